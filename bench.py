@@ -1,0 +1,147 @@
+#!/usr/bin/env python3
+"""Benchmark of the FDFK2D time-stepping hot path on one MI355X.
+
+A step is one iteration of the reference loop (src/FDFK2D.f90:465-532):
+receiver recording, PML memories, the fused interior stencil, the boundary
+frame (free surface, PML strips, hybrid ring).  The workload is the single
+large solve (8001 x 2000 points, dx = dz = 50 m, dt = 0.003 s, 40-cell PML,
+FD order 6, 401 receivers, seeded layered + 5% smoothed random model).
+value = padded-grid cell-updates per second (NX*NZ per step, src/FDFK2D.f90
+loop bounds), inputs resident in HBM.
+
+Prints one JSON line with the roofline of the interior kernel (36
+algorithmic bytes per cell-update, HIP events on the solver's stream) and a
+CPU baseline: the FP32 CPU oracle (the reference's separate operator
+passes, OpenMP) timed on a bounded sample of the 1001 x 400 workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_CELL = 36  # u^n (8) + u^{n-1} (8) + u^{n+1} (8) + dt^2/rho, lambda, mu (12)
+
+
+def gpu_bench(steps, warmup, workload):
+    from fdfk2d_amd import configs, lib
+    nt = warmup + steps + 1
+    cfg = configs.c2(nt=nt) if workload == "c2" else configs.c1(nt=nt)
+    s = lib.Solver(cfg["nx"], cfg["nz"], cfg["npml"], cfg["norder"], cfg["dx"], cfg["dz"], cfg["dt"],
+                   nt, cfg["c"], rx=cfg["rx"], rz=cfg["rz"])
+    bg_mode = "fk-gpu"
+    try:
+        s.compute_background(cfg["fk_left"], cfg["fk_right"], cfg["src_is_p"], cfg["f0"], cfg["ds"],
+                             cfg["rayp"], cfg["bazs"])
+    except lib.FdfkError:
+        bg_mode = "zero"
+        s.set_background([np.zeros(n, np.float32) for n in lib.bg_sizes(cfg["nx"], cfg["nz"], 5, nt)])
+    s.run(warmup)
+    s.sync()
+    s.profile(True)
+    t0 = time.perf_counter()
+    s.run(steps)
+    s.sync()
+    t1 = time.perf_counter()
+    st = s.kernel_stats()
+    NX, NZ = cfg["NX"], cfg["NZ"]
+    nbd = 5
+    int_cells = (cfg["nx"] - 2 * nbd) * (cfg["nz"] - nbd - 2 * 3 + 1)
+    sx, sz = s.seismograms()
+    finite = bool(np.isfinite(sx).all() and np.isfinite(sz).all())
+    s.close()
+    wall = t1 - t0
+    return dict(cfg=cfg, NX=NX, NZ=NZ, wall=wall, stats=st, int_cells=int_cells, bg_mode=bg_mode,
+                finite=finite)
+
+
+def cpu_baseline(sample_steps):
+    from fdfk2d_amd import configs, refstate
+    from oracle import pyoracle as po
+    cfg = configs.c1(nt=sample_steps)
+    st = refstate.make_state(cfg["nx"], cfg["nz"], cfg["npml"], 3, cfg["dx"], cfg["dz"], cfg["dt"],
+                             sample_steps, cfg["c"])
+    st["bg"] = [np.zeros(n, np.float32) for n in po.bg_shapes(cfg["nx"], cfg["nz"], 5, sample_steps)]
+    po.updatedispl(st, 0)  # first-touch / thread start-up outside the timed sample
+    t0 = time.perf_counter()
+    for it in range(1, sample_steps):
+        po.updatedispl(st, it)
+    dt = time.perf_counter() - t0
+    n = sample_steps - 1
+    cells = cfg["NX"] * cfg["NZ"]
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return dict(value=round(cells * n / dt / 1e9, 6), unit="Gcell-updates/s", cores=threads,
+                kind="port",
+                sample=f"CPU oracle, {n} iterations of the 1001x400 (1041x420 padded) workload, "
+                       f"{dt:.1f} s, OMP_NUM_THREADS={threads}")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2"])
+    ap.add_argument("--cpu-steps", type=int, default=41)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py: this build measures one GPU; the hot path shards as x-slabs "
+              "(see DESIGN.md) but multi-GPU runs are not part of it", file=sys.stderr)
+        sys.exit(2)
+    r = gpu_bench(a.steps, a.warmup, a.workload)
+    cells = r["NX"] * r["NZ"]
+    value = cells * a.steps / r["wall"] / 1e9
+    ms = r["wall"] / a.steps * 1e3
+    st = r["stats"]
+    int_ms = st["interior_ms"] / max(st["interior_n"], 1)
+    achieved = BYTES_PER_CELL * r["int_cells"] / (int_ms * 1e-3) / 1e9
+    cfg = r["cfg"]
+    full_steps = 20000
+    out = {
+        "metric": "Gcell-updates/s",
+        "value": round(value, 3),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "single large solve 8001x2000, dx=dz=50 m, dt=0.003 s, 40-cell PML, order 6, "
+                        "401 receivers" if a.workload == "c2" else "1001x400, dx=200 m, dt=0.01 s",
+            "grid_padded": [r["NX"], r["NZ"]],
+            "background": r["bg_mode"],
+            "solves_per_hour_at_20000_steps": round(3600.0 / (ms * 1e-3 * full_steps), 3),
+            "finite": r["finite"],
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "kernel": "k_interior",
+            "kernel_avg_ms": round(int_ms, 5),
+            "bytes_per_cell": BYTES_PER_CELL,
+            "cells_per_launch": r["int_cells"],
+        },
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(a.cpu_steps)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,656 @@
+// Host side of the C ABI declared in include/fdfk.h.
+//
+// A solver handle owns every array of one simulation on the GPU (fields at
+// two time levels, materials, PML strips and memories, the hybrid
+// background, receivers and seismograms) and a HIP stream; fdfk_run queues
+// the per-iteration kernels of the reference loop (src/FDFK2D.f90:465-532)
+// without host synchronisation.
+#include "fdfk.h"
+#include "fdfk_internal.h"
+// single translation unit: the kernels are compiled with the host code
+#include "fdfk_kernels.hip"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace fdfk {
+
+static thread_local std::string g_err;
+static int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+#define HIPCHK(expr)                                                                     \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return fail(FDFK_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// FD coefficients: closed forms of src/Submain.f90:1111-1198 (fd_coef0:
+// staggered first derivative, fd_coef1: collocated first derivative)
+static void coef_table(int norder, bool staggered, double *c)
+{
+    std::memset(c, 0, sizeof(double) * norder * norder);
+    for (int i = 1; i <= norder; ++i)
+        for (int j = 1; j <= i; ++j) {
+            double num = 1.0, lo = 1.0, hi = 1.0;
+            auto node = [&](int k) { return staggered ? (double)((2 * k - 1) * (2 * k - 1)) : (double)(k * k); };
+            for (int k = 1; k <= j - 1; ++k) { num *= node(k); lo *= node(j) - node(k); }
+            for (int k = j + 1; k <= i; ++k) { num *= node(k); hi *= node(k) - node(j); }
+            num *= (j % 2 == 1) ? 1.0 : -1.0;
+            lo *= staggered ? (double)(2 * j - 1) : 2.0 * (double)j;
+            c[(j - 1) + norder * (i - 1)] = num / lo / hi;
+        }
+}
+
+}  // namespace fdfk
+
+using namespace fdfk;
+
+struct fdfk_solver {
+    Geo g{};
+    int nt = 0, nstp = 1, nsis = 0, nsamp = 1, nrcv = 0, precision = 0;
+    int it = 0, istp = 0, isis = 0;
+    float x0 = 0, z0 = 0, vpml = 0;
+    double R0 = 1e-3;
+    int npower = 2;
+    hipStream_t stream = nullptr;
+    std::vector<void *> allocs;
+    int64_t bytes = 0;
+    // fields
+    float *ux[2] = {nullptr, nullptr}, *uz[2] = {nullptr, nullptr};
+    int cur = 0;
+    float *bf = nullptr, *lam = nullptr, *mu = nullptr;
+    Coefs *dcoef = nullptr;
+    Coefs hcoef{};
+    // strips
+    float *strip_fields[3][2][3][2] = {};  // [strip][comp][part][buffer]
+    float *psx[3][2] = {}, *psz[3][2] = {};
+    float *ax = nullptr, *bx = nullptr, *az = nullptr, *bz = nullptr;
+    std::vector<float> h_ax, h_bx, h_az, h_bz, h_dx, h_dz;
+    long strip_cells[3] = {0, 0, 0};
+    // background
+    float *bg[12] = {};
+    long bg_n[12] = {};
+    bool bg_valid = false;
+    // receivers
+    float *rx = nullptr, *rz = nullptr, *seisx = nullptr, *seisz = nullptr;
+    float *stage = nullptr;   // (2, NZ, NX) staging for reference-layout copies
+    // interior kernel
+    int lx = 90, nt_threads = 256;
+    // profiling
+    bool prof = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_int, ev_step;
+    double int_ms = 0, step_ms = 0;
+    int64_t int_n = 0, step_n = 0;
+
+    template <class T>
+    int dalloc(T **p, size_t n)
+    {
+        void *q = nullptr;
+        if (n == 0) n = 1;
+        hipError_t e = hipMalloc(&q, n * sizeof(T));
+        if (e != hipSuccess) return fail(FDFK_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+        allocs.push_back(q);
+        bytes += (int64_t)(n * sizeof(T));
+        *p = (T *)q;
+        return 0;
+    }
+    ~fdfk_solver()
+    {
+        if (stream) hipStreamSynchronize(stream);
+        for (auto &e : ev_int) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
+        for (auto &e : ev_step) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
+        for (void *p : allocs) hipFree(p);
+        if (stream) hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+int build_damping(fdfk_solver *s)
+{
+    const Geo &g = s->g;
+    const int nx = g.nx, nz = g.nz, npml = g.npml, nbd = g.nbd;
+    const int leff = npml - nbd;
+    const double lr = std::log(s->R0);
+    const double d0x = leff > 0 ? -(double)(s->npower + 1) * (double)s->vpml * lr / (2.0 * leff * (double)g.dx) : 0.0;
+    const double d0z = leff > 0 ? -(double)(s->npower + 1) * (double)s->vpml * lr / (2.0 * leff * (double)g.dz) : 0.0;
+    s->h_dx.assign(g.NX, 0.f); s->h_ax.assign(g.NX, 0.f); s->h_bx.assign(g.NX, 1.f);
+    s->h_dz.assign(g.NZ, 0.f); s->h_az.assign(g.NZ, 0.f); s->h_bz.assign(g.NZ, 1.f);
+    auto prof = [&](int q, double d0) {
+        if (q <= nbd || leff <= 0) return 0.0;
+        double r = (double)(q - nbd) / (double)leff, p = 1.0;
+        for (int e = 0; e < s->npower; ++e) p *= r;
+        return d0 * p;
+    };
+    const float fdt = (float)g.dt;
+    for (int j = 0; j < g.NX; ++j) {
+        int ix = j + 1 - npml;
+        int q = ix <= 0 ? 1 - ix : (ix > nx ? ix - nx : 0);
+        float d = (float)prof(q, d0x);
+        s->h_dx[j] = d;
+        s->h_ax[j] = 0.5f * d * fdt;
+        s->h_bx[j] = (float)std::exp(-(double)d * g.dt);
+    }
+    for (int k = 0; k < g.NZ; ++k) {
+        int iz = k + 1;
+        float d = iz > nz ? (float)prof(iz - nz, d0z) : 0.0f;
+        s->h_dz[k] = d;
+        s->h_az[k] = 0.5f * d * fdt;
+        s->h_bz[k] = (float)std::exp(-(double)d * g.dt);
+    }
+    int rc;
+    if ((rc = s->dalloc(&s->ax, g.NX)) || (rc = s->dalloc(&s->bx, g.NX)) ||
+        (rc = s->dalloc(&s->az, g.NZ)) || (rc = s->dalloc(&s->bz, g.NZ)))
+        return rc;
+    HIPCHK(hipMemcpy(s->ax, s->h_ax.data(), g.NX * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(s->bx, s->h_bx.data(), g.NX * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(s->az, s->h_az.data(), g.NZ * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(s->bz, s->h_bz.data(), g.NZ * 4, hipMemcpyHostToDevice));
+    return 0;
+}
+
+StepArgs step_args(fdfk_solver *s)
+{
+    StepArgs a{};
+    a.g = s->g;
+    a.cf = s->dcoef;
+    a.ux = s->ux[s->cur];
+    a.uz = s->uz[s->cur];
+    a.uxp = s->ux[1 - s->cur];
+    a.uzp = s->uz[1 - s->cur];
+    a.bf = s->bf; a.lam = s->lam; a.mu = s->mu;
+    for (int st = 0; st < 3; ++st)
+        for (int c = 0; c < 2; ++c) {
+            for (int p = 0; p < 3; ++p) {
+                a.st.cur[st][c][p] = s->strip_fields[st][c][p][s->cur];
+                a.st.prev[st][c][p] = s->strip_fields[st][c][p][1 - s->cur];
+            }
+            a.st.psx[st][c] = s->psx[st][c];
+            a.st.psz[st][c] = s->psz[st][c];
+        }
+    a.st.ax = s->ax; a.st.bx = s->bx; a.st.az = s->az; a.st.bz = s->bz;
+    for (int i = 0; i < 12; ++i) a.bg.a[i] = s->bg[i];
+    a.bg.valid = s->bg_valid && s->it <= s->nt;
+    a.it = s->it;
+    return a;
+}
+
+template <int NO, int NT>
+void launch_interior_t(fdfk_solver *s)
+{
+    IntArgs<NO> a{};
+    const Geo &g = s->g;
+    a.g = g;
+    const double *c0 = s->hcoef.c0, *c1 = s->hcoef.c1;
+    const int n = g.norder;
+    for (int k = 1; k <= NO; ++k) {
+        double a0 = c0[(k - 1) + n * (NO - 1)], b0 = c1[(k - 1) + n * (NO - 1)];
+        a.k.ai[k - 1] = (float)a0;
+        a.k.aox[k - 1] = (float)(2.0 * a0 / ((double)g.dx * g.dx));
+        a.k.aoz[k - 1] = (float)(2.0 * a0 / ((double)g.dz * g.dz));
+        a.k.bi[k - 1] = (float)b0;
+        a.k.bo[k - 1] = (float)(b0 / ((double)g.dx * g.dz));
+    }
+    a.ux = s->ux[s->cur]; a.uz = s->uz[s->cur];
+    a.uxp = s->ux[1 - s->cur]; a.uzp = s->uz[1 - s->cur];
+    a.bf = s->bf; a.lam = s->lam; a.mu = s->mu;
+    constexpr int TZ = NT - 2 * (2 * NO - 1);
+    int ncols = g.ix1 - g.ix0 + 1, nrows = g.iz1 - g.iz0 + 1;
+    a.lx = s->lx;
+    a.nbx = (ncols + a.lx - 1) / a.lx;
+    int nbz = (nrows + TZ - 1) / TZ;
+    hipLaunchKernelGGL((k_interior<NO, NT>), dim3(a.nbx * nbz), dim3(NT), 0, s->stream, a);
+}
+
+void launch_interior(fdfk_solver *s)
+{
+    switch (s->g.norder) {
+    case 1: launch_interior_t<1, 256>(s); break;
+    case 2: launch_interior_t<2, 256>(s); break;
+    case 3:
+        if (s->nt_threads == 128) launch_interior_t<3, 128>(s);
+        else if (s->nt_threads == 512) launch_interior_t<3, 512>(s);
+        else launch_interior_t<3, 256>(s);
+        break;
+    default: launch_interior_t<4, 256>(s); break;
+    }
+}
+
+Bands make_bands(const Geo &g)
+{
+    Bands b{};
+    b.W = g.nx + 2 * g.npml - 2;
+    b.ntop = (long)(g.iz0 - 1) * b.W;
+    b.nbot = (long)(g.nz + g.npml - 1 - g.iz1) * b.W;
+    long rows = g.iz1 - g.iz0 + 1;
+    b.nleft = rows * (long)(g.ix0 - (2 - g.npml));
+    b.nright = rows * (long)(g.nx + g.npml - 1 - g.ix1);
+    return b;
+}
+
+int one_step(fdfk_solver *s, bool record)
+{
+    const Geo &g = s->g;
+    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
+    if (s->prof) {
+        hipEventCreate(&e0); hipEventCreate(&e1); hipEventCreate(&e2); hipEventCreate(&e3);
+        hipEventRecord(e0, s->stream);
+    }
+    if (record && s->nrcv > 0 && s->istp <= s->nsis) {
+        RecArgs r{};
+        r.g = g; r.x0 = s->x0; r.z0 = s->z0; r.nrcv = s->nrcv; r.nsamp = s->nsamp; r.istp = s->istp;
+        r.rx = s->rx; r.rz = s->rz; r.ux = s->ux[s->cur]; r.uz = s->uz[s->cur];
+        r.seisx = s->seisx; r.seisz = s->seisz;
+        hipLaunchKernelGGL(k_record, dim3((s->nrcv + 63) / 64), dim3(64), 0, s->stream, r);
+    }
+    StepArgs a = step_args(s);
+    long npsi = 2L * g.npml * g.nz + (long)g.NX * g.npml;
+    hipLaunchKernelGGL(k_psi, dim3((unsigned)((npsi + 255) / 256)), dim3(256), 0, s->stream, a);
+    if (s->prof) hipEventRecord(e1, s->stream);
+    launch_interior(s);
+    if (s->prof) hipEventRecord(e2, s->stream);
+    Bands b = make_bands(g);
+    long nb = b.ntop + b.nbot + b.nleft + b.nright;
+    hipLaunchKernelGGL(k_boundary, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, s->stream, a, b);
+    if (s->prof) {
+        hipEventRecord(e3, s->stream);
+        s->ev_int.emplace_back(e1, e2);
+        s->ev_step.emplace_back(e0, e3);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(FDFK_EDEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
+    s->cur = 1 - s->cur;
+    s->it += 1;
+    return 0;
+}
+
+int validate(const fdfk_job *j)
+{
+    if (!j) return fail(FDFK_EINVAL, "null job");
+    if (j->is_pml_top) return fail(FDFK_ENOSYS, "is_PML_top = .true. is not supported (free surface only)");
+    if (j->norder < 1 || j->norder > 4) return fail(FDFK_EINVAL, "norder must be 1..4 (FD order 2..8)");
+    int nbd = 2 * j->norder - 1;
+    if (j->npml < nbd) return fail(FDFK_EINVAL, "npml must be >= 2*norder-1");
+    if (j->nx < 2 * nbd + 2) return fail(FDFK_EINVAL, "nx too small for the hybrid ring");
+    if (j->nz < nbd + 2 * j->norder + 1) return fail(FDFK_EINVAL, "nz too small");
+    if (!(j->dx > 0) || !(j->dz > 0) || !(j->dt > 0)) return fail(FDFK_EINVAL, "dx, dz, dt must be > 0");
+    if (j->nt < 0 || j->nstp < 1) return fail(FDFK_EINVAL, "nt >= 0 and nstp >= 1 required");
+    if (!j->c) return fail(FDFK_EINVAL, "material array c is required");
+    if (j->nrcv < 0 || (j->nrcv > 0 && (!j->rx || !j->rz))) return fail(FDFK_EINVAL, "receiver arrays");
+    if (j->precision != FDFK_FP32) return fail(FDFK_ENOSYS, "only FP32 storage in this entry point");
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *fdfk_last_error(void) { return g_err.c_str(); }
+
+int fdfk_create(const fdfk_job *job, fdfk_solver **out)
+{
+    if (!out) return fail(FDFK_EINVAL, "null output pointer");
+    *out = nullptr;
+    int rc = validate(job);
+    if (rc) return rc;
+    auto *s = new fdfk_solver();
+    Geo &g = s->g;
+    g.nx = job->nx; g.nz = job->nz; g.npml = job->npml; g.norder = job->norder;
+    g.nbd = 2 * g.norder - 1;
+    g.NX = g.nx + 2 * g.npml; g.NZ = g.nz + g.npml;
+    g.ld = (g.NZ + 31) / 32 * 32;
+    g.dx = job->dx; g.dz = job->dz; g.dt = job->dt;
+    g.ix0 = g.nbd + 1; g.ix1 = g.nx - g.nbd;
+    g.iz0 = 2 * g.norder; g.iz1 = g.nz - g.nbd;
+    s->nt = job->nt; s->nstp = job->nstp;
+    s->nsis = job->nt / job->nstp;
+    s->nsamp = s->nsis + 1;
+    s->nrcv = job->nrcv;
+    s->x0 = job->x0; s->z0 = job->z0;
+    s->R0 = job->R0; s->npower = job->npower;
+    if (const char *e = std::getenv("FDFK_LX")) s->lx = std::max(9, std::atoi(e));
+    if (const char *e = std::getenv("FDFK_NT")) s->nt_threads = std::atoi(e);
+    const long ncell = (long)g.NX * g.NZ;
+    // vpml = max Vp (src/FDFK2D.f90:416)
+    float vpml = job->vpml;
+    if (!(vpml > 0)) {
+        vpml = 0.0f;
+        for (long i = 0; i < ncell; ++i) {
+            float v = std::sqrt((job->c[3 * i + 1] + 2.0f * job->c[3 * i + 2]) / job->c[3 * i]);
+            if (v > vpml) vpml = v;
+        }
+    }
+    s->vpml = vpml;
+    coef_table(g.norder, true, s->hcoef.c0);
+    coef_table(g.norder, false, s->hcoef.c1);
+    auto bad = [&](int code) { delete s; return code; };
+    if (hipStreamCreate(&s->stream) != hipSuccess) return bad(fail(FDFK_EDEVICE, "hipStreamCreate failed"));
+    const size_t nf = (size_t)g.NX * g.ld;
+    if ((rc = s->dalloc(&s->ux[0], nf)) || (rc = s->dalloc(&s->ux[1], nf)) ||
+        (rc = s->dalloc(&s->uz[0], nf)) || (rc = s->dalloc(&s->uz[1], nf)) ||
+        (rc = s->dalloc(&s->bf, nf)) || (rc = s->dalloc(&s->lam, nf)) || (rc = s->dalloc(&s->mu, nf)) ||
+        (rc = s->dalloc(&s->dcoef, 1)) || (rc = s->dalloc(&s->stage, (size_t)2 * ncell)))
+        return bad(rc);
+    for (int q = 0; q < 2; ++q) {
+        if (hipMemset(s->ux[q], 0, nf * 4) || hipMemset(s->uz[q], 0, nf * 4)) return bad(fail(FDFK_EDEVICE, "memset"));
+    }
+    if (hipMemset(s->bf, 0, nf * 4) || hipMemset(s->lam, 0, nf * 4) || hipMemset(s->mu, 0, nf * 4))
+        return bad(fail(FDFK_EDEVICE, "memset"));
+    if (hipMemcpy(s->dcoef, &s->hcoef, sizeof(Coefs), hipMemcpyHostToDevice) != hipSuccess)
+        return bad(fail(FDFK_EDEVICE, "coef upload"));
+    // materials: reference layout -> dt^2/rho, lambda, mu
+    float *dc = nullptr;
+    if (hipMalloc(&dc, (size_t)3 * ncell * 4) != hipSuccess) return bad(fail(FDFK_ENOMEM, "staging"));
+    hipMemcpy(dc, job->c, (size_t)3 * ncell * 4, hipMemcpyHostToDevice);
+    hipLaunchKernelGGL(k_import_material, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, s->stream, g, dc,
+                       (float)(g.dt * g.dt), s->bf, s->lam, s->mu);
+    hipStreamSynchronize(s->stream);
+    hipFree(dc);
+    // PML strips
+    s->strip_cells[S_L] = s->strip_cells[S_R] = (long)g.npml * g.nz;
+    s->strip_cells[S_B] = (long)g.NX * g.npml;
+    for (int st = 0; st < 3; ++st) {
+        size_t n = (size_t)s->strip_cells[st];
+        for (int c = 0; c < 2; ++c) {
+            for (int p = 0; p < 3; ++p)
+                for (int b = 0; b < 2; ++b) {
+                    if ((rc = s->dalloc(&s->strip_fields[st][c][p][b], n))) return bad(rc);
+                    hipMemset(s->strip_fields[st][c][p][b], 0, n * 4);
+                }
+            if ((rc = s->dalloc(&s->psx[st][c], n))) return bad(rc);
+            hipMemset(s->psx[st][c], 0, n * 4);
+            if (st == S_B) {
+                if ((rc = s->dalloc(&s->psz[st][c], n))) return bad(rc);
+                hipMemset(s->psz[st][c], 0, n * 4);
+            }
+        }
+    }
+    if ((rc = build_damping(s))) return bad(rc);
+    // receivers
+    if (s->nrcv > 0) {
+        if ((rc = s->dalloc(&s->rx, s->nrcv)) || (rc = s->dalloc(&s->rz, s->nrcv)) ||
+            (rc = s->dalloc(&s->seisx, (size_t)s->nrcv * s->nsamp)) ||
+            (rc = s->dalloc(&s->seisz, (size_t)s->nrcv * s->nsamp)))
+            return bad(rc);
+        hipMemcpy(s->rx, job->rx, s->nrcv * 4, hipMemcpyHostToDevice);
+        hipMemcpy(s->rz, job->rz, s->nrcv * 4, hipMemcpyHostToDevice);
+        hipMemset(s->seisx, 0, (size_t)s->nrcv * s->nsamp * 4);
+        hipMemset(s->seisz, 0, (size_t)s->nrcv * s->nsamp * 4);
+    }
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) return bad(fail(FDFK_EDEVICE, std::string("setup: ") + hipGetErrorString(e)));
+    *out = s;
+    return 0;
+}
+
+static void bg_sizes(const Geo &g, int nt, long n[12])
+{
+    long T = nt + 1;
+    long reg_lr = (long)(g.nz - g.nbd) * g.nbd * T, reg_b = (long)g.nbd * g.nx * T;
+    long pml_lr = (long)g.nz * g.nbd * T, pml_b = (long)g.nbd * (g.nx + 2 * g.nbd) * T;
+    long v[12] = {reg_lr, reg_lr, reg_lr, reg_lr, reg_b, reg_b, pml_lr, pml_lr, pml_lr, pml_lr, pml_b, pml_b};
+    for (int i = 0; i < 12; ++i) n[i] = v[i];
+}
+
+int fdfk_set_background(fdfk_solver *s, const float *const bg[12])
+{
+    if (!s || !bg) return fail(FDFK_EINVAL, "null argument");
+    long n[12];
+    bg_sizes(s->g, s->nt, n);
+    for (int i = 0; i < 12; ++i) {
+        if (!bg[i]) return fail(FDFK_EINVAL, "null background array");
+        if (!s->bg[i]) {
+            int rc = s->dalloc(&s->bg[i], (size_t)n[i]);
+            if (rc) return rc;
+            s->bg_n[i] = n[i];
+        }
+        HIPCHK(hipMemcpyAsync(s->bg[i], bg[i], (size_t)n[i] * 4, hipMemcpyHostToDevice, s->stream));
+    }
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->bg_valid = true;
+    return 0;
+}
+
+int fdfk_get_background(fdfk_solver *s, float *const bg[12])
+{
+    if (!s || !bg) return fail(FDFK_EINVAL, "null argument");
+    if (!s->bg_valid) return fail(FDFK_EINVAL, "no background on the device");
+    for (int i = 0; i < 12; ++i)
+        HIPCHK(hipMemcpyAsync(bg[i], s->bg[i], (size_t)s->bg_n[i] * 4, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+int fdfk_run(fdfk_solver *s, int32_t nsteps)
+{
+    if (!s) return fail(FDFK_EINVAL, "null solver");
+    if (nsteps < 0) return fail(FDFK_EINVAL, "nsteps < 0");
+    for (int k = 0; k < nsteps; ++k) {
+        if (s->it > s->nt) return fail(FDFK_ERANGE, "all iterations done");
+        bool rec = (s->isis == s->it) && (s->istp <= s->nsis);
+        int rc = one_step(s, rec);
+        if (rc) return rc;
+        if (rec) { s->istp += 1; s->isis += s->nstp; }
+    }
+    return 0;
+}
+
+int fdfk_iteration(const fdfk_solver *s) { return s ? s->it : FDFK_EINVAL; }
+
+int fdfk_sync(fdfk_solver *s)
+{
+    if (!s) return fail(FDFK_EINVAL, "null solver");
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+int fdfk_seismograms(fdfk_solver *s, float *sx, float *sz)
+{
+    if (!s || !sx || !sz) return fail(FDFK_EINVAL, "null argument");
+    if (s->nrcv == 0) return 0;
+    size_t n = (size_t)s->nrcv * s->nsamp * 4;
+    HIPCHK(hipMemcpyAsync(sx, s->seisx, n, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipMemcpyAsync(sz, s->seisz, n, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+int fdfk_get_state(fdfk_solver *s, float *ux, float *uz)
+{
+    if (!s || !ux || !uz) return fail(FDFK_EINVAL, "null argument");
+    const Geo &g = s->g;
+    long n = (long)g.NX * g.NZ;
+    dim3 gr((unsigned)((n + 255) / 256)), bl(256);
+    hipLaunchKernelGGL(k_export_levels, gr, bl, 0, s->stream, g, s->ux[s->cur], s->ux[1 - s->cur], s->stage);
+    HIPCHK(hipMemcpyAsync(ux, s->stage, (size_t)2 * n * 4, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    hipLaunchKernelGGL(k_export_levels, gr, bl, 0, s->stream, g, s->uz[s->cur], s->uz[1 - s->cur], s->stage);
+    HIPCHK(hipMemcpyAsync(uz, s->stage, (size_t)2 * n * 4, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+int fdfk_set_state(fdfk_solver *s, const float *ux, const float *uz)
+{
+    if (!s || !ux || !uz) return fail(FDFK_EINVAL, "null argument");
+    const Geo &g = s->g;
+    long n = (long)g.NX * g.NZ;
+    dim3 gr((unsigned)((n + 255) / 256)), bl(256);
+    HIPCHK(hipMemcpyAsync(s->stage, ux, (size_t)2 * n * 4, hipMemcpyHostToDevice, s->stream));
+    hipLaunchKernelGGL(k_import_levels, gr, bl, 0, s->stream, g, s->stage, s->ux[s->cur], s->ux[1 - s->cur]);
+    HIPCHK(hipStreamSynchronize(s->stream));
+    HIPCHK(hipMemcpyAsync(s->stage, uz, (size_t)2 * n * 4, hipMemcpyHostToDevice, s->stream));
+    hipLaunchKernelGGL(k_import_levels, gr, bl, 0, s->stream, g, s->stage, s->uz[s->cur], s->uz[1 - s->cur]);
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+int fdfk_profile(fdfk_solver *s, int32_t enable)
+{
+    if (!s) return fail(FDFK_EINVAL, "null solver");
+    s->prof = enable != 0;
+    return 0;
+}
+
+int fdfk_kernel_stats(fdfk_solver *s, double *int_ms, int64_t *int_n, double *step_ms, int64_t *step_n)
+{
+    if (!s) return fail(FDFK_EINVAL, "null solver");
+    HIPCHK(hipStreamSynchronize(s->stream));
+    for (auto &e : s->ev_int) {
+        float ms = 0;
+        hipEventElapsedTime(&ms, e.first, e.second);
+        s->int_ms += ms; s->int_n += 1;
+    }
+    for (auto &e : s->ev_step) {
+        float ms = 0;
+        hipEventElapsedTime(&ms, e.first, e.second);
+        s->step_ms += ms; s->step_n += 1;
+        hipEventDestroy(e.first); hipEventDestroy(e.second);
+    }
+    for (auto &e : s->ev_int) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
+    s->ev_int.clear(); s->ev_step.clear();
+    if (int_ms) *int_ms = s->int_ms;
+    if (int_n) *int_n = s->int_n;
+    if (step_ms) *step_ms = s->step_ms;
+    if (step_n) *step_n = s->step_n;
+    s->int_ms = s->step_ms = 0; s->int_n = s->step_n = 0;
+    return 0;
+}
+
+int64_t fdfk_device_bytes(const fdfk_solver *s) { return s ? s->bytes : 0; }
+
+void fdfk_destroy(fdfk_solver *s) { delete s; }
+
+// ---------------------------------------------------------------------
+// compatibility layer: the reference's external procedures
+// ---------------------------------------------------------------------
+struct CompatEntry {
+    fdfk_solver *s = nullptr;
+    int last_it = -2;
+    int nx, nz, npml, norder, nt;
+    const float *c, *bg0;
+};
+static std::map<const float *, CompatEntry> g_compat;
+static std::mutex g_compat_mu;
+
+static int strip_import(fdfk_solver *s, int st, float *const parts[2][3], float *const psx[2], float *const psz[2])
+{
+    long n = s->strip_cells[st];
+    float *tmp = s->stage;  // 2*NX*NZ floats >= 2*n
+    dim3 gr((unsigned)((n + 255) / 256)), bl(256);
+    for (int c = 0; c < 2; ++c) {
+        for (int p = 0; p < 3; ++p) {
+            HIPCHK(hipMemcpyAsync(tmp, parts[c][p], (size_t)2 * n * 4, hipMemcpyHostToDevice, s->stream));
+            hipLaunchKernelGGL(k_split_levels, gr, bl, 0, s->stream, n, tmp, s->strip_fields[st][c][p][s->cur],
+                               s->strip_fields[st][c][p][1 - s->cur]);
+            HIPCHK(hipStreamSynchronize(s->stream));
+        }
+        HIPCHK(hipMemcpy(s->psx[st][c], psx[c], (size_t)n * 4, hipMemcpyHostToDevice));
+        if (st == S_B) HIPCHK(hipMemcpy(s->psz[st][c], psz[c], (size_t)n * 4, hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+static int strip_export(fdfk_solver *s, int st, float *const parts[2][3], float *const psx[2], float *const psz[2])
+{
+    long n = s->strip_cells[st];
+    float *tmp = s->stage;
+    dim3 gr((unsigned)((n + 255) / 256)), bl(256);
+    for (int c = 0; c < 2; ++c) {
+        for (int p = 0; p < 3; ++p) {
+            hipLaunchKernelGGL(k_join_levels, gr, bl, 0, s->stream, n, s->strip_fields[st][c][p][s->cur],
+                               s->strip_fields[st][c][p][1 - s->cur], tmp);
+            HIPCHK(hipMemcpyAsync(parts[c][p], tmp, (size_t)2 * n * 4, hipMemcpyDeviceToHost, s->stream));
+            HIPCHK(hipStreamSynchronize(s->stream));
+        }
+        HIPCHK(hipMemcpy(psx[c], s->psx[st][c], (size_t)n * 4, hipMemcpyDeviceToHost));
+        if (st == S_B) HIPCHK(hipMemcpy(psz[c], s->psz[st][c], (size_t)n * 4, hipMemcpyDeviceToHost));
+    }
+    return 0;
+}
+
+void updatedispl_(const int32_t *norder, const int32_t *nx, const int32_t *nz, const int32_t *npml,
+    const int32_t *izt, const int32_t *is_pml_top, const int32_t *npower, const int32_t *nbd,
+    const int32_t *it, const int32_t *nt, const float *dx, const float *dz, const double *dt,
+    const float *vpml, const double *R0,
+    const float *uxl_reg, const float *uzl_reg, const float *uxr_reg, const float *uzr_reg,
+    const float *uxb_reg, const float *uzb_reg, const float *uxl_pml, const float *uzl_pml,
+    const float *uxr_pml, const float *uzr_pml, const float *uxb_pml, const float *uzb_pml,
+    const float *c, const double *coef0, const double *coef1, float *ux, float *uz,
+    float *bpxlx, float *bpzlx, float *bpxrx, float *bpzrx, float *bpxtx, float *bpxtz,
+    float *bpztx, float *bpztz, float *bpxbx, float *bpxbz, float *bpzbx, float *bpzbz,
+    float *bux1l, float *bux2l, float *bux3l, float *buz1l, float *buz2l, float *buz3l,
+    float *bux1r, float *bux2r, float *bux3r, float *buz1r, float *buz2r, float *buz3r,
+    float *bux1b, float *bux2b, float *bux3b, float *buz1b, float *buz2b, float *buz3b,
+    float *bux1t, float *bux2t, float *bux3t, float *buz1t, float *buz2t, float *buz3t)
+{
+    (void)coef0; (void)coef1; (void)bpxtx; (void)bpxtz; (void)bpztx; (void)bpztz;
+    (void)bux1t; (void)bux2t; (void)bux3t; (void)buz1t; (void)buz2t; (void)buz3t;
+    std::lock_guard<std::mutex> lk(g_compat_mu);
+    auto die = [](const std::string &m) { std::fprintf(stderr, "updatedispl (fdfk2d_amd): %s\n", m.c_str()); };
+    if (*is_pml_top || *izt != 1) { die("only is_PML_top = .false. is supported"); return; }
+    if (*nbd != 2 * *norder - 1) { die("nbd must equal 2*norder-1"); return; }
+    auto itr = g_compat.find(ux);
+    bool fresh = itr == g_compat.end() || itr->second.nx != *nx || itr->second.nz != *nz ||
+                 itr->second.npml != *npml || itr->second.norder != *norder || itr->second.nt != *nt ||
+                 itr->second.c != c || itr->second.bg0 != uxl_reg;
+    if (fresh) {
+        if (itr != g_compat.end()) { fdfk_destroy(itr->second.s); g_compat.erase(itr); }
+        fdfk_job j{};
+        j.nx = *nx; j.nz = *nz; j.npml = *npml; j.norder = *norder; j.npower = *npower;
+        j.dx = *dx; j.dz = *dz; j.dt = *dt; j.R0 = *R0; j.vpml = *vpml; j.nt = *nt; j.nstp = 1;
+        j.c = c; j.precision = FDFK_FP32;
+        fdfk_solver *s = nullptr;
+        if (fdfk_create(&j, &s)) { die(fdfk_last_error()); return; }
+        const float *bg[12] = {uxl_reg, uzl_reg, uxr_reg, uzr_reg, uxb_reg, uzb_reg,
+                               uxl_pml, uzl_pml, uxr_pml, uzr_pml, uxb_pml, uzb_pml};
+        if (fdfk_set_background(s, bg)) { die(fdfk_last_error()); fdfk_destroy(s); return; }
+        CompatEntry e;
+        e.s = s; e.nx = *nx; e.nz = *nz; e.npml = *npml; e.norder = *norder; e.nt = *nt; e.c = c; e.bg0 = uxl_reg;
+        itr = g_compat.emplace(ux, e).first;
+    }
+    CompatEntry &e = itr->second;
+    fdfk_solver *s = e.s;
+    float *const pl[2][3] = {{bux1l, bux2l, bux3l}, {buz1l, buz2l, buz3l}};
+    float *const pr[2][3] = {{bux1r, bux2r, bux3r}, {buz1r, buz2r, buz3r}};
+    float *const pb[2][3] = {{bux1b, bux2b, bux3b}, {buz1b, buz2b, buz3b}};
+    float *const xl[2] = {bpxlx, bpzlx}, *const xr[2] = {bpxrx, bpzrx}, *const xb[2] = {bpxbx, bpzbx};
+    float *const zb[2] = {bpxbz, bpzbz}, *const none[2] = {nullptr, nullptr};
+    if (e.last_it != *it - 1) {
+        // host state is authoritative: import fields, split fields and memories
+        if (fdfk_set_state(s, ux, uz) || strip_import(s, S_L, pl, xl, none) ||
+            strip_import(s, S_R, pr, xr, none) || strip_import(s, S_B, pb, xb, zb)) {
+            die(fdfk_last_error());
+            return;
+        }
+    }
+    s->it = *it;
+    if (one_step(s, false)) { die(fdfk_last_error()); return; }
+    if (fdfk_get_state(s, ux, uz) || strip_export(s, S_L, pl, xl, none) ||
+        strip_export(s, S_R, pr, xr, none) || strip_export(s, S_B, pb, xb, zb)) {
+        die(fdfk_last_error());
+        return;
+    }
+    e.last_it = *it;
+}
+
+void fdfk_compat_release(void)
+{
+    std::lock_guard<std::mutex> lk(g_compat_mu);
+    for (auto &kv : g_compat) fdfk_destroy(kv.second.s);
+    g_compat.clear();
+}
+
+}  // extern "C"
+
+#include "fdfk_fk.hip"

@@ -1,0 +1,116 @@
+// Internal definitions shared by the HIP kernels and the host API of the
+// FDFK2D time-stepping library (gfx950).  Not part of the public ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace fdfk {
+
+constexpr int kMaxOrder = 8;
+
+// Grid description.  Cell (ix, iz) with ix in 1-npml..nx+npml (reference
+// column index) and iz in 1..nz+npml (free surface at iz = 1) lives at
+//   (ix + npml - 1) * ld + (iz - 1)
+// i.e. z is the contiguous (wavefront-coalesced) axis, like the reference's
+// Fortran arrays, with the column pitch ld padded to a 128-byte multiple.
+struct Geo {
+    int nx, nz, npml, norder, nbd;
+    int NX, NZ, ld;
+    float dx, dz;
+    double dt;
+    // interior rectangle handled by the tiled stencil kernel: full-order
+    // stencils, no PML, no hybrid correction, no free-surface closure
+    int ix0, ix1, iz0, iz1;
+};
+
+__host__ __device__ inline size_t cell(const Geo &g, int iz, int ix)
+{
+    return (size_t)(ix + g.npml - 1) * (size_t)g.ld + (size_t)(iz - 1);
+}
+
+// FD coefficient tables coef(j, i) = c[(j-1) + norder*(i-1)] in FP64 (as the
+// reference's fd_coef0/fd_coef1) plus full-order FP32 copies for the
+// interior kernel.
+struct Coefs {
+    double c0[kMaxOrder * kMaxOrder];
+    double c1[kMaxOrder * kMaxOrder];
+};
+
+// Padding strips of the PML (left, right, bottom).  Cell offsets inside a
+// strip are column-major with z fastest:
+//   L: (ix - (1-npml)) * nz + (iz - 1)          ix in 1-npml..0,     iz in 1..nz
+//   R: (ix - (nx+1))   * nz + (iz - 1)          ix in nx+1..nx+npml, iz in 1..nz
+//   B: (ix - (1-npml)) * npml + (iz - (nz+1))   all ix,              iz in nz+1..nz+npml
+// which is exactly the reference's bux1l/bpxlx/... cell order.
+enum { S_L = 0, S_R = 1, S_B = 2, S_NONE = -1 };
+
+struct Strips {
+    // split fields u = u1 + u2 + u3: [strip][comp][part] -> cur / prev
+    float *cur[3][2][3];
+    float *prev[3][2][3];
+    // first-derivative memories psi: [strip][comp] x-direction, z (bottom only)
+    float *psx[3][2];
+    float *psz[3][2];
+    // damping: per padded column / row
+    const float *ax, *bx;   // 0.5*d*dt and exp(-d*dt) per column (NX)
+    const float *az, *bz;   // same per row (NZ)
+};
+
+// Twelve background arrays, reference layout and order:
+// 0 uxl_reg 1 uzl_reg 2 uxr_reg 3 uzr_reg 4 uxb_reg 5 uzb_reg
+// 6 uxl_pml 7 uzl_pml 8 uxr_pml 9 uzr_pml 10 uxb_pml 11 uzb_pml
+struct Background {
+    const float *a[12];
+    int valid;
+};
+
+struct StepArgs {
+    Geo g;
+    const Coefs *cf;
+    const float *ux, *uz;          // level n
+    float *uxp, *uzp;              // level n-1 in, n+1 out
+    const float *bf, *lam, *mu;    // dt^2/rho, lambda, mu
+    Strips st;
+    Background bg;
+    int it;
+};
+
+// boundary-kernel work decomposition: the updated cells outside the
+// interior rectangle, as four bands (top, bottom, left, right)
+struct Bands {
+    long ntop, nbot, nleft, nright;
+    int W;  // updated columns 2-npml .. nx+npml-1
+};
+
+struct RecArgs {
+    Geo g;
+    float x0, z0;
+    int nrcv, nsamp, istp;
+    const float *rx, *rz;
+    const float *ux, *uz;
+    float *seisx, *seisz;  // (nsamp, nrcv), time fastest
+};
+
+// full-order FP32 coefficients of the interior kernel
+template <int NO>
+struct ICoef {
+    float ai[NO];   // staggered inner  a_j = coef0(j,NO)
+    float aox[NO];  // x outer 2*coef0(k,NO)/dx^2
+    float aoz[NO];  // z outer 2*coef0(k,NO)/dz^2
+    float bi[NO];   // central inner    b_j = coef1(j,NO)
+    float bo[NO];   // mixed outer      coef1(k,NO)/(dx*dz)
+};
+
+template <int NO>
+struct IntArgs {
+    Geo g;
+    ICoef<NO> k;
+    const float *ux, *uz;
+    float *uxp, *uzp;
+    const float *bf, *lam, *mu;
+    int lx;   // output columns per block
+    int nbx;  // blocks along x
+};
+
+}  // namespace fdfk

@@ -1,0 +1,683 @@
+// HIP kernels (gfx950) of the FDFK2D time step: the updatedispl role
+// (src/FDFK2D.f90:516-522, binary-only in the reference) and the receiver
+// recorder loadseis (src/Submain.f90:33-116).
+//
+// One step n -> n+1 is three launches on the handle's stream:
+//   k_psi       first-derivative PML memories psi (level n)       strips only
+//   k_interior  full-order fused stencil + leapfrog                 ~95% of cells
+//   k_boundary  everything else: free-surface rows, reduced-order   the frame
+//               edge stencils, split-field PML, hybrid TF/SF ring
+// and, on recording iterations, k_record before them.
+//
+// k_interior is the HBM-bound hot kernel: 64-wide wavefronts run along z
+// (coalesced rows), each block marches along x keeping x-neighbourhoods in
+// register rings (no x halo re-reads) and exchanging z-neighbourhoods
+// through LDS.  It reads u^n, u^{n-1}, dt^2/rho, lambda, mu once and writes
+// u^{n+1} in place of u^{n-1}: 36 algorithmic bytes per cell-update in FP32.
+//
+// k_boundary, k_psi and k_record reproduce the CPU oracle's arithmetic
+// exactly (FP64 accumulation rounded to FP32 after every term, the
+// Fortran REAL(4)/REAL(8) mixing of src/Submain.f90), so on the frame the
+// GPU and the oracle agree to the last bit given the same inputs.
+#include "fdfk_internal.h"
+
+namespace fdfk {
+
+// ---------------------------------------------------------------------
+// exact (oracle-arithmetic) pointwise operators
+// ---------------------------------------------------------------------
+#pragma clang fp contract(off)
+
+__device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b; }
+__device__ __forceinline__ float accd(float s, double c, float d)
+{
+    return (float)((double)s + c * (double)d);
+}
+__device__ __forceinline__ float hmean(float a, float b) { return 2.0f / (1.0f / a + 1.0f / b); }
+__device__ __forceinline__ float meff_of(float lam, float mu)
+{
+    float m = lam + 2.0f * mu;
+    return m != 0.0f ? (4.0f * mu) * (lam + mu) / m : 0.0f;
+}
+__device__ __forceinline__ float matval(int mode, float lam, float mu)
+{
+    return mode == 0 ? lam + 2.0f * mu : (mode == 1 ? mu : meff_of(lam, mu));
+}
+
+struct FieldRd {
+    const float *p;
+    const Geo *g;
+    __device__ float operator()(int iz, int ix) const { return p[cell(*g, iz, ix)]; }
+};
+
+// background cell offset in the twelve reference arrays (FDFK2D.f90:330-355)
+// arr: 0 left, 1 right, 2 bottom; which: 0 inside (reg), 1 outside (pml)
+__device__ __forceinline__ long bg_off(const Geo &g, int iz, int ix, int it, int &arr, int &which)
+{
+    const int nx = g.nx, nz = g.nz, nbd = g.nbd;
+    if (iz >= 1 && iz <= nz - nbd && ix >= 1 && ix <= nbd) {
+        arr = 0; which = 0;
+        return (long)(iz - 1) + (long)(nz - nbd) * ((long)(ix - 1) + (long)nbd * it);
+    }
+    if (iz >= 1 && iz <= nz - nbd && ix >= nx - nbd + 1 && ix <= nx) {
+        arr = 1; which = 0;
+        return (long)(iz - 1) + (long)(nz - nbd) * ((long)(ix - (nx - nbd + 1)) + (long)nbd * it);
+    }
+    if (iz >= nz - nbd + 1 && iz <= nz && ix >= 1 && ix <= nx) {
+        arr = 2; which = 0;
+        return (long)(iz - (nz - nbd + 1)) + (long)nbd * ((long)(ix - 1) + (long)nx * it);
+    }
+    if (iz >= 1 && iz <= nz && ix >= 1 - nbd && ix <= 0) {
+        arr = 0; which = 1;
+        return (long)(iz - 1) + (long)nz * ((long)(ix - (1 - nbd)) + (long)nbd * it);
+    }
+    if (iz >= 1 && iz <= nz && ix >= nx + 1 && ix <= nx + nbd) {
+        arr = 1; which = 1;
+        return (long)(iz - 1) + (long)nz * ((long)(ix - (nx + 1)) + (long)nbd * it);
+    }
+    if (iz >= nz + 1 && iz <= nz + nbd && ix >= 1 - nbd && ix <= nx + nbd) {
+        arr = 2; which = 1;
+        return (long)(iz - (nz + 1)) + (long)nbd * ((long)(ix - (1 - nbd)) + (long)(nx + 2 * nbd) * it);
+    }
+    return -1;
+}
+
+// background value of one component restricted to the inside (which = 0)
+// or outside (which = 1) ring strips, zero elsewhere
+struct MaskRd {
+    const Background *bg;
+    const Geo *g;
+    int it, which, comp;
+    __device__ float operator()(int iz, int ix) const
+    {
+        int arr, wh;
+        long o = bg_off(*g, iz, ix, it, arr, wh);
+        if (o < 0 || wh != which) return 0.0f;
+        return bg->a[6 * wh + 2 * arr + comp][o];
+    }
+};
+
+template <class R>
+__device__ float cdx_sum(const Geo &g, const Coefs *cf, const R &f, int iz, int ix)
+{
+    const int no = g.norder;
+    int nox = imin(imin(ix + g.npml - 1, g.nx + g.npml - ix), no);
+    float s = 0.0f;
+    for (int k = 1; k <= nox; ++k) s = accd(s, cf->c1[(k - 1) + no * (nox - 1)], f(iz, ix + k) - f(iz, ix - k));
+    return s;
+}
+template <class R>
+__device__ float cdz_sum(const Geo &g, const Coefs *cf, const R &f, int iz, int ix)
+{
+    const int no = g.norder;
+    int noz = imin(imin(iz - 1, g.nz + g.npml - iz), no);
+    float s = 0.0f;
+    for (int k = 1; k <= noz; ++k) s = accd(s, cf->c1[(k - 1) + no * (noz - 1)], f(iz + k, ix) - f(iz - k, ix));
+    return s;
+}
+
+template <class R>
+__device__ float x2_mode(const Geo &g, const Coefs *cf, const float *lam, const float *mu, int mode,
+                         const R &u, int iz, int ix)
+{
+    const int no = g.norder, nx = g.nx, npml = g.npml;
+    int nox = imin(imin(ix + npml - 1, nx + npml - ix), no);
+    float s = 0.0f;
+    auto M = [&](int jx) { size_t c = cell(g, iz, jx); return matval(mode, lam[c], mu[c]); };
+    for (int k = 1; k <= nox; ++k) {
+        int i = (2 * k - 1) / 2;
+        int n1 = imin(imin(ix + i + npml - 1, nx + npml - ix - i), no);
+        float s1 = 0.0f;
+        for (int j = 1; j <= n1; ++j) s1 = accd(s1, cf->c0[(j - 1) + no * (n1 - 1)], u(iz, ix + j + k - 1) - u(iz, ix - j + k));
+        int n2 = imin(imin(ix - i + npml - 1, nx + npml - ix + i), no);
+        float s2 = 0.0f;
+        for (int j = 1; j <= n2; ++j) s2 = accd(s2, cf->c0[(j - 1) + no * (n2 - 1)], u(iz, ix + j - k) - u(iz, ix - j - k + 1));
+        float t = hmean(M(ix + k), M(ix + k - 1)) * s1 - hmean(M(ix - k + 1), M(ix - k)) * s2;
+        s = accd(s, cf->c0[(k - 1) + no * (nox - 1)], t);
+    }
+    bool in_reg = (iz >= 1 && iz <= g.nz && ix >= 1 && ix <= nx);
+    float dx2 = in_reg ? (float)(1.0 / (double)(g.dx * g.dx)) : 1.0f / (g.dx * g.dx);
+    return s * dx2;
+}
+
+template <class R>
+__device__ float z2_mode(const Geo &g, const Coefs *cf, const float *lam, const float *mu, int mode,
+                         const R &u, int iz, int ix)
+{
+    const int no = g.norder, nz = g.nz, npml = g.npml, izt = 1;
+    int noz = imin(imin(iz - izt, nz + npml - iz), no);
+    float s = 0.0f;
+    auto M = [&](int jz) { size_t c = cell(g, jz, ix); return matval(mode, lam[c], mu[c]); };
+    for (int k = 1; k <= noz; ++k) {
+        int i = (2 * k - 1) / 2;
+        int n1 = imin(imin(iz + i - izt, nz + npml - iz - i), no);
+        float s1 = 0.0f;
+        for (int j = 1; j <= n1; ++j) s1 = accd(s1, cf->c0[(j - 1) + no * (n1 - 1)], u(iz + j + k - 1, ix) - u(iz - j + k, ix));
+        int n2 = imin(imin(iz - i - izt, nz + npml - iz + i), no);
+        float s2 = 0.0f;
+        for (int j = 1; j <= n2; ++j) s2 = accd(s2, cf->c0[(j - 1) + no * (n2 - 1)], u(iz + j - k, ix) - u(iz - j - k + 1, ix));
+        float t = hmean(M(iz + k), M(iz + k - 1)) * s1 - hmean(M(iz - k + 1), M(iz - k)) * s2;
+        s = accd(s, cf->c0[(k - 1) + no * (noz - 1)], t);
+    }
+    return s * (1.0f / (g.dz * g.dz));
+}
+
+// Dx(mat * Dz u)  (diffzx at one output cell)
+template <class R>
+__device__ float zx_pt(const Geo &g, const Coefs *cf, const float *mat, const R &u, int iz, int ix)
+{
+    const int no = g.norder;
+    int nox = imin(imin(ix + g.npml - 1, g.nx + g.npml - ix), no);
+    float s = 0.0f;
+    for (int k = 1; k <= nox; ++k) {
+        float vp = mat[cell(g, iz, ix + k)] * cdz_sum(g, cf, u, iz, ix + k) / g.dz;
+        float vm = mat[cell(g, iz, ix - k)] * cdz_sum(g, cf, u, iz, ix - k) / g.dz;
+        s = accd(s, cf->c1[(k - 1) + no * (nox - 1)], vp - vm);
+    }
+    return s / g.dx;
+}
+// Dz(mat * Dx u)  (diffxz at one output cell)
+template <class R>
+__device__ float xz_pt(const Geo &g, const Coefs *cf, const float *mat, const R &u, int iz, int ix)
+{
+    const int no = g.norder;
+    int noz = imin(imin(iz - 1, g.nz + g.npml - iz), no);
+    float s = 0.0f;
+    for (int k = 1; k <= noz; ++k) {
+        float vp = mat[cell(g, iz + k, ix)] * cdx_sum(g, cf, u, iz + k, ix) / g.dx;
+        float vm = mat[cell(g, iz - k, ix)] * cdx_sum(g, cf, u, iz - k, ix) / g.dx;
+        s = accd(s, cf->c1[(k - 1) + no * (noz - 1)], vp - vm);
+    }
+    return s / g.dz;
+}
+
+// The six RHS parts {Xx, Zx, Mx, Xz, Zz, Mz} at one cell (free surface at iz = 1).
+template <class R>
+__device__ void eval_parts(const Geo &g, const Coefs *cf, const float *lam, const float *mu,
+                           const R &ux, const R &uz, int iz, int ix, float p[6])
+{
+    if (iz == 1) {
+        // half-cell traction-free closure of the surface row
+        size_t a = cell(g, 1, ix), b = cell(g, 2, ix);
+        float sdz2 = 2.0f / (g.dz * g.dz);
+        float M1 = lam[a] + 2.0f * mu[a], M2 = lam[b] + 2.0f * mu[b];
+        p[0] = x2_mode(g, cf, lam, mu, 2, ux, iz, ix);
+        p[1] = (hmean(mu[a], mu[b]) * (ux(2, ix) - ux(1, ix))) * sdz2;
+        float vm1 = mu[a] * cdx_sum(g, cf, uz, 1, ix) / g.dx;
+        float vm2 = mu[b] * cdx_sum(g, cf, uz, 2, ix) / g.dx;
+        p[2] = (vm1 + vm2) / g.dz;
+        p[3] = 0.0f;
+        p[4] = (hmean(M1, M2) * (uz(2, ix) - uz(1, ix))) * sdz2;
+        float vl1 = lam[a] * cdx_sum(g, cf, ux, 1, ix) / g.dx;
+        float vl2 = lam[b] * cdx_sum(g, cf, ux, 2, ix) / g.dx;
+        p[5] = (vl1 + vl2) / g.dz;
+        return;
+    }
+    p[0] = x2_mode(g, cf, lam, mu, 0, ux, iz, ix);
+    p[1] = z2_mode(g, cf, lam, mu, 1, ux, iz, ix);
+    p[2] = zx_pt(g, cf, lam, uz, iz, ix) + xz_pt(g, cf, mu, uz, iz, ix);
+    p[3] = x2_mode(g, cf, lam, mu, 1, uz, iz, ix);
+    p[4] = z2_mode(g, cf, lam, mu, 0, uz, iz, ix);
+    p[5] = zx_pt(g, cf, mu, ux, iz, ix) + xz_pt(g, cf, lam, ux, iz, ix);
+}
+
+__device__ __forceinline__ int strip_of(const Geo &g, int iz, int ix)
+{
+    if (iz >= 1 && iz <= g.nz) {
+        if (ix <= 0) return S_L;
+        if (ix > g.nx) return S_R;
+        return S_NONE;
+    }
+    return S_B;
+}
+__device__ __forceinline__ size_t strip_off(const Geo &g, int s, int iz, int ix)
+{
+    if (s == S_L) return (size_t)(ix - (1 - g.npml)) * g.nz + (size_t)(iz - 1);
+    if (s == S_R) return (size_t)(ix - (g.nx + 1)) * g.nz + (size_t)(iz - 1);
+    return (size_t)(ix - (1 - g.npml)) * g.npml + (size_t)(iz - (g.nz + 1));
+}
+
+struct PsiRd {
+    const Strips *st;
+    const Geo *g;
+    int comp, zdir;
+    __device__ float operator()(int iz, int ix) const
+    {
+        int s = strip_of(*g, iz, ix);
+        if (s == S_NONE) return 0.0f;
+        if (zdir) {
+            if (s != S_B) return 0.0f;
+            return st->psz[s][comp][strip_off(*g, s, iz, ix)];
+        }
+        return st->psx[s][comp][strip_off(*g, s, iz, ix)];
+    }
+};
+
+// ---------------------------------------------------------------------
+// boundary kernel: all updated cells outside the interior rectangle
+// ---------------------------------------------------------------------
+
+__device__ __forceinline__ bool band_cell(const Geo &g, const Bands &b, long t, int &iz, int &ix)
+{
+    const int xfirst = 2 - g.npml;
+    if (t < b.ntop) {
+        int rows = g.iz0 - 1;
+        ix = xfirst + (int)(t / rows); iz = 1 + (int)(t % rows);
+        return true;
+    }
+    t -= b.ntop;
+    if (t < b.nbot) {
+        int rows = g.nz + g.npml - 1 - g.iz1;
+        ix = xfirst + (int)(t / rows); iz = g.iz1 + 1 + (int)(t % rows);
+        return true;
+    }
+    t -= b.nbot;
+    int rows = g.iz1 - g.iz0 + 1;
+    if (t < b.nleft) {
+        ix = xfirst + (int)(t / rows); iz = g.iz0 + (int)(t % rows);
+        return true;
+    }
+    t -= b.nleft;
+    if (t < b.nright) {
+        ix = g.ix1 + 1 + (int)(t / rows); iz = g.iz0 + (int)(t % rows);
+        return true;
+    }
+    return false;
+}
+
+__global__ __launch_bounds__(256) void k_boundary(StepArgs a, Bands b)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int iz, ix;
+    if (!band_cell(a.g, b, t, iz, ix)) return;
+    const Geo &g = a.g;
+    const Coefs *cf = a.cf;
+    FieldRd rx{a.ux, &g}, rz{a.uz, &g};
+    float p[6];
+    eval_parts(g, cf, a.lam, a.mu, rx, rz, iz, ix, p);
+    const int s = strip_of(g, iz, ix);
+    const bool surf = (iz == 1);
+    if (s != S_NONE) {
+        // PML memory corrections -Dx(Mx psi_x), -Dz(Mz psi_z)
+        PsiRd px0{&a.st, &g, 0, 0}, px1{&a.st, &g, 1, 0}, pz0{&a.st, &g, 0, 1}, pz1{&a.st, &g, 1, 1};
+        const int no = g.norder;
+        int nox = imin(imin(ix + g.npml - 1, g.nx + g.npml - ix), no);
+        int noz = imin(imin(iz - 1, g.nz + g.npml - iz), no);
+        float sx0 = 0.0f, sx1 = 0.0f, sz0 = 0.0f, sz1 = 0.0f;
+        for (int k = 1; k <= nox; ++k) {
+            size_t ca = cell(g, iz, ix + k), cb = cell(g, iz, ix - k);
+            float ma = surf ? meff_of(a.lam[ca], a.mu[ca]) : a.lam[ca] + 2.0f * a.mu[ca];
+            float mb = surf ? meff_of(a.lam[cb], a.mu[cb]) : a.lam[cb] + 2.0f * a.mu[cb];
+            double c = cf->c1[(k - 1) + no * (nox - 1)];
+            sx0 = accd(sx0, c, ma * px0(iz, ix + k) - mb * px0(iz, ix - k));
+            sx1 = accd(sx1, c, a.mu[ca] * px1(iz, ix + k) - a.mu[cb] * px1(iz, ix - k));
+        }
+        for (int k = 1; k <= noz; ++k) {
+            size_t ca = cell(g, iz + k, ix), cb = cell(g, iz - k, ix);
+            double c = cf->c1[(k - 1) + no * (noz - 1)];
+            sz0 = accd(sz0, c, a.mu[ca] * pz0(iz + k, ix) - a.mu[cb] * pz0(iz - k, ix));
+            sz1 = accd(sz1, c, (a.lam[ca] + 2.0f * a.mu[ca]) * pz1(iz + k, ix) -
+                                   (a.lam[cb] + 2.0f * a.mu[cb]) * pz1(iz - k, ix));
+        }
+        p[0] = p[0] - sx0 / g.dx;
+        if (!surf) {
+            p[3] = p[3] - sx1 / g.dx;
+            p[1] = p[1] - sz0 / g.dz;
+            p[4] = p[4] - sz1 / g.dz;
+        }
+    }
+    if (a.bg.valid) {
+        int arr, which;
+        long o = bg_off(g, iz, ix, a.it, arr, which);
+        if (o >= 0) {
+            // inside ring adds L[u0 outside], outside ring subtracts L[u0 inside]
+            int other = 1 - which;
+            MaskRd mx{&a.bg, &g, a.it, other, 0}, mz{&a.bg, &g, a.it, other, 1};
+            float q[6];
+            eval_parts(g, cf, a.lam, a.mu, mx, mz, iz, ix, q);
+            if (which == 0)
+                for (int m = 0; m < 6; ++m) p[m] = p[m] + q[m];
+            else
+                for (int m = 0; m < 6; ++m) p[m] = p[m] - q[m];
+        }
+    }
+    const size_t c = cell(g, iz, ix);
+    const float bf = a.bf[c];
+    if (s == S_NONE) {
+        float rxs = (p[0] + p[1]) + p[2], rzs = (p[3] + p[4]) + p[5];
+        a.uxp[c] = (2.0f * a.ux[c] - a.uxp[c]) + bf * rxs;
+        a.uzp[c] = (2.0f * a.uz[c] - a.uzp[c]) + bf * rzs;
+        return;
+    }
+    const size_t o = strip_off(g, s, iz, ix);
+    const float ax = a.st.ax[ix + g.npml - 1];
+    const float az = a.st.az[iz - 1];
+    const float c3 = 2.0f - 4.0f * ax * az;
+    for (int comp = 0; comp < 2; ++comp) {
+        float *b1c = a.st.cur[s][comp][0], *b2c = a.st.cur[s][comp][1], *b3c = a.st.cur[s][comp][2];
+        float *b1p = a.st.prev[s][comp][0], *b2p = a.st.prev[s][comp][1], *b3p = a.st.prev[s][comp][2];
+        float X = p[3 * comp], Z = p[3 * comp + 1], Mm = p[3 * comp + 2];
+        float n1 = (2.0f * b1c[o] - (1.0f - ax) * b1p[o] + bf * X) / (1.0f + ax);
+        float n2 = (2.0f * b2c[o] - (1.0f - az) * b2p[o] + bf * Z) / (1.0f + az);
+        float n3 = (c3 * b3c[o] - (1.0f - ax - az) * b3p[o] + bf * Mm) / (1.0f + ax + az);
+        b1p[o] = n1; b2p[o] = n2; b3p[o] = n3;
+        float tot = (n1 + n2) + n3;
+        if (comp == 0) a.uxp[c] = tot; else a.uzp[c] = tot;
+    }
+}
+
+// ---------------------------------------------------------------------
+// PML first-derivative memories (phase A of the step)
+// ---------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_psi(StepArgs a)
+{
+    const Geo &g = a.g;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long nlr = (long)g.npml * g.nz;
+    long nb = (long)g.NX * g.npml;
+    int s, iz, ix;
+    if (t < nlr) { s = S_L; ix = (1 - g.npml) + (int)(t / g.nz); iz = 1 + (int)(t % g.nz); }
+    else if (t < 2 * nlr) { t -= nlr; s = S_R; ix = g.nx + 1 + (int)(t / g.nz); iz = 1 + (int)(t % g.nz); }
+    else if (t < 2 * nlr + nb) { t -= 2 * nlr; s = S_B; ix = (1 - g.npml) + (int)(t / g.npml); iz = g.nz + 1 + (int)(t % g.npml); }
+    else return;
+    const size_t o = strip_off(g, s, iz, ix);
+    FieldRd rx{a.ux, &g}, rz{a.uz, &g};
+    const int jx = ix + g.npml - 1;
+    if (a.st.ax[jx] > 0.0f) {
+        float b = a.st.bx[jx], c = 1.0f - b;
+        float gx = cdx_sum(g, a.cf, rx, iz, ix) / g.dx;
+        float gz = cdx_sum(g, a.cf, rz, iz, ix) / g.dx;
+        a.st.psx[s][0][o] = b * a.st.psx[s][0][o] + c * gx;
+        a.st.psx[s][1][o] = b * a.st.psx[s][1][o] + c * gz;
+    }
+    if (s == S_B && a.st.az[iz - 1] > 0.0f) {
+        float b = a.st.bz[iz - 1], c = 1.0f - b;
+        float gx = cdz_sum(g, a.cf, rx, iz, ix) / g.dz;
+        float gz = cdz_sum(g, a.cf, rz, iz, ix) / g.dz;
+        a.st.psz[s][0][o] = b * a.st.psz[s][0][o] + c * gx;
+        a.st.psz[s][1][o] = b * a.st.psz[s][1][o] + c * gz;
+    }
+}
+
+// ---------------------------------------------------------------------
+// receivers: 7x7 discretised-Dirac interpolation (Submain.f90:3-116)
+// ---------------------------------------------------------------------
+__device__ double dirac_w(double x)
+{
+    const double C1_0 = 1.0, C1_2 = -5.0 / 4.0, C1_3 = -35.0 / 12.0, C1_4 = 21.0 / 4.0, C1_5 = -25.0 / 12.0;
+    const double C2_0 = -4.0, C2_1 = 75.0 / 4.0, C2_2 = -245.0 / 8.0, C2_3 = 545.0 / 24.0,
+                 C2_4 = -63.0 / 8.0, C2_5 = 25.0 / 24.0;
+    const double C3_0 = 18.0, C3_1 = -153.0 / 4.0, C3_2 = 255.0 / 8.0, C3_3 = -313.0 / 24.0,
+                 C3_4 = 21.0 / 8.0, C3_5 = -5.0 / 24.0;
+    double x2 = x * x, x3 = x2 * x;
+    if (x <= 1.0) return C1_0 + x2 * (C1_2 + x2 * C1_4) + x3 * (C1_3 + x2 * C1_5);
+    if (x <= 2.0) return C2_0 + C2_1 * x + x2 * (C2_2 + x2 * C2_4) + x3 * (C2_3 + x2 * C2_5);
+    if (x <= 3.0) return C3_0 + C3_1 * x + x2 * (C3_2 + x2 * C3_4) + x3 * (C3_3 + x2 * C3_5);
+    return 0.0;
+}
+
+
+__global__ __launch_bounds__(64) void k_record(RecArgs r)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= r.nrcv) return;
+    const Geo &g = r.g;
+    const float rx = r.rx[i], rz = r.rz[i], x0 = r.x0, z0 = r.z0, dx = g.dx, dz = g.dz;
+    int irx = (int)floorf((rx - x0) / dx) + 1;
+    int irz = (int)floorf((rz - z0) / dz) + 1;
+    float sx = 0.0f, sz = 0.0f;
+    int xa = irx - 3 > 1 - g.npml ? irx - 3 : 1 - g.npml;
+    int xb = irx + 3 < g.nx + g.npml ? irx + 3 : g.nx + g.npml;
+    int za = irz - 3 > 1 ? irz - 3 : 1;
+    int zb = irz + 3 < g.nz + g.npml ? irz + 3 : g.nz + g.npml;
+    for (int ix = xa; ix <= xb; ++ix) {
+        double x = fabs((double)(x0 + (float)(ix - 1) * dx) - (double)rx) / (double)dx;
+        double bax = dirac_w(x);
+        for (int iz = za; iz <= zb; ++iz) {
+            double z = fabs((double)(z0 + (float)(iz - 1) * dz) - (double)rz) / (double)dz;
+            double ba = bax * dirac_w(z);
+            size_t c = cell(g, iz, ix);
+            sx = (float)((double)sx + ba * (double)r.ux[c]);
+            sz = (float)((double)sz + ba * (double)r.uz[c]);
+        }
+    }
+    r.seisx[(size_t)i * r.nsamp + r.istp] = sx;
+    r.seisz[(size_t)i * r.nsamp + r.istp] = sz;
+}
+
+#pragma clang fp contract(on)
+
+// ---------------------------------------------------------------------
+// interior kernel: full-order fused stencil, marching along x
+// ---------------------------------------------------------------------
+
+template <int NO, int NT>
+struct IntKernel {
+    static constexpr int HZ = 2 * NO - 1;   // z halo lanes on each side
+    static constexpr int TZ = NT - 2 * HZ;  // output rows per block
+    static constexpr int P = 3 * NO;        // register-ring period
+    static constexpr int PAD = NO + 1;      // LDS guard entries on each side
+    static constexpr int WARM = 4 * NO - 2; // warm-up columns before the first output
+
+    struct Smem {
+        float uA[2][NT + 2 * PAD];   // u at column ix+NO (for Vx)
+        float uB[2][NT + 2 * PAD];   // u at column ix (for Fz)
+        float rM[2][NT + 2 * PAD];   // 1/(l+2m), 1/m at column ix
+        float fz[2][NT + 2 * PAD];   // z-fluxes at column ix
+        float vz[2][NT + 2 * PAD];   // m*Dx uz, l*Dx ux at column ix
+    };
+
+    static __device__ __forceinline__ int sl(int ph, int j) { return ((ph + j) % P + P) % P; }
+
+    template <int PH>
+    static __device__ __forceinline__ void step(const IntArgs<NO> &a, Smem &S, int lane, int r, bool rowok,
+                                                bool outrow, int ix, int xs, int xe,
+                                                float (&U)[2][P], float (&FX)[2][P], float (&VX)[2][P],
+                                                float &rMa_prev, float &rMm_prev)
+    {
+        if (ix >= xe) return;  // block-uniform
+        const Geo &g = a.g;
+        const size_t colpitch = (size_t)g.ld;
+        const size_t rowoff = (size_t)(r - 1);
+        auto at = [&](int c) { return (size_t)(c + g.npml - 1) * colpitch + rowoff; };
+        // 1. new u column ix+2NO-1, materials at ix+NO
+        {
+            float nx_ = 0.0f, nz_ = 0.0f, l1 = 1.0f, m1 = 1.0f;
+            if (rowok) {
+                size_t cn = at(ix + 2 * NO - 1), cm = at(ix + NO);
+                nx_ = a.ux[cn]; nz_ = a.uz[cn];
+                l1 = a.lam[cm]; m1 = a.mu[cm];
+            }
+            U[0][sl(PH, 2 * NO - 1)] = nx_;
+            U[1][sl(PH, 2 * NO - 1)] = nz_;
+            float rMa = 1.0f / (l1 + 2.0f * m1), rMm = 1.0f / m1;
+            if (ix >= xs - 2 * NO) {
+                // x-fluxes at half-point h = ix+NO-1 (between h and h+1)
+                float ha = 1.0f / (rMa_prev + rMa), hm = 1.0f / (rMm_prev + rMm);
+                float s0 = 0.0f, s1 = 0.0f;
+#pragma unroll
+                for (int j = 1; j <= NO; ++j) {
+                    s0 += a.k.ai[j - 1] * (U[0][sl(PH, NO - 1 + j)] - U[0][sl(PH, NO - j)]);
+                    s1 += a.k.ai[j - 1] * (U[1][sl(PH, NO - 1 + j)] - U[1][sl(PH, NO - j)]);
+                }
+                FX[0][sl(PH, NO - 1)] = ha * s0;
+                FX[1][sl(PH, NO - 1)] = hm * s1;
+                // stage A: share u at ix+NO and ix, and 1/M at ix
+                S.uA[0][PAD + lane] = U[0][sl(PH, NO)];
+                S.uA[1][PAD + lane] = U[1][sl(PH, NO)];
+                S.uB[0][PAD + lane] = U[0][sl(PH, 0)];
+                S.uB[1][PAD + lane] = U[1][sl(PH, 0)];
+            }
+            rMa_prev = rMa;
+            rMm_prev = rMm;
+            if (ix < xs - 2 * NO) return;
+            // materials at column ix
+            float l0 = 1.0f, m0 = 1.0f;
+            if (rowok) {
+                size_t c0 = at(ix);
+                l0 = a.lam[c0]; m0 = a.mu[c0];
+            }
+            float rMa0 = 1.0f / (l0 + 2.0f * m0), rMm0 = 1.0f / m0;
+            S.rM[0][PAD + lane] = rMa0;
+            S.rM[1][PAD + lane] = rMm0;
+            __syncthreads();
+            // Vx at column ix+NO: l*Dz uz and m*Dz ux
+            {
+                float dzz = 0.0f, dzx = 0.0f;
+#pragma unroll
+                for (int j = 1; j <= NO; ++j) {
+                    dzz += a.k.bi[j - 1] * (S.uA[1][PAD + lane + j] - S.uA[1][PAD + lane - j]);
+                    dzx += a.k.bi[j - 1] * (S.uA[0][PAD + lane + j] - S.uA[0][PAD + lane - j]);
+                }
+                VX[0][sl(PH, NO)] = l1 * dzz;
+                VX[1][sl(PH, NO)] = m1 * dzx;
+            }
+            if (ix < xs) {
+                __syncthreads();  // keep the two-barrier cadence uniform
+                return;
+            }
+            // z-fluxes at (r+1/2, ix) and Vz at (r, ix)
+            float sfx = 0.0f, sfz = 0.0f;
+#pragma unroll
+            for (int j = 1; j <= NO; ++j) {
+                sfx += a.k.ai[j - 1] * (S.uB[0][PAD + lane + j] - S.uB[0][PAD + lane + 1 - j]);
+                sfz += a.k.ai[j - 1] * (S.uB[1][PAD + lane + j] - S.uB[1][PAD + lane + 1 - j]);
+            }
+            float fzx = sfx / (rMm0 + S.rM[1][PAD + lane + 1]);
+            float fzz = sfz / (rMa0 + S.rM[0][PAD + lane + 1]);
+            float dxz = 0.0f, dxx = 0.0f;
+#pragma unroll
+            for (int k = 1; k <= NO; ++k) {
+                dxz += a.k.bi[k - 1] * (U[1][sl(PH, k)] - U[1][sl(PH, -k)]);
+                dxx += a.k.bi[k - 1] * (U[0][sl(PH, k)] - U[0][sl(PH, -k)]);
+            }
+            __syncthreads();  // all reads of uA/uB/rM for this column are done
+            S.fz[0][PAD + lane] = fzx;
+            S.fz[1][PAD + lane] = fzz;
+            S.vz[0][PAD + lane] = m0 * dxz;
+            S.vz[1][PAD + lane] = l0 * dxx;
+            __syncthreads();
+            if (outrow) {
+                float Xx = 0.0f, Xz = 0.0f, Zx = 0.0f, Zz = 0.0f, Mx = 0.0f, Mz = 0.0f;
+#pragma unroll
+                for (int k = 1; k <= NO; ++k) {
+                    Xx += a.k.aox[k - 1] * (FX[0][sl(PH, k - 1)] - FX[0][sl(PH, -k)]);
+                    Xz += a.k.aox[k - 1] * (FX[1][sl(PH, k - 1)] - FX[1][sl(PH, -k)]);
+                    Zx += a.k.aoz[k - 1] * (S.fz[0][PAD + lane + k - 1] - S.fz[0][PAD + lane - k]);
+                    Zz += a.k.aoz[k - 1] * (S.fz[1][PAD + lane + k - 1] - S.fz[1][PAD + lane - k]);
+                    Mx += a.k.bo[k - 1] * ((VX[0][sl(PH, k)] - VX[0][sl(PH, -k)]) +
+                                          (S.vz[0][PAD + lane + k] - S.vz[0][PAD + lane - k]));
+                    Mz += a.k.bo[k - 1] * ((VX[1][sl(PH, k)] - VX[1][sl(PH, -k)]) +
+                                          (S.vz[1][PAD + lane + k] - S.vz[1][PAD + lane - k]));
+                }
+                size_t c = at(ix);
+                float bf = a.bf[c];
+                float upx = a.uxp[c], upz = a.uzp[c];
+                a.uxp[c] = (2.0f * U[0][sl(PH, 0)] - upx) + bf * ((Xx + Zx) + Mx);
+                a.uzp[c] = (2.0f * U[1][sl(PH, 0)] - upz) + bf * ((Xz + Zz) + Mz);
+            }
+        }
+    }
+
+    static __device__ void run(const IntArgs<NO> &a)
+    {
+        __shared__ Smem S;
+        const Geo &g = a.g;
+        const int lane = threadIdx.x;
+        // zero the LDS guards once
+        for (int i = lane; i < NT + 2 * PAD; i += NT) {
+            if (i < PAD || i >= NT + PAD)
+                for (int q = 0; q < 2; ++q) { S.uA[q][i] = 0.0f; S.uB[q][i] = 0.0f; S.rM[q][i] = 1.0f; S.fz[q][i] = 0.0f; S.vz[q][i] = 0.0f; }
+        }
+        const int bxid = blockIdx.x % a.nbx, bzid = blockIdx.x / a.nbx;
+        const int z0 = g.iz0 + bzid * TZ;
+        const int r = z0 - HZ + lane;
+        const bool rowok = (r >= g.iz0 - HZ) && (r <= g.iz1 + HZ);
+        const bool outrow = (lane >= HZ) && (lane < HZ + TZ) && (r <= g.iz1);
+        const int xs = g.ix0 + bxid * a.lx;
+        const int xe = min(xs + a.lx, g.ix1 + 1);
+        float U[2][P], FX[2][P], VX[2][P];
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int i = 0; i < P; ++i) { U[q][i] = 0.0f; FX[q][i] = 0.0f; VX[q][i] = 0.0f; }
+        float rMa_prev = 1.0f, rMm_prev = 1.0f;
+        __syncthreads();
+        for (int base = xs - WARM; base < xe; base += P) {
+            step<0>(a, S, lane, r, rowok, outrow, base + 0, xs, xe, U, FX, VX, rMa_prev, rMm_prev);
+            if constexpr (P > 1) step<1>(a, S, lane, r, rowok, outrow, base + 1, xs, xe, U, FX, VX, rMa_prev, rMm_prev);
+            if constexpr (P > 2) step<2>(a, S, lane, r, rowok, outrow, base + 2, xs, xe, U, FX, VX, rMa_prev, rMm_prev);
+            if constexpr (P > 3) step<3>(a, S, lane, r, rowok, outrow, base + 3, xs, xe, U, FX, VX, rMa_prev, rMm_prev);
+            if constexpr (P > 4) step<4>(a, S, lane, r, rowok, outrow, base + 4, xs, xe, U, FX, VX, rMa_prev, rMm_prev);
+            if constexpr (P > 5) step<5>(a, S, lane, r, rowok, outrow, base + 5, xs, xe, U, FX, VX, rMa_prev, rMm_prev);
+            if constexpr (P > 6) step<6>(a, S, lane, r, rowok, outrow, base + 6, xs, xe, U, FX, VX, rMa_prev, rMm_prev);
+            if constexpr (P > 7) step<7>(a, S, lane, r, rowok, outrow, base + 7, xs, xe, U, FX, VX, rMa_prev, rMm_prev);
+            if constexpr (P > 8) step<8>(a, S, lane, r, rowok, outrow, base + 8, xs, xe, U, FX, VX, rMa_prev, rMm_prev);
+            if constexpr (P > 9) step<9>(a, S, lane, r, rowok, outrow, base + 9, xs, xe, U, FX, VX, rMa_prev, rMm_prev);
+            if constexpr (P > 10) step<10>(a, S, lane, r, rowok, outrow, base + 10, xs, xe, U, FX, VX, rMa_prev, rMm_prev);
+            if constexpr (P > 11) step<11>(a, S, lane, r, rowok, outrow, base + 11, xs, xe, U, FX, VX, rMa_prev, rMm_prev);
+        }
+    }
+};
+
+template <int NO, int NT>
+__global__ __launch_bounds__(NT) void k_interior(IntArgs<NO> a)
+{
+    IntKernel<NO, NT>::run(a);
+}
+
+
+// ---------------------------------------------------------------------
+// layout conversion helpers
+// ---------------------------------------------------------------------
+// reference (1:3, NZ, NX) -> bf = dt^2/rho, lam, mu in the padded layout
+__global__ void k_import_material(Geo g, const float *c, float dt2, float *bf, float *lam, float *mu)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long n = (long)g.NX * g.NZ;
+    if (t >= n) return;
+    int j = (int)(t / g.NZ), k = (int)(t % g.NZ);
+    size_t d = (size_t)j * g.ld + k;
+    bf[d] = dt2 / c[3 * t + 0];
+    lam[d] = c[3 * t + 1];
+    mu[d] = c[3 * t + 2];
+}
+
+// reference (1:2, NZ, NX) <-> two padded arrays (level 1 -> cur, level 2 -> prev)
+__global__ void k_import_levels(Geo g, const float *ref2, float *cur, float *prev)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long n = (long)g.NX * g.NZ;
+    if (t >= n) return;
+    int j = (int)(t / g.NZ), k = (int)(t % g.NZ);
+    size_t d = (size_t)j * g.ld + k;
+    cur[d] = ref2[2 * t];
+    prev[d] = ref2[2 * t + 1];
+}
+__global__ void k_export_levels(Geo g, const float *cur, const float *prev, float *ref2)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long n = (long)g.NX * g.NZ;
+    if (t >= n) return;
+    int j = (int)(t / g.NZ), k = (int)(t % g.NZ);
+    size_t d = (size_t)j * g.ld + k;
+    ref2[2 * t] = cur[d];
+    ref2[2 * t + 1] = prev[d];
+}
+// strip arrays: reference (1:2, cells) <-> cur/prev
+__global__ void k_split_levels(long n, const float *ref2, float *cur, float *prev)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    cur[t] = ref2[2 * t];
+    prev[t] = ref2[2 * t + 1];
+}
+__global__ void k_join_levels(long n, const float *cur, const float *prev, float *ref2)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    ref2[2 * t] = cur[t];
+    ref2[2 * t + 1] = prev[t];
+}
+
+}  // namespace fdfk

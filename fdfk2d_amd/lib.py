@@ -1,0 +1,220 @@
+"""ctypes binding of the C ABI in include/fdfk.h (fdfk2d_amd/libfdfk.so).
+
+The library is loaded from the package directory; if it is missing the
+import fails loudly -- there is no CPU fallback for the time-stepping path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfdfk.so")
+
+FDFK_FP32, FDFK_BF16, FDFK_FP16 = 0, 1, 2
+
+
+class FdfkJob(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("nz", C.c_int32), ("npml", C.c_int32), ("norder", C.c_int32),
+        ("is_pml_top", C.c_int32), ("npower", C.c_int32),
+        ("x0", C.c_float), ("z0", C.c_float), ("dx", C.c_float), ("dz", C.c_float),
+        ("dt", C.c_double), ("R0", C.c_double), ("vpml", C.c_float),
+        ("nt", C.c_int32), ("nstp", C.c_int32), ("nrcv", C.c_int32),
+        ("rx", C.c_void_p), ("rz", C.c_void_p), ("c", C.c_void_p),
+        ("precision", C.c_int32),
+    ]
+
+
+class FdfkFK(C.Structure):
+    _fields_ = [
+        ("nlayer_left", C.c_int32), ("nlayer_right", C.c_int32),
+        ("fk_left", C.c_void_p), ("fk_right", C.c_void_p),
+        ("src_is_p", C.c_int32), ("ds", C.c_float), ("f0", C.c_float),
+        ("rayp", C.c_double), ("bazs", C.c_void_p),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: run `python -m fdfk2d_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        L.fdfk_last_error.restype = C.c_char_p
+        L.fdfk_device_bytes.restype = C.c_int64
+        L.fdfk_create.argtypes = [C.POINTER(FdfkJob), C.POINTER(C.c_void_p)]
+        for name in ("fdfk_run", "fdfk_profile"):
+            getattr(L, name).argtypes = [C.c_void_p, C.c_int32]
+        for name in ("fdfk_sync", "fdfk_iteration", "fdfk_destroy", "fdfk_device_bytes"):
+            getattr(L, name).argtypes = [C.c_void_p]
+        L.fdfk_destroy.restype = None
+        L.fdfk_set_background.argtypes = [C.c_void_p, C.c_void_p]
+        L.fdfk_get_background.argtypes = [C.c_void_p, C.c_void_p]
+        L.fdfk_compute_background.argtypes = [C.c_void_p, C.POINTER(FdfkFK)]
+        L.fdfk_seismograms.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.fdfk_get_state.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.fdfk_set_state.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.fdfk_kernel_stats.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64),
+                                        C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+        L.updatedispl_.restype = None
+        L.comput_background_wavefield_.restype = None
+        L.fdfk_compat_release.restype = None
+        _lib = L
+    return _lib
+
+
+class FdfkError(RuntimeError):
+    pass
+
+
+def _check(rc):
+    if rc != 0:
+        raise FdfkError(f"fdfk error {rc}: {lib().fdfk_last_error().decode()}")
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def bg_sizes(nx, nz, nbd, nt):
+    """Element counts of the twelve background arrays (src/FDFK2D.f90:330-355)."""
+    T = nt + 1
+    reg_lr, reg_b = (nz - nbd) * nbd * T, nbd * nx * T
+    pml_lr, pml_b = nz * nbd * T, nbd * (nx + 2 * nbd) * T
+    return [reg_lr] * 4 + [reg_b] * 2 + [pml_lr] * 4 + [pml_b] * 2
+
+
+class Solver:
+    """One simulation resident on the GPU (a fdfk_solver handle)."""
+
+    def __init__(self, nx, nz, npml, norder, dx, dz, dt, nt, c, x0=0.0, z0=0.0, nstp=1,
+                 rx=None, rz=None, R0=1e-3, npower=2, vpml=0.0, precision=FDFK_FP32):
+        self.nx, self.nz, self.npml, self.norder, self.nt = nx, nz, npml, norder, nt
+        self.nbd = 2 * norder - 1
+        self.NX, self.NZ = nx + 2 * npml, nz + npml
+        self.nstp = nstp
+        self.nsamp = nt // nstp + 1
+        self._c = np.ascontiguousarray(c, np.float32)
+        assert self._c.size == 3 * self.NX * self.NZ, "c must be (3, NZ, NX) reference layout"
+        self._rx = None if rx is None else np.ascontiguousarray(rx, np.float32)
+        self._rz = None if rz is None else np.ascontiguousarray(rz, np.float32)
+        self.nrcv = 0 if rx is None else self._rx.size
+        job = FdfkJob(nx=nx, nz=nz, npml=npml, norder=norder, is_pml_top=0, npower=npower,
+                      x0=x0, z0=z0, dx=dx, dz=dz, dt=dt, R0=R0, vpml=vpml, nt=nt, nstp=nstp,
+                      nrcv=self.nrcv, rx=_ptr(self._rx), rz=_ptr(self._rz), c=_ptr(self._c),
+                      precision=precision)
+        h = C.c_void_p()
+        _check(lib().fdfk_create(C.byref(job), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fdfk_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def set_background(self, arrays):
+        arrs = [np.ascontiguousarray(a, np.float32) for a in arrays]
+        sizes = bg_sizes(self.nx, self.nz, self.nbd, self.nt)
+        for a, n in zip(arrs, sizes):
+            assert a.size == n, (a.size, n)
+        ptrs = (C.c_void_p * 12)(*[a.ctypes.data for a in arrs])
+        _check(lib().fdfk_set_background(self.h, ptrs))
+
+    def compute_background(self, fk_left, fk_right, src_is_p, f0, ds, rayp, bazs):
+        fl = np.ascontiguousarray(fk_left, np.float32)
+        fr = np.ascontiguousarray(fk_right, np.float32)
+        bz = np.ascontiguousarray(bazs, np.float64)
+        fk = FdfkFK(nlayer_left=fl.size // 4, nlayer_right=fr.size // 4, fk_left=_ptr(fl),
+                    fk_right=_ptr(fr), src_is_p=1 if src_is_p else 0, ds=ds, f0=f0, rayp=rayp,
+                    bazs=_ptr(bz))
+        _check(lib().fdfk_compute_background(self.h, C.byref(fk)))
+
+    def get_background(self):
+        arrs = [np.zeros(n, np.float32) for n in bg_sizes(self.nx, self.nz, self.nbd, self.nt)]
+        ptrs = (C.c_void_p * 12)(*[a.ctypes.data for a in arrs])
+        _check(lib().fdfk_get_background(self.h, ptrs))
+        return arrs
+
+    def run(self, nsteps):
+        _check(lib().fdfk_run(self.h, int(nsteps)))
+
+    def sync(self):
+        _check(lib().fdfk_sync(self.h))
+
+    @property
+    def iteration(self):
+        return lib().fdfk_iteration(self.h)
+
+    def seismograms(self):
+        sx = np.zeros((self.nrcv, self.nsamp), np.float32)
+        sz = np.zeros((self.nrcv, self.nsamp), np.float32)
+        _check(lib().fdfk_seismograms(self.h, _ptr(sx), _ptr(sz)))
+        return sx, sz
+
+    def get_state(self):
+        n = 2 * self.NX * self.NZ
+        ux, uz = np.zeros(n, np.float32), np.zeros(n, np.float32)
+        _check(lib().fdfk_get_state(self.h, _ptr(ux), _ptr(uz)))
+        return ux, uz
+
+    def set_state(self, ux, uz):
+        ux = np.ascontiguousarray(ux, np.float32)
+        uz = np.ascontiguousarray(uz, np.float32)
+        _check(lib().fdfk_set_state(self.h, _ptr(ux), _ptr(uz)))
+
+    def profile(self, on=True):
+        _check(lib().fdfk_profile(self.h, 1 if on else 0))
+
+    def kernel_stats(self):
+        a, b, c, d = C.c_double(), C.c_int64(), C.c_double(), C.c_int64()
+        _check(lib().fdfk_kernel_stats(self.h, C.byref(a), C.byref(b), C.byref(c), C.byref(d)))
+        return dict(interior_ms=a.value, interior_n=b.value, step_ms=c.value, step_n=d.value)
+
+    @property
+    def device_bytes(self):
+        return lib().fdfk_device_bytes(self.h)
+
+
+def _i(v):
+    return C.byref(C.c_int32(int(v)))
+
+
+def _f(v):
+    return C.byref(C.c_float(float(v)))
+
+
+def _d(v):
+    return C.byref(C.c_double(float(v)))
+
+
+def updatedispl(st, it):
+    """Drop-in path: call updatedispl_ exactly as src/FDFK2D.f90:516-522 does,
+    on a reference-layout state (see fdfk2d_amd.refstate)."""
+    bg = st.get("bg")
+    bgp = [_ptr(x) for x in bg] if bg is not None else [None] * 12
+    lib().updatedispl_(
+        _i(st["norder"]), _i(st["nx"]), _i(st["nz"]), _i(st["npml"]), _i(st["izt"]),
+        _i(st["pml_top"]), _i(st["npower"]), _i(st["nbd"]), _i(it), _i(st["nt"]),
+        _f(st["dx"]), _f(st["dz"]), _d(st["dt"]), _f(st["vpml"]), _d(st["R0"]),
+        *bgp, _ptr(st["c"]), _ptr(st["coef0"]), _ptr(st["coef1"]), _ptr(st["ux"]), _ptr(st["uz"]),
+        *[_ptr(x) for x in st["bp"]], *[_ptr(x) for x in st["bu"]])
+
+
+def comput_background_wavefield(nx, nz, npml, izt, nbd, nt, src_is_p, x0, z0, dx, dz, dt, ds,
+                                f0, rayp, bazs, fk_left, fk_right):
+    """Drop-in path for comput_background_wavefield (src/FDFK2D.f90:432-434)."""
+    arrs = [np.zeros(n, np.float32) for n in bg_sizes(nx, nz, nbd, nt)]
+    fl = np.ascontiguousarray(fk_left, np.float32)
+    fr = np.ascontiguousarray(fk_right, np.float32)
+    bz = np.ascontiguousarray(bazs, np.float64)
+    lib().comput_background_wavefield_(
+        _i(nx), _i(nz), _i(npml), _i(izt), _i(nbd), _i(nt), _i(fl.size // 4), _i(fr.size // 4),
+        _i(1 if src_is_p else 0), _f(x0), _f(z0), _f(dx), _f(dz), _d(dt), _f(ds), _f(f0),
+        _d(rayp), _ptr(bz), _ptr(fl), _ptr(fr), *[_ptr(a) for a in arrs])
+    return arrs

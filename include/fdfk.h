@@ -1,0 +1,135 @@
+/*
+ * fdfk.h -- C ABI of the MI355X (gfx950) FDFK2D time-stepping library.
+ *
+ * Plain C types only (no torch, no HIP types).  Every function returns 0 on
+ * success or a negative errno-style code; fdfk_last_error() gives the text of
+ * the last failure on the calling thread.  A solver handle is not
+ * thread-safe; separate handles may be used concurrently.
+ *
+ * Array layouts are the reference's Fortran ones (column-major, z fastest):
+ *   c     (1:3, izt:nz+npml, 1-npml:nx+npml)   rho, lambda, mu       (FP32)
+ *   ux/uz (1:2, izt:nz+npml, 1-npml:nx+npml)   level 1 = time n, level 2 = n-1
+ *   seisx/seisz (0:nsis, 1:nrcv)               time fastest per receiver
+ *   the twelve background arrays uxl_reg..uzb_pml with the shapes of
+ *   src/FDFK2D.f90:330-355 (time slowest, 0:nt).
+ * Only the free-surface configuration (is_PML_top = .false., izt = 1) is
+ * supported by this build.
+ */
+#ifndef FDFK_H
+#define FDFK_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FDFK_OK 0
+#define FDFK_EINVAL (-22)
+#define FDFK_ENOMEM (-12)
+#define FDFK_EDEVICE (-5)
+#define FDFK_ERANGE (-34)
+#define FDFK_ENOSYS (-38)
+
+/* storage precision of wavefield and material arrays (arithmetic is FP32) */
+#define FDFK_FP32 0
+#define FDFK_BF16 1
+#define FDFK_FP16 2
+
+typedef struct fdfk_solver fdfk_solver;
+
+/* One simulation: the scalars of src/Input.f90:get_parameter plus the arrays
+ * of read_data, i.e. the state simulate_wavefield (src/FDFK2D.f90:159) sets up. */
+typedef struct fdfk_job {
+    int32_t nx, nz, npml, norder;   /* norder = FD order / 2 (Input.f90:130-131) */
+    int32_t is_pml_top;             /* must be 0 */
+    int32_t npower;                 /* PML profile power (FDFK2D.f90:417: 2) */
+    float x0, z0, dx, dz;
+    double dt;
+    double R0;                      /* PML reflection coefficient (FDFK2D.f90:421) */
+    float vpml;                     /* <= 0: max Vp of c (FDFK2D.f90:416) */
+    int32_t nt;                     /* iterations it = 0..nt (FDFK2D.f90:311) */
+    int32_t nstp;                   /* record every nstp iterations (FDFK2D.f90:313) */
+    int32_t nrcv;
+    const float *rx, *rz;           /* receiver coordinates (m) */
+    const float *c;                 /* rho, lambda, mu (reference layout) */
+    int32_t precision;              /* FDFK_FP32 / FDFK_BF16 / FDFK_FP16 */
+} fdfk_job;
+
+/* FK background description: the inputs of comput_background_wavefield
+ * (src/FDFK2D.f90:432-434). */
+typedef struct fdfk_fk {
+    int32_t nlayer_left, nlayer_right;
+    const float *fk_left, *fk_right; /* (4, nlayer): ztop, vp, vs, rho per layer */
+    int32_t src_is_p;                /* P (1) or SV (0) incidence */
+    float ds, f0;                    /* source strength (m) and frequency (Hz) */
+    double rayp;                     /* s/m (|rayp| >= 1e-9) */
+    const double *bazs;              /* back-azimuth per padded column (rad), NX values */
+} fdfk_fk;
+
+/* --- handle API: state stays resident on the GPU between calls --- */
+/* replaces simulate_wavefield's allocation + setup (src/FDFK2D.f90:253-421) */
+int fdfk_create(const fdfk_job *job, fdfk_solver **out);
+/* upload a host background (twelve reference-layout arrays, nt+1 steps each) */
+int fdfk_set_background(fdfk_solver *s, const float *const bg[12]);
+/* compute the FK background on the GPU (FP64), replaces
+ * comput_background_wavefield (src/FDFK2D.f90:432-434) */
+int fdfk_compute_background(fdfk_solver *s, const fdfk_fk *fk);
+/* copy the device background back in the reference layout (twelve arrays) */
+int fdfk_get_background(fdfk_solver *s, float *const bg[12]);
+/* advance nsteps iterations of the loop src/FDFK2D.f90:465-532 (record,
+ * then updatedispl); recording stays on the device */
+int fdfk_run(fdfk_solver *s, int32_t nsteps);
+/* next iteration index it (0..nt+1) */
+int fdfk_iteration(const fdfk_solver *s);
+/* seismograms (0:nsis, 1:nrcv), before the cos(baz) projection of
+ * src/FDFK2D.f90:537-543 */
+int fdfk_seismograms(fdfk_solver *s, float *seisx, float *seisz);
+/* current two-level fields in the reference layout */
+int fdfk_get_state(fdfk_solver *s, float *ux, float *uz);
+int fdfk_set_state(fdfk_solver *s, const float *ux, const float *uz);
+/* wait for all queued work of this handle */
+int fdfk_sync(fdfk_solver *s);
+/* kernel timing of the interior stencil kernel (HIP events on the handle's
+ * stream); enable before fdfk_run, read after fdfk_sync */
+int fdfk_profile(fdfk_solver *s, int32_t enable);
+int fdfk_kernel_stats(fdfk_solver *s, double *interior_ms_total, int64_t *interior_launches,
+                      double *step_ms_total, int64_t *steps);
+/* bytes of device memory held by the handle */
+int64_t fdfk_device_bytes(const fdfk_solver *s);
+void fdfk_destroy(fdfk_solver *s);
+const char *fdfk_last_error(void);
+
+/* --- drop-in compatibility: the exact external procedures the reference
+ * driver links (all arguments by reference, gfortran/flang mangling).  Device
+ * mirrors are keyed by the host ux pointer; every call copies level 1/2 back
+ * because the reference driver reads ux(1,...) on the host each iteration. --- */
+/* replaces updatedispl (called at src/FDFK2D.f90:516-522) */
+void updatedispl_(const int32_t *norder, const int32_t *nx, const int32_t *nz,
+    const int32_t *npml, const int32_t *izt, const int32_t *is_pml_top, const int32_t *npower,
+    const int32_t *nbd, const int32_t *it, const int32_t *nt, const float *dx, const float *dz,
+    const double *dt, const float *vpml, const double *R0,
+    const float *uxl_reg, const float *uzl_reg, const float *uxr_reg, const float *uzr_reg,
+    const float *uxb_reg, const float *uzb_reg, const float *uxl_pml, const float *uzl_pml,
+    const float *uxr_pml, const float *uzr_pml, const float *uxb_pml, const float *uzb_pml,
+    const float *c, const double *coef0, const double *coef1, float *ux, float *uz,
+    float *bpxlx, float *bpzlx, float *bpxrx, float *bpzrx, float *bpxtx, float *bpxtz,
+    float *bpztx, float *bpztz, float *bpxbx, float *bpxbz, float *bpzbx, float *bpzbz,
+    float *bux1l, float *bux2l, float *bux3l, float *buz1l, float *buz2l, float *buz3l,
+    float *bux1r, float *bux2r, float *bux3r, float *buz1r, float *buz2r, float *buz3r,
+    float *bux1b, float *bux2b, float *bux3b, float *buz1b, float *buz2b, float *buz3b,
+    float *bux1t, float *bux2t, float *bux3t, float *buz1t, float *buz2t, float *buz3t);
+/* replaces comput_background_wavefield (called at src/FDFK2D.f90:432-434) */
+void comput_background_wavefield_(const int32_t *nx, const int32_t *nz, const int32_t *npml,
+    const int32_t *izt, const int32_t *nbd, const int32_t *nt, const int32_t *nlayer_left,
+    const int32_t *nlayer_right, const int32_t *src_type, const float *x0, const float *z0,
+    const float *dx, const float *dz, const double *dt, const float *ds, const float *f0,
+    const double *rayp, const double *bazs, const float *fk_left, const float *fk_right,
+    float *uxl_reg, float *uzl_reg, float *uxr_reg, float *uzr_reg, float *uxb_reg,
+    float *uzb_reg, float *uxl_pml, float *uzl_pml, float *uxr_pml, float *uzr_pml,
+    float *uxb_pml, float *uzb_pml);
+/* releases the compatibility mirrors (optional) */
+void fdfk_compat_release(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

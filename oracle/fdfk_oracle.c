@@ -1,0 +1,991 @@
+/*
+ * CPU oracle for the FDFK2D time-stepping path -- TEST INFRASTRUCTURE ONLY.
+ * See fdfk_oracle.h for what is a literal restatement of reference source
+ * (pinned bit-for-bit against oracle/_ref) and what is this project's own
+ * formulation of the binary-only parts (pinned by invariants in tests/).
+ *
+ * Floating-point semantics: Fortran evaluates `mysum + coef(k,n)*(a-b)` with
+ * mysum, a, b REAL(4) and coef REAL(8) as: (a-b) in FP32, product and sum in
+ * FP64, result rounded to FP32 on assignment.  The helpers below reproduce
+ * that exactly; this file must be compiled without FP contraction
+ * (-ffp-contract=off) so no FMA changes the roundings.
+ */
+#include "fdfk_oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdint.h>
+
+#define CO(coef, j, i) (coef)[((j) - 1) + norder * ((i) - 1)]
+static inline int imin(int a, int b) { return a < b ? a : b; }
+static inline int imax(int a, int b) { return a > b ? a : b; }
+/* Fortran REAL(4) accumulate with a REAL(8) coefficient */
+static inline float acc(float s, double c, float d) { return (float)((double)s + c * (double)d); }
+
+/* ------------------------------------------------------------------ */
+/* FD coefficients, src/Submain.f90:1111-1243                          */
+/* ------------------------------------------------------------------ */
+void orc_fd_coef0(int norder, double *coef)
+{
+    memset(coef, 0, sizeof(double) * norder * norder);
+    for (int i = 1; i <= norder; ++i)
+        for (int j = 1; j <= i; ++j) {
+            double t1 = 1.0, t2 = 1.0, t3 = 1.0;
+            for (int k = 1; k <= j - 1; ++k) {
+                t1 = t1 * (double)((2 * k - 1) * (2 * k - 1));
+                t2 = t2 * ((double)((2 * j - 1) * (2 * j - 1)) - (double)((2 * k - 1) * (2 * k - 1)));
+            }
+            for (int k = j + 1; k <= i; ++k) {
+                t1 = t1 * (double)((2 * k - 1) * (2 * k - 1));
+                t3 = t3 * ((double)((2 * k - 1) * (2 * k - 1)) - (double)((2 * j - 1) * (2 * j - 1)));
+            }
+            t1 = t1 * (double)((j + 1) % 2 == 0 ? 1 : -1);
+            t2 = t2 * (double)(2 * j - 1);
+            CO(coef, j, i) = t1 / t2 / t3;
+        }
+}
+
+void orc_fd_coef1(int norder, double *coef)
+{
+    memset(coef, 0, sizeof(double) * norder * norder);
+    for (int i = 1; i <= norder; ++i)
+        for (int j = 1; j <= i; ++j) {
+            double t1 = 1.0, t2 = 1.0, t3 = 1.0;
+            for (int k = 1; k <= j - 1; ++k) {
+                t1 = t1 * (double)(k * k);
+                t2 = t2 * ((double)(j * j) - (double)(k * k));
+            }
+            for (int k = j + 1; k <= i; ++k) {
+                t1 = t1 * (double)(k * k);
+                t3 = t3 * ((double)(k * k) - (double)(j * j));
+            }
+            t1 = t1 * (double)((j + 1) % 2 == 0 ? 1 : -1);
+            t2 = t2 * (2.0 * (double)j);
+            CO(coef, j, i) = t1 / t2 / t3;
+        }
+}
+
+void orc_fd_coef2(int norder, double *coef)
+{
+    memset(coef, 0, sizeof(double) * norder * norder);
+    for (int i = 1; i <= norder; ++i)
+        for (int j = 1; j <= i; ++j) {
+            double t1 = 1.0, t2 = 1.0, t3 = 1.0;
+            for (int k = 1; k <= j - 1; ++k) {
+                t1 = t1 * (double)(k * k);
+                t2 = t2 * ((double)(j * j) - (double)(k * k));
+            }
+            for (int k = j + 1; k <= i; ++k) {
+                t1 = t1 * (double)(k * k);
+                t3 = t3 * ((double)(k * k) - (double)(j * j));
+            }
+            t1 = t1 * (double)((j + 1) % 2 == 0 ? 1 : -1);
+            t2 = t2 * (double)(j * j);
+            CO(coef, j, i) = t1 / t2 / t3;
+        }
+}
+
+/* src/Submain.f90:118-164 */
+int orc_check_stability(int nx, int nz, int npml, int izt, int norder, float dx, float dz,
+                        double dt, const double *coef2, const float *c, float *dt_stable)
+{
+    float h = dx > dz ? dx : dz;
+    float cx = (h / dx) * (h / dx);
+    float cz = (h / dz) * (h / dz);
+    double sa = 0.0, s = 0.0;
+    for (int j = 1; j <= norder; ++j) sa += fabs(CO(coef2, j, norder));
+    for (int j = 1; j <= norder; ++j) s += CO(coef2, j, norder);
+    float cfl = (float)sqrt(2.0 / ((double)(cx + cz) * (sa + s)));
+    long ncell = (long)(nz + npml - izt + 1) * (long)(nx + 2 * npml);
+    float vpmax = -INFINITY;
+    for (long i = 0; i < ncell; ++i) {
+        float v = sqrtf((c[3 * i + 1] + 2.0f * c[3 * i + 2]) / c[3 * i + 0]);
+        if (v > vpmax) vpmax = v;
+    }
+    float dts = cfl * h / vpmax;
+    *dt_stable = dts;
+    if (dt > (double)dts) return 1;
+    if (dt < (double)(0.4f * dts)) return 2;
+    return 0;
+}
+
+/* 2-D array helpers: a(izt:nz+npml, 1-npml:nx+npml), z fastest */
+#define DIMS int NZ = nz + npml - izt + 1; (void)NZ
+#define A2(a, iz, ix) (a)[(size_t)((iz) - izt) + (size_t)NZ * (size_t)((ix) + npml - 1)]
+
+/* src/Submain.f90:167-212 */
+float orc_diff1(int norder, int dire, int nx, int nz, int npml, int izt, int ix, int iz,
+                float d, const double *coef, const float *u)
+{
+    DIMS;
+    int nox = imin(imin(ix + npml - 1, nx + npml - ix), norder);
+    int noz = imin(imin(iz - izt, nz + npml - iz), norder);
+    int kx = 2 - dire, kz = dire - 1;
+    int node = kx * nox + kz * noz;
+    float s = 0.0f;
+    for (int k = 1; k <= node; ++k) {
+        int lx2 = ix + kx * k, lx1 = ix - kx * k, lz2 = iz + kz * k, lz1 = iz - kz * k;
+        s = acc(s, CO(coef, k, node), A2(u, lz2, lx2) - A2(u, lz1, lx1));
+    }
+    return s / d;
+}
+
+/* src/Submain.f90:215-249; u(1-npml:nx+npml) */
+float orc_diffx1(int norder, int nx, int npml, int ix, float dx, const double *coef, const float *u)
+{
+    int nox = imin(imin(ix + npml - 1, nx + npml - ix), norder);
+    float s = 0.0f;
+    for (int k = 1; k <= nox; ++k)
+        s = acc(s, CO(coef, k, nox), u[ix + k + npml - 1] - u[ix - k + npml - 1]);
+    return s / dx;
+}
+
+/* ------------------------------------------------------------------ */
+/* mixed operators, src/Submain.f90:251-762                            */
+/* ------------------------------------------------------------------ */
+void orc_diffzx_reg(int norder, int nx, int nz, int npml, int izt, float dx, float dz,
+                    const double *coef, const float *mat, const float *u, float *out)
+{
+    DIMS;
+    size_t n = (size_t)NZ * (size_t)(nx + 2 * npml);
+    float *v = (float *)calloc(n, sizeof(float));
+#pragma omp parallel for schedule(dynamic)
+    for (int iz = 2; iz <= nz; ++iz) {
+        int noz = imin(imin(iz - izt, nz + npml - iz), norder);
+        for (int ix = 1 - norder; ix <= nx + norder; ++ix) {
+            float s = 0.0f;
+            for (int k = 1; k <= noz; ++k) s = acc(s, CO(coef, k, noz), A2(u, iz + k, ix) - A2(u, iz - k, ix));
+            A2(v, iz, ix) = A2(mat, iz, ix) * s / dz;
+        }
+    }
+    memset(out, 0, n * sizeof(float));
+#pragma omp parallel for schedule(dynamic)
+    for (int ix = 1; ix <= nx; ++ix) {
+        int nox = imin(imin(ix + npml - 1, nx + npml - ix), norder);
+        for (int iz = 2; iz <= nz; ++iz) {
+            float s = 0.0f;
+            for (int k = 1; k <= nox; ++k) s = acc(s, CO(coef, k, nox), A2(v, iz, ix + k) - A2(v, iz, ix - k));
+            A2(out, iz, ix) = s / dx;
+        }
+    }
+    free(v);
+}
+
+void orc_diffzx_pml(int norder, int nx, int nz, int npml, int izt, float dx, float dz,
+                    const double *coef, const float *mat, const float *u, float *out)
+{
+    DIMS;
+    size_t n = (size_t)NZ * (size_t)(nx + 2 * npml);
+    float *v = (float *)calloc(n, sizeof(float));
+#pragma omp parallel for schedule(dynamic)
+    for (int iz = izt + 1; iz <= nz + npml - 1; ++iz) {
+        int noz = imin(imin(iz - izt, nz + npml - iz), norder);
+        for (int ix = 2 - npml; ix <= nx + npml - 1; ++ix) {
+            /* mask: 1 except (1:nz, norder+1:nx-norder) and row izt */
+            if (iz >= 1 && iz <= nz && ix >= norder + 1 && ix <= nx - norder) continue;
+            float s = 0.0f;
+            for (int k = 1; k <= noz; ++k) s = acc(s, CO(coef, k, noz), A2(u, iz + k, ix) - A2(u, iz - k, ix));
+            A2(v, iz, ix) = A2(mat, iz, ix) * s / dz;
+        }
+    }
+    memset(out, 0, n * sizeof(float));
+#pragma omp parallel for schedule(dynamic)
+    for (int ix = 2 - npml; ix <= nx + npml - 1; ++ix) {
+        int nox = imin(imin(ix + npml - 1, nx + npml - ix), norder);
+        for (int iz = izt + 1; iz <= nz + npml - 1; ++iz) {
+            if (iz >= 1 && iz <= nz && ix >= 1 && ix <= nx) continue;
+            float s = 0.0f;
+            for (int k = 1; k <= nox; ++k) s = acc(s, CO(coef, k, nox), A2(v, iz, ix + k) - A2(v, iz, ix - k));
+            A2(out, iz, ix) = s / dx;
+        }
+    }
+    free(v);
+}
+
+void orc_diffxz_reg(int norder, int nx, int nz, int npml, int izt, float dx, float dz,
+                    const double *coef, const float *mat, const float *u, float *out)
+{
+    DIMS;
+    size_t n = (size_t)NZ * (size_t)(nx + 2 * npml);
+    float *v = (float *)calloc(n, sizeof(float));
+#pragma omp parallel for schedule(dynamic)
+    for (int ix = 1; ix <= nx; ++ix) {
+        int nox = imin(imin(ix + npml - 1, nx + npml - ix), norder);
+        for (int iz = 1; iz <= nz + norder; ++iz) {
+            float s = 0.0f;
+            for (int k = 1; k <= nox; ++k) s = acc(s, CO(coef, k, nox), A2(u, iz, ix + k) - A2(u, iz, ix - k));
+            A2(v, iz, ix) = A2(mat, iz, ix) * s / dx;
+        }
+    }
+    memset(out, 0, n * sizeof(float));
+#pragma omp parallel for schedule(dynamic)
+    for (int iz = 2; iz <= nz; ++iz) {
+        int noz = imin(imin(iz - izt, nz + npml - iz), norder);
+        for (int ix = 1; ix <= nx; ++ix) {
+            float s = 0.0f;
+            for (int k = 1; k <= noz; ++k) s = acc(s, CO(coef, k, noz), A2(v, iz + k, ix) - A2(v, iz - k, ix));
+            A2(out, iz, ix) = s / dz;
+        }
+    }
+    free(v);
+}
+
+void orc_diffxz_pml(int norder, int nx, int nz, int npml, int izt, float dx, float dz,
+                    const double *coef, const float *mat, const float *u, float *out)
+{
+    DIMS;
+    size_t n = (size_t)NZ * (size_t)(nx + 2 * npml);
+    float *v = (float *)calloc(n, sizeof(float));
+#pragma omp parallel for schedule(dynamic)
+    for (int ix = 2 - npml; ix <= nx + npml - 1; ++ix) {
+        int nox = imin(imin(ix + npml - 1, nx + npml - ix), norder);
+        for (int iz = izt; iz <= nz + npml - 1; ++iz) {
+            if (iz >= 1 && iz <= nz - norder && ix >= 1 && ix <= nx) continue;
+            float s = 0.0f;
+            for (int k = 1; k <= nox; ++k) s = acc(s, CO(coef, k, nox), A2(u, iz, ix + k) - A2(u, iz, ix - k));
+            A2(v, iz, ix) = A2(mat, iz, ix) * s / dx;
+        }
+    }
+    memset(out, 0, n * sizeof(float));
+#pragma omp parallel for schedule(dynamic)
+    for (int iz = izt + 1; iz <= nz + npml - 1; ++iz) {
+        int noz = imin(imin(iz - izt, nz + npml - iz), norder);
+        for (int ix = 2 - npml; ix <= nx + npml - 1; ++ix) {
+            if (iz >= 1 && iz <= nz && ix >= 1 && ix <= nx) continue;
+            float s = 0.0f;
+            for (int k = 1; k <= noz; ++k) s = acc(s, CO(coef, k, noz), A2(v, iz + k, ix) - A2(v, iz - k, ix));
+            A2(out, iz, ix) = s / dz;
+        }
+    }
+    free(v);
+}
+
+/* ------------------------------------------------------------------ */
+/* second-derivative operators, src/Submain.f90:764-1109               */
+/* ------------------------------------------------------------------ */
+static inline float hmean(float a, float b) { return 2.0f / (1.0f / a + 1.0f / b); }
+
+/* one output of diffx2 at (iz, ix): the body of the loops at :815-837 */
+static inline float x2_point(int norder, int nx, int npml, int izt, int NZ, const double *coef,
+                             const float *mat, const float *u, int iz, int ix)
+{
+    int nox = imin(imin(ix + npml - 1, nx + npml - ix), norder);
+    float s = 0.0f;
+    for (int k = 1; k <= nox; ++k) {
+        int i = (2 * k - 1) / 2;
+        int n1 = imin(imin(ix + i + npml - 1, nx + npml - ix - i), norder);
+        float s1 = 0.0f;
+        for (int j = 1; j <= n1; ++j) s1 = acc(s1, CO(coef, j, n1), A2(u, iz, ix + j + k - 1) - A2(u, iz, ix - j + k));
+        int n2 = imin(imin(ix - i + npml - 1, nx + npml - ix + i), norder);
+        float s2 = 0.0f;
+        for (int j = 1; j <= n2; ++j) s2 = acc(s2, CO(coef, j, n2), A2(u, iz, ix + j - k) - A2(u, iz, ix - j - k + 1));
+        float t = hmean(A2(mat, iz, ix + k), A2(mat, iz, ix + k - 1)) * s1 -
+                  hmean(A2(mat, iz, ix - k + 1), A2(mat, iz, ix - k)) * s2;
+        s = acc(s, CO(coef, k, nox), t);
+    }
+    return s;
+}
+
+static inline float z2_point(int norder, int nz, int npml, int izt, int NZ, const double *coef,
+                             const float *mat, const float *u, int iz, int ix)
+{
+    int noz = imin(imin(iz - izt, nz + npml - iz), norder);
+    float s = 0.0f;
+    for (int k = 1; k <= noz; ++k) {
+        int i = (2 * k - 1) / 2;
+        int n1 = imin(imin(iz + i - izt, nz + npml - iz - i), norder);
+        float s1 = 0.0f;
+        for (int j = 1; j <= n1; ++j) s1 = acc(s1, CO(coef, j, n1), A2(u, iz + j + k - 1, ix) - A2(u, iz - j + k, ix));
+        int n2 = imin(imin(iz - i - izt, nz + npml - iz + i), norder);
+        float s2 = 0.0f;
+        for (int j = 1; j <= n2; ++j) s2 = acc(s2, CO(coef, j, n2), A2(u, iz + j - k, ix) - A2(u, iz - j - k + 1, ix));
+        float t = hmean(A2(mat, iz + k, ix), A2(mat, iz + k - 1, ix)) * s1 -
+                  hmean(A2(mat, iz - k + 1, ix), A2(mat, iz - k, ix)) * s2;
+        s = acc(s, CO(coef, k, noz), t);
+    }
+    return s;
+}
+
+void orc_diffx2_reg(int norder, int nx, int nz, int npml, int izt, float dx,
+                    const double *coef, const float *mat, const float *u, float *out)
+{
+    DIMS;
+    memset(out, 0, (size_t)NZ * (size_t)(nx + 2 * npml) * sizeof(float));
+    float dx2 = (float)(1.0 / (double)(dx * dx));
+#pragma omp parallel for schedule(dynamic)
+    for (int ix = 1; ix <= nx; ++ix)
+        for (int iz = izt; iz <= nz; ++iz) {
+            if (iz < 1) continue; /* mask(1:nz,1:nx) */
+            A2(out, iz, ix) = x2_point(norder, nx, npml, izt, NZ, coef, mat, u, iz, ix) * dx2;
+        }
+}
+
+void orc_diffx2_pml(int norder, int nx, int nz, int npml, int izt, float dx,
+                    const double *coef, const float *mat, const float *u, float *out)
+{
+    DIMS;
+    memset(out, 0, (size_t)NZ * (size_t)(nx + 2 * npml) * sizeof(float));
+    float dx2 = 1.0f / (dx * dx);
+#pragma omp parallel for schedule(dynamic)
+    for (int ix = 2 - npml; ix <= nx + npml - 1; ++ix)
+        for (int iz = izt; iz <= nz + npml - 1; ++iz) {
+            if (iz >= 1 && iz <= nz && ix >= 1 && ix <= nx) continue;
+            A2(out, iz, ix) = x2_point(norder, nx, npml, izt, NZ, coef, mat, u, iz, ix) * dx2;
+        }
+}
+
+void orc_diffz2_reg(int norder, int nx, int nz, int npml, int izt, float dz,
+                    const double *coef, const float *mat, const float *u, float *out)
+{
+    DIMS;
+    memset(out, 0, (size_t)NZ * (size_t)(nx + 2 * npml) * sizeof(float));
+    float dz2 = 1.0f / (dz * dz);
+#pragma omp parallel for schedule(dynamic)
+    for (int iz = 2; iz <= nz; ++iz)
+        for (int ix = 1; ix <= nx; ++ix)
+            A2(out, iz, ix) = z2_point(norder, nz, npml, izt, NZ, coef, mat, u, iz, ix) * dz2;
+}
+
+void orc_diffz2_pml(int norder, int nx, int nz, int npml, int izt, float dz,
+                    const double *coef, const float *mat, const float *u, float *out)
+{
+    DIMS;
+    memset(out, 0, (size_t)NZ * (size_t)(nx + 2 * npml) * sizeof(float));
+    float dz2 = 1.0f / (dz * dz);
+#pragma omp parallel for schedule(dynamic)
+    for (int iz = izt + 1; iz <= nz + npml - 1; ++iz)
+        for (int ix = 1 - npml; ix <= nx + npml; ++ix) {
+            if (iz >= 1 && iz <= nz && ix >= 1 && ix <= nx) continue;
+            A2(out, iz, ix) = z2_point(norder, nz, npml, izt, NZ, coef, mat, u, iz, ix) * dz2;
+        }
+}
+
+/* ------------------------------------------------------------------ */
+/* receiver interpolation, src/Submain.f90:3-116                       */
+/* ------------------------------------------------------------------ */
+static double dirac_w(double x)
+{
+    const double C1_0 = 1.0, C1_2 = -5.0 / 4.0, C1_3 = -35.0 / 12.0, C1_4 = 21.0 / 4.0, C1_5 = -25.0 / 12.0;
+    const double C2_0 = -4.0, C2_1 = 75.0 / 4.0, C2_2 = -245.0 / 8.0, C2_3 = 545.0 / 24.0,
+                 C2_4 = -63.0 / 8.0, C2_5 = 25.0 / 24.0;
+    const double C3_0 = 18.0, C3_1 = -153.0 / 4.0, C3_2 = 255.0 / 8.0, C3_3 = -313.0 / 24.0,
+                 C3_4 = 21.0 / 8.0, C3_5 = -5.0 / 24.0;
+    double x2 = x * x, x3 = x2 * x;
+    if (x <= 1.0) return C1_0 + x2 * (C1_2 + x2 * C1_4) + x3 * (C1_3 + x2 * C1_5);
+    if (x <= 2.0) return C2_0 + C2_1 * x + x2 * (C2_2 + x2 * C2_4) + x3 * (C2_3 + x2 * C2_5);
+    if (x <= 3.0) return C3_0 + C3_1 * x + x2 * (C3_2 + x2 * C3_4) + x3 * (C3_3 + x2 * C3_5);
+    return 0.0;
+}
+
+void orc_loadseis(int nx, int nz, int npml, int izt, float x0, float z0, float dx, float dz,
+                  float rx, float rz, const float *ux, const float *uz, float *dsx, float *dsz)
+{
+    int NZ = nz + npml - izt + 1;
+    int irx = (int)floorf((rx - x0) / dx) + 1;
+    int irz = (int)floorf((rz - z0) / dz) + 1;
+    float sx = 0.0f, sz = 0.0f;
+    for (int ix = imax(1 - npml, irx - 3); ix <= imin(nx + npml, irx + 3); ++ix) {
+        double x = fabs((double)(x0 + (float)(ix - 1) * dx) - (double)rx) / (double)dx;
+        double bax = dirac_w(x);
+        for (int iz = imax(izt, irz - 3); iz <= imin(nz + npml, irz + 3); ++iz) {
+            double z = fabs((double)(z0 + (float)(iz - 1) * dz) - (double)rz) / (double)dz;
+            double baz = dirac_w(z);
+            double ba = bax * baz;
+            size_t off = 2 * ((size_t)(iz - izt) + (size_t)NZ * (size_t)(ix + npml - 1));
+            sx = (float)((double)sx + ba * (double)ux[off]);
+            sz = (float)((double)sz + ba * (double)uz[off]);
+        }
+    }
+    *dsx = sx;
+    *dsz = sz;
+}
+
+/* src/Input.f90:364-378 */
+void orc_material_law(long ncell, float *c)
+{
+    for (long i = 0; i < ncell; ++i) {
+        float vp = c[3 * i + 1], vs = c[3 * i + 2];
+        float rho = (vp + 980.0f) / 2.760f;
+        c[3 * i + 0] = rho;
+        c[3 * i + 1] = rho * (vp * vp - 2.0f * vs * vs);
+        c[3 * i + 2] = rho * vs * vs;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* geometry: src/calc_backazimuth.f90, coord_transform.f90,            */
+/* define_rotation_matrix.f90                                          */
+/* ------------------------------------------------------------------ */
+static const double O_PI = 3.14159265358979323846;
+void orc_backaz(double evla, double evlo, double stla, double stlo, double *baz)
+{
+    const double deg2rad = O_PI / 180.0, rad2deg = 180.0 / O_PI, twoPI = 2.0 * O_PI, PIo2 = 0.5 * O_PI;
+    double ell = 1.0 / 298.257;
+    double coeff = (1.0 - ell) * (1.0 - ell);
+    double scola = PIo2 - atan(coeff * tan(stla * deg2rad));
+    double ecola = PIo2 - atan(coeff * tan(evla * deg2rad));
+    double slon = stlo * deg2rad, elon = evlo * deg2rad;
+    double a = sin(scola) * cos(slon), b = sin(scola) * sin(slon), c = cos(scola);
+    double d = sin(slon), e = -cos(slon), g = -c * e, h = c * d, k = -sin(scola);
+    double aa = sin(ecola) * cos(elon), bb = sin(ecola) * sin(elon), cc = cos(ecola);
+    (void)aa; (void)bb; (void)cc; (void)k;
+    double rhs1 = (aa - d) * (aa - d) + (bb - e) * (bb - e) + cc * cc - 2.0;
+    double rhs2 = (aa - g) * (aa - g) + (bb - h) * (bb - h) + (cc - k) * (cc - k) - 2.0;
+    double dbaz = atan2(rhs1, rhs2);
+    if (dbaz < 0.0) dbaz = dbaz + twoPI;
+    double bz = dbaz * rad2deg;
+    if (fabs(bz - 360.0) < 1.0e-5) bz = 0.0;
+    *baz = bz;
+    (void)a; (void)b;
+}
+
+void orc_backazimuth_grids(double lat_org, double lon_org, double az_org, int n,
+                           const float *coordx, double evla, double evlo, double *bazs)
+{
+    const double deg2rad = O_PI / 180.0, rad2deg = 180.0 / O_PI;
+    double az = az_org * deg2rad, lat = lat_org * deg2rad, lon = lon_org * deg2rad;
+    double st = sin(lat), ct = cos(lat), sp = sin(lon), cp = cos(lon), sa = sin(az), ca = cos(az);
+    double f1 = st * cp;
+    /* first column of inv_rotate_matrix (define_rotation_matrix.f90:79-87) */
+    double i11 = -sa * sp + ca * f1;
+    double i21 = sa * cp + ca * f1;
+    double i31 = -ca * ct;
+    /* geogr2cart(lat_org, lon_org) with Earth_raduis = 6371.e3 (REAL(4)) */
+    double R = (double)6371.e3f;
+    double colat = (90.0 - lat_org) * deg2rad, lo = lon_org * deg2rad;
+    double xo = R * sin(colat) * cos(lo), yo = R * sin(colat) * sin(lo), zo = R * cos(colat);
+    for (int i = 0; i < n; ++i) {
+        double cx = (double)coordx[i];
+        double x = i11 * cx + xo, y = i21 * cx + yo, z = i31 * cx + zo;
+        double r = hypot(x, y), stla, stlo;
+        if (r < 1.0e-6) {
+            stla = z > 0.0 ? 90.0 : -90.0;
+            stlo = 0.0;
+        } else {
+            stla = atan2(z, r) * rad2deg;
+            stlo = atan2(y, x) * rad2deg;
+            if (fabs(stla) < (double)1.e-9f) stla = 0.0;
+            if (fabs(stlo) < (double)1.e-9f) stlo = 0.0;
+        }
+        double baz;
+        orc_backaz(evla, evlo, stla, stlo, &baz);
+        bazs[i] = baz * deg2rad;
+    }
+}
+
+/* ================================================================== */
+/* (2) this project's formulation of updatedispl                       */
+/* ================================================================== */
+/*
+ * State: u = level 1 (time n), level 2 (time n-1) in the (1:2,...) arrays.
+ * One call advances to n+1 and leaves level 1 = u^{n+1}, level 2 = u^n.
+ *
+ *  rho u_tt = X + Z + M, per component, where (Submain.f90 operators)
+ *    ux: X = diffx2[l+2m](ux), Z = diffz2[m](ux), M = diffzx[l](uz) + diffxz[m](uz)
+ *    uz: X = diffx2[m](uz),    Z = diffz2[l+2m](uz), M = diffzx[m](ux) + diffxz[l](ux)
+ *  Free surface (izt == 1), row iz = 1, half-cell traction-free closure:
+ *    ux: X = diffx2[meff](ux), meff = 4m(l+m)/(l+2m)   (sigma_zz = 0 eliminates d_z uz)
+ *        Z = 2/dz^2 H(m1,m2)(ux2-ux1),  M = (m1 Dx uz1 + m2 Dx uz2)/dz
+ *    uz: X = 0,  Z = 2/dz^2 H(M1,M2)(uz2-uz1),  M = (l1 Dx ux1 + l2 Dx ux2)/dz
+ *  PML (stretched coordinates, split u = u1 + u2 + u3 in the padding strips):
+ *    rho (u1_tt + dx u1_t)               = X - Dx(Mx psi_x)
+ *    rho (u2_tt + dz u2_t)               = Z - Dz(Mz psi_z)
+ *    rho (u3_tt + (dx+dz) u3_t + dx dz u3) = M
+ *    psi_t + d psi = d * D u   (first-derivative memory, the bp* arrays)
+ *  Damping d = d0 ((q-nbd)/(npml-nbd))^npower for q > nbd cells into the
+ *  left/right/bottom strips (the first nbd cells are an undamped buffer that
+ *  holds the hybrid ring), d0 = -(npower+1) vpml ln(R0) / (2 (npml-nbd) h);
+ *  the optional top strip has no buffer.
+ *  Hybrid TF/SF: the physical region 1..nx x 1..nz holds the total field,
+ *  the padding holds the scattered field.  Inside-ring cells add
+ *  L[u0 on outside strips], outside-ring cells subtract L[u0 on inside strips].
+ */
+
+typedef struct {
+    int norder, nx, nz, npml, izt, pml_top, npower, nbd, it, nt;
+    float dx, dz;
+    double dt;
+    int NZ, NX;
+} gdims;
+
+static inline size_t g2(const gdims *g, int iz, int ix)
+{
+    return (size_t)(iz - g->izt) + (size_t)g->NZ * (size_t)(ix + g->npml - 1);
+}
+
+void orc_damping(int nx, int nz, int npml, int nbd, int is_pml_top, int npower, float dx,
+                 float dz, float vpml, double R0, float *dpx, float *dpz)
+{
+    int izt = is_pml_top ? 1 - npml : 1;
+    int NX = nx + 2 * npml, NZ = nz + npml - izt + 1;
+    int leff = npml - nbd;
+    double lr = log(R0);
+    double d0x = leff > 0 ? -(double)(npower + 1) * (double)vpml * lr / (2.0 * leff * (double)dx) : 0.0;
+    double d0z = leff > 0 ? -(double)(npower + 1) * (double)vpml * lr / (2.0 * leff * (double)dz) : 0.0;
+    double d0t = -(double)(npower + 1) * (double)vpml * lr / (2.0 * npml * (double)dz);
+    for (int j = 0; j < NX; ++j) {
+        int ix = j + 1 - npml;
+        int q = ix <= 0 ? 1 - ix : (ix > nx ? ix - nx : 0);
+        double d = 0.0;
+        if (q > nbd && leff > 0) {
+            double r = (double)(q - nbd) / (double)leff, p = 1.0;
+            for (int e = 0; e < npower; ++e) p *= r;
+            d = d0x * p;
+        }
+        dpx[j] = (float)d;
+    }
+    for (int j = 0; j < NZ; ++j) {
+        int iz = j + izt;
+        double d = 0.0;
+        if (iz > nz) {
+            int q = iz - nz;
+            if (q > nbd && leff > 0) {
+                double r = (double)(q - nbd) / (double)leff, p = 1.0;
+                for (int e = 0; e < npower; ++e) p *= r;
+                d = d0z * p;
+            }
+        } else if (iz <= 0) {
+            double r = (double)(1 - iz) / (double)npml, p = 1.0;
+            for (int e = 0; e < npower; ++e) p *= r;
+            d = d0t * p;
+        }
+        dpz[j] = (float)d;
+    }
+}
+
+/* central first derivative along x / z with the reduced-order rule of
+ * diff1 (Submain.f90:192-209), times nothing; returns mysum (not divided) */
+static inline float cdx_sum(const gdims *g, const double *coef1, const float *f, int iz, int ix)
+{
+    int norder = g->norder;
+    int nox = imin(imin(ix + g->npml - 1, g->nx + g->npml - ix), norder);
+    float s = 0.0f;
+    for (int k = 1; k <= nox; ++k) s = acc(s, CO(coef1, k, nox), f[g2(g, iz, ix + k)] - f[g2(g, iz, ix - k)]);
+    return s;
+}
+static inline float cdz_sum(const gdims *g, const double *coef1, const float *f, int iz, int ix)
+{
+    int norder = g->norder;
+    int noz = imin(imin(iz - g->izt, g->nz + g->npml - iz), norder);
+    float s = 0.0f;
+    for (int k = 1; k <= noz; ++k) s = acc(s, CO(coef1, k, noz), f[g2(g, iz + k, ix)] - f[g2(g, iz - k, ix)]);
+    return s;
+}
+
+static inline float meff_of(float lam, float mu)
+{
+    float m = lam + 2.0f * mu;
+    return m != 0.0f ? (4.0f * mu) * (lam + mu) / m : 0.0f;
+}
+
+/* D2x with modulus given by a callback-free pair: mat row = lam+2mu (mode 0),
+ * mu (mode 1), meff (mode 2) evaluated on the fly */
+static inline float matval(int mode, float lam, float mu)
+{
+    return mode == 0 ? lam + 2.0f * mu : (mode == 1 ? mu : meff_of(lam, mu));
+}
+
+static float x2_point_mode(const gdims *g, const double *coef0, const float *lam, const float *mu,
+                           int mode, const float *u, int iz, int ix)
+{
+    int norder = g->norder, nx = g->nx, npml = g->npml;
+    int nox = imin(imin(ix + npml - 1, nx + npml - ix), norder);
+    float s = 0.0f;
+#define MAT(jx) matval(mode, lam[g2(g, iz, (jx))], mu[g2(g, iz, (jx))])
+#define U(jx) u[g2(g, iz, (jx))]
+    for (int k = 1; k <= nox; ++k) {
+        int i = (2 * k - 1) / 2;
+        int n1 = imin(imin(ix + i + npml - 1, nx + npml - ix - i), norder);
+        float s1 = 0.0f;
+        for (int j = 1; j <= n1; ++j) s1 = acc(s1, CO(coef0, j, n1), U(ix + j + k - 1) - U(ix - j + k));
+        int n2 = imin(imin(ix - i + npml - 1, nx + npml - ix + i), norder);
+        float s2 = 0.0f;
+        for (int j = 1; j <= n2; ++j) s2 = acc(s2, CO(coef0, j, n2), U(ix + j - k) - U(ix - j - k + 1));
+        float t = hmean(MAT(ix + k), MAT(ix + k - 1)) * s1 - hmean(MAT(ix - k + 1), MAT(ix - k)) * s2;
+        s = acc(s, CO(coef0, k, nox), t);
+    }
+#undef MAT
+#undef U
+    int in_reg = (iz >= 1 && iz <= g->nz && ix >= 1 && ix <= nx);
+    float dx2 = in_reg ? (float)(1.0 / (double)(g->dx * g->dx)) : 1.0f / (g->dx * g->dx);
+    return s * dx2;
+}
+
+static float z2_point_mode(const gdims *g, const double *coef0, const float *lam, const float *mu,
+                           int mode, const float *u, int iz, int ix)
+{
+    int norder = g->norder, nz = g->nz, npml = g->npml, izt = g->izt;
+    int noz = imin(imin(iz - izt, nz + npml - iz), norder);
+    float s = 0.0f;
+#define MAT(jz) matval(mode, lam[g2(g, (jz), ix)], mu[g2(g, (jz), ix)])
+#define U(jz) u[g2(g, (jz), ix)]
+    for (int k = 1; k <= noz; ++k) {
+        int i = (2 * k - 1) / 2;
+        int n1 = imin(imin(iz + i - izt, nz + npml - iz - i), norder);
+        float s1 = 0.0f;
+        for (int j = 1; j <= n1; ++j) s1 = acc(s1, CO(coef0, j, n1), U(iz + j + k - 1) - U(iz - j + k));
+        int n2 = imin(imin(iz - i - izt, nz + npml - iz + i), norder);
+        float s2 = 0.0f;
+        for (int j = 1; j <= n2; ++j) s2 = acc(s2, CO(coef0, j, n2), U(iz + j - k) - U(iz - j - k + 1));
+        float t = hmean(MAT(iz + k), MAT(iz + k - 1)) * s1 - hmean(MAT(iz - k + 1), MAT(iz - k)) * s2;
+        s = acc(s, CO(coef0, k, noz), t);
+    }
+#undef MAT
+#undef U
+    return s * (1.0f / (g->dz * g->dz));
+}
+
+/* Dx( mat * Dz u ): diffzx at one output cell */
+static float zx_point(const gdims *g, const double *coef1, const float *mat, const float *u, int iz, int ix)
+{
+    int norder = g->norder;
+    int nox = imin(imin(ix + g->npml - 1, g->nx + g->npml - ix), norder);
+    float s = 0.0f;
+    for (int k = 1; k <= nox; ++k) {
+        float vp = mat[g2(g, iz, ix + k)] * cdz_sum(g, coef1, u, iz, ix + k) / g->dz;
+        float vm = mat[g2(g, iz, ix - k)] * cdz_sum(g, coef1, u, iz, ix - k) / g->dz;
+        s = acc(s, CO(coef1, k, nox), vp - vm);
+    }
+    return s / g->dx;
+}
+/* Dz( mat * Dx u ): diffxz at one output cell */
+static float xz_point(const gdims *g, const double *coef1, const float *mat, const float *u, int iz, int ix)
+{
+    int norder = g->norder;
+    int noz = imin(imin(iz - g->izt, g->nz + g->npml - iz), norder);
+    float s = 0.0f;
+    for (int k = 1; k <= noz; ++k) {
+        float vp = mat[g2(g, iz + k, ix)] * cdx_sum(g, coef1, u, iz + k, ix) / g->dx;
+        float vm = mat[g2(g, iz - k, ix)] * cdx_sum(g, coef1, u, iz - k, ix) / g->dx;
+        s = acc(s, CO(coef1, k, noz), vp - vm);
+    }
+    return s / g->dz;
+}
+
+/* the six RHS parts at one updated cell; lam/mu 2-D planes */
+static void eval_parts_g(const gdims *g, const double *coef0, const double *coef1, const float *lam,
+                         const float *mu, const float *ux, const float *uz, int iz, int ix, float p[6])
+{
+    if (iz == g->izt && g->izt == 1) {
+        /* free-surface row: half-cell traction-free closure */
+        size_t a = g2(g, 1, ix), b = g2(g, 2, ix);
+        float sdz2 = 2.0f / (g->dz * g->dz);
+        float M1 = lam[a] + 2.0f * mu[a], M2 = lam[b] + 2.0f * mu[b];
+        p[0] = x2_point_mode(g, coef0, lam, mu, 2, ux, iz, ix);
+        p[1] = (hmean(mu[a], mu[b]) * (ux[b] - ux[a])) * sdz2;
+        float vm1 = mu[a] * cdx_sum(g, coef1, uz, 1, ix) / g->dx;
+        float vm2 = mu[b] * cdx_sum(g, coef1, uz, 2, ix) / g->dx;
+        p[2] = (vm1 + vm2) / g->dz;
+        p[3] = 0.0f;
+        p[4] = (hmean(M1, M2) * (uz[b] - uz[a])) * sdz2;
+        float vl1 = lam[a] * cdx_sum(g, coef1, ux, 1, ix) / g->dx;
+        float vl2 = lam[b] * cdx_sum(g, coef1, ux, 2, ix) / g->dx;
+        p[5] = (vl1 + vl2) / g->dz;
+        return;
+    }
+    p[0] = x2_point_mode(g, coef0, lam, mu, 0, ux, iz, ix);
+    p[1] = z2_point_mode(g, coef0, lam, mu, 1, ux, iz, ix);
+    p[2] = zx_point(g, coef1, lam, uz, iz, ix) + xz_point(g, coef1, mu, uz, iz, ix);
+    p[3] = x2_point_mode(g, coef0, lam, mu, 1, uz, iz, ix);
+    p[4] = z2_point_mode(g, coef0, lam, mu, 0, uz, iz, ix);
+    p[5] = zx_point(g, coef1, mu, ux, iz, ix) + xz_point(g, coef1, lam, ux, iz, ix);
+}
+
+void orc_eval_parts(int norder, int nx, int nz, int npml, int izt, float dx, float dz,
+                    const double *coef0, const double *coef1, const float *lam2d,
+                    const float *mu2d, const float *ux2d, const float *uz2d,
+                    int iz, int ix, float parts[6])
+{
+    gdims g;
+    memset(&g, 0, sizeof g);
+    g.norder = norder; g.nx = nx; g.nz = nz; g.npml = npml; g.izt = izt; g.dx = dx; g.dz = dz;
+    g.NZ = nz + npml - izt + 1; g.NX = nx + 2 * npml;
+    eval_parts_g(&g, coef0, coef1, lam2d, mu2d, ux2d, uz2d, iz, ix, parts);
+}
+
+/* --- strip storage --- */
+enum { S_NONE = -1, S_L = 0, S_R = 1, S_B = 2, S_T = 3 };
+static inline int strip_of(const gdims *g, int iz, int ix)
+{
+    if (iz >= 1 && iz <= g->nz) {
+        if (ix <= 0) return S_L;
+        if (ix > g->nx) return S_R;
+        return S_NONE;
+    }
+    if (iz > g->nz) return S_B;
+    return S_T;
+}
+/* cell offset (without level) inside a strip array */
+static inline size_t strip_off(const gdims *g, int s, int iz, int ix)
+{
+    switch (s) {
+    case S_L: return (size_t)(iz - 1) + (size_t)g->nz * (size_t)(ix - (1 - g->npml));
+    case S_R: return (size_t)(iz - 1) + (size_t)g->nz * (size_t)(ix - (g->nx + 1));
+    case S_B: return (size_t)(iz - (g->nz + 1)) + (size_t)g->npml * (size_t)(ix - (1 - g->npml));
+    default:  return (size_t)(iz - (1 - g->npml)) + (size_t)g->npml * (size_t)(ix - (1 - g->npml));
+    }
+}
+
+/* background ring: returns pointer offset in the right array or -1;
+ * which: 0 = inside (reg) arrays, 1 = outside (pml) arrays */
+static inline long bg_off(const gdims *g, int iz, int ix, int it, int *arr, int *which)
+{
+    int nx = g->nx, nz = g->nz, nbd = g->nbd;
+    if (iz >= 1 && iz <= nz - nbd && ix >= 1 && ix <= nbd) {
+        *arr = 0; *which = 0;
+        return (long)((size_t)(iz - 1) + (size_t)(nz - nbd) * ((size_t)(ix - 1) + (size_t)nbd * it));
+    }
+    if (iz >= 1 && iz <= nz - nbd && ix >= nx - nbd + 1 && ix <= nx) {
+        *arr = 1; *which = 0;
+        return (long)((size_t)(iz - 1) + (size_t)(nz - nbd) * ((size_t)(ix - (nx - nbd + 1)) + (size_t)nbd * it));
+    }
+    if (iz >= nz - nbd + 1 && iz <= nz && ix >= 1 && ix <= nx) {
+        *arr = 2; *which = 0;
+        return (long)((size_t)(iz - (nz - nbd + 1)) + (size_t)nbd * ((size_t)(ix - 1) + (size_t)nx * it));
+    }
+    if (iz >= 1 && iz <= nz && ix >= 1 - nbd && ix <= 0) {
+        *arr = 0; *which = 1;
+        return (long)((size_t)(iz - 1) + (size_t)nz * ((size_t)(ix - (1 - nbd)) + (size_t)nbd * it));
+    }
+    if (iz >= 1 && iz <= nz && ix >= nx + 1 && ix <= nx + nbd) {
+        *arr = 1; *which = 1;
+        return (long)((size_t)(iz - 1) + (size_t)nz * ((size_t)(ix - (nx + 1)) + (size_t)nbd * it));
+    }
+    if (iz >= nz + 1 && iz <= nz + nbd && ix >= 1 - nbd && ix <= nx + nbd) {
+        *arr = 2; *which = 1;
+        return (long)((size_t)(iz - (nz + 1)) + (size_t)nbd * ((size_t)(ix - (1 - nbd)) + (size_t)(nx + 2 * nbd) * it));
+    }
+    return -1;
+}
+
+void orc_updatedispl(const int *norder_, const int *nx_, const int *nz_, const int *npml_,
+    const int *izt_, const int *is_pml_top_, const int *npower_, const int *nbd_, const int *it_,
+    const int *nt_, const float *dx_, const float *dz_, const double *dt_, const float *vpml_,
+    const double *R0_,
+    const float *uxl_reg, const float *uzl_reg, const float *uxr_reg, const float *uzr_reg,
+    const float *uxb_reg, const float *uzb_reg, const float *uxl_pml, const float *uzl_pml,
+    const float *uxr_pml, const float *uzr_pml, const float *uxb_pml, const float *uzb_pml,
+    const float *c, const double *coef0, const double *coef1, float *ux, float *uz,
+    float *bpxlx, float *bpzlx, float *bpxrx, float *bpzrx, float *bpxtx, float *bpxtz,
+    float *bpztx, float *bpztz, float *bpxbx, float *bpxbz, float *bpzbx, float *bpzbz,
+    float *bux1l, float *bux2l, float *bux3l, float *buz1l, float *buz2l, float *buz3l,
+    float *bux1r, float *bux2r, float *bux3r, float *buz1r, float *buz2r, float *buz3r,
+    float *bux1b, float *bux2b, float *bux3b, float *buz1b, float *buz2b, float *buz3b,
+    float *bux1t, float *bux2t, float *bux3t, float *buz1t, float *buz2t, float *buz3t)
+{
+    gdims G, *g = &G;
+    G.norder = *norder_; G.nx = *nx_; G.nz = *nz_; G.npml = *npml_; G.izt = *izt_;
+    G.pml_top = *is_pml_top_; G.npower = *npower_; G.nbd = *nbd_; G.it = *it_; G.nt = *nt_;
+    G.dx = *dx_; G.dz = *dz_; G.dt = *dt_;
+    G.NZ = G.nz + G.npml - G.izt + 1; G.NX = G.nx + 2 * G.npml;
+    const int norder = G.norder, nx = G.nx, nz = G.nz, npml = G.npml, izt = G.izt, nbd = G.nbd;
+    const size_t ncell = (size_t)G.NZ * (size_t)G.NX;
+    (void)norder;
+
+    /* copy-in of the level-1 fields and the material planes */
+    float *u1x = malloc(ncell * sizeof(float)), *u1z = malloc(ncell * sizeof(float));
+    float *rho = malloc(ncell * sizeof(float)), *lam = malloc(ncell * sizeof(float));
+    float *mu = malloc(ncell * sizeof(float)), *lp2m = malloc(ncell * sizeof(float));
+    for (size_t i = 0; i < ncell; ++i) {
+        u1x[i] = ux[2 * i]; u1z[i] = uz[2 * i];
+        rho[i] = c[3 * i]; lam[i] = c[3 * i + 1]; mu[i] = c[3 * i + 2];
+        lp2m[i] = lam[i] + 2.0f * mu[i];
+    }
+    float *dpx = malloc(G.NX * sizeof(float)), *dpz = malloc(G.NZ * sizeof(float));
+    orc_damping(nx, nz, npml, nbd, G.pml_top, G.npower, G.dx, G.dz, *vpml_, *R0_, dpx, dpz);
+
+    float *bpx[4][2] = {{bpxlx, bpzlx}, {bpxrx, bpzrx}, {bpxbx, bpzbx}, {bpxtx, bpztx}};
+    float *bpz[4][2] = {{NULL, NULL}, {NULL, NULL}, {bpxbz, bpzbz}, {bpxtz, bpztz}};
+    float *bu[4][2][3] = {{{bux1l, bux2l, bux3l}, {buz1l, buz2l, buz3l}},
+                          {{bux1r, bux2r, bux3r}, {buz1r, buz2r, buz3r}},
+                          {{bux1b, bux2b, bux3b}, {buz1b, buz2b, buz3b}},
+                          {{bux1t, bux2t, bux3t}, {buz1t, buz2t, buz3t}}};
+    const int nstrip = G.pml_top ? 4 : 3;
+
+    /* ---- phase A: first-derivative memories psi (level n) ---- */
+    for (int s = 0; s < nstrip; ++s) {
+        int iz0, iz1, ix0, ix1;
+        if (s == S_L) { iz0 = 1; iz1 = nz; ix0 = 1 - npml; ix1 = 0; }
+        else if (s == S_R) { iz0 = 1; iz1 = nz; ix0 = nx + 1; ix1 = nx + npml; }
+        else if (s == S_B) { iz0 = nz + 1; iz1 = nz + npml; ix0 = 1 - npml; ix1 = nx + npml; }
+        else { iz0 = 1 - npml; iz1 = 0; ix0 = 1 - npml; ix1 = nx + npml; }
+#pragma omp parallel for schedule(dynamic)
+        for (int ix = ix0; ix <= ix1; ++ix)
+            for (int iz = iz0; iz <= iz1; ++iz) {
+                size_t o = strip_off(g, s, iz, ix);
+                float ddx = dpx[ix + npml - 1], ddz = dpz[iz - izt];
+                if (ddx > 0.0f) {
+                    float b = (float)exp(-(double)ddx * G.dt), a = 1.0f - b;
+                    float gx = cdx_sum(g, coef1, u1x, iz, ix) / G.dx;
+                    float gz = cdx_sum(g, coef1, u1z, iz, ix) / G.dx;
+                    bpx[s][0][o] = b * bpx[s][0][o] + a * gx;
+                    bpx[s][1][o] = b * bpx[s][1][o] + a * gz;
+                }
+                if (bpz[s][0] && ddz > 0.0f) {
+                    float b = (float)exp(-(double)ddz * G.dt), a = 1.0f - b;
+                    float gx = cdz_sum(g, coef1, u1x, iz, ix) / G.dz;
+                    float gz = cdz_sum(g, coef1, u1z, iz, ix) / G.dz;
+                    bpz[s][0][o] = b * bpz[s][0][o] + a * gx;
+                    bpz[s][1][o] = b * bpz[s][1][o] + a * gz;
+                }
+            }
+    }
+
+    /* full-grid psi views (zero outside the strips) for the correction stencils */
+    float *psx[2], *psz[2];
+    for (int q = 0; q < 2; ++q) {
+        psx[q] = calloc(ncell, sizeof(float));
+        psz[q] = calloc(ncell, sizeof(float));
+    }
+    for (int s = 0; s < nstrip; ++s) {
+        int iz0, iz1, ix0, ix1;
+        if (s == S_L) { iz0 = 1; iz1 = nz; ix0 = 1 - npml; ix1 = 0; }
+        else if (s == S_R) { iz0 = 1; iz1 = nz; ix0 = nx + 1; ix1 = nx + npml; }
+        else if (s == S_B) { iz0 = nz + 1; iz1 = nz + npml; ix0 = 1 - npml; ix1 = nx + npml; }
+        else { iz0 = 1 - npml; iz1 = 0; ix0 = 1 - npml; ix1 = nx + npml; }
+        for (int ix = ix0; ix <= ix1; ++ix)
+            for (int iz = iz0; iz <= iz1; ++iz) {
+                size_t o = strip_off(g, s, iz, ix), gi = g2(g, iz, ix);
+                for (int q = 0; q < 2; ++q) {
+                    psx[q][gi] = bpx[s][q][o];
+                    if (bpz[s][q]) psz[q][gi] = bpz[s][q][o];
+                }
+            }
+    }
+
+    /* ---- phase B: operator passes (the Submain operators, reg + pml) ---- */
+    float *t1 = malloc(ncell * sizeof(float)), *t2 = malloc(ncell * sizeof(float));
+    float *P[6];
+    for (int q = 0; q < 6; ++q) P[q] = calloc(ncell, sizeof(float));
+    float dxf = G.dx, dzf = G.dz;
+#define ADD2(dst) for (size_t i = 0; i < ncell; ++i) dst[i] = dst[i] + (t1[i] + t2[i])
+    orc_diffx2_reg(norder, nx, nz, npml, izt, dxf, coef0, lp2m, u1x, t1);
+    orc_diffx2_pml(norder, nx, nz, npml, izt, dxf, coef0, lp2m, u1x, t2); ADD2(P[0]);
+    orc_diffz2_reg(norder, nx, nz, npml, izt, dzf, coef0, mu, u1x, t1);
+    orc_diffz2_pml(norder, nx, nz, npml, izt, dzf, coef0, mu, u1x, t2); ADD2(P[1]);
+    orc_diffzx_reg(norder, nx, nz, npml, izt, dxf, dzf, coef1, lam, u1z, t1);
+    orc_diffzx_pml(norder, nx, nz, npml, izt, dxf, dzf, coef1, lam, u1z, t2); ADD2(P[2]);
+    orc_diffxz_reg(norder, nx, nz, npml, izt, dxf, dzf, coef1, mu, u1z, t1);
+    orc_diffxz_pml(norder, nx, nz, npml, izt, dxf, dzf, coef1, mu, u1z, t2); ADD2(P[2]);
+    orc_diffx2_reg(norder, nx, nz, npml, izt, dxf, coef0, mu, u1z, t1);
+    orc_diffx2_pml(norder, nx, nz, npml, izt, dxf, coef0, mu, u1z, t2); ADD2(P[3]);
+    orc_diffz2_reg(norder, nx, nz, npml, izt, dzf, coef0, lp2m, u1z, t1);
+    orc_diffz2_pml(norder, nx, nz, npml, izt, dzf, coef0, lp2m, u1z, t2); ADD2(P[4]);
+    orc_diffzx_reg(norder, nx, nz, npml, izt, dxf, dzf, coef1, mu, u1x, t1);
+    orc_diffzx_pml(norder, nx, nz, npml, izt, dxf, dzf, coef1, mu, u1x, t2); ADD2(P[5]);
+    orc_diffxz_reg(norder, nx, nz, npml, izt, dxf, dzf, coef1, lam, u1x, t1);
+    orc_diffxz_pml(norder, nx, nz, npml, izt, dxf, dzf, coef1, lam, u1x, t2); ADD2(P[5]);
+#undef ADD2
+
+    /* masked background fields for the hybrid correction */
+    const float *bgx[2][3] = {{uxl_reg, uxr_reg, uxb_reg}, {uxl_pml, uxr_pml, uxb_pml}};
+    const float *bgz[2][3] = {{uzl_reg, uzr_reg, uzb_reg}, {uzl_pml, uzr_pml, uzb_pml}};
+    int have_bg = uxl_reg && uzl_reg && uxr_reg && uzr_reg && uxb_reg && uzb_reg && uxl_pml &&
+                  uzl_pml && uxr_pml && uzr_pml && uxb_pml && uzb_pml && G.it >= 0 && G.it <= G.nt;
+    float *wTx = NULL, *wTz = NULL, *wSx = NULL, *wSz = NULL;
+    if (have_bg) {
+        wTx = calloc(ncell, sizeof(float)); wTz = calloc(ncell, sizeof(float));
+        wSx = calloc(ncell, sizeof(float)); wSz = calloc(ncell, sizeof(float));
+        for (int ix = 1 - nbd; ix <= nx + nbd; ++ix)
+            for (int iz = 1; iz <= nz + nbd; ++iz) {
+                int arr, which;
+                long o = bg_off(g, iz, ix, G.it, &arr, &which);
+                if (o < 0) continue;
+                size_t gi = g2(g, iz, ix);
+                if (which == 0) { wTx[gi] = bgx[0][arr][o]; wTz[gi] = bgz[0][arr][o]; }
+                else { wSx[gi] = bgx[1][arr][o]; wSz[gi] = bgz[1][arr][o]; }
+            }
+    }
+
+    const float dt2 = (float)(G.dt * G.dt);
+    const int iz_first = (izt == 1) ? 1 : izt + 1;
+    float *newx = malloc(ncell * sizeof(float)), *newz = malloc(ncell * sizeof(float));
+    memcpy(newx, u1x, ncell * sizeof(float)); /* cells not updated keep their value */
+    memcpy(newz, u1z, ncell * sizeof(float));
+
+#pragma omp parallel for schedule(dynamic)
+    for (int ix = 2 - npml; ix <= nx + npml - 1; ++ix) {
+        for (int iz = iz_first; iz <= nz + npml - 1; ++iz) {
+            size_t gi = g2(g, iz, ix);
+            float p[6];
+            for (int q = 0; q < 6; ++q) p[q] = P[q][gi];
+            if (iz == 1 && izt == 1) eval_parts_g(g, coef0, coef1, lam, mu, u1x, u1z, iz, ix, p);
+
+            int s = strip_of(g, iz, ix);
+            /* PML memory corrections (nonzero only next to damped cells) */
+            {
+                int surf = (iz == 1 && izt == 1);
+                int nox = imin(imin(ix + npml - 1, nx + npml - ix), G.norder);
+                int noz = imin(imin(iz - izt, nz + npml - iz), G.norder);
+                float sx0 = 0.0f, sx1 = 0.0f, sz0 = 0.0f, sz1 = 0.0f;
+                for (int k = 1; k <= nox; ++k) {
+                    size_t a = g2(g, iz, ix + k), b = g2(g, iz, ix - k);
+                    float ma = surf ? meff_of(lam[a], mu[a]) : lp2m[a];
+                    float mb = surf ? meff_of(lam[b], mu[b]) : lp2m[b];
+                    sx0 = acc(sx0, CO(coef1, k, nox), ma * psx[0][a] - mb * psx[0][b]);
+                    sx1 = acc(sx1, CO(coef1, k, nox), mu[a] * psx[1][a] - mu[b] * psx[1][b]);
+                }
+                for (int k = 1; k <= noz; ++k) {
+                    size_t a = g2(g, iz + k, ix), b = g2(g, iz - k, ix);
+                    sz0 = acc(sz0, CO(coef1, k, noz), mu[a] * psz[0][a] - mu[b] * psz[0][b]);
+                    sz1 = acc(sz1, CO(coef1, k, noz), lp2m[a] * psz[1][a] - lp2m[b] * psz[1][b]);
+                }
+                p[0] = p[0] - sx0 / G.dx;
+                if (!surf) p[3] = p[3] - sx1 / G.dx;
+                if (!surf) { p[1] = p[1] - sz0 / G.dz; p[4] = p[4] - sz1 / G.dz; }
+            }
+            /* hybrid TF/SF correction */
+            if (have_bg) {
+                int arr, which;
+                long o = bg_off(g, iz, ix, G.it, &arr, &which);
+                if (o >= 0) {
+                    float q[6];
+                    if (which == 0) {
+                        eval_parts_g(g, coef0, coef1, lam, mu, wSx, wSz, iz, ix, q);
+                        for (int m = 0; m < 6; ++m) p[m] = p[m] + q[m];
+                    } else {
+                        eval_parts_g(g, coef0, coef1, lam, mu, wTx, wTz, iz, ix, q);
+                        for (int m = 0; m < 6; ++m) p[m] = p[m] - q[m];
+                    }
+                }
+            }
+
+            float bf = dt2 / rho[gi];
+            if (s == S_NONE) {
+                float rx = (p[0] + p[1]) + p[2], rz = (p[3] + p[4]) + p[5];
+                newx[gi] = (2.0f * u1x[gi] - ux[2 * gi + 1]) + bf * rx;
+                newz[gi] = (2.0f * u1z[gi] - uz[2 * gi + 1]) + bf * rz;
+            } else {
+                size_t o = strip_off(g, s, iz, ix);
+                float ax = 0.5f * dpx[ix + npml - 1] * (float)G.dt;
+                float az = 0.5f * dpz[iz - izt] * (float)G.dt;
+                float c3 = 2.0f - 4.0f * ax * az;
+                for (int comp = 0; comp < 2; ++comp) {
+                    float *b1 = bu[s][comp][0], *b2 = bu[s][comp][1], *b3 = bu[s][comp][2];
+                    float X = p[3 * comp], Z = p[3 * comp + 1], Mm = p[3 * comp + 2];
+                    float n1 = (2.0f * b1[2 * o] - (1.0f - ax) * b1[2 * o + 1] + bf * X) / (1.0f + ax);
+                    float n2 = (2.0f * b2[2 * o] - (1.0f - az) * b2[2 * o + 1] + bf * Z) / (1.0f + az);
+                    float n3 = (c3 * b3[2 * o] - (1.0f - ax - az) * b3[2 * o + 1] + bf * Mm) / (1.0f + ax + az);
+                    b1[2 * o + 1] = b1[2 * o]; b1[2 * o] = n1;
+                    b2[2 * o + 1] = b2[2 * o]; b2[2 * o] = n2;
+                    b3[2 * o + 1] = b3[2 * o]; b3[2 * o] = n3;
+                    float tot = (n1 + n2) + n3;
+                    if (comp == 0) newx[gi] = tot; else newz[gi] = tot;
+                }
+            }
+        }
+    }
+    /* level shift: level 2 <- level 1, level 1 <- new */
+    for (size_t i = 0; i < ncell; ++i) {
+        ux[2 * i + 1] = ux[2 * i]; ux[2 * i] = newx[i];
+        uz[2 * i + 1] = uz[2 * i]; uz[2 * i] = newz[i];
+    }
+
+    free(newx); free(newz);
+    free(wTx); free(wTz); free(wSx); free(wSz);
+    for (int q = 0; q < 6; ++q) free(P[q]);
+    free(t1); free(t2);
+    for (int q = 0; q < 2; ++q) { free(psx[q]); free(psz[q]); }
+    free(dpx); free(dpz);
+    free(u1x); free(u1z); free(rho); free(lam); free(mu); free(lp2m);
+}

@@ -1,0 +1,226 @@
+"""ctypes bindings to the CPU oracle (oracle/build/liboracle.so) and to the
+reference's own source-visible Fortran routines (oracle/_ref/libfdfk_ref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, never by the product package.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "build", "liboracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libfdfk_ref.so")
+
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+
+
+def _i(v):
+    return C.byref(C.c_int(int(v)))
+
+
+def _f(v):
+    return C.byref(C.c_float(float(v)))
+
+
+def _d(v):
+    return C.byref(C.c_double(float(v)))
+
+
+def _p(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.c_void_p)
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        _lib = C.CDLL(LIB_PATH)
+        _lib.orc_diffx1.restype = C.c_float
+        _lib.orc_diff1.restype = C.c_float
+        _lib.orc_check_stability.restype = C.c_int
+        _lib.orc_fk_traces.restype = C.c_int
+    return _lib
+
+
+def ref_available():
+    return os.path.exists(REF_PATH)
+
+
+def ref():
+    """Reference routines compiled from src/*.f90 (lazy binding: the LAPACK
+    symbols used only by zoeppritz stay unresolved and are never called)."""
+    global _ref
+    if _ref is None:
+        _ref = C.CDLL(REF_PATH, mode=os.RTLD_LAZY | os.RTLD_LOCAL)
+    return _ref
+
+
+def fd_coefs(norder):
+    out = []
+    for name in ("orc_fd_coef0", "orc_fd_coef1", "orc_fd_coef2"):
+        c = np.zeros(norder * norder, np.float64)
+        getattr(lib(), name)(C.c_int(norder), _p(c))
+        out.append(c)
+    return out
+
+
+OPS = ("diffzx_reg", "diffzx_pml", "diffxz_reg", "diffxz_pml",
+       "diffx2_reg", "diffx2_pml", "diffz2_reg", "diffz2_pml")
+
+
+def op(name, norder, nx, nz, npml, izt, dx, dz, coef, mat, u):
+    """Apply one oracle operator to 2-D arrays shaped (NX, NZ) (z fastest)."""
+    out = np.zeros_like(u)
+    L = lib()
+    fn = getattr(L, "orc_" + name)
+    ints = [C.c_int(norder), C.c_int(nx), C.c_int(nz), C.c_int(npml), C.c_int(izt)]
+    if name.startswith("diffx2"):
+        fn(*ints, C.c_float(dx), _p(coef), _p(mat), _p(u), _p(out))
+    elif name.startswith("diffz2"):
+        fn(*ints, C.c_float(dz), _p(coef), _p(mat), _p(u), _p(out))
+    else:
+        fn(*ints, C.c_float(dx), C.c_float(dz), _p(coef), _p(mat), _p(u), _p(out))
+    return out
+
+
+def ref_op(name, norder, nx, nz, npml, izt, dx, dz, coef, mat, u):
+    """The same operator from the reference's compiled Submain.f90."""
+    out = np.zeros_like(u)
+    fn = getattr(ref(), name + "_")
+    ints = [_i(norder), _i(nx), _i(nz), _i(npml), _i(izt)]
+    if name.startswith("diffx2"):
+        fn(*ints, _f(dx), _p(coef), _p(mat), _p(u), _p(out))
+    elif name.startswith("diffz2"):
+        fn(*ints, _f(dz), _p(coef), _p(mat), _p(u), _p(out))
+    else:
+        fn(*ints, _f(dx), _f(dz), _p(coef), _p(mat), _p(u), _p(out))
+    return out
+
+
+def loadseis(nx, nz, npml, izt, x0, z0, dx, dz, rx, rz, ux, uz):
+    sx, sz = C.c_float(), C.c_float()
+    lib().orc_loadseis(C.c_int(nx), C.c_int(nz), C.c_int(npml), C.c_int(izt), C.c_float(x0),
+                       C.c_float(z0), C.c_float(dx), C.c_float(dz), C.c_float(rx),
+                       C.c_float(rz), _p(ux), _p(uz), C.byref(sx), C.byref(sz))
+    return sx.value, sz.value
+
+
+def ref_loadseis(nx, nz, npml, izt, x0, z0, dx, dz, rx, rz, ux, uz):
+    sx, sz = C.c_float(), C.c_float()
+    ref().loadseis_(_i(nx), _i(nz), _i(npml), _i(izt), _f(x0), _f(z0), _f(dx), _f(dz),
+                    _f(rx), _f(rz), _p(ux), _p(uz), C.byref(sx), C.byref(sz))
+    return sx.value, sz.value
+
+
+def check_stability(nx, nz, npml, izt, norder, dx, dz, dt, coef2, c):
+    dts = C.c_float()
+    rc = lib().orc_check_stability(C.c_int(nx), C.c_int(nz), C.c_int(npml), C.c_int(izt),
+                                   C.c_int(norder), C.c_float(dx), C.c_float(dz),
+                                   C.c_double(dt), _p(coef2), _p(c), C.byref(dts))
+    return rc, dts.value
+
+
+def material_law(c):
+    lib().orc_material_law(C.c_long(c.size // 3), _p(c))
+
+
+def backazimuth_grids(lat_org, lon_org, az_org, coordx, evla, evlo):
+    coordx = np.ascontiguousarray(coordx, np.float32)
+    out = np.zeros(coordx.size, np.float64)
+    lib().orc_backazimuth_grids(C.c_double(lat_org), C.c_double(lon_org), C.c_double(az_org),
+                                C.c_int(coordx.size), _p(coordx), C.c_double(evla),
+                                C.c_double(evlo), _p(out))
+    return out
+
+
+def ref_backazimuth_grids(lat_org, lon_org, az_org, coordx, evla, evlo):
+    R = ref()
+    for nm, v in (("lat_org", lat_org), ("lon_org", lon_org), ("az_org", az_org)):
+        C.c_double.in_dll(R, "_QMfdfk_parE" + nm).value = v
+    coordx = np.ascontiguousarray(coordx, np.float32)
+    out = np.zeros(coordx.size, np.float64)
+    R.compute_backazimuth_grids_(_i(coordx.size), _p(coordx), _d(evla), _d(evlo), _p(out))
+    return out
+
+
+def eval_parts(norder, nx, nz, npml, izt, dx, dz, coef0, coef1, lam, mu, ux, uz, iz, ix):
+    parts = np.zeros(6, np.float32)
+    lib().orc_eval_parts(C.c_int(norder), C.c_int(nx), C.c_int(nz), C.c_int(npml),
+                         C.c_int(izt), C.c_float(dx), C.c_float(dz), _p(coef0), _p(coef1),
+                         _p(lam), _p(mu), _p(ux), _p(uz), C.c_int(iz), C.c_int(ix),
+                         _p(parts))
+    return parts
+
+
+def damping(nx, nz, npml, nbd, pml_top, npower, dx, dz, vpml, R0):
+    izt = 1 - npml if pml_top else 1
+    dpx = np.zeros(nx + 2 * npml, np.float32)
+    dpz = np.zeros(nz + npml - izt + 1, np.float32)
+    lib().orc_damping(C.c_int(nx), C.c_int(nz), C.c_int(npml), C.c_int(nbd), C.c_int(pml_top),
+                      C.c_int(npower), C.c_float(dx), C.c_float(dz), C.c_float(vpml),
+                      C.c_double(R0), _p(dpx), _p(dpz))
+    return dpx, dpz
+
+
+def updatedispl(st, it):
+    """One oracle time step on a state dict produced by fdfk2d_amd-style
+    allocation (see tests/_state.py).  Arrays are mutated in place."""
+    a = st
+    bg = a.get("bg")
+    bgp = [_p(x) for x in bg] if bg is not None else [None] * 12
+    lib().orc_updatedispl(
+        _i(a["norder"]), _i(a["nx"]), _i(a["nz"]), _i(a["npml"]), _i(a["izt"]),
+        _i(a["pml_top"]), _i(a["npower"]), _i(a["nbd"]), _i(it), _i(a["nt"]),
+        _f(a["dx"]), _f(a["dz"]), _d(a["dt"]), _f(a["vpml"]), _d(a["R0"]),
+        *bgp, _p(a["c"]), _p(a["coef0"]), _p(a["coef1"]), _p(a["ux"]), _p(a["uz"]),
+        *[_p(x) for x in a["bp"]], *[_p(x) for x in a["bu"]])
+
+
+def background(nx, nz, npml, izt, nbd, nt, src_is_p, x0, z0, dx, dz, dt, ds, f0, rayp, bazs,
+               fk_left, fk_right):
+    """Oracle FK background in the reference's twelve-array layout."""
+    shapes = bg_shapes(nx, nz, nbd, nt)
+    arrs = [np.zeros(s, np.float32) for s in shapes]
+    fl = np.ascontiguousarray(fk_left, np.float32)
+    fr = np.ascontiguousarray(fk_right, np.float32)
+    bz = np.ascontiguousarray(bazs, np.float64)
+    lib().orc_comput_background_wavefield(
+        _i(nx), _i(nz), _i(npml), _i(izt), _i(nbd), _i(nt), _i(fl.size // 4), _i(fr.size // 4),
+        _i(1 if src_is_p else 0), _f(x0), _f(z0), _f(dx), _f(dz), _d(dt), _f(ds), _f(f0),
+        _d(rayp), _p(bz), _p(fl), _p(fr), *[_p(x) for x in arrs])
+    return arrs
+
+
+def bg_shapes(nx, nz, nbd, nt):
+    """Element counts of uxl_reg..uzb_pml (src/FDFK2D.f90:330-355)."""
+    T = nt + 1
+    reg_lr = (nz - nbd) * nbd * T
+    reg_b = nbd * nx * T
+    pml_lr = nz * nbd * T
+    pml_b = nbd * (nx + 2 * nbd) * T
+    return [reg_lr, reg_lr, reg_lr, reg_lr, reg_b, reg_b,
+            pml_lr, pml_lr, pml_lr, pml_lr, pml_b, pml_b]
+
+
+def fk_traces(fkmodel, src_is_p, p, f0, ds, dt, ntq, z, zref, t0):
+    fk = np.ascontiguousarray(fkmodel, np.float32)
+    z = np.ascontiguousarray(z, np.float64)
+    ux = np.zeros((z.size, ntq), np.float64)
+    uz = np.zeros((z.size, ntq), np.float64)
+    rc = lib().orc_fk_traces(C.c_int(fk.size // 4), _p(fk), C.c_int(1 if src_is_p else 0),
+                             C.c_double(p), C.c_float(f0), C.c_float(ds), C.c_double(dt),
+                             C.c_int(ntq), C.c_int(z.size), _p(z), C.c_double(zref),
+                             C.c_double(t0), _p(ux), _p(uz))
+    if rc:
+        raise ValueError("invalid FK model")
+    return ux, uz
