@@ -19,6 +19,7 @@ FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-munsafe-fp-atomics",
     "-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(HERE, "csrc"),
+    "-L/opt/rocm/lib", "-lhipfft", "-Wl,-rpath,/opt/rocm/lib",
 ]
 
 

@@ -78,10 +78,13 @@ struct fdfk_solver {
     float *ax = nullptr, *bx = nullptr, *az = nullptr, *bz = nullptr;
     std::vector<float> h_ax, h_bx, h_az, h_bz, h_dx, h_dz;
     long strip_cells[3] = {0, 0, 0};
+    Strips *d_strips[2] = {nullptr, nullptr};  // device copies for the two buffer parities
     // background
     float *bg[12] = {};
     long bg_n[12] = {};
     bool bg_valid = false;
+    Background *d_bg = nullptr;
+    float *ringx = nullptr, *ringz = nullptr;
     // receivers
     float *rx = nullptr, *rz = nullptr, *seisx = nullptr, *seisz = nullptr;
     float *stage = nullptr;   // (2, NZ, NX) staging for reference-layout copies
@@ -160,6 +163,29 @@ int build_damping(fdfk_solver *s)
     return 0;
 }
 
+int upload_strips(fdfk_solver *s)
+{
+    for (int par = 0; par < 2; ++par) {
+        Strips h{};
+        for (int st = 0; st < 3; ++st)
+            for (int c = 0; c < 2; ++c) {
+                for (int p = 0; p < 3; ++p) {
+                    h.cur[st][c][p] = s->strip_fields[st][c][p][par];
+                    h.prev[st][c][p] = s->strip_fields[st][c][p][1 - par];
+                }
+                h.psx[st][c] = s->psx[st][c];
+                h.psz[st][c] = s->psz[st][c];
+            }
+        h.ax = s->ax; h.bx = s->bx; h.az = s->az; h.bz = s->bz;
+        if (!s->d_strips[par]) {
+            int rc = s->dalloc(&s->d_strips[par], 1);
+            if (rc) return rc;
+        }
+        HIPCHK(hipMemcpy(s->d_strips[par], &h, sizeof(Strips), hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
 StepArgs step_args(fdfk_solver *s)
 {
     StepArgs a{};
@@ -170,20 +196,29 @@ StepArgs step_args(fdfk_solver *s)
     a.uxp = s->ux[1 - s->cur];
     a.uzp = s->uz[1 - s->cur];
     a.bf = s->bf; a.lam = s->lam; a.mu = s->mu;
-    for (int st = 0; st < 3; ++st)
-        for (int c = 0; c < 2; ++c) {
-            for (int p = 0; p < 3; ++p) {
-                a.st.cur[st][c][p] = s->strip_fields[st][c][p][s->cur];
-                a.st.prev[st][c][p] = s->strip_fields[st][c][p][1 - s->cur];
-            }
-            a.st.psx[st][c] = s->psx[st][c];
-            a.st.psz[st][c] = s->psz[st][c];
-        }
-    a.st.ax = s->ax; a.st.bx = s->bx; a.st.az = s->az; a.st.bz = s->bz;
-    for (int i = 0; i < 12; ++i) a.bg.a[i] = s->bg[i];
-    a.bg.valid = s->bg_valid && s->it <= s->nt;
+    a.st = s->d_strips[s->cur];
+    a.ringx = s->ringx; a.ringz = s->ringz;
+    a.ring_valid = s->bg_valid && s->it <= s->nt;
     a.it = s->it;
     return a;
+}
+
+template <int NO>
+ICoef<NO> make_icoef(const fdfk_solver *s)
+{
+    ICoef<NO> k{};
+    const Geo &g = s->g;
+    const double *c0 = s->hcoef.c0, *c1 = s->hcoef.c1;
+    const int n = g.norder;
+    for (int j = 1; j <= NO; ++j) {
+        double a0 = c0[(j - 1) + n * (NO - 1)], b0 = c1[(j - 1) + n * (NO - 1)];
+        k.ai[j - 1] = (float)a0;
+        k.aox[j - 1] = (float)(2.0 * a0 / ((double)g.dx * g.dx));
+        k.aoz[j - 1] = (float)(2.0 * a0 / ((double)g.dz * g.dz));
+        k.bi[j - 1] = (float)b0;
+        k.bo[j - 1] = (float)(b0 / ((double)g.dx * g.dz));
+    }
+    return k;
 }
 
 template <int NO, int NT>
@@ -192,16 +227,7 @@ void launch_interior_t(fdfk_solver *s)
     IntArgs<NO> a{};
     const Geo &g = s->g;
     a.g = g;
-    const double *c0 = s->hcoef.c0, *c1 = s->hcoef.c1;
-    const int n = g.norder;
-    for (int k = 1; k <= NO; ++k) {
-        double a0 = c0[(k - 1) + n * (NO - 1)], b0 = c1[(k - 1) + n * (NO - 1)];
-        a.k.ai[k - 1] = (float)a0;
-        a.k.aox[k - 1] = (float)(2.0 * a0 / ((double)g.dx * g.dx));
-        a.k.aoz[k - 1] = (float)(2.0 * a0 / ((double)g.dz * g.dz));
-        a.k.bi[k - 1] = (float)b0;
-        a.k.bo[k - 1] = (float)(b0 / ((double)g.dx * g.dz));
-    }
+    a.k = make_icoef<NO>(s);
     a.ux = s->ux[s->cur]; a.uz = s->uz[s->cur];
     a.uxp = s->ux[1 - s->cur]; a.uzp = s->uz[1 - s->cur];
     a.bf = s->bf; a.lam = s->lam; a.mu = s->mu;
@@ -227,16 +253,39 @@ void launch_interior(fdfk_solver *s)
     }
 }
 
-Bands make_bands(const Geo &g)
+void add_rect(Rects &r, int x0, int x1, int z0, int z1)
 {
-    Bands b{};
-    b.W = g.nx + 2 * g.npml - 2;
-    b.ntop = (long)(g.iz0 - 1) * b.W;
-    b.nbot = (long)(g.nz + g.npml - 1 - g.iz1) * b.W;
-    long rows = g.iz1 - g.iz0 + 1;
-    b.nleft = rows * (long)(g.ix0 - (2 - g.npml));
-    b.nright = rows * (long)(g.nx + g.npml - 1 - g.ix1);
-    return b;
+    if (x1 < x0 || z1 < z0) return;
+    int q = r.n++;
+    r.x0[q] = x0; r.x1[q] = x1; r.z0[q] = z0; r.z1[q] = z1;
+    r.off[q + 1] = r.off[q] + (long)(x1 - x0 + 1) * (z1 - z0 + 1);
+}
+
+// full-order frame: the full-order rectangle minus the interior rectangle
+Rects full_rects(const Geo &g)
+{
+    const int NO = g.norder;
+    int xf0 = 2 * NO - g.npml, xf1 = g.nx + g.npml + 1 - 2 * NO, zf1 = g.nz + g.npml + 1 - 2 * NO;
+    Rects r{};
+    add_rect(r, xf0, xf1, g.iz1 + 1, zf1);
+    add_rect(r, xf0, g.ix0 - 1, g.iz0, g.iz1);
+    add_rect(r, g.ix1 + 1, xf1, g.iz0, g.iz1);
+    return r;
+}
+
+// reduced-order edge bands: top rows (free surface), bottom rows and the
+// columns next to the padded-grid boundary
+Rects edge_rects(const Geo &g)
+{
+    const int NO = g.norder;
+    int xf0 = 2 * NO - g.npml, xf1 = g.nx + g.npml + 1 - 2 * NO, zf1 = g.nz + g.npml + 1 - 2 * NO;
+    int xa = 2 - g.npml, xb = g.nx + g.npml - 1;
+    Rects r{};
+    add_rect(r, xa, xb, 1, g.iz0 - 1);
+    add_rect(r, xa, xb, zf1 + 1, g.nz + g.npml - 1);
+    add_rect(r, xa, xf0 - 1, g.iz0, zf1);
+    add_rect(r, xf1 + 1, xb, g.iz0, zf1);
+    return r;
 }
 
 int one_step(fdfk_solver *s, bool record)
@@ -255,14 +304,25 @@ int one_step(fdfk_solver *s, bool record)
         hipLaunchKernelGGL(k_record, dim3((s->nrcv + 63) / 64), dim3(64), 0, s->stream, r);
     }
     StepArgs a = step_args(s);
+    if (a.ring_valid) {
+        long nr = (long)(g.nx + 2 * g.nbd) * (g.nz + g.nbd);
+        hipLaunchKernelGGL(k_ring_prep, dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, s->stream, g, s->d_bg,
+                           s->it, s->ringx, s->ringz);
+    }
     long npsi = 2L * g.npml * g.nz + (long)g.NX * g.npml;
     hipLaunchKernelGGL(k_psi, dim3((unsigned)((npsi + 255) / 256)), dim3(256), 0, s->stream, a);
     if (s->prof) hipEventRecord(e1, s->stream);
     launch_interior(s);
     if (s->prof) hipEventRecord(e2, s->stream);
-    Bands b = make_bands(g);
-    long nb = b.ntop + b.nbot + b.nleft + b.nright;
-    hipLaunchKernelGGL(k_boundary, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, s->stream, a, b);
+    Rects rf = full_rects(g), re = edge_rects(g);
+    dim3 fg((unsigned)((rf.off[rf.n] + 255) / 256)), eg((unsigned)((re.off[re.n] + 255) / 256)), fb(256);
+    if (rf.off[rf.n] > 0) switch (g.norder) {
+    case 1: hipLaunchKernelGGL(k_frame_full<1>, fg, fb, 0, s->stream, a, rf, make_icoef<1>(s)); break;
+    case 2: hipLaunchKernelGGL(k_frame_full<2>, fg, fb, 0, s->stream, a, rf, make_icoef<2>(s)); break;
+    case 3: hipLaunchKernelGGL(k_frame_full<3>, fg, fb, 0, s->stream, a, rf, make_icoef<3>(s)); break;
+    default: hipLaunchKernelGGL(k_frame_full<4>, fg, fb, 0, s->stream, a, rf, make_icoef<4>(s)); break;
+    }
+    hipLaunchKernelGGL(k_frame_edge, eg, fb, 0, s->stream, a, re);
     if (s->prof) {
         hipEventRecord(e3, s->stream);
         s->ev_int.emplace_back(e1, e2);
@@ -377,6 +437,7 @@ int fdfk_create(const fdfk_job *job, fdfk_solver **out)
         }
     }
     if ((rc = build_damping(s))) return bad(rc);
+    if ((rc = upload_strips(s))) return bad(rc);
     // receivers
     if (s->nrcv > 0) {
         if ((rc = s->dalloc(&s->rx, s->nrcv)) || (rc = s->dalloc(&s->rz, s->nrcv)) ||
@@ -403,18 +464,40 @@ static void bg_sizes(const Geo &g, int nt, long n[12])
     for (int i = 0; i < 12; ++i) n[i] = v[i];
 }
 
+// device storage of the twelve background arrays, their pointer table and
+// the two full-grid ring arrays
+static int alloc_background(fdfk_solver *s)
+{
+    if (s->d_bg) return 0;
+    long n[12];
+    bg_sizes(s->g, s->nt, n);
+    Background h{};
+    for (int i = 0; i < 12; ++i) {
+        int rc = s->dalloc(&s->bg[i], (size_t)n[i]);
+        if (rc) return rc;
+        s->bg_n[i] = n[i];
+        h.a[i] = s->bg[i];
+    }
+    h.valid = 1;
+    const size_t nf = (size_t)s->g.NX * s->g.ld;
+    int rc;
+    if ((rc = s->dalloc(&s->d_bg, 1)) || (rc = s->dalloc(&s->ringx, nf)) || (rc = s->dalloc(&s->ringz, nf)))
+        return rc;
+    HIPCHK(hipMemcpy(s->d_bg, &h, sizeof(Background), hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(s->ringx, 0, nf * 4));
+    HIPCHK(hipMemset(s->ringz, 0, nf * 4));
+    return 0;
+}
+
 int fdfk_set_background(fdfk_solver *s, const float *const bg[12])
 {
     if (!s || !bg) return fail(FDFK_EINVAL, "null argument");
     long n[12];
     bg_sizes(s->g, s->nt, n);
+    int rc = alloc_background(s);
+    if (rc) return rc;
     for (int i = 0; i < 12; ++i) {
         if (!bg[i]) return fail(FDFK_EINVAL, "null background array");
-        if (!s->bg[i]) {
-            int rc = s->dalloc(&s->bg[i], (size_t)n[i]);
-            if (rc) return rc;
-            s->bg_n[i] = n[i];
-        }
         HIPCHK(hipMemcpyAsync(s->bg[i], bg[i], (size_t)n[i] * 4, hipMemcpyHostToDevice, s->stream));
     }
     HIPCHK(hipStreamSynchronize(s->stream));

@@ -1,15 +1,429 @@
-// FK background on the GPU (FP64): placeholder until the propagator kernels land.
+// FK background on the GPU in FP64: the comput_background_wavefield role
+// (src/FDFK2D.f90:432-434, binary-only libFK3D in the reference).
+//
+// Plane P or SV wave incident from the half-space of a 1-D layered model
+// with a free surface (this project's formulation, z down, e^{iw(px+qz-t)}):
+//   k_fk_amp    one thread per frequency: propagate the incident and the two
+//               reflected half-space waves up through the layer matrices
+//               Q_j = E_j^-1 E_{j+1}, impose zero traction at z = 0, keep the
+//               four amplitudes of every layer
+//   k_fk_spec   one thread per (frequency, depth): displacement spectrum
+//               E_j[0:2] Lambda_j(z - z_j) a_j at each ring depth
+//   hipFFT Z2D  frequency -> time per depth (complex frequency w + i eps
+//               against wrap-around, undone by e^{eps t})
+//   k_fk_fill   every ring point and iteration: the depth trace delayed by
+//               p |x - x_entry| (plane-wave translation invariance), 4-point
+//               cubic Lagrange interpolation, written in the reference's
+//               twelve-array layout (src/FDFK2D.f90:330-355)
+// Incident pulse ds exp(-(pi f0 (t - t0))^2), t0 = 1.5/f0, centred at the
+// entry column at depth max(ring bottom, half-space top).
+#include <hipfft/hipfft.h>
+
+namespace fdfk {
+
+struct zc {
+    double r, i;
+};
+__host__ __device__ inline zc zmk(double r, double i) { return zc{r, i}; }
+__host__ __device__ inline zc operator+(zc a, zc b) { return zc{a.r + b.r, a.i + b.i}; }
+__host__ __device__ inline zc operator-(zc a, zc b) { return zc{a.r - b.r, a.i - b.i}; }
+__host__ __device__ inline zc operator*(zc a, zc b) { return zc{a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r}; }
+__host__ __device__ inline zc operator*(double s, zc a) { return zc{s * a.r, s * a.i}; }
+__host__ __device__ inline zc zdiv(zc a, zc b)
+{
+    double d = b.r * b.r + b.i * b.i;
+    return zc{(a.r * b.r + a.i * b.i) / d, (a.i * b.r - a.r * b.i) / d};
+}
+__host__ __device__ inline zc zexp(zc a)
+{
+    double e = exp(a.r);
+    return zc{e * cos(a.i), e * sin(a.i)};
+}
+__host__ __device__ inline zc zconj(zc a) { return zc{a.r, -a.i}; }
+__host__ __device__ inline double zabs(zc a) { return hypot(a.r, a.i); }
+__host__ __device__ inline zc zI(zc a) { return zc{-a.i, a.r}; }  // i*a
+
+constexpr int kMaxLayer = 16;
+
+struct FKModel {
+    int nl;
+    double zt[kMaxLayer];
+    zc ea[kMaxLayer], eb[kMaxLayer];
+    zc E[kMaxLayer][4][4];
+    zc Q[kMaxLayer][4][4];
+    double tref;
+};
+
+static zc vslow_h(double v, double p)
+{
+    double a = 1.0 / (v * v) - p * p;
+    return a >= 0.0 ? zmk(std::sqrt(a), 0.0) : zmk(0.0, std::sqrt(-a));
+}
+
+// E (rows ux, uz, szz/iw, sxz/iw; columns P down, S down, P up, S up)
+static void emat_h(double vp, double vs, double rho, double p, zc ea, zc eb, zc E[4][4])
+{
+    double g = 1.0 - 2.0 * vs * vs * p * p;
+    E[0][0] = zmk(vp * p, 0); E[0][1] = vs * eb; E[0][2] = zmk(vp * p, 0); E[0][3] = vs * eb;
+    E[1][0] = vp * ea; E[1][1] = zmk(-vs * p, 0); E[1][2] = -vp * ea; E[1][3] = zmk(vs * p, 0);
+    E[2][0] = zmk(rho * vp * g, 0); E[2][1] = (-2.0 * rho * vs * vs * vs * p) * eb;
+    E[2][2] = zmk(rho * vp * g, 0); E[2][3] = (-2.0 * rho * vs * vs * vs * p) * eb;
+    E[3][0] = (2.0 * rho * vs * vs * vp * p) * ea; E[3][1] = zmk(rho * vs * g, 0);
+    E[3][2] = (-2.0 * rho * vs * vs * vp * p) * ea; E[3][3] = zmk(-rho * vs * g, 0);
+}
+
+// A X = B, 4x4 complex, B is 4x4 (Gaussian elimination, partial pivoting)
+static bool solve4_h(zc A[4][4], zc B[4][4])
+{
+    for (int c = 0; c < 4; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < 4; ++r)
+            if (zabs(A[r][c]) > zabs(A[piv][c])) piv = r;
+        if (zabs(A[piv][c]) == 0.0) return false;
+        if (piv != c)
+            for (int k = 0; k < 4; ++k) { std::swap(A[c][k], A[piv][k]); std::swap(B[c][k], B[piv][k]); }
+        for (int r = c + 1; r < 4; ++r) {
+            zc f = zdiv(A[r][c], A[c][c]);
+            for (int k = c; k < 4; ++k) A[r][k] = A[r][k] - f * A[c][k];
+            for (int k = 0; k < 4; ++k) B[r][k] = B[r][k] - f * B[c][k];
+        }
+    }
+    for (int c = 3; c >= 0; --c)
+        for (int k = 0; k < 4; ++k) {
+            zc s = B[c][k];
+            for (int j = c + 1; j < 4; ++j) s = s - A[c][j] * B[j][k];
+            B[c][k] = zdiv(s, A[c][c]);
+        }
+    return true;
+}
+
+static bool fk_model_setup(FKModel &m, int nl, const float *fk, double p)
+{
+    if (nl < 1 || nl > kMaxLayer) return false;
+    m.nl = nl;
+    for (int j = 0; j < nl; ++j) {
+        double zt = fk[4 * j], vp = fk[4 * j + 1], vs = fk[4 * j + 2], rho = fk[4 * j + 3];
+        if (!(vs > 0.0) || !(vp > vs) || !(rho > 0.0)) return false;
+        m.zt[j] = zt;
+        m.ea[j] = vslow_h(vp, p);
+        m.eb[j] = vslow_h(vs, p);
+        emat_h(vp, vs, rho, p, m.ea[j], m.eb[j], m.E[j]);
+    }
+    for (int j = 0; j + 1 < nl; ++j) {
+        zc A[4][4], B[4][4];
+        for (int r = 0; r < 4; ++r)
+            for (int c = 0; c < 4; ++c) { A[r][c] = m.E[j][r][c]; B[r][c] = m.E[j + 1][r][c]; }
+        if (!solve4_h(A, B)) return false;
+        for (int r = 0; r < 4; ++r)
+            for (int c = 0; c < 4; ++c) m.Q[j][r][c] = B[r][c];
+    }
+    return true;
+}
+
+struct FKRun {
+    int nfft, nk;          // nk = nfft/2 frequencies used (Nyquist dropped)
+    double T, eps, dtf;
+    double f0, ds;
+    int src_is_p;
+    int nzq;               // ring depths z0 + q dz, q = 0..nzq-1
+};
+
+// per frequency: amplitudes of all layers, amps[(k*nl + j)*4 + c]
+__global__ void k_fk_amp(const FKModel *mp, FKRun r, zc *amps, zc *work)
+{
+    const FKModel &m = *mp;
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= r.nk) return;
+    const int N = m.nl;
+    const double PI = 3.14159265358979323846;
+    zc w = zmk(2.0 * PI * k / r.T, r.eps);
+    zc x = zmk(w.r / (2.0 * PI * r.f0), w.i / (2.0 * PI * r.f0));
+    zc S = (r.ds / (r.f0 * sqrt(PI))) * zexp(zmk(-(x.r * x.r - x.i * x.i), -2.0 * x.r * x.i));
+    zc ainc = S * zexp(zI(w * zmk(m.tref, 0.0)));
+    // basis amplitudes of each layer: work[((k*N + j)*3 + b)*4 + c]
+    zc *ab = work + (size_t)k * N * 12;
+    for (int b = 0; b < 3; ++b)
+        for (int c = 0; c < 4; ++c) ab[((N - 1) * 3 + b) * 4 + c] = zmk(0, 0);
+    ab[((N - 1) * 3 + 0) * 4 + (r.src_is_p ? 2 : 3)] = zmk(1, 0);
+    ab[((N - 1) * 3 + 1) * 4 + 0] = zmk(1, 0);
+    ab[((N - 1) * 3 + 2) * 4 + 1] = zmk(1, 0);
+    for (int j = N - 2; j >= 0; --j) {
+        double h = m.zt[j + 1] - m.zt[j];
+        zc iwh_a = zI(w * m.ea[j]), iwh_b = zI(w * m.eb[j]);
+        zc inv[4] = {zexp(-h * iwh_a), zexp(-h * iwh_b), zexp(h * iwh_a), zexp(h * iwh_b)};
+        for (int b = 0; b < 3; ++b)
+            for (int rr = 0; rr < 4; ++rr) {
+                zc s = zmk(0, 0);
+                for (int c = 0; c < 4; ++c) s = s + m.Q[j][rr][c] * ab[((j + 1) * 3 + b) * 4 + c];
+                ab[(j * 3 + b) * 4 + rr] = inv[rr] * s;
+            }
+    }
+    zc T[3][2];
+    for (int b = 0; b < 3; ++b)
+        for (int rr = 0; rr < 2; ++rr) {
+            zc s = zmk(0, 0);
+            for (int c = 0; c < 4; ++c) s = s + m.E[0][2 + rr][c] * ab[(0 * 3 + b) * 4 + c];
+            T[b][rr] = s;
+        }
+    zc det = T[1][0] * T[2][1] - T[2][0] * T[1][1];
+    zc r0 = zmk(0, 0) - T[0][0] * ainc, r1 = zmk(0, 0) - T[0][1] * ainc;
+    zc x1 = zdiv(r0 * T[2][1] - T[2][0] * r1, det);
+    zc x2 = zdiv(T[1][0] * r1 - r0 * T[1][1], det);
+    for (int j = 0; j < N; ++j)
+        for (int c = 0; c < 4; ++c)
+            amps[((size_t)k * N + j) * 4 + c] =
+                ainc * ab[(j * 3 + 0) * 4 + c] + x1 * ab[(j * 3 + 1) * 4 + c] + x2 * ab[(j * 3 + 2) * 4 + c];
+}
+
+// displacement spectra at the ring depths, conjugated for the e^{+} inverse
+// transform of hipFFT: spec[((comp*nzq + q)*(nfft/2+1)) + k]
+__global__ void k_fk_spec(const FKModel *mp, FKRun r, const zc *amps, const int *lay, const double *dzl, zc *spec)
+{
+    const FKModel &m = *mp;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long nkp = r.nfft / 2 + 1;
+    if (t >= (long)r.nzq * nkp) return;
+    int q = (int)(t / nkp), k = (int)(t % nkp);
+    zc ux = zmk(0, 0), uz = zmk(0, 0);
+    if (k < r.nk) {
+        const double PI = 3.14159265358979323846;
+        zc w = zmk(2.0 * PI * k / r.T, r.eps);
+        int j = lay[q];
+        double d = dzl[q];
+        zc iwa = zI(w * m.ea[j]), iwb = zI(w * m.eb[j]);
+        zc ph[4] = {zexp(d * iwa), zexp(d * iwb), zexp(-d * iwa), zexp(-d * iwb)};
+        const zc *a = amps + ((size_t)k * m.nl + j) * 4;
+        for (int c = 0; c < 4; ++c) {
+            zc pa = ph[c] * a[c];
+            ux = ux + m.E[j][0][c] * pa;
+            uz = uz + m.E[j][1][c] * pa;
+        }
+    }
+    spec[(size_t)q * nkp + k] = zconj(ux);
+    spec[((size_t)r.nzq + q) * nkp + k] = zconj(uz);
+}
+
+// undo the complex-frequency damping and the 1/T of the discrete inverse
+__global__ void k_fk_scale(FKRun r, double *g, long ntr)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntr * r.nfft) return;
+    int n = (int)(t % r.nfft);
+    g[t] = g[t] * exp(r.eps * n * r.dtf) / r.T;
+}
+
+__device__ inline double fk_interp(const double *g, int nfft, double dtf, double t)
+{
+    if (t < 0.0) return 0.0;
+    double u = t / dtf;
+    int i = (int)floor(u);
+    double f = u - i;
+    if (i + 2 >= nfft / 2) return 0.0;
+    double gm1 = i >= 1 ? g[i - 1] : 0.0, g0 = g[i], g1 = g[i + 1], g2 = g[i + 2];
+    double wm1 = -f * (f - 1.0) * (f - 2.0) / 6.0, w0 = (f + 1.0) * (f - 1.0) * (f - 2.0) / 2.0;
+    double w1 = -(f + 1.0) * f * (f - 2.0) / 2.0, w2 = (f + 1.0) * f * (f - 1.0) / 6.0;
+    return wm1 * gm1 + w0 * g0 + w1 * g1 + w2 * g2;
+}
+
+struct FillArgs {
+    Geo g;
+    int nt, nfft, nzq;
+    double dt, dtf, x0, dx, p, sgn, xe;
+    const double *gl, *gr;      // time traces [comp][q][nfft] of the left / right model
+    float *ax, *az;             // destination arrays (one strip)
+    int strip;                  // 0 L reg, 1 R reg, 2 B reg, 3 L pml, 4 R pml, 5 B pml
+    int rows, cols, row0, col0; // strip shape: rows row0.., columns col0..
+};
+
+__global__ void k_fk_fill(FillArgs f)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long per = (long)f.rows * f.cols;
+    if (t >= per * (f.nt + 1)) return;
+    int it = (int)(t / per);
+    long rem = t % per;
+    int ix = f.col0 + (int)(rem / f.rows), iz = f.row0 + (int)(rem % f.rows);
+    double x = f.x0 + (double)(ix - 1) * f.dx;
+    double tau = f.p * (f.sgn > 0 ? (x - f.xe) : (f.xe - x));
+    double tt = it * f.dt - tau;
+    int q = iz - 1;
+    size_t nz2 = (size_t)f.nzq * f.nfft;
+    const double *gxl = f.gl + (size_t)q * f.nfft, *gzl = f.gl + nz2 + (size_t)q * f.nfft;
+    const double *gxr = f.gr + (size_t)q * f.nfft, *gzr = f.gr + nz2 + (size_t)q * f.nfft;
+    double bx, bz;
+    bool side = (f.strip == 0 || f.strip == 1 || f.strip == 3 || f.strip == 4);
+    if (side) {
+        bool left = (f.strip == 0 || f.strip == 3);
+        bx = fk_interp(left ? gxl : gxr, f.nfft, f.dtf, tt);
+        bz = fk_interp(left ? gzl : gzr, f.nfft, f.dtf, tt);
+    } else {
+        double w = (double)(ix - 1) / (double)(f.g.nx - 1);
+        w = w < 0.0 ? 0.0 : (w > 1.0 ? 1.0 : w);
+        double lx = fk_interp(gxl, f.nfft, f.dtf, tt), lz = fk_interp(gzl, f.nfft, f.dtf, tt);
+        double rx = fk_interp(gxr, f.nfft, f.dtf, tt), rz = fk_interp(gzr, f.nfft, f.dtf, tt);
+        bx = (1.0 - w) * lx + w * rx;
+        bz = (1.0 - w) * lz + w * rz;
+    }
+    size_t o = (size_t)(iz - f.row0) + (size_t)f.rows * ((size_t)(ix - f.col0) + (size_t)f.cols * it);
+    f.ax[o] = (float)(f.sgn * bx);
+    f.az[o] = (float)bz;
+}
+
+}  // namespace fdfk
+
+namespace {
+
+struct DevBuf {
+    void *p = nullptr;
+    ~DevBuf() { if (p) hipFree(p); }
+    template <class T> T *as() { return (T *)p; }
+};
+
+int next_pow2(long n)
+{
+    int p = 1;
+    while (p < n) p <<= 1;
+    return p;
+}
+
+// traces of one model on the device: g[comp][q][nfft]
+int fk_traces_dev(fdfk_solver *s, FKModel &m, const FKRun &r, const double *zq, double *g_out)
+{
+    const int nzq = r.nzq;
+    std::vector<int> lay(nzq);
+    std::vector<double> dzl(nzq);
+    for (int q = 0; q < nzq; ++q) {
+        int j = 0;
+        while (j + 1 < m.nl && zq[q] >= m.zt[j + 1]) ++j;
+        lay[q] = j;
+        dzl[q] = zq[q] - m.zt[j];
+    }
+    DevBuf amps, work, dlay, ddz, spec, dm;
+    const long nkp = r.nfft / 2 + 1;
+    HIPCHK(hipMalloc(&dm.p, sizeof(FKModel)));
+    HIPCHK(hipMemcpy(dm.p, &m, sizeof(FKModel), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&amps.p, (size_t)r.nk * m.nl * 4 * sizeof(zc)));
+    HIPCHK(hipMalloc(&work.p, (size_t)r.nk * m.nl * 12 * sizeof(zc)));
+    HIPCHK(hipMalloc(&dlay.p, nzq * sizeof(int)));
+    HIPCHK(hipMalloc(&ddz.p, nzq * sizeof(double)));
+    HIPCHK(hipMalloc(&spec.p, (size_t)2 * nzq * nkp * sizeof(zc)));
+    HIPCHK(hipMemcpy(dlay.p, lay.data(), nzq * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ddz.p, dzl.data(), nzq * sizeof(double), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_fk_amp, dim3((r.nk + 127) / 128), dim3(128), 0, s->stream, dm.as<FKModel>(), r, amps.as<zc>(),
+                       work.as<zc>());
+    long ns = (long)nzq * nkp;
+    hipLaunchKernelGGL(k_fk_spec, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, s->stream, dm.as<FKModel>(), r,
+                       amps.as<zc>(), dlay.as<int>(), ddz.as<double>(), spec.as<zc>());
+    hipfftHandle plan;
+    int n[1] = {r.nfft};
+    if (hipfftPlanMany(&plan, 1, n, nullptr, 1, (int)nkp, nullptr, 1, r.nfft, HIPFFT_Z2D, 2 * nzq) != HIPFFT_SUCCESS)
+        return fail(FDFK_EDEVICE, "hipfftPlanMany failed");
+    hipfftSetStream(plan, s->stream);
+    hipfftResult fr = hipfftExecZ2D(plan, (hipfftDoubleComplex *)spec.p, (hipfftDoubleReal *)g_out);
+    hipStreamSynchronize(s->stream);
+    hipfftDestroy(plan);
+    if (fr != HIPFFT_SUCCESS) return fail(FDFK_EDEVICE, "hipfftExecZ2D failed");
+    long tot = 2L * nzq * r.nfft;
+    hipLaunchKernelGGL(k_fk_scale, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s->stream, r, g_out,
+                       2L * nzq);
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+}  // namespace
+
 extern "C" int fdfk_compute_background(fdfk_solver *s, const fdfk_fk *fk)
 {
-    (void)s; (void)fk;
-    return fail(FDFK_ENOSYS, "GPU FK background not built yet");
+    if (!s || !fk || !fk->fk_left || !fk->fk_right || !fk->bazs) return fail(FDFK_EINVAL, "null argument");
+    if (!(fk->f0 > 0)) return fail(FDFK_EINVAL, "f0 must be > 0");
+    const Geo &g = s->g;
+    const int nbd = g.nbd, nx = g.nx, nz = g.nz, nt = s->nt;
+    double cs = 0.0;
+    for (int i = 0; i < g.NX; ++i) cs += std::cos(fk->bazs[i]);
+    double rayp = fk->rayp;
+    if (std::fabs(rayp) < 1e-9) rayp = rayp < 0 ? -1e-9 : 1e-9;  // src/FDFK2D.f90:431
+    const double sgn = (rayp < 0.0 ? -1.0 : 1.0) * (cs < 0.0 ? -1.0 : 1.0);
+    const double p = std::fabs(rayp);
+    const double f0 = fk->f0, t0 = 1.5 / f0, dt = g.dt;
+    FKModel ml{}, mr{};
+    if (!fk_model_setup(ml, fk->nlayer_left, fk->fk_left, p) || !fk_model_setup(mr, fk->nlayer_right, fk->fk_right, p))
+        return fail(FDFK_EINVAL, "invalid FK model (need 1..16 solid layers, vp > vs > 0)");
+    const int nzq = nz + nbd;
+    std::vector<double> zq(nzq);
+    for (int q = 0; q < nzq; ++q) zq[q] = (double)s->z0 + (double)q * g.dz;
+    const double xl = (double)s->x0 + (double)(1 - nbd - 1) * g.dx, xr = (double)s->x0 + (double)(nx + nbd - 1) * g.dx;
+    const double xe = sgn > 0 ? xl : xr;
+    const double zref = std::max(std::max(zq[nzq - 1], ml.zt[ml.nl - 1]), std::max(zq[nzq - 1], mr.zt[mr.nl - 1]));
+    FKRun r{};
+    r.dtf = dt / 2.0;
+    const double tneed = nt * dt + p * (xr - xl) + 2.0 * t0 + 10.0 / f0;
+    r.nfft = next_pow2((long)std::ceil(2.0 * tneed / r.dtf));
+    r.nk = r.nfft / 2;
+    r.T = r.nfft * r.dtf;
+    r.eps = 27.6 / r.T;
+    r.f0 = f0; r.ds = fk->ds; r.src_is_p = fk->src_is_p ? 1 : 0; r.nzq = nzq;
+    ml.tref = t0 + (r.src_is_p ? ml.ea[ml.nl - 1].r : ml.eb[ml.nl - 1].r) * (zref - ml.zt[ml.nl - 1]);
+    mr.tref = t0 + (r.src_is_p ? mr.ea[mr.nl - 1].r : mr.eb[mr.nl - 1].r) * (zref - mr.zt[mr.nl - 1]);
+    const bool same = fk->nlayer_left == fk->nlayer_right &&
+                      std::memcmp(fk->fk_left, fk->fk_right, sizeof(float) * 4 * fk->nlayer_left) == 0;
+    int rc = alloc_background(s);
+    if (rc) return rc;
+    DevBuf gl, gr;
+    const size_t ntr = (size_t)2 * nzq * r.nfft;
+    HIPCHK(hipMalloc(&gl.p, ntr * sizeof(double)));
+    if ((rc = fk_traces_dev(s, ml, r, zq.data(), gl.as<double>()))) return rc;
+    if (!same) {
+        HIPCHK(hipMalloc(&gr.p, ntr * sizeof(double)));
+        if ((rc = fk_traces_dev(s, mr, r, zq.data(), gr.as<double>()))) return rc;
+    }
+    FillArgs f{};
+    f.g = g; f.nt = nt; f.nfft = r.nfft; f.nzq = nzq; f.dt = dt; f.dtf = r.dtf;
+    f.x0 = s->x0; f.dx = g.dx; f.p = p; f.sgn = sgn; f.xe = xe;
+    f.gl = gl.as<double>(); f.gr = same ? gl.as<double>() : gr.as<double>();
+    // strip shapes: (rows, cols, row0, col0) of the twelve arrays, FDFK2D.f90:330-355
+    const int shp[6][4] = {{nz - nbd, nbd, 1, 1}, {nz - nbd, nbd, 1, nx - nbd + 1}, {nbd, nx, nz - nbd + 1, 1},
+                           {nz, nbd, 1, 1 - nbd}, {nz, nbd, 1, nx + 1}, {nbd, nx + 2 * nbd, nz + 1, 1 - nbd}};
+    for (int k = 0; k < 6; ++k) {
+        f.strip = k;
+        f.rows = shp[k][0]; f.cols = shp[k][1]; f.row0 = shp[k][2]; f.col0 = shp[k][3];
+        f.ax = s->bg[2 * k]; f.az = s->bg[2 * k + 1];
+        long n = (long)f.rows * f.cols * (nt + 1);
+        hipLaunchKernelGGL(k_fk_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s->stream, f);
+    }
+    HIPCHK(hipStreamSynchronize(s->stream));
+    HIPCHK(hipGetLastError());
+    s->bg_valid = true;
+    return 0;
 }
-extern "C" void comput_background_wavefield_(const int32_t *, const int32_t *, const int32_t *,
-    const int32_t *, const int32_t *, const int32_t *, const int32_t *, const int32_t *,
-    const int32_t *, const float *, const float *, const float *, const float *, const double *,
-    const float *, const float *, const double *, const double *, const float *, const float *,
-    float *, float *, float *, float *, float *, float *, float *, float *, float *, float *,
-    float *, float *)
+
+extern "C" void comput_background_wavefield_(const int32_t *nx, const int32_t *nz, const int32_t *npml,
+    const int32_t *izt, const int32_t *nbd, const int32_t *nt, const int32_t *nlayer_left,
+    const int32_t *nlayer_right, const int32_t *src_type, const float *x0, const float *z0,
+    const float *dx, const float *dz, const double *dt, const float *ds, const float *f0,
+    const double *rayp, const double *bazs, const float *fk_left, const float *fk_right,
+    float *uxl_reg, float *uzl_reg, float *uxr_reg, float *uzr_reg, float *uxb_reg,
+    float *uzb_reg, float *uxl_pml, float *uzl_pml, float *uxr_pml, float *uzr_pml,
+    float *uxb_pml, float *uzb_pml)
 {
-    std::fprintf(stderr, "comput_background_wavefield (fdfk2d_amd): not built yet\n");
+    auto die = [](const char *m) { std::fprintf(stderr, "comput_background_wavefield (fdfk2d_amd): %s\n", m); };
+    if (*izt != 1) { die("only is_PML_top = .false. is supported"); return; }
+    int norder = (*nbd + 1) / 2;
+    // a homogeneous dummy material: only geometry and time axis matter here
+    const int NX = *nx + 2 * *npml, NZ = *nz + *npml;
+    std::vector<float> c((size_t)3 * NX * NZ);
+    for (size_t i = 0; i < c.size(); i += 3) { c[i] = 2000.f; c[i + 1] = 2e10f; c[i + 2] = 1e10f; }
+    fdfk_job j{};
+    j.nx = *nx; j.nz = *nz; j.npml = *npml; j.norder = norder; j.npower = 2;
+    j.x0 = *x0; j.z0 = *z0; j.dx = *dx; j.dz = *dz; j.dt = *dt; j.R0 = 1e-3; j.vpml = 1.f;
+    j.nt = *nt; j.nstp = 1; j.c = c.data(); j.precision = FDFK_FP32;
+    fdfk_solver *s = nullptr;
+    if (fdfk_create(&j, &s)) { die(fdfk_last_error()); return; }
+    fdfk_fk fk{};
+    fk.nlayer_left = *nlayer_left; fk.nlayer_right = *nlayer_right;
+    fk.fk_left = fk_left; fk.fk_right = fk_right; fk.src_is_p = *src_type != 0;
+    fk.ds = *ds; fk.f0 = *f0; fk.rayp = *rayp; fk.bazs = bazs;
+    float *const out[12] = {uxl_reg, uzl_reg, uxr_reg, uzr_reg, uxb_reg, uzb_reg,
+                            uxl_pml, uzl_pml, uxr_pml, uzr_pml, uxb_pml, uzb_pml};
+    if (fdfk_compute_background(s, &fk) || fdfk_get_background(s, out)) die(fdfk_last_error());
+    fdfk_destroy(s);
 }

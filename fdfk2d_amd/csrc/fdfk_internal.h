@@ -65,22 +65,36 @@ struct Background {
     int valid;
 };
 
+// full-order FP32 coefficients of the fused stencils
+template <int NO>
+struct ICoef {
+    float ai[NO];   // staggered inner  a_j = coef0(j,NO)
+    float aox[NO];  // x outer 2*coef0(k,NO)/dx^2
+    float aoz[NO];  // z outer 2*coef0(k,NO)/dz^2
+    float bi[NO];   // central inner    b_j = coef1(j,NO)
+    float bo[NO];   // mixed outer      coef1(k,NO)/(dx*dz)
+};
+
+// Strips and Background live in device memory (read through a pointer) so
+// that runtime strip / array indices never force the tables into scratch.
 struct StepArgs {
     Geo g;
     const Coefs *cf;
     const float *ux, *uz;          // level n
     float *uxp, *uzp;              // level n-1 in, n+1 out
     const float *bf, *lam, *mu;    // dt^2/rho, lambda, mu
-    Strips st;
-    Background bg;
+    const Strips *st;
+    const float *ringx, *ringz;    // background u0 on the hybrid ring at this step (0 elsewhere)
+    int ring_valid;
     int it;
 };
 
-// boundary-kernel work decomposition: the updated cells outside the
-// interior rectangle, as four bands (top, bottom, left, right)
-struct Bands {
-    long ntop, nbot, nleft, nright;
-    int W;  // updated columns 2-npml .. nx+npml-1
+// frame-kernel work decomposition: up to four rectangles of cells
+// (columns x0..x1, rows z0..z1), enumerated z fastest; off[] are prefix counts
+struct Rects {
+    int n;
+    int x0[4], x1[4], z0[4], z1[4];
+    long off[5];
 };
 
 struct RecArgs {
@@ -90,16 +104,6 @@ struct RecArgs {
     const float *rx, *rz;
     const float *ux, *uz;
     float *seisx, *seisz;  // (nsamp, nrcv), time fastest
-};
-
-// full-order FP32 coefficients of the interior kernel
-template <int NO>
-struct ICoef {
-    float ai[NO];   // staggered inner  a_j = coef0(j,NO)
-    float aox[NO];  // x outer 2*coef0(k,NO)/dx^2
-    float aoz[NO];  // z outer 2*coef0(k,NO)/dz^2
-    float bi[NO];   // central inner    b_j = coef1(j,NO)
-    float bo[NO];   // mixed outer      coef1(k,NO)/(dx*dz)
 };
 
 template <int NO>

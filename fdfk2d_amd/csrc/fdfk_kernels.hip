@@ -2,10 +2,11 @@
 // (src/FDFK2D.f90:516-522, binary-only in the reference) and the receiver
 // recorder loadseis (src/Submain.f90:33-116).
 //
-// One step n -> n+1 is three launches on the handle's stream:
+// One step n -> n+1 is four launches on the handle's stream:
+//   k_ring_prep background u0 on the hybrid ring at this step    ring only
 //   k_psi       first-derivative PML memories psi (level n)       strips only
-//   k_interior  full-order fused stencil + leapfrog                 ~95% of cells
-//   k_boundary  everything else: free-surface rows, reduced-order   the frame
+//   k_interior  full-order fused stencil + leapfrog                 ~97% of cells
+//   k_frame     everything else: free-surface rows, reduced-order   the frame
 //               edge stencils, split-field PML, hybrid TF/SF ring
 // and, on recording iterations, k_record before them.
 //
@@ -15,10 +16,13 @@
 // through LDS.  It reads u^n, u^{n-1}, dt^2/rho, lambda, mu once and writes
 // u^{n+1} in place of u^{n-1}: 36 algorithmic bytes per cell-update in FP32.
 //
-// k_boundary, k_psi and k_record reproduce the CPU oracle's arithmetic
-// exactly (FP64 accumulation rounded to FP32 after every term, the
-// Fortran REAL(4)/REAL(8) mixing of src/Submain.f90), so on the frame the
-// GPU and the oracle agree to the last bit given the same inputs.
+// k_frame evaluates full-order cells with the same FP32 shared-flux
+// stencil as k_interior and the cells next to the padded-grid edges and the
+// free surface (reduced-order stencils, src/Submain.f90:192-195) with the
+// CPU oracle's exact arithmetic (FP64 accumulation rounded to FP32 after
+// every term, the Fortran REAL(4)/REAL(8) mixing); k_psi and k_record use
+// the exact arithmetic too.  The hybrid TF/SF correction enters as reads of
+// the per-step ring field (k_ring_prep) instead of a second stencil pass.
 #include "fdfk_internal.h"
 
 namespace fdfk {
@@ -34,14 +38,9 @@ __device__ __forceinline__ float accd(float s, double c, float d)
     return (float)((double)s + c * (double)d);
 }
 __device__ __forceinline__ float hmean(float a, float b) { return 2.0f / (1.0f / a + 1.0f / b); }
-__device__ __forceinline__ float meff_of(float lam, float mu)
-{
-    float m = lam + 2.0f * mu;
-    return m != 0.0f ? (4.0f * mu) * (lam + mu) / m : 0.0f;
-}
 __device__ __forceinline__ float matval(int mode, float lam, float mu)
 {
-    return mode == 0 ? lam + 2.0f * mu : (mode == 1 ? mu : meff_of(lam, mu));
+    return mode == 0 ? lam + 2.0f * mu : mu;
 }
 
 struct FieldRd {
@@ -82,18 +81,28 @@ __device__ __forceinline__ long bg_off(const Geo &g, int iz, int ix, int it, int
     return -1;
 }
 
-// background value of one component restricted to the inside (which = 0)
-// or outside (which = 1) ring strips, zero elsewhere
-struct MaskRd {
-    const Background *bg;
+// side of the hybrid interface: 0 total-field region (1..nx x 1..nz), 1 padding
+__device__ __forceinline__ int tf_side(const Geo &g, int iz, int ix)
+{
+    return (ix >= 1 && ix <= g.nx && iz <= g.nz) ? 0 : 1;
+}
+
+// Field read for a ring cell of side sigma: neighbours on the other side
+// are seen in the output cell's representation, i.e. the inside ring reads
+// u + u0 outside (total field) and the outside ring reads u - u0 inside
+// (scattered field).  By linearity this equals L[u] +/- L[u0 masked], the
+// TF/SF correction of the hybrid method.
+// sigma = -1: plain read (cells outside the ring)
+struct CorrRd {
+    const float *p, *ring;
     const Geo *g;
-    int it, which, comp;
+    int sigma;
     __device__ float operator()(int iz, int ix) const
     {
-        int arr, wh;
-        long o = bg_off(*g, iz, ix, it, arr, wh);
-        if (o < 0 || wh != which) return 0.0f;
-        return bg->a[6 * wh + 2 * arr + comp][o];
+        size_t c = cell(*g, iz, ix);
+        float v = p[c];
+        if (sigma >= 0 && tf_side(*g, iz, ix) != sigma) v = sigma == 0 ? v + ring[c] : v - ring[c];
+        return v;
     }
 };
 
@@ -197,20 +206,31 @@ __device__ void eval_parts(const Geo &g, const Coefs *cf, const float *lam, cons
                            const R &ux, const R &uz, int iz, int ix, float p[6])
 {
     if (iz == 1) {
-        // half-cell traction-free closure of the surface row
+        // free-surface row: half-cell closure derived from the strain energy
+        // of the top layer (rows 1-2, trapezoidal in z); traction free naturally
         size_t a = cell(g, 1, ix), b = cell(g, 2, ix);
         float sdz2 = 2.0f / (g.dz * g.dz);
         float M1 = lam[a] + 2.0f * mu[a], M2 = lam[b] + 2.0f * mu[b];
-        p[0] = x2_mode(g, cf, lam, mu, 2, ux, iz, ix);
+        const int no = g.norder;
+        int nox = imin(imin(ix + g.npml - 1, g.nx + g.npml - ix), no);
+        float sl = 0.0f, sm = 0.0f;
+        for (int k = 1; k <= nox; ++k) {
+            size_t a1 = cell(g, 1, ix + k), b1 = cell(g, 2, ix + k), a2 = cell(g, 1, ix - k), b2 = cell(g, 2, ix - k);
+            float lp = 0.5f * (lam[a1] + lam[b1]), lm = 0.5f * (lam[a2] + lam[b2]);
+            float mp = 0.5f * (mu[a1] + mu[b1]), mm = 0.5f * (mu[a2] + mu[b2]);
+            double c = cf->c1[(k - 1) + no * (nox - 1)];
+            sl = accd(sl, c, lp * (uz(2, ix + k) - uz(1, ix + k)) - lm * (uz(2, ix - k) - uz(1, ix - k)));
+            sm = accd(sm, c, mp * (ux(2, ix + k) - ux(1, ix + k)) - mm * (ux(2, ix - k) - ux(1, ix - k)));
+        }
+        float lbar = 0.5f * (lam[a] + lam[b]), mbar = 0.5f * (mu[a] + mu[b]);
+        float dxuz = (cdx_sum(g, cf, uz, 1, ix) + cdx_sum(g, cf, uz, 2, ix)) / g.dx;
+        float dxux = (cdx_sum(g, cf, ux, 1, ix) + cdx_sum(g, cf, ux, 2, ix)) / g.dx;
+        p[0] = x2_mode(g, cf, lam, mu, 0, ux, iz, ix);
         p[1] = (hmean(mu[a], mu[b]) * (ux(2, ix) - ux(1, ix))) * sdz2;
-        float vm1 = mu[a] * cdx_sum(g, cf, uz, 1, ix) / g.dx;
-        float vm2 = mu[b] * cdx_sum(g, cf, uz, 2, ix) / g.dx;
-        p[2] = (vm1 + vm2) / g.dz;
-        p[3] = 0.0f;
+        p[2] = (sl / g.dx + mbar * dxuz) / g.dz;
+        p[3] = x2_mode(g, cf, lam, mu, 1, uz, iz, ix);
         p[4] = (hmean(M1, M2) * (uz(2, ix) - uz(1, ix))) * sdz2;
-        float vl1 = lam[a] * cdx_sum(g, cf, ux, 1, ix) / g.dx;
-        float vl2 = lam[b] * cdx_sum(g, cf, ux, 2, ix) / g.dx;
-        p[5] = (vl1 + vl2) / g.dz;
+        p[5] = (sm / g.dx + lbar * dxux) / g.dz;
         return;
     }
     p[0] = x2_mode(g, cf, lam, mu, 0, ux, iz, ix);
@@ -237,6 +257,7 @@ __device__ __forceinline__ size_t strip_off(const Geo &g, int s, int iz, int ix)
     return (size_t)(ix - (1 - g.npml)) * g.npml + (size_t)(iz - (g.nz + 1));
 }
 
+// psi_x / psi_z of one component, zero outside the strips that carry it
 struct PsiRd {
     const Strips *st;
     const Geo *g;
@@ -244,101 +265,173 @@ struct PsiRd {
     __device__ float operator()(int iz, int ix) const
     {
         int s = strip_of(*g, iz, ix);
-        if (s == S_NONE) return 0.0f;
-        if (zdir) {
-            if (s != S_B) return 0.0f;
-            return st->psz[s][comp][strip_off(*g, s, iz, ix)];
-        }
-        return st->psx[s][comp][strip_off(*g, s, iz, ix)];
+        if (s == S_NONE || (zdir && s != S_B)) return 0.0f;
+        const float *b = zdir ? st->psz[s][comp] : st->psx[s][comp];
+        return b[strip_off(*g, s, iz, ix)];
     }
 };
 
-// ---------------------------------------------------------------------
-// boundary kernel: all updated cells outside the interior rectangle
-// ---------------------------------------------------------------------
+#pragma clang fp contract(on)
 
-__device__ __forceinline__ bool band_cell(const Geo &g, const Bands &b, long t, int &iz, int &ix)
+// ---------------------------------------------------------------------
+// FP32 full-order pointwise stencil (frame cells away from the padded-grid
+// edges and the surface rows): same shared-flux arithmetic as k_interior
+// ---------------------------------------------------------------------
+template <int NO, class R>
+__device__ __forceinline__ void parts_full(const Geo &g, const ICoef<NO> &k, const float *lam,
+                                           const float *mu, const R &ux, const R &uz, int iz, int ix,
+                                           float p[6])
 {
-    const int xfirst = 2 - g.npml;
-    if (t < b.ntop) {
-        int rows = g.iz0 - 1;
-        ix = xfirst + (int)(t / rows); iz = 1 + (int)(t % rows);
-        return true;
+    constexpr int NL = 4 * NO - 1;
+    constexpr int NM = 2 * NO + 1;
+    float Xx = 0.0f, Xz = 0.0f, Zx = 0.0f, Zz = 0.0f, Mx = 0.0f, Mz = 0.0f;
+    float lax[NM], mux[NM], laz[NM], muz[NM];
+#pragma unroll
+    for (int j = 0; j < NM; ++j) {
+        size_t cx = cell(g, iz, ix - NO + j), cz = cell(g, iz - NO + j, ix);
+        lax[j] = lam[cx]; mux[j] = mu[cx];
+        laz[j] = lam[cz]; muz[j] = mu[cz];
     }
-    t -= b.ntop;
-    if (t < b.nbot) {
-        int rows = g.nz + g.npml - 1 - g.iz1;
-        ix = xfirst + (int)(t / rows); iz = g.iz1 + 1 + (int)(t % rows);
-        return true;
+    {   // x: D2x[l+2m](ux), D2x[m](uz)
+        float lx0[NL], lx1[NL], ra[NM], rm[NM];
+#pragma unroll
+        for (int j = 0; j < NL; ++j) { lx0[j] = ux(iz, ix - 2 * NO + 1 + j); lx1[j] = uz(iz, ix - 2 * NO + 1 + j); }
+#pragma unroll
+        for (int j = 0; j < NM; ++j) { ra[j] = 1.0f / (lax[j] + 2.0f * mux[j]); rm[j] = 1.0f / mux[j]; }
+        float Fa[2 * NO], Fm[2 * NO];
+#pragma unroll
+        for (int q = 0; q < 2 * NO; ++q) {
+            float sa = 0.0f, sm = 0.0f;
+#pragma unroll
+            for (int jj = 1; jj <= NO; ++jj) {
+                sa += k.ai[jj - 1] * (lx0[NO + q + jj - 1] - lx0[NO + q - jj]);
+                sm += k.ai[jj - 1] * (lx1[NO + q + jj - 1] - lx1[NO + q - jj]);
+            }
+            Fa[q] = sa / (ra[q] + ra[q + 1]);
+            Fm[q] = sm / (rm[q] + rm[q + 1]);
+        }
+#pragma unroll
+        for (int kk = 1; kk <= NO; ++kk) {
+            Xx += k.aox[kk - 1] * (Fa[NO + kk - 1] - Fa[NO - kk]);
+            Xz += k.aox[kk - 1] * (Fm[NO + kk - 1] - Fm[NO - kk]);
+        }
     }
-    t -= b.nbot;
-    int rows = g.iz1 - g.iz0 + 1;
-    if (t < b.nleft) {
-        ix = xfirst + (int)(t / rows); iz = g.iz0 + (int)(t % rows);
-        return true;
+    {   // z: D2z[m](ux), D2z[l+2m](uz)
+        float lz0[NL], lz1[NL], ra[NM], rm[NM];
+#pragma unroll
+        for (int j = 0; j < NL; ++j) { lz0[j] = ux(iz - 2 * NO + 1 + j, ix); lz1[j] = uz(iz - 2 * NO + 1 + j, ix); }
+#pragma unroll
+        for (int j = 0; j < NM; ++j) { ra[j] = 1.0f / (laz[j] + 2.0f * muz[j]); rm[j] = 1.0f / muz[j]; }
+        float Fm[2 * NO], Fa[2 * NO];
+#pragma unroll
+        for (int q = 0; q < 2 * NO; ++q) {
+            float sm = 0.0f, sa = 0.0f;
+#pragma unroll
+            for (int jj = 1; jj <= NO; ++jj) {
+                sm += k.ai[jj - 1] * (lz0[NO + q + jj - 1] - lz0[NO + q - jj]);
+                sa += k.ai[jj - 1] * (lz1[NO + q + jj - 1] - lz1[NO + q - jj]);
+            }
+            Fm[q] = sm / (rm[q] + rm[q + 1]);
+            Fa[q] = sa / (ra[q] + ra[q + 1]);
+        }
+#pragma unroll
+        for (int kk = 1; kk <= NO; ++kk) {
+            Zx += k.aoz[kk - 1] * (Fm[NO + kk - 1] - Fm[NO - kk]);
+            Zz += k.aoz[kk - 1] * (Fa[NO + kk - 1] - Fa[NO - kk]);
+        }
     }
-    t -= b.nleft;
-    if (t < b.nright) {
-        ix = g.ix1 + 1 + (int)(t / rows); iz = g.iz0 + (int)(t % rows);
-        return true;
+    // mixed: Dx(l Dz uz) + Dz(m Dx uz) for ux;  Dx(m Dz ux) + Dz(l Dx ux) for uz
+#pragma unroll 1
+    for (int kk = 1; kk <= NO; ++kk) {
+        float dzp_z = 0.0f, dzm_z = 0.0f, dzp_x = 0.0f, dzm_x = 0.0f;  // Dz at columns ix+kk / ix-kk
+        float dxp_z = 0.0f, dxm_z = 0.0f, dxp_x = 0.0f, dxm_x = 0.0f;  // Dx at rows iz+kk / iz-kk
+#pragma unroll
+        for (int j = 1; j <= NO; ++j) {
+            float b = k.bi[j - 1];
+            dzp_z += b * (uz(iz + j, ix + kk) - uz(iz - j, ix + kk));
+            dzm_z += b * (uz(iz + j, ix - kk) - uz(iz - j, ix - kk));
+            dzp_x += b * (ux(iz + j, ix + kk) - ux(iz - j, ix + kk));
+            dzm_x += b * (ux(iz + j, ix - kk) - ux(iz - j, ix - kk));
+            dxp_z += b * (uz(iz + kk, ix + j) - uz(iz + kk, ix - j));
+            dxm_z += b * (uz(iz - kk, ix + j) - uz(iz - kk, ix - j));
+            dxp_x += b * (ux(iz + kk, ix + j) - ux(iz + kk, ix - j));
+            dxm_x += b * (ux(iz - kk, ix + j) - ux(iz - kk, ix - j));
+        }
+        float bo = k.bo[kk - 1];
+        size_t cxp = cell(g, iz, ix + kk), cxm = cell(g, iz, ix - kk);
+        size_t czp = cell(g, iz + kk, ix), czm = cell(g, iz - kk, ix);
+        Mx += bo * ((lam[cxp] * dzp_z - lam[cxm] * dzm_z) + (mu[czp] * dxp_z - mu[czm] * dxm_z));
+        Mz += bo * ((mu[cxp] * dzp_x - mu[cxm] * dzm_x) + (lam[czp] * dxp_x - lam[czm] * dxm_x));
+    }
+    p[0] = Xx; p[1] = Zx; p[2] = Mx; p[3] = Xz; p[4] = Zz; p[5] = Mz;
+}
+
+#pragma clang fp contract(off)
+
+// ---------------------------------------------------------------------
+// frame kernels: every updated cell outside the interior rectangle, split
+// into the full-order part (k_frame_full) and the reduced-order edge bands
+// next to the padded-grid boundary and the free surface (k_frame_edge)
+// ---------------------------------------------------------------------
+__device__ __forceinline__ bool rect_cell(const Rects &r, long t, int &iz, int &ix)
+{
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if (q < r.n && t < r.off[q + 1]) {
+            long u = t - r.off[q];
+            int rows = r.z1[q] - r.z0[q] + 1;
+            ix = r.x0[q] + (int)(u / rows);
+            iz = r.z0[q] + (int)(u % rows);
+            return true;
+        }
     }
     return false;
 }
 
-__global__ __launch_bounds__(256) void k_boundary(StepArgs a, Bands b)
+__device__ __forceinline__ CorrRd frame_reader(const StepArgs &a, const float *u, const float *ring, int iz, int ix)
 {
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    int iz, ix;
-    if (!band_cell(a.g, b, t, iz, ix)) return;
+    int sig = -1;
+    if (a.ring_valid) {
+        int arr, which;
+        if (bg_off(a.g, iz, ix, 0, arr, which) >= 0) sig = which;
+    }
+    return CorrRd{u, ring, &a.g, sig};
+}
+
+// PML memory corrections and the (split-field) leapfrog update of one cell
+__device__ __forceinline__ void frame_update(const StepArgs &a, int iz, int ix, float p[6])
+{
     const Geo &g = a.g;
     const Coefs *cf = a.cf;
-    FieldRd rx{a.ux, &g}, rz{a.uz, &g};
-    float p[6];
-    eval_parts(g, cf, a.lam, a.mu, rx, rz, iz, ix, p);
     const int s = strip_of(g, iz, ix);
     const bool surf = (iz == 1);
     if (s != S_NONE) {
-        // PML memory corrections -Dx(Mx psi_x), -Dz(Mz psi_z)
-        PsiRd px0{&a.st, &g, 0, 0}, px1{&a.st, &g, 1, 0}, pz0{&a.st, &g, 0, 1}, pz1{&a.st, &g, 1, 1};
+        // -Dx(Mx psi_x), -Dz(Mz psi_z)
+        PsiRd px0{a.st, &g, 0, 0}, px1{a.st, &g, 1, 0}, pz0{a.st, &g, 0, 1}, pz1{a.st, &g, 1, 1};
         const int no = g.norder;
         int nox = imin(imin(ix + g.npml - 1, g.nx + g.npml - ix), no);
         int noz = imin(imin(iz - 1, g.nz + g.npml - iz), no);
         float sx0 = 0.0f, sx1 = 0.0f, sz0 = 0.0f, sz1 = 0.0f;
-        for (int k = 1; k <= nox; ++k) {
-            size_t ca = cell(g, iz, ix + k), cb = cell(g, iz, ix - k);
-            float ma = surf ? meff_of(a.lam[ca], a.mu[ca]) : a.lam[ca] + 2.0f * a.mu[ca];
-            float mb = surf ? meff_of(a.lam[cb], a.mu[cb]) : a.lam[cb] + 2.0f * a.mu[cb];
-            double c = cf->c1[(k - 1) + no * (nox - 1)];
-            sx0 = accd(sx0, c, ma * px0(iz, ix + k) - mb * px0(iz, ix - k));
-            sx1 = accd(sx1, c, a.mu[ca] * px1(iz, ix + k) - a.mu[cb] * px1(iz, ix - k));
+        for (int kk = 1; kk <= nox; ++kk) {
+            size_t ca = cell(g, iz, ix + kk), cb = cell(g, iz, ix - kk);
+            float ma = a.lam[ca] + 2.0f * a.mu[ca], mb = a.lam[cb] + 2.0f * a.mu[cb];
+            double c = cf->c1[(kk - 1) + no * (nox - 1)];
+            sx0 = accd(sx0, c, ma * px0(iz, ix + kk) - mb * px0(iz, ix - kk));
+            sx1 = accd(sx1, c, a.mu[ca] * px1(iz, ix + kk) - a.mu[cb] * px1(iz, ix - kk));
         }
-        for (int k = 1; k <= noz; ++k) {
-            size_t ca = cell(g, iz + k, ix), cb = cell(g, iz - k, ix);
-            double c = cf->c1[(k - 1) + no * (noz - 1)];
-            sz0 = accd(sz0, c, a.mu[ca] * pz0(iz + k, ix) - a.mu[cb] * pz0(iz - k, ix));
-            sz1 = accd(sz1, c, (a.lam[ca] + 2.0f * a.mu[ca]) * pz1(iz + k, ix) -
-                                   (a.lam[cb] + 2.0f * a.mu[cb]) * pz1(iz - k, ix));
-        }
+        if (s == S_B)
+            for (int kk = 1; kk <= noz; ++kk) {
+                size_t ca = cell(g, iz + kk, ix), cb = cell(g, iz - kk, ix);
+                double c = cf->c1[(kk - 1) + no * (noz - 1)];
+                sz0 = accd(sz0, c, a.mu[ca] * pz0(iz + kk, ix) - a.mu[cb] * pz0(iz - kk, ix));
+                sz1 = accd(sz1, c, (a.lam[ca] + 2.0f * a.mu[ca]) * pz1(iz + kk, ix) -
+                                       (a.lam[cb] + 2.0f * a.mu[cb]) * pz1(iz - kk, ix));
+            }
         p[0] = p[0] - sx0 / g.dx;
+        p[3] = p[3] - sx1 / g.dx;
         if (!surf) {
-            p[3] = p[3] - sx1 / g.dx;
             p[1] = p[1] - sz0 / g.dz;
             p[4] = p[4] - sz1 / g.dz;
-        }
-    }
-    if (a.bg.valid) {
-        int arr, which;
-        long o = bg_off(g, iz, ix, a.it, arr, which);
-        if (o >= 0) {
-            // inside ring adds L[u0 outside], outside ring subtracts L[u0 inside]
-            int other = 1 - which;
-            MaskRd mx{&a.bg, &g, a.it, other, 0}, mz{&a.bg, &g, a.it, other, 1};
-            float q[6];
-            eval_parts(g, cf, a.lam, a.mu, mx, mz, iz, ix, q);
-            if (which == 0)
-                for (int m = 0; m < 6; ++m) p[m] = p[m] + q[m];
-            else
-                for (int m = 0; m < 6; ++m) p[m] = p[m] - q[m];
         }
     }
     const size_t c = cell(g, iz, ix);
@@ -349,13 +442,15 @@ __global__ __launch_bounds__(256) void k_boundary(StepArgs a, Bands b)
         a.uzp[c] = (2.0f * a.uz[c] - a.uzp[c]) + bf * rzs;
         return;
     }
+    const Strips *st = a.st;
     const size_t o = strip_off(g, s, iz, ix);
-    const float ax = a.st.ax[ix + g.npml - 1];
-    const float az = a.st.az[iz - 1];
+    const float ax = st->ax[ix + g.npml - 1];
+    const float az = st->az[iz - 1];
     const float c3 = 2.0f - 4.0f * ax * az;
+#pragma unroll
     for (int comp = 0; comp < 2; ++comp) {
-        float *b1c = a.st.cur[s][comp][0], *b2c = a.st.cur[s][comp][1], *b3c = a.st.cur[s][comp][2];
-        float *b1p = a.st.prev[s][comp][0], *b2p = a.st.prev[s][comp][1], *b3p = a.st.prev[s][comp][2];
+        float *b1c = st->cur[s][comp][0], *b2c = st->cur[s][comp][1], *b3c = st->cur[s][comp][2];
+        float *b1p = st->prev[s][comp][0], *b2p = st->prev[s][comp][1], *b3p = st->prev[s][comp][2];
         float X = p[3 * comp], Z = p[3 * comp + 1], Mm = p[3 * comp + 2];
         float n1 = (2.0f * b1c[o] - (1.0f - ax) * b1p[o] + bf * X) / (1.0f + ax);
         float n2 = (2.0f * b2c[o] - (1.0f - az) * b2p[o] + bf * Z) / (1.0f + az);
@@ -364,6 +459,47 @@ __global__ __launch_bounds__(256) void k_boundary(StepArgs a, Bands b)
         float tot = (n1 + n2) + n3;
         if (comp == 0) a.uxp[c] = tot; else a.uzp[c] = tot;
     }
+}
+
+template <int NO>
+__global__ __launch_bounds__(256) void k_frame_full(StepArgs a, Rects r, ICoef<NO> k)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int iz, ix;
+    if (!rect_cell(r, t, iz, ix)) return;
+    CorrRd rx = frame_reader(a, a.ux, a.ringx, iz, ix), rz = frame_reader(a, a.uz, a.ringz, iz, ix);
+    float p[6];
+    parts_full<NO>(a.g, k, a.lam, a.mu, rx, rz, iz, ix, p);
+    frame_update(a, iz, ix, p);
+}
+
+__global__ __launch_bounds__(256) void k_frame_edge(StepArgs a, Rects r)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int iz, ix;
+    if (!rect_cell(r, t, iz, ix)) return;
+    CorrRd rx = frame_reader(a, a.ux, a.ringx, iz, ix), rz = frame_reader(a, a.uz, a.ringz, iz, ix);
+    float p[6];
+    eval_parts(a.g, a.cf, a.lam, a.mu, rx, rz, iz, ix, p);
+    frame_update(a, iz, ix, p);
+}
+
+// background on the hybrid ring at iteration it, scattered into the
+// full-grid ring arrays (positions are fixed; values change every step)
+__global__ __launch_bounds__(256) void k_ring_prep(Geo g, const Background *bg, int it, float *ringx, float *ringz)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int nx = g.nx, nz = g.nz, nbd = g.nbd;
+    // enumerate the ring rectangle rows 1..nz+nbd, columns 1-nbd..nx+nbd
+    long W = nx + 2 * nbd, H = nz + nbd;
+    if (t >= W * H) return;
+    int ix = (1 - nbd) + (int)(t / H), iz = 1 + (int)(t % H);
+    int arr, which;
+    long o = bg_off(g, iz, ix, it, arr, which);
+    if (o < 0) return;
+    size_t c = cell(g, iz, ix);
+    ringx[c] = bg->a[6 * which + 2 * arr][o];
+    ringz[c] = bg->a[6 * which + 2 * arr + 1][o];
 }
 
 // ---------------------------------------------------------------------
@@ -383,19 +519,22 @@ __global__ __launch_bounds__(256) void k_psi(StepArgs a)
     const size_t o = strip_off(g, s, iz, ix);
     FieldRd rx{a.ux, &g}, rz{a.uz, &g};
     const int jx = ix + g.npml - 1;
-    if (a.st.ax[jx] > 0.0f) {
-        float b = a.st.bx[jx], c = 1.0f - b;
+    const Strips *st = a.st;
+    if (st->ax[jx] > 0.0f) {
+        float b = st->bx[jx], c = 1.0f - b;
         float gx = cdx_sum(g, a.cf, rx, iz, ix) / g.dx;
         float gz = cdx_sum(g, a.cf, rz, iz, ix) / g.dx;
-        a.st.psx[s][0][o] = b * a.st.psx[s][0][o] + c * gx;
-        a.st.psx[s][1][o] = b * a.st.psx[s][1][o] + c * gz;
+        float *p0 = st->psx[s][0], *p1 = st->psx[s][1];
+        p0[o] = b * p0[o] + c * gx;
+        p1[o] = b * p1[o] + c * gz;
     }
-    if (s == S_B && a.st.az[iz - 1] > 0.0f) {
-        float b = a.st.bz[iz - 1], c = 1.0f - b;
+    if (s == S_B && st->az[iz - 1] > 0.0f) {
+        float b = st->bz[iz - 1], c = 1.0f - b;
         float gx = cdz_sum(g, a.cf, rx, iz, ix) / g.dz;
         float gz = cdz_sum(g, a.cf, rz, iz, ix) / g.dz;
-        a.st.psz[s][0][o] = b * a.st.psz[s][0][o] + c * gx;
-        a.st.psz[s][1][o] = b * a.st.psz[s][1][o] + c * gz;
+        float *p0 = st->psz[s][0], *p1 = st->psz[s][1];
+        p0[o] = b * p0[o] + c * gx;
+        p1[o] = b * p1[o] + c * gz;
     }
 }
 
@@ -631,16 +770,24 @@ __global__ __launch_bounds__(NT) void k_interior(IntArgs<NO> a)
 // layout conversion helpers
 // ---------------------------------------------------------------------
 // reference (1:3, NZ, NX) -> bf = dt^2/rho, lam, mu in the padded layout
+// The damped part of the PML (more than nbd cells into the padding) takes
+// the material of the outermost undamped cell along the normal: the
+// absorbing layers are normal-invariant, which keeps heterogeneous damped
+// corners stable.
 __global__ void k_import_material(Geo g, const float *c, float dt2, float *bf, float *lam, float *mu)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     long n = (long)g.NX * g.NZ;
     if (t >= n) return;
     int j = (int)(t / g.NZ), k = (int)(t % g.NZ);
+    int ix = j + 1 - g.npml, iz = k + 1;
+    int sx = ix < 1 - g.nbd ? 1 - g.nbd : (ix > g.nx + g.nbd ? g.nx + g.nbd : ix);
+    int sz = iz > g.nz + g.nbd ? g.nz + g.nbd : iz;
+    long src = (long)(sx + g.npml - 1) * g.NZ + (sz - 1);
     size_t d = (size_t)j * g.ld + k;
-    bf[d] = dt2 / c[3 * t + 0];
-    lam[d] = c[3 * t + 1];
-    mu[d] = c[3 * t + 2];
+    bf[d] = dt2 / c[3 * src + 0];
+    lam[d] = c[3 * src + 1];
+    mu[d] = c[3 * src + 2];
 }
 
 // reference (1:2, NZ, NX) <-> two padded arrays (level 1 -> cur, level 2 -> prev)

@@ -483,10 +483,13 @@ void orc_backazimuth_grids(double lat_org, double lon_org, double az_org, int n,
  *  rho u_tt = X + Z + M, per component, where (Submain.f90 operators)
  *    ux: X = diffx2[l+2m](ux), Z = diffz2[m](ux), M = diffzx[l](uz) + diffxz[m](uz)
  *    uz: X = diffx2[m](uz),    Z = diffz2[l+2m](uz), M = diffzx[m](ux) + diffxz[l](ux)
- *  Free surface (izt == 1), row iz = 1, half-cell traction-free closure:
- *    ux: X = diffx2[meff](ux), meff = 4m(l+m)/(l+2m)   (sigma_zz = 0 eliminates d_z uz)
- *        Z = 2/dz^2 H(m1,m2)(ux2-ux1),  M = (m1 Dx uz1 + m2 Dx uz2)/dz
- *    uz: X = 0,  Z = 2/dz^2 H(M1,M2)(uz2-uz1),  M = (l1 Dx ux1 + l2 Dx ux2)/dz
+ *  Free surface (izt == 1), row iz = 1: half-cell closure derived from the
+ *  strain energy of the top layer (rows 1-2, trapezoidal rule in z), so the
+ *  zero traction is the natural boundary condition (lb, mb = layer means):
+ *    ux: X = diffx2[l+2m](ux), Z = 2/dz^2 H(m1,m2)(ux2-ux1),
+ *        M = [Dx(lb (uz2-uz1)) + mb (Dx uz1 + Dx uz2)]/dz
+ *    uz: X = diffx2[m](uz),    Z = 2/dz^2 H(M1,M2)(uz2-uz1),
+ *        M = [Dx(mb (ux2-ux1)) + lb (Dx ux1 + Dx ux2)]/dz
  *  PML (stretched coordinates, split u = u1 + u2 + u3 in the padding strips):
  *    rho (u1_tt + dx u1_t)               = X - Dx(Mx psi_x)
  *    rho (u2_tt + dz u2_t)               = Z - Dz(Mz psi_z)
@@ -495,7 +498,9 @@ void orc_backazimuth_grids(double lat_org, double lon_org, double az_org, int n,
  *  Damping d = d0 ((q-nbd)/(npml-nbd))^npower for q > nbd cells into the
  *  left/right/bottom strips (the first nbd cells are an undamped buffer that
  *  holds the hybrid ring), d0 = -(npower+1) vpml ln(R0) / (2 (npml-nbd) h);
- *  the optional top strip has no buffer.
+ *  the optional top strip has no buffer.  The damped cells use the material
+ *  of the outermost undamped cell along the normal (normal-invariant layers;
+ *  heterogeneous damped corners are unstable otherwise).
  *  Hybrid TF/SF: the physical region 1..nx x 1..nz holds the total field,
  *  the padding holds the scattered field.  Inside-ring cells add
  *  L[u0 on outside strips], outside-ring cells subtract L[u0 on inside strips].
@@ -572,17 +577,10 @@ static inline float cdz_sum(const gdims *g, const double *coef1, const float *f,
     return s;
 }
 
-static inline float meff_of(float lam, float mu)
-{
-    float m = lam + 2.0f * mu;
-    return m != 0.0f ? (4.0f * mu) * (lam + mu) / m : 0.0f;
-}
-
-/* D2x with modulus given by a callback-free pair: mat row = lam+2mu (mode 0),
- * mu (mode 1), meff (mode 2) evaluated on the fly */
+/* modulus of the second-derivative terms: lam+2mu (mode 0) or mu (mode 1) */
 static inline float matval(int mode, float lam, float mu)
 {
-    return mode == 0 ? lam + 2.0f * mu : (mode == 1 ? mu : meff_of(lam, mu));
+    return mode == 0 ? lam + 2.0f * mu : mu;
 }
 
 static float x2_point_mode(const gdims *g, const double *coef0, const float *lam, const float *mu,
@@ -667,20 +665,31 @@ static void eval_parts_g(const gdims *g, const double *coef0, const double *coef
                          const float *mu, const float *ux, const float *uz, int iz, int ix, float p[6])
 {
     if (iz == g->izt && g->izt == 1) {
-        /* free-surface row: half-cell traction-free closure */
+        /* free-surface row: half-cell closure from the strain energy of the
+         * top layer (rows 1-2, trapezoidal in z), traction free naturally */
         size_t a = g2(g, 1, ix), b = g2(g, 2, ix);
         float sdz2 = 2.0f / (g->dz * g->dz);
         float M1 = lam[a] + 2.0f * mu[a], M2 = lam[b] + 2.0f * mu[b];
-        p[0] = x2_point_mode(g, coef0, lam, mu, 2, ux, iz, ix);
+        int nox = imin(imin(ix + g->npml - 1, g->nx + g->npml - ix), g->norder);
+        int norder = g->norder;
+        /* Dx( lbar (uz2-uz1)/dz ) and Dx( mbar (ux2-ux1)/dz ) */
+        float sl = 0.0f, sm = 0.0f;
+        for (int k = 1; k <= nox; ++k) {
+            size_t a1 = g2(g, 1, ix + k), b1 = g2(g, 2, ix + k), a2 = g2(g, 1, ix - k), b2 = g2(g, 2, ix - k);
+            float lp = 0.5f * (lam[a1] + lam[b1]), lm = 0.5f * (lam[a2] + lam[b2]);
+            float mp = 0.5f * (mu[a1] + mu[b1]), mm = 0.5f * (mu[a2] + mu[b2]);
+            sl = acc(sl, CO(coef1, k, nox), lp * (uz[b1] - uz[a1]) - lm * (uz[b2] - uz[a2]));
+            sm = acc(sm, CO(coef1, k, nox), mp * (ux[b1] - ux[a1]) - mm * (ux[b2] - ux[a2]));
+        }
+        float lbar = 0.5f * (lam[a] + lam[b]), mbar = 0.5f * (mu[a] + mu[b]);
+        float dxuz = (cdx_sum(g, coef1, uz, 1, ix) + cdx_sum(g, coef1, uz, 2, ix)) / g->dx;
+        float dxux = (cdx_sum(g, coef1, ux, 1, ix) + cdx_sum(g, coef1, ux, 2, ix)) / g->dx;
+        p[0] = x2_point_mode(g, coef0, lam, mu, 0, ux, iz, ix);
         p[1] = (hmean(mu[a], mu[b]) * (ux[b] - ux[a])) * sdz2;
-        float vm1 = mu[a] * cdx_sum(g, coef1, uz, 1, ix) / g->dx;
-        float vm2 = mu[b] * cdx_sum(g, coef1, uz, 2, ix) / g->dx;
-        p[2] = (vm1 + vm2) / g->dz;
-        p[3] = 0.0f;
+        p[2] = (sl / g->dx + mbar * dxuz) / g->dz;
+        p[3] = x2_point_mode(g, coef0, lam, mu, 1, uz, iz, ix);
         p[4] = (hmean(M1, M2) * (uz[b] - uz[a])) * sdz2;
-        float vl1 = lam[a] * cdx_sum(g, coef1, ux, 1, ix) / g->dx;
-        float vl2 = lam[b] * cdx_sum(g, coef1, ux, 2, ix) / g->dx;
-        p[5] = (vl1 + vl2) / g->dz;
+        p[5] = (sm / g->dx + lbar * dxux) / g->dz;
         return;
     }
     p[0] = x2_point_mode(g, coef0, lam, mu, 0, ux, iz, ix);
@@ -786,11 +795,19 @@ void orc_updatedispl(const int *norder_, const int *nx_, const int *nz_, const i
     float *u1x = malloc(ncell * sizeof(float)), *u1z = malloc(ncell * sizeof(float));
     float *rho = malloc(ncell * sizeof(float)), *lam = malloc(ncell * sizeof(float));
     float *mu = malloc(ncell * sizeof(float)), *lp2m = malloc(ncell * sizeof(float));
-    for (size_t i = 0; i < ncell; ++i) {
-        u1x[i] = ux[2 * i]; u1z[i] = uz[2 * i];
-        rho[i] = c[3 * i]; lam[i] = c[3 * i + 1]; mu[i] = c[3 * i + 2];
-        lp2m[i] = lam[i] + 2.0f * mu[i];
-    }
+    /* material seen by the stencils: the damped part of the PML (more than
+     * nbd cells into the padding) repeats the outermost undamped cell along
+     * the normal, so the layers are normal-invariant (heterogeneous damped
+     * corners are unstable otherwise) */
+    for (int ix = 1 - npml; ix <= nx + npml; ++ix)
+        for (int iz = izt; iz <= nz + npml; ++iz) {
+            int sx = ix < 1 - nbd ? 1 - nbd : (ix > nx + nbd ? nx + nbd : ix);
+            int sz = iz > nz + nbd ? nz + nbd : iz;
+            size_t i = g2(g, iz, ix), j = g2(g, sz, sx);
+            u1x[i] = ux[2 * i]; u1z[i] = uz[2 * i];
+            rho[i] = c[3 * j]; lam[i] = c[3 * j + 1]; mu[i] = c[3 * j + 2];
+            lp2m[i] = lam[i] + 2.0f * mu[i];
+        }
     float *dpx = malloc(G.NX * sizeof(float)), *dpz = malloc(G.NZ * sizeof(float));
     orc_damping(nx, nz, npml, nbd, G.pml_top, G.npower, G.dx, G.dz, *vpml_, *R0_, dpx, dpz);
 
@@ -920,8 +937,7 @@ void orc_updatedispl(const int *norder_, const int *nx_, const int *nz_, const i
                 float sx0 = 0.0f, sx1 = 0.0f, sz0 = 0.0f, sz1 = 0.0f;
                 for (int k = 1; k <= nox; ++k) {
                     size_t a = g2(g, iz, ix + k), b = g2(g, iz, ix - k);
-                    float ma = surf ? meff_of(lam[a], mu[a]) : lp2m[a];
-                    float mb = surf ? meff_of(lam[b], mu[b]) : lp2m[b];
+                    float ma = lp2m[a], mb = lp2m[b];
                     sx0 = acc(sx0, CO(coef1, k, nox), ma * psx[0][a] - mb * psx[0][b]);
                     sx1 = acc(sx1, CO(coef1, k, nox), mu[a] * psx[1][a] - mu[b] * psx[1][b]);
                 }
@@ -931,7 +947,7 @@ void orc_updatedispl(const int *norder_, const int *nx_, const int *nz_, const i
                     sz1 = acc(sz1, CO(coef1, k, noz), lp2m[a] * psz[1][a] - lp2m[b] * psz[1][b]);
                 }
                 p[0] = p[0] - sx0 / G.dx;
-                if (!surf) p[3] = p[3] - sx1 / G.dx;
+                p[3] = p[3] - sx1 / G.dx;
                 if (!surf) { p[1] = p[1] - sz0 / G.dz; p[4] = p[4] - sz1 / G.dz; }
             }
             /* hybrid TF/SF correction */

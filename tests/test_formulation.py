@@ -1,0 +1,167 @@
+"""Pins of this project's formulation of the binary-only updatedispl
+(PML, free surface, hybrid TF/SF injection, leapfrog) through the CPU oracle:
+
+* the pointwise operator evaluation equals the literal operator passes
+  (which are themselves pinned to the reference's Fortran) bit for bit;
+* the collocated second-derivative composition is exact on polynomials up
+  to degree 2*norder (closed form);
+* PML: a pulse leaving the domain loses >= 99.9% of its energy, and the
+  energy decays monotonically once the pulse is inside the layers;
+* free surface + leapfrog + PML: long runs in a cell-wise random model
+  (random also in the near-surface rows and the damped corners) stay
+  bounded at 0.97 of the reference's CFL limit;
+* hybrid self-consistency: with the FD model equal to the FK model the
+  surface seismograms equal the FK surface response and the padding holds
+  only a small scattered field.
+"""
+import numpy as np
+import pytest
+
+from fdfk2d_amd import refstate
+from oracle import pyoracle as po
+from tests._cases import Case, fk_model, layered_grid, oracle_run, rel_l2
+
+
+def _planes(c, NX, NZ):
+    c3 = c.reshape(NX, NZ, 3)
+    return np.ascontiguousarray(c3[..., 1]), np.ascontiguousarray(c3[..., 2])
+
+
+@pytest.mark.parametrize("norder", [1, 2, 3, 4])
+def test_pointwise_parts_equal_literal_passes(norder):
+    nx, nz, npml = 27, 19, 9
+    NX, NZ = nx + 2 * npml, nz + npml
+    rng = np.random.default_rng(norder)
+    vp = rng.uniform(5000, 7000, (NX, NZ)).astype(np.float32)
+    vs = (vp / rng.uniform(1.6, 1.9, (NX, NZ))).astype(np.float32)
+    c = refstate.material_law(vp, vs)
+    lam, mu = _planes(c, NX, NZ)
+    ux = rng.standard_normal((NX, NZ)).astype(np.float32)
+    uz = rng.standard_normal((NX, NZ)).astype(np.float32)
+    ux[0], ux[-1], ux[:, -1] = 0, 0, 0
+    uz[0], uz[-1], uz[:, -1] = 0, 0, 0
+    c0, c1, _ = po.fd_coefs(norder)
+    lp2m = (lam + np.float32(2.0) * mu).astype(np.float32)
+    dx, dz = 200.0, 150.0
+
+    def both(name, mat, u, coef):
+        return (po.op(name + "_reg", norder, nx, nz, npml, 1, dx, dz, coef, mat, u) +
+                po.op(name + "_pml", norder, nx, nz, npml, 1, dx, dz, coef, mat, u))
+    Xx = both("diffx2", lp2m, ux, c0)
+    Zx = both("diffz2", mu, ux, c0)
+    Mx = both("diffzx", lam, uz, c1) + both("diffxz", mu, uz, c1)
+    Xz = both("diffx2", mu, uz, c0)
+    Zz = both("diffz2", lp2m, uz, c0)
+    Mz = both("diffzx", mu, ux, c1) + both("diffxz", lam, ux, c1)
+    ref = [Xx, Zx, Mx, Xz, Zz, Mz]
+    for ix in range(2 - npml, nx + npml):
+        for iz in range(2, nz + npml):
+            p = po.eval_parts(norder, nx, nz, npml, 1, dx, dz, c0, c1, lam, mu, ux, uz, iz, ix)
+            want = np.array([r[ix + npml - 1, iz - 1] for r in ref], np.float32)
+            assert np.array_equal(p.view(np.uint32), want.view(np.uint32)), (ix, iz)
+
+
+@pytest.mark.parametrize("norder", [1, 2, 3, 4])
+def test_second_derivative_exact_on_polynomials(norder):
+    nx, nz, npml = 30, 12, 8
+    NX, NZ = nx + 2 * npml, nz + npml
+    c0, _, _ = po.fd_coefs(norder)
+    x = (np.arange(NX) - npml).astype(np.float64) / NX
+    mat = np.ones((NX, NZ), np.float32)
+    ix = np.arange(2 * norder, nx + 1 - 2 * norder)  # full-order columns inside 1..nx
+    for k in range(2, 2 * norder + 1):
+        u = np.repeat((x ** k)[:, None], NZ, 1).astype(np.float32)
+        out = po.op("diffx2_reg", norder, nx, nz, npml, 1, 1.0, 1.0, c0, mat, u)
+        exact = k * (k - 1) * x ** (k - 2) / NX ** 2
+        got = out[ix + npml - 1, 5]
+        assert np.allclose(got, exact[ix + npml - 1], rtol=2e-3, atol=2e-3 * np.abs(exact).max())
+
+
+def _energy(st, c, NX, NZ, dt):
+    ux = st["ux"].reshape(NX, NZ, 2)
+    uz = st["uz"].reshape(NX, NZ, 2)
+    rho = c.reshape(NX, NZ, 3)[..., 0].astype(np.float64)
+    vx = (ux[..., 0].astype(np.float64) - ux[..., 1]) / dt
+    vz = (uz[..., 0].astype(np.float64) - uz[..., 1]) / dt
+    return float((rho * (vx * vx + vz * vz)).sum())
+
+
+def test_pml_absorbs_outgoing_pulse():
+    nx, nz, npml, norder = 101, 60, 20, 3
+    NX, NZ = nx + 2 * npml, nz + npml
+    dx, dt = 200.0, 0.01
+    vp, vs = layered_grid(nx, nz, npml, dx, dx, [(0.0, 6000.0, 3450.0)])
+    c = refstate.material_law(vp, vs)
+    st = refstate.make_state(nx, nz, npml, norder, dx, dx, dt, 2000, c)
+    X, Z = np.meshgrid((np.arange(NX) - npml - 50) * dx, (np.arange(NZ) - 40) * dx, indexing="ij")
+    pulse = (1e-3 * np.exp(-(X * X + Z * Z) / (2 * 1200.0 ** 2))).astype(np.float32)
+    ux = st["ux"].reshape(NX, NZ, 2)
+    ux[..., 0] = pulse
+    ux[..., 1] = pulse
+    e0 = None
+    energies = []
+    for it in range(900):
+        po.updatedispl(st, it)
+        if it == 5:
+            e0 = _energy(st, c, NX, NZ, dt)
+        if it % 50 == 0:
+            energies.append(_energy(st, c, NX, NZ, dt))
+    ef = _energy(st, c, NX, NZ, dt)
+    assert np.isfinite(ef)
+    assert ef < 1e-3 * e0
+    # after the first crossings the energy only decreases
+    tail = energies[-8:]
+    assert all(b <= a * 1.05 for a, b in zip(tail, tail[1:]))
+
+
+def test_long_run_stable_at_cfl_limit():
+    nx, nz, npml, norder = 61, 40, 12, 3
+    NX, NZ = nx + 2 * npml, nz + npml
+    dx = 200.0
+    rng = np.random.default_rng(11)
+    vp = rng.uniform(5500.0, 8000.0, (NX, NZ)).astype(np.float32)
+    vs = (vp / 1.75).astype(np.float32)
+    vs[:, :6] *= np.float32(0.6)   # slow, rough near-surface layer
+    c = refstate.material_law(vp, vs)
+    c2 = po.fd_coefs(norder)[2]
+    rc, dts = po.check_stability(nx, nz, npml, 1, norder, dx, dx, 0.01, c2, c)
+    dt = 0.97 * dts
+    st = refstate.make_state(nx, nz, npml, norder, dx, dx, dt, 3000, c)
+    ux = st["ux"].reshape(NX, NZ, 2)
+    ux[npml + 30, 3, :] = 1e-3
+    for it in range(4000):
+        po.updatedispl(st, it)
+    m = float(np.abs(st["ux"]).max())
+    assert np.isfinite(m) and m < 1e-3
+
+
+def test_hybrid_self_consistency_layered():
+    """FD model == FK model: surface seismograms match the FK surface response.
+
+    The FD grid switches material between the rows at 5800 and 6000 m; the
+    harmonic-mean fluxes put that interface at the half point 5900 m, which
+    is where the FK model carries it."""
+    layers = ((0.0, 6000.0, 3450.0), (6000.0, 8000.0, 4450.0))
+    case = Case(nx=151, nz=60, npml=20, nt=700, layers=layers, anomaly=None, f0=1.0, nrcv=7)
+    case.fk = fk_model(((0.0, 6000.0, 3450.0), (5900.0, 8000.0, 4450.0)))
+    bg = case.oracle_background()
+    st = case.state()
+    fx, fz = oracle_run(case, st, bg, case.nt + 1)
+    # FK surface traces at the receiver columns: translation of the depth-0 trace
+    p = case.rayp
+    xe = (case.nx + case.nbd - 1) * case.dx            # wave toward -x (baz = pi)
+    ux, uz = po.fk_traces(case.fk, True, p, case.f0, case.ds, case.dt, case.nt + 400,
+                          np.array([0.0]), (case.nz + case.nbd - 1) * case.dz, 1.5 / case.f0)
+    n = np.arange(case.nt + 1)
+    errs = []
+    for i, rx in enumerate(case.rx):
+        sh = p * (xe - rx) / case.dt
+        kx = -np.interp(n - sh, np.arange(ux.shape[1]), ux[0])
+        kz = np.interp(n - sh, np.arange(uz.shape[1]), uz[0])
+        errs.append((rel_l2(fx[i], kx), rel_l2(fz[i], kz)))
+    errs = np.array(errs)
+    assert errs[:, 0].max() < 0.01 and errs[:, 1].max() < 0.005, errs
+    # scattered field in the padding stays small relative to the incident field
+    NX, NZ = case.NX, case.NZ
+    uxf = st["ux"].reshape(NX, NZ, 2)[..., 0]
+    assert np.abs(uxf[: case.npml - case.nbd]).max() < 0.005 * np.abs(fx).max()
