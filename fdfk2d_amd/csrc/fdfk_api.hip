@@ -89,7 +89,7 @@ struct fdfk_solver {
     float *rx = nullptr, *rz = nullptr, *seisx = nullptr, *seisz = nullptr;
     float *stage = nullptr;   // (2, NZ, NX) staging for reference-layout copies
     // interior kernel
-    int lx = 90, nt_threads = 256;
+    int lx = 90, nt_threads = 256, kernel_variant = 2;
     // profiling
     bool prof = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_int, ev_step;
@@ -235,8 +235,11 @@ void launch_interior_t(fdfk_solver *s)
     int ncols = g.ix1 - g.ix0 + 1, nrows = g.iz1 - g.iz0 + 1;
     a.lx = s->lx;
     a.nbx = (ncols + a.lx - 1) / a.lx;
-    int nbz = (nrows + TZ - 1) / TZ;
-    hipLaunchKernelGGL((k_interior<NO, NT>), dim3(a.nbx * nbz), dim3(NT), 0, s->stream, a);
+    a.nbz = (nrows + TZ - 1) / TZ;
+    if (s->kernel_variant == 1)
+        hipLaunchKernelGGL((k_interior<NO, NT>), dim3(a.nbx * a.nbz), dim3(NT), 0, s->stream, a);
+    else
+        hipLaunchKernelGGL((k_interior2<NO, NT>), dim3(a.nbx * a.nbz), dim3(NT), 0, s->stream, a);
 }
 
 void launch_interior(fdfk_solver *s)
@@ -381,6 +384,7 @@ int fdfk_create(const fdfk_job *job, fdfk_solver **out)
     s->R0 = job->R0; s->npower = job->npower;
     if (const char *e = std::getenv("FDFK_LX")) s->lx = std::max(9, std::atoi(e));
     if (const char *e = std::getenv("FDFK_NT")) s->nt_threads = std::atoi(e);
+    if (const char *e = std::getenv("FDFK_KERNEL")) s->kernel_variant = std::atoi(e);
     const long ncell = (long)g.NX * g.NZ;
     // vpml = max Vp (src/FDFK2D.f90:416)
     float vpml = job->vpml;

@@ -115,6 +115,7 @@ struct IntArgs {
     const float *bf, *lam, *mu;
     int lx;   // output columns per block
     int nbx;  // blocks along x
+    int nbz;  // blocks along z
 };
 
 }  // namespace fdfk

@@ -767,6 +767,204 @@ __global__ __launch_bounds__(NT) void k_interior(IntArgs<NO> a)
 
 
 // ---------------------------------------------------------------------
+// interior kernel, pipelined variant: one barrier per column (double-
+// buffered LDS), loads issued one column ahead, outputs of column ix-1
+// written while column ix is staged, hardware reciprocals
+// ---------------------------------------------------------------------
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+template <int NO, int NT>
+struct IntKernel2 {
+    static constexpr int HZ = 2 * NO - 1;
+    static constexpr int TZ = NT - 2 * HZ;
+    static constexpr int P0 = 3 * NO + 1;
+    static constexpr int P = P0 + (P0 & 1);     // even ring period: LDS parity is compile-time
+    static constexpr int PAD = NO + 1;
+    static constexpr int WARM = 4 * NO - 2;
+    static constexpr int LW = NT + 2 * PAD;
+
+    struct Smem {
+        float uA[2][2][LW];   // [parity][comp] u at column ix+NO
+        float uB[2][2][LW];   // u at column ix
+        float rM[2][2][LW];   // 1/(l+2m), 1/m at column ix
+        float fz[2][2][LW];   // z-fluxes at column ix
+        float vz[2][2][LW];   // m*Dx uz, l*Dx ux at column ix
+    };
+
+    static __device__ __forceinline__ int sl(int ph, int j) { return ((ph + j) % P + P) % P; }
+
+    struct Regs {
+        float U[2][P], FX[2][P], VX[2][P];
+        float rMa_prev, rMm_prev;
+        float pu[2][2];     // [parity][comp] next new u column
+        float pl[2][2];     // [parity][l,m] next lead materials (ix+NO)
+        float p0[2][2];     // [parity][l,m] next column materials (ix)
+        float po[2][3];     // [parity][bf, uxp, uzp] of column ix (used one step later)
+    };
+
+    template <int PH>
+    static __device__ __forceinline__ void step(const IntArgs<NO> &a, Smem &S, Regs &R, int lane, int r,
+                                                bool rowok, bool outrow, int ix, int xs, int xe)
+    {
+        constexpr int PA = PH & 1, PB = 1 - PA;
+        const Geo &g = a.g;
+        const size_t rowoff = (size_t)(r - 1);
+        auto at = [&](int c) { return (size_t)(c + g.npml - 1) * (size_t)g.ld + rowoff; };
+        const bool active = ix < xe;
+        float l1 = 1.0f, m1 = 1.0f, l0 = 1.0f, m0 = 1.0f;
+        if (active) {
+            // consume the prefetched column ix+2NO-1 and lead materials, prefetch the next ones
+            R.U[0][sl(PH, 2 * NO - 1)] = R.pu[PA][0];
+            R.U[1][sl(PH, 2 * NO - 1)] = R.pu[PA][1];
+            l1 = R.pl[PA][0]; m1 = R.pl[PA][1];
+            l0 = R.p0[PA][0]; m0 = R.p0[PA][1];
+            if (rowok && ix + 1 < xe) {
+                size_t cn = at(ix + 2 * NO), cm = at(ix + 1 + NO), c0 = at(ix + 1);
+                R.pu[PB][0] = a.ux[cn]; R.pu[PB][1] = a.uz[cn];
+                R.pl[PB][0] = a.lam[cm]; R.pl[PB][1] = a.mu[cm];
+                R.p0[PB][0] = a.lam[c0]; R.p0[PB][1] = a.mu[c0];
+            }
+            if (outrow && ix >= xs) {
+                size_t c = at(ix);
+                R.po[PA][0] = a.bf[c]; R.po[PA][1] = a.uxp[c]; R.po[PA][2] = a.uzp[c];
+            }
+            float rMa = frcp(l1 + 2.0f * m1), rMm = frcp(m1);
+            if (ix >= xs - 2 * NO) {
+                float ha = frcp(R.rMa_prev + rMa), hm = frcp(R.rMm_prev + rMm);
+                float s0 = 0.0f, s1 = 0.0f;
+#pragma unroll
+                for (int j = 1; j <= NO; ++j) {
+                    s0 += a.k.ai[j - 1] * (R.U[0][sl(PH, NO - 1 + j)] - R.U[0][sl(PH, NO - j)]);
+                    s1 += a.k.ai[j - 1] * (R.U[1][sl(PH, NO - 1 + j)] - R.U[1][sl(PH, NO - j)]);
+                }
+                R.FX[0][sl(PH, NO - 1)] = ha * s0;
+                R.FX[1][sl(PH, NO - 1)] = hm * s1;
+                S.uA[PA][0][PAD + lane] = R.U[0][sl(PH, NO)];
+                S.uA[PA][1][PAD + lane] = R.U[1][sl(PH, NO)];
+                S.uB[PA][0][PAD + lane] = R.U[0][sl(PH, 0)];
+                S.uB[PA][1][PAD + lane] = R.U[1][sl(PH, 0)];
+                S.rM[PA][0][PAD + lane] = frcp(l0 + 2.0f * m0);
+                S.rM[PA][1][PAD + lane] = frcp(m0);
+            }
+            R.rMa_prev = rMa;
+            R.rMm_prev = rMm;
+        }
+        if (ix < xs - 2 * NO) return;  // block-uniform: no LDS traffic yet
+        __syncthreads();
+        if (active) {
+            // Vx at column ix+NO
+            float dzz = 0.0f, dzx = 0.0f;
+#pragma unroll
+            for (int j = 1; j <= NO; ++j) {
+                dzz += a.k.bi[j - 1] * (S.uA[PA][1][PAD + lane + j] - S.uA[PA][1][PAD + lane - j]);
+                dzx += a.k.bi[j - 1] * (S.uA[PA][0][PAD + lane + j] - S.uA[PA][0][PAD + lane - j]);
+            }
+            R.VX[0][sl(PH, NO)] = l1 * dzz;
+            R.VX[1][sl(PH, NO)] = m1 * dzx;
+            if (ix >= xs) {
+                float sfx = 0.0f, sfz = 0.0f;
+#pragma unroll
+                for (int j = 1; j <= NO; ++j) {
+                    sfx += a.k.ai[j - 1] * (S.uB[PA][0][PAD + lane + j] - S.uB[PA][0][PAD + lane + 1 - j]);
+                    sfz += a.k.ai[j - 1] * (S.uB[PA][1][PAD + lane + j] - S.uB[PA][1][PAD + lane + 1 - j]);
+                }
+                float rMa0 = S.rM[PA][0][PAD + lane], rMm0 = S.rM[PA][1][PAD + lane];
+                float fzx = sfx * frcp(rMm0 + S.rM[PA][1][PAD + lane + 1]);
+                float fzz = sfz * frcp(rMa0 + S.rM[PA][0][PAD + lane + 1]);
+                float dxz = 0.0f, dxx = 0.0f;
+#pragma unroll
+                for (int k = 1; k <= NO; ++k) {
+                    dxz += a.k.bi[k - 1] * (R.U[1][sl(PH, k)] - R.U[1][sl(PH, -k)]);
+                    dxx += a.k.bi[k - 1] * (R.U[0][sl(PH, k)] - R.U[0][sl(PH, -k)]);
+                }
+                S.fz[PA][0][PAD + lane] = fzx;
+                S.fz[PA][1][PAD + lane] = fzz;
+                S.vz[PA][0][PAD + lane] = m0 * dxz;
+                S.vz[PA][1][PAD + lane] = l0 * dxx;
+            }
+        }
+        // outputs of column ix-1 (its z-fluxes were staged in the previous step)
+        if (outrow && ix - 1 >= xs && ix - 1 < xe) {
+            float Xx = 0.0f, Xz = 0.0f, Zx = 0.0f, Zz = 0.0f, Mx = 0.0f, Mz = 0.0f;
+#pragma unroll
+            for (int k = 1; k <= NO; ++k) {
+                Xx += a.k.aox[k - 1] * (R.FX[0][sl(PH, k - 2)] - R.FX[0][sl(PH, -k - 1)]);
+                Xz += a.k.aox[k - 1] * (R.FX[1][sl(PH, k - 2)] - R.FX[1][sl(PH, -k - 1)]);
+                Zx += a.k.aoz[k - 1] * (S.fz[PB][0][PAD + lane + k - 1] - S.fz[PB][0][PAD + lane - k]);
+                Zz += a.k.aoz[k - 1] * (S.fz[PB][1][PAD + lane + k - 1] - S.fz[PB][1][PAD + lane - k]);
+                Mx += a.k.bo[k - 1] * ((R.VX[0][sl(PH, k - 1)] - R.VX[0][sl(PH, -k - 1)]) +
+                                      (S.vz[PB][0][PAD + lane + k] - S.vz[PB][0][PAD + lane - k]));
+                Mz += a.k.bo[k - 1] * ((R.VX[1][sl(PH, k - 1)] - R.VX[1][sl(PH, -k - 1)]) +
+                                      (S.vz[PB][1][PAD + lane + k] - S.vz[PB][1][PAD + lane - k]));
+            }
+            size_t c = at(ix - 1);
+            float bf = R.po[PB][0];
+            a.uxp[c] = (2.0f * R.U[0][sl(PH, -1)] - R.po[PB][1]) + bf * ((Xx + Zx) + Mx);
+            a.uzp[c] = (2.0f * R.U[1][sl(PH, -1)] - R.po[PB][2]) + bf * ((Xz + Zz) + Mz);
+        }
+    }
+
+    static __device__ void run(const IntArgs<NO> &a)
+    {
+        __shared__ Smem S;
+        const Geo &g = a.g;
+        const int lane = threadIdx.x;
+        for (int i = lane; i < LW; i += NT)
+            if (i < PAD || i >= NT + PAD)
+                for (int q = 0; q < 2; ++q)
+                    for (int c = 0; c < 2; ++c) {
+                        S.uA[q][c][i] = 0.0f; S.uB[q][c][i] = 0.0f; S.rM[q][c][i] = 1.0f;
+                        S.fz[q][c][i] = 0.0f; S.vz[q][c][i] = 0.0f;
+                    }
+        // block -> tile: consecutive blocks walk down z first so that the x
+        // neighbours sharing warm-up columns run close in time
+        const int nbz = a.nbz;
+        const int bzid = blockIdx.x % nbz, bxid = blockIdx.x / nbz;
+        const int z0 = g.iz0 + bzid * TZ;
+        const int r = z0 - HZ + lane;
+        const bool rowok = (r >= g.iz0 - HZ) && (r <= g.iz1 + HZ);
+        const bool outrow = (lane >= HZ) && (lane < HZ + TZ) && (r <= g.iz1);
+        const int xs = g.ix0 + bxid * a.lx;
+        const int xe = min(xs + a.lx, g.ix1 + 1);
+        Regs R;
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int i = 0; i < P; ++i) { R.U[q][i] = 0.0f; R.FX[q][i] = 0.0f; R.VX[q][i] = 0.0f; }
+        R.rMa_prev = 1.0f; R.rMm_prev = 1.0f;
+        const int x0 = xs - WARM;
+        // prefetch for the first step
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            R.pu[0][q] = 0.0f; R.pu[1][q] = 0.0f; R.pl[0][q] = 1.0f; R.pl[1][q] = 1.0f;
+            R.p0[0][q] = 1.0f; R.p0[1][q] = 1.0f;
+        }
+#pragma unroll
+        for (int q = 0; q < 3; ++q) { R.po[0][q] = 0.0f; R.po[1][q] = 0.0f; }
+        if (rowok) {
+            size_t rowoff = (size_t)(r - 1);
+            auto at = [&](int c) { return (size_t)(c + g.npml - 1) * (size_t)g.ld + rowoff; };
+            R.pu[0][0] = a.ux[at(x0 + 2 * NO - 1)]; R.pu[0][1] = a.uz[at(x0 + 2 * NO - 1)];
+            R.pl[0][0] = a.lam[at(x0 + NO)]; R.pl[0][1] = a.mu[at(x0 + NO)];
+            R.p0[0][0] = a.lam[at(x0)]; R.p0[0][1] = a.mu[at(x0)];
+        }
+        __syncthreads();
+        for (int base = x0; base <= xe; base += P) {
+#define IK2_STEP(PH) if constexpr (P > PH) { if (base + PH <= xe) step<PH>(a, S, R, lane, r, rowok, outrow, base + PH, xs, xe); }
+            IK2_STEP(0) IK2_STEP(1) IK2_STEP(2) IK2_STEP(3) IK2_STEP(4) IK2_STEP(5) IK2_STEP(6)
+            IK2_STEP(7) IK2_STEP(8) IK2_STEP(9) IK2_STEP(10) IK2_STEP(11) IK2_STEP(12) IK2_STEP(13)
+#undef IK2_STEP
+        }
+    }
+};
+
+template <int NO, int NT>
+__global__ __launch_bounds__(NT) void k_interior2(IntArgs<NO> a)
+{
+    IntKernel2<NO, NT>::run(a);
+}
+
+// ---------------------------------------------------------------------
 // layout conversion helpers
 // ---------------------------------------------------------------------
 // reference (1:3, NZ, NX) -> bf = dt^2/rho, lam, mu in the padded layout

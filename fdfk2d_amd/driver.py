@@ -1,0 +1,84 @@
+"""FDFK2D driver on the GPU path: the command line and loop of
+src/FDFK2D.f90:32-588 with the time step and the FK background on the
+MI355X (C ABI in include/fdfk.h).
+
+    python -m fdfk2d_amd.driver ./input inpar.dat ./seismograms seis
+
+Reads the unchanged input files, writes <outpath>/<prefix>x.su and z.su
+exactly as the reference (radial projection by cos(baz) included).
+"""
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import geometry, io, refstate
+from .lib import Solver
+
+
+def simulate(job, log=print):
+    fd, src = job["fd"], job["src"]
+    nx, nz, npml, norder = job["nx"], job["nz"], fd["npml"], fd["norder"]
+    dt = fd["dt"]
+    nt = int(np.rint(fd["tmax"] / dt))                       # FDFK2D.f90:311-315
+    tstp = fd["tstp"] if fd["tstp"] >= dt else dt
+    nstp = int(np.rint(tstp / dt))
+    nsis = nt // nstp
+    tstp = np.float32(nstp * dt)
+    dts = refstate.stable_dt(norder, fd["dx"], fd["dz"], job["c"])
+    if dt > dts:                                             # Submain.f90:149-155
+        raise ValueError(f"The dt is too large to stablize the simulation; stable dt = {dts:.7f}")
+    bazs = geometry.column_bazs(nx, npml, fd["x0"], fd["dx"], fd["lat_org"], fd["lon_org"],
+                                fd["az_org"], src["evla"], src["evlo"])
+    if abs(np.cos(bazs.sum() / (nx + 2 * npml + 1))) < 0.5 * np.sqrt(3.0):
+        log("WARNING: the profile is more than 30 deg off the back-azimuth (FDFK2D.f90:388-401)")
+    rayp = src["rayp"]
+    if abs(rayp) < 1e-9:
+        rayp = 1e-9 * np.sign(rayp if rayp != 0 else 1.0)
+    s = Solver(nx, nz, npml, norder, fd["dx"], fd["dz"], dt, nt, job["c"], x0=fd["x0"], z0=fd["z0"],
+               nstp=nstp, rx=job["rx"], rz=job["rz"], R0=1e-3, npower=2,
+               vpml=refstate.vp_max(job["c"]))
+    t0 = time.perf_counter()
+    s.compute_background(job["fk_left"], job["fk_right"], src["src_is_p"], src["f0"], src["ds"],
+                         rayp, bazs)
+    s.sync()
+    t1 = time.perf_counter()
+    log(f"background (FK, GPU FP64): {t1 - t0:.2f} s")
+    s.run(nt + 1)
+    s.sync()
+    t2 = time.perf_counter()
+    cells = (nx + 2 * npml) * (nz + npml) * (nt + 1)
+    log(f"time loop: {nt + 1} iterations in {t2 - t1:.2f} s ({cells / (t2 - t1) / 1e9:.2f} Gcell/s)")
+    sx, sz = s.seismograms()
+    s.close()
+    # radial projection (FDFK2D.f90:537-543)
+    br = geometry.backazimuth_grids(fd["lat_org"], fd["lon_org"], fd["az_org"], job["rx"],
+                                    src["evla"], src["evlo"]) - fd["az_org"] * np.pi / 180.0
+    sx = (sx.astype(np.float64) * np.cos(br)[:, None]).astype(np.float32)
+    return sx, sz, dict(nt=nt, nsis=nsis, tstp=float(tstp))
+
+
+def main(argv=None):
+    argv = sys.argv[1:] if argv is None else argv
+    if len(argv) < 3:
+        print("FDFK2D (MI355X path)\n./driver ./inpath inpar.dat ./outpath seism")
+        return 0
+    prefix, finpar, outpath = argv[:3]
+    fsis = argv[3] if len(argv) > 3 else "seis"
+    job = io.read_job(prefix, finpar)
+    # *_without_pml.su copies of the models (Input.f90:305-341)
+    for key, arr in (("vp_model", job["vp"]), ("vs_model", job["vs"])):
+        name = job["names"][key]
+        io.write_su(os.path.join(prefix, name[:-3] + "_without_pml.su"),
+                    arr[job["fd"]["npml"]: job["fd"]["npml"] + job["nx"], : job["nz"]], 1000)
+    sx, sz, info = simulate(job)
+    os.makedirs(outpath, exist_ok=True)
+    dh = int(np.float32(info["tstp"]) * np.float32(1e6))
+    io.write_su(os.path.join(outpath, fsis + "x.su"), sx, dh)
+    io.write_su(os.path.join(outpath, fsis + "z.su"), sz, dh)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

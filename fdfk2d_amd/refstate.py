@@ -19,8 +19,10 @@ def material_law(vp, vs):
     return np.stack([rho, lam, mu], axis=-1).astype(np.float32).reshape(-1)
 
 
-def fd_coef(norder, staggered):
-    """coef(j, i) as a flat Fortran (norder, norder) array (Submain.f90:1111-1198)."""
+def fd_coef(norder, staggered, second=False):
+    """coef(j, i) as a flat Fortran (norder, norder) array: staggered first
+    derivative (fd_coef0), collocated first (fd_coef1) or collocated second
+    derivative (fd_coef2), src/Submain.f90:1111-1243."""
     c = np.zeros(norder * norder, np.float64)
     for i in range(1, norder + 1):
         for j in range(1, i + 1):
@@ -33,9 +35,22 @@ def fd_coef(norder, staggered):
                 num *= node(k)
                 hi *= node(k) - node(j)
             num *= 1.0 if j % 2 == 1 else -1.0
-            lo *= float(2 * j - 1) if staggered else 2.0 * j
+            if second:
+                lo *= float(j * j)
+            else:
+                lo *= float(2 * j - 1) if staggered else 2.0 * j
             c[(j - 1) + norder * (i - 1)] = num / lo / hi
     return c
+
+
+def stable_dt(norder, dx, dz, c):
+    """dt_stable of check_stability (src/Submain.f90:139-146)."""
+    c2 = fd_coef(norder, False, second=True)
+    col = c2[(norder - 1) * norder: norder * norder]
+    h = max(dx, dz)
+    cx, cz = (h / dx) ** 2, (h / dz) ** 2
+    cfl = np.sqrt(2.0 / ((cx + cz) * (np.abs(col).sum() + col.sum())))
+    return float(np.float32(cfl) * np.float32(h) / np.float32(vp_max(c)))
 
 
 def vp_max(c):

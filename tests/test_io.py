@@ -1,0 +1,64 @@
+"""Host-side formats: the input files and SU files round-trip through the
+package's writers and readers (src/Input.f90 layouts), the material law and
+geometry restatements agree with the oracle."""
+import numpy as np
+
+from fdfk2d_amd import geometry, io, refstate
+from oracle import pyoracle as po
+from tests._cases import layered_grid
+
+
+def test_inputs_round_trip(tmp_path):
+    nx, nz, npml, dx = 41, 20, 10, 200.0
+    vp, vs = layered_grid(nx, nz, npml, dx, dx, [(0.0, 6000.0, 3450.0), (2000.0, 8000.0, 4450.0)])
+    rx = np.arange(0, 8001, 1000, dtype=np.float32)
+    fk = [(0.0, 6000.0, 3450.0), (1900.0, 8000.0, 4450.0)]
+    io.write_inputs(str(tmp_path), order=6, x0=0.0, xn=8000.0, z0=0.0, zn=3800.0, dx=dx, dz=dx,
+                    dt=0.01, tmax=1.0, tstp=0.02, npml=npml, lat_org=45.0, lon_org=0.0, az_org=0.0,
+                    f0=1.0, ds=1e-3, src_is_p=False, rayp_sdeg=9.535, evla=-42.0, evlo=10.0, rx=rx,
+                    rz=np.zeros_like(rx), fk_left=fk, fk_right=fk, vp=vp, vs=vs)
+    job = io.read_job(str(tmp_path), "inpar.dat")
+    assert (job["nx"], job["nz"], job["fd"]["norder"], job["fd"]["npml"]) == (nx, nz, 3, npml)
+    assert job["src"]["src_is_p"] is False
+    assert abs(job["src"]["rayp"] - 9.535 * 180 / (np.pi * 6371e3)) < 1e-15
+    assert np.array_equal(job["rx"], rx)
+    assert np.array_equal(job["vp"], vp) and np.array_equal(job["vs"], vs)
+    assert job["fk_left"].shape == (2, 4) and job["fk_left"][1, 0] == 1900.0
+    c = refstate.material_law(vp, vs)
+    ref = np.stack([np.zeros_like(vp), vp, vs], -1).reshape(-1).astype(np.float32)
+    po.material_law(ref)
+    assert np.array_equal(c, ref) and np.array_equal(job["c"], c)
+
+
+def test_fk_model_reader_fortran_exponents(tmp_path):
+    p = tmp_path / "fk.dat"
+    p.write_text("nlayer\n  2\nVp Vs z\n6000.0    3450.0    0.d3\n8000.0    4450.0    35.d3\n"
+                 "9000.0 5000.0 99.d3\n")
+    fk = io.read_fk_model(str(p))
+    assert fk.shape == (2, 4) and fk[1, 0] == 35000.0
+    assert fk[0, 3] == np.float32((np.float32(6000.0) + np.float32(980.0)) / np.float32(2.76))
+
+
+def test_su_round_trip(tmp_path):
+    tr = np.random.default_rng(0).standard_normal((7, 33)).astype(np.float32)
+    io.write_su(str(tmp_path / "a.su"), tr, 10000)
+    back, head = io.read_su(str(tmp_path / "a.su"))
+    assert np.array_equal(back, tr) and head[57] == 33 and head[58] == 10000
+    raw = (tmp_path / "a.su").read_bytes()
+    assert len(raw) == 7 * (240 + 4 * 33)
+
+
+def test_geometry_matches_oracle_restatement():
+    x = (np.arange(-20, 1022) * 200.0).astype(np.float32)
+    for args in [(45.0, 0.0, 0.0, -42.0, 10.0), (40.372214, 88.173868, 134.1466634471141, -0.0451, 124.2812)]:
+        a = geometry.backazimuth_grids(args[0], args[1], args[2], x, args[3], args[4])
+        b = po.backazimuth_grids(args[0], args[1], args[2], x, args[3], args[4])
+        assert np.abs(a - b).max() < 1e-12
+
+
+def test_stable_dt_matches_oracle():
+    nx, nz, npml = 41, 20, 10
+    vp, vs = layered_grid(nx, nz, npml, 50.0, 50.0, [(0.0, 5000.0, 2900.0), (600.0, 7800.0, 4500.0)])
+    c = refstate.material_law(vp, vs)
+    _, dts = po.check_stability(nx, nz, npml, 1, 3, 50.0, 50.0, 0.001, po.fd_coefs(3)[2], c)
+    assert abs(refstate.stable_dt(3, 50.0, 50.0, c) - dts) <= 1e-7 * dts
