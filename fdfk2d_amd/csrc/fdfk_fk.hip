@@ -342,7 +342,8 @@ extern "C" int fdfk_compute_background(fdfk_solver *s, const fdfk_fk *fk)
     for (int i = 0; i < g.NX; ++i) cs += std::cos(fk->bazs[i]);
     double rayp = fk->rayp;
     if (std::fabs(rayp) < 1e-9) rayp = rayp < 0 ? -1e-9 : 1e-9;  // src/FDFK2D.f90:431
-    const double sgn = (rayp < 0.0 ? -1.0 : 1.0) * (cs < 0.0 ? -1.0 : 1.0);
+    // travel away from the event: along-profile slowness -p cos(baz)
+    const double sgn = (rayp < 0.0 ? -1.0 : 1.0) * (cs < 0.0 ? 1.0 : -1.0);
     const double p = std::fabs(rayp);
     const double f0 = fk->f0, t0 = 1.5 / f0, dt = g.dt;
     FKModel ml{}, mr{};

@@ -20,8 +20,12 @@
  *  - per depth: IFFT to an oversampled time grid (dt/2), then each ring column
  *    is the same trace delayed by p |x - x_entry| (plane-wave translation
  *    invariance), read with 4-point cubic Lagrange interpolation.
- *  The profile direction sign s = sign(rayp) * sign(sum cos(bazs)) orients the
- *  wave along +x (s > 0) or -x; ux carries that sign.  Left strips use the left
+ *  The wave travels away from the event: its along-profile slowness is
+ *  -p cos(baz) (propagation azimuth = back-azimuth + 180 deg), so the
+ *  direction sign is s = -sign(rayp) * sign(sum cos(bazs)); s > 0 is +x, and
+ *  ux carries s (mirror image of the +x solution).  With the driver's radial
+ *  projection seisx * cos(baz) (src/FDFK2D.f90:540-542) and z down, R/Z of the
+ *  direct P is positive, as in the reference's published P receiver function.  Left strips use the left
  *  model, right strips the right model, the bottom strip blends them linearly
  *  in x (identical when the models agree).
  */
@@ -305,7 +309,7 @@ void orc_comput_background_wavefield(const int *nx_, const int *nz_, const int *
     (void)izt_;
     double cs = 0.0;
     for (int i = 0; i < nx + 2 * npml; ++i) cs += cos(bazs[i]);
-    double sgn = ((*rayp_ < 0.0) ? -1.0 : 1.0) * (cs < 0.0 ? -1.0 : 1.0);
+    double sgn = ((*rayp_ < 0.0) ? -1.0 : 1.0) * (cs < 0.0 ? 1.0 : -1.0);
     double p = fabs(*rayp_);
     double t0 = 1.5 / (double)*f0_;
 

@@ -149,14 +149,14 @@ def test_hybrid_self_consistency_layered():
     fx, fz = oracle_run(case, st, bg, case.nt + 1)
     # FK surface traces at the receiver columns: translation of the depth-0 trace
     p = case.rayp
-    xe = (case.nx + case.nbd - 1) * case.dx            # wave toward -x (baz = pi)
+    xe = (1 - case.nbd - 1) * case.dx                  # baz = pi: wave toward +x, enters on the left
     ux, uz = po.fk_traces(case.fk, True, p, case.f0, case.ds, case.dt, case.nt + 400,
                           np.array([0.0]), (case.nz + case.nbd - 1) * case.dz, 1.5 / case.f0)
     n = np.arange(case.nt + 1)
     errs = []
     for i, rx in enumerate(case.rx):
-        sh = p * (xe - rx) / case.dt
-        kx = -np.interp(n - sh, np.arange(ux.shape[1]), ux[0])
+        sh = p * (rx - xe) / case.dt
+        kx = np.interp(n - sh, np.arange(ux.shape[1]), ux[0])
         kz = np.interp(n - sh, np.arange(uz.shape[1]), uz[0])
         errs.append((rel_l2(fx[i], kx), rel_l2(fz[i], kz)))
     errs = np.array(errs)
