@@ -304,11 +304,11 @@ int one_step(fdfk_solver *s, bool record)
         r.g = g; r.x0 = s->x0; r.z0 = s->z0; r.nrcv = s->nrcv; r.nsamp = s->nsamp; r.istp = s->istp;
         r.rx = s->rx; r.rz = s->rz; r.ux = s->ux[s->cur]; r.uz = s->uz[s->cur];
         r.seisx = s->seisx; r.seisz = s->seisz;
-        hipLaunchKernelGGL(k_record, dim3((s->nrcv + 63) / 64), dim3(64), 0, s->stream, r);
+        hipLaunchKernelGGL(k_record, dim3(s->nrcv), dim3(64), 0, s->stream, r);
     }
     StepArgs a = step_args(s);
     if (a.ring_valid) {
-        long nr = (long)(g.nx + 2 * g.nbd) * (g.nz + g.nbd);
+        long nr = ring_count(g);
         hipLaunchKernelGGL(k_ring_prep, dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, s->stream, g, s->d_bg,
                            s->it, s->ringx, s->ringz);
     }
@@ -398,6 +398,10 @@ int fdfk_create(const fdfk_job *job, fdfk_solver **out)
     s->vpml = vpml;
     coef_table(g.norder, true, s->hcoef.c0);
     coef_table(g.norder, false, s->hcoef.c1);
+    for (int i = 0; i < kMaxOrder * kMaxOrder; ++i) {
+        s->hcoef.c0f[i] = (float)s->hcoef.c0[i];
+        s->hcoef.c1f[i] = (float)s->hcoef.c1[i];
+    }
     auto bad = [&](int code) { delete s; return code; };
     if (hipStreamCreate(&s->stream) != hipSuccess) return bad(fail(FDFK_EDEVICE, "hipStreamCreate failed"));
     const size_t nf = (size_t)g.NX * g.ld;
@@ -741,3 +745,4 @@ void fdfk_compat_release(void)
 }  // extern "C"
 
 #include "fdfk_fk.hip"
+#include "fdfk_batch.hip"

@@ -1,0 +1,343 @@
+// Batched execution: many resident simulations advanced together, one
+// launch per kernel type per step for all of them (a tile work-list over
+// jobs).  Each job keeps its own HBM state (a fdfk_solver handle); the batch
+// only builds, every step, a table of per-job kernel arguments plus prefix
+// sums of each job's block counts, uploads it from pinned memory and
+// launches the same kernel bodies as the single-job path.  A block finds
+// its job by binary search over the prefix sums.
+//
+// Jobs may have different grids, step counts, receivers and backgrounds;
+// they must share the FD order.  Small grids that cannot fill 256 CUs alone
+// are packed into one launch, and the per-step launch count is independent
+// of the number of jobs.
+
+namespace fdfk {
+
+constexpr int kMaxBatch = 256;
+enum { KB_RING = 0, KB_PSI, KB_INT, KB_FFULL, KB_FEDGE, KB_REC, KB_N };
+
+template <int NO>
+struct BatchJob {
+    StepArgs sa;
+    IntArgs<NO> ia;
+    ICoef<NO> k;
+    Rects rf, re;
+    RecArgs ra;
+    const Background *bg;
+    float *ringx, *ringz;
+};
+
+struct BatchPrefix {
+    int n;
+    int pre[KB_N][kMaxBatch + 1];
+};
+
+__device__ __forceinline__ int find_job(const int *pre, int n, int bid)
+{
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (pre[mid] <= bid) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+template <int NO>
+__global__ __launch_bounds__(256) void kb_ring(const BatchJob<NO> *J, const BatchPrefix *P)
+{
+    int j = find_job(P->pre[KB_RING], P->n, blockIdx.x);
+    const BatchJob<NO> &b = J[j];
+    ring_prep_body(b.sa.g, b.bg, b.sa.it, b.ringx, b.ringz, blockIdx.x - P->pre[KB_RING][j]);
+}
+
+template <int NO>
+__global__ __launch_bounds__(256) void kb_psi(const BatchJob<NO> *J, const BatchPrefix *P)
+{
+    int j = find_job(P->pre[KB_PSI], P->n, blockIdx.x);
+    psi_body(J[j].sa, blockIdx.x - P->pre[KB_PSI][j]);
+}
+
+template <int NO, int NT>
+__global__ __launch_bounds__(NT) void kb_interior(const BatchJob<NO> *J, const BatchPrefix *P)
+{
+    int j = find_job(P->pre[KB_INT], P->n, blockIdx.x);
+    IntKernel2<NO, NT>::run(J[j].ia, blockIdx.x - P->pre[KB_INT][j]);
+}
+
+template <int NO>
+__global__ __launch_bounds__(256) void kb_frame_full(const BatchJob<NO> *J, const BatchPrefix *P)
+{
+    int j = find_job(P->pre[KB_FFULL], P->n, blockIdx.x);
+    const BatchJob<NO> &b = J[j];
+    frame_full_body<NO>(b.sa, b.rf, b.k, blockIdx.x - P->pre[KB_FFULL][j]);
+}
+
+template <int NO>
+__global__ __launch_bounds__(256) void kb_frame_edge(const BatchJob<NO> *J, const BatchPrefix *P)
+{
+    int j = find_job(P->pre[KB_FEDGE], P->n, blockIdx.x);
+    frame_edge_body(J[j].sa, J[j].re, blockIdx.x - P->pre[KB_FEDGE][j]);
+}
+
+template <int NO>
+__global__ __launch_bounds__(64) void kb_record(const BatchJob<NO> *J, const BatchPrefix *P)
+{
+    int j = find_job(P->pre[KB_REC], P->n, blockIdx.x);
+    record_body(J[j].ra, blockIdx.x - P->pre[KB_REC][j]);
+}
+
+}  // namespace fdfk
+
+struct fdfk_batch {
+    int norder = 3;
+    hipStream_t stream = nullptr;
+    std::vector<fdfk_solver *> members;
+    static constexpr int R = 4;
+    void *dtab[R] = {}, *htab[R] = {};
+    hipEvent_t ev[R] = {};
+    int slot = 0;
+    size_t tabbytes = 0;
+    bool prof = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_int, ev_step;
+    ~fdfk_batch()
+    {
+        if (stream) hipStreamSynchronize(stream);
+        for (int i = 0; i < R; ++i) {
+            if (dtab[i]) hipFree(dtab[i]);
+            if (htab[i]) hipHostFree(htab[i]);
+            if (ev[i]) hipEventDestroy(ev[i]);
+        }
+        for (auto &e : ev_int) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
+        for (auto &e : ev_step) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
+        if (stream) hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+template <int NO>
+size_t batch_table_bytes()
+{
+    return sizeof(BatchJob<NO>) * kMaxBatch + sizeof(BatchPrefix);
+}
+
+// fill one job's arguments for the step it is about to take; returns the
+// block counts of the six kernels
+template <int NO>
+void fill_job(fdfk_solver *s, BatchJob<NO> &b, int nb[KB_N], bool &record)
+{
+    const Geo &g = s->g;
+    b.sa = step_args(s);
+    b.bg = s->d_bg;
+    b.ringx = s->ringx;
+    b.ringz = s->ringz;
+    b.k = make_icoef<NO>(s);
+    IntArgs<NO> &a = b.ia;
+    a = IntArgs<NO>{};
+    a.g = g;
+    a.k = b.k;
+    a.ux = s->ux[s->cur]; a.uz = s->uz[s->cur];
+    a.uxp = s->ux[1 - s->cur]; a.uzp = s->uz[1 - s->cur];
+    a.bf = s->bf; a.lam = s->lam; a.mu = s->mu;
+    constexpr int TZ = 256 - 2 * (2 * NO - 1);
+    a.lx = s->lx;
+    a.nbx = (g.ix1 - g.ix0 + 1 + a.lx - 1) / a.lx;
+    a.nbz = (g.iz1 - g.iz0 + 1 + TZ - 1) / TZ;
+    b.rf = full_rects(g);
+    b.re = edge_rects(g);
+    record = (s->isis == s->it) && (s->istp <= s->nsis) && s->nrcv > 0;
+    RecArgs &r = b.ra;
+    r = RecArgs{};
+    r.g = g; r.x0 = s->x0; r.z0 = s->z0; r.nrcv = s->nrcv; r.nsamp = s->nsamp; r.istp = s->istp;
+    r.rx = s->rx; r.rz = s->rz; r.ux = s->ux[s->cur]; r.uz = s->uz[s->cur];
+    r.seisx = s->seisx; r.seisz = s->seisz;
+    long nring = ring_count(g);
+    long npsi = 2L * g.npml * g.nz + (long)g.NX * g.npml;
+    nb[KB_RING] = b.sa.ring_valid ? (int)((nring + 255) / 256) : 0;
+    nb[KB_PSI] = (int)((npsi + 255) / 256);
+    nb[KB_INT] = a.nbx * a.nbz;
+    nb[KB_FFULL] = (int)((b.rf.off[b.rf.n] + 255) / 256);
+    nb[KB_FEDGE] = (int)((b.re.off[b.re.n] + 255) / 256);
+    nb[KB_REC] = record ? s->nrcv : 0;
+}
+
+template <int NO>
+int batch_step(fdfk_batch *B)
+{
+    std::vector<fdfk_solver *> act;
+    for (auto *s : B->members)
+        if (s->it <= s->nt) act.push_back(s);
+    if (act.empty()) return 0;
+    const int slot = B->slot;
+    HIPCHK(hipEventSynchronize(B->ev[slot]));   // the table slot is free again
+    auto *J = (BatchJob<NO> *)B->htab[slot];
+    auto *P = (BatchPrefix *)((char *)B->htab[slot] + sizeof(BatchJob<NO>) * kMaxBatch);
+    P->n = (int)act.size();
+    for (int k = 0; k < KB_N; ++k) P->pre[k][0] = 0;
+    std::vector<char> rec(act.size());
+    for (size_t j = 0; j < act.size(); ++j) {
+        int nb[KB_N];
+        bool r = false;
+        fill_job<NO>(act[j], J[j], nb, r);
+        rec[j] = r;
+        for (int k = 0; k < KB_N; ++k) P->pre[k][j + 1] = P->pre[k][j] + nb[k];
+    }
+    // copy only the used part of the job table plus the prefix block
+    auto *dJ = (BatchJob<NO> *)B->dtab[slot];
+    auto *dP = (BatchPrefix *)((char *)B->dtab[slot] + sizeof(BatchJob<NO>) * kMaxBatch);
+    HIPCHK(hipMemcpyAsync(dJ, J, sizeof(BatchJob<NO>) * act.size(), hipMemcpyHostToDevice, B->stream));
+    HIPCHK(hipMemcpyAsync(dP, P, sizeof(BatchPrefix), hipMemcpyHostToDevice, B->stream));
+    const int n = P->n;
+    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
+    if (B->prof) {
+        hipEventCreate(&e0); hipEventCreate(&e1); hipEventCreate(&e2); hipEventCreate(&e3);
+        hipEventRecord(e0, B->stream);
+    }
+    if (P->pre[KB_REC][n]) hipLaunchKernelGGL(kb_record<NO>, dim3(P->pre[KB_REC][n]), dim3(64), 0, B->stream, dJ, dP);
+    if (P->pre[KB_RING][n]) hipLaunchKernelGGL(kb_ring<NO>, dim3(P->pre[KB_RING][n]), dim3(256), 0, B->stream, dJ, dP);
+    if (P->pre[KB_PSI][n]) hipLaunchKernelGGL(kb_psi<NO>, dim3(P->pre[KB_PSI][n]), dim3(256), 0, B->stream, dJ, dP);
+    if (B->prof) hipEventRecord(e1, B->stream);
+    if (P->pre[KB_INT][n]) hipLaunchKernelGGL((kb_interior<NO, 256>), dim3(P->pre[KB_INT][n]), dim3(256), 0, B->stream, dJ, dP);
+    if (B->prof) hipEventRecord(e2, B->stream);
+    if (P->pre[KB_FFULL][n]) hipLaunchKernelGGL(kb_frame_full<NO>, dim3(P->pre[KB_FFULL][n]), dim3(256), 0, B->stream, dJ, dP);
+    if (P->pre[KB_FEDGE][n]) hipLaunchKernelGGL(kb_frame_edge<NO>, dim3(P->pre[KB_FEDGE][n]), dim3(256), 0, B->stream, dJ, dP);
+    if (B->prof) {
+        hipEventRecord(e3, B->stream);
+        B->ev_int.emplace_back(e1, e2);
+        B->ev_step.emplace_back(e0, e3);
+    }
+    HIPCHK(hipEventRecord(B->ev[slot], B->stream));
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(FDFK_EDEVICE, std::string("batch launch: ") + hipGetErrorString(e));
+    B->slot = (slot + 1) % fdfk_batch::R;
+    for (size_t j = 0; j < act.size(); ++j) {
+        fdfk_solver *s = act[j];
+        if (rec[j]) { s->istp += 1; s->isis += s->nstp; }
+        s->cur = 1 - s->cur;
+        s->it += 1;
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fdfk_batch_create(int32_t norder, fdfk_batch **out)
+{
+    if (!out) return fail(FDFK_EINVAL, "null output pointer");
+    *out = nullptr;
+    if (norder < 1 || norder > 4) return fail(FDFK_EINVAL, "norder must be 1..4");
+    auto *B = new fdfk_batch();
+    B->norder = norder;
+    switch (norder) {
+    case 1: B->tabbytes = batch_table_bytes<1>(); break;
+    case 2: B->tabbytes = batch_table_bytes<2>(); break;
+    case 3: B->tabbytes = batch_table_bytes<3>(); break;
+    default: B->tabbytes = batch_table_bytes<4>(); break;
+    }
+    if (hipStreamCreate(&B->stream) != hipSuccess) { delete B; return fail(FDFK_EDEVICE, "hipStreamCreate"); }
+    for (int i = 0; i < fdfk_batch::R; ++i) {
+        if (hipMalloc(&B->dtab[i], B->tabbytes) != hipSuccess ||
+            hipHostMalloc(&B->htab[i], B->tabbytes, hipHostMallocDefault) != hipSuccess ||
+            hipEventCreate(&B->ev[i]) != hipSuccess) {
+            delete B;
+            return fail(FDFK_ENOMEM, "batch table allocation");
+        }
+        hipEventRecord(B->ev[i], B->stream);
+    }
+    *out = B;
+    return 0;
+}
+
+int fdfk_batch_add(fdfk_batch *B, fdfk_solver *s)
+{
+    if (!B || !s) return fail(FDFK_EINVAL, "null argument");
+    if (s->g.norder != B->norder) return fail(FDFK_EINVAL, "solver FD order differs from the batch");
+    if ((int)B->members.size() >= kMaxBatch) return fail(FDFK_ERANGE, "batch is full");
+    for (auto *m : B->members)
+        if (m == s) return fail(FDFK_EINVAL, "solver already in the batch");
+    HIPCHK(hipStreamSynchronize(s->stream));   // its own queued work (background, state) is done
+    B->members.push_back(s);
+    return 0;
+}
+
+int fdfk_batch_remove(fdfk_batch *B, fdfk_solver *s)
+{
+    if (!B || !s) return fail(FDFK_EINVAL, "null argument");
+    HIPCHK(hipStreamSynchronize(B->stream));
+    for (size_t i = 0; i < B->members.size(); ++i)
+        if (B->members[i] == s) {
+            B->members.erase(B->members.begin() + i);
+            return 0;
+        }
+    return fail(FDFK_EINVAL, "solver not in the batch");
+}
+
+int fdfk_batch_run(fdfk_batch *B, int32_t nsteps)
+{
+    if (!B) return fail(FDFK_EINVAL, "null batch");
+    for (int k = 0; k < nsteps; ++k) {
+        int rc;
+        switch (B->norder) {
+        case 1: rc = batch_step<1>(B); break;
+        case 2: rc = batch_step<2>(B); break;
+        case 3: rc = batch_step<3>(B); break;
+        default: rc = batch_step<4>(B); break;
+        }
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+int fdfk_batch_active(const fdfk_batch *B)
+{
+    if (!B) return FDFK_EINVAL;
+    int n = 0;
+    for (auto *s : B->members) n += s->it <= s->nt;
+    return n;
+}
+
+int fdfk_batch_sync(fdfk_batch *B)
+{
+    if (!B) return fail(FDFK_EINVAL, "null batch");
+    HIPCHK(hipStreamSynchronize(B->stream));
+    return 0;
+}
+
+int fdfk_batch_profile(fdfk_batch *B, int32_t enable)
+{
+    if (!B) return fail(FDFK_EINVAL, "null batch");
+    B->prof = enable != 0;
+    return 0;
+}
+
+int fdfk_batch_kernel_stats(fdfk_batch *B, double *int_ms, int64_t *int_n, double *step_ms, int64_t *step_n)
+{
+    if (!B) return fail(FDFK_EINVAL, "null batch");
+    HIPCHK(hipStreamSynchronize(B->stream));
+    double a = 0, c = 0;
+    int64_t na = 0, nc = 0;
+    for (auto &e : B->ev_int) {
+        float ms = 0;
+        hipEventElapsedTime(&ms, e.first, e.second);
+        a += ms; ++na;
+        hipEventDestroy(e.first); hipEventDestroy(e.second);
+    }
+    for (auto &e : B->ev_step) {
+        float ms = 0;
+        hipEventElapsedTime(&ms, e.first, e.second);
+        c += ms; ++nc;
+        hipEventDestroy(e.first); hipEventDestroy(e.second);
+    }
+    B->ev_int.clear(); B->ev_step.clear();
+    if (int_ms) *int_ms = a;
+    if (int_n) *int_n = na;
+    if (step_ms) *step_ms = c;
+    if (step_n) *step_n = nc;
+    return 0;
+}
+
+void fdfk_batch_destroy(fdfk_batch *B) { delete B; }
+
+}  // extern "C"

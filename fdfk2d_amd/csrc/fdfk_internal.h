@@ -29,12 +29,32 @@ __host__ __device__ inline size_t cell(const Geo &g, int iz, int ix)
     return (size_t)(ix + g.npml - 1) * (size_t)g.ld + (size_t)(iz - 1);
 }
 
+// Cells per time step of the six hybrid-ring background arrays in the order
+// of src/FDFK2D.f90:330-355: left/right/bottom inside (reg), then
+// left/right/bottom outside (pml).
+__host__ __device__ inline void ring_sizes(const Geo &g, long n[6])
+{
+    const long nx = g.nx, nz = g.nz, nbd = g.nbd;
+    n[0] = n[1] = (nz - nbd) * nbd;
+    n[2] = nbd * nx;
+    n[3] = n[4] = nz * nbd;
+    n[5] = nbd * (nx + 2 * nbd);
+}
+__host__ __device__ inline long ring_count(const Geo &g)
+{
+    long n[6];
+    ring_sizes(g, n);
+    return n[0] + n[1] + n[2] + n[3] + n[4] + n[5];
+}
+
 // FD coefficient tables coef(j, i) = c[(j-1) + norder*(i-1)] in FP64 (as the
 // reference's fd_coef0/fd_coef1) plus full-order FP32 copies for the
 // interior kernel.
 struct Coefs {
     double c0[kMaxOrder * kMaxOrder];
     double c1[kMaxOrder * kMaxOrder];
+    float c0f[kMaxOrder * kMaxOrder];   // FP32 copies used by the pointwise kernels
+    float c1f[kMaxOrder * kMaxOrder];
 };
 
 // Padding strips of the PML (left, right, bottom).  Cell offsets inside a

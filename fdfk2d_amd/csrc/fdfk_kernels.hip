@@ -16,28 +16,25 @@
 // through LDS.  It reads u^n, u^{n-1}, dt^2/rho, lambda, mu once and writes
 // u^{n+1} in place of u^{n-1}: 36 algorithmic bytes per cell-update in FP32.
 //
-// k_frame evaluates full-order cells with the same FP32 shared-flux
-// stencil as k_interior and the cells next to the padded-grid edges and the
-// free surface (reduced-order stencils, src/Submain.f90:192-195) with the
-// CPU oracle's exact arithmetic (FP64 accumulation rounded to FP32 after
-// every term, the Fortran REAL(4)/REAL(8) mixing); k_psi and k_record use
-// the exact arithmetic too.  The hybrid TF/SF correction enters as reads of
+// k_frame_full evaluates full-order frame cells with the same FP32
+// shared-flux stencil as k_interior; k_frame_edge evaluates the cells next to
+// the padded-grid edges and the free surface with the reduced-order stencils
+// of src/Submain.f90:192-195 (FP32, the operator structure of the literal
+// restatement).  k_record keeps the reference's FP64 interpolation weights
+// and accumulation exactly.  The hybrid TF/SF correction enters as reads of
 // the per-step ring field (k_ring_prep) instead of a second stencil pass.
 #include "fdfk_internal.h"
 
 namespace fdfk {
 
 // ---------------------------------------------------------------------
-// exact (oracle-arithmetic) pointwise operators
+// pointwise operators with the reduced-order rule of the reference
 // ---------------------------------------------------------------------
-#pragma clang fp contract(off)
-
 __device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b; }
-__device__ __forceinline__ float accd(float s, double c, float d)
-{
-    return (float)((double)s + c * (double)d);
-}
-__device__ __forceinline__ float hmean(float a, float b) { return 2.0f / (1.0f / a + 1.0f / b); }
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float accf(float s, float c, float d) { return s + c * d; }
+// harmonic mean 2/(1/a + 1/b); a zero modulus gives 0 (rcp(0) = inf)
+__device__ __forceinline__ float hmean(float a, float b) { return 2.0f * frcp(frcp(a) + frcp(b)); }
 __device__ __forceinline__ float matval(int mode, float lam, float mu)
 {
     return mode == 0 ? lam + 2.0f * mu : mu;
@@ -112,7 +109,7 @@ __device__ float cdx_sum(const Geo &g, const Coefs *cf, const R &f, int iz, int 
     const int no = g.norder;
     int nox = imin(imin(ix + g.npml - 1, g.nx + g.npml - ix), no);
     float s = 0.0f;
-    for (int k = 1; k <= nox; ++k) s = accd(s, cf->c1[(k - 1) + no * (nox - 1)], f(iz, ix + k) - f(iz, ix - k));
+    for (int k = 1; k <= nox; ++k) s = accf(s, cf->c1f[(k - 1) + no * (nox - 1)], f(iz, ix + k) - f(iz, ix - k));
     return s;
 }
 template <class R>
@@ -121,7 +118,7 @@ __device__ float cdz_sum(const Geo &g, const Coefs *cf, const R &f, int iz, int 
     const int no = g.norder;
     int noz = imin(imin(iz - 1, g.nz + g.npml - iz), no);
     float s = 0.0f;
-    for (int k = 1; k <= noz; ++k) s = accd(s, cf->c1[(k - 1) + no * (noz - 1)], f(iz + k, ix) - f(iz - k, ix));
+    for (int k = 1; k <= noz; ++k) s = accf(s, cf->c1f[(k - 1) + no * (noz - 1)], f(iz + k, ix) - f(iz - k, ix));
     return s;
 }
 
@@ -137,12 +134,12 @@ __device__ float x2_mode(const Geo &g, const Coefs *cf, const float *lam, const 
         int i = (2 * k - 1) / 2;
         int n1 = imin(imin(ix + i + npml - 1, nx + npml - ix - i), no);
         float s1 = 0.0f;
-        for (int j = 1; j <= n1; ++j) s1 = accd(s1, cf->c0[(j - 1) + no * (n1 - 1)], u(iz, ix + j + k - 1) - u(iz, ix - j + k));
+        for (int j = 1; j <= n1; ++j) s1 = accf(s1, cf->c0f[(j - 1) + no * (n1 - 1)], u(iz, ix + j + k - 1) - u(iz, ix - j + k));
         int n2 = imin(imin(ix - i + npml - 1, nx + npml - ix + i), no);
         float s2 = 0.0f;
-        for (int j = 1; j <= n2; ++j) s2 = accd(s2, cf->c0[(j - 1) + no * (n2 - 1)], u(iz, ix + j - k) - u(iz, ix - j - k + 1));
+        for (int j = 1; j <= n2; ++j) s2 = accf(s2, cf->c0f[(j - 1) + no * (n2 - 1)], u(iz, ix + j - k) - u(iz, ix - j - k + 1));
         float t = hmean(M(ix + k), M(ix + k - 1)) * s1 - hmean(M(ix - k + 1), M(ix - k)) * s2;
-        s = accd(s, cf->c0[(k - 1) + no * (nox - 1)], t);
+        s = accf(s, cf->c0f[(k - 1) + no * (nox - 1)], t);
     }
     bool in_reg = (iz >= 1 && iz <= g.nz && ix >= 1 && ix <= nx);
     float dx2 = in_reg ? (float)(1.0 / (double)(g.dx * g.dx)) : 1.0f / (g.dx * g.dx);
@@ -161,12 +158,12 @@ __device__ float z2_mode(const Geo &g, const Coefs *cf, const float *lam, const 
         int i = (2 * k - 1) / 2;
         int n1 = imin(imin(iz + i - izt, nz + npml - iz - i), no);
         float s1 = 0.0f;
-        for (int j = 1; j <= n1; ++j) s1 = accd(s1, cf->c0[(j - 1) + no * (n1 - 1)], u(iz + j + k - 1, ix) - u(iz - j + k, ix));
+        for (int j = 1; j <= n1; ++j) s1 = accf(s1, cf->c0f[(j - 1) + no * (n1 - 1)], u(iz + j + k - 1, ix) - u(iz - j + k, ix));
         int n2 = imin(imin(iz - i - izt, nz + npml - iz + i), no);
         float s2 = 0.0f;
-        for (int j = 1; j <= n2; ++j) s2 = accd(s2, cf->c0[(j - 1) + no * (n2 - 1)], u(iz + j - k, ix) - u(iz - j - k + 1, ix));
+        for (int j = 1; j <= n2; ++j) s2 = accf(s2, cf->c0f[(j - 1) + no * (n2 - 1)], u(iz + j - k, ix) - u(iz - j - k + 1, ix));
         float t = hmean(M(iz + k), M(iz + k - 1)) * s1 - hmean(M(iz - k + 1), M(iz - k)) * s2;
-        s = accd(s, cf->c0[(k - 1) + no * (noz - 1)], t);
+        s = accf(s, cf->c0f[(k - 1) + no * (noz - 1)], t);
     }
     return s * (1.0f / (g.dz * g.dz));
 }
@@ -181,7 +178,7 @@ __device__ float zx_pt(const Geo &g, const Coefs *cf, const float *mat, const R 
     for (int k = 1; k <= nox; ++k) {
         float vp = mat[cell(g, iz, ix + k)] * cdz_sum(g, cf, u, iz, ix + k) / g.dz;
         float vm = mat[cell(g, iz, ix - k)] * cdz_sum(g, cf, u, iz, ix - k) / g.dz;
-        s = accd(s, cf->c1[(k - 1) + no * (nox - 1)], vp - vm);
+        s = accf(s, cf->c1f[(k - 1) + no * (nox - 1)], vp - vm);
     }
     return s / g.dx;
 }
@@ -195,7 +192,7 @@ __device__ float xz_pt(const Geo &g, const Coefs *cf, const float *mat, const R 
     for (int k = 1; k <= noz; ++k) {
         float vp = mat[cell(g, iz + k, ix)] * cdx_sum(g, cf, u, iz + k, ix) / g.dx;
         float vm = mat[cell(g, iz - k, ix)] * cdx_sum(g, cf, u, iz - k, ix) / g.dx;
-        s = accd(s, cf->c1[(k - 1) + no * (noz - 1)], vp - vm);
+        s = accf(s, cf->c1f[(k - 1) + no * (noz - 1)], vp - vm);
     }
     return s / g.dz;
 }
@@ -218,9 +215,9 @@ __device__ void eval_parts(const Geo &g, const Coefs *cf, const float *lam, cons
             size_t a1 = cell(g, 1, ix + k), b1 = cell(g, 2, ix + k), a2 = cell(g, 1, ix - k), b2 = cell(g, 2, ix - k);
             float lp = 0.5f * (lam[a1] + lam[b1]), lm = 0.5f * (lam[a2] + lam[b2]);
             float mp = 0.5f * (mu[a1] + mu[b1]), mm = 0.5f * (mu[a2] + mu[b2]);
-            double c = cf->c1[(k - 1) + no * (nox - 1)];
-            sl = accd(sl, c, lp * (uz(2, ix + k) - uz(1, ix + k)) - lm * (uz(2, ix - k) - uz(1, ix - k)));
-            sm = accd(sm, c, mp * (ux(2, ix + k) - ux(1, ix + k)) - mm * (ux(2, ix - k) - ux(1, ix - k)));
+            float c = cf->c1f[(k - 1) + no * (nox - 1)];
+            sl = accf(sl, c, lp * (uz(2, ix + k) - uz(1, ix + k)) - lm * (uz(2, ix - k) - uz(1, ix - k)));
+            sm = accf(sm, c, mp * (ux(2, ix + k) - ux(1, ix + k)) - mm * (ux(2, ix - k) - ux(1, ix - k)));
         }
         float lbar = 0.5f * (lam[a] + lam[b]), mbar = 0.5f * (mu[a] + mu[b]);
         float dxuz = (cdx_sum(g, cf, uz, 1, ix) + cdx_sum(g, cf, uz, 2, ix)) / g.dx;
@@ -270,8 +267,6 @@ struct PsiRd {
         return b[strip_off(*g, s, iz, ix)];
     }
 };
-
-#pragma clang fp contract(on)
 
 // ---------------------------------------------------------------------
 // FP32 full-order pointwise stencil (frame cells away from the padded-grid
@@ -366,8 +361,6 @@ __device__ __forceinline__ void parts_full(const Geo &g, const ICoef<NO> &k, con
     p[0] = Xx; p[1] = Zx; p[2] = Mx; p[3] = Xz; p[4] = Zz; p[5] = Mz;
 }
 
-#pragma clang fp contract(off)
-
 // ---------------------------------------------------------------------
 // frame kernels: every updated cell outside the interior rectangle, split
 // into the full-order part (k_frame_full) and the reduced-order edge bands
@@ -415,16 +408,16 @@ __device__ __forceinline__ void frame_update(const StepArgs &a, int iz, int ix, 
         for (int kk = 1; kk <= nox; ++kk) {
             size_t ca = cell(g, iz, ix + kk), cb = cell(g, iz, ix - kk);
             float ma = a.lam[ca] + 2.0f * a.mu[ca], mb = a.lam[cb] + 2.0f * a.mu[cb];
-            double c = cf->c1[(kk - 1) + no * (nox - 1)];
-            sx0 = accd(sx0, c, ma * px0(iz, ix + kk) - mb * px0(iz, ix - kk));
-            sx1 = accd(sx1, c, a.mu[ca] * px1(iz, ix + kk) - a.mu[cb] * px1(iz, ix - kk));
+            float c = cf->c1f[(kk - 1) + no * (nox - 1)];
+            sx0 = accf(sx0, c, ma * px0(iz, ix + kk) - mb * px0(iz, ix - kk));
+            sx1 = accf(sx1, c, a.mu[ca] * px1(iz, ix + kk) - a.mu[cb] * px1(iz, ix - kk));
         }
         if (s == S_B)
             for (int kk = 1; kk <= noz; ++kk) {
                 size_t ca = cell(g, iz + kk, ix), cb = cell(g, iz - kk, ix);
-                double c = cf->c1[(kk - 1) + no * (noz - 1)];
-                sz0 = accd(sz0, c, a.mu[ca] * pz0(iz + kk, ix) - a.mu[cb] * pz0(iz - kk, ix));
-                sz1 = accd(sz1, c, (a.lam[ca] + 2.0f * a.mu[ca]) * pz1(iz + kk, ix) -
+                float c = cf->c1f[(kk - 1) + no * (noz - 1)];
+                sz0 = accf(sz0, c, a.mu[ca] * pz0(iz + kk, ix) - a.mu[cb] * pz0(iz - kk, ix));
+                sz1 = accf(sz1, c, (a.lam[ca] + 2.0f * a.mu[ca]) * pz1(iz + kk, ix) -
                                        (a.lam[cb] + 2.0f * a.mu[cb]) * pz1(iz - kk, ix));
             }
         p[0] = p[0] - sx0 / g.dx;
@@ -462,9 +455,9 @@ __device__ __forceinline__ void frame_update(const StepArgs &a, int iz, int ix, 
 }
 
 template <int NO>
-__global__ __launch_bounds__(256) void k_frame_full(StepArgs a, Rects r, ICoef<NO> k)
+__device__ __forceinline__ void frame_full_body(const StepArgs &a, const Rects &r, const ICoef<NO> &k, long bid)
 {
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long t = bid * blockDim.x + threadIdx.x;
     int iz, ix;
     if (!rect_cell(r, t, iz, ix)) return;
     CorrRd rx = frame_reader(a, a.ux, a.ringx, iz, ix), rz = frame_reader(a, a.uz, a.ringz, iz, ix);
@@ -472,10 +465,15 @@ __global__ __launch_bounds__(256) void k_frame_full(StepArgs a, Rects r, ICoef<N
     parts_full<NO>(a.g, k, a.lam, a.mu, rx, rz, iz, ix, p);
     frame_update(a, iz, ix, p);
 }
-
-__global__ __launch_bounds__(256) void k_frame_edge(StepArgs a, Rects r)
+template <int NO>
+__global__ __launch_bounds__(256) void k_frame_full(StepArgs a, Rects r, ICoef<NO> k)
 {
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    frame_full_body<NO>(a, r, k, blockIdx.x);
+}
+
+__device__ __forceinline__ void frame_edge_body(const StepArgs &a, const Rects &r, long bid)
+{
+    long t = bid * blockDim.x + threadIdx.x;
     int iz, ix;
     if (!rect_cell(r, t, iz, ix)) return;
     CorrRd rx = frame_reader(a, a.ux, a.ringx, iz, ix), rz = frame_reader(a, a.uz, a.ringz, iz, ix);
@@ -483,32 +481,52 @@ __global__ __launch_bounds__(256) void k_frame_edge(StepArgs a, Rects r)
     eval_parts(a.g, a.cf, a.lam, a.mu, rx, rz, iz, ix, p);
     frame_update(a, iz, ix, p);
 }
+__global__ __launch_bounds__(256) void k_frame_edge(StepArgs a, Rects r)
+{
+    frame_edge_body(a, r, blockIdx.x);
+}
 
 // background on the hybrid ring at iteration it, scattered into the
 // full-grid ring arrays (positions are fixed; values change every step)
+__device__ __forceinline__ void ring_prep_body(const Geo &g, const Background *bg, int it, float *ringx,
+                                               float *ringz, long bid)
+{
+    // one thread per ring cell, walking the six background arrays in their
+    // own (contiguous) order so the reads of step it coalesce
+    long o = bid * blockDim.x + threadIdx.x;
+    long n[6];
+    ring_sizes(g, n);
+    const int nx = g.nx, nz = g.nz, nbd = g.nbd;
+    int k = 0;
+    while (k < 6 && o >= n[k]) { o -= n[k]; ++k; }
+    if (k == 6) return;
+    int iz, ix;
+    switch (k) {
+    case 0: iz = 1 + (int)(o % (nz - nbd)); ix = 1 + (int)(o / (nz - nbd)); break;
+    case 1: iz = 1 + (int)(o % (nz - nbd)); ix = nx - nbd + 1 + (int)(o / (nz - nbd)); break;
+    case 2: iz = nz - nbd + 1 + (int)(o % nbd); ix = 1 + (int)(o / nbd); break;
+    case 3: iz = 1 + (int)(o % nz); ix = 1 - nbd + (int)(o / nz); break;
+    case 4: iz = 1 + (int)(o % nz); ix = nx + 1 + (int)(o / nz); break;
+    default: iz = nz + 1 + (int)(o % nbd); ix = 1 - nbd + (int)(o / nbd); break;
+    }
+    const int which = k / 3, arr = k % 3;
+    const long src = o + n[k] * (long)it;
+    const size_t c = cell(g, iz, ix);
+    ringx[c] = bg->a[6 * which + 2 * arr][src];
+    ringz[c] = bg->a[6 * which + 2 * arr + 1][src];
+}
 __global__ __launch_bounds__(256) void k_ring_prep(Geo g, const Background *bg, int it, float *ringx, float *ringz)
 {
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int nx = g.nx, nz = g.nz, nbd = g.nbd;
-    // enumerate the ring rectangle rows 1..nz+nbd, columns 1-nbd..nx+nbd
-    long W = nx + 2 * nbd, H = nz + nbd;
-    if (t >= W * H) return;
-    int ix = (1 - nbd) + (int)(t / H), iz = 1 + (int)(t % H);
-    int arr, which;
-    long o = bg_off(g, iz, ix, it, arr, which);
-    if (o < 0) return;
-    size_t c = cell(g, iz, ix);
-    ringx[c] = bg->a[6 * which + 2 * arr][o];
-    ringz[c] = bg->a[6 * which + 2 * arr + 1][o];
+    ring_prep_body(g, bg, it, ringx, ringz, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------
 // PML first-derivative memories (phase A of the step)
 // ---------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_psi(StepArgs a)
+__device__ __forceinline__ void psi_body(const StepArgs &a, long bid)
 {
     const Geo &g = a.g;
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long t = bid * blockDim.x + threadIdx.x;
     long nlr = (long)g.npml * g.nz;
     long nb = (long)g.NX * g.npml;
     int s, iz, ix;
@@ -537,12 +555,17 @@ __global__ __launch_bounds__(256) void k_psi(StepArgs a)
         p1[o] = b * p1[o] + c * gz;
     }
 }
+__global__ __launch_bounds__(256) void k_psi(StepArgs a)
+{
+    psi_body(a, blockIdx.x);
+}
 
 // ---------------------------------------------------------------------
 // receivers: 7x7 discretised-Dirac interpolation (Submain.f90:3-116)
 // ---------------------------------------------------------------------
 __device__ double dirac_w(double x)
 {
+#pragma clang fp contract(off)
     const double C1_0 = 1.0, C1_2 = -5.0 / 4.0, C1_3 = -35.0 / 12.0, C1_4 = 21.0 / 4.0, C1_5 = -25.0 / 12.0;
     const double C2_0 = -4.0, C2_1 = 75.0 / 4.0, C2_2 = -245.0 / 8.0, C2_3 = 545.0 / 24.0,
                  C2_4 = -63.0 / 8.0, C2_5 = 25.0 / 24.0;
@@ -556,32 +579,49 @@ __device__ double dirac_w(double x)
 }
 
 
-__global__ __launch_bounds__(64) void k_record(RecArgs r)
+// One wave per receiver: lane l < 49 forms the weight and loads the field
+// of stencil point l, then lane 0 accumulates the 49 products in the
+// reference's loop order (ix outer, iz inner) with FP64 products rounded to
+// FP32 after every term, so the sum is the reference's bit for bit.
+__device__ __forceinline__ void record_body(const RecArgs &r, long bid)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+#pragma clang fp contract(off)
+    __shared__ double wx[64], wz[64];
+    const int i = (int)bid;
+    const int l = threadIdx.x;
     if (i >= r.nrcv) return;
     const Geo &g = r.g;
     const float rx = r.rx[i], rz = r.rz[i], x0 = r.x0, z0 = r.z0, dx = g.dx, dz = g.dz;
-    int irx = (int)floorf((rx - x0) / dx) + 1;
-    int irz = (int)floorf((rz - z0) / dz) + 1;
-    float sx = 0.0f, sz = 0.0f;
-    int xa = irx - 3 > 1 - g.npml ? irx - 3 : 1 - g.npml;
-    int xb = irx + 3 < g.nx + g.npml ? irx + 3 : g.nx + g.npml;
-    int za = irz - 3 > 1 ? irz - 3 : 1;
-    int zb = irz + 3 < g.nz + g.npml ? irz + 3 : g.nz + g.npml;
-    for (int ix = xa; ix <= xb; ++ix) {
+    const int irx = (int)floorf((rx - x0) / dx) + 1;
+    const int irz = (int)floorf((rz - z0) / dz) + 1;
+    const int xa = irx - 3 > 1 - g.npml ? irx - 3 : 1 - g.npml;
+    const int xb = irx + 3 < g.nx + g.npml ? irx + 3 : g.nx + g.npml;
+    const int za = irz - 3 > 1 ? irz - 3 : 1;
+    const int zb = irz + 3 < g.nz + g.npml ? irz + 3 : g.nz + g.npml;
+    const int nzr = zb - za + 1, npt = (xb - xa + 1) * nzr;
+    if (l < npt) {
+        const int ix = xa + l / nzr, iz = za + l % nzr;
         double x = fabs((double)(x0 + (float)(ix - 1) * dx) - (double)rx) / (double)dx;
-        double bax = dirac_w(x);
-        for (int iz = za; iz <= zb; ++iz) {
-            double z = fabs((double)(z0 + (float)(iz - 1) * dz) - (double)rz) / (double)dz;
-            double ba = bax * dirac_w(z);
-            size_t c = cell(g, iz, ix);
-            sx = (float)((double)sx + ba * (double)r.ux[c]);
-            sz = (float)((double)sz + ba * (double)r.uz[c]);
-        }
+        double z = fabs((double)(z0 + (float)(iz - 1) * dz) - (double)rz) / (double)dz;
+        double ba = dirac_w(x) * dirac_w(z);
+        size_t c = cell(g, iz, ix);
+        wx[l] = ba * (double)r.ux[c];
+        wz[l] = ba * (double)r.uz[c];
     }
-    r.seisx[(size_t)i * r.nsamp + r.istp] = sx;
-    r.seisz[(size_t)i * r.nsamp + r.istp] = sz;
+    __syncthreads();
+    if (l == 0) {
+        float sx = 0.0f, sz = 0.0f;
+        for (int k = 0; k < npt; ++k) {
+            sx = (float)((double)sx + wx[k]);
+            sz = (float)((double)sz + wz[k]);
+        }
+        r.seisx[(size_t)i * r.nsamp + r.istp] = sx;
+        r.seisz[(size_t)i * r.nsamp + r.istp] = sz;
+    }
+}
+__global__ __launch_bounds__(64) void k_record(RecArgs r)
+{
+    record_body(r, blockIdx.x);
 }
 
 #pragma clang fp contract(on)
@@ -771,8 +811,6 @@ __global__ __launch_bounds__(NT) void k_interior(IntArgs<NO> a)
 // buffered LDS), loads issued one column ahead, outputs of column ix-1
 // written while column ix is staged, hardware reciprocals
 // ---------------------------------------------------------------------
-__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
-
 template <int NO, int NT>
 struct IntKernel2 {
     static constexpr int HZ = 2 * NO - 1;
@@ -904,7 +942,7 @@ struct IntKernel2 {
         }
     }
 
-    static __device__ void run(const IntArgs<NO> &a)
+    static __device__ void run(const IntArgs<NO> &a, int bid)
     {
         __shared__ Smem S;
         const Geo &g = a.g;
@@ -919,7 +957,7 @@ struct IntKernel2 {
         // block -> tile: consecutive blocks walk down z first so that the x
         // neighbours sharing warm-up columns run close in time
         const int nbz = a.nbz;
-        const int bzid = blockIdx.x % nbz, bxid = blockIdx.x / nbz;
+        const int bzid = bid % nbz, bxid = bid / nbz;
         const int z0 = g.iz0 + bzid * TZ;
         const int r = z0 - HZ + lane;
         const bool rowok = (r >= g.iz0 - HZ) && (r <= g.iz1 + HZ);
@@ -961,7 +999,7 @@ struct IntKernel2 {
 template <int NO, int NT>
 __global__ __launch_bounds__(NT) void k_interior2(IntArgs<NO> a)
 {
-    IntKernel2<NO, NT>::run(a);
+    IntKernel2<NO, NT>::run(a, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------

@@ -60,6 +60,15 @@ def lib():
         L.fdfk_set_state.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.fdfk_kernel_stats.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64),
                                         C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+        L.fdfk_batch_create.argtypes = [C.c_int32, C.POINTER(C.c_void_p)]
+        for name in ("fdfk_batch_add", "fdfk_batch_remove"):
+            getattr(L, name).argtypes = [C.c_void_p, C.c_void_p]
+        for name in ("fdfk_batch_run", "fdfk_batch_profile"):
+            getattr(L, name).argtypes = [C.c_void_p, C.c_int32]
+        for name in ("fdfk_batch_active", "fdfk_batch_sync", "fdfk_batch_destroy"):
+            getattr(L, name).argtypes = [C.c_void_p]
+        L.fdfk_batch_destroy.restype = None
+        L.fdfk_batch_kernel_stats.argtypes = L.fdfk_kernel_stats.argtypes
         L.updatedispl_.restype = None
         L.comput_background_wavefield_.restype = None
         L.fdfk_compat_release.restype = None
@@ -179,6 +188,49 @@ class Solver:
     @property
     def device_bytes(self):
         return lib().fdfk_device_bytes(self.h)
+
+
+class Batch:
+    """Resident solvers advanced together (fdfk_batch)."""
+
+    def __init__(self, norder=3):
+        h = C.c_void_p()
+        _check(lib().fdfk_batch_create(int(norder), C.byref(h)))
+        self.h = h
+        self.members = []
+
+    def add(self, solver):
+        _check(lib().fdfk_batch_add(self.h, solver.h))
+        self.members.append(solver)
+
+    def remove(self, solver):
+        _check(lib().fdfk_batch_remove(self.h, solver.h))
+        self.members.remove(solver)
+
+    def run(self, nsteps):
+        _check(lib().fdfk_batch_run(self.h, int(nsteps)))
+
+    @property
+    def active(self):
+        return lib().fdfk_batch_active(self.h)
+
+    def sync(self):
+        _check(lib().fdfk_batch_sync(self.h))
+
+    def profile(self, on=True):
+        _check(lib().fdfk_batch_profile(self.h, 1 if on else 0))
+
+    def kernel_stats(self):
+        a, b, c, d = C.c_double(), C.c_int64(), C.c_double(), C.c_int64()
+        _check(lib().fdfk_batch_kernel_stats(self.h, C.byref(a), C.byref(b), C.byref(c), C.byref(d)))
+        return dict(interior_ms=a.value, interior_n=b.value, step_ms=c.value, step_n=d.value)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fdfk_batch_destroy(self.h)
+            self.h = None
+
+    __del__ = close
 
 
 def _i(v):

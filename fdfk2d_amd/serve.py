@@ -1,0 +1,197 @@
+"""Continuous-batching request server for FDFK2D simulations on one GPU.
+
+Requests arrive over time (a Poisson stream); each is admitted as soon as
+its HBM footprint fits the budget, gets its own resident state (solver
+handle, FK background computed on the GPU), joins the running batch, and
+advances with every other admitted job in the same batched launches.  A job
+leaves the batch when its last iteration is done; its seismograms are read
+back and its memory freed, making room for waiting requests.  Latency is
+measured from arrival to seismograms on the host.
+
+The plain path for comparison serves the same requests one at a time with
+the single-job API (fdfk_run).
+"""
+import collections
+import math
+import time
+
+import numpy as np
+
+from . import configs, geometry, refstate
+from .lib import Batch, Solver
+
+
+# ---------------------------------------------------------------------------
+# request mix
+# ---------------------------------------------------------------------------
+def make_model(nx, nz, npml, dx, seed, perturb=True):
+    """3-layer model with a smoothed 5% random perturbation (the large-solve
+    recipe) on an (NX, NZ) padded grid."""
+    NX, NZ = nx + 2 * npml, nz + npml
+    vsl = [(0.0, 1500.0), (2000.0, 3600.0), (45000.0, 4500.0)]
+    z = np.arange(NZ) * dx
+    vs1 = np.zeros(NZ, np.float32)
+    for zt, v in vsl:
+        vs1[z >= zt] = v
+    vs = np.repeat(vs1[None, :], NX, 0)
+    if perturb:
+        d = configs.smooth_noise((NX, NZ), 5000.0 / dx, seed)
+        vs = (vs * (1.0 + 0.05 * d)).astype(np.float32)
+    vp = (np.float32(1.73) * vs).astype(np.float32)
+    layers = [(zt, 1.73 * v, v) for zt, v in vsl]
+    return vp, vs, configs.fk_layers(layers)
+
+
+def request_mix(n, seed=5, rate=4.0, nx_range=(1000, 8000), nz_range=(400, 2000),
+                steps_range=(3000, 20000), dx_choices=(50.0, 100.0, 200.0), p_range=(0.04, 0.08),
+                npml=20, perturb=True):
+    """Seeded requests: arrival times (Poisson), log-uniform grid sizes,
+    uniform step counts, random P/S and ray parameter."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, n))
+    reqs = []
+    for i in range(n):
+        f = rng.uniform()
+        nx = int(round(math.exp(math.log(nx_range[0]) + f * math.log(nx_range[1] / nx_range[0]))))
+        nz = int(round(math.exp(math.log(nz_range[0]) + f * math.log(nz_range[1] / nz_range[0]))))
+        dx = float(rng.choice(dx_choices))
+        steps = int(rng.integers(steps_range[0], steps_range[1] + 1))
+        reqs.append(dict(id=i, arrival=float(t[i]), nx=nx, nz=nz, dx=dx, nsteps=steps,
+                         src_is_p=bool(rng.integers(0, 2)), p_skm=float(rng.uniform(*p_range)),
+                         seed=int(rng.integers(1 << 30)), npml=npml, perturb=perturb))
+    return reqs
+
+
+def build_job(req):
+    """Materials, FK model, receivers and dt for one request (dt = 0.9 of the
+    reference's stability limit, src/Submain.f90:139-146)."""
+    nx, nz, dx, npml = req["nx"], req["nz"], req["dx"], req["npml"]
+    vp, vs, fk = make_model(nx, nz, npml, dx, req["seed"], req.get("perturb", True))
+    c = refstate.material_law(vp, vs)
+    dt = 0.9 * refstate.stable_dt(3, dx, dx, c)
+    nrcv = max(2, min(401, nx // 20))
+    rx = np.linspace(0.0, (nx - 1) * dx, nrcv).astype(np.float32)
+    bazs = geometry.column_bazs(nx, npml, 0.0, dx, configs.LAT_ORG, configs.LON_ORG, configs.AZ_ORG,
+                                configs.EVLA, configs.EVLO)
+    return dict(c=c, fk=fk, dt=dt, rx=rx, rz=np.zeros_like(rx), bazs=bazs, nt=req["nsteps"] - 1,
+                f0=0.5, ds=1e-3, rayp=req["p_skm"] / 1000.0)
+
+
+def footprint_bytes(req):
+    """(resident, transient) device bytes of a request: fields, materials,
+    ring, PML and the background for all steps stay resident; the FK work
+    arrays (spectra and traces, FP64) exist only while the background is
+    computed.  dt is estimated at 0.9 of the stability limit for Vp <= 9 km/s."""
+    nx, nz, npml = req["nx"], req["nz"], req["npml"]
+    nbd = 5
+    NX, NZ = nx + 2 * npml, nz + npml
+    ld = (NZ + 31) // 32 * 32
+    nt = req["nsteps"] - 1
+    fields = 9 * NX * ld * 4 + 2 * NX * NZ * 4
+    strips = (2 * npml * nz + NX * npml) * 4 * 15
+    ring_pts = 2 * (nz - nbd) * nbd + nbd * nx + 2 * nz * nbd + nbd * (nx + 2 * nbd)
+    bg = 2 * ring_pts * (nt + 1) * 4
+    dt = 0.9 * 0.5752 * req["dx"] / 9000.0
+    f0 = 0.5
+    tneed = nt * dt + req["p_skm"] / 1000.0 * (nx + 2 * nbd) * req["dx"] + 3.0 / f0 + 10.0 / f0
+    nfft = 1 << int(math.ceil(math.log2(2.0 * tneed / (dt / 2.0))))
+    fk_work = 2 * 2 * (nz + nbd) * nfft * 8 * 2
+    return fields + strips + bg, fk_work
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+# ---------------------------------------------------------------------------
+# serving
+# ---------------------------------------------------------------------------
+class Server:
+    def __init__(self, hbm_budget_gb=200.0, chunk=100, log=None):
+        self.budget = hbm_budget_gb * 1e9
+        self.chunk = chunk
+        self.batch = Batch(3)
+        self.used = 0.0
+        self.active = {}
+        self.log = log or (lambda *a: None)
+
+    def _admit(self, req):
+        job = build_job(req)
+        s = Solver(req["nx"], req["nz"], req["npml"], 3, req["dx"], req["dx"], job["dt"], job["nt"],
+                   job["c"], rx=job["rx"], rz=job["rz"])
+        s.compute_background(job["fk"], job["fk"], req["src_is_p"], job["f0"], job["ds"], job["rayp"],
+                             job["bazs"])
+        self.batch.add(s)
+        self.active[req["id"]] = (req, s)
+
+    def run(self, reqs, keep_results=False):
+        """Serve reqs (arrival times relative to the start); returns per
+        request latency and the total wall time."""
+        pending = collections.deque(sorted(reqs, key=lambda r: r["arrival"]))
+        waiting = collections.deque()
+        lat, results = {}, {}
+        t0 = time.perf_counter()
+        while pending or waiting or self.active:
+            now = time.perf_counter() - t0
+            while pending and pending[0]["arrival"] <= now:
+                waiting.append(pending.popleft())
+            while waiting:
+                req = waiting[0]
+                res, work = footprint_bytes(req)
+                if self.used + res + work > self.budget and self.active:
+                    break
+                waiting.popleft()
+                self._admit(req)
+                self.used += res
+            if not self.active:
+                if pending:
+                    time.sleep(max(0.0, min(0.05, pending[0]["arrival"] - now)))
+                continue
+            self.batch.run(self.chunk)
+            if self.batch.active < len(self.active):
+                self.batch.sync()
+                for rid, (req, s) in list(self.active.items()):
+                    if s.iteration > s.nt:
+                        sx, sz = s.seismograms()
+                        self.batch.remove(s)
+                        s.close()
+                        del self.active[rid]
+                        self.used -= footprint_bytes(req)[0]
+                        lat[rid] = time.perf_counter() - t0 - req["arrival"]
+                        if keep_results:
+                            results[rid] = (sx, sz)
+        total = time.perf_counter() - t0
+        return lat, total, results
+
+    def close(self):
+        self.batch.close()
+
+
+def serve_plain(reqs, keep_results=False):
+    """One request at a time with the single-job API, FIFO by arrival."""
+    lat, results = {}, {}
+    t0 = time.perf_counter()
+    for req in sorted(reqs, key=lambda r: r["arrival"]):
+        now = time.perf_counter() - t0
+        if req["arrival"] > now:
+            time.sleep(req["arrival"] - now)
+        job = build_job(req)
+        s = Solver(req["nx"], req["nz"], req["npml"], 3, req["dx"], req["dx"], job["dt"], job["nt"],
+                   job["c"], rx=job["rx"], rz=job["rz"])
+        s.compute_background(job["fk"], job["fk"], req["src_is_p"], job["f0"], job["ds"], job["rayp"],
+                             job["bazs"])
+        s.run(job["nt"] + 1)
+        sx, sz = s.seismograms()
+        s.close()
+        lat[req["id"]] = time.perf_counter() - t0 - req["arrival"]
+        if keep_results:
+            results[req["id"]] = (sx, sz)
+    return lat, time.perf_counter() - t0, results
+
+
+def summarize(lat, total, n):
+    v = np.array(sorted(lat.values()))
+    return dict(requests=n, wall_s=round(total, 3), solves_per_hour=round(n / total * 3600.0, 1),
+                p50_s=round(float(np.percentile(v, 50)), 4), p99_s=round(float(np.percentile(v, 99)), 4))

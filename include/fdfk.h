@@ -98,6 +98,26 @@ int64_t fdfk_device_bytes(const fdfk_solver *s);
 void fdfk_destroy(fdfk_solver *s);
 const char *fdfk_last_error(void);
 
+/* --- batched execution: many resident solvers advanced together with one
+ * launch per kernel type per step (tile work-list over jobs).  Members keep
+ * their own device state; they must share the FD order, and while they are
+ * in a batch only fdfk_batch_run may advance them.  No counterpart in the
+ * reference (one simulation per process there). --- */
+typedef struct fdfk_batch fdfk_batch;
+int fdfk_batch_create(int32_t norder, fdfk_batch **out);
+/* waits for the solver's own queued work (e.g. its background) first */
+int fdfk_batch_add(fdfk_batch *b, fdfk_solver *s);
+int fdfk_batch_remove(fdfk_batch *b, fdfk_solver *s);
+/* nsteps batched steps; members past their last iteration are skipped */
+int fdfk_batch_run(fdfk_batch *b, int32_t nsteps);
+/* number of members with iterations left */
+int fdfk_batch_active(const fdfk_batch *b);
+int fdfk_batch_sync(fdfk_batch *b);
+int fdfk_batch_profile(fdfk_batch *b, int32_t enable);
+int fdfk_batch_kernel_stats(fdfk_batch *b, double *interior_ms_total, int64_t *interior_launches,
+                            double *step_ms_total, int64_t *steps);
+void fdfk_batch_destroy(fdfk_batch *b);
+
 /* --- drop-in compatibility: the exact external procedures the reference
  * driver links (all arguments by reference, gfortran/flang mangling).  Device
  * mirrors are keyed by the host ux pointer; every call copies level 1/2 back

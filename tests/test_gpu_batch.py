@@ -1,0 +1,65 @@
+"""Batched execution and serving on the GPU: every job gives the same
+seismograms alone (single-job API) and in the mix (batched launches with
+other jobs of different sizes, joining at different times)."""
+import numpy as np
+import pytest
+
+from fdfk2d_amd import serve
+from fdfk2d_amd.lib import Batch, Solver
+
+pytestmark = pytest.mark.gpu
+
+
+def _reqs():
+    return [dict(id=0, arrival=0.0, nx=181, nz=61, dx=200.0, nsteps=240, src_is_p=True, p_skm=0.06, seed=1, npml=20),
+            dict(id=1, arrival=0.0, nx=97, nz=45, dx=100.0, nsteps=300, src_is_p=False, p_skm=0.05, seed=2, npml=20),
+            dict(id=2, arrival=0.0, nx=240, nz=80, dx=50.0, nsteps=200, src_is_p=True, p_skm=0.075, seed=3, npml=20)]
+
+
+def _solver(req):
+    job = serve.build_job(req)
+    s = Solver(req["nx"], req["nz"], req["npml"], 3, req["dx"], req["dx"], job["dt"], job["nt"], job["c"],
+               rx=job["rx"], rz=job["rz"])
+    s.compute_background(job["fk"], job["fk"], req["src_is_p"], job["f0"], job["ds"], job["rayp"], job["bazs"])
+    return s
+
+
+def test_batch_equals_alone_with_staggered_joins():
+    print("case: 3 jobs alone vs batched, staggered joins", flush=True)
+    reqs = _reqs()
+    alone = []
+    for r in reqs:
+        s = _solver(r)
+        s.run(s.nt + 1)
+        alone.append(s.seismograms())
+        s.close()
+    b = Batch(3)
+    sol = [_solver(r) for r in reqs]
+    b.add(sol[0])
+    b.run(37)
+    b.add(sol[1])
+    b.run(50)
+    b.add(sol[2])
+    while b.active:
+        b.run(25)
+    b.sync()
+    for s, (ax, az) in zip(sol, alone):
+        gx, gz = s.seismograms()
+        assert np.abs(az).max() > 0
+        assert np.array_equal(gx, ax) and np.array_equal(gz, az)
+        s.close()
+    b.close()
+
+
+def test_server_results_equal_plain_path():
+    print("case: serving 6 requests vs plain serial path", flush=True)
+    reqs = serve.request_mix(6, seed=7, rate=20.0, nx_range=(120, 400), nz_range=(50, 120),
+                             steps_range=(150, 400), dx_choices=(100.0, 200.0))
+    srv = serve.Server(hbm_budget_gb=50.0, chunk=40)
+    lat, total, res = srv.run(reqs, keep_results=True)
+    srv.close()
+    lat_p, total_p, res_p = serve.serve_plain(reqs, keep_results=True)
+    assert sorted(res) == sorted(res_p) == list(range(6))
+    for i in res:
+        assert np.array_equal(res[i][0], res_p[i][0]) and np.array_equal(res[i][1], res_p[i][1])
+    print("served", serve.summarize(lat, total, 6), "plain", serve.summarize(lat_p, total_p, 6), flush=True)

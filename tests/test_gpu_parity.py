@@ -1,8 +1,10 @@
 """GPU parity against the CPU oracle (same seeded inputs, same formulation).
 
-Tolerances: the boundary kernels reproduce the oracle's arithmetic exactly;
-the interior kernel evaluates the same stencil in FP32 with shared fluxes,
-so single steps agree to ~1e-6 relative and seismograms to rel-L2 <= 1e-5.
+Tolerances: the receiver interpolation reproduces the oracle's arithmetic
+exactly; the stencil kernels evaluate the same operators in FP32 (shared
+fluxes in the interior, hardware reciprocals in the harmonic means), so
+single steps agree to ~1e-6 (<= 1e-5 max-relative per array) and
+seismograms to rel-L2 <= 1e-5.
 """
 import numpy as np
 import pytest
@@ -45,11 +47,11 @@ def test_single_step_compat_matches_oracle(case, bg, warm_state):
     lib.updatedispl(sg, 180)
     lib.lib().fdfk_compat_release()
     for k in ("ux", "uz"):
-        assert _max_rel(sg[k], so[k]) < 2e-6, k
+        assert _max_rel(sg[k], so[k]) < 1e-5, k
     for i, (a, b) in enumerate(zip(sg["bu"][:18], so["bu"][:18])):
-        assert _max_rel(a, b) < 2e-6, ("bu", i)
+        assert _max_rel(a, b) < 1e-5, ("bu", i)
     for i in (0, 1, 2, 3, 8, 9, 10, 11):
-        assert _max_rel(sg["bp"][i], so["bp"][i]) < 2e-6, ("bp", i)
+        assert _max_rel(sg["bp"][i], so["bp"][i]) < 1e-5, ("bp", i)
 
 
 def test_seismograms_match_oracle(case, bg):
