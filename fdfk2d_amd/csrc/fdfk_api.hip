@@ -9,6 +9,7 @@
 #include "fdfk_internal.h"
 // single translation unit: the kernels are compiled with the host code
 #include "fdfk_kernels.hip"
+#include "fdfk_frame.hip"
 
 #include <algorithm>
 #include <cmath>
@@ -89,7 +90,8 @@ struct fdfk_solver {
     float *rx = nullptr, *rz = nullptr, *seisx = nullptr, *seisz = nullptr;
     float *stage = nullptr;   // (2, NZ, NX) staging for reference-layout copies
     // interior kernel
-    int lx = 90, nt_threads = 256, kernel_variant = 2;
+    int lx = 45, nt_threads = 256;
+    int flx_side = 8, flx_bottom = 8;   // k_frame_out columns per block (side strips / bottom strip)
     // profiling
     bool prof = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_int, ev_step;
@@ -222,7 +224,7 @@ ICoef<NO> make_icoef(const fdfk_solver *s)
 }
 
 template <int NO, int NT>
-void launch_interior_t(fdfk_solver *s)
+IntArgs<NO> interior_args(fdfk_solver *s)
 {
     IntArgs<NO> a{};
     const Geo &g = s->g;
@@ -231,15 +233,65 @@ void launch_interior_t(fdfk_solver *s)
     a.ux = s->ux[s->cur]; a.uz = s->uz[s->cur];
     a.uxp = s->ux[1 - s->cur]; a.uzp = s->uz[1 - s->cur];
     a.bf = s->bf; a.lam = s->lam; a.mu = s->mu;
+    a.ringx = s->ringx; a.ringz = s->ringz;
+    a.corr = s->bg_valid && s->it <= s->nt;
     constexpr int TZ = NT - 2 * (2 * NO - 1);
     int ncols = g.ix1 - g.ix0 + 1, nrows = g.iz1 - g.iz0 + 1;
     a.lx = s->lx;
     a.nbx = (ncols + a.lx - 1) / a.lx;
     a.nbz = (nrows + TZ - 1) / TZ;
-    if (s->kernel_variant == 1)
-        hipLaunchKernelGGL((k_interior<NO, NT>), dim3(a.nbx * a.nbz), dim3(NT), 0, s->stream, a);
-    else
-        hipLaunchKernelGGL((k_interior2<NO, NT>), dim3(a.nbx * a.nbz), dim3(NT), 0, s->stream, a);
+    return a;
+}
+
+template <int NO, int NT>
+void launch_interior_t(fdfk_solver *s)
+{
+    IntArgs<NO> a = interior_args<NO, NT>(s);
+    hipLaunchKernelGGL((k_interior2<NO, NT>), dim3(a.nbx * a.nbz), dim3(NT), 0, s->stream, a);
+}
+
+// Columns per interior block.  The stencil blocks all take about the same
+// time, so the kernel time is (number of resident rounds) x (columns per
+// block + warm-up columns).  Pick the width that minimises that for the
+// blocks one launch can keep resident (occupancy x CUs).
+template <int NO, int NT>
+int choose_lx(const Geo &g)
+{
+    int dev = 0, ncu = 0, bpc = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_interior2<NO, NT>, NT, 0) != hipSuccess ||
+        ncu <= 0 || bpc <= 0)
+        return 45;
+    constexpr int TZ = NT - 2 * (2 * NO - 1);
+    constexpr int WARM = 4 * NO - 2;
+    const long slots = (long)ncu * bpc;
+    const int ncols = g.ix1 - g.ix0 + 1, nrows = g.iz1 - g.iz0 + 1;
+    const long nbz = (nrows + TZ - 1) / TZ;
+    int best = 45;
+    long best_cost = -1;
+    for (int R = 1; R <= 16; ++R) {
+        long nbx = std::max(1L, R * slots / nbz);
+        int lx = (int)std::max(4L, (ncols + nbx - 1) / nbx);
+        long nb = ((ncols + lx - 1) / lx) * nbz;
+        long rounds = (nb + slots - 1) / slots;
+        long cost = rounds * (lx + WARM);
+        if (best_cost < 0 || cost < best_cost) { best_cost = cost; best = lx; }
+    }
+    return best;
+}
+
+int auto_lx(const fdfk_solver *s)
+{
+    switch (s->g.norder) {
+    case 1: return choose_lx<1, 256>(s->g);
+    case 2: return choose_lx<2, 256>(s->g);
+    case 3:
+        if (s->nt_threads == 128) return choose_lx<3, 128>(s->g);
+        if (s->nt_threads == 512) return choose_lx<3, 512>(s->g);
+        return choose_lx<3, 256>(s->g);
+    default: return choose_lx<4, 256>(s->g);
+    }
 }
 
 void launch_interior(fdfk_solver *s)
@@ -264,24 +316,78 @@ void add_rect(Rects &r, int x0, int x1, int z0, int z1)
     r.off[q + 1] = r.off[q] + (long)(x1 - x0 + 1) * (z1 - z0 + 1);
 }
 
-// full-order frame: the full-order rectangle minus the interior rectangle
-Rects full_rects(const Geo &g)
+// Full-order rectangle: columns xf0..xf1, rows iz0..zf1.  Its part inside
+// the total-field region is the interior kernel's; the rest (the PML strips
+// up to the reduced-order bands) is k_frame_out's.
+void full_order_box(const Geo &g, int &xf0, int &xf1, int &zf1)
 {
-    const int NO = g.norder;
-    int xf0 = 2 * NO - g.npml, xf1 = g.nx + g.npml + 1 - 2 * NO, zf1 = g.nz + g.npml + 1 - 2 * NO;
-    Rects r{};
-    add_rect(r, xf0, xf1, g.iz1 + 1, zf1);
-    add_rect(r, xf0, g.ix0 - 1, g.iz0, g.iz1);
-    add_rect(r, g.ix1 + 1, xf1, g.iz0, g.iz1);
-    return r;
+    xf0 = 2 * g.norder - g.npml;
+    xf1 = g.nx + g.npml + 1 - 2 * g.norder;
+    zf1 = g.nz + g.npml + 1 - 2 * g.norder;
+}
+
+void add_tile(FTiles &t, int x0, int x1, int z0, int z1, int lx, int TZ)
+{
+    if (x1 < x0 || z1 < z0) return;
+    int q = t.n++;
+    t.x0[q] = x0; t.x1[q] = x1; t.z0[q] = z0; t.z1[q] = z1; t.lx[q] = lx;
+    t.nbx[q] = (x1 - x0 + lx) / lx;
+    t.nbz[q] = (z1 - z0 + TZ) / TZ;
+    t.off[q + 1] = t.off[q] + t.nbx[q] * t.nbz[q];
+}
+
+FTiles out_tiles(const Geo &g, int flx_side, int flx_bottom)
+{
+    int xf0, xf1, zf1;
+    full_order_box(g, xf0, xf1, zf1);
+    const int TZ = 64 - 2 * (2 * g.norder - 1);
+    FTiles t{};
+    add_tile(t, xf0, 0, g.iz0, zf1, flx_side, TZ);              // left strip (+ bottom-left corner)
+    add_tile(t, g.nx + 1, xf1, g.iz0, zf1, flx_side, TZ);       // right strip (+ corner)
+    add_tile(t, 1, g.nx, g.nz + 1, zf1, flx_bottom, TZ);        // bottom strip under the model
+    return t;
+}
+
+template <int NO>
+FrameArgs<NO> frame_args(fdfk_solver *s)
+{
+    FrameArgs<NO> f{};
+    f.g = s->g;
+    f.k = make_icoef<NO>(s);
+    f.ux = s->ux[s->cur]; f.uz = s->uz[s->cur];
+    f.uxp = s->ux[1 - s->cur]; f.uzp = s->uz[1 - s->cur];
+    f.bf = s->bf; f.lam = s->lam; f.mu = s->mu;
+    f.st = s->d_strips[s->cur];
+    f.ringx = s->ringx; f.ringz = s->ringz;
+    f.corr = s->bg_valid && s->it <= s->nt;
+    f.t = out_tiles(s->g, s->flx_side, s->flx_bottom);
+    return f;
+}
+
+template <int NO>
+void launch_frame_out_t(fdfk_solver *s)
+{
+    FrameArgs<NO> f = frame_args<NO>(s);
+    if (f.t.n > 0)
+        hipLaunchKernelGGL(k_frame_out<NO>, dim3(f.t.off[f.t.n]), dim3(64), 0, s->stream, f);
+}
+
+void launch_frame_out(fdfk_solver *s)
+{
+    switch (s->g.norder) {
+    case 1: launch_frame_out_t<1>(s); break;
+    case 2: launch_frame_out_t<2>(s); break;
+    case 3: launch_frame_out_t<3>(s); break;
+    default: launch_frame_out_t<4>(s); break;
+    }
 }
 
 // reduced-order edge bands: top rows (free surface), bottom rows and the
 // columns next to the padded-grid boundary
 Rects edge_rects(const Geo &g)
 {
-    const int NO = g.norder;
-    int xf0 = 2 * NO - g.npml, xf1 = g.nx + g.npml + 1 - 2 * NO, zf1 = g.nz + g.npml + 1 - 2 * NO;
+    int xf0, xf1, zf1;
+    full_order_box(g, xf0, xf1, zf1);
     int xa = 2 - g.npml, xb = g.nx + g.npml - 1;
     Rects r{};
     add_rect(r, xa, xb, 1, g.iz0 - 1);
@@ -313,19 +419,27 @@ int one_step(fdfk_solver *s, bool record)
                            s->it, s->ringx, s->ringz);
     }
     long npsi = 2L * g.npml * g.nz + (long)g.NX * g.npml;
-    hipLaunchKernelGGL(k_psi, dim3((unsigned)((npsi + 255) / 256)), dim3(256), 0, s->stream, a);
+    dim3 pg((unsigned)((npsi + 255) / 256));
+    switch (g.norder) {
+    case 1: hipLaunchKernelGGL(k_psi<1>, pg, dim3(256), 0, s->stream, a); break;
+    case 2: hipLaunchKernelGGL(k_psi<2>, pg, dim3(256), 0, s->stream, a); break;
+    case 3: hipLaunchKernelGGL(k_psi<3>, pg, dim3(256), 0, s->stream, a); break;
+    default: hipLaunchKernelGGL(k_psi<4>, pg, dim3(256), 0, s->stream, a); break;
+    }
     if (s->prof) hipEventRecord(e1, s->stream);
     launch_interior(s);
     if (s->prof) hipEventRecord(e2, s->stream);
-    Rects rf = full_rects(g), re = edge_rects(g);
-    dim3 fg((unsigned)((rf.off[rf.n] + 255) / 256)), eg((unsigned)((re.off[re.n] + 255) / 256)), fb(256);
-    if (rf.off[rf.n] > 0) switch (g.norder) {
-    case 1: hipLaunchKernelGGL(k_frame_full<1>, fg, fb, 0, s->stream, a, rf, make_icoef<1>(s)); break;
-    case 2: hipLaunchKernelGGL(k_frame_full<2>, fg, fb, 0, s->stream, a, rf, make_icoef<2>(s)); break;
-    case 3: hipLaunchKernelGGL(k_frame_full<3>, fg, fb, 0, s->stream, a, rf, make_icoef<3>(s)); break;
-    default: hipLaunchKernelGGL(k_frame_full<4>, fg, fb, 0, s->stream, a, rf, make_icoef<4>(s)); break;
+    launch_frame_out(s);
+    Rects re = edge_rects(g);
+    if (re.off[re.n] > 0) {
+        dim3 eg((unsigned)((re.off[re.n] + kEdgeCells - 1) / kEdgeCells)), eb(kEdgeThreads);
+        switch (g.norder) {
+        case 1: hipLaunchKernelGGL(k_frame_edge<1>, eg, eb, 0, s->stream, a, re); break;
+        case 2: hipLaunchKernelGGL(k_frame_edge<2>, eg, eb, 0, s->stream, a, re); break;
+        case 3: hipLaunchKernelGGL(k_frame_edge<3>, eg, eb, 0, s->stream, a, re); break;
+        default: hipLaunchKernelGGL(k_frame_edge<4>, eg, eb, 0, s->stream, a, re); break;
+        }
     }
-    hipLaunchKernelGGL(k_frame_edge, eg, fb, 0, s->stream, a, re);
     if (s->prof) {
         hipEventRecord(e3, s->stream);
         s->ev_int.emplace_back(e1, e2);
@@ -374,17 +488,21 @@ int fdfk_create(const fdfk_job *job, fdfk_solver **out)
     g.NX = g.nx + 2 * g.npml; g.NZ = g.nz + g.npml;
     g.ld = (g.NZ + 31) / 32 * 32;
     g.dx = job->dx; g.dz = job->dz; g.dt = job->dt;
-    g.ix0 = g.nbd + 1; g.ix1 = g.nx - g.nbd;
-    g.iz0 = 2 * g.norder; g.iz1 = g.nz - g.nbd;
+    // interior kernel: the whole total-field region below the surface rows
+    // (full order there because npml >= 2*norder-1)
+    g.ix0 = 1; g.ix1 = g.nx;
+    g.iz0 = 2 * g.norder; g.iz1 = g.nz;
     s->nt = job->nt; s->nstp = job->nstp;
     s->nsis = job->nt / job->nstp;
     s->nsamp = s->nsis + 1;
     s->nrcv = job->nrcv;
     s->x0 = job->x0; s->z0 = job->z0;
     s->R0 = job->R0; s->npower = job->npower;
-    if (const char *e = std::getenv("FDFK_LX")) s->lx = std::max(9, std::atoi(e));
     if (const char *e = std::getenv("FDFK_NT")) s->nt_threads = std::atoi(e);
-    if (const char *e = std::getenv("FDFK_KERNEL")) s->kernel_variant = std::atoi(e);
+    s->lx = auto_lx(s);
+    if (const char *e = std::getenv("FDFK_LX")) s->lx = std::max(4, std::atoi(e));
+    if (const char *e = std::getenv("FDFK_FLX")) s->flx_side = std::max(1, std::atoi(e));
+    if (const char *e = std::getenv("FDFK_FLXB")) s->flx_bottom = std::max(1, std::atoi(e));
     const long ncell = (long)g.NX * g.NZ;
     // vpml = max Vp (src/FDFK2D.f90:416)
     float vpml = job->vpml;

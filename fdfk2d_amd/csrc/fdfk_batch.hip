@@ -14,14 +14,14 @@
 namespace fdfk {
 
 constexpr int kMaxBatch = 256;
-enum { KB_RING = 0, KB_PSI, KB_INT, KB_FFULL, KB_FEDGE, KB_REC, KB_N };
+enum { KB_RING = 0, KB_PSI, KB_INT, KB_FOUT, KB_FEDGE, KB_REC, KB_N };
 
 template <int NO>
 struct BatchJob {
     StepArgs sa;
     IntArgs<NO> ia;
-    ICoef<NO> k;
-    Rects rf, re;
+    FrameArgs<NO> fa;
+    Rects re;
     RecArgs ra;
     const Background *bg;
     float *ringx, *ringz;
@@ -54,7 +54,7 @@ template <int NO>
 __global__ __launch_bounds__(256) void kb_psi(const BatchJob<NO> *J, const BatchPrefix *P)
 {
     int j = find_job(P->pre[KB_PSI], P->n, blockIdx.x);
-    psi_body(J[j].sa, blockIdx.x - P->pre[KB_PSI][j]);
+    psi_body<NO>(J[j].sa, blockIdx.x - P->pre[KB_PSI][j]);
 }
 
 template <int NO, int NT>
@@ -65,18 +65,17 @@ __global__ __launch_bounds__(NT) void kb_interior(const BatchJob<NO> *J, const B
 }
 
 template <int NO>
-__global__ __launch_bounds__(256) void kb_frame_full(const BatchJob<NO> *J, const BatchPrefix *P)
+__global__ __launch_bounds__(64) void kb_frame_out(const BatchJob<NO> *J, const BatchPrefix *P)
 {
-    int j = find_job(P->pre[KB_FFULL], P->n, blockIdx.x);
-    const BatchJob<NO> &b = J[j];
-    frame_full_body<NO>(b.sa, b.rf, b.k, blockIdx.x - P->pre[KB_FFULL][j]);
+    int j = find_job(P->pre[KB_FOUT], P->n, blockIdx.x);
+    OutKernel<NO, 64>::run(J[j].fa, blockIdx.x - P->pre[KB_FOUT][j]);
 }
 
 template <int NO>
-__global__ __launch_bounds__(256) void kb_frame_edge(const BatchJob<NO> *J, const BatchPrefix *P)
+__global__ __launch_bounds__(kEdgeThreads) void kb_frame_edge(const BatchJob<NO> *J, const BatchPrefix *P)
 {
     int j = find_job(P->pre[KB_FEDGE], P->n, blockIdx.x);
-    frame_edge_body(J[j].sa, J[j].re, blockIdx.x - P->pre[KB_FEDGE][j]);
+    frame_edge_body<NO>(J[j].sa, J[j].re, blockIdx.x - P->pre[KB_FEDGE][j]);
 }
 
 template <int NO>
@@ -131,19 +130,8 @@ void fill_job(fdfk_solver *s, BatchJob<NO> &b, int nb[KB_N], bool &record)
     b.bg = s->d_bg;
     b.ringx = s->ringx;
     b.ringz = s->ringz;
-    b.k = make_icoef<NO>(s);
-    IntArgs<NO> &a = b.ia;
-    a = IntArgs<NO>{};
-    a.g = g;
-    a.k = b.k;
-    a.ux = s->ux[s->cur]; a.uz = s->uz[s->cur];
-    a.uxp = s->ux[1 - s->cur]; a.uzp = s->uz[1 - s->cur];
-    a.bf = s->bf; a.lam = s->lam; a.mu = s->mu;
-    constexpr int TZ = 256 - 2 * (2 * NO - 1);
-    a.lx = s->lx;
-    a.nbx = (g.ix1 - g.ix0 + 1 + a.lx - 1) / a.lx;
-    a.nbz = (g.iz1 - g.iz0 + 1 + TZ - 1) / TZ;
-    b.rf = full_rects(g);
+    b.ia = interior_args<NO, 256>(s);
+    b.fa = frame_args<NO>(s);
     b.re = edge_rects(g);
     record = (s->isis == s->it) && (s->istp <= s->nsis) && s->nrcv > 0;
     RecArgs &r = b.ra;
@@ -155,9 +143,9 @@ void fill_job(fdfk_solver *s, BatchJob<NO> &b, int nb[KB_N], bool &record)
     long npsi = 2L * g.npml * g.nz + (long)g.NX * g.npml;
     nb[KB_RING] = b.sa.ring_valid ? (int)((nring + 255) / 256) : 0;
     nb[KB_PSI] = (int)((npsi + 255) / 256);
-    nb[KB_INT] = a.nbx * a.nbz;
-    nb[KB_FFULL] = (int)((b.rf.off[b.rf.n] + 255) / 256);
-    nb[KB_FEDGE] = (int)((b.re.off[b.re.n] + 255) / 256);
+    nb[KB_INT] = b.ia.nbx * b.ia.nbz;
+    nb[KB_FOUT] = b.fa.t.off[b.fa.t.n];
+    nb[KB_FEDGE] = (int)((b.re.off[b.re.n] + kEdgeCells - 1) / kEdgeCells);
     nb[KB_REC] = record ? s->nrcv : 0;
 }
 
@@ -199,8 +187,8 @@ int batch_step(fdfk_batch *B)
     if (B->prof) hipEventRecord(e1, B->stream);
     if (P->pre[KB_INT][n]) hipLaunchKernelGGL((kb_interior<NO, 256>), dim3(P->pre[KB_INT][n]), dim3(256), 0, B->stream, dJ, dP);
     if (B->prof) hipEventRecord(e2, B->stream);
-    if (P->pre[KB_FFULL][n]) hipLaunchKernelGGL(kb_frame_full<NO>, dim3(P->pre[KB_FFULL][n]), dim3(256), 0, B->stream, dJ, dP);
-    if (P->pre[KB_FEDGE][n]) hipLaunchKernelGGL(kb_frame_edge<NO>, dim3(P->pre[KB_FEDGE][n]), dim3(256), 0, B->stream, dJ, dP);
+    if (P->pre[KB_FOUT][n]) hipLaunchKernelGGL(kb_frame_out<NO>, dim3(P->pre[KB_FOUT][n]), dim3(64), 0, B->stream, dJ, dP);
+    if (P->pre[KB_FEDGE][n]) hipLaunchKernelGGL(kb_frame_edge<NO>, dim3(P->pre[KB_FEDGE][n]), dim3(kEdgeThreads), 0, B->stream, dJ, dP);
     if (B->prof) {
         hipEventRecord(e3, B->stream);
         B->ev_int.emplace_back(e1, e2);

@@ -109,8 +109,9 @@ struct StepArgs {
     int it;
 };
 
-// frame-kernel work decomposition: up to four rectangles of cells
-// (columns x0..x1, rows z0..z1), enumerated z fastest; off[] are prefix counts
+// edge-kernel work decomposition: up to four rectangles of cells (columns
+// x0..x1, rows z0..z1), enumerated z fastest (coalesced along the columns,
+// also for the thin row bands); off[] are prefix counts
 struct Rects {
     int n;
     int x0[4], x1[4], z0[4], z1[4];
@@ -133,9 +134,33 @@ struct IntArgs {
     const float *ux, *uz;
     float *uxp, *uzp;
     const float *bf, *lam, *mu;
+    const float *ringx, *ringz;  // background on the hybrid ring (total-field view of the padding)
+    int corr;                    // ring values valid at this step
     int lx;   // output columns per block
     int nbx;  // blocks along x
     int nbz;  // blocks along z
+};
+
+// Tiles of the full-order padding cells (PML strips) for k_frame_out: up to
+// three rectangles (left strip, right strip, bottom strip), each cut into
+// nbx x nbz blocks of lx columns and TZ rows; off[] are block prefix counts.
+struct FTiles {
+    int n;
+    int x0[3], x1[3], z0[3], z1[3], lx[3], nbx[3], nbz[3];
+    int off[4];
+};
+
+template <int NO>
+struct FrameArgs {
+    Geo g;
+    ICoef<NO> k;
+    const float *ux, *uz;
+    float *uxp, *uzp;
+    const float *bf, *lam, *mu;
+    const Strips *st;
+    const float *ringx, *ringz;
+    int corr;
+    FTiles t;
 };
 
 }  // namespace fdfk

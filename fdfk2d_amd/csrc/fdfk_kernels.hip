@@ -40,10 +40,31 @@ __device__ __forceinline__ float matval(int mode, float lam, float mu)
     return mode == 0 ? lam + 2.0f * mu : mu;
 }
 
+// The pointwise operators below run their stencil loops to the full order
+// NO (inner loops unrolled, so the loads of one outer term are issued
+// together) and give the terms beyond the reduced order of the cell a zero
+// coefficient.  Those
+// terms may address cells beyond the padded grid: addresses are clamped to
+// the grid (the values are finite and multiplied by zero).
+__device__ __forceinline__ size_t cellc(const Geo &g, int iz, int ix)
+{
+    ix = ix < 1 - g.npml ? 1 - g.npml : (ix > g.nx + g.npml ? g.nx + g.npml : ix);
+    iz = iz < 1 ? 1 : (iz > g.NZ ? g.NZ : iz);
+    return cell(g, iz, ix);
+}
+// coefficient j of order n from a (NO x NO) table; 0 for j > n (and n < 1)
+template <int NO>
+__device__ __forceinline__ float coefz(const float *tab, int j, int n)
+{
+    const int nn = n < 1 ? 1 : n;
+    const float c = tab[(j - 1) + NO * (nn - 1)];
+    return j <= n ? c : 0.0f;
+}
+
 struct FieldRd {
     const float *p;
     const Geo *g;
-    __device__ float operator()(int iz, int ix) const { return p[cell(*g, iz, ix)]; }
+    __device__ float operator()(int iz, int ix) const { return p[cellc(*g, iz, ix)]; }
 };
 
 // background cell offset in the twelve reference arrays (FDFK2D.f90:330-355)
@@ -96,146 +117,169 @@ struct CorrRd {
     int sigma;
     __device__ float operator()(int iz, int ix) const
     {
-        size_t c = cell(*g, iz, ix);
+        size_t c = cellc(*g, iz, ix);
         float v = p[c];
         if (sigma >= 0 && tf_side(*g, iz, ix) != sigma) v = sigma == 0 ? v + ring[c] : v - ring[c];
         return v;
     }
 };
 
-template <class R>
-__device__ float cdx_sum(const Geo &g, const Coefs *cf, const R &f, int iz, int ix)
+template <int NO, class R>
+__device__ __forceinline__ float cdx_sum(const Geo &g, const Coefs *cf, const R &f, int iz, int ix)
 {
-    const int no = g.norder;
-    int nox = imin(imin(ix + g.npml - 1, g.nx + g.npml - ix), no);
+    const int nox = imin(imin(ix + g.npml - 1, g.nx + g.npml - ix), NO);
     float s = 0.0f;
-    for (int k = 1; k <= nox; ++k) s = accf(s, cf->c1f[(k - 1) + no * (nox - 1)], f(iz, ix + k) - f(iz, ix - k));
+#pragma unroll
+    for (int k = 1; k <= NO; ++k) s = accf(s, coefz<NO>(cf->c1f, k, nox), f(iz, ix + k) - f(iz, ix - k));
     return s;
 }
-template <class R>
-__device__ float cdz_sum(const Geo &g, const Coefs *cf, const R &f, int iz, int ix)
+template <int NO, class R>
+__device__ __forceinline__ float cdz_sum(const Geo &g, const Coefs *cf, const R &f, int iz, int ix)
 {
-    const int no = g.norder;
-    int noz = imin(imin(iz - 1, g.nz + g.npml - iz), no);
+    const int noz = imin(imin(iz - 1, g.nz + g.npml - iz), NO);
     float s = 0.0f;
-    for (int k = 1; k <= noz; ++k) s = accf(s, cf->c1f[(k - 1) + no * (noz - 1)], f(iz + k, ix) - f(iz - k, ix));
+#pragma unroll
+    for (int k = 1; k <= NO; ++k) s = accf(s, coefz<NO>(cf->c1f, k, noz), f(iz + k, ix) - f(iz - k, ix));
     return s;
 }
 
-template <class R>
-__device__ float x2_mode(const Geo &g, const Coefs *cf, const float *lam, const float *mu, int mode,
-                         const R &u, int iz, int ix)
+// D2 staggered composition along x with harmonic means of M (mode 0:
+// lambda+2mu, 1: mu) and the order reduction of diffx2_reg/diffx2_pml
+template <int NO, class R>
+__device__ __forceinline__ float x2_mode(const Geo &g, const Coefs *cf, const float *lam, const float *mu, int mode,
+                                         const R &u, int iz, int ix)
 {
-    const int no = g.norder, nx = g.nx, npml = g.npml;
-    int nox = imin(imin(ix + npml - 1, nx + npml - ix), no);
+    const int nx = g.nx, npml = g.npml;
+    const int nox = imin(imin(ix + npml - 1, nx + npml - ix), NO);
     float s = 0.0f;
-    auto M = [&](int jx) { size_t c = cell(g, iz, jx); return matval(mode, lam[c], mu[c]); };
-    for (int k = 1; k <= nox; ++k) {
-        int i = (2 * k - 1) / 2;
-        int n1 = imin(imin(ix + i + npml - 1, nx + npml - ix - i), no);
-        float s1 = 0.0f;
-        for (int j = 1; j <= n1; ++j) s1 = accf(s1, cf->c0f[(j - 1) + no * (n1 - 1)], u(iz, ix + j + k - 1) - u(iz, ix - j + k));
-        int n2 = imin(imin(ix - i + npml - 1, nx + npml - ix + i), no);
-        float s2 = 0.0f;
-        for (int j = 1; j <= n2; ++j) s2 = accf(s2, cf->c0f[(j - 1) + no * (n2 - 1)], u(iz, ix + j - k) - u(iz, ix - j - k + 1));
+    auto M = [&](int jx) { size_t c = cellc(g, iz, jx); return matval(mode, lam[c], mu[c]); };
+#pragma unroll 1
+    for (int k = 1; k <= NO; ++k) {
+        const int i = k - 1;  // (2k-1)/2
+        const int n1 = imin(imin(ix + i + npml - 1, nx + npml - ix - i), NO);
+        const int n2 = imin(imin(ix - i + npml - 1, nx + npml - ix + i), NO);
+        float s1 = 0.0f, s2 = 0.0f;
+#pragma unroll
+        for (int j = 1; j <= NO; ++j) {
+            s1 = accf(s1, coefz<NO>(cf->c0f, j, n1), u(iz, ix + j + k - 1) - u(iz, ix - j + k));
+            s2 = accf(s2, coefz<NO>(cf->c0f, j, n2), u(iz, ix + j - k) - u(iz, ix - j - k + 1));
+        }
         float t = hmean(M(ix + k), M(ix + k - 1)) * s1 - hmean(M(ix - k + 1), M(ix - k)) * s2;
-        s = accf(s, cf->c0f[(k - 1) + no * (nox - 1)], t);
+        s = accf(s, coefz<NO>(cf->c0f, k, nox), t);
     }
-    bool in_reg = (iz >= 1 && iz <= g.nz && ix >= 1 && ix <= nx);
-    float dx2 = in_reg ? (float)(1.0 / (double)(g.dx * g.dx)) : 1.0f / (g.dx * g.dx);
+    const bool in_reg = (iz >= 1 && iz <= g.nz && ix >= 1 && ix <= nx);
+    const float dx2 = in_reg ? (float)(1.0 / (double)(g.dx * g.dx)) : 1.0f / (g.dx * g.dx);
     return s * dx2;
 }
 
-template <class R>
-__device__ float z2_mode(const Geo &g, const Coefs *cf, const float *lam, const float *mu, int mode,
-                         const R &u, int iz, int ix)
+template <int NO, class R>
+__device__ __forceinline__ float z2_mode(const Geo &g, const Coefs *cf, const float *lam, const float *mu, int mode,
+                                         const R &u, int iz, int ix)
 {
-    const int no = g.norder, nz = g.nz, npml = g.npml, izt = 1;
-    int noz = imin(imin(iz - izt, nz + npml - iz), no);
+    const int nz = g.nz, npml = g.npml, izt = 1;
+    const int noz = imin(imin(iz - izt, nz + npml - iz), NO);
     float s = 0.0f;
-    auto M = [&](int jz) { size_t c = cell(g, jz, ix); return matval(mode, lam[c], mu[c]); };
-    for (int k = 1; k <= noz; ++k) {
-        int i = (2 * k - 1) / 2;
-        int n1 = imin(imin(iz + i - izt, nz + npml - iz - i), no);
-        float s1 = 0.0f;
-        for (int j = 1; j <= n1; ++j) s1 = accf(s1, cf->c0f[(j - 1) + no * (n1 - 1)], u(iz + j + k - 1, ix) - u(iz - j + k, ix));
-        int n2 = imin(imin(iz - i - izt, nz + npml - iz + i), no);
-        float s2 = 0.0f;
-        for (int j = 1; j <= n2; ++j) s2 = accf(s2, cf->c0f[(j - 1) + no * (n2 - 1)], u(iz + j - k, ix) - u(iz - j - k + 1, ix));
+    auto M = [&](int jz) { size_t c = cellc(g, jz, ix); return matval(mode, lam[c], mu[c]); };
+#pragma unroll 1
+    for (int k = 1; k <= NO; ++k) {
+        const int i = k - 1;
+        const int n1 = imin(imin(iz + i - izt, nz + npml - iz - i), NO);
+        const int n2 = imin(imin(iz - i - izt, nz + npml - iz + i), NO);
+        float s1 = 0.0f, s2 = 0.0f;
+#pragma unroll
+        for (int j = 1; j <= NO; ++j) {
+            s1 = accf(s1, coefz<NO>(cf->c0f, j, n1), u(iz + j + k - 1, ix) - u(iz - j + k, ix));
+            s2 = accf(s2, coefz<NO>(cf->c0f, j, n2), u(iz + j - k, ix) - u(iz - j - k + 1, ix));
+        }
         float t = hmean(M(iz + k), M(iz + k - 1)) * s1 - hmean(M(iz - k + 1), M(iz - k)) * s2;
-        s = accf(s, cf->c0f[(k - 1) + no * (noz - 1)], t);
+        s = accf(s, coefz<NO>(cf->c0f, k, noz), t);
     }
     return s * (1.0f / (g.dz * g.dz));
 }
 
 // Dx(mat * Dz u)  (diffzx at one output cell)
-template <class R>
-__device__ float zx_pt(const Geo &g, const Coefs *cf, const float *mat, const R &u, int iz, int ix)
+template <int NO, class R>
+__device__ __forceinline__ float zx_pt(const Geo &g, const Coefs *cf, const float *mat, const R &u, int iz, int ix)
 {
-    const int no = g.norder;
-    int nox = imin(imin(ix + g.npml - 1, g.nx + g.npml - ix), no);
+    const int nox = imin(imin(ix + g.npml - 1, g.nx + g.npml - ix), NO);
     float s = 0.0f;
-    for (int k = 1; k <= nox; ++k) {
-        float vp = mat[cell(g, iz, ix + k)] * cdz_sum(g, cf, u, iz, ix + k) / g.dz;
-        float vm = mat[cell(g, iz, ix - k)] * cdz_sum(g, cf, u, iz, ix - k) / g.dz;
-        s = accf(s, cf->c1f[(k - 1) + no * (nox - 1)], vp - vm);
+#pragma unroll 1
+    for (int k = 1; k <= NO; ++k) {
+        float vp = mat[cellc(g, iz, ix + k)] * cdz_sum<NO>(g, cf, u, iz, ix + k) / g.dz;
+        float vm = mat[cellc(g, iz, ix - k)] * cdz_sum<NO>(g, cf, u, iz, ix - k) / g.dz;
+        s = accf(s, coefz<NO>(cf->c1f, k, nox), vp - vm);
     }
     return s / g.dx;
 }
 // Dz(mat * Dx u)  (diffxz at one output cell)
-template <class R>
-__device__ float xz_pt(const Geo &g, const Coefs *cf, const float *mat, const R &u, int iz, int ix)
+template <int NO, class R>
+__device__ __forceinline__ float xz_pt(const Geo &g, const Coefs *cf, const float *mat, const R &u, int iz, int ix)
 {
-    const int no = g.norder;
-    int noz = imin(imin(iz - 1, g.nz + g.npml - iz), no);
+    const int noz = imin(imin(iz - 1, g.nz + g.npml - iz), NO);
     float s = 0.0f;
-    for (int k = 1; k <= noz; ++k) {
-        float vp = mat[cell(g, iz + k, ix)] * cdx_sum(g, cf, u, iz + k, ix) / g.dx;
-        float vm = mat[cell(g, iz - k, ix)] * cdx_sum(g, cf, u, iz - k, ix) / g.dx;
-        s = accf(s, cf->c1f[(k - 1) + no * (noz - 1)], vp - vm);
+#pragma unroll 1
+    for (int k = 1; k <= NO; ++k) {
+        float vp = mat[cellc(g, iz + k, ix)] * cdx_sum<NO>(g, cf, u, iz + k, ix) / g.dx;
+        float vm = mat[cellc(g, iz - k, ix)] * cdx_sum<NO>(g, cf, u, iz - k, ix) / g.dx;
+        s = accf(s, coefz<NO>(cf->c1f, k, noz), vp - vm);
     }
     return s / g.dz;
 }
 
-// The six RHS parts {Xx, Zx, Mx, Xz, Zz, Mz} at one cell (free surface at iz = 1).
-template <class R>
-__device__ void eval_parts(const Geo &g, const Coefs *cf, const float *lam, const float *mu,
-                           const R &ux, const R &uz, int iz, int ix, float p[6])
+// One of the eight terms that make up the six RHS parts {Xx, Zx, Mx, Xz,
+// Zz, Mz} at one cell (free surface at iz = 1):
+//   0 Xx = Dx(M Dx ux)   1 Zx = Dz(m Dz ux)   2 Dx(l Dz uz)   3 Dz(m Dx uz)
+//   4 Xz = Dx(m Dx uz)   5 Zz = Dz(M Dz uz)   6 Dx(m Dz ux)   7 Dz(l Dx ux)
+// with Mx = t2 + t3 and Mz = t6 + t7.
+template <int NO, class R>
+__device__ __forceinline__ float eval_task(int task, const Geo &g, const Coefs *cf, const float *lam, const float *mu,
+                                           const R &ux, const R &uz, int iz, int ix)
 {
     if (iz == 1) {
         // free-surface row: half-cell closure derived from the strain energy
         // of the top layer (rows 1-2, trapezoidal in z); traction free naturally
-        size_t a = cell(g, 1, ix), b = cell(g, 2, ix);
-        float sdz2 = 2.0f / (g.dz * g.dz);
-        float M1 = lam[a] + 2.0f * mu[a], M2 = lam[b] + 2.0f * mu[b];
-        const int no = g.norder;
-        int nox = imin(imin(ix + g.npml - 1, g.nx + g.npml - ix), no);
-        float sl = 0.0f, sm = 0.0f;
-        for (int k = 1; k <= nox; ++k) {
-            size_t a1 = cell(g, 1, ix + k), b1 = cell(g, 2, ix + k), a2 = cell(g, 1, ix - k), b2 = cell(g, 2, ix - k);
-            float lp = 0.5f * (lam[a1] + lam[b1]), lm = 0.5f * (lam[a2] + lam[b2]);
-            float mp = 0.5f * (mu[a1] + mu[b1]), mm = 0.5f * (mu[a2] + mu[b2]);
-            float c = cf->c1f[(k - 1) + no * (nox - 1)];
-            sl = accf(sl, c, lp * (uz(2, ix + k) - uz(1, ix + k)) - lm * (uz(2, ix - k) - uz(1, ix - k)));
-            sm = accf(sm, c, mp * (ux(2, ix + k) - ux(1, ix + k)) - mm * (ux(2, ix - k) - ux(1, ix - k)));
+        const size_t a = cell(g, 1, ix), b = cell(g, 2, ix);
+        const float sdz2 = 2.0f / (g.dz * g.dz);
+        switch (task) {
+        case 0: return x2_mode<NO>(g, cf, lam, mu, 0, ux, iz, ix);
+        case 4: return x2_mode<NO>(g, cf, lam, mu, 1, uz, iz, ix);
+        case 1: return (hmean(mu[a], mu[b]) * (ux(2, ix) - ux(1, ix))) * sdz2;
+        case 5: {
+            const float M1 = lam[a] + 2.0f * mu[a], M2 = lam[b] + 2.0f * mu[b];
+            return (hmean(M1, M2) * (uz(2, ix) - uz(1, ix))) * sdz2;
         }
-        float lbar = 0.5f * (lam[a] + lam[b]), mbar = 0.5f * (mu[a] + mu[b]);
-        float dxuz = (cdx_sum(g, cf, uz, 1, ix) + cdx_sum(g, cf, uz, 2, ix)) / g.dx;
-        float dxux = (cdx_sum(g, cf, ux, 1, ix) + cdx_sum(g, cf, ux, 2, ix)) / g.dx;
-        p[0] = x2_mode(g, cf, lam, mu, 0, ux, iz, ix);
-        p[1] = (hmean(mu[a], mu[b]) * (ux(2, ix) - ux(1, ix))) * sdz2;
-        p[2] = (sl / g.dx + mbar * dxuz) / g.dz;
-        p[3] = x2_mode(g, cf, lam, mu, 1, uz, iz, ix);
-        p[4] = (hmean(M1, M2) * (uz(2, ix) - uz(1, ix))) * sdz2;
-        p[5] = (sm / g.dx + lbar * dxux) / g.dz;
-        return;
+        case 2:
+        case 6: {
+            // task 2: (Dx(lbar dz uz) + mbar Dx uz) / dz, task 6: (Dx(mbar dz ux) + lbar Dx ux) / dz
+            const bool t2 = task == 2;
+            const R &w = t2 ? uz : ux;
+            const float *A = t2 ? lam : mu;
+            const int nox = imin(imin(ix + g.npml - 1, g.nx + g.npml - ix), NO);
+            float sa = 0.0f;
+#pragma unroll
+            for (int k = 1; k <= NO; ++k) {
+                size_t a1 = cellc(g, 1, ix + k), b1 = cellc(g, 2, ix + k), a2 = cellc(g, 1, ix - k), b2 = cellc(g, 2, ix - k);
+                float ap = 0.5f * (A[a1] + A[b1]), am = 0.5f * (A[a2] + A[b2]);
+                sa = accf(sa, coefz<NO>(cf->c1f, k, nox), ap * (w(2, ix + k) - w(1, ix + k)) - am * (w(2, ix - k) - w(1, ix - k)));
+            }
+            const float *B = t2 ? mu : lam;
+            const float bbar = 0.5f * (B[a] + B[b]);
+            const float dxw = (cdx_sum<NO>(g, cf, w, 1, ix) + cdx_sum<NO>(g, cf, w, 2, ix)) / g.dx;
+            return (sa / g.dx + bbar * dxw) / g.dz;
+        }
+        default: return 0.0f;
+        }
     }
-    p[0] = x2_mode(g, cf, lam, mu, 0, ux, iz, ix);
-    p[1] = z2_mode(g, cf, lam, mu, 1, ux, iz, ix);
-    p[2] = zx_pt(g, cf, lam, uz, iz, ix) + xz_pt(g, cf, mu, uz, iz, ix);
-    p[3] = x2_mode(g, cf, lam, mu, 1, uz, iz, ix);
-    p[4] = z2_mode(g, cf, lam, mu, 0, uz, iz, ix);
-    p[5] = zx_pt(g, cf, mu, ux, iz, ix) + xz_pt(g, cf, lam, ux, iz, ix);
+    switch (task) {
+    case 0: return x2_mode<NO>(g, cf, lam, mu, 0, ux, iz, ix);
+    case 1: return z2_mode<NO>(g, cf, lam, mu, 1, ux, iz, ix);
+    case 2: return zx_pt<NO>(g, cf, lam, uz, iz, ix);
+    case 3: return xz_pt<NO>(g, cf, mu, uz, iz, ix);
+    case 4: return x2_mode<NO>(g, cf, lam, mu, 1, uz, iz, ix);
+    case 5: return z2_mode<NO>(g, cf, lam, mu, 0, uz, iz, ix);
+    case 6: return zx_pt<NO>(g, cf, mu, ux, iz, ix);
+    default: return xz_pt<NO>(g, cf, lam, ux, iz, ix);
+    }
 }
 
 __device__ __forceinline__ int strip_of(const Geo &g, int iz, int ix)
@@ -263,103 +307,11 @@ struct PsiRd {
     {
         int s = strip_of(*g, iz, ix);
         if (s == S_NONE || (zdir && s != S_B)) return 0.0f;
+        if (ix < 1 - g->npml || ix > g->nx + g->npml || iz < 1 || iz > g->NZ) return 0.0f;
         const float *b = zdir ? st->psz[s][comp] : st->psx[s][comp];
         return b[strip_off(*g, s, iz, ix)];
     }
 };
-
-// ---------------------------------------------------------------------
-// FP32 full-order pointwise stencil (frame cells away from the padded-grid
-// edges and the surface rows): same shared-flux arithmetic as k_interior
-// ---------------------------------------------------------------------
-template <int NO, class R>
-__device__ __forceinline__ void parts_full(const Geo &g, const ICoef<NO> &k, const float *lam,
-                                           const float *mu, const R &ux, const R &uz, int iz, int ix,
-                                           float p[6])
-{
-    constexpr int NL = 4 * NO - 1;
-    constexpr int NM = 2 * NO + 1;
-    float Xx = 0.0f, Xz = 0.0f, Zx = 0.0f, Zz = 0.0f, Mx = 0.0f, Mz = 0.0f;
-    float lax[NM], mux[NM], laz[NM], muz[NM];
-#pragma unroll
-    for (int j = 0; j < NM; ++j) {
-        size_t cx = cell(g, iz, ix - NO + j), cz = cell(g, iz - NO + j, ix);
-        lax[j] = lam[cx]; mux[j] = mu[cx];
-        laz[j] = lam[cz]; muz[j] = mu[cz];
-    }
-    {   // x: D2x[l+2m](ux), D2x[m](uz)
-        float lx0[NL], lx1[NL], ra[NM], rm[NM];
-#pragma unroll
-        for (int j = 0; j < NL; ++j) { lx0[j] = ux(iz, ix - 2 * NO + 1 + j); lx1[j] = uz(iz, ix - 2 * NO + 1 + j); }
-#pragma unroll
-        for (int j = 0; j < NM; ++j) { ra[j] = 1.0f / (lax[j] + 2.0f * mux[j]); rm[j] = 1.0f / mux[j]; }
-        float Fa[2 * NO], Fm[2 * NO];
-#pragma unroll
-        for (int q = 0; q < 2 * NO; ++q) {
-            float sa = 0.0f, sm = 0.0f;
-#pragma unroll
-            for (int jj = 1; jj <= NO; ++jj) {
-                sa += k.ai[jj - 1] * (lx0[NO + q + jj - 1] - lx0[NO + q - jj]);
-                sm += k.ai[jj - 1] * (lx1[NO + q + jj - 1] - lx1[NO + q - jj]);
-            }
-            Fa[q] = sa / (ra[q] + ra[q + 1]);
-            Fm[q] = sm / (rm[q] + rm[q + 1]);
-        }
-#pragma unroll
-        for (int kk = 1; kk <= NO; ++kk) {
-            Xx += k.aox[kk - 1] * (Fa[NO + kk - 1] - Fa[NO - kk]);
-            Xz += k.aox[kk - 1] * (Fm[NO + kk - 1] - Fm[NO - kk]);
-        }
-    }
-    {   // z: D2z[m](ux), D2z[l+2m](uz)
-        float lz0[NL], lz1[NL], ra[NM], rm[NM];
-#pragma unroll
-        for (int j = 0; j < NL; ++j) { lz0[j] = ux(iz - 2 * NO + 1 + j, ix); lz1[j] = uz(iz - 2 * NO + 1 + j, ix); }
-#pragma unroll
-        for (int j = 0; j < NM; ++j) { ra[j] = 1.0f / (laz[j] + 2.0f * muz[j]); rm[j] = 1.0f / muz[j]; }
-        float Fm[2 * NO], Fa[2 * NO];
-#pragma unroll
-        for (int q = 0; q < 2 * NO; ++q) {
-            float sm = 0.0f, sa = 0.0f;
-#pragma unroll
-            for (int jj = 1; jj <= NO; ++jj) {
-                sm += k.ai[jj - 1] * (lz0[NO + q + jj - 1] - lz0[NO + q - jj]);
-                sa += k.ai[jj - 1] * (lz1[NO + q + jj - 1] - lz1[NO + q - jj]);
-            }
-            Fm[q] = sm / (rm[q] + rm[q + 1]);
-            Fa[q] = sa / (ra[q] + ra[q + 1]);
-        }
-#pragma unroll
-        for (int kk = 1; kk <= NO; ++kk) {
-            Zx += k.aoz[kk - 1] * (Fm[NO + kk - 1] - Fm[NO - kk]);
-            Zz += k.aoz[kk - 1] * (Fa[NO + kk - 1] - Fa[NO - kk]);
-        }
-    }
-    // mixed: Dx(l Dz uz) + Dz(m Dx uz) for ux;  Dx(m Dz ux) + Dz(l Dx ux) for uz
-#pragma unroll 1
-    for (int kk = 1; kk <= NO; ++kk) {
-        float dzp_z = 0.0f, dzm_z = 0.0f, dzp_x = 0.0f, dzm_x = 0.0f;  // Dz at columns ix+kk / ix-kk
-        float dxp_z = 0.0f, dxm_z = 0.0f, dxp_x = 0.0f, dxm_x = 0.0f;  // Dx at rows iz+kk / iz-kk
-#pragma unroll
-        for (int j = 1; j <= NO; ++j) {
-            float b = k.bi[j - 1];
-            dzp_z += b * (uz(iz + j, ix + kk) - uz(iz - j, ix + kk));
-            dzm_z += b * (uz(iz + j, ix - kk) - uz(iz - j, ix - kk));
-            dzp_x += b * (ux(iz + j, ix + kk) - ux(iz - j, ix + kk));
-            dzm_x += b * (ux(iz + j, ix - kk) - ux(iz - j, ix - kk));
-            dxp_z += b * (uz(iz + kk, ix + j) - uz(iz + kk, ix - j));
-            dxm_z += b * (uz(iz - kk, ix + j) - uz(iz - kk, ix - j));
-            dxp_x += b * (ux(iz + kk, ix + j) - ux(iz + kk, ix - j));
-            dxm_x += b * (ux(iz - kk, ix + j) - ux(iz - kk, ix - j));
-        }
-        float bo = k.bo[kk - 1];
-        size_t cxp = cell(g, iz, ix + kk), cxm = cell(g, iz, ix - kk);
-        size_t czp = cell(g, iz + kk, ix), czm = cell(g, iz - kk, ix);
-        Mx += bo * ((lam[cxp] * dzp_z - lam[cxm] * dzm_z) + (mu[czp] * dxp_z - mu[czm] * dxm_z));
-        Mz += bo * ((mu[cxp] * dzp_x - mu[cxm] * dzm_x) + (lam[czp] * dxp_x - lam[czm] * dxm_x));
-    }
-    p[0] = Xx; p[1] = Zx; p[2] = Mx; p[3] = Xz; p[4] = Zz; p[5] = Mz;
-}
 
 // ---------------------------------------------------------------------
 // frame kernels: every updated cell outside the interior rectangle, split
@@ -392,6 +344,7 @@ __device__ __forceinline__ CorrRd frame_reader(const StepArgs &a, const float *u
 }
 
 // PML memory corrections and the (split-field) leapfrog update of one cell
+template <int NO>
 __device__ __forceinline__ void frame_update(const StepArgs &a, int iz, int ix, float p[6])
 {
     const Geo &g = a.g;
@@ -401,25 +354,27 @@ __device__ __forceinline__ void frame_update(const StepArgs &a, int iz, int ix, 
     if (s != S_NONE) {
         // -Dx(Mx psi_x), -Dz(Mz psi_z)
         PsiRd px0{a.st, &g, 0, 0}, px1{a.st, &g, 1, 0}, pz0{a.st, &g, 0, 1}, pz1{a.st, &g, 1, 1};
-        const int no = g.norder;
-        int nox = imin(imin(ix + g.npml - 1, g.nx + g.npml - ix), no);
-        int noz = imin(imin(iz - 1, g.nz + g.npml - iz), no);
+        const int nox = imin(imin(ix + g.npml - 1, g.nx + g.npml - ix), NO);
+        const int noz = imin(imin(iz - 1, g.nz + g.npml - iz), NO);
         float sx0 = 0.0f, sx1 = 0.0f, sz0 = 0.0f, sz1 = 0.0f;
-        for (int kk = 1; kk <= nox; ++kk) {
-            size_t ca = cell(g, iz, ix + kk), cb = cell(g, iz, ix - kk);
+#pragma unroll
+        for (int kk = 1; kk <= NO; ++kk) {
+            size_t ca = cellc(g, iz, ix + kk), cb = cellc(g, iz, ix - kk);
             float ma = a.lam[ca] + 2.0f * a.mu[ca], mb = a.lam[cb] + 2.0f * a.mu[cb];
-            float c = cf->c1f[(kk - 1) + no * (nox - 1)];
+            float c = coefz<NO>(cf->c1f, kk, nox);
             sx0 = accf(sx0, c, ma * px0(iz, ix + kk) - mb * px0(iz, ix - kk));
             sx1 = accf(sx1, c, a.mu[ca] * px1(iz, ix + kk) - a.mu[cb] * px1(iz, ix - kk));
         }
-        if (s == S_B)
-            for (int kk = 1; kk <= noz; ++kk) {
-                size_t ca = cell(g, iz + kk, ix), cb = cell(g, iz - kk, ix);
-                float c = cf->c1f[(kk - 1) + no * (noz - 1)];
+        if (s == S_B) {
+#pragma unroll
+            for (int kk = 1; kk <= NO; ++kk) {
+                size_t ca = cellc(g, iz + kk, ix), cb = cellc(g, iz - kk, ix);
+                float c = coefz<NO>(cf->c1f, kk, noz);
                 sz0 = accf(sz0, c, a.mu[ca] * pz0(iz + kk, ix) - a.mu[cb] * pz0(iz - kk, ix));
                 sz1 = accf(sz1, c, (a.lam[ca] + 2.0f * a.mu[ca]) * pz1(iz + kk, ix) -
                                        (a.lam[cb] + 2.0f * a.mu[cb]) * pz1(iz - kk, ix));
             }
+        }
         p[0] = p[0] - sx0 / g.dx;
         p[3] = p[3] - sx1 / g.dx;
         if (!surf) {
@@ -454,36 +409,33 @@ __device__ __forceinline__ void frame_update(const StepArgs &a, int iz, int ix, 
     }
 }
 
+// Edge kernel: 512 threads per block, 64 cells per block, eight waves; wave
+// w evaluates term w of the 64 cells (wave-uniform control flow), then wave 0
+// combines the terms and applies the memory corrections and the update.
+constexpr int kEdgeCells = 64;
+constexpr int kEdgeThreads = 8 * kEdgeCells;
 template <int NO>
-__device__ __forceinline__ void frame_full_body(const StepArgs &a, const Rects &r, const ICoef<NO> &k, long bid)
-{
-    long t = bid * blockDim.x + threadIdx.x;
-    int iz, ix;
-    if (!rect_cell(r, t, iz, ix)) return;
-    CorrRd rx = frame_reader(a, a.ux, a.ringx, iz, ix), rz = frame_reader(a, a.uz, a.ringz, iz, ix);
-    float p[6];
-    parts_full<NO>(a.g, k, a.lam, a.mu, rx, rz, iz, ix, p);
-    frame_update(a, iz, ix, p);
-}
-template <int NO>
-__global__ __launch_bounds__(256) void k_frame_full(StepArgs a, Rects r, ICoef<NO> k)
-{
-    frame_full_body<NO>(a, r, k, blockIdx.x);
-}
-
 __device__ __forceinline__ void frame_edge_body(const StepArgs &a, const Rects &r, long bid)
 {
-    long t = bid * blockDim.x + threadIdx.x;
-    int iz, ix;
-    if (!rect_cell(r, t, iz, ix)) return;
-    CorrRd rx = frame_reader(a, a.ux, a.ringx, iz, ix), rz = frame_reader(a, a.uz, a.ringz, iz, ix);
-    float p[6];
-    eval_parts(a.g, a.cf, a.lam, a.mu, rx, rz, iz, ix, p);
-    frame_update(a, iz, ix, p);
+    __shared__ float part[8][kEdgeCells];
+    const int task = threadIdx.x / kEdgeCells, slot = threadIdx.x % kEdgeCells;
+    const long t = bid * kEdgeCells + slot;
+    int iz = 1, ix = 1;
+    const bool ok = rect_cell(r, t, iz, ix);
+    if (ok) {
+        CorrRd rx = frame_reader(a, a.ux, a.ringx, iz, ix), rz = frame_reader(a, a.uz, a.ringz, iz, ix);
+        part[task][slot] = eval_task<NO>(task, a.g, a.cf, a.lam, a.mu, rx, rz, iz, ix);
+    }
+    __syncthreads();
+    if (task != 0 || !ok) return;
+    float p[6] = {part[0][slot], part[1][slot], part[2][slot] + part[3][slot],
+                  part[4][slot], part[5][slot], part[6][slot] + part[7][slot]};
+    frame_update<NO>(a, iz, ix, p);
 }
-__global__ __launch_bounds__(256) void k_frame_edge(StepArgs a, Rects r)
+template <int NO>
+__global__ __launch_bounds__(kEdgeThreads) void k_frame_edge(StepArgs a, Rects r)
 {
-    frame_edge_body(a, r, blockIdx.x);
+    frame_edge_body<NO>(a, r, blockIdx.x);
 }
 
 // background on the hybrid ring at iteration it, scattered into the
@@ -523,6 +475,7 @@ __global__ __launch_bounds__(256) void k_ring_prep(Geo g, const Background *bg, 
 // ---------------------------------------------------------------------
 // PML first-derivative memories (phase A of the step)
 // ---------------------------------------------------------------------
+template <int NO>
 __device__ __forceinline__ void psi_body(const StepArgs &a, long bid)
 {
     const Geo &g = a.g;
@@ -540,24 +493,25 @@ __device__ __forceinline__ void psi_body(const StepArgs &a, long bid)
     const Strips *st = a.st;
     if (st->ax[jx] > 0.0f) {
         float b = st->bx[jx], c = 1.0f - b;
-        float gx = cdx_sum(g, a.cf, rx, iz, ix) / g.dx;
-        float gz = cdx_sum(g, a.cf, rz, iz, ix) / g.dx;
+        float gx = cdx_sum<NO>(g, a.cf, rx, iz, ix) / g.dx;
+        float gz = cdx_sum<NO>(g, a.cf, rz, iz, ix) / g.dx;
         float *p0 = st->psx[s][0], *p1 = st->psx[s][1];
         p0[o] = b * p0[o] + c * gx;
         p1[o] = b * p1[o] + c * gz;
     }
     if (s == S_B && st->az[iz - 1] > 0.0f) {
         float b = st->bz[iz - 1], c = 1.0f - b;
-        float gx = cdz_sum(g, a.cf, rx, iz, ix) / g.dz;
-        float gz = cdz_sum(g, a.cf, rz, iz, ix) / g.dz;
+        float gx = cdz_sum<NO>(g, a.cf, rx, iz, ix) / g.dz;
+        float gz = cdz_sum<NO>(g, a.cf, rz, iz, ix) / g.dz;
         float *p0 = st->psz[s][0], *p1 = st->psz[s][1];
         p0[o] = b * p0[o] + c * gx;
         p1[o] = b * p1[o] + c * gz;
     }
 }
+template <int NO>
 __global__ __launch_bounds__(256) void k_psi(StepArgs a)
 {
-    psi_body(a, blockIdx.x);
+    psi_body<NO>(a, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------
@@ -627,186 +581,6 @@ __global__ __launch_bounds__(64) void k_record(RecArgs r)
 #pragma clang fp contract(on)
 
 // ---------------------------------------------------------------------
-// interior kernel: full-order fused stencil, marching along x
-// ---------------------------------------------------------------------
-
-template <int NO, int NT>
-struct IntKernel {
-    static constexpr int HZ = 2 * NO - 1;   // z halo lanes on each side
-    static constexpr int TZ = NT - 2 * HZ;  // output rows per block
-    static constexpr int P = 3 * NO;        // register-ring period
-    static constexpr int PAD = NO + 1;      // LDS guard entries on each side
-    static constexpr int WARM = 4 * NO - 2; // warm-up columns before the first output
-
-    struct Smem {
-        float uA[2][NT + 2 * PAD];   // u at column ix+NO (for Vx)
-        float uB[2][NT + 2 * PAD];   // u at column ix (for Fz)
-        float rM[2][NT + 2 * PAD];   // 1/(l+2m), 1/m at column ix
-        float fz[2][NT + 2 * PAD];   // z-fluxes at column ix
-        float vz[2][NT + 2 * PAD];   // m*Dx uz, l*Dx ux at column ix
-    };
-
-    static __device__ __forceinline__ int sl(int ph, int j) { return ((ph + j) % P + P) % P; }
-
-    template <int PH>
-    static __device__ __forceinline__ void step(const IntArgs<NO> &a, Smem &S, int lane, int r, bool rowok,
-                                                bool outrow, int ix, int xs, int xe,
-                                                float (&U)[2][P], float (&FX)[2][P], float (&VX)[2][P],
-                                                float &rMa_prev, float &rMm_prev)
-    {
-        if (ix >= xe) return;  // block-uniform
-        const Geo &g = a.g;
-        const size_t colpitch = (size_t)g.ld;
-        const size_t rowoff = (size_t)(r - 1);
-        auto at = [&](int c) { return (size_t)(c + g.npml - 1) * colpitch + rowoff; };
-        // 1. new u column ix+2NO-1, materials at ix+NO
-        {
-            float nx_ = 0.0f, nz_ = 0.0f, l1 = 1.0f, m1 = 1.0f;
-            if (rowok) {
-                size_t cn = at(ix + 2 * NO - 1), cm = at(ix + NO);
-                nx_ = a.ux[cn]; nz_ = a.uz[cn];
-                l1 = a.lam[cm]; m1 = a.mu[cm];
-            }
-            U[0][sl(PH, 2 * NO - 1)] = nx_;
-            U[1][sl(PH, 2 * NO - 1)] = nz_;
-            float rMa = 1.0f / (l1 + 2.0f * m1), rMm = 1.0f / m1;
-            if (ix >= xs - 2 * NO) {
-                // x-fluxes at half-point h = ix+NO-1 (between h and h+1)
-                float ha = 1.0f / (rMa_prev + rMa), hm = 1.0f / (rMm_prev + rMm);
-                float s0 = 0.0f, s1 = 0.0f;
-#pragma unroll
-                for (int j = 1; j <= NO; ++j) {
-                    s0 += a.k.ai[j - 1] * (U[0][sl(PH, NO - 1 + j)] - U[0][sl(PH, NO - j)]);
-                    s1 += a.k.ai[j - 1] * (U[1][sl(PH, NO - 1 + j)] - U[1][sl(PH, NO - j)]);
-                }
-                FX[0][sl(PH, NO - 1)] = ha * s0;
-                FX[1][sl(PH, NO - 1)] = hm * s1;
-                // stage A: share u at ix+NO and ix, and 1/M at ix
-                S.uA[0][PAD + lane] = U[0][sl(PH, NO)];
-                S.uA[1][PAD + lane] = U[1][sl(PH, NO)];
-                S.uB[0][PAD + lane] = U[0][sl(PH, 0)];
-                S.uB[1][PAD + lane] = U[1][sl(PH, 0)];
-            }
-            rMa_prev = rMa;
-            rMm_prev = rMm;
-            if (ix < xs - 2 * NO) return;
-            // materials at column ix
-            float l0 = 1.0f, m0 = 1.0f;
-            if (rowok) {
-                size_t c0 = at(ix);
-                l0 = a.lam[c0]; m0 = a.mu[c0];
-            }
-            float rMa0 = 1.0f / (l0 + 2.0f * m0), rMm0 = 1.0f / m0;
-            S.rM[0][PAD + lane] = rMa0;
-            S.rM[1][PAD + lane] = rMm0;
-            __syncthreads();
-            // Vx at column ix+NO: l*Dz uz and m*Dz ux
-            {
-                float dzz = 0.0f, dzx = 0.0f;
-#pragma unroll
-                for (int j = 1; j <= NO; ++j) {
-                    dzz += a.k.bi[j - 1] * (S.uA[1][PAD + lane + j] - S.uA[1][PAD + lane - j]);
-                    dzx += a.k.bi[j - 1] * (S.uA[0][PAD + lane + j] - S.uA[0][PAD + lane - j]);
-                }
-                VX[0][sl(PH, NO)] = l1 * dzz;
-                VX[1][sl(PH, NO)] = m1 * dzx;
-            }
-            if (ix < xs) {
-                __syncthreads();  // keep the two-barrier cadence uniform
-                return;
-            }
-            // z-fluxes at (r+1/2, ix) and Vz at (r, ix)
-            float sfx = 0.0f, sfz = 0.0f;
-#pragma unroll
-            for (int j = 1; j <= NO; ++j) {
-                sfx += a.k.ai[j - 1] * (S.uB[0][PAD + lane + j] - S.uB[0][PAD + lane + 1 - j]);
-                sfz += a.k.ai[j - 1] * (S.uB[1][PAD + lane + j] - S.uB[1][PAD + lane + 1 - j]);
-            }
-            float fzx = sfx / (rMm0 + S.rM[1][PAD + lane + 1]);
-            float fzz = sfz / (rMa0 + S.rM[0][PAD + lane + 1]);
-            float dxz = 0.0f, dxx = 0.0f;
-#pragma unroll
-            for (int k = 1; k <= NO; ++k) {
-                dxz += a.k.bi[k - 1] * (U[1][sl(PH, k)] - U[1][sl(PH, -k)]);
-                dxx += a.k.bi[k - 1] * (U[0][sl(PH, k)] - U[0][sl(PH, -k)]);
-            }
-            __syncthreads();  // all reads of uA/uB/rM for this column are done
-            S.fz[0][PAD + lane] = fzx;
-            S.fz[1][PAD + lane] = fzz;
-            S.vz[0][PAD + lane] = m0 * dxz;
-            S.vz[1][PAD + lane] = l0 * dxx;
-            __syncthreads();
-            if (outrow) {
-                float Xx = 0.0f, Xz = 0.0f, Zx = 0.0f, Zz = 0.0f, Mx = 0.0f, Mz = 0.0f;
-#pragma unroll
-                for (int k = 1; k <= NO; ++k) {
-                    Xx += a.k.aox[k - 1] * (FX[0][sl(PH, k - 1)] - FX[0][sl(PH, -k)]);
-                    Xz += a.k.aox[k - 1] * (FX[1][sl(PH, k - 1)] - FX[1][sl(PH, -k)]);
-                    Zx += a.k.aoz[k - 1] * (S.fz[0][PAD + lane + k - 1] - S.fz[0][PAD + lane - k]);
-                    Zz += a.k.aoz[k - 1] * (S.fz[1][PAD + lane + k - 1] - S.fz[1][PAD + lane - k]);
-                    Mx += a.k.bo[k - 1] * ((VX[0][sl(PH, k)] - VX[0][sl(PH, -k)]) +
-                                          (S.vz[0][PAD + lane + k] - S.vz[0][PAD + lane - k]));
-                    Mz += a.k.bo[k - 1] * ((VX[1][sl(PH, k)] - VX[1][sl(PH, -k)]) +
-                                          (S.vz[1][PAD + lane + k] - S.vz[1][PAD + lane - k]));
-                }
-                size_t c = at(ix);
-                float bf = a.bf[c];
-                float upx = a.uxp[c], upz = a.uzp[c];
-                a.uxp[c] = (2.0f * U[0][sl(PH, 0)] - upx) + bf * ((Xx + Zx) + Mx);
-                a.uzp[c] = (2.0f * U[1][sl(PH, 0)] - upz) + bf * ((Xz + Zz) + Mz);
-            }
-        }
-    }
-
-    static __device__ void run(const IntArgs<NO> &a)
-    {
-        __shared__ Smem S;
-        const Geo &g = a.g;
-        const int lane = threadIdx.x;
-        // zero the LDS guards once
-        for (int i = lane; i < NT + 2 * PAD; i += NT) {
-            if (i < PAD || i >= NT + PAD)
-                for (int q = 0; q < 2; ++q) { S.uA[q][i] = 0.0f; S.uB[q][i] = 0.0f; S.rM[q][i] = 1.0f; S.fz[q][i] = 0.0f; S.vz[q][i] = 0.0f; }
-        }
-        const int bxid = blockIdx.x % a.nbx, bzid = blockIdx.x / a.nbx;
-        const int z0 = g.iz0 + bzid * TZ;
-        const int r = z0 - HZ + lane;
-        const bool rowok = (r >= g.iz0 - HZ) && (r <= g.iz1 + HZ);
-        const bool outrow = (lane >= HZ) && (lane < HZ + TZ) && (r <= g.iz1);
-        const int xs = g.ix0 + bxid * a.lx;
-        const int xe = min(xs + a.lx, g.ix1 + 1);
-        float U[2][P], FX[2][P], VX[2][P];
-#pragma unroll
-        for (int q = 0; q < 2; ++q)
-#pragma unroll
-            for (int i = 0; i < P; ++i) { U[q][i] = 0.0f; FX[q][i] = 0.0f; VX[q][i] = 0.0f; }
-        float rMa_prev = 1.0f, rMm_prev = 1.0f;
-        __syncthreads();
-        for (int base = xs - WARM; base < xe; base += P) {
-            step<0>(a, S, lane, r, rowok, outrow, base + 0, xs, xe, U, FX, VX, rMa_prev, rMm_prev);
-            if constexpr (P > 1) step<1>(a, S, lane, r, rowok, outrow, base + 1, xs, xe, U, FX, VX, rMa_prev, rMm_prev);
-            if constexpr (P > 2) step<2>(a, S, lane, r, rowok, outrow, base + 2, xs, xe, U, FX, VX, rMa_prev, rMm_prev);
-            if constexpr (P > 3) step<3>(a, S, lane, r, rowok, outrow, base + 3, xs, xe, U, FX, VX, rMa_prev, rMm_prev);
-            if constexpr (P > 4) step<4>(a, S, lane, r, rowok, outrow, base + 4, xs, xe, U, FX, VX, rMa_prev, rMm_prev);
-            if constexpr (P > 5) step<5>(a, S, lane, r, rowok, outrow, base + 5, xs, xe, U, FX, VX, rMa_prev, rMm_prev);
-            if constexpr (P > 6) step<6>(a, S, lane, r, rowok, outrow, base + 6, xs, xe, U, FX, VX, rMa_prev, rMm_prev);
-            if constexpr (P > 7) step<7>(a, S, lane, r, rowok, outrow, base + 7, xs, xe, U, FX, VX, rMa_prev, rMm_prev);
-            if constexpr (P > 8) step<8>(a, S, lane, r, rowok, outrow, base + 8, xs, xe, U, FX, VX, rMa_prev, rMm_prev);
-            if constexpr (P > 9) step<9>(a, S, lane, r, rowok, outrow, base + 9, xs, xe, U, FX, VX, rMa_prev, rMm_prev);
-            if constexpr (P > 10) step<10>(a, S, lane, r, rowok, outrow, base + 10, xs, xe, U, FX, VX, rMa_prev, rMm_prev);
-            if constexpr (P > 11) step<11>(a, S, lane, r, rowok, outrow, base + 11, xs, xe, U, FX, VX, rMa_prev, rMm_prev);
-        }
-    }
-};
-
-template <int NO, int NT>
-__global__ __launch_bounds__(NT) void k_interior(IntArgs<NO> a)
-{
-    IntKernel<NO, NT>::run(a);
-}
-
-
-// ---------------------------------------------------------------------
 // interior kernel, pipelined variant: one barrier per column (double-
 // buffered LDS), loads issued one column ahead, outputs of column ix-1
 // written while column ix is staged, hardware reciprocals
@@ -830,6 +604,11 @@ struct IntKernel2 {
     };
 
     static __device__ __forceinline__ int sl(int ph, int j) { return ((ph + j) % P + P) % P; }
+    // padding cell: read in the total-field view (u + u0 on the hybrid ring)
+    static __device__ __forceinline__ bool outside(const Geo &g, int r, int c)
+    {
+        return c < 1 || c > g.nx || r > g.nz;
+    }
 
     struct Regs {
         float U[2][P], FX[2][P], VX[2][P];
@@ -842,12 +621,15 @@ struct IntKernel2 {
 
     template <int PH>
     static __device__ __forceinline__ void step(const IntArgs<NO> &a, Smem &S, Regs &R, int lane, int r,
-                                                bool rowok, bool outrow, int ix, int xs, int xe)
+                                                bool rowok, bool outrow, bool bc, int ix, int xs, int xe)
     {
         constexpr int PA = PH & 1, PB = 1 - PA;
         const Geo &g = a.g;
         const size_t rowoff = (size_t)(r - 1);
         auto at = [&](int c) { return (size_t)(c + g.npml - 1) * (size_t)g.ld + rowoff; };
+        // material columns of the first warm-up steps may lie left of the
+        // grid; their values are never used, so the address is clamped
+        auto atm = [&](int c) { return at(c < 1 - g.npml ? 1 - g.npml : c); };
         const bool active = ix < xe;
         float l1 = 1.0f, m1 = 1.0f, l0 = 1.0f, m0 = 1.0f;
         if (active) {
@@ -857,8 +639,9 @@ struct IntKernel2 {
             l1 = R.pl[PA][0]; m1 = R.pl[PA][1];
             l0 = R.p0[PA][0]; m0 = R.p0[PA][1];
             if (rowok && ix + 1 < xe) {
-                size_t cn = at(ix + 2 * NO), cm = at(ix + 1 + NO), c0 = at(ix + 1);
+                size_t cn = at(ix + 2 * NO), cm = atm(ix + 1 + NO), c0 = atm(ix + 1);
                 R.pu[PB][0] = a.ux[cn]; R.pu[PB][1] = a.uz[cn];
+                if (bc && outside(a.g, r, ix + 2 * NO)) { R.pu[PB][0] += a.ringx[cn]; R.pu[PB][1] += a.ringz[cn]; }
                 R.pl[PB][0] = a.lam[cm]; R.pl[PB][1] = a.mu[cm];
                 R.p0[PB][0] = a.lam[c0]; R.p0[PB][1] = a.mu[c0];
             }
@@ -964,6 +747,8 @@ struct IntKernel2 {
         const bool outrow = (lane >= HZ) && (lane < HZ + TZ) && (r <= g.iz1);
         const int xs = g.ix0 + bxid * a.lx;
         const int xe = min(xs + a.lx, g.ix1 + 1);
+        // blocks whose loads reach the padding read the ring (block-uniform)
+        const bool bc = a.corr && (xs - WARM + 2 * NO - 1 < 1 || xe + 2 * NO - 1 > g.nx || z0 - HZ + NT - 1 > g.nz);
         Regs R;
 #pragma unroll
         for (int q = 0; q < 2; ++q)
@@ -982,13 +767,17 @@ struct IntKernel2 {
         if (rowok) {
             size_t rowoff = (size_t)(r - 1);
             auto at = [&](int c) { return (size_t)(c + g.npml - 1) * (size_t)g.ld + rowoff; };
-            R.pu[0][0] = a.ux[at(x0 + 2 * NO - 1)]; R.pu[0][1] = a.uz[at(x0 + 2 * NO - 1)];
-            R.pl[0][0] = a.lam[at(x0 + NO)]; R.pl[0][1] = a.mu[at(x0 + NO)];
-            R.p0[0][0] = a.lam[at(x0)]; R.p0[0][1] = a.mu[at(x0)];
+            const size_t cn = at(x0 + 2 * NO - 1);
+            R.pu[0][0] = a.ux[cn]; R.pu[0][1] = a.uz[cn];
+            if (bc && outside(g, r, x0 + 2 * NO - 1)) { R.pu[0][0] += a.ringx[cn]; R.pu[0][1] += a.ringz[cn]; }
+            const int xm = x0 < 1 - g.npml ? 1 - g.npml : x0;
+            const int xl = x0 + NO < 1 - g.npml ? 1 - g.npml : x0 + NO;
+            R.pl[0][0] = a.lam[at(xl)]; R.pl[0][1] = a.mu[at(xl)];
+            R.p0[0][0] = a.lam[at(xm)]; R.p0[0][1] = a.mu[at(xm)];
         }
         __syncthreads();
         for (int base = x0; base <= xe; base += P) {
-#define IK2_STEP(PH) if constexpr (P > PH) { if (base + PH <= xe) step<PH>(a, S, R, lane, r, rowok, outrow, base + PH, xs, xe); }
+#define IK2_STEP(PH) if constexpr (P > PH) { if (base + PH <= xe) step<PH>(a, S, R, lane, r, rowok, outrow, bc, base + PH, xs, xe); }
             IK2_STEP(0) IK2_STEP(1) IK2_STEP(2) IK2_STEP(3) IK2_STEP(4) IK2_STEP(5) IK2_STEP(6)
             IK2_STEP(7) IK2_STEP(8) IK2_STEP(9) IK2_STEP(10) IK2_STEP(11) IK2_STEP(12) IK2_STEP(13)
 #undef IK2_STEP

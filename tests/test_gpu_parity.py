@@ -37,6 +37,17 @@ def _max_rel(a, b):
     return float(np.abs(a.astype(np.float64) - b).max() / max(np.abs(b).max(), 1e-30))
 
 
+def _strip_ok(a, b, utot):
+    """Split-field strip arrays: 1e-5 of the array's own scale, plus a floor
+    of 4 FP32 ulps of the total-field scale.  The first padding row under the
+    hybrid interface holds the scattered field (~1e-5 of the total field
+    here); the oracle forms it as L[u] - L[u0], a difference of total-field
+    sized stencil sums, so it is defined only to the total field's FP32
+    resolution."""
+    err = np.abs(a.astype(np.float64) - b).max()
+    return err <= 1e-5 * np.abs(b).max() + 4.0 * np.finfo(np.float32).eps * utot
+
+
 def test_single_step_compat_matches_oracle(case, bg, warm_state):
     print("case: single step via updatedispl_, it=180", flush=True)
     so = refstate.copy_state(warm_state)
@@ -48,8 +59,9 @@ def test_single_step_compat_matches_oracle(case, bg, warm_state):
     lib.lib().fdfk_compat_release()
     for k in ("ux", "uz"):
         assert _max_rel(sg[k], so[k]) < 1e-5, k
+    utot = max(np.abs(so["ux"]).max(), np.abs(so["uz"]).max())
     for i, (a, b) in enumerate(zip(sg["bu"][:18], so["bu"][:18])):
-        assert _max_rel(a, b) < 1e-5, ("bu", i)
+        assert _strip_ok(a, b, utot), ("bu", i)
     for i in (0, 1, 2, 3, 8, 9, 10, 11):
         assert _max_rel(sg["bp"][i], so["bp"][i]) < 1e-5, ("bp", i)
 
