@@ -51,8 +51,9 @@ def gpu_bench(steps, warmup, workload):
     t1 = time.perf_counter()
     st = s.kernel_stats()
     NX, NZ = cfg["NX"], cfg["NZ"]
-    nbd = 5
-    int_cells = (cfg["nx"] - 2 * nbd) * (cfg["nz"] - nbd - 2 * 3 + 1)
+    # the interior kernel covers the total-field region below the surface
+    # rows: columns 1..nx, rows 2*norder..nz
+    int_cells = cfg["nx"] * (cfg["nz"] - 2 * cfg["norder"] + 1)
     sx, sz = s.seismograms()
     finite = bool(np.isfinite(sx).all() and np.isfinite(sz).all())
     s.close()
@@ -82,6 +83,20 @@ def cpu_baseline(sample_steps):
                        f"{dt:.1f} s, OMP_NUM_THREADS={threads}")
 
 
+def copy_calibration(mib=512, reps=5):
+    """Plain device-to-device copies of a known size (torch's copy kernel):
+    the reference point that converts the FETCH_SIZE / WRITE_SIZE counters of
+    a profiler run into bytes."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n * 4, "reps": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -90,7 +105,12 @@ def main():
     ap.add_argument("--workload", default="c2", choices=["c1", "c2"])
     ap.add_argument("--cpu-steps", type=int, default=41)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="only run the copy kernel used to calibrate HBM byte counters")
     a = ap.parse_args()
+    if a.copy_calib:
+        copy_calibration()
+        return
     if a.gpus != 1:
         print("bench.py: this build measures one GPU; the hot path shards as x-slabs "
               "(see DESIGN.md) but multi-GPU runs are not part of it", file=sys.stderr)
@@ -132,7 +152,7 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": None,
-            "kernel": "k_interior",
+            "kernel": "k_interior2",
             "kernel_avg_ms": round(int_ms, 5),
             "bytes_per_cell": BYTES_PER_CELL,
             "cells_per_launch": r["int_cells"],

@@ -580,10 +580,24 @@ __global__ __launch_bounds__(64) void k_record(RecArgs r)
 
 #pragma clang fp contract(on)
 
+// word 3 of a gfx9 buffer resource (32-bit raw data, no swizzle)
+constexpr int kBufferFlags = 0x00020000;
+
 // ---------------------------------------------------------------------
-// interior kernel, pipelined variant: one barrier per column (double-
-// buffered LDS), loads issued one column ahead, outputs of column ix-1
-// written while column ix is staged, hardware reciprocals
+// interior kernel: full-order fused stencil over the total-field region.
+// Lanes run along z (one row each, HZ halo rows on either side of the TZ
+// output rows), the block marches along x one column per step.  Per step:
+//   A  take the prefetched column ix+2NO-1 into the register ring, prefetch
+//      column ix+2NO and the materials of the next step, form the x-flux at
+//      the face (ix+NO-1, ix+NO) and stage u(ix+NO), u(ix), 1/M(ix) in LDS;
+//   B  one barrier (LDS double-buffered by column parity);
+//   C  l Dz uz, m Dz ux at ix+NO (x-mixed ring), z-fluxes and l Dx ux,
+//      m Dx uz at ix (to LDS);
+//   D  the update of column ix-1, whose z-terms were staged one step before.
+// The steady state has no branches: every lane loads (rows clamped into the
+// grid; rows beyond the stencil reach of an output are never used) and only
+// the stores are predicated.  Addresses are a per-column scalar base plus a
+// loop-invariant 32-bit lane offset.
 // ---------------------------------------------------------------------
 template <int NO, int NT>
 struct IntKernel2 {
@@ -619,61 +633,80 @@ struct IntKernel2 {
         float po[2][3];     // [parity][bf, uxp, uzp] of column ix (used one step later)
     };
 
+    struct Lane {
+        int lane, r;
+        unsigned ofs;       // clamped row byte offset inside a column
+        bool outrow;
+    };
+    // element at byte offset o of column c: a buffer resource on the column
+    // (scalar base, built per column) plus the lane's 32-bit row offset
+    static __device__ __forceinline__ __amdgpu_buffer_rsrc_t colrsrc(const Geo &g, const float *base, int c)
+    {
+        const float *cb = base + (size_t)(c + g.npml - 1) * (size_t)g.ld;
+        return __builtin_amdgcn_make_buffer_rsrc((void *)cb, 0, g.ld * 4, kBufferFlags);
+    }
+    static __device__ __forceinline__ float at(const Geo &g, const float *base, int c, unsigned o)
+    {
+        return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(colrsrc(g, base, c), o, 0, 0));
+    }
+    static __device__ __forceinline__ void put(const Geo &g, float *base, int c, unsigned o, float v)
+    {
+        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), colrsrc(g, base, c), o, 0, 0);
+    }
+
     template <int PH>
-    static __device__ __forceinline__ void step(const IntArgs<NO> &a, Smem &S, Regs &R, int lane, int r,
-                                                bool rowok, bool outrow, bool bc, int ix, int xs, int xe)
+    static __device__ __forceinline__ void step(const IntArgs<NO> &a, Smem &S, Regs &R, const Lane &L, bool bc,
+                                                int ix, int xs, int xe)
     {
         constexpr int PA = PH & 1, PB = 1 - PA;
         const Geo &g = a.g;
-        const size_t rowoff = (size_t)(r - 1);
-        auto at = [&](int c) { return (size_t)(c + g.npml - 1) * (size_t)g.ld + rowoff; };
-        // material columns of the first warm-up steps may lie left of the
-        // grid; their values are never used, so the address is clamped
-        auto atm = [&](int c) { return at(c < 1 - g.npml ? 1 - g.npml : c); };
-        const bool active = ix < xe;
-        float l1 = 1.0f, m1 = 1.0f, l0 = 1.0f, m0 = 1.0f;
-        if (active) {
-            // consume the prefetched column ix+2NO-1 and lead materials, prefetch the next ones
-            R.U[0][sl(PH, 2 * NO - 1)] = R.pu[PA][0];
-            R.U[1][sl(PH, 2 * NO - 1)] = R.pu[PA][1];
-            l1 = R.pl[PA][0]; m1 = R.pl[PA][1];
-            l0 = R.p0[PA][0]; m0 = R.p0[PA][1];
-            if (rowok && ix + 1 < xe) {
-                size_t cn = at(ix + 2 * NO), cm = atm(ix + 1 + NO), c0 = atm(ix + 1);
-                R.pu[PB][0] = a.ux[cn]; R.pu[PB][1] = a.uz[cn];
-                if (bc && outside(a.g, r, ix + 2 * NO)) { R.pu[PB][0] += a.ringx[cn]; R.pu[PB][1] += a.ringz[cn]; }
-                R.pl[PB][0] = a.lam[cm]; R.pl[PB][1] = a.mu[cm];
-                R.p0[PB][0] = a.lam[c0]; R.p0[PB][1] = a.mu[c0];
+        const int lane = L.lane;
+        const int lo = 1 - g.npml, hi = g.nx + g.npml;
+        auto colc = [&](int c) { return c < lo ? lo : (c > hi ? hi : c); };
+        const unsigned o = L.ofs;
+        // A. ring update, prefetch of the next step, x-flux, LDS staging
+        R.U[0][sl(PH, 2 * NO - 1)] = R.pu[PA][0];
+        R.U[1][sl(PH, 2 * NO - 1)] = R.pu[PA][1];
+        const float l1 = R.pl[PA][0], m1 = R.pl[PA][1];
+        const float l0 = R.p0[PA][0], m0 = R.p0[PA][1];
+        {
+            const int cn = colc(ix + 2 * NO), cm = colc(ix + 1 + NO), c0 = colc(ix + 1), cx = colc(ix);
+            R.pu[PB][0] = at(g, a.ux, cn, o);
+            R.pu[PB][1] = at(g, a.uz, cn, o);
+            if (bc && outside(g, L.r, cn)) {
+                R.pu[PB][0] += at(g, a.ringx, cn, o);
+                R.pu[PB][1] += at(g, a.ringz, cn, o);
             }
-            if (outrow && ix >= xs) {
-                size_t c = at(ix);
-                R.po[PA][0] = a.bf[c]; R.po[PA][1] = a.uxp[c]; R.po[PA][2] = a.uzp[c];
-            }
-            float rMa = frcp(l1 + 2.0f * m1), rMm = frcp(m1);
-            if (ix >= xs - 2 * NO) {
-                float ha = frcp(R.rMa_prev + rMa), hm = frcp(R.rMm_prev + rMm);
-                float s0 = 0.0f, s1 = 0.0f;
-#pragma unroll
-                for (int j = 1; j <= NO; ++j) {
-                    s0 += a.k.ai[j - 1] * (R.U[0][sl(PH, NO - 1 + j)] - R.U[0][sl(PH, NO - j)]);
-                    s1 += a.k.ai[j - 1] * (R.U[1][sl(PH, NO - 1 + j)] - R.U[1][sl(PH, NO - j)]);
-                }
-                R.FX[0][sl(PH, NO - 1)] = ha * s0;
-                R.FX[1][sl(PH, NO - 1)] = hm * s1;
-                S.uA[PA][0][PAD + lane] = R.U[0][sl(PH, NO)];
-                S.uA[PA][1][PAD + lane] = R.U[1][sl(PH, NO)];
-                S.uB[PA][0][PAD + lane] = R.U[0][sl(PH, 0)];
-                S.uB[PA][1][PAD + lane] = R.U[1][sl(PH, 0)];
-                S.rM[PA][0][PAD + lane] = frcp(l0 + 2.0f * m0);
-                S.rM[PA][1][PAD + lane] = frcp(m0);
-            }
-            R.rMa_prev = rMa;
-            R.rMm_prev = rMm;
+            R.pl[PB][0] = at(g, a.lam, cm, o); R.pl[PB][1] = at(g, a.mu, cm, o);
+            R.p0[PB][0] = at(g, a.lam, c0, o); R.p0[PB][1] = at(g, a.mu, c0, o);
+            R.po[PA][0] = at(g, a.bf, cx, o);
+            R.po[PA][1] = at(g, a.uxp, cx, o);
+            R.po[PA][2] = at(g, a.uzp, cx, o);
         }
-        if (ix < xs - 2 * NO) return;  // block-uniform: no LDS traffic yet
+        const float rMa = frcp(l1 + 2.0f * m1), rMm = frcp(m1);
+        {
+            const float ha = frcp(R.rMa_prev + rMa), hm = frcp(R.rMm_prev + rMm);
+            float s0 = 0.0f, s1 = 0.0f;
+#pragma unroll
+            for (int j = 1; j <= NO; ++j) {
+                s0 += a.k.ai[j - 1] * (R.U[0][sl(PH, NO - 1 + j)] - R.U[0][sl(PH, NO - j)]);
+                s1 += a.k.ai[j - 1] * (R.U[1][sl(PH, NO - 1 + j)] - R.U[1][sl(PH, NO - j)]);
+            }
+            R.FX[0][sl(PH, NO - 1)] = ha * s0;
+            R.FX[1][sl(PH, NO - 1)] = hm * s1;
+        }
+        R.rMa_prev = rMa;
+        R.rMm_prev = rMm;
+        S.uA[PA][0][PAD + lane] = R.U[0][sl(PH, NO)];
+        S.uA[PA][1][PAD + lane] = R.U[1][sl(PH, NO)];
+        S.uB[PA][0][PAD + lane] = R.U[0][sl(PH, 0)];
+        S.uB[PA][1][PAD + lane] = R.U[1][sl(PH, 0)];
+        S.rM[PA][0][PAD + lane] = frcp(l0 + 2.0f * m0);
+        S.rM[PA][1][PAD + lane] = frcp(m0);
+        // B.
         __syncthreads();
-        if (active) {
-            // Vx at column ix+NO
+        // C. Vx at ix+NO; z-fluxes and Vz at ix
+        {
             float dzz = 0.0f, dzx = 0.0f;
 #pragma unroll
             for (int j = 1; j <= NO; ++j) {
@@ -682,30 +715,30 @@ struct IntKernel2 {
             }
             R.VX[0][sl(PH, NO)] = l1 * dzz;
             R.VX[1][sl(PH, NO)] = m1 * dzx;
-            if (ix >= xs) {
-                float sfx = 0.0f, sfz = 0.0f;
-#pragma unroll
-                for (int j = 1; j <= NO; ++j) {
-                    sfx += a.k.ai[j - 1] * (S.uB[PA][0][PAD + lane + j] - S.uB[PA][0][PAD + lane + 1 - j]);
-                    sfz += a.k.ai[j - 1] * (S.uB[PA][1][PAD + lane + j] - S.uB[PA][1][PAD + lane + 1 - j]);
-                }
-                float rMa0 = S.rM[PA][0][PAD + lane], rMm0 = S.rM[PA][1][PAD + lane];
-                float fzx = sfx * frcp(rMm0 + S.rM[PA][1][PAD + lane + 1]);
-                float fzz = sfz * frcp(rMa0 + S.rM[PA][0][PAD + lane + 1]);
-                float dxz = 0.0f, dxx = 0.0f;
-#pragma unroll
-                for (int k = 1; k <= NO; ++k) {
-                    dxz += a.k.bi[k - 1] * (R.U[1][sl(PH, k)] - R.U[1][sl(PH, -k)]);
-                    dxx += a.k.bi[k - 1] * (R.U[0][sl(PH, k)] - R.U[0][sl(PH, -k)]);
-                }
-                S.fz[PA][0][PAD + lane] = fzx;
-                S.fz[PA][1][PAD + lane] = fzz;
-                S.vz[PA][0][PAD + lane] = m0 * dxz;
-                S.vz[PA][1][PAD + lane] = l0 * dxx;
-            }
         }
-        // outputs of column ix-1 (its z-fluxes were staged in the previous step)
-        if (outrow && ix - 1 >= xs && ix - 1 < xe) {
+        {
+            float sfx = 0.0f, sfz = 0.0f;
+#pragma unroll
+            for (int j = 1; j <= NO; ++j) {
+                sfx += a.k.ai[j - 1] * (S.uB[PA][0][PAD + lane + j] - S.uB[PA][0][PAD + lane + 1 - j]);
+                sfz += a.k.ai[j - 1] * (S.uB[PA][1][PAD + lane + j] - S.uB[PA][1][PAD + lane + 1 - j]);
+            }
+            const float rMa0 = S.rM[PA][0][PAD + lane], rMm0 = S.rM[PA][1][PAD + lane];
+            const float fzx = sfx * frcp(rMm0 + S.rM[PA][1][PAD + lane + 1]);
+            const float fzz = sfz * frcp(rMa0 + S.rM[PA][0][PAD + lane + 1]);
+            float dxz = 0.0f, dxx = 0.0f;
+#pragma unroll
+            for (int k = 1; k <= NO; ++k) {
+                dxz += a.k.bi[k - 1] * (R.U[1][sl(PH, k)] - R.U[1][sl(PH, -k)]);
+                dxx += a.k.bi[k - 1] * (R.U[0][sl(PH, k)] - R.U[0][sl(PH, -k)]);
+            }
+            S.fz[PA][0][PAD + lane] = fzx;
+            S.fz[PA][1][PAD + lane] = fzz;
+            S.vz[PA][0][PAD + lane] = m0 * dxz;
+            S.vz[PA][1][PAD + lane] = l0 * dxx;
+        }
+        // D. update of column ix-1 (its z-terms were staged in the previous step)
+        {
             float Xx = 0.0f, Xz = 0.0f, Zx = 0.0f, Zz = 0.0f, Mx = 0.0f, Mz = 0.0f;
 #pragma unroll
             for (int k = 1; k <= NO; ++k) {
@@ -718,10 +751,13 @@ struct IntKernel2 {
                 Mz += a.k.bo[k - 1] * ((R.VX[1][sl(PH, k - 1)] - R.VX[1][sl(PH, -k - 1)]) +
                                       (S.vz[PB][1][PAD + lane + k] - S.vz[PB][1][PAD + lane - k]));
             }
-            size_t c = at(ix - 1);
-            float bf = R.po[PB][0];
-            a.uxp[c] = (2.0f * R.U[0][sl(PH, -1)] - R.po[PB][1]) + bf * ((Xx + Zx) + Mx);
-            a.uzp[c] = (2.0f * R.U[1][sl(PH, -1)] - R.po[PB][2]) + bf * ((Xz + Zz) + Mz);
+            const float bf = R.po[PB][0];
+            const float nx_ = (2.0f * R.U[0][sl(PH, -1)] - R.po[PB][1]) + bf * ((Xx + Zx) + Mx);
+            const float nz_ = (2.0f * R.U[1][sl(PH, -1)] - R.po[PB][2]) + bf * ((Xz + Zz) + Mz);
+            if (L.outrow && ix - 1 >= xs && ix - 1 < xe) {
+                put(g, a.uxp, ix - 1, o, nx_);
+                put(g, a.uzp, ix - 1, o, nz_);
+            }
         }
     }
 
@@ -729,8 +765,9 @@ struct IntKernel2 {
     {
         __shared__ Smem S;
         const Geo &g = a.g;
-        const int lane = threadIdx.x;
-        for (int i = lane; i < LW; i += NT)
+        Lane L;
+        L.lane = threadIdx.x;
+        for (int i = L.lane; i < LW; i += NT)
             if (i < PAD || i >= NT + PAD)
                 for (int q = 0; q < 2; ++q)
                     for (int c = 0; c < 2; ++c) {
@@ -742,9 +779,10 @@ struct IntKernel2 {
         const int nbz = a.nbz;
         const int bzid = bid % nbz, bxid = bid / nbz;
         const int z0 = g.iz0 + bzid * TZ;
-        const int r = z0 - HZ + lane;
-        const bool rowok = (r >= g.iz0 - HZ) && (r <= g.iz1 + HZ);
-        const bool outrow = (lane >= HZ) && (lane < HZ + TZ) && (r <= g.iz1);
+        L.r = z0 - HZ + L.lane;
+        const int rc = L.r < 1 ? 1 : (L.r > g.NZ ? g.NZ : L.r);
+        L.ofs = (unsigned)(rc - 1) * 4u;
+        L.outrow = (L.lane >= HZ) && (L.lane < HZ + TZ) && (L.r <= g.iz1);
         const int xs = g.ix0 + bxid * a.lx;
         const int xe = min(xs + a.lx, g.ix1 + 1);
         // blocks whose loads reach the padding read the ring (block-uniform)
@@ -756,28 +794,27 @@ struct IntKernel2 {
             for (int i = 0; i < P; ++i) { R.U[q][i] = 0.0f; R.FX[q][i] = 0.0f; R.VX[q][i] = 0.0f; }
         R.rMa_prev = 1.0f; R.rMm_prev = 1.0f;
         const int x0 = xs - WARM;
-        // prefetch for the first step
+        // prologue: the loads the first step consumes
+        {
+            const int lo = 1 - g.npml;
+            const unsigned o = L.ofs;
+            const int cn = x0 + 2 * NO - 1;
+            const int cm = x0 + NO < lo ? lo : x0 + NO, c0 = x0 < lo ? lo : x0;
+            R.pu[0][0] = at(g, a.ux, cn, o); R.pu[0][1] = at(g, a.uz, cn, o);
+            if (bc && outside(g, L.r, cn)) {
+                R.pu[0][0] += at(g, a.ringx, cn, o);
+                R.pu[0][1] += at(g, a.ringz, cn, o);
+            }
+            R.pl[0][0] = at(g, a.lam, cm, o); R.pl[0][1] = at(g, a.mu, cm, o);
+            R.p0[0][0] = at(g, a.lam, c0, o); R.p0[0][1] = at(g, a.mu, c0, o);
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            R.pu[0][q] = 0.0f; R.pu[1][q] = 0.0f; R.pl[0][q] = 1.0f; R.pl[1][q] = 1.0f;
-            R.p0[0][q] = 1.0f; R.p0[1][q] = 1.0f;
-        }
+            for (int q = 0; q < 2; ++q) { R.pu[1][q] = 0.0f; R.pl[1][q] = 1.0f; R.p0[1][q] = 1.0f; }
 #pragma unroll
-        for (int q = 0; q < 3; ++q) { R.po[0][q] = 0.0f; R.po[1][q] = 0.0f; }
-        if (rowok) {
-            size_t rowoff = (size_t)(r - 1);
-            auto at = [&](int c) { return (size_t)(c + g.npml - 1) * (size_t)g.ld + rowoff; };
-            const size_t cn = at(x0 + 2 * NO - 1);
-            R.pu[0][0] = a.ux[cn]; R.pu[0][1] = a.uz[cn];
-            if (bc && outside(g, r, x0 + 2 * NO - 1)) { R.pu[0][0] += a.ringx[cn]; R.pu[0][1] += a.ringz[cn]; }
-            const int xm = x0 < 1 - g.npml ? 1 - g.npml : x0;
-            const int xl = x0 + NO < 1 - g.npml ? 1 - g.npml : x0 + NO;
-            R.pl[0][0] = a.lam[at(xl)]; R.pl[0][1] = a.mu[at(xl)];
-            R.p0[0][0] = a.lam[at(xm)]; R.p0[0][1] = a.mu[at(xm)];
+            for (int q = 0; q < 3; ++q) { R.po[0][q] = 0.0f; R.po[1][q] = 0.0f; }
         }
         __syncthreads();
         for (int base = x0; base <= xe; base += P) {
-#define IK2_STEP(PH) if constexpr (P > PH) { if (base + PH <= xe) step<PH>(a, S, R, lane, r, rowok, outrow, bc, base + PH, xs, xe); }
+#define IK2_STEP(PH) if constexpr (P > PH) { if (base + PH <= xe) step<PH>(a, S, R, L, bc, base + PH, xs, xe); }
             IK2_STEP(0) IK2_STEP(1) IK2_STEP(2) IK2_STEP(3) IK2_STEP(4) IK2_STEP(5) IK2_STEP(6)
             IK2_STEP(7) IK2_STEP(8) IK2_STEP(9) IK2_STEP(10) IK2_STEP(11) IK2_STEP(12) IK2_STEP(13)
 #undef IK2_STEP
@@ -786,7 +823,7 @@ struct IntKernel2 {
 };
 
 template <int NO, int NT>
-__global__ __launch_bounds__(NT) void k_interior2(IntArgs<NO> a)
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_interior2(IntArgs<NO> a)
 {
     IntKernel2<NO, NT>::run(a, blockIdx.x);
 }
