@@ -91,7 +91,7 @@ struct fdfk_solver {
     float *stage = nullptr;   // (2, NZ, NX) staging for reference-layout copies
     // interior kernel
     int lx = 45, nt_threads = 256;
-    int flx_side = 8, flx_bottom = 8;   // k_frame_out columns per block (side strips / bottom strip)
+    int flx_side = 6, flx_bottom = 6;   // k_frame_out columns per block (side strips / bottom strip)
     // profiling
     bool prof = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_int, ev_step;
@@ -308,12 +308,25 @@ void launch_interior(fdfk_solver *s)
     }
 }
 
-void add_rect(Rects &r, int x0, int x1, int z0, int z1)
+// one edge band cut into tiles of at most kEdgeThreads cells whose staged
+// halo fits the kernel's LDS arrays: thin row bands take all their rows per
+// tile, thin column bands all their columns
+void add_band(ETiles &t, int x0, int x1, int z0, int z1, int norder)
 {
     if (x1 < x0 || z1 < z0) return;
-    int q = r.n++;
-    r.x0[q] = x0; r.x1[q] = x1; r.z0[q] = z0; r.z1[q] = z1;
-    r.off[q + 1] = r.off[q] + (long)(x1 - x0 + 1) * (z1 - z0 + 1);
+    const int H = 2 * norder - 1;
+    const int cols = x1 - x0 + 1, rows = z1 - z0 + 1;
+    int tx, tz;
+    if (rows <= cols) { tz = std::min(rows, kEdgeThreads); tx = std::max(1, kEdgeThreads / tz); }
+    else { tx = std::min(cols, kEdgeThreads); tz = std::max(1, kEdgeThreads / tx); }
+    while ((tx + 2 * H) * (tz + 2 * H) > kEdgeLds) {   // staged halo is unclipped
+        if (tx >= tz) --tx; else --tz;
+    }
+    int q = t.n++;
+    t.x0[q] = x0; t.x1[q] = x1; t.z0[q] = z0; t.z1[q] = z1; t.tx[q] = tx; t.tz[q] = tz;
+    t.nbx[q] = (cols + tx - 1) / tx;
+    t.nbz[q] = (rows + tz - 1) / tz;
+    t.off[q + 1] = t.off[q] + t.nbx[q] * t.nbz[q];
 }
 
 // Full-order rectangle: columns xf0..xf1, rows iz0..zf1.  Its part inside
@@ -384,17 +397,17 @@ void launch_frame_out(fdfk_solver *s)
 
 // reduced-order edge bands: top rows (free surface), bottom rows and the
 // columns next to the padded-grid boundary
-Rects edge_rects(const Geo &g)
+ETiles edge_tiles(const Geo &g)
 {
     int xf0, xf1, zf1;
     full_order_box(g, xf0, xf1, zf1);
     int xa = 2 - g.npml, xb = g.nx + g.npml - 1;
-    Rects r{};
-    add_rect(r, xa, xb, 1, g.iz0 - 1);
-    add_rect(r, xa, xb, zf1 + 1, g.nz + g.npml - 1);
-    add_rect(r, xa, xf0 - 1, g.iz0, zf1);
-    add_rect(r, xf1 + 1, xb, g.iz0, zf1);
-    return r;
+    ETiles t{};
+    add_band(t, xa, xb, 1, g.iz0 - 1, g.norder);
+    add_band(t, xa, xb, zf1 + 1, g.nz + g.npml - 1, g.norder);
+    add_band(t, xa, xf0 - 1, g.iz0, zf1, g.norder);
+    add_band(t, xf1 + 1, xb, g.iz0, zf1, g.norder);
+    return t;
 }
 
 int one_step(fdfk_solver *s, bool record)
@@ -430,9 +443,9 @@ int one_step(fdfk_solver *s, bool record)
     launch_interior(s);
     if (s->prof) hipEventRecord(e2, s->stream);
     launch_frame_out(s);
-    Rects re = edge_rects(g);
+    ETiles re = edge_tiles(g);
     if (re.off[re.n] > 0) {
-        dim3 eg((unsigned)((re.off[re.n] + kEdgeCells - 1) / kEdgeCells)), eb(kEdgeThreads);
+        dim3 eg((unsigned)re.off[re.n]), eb(kEdgeThreads);
         switch (g.norder) {
         case 1: hipLaunchKernelGGL(k_frame_edge<1>, eg, eb, 0, s->stream, a, re); break;
         case 2: hipLaunchKernelGGL(k_frame_edge<2>, eg, eb, 0, s->stream, a, re); break;

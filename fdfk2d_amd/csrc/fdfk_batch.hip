@@ -21,7 +21,7 @@ struct BatchJob {
     StepArgs sa;
     IntArgs<NO> ia;
     FrameArgs<NO> fa;
-    Rects re;
+    ETiles re;
     RecArgs ra;
     const Background *bg;
     float *ringx, *ringz;
@@ -132,7 +132,7 @@ void fill_job(fdfk_solver *s, BatchJob<NO> &b, int nb[KB_N], bool &record)
     b.ringz = s->ringz;
     b.ia = interior_args<NO, 256>(s);
     b.fa = frame_args<NO>(s);
-    b.re = edge_rects(g);
+    b.re = edge_tiles(g);
     record = (s->isis == s->it) && (s->istp <= s->nsis) && s->nrcv > 0;
     RecArgs &r = b.ra;
     r = RecArgs{};
@@ -145,7 +145,7 @@ void fill_job(fdfk_solver *s, BatchJob<NO> &b, int nb[KB_N], bool &record)
     nb[KB_PSI] = (int)((npsi + 255) / 256);
     nb[KB_INT] = b.ia.nbx * b.ia.nbz;
     nb[KB_FOUT] = b.fa.t.off[b.fa.t.n];
-    nb[KB_FEDGE] = (int)((b.re.off[b.re.n] + kEdgeCells - 1) / kEdgeCells);
+    nb[KB_FEDGE] = b.re.off[b.re.n];
     nb[KB_REC] = record ? s->nrcv : 0;
 }
 

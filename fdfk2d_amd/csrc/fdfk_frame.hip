@@ -20,25 +20,27 @@ template <int NO, int NT>
 struct OutKernel {
     static constexpr int HZ = 2 * NO - 1;   // z halo lanes on each side
     static constexpr int TZ = NT - 2 * HZ;  // output rows per block
-    static constexpr int P = 3 * NO;        // register-ring period
+    static constexpr int P0 = 3 * NO + 1;
+    static constexpr int P = P0 + (P0 & 1); // even ring period: LDS parity is compile-time
     static constexpr int PAD = NO + 1;      // LDS guard entries on each side
     static constexpr int WARM = 4 * NO - 2; // warm-up columns before the first output
     static constexpr int LW = NT + 2 * PAD;
 
-    struct Smem {
-        float uA[2][LW];   // u at column ix+NO
-        float uB[2][LW];   // u at column ix
-        float rM[2][LW];   // 1/(l+2m), 1/m at column ix
-        float fz[2][LW];   // z-fluxes at column ix
-        float vz[2][LW];   // m*Dx uz, l*Dx ux at column ix
-        float pz[2][LW];   // m*psi_z(ux), (l+2m)*psi_z(uz) at column ix (bottom strip, else 0)
+    struct Smem {                 // [parity][comp][lane]
+        float uA[2][2][LW];       // u at column ix+NO
+        float uB[2][2][LW];       // u at column ix
+        float rM[2][2][LW];       // 1/(l+2m), 1/m at column ix
+        float fz[2][2][LW];       // z-fluxes at column ix
+        float vz[2][2][LW];       // m*Dx uz, l*Dx ux at column ix
+        float pz[2][2][LW];       // m*psi_z(ux), (l+2m)*psi_z(uz) at column ix (bottom strip, else 0)
     };
 
     // per-lane constants: row, masks, and the lane's strip (the side strip
     // above the bottom strip, the bottom strip below it) with its arrays
     struct Lane {
         int lane, r;
-        bool rowok, outrow, isB;
+        unsigned ofs;    // clamped row byte offset inside a grid column
+        bool outrow, isB;
         float az;
         int cb;          // first column of the lane's strip
         int stride;      // strip column pitch (nz or npml)
@@ -51,161 +53,187 @@ struct OutKernel {
         }
     };
 
+    struct Regs {
+        float U[2][P], FX[2][P], VX[2][P], PX[2][P];
+        float rMa_prev, rMm_prev;
+        float pu[2][2];   // [parity][comp] next new u column (ix+2NO-1 of the next step)
+        float pl[2][2];   // [parity][l,m] next lead materials (ix+NO)
+        float p0[2][2];   // [parity][l,m] next column materials (ix)
+        float pq[2][2];   // [parity][comp] next psi_x at (r, ix+NO)
+        float pr[2][2];   // [parity][comp] next psi_z at (r, ix)
+        float ps[2][2];   // [parity][comp] z memory sums of column ix (used one step later)
+    };
+
     static __device__ __forceinline__ int sl(int ph, int j) { return ((ph + j) % P + P) % P; }
 
-    template <int PH>
-    static __device__ __forceinline__ void step(const FrameArgs<NO> &a, Smem &S, const Lane &L, int ix, int xs,
-                                                int xe, float (&U)[2][P], float (&FX)[2][P], float (&VX)[2][P],
-                                                float (&PX)[2][P], float &rMa_prev, float &rMm_prev)
+    static __device__ __forceinline__ __amdgpu_buffer_rsrc_t colrsrc(const Geo &g, const float *base, int c)
     {
-        if (ix >= xe) return;  // block-uniform
+        const float *cb = base + (size_t)(c + g.npml - 1) * (size_t)g.ld;
+        return __builtin_amdgcn_make_buffer_rsrc((void *)cb, 0, g.ld * 4, kBufferFlags);
+    }
+    static __device__ __forceinline__ float at(const Geo &g, const float *base, int c, unsigned o)
+    {
+        return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(colrsrc(g, base, c), o, 0, 0));
+    }
+    static __device__ __forceinline__ void put(const Geo &g, float *base, int c, unsigned o, float v)
+    {
+        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), colrsrc(g, base, c), o, 0, 0);
+    }
+
+    // the loads one step consumes, issued one step ahead (column ix)
+    static __device__ __forceinline__ void prefetch(const FrameArgs<NO> &a, const Lane &L, Regs &R, int q, int ix,
+                                                    int xs, int xe)
+    {
         const Geo &g = a.g;
-        const int lane = L.lane, r = L.r;
-        const size_t rowoff = (size_t)(r - 1);
-        auto at = [&](int c) { return (size_t)(c + g.npml - 1) * (size_t)g.ld + rowoff; };
-        // material columns of the first warm-up steps may lie left of the
-        // grid; their values are never used, so the address is clamped
-        auto atm = [&](int c) { return at(c < 1 - g.npml ? 1 - g.npml : c); };
-        auto inside = [&](int c) { return c >= 1 && c <= g.nx && r <= g.nz; };
-        const int cnew = ix + 2 * NO - 1, cmx = ix + NO;
-        const bool warm = ix < xs - 2 * NO;   // block-uniform
-        const bool outc = ix >= xs;           // block-uniform
-        // ---- every global load of this step, issued together
-        float nx_ = 0.0f, nz_ = 0.0f, l1 = 1.0f, m1 = 1.0f, l0 = 1.0f, m0 = 1.0f;
-        float q0 = 0.0f, q1 = 0.0f, z0 = 0.0f, z1 = 0.0f, bf = 0.0f, ax = 0.0f;
-        float sc[2][3] = {}, sp[2][3] = {};
-        if (L.rowok) {
-            const size_t cn = at(cnew), cm = atm(cmx);
-            nx_ = a.ux[cn]; nz_ = a.uz[cn];
-            if (a.corr && inside(cnew)) { nx_ -= a.ringx[cn]; nz_ -= a.ringz[cn]; }
-            l1 = a.lam[cm]; m1 = a.mu[cm];
-            if (!warm) {
-                const size_t c0 = atm(ix);
-                l0 = a.lam[c0]; m0 = a.mu[c0];
-            }
-            if (outc && L.isB) {
-                const size_t o = L.off(ix);
-                z0 = L.psz[0][o]; z1 = L.psz[1][o];
-            }
-        }
-        if (!warm && L.outrow && !inside(cmx)) {
-            const size_t o = L.off(cmx);
+        const int lo = 1 - g.npml, hi = g.nx + g.npml;
+        // columns of the first warm-up and the last (unused) prefetch may
+        // lie beyond the grid; their values are never used, so the grid
+        // addresses are clamped and the strip reads skipped
+        auto cc = [&](int c) { return c < lo ? lo : (c > hi ? hi : c); };
+        auto inside = [&](int c) { return c >= 1 && c <= g.nx && L.r <= g.nz; };
+        const int cn = cc(ix + 2 * NO - 1), cm = ix + NO;
+        float vx = at(g, a.ux, cn, L.ofs), vz = at(g, a.uz, cn, L.ofs);
+        if (a.corr && inside(cn)) { vx -= at(g, a.ringx, cn, L.ofs); vz -= at(g, a.ringz, cn, L.ofs); }
+        R.pu[q][0] = vx; R.pu[q][1] = vz;
+        R.pl[q][0] = at(g, a.lam, cc(cm), L.ofs); R.pl[q][1] = at(g, a.mu, cc(cm), L.ofs);
+        R.p0[q][0] = at(g, a.lam, cc(ix), L.ofs); R.p0[q][1] = at(g, a.mu, cc(ix), L.ofs);
+        float q0 = 0.0f, q1 = 0.0f, z0 = 0.0f, z1 = 0.0f;
+        if (L.outrow && cm >= xs - NO && cm < xe + NO && !inside(cm)) {
+            const size_t o = L.off(cm);
             q0 = L.psx[0][o]; q1 = L.psx[1][o];
         }
-        if (outc && L.outrow) {
+        if (L.isB && ix >= xs && ix < xe) {
             const size_t o = L.off(ix);
-            bf = a.bf[at(ix)];
-            ax = a.st->ax[ix + g.npml - 1];
+            z0 = L.psz[0][o]; z1 = L.psz[1][o];
+        }
+        R.pq[q][0] = q0; R.pq[q][1] = q1;
+        R.pr[q][0] = z0; R.pr[q][1] = z1;
+    }
+
+    template <int PH>
+    static __device__ __forceinline__ void step(const FrameArgs<NO> &a, Smem &S, Regs &R, const Lane &L, int ix,
+                                                int xs, int xe)
+    {
+        constexpr int PA = PH & 1, PB = 1 - PA;
+        const Geo &g = a.g;
+        const int lane = L.lane;
+        // A. consume this step's prefetch, prefetch the next step, load the
+        //    update inputs of the output column ix-1
+        R.U[0][sl(PH, 2 * NO - 1)] = R.pu[PA][0];
+        R.U[1][sl(PH, 2 * NO - 1)] = R.pu[PA][1];
+        const float l1 = R.pl[PA][0], m1 = R.pl[PA][1];
+        const float l0 = R.p0[PA][0], m0 = R.p0[PA][1];
+        const float q0 = R.pq[PA][0], q1 = R.pq[PA][1];
+        const float zr0 = R.pr[PA][0], zr1 = R.pr[PA][1];
+        prefetch(a, L, R, PB, ix + 1, xs, xe);
+        const int co = ix - 1;
+        const bool out = L.outrow && co >= xs && co < xe;
+        float bf = 0.0f, ax = 0.0f, sc[2][3] = {}, sp[2][3] = {};
+        if (out) {
+            const size_t o = L.off(co);
+            bf = at(g, a.bf, co, L.ofs);
+            ax = a.st->ax[co + g.npml - 1];
 #pragma unroll
             for (int c = 0; c < 2; ++c)
 #pragma unroll
                 for (int p = 0; p < 3; ++p) { sc[c][p] = L.cur[c][p][o]; sp[c][p] = L.prev[c][p][o]; }
         }
-        // ---- 1. new u column (scattered view) into the ring, x-fluxes
-        U[0][sl(PH, 2 * NO - 1)] = nx_;
-        U[1][sl(PH, 2 * NO - 1)] = nz_;
-        const float rMa = 1.0f / (l1 + 2.0f * m1), rMm = 1.0f / m1;
-        if (!warm) {
-            float ha = 1.0f / (rMa_prev + rMa), hm = 1.0f / (rMm_prev + rMm);
+        // x-flux at the face (ix+NO-1, ix+NO), M psi_x at ix+NO, LDS staging
+        const float rMa = frcp(l1 + 2.0f * m1), rMm = frcp(m1);
+        {
+            const float ha = frcp(R.rMa_prev + rMa), hm = frcp(R.rMm_prev + rMm);
             float s0 = 0.0f, s1 = 0.0f;
 #pragma unroll
             for (int j = 1; j <= NO; ++j) {
-                s0 += a.k.ai[j - 1] * (U[0][sl(PH, NO - 1 + j)] - U[0][sl(PH, NO - j)]);
-                s1 += a.k.ai[j - 1] * (U[1][sl(PH, NO - 1 + j)] - U[1][sl(PH, NO - j)]);
+                s0 += a.k.ai[j - 1] * (R.U[0][sl(PH, NO - 1 + j)] - R.U[0][sl(PH, NO - j)]);
+                s1 += a.k.ai[j - 1] * (R.U[1][sl(PH, NO - 1 + j)] - R.U[1][sl(PH, NO - j)]);
             }
-            FX[0][sl(PH, NO - 1)] = ha * s0;
-            FX[1][sl(PH, NO - 1)] = hm * s1;
-            // M * psi_x at column ix+NO for this lane's own row
-            PX[0][sl(PH, NO)] = (l1 + 2.0f * m1) * q0;
-            PX[1][sl(PH, NO)] = m1 * q1;
-            S.uA[0][PAD + lane] = U[0][sl(PH, NO)];
-            S.uA[1][PAD + lane] = U[1][sl(PH, NO)];
-            S.uB[0][PAD + lane] = U[0][sl(PH, 0)];
-            S.uB[1][PAD + lane] = U[1][sl(PH, 0)];
+            R.FX[0][sl(PH, NO - 1)] = ha * s0;
+            R.FX[1][sl(PH, NO - 1)] = hm * s1;
         }
-        rMa_prev = rMa;
-        rMm_prev = rMm;
-        if (warm) return;
-        // ---- 2. 1/M and M * psi_z at column ix
-        const float rMa0 = 1.0f / (l0 + 2.0f * m0), rMm0 = 1.0f / m0;
-        S.rM[0][PAD + lane] = rMa0;
-        S.rM[1][PAD + lane] = rMm0;
-        S.pz[0][PAD + lane] = m0 * z0;
-        S.pz[1][PAD + lane] = (l0 + 2.0f * m0) * z1;
-        __syncthreads();
-        // ---- 3. Vx at column ix+NO: l*Dz uz and m*Dz ux
+        R.rMa_prev = rMa;
+        R.rMm_prev = rMm;
+        R.PX[0][sl(PH, NO)] = (l1 + 2.0f * m1) * q0;
+        R.PX[1][sl(PH, NO)] = m1 * q1;
+        S.uA[PA][0][PAD + lane] = R.U[0][sl(PH, NO)];
+        S.uA[PA][1][PAD + lane] = R.U[1][sl(PH, NO)];
+        S.uB[PA][0][PAD + lane] = R.U[0][sl(PH, 0)];
+        S.uB[PA][1][PAD + lane] = R.U[1][sl(PH, 0)];
+        S.rM[PA][0][PAD + lane] = frcp(l0 + 2.0f * m0);
+        S.rM[PA][1][PAD + lane] = frcp(m0);
+        S.pz[PA][0][PAD + lane] = m0 * zr0;
+        S.pz[PA][1][PAD + lane] = (l0 + 2.0f * m0) * zr1;
+        // B.
+        lds_barrier();
+        // C. Vx at ix+NO; z-fluxes, Vz and the z memory sums at ix
         {
             float dzz = 0.0f, dzx = 0.0f;
 #pragma unroll
             for (int j = 1; j <= NO; ++j) {
-                dzz += a.k.bi[j - 1] * (S.uA[1][PAD + lane + j] - S.uA[1][PAD + lane - j]);
-                dzx += a.k.bi[j - 1] * (S.uA[0][PAD + lane + j] - S.uA[0][PAD + lane - j]);
+                dzz += a.k.bi[j - 1] * (S.uA[PA][1][PAD + lane + j] - S.uA[PA][1][PAD + lane - j]);
+                dzx += a.k.bi[j - 1] * (S.uA[PA][0][PAD + lane + j] - S.uA[PA][0][PAD + lane - j]);
             }
-            VX[0][sl(PH, NO)] = l1 * dzz;
-            VX[1][sl(PH, NO)] = m1 * dzx;
+            R.VX[0][sl(PH, NO)] = l1 * dzz;
+            R.VX[1][sl(PH, NO)] = m1 * dzx;
         }
-        if (!outc) {
-            __syncthreads();  // keep the barrier cadence uniform
-            return;
-        }
-        // ---- 4. z-fluxes at (r+1/2, ix), Vz at (r, ix), z memory sums at (r, ix)
-        float sfx = 0.0f, sfz = 0.0f, pzx = 0.0f, pzz = 0.0f;
+        {
+            float sfx = 0.0f, sfz = 0.0f, pzx = 0.0f, pzz = 0.0f;
 #pragma unroll
-        for (int j = 1; j <= NO; ++j) {
-            sfx += a.k.ai[j - 1] * (S.uB[0][PAD + lane + j] - S.uB[0][PAD + lane + 1 - j]);
-            sfz += a.k.ai[j - 1] * (S.uB[1][PAD + lane + j] - S.uB[1][PAD + lane + 1 - j]);
-            pzx += a.k.bi[j - 1] * (S.pz[0][PAD + lane + j] - S.pz[0][PAD + lane - j]);
-            pzz += a.k.bi[j - 1] * (S.pz[1][PAD + lane + j] - S.pz[1][PAD + lane - j]);
-        }
-        const float fzx = sfx / (rMm0 + S.rM[1][PAD + lane + 1]);
-        const float fzz = sfz / (rMa0 + S.rM[0][PAD + lane + 1]);
-        float dxz = 0.0f, dxx = 0.0f;
+            for (int j = 1; j <= NO; ++j) {
+                sfx += a.k.ai[j - 1] * (S.uB[PA][0][PAD + lane + j] - S.uB[PA][0][PAD + lane + 1 - j]);
+                sfz += a.k.ai[j - 1] * (S.uB[PA][1][PAD + lane + j] - S.uB[PA][1][PAD + lane + 1 - j]);
+                pzx += a.k.bi[j - 1] * (S.pz[PA][0][PAD + lane + j] - S.pz[PA][0][PAD + lane - j]);
+                pzz += a.k.bi[j - 1] * (S.pz[PA][1][PAD + lane + j] - S.pz[PA][1][PAD + lane - j]);
+            }
+            const float rMa0 = S.rM[PA][0][PAD + lane], rMm0 = S.rM[PA][1][PAD + lane];
+            const float fzx = sfx * frcp(rMm0 + S.rM[PA][1][PAD + lane + 1]);
+            const float fzz = sfz * frcp(rMa0 + S.rM[PA][0][PAD + lane + 1]);
+            float dxz = 0.0f, dxx = 0.0f;
 #pragma unroll
-        for (int k = 1; k <= NO; ++k) {
-            dxz += a.k.bi[k - 1] * (U[1][sl(PH, k)] - U[1][sl(PH, -k)]);
-            dxx += a.k.bi[k - 1] * (U[0][sl(PH, k)] - U[0][sl(PH, -k)]);
+            for (int k = 1; k <= NO; ++k) {
+                dxz += a.k.bi[k - 1] * (R.U[1][sl(PH, k)] - R.U[1][sl(PH, -k)]);
+                dxx += a.k.bi[k - 1] * (R.U[0][sl(PH, k)] - R.U[0][sl(PH, -k)]);
+            }
+            S.fz[PA][0][PAD + lane] = fzx;
+            S.fz[PA][1][PAD + lane] = fzz;
+            S.vz[PA][0][PAD + lane] = m0 * dxz;
+            S.vz[PA][1][PAD + lane] = l0 * dxx;
+            R.ps[PA][0] = pzx;
+            R.ps[PA][1] = pzz;
         }
-        __syncthreads();  // all reads of uA/uB/rM/pz for this column are done
-        S.fz[0][PAD + lane] = fzx;
-        S.fz[1][PAD + lane] = fzz;
-        S.vz[0][PAD + lane] = m0 * dxz;
-        S.vz[1][PAD + lane] = l0 * dxx;
-        __syncthreads();
-        if (!L.outrow) return;
-        // ---- 5. the six parts, memory corrections, split update
+        // D. column ix-1: six parts, memory corrections, split update
+        if (!out) return;
         float Xx = 0.0f, Xz = 0.0f, Zx = 0.0f, Zz = 0.0f, Mx = 0.0f, Mz = 0.0f, sx0 = 0.0f, sx1 = 0.0f;
 #pragma unroll
         for (int k = 1; k <= NO; ++k) {
-            Xx += a.k.aox[k - 1] * (FX[0][sl(PH, k - 1)] - FX[0][sl(PH, -k)]);
-            Xz += a.k.aox[k - 1] * (FX[1][sl(PH, k - 1)] - FX[1][sl(PH, -k)]);
-            Zx += a.k.aoz[k - 1] * (S.fz[0][PAD + lane + k - 1] - S.fz[0][PAD + lane - k]);
-            Zz += a.k.aoz[k - 1] * (S.fz[1][PAD + lane + k - 1] - S.fz[1][PAD + lane - k]);
-            Mx += a.k.bo[k - 1] * ((VX[0][sl(PH, k)] - VX[0][sl(PH, -k)]) +
-                                  (S.vz[0][PAD + lane + k] - S.vz[0][PAD + lane - k]));
-            Mz += a.k.bo[k - 1] * ((VX[1][sl(PH, k)] - VX[1][sl(PH, -k)]) +
-                                  (S.vz[1][PAD + lane + k] - S.vz[1][PAD + lane - k]));
-            sx0 += a.k.bi[k - 1] * (PX[0][sl(PH, k)] - PX[0][sl(PH, -k)]);
-            sx1 += a.k.bi[k - 1] * (PX[1][sl(PH, k)] - PX[1][sl(PH, -k)]);
+            Xx += a.k.aox[k - 1] * (R.FX[0][sl(PH, k - 2)] - R.FX[0][sl(PH, -k - 1)]);
+            Xz += a.k.aox[k - 1] * (R.FX[1][sl(PH, k - 2)] - R.FX[1][sl(PH, -k - 1)]);
+            Zx += a.k.aoz[k - 1] * (S.fz[PB][0][PAD + lane + k - 1] - S.fz[PB][0][PAD + lane - k]);
+            Zz += a.k.aoz[k - 1] * (S.fz[PB][1][PAD + lane + k - 1] - S.fz[PB][1][PAD + lane - k]);
+            Mx += a.k.bo[k - 1] * ((R.VX[0][sl(PH, k - 1)] - R.VX[0][sl(PH, -k - 1)]) +
+                                  (S.vz[PB][0][PAD + lane + k] - S.vz[PB][0][PAD + lane - k]));
+            Mz += a.k.bo[k - 1] * ((R.VX[1][sl(PH, k - 1)] - R.VX[1][sl(PH, -k - 1)]) +
+                                  (S.vz[PB][1][PAD + lane + k] - S.vz[PB][1][PAD + lane - k]));
+            sx0 += a.k.bi[k - 1] * (R.PX[0][sl(PH, k - 1)] - R.PX[0][sl(PH, -k - 1)]);
+            sx1 += a.k.bi[k - 1] * (R.PX[1][sl(PH, k - 1)] - R.PX[1][sl(PH, -k - 1)]);
         }
         Xx = Xx - sx0 / g.dx;
         Xz = Xz - sx1 / g.dx;
         if (L.isB) {
-            Zx = Zx - pzx / g.dz;
-            Zz = Zz - pzz / g.dz;
+            Zx = Zx - R.ps[PB][0] / g.dz;
+            Zz = Zz - R.ps[PB][1] / g.dz;
         }
         const float az = L.az;
         const float c3 = 2.0f - 4.0f * ax * az;
         const float pa[2][3] = {{Xx, Zx, Mx}, {Xz, Zz, Mz}};
-        const size_t o = L.off(ix);
-        const size_t c = at(ix);
+        const size_t o = L.off(co);
 #pragma unroll
         for (int comp = 0; comp < 2; ++comp) {
             float n1 = (2.0f * sc[comp][0] - (1.0f - ax) * sp[comp][0] + bf * pa[comp][0]) / (1.0f + ax);
             float n2 = (2.0f * sc[comp][1] - (1.0f - az) * sp[comp][1] + bf * pa[comp][1]) / (1.0f + az);
             float n3 = (c3 * sc[comp][2] - (1.0f - ax - az) * sp[comp][2] + bf * pa[comp][2]) / (1.0f + ax + az);
             L.prev[comp][0][o] = n1; L.prev[comp][1][o] = n2; L.prev[comp][2][o] = n3;
-            float tot = (n1 + n2) + n3;
-            if (comp == 0) a.uxp[c] = tot; else a.uzp[c] = tot;
+            put(g, comp == 0 ? a.uxp : a.uzp, co, L.ofs, (n1 + n2) + n3);
         }
     }
 
@@ -226,15 +254,18 @@ struct OutKernel {
         L.lane = threadIdx.x;
         for (int i = L.lane; i < LW; i += NT)
             if (i < PAD || i >= NT + PAD)
-                for (int c = 0; c < 2; ++c) {
-                    S.uA[c][i] = 0.0f; S.uB[c][i] = 0.0f; S.rM[c][i] = 1.0f;
-                    S.fz[c][i] = 0.0f; S.vz[c][i] = 0.0f; S.pz[c][i] = 0.0f;
-                }
+                for (int p = 0; p < 2; ++p)
+                    for (int c = 0; c < 2; ++c) {
+                        S.uA[p][c][i] = 0.0f; S.uB[p][c][i] = 0.0f; S.rM[p][c][i] = 1.0f;
+                        S.fz[p][c][i] = 0.0f; S.vz[p][c][i] = 0.0f; S.pz[p][c][i] = 0.0f;
+                    }
         L.r = zs - HZ + L.lane;
-        L.rowok = L.r >= 1 && L.r <= g.NZ;
+        const bool rowok = L.r >= 1 && L.r <= g.NZ;
+        const int rc = L.r < 1 ? 1 : (L.r > g.NZ ? g.NZ : L.r);
+        L.ofs = (unsigned)(rc - 1) * 4u;
         L.outrow = L.lane >= HZ && L.lane < HZ + TZ && L.r <= ze;
-        L.isB = L.r > g.nz;
-        L.az = L.rowok ? a.st->az[L.r - 1] : 0.0f;
+        L.isB = rowok && L.r > g.nz;
+        L.az = rowok ? a.st->az[L.r - 1] : 0.0f;
         // the side strip of this tile's rows above the bottom strip
         const int sside = xs <= 0 ? S_L : S_R;
         const int s = L.isB ? S_B : sside;
@@ -252,17 +283,22 @@ struct OutKernel {
                 L.prev[c][p] = L.isB ? st->prev[S_B][c][p] : st->prev[sside][c][p];
             }
         }
-        float U[2][P], FX[2][P], VX[2][P], PX[2][P];
+        Regs R;
 #pragma unroll
-        for (int c = 0; c < 2; ++c)
+        for (int c = 0; c < 2; ++c) {
 #pragma unroll
-            for (int i = 0; i < P; ++i) { U[c][i] = 0.0f; FX[c][i] = 0.0f; VX[c][i] = 0.0f; PX[c][i] = 0.0f; }
-        float rMa_prev = 1.0f, rMm_prev = 1.0f;
+            for (int i = 0; i < P; ++i) { R.U[c][i] = 0.0f; R.FX[c][i] = 0.0f; R.VX[c][i] = 0.0f; R.PX[c][i] = 0.0f; }
+            R.pu[1][c] = 0.0f; R.pl[1][c] = 1.0f; R.p0[1][c] = 1.0f; R.pq[1][c] = 0.0f; R.pr[1][c] = 0.0f;
+            R.ps[0][c] = 0.0f; R.ps[1][c] = 0.0f;
+        }
+        R.rMa_prev = 1.0f; R.rMm_prev = 1.0f;
+        const int x0 = xs - WARM;
+        prefetch(a, L, R, 0, x0, xs, xe);
         __syncthreads();
-        for (int base = xs - WARM; base < xe; base += P) {
-#define OK_STEP(PH) if constexpr (P > PH) step<PH>(a, S, L, base + PH, xs, xe, U, FX, VX, PX, rMa_prev, rMm_prev);
-            OK_STEP(0) OK_STEP(1) OK_STEP(2) OK_STEP(3) OK_STEP(4) OK_STEP(5)
-            OK_STEP(6) OK_STEP(7) OK_STEP(8) OK_STEP(9) OK_STEP(10) OK_STEP(11)
+        for (int base = x0; base <= xe; base += P) {
+#define OK_STEP(PH) if constexpr (P > PH) { if (base + PH <= xe) step<PH>(a, S, R, L, base + PH, xs, xe); }
+            OK_STEP(0) OK_STEP(1) OK_STEP(2) OK_STEP(3) OK_STEP(4) OK_STEP(5) OK_STEP(6)
+            OK_STEP(7) OK_STEP(8) OK_STEP(9) OK_STEP(10) OK_STEP(11) OK_STEP(12) OK_STEP(13)
 #undef OK_STEP
         }
     }

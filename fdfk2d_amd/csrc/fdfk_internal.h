@@ -109,13 +109,13 @@ struct StepArgs {
     int it;
 };
 
-// edge-kernel work decomposition: up to four rectangles of cells (columns
-// x0..x1, rows z0..z1), enumerated z fastest (coalesced along the columns,
-// also for the thin row bands); off[] are prefix counts
-struct Rects {
+// edge-kernel work decomposition: up to four bands (columns x0..x1, rows
+// z0..z1), each cut into nbx x nbz tiles of tx x tz cells; off[] are block
+// prefix counts
+struct ETiles {
     int n;
-    int x0[4], x1[4], z0[4], z1[4];
-    long off[5];
+    int x0[4], x1[4], z0[4], z1[4], tx[4], tz[4], nbx[4], nbz[4];
+    int off[5];
 };
 
 struct RecArgs {

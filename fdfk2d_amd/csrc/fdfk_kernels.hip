@@ -105,25 +105,6 @@ __device__ __forceinline__ int tf_side(const Geo &g, int iz, int ix)
     return (ix >= 1 && ix <= g.nx && iz <= g.nz) ? 0 : 1;
 }
 
-// Field read for a ring cell of side sigma: neighbours on the other side
-// are seen in the output cell's representation, i.e. the inside ring reads
-// u + u0 outside (total field) and the outside ring reads u - u0 inside
-// (scattered field).  By linearity this equals L[u] +/- L[u0 masked], the
-// TF/SF correction of the hybrid method.
-// sigma = -1: plain read (cells outside the ring)
-struct CorrRd {
-    const float *p, *ring;
-    const Geo *g;
-    int sigma;
-    __device__ float operator()(int iz, int ix) const
-    {
-        size_t c = cellc(*g, iz, ix);
-        float v = p[c];
-        if (sigma >= 0 && tf_side(*g, iz, ix) != sigma) v = sigma == 0 ? v + ring[c] : v - ring[c];
-        return v;
-    }
-};
-
 template <int NO, class R>
 __device__ __forceinline__ float cdx_sum(const Geo &g, const Coefs *cf, const R &f, int iz, int ix)
 {
@@ -145,14 +126,14 @@ __device__ __forceinline__ float cdz_sum(const Geo &g, const Coefs *cf, const R 
 
 // D2 staggered composition along x with harmonic means of M (mode 0:
 // lambda+2mu, 1: mu) and the order reduction of diffx2_reg/diffx2_pml
-template <int NO, class R>
-__device__ __forceinline__ float x2_mode(const Geo &g, const Coefs *cf, const float *lam, const float *mu, int mode,
+template <int NO, class R, class MR>
+__device__ __forceinline__ float x2_mode(const Geo &g, const Coefs *cf, const MR &lam, const MR &mu, int mode,
                                          const R &u, int iz, int ix)
 {
     const int nx = g.nx, npml = g.npml;
     const int nox = imin(imin(ix + npml - 1, nx + npml - ix), NO);
     float s = 0.0f;
-    auto M = [&](int jx) { size_t c = cellc(g, iz, jx); return matval(mode, lam[c], mu[c]); };
+    auto M = [&](int jx) { return matval(mode, lam(iz, jx), mu(iz, jx)); };
 #pragma unroll 1
     for (int k = 1; k <= NO; ++k) {
         const int i = k - 1;  // (2k-1)/2
@@ -172,14 +153,14 @@ __device__ __forceinline__ float x2_mode(const Geo &g, const Coefs *cf, const fl
     return s * dx2;
 }
 
-template <int NO, class R>
-__device__ __forceinline__ float z2_mode(const Geo &g, const Coefs *cf, const float *lam, const float *mu, int mode,
+template <int NO, class R, class MR>
+__device__ __forceinline__ float z2_mode(const Geo &g, const Coefs *cf, const MR &lam, const MR &mu, int mode,
                                          const R &u, int iz, int ix)
 {
     const int nz = g.nz, npml = g.npml, izt = 1;
     const int noz = imin(imin(iz - izt, nz + npml - iz), NO);
     float s = 0.0f;
-    auto M = [&](int jz) { size_t c = cellc(g, jz, ix); return matval(mode, lam[c], mu[c]); };
+    auto M = [&](int jz) { return matval(mode, lam(jz, ix), mu(jz, ix)); };
 #pragma unroll 1
     for (int k = 1; k <= NO; ++k) {
         const int i = k - 1;
@@ -198,29 +179,29 @@ __device__ __forceinline__ float z2_mode(const Geo &g, const Coefs *cf, const fl
 }
 
 // Dx(mat * Dz u)  (diffzx at one output cell)
-template <int NO, class R>
-__device__ __forceinline__ float zx_pt(const Geo &g, const Coefs *cf, const float *mat, const R &u, int iz, int ix)
+template <int NO, class R, class MR>
+__device__ __forceinline__ float zx_pt(const Geo &g, const Coefs *cf, const MR &mat, const R &u, int iz, int ix)
 {
     const int nox = imin(imin(ix + g.npml - 1, g.nx + g.npml - ix), NO);
     float s = 0.0f;
 #pragma unroll 1
     for (int k = 1; k <= NO; ++k) {
-        float vp = mat[cellc(g, iz, ix + k)] * cdz_sum<NO>(g, cf, u, iz, ix + k) / g.dz;
-        float vm = mat[cellc(g, iz, ix - k)] * cdz_sum<NO>(g, cf, u, iz, ix - k) / g.dz;
+        float vp = mat(iz, ix + k) * cdz_sum<NO>(g, cf, u, iz, ix + k) / g.dz;
+        float vm = mat(iz, ix - k) * cdz_sum<NO>(g, cf, u, iz, ix - k) / g.dz;
         s = accf(s, coefz<NO>(cf->c1f, k, nox), vp - vm);
     }
     return s / g.dx;
 }
 // Dz(mat * Dx u)  (diffxz at one output cell)
-template <int NO, class R>
-__device__ __forceinline__ float xz_pt(const Geo &g, const Coefs *cf, const float *mat, const R &u, int iz, int ix)
+template <int NO, class R, class MR>
+__device__ __forceinline__ float xz_pt(const Geo &g, const Coefs *cf, const MR &mat, const R &u, int iz, int ix)
 {
     const int noz = imin(imin(iz - 1, g.nz + g.npml - iz), NO);
     float s = 0.0f;
 #pragma unroll 1
     for (int k = 1; k <= NO; ++k) {
-        float vp = mat[cellc(g, iz + k, ix)] * cdx_sum<NO>(g, cf, u, iz + k, ix) / g.dx;
-        float vm = mat[cellc(g, iz - k, ix)] * cdx_sum<NO>(g, cf, u, iz - k, ix) / g.dx;
+        float vp = mat(iz + k, ix) * cdx_sum<NO>(g, cf, u, iz + k, ix) / g.dx;
+        float vm = mat(iz - k, ix) * cdx_sum<NO>(g, cf, u, iz - k, ix) / g.dx;
         s = accf(s, coefz<NO>(cf->c1f, k, noz), vp - vm);
     }
     return s / g.dz;
@@ -231,21 +212,20 @@ __device__ __forceinline__ float xz_pt(const Geo &g, const Coefs *cf, const floa
 //   0 Xx = Dx(M Dx ux)   1 Zx = Dz(m Dz ux)   2 Dx(l Dz uz)   3 Dz(m Dx uz)
 //   4 Xz = Dx(m Dx uz)   5 Zz = Dz(M Dz uz)   6 Dx(m Dz ux)   7 Dz(l Dx ux)
 // with Mx = t2 + t3 and Mz = t6 + t7.
-template <int NO, class R>
-__device__ __forceinline__ float eval_task(int task, const Geo &g, const Coefs *cf, const float *lam, const float *mu,
+template <int NO, class R, class MR>
+__device__ __forceinline__ float eval_task(int task, const Geo &g, const Coefs *cf, const MR &lam, const MR &mu,
                                            const R &ux, const R &uz, int iz, int ix)
 {
     if (iz == 1) {
         // free-surface row: half-cell closure derived from the strain energy
         // of the top layer (rows 1-2, trapezoidal in z); traction free naturally
-        const size_t a = cell(g, 1, ix), b = cell(g, 2, ix);
         const float sdz2 = 2.0f / (g.dz * g.dz);
         switch (task) {
         case 0: return x2_mode<NO>(g, cf, lam, mu, 0, ux, iz, ix);
         case 4: return x2_mode<NO>(g, cf, lam, mu, 1, uz, iz, ix);
-        case 1: return (hmean(mu[a], mu[b]) * (ux(2, ix) - ux(1, ix))) * sdz2;
+        case 1: return (hmean(mu(1, ix), mu(2, ix)) * (ux(2, ix) - ux(1, ix))) * sdz2;
         case 5: {
-            const float M1 = lam[a] + 2.0f * mu[a], M2 = lam[b] + 2.0f * mu[b];
+            const float M1 = lam(1, ix) + 2.0f * mu(1, ix), M2 = lam(2, ix) + 2.0f * mu(2, ix);
             return (hmean(M1, M2) * (uz(2, ix) - uz(1, ix))) * sdz2;
         }
         case 2:
@@ -253,17 +233,16 @@ __device__ __forceinline__ float eval_task(int task, const Geo &g, const Coefs *
             // task 2: (Dx(lbar dz uz) + mbar Dx uz) / dz, task 6: (Dx(mbar dz ux) + lbar Dx ux) / dz
             const bool t2 = task == 2;
             const R &w = t2 ? uz : ux;
-            const float *A = t2 ? lam : mu;
+            const MR &A = t2 ? lam : mu;
             const int nox = imin(imin(ix + g.npml - 1, g.nx + g.npml - ix), NO);
             float sa = 0.0f;
 #pragma unroll
             for (int k = 1; k <= NO; ++k) {
-                size_t a1 = cellc(g, 1, ix + k), b1 = cellc(g, 2, ix + k), a2 = cellc(g, 1, ix - k), b2 = cellc(g, 2, ix - k);
-                float ap = 0.5f * (A[a1] + A[b1]), am = 0.5f * (A[a2] + A[b2]);
+                float ap = 0.5f * (A(1, ix + k) + A(2, ix + k)), am = 0.5f * (A(1, ix - k) + A(2, ix - k));
                 sa = accf(sa, coefz<NO>(cf->c1f, k, nox), ap * (w(2, ix + k) - w(1, ix + k)) - am * (w(2, ix - k) - w(1, ix - k)));
             }
-            const float *B = t2 ? mu : lam;
-            const float bbar = 0.5f * (B[a] + B[b]);
+            const MR &B = t2 ? mu : lam;
+            const float bbar = 0.5f * (B(1, ix) + B(2, ix));
             const float dxw = (cdx_sum<NO>(g, cf, w, 1, ix) + cdx_sum<NO>(g, cf, w, 2, ix)) / g.dx;
             return (sa / g.dx + bbar * dxw) / g.dz;
         }
@@ -312,36 +291,6 @@ struct PsiRd {
         return b[strip_off(*g, s, iz, ix)];
     }
 };
-
-// ---------------------------------------------------------------------
-// frame kernels: every updated cell outside the interior rectangle, split
-// into the full-order part (k_frame_full) and the reduced-order edge bands
-// next to the padded-grid boundary and the free surface (k_frame_edge)
-// ---------------------------------------------------------------------
-__device__ __forceinline__ bool rect_cell(const Rects &r, long t, int &iz, int &ix)
-{
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        if (q < r.n && t < r.off[q + 1]) {
-            long u = t - r.off[q];
-            int rows = r.z1[q] - r.z0[q] + 1;
-            ix = r.x0[q] + (int)(u / rows);
-            iz = r.z0[q] + (int)(u % rows);
-            return true;
-        }
-    }
-    return false;
-}
-
-__device__ __forceinline__ CorrRd frame_reader(const StepArgs &a, const float *u, const float *ring, int iz, int ix)
-{
-    int sig = -1;
-    if (a.ring_valid) {
-        int arr, which;
-        if (bg_off(a.g, iz, ix, 0, arr, which) >= 0) sig = which;
-    }
-    return CorrRd{u, ring, &a.g, sig};
-}
 
 // PML memory corrections and the (split-field) leapfrog update of one cell
 template <int NO>
@@ -409,33 +358,109 @@ __device__ __forceinline__ void frame_update(const StepArgs &a, int iz, int ix, 
     }
 }
 
-// Edge kernel: 512 threads per block, 64 cells per block, eight waves; wave
-// w evaluates term w of the 64 cells (wave-uniform control flow), then wave 0
-// combines the terms and applies the memory corrections and the update.
-constexpr int kEdgeCells = 64;
-constexpr int kEdgeThreads = 8 * kEdgeCells;
-template <int NO>
-__device__ __forceinline__ void frame_edge_body(const StepArgs &a, const Rects &r, long bid)
-{
-    __shared__ float part[8][kEdgeCells];
-    const int task = threadIdx.x / kEdgeCells, slot = threadIdx.x % kEdgeCells;
-    const long t = bid * kEdgeCells + slot;
-    int iz = 1, ix = 1;
-    const bool ok = rect_cell(r, t, iz, ix);
-    if (ok) {
-        CorrRd rx = frame_reader(a, a.ux, a.ringx, iz, ix), rz = frame_reader(a, a.uz, a.ringz, iz, ix);
-        part[task][slot] = eval_task<NO>(task, a.g, a.cf, a.lam, a.mu, rx, rz, iz, ix);
+// Edge kernel: one block per tile of an edge band (at most kEdgeThreads
+// cells).  The tile and its full stencil halo (2*NO-1 cells each way; cells
+// beyond the grid are staged as zeros) of u, lambda, mu and the ring
+// background are staged in LDS once; each thread then evaluates one cell
+// from LDS (the eight terms in turn), applies the memory corrections and
+// the update.  The operators always run to the full order with zero
+// coefficients beyond a cell's reduced order, so every read stays inside the
+// staged halo and needs no clamping.
+constexpr int kEdgeThreads = 256;
+constexpr int kEdgeLds = 1152;   // staged cells per array (host checks the tile fits)
+
+struct LdsTile {
+    int lx0, lz0, tw;
+    __device__ __forceinline__ int idx(int iz, int ix) const { return (ix - lx0) * tw + (iz - lz0); }
+};
+// Field reader on the staged tile.  With RING, for a ring cell of side
+// sigma, neighbours on the other side are seen in the output cell's
+// representation: the inside ring reads u + u0 outside (total field), the
+// outside ring reads u - u0 inside (scattered field).  By linearity this
+// equals L[u] +/- L[u0 masked], the TF/SF correction of the hybrid method.
+// sigma = -1: plain read (cells outside the ring).
+template <bool RING>
+struct LdsRd {
+    const LdsTile *t;
+    const float *p, *ring;
+    const Geo *g;
+    int sigma;
+    __device__ __forceinline__ float operator()(int iz, int ix) const
+    {
+        const int k = t->idx(iz, ix);
+        float v = p[k];
+        if (RING && sigma >= 0 && tf_side(*g, iz, ix) != sigma) v = sigma == 0 ? v + ring[k] : v - ring[k];
+        return v;
     }
-    __syncthreads();
-    if (task != 0 || !ok) return;
-    float p[6] = {part[0][slot], part[1][slot], part[2][slot] + part[3][slot],
-                  part[4][slot], part[5][slot], part[6][slot] + part[7][slot]};
+};
+struct LdsMat {
+    const LdsTile *t;
+    const float *p;
+    __device__ __forceinline__ float operator()(int iz, int ix) const { return p[t->idx(iz, ix)]; }
+};
+
+template <int NO, bool RING>
+__device__ __forceinline__ void edge_cell(const StepArgs &a, const LdsTile &T, const float *sux, const float *suz,
+                                          const float *slam, const float *smu, const float *srx, const float *srz,
+                                          int iz, int ix, int sig)
+{
+    const Geo &g = a.g;
+    LdsRd<RING> rx{&T, sux, srx, &g, sig}, rz{&T, suz, srz, &g, sig};
+    LdsMat lam{&T, slam}, mu{&T, smu};
+    float e[8];
+#pragma unroll 1
+    for (int k = 0; k < 8; ++k) e[k] = eval_task<NO>(k, g, a.cf, lam, mu, rx, rz, iz, ix);
+    float p[6] = {e[0], e[1], e[2] + e[3], e[4], e[5], e[6] + e[7]};
     frame_update<NO>(a, iz, ix, p);
 }
+
 template <int NO>
-__global__ __launch_bounds__(kEdgeThreads) void k_frame_edge(StepArgs a, Rects r)
+__device__ __forceinline__ void frame_edge_body(const StepArgs &a, const ETiles &et, long bid)
 {
-    frame_edge_body<NO>(a, r, blockIdx.x);
+    __shared__ float sux[kEdgeLds], suz[kEdgeLds], slam[kEdgeLds], smu[kEdgeLds], srx[kEdgeLds], srz[kEdgeLds];
+    const Geo &g = a.g;
+    int q = 0;
+    while (q + 1 < et.n && bid >= et.off[q + 1]) ++q;
+    const int loc = (int)(bid - et.off[q]);
+    const int bx = loc / et.nbz[q], bz = loc % et.nbz[q];
+    const int cx0 = et.x0[q] + bx * et.tx[q], cx1 = min(cx0 + et.tx[q] - 1, et.x1[q]);
+    const int cz0 = et.z0[q] + bz * et.tz[q], cz1 = min(cz0 + et.tz[q] - 1, et.z1[q]);
+    constexpr int H = 2 * NO - 1;
+    LdsTile T;
+    T.lx0 = cx0 - H; T.lz0 = cz0 - H;
+    T.tw = cz1 - cz0 + 1 + 2 * H;
+    const int ne = (cx1 - cx0 + 1 + 2 * H) * T.tw;
+    // the tile holds ring cells iff it meets the ring's bounding box
+    // (columns 1-nbd..nx+nbd, rows 1..nz+nbd) minus the region inside it
+    const bool ring = a.ring_valid && !(cx1 < 1 - g.nbd || cx0 > g.nx + g.nbd || cz0 > g.nz + g.nbd ||
+                                        (cx0 > g.nbd && cx1 <= g.nx - g.nbd && cz1 <= g.nz - g.nbd));
+    for (int e = threadIdx.x; e < ne; e += kEdgeThreads) {
+        const int ix = T.lx0 + e / T.tw, iz = T.lz0 + e % T.tw;
+        float vx = 0.0f, vz = 0.0f, vl = 0.0f, vm = 0.0f, wx = 0.0f, wz = 0.0f;
+        if (ix >= 1 - g.npml && ix <= g.nx + g.npml && iz >= 1 && iz <= g.NZ) {
+            const size_t c = cell(g, iz, ix);
+            vx = a.ux[c]; vz = a.uz[c]; vl = a.lam[c]; vm = a.mu[c];
+            if (ring) { wx = a.ringx[c]; wz = a.ringz[c]; }
+        }
+        sux[e] = vx; suz[e] = vz; slam[e] = vl; smu[e] = vm; srx[e] = wx; srz[e] = wz;
+    }
+    __syncthreads();
+    const int rows = cz1 - cz0 + 1;
+    const int t = threadIdx.x;
+    if (t >= rows * (cx1 - cx0 + 1)) return;
+    const int ix = cx0 + t / rows, iz = cz0 + t % rows;
+    if (ring) {
+        int sig = -1, arr, which;
+        if (bg_off(g, iz, ix, 0, arr, which) >= 0) sig = which;
+        edge_cell<NO, true>(a, T, sux, suz, slam, smu, srx, srz, iz, ix, sig);
+    } else {
+        edge_cell<NO, false>(a, T, sux, suz, slam, smu, srx, srz, iz, ix, -1);
+    }
+}
+template <int NO>
+__global__ __launch_bounds__(kEdgeThreads) void k_frame_edge(StepArgs a, ETiles t)
+{
+    frame_edge_body<NO>(a, t, blockIdx.x);
 }
 
 // background on the hybrid ring at iteration it, scattered into the
@@ -583,6 +608,18 @@ __global__ __launch_bounds__(64) void k_record(RecArgs r)
 // word 3 of a gfx9 buffer resource (32-bit raw data, no swizzle)
 constexpr int kBufferFlags = 0x00020000;
 
+// Workgroup barrier for LDS exchange only.  __syncthreads() also waits for
+// every outstanding global load (gfx9 counts loads and stores in one
+// counter), which would drain the one-column-ahead prefetch of the marching
+// kernels at every column.  Their lanes never read global data written by
+// another lane of the same launch, so waiting for the wave's own LDS
+// operations before the barrier is sufficient; the "memory" clobber keeps
+// the compiler from moving LDS accesses across it.
+__device__ __forceinline__ void lds_barrier()
+{
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
 // ---------------------------------------------------------------------
 // interior kernel: full-order fused stencil over the total-field region.
 // Lanes run along z (one row each, HZ halo rows on either side of the TZ
@@ -704,7 +741,7 @@ struct IntKernel2 {
         S.rM[PA][0][PAD + lane] = frcp(l0 + 2.0f * m0);
         S.rM[PA][1][PAD + lane] = frcp(m0);
         // B.
-        __syncthreads();
+        lds_barrier();
         // C. Vx at ix+NO; z-fluxes and Vz at ix
         {
             float dzz = 0.0f, dzx = 0.0f;
