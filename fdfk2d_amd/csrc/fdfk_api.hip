@@ -91,7 +91,6 @@ struct fdfk_solver {
     float *stage = nullptr;   // (2, NZ, NX) staging for reference-layout copies
     // interior kernel
     int lx = 45, nt_threads = 256;
-    int flx_side = 6, flx_bottom = 6;   // k_frame_out columns per block (side strips / bottom strip)
     // profiling
     bool prof = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_int, ev_step;
@@ -308,106 +307,62 @@ void launch_interior(fdfk_solver *s)
     }
 }
 
-// one edge band cut into tiles of at most kEdgeThreads cells whose staged
-// halo fits the kernel's LDS arrays: thin row bands take all their rows per
-// tile, thin column bands all their columns
-void add_band(ETiles &t, int x0, int x1, int z0, int z1, int norder)
+// one frame band cut into tiles of at most kFrameThreads cells whose staged
+// halo and intermediates fit the kernel's LDS: thin bands take all their
+// rows (columns) per tile, wide ones 16 x 16 tiles
+void add_band(ETiles &t, int x0, int x1, int z0, int z1, int norder, int view)
 {
     if (x1 < x0 || z1 < z0) return;
-    const int H = 2 * norder - 1;
     const int cols = x1 - x0 + 1, rows = z1 - z0 + 1;
     int tx, tz;
-    if (rows <= cols) { tz = std::min(rows, kEdgeThreads); tx = std::max(1, kEdgeThreads / tz); }
-    else { tx = std::min(cols, kEdgeThreads); tz = std::max(1, kEdgeThreads / tx); }
-    while ((tx + 2 * H) * (tz + 2 * H) > kEdgeLds) {   // staged halo is unclipped
+    if (rows <= 8) { tz = rows; tx = std::max(1, kFrameThreads / tz); }
+    else if (cols <= 8) { tx = cols; tz = std::max(1, kFrameThreads / tx); }
+    else { tx = 16; tz = 16; }
+    while (frame_lds_floats(tx, tz, norder) > kFrameLds) {
         if (tx >= tz) --tx; else --tz;
     }
     int q = t.n++;
-    t.x0[q] = x0; t.x1[q] = x1; t.z0[q] = z0; t.z1[q] = z1; t.tx[q] = tx; t.tz[q] = tz;
+    t.x0[q] = x0; t.x1[q] = x1; t.z0[q] = z0; t.z1[q] = z1; t.tx[q] = tx; t.tz[q] = tz; t.view[q] = view;
     t.nbx[q] = (cols + tx - 1) / tx;
     t.nbz[q] = (rows + tz - 1) / tz;
     t.off[q + 1] = t.off[q] + t.nbx[q] * t.nbz[q];
 }
 
-// Full-order rectangle: columns xf0..xf1, rows iz0..zf1.  Its part inside
-// the total-field region is the interior kernel's; the rest (the PML strips
-// up to the reduced-order bands) is k_frame_out's.
-void full_order_box(const Geo &g, int &xf0, int &xf1, int &zf1)
+// every updated cell outside the interior rectangle, in four bands that
+// each lie on one side of the hybrid interface: the surface rows of the
+// total-field region, the left and right strips (all rows of the model
+// depth), and the bottom strip (all columns)
+ETiles frame_tiles(const Geo &g)
 {
-    xf0 = 2 * g.norder - g.npml;
-    xf1 = g.nx + g.npml + 1 - 2 * g.norder;
-    zf1 = g.nz + g.npml + 1 - 2 * g.norder;
-}
-
-void add_tile(FTiles &t, int x0, int x1, int z0, int z1, int lx, int TZ)
-{
-    if (x1 < x0 || z1 < z0) return;
-    int q = t.n++;
-    t.x0[q] = x0; t.x1[q] = x1; t.z0[q] = z0; t.z1[q] = z1; t.lx[q] = lx;
-    t.nbx[q] = (x1 - x0 + lx) / lx;
-    t.nbz[q] = (z1 - z0 + TZ) / TZ;
-    t.off[q + 1] = t.off[q] + t.nbx[q] * t.nbz[q];
-}
-
-FTiles out_tiles(const Geo &g, int flx_side, int flx_bottom)
-{
-    int xf0, xf1, zf1;
-    full_order_box(g, xf0, xf1, zf1);
-    const int TZ = 64 - 2 * (2 * g.norder - 1);
-    FTiles t{};
-    add_tile(t, xf0, 0, g.iz0, zf1, flx_side, TZ);              // left strip (+ bottom-left corner)
-    add_tile(t, g.nx + 1, xf1, g.iz0, zf1, flx_side, TZ);       // right strip (+ corner)
-    add_tile(t, 1, g.nx, g.nz + 1, zf1, flx_bottom, TZ);        // bottom strip under the model
-    return t;
-}
-
-template <int NO>
-FrameArgs<NO> frame_args(fdfk_solver *s)
-{
-    FrameArgs<NO> f{};
-    f.g = s->g;
-    f.k = make_icoef<NO>(s);
-    f.ux = s->ux[s->cur]; f.uz = s->uz[s->cur];
-    f.uxp = s->ux[1 - s->cur]; f.uzp = s->uz[1 - s->cur];
-    f.bf = s->bf; f.lam = s->lam; f.mu = s->mu;
-    f.st = s->d_strips[s->cur];
-    f.ringx = s->ringx; f.ringz = s->ringz;
-    f.corr = s->bg_valid && s->it <= s->nt;
-    f.t = out_tiles(s->g, s->flx_side, s->flx_bottom);
-    return f;
-}
-
-template <int NO>
-void launch_frame_out_t(fdfk_solver *s)
-{
-    FrameArgs<NO> f = frame_args<NO>(s);
-    if (f.t.n > 0)
-        hipLaunchKernelGGL(k_frame_out<NO>, dim3(f.t.off[f.t.n]), dim3(64), 0, s->stream, f);
-}
-
-void launch_frame_out(fdfk_solver *s)
-{
-    switch (s->g.norder) {
-    case 1: launch_frame_out_t<1>(s); break;
-    case 2: launch_frame_out_t<2>(s); break;
-    case 3: launch_frame_out_t<3>(s); break;
-    default: launch_frame_out_t<4>(s); break;
-    }
-}
-
-// reduced-order edge bands: top rows (free surface), bottom rows and the
-// columns next to the padded-grid boundary
-ETiles edge_tiles(const Geo &g)
-{
-    int xf0, xf1, zf1;
-    full_order_box(g, xf0, xf1, zf1);
-    int xa = 2 - g.npml, xb = g.nx + g.npml - 1;
+    const int xa = 2 - g.npml, xb = g.nx + g.npml - 1;
     ETiles t{};
-    add_band(t, xa, xb, 1, g.iz0 - 1, g.norder);
-    add_band(t, xa, xb, zf1 + 1, g.nz + g.npml - 1, g.norder);
-    add_band(t, xa, xf0 - 1, g.iz0, zf1, g.norder);
-    add_band(t, xf1 + 1, xb, g.iz0, zf1, g.norder);
+    add_band(t, 1, g.nx, 1, g.iz0 - 1, g.norder, 0);
+    add_band(t, xa, 0, 1, g.nz, g.norder, 1);
+    add_band(t, g.nx + 1, xb, 1, g.nz, g.norder, 1);
+    add_band(t, xa, xb, g.nz + 1, g.nz + g.npml - 1, g.norder, 1);
     return t;
+}
+
+// dynamic LDS bytes of a frame launch: its largest tile
+size_t frame_lds_bytes(const ETiles &t, int norder)
+{
+    int m = 0;
+    for (int q = 0; q < t.n; ++q) m = std::max(m, frame_lds_floats(t.tx[q], t.tz[q], norder));
+    return (size_t)m * sizeof(float);
+}
+
+void launch_frame(fdfk_solver *s, const StepArgs &a)
+{
+    ETiles t = frame_tiles(s->g);
+    if (t.off[t.n] == 0) return;
+    dim3 fg((unsigned)t.off[t.n]), fb(kFrameThreads);
+    const size_t sh = frame_lds_bytes(t, s->g.norder);
+    switch (s->g.norder) {
+    case 1: hipLaunchKernelGGL(k_frame<1>, fg, fb, sh, s->stream, a, t); break;
+    case 2: hipLaunchKernelGGL(k_frame<2>, fg, fb, sh, s->stream, a, t); break;
+    case 3: hipLaunchKernelGGL(k_frame<3>, fg, fb, sh, s->stream, a, t); break;
+    default: hipLaunchKernelGGL(k_frame<4>, fg, fb, sh, s->stream, a, t); break;
+    }
 }
 
 int one_step(fdfk_solver *s, bool record)
@@ -442,17 +397,7 @@ int one_step(fdfk_solver *s, bool record)
     if (s->prof) hipEventRecord(e1, s->stream);
     launch_interior(s);
     if (s->prof) hipEventRecord(e2, s->stream);
-    launch_frame_out(s);
-    ETiles re = edge_tiles(g);
-    if (re.off[re.n] > 0) {
-        dim3 eg((unsigned)re.off[re.n]), eb(kEdgeThreads);
-        switch (g.norder) {
-        case 1: hipLaunchKernelGGL(k_frame_edge<1>, eg, eb, 0, s->stream, a, re); break;
-        case 2: hipLaunchKernelGGL(k_frame_edge<2>, eg, eb, 0, s->stream, a, re); break;
-        case 3: hipLaunchKernelGGL(k_frame_edge<3>, eg, eb, 0, s->stream, a, re); break;
-        default: hipLaunchKernelGGL(k_frame_edge<4>, eg, eb, 0, s->stream, a, re); break;
-        }
-    }
+    launch_frame(s, a);
     if (s->prof) {
         hipEventRecord(e3, s->stream);
         s->ev_int.emplace_back(e1, e2);
@@ -514,8 +459,6 @@ int fdfk_create(const fdfk_job *job, fdfk_solver **out)
     if (const char *e = std::getenv("FDFK_NT")) s->nt_threads = std::atoi(e);
     s->lx = auto_lx(s);
     if (const char *e = std::getenv("FDFK_LX")) s->lx = std::max(4, std::atoi(e));
-    if (const char *e = std::getenv("FDFK_FLX")) s->flx_side = std::max(1, std::atoi(e));
-    if (const char *e = std::getenv("FDFK_FLXB")) s->flx_bottom = std::max(1, std::atoi(e));
     const long ncell = (long)g.NX * g.NZ;
     // vpml = max Vp (src/FDFK2D.f90:416)
     float vpml = job->vpml;

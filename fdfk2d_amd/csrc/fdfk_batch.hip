@@ -14,14 +14,13 @@
 namespace fdfk {
 
 constexpr int kMaxBatch = 256;
-enum { KB_RING = 0, KB_PSI, KB_INT, KB_FOUT, KB_FEDGE, KB_REC, KB_N };
+enum { KB_RING = 0, KB_PSI, KB_INT, KB_FRAME, KB_REC, KB_N };
 
 template <int NO>
 struct BatchJob {
     StepArgs sa;
     IntArgs<NO> ia;
-    FrameArgs<NO> fa;
-    ETiles re;
+    ETiles ft;
     RecArgs ra;
     const Background *bg;
     float *ringx, *ringz;
@@ -65,17 +64,10 @@ __global__ __launch_bounds__(NT) void kb_interior(const BatchJob<NO> *J, const B
 }
 
 template <int NO>
-__global__ __launch_bounds__(64) void kb_frame_out(const BatchJob<NO> *J, const BatchPrefix *P)
+__global__ __launch_bounds__(kFrameThreads) void kb_frame(const BatchJob<NO> *J, const BatchPrefix *P)
 {
-    int j = find_job(P->pre[KB_FOUT], P->n, blockIdx.x);
-    OutKernel<NO, 64>::run(J[j].fa, blockIdx.x - P->pre[KB_FOUT][j]);
-}
-
-template <int NO>
-__global__ __launch_bounds__(kEdgeThreads) void kb_frame_edge(const BatchJob<NO> *J, const BatchPrefix *P)
-{
-    int j = find_job(P->pre[KB_FEDGE], P->n, blockIdx.x);
-    frame_edge_body<NO>(J[j].sa, J[j].re, blockIdx.x - P->pre[KB_FEDGE][j]);
+    int j = find_job(P->pre[KB_FRAME], P->n, blockIdx.x);
+    frame_body<NO>(J[j].sa, J[j].ft, blockIdx.x - P->pre[KB_FRAME][j]);
 }
 
 template <int NO>
@@ -121,7 +113,7 @@ size_t batch_table_bytes()
 }
 
 // fill one job's arguments for the step it is about to take; returns the
-// block counts of the six kernels
+// block counts of the kernels
 template <int NO>
 void fill_job(fdfk_solver *s, BatchJob<NO> &b, int nb[KB_N], bool &record)
 {
@@ -131,8 +123,7 @@ void fill_job(fdfk_solver *s, BatchJob<NO> &b, int nb[KB_N], bool &record)
     b.ringx = s->ringx;
     b.ringz = s->ringz;
     b.ia = interior_args<NO, 256>(s);
-    b.fa = frame_args<NO>(s);
-    b.re = edge_tiles(g);
+    b.ft = frame_tiles(g);
     record = (s->isis == s->it) && (s->istp <= s->nsis) && s->nrcv > 0;
     RecArgs &r = b.ra;
     r = RecArgs{};
@@ -144,8 +135,7 @@ void fill_job(fdfk_solver *s, BatchJob<NO> &b, int nb[KB_N], bool &record)
     nb[KB_RING] = b.sa.ring_valid ? (int)((nring + 255) / 256) : 0;
     nb[KB_PSI] = (int)((npsi + 255) / 256);
     nb[KB_INT] = b.ia.nbx * b.ia.nbz;
-    nb[KB_FOUT] = b.fa.t.off[b.fa.t.n];
-    nb[KB_FEDGE] = b.re.off[b.re.n];
+    nb[KB_FRAME] = b.ft.off[b.ft.n];
     nb[KB_REC] = record ? s->nrcv : 0;
 }
 
@@ -187,8 +177,9 @@ int batch_step(fdfk_batch *B)
     if (B->prof) hipEventRecord(e1, B->stream);
     if (P->pre[KB_INT][n]) hipLaunchKernelGGL((kb_interior<NO, 256>), dim3(P->pre[KB_INT][n]), dim3(256), 0, B->stream, dJ, dP);
     if (B->prof) hipEventRecord(e2, B->stream);
-    if (P->pre[KB_FOUT][n]) hipLaunchKernelGGL(kb_frame_out<NO>, dim3(P->pre[KB_FOUT][n]), dim3(64), 0, B->stream, dJ, dP);
-    if (P->pre[KB_FEDGE][n]) hipLaunchKernelGGL(kb_frame_edge<NO>, dim3(P->pre[KB_FEDGE][n]), dim3(kEdgeThreads), 0, B->stream, dJ, dP);
+    size_t fsh = 0;
+    for (size_t j = 0; j < act.size(); ++j) fsh = std::max(fsh, frame_lds_bytes(J[j].ft, NO));
+    if (P->pre[KB_FRAME][n]) hipLaunchKernelGGL(kb_frame<NO>, dim3(P->pre[KB_FRAME][n]), dim3(kFrameThreads), fsh, B->stream, dJ, dP);
     if (B->prof) {
         hipEventRecord(e3, B->stream);
         B->ev_int.emplace_back(e1, e2);

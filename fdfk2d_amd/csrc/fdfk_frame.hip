@@ -1,313 +1,208 @@
-// Full-order padding cells (the PML strips beyond the hybrid interface):
-// a marching kernel with the interior kernel's shared-flux structure (lanes
-// along z, one column per step along x, register rings for x-neighbours,
-// LDS for z-neighbours) plus what the strips need:
-//   * scattered-field view: every read of a cell inside the total-field
-//     region subtracts the background u0 there (the reads of an outside
-//     hybrid-ring cell, src/FDFK2D.f90:516-522 / libHybrid);
-//   * the stretched-coordinate memory corrections -Dx(M psi_x) and, in the
-//     bottom strip, -Dz(M psi_z) with the psi of this step (k_psi);
-//   * the split-field leapfrog update of the three parts per component.
-// The reduced-order cells next to the padded-grid boundary and the surface
-// rows stay with k_frame_edge.
-//
-// Every output cell of a tile is outside the total-field region, so one
-// view serves the whole tile.
-
+// ---------------------------------------------------------------------
+// Frame kernel: every updated cell outside the interior rectangle (the
+// surface rows of the total-field region, the PML strips and the cells
+// next to the padded-grid boundary), in 2-D tiles.  A tile lies on one side
+// of the hybrid interface, so the view of the ring (total field inside,
+// scattered field outside, the corrected reads of libHybrid) is applied
+// once when the tile and its halo of 2*NO-1 cells are staged in LDS (cells
+// beyond the grid staged as zeros).  Phase 1 evaluates, for the tile, the
+// shared intermediate quantities with the reduced-order rule of
+// src/Submain.f90:192-195:
+//   FX1/FX2  x-fluxes h*D1 at faces (f, f+1), orders n1(f) / n2(f)
+//   FZ1/FZ2  z-fluxes
+//   DZ, DX   collocated first derivatives of u (cdz_sum / cdx_sum)
+// Phase 2 combines them per cell into the six RHS parts (with the
+// free-surface closure on row 1), then frame_update applies the memory
+// corrections and the (split) update.
+// ---------------------------------------------------------------------
 namespace fdfk {
 
-template <int NO, int NT>
-struct OutKernel {
-    static constexpr int HZ = 2 * NO - 1;   // z halo lanes on each side
-    static constexpr int TZ = NT - 2 * HZ;  // output rows per block
-    static constexpr int P0 = 3 * NO + 1;
-    static constexpr int P = P0 + (P0 & 1); // even ring period: LDS parity is compile-time
-    static constexpr int PAD = NO + 1;      // LDS guard entries on each side
-    static constexpr int WARM = 4 * NO - 2; // warm-up columns before the first output
-    static constexpr int LW = NT + 2 * PAD;
+constexpr int kFrameThreads = 256;
+constexpr int kFrameLds = 12288;   // max floats of LDS per block (host keeps tiles within it)
 
-    struct Smem {                 // [parity][comp][lane]
-        float uA[2][2][LW];       // u at column ix+NO
-        float uB[2][2][LW];       // u at column ix
-        float rM[2][2][LW];       // 1/(l+2m), 1/m at column ix
-        float fz[2][2][LW];       // z-fluxes at column ix
-        float vz[2][2][LW];       // m*Dx uz, l*Dx ux at column ix
-        float pz[2][2][LW];       // m*psi_z(ux), (l+2m)*psi_z(uz) at column ix (bottom strip, else 0)
-    };
-
-    // per-lane constants: row, masks, and the lane's strip (the side strip
-    // above the bottom strip, the bottom strip below it) with its arrays
-    struct Lane {
-        int lane, r;
-        unsigned ofs;    // clamped row byte offset inside a grid column
-        bool outrow, isB;
-        float az;
-        int cb;          // first column of the lane's strip
-        int stride;      // strip column pitch (nz or npml)
-        int rb;          // row offset inside a strip column
-        const float *psx[2], *psz[2];
-        float *cur[2][3], *prev[2][3];
-        __device__ __forceinline__ size_t off(int col) const
-        {
-            return (size_t)(col - cb) * (size_t)stride + (size_t)rb;
-        }
-    };
-
-    struct Regs {
-        float U[2][P], FX[2][P], VX[2][P], PX[2][P];
-        float rMa_prev, rMm_prev;
-        float pu[2][2];   // [parity][comp] next new u column (ix+2NO-1 of the next step)
-        float pl[2][2];   // [parity][l,m] next lead materials (ix+NO)
-        float p0[2][2];   // [parity][l,m] next column materials (ix)
-        float pq[2][2];   // [parity][comp] next psi_x at (r, ix+NO)
-        float pr[2][2];   // [parity][comp] next psi_z at (r, ix)
-        float ps[2][2];   // [parity][comp] z memory sums of column ix (used one step later)
-    };
-
-    static __device__ __forceinline__ int sl(int ph, int j) { return ((ph + j) % P + P) % P; }
-
-    static __device__ __forceinline__ __amdgpu_buffer_rsrc_t colrsrc(const Geo &g, const float *base, int c)
-    {
-        const float *cb = base + (size_t)(c + g.npml - 1) * (size_t)g.ld;
-        return __builtin_amdgcn_make_buffer_rsrc((void *)cb, 0, g.ld * 4, kBufferFlags);
-    }
-    static __device__ __forceinline__ float at(const Geo &g, const float *base, int c, unsigned o)
-    {
-        return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(colrsrc(g, base, c), o, 0, 0));
-    }
-    static __device__ __forceinline__ void put(const Geo &g, float *base, int c, unsigned o, float v)
-    {
-        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), colrsrc(g, base, c), o, 0, 0);
-    }
-
-    // the loads one step consumes, issued one step ahead (column ix)
-    static __device__ __forceinline__ void prefetch(const FrameArgs<NO> &a, const Lane &L, Regs &R, int q, int ix,
-                                                    int xs, int xe)
-    {
-        const Geo &g = a.g;
-        const int lo = 1 - g.npml, hi = g.nx + g.npml;
-        // columns of the first warm-up and the last (unused) prefetch may
-        // lie beyond the grid; their values are never used, so the grid
-        // addresses are clamped and the strip reads skipped
-        auto cc = [&](int c) { return c < lo ? lo : (c > hi ? hi : c); };
-        auto inside = [&](int c) { return c >= 1 && c <= g.nx && L.r <= g.nz; };
-        const int cn = cc(ix + 2 * NO - 1), cm = ix + NO;
-        float vx = at(g, a.ux, cn, L.ofs), vz = at(g, a.uz, cn, L.ofs);
-        if (a.corr && inside(cn)) { vx -= at(g, a.ringx, cn, L.ofs); vz -= at(g, a.ringz, cn, L.ofs); }
-        R.pu[q][0] = vx; R.pu[q][1] = vz;
-        R.pl[q][0] = at(g, a.lam, cc(cm), L.ofs); R.pl[q][1] = at(g, a.mu, cc(cm), L.ofs);
-        R.p0[q][0] = at(g, a.lam, cc(ix), L.ofs); R.p0[q][1] = at(g, a.mu, cc(ix), L.ofs);
-        float q0 = 0.0f, q1 = 0.0f, z0 = 0.0f, z1 = 0.0f;
-        if (L.outrow && cm >= xs - NO && cm < xe + NO && !inside(cm)) {
-            const size_t o = L.off(cm);
-            q0 = L.psx[0][o]; q1 = L.psx[1][o];
-        }
-        if (L.isB && ix >= xs && ix < xe) {
-            const size_t o = L.off(ix);
-            z0 = L.psz[0][o]; z1 = L.psz[1][o];
-        }
-        R.pq[q][0] = q0; R.pq[q][1] = q1;
-        R.pr[q][0] = z0; R.pr[q][1] = z1;
-    }
-
-    template <int PH>
-    static __device__ __forceinline__ void step(const FrameArgs<NO> &a, Smem &S, Regs &R, const Lane &L, int ix,
-                                                int xs, int xe)
-    {
-        constexpr int PA = PH & 1, PB = 1 - PA;
-        const Geo &g = a.g;
-        const int lane = L.lane;
-        // A. consume this step's prefetch, prefetch the next step, load the
-        //    update inputs of the output column ix-1
-        R.U[0][sl(PH, 2 * NO - 1)] = R.pu[PA][0];
-        R.U[1][sl(PH, 2 * NO - 1)] = R.pu[PA][1];
-        const float l1 = R.pl[PA][0], m1 = R.pl[PA][1];
-        const float l0 = R.p0[PA][0], m0 = R.p0[PA][1];
-        const float q0 = R.pq[PA][0], q1 = R.pq[PA][1];
-        const float zr0 = R.pr[PA][0], zr1 = R.pr[PA][1];
-        prefetch(a, L, R, PB, ix + 1, xs, xe);
-        const int co = ix - 1;
-        const bool out = L.outrow && co >= xs && co < xe;
-        float bf = 0.0f, ax = 0.0f, sc[2][3] = {}, sp[2][3] = {};
-        if (out) {
-            const size_t o = L.off(co);
-            bf = at(g, a.bf, co, L.ofs);
-            ax = a.st->ax[co + g.npml - 1];
-#pragma unroll
-            for (int c = 0; c < 2; ++c)
-#pragma unroll
-                for (int p = 0; p < 3; ++p) { sc[c][p] = L.cur[c][p][o]; sp[c][p] = L.prev[c][p][o]; }
-        }
-        // x-flux at the face (ix+NO-1, ix+NO), M psi_x at ix+NO, LDS staging
-        const float rMa = frcp(l1 + 2.0f * m1), rMm = frcp(m1);
-        {
-            const float ha = frcp(R.rMa_prev + rMa), hm = frcp(R.rMm_prev + rMm);
-            float s0 = 0.0f, s1 = 0.0f;
-#pragma unroll
-            for (int j = 1; j <= NO; ++j) {
-                s0 += a.k.ai[j - 1] * (R.U[0][sl(PH, NO - 1 + j)] - R.U[0][sl(PH, NO - j)]);
-                s1 += a.k.ai[j - 1] * (R.U[1][sl(PH, NO - 1 + j)] - R.U[1][sl(PH, NO - j)]);
-            }
-            R.FX[0][sl(PH, NO - 1)] = ha * s0;
-            R.FX[1][sl(PH, NO - 1)] = hm * s1;
-        }
-        R.rMa_prev = rMa;
-        R.rMm_prev = rMm;
-        R.PX[0][sl(PH, NO)] = (l1 + 2.0f * m1) * q0;
-        R.PX[1][sl(PH, NO)] = m1 * q1;
-        S.uA[PA][0][PAD + lane] = R.U[0][sl(PH, NO)];
-        S.uA[PA][1][PAD + lane] = R.U[1][sl(PH, NO)];
-        S.uB[PA][0][PAD + lane] = R.U[0][sl(PH, 0)];
-        S.uB[PA][1][PAD + lane] = R.U[1][sl(PH, 0)];
-        S.rM[PA][0][PAD + lane] = frcp(l0 + 2.0f * m0);
-        S.rM[PA][1][PAD + lane] = frcp(m0);
-        S.pz[PA][0][PAD + lane] = m0 * zr0;
-        S.pz[PA][1][PAD + lane] = (l0 + 2.0f * m0) * zr1;
-        // B.
-        lds_barrier();
-        // C. Vx at ix+NO; z-fluxes, Vz and the z memory sums at ix
-        {
-            float dzz = 0.0f, dzx = 0.0f;
-#pragma unroll
-            for (int j = 1; j <= NO; ++j) {
-                dzz += a.k.bi[j - 1] * (S.uA[PA][1][PAD + lane + j] - S.uA[PA][1][PAD + lane - j]);
-                dzx += a.k.bi[j - 1] * (S.uA[PA][0][PAD + lane + j] - S.uA[PA][0][PAD + lane - j]);
-            }
-            R.VX[0][sl(PH, NO)] = l1 * dzz;
-            R.VX[1][sl(PH, NO)] = m1 * dzx;
-        }
-        {
-            float sfx = 0.0f, sfz = 0.0f, pzx = 0.0f, pzz = 0.0f;
-#pragma unroll
-            for (int j = 1; j <= NO; ++j) {
-                sfx += a.k.ai[j - 1] * (S.uB[PA][0][PAD + lane + j] - S.uB[PA][0][PAD + lane + 1 - j]);
-                sfz += a.k.ai[j - 1] * (S.uB[PA][1][PAD + lane + j] - S.uB[PA][1][PAD + lane + 1 - j]);
-                pzx += a.k.bi[j - 1] * (S.pz[PA][0][PAD + lane + j] - S.pz[PA][0][PAD + lane - j]);
-                pzz += a.k.bi[j - 1] * (S.pz[PA][1][PAD + lane + j] - S.pz[PA][1][PAD + lane - j]);
-            }
-            const float rMa0 = S.rM[PA][0][PAD + lane], rMm0 = S.rM[PA][1][PAD + lane];
-            const float fzx = sfx * frcp(rMm0 + S.rM[PA][1][PAD + lane + 1]);
-            const float fzz = sfz * frcp(rMa0 + S.rM[PA][0][PAD + lane + 1]);
-            float dxz = 0.0f, dxx = 0.0f;
-#pragma unroll
-            for (int k = 1; k <= NO; ++k) {
-                dxz += a.k.bi[k - 1] * (R.U[1][sl(PH, k)] - R.U[1][sl(PH, -k)]);
-                dxx += a.k.bi[k - 1] * (R.U[0][sl(PH, k)] - R.U[0][sl(PH, -k)]);
-            }
-            S.fz[PA][0][PAD + lane] = fzx;
-            S.fz[PA][1][PAD + lane] = fzz;
-            S.vz[PA][0][PAD + lane] = m0 * dxz;
-            S.vz[PA][1][PAD + lane] = l0 * dxx;
-            R.ps[PA][0] = pzx;
-            R.ps[PA][1] = pzz;
-        }
-        // D. column ix-1: six parts, memory corrections, split update
-        if (!out) return;
-        float Xx = 0.0f, Xz = 0.0f, Zx = 0.0f, Zz = 0.0f, Mx = 0.0f, Mz = 0.0f, sx0 = 0.0f, sx1 = 0.0f;
-#pragma unroll
-        for (int k = 1; k <= NO; ++k) {
-            Xx += a.k.aox[k - 1] * (R.FX[0][sl(PH, k - 2)] - R.FX[0][sl(PH, -k - 1)]);
-            Xz += a.k.aox[k - 1] * (R.FX[1][sl(PH, k - 2)] - R.FX[1][sl(PH, -k - 1)]);
-            Zx += a.k.aoz[k - 1] * (S.fz[PB][0][PAD + lane + k - 1] - S.fz[PB][0][PAD + lane - k]);
-            Zz += a.k.aoz[k - 1] * (S.fz[PB][1][PAD + lane + k - 1] - S.fz[PB][1][PAD + lane - k]);
-            Mx += a.k.bo[k - 1] * ((R.VX[0][sl(PH, k - 1)] - R.VX[0][sl(PH, -k - 1)]) +
-                                  (S.vz[PB][0][PAD + lane + k] - S.vz[PB][0][PAD + lane - k]));
-            Mz += a.k.bo[k - 1] * ((R.VX[1][sl(PH, k - 1)] - R.VX[1][sl(PH, -k - 1)]) +
-                                  (S.vz[PB][1][PAD + lane + k] - S.vz[PB][1][PAD + lane - k]));
-            sx0 += a.k.bi[k - 1] * (R.PX[0][sl(PH, k - 1)] - R.PX[0][sl(PH, -k - 1)]);
-            sx1 += a.k.bi[k - 1] * (R.PX[1][sl(PH, k - 1)] - R.PX[1][sl(PH, -k - 1)]);
-        }
-        Xx = Xx - sx0 / g.dx;
-        Xz = Xz - sx1 / g.dx;
-        if (L.isB) {
-            Zx = Zx - R.ps[PB][0] / g.dz;
-            Zz = Zz - R.ps[PB][1] / g.dz;
-        }
-        const float az = L.az;
-        const float c3 = 2.0f - 4.0f * ax * az;
-        const float pa[2][3] = {{Xx, Zx, Mx}, {Xz, Zz, Mz}};
-        const size_t o = L.off(co);
-#pragma unroll
-        for (int comp = 0; comp < 2; ++comp) {
-            float n1 = (2.0f * sc[comp][0] - (1.0f - ax) * sp[comp][0] + bf * pa[comp][0]) / (1.0f + ax);
-            float n2 = (2.0f * sc[comp][1] - (1.0f - az) * sp[comp][1] + bf * pa[comp][1]) / (1.0f + az);
-            float n3 = (c3 * sc[comp][2] - (1.0f - ax - az) * sp[comp][2] + bf * pa[comp][2]) / (1.0f + ax + az);
-            L.prev[comp][0][o] = n1; L.prev[comp][1][o] = n2; L.prev[comp][2][o] = n3;
-            put(g, comp == 0 ? a.uxp : a.uzp, co, L.ofs, (n1 + n2) + n3);
-        }
-    }
-
-    static __device__ void run(const FrameArgs<NO> &a, int bid)
-    {
-        __shared__ Smem S;
-        const Geo &g = a.g;
-        const FTiles &t = a.t;
-        int q = 0;
-        while (q + 1 < t.n && bid >= t.off[q + 1]) ++q;
-        const int loc = bid - t.off[q];
-        const int bx = loc / t.nbz[q], bz = loc % t.nbz[q];
-        const int xs = t.x0[q] + bx * t.lx[q];
-        const int xe = min(xs + t.lx[q], t.x1[q] + 1);
-        const int zs = t.z0[q] + bz * TZ;
-        const int ze = min(zs + TZ - 1, t.z1[q]);
-        Lane L;
-        L.lane = threadIdx.x;
-        for (int i = L.lane; i < LW; i += NT)
-            if (i < PAD || i >= NT + PAD)
-                for (int p = 0; p < 2; ++p)
-                    for (int c = 0; c < 2; ++c) {
-                        S.uA[p][c][i] = 0.0f; S.uB[p][c][i] = 0.0f; S.rM[p][c][i] = 1.0f;
-                        S.fz[p][c][i] = 0.0f; S.vz[p][c][i] = 0.0f; S.pz[p][c][i] = 0.0f;
-                    }
-        L.r = zs - HZ + L.lane;
-        const bool rowok = L.r >= 1 && L.r <= g.NZ;
-        const int rc = L.r < 1 ? 1 : (L.r > g.NZ ? g.NZ : L.r);
-        L.ofs = (unsigned)(rc - 1) * 4u;
-        L.outrow = L.lane >= HZ && L.lane < HZ + TZ && L.r <= ze;
-        L.isB = rowok && L.r > g.nz;
-        L.az = rowok ? a.st->az[L.r - 1] : 0.0f;
-        // the side strip of this tile's rows above the bottom strip
-        const int sside = xs <= 0 ? S_L : S_R;
-        const int s = L.isB ? S_B : sside;
-        L.cb = (s == S_R) ? g.nx + 1 : 1 - g.npml;
-        L.stride = L.isB ? g.npml : g.nz;
-        L.rb = L.isB ? L.r - g.nz - 1 : L.r - 1;
-        const Strips *st = a.st;
-#pragma unroll
-        for (int c = 0; c < 2; ++c) {
-            L.psx[c] = L.isB ? st->psx[S_B][c] : st->psx[sside][c];
-            L.psz[c] = st->psz[S_B][c];
-#pragma unroll
-            for (int p = 0; p < 3; ++p) {
-                L.cur[c][p] = L.isB ? st->cur[S_B][c][p] : st->cur[sside][c][p];
-                L.prev[c][p] = L.isB ? st->prev[S_B][c][p] : st->prev[sside][c][p];
-            }
-        }
-        Regs R;
-#pragma unroll
-        for (int c = 0; c < 2; ++c) {
-#pragma unroll
-            for (int i = 0; i < P; ++i) { R.U[c][i] = 0.0f; R.FX[c][i] = 0.0f; R.VX[c][i] = 0.0f; R.PX[c][i] = 0.0f; }
-            R.pu[1][c] = 0.0f; R.pl[1][c] = 1.0f; R.p0[1][c] = 1.0f; R.pq[1][c] = 0.0f; R.pr[1][c] = 0.0f;
-            R.ps[0][c] = 0.0f; R.ps[1][c] = 0.0f;
-        }
-        R.rMa_prev = 1.0f; R.rMm_prev = 1.0f;
-        const int x0 = xs - WARM;
-        prefetch(a, L, R, 0, x0, xs, xe);
-        __syncthreads();
-        for (int base = x0; base <= xe; base += P) {
-#define OK_STEP(PH) if constexpr (P > PH) { if (base + PH <= xe) step<PH>(a, S, R, L, base + PH, xs, xe); }
-            OK_STEP(0) OK_STEP(1) OK_STEP(2) OK_STEP(3) OK_STEP(4) OK_STEP(5) OK_STEP(6)
-            OK_STEP(7) OK_STEP(8) OK_STEP(9) OK_STEP(10) OK_STEP(11) OK_STEP(12) OK_STEP(13)
-#undef OK_STEP
-        }
-    }
-};
+__host__ __device__ inline int frame_lds_floats(int tx, int tz, int NO)
+{
+    const int H = 2 * NO - 1;
+    return 4 * (tx + 2 * H) * (tz + 2 * H) + 4 * (tx + 2 * NO - 1) * tz + 4 * tx * (tz + 2 * NO - 1) +
+           2 * (tx + 2 * NO) * tz + 2 * tx * (tz + 2 * NO);
+}
 
 template <int NO>
-__global__ __launch_bounds__(64) void k_frame_out(FrameArgs<NO> a)
+__device__ __forceinline__ void frame_body(const StepArgs &a, const ETiles &et, long bid)
 {
-    OutKernel<NO, 64>::run(a, blockIdx.x);
+    extern __shared__ float sm[];   // frame_lds_floats(tx, tz, NO) of the largest tile of the launch
+    const Geo &g = a.g;
+    const Coefs *cf = a.cf;
+    constexpr int H = 2 * NO - 1;
+    int q = 0;
+    while (q + 1 < et.n && bid >= et.off[q + 1]) ++q;
+    const int loc = (int)(bid - et.off[q]);
+    const int bx = loc / et.nbz[q], bz = loc % et.nbz[q];
+    const int cx0 = et.x0[q] + bx * et.tx[q], cx1 = min(cx0 + et.tx[q] - 1, et.x1[q]);
+    const int cz0 = et.z0[q] + bz * et.tz[q], cz1 = min(cz0 + et.tz[q] - 1, et.z1[q]);
+    const int view = et.view[q];
+    const int tx = cx1 - cx0 + 1, tz = cz1 - cz0 + 1;
+    const int SX = tx + 2 * H, SZ = tz + 2 * H, lx0 = cx0 - H, lz0 = cz0 - H;
+    const int NFX = tx + 2 * NO - 1, fx0 = cx0 - NO;   // x faces f..f+1, f in fx0..fx0+NFX-1
+    const int NFZ = tz + 2 * NO - 1, fz0 = cz0 - NO;   // z faces
+    const int NDZ = tx + 2 * NO, dz0 = cx0 - NO;       // DZ columns
+    const int NDX = tz + 2 * NO, dx0 = cz0 - NO;       // DX rows
+    float *sux = sm, *suz = sux + SX * SZ, *slam = suz + SX * SZ, *smu = slam + SX * SZ;
+    float *fx1x = smu + SX * SZ, *fx1z = fx1x + NFX * tz, *fx2x = fx1z + NFX * tz, *fx2z = fx2x + NFX * tz;
+    float *fz1x = fx2z + NFX * tz, *fz1z = fz1x + tx * NFZ, *fz2x = fz1z + tx * NFZ, *fz2z = fz2x + tx * NFZ;
+    float *dzx = fz2z + tx * NFZ, *dzz = dzx + NDZ * tz;
+    float *dxx = dzz + NDZ * tz, *dxz = dxx + tx * NDX;
+    auto S = [&](int z, int x) { return (x - lx0) * SZ + (z - lz0); };
+    // stage u (in the tile's view), lambda, mu
+    for (int e = threadIdx.x; e < SX * SZ; e += kFrameThreads) {
+        const int x = lx0 + e / SZ, z = lz0 + e % SZ;
+        float vx = 0.0f, vz = 0.0f, vl = 0.0f, vm = 0.0f;
+        if (x >= 1 - g.npml && x <= g.nx + g.npml && z >= 1 && z <= g.NZ) {
+            const size_t c = cell(g, z, x);
+            vx = a.ux[c]; vz = a.uz[c]; vl = a.lam[c]; vm = a.mu[c];
+            if (a.ring_valid) {
+                const int side = tf_side(g, z, x);
+                if (view == 1 && side == 0) { vx -= a.ringx[c]; vz -= a.ringz[c]; }
+                else if (view == 0 && side == 1) { vx += a.ringx[c]; vz += a.ringz[c]; }
+            }
+        }
+        sux[e] = vx; suz[e] = vz; slam[e] = vl; smu[e] = vm;
+    }
+    __syncthreads();
+    const int nxp = g.nx + g.npml, NZ = g.NZ;
+    // phase 1a: x-fluxes (mode 0 on ux, mode 1 on uz)
+    for (int e = threadIdx.x; e < NFX * tz; e += kFrameThreads) {
+        const int f = fx0 + e / tz, z = cz0 + e % tz;
+        const int n1 = imin(imin(f + g.npml - 1, nxp - f), NO), n2 = imin(imin(f + g.npml, nxp - f - 1), NO);
+        const int a0 = S(z, f), a1 = S(z, f + 1);
+        const float ha = hmean(slam[a1] + 2.0f * smu[a1], slam[a0] + 2.0f * smu[a0]);
+        const float hm = hmean(smu[a1], smu[a0]);
+        float s1x = 0.0f, s2x = 0.0f, s1z = 0.0f, s2z = 0.0f;
+#pragma unroll
+        for (int j = 1; j <= NO; ++j) {
+            const int p = S(z, f + j), m = S(z, f + 1 - j);
+            const float dux = sux[p] - sux[m], duz = suz[p] - suz[m];
+            const float c1 = coefz<NO>(cf->c0f, j, n1), c2 = coefz<NO>(cf->c0f, j, n2);
+            s1x = accf(s1x, c1, dux); s2x = accf(s2x, c2, dux);
+            s1z = accf(s1z, c1, duz); s2z = accf(s2z, c2, duz);
+        }
+        fx1x[e] = ha * s1x; fx2x[e] = ha * s2x;
+        fx1z[e] = hm * s1z; fx2z[e] = hm * s2z;
+    }
+    // phase 1b: z-fluxes (mode 1 on ux, mode 0 on uz)
+    for (int e = threadIdx.x; e < tx * NFZ; e += kFrameThreads) {
+        const int x = cx0 + e / NFZ, f = fz0 + e % NFZ;
+        const int n1 = imin(imin(f - 1, NZ - f), NO), n2 = imin(imin(f, NZ - f - 1), NO);
+        const int a0 = S(f, x), a1 = S(f + 1, x);
+        const float ha = hmean(slam[a1] + 2.0f * smu[a1], slam[a0] + 2.0f * smu[a0]);
+        const float hm = hmean(smu[a1], smu[a0]);
+        float s1x = 0.0f, s2x = 0.0f, s1z = 0.0f, s2z = 0.0f;
+#pragma unroll
+        for (int j = 1; j <= NO; ++j) {
+            const int p = S(f + j, x), m = S(f + 1 - j, x);
+            const float dux = sux[p] - sux[m], duz = suz[p] - suz[m];
+            const float c1 = coefz<NO>(cf->c0f, j, n1), c2 = coefz<NO>(cf->c0f, j, n2);
+            s1x = accf(s1x, c1, dux); s2x = accf(s2x, c2, dux);
+            s1z = accf(s1z, c1, duz); s2z = accf(s2z, c2, duz);
+        }
+        fz1x[e] = hm * s1x; fz2x[e] = hm * s2x;
+        fz1z[e] = ha * s1z; fz2z[e] = ha * s2z;
+    }
+    // phase 1c: collocated Dz (cdz_sum) at columns dz0.., rows of the tile
+    for (int e = threadIdx.x; e < NDZ * tz; e += kFrameThreads) {
+        const int x = dz0 + e / tz, z = cz0 + e % tz;
+        const int n = imin(imin(z - 1, NZ - z), NO);
+        float sx = 0.0f, sz = 0.0f;
+#pragma unroll
+        for (int j = 1; j <= NO; ++j) {
+            const int p = S(z + j, x), m = S(z - j, x);
+            const float c = coefz<NO>(cf->c1f, j, n);
+            sx = accf(sx, c, sux[p] - sux[m]);
+            sz = accf(sz, c, suz[p] - suz[m]);
+        }
+        dzx[e] = sx; dzz[e] = sz;
+    }
+    // phase 1d: collocated Dx (cdx_sum) at the tile columns, rows dx0..
+    for (int e = threadIdx.x; e < tx * NDX; e += kFrameThreads) {
+        const int x = cx0 + e / NDX, z = dx0 + e % NDX;
+        const int n = imin(imin(x + g.npml - 1, nxp - x), NO);
+        float sx = 0.0f, sz = 0.0f;
+#pragma unroll
+        for (int j = 1; j <= NO; ++j) {
+            const int p = S(z, x + j), m = S(z, x - j);
+            const float c = coefz<NO>(cf->c1f, j, n);
+            sx = accf(sx, c, sux[p] - sux[m]);
+            sz = accf(sz, c, suz[p] - suz[m]);
+        }
+        dxx[e] = sx; dxz[e] = sz;
+    }
+    __syncthreads();
+    // phase 2: one thread per cell
+    const int t = threadIdx.x;
+    if (t >= tx * tz) return;
+    const int ix = cx0 + t / tz, iz = cz0 + t % tz;
+    const int nox = imin(imin(ix + g.npml - 1, nxp - ix), NO);
+    const int noz = imin(imin(iz - 1, NZ - iz), NO);
+    auto FX = [&](const float *b, int f) { return b[(f - fx0) * tz + (iz - cz0)]; };
+    auto FZ = [&](const float *b, int f) { return b[(ix - cx0) * NFZ + (f - fz0)]; };
+    auto DZ = [&](const float *b, int x) { return b[(x - dz0) * tz + (iz - cz0)]; };
+    auto DX = [&](const float *b, int z) { return b[(ix - cx0) * NDX + (z - dx0)]; };
+    const bool in_reg = (iz >= 1 && iz <= g.nz && ix >= 1 && ix <= g.nx);
+    const float dx2 = in_reg ? (float)(1.0 / (double)(g.dx * g.dx)) : 1.0f / (g.dx * g.dx);
+    float Xx = 0.0f, Xz = 0.0f, Zx = 0.0f, Zz = 0.0f, zxl = 0.0f, zxm = 0.0f, xzm = 0.0f, xzl = 0.0f;
+#pragma unroll
+    for (int k = 1; k <= NO; ++k) {
+        const float c0x = coefz<NO>(cf->c0f, k, nox), c0z = coefz<NO>(cf->c0f, k, noz);
+        const float c1x = coefz<NO>(cf->c1f, k, nox), c1z = coefz<NO>(cf->c1f, k, noz);
+        Xx = accf(Xx, c0x, FX(fx1x, ix + k - 1) - FX(fx2x, ix - k));
+        Xz = accf(Xz, c0x, FX(fx1z, ix + k - 1) - FX(fx2z, ix - k));
+        Zx = accf(Zx, c0z, FZ(fz1x, iz + k - 1) - FZ(fz2x, iz - k));
+        Zz = accf(Zz, c0z, FZ(fz1z, iz + k - 1) - FZ(fz2z, iz - k));
+        const int xp = S(iz, ix + k), xm = S(iz, ix - k), zp = S(iz + k, ix), zm = S(iz - k, ix);
+        // Dx(l Dz uz), Dx(m Dz ux), Dz(m Dx uz), Dz(l Dx ux)
+        zxl = accf(zxl, c1x, slam[xp] * DZ(dzz, ix + k) / g.dz - slam[xm] * DZ(dzz, ix - k) / g.dz);
+        zxm = accf(zxm, c1x, smu[xp] * DZ(dzx, ix + k) / g.dz - smu[xm] * DZ(dzx, ix - k) / g.dz);
+        xzm = accf(xzm, c1z, smu[zp] * DX(dxz, iz + k) / g.dx - smu[zm] * DX(dxz, iz - k) / g.dx);
+        xzl = accf(xzl, c1z, slam[zp] * DX(dxx, iz + k) / g.dx - slam[zm] * DX(dxx, iz - k) / g.dx);
+    }
+    float p[6];
+    p[0] = Xx * dx2;
+    p[3] = Xz * dx2;
+    if (iz == 1) {
+        // free-surface row: half-cell closure derived from the strain energy
+        // of the top layer (rows 1-2, trapezoidal in z); traction free naturally
+        const int r1 = S(1, ix), r2 = S(2, ix);
+        const float sdz2 = 2.0f / (g.dz * g.dz);
+        p[1] = (hmean(smu[r1], smu[r2]) * (sux[r2] - sux[r1])) * sdz2;
+        p[4] = (hmean(slam[r1] + 2.0f * smu[r1], slam[r2] + 2.0f * smu[r2]) * (suz[r2] - suz[r1])) * sdz2;
+        float sa = 0.0f, sb = 0.0f;
+#pragma unroll
+        for (int k = 1; k <= NO; ++k) {
+            const int p1 = S(1, ix + k), p2 = S(2, ix + k), m1 = S(1, ix - k), m2 = S(2, ix - k);
+            const float c = coefz<NO>(cf->c1f, k, nox);
+            const float lp = 0.5f * (slam[p1] + slam[p2]), lm = 0.5f * (slam[m1] + slam[m2]);
+            const float mp = 0.5f * (smu[p1] + smu[p2]), mm = 0.5f * (smu[m1] + smu[m2]);
+            sa = accf(sa, c, lp * (suz[p2] - suz[p1]) - lm * (suz[m2] - suz[m1]));
+            sb = accf(sb, c, mp * (sux[p2] - sux[p1]) - mm * (sux[m2] - sux[m1]));
+        }
+        const float lbar = 0.5f * (slam[r1] + slam[r2]), mbar = 0.5f * (smu[r1] + smu[r2]);
+        const float dxuz = (DX(dxz, 1) + DX(dxz, 2)) / g.dx;
+        const float dxux = (DX(dxx, 1) + DX(dxx, 2)) / g.dx;
+        p[2] = (sa / g.dx + mbar * dxuz) / g.dz;
+        p[5] = (sb / g.dx + lbar * dxux) / g.dz;
+    } else {
+        p[1] = Zx * (1.0f / (g.dz * g.dz));
+        p[4] = Zz * (1.0f / (g.dz * g.dz));
+        p[2] = zxl / g.dx + xzm / g.dz;
+        p[5] = zxm / g.dx + xzl / g.dz;
+    }
+    frame_update<NO>(a, iz, ix, p);
+}
+
+template <int NO>
+__global__ __launch_bounds__(kFrameThreads) void k_frame(StepArgs a, ETiles t)
+{
+    frame_body<NO>(a, t, blockIdx.x);
 }
 
 }  // namespace fdfk

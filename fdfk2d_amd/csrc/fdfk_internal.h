@@ -109,12 +109,13 @@ struct StepArgs {
     int it;
 };
 
-// edge-kernel work decomposition: up to four bands (columns x0..x1, rows
-// z0..z1), each cut into nbx x nbz tiles of tx x tz cells; off[] are block
-// prefix counts
+// frame-kernel work decomposition: up to four bands (columns x0..x1, rows
+// z0..z1), each cut into nbx x nbz tiles of tx x tz cells and lying on one
+// side of the hybrid interface (view 0: total-field region, 1: padding);
+// off[] are block prefix counts
 struct ETiles {
     int n;
-    int x0[4], x1[4], z0[4], z1[4], tx[4], tz[4], nbx[4], nbz[4];
+    int x0[4], x1[4], z0[4], z1[4], tx[4], tz[4], nbx[4], nbz[4], view[4];
     int off[5];
 };
 
@@ -139,28 +140,6 @@ struct IntArgs {
     int lx;   // output columns per block
     int nbx;  // blocks along x
     int nbz;  // blocks along z
-};
-
-// Tiles of the full-order padding cells (PML strips) for k_frame_out: up to
-// three rectangles (left strip, right strip, bottom strip), each cut into
-// nbx x nbz blocks of lx columns and TZ rows; off[] are block prefix counts.
-struct FTiles {
-    int n;
-    int x0[3], x1[3], z0[3], z1[3], lx[3], nbx[3], nbz[3];
-    int off[4];
-};
-
-template <int NO>
-struct FrameArgs {
-    Geo g;
-    ICoef<NO> k;
-    const float *ux, *uz;
-    float *uxp, *uzp;
-    const float *bf, *lam, *mu;
-    const Strips *st;
-    const float *ringx, *ringz;
-    int corr;
-    FTiles t;
 };
 
 }  // namespace fdfk

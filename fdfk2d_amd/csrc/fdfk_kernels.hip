@@ -5,24 +5,23 @@
 // One step n -> n+1 is four launches on the handle's stream:
 //   k_ring_prep background u0 on the hybrid ring at this step    ring only
 //   k_psi       first-derivative PML memories psi (level n)       strips only
-//   k_interior  full-order fused stencil + leapfrog                 ~97% of cells
-//   k_frame     everything else: free-surface rows, reduced-order   the frame
-//               edge stencils, split-field PML, hybrid TF/SF ring
+//   k_interior2 full-order fused stencil + leapfrog over the       ~97% of cells
+//               total-field region below the surface rows
+//   k_frame     everything else (fdfk_frame.hip): free-surface     the frame
+//               rows, reduced-order edge stencils, split-field
+//               PML strips, hybrid TF/SF ring
 // and, on recording iterations, k_record before them.
 //
-// k_interior is the HBM-bound hot kernel: 64-wide wavefronts run along z
-// (coalesced rows), each block marches along x keeping x-neighbourhoods in
-// register rings (no x halo re-reads) and exchanging z-neighbourhoods
-// through LDS.  It reads u^n, u^{n-1}, dt^2/rho, lambda, mu once and writes
-// u^{n+1} in place of u^{n-1}: 36 algorithmic bytes per cell-update in FP32.
+// k_interior2 is the hot kernel: 64-wide wavefronts run along z (coalesced
+// rows), each block marches along x keeping x-neighbourhoods in register
+// rings (no x halo re-reads) and exchanging z-neighbourhoods through LDS.
+// It reads u^n, u^{n-1}, dt^2/rho, lambda, mu once and writes u^{n+1} in
+// place of u^{n-1}: 36 algorithmic bytes per cell-update in FP32.
 //
-// k_frame_full evaluates full-order frame cells with the same FP32
-// shared-flux stencil as k_interior; k_frame_edge evaluates the cells next to
-// the padded-grid edges and the free surface with the reduced-order stencils
-// of src/Submain.f90:192-195 (FP32, the operator structure of the literal
-// restatement).  k_record keeps the reference's FP64 interpolation weights
-// and accumulation exactly.  The hybrid TF/SF correction enters as reads of
-// the per-step ring field (k_ring_prep) instead of a second stencil pass.
+// k_record keeps the reference's FP64 interpolation weights and
+// accumulation exactly.  The hybrid TF/SF correction enters as reads of the
+// per-step ring field (k_ring_prep) in the view of the output cell's side
+// instead of a second stencil pass.
 #include "fdfk_internal.h"
 
 namespace fdfk {
@@ -35,17 +34,12 @@ __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x)
 __device__ __forceinline__ float accf(float s, float c, float d) { return s + c * d; }
 // harmonic mean 2/(1/a + 1/b); a zero modulus gives 0 (rcp(0) = inf)
 __device__ __forceinline__ float hmean(float a, float b) { return 2.0f * frcp(frcp(a) + frcp(b)); }
-__device__ __forceinline__ float matval(int mode, float lam, float mu)
-{
-    return mode == 0 ? lam + 2.0f * mu : mu;
-}
 
-// The pointwise operators below run their stencil loops to the full order
-// NO (inner loops unrolled, so the loads of one outer term are issued
-// together) and give the terms beyond the reduced order of the cell a zero
-// coefficient.  Those
-// terms may address cells beyond the padded grid: addresses are clamped to
-// the grid (the values are finite and multiplied by zero).
+// The pointwise operators below (PML memories and their corrections) run
+// their stencil loops to the full order NO and give the terms beyond the
+// reduced order of the cell a zero coefficient.  Those terms may address
+// cells beyond the padded grid: addresses are clamped to the grid (the
+// values are finite and multiplied by zero).
 __device__ __forceinline__ size_t cellc(const Geo &g, int iz, int ix)
 {
     ix = ix < 1 - g.npml ? 1 - g.npml : (ix > g.nx + g.npml ? g.nx + g.npml : ix);
@@ -122,143 +116,6 @@ __device__ __forceinline__ float cdz_sum(const Geo &g, const Coefs *cf, const R 
 #pragma unroll
     for (int k = 1; k <= NO; ++k) s = accf(s, coefz<NO>(cf->c1f, k, noz), f(iz + k, ix) - f(iz - k, ix));
     return s;
-}
-
-// D2 staggered composition along x with harmonic means of M (mode 0:
-// lambda+2mu, 1: mu) and the order reduction of diffx2_reg/diffx2_pml
-template <int NO, class R, class MR>
-__device__ __forceinline__ float x2_mode(const Geo &g, const Coefs *cf, const MR &lam, const MR &mu, int mode,
-                                         const R &u, int iz, int ix)
-{
-    const int nx = g.nx, npml = g.npml;
-    const int nox = imin(imin(ix + npml - 1, nx + npml - ix), NO);
-    float s = 0.0f;
-    auto M = [&](int jx) { return matval(mode, lam(iz, jx), mu(iz, jx)); };
-#pragma unroll 1
-    for (int k = 1; k <= NO; ++k) {
-        const int i = k - 1;  // (2k-1)/2
-        const int n1 = imin(imin(ix + i + npml - 1, nx + npml - ix - i), NO);
-        const int n2 = imin(imin(ix - i + npml - 1, nx + npml - ix + i), NO);
-        float s1 = 0.0f, s2 = 0.0f;
-#pragma unroll
-        for (int j = 1; j <= NO; ++j) {
-            s1 = accf(s1, coefz<NO>(cf->c0f, j, n1), u(iz, ix + j + k - 1) - u(iz, ix - j + k));
-            s2 = accf(s2, coefz<NO>(cf->c0f, j, n2), u(iz, ix + j - k) - u(iz, ix - j - k + 1));
-        }
-        float t = hmean(M(ix + k), M(ix + k - 1)) * s1 - hmean(M(ix - k + 1), M(ix - k)) * s2;
-        s = accf(s, coefz<NO>(cf->c0f, k, nox), t);
-    }
-    const bool in_reg = (iz >= 1 && iz <= g.nz && ix >= 1 && ix <= nx);
-    const float dx2 = in_reg ? (float)(1.0 / (double)(g.dx * g.dx)) : 1.0f / (g.dx * g.dx);
-    return s * dx2;
-}
-
-template <int NO, class R, class MR>
-__device__ __forceinline__ float z2_mode(const Geo &g, const Coefs *cf, const MR &lam, const MR &mu, int mode,
-                                         const R &u, int iz, int ix)
-{
-    const int nz = g.nz, npml = g.npml, izt = 1;
-    const int noz = imin(imin(iz - izt, nz + npml - iz), NO);
-    float s = 0.0f;
-    auto M = [&](int jz) { return matval(mode, lam(jz, ix), mu(jz, ix)); };
-#pragma unroll 1
-    for (int k = 1; k <= NO; ++k) {
-        const int i = k - 1;
-        const int n1 = imin(imin(iz + i - izt, nz + npml - iz - i), NO);
-        const int n2 = imin(imin(iz - i - izt, nz + npml - iz + i), NO);
-        float s1 = 0.0f, s2 = 0.0f;
-#pragma unroll
-        for (int j = 1; j <= NO; ++j) {
-            s1 = accf(s1, coefz<NO>(cf->c0f, j, n1), u(iz + j + k - 1, ix) - u(iz - j + k, ix));
-            s2 = accf(s2, coefz<NO>(cf->c0f, j, n2), u(iz + j - k, ix) - u(iz - j - k + 1, ix));
-        }
-        float t = hmean(M(iz + k), M(iz + k - 1)) * s1 - hmean(M(iz - k + 1), M(iz - k)) * s2;
-        s = accf(s, coefz<NO>(cf->c0f, k, noz), t);
-    }
-    return s * (1.0f / (g.dz * g.dz));
-}
-
-// Dx(mat * Dz u)  (diffzx at one output cell)
-template <int NO, class R, class MR>
-__device__ __forceinline__ float zx_pt(const Geo &g, const Coefs *cf, const MR &mat, const R &u, int iz, int ix)
-{
-    const int nox = imin(imin(ix + g.npml - 1, g.nx + g.npml - ix), NO);
-    float s = 0.0f;
-#pragma unroll 1
-    for (int k = 1; k <= NO; ++k) {
-        float vp = mat(iz, ix + k) * cdz_sum<NO>(g, cf, u, iz, ix + k) / g.dz;
-        float vm = mat(iz, ix - k) * cdz_sum<NO>(g, cf, u, iz, ix - k) / g.dz;
-        s = accf(s, coefz<NO>(cf->c1f, k, nox), vp - vm);
-    }
-    return s / g.dx;
-}
-// Dz(mat * Dx u)  (diffxz at one output cell)
-template <int NO, class R, class MR>
-__device__ __forceinline__ float xz_pt(const Geo &g, const Coefs *cf, const MR &mat, const R &u, int iz, int ix)
-{
-    const int noz = imin(imin(iz - 1, g.nz + g.npml - iz), NO);
-    float s = 0.0f;
-#pragma unroll 1
-    for (int k = 1; k <= NO; ++k) {
-        float vp = mat(iz + k, ix) * cdx_sum<NO>(g, cf, u, iz + k, ix) / g.dx;
-        float vm = mat(iz - k, ix) * cdx_sum<NO>(g, cf, u, iz - k, ix) / g.dx;
-        s = accf(s, coefz<NO>(cf->c1f, k, noz), vp - vm);
-    }
-    return s / g.dz;
-}
-
-// One of the eight terms that make up the six RHS parts {Xx, Zx, Mx, Xz,
-// Zz, Mz} at one cell (free surface at iz = 1):
-//   0 Xx = Dx(M Dx ux)   1 Zx = Dz(m Dz ux)   2 Dx(l Dz uz)   3 Dz(m Dx uz)
-//   4 Xz = Dx(m Dx uz)   5 Zz = Dz(M Dz uz)   6 Dx(m Dz ux)   7 Dz(l Dx ux)
-// with Mx = t2 + t3 and Mz = t6 + t7.
-template <int NO, class R, class MR>
-__device__ __forceinline__ float eval_task(int task, const Geo &g, const Coefs *cf, const MR &lam, const MR &mu,
-                                           const R &ux, const R &uz, int iz, int ix)
-{
-    if (iz == 1) {
-        // free-surface row: half-cell closure derived from the strain energy
-        // of the top layer (rows 1-2, trapezoidal in z); traction free naturally
-        const float sdz2 = 2.0f / (g.dz * g.dz);
-        switch (task) {
-        case 0: return x2_mode<NO>(g, cf, lam, mu, 0, ux, iz, ix);
-        case 4: return x2_mode<NO>(g, cf, lam, mu, 1, uz, iz, ix);
-        case 1: return (hmean(mu(1, ix), mu(2, ix)) * (ux(2, ix) - ux(1, ix))) * sdz2;
-        case 5: {
-            const float M1 = lam(1, ix) + 2.0f * mu(1, ix), M2 = lam(2, ix) + 2.0f * mu(2, ix);
-            return (hmean(M1, M2) * (uz(2, ix) - uz(1, ix))) * sdz2;
-        }
-        case 2:
-        case 6: {
-            // task 2: (Dx(lbar dz uz) + mbar Dx uz) / dz, task 6: (Dx(mbar dz ux) + lbar Dx ux) / dz
-            const bool t2 = task == 2;
-            const R &w = t2 ? uz : ux;
-            const MR &A = t2 ? lam : mu;
-            const int nox = imin(imin(ix + g.npml - 1, g.nx + g.npml - ix), NO);
-            float sa = 0.0f;
-#pragma unroll
-            for (int k = 1; k <= NO; ++k) {
-                float ap = 0.5f * (A(1, ix + k) + A(2, ix + k)), am = 0.5f * (A(1, ix - k) + A(2, ix - k));
-                sa = accf(sa, coefz<NO>(cf->c1f, k, nox), ap * (w(2, ix + k) - w(1, ix + k)) - am * (w(2, ix - k) - w(1, ix - k)));
-            }
-            const MR &B = t2 ? mu : lam;
-            const float bbar = 0.5f * (B(1, ix) + B(2, ix));
-            const float dxw = (cdx_sum<NO>(g, cf, w, 1, ix) + cdx_sum<NO>(g, cf, w, 2, ix)) / g.dx;
-            return (sa / g.dx + bbar * dxw) / g.dz;
-        }
-        default: return 0.0f;
-        }
-    }
-    switch (task) {
-    case 0: return x2_mode<NO>(g, cf, lam, mu, 0, ux, iz, ix);
-    case 1: return z2_mode<NO>(g, cf, lam, mu, 1, ux, iz, ix);
-    case 2: return zx_pt<NO>(g, cf, lam, uz, iz, ix);
-    case 3: return xz_pt<NO>(g, cf, mu, uz, iz, ix);
-    case 4: return x2_mode<NO>(g, cf, lam, mu, 1, uz, iz, ix);
-    case 5: return z2_mode<NO>(g, cf, lam, mu, 0, uz, iz, ix);
-    case 6: return zx_pt<NO>(g, cf, mu, ux, iz, ix);
-    default: return xz_pt<NO>(g, cf, lam, ux, iz, ix);
-    }
 }
 
 __device__ __forceinline__ int strip_of(const Geo &g, int iz, int ix)
@@ -356,111 +213,6 @@ __device__ __forceinline__ void frame_update(const StepArgs &a, int iz, int ix, 
         float tot = (n1 + n2) + n3;
         if (comp == 0) a.uxp[c] = tot; else a.uzp[c] = tot;
     }
-}
-
-// Edge kernel: one block per tile of an edge band (at most kEdgeThreads
-// cells).  The tile and its full stencil halo (2*NO-1 cells each way; cells
-// beyond the grid are staged as zeros) of u, lambda, mu and the ring
-// background are staged in LDS once; each thread then evaluates one cell
-// from LDS (the eight terms in turn), applies the memory corrections and
-// the update.  The operators always run to the full order with zero
-// coefficients beyond a cell's reduced order, so every read stays inside the
-// staged halo and needs no clamping.
-constexpr int kEdgeThreads = 256;
-constexpr int kEdgeLds = 1152;   // staged cells per array (host checks the tile fits)
-
-struct LdsTile {
-    int lx0, lz0, tw;
-    __device__ __forceinline__ int idx(int iz, int ix) const { return (ix - lx0) * tw + (iz - lz0); }
-};
-// Field reader on the staged tile.  With RING, for a ring cell of side
-// sigma, neighbours on the other side are seen in the output cell's
-// representation: the inside ring reads u + u0 outside (total field), the
-// outside ring reads u - u0 inside (scattered field).  By linearity this
-// equals L[u] +/- L[u0 masked], the TF/SF correction of the hybrid method.
-// sigma = -1: plain read (cells outside the ring).
-template <bool RING>
-struct LdsRd {
-    const LdsTile *t;
-    const float *p, *ring;
-    const Geo *g;
-    int sigma;
-    __device__ __forceinline__ float operator()(int iz, int ix) const
-    {
-        const int k = t->idx(iz, ix);
-        float v = p[k];
-        if (RING && sigma >= 0 && tf_side(*g, iz, ix) != sigma) v = sigma == 0 ? v + ring[k] : v - ring[k];
-        return v;
-    }
-};
-struct LdsMat {
-    const LdsTile *t;
-    const float *p;
-    __device__ __forceinline__ float operator()(int iz, int ix) const { return p[t->idx(iz, ix)]; }
-};
-
-template <int NO, bool RING>
-__device__ __forceinline__ void edge_cell(const StepArgs &a, const LdsTile &T, const float *sux, const float *suz,
-                                          const float *slam, const float *smu, const float *srx, const float *srz,
-                                          int iz, int ix, int sig)
-{
-    const Geo &g = a.g;
-    LdsRd<RING> rx{&T, sux, srx, &g, sig}, rz{&T, suz, srz, &g, sig};
-    LdsMat lam{&T, slam}, mu{&T, smu};
-    float e[8];
-#pragma unroll 1
-    for (int k = 0; k < 8; ++k) e[k] = eval_task<NO>(k, g, a.cf, lam, mu, rx, rz, iz, ix);
-    float p[6] = {e[0], e[1], e[2] + e[3], e[4], e[5], e[6] + e[7]};
-    frame_update<NO>(a, iz, ix, p);
-}
-
-template <int NO>
-__device__ __forceinline__ void frame_edge_body(const StepArgs &a, const ETiles &et, long bid)
-{
-    __shared__ float sux[kEdgeLds], suz[kEdgeLds], slam[kEdgeLds], smu[kEdgeLds], srx[kEdgeLds], srz[kEdgeLds];
-    const Geo &g = a.g;
-    int q = 0;
-    while (q + 1 < et.n && bid >= et.off[q + 1]) ++q;
-    const int loc = (int)(bid - et.off[q]);
-    const int bx = loc / et.nbz[q], bz = loc % et.nbz[q];
-    const int cx0 = et.x0[q] + bx * et.tx[q], cx1 = min(cx0 + et.tx[q] - 1, et.x1[q]);
-    const int cz0 = et.z0[q] + bz * et.tz[q], cz1 = min(cz0 + et.tz[q] - 1, et.z1[q]);
-    constexpr int H = 2 * NO - 1;
-    LdsTile T;
-    T.lx0 = cx0 - H; T.lz0 = cz0 - H;
-    T.tw = cz1 - cz0 + 1 + 2 * H;
-    const int ne = (cx1 - cx0 + 1 + 2 * H) * T.tw;
-    // the tile holds ring cells iff it meets the ring's bounding box
-    // (columns 1-nbd..nx+nbd, rows 1..nz+nbd) minus the region inside it
-    const bool ring = a.ring_valid && !(cx1 < 1 - g.nbd || cx0 > g.nx + g.nbd || cz0 > g.nz + g.nbd ||
-                                        (cx0 > g.nbd && cx1 <= g.nx - g.nbd && cz1 <= g.nz - g.nbd));
-    for (int e = threadIdx.x; e < ne; e += kEdgeThreads) {
-        const int ix = T.lx0 + e / T.tw, iz = T.lz0 + e % T.tw;
-        float vx = 0.0f, vz = 0.0f, vl = 0.0f, vm = 0.0f, wx = 0.0f, wz = 0.0f;
-        if (ix >= 1 - g.npml && ix <= g.nx + g.npml && iz >= 1 && iz <= g.NZ) {
-            const size_t c = cell(g, iz, ix);
-            vx = a.ux[c]; vz = a.uz[c]; vl = a.lam[c]; vm = a.mu[c];
-            if (ring) { wx = a.ringx[c]; wz = a.ringz[c]; }
-        }
-        sux[e] = vx; suz[e] = vz; slam[e] = vl; smu[e] = vm; srx[e] = wx; srz[e] = wz;
-    }
-    __syncthreads();
-    const int rows = cz1 - cz0 + 1;
-    const int t = threadIdx.x;
-    if (t >= rows * (cx1 - cx0 + 1)) return;
-    const int ix = cx0 + t / rows, iz = cz0 + t % rows;
-    if (ring) {
-        int sig = -1, arr, which;
-        if (bg_off(g, iz, ix, 0, arr, which) >= 0) sig = which;
-        edge_cell<NO, true>(a, T, sux, suz, slam, smu, srx, srz, iz, ix, sig);
-    } else {
-        edge_cell<NO, false>(a, T, sux, suz, slam, smu, srx, srz, iz, ix, -1);
-    }
-}
-template <int NO>
-__global__ __launch_bounds__(kEdgeThreads) void k_frame_edge(StepArgs a, ETiles t)
-{
-    frame_edge_body<NO>(a, t, blockIdx.x);
 }
 
 // background on the hybrid ring at iteration it, scattered into the
