@@ -234,7 +234,7 @@ IntArgs<NO> interior_args(fdfk_solver *s)
     a.bf = s->bf; a.lam = s->lam; a.mu = s->mu;
     a.ringx = s->ringx; a.ringz = s->ringz;
     a.corr = s->bg_valid && s->it <= s->nt;
-    constexpr int TZ = NT - 2 * (2 * NO - 1);
+    constexpr int TZ = IntKernel2<NO, NT>::TZ;
     int ncols = g.ix1 - g.ix0 + 1, nrows = g.iz1 - g.iz0 + 1;
     a.lx = s->lx;
     a.nbx = (ncols + a.lx - 1) / a.lx;
@@ -262,8 +262,8 @@ int choose_lx(const Geo &g)
         hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_interior2<NO, NT>, NT, 0) != hipSuccess ||
         ncu <= 0 || bpc <= 0)
         return 45;
-    constexpr int TZ = NT - 2 * (2 * NO - 1);
-    constexpr int WARM = 4 * NO - 2;
+    constexpr int TZ = IntKernel2<NO, NT>::TZ;
+    constexpr int WARM = IntKernel2<NO, NT>::WARM;
     const long slots = (long)ncu * bpc;
     const int ncols = g.ix1 - g.ix0 + 1, nrows = g.iz1 - g.iz0 + 1;
     const long nbz = (nrows + TZ - 1) / TZ;
@@ -308,16 +308,22 @@ void launch_interior(fdfk_solver *s)
 }
 
 // one frame band cut into tiles of at most kFrameThreads cells whose staged
-// halo and intermediates fit the kernel's LDS: thin bands take all their
-// rows (columns) per tile, wide ones 16 x 16 tiles
+// halo and intermediates fit the kernel's LDS
 void add_band(ETiles &t, int x0, int x1, int z0, int z1, int norder, int view)
 {
     if (x1 < x0 || z1 < z0) return;
     const int cols = x1 - x0 + 1, rows = z1 - z0 + 1;
+    // the thin side of a band is split into equal tiles of at most 16
     int tx, tz;
-    if (rows <= 8) { tz = rows; tx = std::max(1, kFrameThreads / tz); }
-    else if (cols <= 8) { tx = cols; tz = std::max(1, kFrameThreads / tx); }
-    else { tx = 16; tz = 16; }
+    if (rows <= cols) {
+        const int nz = (rows + 15) / 16;
+        tz = (rows + nz - 1) / nz;
+        tx = std::max(1, kFrameThreads / tz);
+    } else {
+        const int nxt = (cols + 15) / 16;
+        tx = (cols + nxt - 1) / nxt;
+        tz = std::max(1, kFrameThreads / tx);
+    }
     while (frame_lds_floats(tx, tz, norder) > kFrameLds) {
         if (tx >= tz) --tx; else --tz;
     }

@@ -391,19 +391,24 @@ __device__ __forceinline__ void lds_barrier()
 template <int NO, int NT>
 struct IntKernel2 {
     static constexpr int HZ = 2 * NO - 1;
-    static constexpr int TZ = NT - 2 * HZ;
+    static constexpr int TZ = NT - 2 * HZ;      // output rows per block
     static constexpr int P0 = 3 * NO + 1;
     static constexpr int P = P0 + (P0 & 1);     // even ring period: LDS parity is compile-time
     static constexpr int PAD = NO + 1;
     static constexpr int WARM = 4 * NO - 2;
     static constexpr int LW = NT + 2 * PAD;
 
+    // LDS, lane-major: row r of the block holds SW words, ten per column
+    // parity (uA, uB, rM, fz, vz for the two components); the odd stride
+    // keeps the 64 lanes of a wavefront on distinct banks, and every access
+    // of a step is one base register plus an immediate offset
+    //   uA: u at column ix+NO   uB: u at column ix   rM: 1/(l+2m), 1/m at ix
+    //   fz: z-fluxes at ix      vz: m*Dx uz, l*Dx ux at ix
+    enum { UA = 0, UB = 2, RM = 4, FZ = 6, VZ = 8 };
+    static constexpr int SW = 21;
     struct Smem {
-        float uA[2][2][LW];   // [parity][comp] u at column ix+NO
-        float uB[2][2][LW];   // u at column ix
-        float rM[2][2][LW];   // 1/(l+2m), 1/m at column ix
-        float fz[2][2][LW];   // z-fluxes at column ix
-        float vz[2][2][LW];   // m*Dx uz, l*Dx ux at column ix
+        float m[LW * SW];
+        __device__ __forceinline__ float &at(int par, int q, int c, int i) { return m[i * SW + par * 10 + q + c]; }
     };
 
     static __device__ __forceinline__ int sl(int ph, int j) { return ((ph + j) % P + P) % P; }
@@ -427,25 +432,32 @@ struct IntKernel2 {
         unsigned ofs;       // clamped row byte offset inside a column
         bool outrow;
     };
-    // element at byte offset o of column c: a buffer resource on the column
-    // (scalar base, built per column) plus the lane's 32-bit row offset
-    static __device__ __forceinline__ __amdgpu_buffer_rsrc_t colrsrc(const Geo &g, const float *base, int c)
+    // Buffer resources on every array, based at the block's first column xb
+    // (built once per block); an element is rsrc + scalar column offset
+    // (c - xb) * ld * 4 + the lane's 32-bit row offset.  Offsets stay far
+    // below 4 GiB because a block spans lx + 6*NO columns.
+    struct Rsrc {
+        __amdgpu_buffer_rsrc_t ux, uz, uxp, uzp, bf, lam, mu, rx, rz;
+        int xb, pitch;   // first column, column pitch in bytes
+        __device__ __forceinline__ int soff(int c) const { return (c - xb) * pitch; }
+    };
+    static __device__ __forceinline__ __amdgpu_buffer_rsrc_t mk(const Geo &g, const float *base, int xb)
     {
-        const float *cb = base + (size_t)(c + g.npml - 1) * (size_t)g.ld;
-        return __builtin_amdgcn_make_buffer_rsrc((void *)cb, 0, g.ld * 4, kBufferFlags);
+        const float *cb = base + (size_t)(xb + g.npml - 1) * (size_t)g.ld;
+        return __builtin_amdgcn_make_buffer_rsrc((void *)cb, 0, 0x7fffffff, kBufferFlags);
     }
-    static __device__ __forceinline__ float at(const Geo &g, const float *base, int c, unsigned o)
+    static __device__ __forceinline__ float ld(__amdgpu_buffer_rsrc_t r, unsigned o, int so)
     {
-        return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(colrsrc(g, base, c), o, 0, 0));
+        return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, o, so, 0));
     }
-    static __device__ __forceinline__ void put(const Geo &g, float *base, int c, unsigned o, float v)
+    static __device__ __forceinline__ void st(__amdgpu_buffer_rsrc_t r, unsigned o, int so, float v)
     {
-        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), colrsrc(g, base, c), o, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, o, so, 0);
     }
 
     template <int PH>
-    static __device__ __forceinline__ void step(const IntArgs<NO> &a, Smem &S, Regs &R, const Lane &L, bool bc,
-                                                int ix, int xs, int xe)
+    static __device__ __forceinline__ void step(const IntArgs<NO> &a, const Rsrc &B, Smem &S, Regs &R, const Lane &L,
+                                                bool bc, int ix, int xs, int xe)
     {
         constexpr int PA = PH & 1, PB = 1 - PA;
         const Geo &g = a.g;
@@ -460,17 +472,20 @@ struct IntKernel2 {
         const float l0 = R.p0[PA][0], m0 = R.p0[PA][1];
         {
             const int cn = colc(ix + 2 * NO), cm = colc(ix + 1 + NO), c0 = colc(ix + 1), cx = colc(ix);
-            R.pu[PB][0] = at(g, a.ux, cn, o);
-            R.pu[PB][1] = at(g, a.uz, cn, o);
+            const int sn = B.soff(cn), sm = B.soff(cm), s0 = B.soff(c0), sx = B.soff(cx);
+            R.pu[PB][0] = ld(B.ux, o, sn);
+            R.pu[PB][1] = ld(B.uz, o, sn);
             if (bc && outside(g, L.r, cn)) {
-                R.pu[PB][0] += at(g, a.ringx, cn, o);
-                R.pu[PB][1] += at(g, a.ringz, cn, o);
+                R.pu[PB][0] += ld(B.rx, o, sn);
+                R.pu[PB][1] += ld(B.rz, o, sn);
             }
-            R.pl[PB][0] = at(g, a.lam, cm, o); R.pl[PB][1] = at(g, a.mu, cm, o);
-            R.p0[PB][0] = at(g, a.lam, c0, o); R.p0[PB][1] = at(g, a.mu, c0, o);
-            R.po[PA][0] = at(g, a.bf, cx, o);
-            R.po[PA][1] = at(g, a.uxp, cx, o);
-            R.po[PA][2] = at(g, a.uzp, cx, o);
+            R.pl[PB][0] = ld(B.lam, o, sm); R.pl[PB][1] = ld(B.mu, o, sm);
+            R.p0[PB][0] = ld(B.lam, o, s0); R.p0[PB][1] = ld(B.mu, o, s0);
+            if (ix >= xs) {   // update inputs only for output columns (block-uniform)
+                R.po[PA][0] = ld(B.bf, o, sx);
+                R.po[PA][1] = ld(B.uxp, o, sx);
+                R.po[PA][2] = ld(B.uzp, o, sx);
+            }
         }
         const float rMa = frcp(l1 + 2.0f * m1), rMm = frcp(m1);
         {
@@ -486,12 +501,12 @@ struct IntKernel2 {
         }
         R.rMa_prev = rMa;
         R.rMm_prev = rMm;
-        S.uA[PA][0][PAD + lane] = R.U[0][sl(PH, NO)];
-        S.uA[PA][1][PAD + lane] = R.U[1][sl(PH, NO)];
-        S.uB[PA][0][PAD + lane] = R.U[0][sl(PH, 0)];
-        S.uB[PA][1][PAD + lane] = R.U[1][sl(PH, 0)];
-        S.rM[PA][0][PAD + lane] = frcp(l0 + 2.0f * m0);
-        S.rM[PA][1][PAD + lane] = frcp(m0);
+        S.at(PA, UA, 0, PAD + lane) = R.U[0][sl(PH, NO)];
+        S.at(PA, UA, 1, PAD + lane) = R.U[1][sl(PH, NO)];
+        S.at(PA, UB, 0, PAD + lane) = R.U[0][sl(PH, 0)];
+        S.at(PA, UB, 1, PAD + lane) = R.U[1][sl(PH, 0)];
+        S.at(PA, RM, 0, PAD + lane) = frcp(l0 + 2.0f * m0);
+        S.at(PA, RM, 1, PAD + lane) = frcp(m0);
         // B.
         lds_barrier();
         // C. Vx at ix+NO; z-fluxes and Vz at ix
@@ -499,8 +514,8 @@ struct IntKernel2 {
             float dzz = 0.0f, dzx = 0.0f;
 #pragma unroll
             for (int j = 1; j <= NO; ++j) {
-                dzz += a.k.bi[j - 1] * (S.uA[PA][1][PAD + lane + j] - S.uA[PA][1][PAD + lane - j]);
-                dzx += a.k.bi[j - 1] * (S.uA[PA][0][PAD + lane + j] - S.uA[PA][0][PAD + lane - j]);
+                dzz += a.k.bi[j - 1] * (S.at(PA, UA, 1, PAD + lane + j) - S.at(PA, UA, 1, PAD + lane - j));
+                dzx += a.k.bi[j - 1] * (S.at(PA, UA, 0, PAD + lane + j) - S.at(PA, UA, 0, PAD + lane - j));
             }
             R.VX[0][sl(PH, NO)] = l1 * dzz;
             R.VX[1][sl(PH, NO)] = m1 * dzx;
@@ -509,22 +524,22 @@ struct IntKernel2 {
             float sfx = 0.0f, sfz = 0.0f;
 #pragma unroll
             for (int j = 1; j <= NO; ++j) {
-                sfx += a.k.ai[j - 1] * (S.uB[PA][0][PAD + lane + j] - S.uB[PA][0][PAD + lane + 1 - j]);
-                sfz += a.k.ai[j - 1] * (S.uB[PA][1][PAD + lane + j] - S.uB[PA][1][PAD + lane + 1 - j]);
+                sfx += a.k.ai[j - 1] * (S.at(PA, UB, 0, PAD + lane + j) - S.at(PA, UB, 0, PAD + lane + 1 - j));
+                sfz += a.k.ai[j - 1] * (S.at(PA, UB, 1, PAD + lane + j) - S.at(PA, UB, 1, PAD + lane + 1 - j));
             }
-            const float rMa0 = S.rM[PA][0][PAD + lane], rMm0 = S.rM[PA][1][PAD + lane];
-            const float fzx = sfx * frcp(rMm0 + S.rM[PA][1][PAD + lane + 1]);
-            const float fzz = sfz * frcp(rMa0 + S.rM[PA][0][PAD + lane + 1]);
+            const float rMa0 = S.at(PA, RM, 0, PAD + lane), rMm0 = S.at(PA, RM, 1, PAD + lane);
+            const float fzx = sfx * frcp(rMm0 + S.at(PA, RM, 1, PAD + lane + 1));
+            const float fzz = sfz * frcp(rMa0 + S.at(PA, RM, 0, PAD + lane + 1));
             float dxz = 0.0f, dxx = 0.0f;
 #pragma unroll
             for (int k = 1; k <= NO; ++k) {
                 dxz += a.k.bi[k - 1] * (R.U[1][sl(PH, k)] - R.U[1][sl(PH, -k)]);
                 dxx += a.k.bi[k - 1] * (R.U[0][sl(PH, k)] - R.U[0][sl(PH, -k)]);
             }
-            S.fz[PA][0][PAD + lane] = fzx;
-            S.fz[PA][1][PAD + lane] = fzz;
-            S.vz[PA][0][PAD + lane] = m0 * dxz;
-            S.vz[PA][1][PAD + lane] = l0 * dxx;
+            S.at(PA, FZ, 0, PAD + lane) = fzx;
+            S.at(PA, FZ, 1, PAD + lane) = fzz;
+            S.at(PA, VZ, 0, PAD + lane) = m0 * dxz;
+            S.at(PA, VZ, 1, PAD + lane) = l0 * dxx;
         }
         // D. update of column ix-1 (its z-terms were staged in the previous step)
         {
@@ -533,19 +548,19 @@ struct IntKernel2 {
             for (int k = 1; k <= NO; ++k) {
                 Xx += a.k.aox[k - 1] * (R.FX[0][sl(PH, k - 2)] - R.FX[0][sl(PH, -k - 1)]);
                 Xz += a.k.aox[k - 1] * (R.FX[1][sl(PH, k - 2)] - R.FX[1][sl(PH, -k - 1)]);
-                Zx += a.k.aoz[k - 1] * (S.fz[PB][0][PAD + lane + k - 1] - S.fz[PB][0][PAD + lane - k]);
-                Zz += a.k.aoz[k - 1] * (S.fz[PB][1][PAD + lane + k - 1] - S.fz[PB][1][PAD + lane - k]);
+                Zx += a.k.aoz[k - 1] * (S.at(PB, FZ, 0, PAD + lane + k - 1) - S.at(PB, FZ, 0, PAD + lane - k));
+                Zz += a.k.aoz[k - 1] * (S.at(PB, FZ, 1, PAD + lane + k - 1) - S.at(PB, FZ, 1, PAD + lane - k));
                 Mx += a.k.bo[k - 1] * ((R.VX[0][sl(PH, k - 1)] - R.VX[0][sl(PH, -k - 1)]) +
-                                      (S.vz[PB][0][PAD + lane + k] - S.vz[PB][0][PAD + lane - k]));
+                                      (S.at(PB, VZ, 0, PAD + lane + k) - S.at(PB, VZ, 0, PAD + lane - k)));
                 Mz += a.k.bo[k - 1] * ((R.VX[1][sl(PH, k - 1)] - R.VX[1][sl(PH, -k - 1)]) +
-                                      (S.vz[PB][1][PAD + lane + k] - S.vz[PB][1][PAD + lane - k]));
+                                      (S.at(PB, VZ, 1, PAD + lane + k) - S.at(PB, VZ, 1, PAD + lane - k)));
             }
             const float bf = R.po[PB][0];
             const float nx_ = (2.0f * R.U[0][sl(PH, -1)] - R.po[PB][1]) + bf * ((Xx + Zx) + Mx);
             const float nz_ = (2.0f * R.U[1][sl(PH, -1)] - R.po[PB][2]) + bf * ((Xz + Zz) + Mz);
             if (L.outrow && ix - 1 >= xs && ix - 1 < xe) {
-                put(g, a.uxp, ix - 1, o, nx_);
-                put(g, a.uzp, ix - 1, o, nz_);
+                st(B.uxp, o, B.soff(ix - 1), nx_);
+                st(B.uzp, o, B.soff(ix - 1), nz_);
             }
         }
     }
@@ -560,8 +575,8 @@ struct IntKernel2 {
             if (i < PAD || i >= NT + PAD)
                 for (int q = 0; q < 2; ++q)
                     for (int c = 0; c < 2; ++c) {
-                        S.uA[q][c][i] = 0.0f; S.uB[q][c][i] = 0.0f; S.rM[q][c][i] = 1.0f;
-                        S.fz[q][c][i] = 0.0f; S.vz[q][c][i] = 0.0f;
+                        S.at(q, UA, c, i) = 0.0f; S.at(q, UB, c, i) = 0.0f; S.at(q, RM, c, i) = 1.0f;
+                        S.at(q, FZ, c, i) = 0.0f; S.at(q, VZ, c, i) = 0.0f;
                     }
         // block -> tile: consecutive blocks walk down z first so that the x
         // neighbours sharing warm-up columns run close in time
@@ -583,19 +598,30 @@ struct IntKernel2 {
             for (int i = 0; i < P; ++i) { R.U[q][i] = 0.0f; R.FX[q][i] = 0.0f; R.VX[q][i] = 0.0f; }
         R.rMa_prev = 1.0f; R.rMm_prev = 1.0f;
         const int x0 = xs - WARM;
+        Rsrc B;
+        {
+            const int lo = 1 - g.npml;
+            B.xb = x0 < lo ? lo : x0;   // every column a step touches is clamped to >= lo
+            B.pitch = g.ld * 4;
+            B.ux = mk(g, a.ux, B.xb); B.uz = mk(g, a.uz, B.xb);
+            B.uxp = mk(g, a.uxp, B.xb); B.uzp = mk(g, a.uzp, B.xb);
+            B.bf = mk(g, a.bf, B.xb); B.lam = mk(g, a.lam, B.xb); B.mu = mk(g, a.mu, B.xb);
+            B.rx = mk(g, a.ringx, B.xb); B.rz = mk(g, a.ringz, B.xb);
+        }
         // prologue: the loads the first step consumes
         {
             const int lo = 1 - g.npml;
             const unsigned o = L.ofs;
             const int cn = x0 + 2 * NO - 1;
             const int cm = x0 + NO < lo ? lo : x0 + NO, c0 = x0 < lo ? lo : x0;
-            R.pu[0][0] = at(g, a.ux, cn, o); R.pu[0][1] = at(g, a.uz, cn, o);
+            const int sn = B.soff(cn), sm = B.soff(cm), s0 = B.soff(c0);
+            R.pu[0][0] = ld(B.ux, o, sn); R.pu[0][1] = ld(B.uz, o, sn);
             if (bc && outside(g, L.r, cn)) {
-                R.pu[0][0] += at(g, a.ringx, cn, o);
-                R.pu[0][1] += at(g, a.ringz, cn, o);
+                R.pu[0][0] += ld(B.rx, o, sn);
+                R.pu[0][1] += ld(B.rz, o, sn);
             }
-            R.pl[0][0] = at(g, a.lam, cm, o); R.pl[0][1] = at(g, a.mu, cm, o);
-            R.p0[0][0] = at(g, a.lam, c0, o); R.p0[0][1] = at(g, a.mu, c0, o);
+            R.pl[0][0] = ld(B.lam, o, sm); R.pl[0][1] = ld(B.mu, o, sm);
+            R.p0[0][0] = ld(B.lam, o, s0); R.p0[0][1] = ld(B.mu, o, s0);
 #pragma unroll
             for (int q = 0; q < 2; ++q) { R.pu[1][q] = 0.0f; R.pl[1][q] = 1.0f; R.p0[1][q] = 1.0f; }
 #pragma unroll
@@ -603,7 +629,7 @@ struct IntKernel2 {
         }
         __syncthreads();
         for (int base = x0; base <= xe; base += P) {
-#define IK2_STEP(PH) if constexpr (P > PH) { if (base + PH <= xe) step<PH>(a, S, R, L, bc, base + PH, xs, xe); }
+#define IK2_STEP(PH) if constexpr (P > PH) { if (base + PH <= xe) step<PH>(a, B, S, R, L, bc, base + PH, xs, xe); }
             IK2_STEP(0) IK2_STEP(1) IK2_STEP(2) IK2_STEP(3) IK2_STEP(4) IK2_STEP(5) IK2_STEP(6)
             IK2_STEP(7) IK2_STEP(8) IK2_STEP(9) IK2_STEP(10) IK2_STEP(11) IK2_STEP(12) IK2_STEP(13)
 #undef IK2_STEP
