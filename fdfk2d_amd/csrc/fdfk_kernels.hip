@@ -481,11 +481,9 @@ struct IntKernel2 {
             }
             R.pl[PB][0] = ld(B.lam, o, sm); R.pl[PB][1] = ld(B.mu, o, sm);
             R.p0[PB][0] = ld(B.lam, o, s0); R.p0[PB][1] = ld(B.mu, o, s0);
-            if (ix >= xs) {   // update inputs only for output columns (block-uniform)
-                R.po[PA][0] = ld(B.bf, o, sx);
-                R.po[PA][1] = ld(B.uxp, o, sx);
-                R.po[PA][2] = ld(B.uzp, o, sx);
-            }
+            R.po[PA][0] = ld(B.bf, o, sx);
+            R.po[PA][1] = ld(B.uxp, o, sx);
+            R.po[PA][2] = ld(B.uzp, o, sx);
         }
         const float rMa = frcp(l1 + 2.0f * m1), rMm = frcp(m1);
         {
