@@ -27,6 +27,10 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL = 36  # u^n (8) + u^{n-1} (8) + u^{n+1} (8) + dt^2/rho, lambda, mu (12)
+# HBM bytes per cell-update of k_interior2 on the 8001x2000 workload, from
+# rocprofv3 FETCH_SIZE / WRITE_SIZE scaled by a plain device copy
+# (profiles/k_interior2_8001x2000_hbm_traffic.md)
+MEASURED_TRAFFIC_B_PER_CELL = 47.56
 
 
 def gpu_bench(steps, warmup, workload):
@@ -151,7 +155,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": (round(MEASURED_TRAFFIC_B_PER_CELL * r["int_cells"]) if a.workload == "c2" else None),
+            "traffic_source": "rocprofv3 FETCH_SIZE+WRITE_SIZE per launch, copy-calibrated "
+                              "(profiles/k_interior2_8001x2000_hbm_traffic.md)",
             "kernel": "k_interior2",
             "kernel_avg_ms": round(int_ms, 5),
             "bytes_per_cell": BYTES_PER_CELL,

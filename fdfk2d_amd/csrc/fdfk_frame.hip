@@ -24,7 +24,26 @@ __host__ __device__ inline int frame_lds_floats(int tx, int tz, int NO)
 {
     const int H = 2 * NO - 1;
     return 4 * (tx + 2 * H) * (tz + 2 * H) + 4 * (tx + 2 * NO - 1) * tz + 4 * tx * (tz + 2 * NO - 1) +
-           2 * (tx + 2 * NO) * tz + 2 * tx * (tz + 2 * NO);
+           4 * (tx + 2 * NO) * tz + 4 * tx * (tz + 2 * NO);
+}
+
+// e / d and e % d for 0 <= e < 2^20, 1 <= d <= 4096 through a float
+// reciprocal: (e + 1/2) / d lies at least 1/(2d) away from an integer, far
+// beyond the float rounding error, so the truncation is exact
+struct FDiv {
+    int d;
+    float inv;
+    __device__ __forceinline__ explicit FDiv(int d_) : d(d_), inv(1.0f / (float)d_) {}
+    __device__ __forceinline__ int quo(int e) const { return (int)(((float)e + 0.5f) * inv); }
+};
+
+// coefficient j of order n from a staged (NO x NO) table; 0 for j > n
+template <int NO>
+__device__ __forceinline__ float coefs(const float *tab, int j, int n)
+{
+    const int nn = n < 1 ? 1 : n;
+    const float c = tab[(j - 1) + NO * (nn - 1)];
+    return j <= n ? c : 0.0f;
 }
 
 template <int NO>
@@ -52,10 +71,22 @@ __device__ __forceinline__ void frame_body(const StepArgs &a, const ETiles &et, 
     float *fz1x = fx2z + NFX * tz, *fz1z = fz1x + tx * NFZ, *fz2x = fz1z + tx * NFZ, *fz2z = fz2x + tx * NFZ;
     float *dzx = fz2z + tx * NFZ, *dzz = dzx + NDZ * tz;
     float *dxx = dzz + NDZ * tz, *dxz = dxx + tx * NDX;
+    float *mpx0 = dxz + tx * NDX, *mpx1 = mpx0 + NDZ * tz;   // M psi_x on the DZ index space
+    float *mpz0 = mpx1 + NDZ * tz, *mpz1 = mpz0 + tx * NDX;  // M psi_z on the DX index space
+    // the band's PML strip (uniform per band): none in the total-field region
+    const int sb = view == 0 ? S_NONE : (cz0 > g.nz ? S_B : (cx1 <= 0 ? S_L : S_R));
+    // the two coefficient tables, staged once per block
+    __shared__ float c0s[NO * NO], c1s[NO * NO];
+    if (threadIdx.x < NO * NO) {
+        c0s[threadIdx.x] = cf->c0f[threadIdx.x];
+        c1s[threadIdx.x] = cf->c1f[threadIdx.x];
+    }
+    const FDiv dSZ(SZ), dtz(tz), dNFZ(NFZ), dNDX(NDX);
     auto S = [&](int z, int x) { return (x - lx0) * SZ + (z - lz0); };
     // stage u (in the tile's view), lambda, mu
     for (int e = threadIdx.x; e < SX * SZ; e += kFrameThreads) {
-        const int x = lx0 + e / SZ, z = lz0 + e % SZ;
+        const int q = dSZ.quo(e);
+        const int x = lx0 + q, z = lz0 + (e - q * SZ);
         float vx = 0.0f, vz = 0.0f, vl = 0.0f, vm = 0.0f;
         if (x >= 1 - g.npml && x <= g.nx + g.npml && z >= 1 && z <= g.NZ) {
             const size_t c = cell(g, z, x);
@@ -72,7 +103,8 @@ __device__ __forceinline__ void frame_body(const StepArgs &a, const ETiles &et, 
     const int nxp = g.nx + g.npml, NZ = g.NZ;
     // phase 1a: x-fluxes (mode 0 on ux, mode 1 on uz)
     for (int e = threadIdx.x; e < NFX * tz; e += kFrameThreads) {
-        const int f = fx0 + e / tz, z = cz0 + e % tz;
+        const int q = dtz.quo(e);
+        const int f = fx0 + q, z = cz0 + (e - q * tz);
         const int n1 = imin(imin(f + g.npml - 1, nxp - f), NO), n2 = imin(imin(f + g.npml, nxp - f - 1), NO);
         const int a0 = S(z, f), a1 = S(z, f + 1);
         const float ha = hmean(slam[a1] + 2.0f * smu[a1], slam[a0] + 2.0f * smu[a0]);
@@ -82,7 +114,7 @@ __device__ __forceinline__ void frame_body(const StepArgs &a, const ETiles &et, 
         for (int j = 1; j <= NO; ++j) {
             const int p = S(z, f + j), m = S(z, f + 1 - j);
             const float dux = sux[p] - sux[m], duz = suz[p] - suz[m];
-            const float c1 = coefz<NO>(cf->c0f, j, n1), c2 = coefz<NO>(cf->c0f, j, n2);
+            const float c1 = coefs<NO>(c0s, j, n1), c2 = coefs<NO>(c0s, j, n2);
             s1x = accf(s1x, c1, dux); s2x = accf(s2x, c2, dux);
             s1z = accf(s1z, c1, duz); s2z = accf(s2z, c2, duz);
         }
@@ -91,7 +123,8 @@ __device__ __forceinline__ void frame_body(const StepArgs &a, const ETiles &et, 
     }
     // phase 1b: z-fluxes (mode 1 on ux, mode 0 on uz)
     for (int e = threadIdx.x; e < tx * NFZ; e += kFrameThreads) {
-        const int x = cx0 + e / NFZ, f = fz0 + e % NFZ;
+        const int q = dNFZ.quo(e);
+        const int x = cx0 + q, f = fz0 + (e - q * NFZ);
         const int n1 = imin(imin(f - 1, NZ - f), NO), n2 = imin(imin(f, NZ - f - 1), NO);
         const int a0 = S(f, x), a1 = S(f + 1, x);
         const float ha = hmean(slam[a1] + 2.0f * smu[a1], slam[a0] + 2.0f * smu[a0]);
@@ -101,7 +134,7 @@ __device__ __forceinline__ void frame_body(const StepArgs &a, const ETiles &et, 
         for (int j = 1; j <= NO; ++j) {
             const int p = S(f + j, x), m = S(f + 1 - j, x);
             const float dux = sux[p] - sux[m], duz = suz[p] - suz[m];
-            const float c1 = coefz<NO>(cf->c0f, j, n1), c2 = coefz<NO>(cf->c0f, j, n2);
+            const float c1 = coefs<NO>(c0s, j, n1), c2 = coefs<NO>(c0s, j, n2);
             s1x = accf(s1x, c1, dux); s2x = accf(s2x, c2, dux);
             s1z = accf(s1z, c1, duz); s2z = accf(s2z, c2, duz);
         }
@@ -110,13 +143,14 @@ __device__ __forceinline__ void frame_body(const StepArgs &a, const ETiles &et, 
     }
     // phase 1c: collocated Dz (cdz_sum) at columns dz0.., rows of the tile
     for (int e = threadIdx.x; e < NDZ * tz; e += kFrameThreads) {
-        const int x = dz0 + e / tz, z = cz0 + e % tz;
+        const int q = dtz.quo(e);
+        const int x = dz0 + q, z = cz0 + (e - q * tz);
         const int n = imin(imin(z - 1, NZ - z), NO);
         float sx = 0.0f, sz = 0.0f;
 #pragma unroll
         for (int j = 1; j <= NO; ++j) {
             const int p = S(z + j, x), m = S(z - j, x);
-            const float c = coefz<NO>(cf->c1f, j, n);
+            const float c = coefs<NO>(c1s, j, n);
             sx = accf(sx, c, sux[p] - sux[m]);
             sz = accf(sz, c, suz[p] - suz[m]);
         }
@@ -124,23 +158,57 @@ __device__ __forceinline__ void frame_body(const StepArgs &a, const ETiles &et, 
     }
     // phase 1d: collocated Dx (cdx_sum) at the tile columns, rows dx0..
     for (int e = threadIdx.x; e < tx * NDX; e += kFrameThreads) {
-        const int x = cx0 + e / NDX, z = dx0 + e % NDX;
+        const int q = dNDX.quo(e);
+        const int x = cx0 + q, z = dx0 + (e - q * NDX);
         const int n = imin(imin(x + g.npml - 1, nxp - x), NO);
         float sx = 0.0f, sz = 0.0f;
 #pragma unroll
         for (int j = 1; j <= NO; ++j) {
             const int p = S(z, x + j), m = S(z, x - j);
-            const float c = coefz<NO>(cf->c1f, j, n);
+            const float c = coefs<NO>(c1s, j, n);
             sx = accf(sx, c, sux[p] - sux[m]);
             sz = accf(sz, c, suz[p] - suz[m]);
         }
         dxx[e] = sx; dxz[e] = sz;
     }
+    // phase 1e: M psi of this step (k_psi) around the tile, as the memory
+    // corrections read them: zero outside the strips, psi_z only in the
+    // bottom strip
+    if (sb != S_NONE) {
+        const Strips *st = a.st;
+        for (int e = threadIdx.x; e < NDZ * tz; e += kFrameThreads) {
+            const int qq = dtz.quo(e);
+            const int x = dz0 + qq, z = cz0 + (e - qq * tz);
+            float v0 = 0.0f, v1 = 0.0f;
+            const int s2 = strip_of(g, z, x);
+            if (s2 != S_NONE && x >= 1 - g.npml && x <= nxp && z <= NZ) {
+                const size_t o = strip_off(g, s2, z, x);
+                const int k = S(z, x);
+                v0 = (slam[k] + 2.0f * smu[k]) * st->psx[s2][0][o];
+                v1 = smu[k] * st->psx[s2][1][o];
+            }
+            mpx0[e] = v0; mpx1[e] = v1;
+        }
+        if (sb == S_B)
+            for (int e = threadIdx.x; e < tx * NDX; e += kFrameThreads) {
+                const int qq = dNDX.quo(e);
+                const int x = cx0 + qq, z = dx0 + (e - qq * NDX);
+                float v0 = 0.0f, v1 = 0.0f;
+                if (z > g.nz && z <= NZ) {
+                    const size_t o = strip_off(g, S_B, z, x);
+                    const int k = S(z, x);
+                    v0 = smu[k] * st->psz[S_B][0][o];
+                    v1 = (slam[k] + 2.0f * smu[k]) * st->psz[S_B][1][o];
+                }
+                mpz0[e] = v0; mpz1[e] = v1;
+            }
+    }
     __syncthreads();
     // phase 2: one thread per cell
     const int t = threadIdx.x;
     if (t >= tx * tz) return;
-    const int ix = cx0 + t / tz, iz = cz0 + t % tz;
+    const int tq = dtz.quo(t);
+    const int ix = cx0 + tq, iz = cz0 + (t - tq * tz);
     const int nox = imin(imin(ix + g.npml - 1, nxp - ix), NO);
     const int noz = imin(imin(iz - 1, NZ - iz), NO);
     auto FX = [&](const float *b, int f) { return b[(f - fx0) * tz + (iz - cz0)]; };
@@ -152,8 +220,8 @@ __device__ __forceinline__ void frame_body(const StepArgs &a, const ETiles &et, 
     float Xx = 0.0f, Xz = 0.0f, Zx = 0.0f, Zz = 0.0f, zxl = 0.0f, zxm = 0.0f, xzm = 0.0f, xzl = 0.0f;
 #pragma unroll
     for (int k = 1; k <= NO; ++k) {
-        const float c0x = coefz<NO>(cf->c0f, k, nox), c0z = coefz<NO>(cf->c0f, k, noz);
-        const float c1x = coefz<NO>(cf->c1f, k, nox), c1z = coefz<NO>(cf->c1f, k, noz);
+        const float c0x = coefs<NO>(c0s, k, nox), c0z = coefs<NO>(c0s, k, noz);
+        const float c1x = coefs<NO>(c1s, k, nox), c1z = coefs<NO>(c1s, k, noz);
         Xx = accf(Xx, c0x, FX(fx1x, ix + k - 1) - FX(fx2x, ix - k));
         Xz = accf(Xz, c0x, FX(fx1z, ix + k - 1) - FX(fx2z, ix - k));
         Zx = accf(Zx, c0z, FZ(fz1x, iz + k - 1) - FZ(fz2x, iz - k));
@@ -179,7 +247,7 @@ __device__ __forceinline__ void frame_body(const StepArgs &a, const ETiles &et, 
 #pragma unroll
         for (int k = 1; k <= NO; ++k) {
             const int p1 = S(1, ix + k), p2 = S(2, ix + k), m1 = S(1, ix - k), m2 = S(2, ix - k);
-            const float c = coefz<NO>(cf->c1f, k, nox);
+            const float c = coefs<NO>(c1s, k, nox);
             const float lp = 0.5f * (slam[p1] + slam[p2]), lm = 0.5f * (slam[m1] + slam[m2]);
             const float mp = 0.5f * (smu[p1] + smu[p2]), mm = 0.5f * (smu[m1] + smu[m2]);
             sa = accf(sa, c, lp * (suz[p2] - suz[p1]) - lm * (suz[m2] - suz[m1]));
@@ -196,7 +264,54 @@ __device__ __forceinline__ void frame_body(const StepArgs &a, const ETiles &et, 
         p[2] = zxl / g.dx + xzm / g.dz;
         p[5] = zxm / g.dx + xzl / g.dz;
     }
-    frame_update<NO>(a, iz, ix, p);
+    // memory corrections -Dx(M psi_x), -Dz(M psi_z) (the bottom strip only)
+    if (sb != S_NONE) {
+        float sx0 = 0.0f, sx1 = 0.0f, sz0 = 0.0f, sz1 = 0.0f;
+#pragma unroll
+        for (int k = 1; k <= NO; ++k) {
+            const float c = coefs<NO>(c1s, k, nox);
+            sx0 = accf(sx0, c, DZ(mpx0, ix + k) - DZ(mpx0, ix - k));
+            sx1 = accf(sx1, c, DZ(mpx1, ix + k) - DZ(mpx1, ix - k));
+        }
+        p[0] = p[0] - sx0 / g.dx;
+        p[3] = p[3] - sx1 / g.dx;
+        if (sb == S_B) {
+#pragma unroll
+            for (int k = 1; k <= NO; ++k) {
+                const float c = coefs<NO>(c1s, k, noz);
+                sz0 = accf(sz0, c, DX(mpz0, iz + k) - DX(mpz0, iz - k));
+                sz1 = accf(sz1, c, DX(mpz1, iz + k) - DX(mpz1, iz - k));
+            }
+            p[1] = p[1] - sz0 / g.dz;
+            p[4] = p[4] - sz1 / g.dz;
+        }
+    }
+    // update: plain leapfrog in the total-field region, split parts in a strip
+    const size_t c = cell(g, iz, ix);
+    const float bf = a.bf[c];
+    if (sb == S_NONE) {
+        const float rxs = (p[0] + p[1]) + p[2], rzs = (p[3] + p[4]) + p[5];
+        a.uxp[c] = (2.0f * a.ux[c] - a.uxp[c]) + bf * rxs;
+        a.uzp[c] = (2.0f * a.uz[c] - a.uzp[c]) + bf * rzs;
+        return;
+    }
+    const Strips *st = a.st;
+    const size_t o = strip_off(g, sb, iz, ix);
+    const float ax = st->ax[ix + g.npml - 1];
+    const float az = st->az[iz - 1];
+    const float c3 = 2.0f - 4.0f * ax * az;
+#pragma unroll
+    for (int comp = 0; comp < 2; ++comp) {
+        const float *b1c = st->cur[sb][comp][0], *b2c = st->cur[sb][comp][1], *b3c = st->cur[sb][comp][2];
+        float *b1p = st->prev[sb][comp][0], *b2p = st->prev[sb][comp][1], *b3p = st->prev[sb][comp][2];
+        const float X = p[3 * comp], Z = p[3 * comp + 1], Mm = p[3 * comp + 2];
+        const float n1 = (2.0f * b1c[o] - (1.0f - ax) * b1p[o] + bf * X) / (1.0f + ax);
+        const float n2 = (2.0f * b2c[o] - (1.0f - az) * b2p[o] + bf * Z) / (1.0f + az);
+        const float n3 = (c3 * b3c[o] - (1.0f - ax - az) * b3p[o] + bf * Mm) / (1.0f + ax + az);
+        b1p[o] = n1; b2p[o] = n2; b3p[o] = n3;
+        const float tot = (n1 + n2) + n3;
+        if (comp == 0) a.uxp[c] = tot; else a.uzp[c] = tot;
+    }
 }
 
 template <int NO>

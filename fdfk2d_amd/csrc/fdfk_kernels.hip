@@ -134,87 +134,6 @@ __device__ __forceinline__ size_t strip_off(const Geo &g, int s, int iz, int ix)
     return (size_t)(ix - (1 - g.npml)) * g.npml + (size_t)(iz - (g.nz + 1));
 }
 
-// psi_x / psi_z of one component, zero outside the strips that carry it
-struct PsiRd {
-    const Strips *st;
-    const Geo *g;
-    int comp, zdir;
-    __device__ float operator()(int iz, int ix) const
-    {
-        int s = strip_of(*g, iz, ix);
-        if (s == S_NONE || (zdir && s != S_B)) return 0.0f;
-        if (ix < 1 - g->npml || ix > g->nx + g->npml || iz < 1 || iz > g->NZ) return 0.0f;
-        const float *b = zdir ? st->psz[s][comp] : st->psx[s][comp];
-        return b[strip_off(*g, s, iz, ix)];
-    }
-};
-
-// PML memory corrections and the (split-field) leapfrog update of one cell
-template <int NO>
-__device__ __forceinline__ void frame_update(const StepArgs &a, int iz, int ix, float p[6])
-{
-    const Geo &g = a.g;
-    const Coefs *cf = a.cf;
-    const int s = strip_of(g, iz, ix);
-    const bool surf = (iz == 1);
-    if (s != S_NONE) {
-        // -Dx(Mx psi_x), -Dz(Mz psi_z)
-        PsiRd px0{a.st, &g, 0, 0}, px1{a.st, &g, 1, 0}, pz0{a.st, &g, 0, 1}, pz1{a.st, &g, 1, 1};
-        const int nox = imin(imin(ix + g.npml - 1, g.nx + g.npml - ix), NO);
-        const int noz = imin(imin(iz - 1, g.nz + g.npml - iz), NO);
-        float sx0 = 0.0f, sx1 = 0.0f, sz0 = 0.0f, sz1 = 0.0f;
-#pragma unroll
-        for (int kk = 1; kk <= NO; ++kk) {
-            size_t ca = cellc(g, iz, ix + kk), cb = cellc(g, iz, ix - kk);
-            float ma = a.lam[ca] + 2.0f * a.mu[ca], mb = a.lam[cb] + 2.0f * a.mu[cb];
-            float c = coefz<NO>(cf->c1f, kk, nox);
-            sx0 = accf(sx0, c, ma * px0(iz, ix + kk) - mb * px0(iz, ix - kk));
-            sx1 = accf(sx1, c, a.mu[ca] * px1(iz, ix + kk) - a.mu[cb] * px1(iz, ix - kk));
-        }
-        if (s == S_B) {
-#pragma unroll
-            for (int kk = 1; kk <= NO; ++kk) {
-                size_t ca = cellc(g, iz + kk, ix), cb = cellc(g, iz - kk, ix);
-                float c = coefz<NO>(cf->c1f, kk, noz);
-                sz0 = accf(sz0, c, a.mu[ca] * pz0(iz + kk, ix) - a.mu[cb] * pz0(iz - kk, ix));
-                sz1 = accf(sz1, c, (a.lam[ca] + 2.0f * a.mu[ca]) * pz1(iz + kk, ix) -
-                                       (a.lam[cb] + 2.0f * a.mu[cb]) * pz1(iz - kk, ix));
-            }
-        }
-        p[0] = p[0] - sx0 / g.dx;
-        p[3] = p[3] - sx1 / g.dx;
-        if (!surf) {
-            p[1] = p[1] - sz0 / g.dz;
-            p[4] = p[4] - sz1 / g.dz;
-        }
-    }
-    const size_t c = cell(g, iz, ix);
-    const float bf = a.bf[c];
-    if (s == S_NONE) {
-        float rxs = (p[0] + p[1]) + p[2], rzs = (p[3] + p[4]) + p[5];
-        a.uxp[c] = (2.0f * a.ux[c] - a.uxp[c]) + bf * rxs;
-        a.uzp[c] = (2.0f * a.uz[c] - a.uzp[c]) + bf * rzs;
-        return;
-    }
-    const Strips *st = a.st;
-    const size_t o = strip_off(g, s, iz, ix);
-    const float ax = st->ax[ix + g.npml - 1];
-    const float az = st->az[iz - 1];
-    const float c3 = 2.0f - 4.0f * ax * az;
-#pragma unroll
-    for (int comp = 0; comp < 2; ++comp) {
-        float *b1c = st->cur[s][comp][0], *b2c = st->cur[s][comp][1], *b3c = st->cur[s][comp][2];
-        float *b1p = st->prev[s][comp][0], *b2p = st->prev[s][comp][1], *b3p = st->prev[s][comp][2];
-        float X = p[3 * comp], Z = p[3 * comp + 1], Mm = p[3 * comp + 2];
-        float n1 = (2.0f * b1c[o] - (1.0f - ax) * b1p[o] + bf * X) / (1.0f + ax);
-        float n2 = (2.0f * b2c[o] - (1.0f - az) * b2p[o] + bf * Z) / (1.0f + az);
-        float n3 = (c3 * b3c[o] - (1.0f - ax - az) * b3p[o] + bf * Mm) / (1.0f + ax + az);
-        b1p[o] = n1; b2p[o] = n2; b3p[o] = n3;
-        float tot = (n1 + n2) + n3;
-        if (comp == 0) a.uxp[c] = tot; else a.uzp[c] = tot;
-    }
-}
-
 // background on the hybrid ring at iteration it, scattered into the
 // full-grid ring arrays (positions are fixed; values change every step)
 __device__ __forceinline__ void ring_prep_body(const Geo &g, const Background *bg, int it, float *ringx,
