@@ -249,48 +249,68 @@ void launch_interior_t(fdfk_solver *s)
     hipLaunchKernelGGL((k_interior2<NO, NT>), dim3(a.nbx * a.nbz), dim3(NT), 0, s->stream, a);
 }
 
-// Columns per interior block.  The stencil blocks all take about the same
-// time, so the kernel time is (number of resident rounds) x (columns per
-// block + warm-up columns).  Pick the width that minimises that for the
-// blocks one launch can keep resident (occupancy x CUs).
+// Interior block shape.  The stencil blocks all take about the same time,
+// so for a block height NT the kernel time is (number of resident rounds) x
+// (columns per block + warm-up columns); pick the column width that
+// minimises it for the blocks one launch keeps resident (occupancy x CUs).
+// The height is 256 lanes (measured fastest on the 8001x2000 solve; 128 and
+// 192 can be requested for experiments).
+struct IntPlan {
+    int lx = 45;
+    long cost = -1;
+};
+
 template <int NO, int NT>
-int choose_lx(const Geo &g)
+IntPlan plan_lx(const Geo &g)
 {
+    IntPlan best;
     int dev = 0, ncu = 0, bpc = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
         hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_interior2<NO, NT>, NT, 0) != hipSuccess ||
         ncu <= 0 || bpc <= 0)
-        return 45;
+        return best;
     constexpr int TZ = IntKernel2<NO, NT>::TZ;
     constexpr int WARM = IntKernel2<NO, NT>::WARM;
     const long slots = (long)ncu * bpc;
     const int ncols = g.ix1 - g.ix0 + 1, nrows = g.iz1 - g.iz0 + 1;
     const long nbz = (nrows + TZ - 1) / TZ;
-    int best = 45;
-    long best_cost = -1;
     for (int R = 1; R <= 16; ++R) {
         long nbx = std::max(1L, R * slots / nbz);
         int lx = (int)std::max(4L, (ncols + nbx - 1) / nbx);
         long nb = ((ncols + lx - 1) / lx) * nbz;
         long rounds = (nb + slots - 1) / slots;
         long cost = rounds * (lx + WARM);
-        if (best_cost < 0 || cost < best_cost) { best_cost = cost; best = lx; }
+        if (best.cost < 0 || cost < best.cost) { best.cost = cost; best.lx = lx; }
     }
     return best;
 }
 
-int auto_lx(const fdfk_solver *s)
+// block height (threads along z) and column width of the interior launch;
+// FDFK_NT / FDFK_LX override them
+void plan_interior(fdfk_solver *s)
 {
-    switch (s->g.norder) {
-    case 1: return choose_lx<1, 256>(s->g);
-    case 2: return choose_lx<2, 256>(s->g);
+    const Geo &g = s->g;
+    const char *ent = std::getenv("FDFK_NT");
+    const int want = ent ? std::atoi(ent) : 0;
+    IntPlan p;
+    int nt = 256;
+    auto consider = [&](int n, IntPlan q) {
+        if (q.cost >= 0 && (p.cost < 0 || q.cost < p.cost)) { p = q; nt = n; }
+    };
+    switch (g.norder) {
+    case 1: consider(256, plan_lx<1, 256>(g)); break;
+    case 2: consider(256, plan_lx<2, 256>(g)); break;
     case 3:
-        if (s->nt_threads == 128) return choose_lx<3, 128>(s->g);
-        if (s->nt_threads == 512) return choose_lx<3, 512>(s->g);
-        return choose_lx<3, 256>(s->g);
-    default: return choose_lx<4, 256>(s->g);
+        if (want == 192) consider(192, plan_lx<3, 192>(g));
+        else if (want == 128) consider(128, plan_lx<3, 128>(g));
+        else consider(256, plan_lx<3, 256>(g));
+        break;
+    default: consider(256, plan_lx<4, 256>(g)); break;
     }
+    s->nt_threads = nt;
+    s->lx = p.lx;
+    if (const char *e = std::getenv("FDFK_LX")) s->lx = std::max(4, std::atoi(e));
 }
 
 void launch_interior(fdfk_solver *s)
@@ -300,7 +320,7 @@ void launch_interior(fdfk_solver *s)
     case 2: launch_interior_t<2, 256>(s); break;
     case 3:
         if (s->nt_threads == 128) launch_interior_t<3, 128>(s);
-        else if (s->nt_threads == 512) launch_interior_t<3, 512>(s);
+        else if (s->nt_threads == 192) launch_interior_t<3, 192>(s);
         else launch_interior_t<3, 256>(s);
         break;
     default: launch_interior_t<4, 256>(s); break;
@@ -462,9 +482,7 @@ int fdfk_create(const fdfk_job *job, fdfk_solver **out)
     s->nrcv = job->nrcv;
     s->x0 = job->x0; s->z0 = job->z0;
     s->R0 = job->R0; s->npower = job->npower;
-    if (const char *e = std::getenv("FDFK_NT")) s->nt_threads = std::atoi(e);
-    s->lx = auto_lx(s);
-    if (const char *e = std::getenv("FDFK_LX")) s->lx = std::max(4, std::atoi(e));
+    plan_interior(s);
     const long ncell = (long)g.NX * g.NZ;
     // vpml = max Vp (src/FDFK2D.f90:416)
     float vpml = job->vpml;
