@@ -101,6 +101,87 @@ def copy_calibration(mib=512, reps=5):
     print(json.dumps({"copy_calibration_bytes": n * 4, "reps": reps}), flush=True)
 
 
+def serve_bench(n, seed=5, rate=200.0):
+    """Scaled continuous-batching serving run (the request mix of the serving
+    workload with smaller grids and step counts so that both the batched and
+    the one-at-a-time plain path finish in about a minute): p50/p99 latency
+    and solves per hour for both, plus the equality of their seismograms."""
+    from fdfk2d_amd import serve
+    reqs = serve.request_mix(n, seed=seed, rate=rate, nx_range=(400, 2000), nz_range=(160, 500),
+                             steps_range=(600, 2500), dx_choices=(100.0, 200.0))
+    srv = serve.Server(hbm_budget_gb=200.0, chunk=100)
+    lat_b, tot_b, res_b = srv.run(reqs, keep_results=True)
+    srv.close()
+    lat_p, tot_p, res_p = serve.serve_plain(reqs, keep_results=True)
+    worst = 0.0
+    for rid in res_p:
+        for a_, b_ in zip(res_b[rid], res_p[rid]):
+            worst = max(worst, serve.rel_l2(a_, b_))
+    b = serve.summarize(lat_b, tot_b, n)
+    p = serve.summarize(lat_p, tot_p, n)
+    cells = sum((r["nx"] + 2 * r["npml"]) * (r["nz"] + r["npml"]) * r["nsteps"] for r in reqs)
+    return {"metric": "solves_per_hour", "value": b["solves_per_hour"], "unit": "solves/h",
+            "higher_is_better": True, "data": "synthetic",
+            "config": {"workload": f"serving: {n} Poisson arrivals at {rate}/s (seed {seed}), grids "
+                                   "400..2000 x 160..500 log-uniform, 600..2500 steps, dx 100/200 m, "
+                                   "HBM budget 200 GB", "cell_updates": cells},
+            "batched": b, "plain": p, "speedup_solves_per_hour": round(b["solves_per_hour"] / p["solves_per_hour"], 3),
+            "max_rel_l2_batched_vs_plain": worst}
+
+
+def batch_bench(njobs, seed=3, steps=1000):
+    """Scaled batched workload: njobs independent solves of mixed sizes
+    advanced together (one launch per kernel type per step) against the
+    same solves run one after another with the single-job API."""
+    from fdfk2d_amd import serve
+    from fdfk2d_amd.lib import Batch, Solver
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for i in range(njobs):
+        nx = int(rng.integers(500, 2001))
+        nz = int(rng.integers(200, 501))
+        reqs.append(dict(id=i, arrival=0.0, nx=nx, nz=nz, dx=100.0, nsteps=steps, src_is_p=bool(i % 2 == 0),
+                         p_skm=float(rng.uniform(0.04, 0.08)), seed=int(rng.integers(1 << 30)), npml=20))
+
+    def make(r):
+        job = serve.build_job(r)
+        s = Solver(r["nx"], r["nz"], r["npml"], 3, r["dx"], r["dx"], job["dt"], job["nt"], job["c"],
+                   rx=job["rx"], rz=job["rz"])
+        s.compute_background(job["fk"], job["fk"], r["src_is_p"], job["f0"], job["ds"], job["rayp"], job["bazs"])
+        return s
+
+    sol = [make(r) for r in reqs]
+    for s in sol:
+        s.sync()
+    t0 = time.perf_counter()
+    for s in sol:
+        s.run(s.nt + 1)
+    for s in sol:
+        s.sync()
+    t_seq = time.perf_counter() - t0
+    for s in sol:
+        s.close()
+    sol = [make(r) for r in reqs]
+    B = Batch(3)
+    for s in sol:
+        B.add(s)
+    t0 = time.perf_counter()
+    while B.active:
+        B.run(100)
+    B.sync()
+    t_bat = time.perf_counter() - t0
+    for s in sol:
+        s.close()
+    B.close()
+    cells = sum((r["nx"] + 40) * (r["nz"] + 20) * r["nsteps"] for r in reqs)
+    return {"metric": "Gcell-updates/s", "value": round(cells / t_bat / 1e9, 3), "unit": "Gcell-updates/s",
+            "higher_is_better": True, "data": "synthetic",
+            "config": {"workload": f"batch of {njobs} solves (seed {seed}), nx 500..2000, nz 200..500, "
+                                   f"dx 100 m, {steps} steps each"},
+            "batched_s": round(t_bat, 3), "sequential_s": round(t_seq, 3),
+            "sequential_gcells": round(cells / t_seq / 1e9, 3), "speedup": round(t_seq / t_bat, 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -111,9 +192,19 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run the copy kernel used to calibrate HBM byte counters")
+    ap.add_argument("--serve", type=int, default=0, metavar="N",
+                    help="serving measurement with N requests instead of the default line")
+    ap.add_argument("--batch", type=int, default=0, metavar="N",
+                    help="batched-execution measurement with N jobs instead of the default line")
     a = ap.parse_args()
     if a.copy_calib:
         copy_calibration()
+        return
+    if a.serve:
+        print(json.dumps(serve_bench(a.serve)), flush=True)
+        return
+    if a.batch:
+        print(json.dumps(batch_bench(a.batch)), flush=True)
         return
     if a.gpus != 1:
         print("bench.py: this build measures one GPU; the hot path shards as x-slabs "

@@ -114,8 +114,45 @@ size_t batch_table_bytes()
 
 // fill one job's arguments for the step it is about to take; returns the
 // block counts of the kernels
+// Interior column width shared by the members of one batched step: the
+// launch holds every member's blocks, so the one-round rule of the
+// single-job plan (narrow tiles for small grids) would leave the launch
+// dominated by warm-up columns.  Minimise rounds x (lx + warm-up) over the
+// whole batch instead.
 template <int NO>
-void fill_job(fdfk_solver *s, BatchJob<NO> &b, int nb[KB_N], bool &record)
+int batch_lx(const std::vector<fdfk_solver *> &act)
+{
+    constexpr int NT = 256;
+    constexpr int TZ = IntKernel2<NO, NT>::TZ;
+    constexpr int WARM = IntKernel2<NO, NT>::WARM;
+    static long slots = 0;
+    if (slots == 0) {
+        int dev = 0, ncu = 0, bpc = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, kb_interior<NO, NT>, NT, 0) == hipSuccess &&
+            ncu > 0 && bpc > 0)
+            slots = (long)ncu * bpc;
+        else
+            slots = 768;
+    }
+    int best = 64;
+    long best_cost = -1;
+    for (int lx = 8; lx <= 256; lx += 8) {
+        long nb = 0;
+        for (auto *s : act) {
+            const Geo &g = s->g;
+            const long nbz = (g.iz1 - g.iz0 + TZ) / TZ;
+            nb += nbz * ((g.ix1 - g.ix0 + lx) / lx);
+        }
+        const long cost = ((nb + slots - 1) / slots) * (lx + WARM);
+        if (best_cost < 0 || cost < best_cost) { best_cost = cost; best = lx; }
+    }
+    return best;
+}
+
+template <int NO>
+void fill_job(fdfk_solver *s, BatchJob<NO> &b, int nb[KB_N], bool &record, int lx)
 {
     const Geo &g = s->g;
     b.sa = step_args(s);
@@ -123,6 +160,8 @@ void fill_job(fdfk_solver *s, BatchJob<NO> &b, int nb[KB_N], bool &record)
     b.ringx = s->ringx;
     b.ringz = s->ringz;
     b.ia = interior_args<NO, 256>(s);
+    b.ia.lx = lx;
+    b.ia.nbx = (g.ix1 - g.ix0 + lx) / lx;
     b.ft = frame_tiles(g);
     record = (s->isis == s->it) && (s->istp <= s->nsis) && s->nrcv > 0;
     RecArgs &r = b.ra;
@@ -153,10 +192,11 @@ int batch_step(fdfk_batch *B)
     P->n = (int)act.size();
     for (int k = 0; k < KB_N; ++k) P->pre[k][0] = 0;
     std::vector<char> rec(act.size());
+    const int lx = batch_lx<NO>(act);
     for (size_t j = 0; j < act.size(); ++j) {
         int nb[KB_N];
         bool r = false;
-        fill_job<NO>(act[j], J[j], nb, r);
+        fill_job<NO>(act[j], J[j], nb, r, lx);
         rec[j] = r;
         for (int k = 0; k < KB_N; ++k) P->pre[k][j + 1] = P->pre[k][j] + nb[k];
     }
