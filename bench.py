@@ -188,7 +188,7 @@ def main():
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="c2", choices=["c1", "c2"])
-    ap.add_argument("--cpu-steps", type=int, default=41)
+    ap.add_argument("--cpu-steps", type=int, default=201)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run the copy kernel used to calibrate HBM byte counters")
