@@ -399,26 +399,25 @@ int one_step(fdfk_solver *s, bool record)
         hipEventCreate(&e0); hipEventCreate(&e1); hipEventCreate(&e2); hipEventCreate(&e3);
         hipEventRecord(e0, s->stream);
     }
-    if (record && s->nrcv > 0 && s->istp <= s->nsis) {
-        RecArgs r{};
+    RecArgs r{};
+    const bool rec = record && s->nrcv > 0 && s->istp <= s->nsis;
+    if (rec) {
         r.g = g; r.x0 = s->x0; r.z0 = s->z0; r.nrcv = s->nrcv; r.nsamp = s->nsamp; r.istp = s->istp;
         r.rx = s->rx; r.rz = s->rz; r.ux = s->ux[s->cur]; r.uz = s->uz[s->cur];
         r.seisx = s->seisx; r.seisz = s->seisz;
-        hipLaunchKernelGGL(k_record, dim3(s->nrcv), dim3(64), 0, s->stream, r);
     }
     StepArgs a = step_args(s);
-    if (a.ring_valid) {
-        long nr = ring_count(g);
-        hipLaunchKernelGGL(k_ring_prep, dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, s->stream, g, s->d_bg,
-                           s->it, s->ringx, s->ringz);
-    }
-    long npsi = 2L * g.npml * g.nz + (long)g.NX * g.npml;
-    dim3 pg((unsigned)((npsi + 255) / 256));
-    switch (g.norder) {
-    case 1: hipLaunchKernelGGL(k_psi<1>, pg, dim3(256), 0, s->stream, a); break;
-    case 2: hipLaunchKernelGGL(k_psi<2>, pg, dim3(256), 0, s->stream, a); break;
-    case 3: hipLaunchKernelGGL(k_psi<3>, pg, dim3(256), 0, s->stream, a); break;
-    default: hipLaunchKernelGGL(k_psi<4>, pg, dim3(256), 0, s->stream, a); break;
+    {
+        const int nring = a.ring_valid ? (int)((ring_count(g) + 255) / 256) : 0;
+        const int npsi = (int)((2L * g.npml * g.nz + (long)g.NX * g.npml + 255) / 256);
+        const int nrec = rec ? (s->nrcv + kRecPerBlock - 1) / kRecPerBlock : 0;
+        dim3 pg((unsigned)(nring + npsi + nrec)), pb(256);
+        switch (g.norder) {
+        case 1: hipLaunchKernelGGL(k_prep<1>, pg, pb, 0, s->stream, a, r, s->d_bg, nring, npsi); break;
+        case 2: hipLaunchKernelGGL(k_prep<2>, pg, pb, 0, s->stream, a, r, s->d_bg, nring, npsi); break;
+        case 3: hipLaunchKernelGGL(k_prep<3>, pg, pb, 0, s->stream, a, r, s->d_bg, nring, npsi); break;
+        default: hipLaunchKernelGGL(k_prep<4>, pg, pb, 0, s->stream, a, r, s->d_bg, nring, npsi); break;
+        }
     }
     if (s->prof) hipEventRecord(e1, s->stream);
     launch_interior(s);

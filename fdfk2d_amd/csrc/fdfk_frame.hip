@@ -171,7 +171,7 @@ __device__ __forceinline__ void frame_body(const StepArgs &a, const ETiles &et, 
         }
         dxx[e] = sx; dxz[e] = sz;
     }
-    // phase 1e: M psi of this step (k_psi) around the tile, as the memory
+    // phase 1e: M psi of this step (k_prep) around the tile, as the memory
     // corrections read them: zero outside the strips, psi_z only in the
     // bottom strip
     if (sb != S_NONE) {

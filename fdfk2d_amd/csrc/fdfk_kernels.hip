@@ -2,15 +2,15 @@
 // (src/FDFK2D.f90:516-522, binary-only in the reference) and the receiver
 // recorder loadseis (src/Submain.f90:33-116).
 //
-// One step n -> n+1 is four launches on the handle's stream:
-//   k_ring_prep background u0 on the hybrid ring at this step    ring only
-//   k_psi       first-derivative PML memories psi (level n)       strips only
+// One step n -> n+1 is three launches on the handle's stream:
+//   k_prep      background u0 on the hybrid ring at this step,     ring, strips,
+//               first-derivative PML memories psi (level n) and,  receivers
+//               on recording iterations, the receivers (level n)
 //   k_interior2 full-order fused stencil + leapfrog over the       ~97% of cells
 //               total-field region below the surface rows
 //   k_frame     everything else (fdfk_frame.hip): free-surface     the frame
 //               rows, reduced-order edge stencils, split-field
 //               PML strips, hybrid TF/SF ring
-// and, on recording iterations, k_record before them.
 //
 // k_interior2 is the hot kernel: 64-wide wavefronts run along z (coalesced
 // rows), each block marches along x keeping x-neighbourhoods in register
@@ -18,10 +18,10 @@
 // It reads u^n, u^{n-1}, dt^2/rho, lambda, mu once and writes u^{n+1} in
 // place of u^{n-1}: 36 algorithmic bytes per cell-update in FP32.
 //
-// k_record keeps the reference's FP64 interpolation weights and
+// The recorder keeps the reference's FP64 interpolation weights and
 // accumulation exactly.  The hybrid TF/SF correction enters as reads of the
-// per-step ring field (k_ring_prep) in the view of the output cell's side
-// instead of a second stencil pass.
+// per-step ring field in the view of the output cell's side instead of a
+// second stencil pass.
 #include "fdfk_internal.h"
 
 namespace fdfk {
@@ -163,10 +163,7 @@ __device__ __forceinline__ void ring_prep_body(const Geo &g, const Background *b
     ringx[c] = bg->a[6 * which + 2 * arr][src];
     ringz[c] = bg->a[6 * which + 2 * arr + 1][src];
 }
-__global__ __launch_bounds__(256) void k_ring_prep(Geo g, const Background *bg, int it, float *ringx, float *ringz)
-{
-    ring_prep_body(g, bg, it, ringx, ringz, blockIdx.x);
-}
+
 
 // ---------------------------------------------------------------------
 // PML first-derivative memories (phase A of the step)
@@ -204,11 +201,7 @@ __device__ __forceinline__ void psi_body(const StepArgs &a, long bid)
         p1[o] = b * p1[o] + c * gz;
     }
 }
-template <int NO>
-__global__ __launch_bounds__(256) void k_psi(StepArgs a)
-{
-    psi_body<NO>(a, blockIdx.x);
-}
+
 
 // ---------------------------------------------------------------------
 // receivers: 7x7 discretised-Dirac interpolation (Submain.f90:3-116)
@@ -229,17 +222,20 @@ __device__ double dirac_w(double x)
 }
 
 
-// One wave per receiver: lane l < 49 forms the weight and loads the field
-// of stencil point l, then lane 0 accumulates the 49 products in the
-// reference's loop order (ix outer, iz inner) with FP64 products rounded to
-// FP32 after every term, so the sum is the reference's bit for bit.
+// One wave per receiver (blockDim.x / 64 receivers per block): lane l < 49
+// forms the weight and loads the field of stencil point l, then lane 0
+// accumulates the 49 products in the reference's loop order (ix outer, iz
+// inner) with FP64 products rounded to FP32 after every term, so the sum is
+// the reference's bit for bit.
+constexpr int kRecPerBlock = 4;   // receivers per block in the fused prep kernel
 __device__ __forceinline__ void record_body(const RecArgs &r, long bid)
 {
 #pragma clang fp contract(off)
-    __shared__ double wx[64], wz[64];
-    const int i = (int)bid;
-    const int l = threadIdx.x;
-    if (i >= r.nrcv) return;
+    __shared__ double wx[kRecPerBlock][64], wz[kRecPerBlock][64];
+    const int w = threadIdx.x / 64;
+    const int i = (int)bid * (int)(blockDim.x / 64) + w;
+    const int l = threadIdx.x % 64;
+    if (i >= r.nrcv) return;   // wave-uniform; the exchange below stays inside the wave
     const Geo &g = r.g;
     const float rx = r.rx[i], rz = r.rz[i], x0 = r.x0, z0 = r.z0, dx = g.dx, dz = g.dz;
     const int irx = (int)floorf((rx - x0) / dx) + 1;
@@ -255,23 +251,35 @@ __device__ __forceinline__ void record_body(const RecArgs &r, long bid)
         double z = fabs((double)(z0 + (float)(iz - 1) * dz) - (double)rz) / (double)dz;
         double ba = dirac_w(x) * dirac_w(z);
         size_t c = cell(g, iz, ix);
-        wx[l] = ba * (double)r.ux[c];
-        wz[l] = ba * (double)r.uz[c];
+        wx[w][l] = ba * (double)r.ux[c];
+        wz[w][l] = ba * (double)r.uz[c];
     }
-    __syncthreads();
+    // the wave's own LDS writes are complete and visible to its lanes
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     if (l == 0) {
         float sx = 0.0f, sz = 0.0f;
         for (int k = 0; k < npt; ++k) {
-            sx = (float)((double)sx + wx[k]);
-            sz = (float)((double)sz + wz[k]);
+            sx = (float)((double)sx + wx[w][k]);
+            sz = (float)((double)sz + wz[w][k]);
         }
         r.seisx[(size_t)i * r.nsamp + r.istp] = sx;
         r.seisz[(size_t)i * r.nsamp + r.istp] = sz;
     }
 }
-__global__ __launch_bounds__(64) void k_record(RecArgs r)
+// The three small per-step kernels in one launch (they are independent:
+// the ring gather writes the ring field, the PML memories read u^n and
+// write psi, the recorder reads u^n): blocks [0, nring) gather the ring,
+// [nring, nring + npsi) update the memories, the rest record receivers.
+template <int NO>
+__global__ __launch_bounds__(256) void k_prep(StepArgs a, RecArgs r, const Background *bg, int nring, int npsi)
 {
-    record_body(r, blockIdx.x);
+    int b = blockIdx.x;
+    if (b < nring) { ring_prep_body(a.g, bg, a.it, (float *)a.ringx, (float *)a.ringz, b); return; }
+    b -= nring;
+    if (b < npsi) { psi_body<NO>(a, b); return; }
+    record_body(r, b - npsi);
 }
 
 #pragma clang fp contract(on)
