@@ -83,21 +83,32 @@ __device__ __forceinline__ void frame_body(const StepArgs &a, const ETiles &et, 
     }
     const FDiv dSZ(SZ), dtz(tz), dNFZ(NFZ), dNDX(NDX);
     auto S = [&](int z, int x) { return (x - lx0) * SZ + (z - lz0); };
-    // stage u (in the tile's view), lambda, mu
-    for (int e = threadIdx.x; e < SX * SZ; e += kFrameThreads) {
-        const int q = dSZ.quo(e);
-        const int x = lx0 + q, z = lz0 + (e - q * SZ);
-        float vx = 0.0f, vz = 0.0f, vl = 0.0f, vm = 0.0f;
-        if (x >= 1 - g.npml && x <= g.nx + g.npml && z >= 1 && z <= g.NZ) {
-            const size_t c = cell(g, z, x);
-            vx = a.ux[c]; vz = a.uz[c]; vl = a.lam[c]; vm = a.mu[c];
-            if (a.ring_valid) {
-                const int side = tf_side(g, z, x);
-                if (view == 1 && side == 0) { vx -= a.ringx[c]; vz -= a.ringz[c]; }
-                else if (view == 0 && side == 1) { vx += a.ringx[c]; vz += a.ringz[c]; }
+    // stage u (in the tile's view), lambda, mu; four elements per thread
+    // per pass so that their loads are in flight together
+    constexpr int SU = 4;
+    for (int e0 = threadIdx.x; e0 < SX * SZ; e0 += SU * kFrameThreads) {
+        float vx[SU], vz[SU], vl[SU], vm[SU];
+#pragma unroll
+        for (int u = 0; u < SU; ++u) {
+            const int e = e0 + u * kFrameThreads;
+            const int q = dSZ.quo(e);
+            const int x = lx0 + q, z = lz0 + (e - q * SZ);
+            vx[u] = 0.0f; vz[u] = 0.0f; vl[u] = 0.0f; vm[u] = 0.0f;
+            if (e < SX * SZ && x >= 1 - g.npml && x <= g.nx + g.npml && z >= 1 && z <= g.NZ) {
+                const size_t c = cell(g, z, x);
+                vx[u] = a.ux[c]; vz[u] = a.uz[c]; vl[u] = a.lam[c]; vm[u] = a.mu[c];
+                if (a.ring_valid) {
+                    const int side = tf_side(g, z, x);
+                    if (view == 1 && side == 0) { vx[u] -= a.ringx[c]; vz[u] -= a.ringz[c]; }
+                    else if (view == 0 && side == 1) { vx[u] += a.ringx[c]; vz[u] += a.ringz[c]; }
+                }
             }
         }
-        sux[e] = vx; suz[e] = vz; slam[e] = vl; smu[e] = vm;
+#pragma unroll
+        for (int u = 0; u < SU; ++u) {
+            const int e = e0 + u * kFrameThreads;
+            if (e < SX * SZ) { sux[e] = vx[u]; suz[e] = vz[u]; slam[e] = vl[u]; smu[e] = vm[u]; }
+        }
     }
     __syncthreads();
     const int nxp = g.nx + g.npml, NZ = g.NZ;
@@ -175,17 +186,19 @@ __device__ __forceinline__ void frame_body(const StepArgs &a, const ETiles &et, 
     // corrections read them: zero outside the strips, psi_z only in the
     // bottom strip
     if (sb != S_NONE) {
+        // around a band's tile a cell is either in the band's strip or in
+        // the total-field region (psi 0): one strip, uniform pointers
         const Strips *st = a.st;
+        const float *ps0 = st->psx[sb][0], *ps1 = st->psx[sb][1];
         for (int e = threadIdx.x; e < NDZ * tz; e += kFrameThreads) {
             const int qq = dtz.quo(e);
             const int x = dz0 + qq, z = cz0 + (e - qq * tz);
             float v0 = 0.0f, v1 = 0.0f;
-            const int s2 = strip_of(g, z, x);
-            if (s2 != S_NONE && x >= 1 - g.npml && x <= nxp && z <= NZ) {
-                const size_t o = strip_off(g, s2, z, x);
+            if (strip_of(g, z, x) == sb && x >= 1 - g.npml && x <= nxp && z <= NZ) {
+                const size_t o = strip_off(g, sb, z, x);
                 const int k = S(z, x);
-                v0 = (slam[k] + 2.0f * smu[k]) * st->psx[s2][0][o];
-                v1 = smu[k] * st->psx[s2][1][o];
+                v0 = (slam[k] + 2.0f * smu[k]) * ps0[o];
+                v1 = smu[k] * ps1[o];
             }
             mpx0[e] = v0; mpx1[e] = v1;
         }
