@@ -83,6 +83,67 @@ __device__ __forceinline__ void frame_body(const StepArgs &a, const ETiles &et, 
     }
     const FDiv dSZ(SZ), dtz(tz), dNFZ(NFZ), dNDX(NDX);
     auto S = [&](int z, int x) { return (x - lx0) * SZ + (z - lz0); };
+    const int nxp = g.nx + g.npml, NZ = g.NZ;
+    // Global loads whose addresses do not depend on LDS are issued first so
+    // that they overlap the staging and phase 1: the raw psi around the tile
+    // (the first PF entries of each thread) and the update inputs of the
+    // thread's own cell.
+    constexpr int PF = 3;
+    const Strips *st = a.st;
+    float rpx[PF][2], rpz[PF][2];
+    if (sb != S_NONE) {
+        const float *ps0 = st->psx[sb][0], *ps1 = st->psx[sb][1];
+#pragma unroll
+        for (int i = 0; i < PF; ++i) {
+            const int e = threadIdx.x + i * kFrameThreads;
+            const int qq = dtz.quo(e);
+            const int x = dz0 + qq, z = cz0 + (e - qq * tz);
+            rpx[i][0] = 0.0f; rpx[i][1] = 0.0f;
+            if (e < NDZ * tz && strip_of(g, z, x) == sb && x >= 1 - g.npml && x <= nxp && z <= NZ) {
+                const size_t o = strip_off(g, sb, z, x);
+                rpx[i][0] = ps0[o]; rpx[i][1] = ps1[o];
+            }
+        }
+        if (sb == S_B) {
+            const float *pz0 = st->psz[S_B][0], *pz1 = st->psz[S_B][1];
+#pragma unroll
+            for (int i = 0; i < PF; ++i) {
+                const int e = threadIdx.x + i * kFrameThreads;
+                const int qq = dNDX.quo(e);
+                const int x = cx0 + qq, z = dx0 + (e - qq * NDX);
+                rpz[i][0] = 0.0f; rpz[i][1] = 0.0f;
+                if (e < tx * NDX && z > g.nz && z <= NZ) {
+                    const size_t o = strip_off(g, S_B, z, x);
+                    rpz[i][0] = pz0[o]; rpz[i][1] = pz1[o];
+                }
+            }
+        }
+    }
+    const int t = threadIdx.x;
+    const bool own = t < tx * tz;
+    const int tq = dtz.quo(t);
+    const int ix = cx0 + tq, iz = cz0 + (t - tq * tz);
+    float ubf = 0.0f, uax = 0.0f, uaz = 0.0f, ucur[2][3] = {}, uprv[2][3] = {};
+    size_t uc = 0, uo = 0;
+    if (own) {
+        uc = cell(g, iz, ix);
+        ubf = a.bf[uc];
+        if (sb == S_NONE) {
+            ucur[0][0] = a.ux[uc]; ucur[1][0] = a.uz[uc];
+            uprv[0][0] = a.uxp[uc]; uprv[1][0] = a.uzp[uc];
+        } else {
+            uo = strip_off(g, sb, iz, ix);
+            uax = st->ax[ix + g.npml - 1];
+            uaz = st->az[iz - 1];
+#pragma unroll
+            for (int comp = 0; comp < 2; ++comp)
+#pragma unroll
+                for (int q2 = 0; q2 < 3; ++q2) {
+                    ucur[comp][q2] = st->cur[sb][comp][q2][uo];
+                    uprv[comp][q2] = st->prev[sb][comp][q2][uo];
+                }
+        }
+    }
     // stage u (in the tile's view), lambda, mu; four elements per thread
     // per pass so that their loads are in flight together
     constexpr int SU = 4;
@@ -111,7 +172,6 @@ __device__ __forceinline__ void frame_body(const StepArgs &a, const ETiles &et, 
         }
     }
     __syncthreads();
-    const int nxp = g.nx + g.npml, NZ = g.NZ;
     // phase 1a: x-fluxes (mode 0 on ux, mode 1 on uz)
     for (int e = threadIdx.x; e < NFX * tz; e += kFrameThreads) {
         const int q = dtz.quo(e);
@@ -188,40 +248,44 @@ __device__ __forceinline__ void frame_body(const StepArgs &a, const ETiles &et, 
     if (sb != S_NONE) {
         // around a band's tile a cell is either in the band's strip or in
         // the total-field region (psi 0): one strip, uniform pointers
-        const Strips *st = a.st;
         const float *ps0 = st->psx[sb][0], *ps1 = st->psx[sb][1];
-        for (int e = threadIdx.x; e < NDZ * tz; e += kFrameThreads) {
+        for (int i = 0, e = threadIdx.x; e < NDZ * tz; ++i, e += kFrameThreads) {
             const int qq = dtz.quo(e);
             const int x = dz0 + qq, z = cz0 + (e - qq * tz);
-            float v0 = 0.0f, v1 = 0.0f;
-            if (strip_of(g, z, x) == sb && x >= 1 - g.npml && x <= nxp && z <= NZ) {
+            float r0 = 0.0f, r1 = 0.0f;
+            if (i < PF) {
+#pragma unroll
+                for (int j = 0; j < PF; ++j)
+                    if (j == i) { r0 = rpx[j][0]; r1 = rpx[j][1]; }
+            } else if (strip_of(g, z, x) == sb && x >= 1 - g.npml && x <= nxp && z <= NZ) {
                 const size_t o = strip_off(g, sb, z, x);
-                const int k = S(z, x);
-                v0 = (slam[k] + 2.0f * smu[k]) * ps0[o];
-                v1 = smu[k] * ps1[o];
+                r0 = ps0[o]; r1 = ps1[o];
             }
-            mpx0[e] = v0; mpx1[e] = v1;
+            const int k = S(z, x);
+            mpx0[e] = (slam[k] + 2.0f * smu[k]) * r0;
+            mpx1[e] = smu[k] * r1;
         }
         if (sb == S_B)
-            for (int e = threadIdx.x; e < tx * NDX; e += kFrameThreads) {
+            for (int i = 0, e = threadIdx.x; e < tx * NDX; ++i, e += kFrameThreads) {
                 const int qq = dNDX.quo(e);
                 const int x = cx0 + qq, z = dx0 + (e - qq * NDX);
-                float v0 = 0.0f, v1 = 0.0f;
-                if (z > g.nz && z <= NZ) {
+                float r0 = 0.0f, r1 = 0.0f;
+                if (i < PF) {
+#pragma unroll
+                    for (int j = 0; j < PF; ++j)
+                        if (j == i) { r0 = rpz[j][0]; r1 = rpz[j][1]; }
+                } else if (z > g.nz && z <= NZ) {
                     const size_t o = strip_off(g, S_B, z, x);
-                    const int k = S(z, x);
-                    v0 = smu[k] * st->psz[S_B][0][o];
-                    v1 = (slam[k] + 2.0f * smu[k]) * st->psz[S_B][1][o];
+                    r0 = st->psz[S_B][0][o]; r1 = st->psz[S_B][1][o];
                 }
-                mpz0[e] = v0; mpz1[e] = v1;
+                const int k = S(z, x);
+                mpz0[e] = smu[k] * r0;
+                mpz1[e] = (slam[k] + 2.0f * smu[k]) * r1;
             }
     }
     __syncthreads();
     // phase 2: one thread per cell
-    const int t = threadIdx.x;
-    if (t >= tx * tz) return;
-    const int tq = dtz.quo(t);
-    const int ix = cx0 + tq, iz = cz0 + (t - tq * tz);
+    if (!own) return;
     const int nox = imin(imin(ix + g.npml - 1, nxp - ix), NO);
     const int noz = imin(imin(iz - 1, NZ - iz), NO);
     auto FX = [&](const float *b, int f) { return b[(f - fx0) * tz + (iz - cz0)]; };
@@ -300,30 +364,23 @@ __device__ __forceinline__ void frame_body(const StepArgs &a, const ETiles &et, 
         }
     }
     // update: plain leapfrog in the total-field region, split parts in a strip
-    const size_t c = cell(g, iz, ix);
-    const float bf = a.bf[c];
     if (sb == S_NONE) {
         const float rxs = (p[0] + p[1]) + p[2], rzs = (p[3] + p[4]) + p[5];
-        a.uxp[c] = (2.0f * a.ux[c] - a.uxp[c]) + bf * rxs;
-        a.uzp[c] = (2.0f * a.uz[c] - a.uzp[c]) + bf * rzs;
+        a.uxp[uc] = (2.0f * ucur[0][0] - uprv[0][0]) + ubf * rxs;
+        a.uzp[uc] = (2.0f * ucur[1][0] - uprv[1][0]) + ubf * rzs;
         return;
     }
-    const Strips *st = a.st;
-    const size_t o = strip_off(g, sb, iz, ix);
-    const float ax = st->ax[ix + g.npml - 1];
-    const float az = st->az[iz - 1];
+    const float ax = uax, az = uaz;
     const float c3 = 2.0f - 4.0f * ax * az;
 #pragma unroll
     for (int comp = 0; comp < 2; ++comp) {
-        const float *b1c = st->cur[sb][comp][0], *b2c = st->cur[sb][comp][1], *b3c = st->cur[sb][comp][2];
-        float *b1p = st->prev[sb][comp][0], *b2p = st->prev[sb][comp][1], *b3p = st->prev[sb][comp][2];
         const float X = p[3 * comp], Z = p[3 * comp + 1], Mm = p[3 * comp + 2];
-        const float n1 = (2.0f * b1c[o] - (1.0f - ax) * b1p[o] + bf * X) / (1.0f + ax);
-        const float n2 = (2.0f * b2c[o] - (1.0f - az) * b2p[o] + bf * Z) / (1.0f + az);
-        const float n3 = (c3 * b3c[o] - (1.0f - ax - az) * b3p[o] + bf * Mm) / (1.0f + ax + az);
-        b1p[o] = n1; b2p[o] = n2; b3p[o] = n3;
+        const float n1 = (2.0f * ucur[comp][0] - (1.0f - ax) * uprv[comp][0] + ubf * X) / (1.0f + ax);
+        const float n2 = (2.0f * ucur[comp][1] - (1.0f - az) * uprv[comp][1] + ubf * Z) / (1.0f + az);
+        const float n3 = (c3 * ucur[comp][2] - (1.0f - ax - az) * uprv[comp][2] + ubf * Mm) / (1.0f + ax + az);
+        st->prev[sb][comp][0][uo] = n1; st->prev[sb][comp][1][uo] = n2; st->prev[sb][comp][2][uo] = n3;
         const float tot = (n1 + n2) + n3;
-        if (comp == 0) a.uxp[c] = tot; else a.uzp[c] = tot;
+        if (comp == 0) a.uxp[uc] = tot; else a.uzp[uc] = tot;
     }
 }
 
