@@ -9,10 +9,12 @@ FD order 6, 401 receivers, seeded layered + 5% smoothed random model).
 value = padded-grid cell-updates per second (NX*NZ per step, src/FDFK2D.f90
 loop bounds), inputs resident in HBM.
 
-Prints one JSON line with the roofline of the interior kernel (36
-algorithmic bytes per cell-update, HIP events on the solver's stream) and a
-CPU baseline: the FP32 CPU oracle (the reference's separate operator
-passes, OpenMP) timed on a bounded sample of the 1001 x 400 workload.
+Prints one JSON line with the roofline of the dominant kernel, k_step (the
+interior and the frame of a step in one launch; algorithmic bytes from
+step_alg_bytes, duration from HIP events on the solver's stream in a
+separate profiled pass after the timed steps) and a CPU baseline: the FP32
+CPU oracle (the reference's separate operator passes, OpenMP) timed on a
+bounded sample of the 1001 x 400 workload.
 """
 import argparse
 import json
@@ -26,16 +28,33 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_CELL = 36  # u^n (8) + u^{n-1} (8) + u^{n+1} (8) + dt^2/rho, lambda, mu (12)
-# HBM bytes per cell-update of k_interior2 on the 8001x2000 workload, from
-# rocprofv3 FETCH_SIZE / WRITE_SIZE scaled by a plain device copy
-# (profiles/k_interior2_8001x2000_hbm_traffic.md)
-MEASURED_TRAFFIC_B_PER_CELL = 47.56
+PROFILED_STEPS = 50
+# HBM bytes per k_step launch on the 8001x2000 workload, from rocprofv3
+# FETCH_SIZE / WRITE_SIZE scaled by a plain device copy
+# (profiles/k_step_8001x2000_hbm_traffic.md); None until measured
+MEASURED_TRAFFIC_BYTES = None
+
+
+def step_alg_bytes(nx, nz, npml, norder):
+    """Algorithmic HBM bytes of one k_step launch and its cell count.
+
+    Total-field cells (the interior, columns 1..nx x rows 2*norder..nz, and
+    the surface rows 1..2*norder-1): 36 B each.  PML strip cells (left and
+    right strips over rows 1..nz, bottom strip over all updated columns)
+    read u^n (8 B) and the materials (12 B) and write u^{n+1} (8 B) like the
+    others, but carry the split field instead of u^{n-1}: three parts x two
+    components read at two levels and written at one (72 B), plus the
+    first-derivative memories, x only in the side strips (8 B), x and z in
+    the bottom strip (16 B)."""
+    tf = nx * nz
+    side = 2 * (npml - 1) * nz
+    bottom = (nx + 2 * (npml - 1)) * (npml - 1)
+    return 36 * tf + (28 + 72 + 8) * side + (28 + 72 + 16) * bottom, tf + side + bottom
 
 
 def gpu_bench(steps, warmup, workload):
     from fdfk2d_amd import configs, lib
-    nt = warmup + steps + 1
+    nt = warmup + steps + PROFILED_STEPS + 1
     cfg = configs.c2(nt=nt) if workload == "c2" else configs.c1(nt=nt)
     s = lib.Solver(cfg["nx"], cfg["nz"], cfg["npml"], cfg["norder"], cfg["dx"], cfg["dz"], cfg["dt"],
                    nt, cfg["c"], rx=cfg["rx"], rz=cfg["rz"])
@@ -48,22 +67,24 @@ def gpu_bench(steps, warmup, workload):
         s.set_background([np.zeros(n, np.float32) for n in lib.bg_sizes(cfg["nx"], cfg["nz"], 5, nt)])
     s.run(warmup)
     s.sync()
-    s.profile(True)
     t0 = time.perf_counter()
     s.run(steps)
     s.sync()
     t1 = time.perf_counter()
+    # kernel durations from a separate pass, so that the timed steps carry
+    # no event packets
+    s.profile(True)
+    s.run(PROFILED_STEPS)
+    s.sync()
     st = s.kernel_stats()
     NX, NZ = cfg["NX"], cfg["NZ"]
-    # the interior kernel covers the total-field region below the surface
-    # rows: columns 1..nx, rows 2*norder..nz
-    int_cells = cfg["nx"] * (cfg["nz"] - 2 * cfg["norder"] + 1)
+    alg_bytes, alg_cells = step_alg_bytes(cfg["nx"], cfg["nz"], cfg["npml"], cfg["norder"])
     sx, sz = s.seismograms()
     finite = bool(np.isfinite(sx).all() and np.isfinite(sz).all())
     s.close()
     wall = t1 - t0
-    return dict(cfg=cfg, NX=NX, NZ=NZ, wall=wall, stats=st, int_cells=int_cells, bg_mode=bg_mode,
-                finite=finite)
+    return dict(cfg=cfg, NX=NX, NZ=NZ, wall=wall, stats=st, alg_bytes=alg_bytes, alg_cells=alg_cells,
+                bg_mode=bg_mode, finite=finite)
 
 
 def cpu_baseline(sample_steps):
@@ -215,8 +236,8 @@ def main():
     value = cells * a.steps / r["wall"] / 1e9
     ms = r["wall"] / a.steps * 1e3
     st = r["stats"]
-    int_ms = st["interior_ms"] / max(st["interior_n"], 1)
-    achieved = BYTES_PER_CELL * r["int_cells"] / (int_ms * 1e-3) / 1e9
+    k_ms = st["interior_ms"] / max(st["interior_n"], 1)
+    achieved = r["alg_bytes"] / (k_ms * 1e-3) / 1e9
     cfg = r["cfg"]
     full_steps = 20000
     out = {
@@ -246,13 +267,13 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": (round(MEASURED_TRAFFIC_B_PER_CELL * r["int_cells"]) if a.workload == "c2" else None),
+            "traffic": (MEASURED_TRAFFIC_BYTES if a.workload == "c2" else None),
             "traffic_source": "rocprofv3 FETCH_SIZE+WRITE_SIZE per launch, copy-calibrated "
-                              "(profiles/k_interior2_8001x2000_hbm_traffic.md)",
-            "kernel": "k_interior2",
-            "kernel_avg_ms": round(int_ms, 5),
-            "bytes_per_cell": BYTES_PER_CELL,
-            "cells_per_launch": r["int_cells"],
+                              "(profiles/k_step_8001x2000_hbm_traffic.md)",
+            "kernel": "k_step",
+            "kernel_avg_ms": round(k_ms, 5),
+            "alg_bytes_per_launch": r["alg_bytes"],
+            "cells_per_launch": r["alg_cells"],
         },
     }
     if not a.no_cpu:

@@ -91,6 +91,7 @@ struct fdfk_solver {
     float *stage = nullptr;   // (2, NZ, NX) staging for reference-layout copies
     // interior kernel
     int lx = 45, nt_threads = 256;
+    bool fused = true;   // interior and frame in one launch (k_step)
     // profiling
     bool prof = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_int, ev_step;
@@ -260,15 +261,29 @@ struct IntPlan {
     long cost = -1;
 };
 
+ETiles frame_tiles(const Geo &g);
+size_t frame_lds_bytes(const ETiles &t, int norder);
+
+template <int NO>
+size_t step_lds_bytes(const Geo &g)
+{
+    return std::max(sizeof(typename IntKernel2<NO, 256>::Smem), frame_lds_bytes(frame_tiles(g), g.norder));
+}
+
 template <int NO, int NT>
-IntPlan plan_lx(const Geo &g)
+IntPlan plan_lx(const Geo &g, bool fused)
 {
     IntPlan best;
     int dev = 0, ncu = 0, bpc = 0;
+    hipError_t oe;
+    if constexpr (NT == 256)
+        oe = fused ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_step<NO>, NT, step_lds_bytes<NO>(g))
+                   : hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_interior2<NO, NT>, NT, 0);
+    else
+        oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_interior2<NO, NT>, NT, 0);
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_interior2<NO, NT>, NT, 0) != hipSuccess ||
-        ncu <= 0 || bpc <= 0)
+        oe != hipSuccess || ncu <= 0 || bpc <= 0)
         return best;
     constexpr int TZ = IntKernel2<NO, NT>::TZ;
     constexpr int WARM = IntKernel2<NO, NT>::WARM;
@@ -287,28 +302,32 @@ IntPlan plan_lx(const Geo &g)
 }
 
 // block height (threads along z) and column width of the interior launch;
-// FDFK_NT / FDFK_LX override them
+// FDFK_NT / FDFK_LX override them (FDFK_FUSED=0 launches the interior and
+// the frame separately)
 void plan_interior(fdfk_solver *s)
 {
     const Geo &g = s->g;
     const char *ent = std::getenv("FDFK_NT");
     const int want = ent ? std::atoi(ent) : 0;
+    const char *ef = std::getenv("FDFK_FUSED");
+    const bool fused = !(ef && std::atoi(ef) == 0) && want != 128 && want != 192;
     IntPlan p;
     int nt = 256;
     auto consider = [&](int n, IntPlan q) {
         if (q.cost >= 0 && (p.cost < 0 || q.cost < p.cost)) { p = q; nt = n; }
     };
     switch (g.norder) {
-    case 1: consider(256, plan_lx<1, 256>(g)); break;
-    case 2: consider(256, plan_lx<2, 256>(g)); break;
+    case 1: consider(256, plan_lx<1, 256>(g, fused)); break;
+    case 2: consider(256, plan_lx<2, 256>(g, fused)); break;
     case 3:
-        if (want == 192) consider(192, plan_lx<3, 192>(g));
-        else if (want == 128) consider(128, plan_lx<3, 128>(g));
-        else consider(256, plan_lx<3, 256>(g));
+        if (want == 192) consider(192, plan_lx<3, 192>(g, false));
+        else if (want == 128) consider(128, plan_lx<3, 128>(g, false));
+        else consider(256, plan_lx<3, 256>(g, fused));
         break;
-    default: consider(256, plan_lx<4, 256>(g)); break;
+    default: consider(256, plan_lx<4, 256>(g, fused)); break;
     }
     s->nt_threads = nt;
+    s->fused = fused && nt == 256;
     s->lx = p.lx;
     if (const char *e = std::getenv("FDFK_LX")) s->lx = std::max(4, std::atoi(e));
 }
@@ -377,6 +396,26 @@ size_t frame_lds_bytes(const ETiles &t, int norder)
     return (size_t)m * sizeof(float);
 }
 
+template <int NO>
+void launch_step_t(fdfk_solver *s, const StepArgs &a)
+{
+    IntArgs<NO> ia = interior_args<NO, 256>(s);
+    ETiles t = frame_tiles(s->g);
+    const int nint = ia.nbx * ia.nbz;
+    hipLaunchKernelGGL(k_step<NO>, dim3((unsigned)(nint + t.off[t.n])), dim3(256), step_lds_bytes<NO>(s->g),
+                       s->stream, ia, a, t, nint);
+}
+
+void launch_step(fdfk_solver *s, const StepArgs &a)
+{
+    switch (s->g.norder) {
+    case 1: launch_step_t<1>(s, a); break;
+    case 2: launch_step_t<2>(s, a); break;
+    case 3: launch_step_t<3>(s, a); break;
+    default: launch_step_t<4>(s, a); break;
+    }
+}
+
 void launch_frame(fdfk_solver *s, const StepArgs &a)
 {
     ETiles t = frame_tiles(s->g);
@@ -420,9 +459,14 @@ int one_step(fdfk_solver *s, bool record)
         }
     }
     if (s->prof) hipEventRecord(e1, s->stream);
-    launch_interior(s);
-    if (s->prof) hipEventRecord(e2, s->stream);
-    launch_frame(s, a);
+    if (s->fused) {
+        launch_step(s, a);
+        if (s->prof) hipEventRecord(e2, s->stream);
+    } else {
+        launch_interior(s);
+        if (s->prof) hipEventRecord(e2, s->stream);
+        launch_frame(s, a);
+    }
     if (s->prof) {
         hipEventRecord(e3, s->stream);
         s->ev_int.emplace_back(e1, e2);

@@ -1,5 +1,5 @@
-// Batched execution: many resident simulations advanced together, one
-// launch per kernel type per step for all of them (a tile work-list over
+// Batched execution: many resident simulations advanced together, two
+// launches per step for all of them (kb_prep, kb_step: a tile work-list over
 // jobs).  Each job keeps its own HBM state (a fdfk_solver handle); the batch
 // only builds, every step, a table of per-job kernel arguments plus prefix
 // sums of each job's block counts, uploads it from pinned memory and
@@ -41,40 +41,51 @@ __device__ __forceinline__ int find_job(const int *pre, int n, int bid)
     return lo;
 }
 
+// one launch for the per-step preparation of every member: the hybrid ring
+// background, the PML memories and (on recording iterations) the receivers,
+// in that block order; 256-lane blocks, kRecPerBlock receivers per block
 template <int NO>
-__global__ __launch_bounds__(256) void kb_ring(const BatchJob<NO> *J, const BatchPrefix *P)
+__global__ __launch_bounds__(256) void kb_prep(const BatchJob<NO> *J, const BatchPrefix *P)
 {
-    int j = find_job(P->pre[KB_RING], P->n, blockIdx.x);
-    const BatchJob<NO> &b = J[j];
-    ring_prep_body(b.sa.g, b.bg, b.sa.it, b.ringx, b.ringz, blockIdx.x - P->pre[KB_RING][j]);
+    const int n = P->n;
+    int b = blockIdx.x;
+    const int nr = P->pre[KB_RING][n], np = P->pre[KB_PSI][n];
+    if (b < nr) {
+        const int j = find_job(P->pre[KB_RING], n, b);
+        const BatchJob<NO> &q = J[j];
+        ring_prep_body(q.sa.g, q.bg, q.sa.it, q.ringx, q.ringz, b - P->pre[KB_RING][j]);
+        return;
+    }
+    b -= nr;
+    if (b < np) {
+        const int j = find_job(P->pre[KB_PSI], n, b);
+        psi_body<NO>(J[j].sa, b - P->pre[KB_PSI][j]);
+        return;
+    }
+    b -= np;
+    const int j = find_job(P->pre[KB_REC], n, b);
+    record_body(J[j].ra, b - P->pre[KB_REC][j]);
 }
 
+// the interior tiles of every member, then their frame tiles, in one launch
+// (the single-job k_step arrangement)
 template <int NO>
-__global__ __launch_bounds__(256) void kb_psi(const BatchJob<NO> *J, const BatchPrefix *P)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void kb_step(const BatchJob<NO> *J,
+                                                                                       const BatchPrefix *P)
 {
-    int j = find_job(P->pre[KB_PSI], P->n, blockIdx.x);
-    psi_body<NO>(J[j].sa, blockIdx.x - P->pre[KB_PSI][j]);
-}
-
-template <int NO, int NT>
-__global__ __launch_bounds__(NT) void kb_interior(const BatchJob<NO> *J, const BatchPrefix *P)
-{
-    int j = find_job(P->pre[KB_INT], P->n, blockIdx.x);
-    IntKernel2<NO, NT>::run(J[j].ia, blockIdx.x - P->pre[KB_INT][j]);
-}
-
-template <int NO>
-__global__ __launch_bounds__(kFrameThreads) void kb_frame(const BatchJob<NO> *J, const BatchPrefix *P)
-{
-    int j = find_job(P->pre[KB_FRAME], P->n, blockIdx.x);
-    frame_body<NO>(J[j].sa, J[j].ft, blockIdx.x - P->pre[KB_FRAME][j]);
-}
-
-template <int NO>
-__global__ __launch_bounds__(64) void kb_record(const BatchJob<NO> *J, const BatchPrefix *P)
-{
-    int j = find_job(P->pre[KB_REC], P->n, blockIdx.x);
-    record_body(J[j].ra, blockIdx.x - P->pre[KB_REC][j]);
+    extern __shared__ float sm[];
+    const int n = P->n;
+    const int ni = P->pre[KB_INT][n];
+    if ((int)blockIdx.x < ni) {
+        const int t = xcd_tile(blockIdx.x, ni);
+        const int j = find_job(P->pre[KB_INT], n, t);
+        IntKernel2<NO, 256>::run(J[j].ia, t - P->pre[KB_INT][j],
+                                 *reinterpret_cast<typename IntKernel2<NO, 256>::Smem *>(sm));
+    } else {
+        const int b = (int)blockIdx.x - ni;
+        const int j = find_job(P->pre[KB_FRAME], n, b);
+        frame_body<NO>(J[j].sa, J[j].ft, b - P->pre[KB_FRAME][j]);
+    }
 }
 
 }  // namespace fdfk
@@ -127,10 +138,12 @@ int batch_lx(const std::vector<fdfk_solver *> &act)
     constexpr int WARM = IntKernel2<NO, NT>::WARM;
     static long slots = 0;
     if (slots == 0) {
+        // occupancy at the largest dynamic LDS a batched step can ask for
+        const size_t sh = std::max(sizeof(typename IntKernel2<NO, NT>::Smem), (size_t)kFrameLds * sizeof(float));
         int dev = 0, ncu = 0, bpc = 0;
         if (hipGetDevice(&dev) == hipSuccess &&
             hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, kb_interior<NO, NT>, NT, 0) == hipSuccess &&
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, kb_step<NO>, NT, sh) == hipSuccess &&
             ncu > 0 && bpc > 0)
             slots = (long)ncu * bpc;
         else
@@ -175,7 +188,7 @@ void fill_job(fdfk_solver *s, BatchJob<NO> &b, int nb[KB_N], bool &record, int l
     nb[KB_PSI] = (int)((npsi + 255) / 256);
     nb[KB_INT] = b.ia.nbx * b.ia.nbz;
     nb[KB_FRAME] = b.ft.off[b.ft.n];
-    nb[KB_REC] = record ? s->nrcv : 0;
+    nb[KB_REC] = record ? (s->nrcv + kRecPerBlock - 1) / kRecPerBlock : 0;
 }
 
 template <int NO>
@@ -211,15 +224,14 @@ int batch_step(fdfk_batch *B)
         hipEventCreate(&e0); hipEventCreate(&e1); hipEventCreate(&e2); hipEventCreate(&e3);
         hipEventRecord(e0, B->stream);
     }
-    if (P->pre[KB_REC][n]) hipLaunchKernelGGL(kb_record<NO>, dim3(P->pre[KB_REC][n]), dim3(64), 0, B->stream, dJ, dP);
-    if (P->pre[KB_RING][n]) hipLaunchKernelGGL(kb_ring<NO>, dim3(P->pre[KB_RING][n]), dim3(256), 0, B->stream, dJ, dP);
-    if (P->pre[KB_PSI][n]) hipLaunchKernelGGL(kb_psi<NO>, dim3(P->pre[KB_PSI][n]), dim3(256), 0, B->stream, dJ, dP);
+    const int nprep = P->pre[KB_RING][n] + P->pre[KB_PSI][n] + P->pre[KB_REC][n];
+    if (nprep) hipLaunchKernelGGL(kb_prep<NO>, dim3(nprep), dim3(256), 0, B->stream, dJ, dP);
     if (B->prof) hipEventRecord(e1, B->stream);
-    if (P->pre[KB_INT][n]) hipLaunchKernelGGL((kb_interior<NO, 256>), dim3(P->pre[KB_INT][n]), dim3(256), 0, B->stream, dJ, dP);
+    size_t sh = sizeof(typename IntKernel2<NO, 256>::Smem);
+    for (size_t j = 0; j < act.size(); ++j) sh = std::max(sh, frame_lds_bytes(J[j].ft, NO));
+    const int nstep = P->pre[KB_INT][n] + P->pre[KB_FRAME][n];
+    if (nstep) hipLaunchKernelGGL(kb_step<NO>, dim3(nstep), dim3(256), sh, B->stream, dJ, dP);
     if (B->prof) hipEventRecord(e2, B->stream);
-    size_t fsh = 0;
-    for (size_t j = 0; j < act.size(); ++j) fsh = std::max(fsh, frame_lds_bytes(J[j].ft, NO));
-    if (P->pre[KB_FRAME][n]) hipLaunchKernelGGL(kb_frame<NO>, dim3(P->pre[KB_FRAME][n]), dim3(kFrameThreads), fsh, B->stream, dJ, dP);
     if (B->prof) {
         hipEventRecord(e3, B->stream);
         B->ev_int.emplace_back(e1, e2);

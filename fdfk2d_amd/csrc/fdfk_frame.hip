@@ -390,4 +390,23 @@ __global__ __launch_bounds__(kFrameThreads) void k_frame(StepArgs a, ETiles t)
     frame_body<NO>(a, t, blockIdx.x);
 }
 
+// The interior and the frame of one step in one launch: blocks
+// [0, nint) are interior tiles, the rest frame tiles.  Both read level n
+// only and write disjoint cells of level n+1, so any interleaving is valid.
+// Workgroups are dispatched in id order, so the interior tiles take every
+// resident slot first and the frame tiles fill the slots that interior
+// tiles release at the end of their march, instead of running after the
+// slowest of them.  Dynamic LDS: the larger of the two needs.
+static_assert(kFrameThreads == 256, "the fused step launches 256-lane blocks");
+template <int NO>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_step(IntArgs<NO> ia, StepArgs sa,
+                                                                                      ETiles ft, int nint)
+{
+    extern __shared__ float sm[];
+    if ((int)blockIdx.x < nint)
+        IntKernel2<NO, 256>::run(ia, xcd_tile(blockIdx.x, nint), *reinterpret_cast<typename IntKernel2<NO, 256>::Smem *>(sm));
+    else
+        frame_body<NO>(sa, ft, (long)blockIdx.x - nint);
+}
+
 }  // namespace fdfk

@@ -299,6 +299,19 @@ __device__ __forceinline__ void lds_barrier()
     asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
+// MI355X hands the workgroups of a launch to its 8 XCDs round-robin
+// (workgroup p runs on XCD p % 8), and each XCD has its own L2.  This maps
+// the hardware id to a tile id so that every XCD owns one contiguous range
+// of tile ids: tiles that read the same cache lines (the halo rows shared by
+// z-neighbours) then meet in one L2 instead of being fetched from HBM twice.
+// A bijection on [0, n) for any n.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_tile(int p, int n)
+{
+    const int x = p % kXcds, k = p / kXcds, q = n / kXcds, r = n % kXcds;
+    return x * q + (x < r ? x : r) + k;
+}
+
 // ---------------------------------------------------------------------
 // interior kernel: full-order fused stencil over the total-field region.
 // Lanes run along z (one row each, HZ halo rows on either side of the TZ
@@ -339,11 +352,6 @@ struct IntKernel2 {
     };
 
     static __device__ __forceinline__ int sl(int ph, int j) { return ((ph + j) % P + P) % P; }
-    // padding cell: read in the total-field view (u + u0 on the hybrid ring)
-    static __device__ __forceinline__ bool outside(const Geo &g, int r, int c)
-    {
-        return c < 1 || c > g.nx || r > g.nz;
-    }
 
     struct Regs {
         float U[2][P], FX[2][P], VX[2][P];
@@ -352,13 +360,23 @@ struct IntKernel2 {
         float pl[2][2];     // [parity][l,m] next lead materials (ix+NO)
         float p0[2][2];     // [parity][l,m] next column materials (ix)
         float po[2][3];     // [parity][bf, uxp, uzp] of column ix (used one step later)
+        float pr[2][2];     // [parity][comp] u0 of the next new column on the outer ring
     };
 
     struct Lane {
         int lane, r;
         unsigned ofs;       // clamped row byte offset inside a column
         bool outrow;
+        bool rb, rr;        // row on the bottom outer ring / not below it
     };
+    // Padding cells are read in the total-field view, u + u0.  u0 is nonzero
+    // only on the outer ring (nbd cells wide beside and below the physical
+    // region), so only those cells load it; c is step-uniform.
+    static __device__ __forceinline__ bool ring_cell(const Geo &g, const Lane &L, int c)
+    {
+        if (c >= 1 && c <= g.nx) return L.rb;
+        return c >= 1 - g.nbd && c <= g.nx + g.nbd && L.rr;
+    }
     // Buffer resources on every array, based at the block's first column xb
     // (built once per block); an element is rsrc + scalar column offset
     // (c - xb) * ld * 4 + the lane's 32-bit row offset.  Offsets stay far
@@ -392,9 +410,17 @@ struct IntKernel2 {
         const int lo = 1 - g.npml, hi = g.nx + g.npml;
         auto colc = [&](int c) { return c < lo ? lo : (c > hi ? hi : c); };
         const unsigned o = L.ofs;
-        // A. ring update, prefetch of the next step, x-flux, LDS staging
-        R.U[0][sl(PH, 2 * NO - 1)] = R.pu[PA][0];
-        R.U[1][sl(PH, 2 * NO - 1)] = R.pu[PA][1];
+        // A. ring update, prefetch of the next step, x-flux, LDS staging.
+        // The u0 of the new column is added here, one step after its load,
+        // so that its latency stays hidden like every other load's.
+        if (bc) {
+            const bool rg = ring_cell(g, L, colc(ix + 2 * NO - 1));
+            R.U[0][sl(PH, 2 * NO - 1)] = rg ? R.pu[PA][0] + R.pr[PA][0] : R.pu[PA][0];
+            R.U[1][sl(PH, 2 * NO - 1)] = rg ? R.pu[PA][1] + R.pr[PA][1] : R.pu[PA][1];
+        } else {
+            R.U[0][sl(PH, 2 * NO - 1)] = R.pu[PA][0];
+            R.U[1][sl(PH, 2 * NO - 1)] = R.pu[PA][1];
+        }
         const float l1 = R.pl[PA][0], m1 = R.pl[PA][1];
         const float l0 = R.p0[PA][0], m0 = R.p0[PA][1];
         {
@@ -402,9 +428,9 @@ struct IntKernel2 {
             const int sn = B.soff(cn), sm = B.soff(cm), s0 = B.soff(c0), sx = B.soff(cx);
             R.pu[PB][0] = ld(B.ux, o, sn);
             R.pu[PB][1] = ld(B.uz, o, sn);
-            if (bc && outside(g, L.r, cn)) {
-                R.pu[PB][0] += ld(B.rx, o, sn);
-                R.pu[PB][1] += ld(B.rz, o, sn);
+            if (bc && ring_cell(g, L, cn)) {
+                R.pr[PB][0] = ld(B.rx, o, sn);
+                R.pr[PB][1] = ld(B.rz, o, sn);
             }
             R.pl[PB][0] = ld(B.lam, o, sm); R.pl[PB][1] = ld(B.mu, o, sm);
             R.p0[PB][0] = ld(B.lam, o, s0); R.p0[PB][1] = ld(B.mu, o, s0);
@@ -493,6 +519,10 @@ struct IntKernel2 {
     static __device__ void run(const IntArgs<NO> &a, int bid)
     {
         __shared__ Smem S;
+        run(a, bid, S);
+    }
+    static __device__ void run(const IntArgs<NO> &a, int bid, Smem &S)
+    {
         const Geo &g = a.g;
         Lane L;
         L.lane = threadIdx.x;
@@ -503,8 +533,8 @@ struct IntKernel2 {
                         S.at(q, UA, c, i) = 0.0f; S.at(q, UB, c, i) = 0.0f; S.at(q, RM, c, i) = 1.0f;
                         S.at(q, FZ, c, i) = 0.0f; S.at(q, VZ, c, i) = 0.0f;
                     }
-        // block -> tile: consecutive blocks walk down z first so that the x
-        // neighbours sharing warm-up columns run close in time
+        // tile id -> tile: consecutive ids walk down z first, so the z
+        // neighbours of one column of tiles share an XCD (xcd_tile)
         const int nbz = a.nbz;
         const int bzid = bid % nbz, bxid = bid / nbz;
         const int z0 = g.iz0 + bzid * TZ;
@@ -512,6 +542,8 @@ struct IntKernel2 {
         const int rc = L.r < 1 ? 1 : (L.r > g.NZ ? g.NZ : L.r);
         L.ofs = (unsigned)(rc - 1) * 4u;
         L.outrow = (L.lane >= HZ) && (L.lane < HZ + TZ) && (L.r <= g.iz1);
+        L.rr = L.r <= g.nz + g.nbd;
+        L.rb = L.r > g.nz && L.rr;
         const int xs = g.ix0 + bxid * a.lx;
         const int xe = min(xs + a.lx, g.ix1 + 1);
         // blocks whose loads reach the padding read the ring (block-uniform)
@@ -522,6 +554,8 @@ struct IntKernel2 {
 #pragma unroll
             for (int i = 0; i < P; ++i) { R.U[q][i] = 0.0f; R.FX[q][i] = 0.0f; R.VX[q][i] = 0.0f; }
         R.rMa_prev = 1.0f; R.rMm_prev = 1.0f;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) { R.pr[q][0] = 0.0f; R.pr[q][1] = 0.0f; }
         const int x0 = xs - WARM;
         Rsrc B;
         {
@@ -541,9 +575,9 @@ struct IntKernel2 {
             const int cm = x0 + NO < lo ? lo : x0 + NO, c0 = x0 < lo ? lo : x0;
             const int sn = B.soff(cn), sm = B.soff(cm), s0 = B.soff(c0);
             R.pu[0][0] = ld(B.ux, o, sn); R.pu[0][1] = ld(B.uz, o, sn);
-            if (bc && outside(g, L.r, cn)) {
-                R.pu[0][0] += ld(B.rx, o, sn);
-                R.pu[0][1] += ld(B.rz, o, sn);
+            if (bc && ring_cell(g, L, cn)) {
+                R.pr[0][0] = ld(B.rx, o, sn);
+                R.pr[0][1] = ld(B.rz, o, sn);
             }
             R.pl[0][0] = ld(B.lam, o, sm); R.pl[0][1] = ld(B.mu, o, sm);
             R.p0[0][0] = ld(B.lam, o, s0); R.p0[0][1] = ld(B.mu, o, s0);
@@ -553,7 +587,16 @@ struct IntKernel2 {
             for (int q = 0; q < 3; ++q) { R.po[0][q] = 0.0f; R.po[1][q] = 0.0f; }
         }
         __syncthreads();
-        for (int base = x0; base <= xe; base += P) {
+        // whole ring periods without per-step guards (no merges, so the
+        // ring registers need no copies), then the guarded remainder
+        int base = x0;
+        for (; base + P - 1 <= xe; base += P) {
+#define IK2_STEP(PH) if constexpr (P > PH) step<PH>(a, B, S, R, L, bc, base + PH, xs, xe);
+            IK2_STEP(0) IK2_STEP(1) IK2_STEP(2) IK2_STEP(3) IK2_STEP(4) IK2_STEP(5) IK2_STEP(6)
+            IK2_STEP(7) IK2_STEP(8) IK2_STEP(9) IK2_STEP(10) IK2_STEP(11) IK2_STEP(12) IK2_STEP(13)
+#undef IK2_STEP
+        }
+        if (base <= xe) {
 #define IK2_STEP(PH) if constexpr (P > PH) { if (base + PH <= xe) step<PH>(a, B, S, R, L, bc, base + PH, xs, xe); }
             IK2_STEP(0) IK2_STEP(1) IK2_STEP(2) IK2_STEP(3) IK2_STEP(4) IK2_STEP(5) IK2_STEP(6)
             IK2_STEP(7) IK2_STEP(8) IK2_STEP(9) IK2_STEP(10) IK2_STEP(11) IK2_STEP(12) IK2_STEP(13)
@@ -565,7 +608,7 @@ struct IntKernel2 {
 template <int NO, int NT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_interior2(IntArgs<NO> a)
 {
-    IntKernel2<NO, NT>::run(a, blockIdx.x);
+    IntKernel2<NO, NT>::run(a, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 // ---------------------------------------------------------------------

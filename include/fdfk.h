@@ -88,8 +88,10 @@ int fdfk_get_state(fdfk_solver *s, float *ux, float *uz);
 int fdfk_set_state(fdfk_solver *s, const float *ux, const float *uz);
 /* wait for all queued work of this handle */
 int fdfk_sync(fdfk_solver *s);
-/* kernel timing of the interior stencil kernel (HIP events on the handle's
- * stream); enable before fdfk_run, read after fdfk_sync */
+/* kernel timing of the stencil launch of each step (HIP events on the
+ * handle's stream): k_step, the interior and the frame in one launch (or
+ * k_interior2 alone when the environment sets FDFK_FUSED=0); enable before
+ * fdfk_run, read after fdfk_sync */
 int fdfk_profile(fdfk_solver *s, int32_t enable);
 int fdfk_kernel_stats(fdfk_solver *s, double *interior_ms_total, int64_t *interior_launches,
                       double *step_ms_total, int64_t *steps);
