@@ -82,6 +82,9 @@ __device__ __forceinline__ void frame_body(const StepArgs &a, const ETiles &et, 
         c1s[threadIdx.x] = cf->c1f[threadIdx.x];
     }
     const FDiv dSZ(SZ), dtz(tz), dNFZ(NFZ), dNDX(NDX);
+    // 1/dx and 1/dz are folded into the collocated derivatives and the
+    // memory corrections (products instead of divisions)
+    const float rdx = 1.0f / g.dx, rdz = 1.0f / g.dz;
     auto S = [&](int z, int x) { return (x - lx0) * SZ + (z - lz0); };
     const int nxp = g.nx + g.npml, NZ = g.NZ;
     // Global loads whose addresses do not depend on LDS are issued first so
@@ -225,7 +228,7 @@ __device__ __forceinline__ void frame_body(const StepArgs &a, const ETiles &et, 
             sx = accf(sx, c, sux[p] - sux[m]);
             sz = accf(sz, c, suz[p] - suz[m]);
         }
-        dzx[e] = sx; dzz[e] = sz;
+        dzx[e] = sx * rdz; dzz[e] = sz * rdz;
     }
     // phase 1d: collocated Dx (cdx_sum) at the tile columns, rows dx0..
     for (int e = threadIdx.x; e < tx * NDX; e += kFrameThreads) {
@@ -240,7 +243,7 @@ __device__ __forceinline__ void frame_body(const StepArgs &a, const ETiles &et, 
             sx = accf(sx, c, sux[p] - sux[m]);
             sz = accf(sz, c, suz[p] - suz[m]);
         }
-        dxx[e] = sx; dxz[e] = sz;
+        dxx[e] = sx * rdx; dxz[e] = sz * rdx;
     }
     // phase 1e: M psi of this step (k_prep) around the tile, as the memory
     // corrections read them: zero outside the strips, psi_z only in the
@@ -305,10 +308,10 @@ __device__ __forceinline__ void frame_body(const StepArgs &a, const ETiles &et, 
         Zz = accf(Zz, c0z, FZ(fz1z, iz + k - 1) - FZ(fz2z, iz - k));
         const int xp = S(iz, ix + k), xm = S(iz, ix - k), zp = S(iz + k, ix), zm = S(iz - k, ix);
         // Dx(l Dz uz), Dx(m Dz ux), Dz(m Dx uz), Dz(l Dx ux)
-        zxl = accf(zxl, c1x, slam[xp] * DZ(dzz, ix + k) / g.dz - slam[xm] * DZ(dzz, ix - k) / g.dz);
-        zxm = accf(zxm, c1x, smu[xp] * DZ(dzx, ix + k) / g.dz - smu[xm] * DZ(dzx, ix - k) / g.dz);
-        xzm = accf(xzm, c1z, smu[zp] * DX(dxz, iz + k) / g.dx - smu[zm] * DX(dxz, iz - k) / g.dx);
-        xzl = accf(xzl, c1z, slam[zp] * DX(dxx, iz + k) / g.dx - slam[zm] * DX(dxx, iz - k) / g.dx);
+        zxl = accf(zxl, c1x, slam[xp] * DZ(dzz, ix + k) - slam[xm] * DZ(dzz, ix - k));
+        zxm = accf(zxm, c1x, smu[xp] * DZ(dzx, ix + k) - smu[xm] * DZ(dzx, ix - k));
+        xzm = accf(xzm, c1z, smu[zp] * DX(dxz, iz + k) - smu[zm] * DX(dxz, iz - k));
+        xzl = accf(xzl, c1z, slam[zp] * DX(dxx, iz + k) - slam[zm] * DX(dxx, iz - k));
     }
     float p[6];
     p[0] = Xx * dx2;
@@ -331,15 +334,15 @@ __device__ __forceinline__ void frame_body(const StepArgs &a, const ETiles &et, 
             sb = accf(sb, c, mp * (sux[p2] - sux[p1]) - mm * (sux[m2] - sux[m1]));
         }
         const float lbar = 0.5f * (slam[r1] + slam[r2]), mbar = 0.5f * (smu[r1] + smu[r2]);
-        const float dxuz = (DX(dxz, 1) + DX(dxz, 2)) / g.dx;
-        const float dxux = (DX(dxx, 1) + DX(dxx, 2)) / g.dx;
-        p[2] = (sa / g.dx + mbar * dxuz) / g.dz;
-        p[5] = (sb / g.dx + lbar * dxux) / g.dz;
+        const float dxuz = DX(dxz, 1) + DX(dxz, 2);
+        const float dxux = DX(dxx, 1) + DX(dxx, 2);
+        p[2] = (sa * rdx + mbar * dxuz) * rdz;
+        p[5] = (sb * rdx + lbar * dxux) * rdz;
     } else {
         p[1] = Zx * (1.0f / (g.dz * g.dz));
         p[4] = Zz * (1.0f / (g.dz * g.dz));
-        p[2] = zxl / g.dx + xzm / g.dz;
-        p[5] = zxm / g.dx + xzl / g.dz;
+        p[2] = zxl * rdx + xzm * rdz;
+        p[5] = zxm * rdx + xzl * rdz;
     }
     // memory corrections -Dx(M psi_x), -Dz(M psi_z) (the bottom strip only)
     if (sb != S_NONE) {
@@ -350,8 +353,8 @@ __device__ __forceinline__ void frame_body(const StepArgs &a, const ETiles &et, 
             sx0 = accf(sx0, c, DZ(mpx0, ix + k) - DZ(mpx0, ix - k));
             sx1 = accf(sx1, c, DZ(mpx1, ix + k) - DZ(mpx1, ix - k));
         }
-        p[0] = p[0] - sx0 / g.dx;
-        p[3] = p[3] - sx1 / g.dx;
+        p[0] = p[0] - sx0 * rdx;
+        p[3] = p[3] - sx1 * rdx;
         if (sb == S_B) {
 #pragma unroll
             for (int k = 1; k <= NO; ++k) {
@@ -359,8 +362,8 @@ __device__ __forceinline__ void frame_body(const StepArgs &a, const ETiles &et, 
                 sz0 = accf(sz0, c, DX(mpz0, iz + k) - DX(mpz0, iz - k));
                 sz1 = accf(sz1, c, DX(mpz1, iz + k) - DX(mpz1, iz - k));
             }
-            p[1] = p[1] - sz0 / g.dz;
-            p[4] = p[4] - sz1 / g.dz;
+            p[1] = p[1] - sz0 * rdz;
+            p[4] = p[4] - sz1 * rdz;
         }
     }
     // update: plain leapfrog in the total-field region, split parts in a strip
@@ -372,12 +375,13 @@ __device__ __forceinline__ void frame_body(const StepArgs &a, const ETiles &et, 
     }
     const float ax = uax, az = uaz;
     const float c3 = 2.0f - 4.0f * ax * az;
+    const float r1 = 1.0f / (1.0f + ax), r2 = 1.0f / (1.0f + az), r3 = 1.0f / (1.0f + ax + az);
 #pragma unroll
     for (int comp = 0; comp < 2; ++comp) {
         const float X = p[3 * comp], Z = p[3 * comp + 1], Mm = p[3 * comp + 2];
-        const float n1 = (2.0f * ucur[comp][0] - (1.0f - ax) * uprv[comp][0] + ubf * X) / (1.0f + ax);
-        const float n2 = (2.0f * ucur[comp][1] - (1.0f - az) * uprv[comp][1] + ubf * Z) / (1.0f + az);
-        const float n3 = (c3 * ucur[comp][2] - (1.0f - ax - az) * uprv[comp][2] + ubf * Mm) / (1.0f + ax + az);
+        const float n1 = (2.0f * ucur[comp][0] - (1.0f - ax) * uprv[comp][0] + ubf * X) * r1;
+        const float n2 = (2.0f * ucur[comp][1] - (1.0f - az) * uprv[comp][1] + ubf * Z) * r2;
+        const float n3 = (c3 * ucur[comp][2] - (1.0f - ax - az) * uprv[comp][2] + ubf * Mm) * r3;
         st->prev[sb][comp][0][uo] = n1; st->prev[sb][comp][1][uo] = n2; st->prev[sb][comp][2][uo] = n3;
         const float tot = (n1 + n2) + n3;
         if (comp == 0) a.uxp[uc] = tot; else a.uzp[uc] = tot;

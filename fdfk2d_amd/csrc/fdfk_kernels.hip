@@ -409,12 +409,20 @@ struct IntKernel2 {
         const int lane = L.lane;
         const int lo = 1 - g.npml, hi = g.nx + g.npml;
         auto colc = [&](int c) { return c < lo ? lo : (c > hi ? hi : c); };
+        // Columns whose loads matter: u within the x-reach 2NO-1 of the
+        // tile, lead materials within NO (face means, Dx of l Dz uz), column
+        // materials and level n-1 on the tile only.  Loads outside are
+        // redirected to the nearest such column (a cache hit) instead of
+        // fetching data that only feeds unstored updates.
+        auto cu = [&](int c) { return colc(c < xs - (2 * NO - 1) ? xs - (2 * NO - 1) : (c > xe + 2 * NO - 2 ? xe + 2 * NO - 2 : c)); };
+        auto cl = [&](int c) { return colc(c < xs - NO ? xs - NO : (c > xe + NO - 1 ? xe + NO - 1 : c)); };
+        auto ct = [&](int c) { return c < xs ? xs : (c > xe - 1 ? xe - 1 : c); };
         const unsigned o = L.ofs;
         // A. ring update, prefetch of the next step, x-flux, LDS staging.
         // The u0 of the new column is added here, one step after its load,
         // so that its latency stays hidden like every other load's.
         if (bc) {
-            const bool rg = ring_cell(g, L, colc(ix + 2 * NO - 1));
+            const bool rg = ring_cell(g, L, cu(ix + 2 * NO - 1));
             R.U[0][sl(PH, 2 * NO - 1)] = rg ? R.pu[PA][0] + R.pr[PA][0] : R.pu[PA][0];
             R.U[1][sl(PH, 2 * NO - 1)] = rg ? R.pu[PA][1] + R.pr[PA][1] : R.pu[PA][1];
         } else {
@@ -424,7 +432,7 @@ struct IntKernel2 {
         const float l1 = R.pl[PA][0], m1 = R.pl[PA][1];
         const float l0 = R.p0[PA][0], m0 = R.p0[PA][1];
         {
-            const int cn = colc(ix + 2 * NO), cm = colc(ix + 1 + NO), c0 = colc(ix + 1), cx = colc(ix);
+            const int cn = cu(ix + 2 * NO), cm = cl(ix + 1 + NO), c0 = ct(ix + 1), cx = ct(ix);
             const int sn = B.soff(cn), sm = B.soff(cm), s0 = B.soff(c0), sx = B.soff(cx);
             R.pu[PB][0] = ld(B.ux, o, sn);
             R.pu[PB][1] = ld(B.uz, o, sn);
@@ -571,8 +579,8 @@ struct IntKernel2 {
         {
             const int lo = 1 - g.npml;
             const unsigned o = L.ofs;
-            const int cn = x0 + 2 * NO - 1;
-            const int cm = x0 + NO < lo ? lo : x0 + NO, c0 = x0 < lo ? lo : x0;
+            const int cn = x0 + 2 * NO - 1;   // = xs - (2NO-1), the first u column the tile reaches
+            const int cm = xs - NO > lo ? xs - NO : lo, c0 = xs;   // step x0 feeds no stored update
             const int sn = B.soff(cn), sm = B.soff(cm), s0 = B.soff(c0);
             R.pu[0][0] = ld(B.ux, o, sn); R.pu[0][1] = ld(B.uz, o, sn);
             if (bc && ring_cell(g, L, cn)) {
