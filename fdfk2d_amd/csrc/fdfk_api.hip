@@ -223,6 +223,35 @@ ICoef<NO> make_icoef(const fdfk_solver *s)
     return k;
 }
 
+// Interior tiling for a column width lx: nbx columns of tiles, nbz bands of
+// full-height tiles (TZ rows); a last band of at most 64 - 2HZ rows goes to
+// one-wave segments, NT/64 of them per block (nsb blocks from row zseg on).
+template <int NO, int NT>
+void int_tiling(const Geo &g, int lx, int &nbx, int &nbz, int &nsb, int &zseg)
+{
+    constexpr int TZ = IntKernel2<NO, NT>::TZ, HZ = IntKernel2<NO, NT>::HZ, W = NT / 64;
+    const int ncols = g.ix1 - g.ix0 + 1, nrows = g.iz1 - g.iz0 + 1;
+    nbx = (ncols + lx - 1) / lx;
+    const int full = nrows / TZ, rem = nrows - full * TZ;
+    if (W > 1 && full > 0 && rem > 0 && rem <= 64 - 2 * HZ) {
+        nbz = full;
+        zseg = g.iz0 + full * TZ;
+        nsb = (nbx + W - 1) / W;
+    } else {
+        nbz = (nrows + TZ - 1) / TZ;
+        zseg = 0;
+        nsb = 0;
+    }
+}
+
+template <int NO, int NT>
+long int_blocks(const Geo &g, int lx)
+{
+    int nbx, nbz, nsb, zseg;
+    int_tiling<NO, NT>(g, lx, nbx, nbz, nsb, zseg);
+    return (long)nbx * nbz + nsb;
+}
+
 template <int NO, int NT>
 IntArgs<NO> interior_args(fdfk_solver *s)
 {
@@ -235,11 +264,8 @@ IntArgs<NO> interior_args(fdfk_solver *s)
     a.bf = s->bf; a.lam = s->lam; a.mu = s->mu;
     a.ringx = s->ringx; a.ringz = s->ringz;
     a.corr = s->bg_valid && s->it <= s->nt;
-    constexpr int TZ = IntKernel2<NO, NT>::TZ;
-    int ncols = g.ix1 - g.ix0 + 1, nrows = g.iz1 - g.iz0 + 1;
     a.lx = s->lx;
-    a.nbx = (ncols + a.lx - 1) / a.lx;
-    a.nbz = (nrows + TZ - 1) / TZ;
+    int_tiling<NO, NT>(g, a.lx, a.nbx, a.nbz, a.nsb, a.zseg);
     return a;
 }
 
@@ -247,7 +273,7 @@ template <int NO, int NT>
 void launch_interior_t(fdfk_solver *s)
 {
     IntArgs<NO> a = interior_args<NO, NT>(s);
-    hipLaunchKernelGGL((k_interior2<NO, NT>), dim3(a.nbx * a.nbz), dim3(NT), 0, s->stream, a);
+    hipLaunchKernelGGL((k_interior2<NO, NT>), dim3(a.nbx * a.nbz + a.nsb), dim3(NT), 0, s->stream, a);
 }
 
 // Interior block shape.  The stencil blocks all take about the same time,
@@ -285,17 +311,12 @@ IntPlan plan_lx(const Geo &g, bool fused)
         hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
         oe != hipSuccess || ncu <= 0 || bpc <= 0)
         return best;
-    constexpr int TZ = IntKernel2<NO, NT>::TZ;
     constexpr int WARM = IntKernel2<NO, NT>::WARM;
     const long slots = (long)ncu * bpc;
-    const int ncols = g.ix1 - g.ix0 + 1, nrows = g.iz1 - g.iz0 + 1;
-    const long nbz = (nrows + TZ - 1) / TZ;
-    for (int R = 1; R <= 16; ++R) {
-        long nbx = std::max(1L, R * slots / nbz);
-        int lx = (int)std::max(4L, (ncols + nbx - 1) / nbx);
-        long nb = ((ncols + lx - 1) / lx) * nbz;
-        long rounds = (nb + slots - 1) / slots;
-        long cost = rounds * (lx + WARM);
+    const int ncols = g.ix1 - g.ix0 + 1;
+    for (int lx = 4; lx <= std::max(4, ncols); ++lx) {
+        const long rounds = (int_blocks<NO, NT>(g, lx) + slots - 1) / slots;
+        const long cost = rounds * (lx + WARM);
         if (best.cost < 0 || cost < best.cost) { best.cost = cost; best.lx = lx; }
     }
     return best;
@@ -401,7 +422,7 @@ void launch_step_t(fdfk_solver *s, const StepArgs &a)
 {
     IntArgs<NO> ia = interior_args<NO, 256>(s);
     ETiles t = frame_tiles(s->g);
-    const int nint = ia.nbx * ia.nbz;
+    const int nint = ia.nbx * ia.nbz + ia.nsb;
     hipLaunchKernelGGL(k_step<NO>, dim3((unsigned)(nint + t.off[t.n])), dim3(256), step_lds_bytes<NO>(s->g),
                        s->stream, ia, a, t, nint);
 }

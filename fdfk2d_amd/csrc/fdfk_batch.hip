@@ -82,7 +82,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         IntKernel2<NO, 256>::run(J[j].ia, t - P->pre[KB_INT][j],
                                  *reinterpret_cast<typename IntKernel2<NO, 256>::Smem *>(sm));
     } else {
-        const int b = (int)blockIdx.x - ni;
+        const int b = xcd_tile((int)blockIdx.x - ni, (int)gridDim.x - ni);
         const int j = find_job(P->pre[KB_FRAME], n, b);
         frame_body<NO>(J[j].sa, J[j].ft, b - P->pre[KB_FRAME][j]);
     }
@@ -134,7 +134,6 @@ template <int NO>
 int batch_lx(const std::vector<fdfk_solver *> &act)
 {
     constexpr int NT = 256;
-    constexpr int TZ = IntKernel2<NO, NT>::TZ;
     constexpr int WARM = IntKernel2<NO, NT>::WARM;
     static long slots = 0;
     if (slots == 0) {
@@ -153,11 +152,7 @@ int batch_lx(const std::vector<fdfk_solver *> &act)
     long best_cost = -1;
     for (int lx = 8; lx <= 256; lx += 8) {
         long nb = 0;
-        for (auto *s : act) {
-            const Geo &g = s->g;
-            const long nbz = (g.iz1 - g.iz0 + TZ) / TZ;
-            nb += nbz * ((g.ix1 - g.ix0 + lx) / lx);
-        }
+        for (auto *s : act) nb += int_blocks<NO, NT>(s->g, lx);
         const long cost = ((nb + slots - 1) / slots) * (lx + WARM);
         if (best_cost < 0 || cost < best_cost) { best_cost = cost; best = lx; }
     }
@@ -174,7 +169,7 @@ void fill_job(fdfk_solver *s, BatchJob<NO> &b, int nb[KB_N], bool &record, int l
     b.ringz = s->ringz;
     b.ia = interior_args<NO, 256>(s);
     b.ia.lx = lx;
-    b.ia.nbx = (g.ix1 - g.ix0 + lx) / lx;
+    int_tiling<NO, 256>(g, lx, b.ia.nbx, b.ia.nbz, b.ia.nsb, b.ia.zseg);
     b.ft = frame_tiles(g);
     record = (s->isis == s->it) && (s->istp <= s->nsis) && s->nrcv > 0;
     RecArgs &r = b.ra;
@@ -186,7 +181,7 @@ void fill_job(fdfk_solver *s, BatchJob<NO> &b, int nb[KB_N], bool &record, int l
     long npsi = 2L * g.npml * g.nz + (long)g.NX * g.npml;
     nb[KB_RING] = b.sa.ring_valid ? (int)((nring + 255) / 256) : 0;
     nb[KB_PSI] = (int)((npsi + 255) / 256);
-    nb[KB_INT] = b.ia.nbx * b.ia.nbz;
+    nb[KB_INT] = b.ia.nbx * b.ia.nbz + b.ia.nsb;
     nb[KB_FRAME] = b.ft.off[b.ft.n];
     nb[KB_REC] = record ? (s->nrcv + kRecPerBlock - 1) / kRecPerBlock : 0;
 }

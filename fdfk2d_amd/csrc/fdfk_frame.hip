@@ -391,7 +391,7 @@ __device__ __forceinline__ void frame_body(const StepArgs &a, const ETiles &et, 
 template <int NO>
 __global__ __launch_bounds__(kFrameThreads) void k_frame(StepArgs a, ETiles t)
 {
-    frame_body<NO>(a, t, blockIdx.x);
+    frame_body<NO>(a, t, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 // The interior and the frame of one step in one launch: blocks
@@ -409,8 +409,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     extern __shared__ float sm[];
     if ((int)blockIdx.x < nint)
         IntKernel2<NO, 256>::run(ia, xcd_tile(blockIdx.x, nint), *reinterpret_cast<typename IntKernel2<NO, 256>::Smem *>(sm));
-    else
-        frame_body<NO>(sa, ft, (long)blockIdx.x - nint);
+    else   // frame tiles grouped per XCD too: neighbouring tiles share halo lines in one L2
+        frame_body<NO>(sa, ft, xcd_tile((int)blockIdx.x - nint, (int)gridDim.x - nint));
 }
 
 }  // namespace fdfk

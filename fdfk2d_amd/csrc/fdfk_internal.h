@@ -139,7 +139,9 @@ struct IntArgs {
     int corr;                    // ring values valid at this step
     int lx;   // output columns per block
     int nbx;  // blocks along x
-    int nbz;  // blocks along z
+    int nbz;  // full-height blocks along z
+    int nsb;  // blocks of one-wave segments over the last, partial band of rows (0: none)
+    int zseg; // first row of that band
 };
 
 }  // namespace fdfk

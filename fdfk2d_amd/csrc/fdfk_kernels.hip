@@ -541,21 +541,41 @@ struct IntKernel2 {
                         S.at(q, UA, c, i) = 0.0f; S.at(q, UB, c, i) = 0.0f; S.at(q, RM, c, i) = 1.0f;
                         S.at(q, FZ, c, i) = 0.0f; S.at(q, VZ, c, i) = 0.0f;
                     }
-        // tile id -> tile: consecutive ids walk down z first, so the z
-        // neighbours of one column of tiles share an XCD (xcd_tile)
-        const int nbz = a.nbz;
-        const int bzid = bid % nbz, bxid = bid / nbz;
-        const int z0 = g.iz0 + bzid * TZ;
-        L.r = z0 - HZ + L.lane;
+        // tile id -> tile.  Ids below nbz*nbx are full-height tiles;
+        // consecutive ids walk down z first, so the z neighbours of one
+        // column of tiles share an XCD (xcd_tile).  A last band of rows too
+        // short for a full tile (at most 64 - 2HZ rows) is cut into 64-lane
+        // segments instead, each wave of a block marching its own columns:
+        // a full-height tile there would spend every step on mostly idle
+        // lanes.  Lanes of a segment read their LDS neighbours across the
+        // segment's edge only from halo lanes, whose results are not stored.
+        const int nmain = a.nbz * a.nbx;
+        int z0, xs, xe, xl, li = L.lane, span = NT;
+        bool live = true;
+        if (bid < nmain) {
+            const int bzid = bid % a.nbz, bxid = bid / a.nbz;
+            z0 = g.iz0 + bzid * TZ;
+            xs = g.ix0 + bxid * a.lx;
+            xe = min(xs + a.lx, g.ix1 + 1);
+            xl = xe;
+        } else {
+            const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+            xs = g.ix0 + ((bid - nmain) * (NT / 64) + w) * a.lx;
+            if (xs > g.ix1) { live = false; xs = g.ix1; }   // past the last column: march, store nothing
+            xe = min(xs + a.lx, g.ix1 + 1);
+            xl = xs + a.lx;   // the same step count for every wave: one barrier per step
+            z0 = a.zseg;
+            li = L.lane & 63;
+            span = 64;
+        }
+        L.r = z0 - HZ + li;
         const int rc = L.r < 1 ? 1 : (L.r > g.NZ ? g.NZ : L.r);
         L.ofs = (unsigned)(rc - 1) * 4u;
-        L.outrow = (L.lane >= HZ) && (L.lane < HZ + TZ) && (L.r <= g.iz1);
+        L.outrow = live && (li >= HZ) && (li < span - HZ) && (L.r <= g.iz1);
         L.rr = L.r <= g.nz + g.nbd;
         L.rb = L.r > g.nz && L.rr;
-        const int xs = g.ix0 + bxid * a.lx;
-        const int xe = min(xs + a.lx, g.ix1 + 1);
-        // blocks whose loads reach the padding read the ring (block-uniform)
-        const bool bc = a.corr && (xs - WARM + 2 * NO - 1 < 1 || xe + 2 * NO - 1 > g.nx || z0 - HZ + NT - 1 > g.nz);
+        // tiles whose loads reach the padding read the ring (wave-uniform)
+        const bool bc = a.corr && (xs - WARM + 2 * NO - 1 < 1 || xe + 2 * NO - 1 > g.nx || z0 - HZ + span - 1 > g.nz);
         Regs R;
 #pragma unroll
         for (int q = 0; q < 2; ++q)
@@ -598,14 +618,14 @@ struct IntKernel2 {
         // whole ring periods without per-step guards (no merges, so the
         // ring registers need no copies), then the guarded remainder
         int base = x0;
-        for (; base + P - 1 <= xe; base += P) {
+        for (; base + P - 1 <= xl; base += P) {
 #define IK2_STEP(PH) if constexpr (P > PH) step<PH>(a, B, S, R, L, bc, base + PH, xs, xe);
             IK2_STEP(0) IK2_STEP(1) IK2_STEP(2) IK2_STEP(3) IK2_STEP(4) IK2_STEP(5) IK2_STEP(6)
             IK2_STEP(7) IK2_STEP(8) IK2_STEP(9) IK2_STEP(10) IK2_STEP(11) IK2_STEP(12) IK2_STEP(13)
 #undef IK2_STEP
         }
-        if (base <= xe) {
-#define IK2_STEP(PH) if constexpr (P > PH) { if (base + PH <= xe) step<PH>(a, B, S, R, L, bc, base + PH, xs, xe); }
+        if (base <= xl) {
+#define IK2_STEP(PH) if constexpr (P > PH) { if (base + PH <= xl) step<PH>(a, B, S, R, L, bc, base + PH, xs, xe); }
             IK2_STEP(0) IK2_STEP(1) IK2_STEP(2) IK2_STEP(3) IK2_STEP(4) IK2_STEP(5) IK2_STEP(6)
             IK2_STEP(7) IK2_STEP(8) IK2_STEP(9) IK2_STEP(10) IK2_STEP(11) IK2_STEP(12) IK2_STEP(13)
 #undef IK2_STEP

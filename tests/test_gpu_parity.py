@@ -6,6 +6,8 @@ fluxes in the interior, hardware reciprocals in the harmonic means), so
 single steps agree to ~1e-6 (<= 1e-5 max-relative per array) and
 seismograms to rel-L2 <= 1e-5.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -94,3 +96,50 @@ def test_compat_sequence_matches_handle(case, bg):
     ux, uz = s.get_state()
     s.close()
     assert np.array_equal(ux, st["ux"]) and np.array_equal(uz, st["uz"])
+
+
+def test_fused_step_equals_separate_launches(case, bg, monkeypatch):
+    """k_step runs the interior and frame tiles of a step in one launch; with
+    FDFK_FUSED=0 the same tile bodies run as k_interior2 then k_frame.  The
+    arithmetic is the same, so fields and seismograms agree bit for bit."""
+    print("case: 120 steps, fused launch vs separate launches", flush=True)
+    out = []
+    for fused in ("1", "0"):
+        monkeypatch.setenv("FDFK_FUSED", fused)
+        s = lib.Solver(case.nx, case.nz, case.npml, case.norder, case.dx, case.dz, case.dt, case.nt,
+                       case.c, rx=case.rx, rz=case.rz)
+        s.set_background(bg)
+        s.run(120)
+        out.append(s.get_state() + s.seismograms())
+        s.close()
+    assert np.abs(out[0][0]).max() > 0
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
+
+
+def test_partial_band_segments_match_oracle():
+    """nz = 262: the interior rows 6..262 are one band of 246-row tiles plus
+    an 11-row band, which the interior cuts into one-wave segments (each
+    wave of a block marching its own columns).  Seismograms (900 iterations:
+    the incident P reaches the surface from the 52 km deep ring) and final
+    fields against the oracle, fused and separate launches."""
+    print("case: 121 x 262, 900 iterations, partial-band segments", flush=True)
+    c = Case(nz=262, nt=900)
+    bg = c.oracle_background()
+    st = c.state()
+    ox, oz = oracle_run(c, st, bg, c.nt + 1)
+    for fused in ("1", "0"):
+        os.environ["FDFK_FUSED"] = fused
+        try:
+            s = lib.Solver(c.nx, c.nz, c.npml, c.norder, c.dx, c.dz, c.dt, c.nt, c.c, rx=c.rx, rz=c.rz)
+            s.set_background(bg)
+            s.run(c.nt + 1)
+            gx, gz = s.seismograms()
+            ux, uz = s.get_state()
+            s.close()
+        finally:
+            os.environ.pop("FDFK_FUSED", None)
+        assert np.abs(oz).max() > 1e-4
+        assert rel_l2(gx, ox) < 1e-5 and rel_l2(gz, oz) < 1e-5, fused
+        # both time levels, the reference's ux(1:2, ...) layout
+        assert rel_l2(ux, np.ravel(st["ux"])) < 1e-5 and rel_l2(uz, np.ravel(st["uz"])) < 1e-5, fused
