@@ -32,7 +32,7 @@ PROFILED_STEPS = 50
 # HBM bytes per k_step launch on the 8001x2000 workload, from rocprofv3
 # FETCH_SIZE / WRITE_SIZE scaled by a plain device copy
 # (profiles/k_step_8001x2000_hbm_traffic.md)
-MEASURED_TRAFFIC_BYTES = 862106825
+MEASURED_TRAFFIC_BYTES = 726348914
 
 
 def step_alg_bytes(nx, nz, npml, norder):
