@@ -70,7 +70,7 @@ __global__ __launch_bounds__(256) void kb_prep(const BatchJob<NO> *J, const Batc
 // the interior tiles of every member, then their frame tiles, in one launch
 // (the single-job k_step arrangement)
 template <int NO>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void kb_step(const BatchJob<NO> *J,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kIntWaves<NO>))) void kb_step(const BatchJob<NO> *J,
                                                                                        const BatchPrefix *P)
 {
     extern __shared__ float sm[];

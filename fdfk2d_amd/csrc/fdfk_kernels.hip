@@ -306,6 +306,12 @@ __device__ __forceinline__ void lds_barrier()
 // z-neighbours) then meet in one L2 instead of being fetched from HBM twice.
 // A bijection on [0, n) for any n.
 constexpr int kXcds = 8;
+
+// Occupancy target of the kernels that run the interior tiles: 3 waves per
+// SIMD (at most 168 VGPRs) up to FD order 6; the order-8 rings need more
+// registers than that and run at 2 waves per SIMD rather than spill.
+template <int NO>
+constexpr int kIntWaves = NO <= 3 ? 3 : 2;
 __device__ __forceinline__ int xcd_tile(int p, int n)
 {
     const int x = p % kXcds, k = p / kXcds, q = n / kXcds, r = n % kXcds;
@@ -524,12 +530,12 @@ struct IntKernel2 {
         }
     }
 
-    static __device__ void run(const IntArgs<NO> &a, int bid)
+    static __device__ __forceinline__ void run(const IntArgs<NO> &a, int bid)
     {
         __shared__ Smem S;
         run(a, bid, S);
     }
-    static __device__ void run(const IntArgs<NO> &a, int bid, Smem &S)
+    static __device__ __forceinline__ void run(const IntArgs<NO> &a, int bid, Smem &S)
     {
         const Geo &g = a.g;
         Lane L;
@@ -634,7 +640,7 @@ struct IntKernel2 {
 };
 
 template <int NO, int NT>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_interior2(IntArgs<NO> a)
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(kIntWaves<NO>))) void k_interior2(IntArgs<NO> a)
 {
     IntKernel2<NO, NT>::run(a, xcd_tile(blockIdx.x, gridDim.x));
 }

@@ -143,3 +143,32 @@ def test_partial_band_segments_match_oracle():
         assert rel_l2(gx, ox) < 1e-5 and rel_l2(gz, oz) < 1e-5, fused
         # both time levels, the reference's ux(1:2, ...) layout
         assert rel_l2(ux, np.ravel(st["ux"])) < 1e-5 and rel_l2(uz, np.ravel(st["uz"])) < 1e-5, fused
+
+
+@pytest.mark.parametrize("norder", [1, 2, 4])
+def test_fd_orders_match_oracle(norder):
+    """FD orders 2, 4 and 8 (the reference's norder 1, 2, 4) on a 121 x 262
+    grid: after 200 iterations the incident wave fills the lower part of
+    the model, across the last band of interior rows that runs as one-wave
+    segments.  Both time levels against the oracle; fused and separate
+    launches bit for bit."""
+    print("case: norder %d, 121 x 262, 200 iterations" % norder, flush=True)
+    c = Case(norder=norder, nz=262, nt=200)
+    bg = c.oracle_background()
+    st = c.state()
+    oracle_run(c, st, bg, c.nt + 1, record=False)
+    out = []
+    for fused in ("1", "0"):
+        os.environ["FDFK_FUSED"] = fused
+        try:
+            s = lib.Solver(c.nx, c.nz, c.npml, c.norder, c.dx, c.dz, c.dt, c.nt, c.c, rx=c.rx, rz=c.rz)
+            s.set_background(bg)
+            s.run(c.nt + 1)
+            out.append(s.get_state())
+            s.close()
+        finally:
+            os.environ.pop("FDFK_FUSED", None)
+    ux, uz = out[0]
+    assert np.abs(ux).max() > 1e-5
+    assert rel_l2(ux, np.ravel(st["ux"])) < 1e-5 and rel_l2(uz, np.ravel(st["uz"])) < 1e-5
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
