@@ -223,6 +223,24 @@ ICoef<NO> make_icoef(const fdfk_solver *s)
     return k;
 }
 
+// Calls f(std::integral_constant<int, NO>{}) for NO = norder, 1..kMaxOrder
+// (FD orders 2..16, the reference's range: src/Input.f90:119-131).
+template <class F>
+void with_order(int norder, F &&f)
+{
+    switch (norder) {
+    case 1: f(std::integral_constant<int, 1>{}); break;
+    case 2: f(std::integral_constant<int, 2>{}); break;
+    case 3: f(std::integral_constant<int, 3>{}); break;
+    case 4: f(std::integral_constant<int, 4>{}); break;
+    case 5: f(std::integral_constant<int, 5>{}); break;
+    case 6: f(std::integral_constant<int, 6>{}); break;
+    case 7: f(std::integral_constant<int, 7>{}); break;
+    default: f(std::integral_constant<int, 8>{}); break;
+    }
+}
+static_assert(kMaxOrder == 8, "with_order covers norder 1..8");
+
 // Interior tiling for a column width lx: nbx columns of tiles, nbz bands of
 // full-height tiles (TZ rows); a last band of at most 64 - 2HZ rows goes to
 // one-wave segments, NT/64 of them per block (nsb blocks from row zseg on).
@@ -337,16 +355,9 @@ void plan_interior(fdfk_solver *s)
     auto consider = [&](int n, IntPlan q) {
         if (q.cost >= 0 && (p.cost < 0 || q.cost < p.cost)) { p = q; nt = n; }
     };
-    switch (g.norder) {
-    case 1: consider(256, plan_lx<1, 256>(g, fused)); break;
-    case 2: consider(256, plan_lx<2, 256>(g, fused)); break;
-    case 3:
-        if (want == 192) consider(192, plan_lx<3, 192>(g, false));
-        else if (want == 128) consider(128, plan_lx<3, 128>(g, false));
-        else consider(256, plan_lx<3, 256>(g, fused));
-        break;
-    default: consider(256, plan_lx<4, 256>(g, fused)); break;
-    }
+    if (g.norder == 3 && want == 192) consider(192, plan_lx<3, 192>(g, false));
+    else if (g.norder == 3 && want == 128) consider(128, plan_lx<3, 128>(g, false));
+    else with_order(g.norder, [&](auto no) { consider(256, plan_lx<decltype(no)::value, 256>(g, fused)); });
     s->nt_threads = nt;
     s->fused = fused && nt == 256;
     s->lx = p.lx;
@@ -355,16 +366,9 @@ void plan_interior(fdfk_solver *s)
 
 void launch_interior(fdfk_solver *s)
 {
-    switch (s->g.norder) {
-    case 1: launch_interior_t<1, 256>(s); break;
-    case 2: launch_interior_t<2, 256>(s); break;
-    case 3:
-        if (s->nt_threads == 128) launch_interior_t<3, 128>(s);
-        else if (s->nt_threads == 192) launch_interior_t<3, 192>(s);
-        else launch_interior_t<3, 256>(s);
-        break;
-    default: launch_interior_t<4, 256>(s); break;
-    }
+    if (s->g.norder == 3 && s->nt_threads == 128) launch_interior_t<3, 128>(s);
+    else if (s->g.norder == 3 && s->nt_threads == 192) launch_interior_t<3, 192>(s);
+    else with_order(s->g.norder, [&](auto no) { launch_interior_t<decltype(no)::value, 256>(s); });
 }
 
 // one frame band cut into tiles of at most kFrameThreads cells whose staged
@@ -429,12 +433,7 @@ void launch_step_t(fdfk_solver *s, const StepArgs &a)
 
 void launch_step(fdfk_solver *s, const StepArgs &a)
 {
-    switch (s->g.norder) {
-    case 1: launch_step_t<1>(s, a); break;
-    case 2: launch_step_t<2>(s, a); break;
-    case 3: launch_step_t<3>(s, a); break;
-    default: launch_step_t<4>(s, a); break;
-    }
+    with_order(s->g.norder, [&](auto no) { launch_step_t<decltype(no)::value>(s, a); });
 }
 
 void launch_frame(fdfk_solver *s, const StepArgs &a)
@@ -443,12 +442,9 @@ void launch_frame(fdfk_solver *s, const StepArgs &a)
     if (t.off[t.n] == 0) return;
     dim3 fg((unsigned)t.off[t.n]), fb(kFrameThreads);
     const size_t sh = frame_lds_bytes(t, s->g.norder);
-    switch (s->g.norder) {
-    case 1: hipLaunchKernelGGL(k_frame<1>, fg, fb, sh, s->stream, a, t); break;
-    case 2: hipLaunchKernelGGL(k_frame<2>, fg, fb, sh, s->stream, a, t); break;
-    case 3: hipLaunchKernelGGL(k_frame<3>, fg, fb, sh, s->stream, a, t); break;
-    default: hipLaunchKernelGGL(k_frame<4>, fg, fb, sh, s->stream, a, t); break;
-    }
+    with_order(s->g.norder, [&](auto no) {
+        hipLaunchKernelGGL(k_frame<decltype(no)::value>, fg, fb, sh, s->stream, a, t);
+    });
 }
 
 int one_step(fdfk_solver *s, bool record)
@@ -472,12 +468,9 @@ int one_step(fdfk_solver *s, bool record)
         const int npsi = (int)((2L * g.npml * g.nz + (long)g.NX * g.npml + 255) / 256);
         const int nrec = rec ? (s->nrcv + kRecPerBlock - 1) / kRecPerBlock : 0;
         dim3 pg((unsigned)(nring + npsi + nrec)), pb(256);
-        switch (g.norder) {
-        case 1: hipLaunchKernelGGL(k_prep<1>, pg, pb, 0, s->stream, a, r, s->d_bg, nring, npsi); break;
-        case 2: hipLaunchKernelGGL(k_prep<2>, pg, pb, 0, s->stream, a, r, s->d_bg, nring, npsi); break;
-        case 3: hipLaunchKernelGGL(k_prep<3>, pg, pb, 0, s->stream, a, r, s->d_bg, nring, npsi); break;
-        default: hipLaunchKernelGGL(k_prep<4>, pg, pb, 0, s->stream, a, r, s->d_bg, nring, npsi); break;
-        }
+        with_order(g.norder, [&](auto no) {
+            hipLaunchKernelGGL(k_prep<decltype(no)::value>, pg, pb, 0, s->stream, a, r, s->d_bg, nring, npsi);
+        });
     }
     if (s->prof) hipEventRecord(e1, s->stream);
     if (s->fused) {
@@ -504,7 +497,7 @@ int validate(const fdfk_job *j)
 {
     if (!j) return fail(FDFK_EINVAL, "null job");
     if (j->is_pml_top) return fail(FDFK_ENOSYS, "is_PML_top = .true. is not supported (free surface only)");
-    if (j->norder < 1 || j->norder > 4) return fail(FDFK_EINVAL, "norder must be 1..4 (FD order 2..8)");
+    if (j->norder < 1 || j->norder > kMaxOrder) return fail(FDFK_EINVAL, "norder must be 1..8 (FD order 2..16)");
     int nbd = 2 * j->norder - 1;
     if (j->npml < nbd) return fail(FDFK_EINVAL, "npml must be >= 2*norder-1");
     if (j->nx < 2 * nbd + 2) return fail(FDFK_EINVAL, "nx too small for the hybrid ring");

@@ -253,15 +253,10 @@ int fdfk_batch_create(int32_t norder, fdfk_batch **out)
 {
     if (!out) return fail(FDFK_EINVAL, "null output pointer");
     *out = nullptr;
-    if (norder < 1 || norder > 4) return fail(FDFK_EINVAL, "norder must be 1..4");
+    if (norder < 1 || norder > kMaxOrder) return fail(FDFK_EINVAL, "norder must be 1..8");
     auto *B = new fdfk_batch();
     B->norder = norder;
-    switch (norder) {
-    case 1: B->tabbytes = batch_table_bytes<1>(); break;
-    case 2: B->tabbytes = batch_table_bytes<2>(); break;
-    case 3: B->tabbytes = batch_table_bytes<3>(); break;
-    default: B->tabbytes = batch_table_bytes<4>(); break;
-    }
+    with_order(norder, [&](auto no) { B->tabbytes = batch_table_bytes<decltype(no)::value>(); });
     if (hipStreamCreate(&B->stream) != hipSuccess) { delete B; return fail(FDFK_EDEVICE, "hipStreamCreate"); }
     for (int i = 0; i < fdfk_batch::R; ++i) {
         if (hipMalloc(&B->dtab[i], B->tabbytes) != hipSuccess ||
@@ -305,12 +300,7 @@ int fdfk_batch_run(fdfk_batch *B, int32_t nsteps)
     if (!B) return fail(FDFK_EINVAL, "null batch");
     for (int k = 0; k < nsteps; ++k) {
         int rc;
-        switch (B->norder) {
-        case 1: rc = batch_step<1>(B); break;
-        case 2: rc = batch_step<2>(B); break;
-        case 3: rc = batch_step<3>(B); break;
-        default: rc = batch_step<4>(B); break;
-        }
+        with_order(B->norder, [&](auto no) { rc = batch_step<decltype(no)::value>(B); });
         if (rc) return rc;
     }
     return 0;

@@ -308,10 +308,11 @@ __device__ __forceinline__ void lds_barrier()
 constexpr int kXcds = 8;
 
 // Occupancy target of the kernels that run the interior tiles: 3 waves per
-// SIMD (at most 168 VGPRs) up to FD order 6; the order-8 rings need more
-// registers than that and run at 2 waves per SIMD rather than spill.
+// SIMD (at most 168 VGPRs) up to FD order 6; the rings of the higher orders
+// need more registers and run at 2 waves per SIMD (orders 8..12) or 1
+// (orders 14, 16) rather than spill.
 template <int NO>
-constexpr int kIntWaves = NO <= 3 ? 3 : 2;
+constexpr int kIntWaves = NO <= 3 ? 3 : (NO <= 6 ? 2 : 1);
 __device__ __forceinline__ int xcd_tile(int p, int n)
 {
     const int x = p % kXcds, k = p / kXcds, q = n / kXcds, r = n % kXcds;
@@ -622,20 +623,32 @@ struct IntKernel2 {
         }
         __syncthreads();
         // whole ring periods without per-step guards (no merges, so the
-        // ring registers need no copies), then the guarded remainder
+        // ring registers need no copies), then the guarded remainder.  FD
+        // orders above 8 keep only the guarded loop: a second copy of their
+        // long ring period would double an already large body.
+        static_assert(P <= 26, "the step lists below cover ring periods up to 26");
         int base = x0;
-        for (; base + P - 1 <= xl; base += P) {
+        if constexpr (NO <= 4) {
+            for (; base + P - 1 <= xl; base += P) {
 #define IK2_STEP(PH) if constexpr (P > PH) step<PH>(a, B, S, R, L, bc, base + PH, xs, xe);
-            IK2_STEP(0) IK2_STEP(1) IK2_STEP(2) IK2_STEP(3) IK2_STEP(4) IK2_STEP(5) IK2_STEP(6)
-            IK2_STEP(7) IK2_STEP(8) IK2_STEP(9) IK2_STEP(10) IK2_STEP(11) IK2_STEP(12) IK2_STEP(13)
+                IK2_STEP(0) IK2_STEP(1) IK2_STEP(2) IK2_STEP(3) IK2_STEP(4) IK2_STEP(5) IK2_STEP(6)
+                IK2_STEP(7) IK2_STEP(8) IK2_STEP(9) IK2_STEP(10) IK2_STEP(11) IK2_STEP(12) IK2_STEP(13)
 #undef IK2_STEP
+            }
         }
-        if (base <= xl) {
 #define IK2_STEP(PH) if constexpr (P > PH) { if (base + PH <= xl) step<PH>(a, B, S, R, L, bc, base + PH, xs, xe); }
-            IK2_STEP(0) IK2_STEP(1) IK2_STEP(2) IK2_STEP(3) IK2_STEP(4) IK2_STEP(5) IK2_STEP(6)
-            IK2_STEP(7) IK2_STEP(8) IK2_STEP(9) IK2_STEP(10) IK2_STEP(11) IK2_STEP(12) IK2_STEP(13)
-#undef IK2_STEP
+#define IK2_GUARDED_PERIOD                                                                                   \
+    IK2_STEP(0) IK2_STEP(1) IK2_STEP(2) IK2_STEP(3) IK2_STEP(4) IK2_STEP(5) IK2_STEP(6) IK2_STEP(7)        \
+    IK2_STEP(8) IK2_STEP(9) IK2_STEP(10) IK2_STEP(11) IK2_STEP(12) IK2_STEP(13) IK2_STEP(14) IK2_STEP(15) \
+    IK2_STEP(16) IK2_STEP(17) IK2_STEP(18) IK2_STEP(19) IK2_STEP(20) IK2_STEP(21) IK2_STEP(22) IK2_STEP(23) \
+    IK2_STEP(24) IK2_STEP(25)
+        if constexpr (NO <= 4) {
+            if (base <= xl) { IK2_GUARDED_PERIOD }   // fewer than P steps remain
+        } else {
+            for (; base <= xl; base += P) { IK2_GUARDED_PERIOD }
         }
+#undef IK2_GUARDED_PERIOD
+#undef IK2_STEP
     }
 };
 

@@ -27,9 +27,10 @@ def _planes(c, NX, NZ):
     return np.ascontiguousarray(c3[..., 1]), np.ascontiguousarray(c3[..., 2])
 
 
-@pytest.mark.parametrize("norder", [1, 2, 3, 4])
+@pytest.mark.parametrize("norder", [1, 2, 3, 4, 5, 6, 7, 8])
 def test_pointwise_parts_equal_literal_passes(norder):
-    nx, nz, npml = 27, 19, 9
+    # the reference keeps 2*norder <= npml + 1 (src/Input.f90:131)
+    nx, nz, npml = 27, 19, max(9, 2 * norder - 1)
     NX, NZ = nx + 2 * npml, nz + npml
     rng = np.random.default_rng(norder)
     vp = rng.uniform(5000, 7000, (NX, NZ)).astype(np.float32)
@@ -61,9 +62,9 @@ def test_pointwise_parts_equal_literal_passes(norder):
             assert np.array_equal(p.view(np.uint32), want.view(np.uint32)), (ix, iz)
 
 
-@pytest.mark.parametrize("norder", [1, 2, 3, 4])
+@pytest.mark.parametrize("norder", [1, 2, 3, 4, 5, 6, 7, 8])
 def test_second_derivative_exact_on_polynomials(norder):
-    nx, nz, npml = 30, 12, 8
+    nx, nz, npml = 30 + 4 * max(0, norder - 4), 12, max(8, 2 * norder - 1)
     NX, NZ = nx + 2 * npml, nz + npml
     c0, _, _ = po.fd_coefs(norder)
     x = (np.arange(NX) - npml).astype(np.float64) / NX
