@@ -1,7 +1,10 @@
 """Host-side formats: the input files and SU files round-trip through the
 package's writers and readers (src/Input.f90 layouts), the material law and
 geometry restatements agree with the oracle."""
+import os
+
 import numpy as np
+import pytest
 
 from fdfk2d_amd import geometry, io, refstate
 from oracle import pyoracle as po
@@ -62,3 +65,31 @@ def test_stable_dt_matches_oracle():
     c = refstate.material_law(vp, vs)
     _, dts = po.check_stability(nx, nz, npml, 1, 3, 50.0, 50.0, 0.001, po.fd_coefs(3)[2], c)
     assert abs(refstate.stable_dt(3, 50.0, 50.0, c) - dts) <= 1e-7 * dts
+
+
+REF_EXAMPLES = "/root/reference/examples"
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_EXAMPLES), reason="reference examples absent")
+@pytest.mark.parametrize("name", ["test1", "test2", "test3", "test4", "Altyn"])
+def test_reference_example_inputs_parse(name):
+    """Every input set the reference ships (examples/*/input) parses with the
+    package readers, and its FD order lies in the library's range: examples/
+    test1 asks for order 16, i.e. norder 8 after the reference's clamp
+    2*norder <= npml + 1 (src/Input.f90:119-131)."""
+    d = os.path.join(REF_EXAMPLES, name, "input")
+    names = io.read_inpar(d, "inpar.dat")
+    fd = io.read_fd_model(os.path.join(d, names["fd_model"]))
+    src = io.read_source(os.path.join(d, names["source"]))
+    rx, rz = io.read_receivers(os.path.join(d, names["receiver"]))
+    fkl = io.read_fk_model(os.path.join(d, names["fk_left"]))
+    fkr = io.read_fk_model(os.path.join(d, names["fk_right"]))
+    assert 1 <= fd["norder"] <= 8 and 2 * fd["norder"] <= fd["npml"] + 1
+    assert fd["norder"] == min(fd["order"] // 2, (fd["npml"] + 1) // 2)
+    assert not fd["pml_top"]
+    assert fd["dx"] > 0 and fd["dt"] > 0 and fd["xn"] > fd["x0"] and fd["zn"] > fd["z0"]
+    assert src["f0"] > 0 and src["rayp"] > 0
+    assert len(rx) == len(rz) > 0 and np.isfinite(rx).all()
+    assert fkl.shape[1] == 4 and fkr.shape[1] == 4 and (fkl[:, 1] > 0).all()
+    if name == "test1":
+        assert fd["order"] == 16 and fd["norder"] == 8
