@@ -5,7 +5,8 @@ MI355X (C ABI in include/fdfk.h).
     python -m fdfk2d_amd.driver ./input inpar.dat ./seismograms seis
 
 Reads the unchanged input files, writes <outpath>/<prefix>x.su and z.su
-exactly as the reference (radial projection by cos(baz) included).
+exactly as the reference (radial projection by cos(baz) included), and, when
+FD_model.dat asks for them, the wavefield snapshots in ./snapshots/.
 """
 import os
 import sys
@@ -15,6 +16,19 @@ import numpy as np
 
 from . import geometry, io, refstate
 from .lib import Solver
+
+
+def write_snapshot(s, it, nx, nz, npml, dz, snapdir="snapshots"):
+    """Snapshot at the top of iteration it (src/FDFK2D.f90:467-480): level 2
+    of ux(1:2, :, :) and uz, rows 1..nz of columns 1..nx, one SU trace per
+    column, header word 58 = nz and 59 = dz, in ./snapshots/<it>ux.su and
+    <it>uz.su (five-digit iteration, I5.5)."""
+    os.makedirs(snapdir, exist_ok=True)
+    ux, uz = s.get_state()
+    NX, NZ = nx + 2 * npml, nz + npml
+    for name, u in (("ux", ux), ("uz", uz)):
+        lev2 = u.reshape(NX, NZ, 2)[npml: npml + nx, :nz, 1]
+        io.write_su(os.path.join(snapdir, "%05d%s.su" % (it, name)), lev2, int(dz))
 
 
 def simulate(job, log=print):
@@ -45,7 +59,15 @@ def simulate(job, log=print):
     s.sync()
     t1 = time.perf_counter()
     log(f"background (FK, GPU FP64): {t1 - t0:.2f} s")
-    s.run(nt + 1)
+    if fd.get("outsnap") and fd.get("nstep", 0) > 0:
+        done = 0
+        for it in range(fd["nstep"], nt + 1, fd["nstep"]):
+            s.run(it - done)
+            done = it
+            write_snapshot(s, it, nx, nz, npml, fd["dz"])
+        s.run(nt + 1 - done)
+    else:
+        s.run(nt + 1)
     s.sync()
     t2 = time.perf_counter()
     cells = (nx + 2 * npml) * (nz + npml) * (nt + 1)

@@ -155,7 +155,7 @@ def read_job(prefix, finpar):
 
 def write_inputs(prefix, *, order, x0, xn, z0, zn, dx, dz, dt, tmax, tstp, npml, lat_org,
                  lon_org, az_org, f0, ds, src_is_p, rayp_sdeg, evla, evlo, rx, rz, fk_left,
-                 fk_right, vp, vs):
+                 fk_right, vp, vs, outsnap=False, nstep=100):
     """Write a complete input directory in the reference's formats.
     fk_left/right: rows (ztop, vp, vs); vp, vs: padded (NX, NZ) grids."""
     os.makedirs(prefix, exist_ok=True)
@@ -173,7 +173,7 @@ def write_inputs(prefix, *, order, x0, xn, z0, zn, dx, dz, dt, tmax, tstp, npml,
         f.write("latitude of (X0,Z0)       longitude of (X0,Z0)  azimuthe of profile\n")
         f.write(f"  {lat_org!r} {lon_org!r} {az_org!r}\n")
         f.write("output snapshot?(.true./.false.)  nstep\n")
-        f.write("  .false.  100\n")
+        f.write(f"  {L(outsnap)}  {nstep}\n")
     with open(os.path.join(prefix, "Source.dat"), "w") as f:
         f.write("f0(Hz)   streng of source(m)   source type(P: .true.; S: .false.)\n")
         f.write(f"  {f0!r} {ds!r} {L(src_is_p)}\n")

@@ -18,6 +18,9 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 REF_DRIVER = os.path.join(ROOT, "oracle", "_ref", "fdfk2d_ref")
 
 
+SNAP_NSTEP = 100
+
+
 def make_inputs(prefix):
     nx, nz, npml, dx = 101, 50, 14, 200.0
     layers = [(0.0, 6000.0, 3450.0), (6000.0, 8000.0, 4450.0)]
@@ -27,7 +30,8 @@ def make_inputs(prefix):
     io.write_inputs(prefix, order=6, x0=0.0, xn=20000.0, z0=0.0, zn=9800.0, dx=dx, dz=dx, dt=0.01,
                     tmax=2.6, tstp=0.01, npml=npml, lat_org=45.0, lon_org=0.0, az_org=0.0, f0=2.0,
                     ds=1e-3, src_is_p=True, rayp_sdeg=6.6717, evla=-42.0, evlo=10.0, rx=rx,
-                    rz=np.zeros_like(rx), fk_left=fk, fk_right=fk, vp=vp, vs=vs)
+                    rz=np.zeros_like(rx), fk_left=fk, fk_right=fk, vp=vp, vs=vs, outsnap=True,
+                    nstep=SNAP_NSTEP)
 
 
 def oracle_pipeline(job):
@@ -60,7 +64,7 @@ def oracle_pipeline(job):
     return (sx.astype(np.float64) * np.cos(br)[:, None]).astype(np.float32), sz
 
 
-def test_reference_driver_dropin(tmp_path):
+def test_reference_driver_dropin(tmp_path, monkeypatch):
     print("case: reference Fortran driver linked to libfdfk.so, 101x50, 261 iterations", flush=True)
     if not os.path.exists(REF_DRIVER):
         pytest.skip("oracle/_ref/fdfk2d_ref not built (reference tree absent at build time)")
@@ -68,18 +72,29 @@ def test_reference_driver_dropin(tmp_path):
     make_inputs(prefix)
     out_ref = tmp_path / "seis_ref"
     out_ref.mkdir()
+    (tmp_path / "snapshots").mkdir()   # the reference opens ./snapshots/ without creating it
     r = subprocess.run([REF_DRIVER, prefix, "inpar.dat", str(out_ref), "seis"], capture_output=True,
                        text=True, timeout=300, cwd=str(tmp_path))
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     rx_ref, h = io.read_su(str(out_ref / "seisx.su"))
     rz_ref, _ = io.read_su(str(out_ref / "seisz.su"))
     out_pkg = tmp_path / "seis_pkg"
+    pkg_cwd = tmp_path / "pkg"
+    pkg_cwd.mkdir()
+    monkeypatch.chdir(pkg_cwd)
     assert driver.main([prefix, "inpar.dat", str(out_pkg), "seis"]) == 0
     rx_pkg, h2 = io.read_su(str(out_pkg / "seisx.su"))
     rz_pkg, _ = io.read_su(str(out_pkg / "seisz.su"))
     assert np.array_equal(h, h2) and rx_ref.shape == (11, 261)
     assert np.abs(rz_ref).max() > 1e-4
     assert rel_l2(rx_pkg, rx_ref) < 1e-6 and rel_l2(rz_pkg, rz_ref) < 1e-6
+    # snapshots (src/FDFK2D.f90:467-480): the same files, bit for bit
+    snaps = sorted(os.listdir(tmp_path / "snapshots"))
+    assert snaps == ["00100ux.su", "00100uz.su", "00200ux.su", "00200uz.su"]
+    for name in snaps:
+        a, ha = io.read_su(str(tmp_path / "snapshots" / name))
+        b, hb = io.read_su(str(pkg_cwd / "snapshots" / name))
+        assert a.shape == (101, 50) and np.array_equal(ha, hb) and np.array_equal(a, b), name
     job = io.read_job(prefix, "inpar.dat")
     ox, oz = oracle_pipeline(job)
     assert rel_l2(rx_pkg, ox) < 1e-5 and rel_l2(rz_pkg, oz) < 1e-5
