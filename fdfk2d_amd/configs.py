@@ -32,14 +32,13 @@ def fk_layers(layers):
 
 
 def smooth_noise(shape, sigma_cells, seed):
-    """White noise smoothed by a Gaussian kernel (std sigma_cells) via FFT,
-    normalised to unit standard deviation."""
+    """White noise smoothed as utils/gauss_smooth.f90 smooths a model
+    (Gaussian of correlation length sigma_cells, window 3.6 lengths, mirror
+    boundaries; fdfk2d_amd.smooth), normalised to unit standard deviation."""
+    from . import smooth
     rng = np.random.default_rng(seed)
-    w = rng.standard_normal(shape)
-    kx = np.fft.fftfreq(shape[0])[:, None]
-    kz = np.fft.rfftfreq(shape[1])[None, :]
-    filt = np.exp(-2.0 * (np.pi * sigma_cells) ** 2 * (kx * kx + kz * kz))
-    s = np.fft.irfft2(np.fft.rfft2(w) * filt, s=shape)
+    w = rng.standard_normal(shape).astype(np.float32)
+    s = smooth.gaussian_separable(w, sigma_cells, sigma_cells, 1.0, 1.0).astype(np.float64)
     return (s / s.std()).astype(np.float32)
 
 

@@ -109,6 +109,24 @@ def smooth(vel, lambdax, lambdaz, dx, dz, iz0=1, iz1=None, ix0=1, ix1=None, meth
     return out
 
 
+def gaussian_separable(vel, lambdax, lambdaz, dx, dz):
+    """The Gaussian method on the whole model as two 1-D passes: the
+    reference's window exp(-(x/lx)^2/2 - (z/lz)^2/2), truncated at
+    ceil(3.6 l/d) cells and normalised by its sum, is the product of the
+    two 1-D windows truncated and normalised alike, and the mirror fold is
+    per axis.  Same result as smooth() up to rounding, for models too large
+    for the direct window sum (the large-solve perturbation)."""
+    from scipy.ndimage import correlate1d
+    out = np.asarray(vel, np.float32)
+    for axis, lam, d in ((0, lambdax, dx), (1, lambdaz, dz)):
+        n = int(math.ceil(np.float32(3.6) * np.float32(lam) / np.float32(d)))
+        x = np.arange(-n, n + 1) * float(d)
+        w = np.exp(-0.5 * (x / float(lam)) ** 2)
+        # scipy's 'reflect' repeats the edge sample: index 0 -> 1, n+1 -> n
+        out = correlate1d(out.astype(np.float64), w / w.sum(), axis=axis, mode="reflect").astype(np.float32)
+    return out
+
+
 def _window(v, n, default):
     """A window bound as the reference clamps it: outside 1..n -> default."""
     return default if (v <= 0 or v > n) else v

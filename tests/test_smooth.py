@@ -126,3 +126,11 @@ def test_command_line(tmp_path):
     assert int(head[58]) == int(np.int32(200 * 1000).astype(np.int16))
     want = smooth.smooth(v, 600.0, 400.0, 250.0, 200.0, 1, 18, 3, 20, 0)
     assert np.array_equal(out, want)
+
+
+def test_separable_form_equals_the_window_sum():
+    rng = np.random.default_rng(11)
+    v = rng.standard_normal((45, 37)).astype(np.float32)
+    direct = smooth.smooth(v, 1100.0, 700.0, 250.0, 200.0)
+    sep = smooth.gaussian_separable(v, 1100.0, 700.0, 250.0, 200.0)
+    assert np.allclose(sep, direct, rtol=1e-4, atol=1e-5 * np.abs(direct).max())
