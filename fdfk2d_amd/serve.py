@@ -35,7 +35,7 @@ def make_model(nx, nz, npml, dx, seed, perturb=True):
         vs1[z >= zt] = v
     vs = np.repeat(vs1[None, :], NX, 0)
     if perturb:
-        d = configs.smooth_noise((NX, NZ), 5000.0 / dx, seed)
+        d = configs.smooth_noise_fft((NX, NZ), 5000.0 / dx, seed)
         vs = (vs * (1.0 + 0.05 * d)).astype(np.float32)
     vp = (np.float32(1.73) * vs).astype(np.float32)
     layers = [(zt, 1.73 * v, v) for zt, v in vsl]
