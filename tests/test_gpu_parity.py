@@ -174,3 +174,36 @@ def test_fd_orders_match_oracle(norder):
     assert np.abs(ux).max() > 1e-5
     assert rel_l2(ux, np.ravel(st["ux"])) < 1e-5 and rel_l2(uz, np.ravel(st["uz"])) < 1e-5
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("norder", [1, 2, 3, 4, 8])
+def test_minimal_grids_match_oracle(norder):
+    """The smallest grid fdfk_create accepts for each order: nx = 2*nbd + 2,
+    nz = nbd + 2*norder + 1 and npml = nbd (the hybrid ring fills the whole
+    PML, no damped cells), nbd = 2*norder - 1.  One tile of every kind,
+    clipped at both ends.  Both time levels after 61 iterations against the
+    oracle; fused and separate launches bit for bit."""
+    nbd = 2 * norder - 1
+    nx, nz, npml = 2 * nbd + 2, nbd + 2 * norder + 1, nbd
+    print("case: minimal grid norder %d, %d x %d, npml %d" % (norder, nx, nz, npml), flush=True)
+    c = Case(norder=norder, nx=nx, nz=nz, npml=npml, nt=60, nrcv=3)
+    bg = c.oracle_background()
+    st = c.state()
+    ox, oz = oracle_run(c, st, bg, c.nt + 1)
+    out = []
+    for fused in ("1", "0"):
+        os.environ["FDFK_FUSED"] = fused
+        try:
+            s = lib.Solver(c.nx, c.nz, c.npml, c.norder, c.dx, c.dz, c.dt, c.nt, c.c, rx=c.rx, rz=c.rz)
+            s.set_background(bg)
+            s.run(c.nt + 1)
+            out.append(s.get_state() + s.seismograms())
+            s.close()
+        finally:
+            os.environ.pop("FDFK_FUSED", None)
+    ux, uz, gx, gz = out[0]
+    assert np.abs(ux).max() > 0 and np.isfinite(ux).all()
+    assert rel_l2(ux, np.ravel(st["ux"])) < 1e-5 and rel_l2(uz, np.ravel(st["uz"])) < 1e-5
+    assert rel_l2(gx, ox) < 1e-5 and rel_l2(gz, oz) < 1e-5
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
