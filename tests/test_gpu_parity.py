@@ -229,3 +229,36 @@ def test_anisotropic_grid_spacing_matches_oracle(norder):
     assert np.abs(oz).max() > 1e-4
     assert rel_l2(gx, ox) < 1e-5 and rel_l2(gz, oz) < 1e-5
     assert rel_l2(ux, np.ravel(st["ux"])) < 1e-5 and rel_l2(uz, np.ravel(st["uz"])) < 1e-5
+
+
+def test_receivers_off_grid_at_depth_with_origin():
+    """Receivers between grid points, at depth and one left of x0 (inside
+    the padding), with a nonzero grid origin (x0, z0): the recorder's
+    coordinate arithmetic and its clipping at the padded edge
+    (src/Submain.f90:3-116) against the oracle's loadseis, which is pinned
+    bit for bit to the reference's."""
+    print("case: off-grid receivers, x0 = 730 m, z0 = -40 m", flush=True)
+    c = Case(nt=300)
+    x0, z0 = 730.0, -40.0
+    rx = np.array([x0 - 1700.0, x0 + 10.0, x0 + 1234.5, x0 + 5111.1, x0 + 12345.6, x0 + 23999.0],
+                  np.float32)
+    rz = np.array([z0, z0 + 55.5, z0 + 333.3, z0 + 1000.0, z0 + 2500.0, z0 + 120.0], np.float32)
+    bg = c.oracle_background()
+    st = c.state()
+    st["bg"] = bg
+    from oracle import pyoracle as po
+    ox, oz = [], []
+    for it in range(c.nt + 1):
+        row = [po.loadseis(c.nx, c.nz, c.npml, 1, x0, z0, c.dx, c.dz, float(a), float(b), st["ux"], st["uz"])
+               for a, b in zip(rx, rz)]
+        ox.append([r[0] for r in row])
+        oz.append([r[1] for r in row])
+        po.updatedispl(st, it)
+    ox, oz = np.asarray(ox, np.float32).T, np.asarray(oz, np.float32).T
+    s = lib.Solver(c.nx, c.nz, c.npml, c.norder, c.dx, c.dz, c.dt, c.nt, c.c, x0=x0, z0=z0, rx=rx, rz=rz)
+    s.set_background(bg)
+    s.run(c.nt + 1)
+    gx, gz = s.seismograms()
+    s.close()
+    assert np.abs(oz).max() > 1e-4 and np.abs(oz[0]).max() > 0
+    assert rel_l2(gx, ox) < 1e-5 and rel_l2(gz, oz) < 1e-5
