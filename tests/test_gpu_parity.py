@@ -262,3 +262,21 @@ def test_receivers_off_grid_at_depth_with_origin():
     s.close()
     assert np.abs(oz).max() > 1e-4 and np.abs(oz[0]).max() > 0
     assert rel_l2(gx, ox) < 1e-5 and rel_l2(gz, oz) < 1e-5
+
+
+def test_recording_interval():
+    """nstp = 3 (tstp = 3 dt): nt/3 + 1 samples, taken at iterations 0, 3,
+    6, ... before the update (src/FDFK2D.f90:311-317, 497-522), equal to the
+    oracle's every-iteration record at those iterations."""
+    print("case: nstp 3, 301 iterations", flush=True)
+    c = Case(nt=300)
+    bg = c.oracle_background()
+    st = c.state()
+    ox, oz = oracle_run(c, st, bg, c.nt + 1)
+    s = lib.Solver(c.nx, c.nz, c.npml, c.norder, c.dx, c.dz, c.dt, c.nt, c.c, nstp=3, rx=c.rx, rz=c.rz)
+    s.set_background(bg)
+    s.run(c.nt + 1)
+    gx, gz = s.seismograms()
+    s.close()
+    assert gx.shape == (len(c.rx), c.nt // 3 + 1)
+    assert rel_l2(gx, ox[:, ::3]) < 1e-5 and rel_l2(gz, oz[:, ::3]) < 1e-5
