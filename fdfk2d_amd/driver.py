@@ -7,6 +7,11 @@ MI355X (C ABI in include/fdfk.h).
 Reads the unchanged input files, writes <outpath>/<prefix>x.su and z.su
 exactly as the reference (radial projection by cos(baz) included), and, when
 FD_model.dat asks for them, the wavefield snapshots in ./snapshots/.
+
+With --all-events every event line of a multi-event Source.dat is simulated
+(the reference reads only the first, src/Input.f90:142-146), the events
+advanced together as batches of resident solvers, and event k is written to
+<outpath>/<prefix>_<k>x.su and z.su.
 """
 import os
 import sys
@@ -15,7 +20,7 @@ import time
 import numpy as np
 
 from . import geometry, io, refstate
-from .lib import Solver
+from .lib import Batch, Solver
 
 
 def write_snapshot(s, it, nx, nz, npml, dz, snapdir="snapshots"):
@@ -31,8 +36,9 @@ def write_snapshot(s, it, nx, nz, npml, dz, snapdir="snapshots"):
         io.write_su(os.path.join(snapdir, "%05d%s.su" % (it, name)), lev2, int(dz))
 
 
-def simulate(job, log=print):
-    fd, src = job["fd"], job["src"]
+def _prepare(job, src, log):
+    """Solver with its FK background for one event (src/FDFK2D.f90:311-434)."""
+    fd = job["fd"]
     nx, nz, npml, norder = job["nx"], job["nz"], fd["npml"], fd["norder"]
     dt = fd["dt"]
     nt = int(np.rint(fd["tmax"] / dt))                       # FDFK2D.f90:311-315
@@ -53,38 +59,92 @@ def simulate(job, log=print):
     s = Solver(nx, nz, npml, norder, fd["dx"], fd["dz"], dt, nt, job["c"], x0=fd["x0"], z0=fd["z0"],
                nstp=nstp, rx=job["rx"], rz=job["rz"], R0=1e-3, npower=2,
                vpml=refstate.vp_max(job["c"]))
-    t0 = time.perf_counter()
-    s.compute_background(job["fk_left"], job["fk_right"], src["src_is_p"], src["f0"], src["ds"],
-                         rayp, bazs)
-    s.sync()
-    t1 = time.perf_counter()
-    log(f"background (FK, GPU FP64): {t1 - t0:.2f} s")
-    if fd.get("outsnap") and fd.get("nstep", 0) > 0:
-        done = 0
-        for it in range(fd["nstep"], nt + 1, fd["nstep"]):
-            s.run(it - done)
-            done = it
-            write_snapshot(s, it, nx, nz, npml, fd["dz"])
-        s.run(nt + 1 - done)
-    else:
-        s.run(nt + 1)
-    s.sync()
-    t2 = time.perf_counter()
-    cells = (nx + 2 * npml) * (nz + npml) * (nt + 1)
-    log(f"time loop: {nt + 1} iterations in {t2 - t1:.2f} s ({cells / (t2 - t1) / 1e9:.2f} Gcell/s)")
+    try:
+        s.compute_background(job["fk_left"], job["fk_right"], src["src_is_p"], src["f0"], src["ds"],
+                             rayp, bazs)
+        s.sync()
+    except Exception:
+        s.close()
+        raise
+    return s, dict(nt=nt, nsis=nsis, tstp=float(tstp))
+
+
+def _finish(job, src, s):
+    """Seismograms with the radial projection by cos(baz) (FDFK2D.f90:537-543)."""
+    fd = job["fd"]
     sx, sz = s.seismograms()
-    s.close()
-    # radial projection (FDFK2D.f90:537-543)
     br = geometry.backazimuth_grids(fd["lat_org"], fd["lon_org"], fd["az_org"], job["rx"],
                                     src["evla"], src["evlo"]) - fd["az_org"] * np.pi / 180.0
     sx = (sx.astype(np.float64) * np.cos(br)[:, None]).astype(np.float32)
-    return sx, sz, dict(nt=nt, nsis=nsis, tstp=float(tstp))
+    return sx, sz
+
+
+def simulate(job, log=print):
+    fd, src = job["fd"], job["src"]
+    nx, nz, npml = job["nx"], job["nz"], fd["npml"]
+    t0 = time.perf_counter()
+    s, info = _prepare(job, src, log)
+    nt = info["nt"]
+    t1 = time.perf_counter()
+    log(f"background (FK, GPU FP64): {t1 - t0:.2f} s")
+    try:
+        if fd.get("outsnap") and fd.get("nstep", 0) > 0:
+            done = 0
+            for it in range(fd["nstep"], nt + 1, fd["nstep"]):
+                s.run(it - done)
+                done = it
+                write_snapshot(s, it, nx, nz, npml, fd["dz"])
+            s.run(nt + 1 - done)
+        else:
+            s.run(nt + 1)
+        s.sync()
+        t2 = time.perf_counter()
+        cells = (nx + 2 * npml) * (nz + npml) * (nt + 1)
+        log(f"time loop: {nt + 1} iterations in {t2 - t1:.2f} s ({cells / (t2 - t1) / 1e9:.2f} Gcell/s)")
+        sx, sz = _finish(job, src, s)
+    finally:
+        s.close()
+    return sx, sz, info
+
+
+def simulate_events(job, events, hbm_budget_gb=200.0, log=print):
+    """Every event of a multi-event Source.dat on the same model, advanced
+    together as batches of resident solvers (fdfk_batch).  Events are taken
+    in file order, as many per batch as the HBM budget holds; each result is
+    bit-identical to the event run alone.  Returns [(sx, sz)], info."""
+    fd = job["fd"]
+    out, info, k = [], None, 0
+    t0 = time.perf_counter()
+    while k < len(events):
+        batch = Batch(fd["norder"])
+        members = []
+        try:
+            while k < len(events):
+                s, info = _prepare(job, events[k], log)
+                members.append((events[k], s))
+                batch.add(s)
+                k += 1
+                if (len(members) + 1) * s.device_bytes > hbm_budget_gb * 1e9:
+                    break
+            batch.run(info["nt"] + 1)
+            batch.sync()
+            for ev, s in members:
+                out.append(_finish(job, ev, s))
+        finally:
+            for _, s in members:
+                s.close()
+            batch.close()
+        log(f"events {k - len(members) + 1}..{k} of {len(events)} done as one batch "
+            f"({time.perf_counter() - t0:.2f} s so far)")
+    return out, info
 
 
 def main(argv=None):
-    argv = sys.argv[1:] if argv is None else argv
+    argv = list(sys.argv[1:] if argv is None else argv)
+    all_events = "--all-events" in argv
+    argv = [a for a in argv if a != "--all-events"]
     if len(argv) < 3:
-        print("FDFK2D (MI355X path)\n./driver ./inpath inpar.dat ./outpath seism")
+        print("FDFK2D (MI355X path)\n./driver ./inpath inpar.dat ./outpath seism [--all-events]")
         return 0
     prefix, finpar, outpath = argv[:3]
     fsis = argv[3] if len(argv) > 3 else "seis"
@@ -94,8 +154,19 @@ def main(argv=None):
         name = job["names"][key]
         io.write_su(os.path.join(prefix, name[:-3] + "_without_pml.su"),
                     arr[job["fd"]["npml"]: job["fd"]["npml"] + job["nx"], : job["nz"]], 1000)
-    sx, sz, info = simulate(job)
     os.makedirs(outpath, exist_ok=True)
+    if all_events:
+        # every line of Source.dat, <fsis>_<k>x.su and z.su for event k (1-based, file order)
+        events = io.read_events(os.path.join(prefix, job["names"]["source"]))
+        if job["fd"].get("outsnap"):
+            print("snapshots are written for single-event runs only")
+        results, info = simulate_events(job, events)
+        dh = int(np.float32(info["tstp"]) * np.float32(1e6))
+        for k, (sx, sz) in enumerate(results, 1):
+            io.write_su(os.path.join(outpath, f"{fsis}_{k}x.su"), sx, dh)
+            io.write_su(os.path.join(outpath, f"{fsis}_{k}z.su"), sz, dh)
+        return 0
+    sx, sz, info = simulate(job)
     dh = int(np.float32(info["tstp"]) * np.float32(1e6))
     io.write_su(os.path.join(outpath, fsis + "x.su"), sx, dh)
     io.write_su(os.path.join(outpath, fsis + "z.su"), sz, dh)

@@ -73,6 +73,29 @@ def read_source(path):
                 rayp=rayp_sdeg * 180.0 / (np.pi * 6371.0e3), evla=evla, evlo=evlo)
 
 
+def read_events(path):
+    """Every event of a multi-event Source.dat (examples/Altyn/input/Source.dat:4-15):
+    the event of line 4 (the only one the reference reads, Input.f90:142-146)
+    followed by each later line that starts with three numbers (ray parameter
+    in s/deg, latitude, longitude); blank lines and text after '!' are
+    skipped.  f0, strength and source type of line 2 apply to all events."""
+    L = _data_lines(path)
+    first = read_source(path)
+    events = []
+    for ln in L[3:]:
+        t = _tokens(ln.split("!", 1)[0])
+        if len(t) < 3:
+            continue
+        try:
+            rayp_sdeg, evla, evlo = _num(t[0]), _num(t[1]), _num(t[2])
+        except ValueError:
+            continue
+        ev = dict(first, rayp_sdeg=rayp_sdeg, rayp=rayp_sdeg * 180.0 / (np.pi * 6371.0e3),
+                  evla=evla, evlo=evlo)
+        events.append(ev)
+    return events
+
+
 def read_receivers(path):
     """Receiver.dat (Input.f90:153-155, 226-236)."""
     L = _data_lines(path)
@@ -155,9 +178,10 @@ def read_job(prefix, finpar):
 
 def write_inputs(prefix, *, order, x0, xn, z0, zn, dx, dz, dt, tmax, tstp, npml, lat_org,
                  lon_org, az_org, f0, ds, src_is_p, rayp_sdeg, evla, evlo, rx, rz, fk_left,
-                 fk_right, vp, vs, outsnap=False, nstep=100):
+                 fk_right, vp, vs, outsnap=False, nstep=100, more_events=()):
     """Write a complete input directory in the reference's formats.
-    fk_left/right: rows (ztop, vp, vs); vp, vs: padded (NX, NZ) grids."""
+    fk_left/right: rows (ztop, vp, vs); vp, vs: padded (NX, NZ) grids;
+    more_events: further (rayp_sdeg, evla, evlo) lines of Source.dat."""
     os.makedirs(prefix, exist_ok=True)
     L = lambda b: ".true." if b else ".false."
     with open(os.path.join(prefix, "inpar.dat"), "w") as f:
@@ -179,6 +203,10 @@ def write_inputs(prefix, *, order, x0, xn, z0, zn, dx, dz, dt, tmax, tstp, npml,
         f.write(f"  {f0!r} {ds!r} {L(src_is_p)}\n")
         f.write("ray parameter(s/deg)   latitude of source(deg)   longtitude of source(deg)\n")
         f.write(f"  {rayp_sdeg!r} {evla!r} {evlo!r}\n")
+        if more_events:
+            f.write("  \n")
+            for k, (p, la, lo) in enumerate(more_events):
+                f.write(f"  {p!r}  {la!r}\t{lo!r}   ! event {k + 2}\n")
     with open(os.path.join(prefix, "Receiver.dat"), "w") as f:
         f.write("number of receivers:\n")
         f.write(f"  {len(rx)}\n")

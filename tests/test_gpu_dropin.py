@@ -21,7 +21,7 @@ REF_DRIVER = os.path.join(ROOT, "oracle", "_ref", "fdfk2d_ref")
 SNAP_NSTEP = 100
 
 
-def make_inputs(prefix):
+def make_inputs(prefix, outsnap=True, event=(6.6717, -42.0, 10.0), more_events=()):
     nx, nz, npml, dx = 101, 50, 14, 200.0
     layers = [(0.0, 6000.0, 3450.0), (6000.0, 8000.0, 4450.0)]
     vp, vs = layered_grid(nx, nz, npml, dx, dx, layers, anomaly=(9000.0, 4000.0, 1500.0, -0.06))
@@ -29,9 +29,9 @@ def make_inputs(prefix):
     rx = np.arange(0, 20001, 2000, dtype=np.float32)
     io.write_inputs(prefix, order=6, x0=0.0, xn=20000.0, z0=0.0, zn=9800.0, dx=dx, dz=dx, dt=0.01,
                     tmax=2.6, tstp=0.01, npml=npml, lat_org=45.0, lon_org=0.0, az_org=0.0, f0=2.0,
-                    ds=1e-3, src_is_p=True, rayp_sdeg=6.6717, evla=-42.0, evlo=10.0, rx=rx,
-                    rz=np.zeros_like(rx), fk_left=fk, fk_right=fk, vp=vp, vs=vs, outsnap=True,
-                    nstep=SNAP_NSTEP)
+                    ds=1e-3, src_is_p=True, rayp_sdeg=event[0], evla=event[1], evlo=event[2], rx=rx,
+                    rz=np.zeros_like(rx), fk_left=fk, fk_right=fk, vp=vp, vs=vs, outsnap=outsnap,
+                    nstep=SNAP_NSTEP, more_events=more_events)
 
 
 def oracle_pipeline(job):
@@ -98,3 +98,34 @@ def test_reference_driver_dropin(tmp_path, monkeypatch):
     job = io.read_job(prefix, "inpar.dat")
     ox, oz = oracle_pipeline(job)
     assert rel_l2(rx_pkg, ox) < 1e-5 and rel_l2(rz_pkg, oz) < 1e-5
+
+
+def test_all_events_batched_equal_single_runs(tmp_path, monkeypatch):
+    """--all-events on a three-event Source.dat: event k's seismograms are
+    bit-identical to a single-event run of that event, whether the events
+    share one batch or each gets its own (a budget too small for two)."""
+    print("case: driver --all-events, 3 events, 101x50, 261 iterations", flush=True)
+    events = [(6.6717, -42.0, 10.0), (7.2, -35.0, 5.0), (5.9, -50.0, 20.0)]
+    monkeypatch.chdir(tmp_path)
+    multi = str(tmp_path / "multi")
+    make_inputs(multi, outsnap=False, more_events=events[1:])
+    out_all = tmp_path / "out_all"
+    assert driver.main([multi, "inpar.dat", str(out_all), "seis", "--all-events"]) == 0
+    got = []
+    for k, ev in enumerate(events, 1):
+        one = str(tmp_path / f"single{k}")
+        make_inputs(one, outsnap=False, event=ev)
+        out = tmp_path / f"out{k}"
+        assert driver.main([one, "inpar.dat", str(out), "seis"]) == 0
+        for comp in "xz":
+            a, ha = io.read_su(str(out_all / f"seis_{k}{comp}.su"))
+            b, hb = io.read_su(str(out / f"seis{comp}.su"))
+            assert a.shape == (11, 261) and np.array_equal(ha, hb) and np.array_equal(a, b), (k, comp)
+        got.append(io.read_su(str(out_all / f"seis_{k}z.su"))[0])
+    assert rel_l2(got[1], got[0]) > 1e-2 and rel_l2(got[2], got[0]) > 1e-2
+    job = io.read_job(multi, "inpar.dat")
+    ev = io.read_events(os.path.join(multi, "Source.dat"))
+    solo, _ = driver.simulate_events(job, ev, hbm_budget_gb=1e-6, log=lambda *a: None)
+    assert len(solo) == 3
+    for k in range(3):
+        assert np.array_equal(solo[k][1], got[k])

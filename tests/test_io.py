@@ -93,3 +93,35 @@ def test_reference_example_inputs_parse(name):
     assert fkl.shape[1] == 4 and fkr.shape[1] == 4 and (fkl[:, 1] > 0).all()
     if name == "test1":
         assert fd["order"] == 16 and fd["norder"] == 8
+
+
+def test_multi_event_source(tmp_path):
+    """read_events: line 4 first, then every later line that starts with three
+    numbers; comments after '!' and blank lines skipped; f0, strength and
+    type shared."""
+    kw = dict(order=6, x0=0.0, xn=1000.0, z0=0.0, zn=500.0, dx=100.0, dz=100.0, dt=0.01, tmax=1.0,
+              tstp=0.01, npml=10, lat_org=45.0, lon_org=0.0, az_org=0.0, f0=1.5, ds=2e-3,
+              src_is_p=False, rayp_sdeg=6.5, evla=-40.0, evlo=12.0, rx=np.zeros(2, np.float32),
+              rz=np.zeros(2, np.float32), fk_left=[(0.0, 6000.0, 3450.0)],
+              fk_right=[(0.0, 6000.0, 3450.0)], vp=np.full((31, 16), 6000.0, np.float32),
+              vs=np.full((31, 16), 3450.0, np.float32))
+    io.write_inputs(str(tmp_path), more_events=[(7.25, 10.5, -3.0), (5.0, -8.75, 150.25)], **kw)
+    ev = io.read_events(str(tmp_path / "Source.dat"))
+    assert [(e["rayp_sdeg"], e["evla"], e["evlo"]) for e in ev] == \
+        [(6.5, -40.0, 12.0), (7.25, 10.5, -3.0), (5.0, -8.75, 150.25)]
+    assert all(e["f0"] == 1.5 and e["ds"] == 2e-3 and e["src_is_p"] is False for e in ev)
+    assert ev[1]["rayp"] == 7.25 * 180.0 / (np.pi * 6371.0e3)
+    first = io.read_source(str(tmp_path / "Source.dat"))
+    assert ev[0] == first
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_EXAMPLES), reason="reference examples absent")
+def test_reference_multi_event_source_parses():
+    """examples/Altyn/input/Source.dat: eleven events, a blank line after the
+    first, tab separators and '!' comments."""
+    ev = io.read_events(os.path.join(REF_EXAMPLES, "Altyn", "input", "Source.dat"))
+    assert len(ev) == 11
+    assert (ev[0]["rayp_sdeg"], ev[0]["evla"], ev[0]["evlo"]) == (7.379, -0.0451, 124.2812)
+    assert (ev[1]["rayp_sdeg"], ev[1]["evla"], ev[1]["evlo"]) == (5.732, -5.3122, 153.7074)
+    assert (ev[-1]["rayp_sdeg"], ev[-1]["evla"], ev[-1]["evlo"]) == (7.664, 35.2440, 25.2697)
+    assert all(e["f0"] == 1.0 and e["ds"] == 1e-3 and e["src_is_p"] for e in ev)
