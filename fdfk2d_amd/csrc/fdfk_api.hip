@@ -901,3 +901,4 @@ void fdfk_compat_release(void)
 
 #include "fdfk_fk.hip"
 #include "fdfk_batch.hip"
+#include "fdfk_smooth.hip"

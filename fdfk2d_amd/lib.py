@@ -69,6 +69,9 @@ def lib():
             getattr(L, name).argtypes = [C.c_void_p]
         L.fdfk_batch_destroy.restype = None
         L.fdfk_batch_kernel_stats.argtypes = L.fdfk_kernel_stats.argtypes
+        L.fdfk_smooth_model.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_float,
+                                        C.c_float, C.c_float, C.c_float, C.c_int32, C.c_int32,
+                                        C.c_int32, C.c_int32, C.c_int32]
         L.updatedispl_.restype = None
         L.comput_background_wavefield_.restype = None
         L.fdfk_compat_release.restype = None

@@ -5,7 +5,9 @@ the smoothed random perturbation in the large-solve workload).
     python -m fdfk2d_amd.smooth model.su [lambdax lambdaz dx dz iz_first iz_end ix_first ix_end method]
 
 An SU model holds one trace per column x, samples along z.  The smoothed
-model is written to <name up to the first '.'>_smooth.su.
+model is written to <name up to the first '.'>_smooth.su.  With --gpu the
+window sum runs on the GPU (fdfk_smooth_model in libfdfk.so, the same FP32
+arithmetic); method 2 there is the Gaussian as two 1-D FP64 passes.
 
 Differences from the reference, where its behaviour is not defined:
 - The model is allocated vel(1:nx,1:nz) but indexed vel(z, x) and filled
@@ -70,7 +72,9 @@ def weights(lambdax, lambdaz, dx, dz, method=0):
             for kk in range(2 * n + 1):
                 src = wjk[kk] if kk < jj else tz[jj]
                 w[kk, jj] = f32((n - a[jj]) / s2x * float(src))
-    w = (w / w.sum(dtype=f32)).astype(f32)
+    # w = w/sum(w): the FP32 sum runs in the array's memory order, x outer, z inner
+    total = np.cumsum(w.T.ravel(), dtype=f32)[-1]
+    w = (w / total).astype(f32)
     return nwx, nwz, w
 
 
@@ -127,16 +131,34 @@ def gaussian_separable(vel, lambdax, lambdaz, dx, dz):
     return out
 
 
+def smooth_gpu(vel, lambdax, lambdaz, dx, dz, iz0=1, iz1=None, ix0=1, ix1=None, method=0):
+    """smooth() on the GPU (fdfk_smooth_model): methods 0 and 1 with the same
+    FP32 arithmetic as smooth(), method 2 the Gaussian as two FP64 passes
+    (gaussian_separable's arithmetic) inside the window."""
+    from .lib import _check, lib
+    v = np.ascontiguousarray(vel, np.float32)
+    nx, nz = v.shape
+    out = np.empty_like(v)
+    iz1 = nz if iz1 is None else iz1
+    ix1 = nx if ix1 is None else ix1
+    _check(lib().fdfk_smooth_model(v.ctypes.data, out.ctypes.data, nx, nz, float(lambdax),
+                                   float(lambdaz), float(dx), float(dz), int(iz0), int(iz1),
+                                   int(ix0), int(ix1), int(method)))
+    return out
+
+
 def _window(v, n, default):
     """A window bound as the reference clamps it: outside 1..n -> default."""
     return default if (v <= 0 or v > n) else v
 
 
 def main(argv=None):
-    argv = sys.argv[1:] if argv is None else argv
+    argv = list(sys.argv[1:] if argv is None else argv)
+    on_gpu = "--gpu" in argv
+    argv = [a for a in argv if a != "--gpu"]
     if len(argv) < 1:
         print("Usage: python -m fdfk2d_amd.smooth filename [lambdax lambdaz dx dz iz_first iz_end "
-              "ix_first ix_end method]")
+              "ix_first ix_end method] [--gpu]")
         return 0
     infile = argv[0]
     vel, head = io.read_su(infile)
@@ -156,7 +178,8 @@ def main(argv=None):
         method = 0
     iz0, iz1 = min(iz0, iz1), max(iz0, iz1)
     ix0, ix1 = min(ix0, ix1), max(ix0, ix1)
-    out = smooth(vel, lambdax, lambdaz, dx, dz, iz0, iz1, ix0, ix1, method)
+    run = smooth_gpu if on_gpu else smooth
+    out = run(vel, lambdax, lambdaz, dx, dz, iz0, iz1, ix0, ix1, method)
     base = os.path.basename(infile)
     stem = base[: base.index(".")] if "." in base else base
     outfile = os.path.join(os.path.dirname(infile), stem + "_smooth.su")

@@ -120,6 +120,20 @@ int fdfk_batch_kernel_stats(fdfk_batch *b, double *interior_ms_total, int64_t *i
                             double *step_ms_total, int64_t *steps);
 void fdfk_batch_destroy(fdfk_batch *b);
 
+/* --- model smoother: replaces the program utils/gauss_smooth.f90 (weights
+ * :310-360, window sum :366-397).  vel and out hold nx traces of nz FP32
+ * samples (one trace per column x, the SU model order) in host memory; out
+ * may alias vel.  Rows iz0..iz1 of columns ix0..ix1 (1-based, inclusive) are
+ * smoothed, the other cells keep vel's values.  method 0: Gaussian window,
+ * 1: triangular window (equal half-widths in x and z only), both in the
+ * reference's FP32 arithmetic; 2: the Gaussian as two 1-D FP64 passes over
+ * the whole model (same result as 0 up to rounding, 2(2n+1) instead of
+ * (2nx+1)(2nz+1) products per cell).  FDFK_EINVAL when a window half-width
+ * reaches the model's extent (the mirror folds once). --- */
+int fdfk_smooth_model(const float *vel, float *out, int32_t nx, int32_t nz, float lambdax,
+                      float lambdaz, float dx, float dz, int32_t iz0, int32_t iz1, int32_t ix0,
+                      int32_t ix1, int32_t method);
+
 /* --- drop-in compatibility: the exact external procedures the reference
  * driver links (all arguments by reference, gfortran/flang mangling).  Device
  * mirrors are keyed by the host ux pointer; every call copies level 1/2 back

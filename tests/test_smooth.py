@@ -109,7 +109,11 @@ def test_triangular_weights_match_the_in_place_loops():
         for k in range(-n, n + 1):
             W[(k, j)] = np.float32((n - abs(j)) / s2x * float(W[(j, k)]))
     lit = np.array([[W[(k, j)] for j in range(-n, n + 1)] for k in range(-n, n + 1)], np.float32)
-    lit = (lit / lit.sum(dtype=np.float32)).astype(np.float32)
+    tot = np.float32(0.0)
+    for j in range(2 * n + 1):                 # sum(w): memory order of w(k, j), k fastest
+        for k in range(2 * n + 1):
+            tot = np.float32(tot + lit[k, j])
+    lit = (lit / tot).astype(np.float32)
     assert np.array_equal(w, lit)
     with pytest.raises(ValueError):
         smooth.weights(2400.0, 1200.0, 500.0, 500.0, 1)      # unequal half-widths
