@@ -36,6 +36,16 @@ static int fail(int code, const std::string &msg)
             return fail(FDFK_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+// Host-to-device copy on a solver's own stream, waited for before return
+// (sources may be locals).  Creation and background set-up go through the
+// solver's stream, never the null stream, so that they add no ordering
+// point to other solvers' or a batch's streams running at the same time.
+static hipError_t h2d(hipStream_t st, void *dst, const void *src, size_t bytes)
+{
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st);
+    return e == hipSuccess ? hipStreamSynchronize(st) : e;
+}
+
 // FD coefficients: closed forms of src/Submain.f90:1111-1198 (fd_coef0:
 // staggered first derivative, fd_coef1: collocated first derivative)
 static void coef_table(int norder, bool staggered, double *c)
@@ -158,10 +168,10 @@ int build_damping(fdfk_solver *s)
     if ((rc = s->dalloc(&s->ax, g.NX)) || (rc = s->dalloc(&s->bx, g.NX)) ||
         (rc = s->dalloc(&s->az, g.NZ)) || (rc = s->dalloc(&s->bz, g.NZ)))
         return rc;
-    HIPCHK(hipMemcpy(s->ax, s->h_ax.data(), g.NX * 4, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(s->bx, s->h_bx.data(), g.NX * 4, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(s->az, s->h_az.data(), g.NZ * 4, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(s->bz, s->h_bz.data(), g.NZ * 4, hipMemcpyHostToDevice));
+    HIPCHK(h2d(s->stream, s->ax, s->h_ax.data(), g.NX * 4));
+    HIPCHK(h2d(s->stream, s->bx, s->h_bx.data(), g.NX * 4));
+    HIPCHK(h2d(s->stream, s->az, s->h_az.data(), g.NZ * 4));
+    HIPCHK(h2d(s->stream, s->bz, s->h_bz.data(), g.NZ * 4));
     return 0;
 }
 
@@ -183,7 +193,7 @@ int upload_strips(fdfk_solver *s)
             int rc = s->dalloc(&s->d_strips[par], 1);
             if (rc) return rc;
         }
-        HIPCHK(hipMemcpy(s->d_strips[par], &h, sizeof(Strips), hipMemcpyHostToDevice));
+        HIPCHK(h2d(s->stream, s->d_strips[par], &h, sizeof(Strips)));
     }
     return 0;
 }
@@ -566,16 +576,21 @@ int fdfk_create(const fdfk_job *job, fdfk_solver **out)
         (rc = s->dalloc(&s->dcoef, 1)) || (rc = s->dalloc(&s->stage, (size_t)2 * ncell)))
         return bad(rc);
     for (int q = 0; q < 2; ++q) {
-        if (hipMemset(s->ux[q], 0, nf * 4) || hipMemset(s->uz[q], 0, nf * 4)) return bad(fail(FDFK_EDEVICE, "memset"));
+        if (hipMemsetAsync(s->ux[q], 0, nf * 4, s->stream) || hipMemsetAsync(s->uz[q], 0, nf * 4, s->stream))
+            return bad(fail(FDFK_EDEVICE, "memset"));
     }
-    if (hipMemset(s->bf, 0, nf * 4) || hipMemset(s->lam, 0, nf * 4) || hipMemset(s->mu, 0, nf * 4))
+    if (hipMemsetAsync(s->bf, 0, nf * 4, s->stream) || hipMemsetAsync(s->lam, 0, nf * 4, s->stream) ||
+        hipMemsetAsync(s->mu, 0, nf * 4, s->stream))
         return bad(fail(FDFK_EDEVICE, "memset"));
-    if (hipMemcpy(s->dcoef, &s->hcoef, sizeof(Coefs), hipMemcpyHostToDevice) != hipSuccess)
+    if (h2d(s->stream, s->dcoef, &s->hcoef, sizeof(Coefs)) != hipSuccess)
         return bad(fail(FDFK_EDEVICE, "coef upload"));
     // materials: reference layout -> dt^2/rho, lambda, mu
     float *dc = nullptr;
     if (hipMalloc(&dc, (size_t)3 * ncell * 4) != hipSuccess) return bad(fail(FDFK_ENOMEM, "staging"));
-    hipMemcpy(dc, job->c, (size_t)3 * ncell * 4, hipMemcpyHostToDevice);
+    if (h2d(s->stream, dc, job->c, (size_t)3 * ncell * 4) != hipSuccess) {
+        hipFree(dc);
+        return bad(fail(FDFK_EDEVICE, "material upload"));
+    }
     hipLaunchKernelGGL(k_import_material, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, s->stream, g, dc,
                        (float)(g.dt * g.dt), s->bf, s->lam, s->mu);
     hipStreamSynchronize(s->stream);
@@ -589,13 +604,13 @@ int fdfk_create(const fdfk_job *job, fdfk_solver **out)
             for (int p = 0; p < 3; ++p)
                 for (int b = 0; b < 2; ++b) {
                     if ((rc = s->dalloc(&s->strip_fields[st][c][p][b], n))) return bad(rc);
-                    hipMemset(s->strip_fields[st][c][p][b], 0, n * 4);
+                    hipMemsetAsync(s->strip_fields[st][c][p][b], 0, n * 4, s->stream);
                 }
             if ((rc = s->dalloc(&s->psx[st][c], n))) return bad(rc);
-            hipMemset(s->psx[st][c], 0, n * 4);
+            hipMemsetAsync(s->psx[st][c], 0, n * 4, s->stream);
             if (st == S_B) {
                 if ((rc = s->dalloc(&s->psz[st][c], n))) return bad(rc);
-                hipMemset(s->psz[st][c], 0, n * 4);
+                hipMemsetAsync(s->psz[st][c], 0, n * 4, s->stream);
             }
         }
     }
@@ -607,12 +622,14 @@ int fdfk_create(const fdfk_job *job, fdfk_solver **out)
             (rc = s->dalloc(&s->seisx, (size_t)s->nrcv * s->nsamp)) ||
             (rc = s->dalloc(&s->seisz, (size_t)s->nrcv * s->nsamp)))
             return bad(rc);
-        hipMemcpy(s->rx, job->rx, s->nrcv * 4, hipMemcpyHostToDevice);
-        hipMemcpy(s->rz, job->rz, s->nrcv * 4, hipMemcpyHostToDevice);
-        hipMemset(s->seisx, 0, (size_t)s->nrcv * s->nsamp * 4);
-        hipMemset(s->seisz, 0, (size_t)s->nrcv * s->nsamp * 4);
+        if (h2d(s->stream, s->rx, job->rx, s->nrcv * 4) != hipSuccess ||
+            h2d(s->stream, s->rz, job->rz, s->nrcv * 4) != hipSuccess)
+            return bad(fail(FDFK_EDEVICE, "receiver upload"));
+        hipMemsetAsync(s->seisx, 0, (size_t)s->nrcv * s->nsamp * 4, s->stream);
+        hipMemsetAsync(s->seisz, 0, (size_t)s->nrcv * s->nsamp * 4, s->stream);
     }
-    hipError_t e = hipDeviceSynchronize();
+    hipError_t e = hipStreamSynchronize(s->stream);
+    if (e == hipSuccess) e = hipGetLastError();
     if (e != hipSuccess) return bad(fail(FDFK_EDEVICE, std::string("setup: ") + hipGetErrorString(e)));
     *out = s;
     return 0;
@@ -646,9 +663,10 @@ static int alloc_background(fdfk_solver *s)
     int rc;
     if ((rc = s->dalloc(&s->d_bg, 1)) || (rc = s->dalloc(&s->ringx, nf)) || (rc = s->dalloc(&s->ringz, nf)))
         return rc;
-    HIPCHK(hipMemcpy(s->d_bg, &h, sizeof(Background), hipMemcpyHostToDevice));
-    HIPCHK(hipMemset(s->ringx, 0, nf * 4));
-    HIPCHK(hipMemset(s->ringz, 0, nf * 4));
+    HIPCHK(h2d(s->stream, s->d_bg, &h, sizeof(Background)));
+    HIPCHK(hipMemsetAsync(s->ringx, 0, nf * 4, s->stream));
+    HIPCHK(hipMemsetAsync(s->ringz, 0, nf * 4, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
     return 0;
 }
 

@@ -30,6 +30,10 @@ struct BatchPrefix {
     int n;
     int pre[KB_N][kMaxBatch + 1];
 };
+// table layout (host slot and device slot alike): the prefix block, then the
+// jobs from this offset, so that the prefix and the n jobs of a step are one
+// contiguous copy
+constexpr size_t kJobOff = (sizeof(BatchPrefix) + 255) / 256 * 256;
 
 __device__ __forceinline__ int find_job(const int *pre, int n, int bid)
 {
@@ -101,6 +105,14 @@ struct fdfk_batch {
     size_t tabbytes = 0;
     bool prof = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_int, ev_step;
+    // Job templates of the current active set: everything a step's job holds
+    // except the fields that change with the iteration, rebuilt only when
+    // the active set changes (tile width, tilings, frame tiles, coefficients)
+    std::vector<fdfk_solver *> tact;
+    std::vector<char> tjobs;        // BatchJob<norder> per active member
+    std::vector<int> tnb;           // KB_N block counts per active member
+    int tlx = 0;
+    size_t tsh = 0;
     ~fdfk_batch()
     {
         if (stream) hipStreamSynchronize(stream);
@@ -120,7 +132,7 @@ namespace {
 template <int NO>
 size_t batch_table_bytes()
 {
-    return sizeof(BatchJob<NO>) * kMaxBatch + sizeof(BatchPrefix);
+    return kJobOff + sizeof(BatchJob<NO>) * kMaxBatch;
 }
 
 // fill one job's arguments for the step it is about to take; returns the
@@ -186,6 +198,23 @@ void fill_job(fdfk_solver *s, BatchJob<NO> &b, int nb[KB_N], bool &record, int l
     nb[KB_REC] = record ? (s->nrcv + kRecPerBlock - 1) / kRecPerBlock : 0;
 }
 
+// the fields of a job template that change from step to step: the level
+// pointers (they swap every step), the iteration, the recording index and
+// whether this iteration records; nb[KB_REC] follows the recording flag
+template <int NO>
+bool refresh_job(fdfk_solver *s, BatchJob<NO> &b, int nb[KB_N])
+{
+    b.sa = step_args(s);
+    b.ia.ux = s->ux[s->cur]; b.ia.uz = s->uz[s->cur];
+    b.ia.uxp = s->ux[1 - s->cur]; b.ia.uzp = s->uz[1 - s->cur];
+    b.ia.corr = s->bg_valid && s->it <= s->nt;
+    const bool record = (s->isis == s->it) && (s->istp <= s->nsis) && s->nrcv > 0;
+    b.ra.istp = s->istp;
+    b.ra.ux = s->ux[s->cur]; b.ra.uz = s->uz[s->cur];
+    nb[KB_REC] = record ? (s->nrcv + kRecPerBlock - 1) / kRecPerBlock : 0;
+    return record;
+}
+
 template <int NO>
 int batch_step(fdfk_batch *B)
 {
@@ -193,26 +222,40 @@ int batch_step(fdfk_batch *B)
     for (auto *s : B->members)
         if (s->it <= s->nt) act.push_back(s);
     if (act.empty()) return 0;
+    constexpr size_t JB = sizeof(BatchJob<NO>);
+    if (act != B->tact) {
+        B->tlx = batch_lx<NO>(act);
+        B->tjobs.assign(JB * act.size(), 0);
+        B->tnb.assign((size_t)KB_N * act.size(), 0);
+        size_t sh = sizeof(typename IntKernel2<NO, 256>::Smem);
+        for (size_t j = 0; j < act.size(); ++j) {
+            auto &t = ((BatchJob<NO> *)B->tjobs.data())[j];
+            bool r = false;
+            fill_job<NO>(act[j], t, &B->tnb[(size_t)KB_N * j], r, B->tlx);
+            sh = std::max(sh, frame_lds_bytes(t.ft, NO));
+        }
+        B->tsh = sh;
+        B->tact = act;
+    }
     const int slot = B->slot;
     HIPCHK(hipEventSynchronize(B->ev[slot]));   // the table slot is free again
-    auto *J = (BatchJob<NO> *)B->htab[slot];
-    auto *P = (BatchPrefix *)((char *)B->htab[slot] + sizeof(BatchJob<NO>) * kMaxBatch);
+    auto *P = (BatchPrefix *)B->htab[slot];
+    auto *J = (BatchJob<NO> *)((char *)B->htab[slot] + kJobOff);
     P->n = (int)act.size();
     for (int k = 0; k < KB_N; ++k) P->pre[k][0] = 0;
     std::vector<char> rec(act.size());
-    const int lx = batch_lx<NO>(act);
     for (size_t j = 0; j < act.size(); ++j) {
+        std::memcpy((void *)&J[j], B->tjobs.data() + JB * j, JB);
         int nb[KB_N];
-        bool r = false;
-        fill_job<NO>(act[j], J[j], nb, r, lx);
-        rec[j] = r;
+        std::memcpy(nb, &B->tnb[(size_t)KB_N * j], sizeof(nb));
+        rec[j] = refresh_job<NO>(act[j], J[j], nb);
         for (int k = 0; k < KB_N; ++k) P->pre[k][j + 1] = P->pre[k][j] + nb[k];
     }
-    // copy only the used part of the job table plus the prefix block
-    auto *dJ = (BatchJob<NO> *)B->dtab[slot];
-    auto *dP = (BatchPrefix *)((char *)B->dtab[slot] + sizeof(BatchJob<NO>) * kMaxBatch);
-    HIPCHK(hipMemcpyAsync(dJ, J, sizeof(BatchJob<NO>) * act.size(), hipMemcpyHostToDevice, B->stream));
-    HIPCHK(hipMemcpyAsync(dP, P, sizeof(BatchPrefix), hipMemcpyHostToDevice, B->stream));
+    // the prefix block and the used jobs: one copy
+    auto *dP = (BatchPrefix *)B->dtab[slot];
+    auto *dJ = (BatchJob<NO> *)((char *)B->dtab[slot] + kJobOff);
+    HIPCHK(hipMemcpyAsync(B->dtab[slot], B->htab[slot], kJobOff + JB * act.size(), hipMemcpyHostToDevice,
+                          B->stream));
     const int n = P->n;
     hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
     if (B->prof) {
@@ -222,8 +265,7 @@ int batch_step(fdfk_batch *B)
     const int nprep = P->pre[KB_RING][n] + P->pre[KB_PSI][n] + P->pre[KB_REC][n];
     if (nprep) hipLaunchKernelGGL(kb_prep<NO>, dim3(nprep), dim3(256), 0, B->stream, dJ, dP);
     if (B->prof) hipEventRecord(e1, B->stream);
-    size_t sh = sizeof(typename IntKernel2<NO, 256>::Smem);
-    for (size_t j = 0; j < act.size(); ++j) sh = std::max(sh, frame_lds_bytes(J[j].ft, NO));
+    const size_t sh = B->tsh;
     const int nstep = P->pre[KB_INT][n] + P->pre[KB_FRAME][n];
     if (nstep) hipLaunchKernelGGL(kb_step<NO>, dim3(nstep), dim3(256), sh, B->stream, dJ, dP);
     if (B->prof) hipEventRecord(e2, B->stream);
@@ -280,6 +322,7 @@ int fdfk_batch_add(fdfk_batch *B, fdfk_solver *s)
         if (m == s) return fail(FDFK_EINVAL, "solver already in the batch");
     HIPCHK(hipStreamSynchronize(s->stream));   // its own queued work (background, state) is done
     B->members.push_back(s);
+    B->tact.clear();    // a new solver may sit at the address of a removed one
     return 0;
 }
 
@@ -290,6 +333,7 @@ int fdfk_batch_remove(fdfk_batch *B, fdfk_solver *s)
     for (size_t i = 0; i < B->members.size(); ++i)
         if (B->members[i] == s) {
             B->members.erase(B->members.begin() + i);
+            B->tact.clear();
             return 0;
         }
     return fail(FDFK_EINVAL, "solver not in the batch");

@@ -301,14 +301,14 @@ int fk_traces_dev(fdfk_solver *s, FKModel &m, const FKRun &r, const double *zq, 
     DevBuf amps, work, dlay, ddz, spec, dm;
     const long nkp = r.nfft / 2 + 1;
     HIPCHK(hipMalloc(&dm.p, sizeof(FKModel)));
-    HIPCHK(hipMemcpy(dm.p, &m, sizeof(FKModel), hipMemcpyHostToDevice));
+    HIPCHK(h2d(s->stream, dm.p, &m, sizeof(FKModel)));
     HIPCHK(hipMalloc(&amps.p, (size_t)r.nk * m.nl * 4 * sizeof(zc)));
     HIPCHK(hipMalloc(&work.p, (size_t)r.nk * m.nl * 12 * sizeof(zc)));
     HIPCHK(hipMalloc(&dlay.p, nzq * sizeof(int)));
     HIPCHK(hipMalloc(&ddz.p, nzq * sizeof(double)));
     HIPCHK(hipMalloc(&spec.p, (size_t)2 * nzq * nkp * sizeof(zc)));
-    HIPCHK(hipMemcpy(dlay.p, lay.data(), nzq * sizeof(int), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(ddz.p, dzl.data(), nzq * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(h2d(s->stream, dlay.p, lay.data(), nzq * sizeof(int)));
+    HIPCHK(h2d(s->stream, ddz.p, dzl.data(), nzq * sizeof(double)));
     hipLaunchKernelGGL(k_fk_amp, dim3((r.nk + 127) / 128), dim3(128), 0, s->stream, dm.as<FKModel>(), r, amps.as<zc>(),
                        work.as<zc>());
     long ns = (long)nzq * nkp;

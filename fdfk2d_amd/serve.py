@@ -1,18 +1,22 @@
 """Continuous-batching request server for FDFK2D simulations on one GPU.
 
 Requests arrive over time (a Poisson stream); each is admitted as soon as
-its HBM footprint fits the budget, gets its own resident state (solver
-handle, FK background computed on the GPU), joins the running batch, and
-advances with every other admitted job in the same batched launches.  A job
-leaves the batch when its last iteration is done; its seismograms are read
-back and its memory freed, making room for waiting requests.  Latency is
-measured from arrival to seismograms on the host.
+its HBM footprint fits the budget.  A pool of host threads prepares admitted
+requests (model, solver handle, FK background computed on the GPU on the
+solver's own stream) while the main thread keeps the running batch stepping;
+a prepared job joins the batch at the next chunk boundary and advances with
+every other admitted job in the same batched launches.  A job leaves the
+batch when its last iteration is done; its seismograms are read back and its
+memory freed, making room for waiting requests.  Latency is measured from
+arrival to seismograms on the host.
 
 The plain path for comparison serves the same requests one at a time with
 the single-job API (fdfk_run).
 """
 import collections
+import concurrent.futures as cf
 import math
+import os
 import time
 
 import numpy as np
@@ -109,59 +113,84 @@ def rel_l2(a, b):
 # serving
 # ---------------------------------------------------------------------------
 class Server:
-    def __init__(self, hbm_budget_gb=200.0, chunk=100, log=None):
+    def __init__(self, hbm_budget_gb=200.0, chunk=100, log=None, workers=None):
         self.budget = hbm_budget_gb * 1e9
         self.chunk = chunk
+        self.workers = workers or max(1, min(8, os.cpu_count() or 1))
         self.batch = Batch(3)
         self.used = 0.0
         self.active = {}
+        self.prep_parts = dict(model=0.0, create=0.0, background=0.0)
         self.log = log or (lambda *a: None)
 
-    def _admit(self, req):
+    def _prepare(self, req):
+        """Host thread: model, solver and FK background of one request (the
+        library calls release the GIL; the background runs on the solver's
+        own stream, beside the batch's)."""
+        t0 = time.perf_counter()
         job = build_job(req)
+        t1 = time.perf_counter()
         s = Solver(req["nx"], req["nz"], req["npml"], 3, req["dx"], req["dx"], job["dt"], job["nt"],
                    job["c"], rx=job["rx"], rz=job["rz"])
-        s.compute_background(job["fk"], job["fk"], req["src_is_p"], job["f0"], job["ds"], job["rayp"],
-                             job["bazs"])
-        self.batch.add(s)
-        self.active[req["id"]] = (req, s)
+        t2 = time.perf_counter()
+        try:
+            s.compute_background(job["fk"], job["fk"], req["src_is_p"], job["f0"], job["ds"],
+                                 job["rayp"], job["bazs"])
+        except Exception:
+            s.close()
+            raise
+        t3 = time.perf_counter()
+        return s, (t1 - t0, t2 - t1, t3 - t2)
 
     def run(self, reqs, keep_results=False):
         """Serve reqs (arrival times relative to the start); returns per
         request latency and the total wall time."""
         pending = collections.deque(sorted(reqs, key=lambda r: r["arrival"]))
         waiting = collections.deque()
+        preparing = {}
         lat, results = {}, {}
         t0 = time.perf_counter()
-        while pending or waiting or self.active:
-            now = time.perf_counter() - t0
-            while pending and pending[0]["arrival"] <= now:
-                waiting.append(pending.popleft())
-            while waiting:
-                req = waiting[0]
-                res, work = footprint_bytes(req)
-                if self.used + res + work > self.budget and self.active:
-                    break
-                waiting.popleft()
-                self._admit(req)
-                self.used += res
-            if not self.active:
-                if pending:
-                    time.sleep(max(0.0, min(0.05, pending[0]["arrival"] - now)))
-                continue
-            self.batch.run(self.chunk)
-            if self.batch.active < len(self.active):
-                self.batch.sync()
-                for rid, (req, s) in list(self.active.items()):
-                    if s.iteration > s.nt:
-                        sx, sz = s.seismograms()
-                        self.batch.remove(s)
-                        s.close()
-                        del self.active[rid]
-                        self.used -= footprint_bytes(req)[0]
-                        lat[rid] = time.perf_counter() - t0 - req["arrival"]
-                        if keep_results:
-                            results[rid] = (sx, sz)
+        with cf.ThreadPoolExecutor(self.workers) as pool:
+            while pending or waiting or preparing or self.active:
+                now = time.perf_counter() - t0
+                while pending and pending[0]["arrival"] <= now:
+                    waiting.append(pending.popleft())
+                while waiting:
+                    req = waiting[0]
+                    res, work = footprint_bytes(req)
+                    if self.used + res + work > self.budget and (self.active or preparing):
+                        break
+                    waiting.popleft()
+                    preparing[pool.submit(self._prepare, req)] = (req, work)
+                    self.used += res + work
+                for f in [f for f in preparing if f.done()]:
+                    req, work = preparing.pop(f)
+                    s, secs = f.result()
+                    for k, v in zip(("model", "create", "background"), secs):
+                        self.prep_parts[k] += v
+                    self.used -= work                      # FK work arrays are gone
+                    self.batch.add(s)
+                    self.active[req["id"]] = (req, s)
+                if not self.active:
+                    wait = 0.05 if not pending else max(0.0, min(0.05, pending[0]["arrival"] - now))
+                    if preparing:
+                        cf.wait(list(preparing), timeout=wait, return_when=cf.FIRST_COMPLETED)
+                    elif pending:
+                        time.sleep(wait)
+                    continue
+                self.batch.run(self.chunk)
+                if self.batch.active < len(self.active):
+                    self.batch.sync()
+                    for rid, (req, s) in list(self.active.items()):
+                        if s.iteration > s.nt:
+                            sx, sz = s.seismograms()
+                            self.batch.remove(s)
+                            s.close()
+                            del self.active[rid]
+                            self.used -= footprint_bytes(req)[0]
+                            lat[rid] = time.perf_counter() - t0 - req["arrival"]
+                            if keep_results:
+                                results[rid] = (sx, sz)
         total = time.perf_counter() - t0
         return lat, total, results
 
@@ -169,22 +198,33 @@ class Server:
         self.batch.close()
 
 
-def serve_plain(reqs, keep_results=False):
-    """One request at a time with the single-job API, FIFO by arrival."""
+def serve_plain(reqs, keep_results=False, stats=None):
+    """One request at a time with the single-job API, FIFO by arrival.
+    stats, if a dict, accumulates the seconds of each phase."""
     lat, results = {}, {}
+    st = stats if stats is not None else {}
+    for k in ("model", "create", "background", "run"):
+        st.setdefault(k, 0.0)
     t0 = time.perf_counter()
     for req in sorted(reqs, key=lambda r: r["arrival"]):
         now = time.perf_counter() - t0
         if req["arrival"] > now:
             time.sleep(req["arrival"] - now)
+        ta = time.perf_counter()
         job = build_job(req)
+        tb = time.perf_counter()
         s = Solver(req["nx"], req["nz"], req["npml"], 3, req["dx"], req["dx"], job["dt"], job["nt"],
                    job["c"], rx=job["rx"], rz=job["rz"])
+        tc = time.perf_counter()
         s.compute_background(job["fk"], job["fk"], req["src_is_p"], job["f0"], job["ds"], job["rayp"],
                              job["bazs"])
+        td = time.perf_counter()
         s.run(job["nt"] + 1)
         sx, sz = s.seismograms()
         s.close()
+        te = time.perf_counter()
+        for k, v in zip(("model", "create", "background", "run"), (tb - ta, tc - tb, td - tc, te - td)):
+            st[k] += v
         lat[req["id"]] = time.perf_counter() - t0 - req["arrival"]
         if keep_results:
             results[req["id"]] = (sx, sz)
