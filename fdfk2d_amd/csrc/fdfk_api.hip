@@ -75,6 +75,7 @@ struct fdfk_solver {
     double R0 = 1e-3;
     int npower = 2;
     hipStream_t stream = nullptr;
+    hipEvent_t ev_order = nullptr;   // orders this stream against a batch's (join, leave)
     std::vector<void *> allocs;
     int64_t bytes = 0;
     // fields
@@ -123,6 +124,7 @@ struct fdfk_solver {
     ~fdfk_solver()
     {
         if (stream) hipStreamSynchronize(stream);
+        if (ev_order) hipEventDestroy(ev_order);
         for (auto &e : ev_int) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
         for (auto &e : ev_step) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
         for (void *p : allocs) hipFree(p);

@@ -320,7 +320,11 @@ int fdfk_batch_add(fdfk_batch *B, fdfk_solver *s)
     if ((int)B->members.size() >= kMaxBatch) return fail(FDFK_ERANGE, "batch is full");
     for (auto *m : B->members)
         if (m == s) return fail(FDFK_EINVAL, "solver already in the batch");
-    HIPCHK(hipStreamSynchronize(s->stream));   // its own queued work (background, state) is done
+    // the batch's next steps wait for the solver's queued work (background,
+    // state) on the GPU; the host does not wait
+    if (!s->ev_order) HIPCHK(hipEventCreateWithFlags(&s->ev_order, hipEventDisableTiming));
+    HIPCHK(hipEventRecord(s->ev_order, s->stream));
+    HIPCHK(hipStreamWaitEvent(B->stream, s->ev_order, 0));
     B->members.push_back(s);
     B->tact.clear();    // a new solver may sit at the address of a removed one
     return 0;
@@ -329,9 +333,13 @@ int fdfk_batch_add(fdfk_batch *B, fdfk_solver *s)
 int fdfk_batch_remove(fdfk_batch *B, fdfk_solver *s)
 {
     if (!B || !s) return fail(FDFK_EINVAL, "null argument");
-    HIPCHK(hipStreamSynchronize(B->stream));
     for (size_t i = 0; i < B->members.size(); ++i)
         if (B->members[i] == s) {
+            // later work on the solver's stream (read-back, single-job steps,
+            // destroy) waits on the GPU for the batch steps already queued
+            if (!s->ev_order) HIPCHK(hipEventCreateWithFlags(&s->ev_order, hipEventDisableTiming));
+            HIPCHK(hipEventRecord(s->ev_order, B->stream));
+            HIPCHK(hipStreamWaitEvent(s->stream, s->ev_order, 0));
             B->members.erase(B->members.begin() + i);
             B->tact.clear();
             return 0;

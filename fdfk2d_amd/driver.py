@@ -131,9 +131,9 @@ def simulate_events(job, events, hbm_budget_gb=200.0, log=print):
             for ev, s in members:
                 out.append(_finish(job, ev, s))
         finally:
+            batch.close()
             for _, s in members:
                 s.close()
-            batch.close()
         log(f"events {k - len(members) + 1}..{k} of {len(events)} done as one batch "
             f"({time.perf_counter() - t0:.2f} s so far)")
     return out, info

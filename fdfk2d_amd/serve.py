@@ -142,23 +142,34 @@ class Server:
         t3 = time.perf_counter()
         return s, (t1 - t0, t2 - t1, t3 - t2)
 
+    @staticmethod
+    def _finish(s):
+        """Host thread: seismograms of a solver that left the batch (its
+        stream is ordered after the batch's last steps), then its memory."""
+        try:
+            return s.seismograms(), time.perf_counter()
+        finally:
+            s.close()
+
     def run(self, reqs, keep_results=False):
         """Serve reqs (arrival times relative to the start); returns per
-        request latency and the total wall time."""
+        request latency and the total wall time.  The main thread only
+        admits, joins and steps; preparation and read-back run on the pool,
+        so the GPU is never drained while requests come and go."""
         pending = collections.deque(sorted(reqs, key=lambda r: r["arrival"]))
         waiting = collections.deque()
-        preparing = {}
+        preparing, finishing = {}, {}
         lat, results = {}, {}
         t0 = time.perf_counter()
         with cf.ThreadPoolExecutor(self.workers) as pool:
-            while pending or waiting or preparing or self.active:
+            while pending or waiting or preparing or self.active or finishing:
                 now = time.perf_counter() - t0
                 while pending and pending[0]["arrival"] <= now:
                     waiting.append(pending.popleft())
                 while waiting:
                     req = waiting[0]
                     res, work = footprint_bytes(req)
-                    if self.used + res + work > self.budget and (self.active or preparing):
+                    if self.used + res + work > self.budget and (self.active or preparing or finishing):
                         break
                     waiting.popleft()
                     preparing[pool.submit(self._prepare, req)] = (req, work)
@@ -171,26 +182,28 @@ class Server:
                     self.used -= work                      # FK work arrays are gone
                     self.batch.add(s)
                     self.active[req["id"]] = (req, s)
+                for f in [f for f in finishing if f.done()]:
+                    req = finishing.pop(f)
+                    (sx, sz), t_done = f.result()
+                    self.used -= footprint_bytes(req)[0]
+                    lat[req["id"]] = t_done - t0 - req["arrival"]
+                    if keep_results:
+                        results[req["id"]] = (sx, sz)
                 if not self.active:
                     wait = 0.05 if not pending else max(0.0, min(0.05, pending[0]["arrival"] - now))
-                    if preparing:
-                        cf.wait(list(preparing), timeout=wait, return_when=cf.FIRST_COMPLETED)
+                    if preparing or finishing:
+                        cf.wait(list(preparing) + list(finishing), timeout=wait,
+                                return_when=cf.FIRST_COMPLETED)
                     elif pending:
                         time.sleep(wait)
                     continue
                 self.batch.run(self.chunk)
                 if self.batch.active < len(self.active):
-                    self.batch.sync()
                     for rid, (req, s) in list(self.active.items()):
                         if s.iteration > s.nt:
-                            sx, sz = s.seismograms()
                             self.batch.remove(s)
-                            s.close()
                             del self.active[rid]
-                            self.used -= footprint_bytes(req)[0]
-                            lat[rid] = time.perf_counter() - t0 - req["arrival"]
-                            if keep_results:
-                                results[rid] = (sx, sz)
+                            finishing[pool.submit(self._finish, s)] = req
         total = time.perf_counter() - t0
         return lat, total, results
 

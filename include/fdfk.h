@@ -107,8 +107,11 @@ const char *fdfk_last_error(void);
  * reference (one simulation per process there). --- */
 typedef struct fdfk_batch fdfk_batch;
 int fdfk_batch_create(int32_t norder, fdfk_batch **out);
-/* waits for the solver's own queued work (e.g. its background) first */
+/* the batch's later steps are ordered after the solver's queued work (e.g.
+ * its background) on the GPU; neither call waits on the host */
 int fdfk_batch_add(fdfk_batch *b, fdfk_solver *s);
+/* later work on the solver's stream (fdfk_seismograms, fdfk_run,
+ * fdfk_destroy) is ordered after the batch steps already queued */
 int fdfk_batch_remove(fdfk_batch *b, fdfk_solver *s);
 /* nsteps batched steps; members past their last iteration are skipped */
 int fdfk_batch_run(fdfk_batch *b, int32_t nsteps);
