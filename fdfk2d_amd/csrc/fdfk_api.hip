@@ -587,16 +587,17 @@ int fdfk_create(const fdfk_job *job, fdfk_solver **out)
     if (h2d(s->stream, s->dcoef, &s->hcoef, sizeof(Coefs)) != hipSuccess)
         return bad(fail(FDFK_EDEVICE, "coef upload"));
     // materials: reference layout -> dt^2/rho, lambda, mu
+    // staging buffer in the stream's order: freeing it waits for no other stream
     float *dc = nullptr;
-    if (hipMalloc(&dc, (size_t)3 * ncell * 4) != hipSuccess) return bad(fail(FDFK_ENOMEM, "staging"));
+    if (hipMallocAsync((void **)&dc, (size_t)3 * ncell * 4, s->stream) != hipSuccess)
+        return bad(fail(FDFK_ENOMEM, "staging"));
     if (h2d(s->stream, dc, job->c, (size_t)3 * ncell * 4) != hipSuccess) {
-        hipFree(dc);
+        hipFreeAsync(dc, s->stream);
         return bad(fail(FDFK_EDEVICE, "material upload"));
     }
     hipLaunchKernelGGL(k_import_material, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, s->stream, g, dc,
                        (float)(g.dt * g.dt), s->bf, s->lam, s->mu);
-    hipStreamSynchronize(s->stream);
-    hipFree(dc);
+    hipFreeAsync(dc, s->stream);
     // PML strips
     s->strip_cells[S_L] = s->strip_cells[S_R] = (long)g.npml * g.nz;
     s->strip_cells[S_B] = (long)g.NX * g.npml;

@@ -273,11 +273,61 @@ __global__ void k_fk_fill(FillArgs f)
 
 namespace {
 
+// Work buffers of one background computation, allocated and freed in the
+// order of the solver's stream (no device-wide wait when they are freed,
+// so other streams' queued work, such as a running batch, is not waited for)
 struct DevBuf {
     void *p = nullptr;
-    ~DevBuf() { if (p) hipFree(p); }
+    hipStream_t st = nullptr;
+    hipError_t alloc(size_t bytes, hipStream_t stream)
+    {
+        st = stream;
+        return hipMallocAsync(&p, bytes, stream);
+    }
+    ~DevBuf() { if (p) (void)hipFreeAsync(p, st); }
     template <class T> T *as() { return (T *)p; }
 };
+
+// Inverse-FFT plans, kept for the life of the process and shared by shape:
+// a caller takes an idle plan of its shape or makes one, and gives it back.
+// Plan set-up uploads device tables; reusing plans keeps it out of the path
+// of every background computation after the first of a shape.
+struct PlanPool {
+    struct Entry { int nfft, nkp, batch; hipfftHandle h; bool busy; };
+    std::mutex mu;
+    std::vector<Entry> plans;
+};
+PlanPool &plan_pool()
+{
+    static PlanPool *pool = new PlanPool();   // never destroyed: plans outlive static teardown
+    return *pool;
+}
+int plan_take(int nfft, int nkp, int batch, hipfftHandle &h)
+{
+    PlanPool &P = plan_pool();
+    {
+        std::lock_guard<std::mutex> lk(P.mu);
+        for (auto &e : P.plans)
+            if (!e.busy && e.nfft == nfft && e.nkp == nkp && e.batch == batch) {
+                e.busy = true;
+                h = e.h;
+                return 0;
+            }
+    }
+    int n[1] = {nfft};
+    if (hipfftPlanMany(&h, 1, n, nullptr, 1, nkp, nullptr, 1, nfft, HIPFFT_Z2D, batch) != HIPFFT_SUCCESS)
+        return fail(FDFK_EDEVICE, "hipfftPlanMany failed");
+    std::lock_guard<std::mutex> lk(P.mu);
+    P.plans.push_back({nfft, nkp, batch, h, true});
+    return 0;
+}
+void plan_give(hipfftHandle h)
+{
+    PlanPool &P = plan_pool();
+    std::lock_guard<std::mutex> lk(P.mu);
+    for (auto &e : P.plans)
+        if (e.h == h) e.busy = false;
+}
 
 int next_pow2(long n)
 {
@@ -300,13 +350,13 @@ int fk_traces_dev(fdfk_solver *s, FKModel &m, const FKRun &r, const double *zq, 
     }
     DevBuf amps, work, dlay, ddz, spec, dm;
     const long nkp = r.nfft / 2 + 1;
-    HIPCHK(hipMalloc(&dm.p, sizeof(FKModel)));
+    HIPCHK(dm.alloc(sizeof(FKModel), s->stream));
     HIPCHK(h2d(s->stream, dm.p, &m, sizeof(FKModel)));
-    HIPCHK(hipMalloc(&amps.p, (size_t)r.nk * m.nl * 4 * sizeof(zc)));
-    HIPCHK(hipMalloc(&work.p, (size_t)r.nk * m.nl * 12 * sizeof(zc)));
-    HIPCHK(hipMalloc(&dlay.p, nzq * sizeof(int)));
-    HIPCHK(hipMalloc(&ddz.p, nzq * sizeof(double)));
-    HIPCHK(hipMalloc(&spec.p, (size_t)2 * nzq * nkp * sizeof(zc)));
+    HIPCHK(amps.alloc((size_t)r.nk * m.nl * 4 * sizeof(zc), s->stream));
+    HIPCHK(work.alloc((size_t)r.nk * m.nl * 12 * sizeof(zc), s->stream));
+    HIPCHK(dlay.alloc(nzq * sizeof(int), s->stream));
+    HIPCHK(ddz.alloc(nzq * sizeof(double), s->stream));
+    HIPCHK(spec.alloc((size_t)2 * nzq * nkp * sizeof(zc), s->stream));
     HIPCHK(h2d(s->stream, dlay.p, lay.data(), nzq * sizeof(int)));
     HIPCHK(h2d(s->stream, ddz.p, dzl.data(), nzq * sizeof(double)));
     hipLaunchKernelGGL(k_fk_amp, dim3((r.nk + 127) / 128), dim3(128), 0, s->stream, dm.as<FKModel>(), r, amps.as<zc>(),
@@ -315,13 +365,11 @@ int fk_traces_dev(fdfk_solver *s, FKModel &m, const FKRun &r, const double *zq, 
     hipLaunchKernelGGL(k_fk_spec, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, s->stream, dm.as<FKModel>(), r,
                        amps.as<zc>(), dlay.as<int>(), ddz.as<double>(), spec.as<zc>());
     hipfftHandle plan;
-    int n[1] = {r.nfft};
-    if (hipfftPlanMany(&plan, 1, n, nullptr, 1, (int)nkp, nullptr, 1, r.nfft, HIPFFT_Z2D, 2 * nzq) != HIPFFT_SUCCESS)
-        return fail(FDFK_EDEVICE, "hipfftPlanMany failed");
+    if (int rc = plan_take(r.nfft, (int)nkp, 2 * nzq, plan)) return rc;
     hipfftSetStream(plan, s->stream);
     hipfftResult fr = hipfftExecZ2D(plan, (hipfftDoubleComplex *)spec.p, (hipfftDoubleReal *)g_out);
     hipStreamSynchronize(s->stream);
-    hipfftDestroy(plan);
+    plan_give(plan);
     if (fr != HIPFFT_SUCCESS) return fail(FDFK_EDEVICE, "hipfftExecZ2D failed");
     long tot = 2L * nzq * r.nfft;
     hipLaunchKernelGGL(k_fk_scale, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s->stream, r, g_out,
@@ -371,10 +419,10 @@ extern "C" int fdfk_compute_background(fdfk_solver *s, const fdfk_fk *fk)
     if (rc) return rc;
     DevBuf gl, gr;
     const size_t ntr = (size_t)2 * nzq * r.nfft;
-    HIPCHK(hipMalloc(&gl.p, ntr * sizeof(double)));
+    HIPCHK(gl.alloc(ntr * sizeof(double), s->stream));
     if ((rc = fk_traces_dev(s, ml, r, zq.data(), gl.as<double>()))) return rc;
     if (!same) {
-        HIPCHK(hipMalloc(&gr.p, ntr * sizeof(double)));
+        HIPCHK(gr.alloc(ntr * sizeof(double), s->stream));
         if ((rc = fk_traces_dev(s, mr, r, zq.data(), gr.as<double>()))) return rc;
     }
     FillArgs f{};
