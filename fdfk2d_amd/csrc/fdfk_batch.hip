@@ -16,6 +16,13 @@ namespace fdfk {
 constexpr int kMaxBatch = 256;
 enum { KB_RING = 0, KB_PSI, KB_INT, KB_FRAME, KB_REC, KB_N };
 
+// One member's arguments in a batch table.  A table is built when the set
+// of active members changes and then serves every step until the next
+// change: the level pointers alternate (one table per step parity), and the
+// iteration-dependent values follow from dstep, the steps taken since the
+// table was built: iteration sa.it + dstep; a recording at every nstp-th
+// step from rec_first on (rec_left of them, -1: none), into sample
+// ra.istp + (dstep - rec_first) / nstp.
 template <int NO>
 struct BatchJob {
     StepArgs sa;
@@ -24,15 +31,16 @@ struct BatchJob {
     RecArgs ra;
     const Background *bg;
     float *ringx, *ringz;
+    int rec_first, nstp, rec_left;
 };
 
 struct BatchPrefix {
     int n;
     int pre[KB_N][kMaxBatch + 1];
 };
-// table layout (host slot and device slot alike): the prefix block, then the
-// jobs from this offset, so that the prefix and the n jobs of a step are one
-// contiguous copy
+// table layout (host slot and device slot alike): the prefix block, then
+// from this offset the n jobs of even steps followed by the n jobs of odd
+// steps, so that a table is one contiguous copy
 constexpr size_t kJobOff = (sizeof(BatchPrefix) + 255) / 256 * 256;
 
 __device__ __forceinline__ int find_job(const int *pre, int n, int bid)
@@ -49,7 +57,7 @@ __device__ __forceinline__ int find_job(const int *pre, int n, int bid)
 // background, the PML memories and (on recording iterations) the receivers,
 // in that block order; 256-lane blocks, kRecPerBlock receivers per block
 template <int NO>
-__global__ __launch_bounds__(256) void kb_prep(const BatchJob<NO> *J, const BatchPrefix *P)
+__global__ __launch_bounds__(256) void kb_prep(const BatchJob<NO> *J, const BatchPrefix *P, int dstep)
 {
     const int n = P->n;
     int b = blockIdx.x;
@@ -57,7 +65,7 @@ __global__ __launch_bounds__(256) void kb_prep(const BatchJob<NO> *J, const Batc
     if (b < nr) {
         const int j = find_job(P->pre[KB_RING], n, b);
         const BatchJob<NO> &q = J[j];
-        ring_prep_body(q.sa.g, q.bg, q.sa.it, q.ringx, q.ringz, b - P->pre[KB_RING][j]);
+        ring_prep_body(q.sa.g, q.bg, q.sa.it + dstep, q.ringx, q.ringz, b - P->pre[KB_RING][j]);
         return;
     }
     b -= nr;
@@ -68,7 +76,10 @@ __global__ __launch_bounds__(256) void kb_prep(const BatchJob<NO> *J, const Batc
     }
     b -= np;
     const int j = find_job(P->pre[KB_REC], n, b);
-    record_body(J[j].ra, b - P->pre[KB_REC][j]);
+    const BatchJob<NO> &q = J[j];
+    const int d = dstep - q.rec_first;     // block-uniform: the whole block records or leaves
+    if (q.rec_first < 0 || d < 0 || d % q.nstp != 0 || d / q.nstp >= q.rec_left) return;
+    record_body(q.ra, b - P->pre[KB_REC][j], q.ra.istp + d / q.nstp);
 }
 
 // the interior tiles of every member, then their frame tiles, in one launch
@@ -98,28 +109,33 @@ struct fdfk_batch {
     int norder = 3;
     hipStream_t stream = nullptr;
     std::vector<fdfk_solver *> members;
+    // pacing: the host runs at most R steps ahead of the GPU, so that a
+    // solver added to the batch joins within a few steps
     static constexpr int R = 4;
-    void *dtab[R] = {}, *htab[R] = {};
     hipEvent_t ev[R] = {};
     int slot = 0;
+    // job tables of the current active set, double-buffered: a table is
+    // rebuilt (and copied once) only when the active set changes; tev[t]
+    // marks the last step that read table t
+    static constexpr int TS = 2;
+    void *dtab[TS] = {}, *htab[TS] = {};
+    hipEvent_t tev[TS] = {};
+    int tslot = 0;
     size_t tabbytes = 0;
+    std::vector<fdfk_solver *> tact;
+    int tn = 0, dstep = 0, nprep = 0, nstepb = 0;
+    size_t tsh = 0;
     bool prof = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_int, ev_step;
-    // Job templates of the current active set: everything a step's job holds
-    // except the fields that change with the iteration, rebuilt only when
-    // the active set changes (tile width, tilings, frame tiles, coefficients)
-    std::vector<fdfk_solver *> tact;
-    std::vector<char> tjobs;        // BatchJob<norder> per active member
-    std::vector<int> tnb;           // KB_N block counts per active member
-    int tlx = 0;
-    size_t tsh = 0;
     ~fdfk_batch()
     {
         if (stream) hipStreamSynchronize(stream);
-        for (int i = 0; i < R; ++i) {
+        for (int i = 0; i < R; ++i)
+            if (ev[i]) hipEventDestroy(ev[i]);
+        for (int i = 0; i < TS; ++i) {
             if (dtab[i]) hipFree(dtab[i]);
             if (htab[i]) hipHostFree(htab[i]);
-            if (ev[i]) hipEventDestroy(ev[i]);
+            if (tev[i]) hipEventDestroy(tev[i]);
         }
         for (auto &e : ev_int) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
         for (auto &e : ev_step) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
@@ -132,7 +148,7 @@ namespace {
 template <int NO>
 size_t batch_table_bytes()
 {
-    return kJobOff + sizeof(BatchJob<NO>) * kMaxBatch;
+    return kJobOff + 2 * sizeof(BatchJob<NO>) * kMaxBatch;
 }
 
 // fill one job's arguments for the step it is about to take; returns the
@@ -198,21 +214,60 @@ void fill_job(fdfk_solver *s, BatchJob<NO> &b, int nb[KB_N], bool &record, int l
     nb[KB_REC] = record ? (s->nrcv + kRecPerBlock - 1) / kRecPerBlock : 0;
 }
 
-// the fields of a job template that change from step to step: the level
-// pointers (they swap every step), the iteration, the recording index and
-// whether this iteration records; nb[KB_REC] follows the recording flag
+// the level pointers of a job for a step whose current level is c
 template <int NO>
-bool refresh_job(fdfk_solver *s, BatchJob<NO> &b, int nb[KB_N])
+void set_levels(const fdfk_solver *s, BatchJob<NO> &b, int c)
 {
-    b.sa = step_args(s);
-    b.ia.ux = s->ux[s->cur]; b.ia.uz = s->uz[s->cur];
-    b.ia.uxp = s->ux[1 - s->cur]; b.ia.uzp = s->uz[1 - s->cur];
-    b.ia.corr = s->bg_valid && s->it <= s->nt;
-    const bool record = (s->isis == s->it) && (s->istp <= s->nsis) && s->nrcv > 0;
-    b.ra.istp = s->istp;
-    b.ra.ux = s->ux[s->cur]; b.ra.uz = s->uz[s->cur];
-    nb[KB_REC] = record ? (s->nrcv + kRecPerBlock - 1) / kRecPerBlock : 0;
-    return record;
+    b.sa.ux = s->ux[c]; b.sa.uz = s->uz[c];
+    b.sa.uxp = s->ux[1 - c]; b.sa.uzp = s->uz[1 - c];
+    b.sa.st = s->d_strips[c];
+    b.ia.ux = s->ux[c]; b.ia.uz = s->uz[c];
+    b.ia.uxp = s->ux[1 - c]; b.ia.uzp = s->uz[1 - c];
+    b.ra.ux = s->ux[c]; b.ra.uz = s->uz[c];
+}
+
+// build and upload the job tables of a new active set
+template <int NO>
+int build_tables(fdfk_batch *B, const std::vector<fdfk_solver *> &act)
+{
+    constexpr size_t JB = sizeof(BatchJob<NO>);
+    // last step on the old table (recorded even when add/remove cleared tact:
+    // steps in flight may still read it)
+    HIPCHK(hipEventRecord(B->tev[B->tslot], B->stream));
+    B->tslot = (B->tslot + 1) % fdfk_batch::TS;
+    HIPCHK(hipEventSynchronize(B->tev[B->tslot]));   // its previous user is done
+    const int n = (int)act.size();
+    auto *P = (BatchPrefix *)B->htab[B->tslot];
+    auto *J0 = (BatchJob<NO> *)((char *)B->htab[B->tslot] + kJobOff);
+    auto *J1 = J0 + n;
+    P->n = n;
+    for (int k = 0; k < KB_N; ++k) P->pre[k][0] = 0;
+    const int lx = batch_lx<NO>(act);
+    size_t sh = sizeof(typename IntKernel2<NO, 256>::Smem);
+    for (int j = 0; j < n; ++j) {
+        fdfk_solver *s = act[j];
+        int nb[KB_N];
+        bool r = false;
+        fill_job<NO>(s, J0[j], nb, r, lx);
+        const bool rec = s->nrcv > 0 && s->istp <= s->nsis;
+        J0[j].rec_first = rec ? s->isis - s->it : -1;
+        J0[j].nstp = s->nstp;
+        J0[j].rec_left = rec ? s->nsis - s->istp + 1 : 0;
+        nb[KB_REC] = rec ? (s->nrcv + kRecPerBlock - 1) / kRecPerBlock : 0;
+        J1[j] = J0[j];
+        set_levels<NO>(s, J1[j], 1 - s->cur);
+        sh = std::max(sh, frame_lds_bytes(J0[j].ft, NO));
+        for (int k = 0; k < KB_N; ++k) P->pre[k][j + 1] = P->pre[k][j] + nb[k];
+    }
+    HIPCHK(hipMemcpyAsync(B->dtab[B->tslot], B->htab[B->tslot], kJobOff + 2 * JB * n, hipMemcpyHostToDevice,
+                          B->stream));
+    B->tact = act;
+    B->tn = n;
+    B->dstep = 0;
+    B->tsh = sh;
+    B->nprep = P->pre[KB_RING][n] + P->pre[KB_PSI][n] + P->pre[KB_REC][n];
+    B->nstepb = P->pre[KB_INT][n] + P->pre[KB_FRAME][n];
+    return 0;
 }
 
 template <int NO>
@@ -222,52 +277,20 @@ int batch_step(fdfk_batch *B)
     for (auto *s : B->members)
         if (s->it <= s->nt) act.push_back(s);
     if (act.empty()) return 0;
-    constexpr size_t JB = sizeof(BatchJob<NO>);
-    if (act != B->tact) {
-        B->tlx = batch_lx<NO>(act);
-        B->tjobs.assign(JB * act.size(), 0);
-        B->tnb.assign((size_t)KB_N * act.size(), 0);
-        size_t sh = sizeof(typename IntKernel2<NO, 256>::Smem);
-        for (size_t j = 0; j < act.size(); ++j) {
-            auto &t = ((BatchJob<NO> *)B->tjobs.data())[j];
-            bool r = false;
-            fill_job<NO>(act[j], t, &B->tnb[(size_t)KB_N * j], r, B->tlx);
-            sh = std::max(sh, frame_lds_bytes(t.ft, NO));
-        }
-        B->tsh = sh;
-        B->tact = act;
-    }
+    if (act != B->tact)
+        if (int rc = build_tables<NO>(B, act)) return rc;
     const int slot = B->slot;
-    HIPCHK(hipEventSynchronize(B->ev[slot]));   // the table slot is free again
-    auto *P = (BatchPrefix *)B->htab[slot];
-    auto *J = (BatchJob<NO> *)((char *)B->htab[slot] + kJobOff);
-    P->n = (int)act.size();
-    for (int k = 0; k < KB_N; ++k) P->pre[k][0] = 0;
-    std::vector<char> rec(act.size());
-    for (size_t j = 0; j < act.size(); ++j) {
-        std::memcpy((void *)&J[j], B->tjobs.data() + JB * j, JB);
-        int nb[KB_N];
-        std::memcpy(nb, &B->tnb[(size_t)KB_N * j], sizeof(nb));
-        rec[j] = refresh_job<NO>(act[j], J[j], nb);
-        for (int k = 0; k < KB_N; ++k) P->pre[k][j + 1] = P->pre[k][j] + nb[k];
-    }
-    // the prefix block and the used jobs: one copy
-    auto *dP = (BatchPrefix *)B->dtab[slot];
-    auto *dJ = (BatchJob<NO> *)((char *)B->dtab[slot] + kJobOff);
-    HIPCHK(hipMemcpyAsync(B->dtab[slot], B->htab[slot], kJobOff + JB * act.size(), hipMemcpyHostToDevice,
-                          B->stream));
-    const int n = P->n;
+    HIPCHK(hipEventSynchronize(B->ev[slot]));   // at most R steps ahead of the GPU
+    auto *dP = (const BatchPrefix *)B->dtab[B->tslot];
+    auto *dJ = (const BatchJob<NO> *)((char *)B->dtab[B->tslot] + kJobOff) + (B->dstep & 1) * B->tn;
     hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
     if (B->prof) {
         hipEventCreate(&e0); hipEventCreate(&e1); hipEventCreate(&e2); hipEventCreate(&e3);
         hipEventRecord(e0, B->stream);
     }
-    const int nprep = P->pre[KB_RING][n] + P->pre[KB_PSI][n] + P->pre[KB_REC][n];
-    if (nprep) hipLaunchKernelGGL(kb_prep<NO>, dim3(nprep), dim3(256), 0, B->stream, dJ, dP);
+    if (B->nprep) hipLaunchKernelGGL(kb_prep<NO>, dim3(B->nprep), dim3(256), 0, B->stream, dJ, dP, B->dstep);
     if (B->prof) hipEventRecord(e1, B->stream);
-    const size_t sh = B->tsh;
-    const int nstep = P->pre[KB_INT][n] + P->pre[KB_FRAME][n];
-    if (nstep) hipLaunchKernelGGL(kb_step<NO>, dim3(nstep), dim3(256), sh, B->stream, dJ, dP);
+    if (B->nstepb) hipLaunchKernelGGL(kb_step<NO>, dim3(B->nstepb), dim3(256), B->tsh, B->stream, dJ, dP);
     if (B->prof) hipEventRecord(e2, B->stream);
     if (B->prof) {
         hipEventRecord(e3, B->stream);
@@ -278,9 +301,10 @@ int batch_step(fdfk_batch *B)
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(FDFK_EDEVICE, std::string("batch launch: ") + hipGetErrorString(e));
     B->slot = (slot + 1) % fdfk_batch::R;
-    for (size_t j = 0; j < act.size(); ++j) {
-        fdfk_solver *s = act[j];
-        if (rec[j]) { s->istp += 1; s->isis += s->nstp; }
+    B->dstep += 1;
+    // host copies of the members' counters advance as the device tables assume
+    for (fdfk_solver *s : act) {
+        if ((s->isis == s->it) && (s->istp <= s->nsis) && s->nrcv > 0) { s->istp += 1; s->isis += s->nstp; }
         s->cur = 1 - s->cur;
         s->it += 1;
     }
@@ -301,13 +325,20 @@ int fdfk_batch_create(int32_t norder, fdfk_batch **out)
     with_order(norder, [&](auto no) { B->tabbytes = batch_table_bytes<decltype(no)::value>(); });
     if (hipStreamCreate(&B->stream) != hipSuccess) { delete B; return fail(FDFK_EDEVICE, "hipStreamCreate"); }
     for (int i = 0; i < fdfk_batch::R; ++i) {
+        if (hipEventCreateWithFlags(&B->ev[i], hipEventDisableTiming) != hipSuccess) {
+            delete B;
+            return fail(FDFK_EDEVICE, "batch event");
+        }
+        hipEventRecord(B->ev[i], B->stream);
+    }
+    for (int i = 0; i < fdfk_batch::TS; ++i) {
         if (hipMalloc(&B->dtab[i], B->tabbytes) != hipSuccess ||
             hipHostMalloc(&B->htab[i], B->tabbytes, hipHostMallocDefault) != hipSuccess ||
-            hipEventCreate(&B->ev[i]) != hipSuccess) {
+            hipEventCreateWithFlags(&B->tev[i], hipEventDisableTiming) != hipSuccess) {
             delete B;
             return fail(FDFK_ENOMEM, "batch table allocation");
         }
-        hipEventRecord(B->ev[i], B->stream);
+        hipEventRecord(B->tev[i], B->stream);
     }
     *out = B;
     return 0;

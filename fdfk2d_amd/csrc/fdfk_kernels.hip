@@ -228,7 +228,7 @@ __device__ double dirac_w(double x)
 // inner) with FP64 products rounded to FP32 after every term, so the sum is
 // the reference's bit for bit.
 constexpr int kRecPerBlock = 4;   // receivers per block in the fused prep kernel
-__device__ __forceinline__ void record_body(const RecArgs &r, long bid)
+__device__ __forceinline__ void record_body(const RecArgs &r, long bid, int istp)
 {
 #pragma clang fp contract(off)
     __shared__ double wx[kRecPerBlock][64], wz[kRecPerBlock][64];
@@ -264,8 +264,8 @@ __device__ __forceinline__ void record_body(const RecArgs &r, long bid)
             sx = (float)((double)sx + wx[w][k]);
             sz = (float)((double)sz + wz[w][k]);
         }
-        r.seisx[(size_t)i * r.nsamp + r.istp] = sx;
-        r.seisz[(size_t)i * r.nsamp + r.istp] = sz;
+        r.seisx[(size_t)i * r.nsamp + istp] = sx;
+        r.seisz[(size_t)i * r.nsamp + istp] = sz;
     }
 }
 // The three small per-step kernels in one launch (they are independent:
@@ -279,7 +279,7 @@ __global__ __launch_bounds__(256) void k_prep(StepArgs a, RecArgs r, const Backg
     if (b < nring) { ring_prep_body(a.g, bg, a.it, (float *)a.ringx, (float *)a.ringz, b); return; }
     b -= nring;
     if (b < npsi) { psi_body<NO>(a, b); return; }
-    record_body(r, b - npsi);
+    record_body(r, b - npsi, r.istp);
 }
 
 #pragma clang fp contract(on)

@@ -16,25 +16,29 @@ def _reqs():
             dict(id=2, arrival=0.0, nx=240, nz=80, dx=50.0, nsteps=200, src_is_p=True, p_skm=0.075, seed=3, npml=20)]
 
 
-def _solver(req):
+def _solver(req, nstp=1):
     job = serve.build_job(req)
     s = Solver(req["nx"], req["nz"], req["npml"], 3, req["dx"], req["dx"], job["dt"], job["nt"], job["c"],
-               rx=job["rx"], rz=job["rz"])
+               rx=job["rx"], rz=job["rz"], nstp=nstp)
     s.compute_background(job["fk"], job["fk"], req["src_is_p"], job["f0"], job["ds"], job["rayp"], job["bazs"])
     return s
 
 
-def test_batch_equals_alone_with_staggered_joins():
-    print("case: 3 jobs alone vs batched, staggered joins", flush=True)
+@pytest.mark.parametrize("nstp", [1, 3])
+def test_batch_equals_alone_with_staggered_joins(nstp):
+    """Joins after 37 and 87 batched steps (not multiples of nstp): the
+    batch tables are rebuilt at each join and each finish, with the
+    recording phase of every member carried across."""
+    print(f"case: 3 jobs alone vs batched, staggered joins, nstp {nstp}", flush=True)
     reqs = _reqs()
     alone = []
     for r in reqs:
-        s = _solver(r)
+        s = _solver(r, nstp)
         s.run(s.nt + 1)
         alone.append(s.seismograms())
         s.close()
     b = Batch(3)
-    sol = [_solver(r) for r in reqs]
+    sol = [_solver(r, nstp) for r in reqs]
     b.add(sol[0])
     b.run(37)
     b.add(sol[1])
