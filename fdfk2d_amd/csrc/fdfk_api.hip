@@ -46,6 +46,49 @@ static hipError_t h2d(hipStream_t st, void *dst, const void *src, size_t bytes)
     return e == hipSuccess ? hipStreamSynchronize(st) : e;
 }
 
+// Device blocks of destroyed solvers, kept for later solvers (per device,
+// by size).  hipFree waits for the whole device and holds up other threads'
+// launches meanwhile, so freeing a solver while others run (a batch, a
+// server's requests) would stall them all; a destroyed solver's blocks go
+// here instead, once its stream (ordered after any batch steps that used
+// them) is idle.  A request takes the smallest cached block of at least its
+// size and at most twice it, else allocates.  fdfk_trim_memory frees the cache.
+struct BlockCache {
+    std::mutex mu;
+    std::map<int, std::multimap<size_t, void *>> free;   // device -> size -> block
+};
+static BlockCache &block_cache()
+{
+    static BlockCache *c = new BlockCache();   // outlives static teardown
+    return *c;
+}
+static hipError_t block_get(size_t bytes, void **p, size_t *got)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    {
+        BlockCache &c = block_cache();
+        std::lock_guard<std::mutex> lk(c.mu);
+        auto &m = c.free[dev];
+        auto it = m.lower_bound(bytes);
+        if (it != m.end() && it->first <= 2 * bytes) {
+            *p = it->second;
+            *got = it->first;
+            m.erase(it);
+            return hipSuccess;
+        }
+    }
+    *got = bytes;
+    return hipMalloc(p, bytes);
+}
+static void block_put(int dev, void *p, size_t bytes)
+{
+    BlockCache &c = block_cache();
+    std::lock_guard<std::mutex> lk(c.mu);
+    c.free[dev].emplace(bytes, p);
+}
+
 // FD coefficients: closed forms of src/Submain.f90:1111-1198 (fd_coef0:
 // staggered first derivative, fd_coef1: collocated first derivative)
 static void coef_table(int norder, bool staggered, double *c)
@@ -76,7 +119,8 @@ struct fdfk_solver {
     int npower = 2;
     hipStream_t stream = nullptr;
     hipEvent_t ev_order = nullptr;   // orders this stream against a batch's (join, leave)
-    std::vector<void *> allocs;
+    std::vector<std::pair<void *, size_t>> allocs;   // device blocks and their sizes
+    int device = 0;
     int64_t bytes = 0;
     // fields
     float *ux[2] = {nullptr, nullptr}, *uz[2] = {nullptr, nullptr};
@@ -114,9 +158,10 @@ struct fdfk_solver {
     {
         void *q = nullptr;
         if (n == 0) n = 1;
-        hipError_t e = hipMalloc(&q, n * sizeof(T));
+        size_t got = 0;
+        hipError_t e = block_get(n * sizeof(T), &q, &got);
         if (e != hipSuccess) return fail(FDFK_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
-        allocs.push_back(q);
+        allocs.emplace_back(q, got);
         bytes += (int64_t)(n * sizeof(T));
         *p = (T *)q;
         return 0;
@@ -127,7 +172,7 @@ struct fdfk_solver {
         if (ev_order) hipEventDestroy(ev_order);
         for (auto &e : ev_int) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
         for (auto &e : ev_step) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
-        for (void *p : allocs) hipFree(p);
+        for (auto &a : allocs) block_put(device, a.first, a.second);   // stream idle: reusable
         if (stream) hipStreamDestroy(stream);
     }
 };
@@ -528,6 +573,20 @@ extern "C" {
 
 const char *fdfk_last_error(void) { return g_err.c_str(); }
 
+int fdfk_trim_memory(void)
+{
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    std::multimap<size_t, void *> blocks;
+    {
+        BlockCache &c = block_cache();
+        std::lock_guard<std::mutex> lk(c.mu);
+        blocks.swap(c.free[dev]);
+    }
+    for (auto &b : blocks) HIPCHK(hipFree(b.second));
+    return FDFK_OK;
+}
+
 int fdfk_create(const fdfk_job *job, fdfk_solver **out)
 {
     if (!out) return fail(FDFK_EINVAL, "null output pointer");
@@ -570,7 +629,8 @@ int fdfk_create(const fdfk_job *job, fdfk_solver **out)
         s->hcoef.c1f[i] = (float)s->hcoef.c1[i];
     }
     auto bad = [&](int code) { delete s; return code; };
-    if (hipStreamCreate(&s->stream) != hipSuccess) return bad(fail(FDFK_EDEVICE, "hipStreamCreate failed"));
+    if (hipGetDevice(&s->device) != hipSuccess || hipStreamCreate(&s->stream) != hipSuccess)
+        return bad(fail(FDFK_EDEVICE, "hipStreamCreate failed"));
     const size_t nf = (size_t)g.NX * g.ld;
     if ((rc = s->dalloc(&s->ux[0], nf)) || (rc = s->dalloc(&s->ux[1], nf)) ||
         (rc = s->dalloc(&s->uz[0], nf)) || (rc = s->dalloc(&s->uz[1], nf)) ||

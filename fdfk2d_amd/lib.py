@@ -88,6 +88,11 @@ def _check(rc):
         raise FdfkError(f"fdfk error {rc}: {lib().fdfk_last_error().decode()}")
 
 
+def trim_memory():
+    """Give device memory that no solver uses back to the driver (fdfk_trim_memory)."""
+    _check(lib().fdfk_trim_memory())
+
+
 def _ptr(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
 

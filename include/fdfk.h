@@ -99,6 +99,11 @@ int fdfk_kernel_stats(fdfk_solver *s, double *interior_ms_total, int64_t *interi
 int64_t fdfk_device_bytes(const fdfk_solver *s);
 void fdfk_destroy(fdfk_solver *s);
 const char *fdfk_last_error(void);
+/* fdfk_destroy keeps a solver's device blocks for later solvers instead of
+ * freeing them (a free waits for the whole device, which would stall other
+ * solvers and batches running meanwhile); this frees the kept blocks of the
+ * current device.  No counterpart in the reference. */
+int fdfk_trim_memory(void);
 
 /* --- batched execution: many resident solvers advanced together with one
  * launch per kernel type per step (tile work-list over jobs).  Members keep

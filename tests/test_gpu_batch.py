@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 from fdfk2d_amd import serve
-from fdfk2d_amd.lib import Batch, Solver
+from fdfk2d_amd.lib import Batch, Solver, trim_memory
 
 pytestmark = pytest.mark.gpu
 
@@ -62,6 +62,7 @@ def test_server_results_equal_plain_path():
     srv = serve.Server(hbm_budget_gb=50.0, chunk=40)
     lat, total, res = srv.run(reqs, keep_results=True)
     srv.close()
+    trim_memory()           # every solver is closed: the pool's memory goes back
     lat_p, total_p, res_p = serve.serve_plain(reqs, keep_results=True)
     assert sorted(res) == sorted(res_p) == list(range(6))
     for i in res:
