@@ -16,6 +16,8 @@
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
+#include <array>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <string>
@@ -139,6 +141,11 @@ struct fdfk_solver {
     float *bg[12] = {};
     long bg_n[12] = {};
     bool bg_valid = false;
+    // background storage: step it at slab it % bg_period; with bg_block > 0
+    // the arrays hold two blocks and bg_fill(stream, t0, t1) synthesises
+    // steps [t0, t1) from the kept FK traces
+    int bg_period = 0, bg_block = 0;
+    std::function<int(hipStream_t, int, int)> bg_fill;
     Background *d_bg = nullptr;
     float *ringx = nullptr, *ringz = nullptr;
     // receivers
@@ -504,9 +511,21 @@ void launch_frame(fdfk_solver *s, const StepArgs &a)
     });
 }
 
+// Blocked background: entering block j >= 1 (iteration jB), synthesise
+// block j+1 into the half of the ring that held block j-1, whose last step
+// is already queued before this point of the stream.
+int bg_refill(fdfk_solver *s, hipStream_t stream)
+{
+    const int B = s->bg_block;
+    if (B <= 0 || !s->bg_fill || s->it < B || s->it % B != 0) return 0;
+    const int t0 = s->it + B, t1 = std::min(s->it + 2 * B, s->nt + 1);
+    return t0 < t1 ? s->bg_fill(stream, t0, t1) : 0;
+}
+
 int one_step(fdfk_solver *s, bool record)
 {
     const Geo &g = s->g;
+    if (int rc = bg_refill(s, s->stream)) return rc;
     hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
     if (s->prof) {
         hipEventCreate(&e0); hipEventCreate(&e1); hipEventCreate(&e2); hipEventCreate(&e3);
@@ -709,11 +728,13 @@ static void bg_sizes(const Geo &g, int nt, long n[12])
 
 // device storage of the twelve background arrays, their pointer table and
 // the two full-grid ring arrays
-static int alloc_background(fdfk_solver *s)
+static int alloc_background(fdfk_solver *s, int period)
 {
-    if (s->d_bg) return 0;
+    if (s->d_bg)
+        return s->bg_period == period ? 0
+                                      : fail(FDFK_EINVAL, "background storage exists with another block size");
     long n[12];
-    bg_sizes(s->g, s->nt, n);
+    bg_sizes(s->g, period - 1, n);
     Background h{};
     for (int i = 0; i < 12; ++i) {
         int rc = s->dalloc(&s->bg[i], (size_t)n[i]);
@@ -722,6 +743,8 @@ static int alloc_background(fdfk_solver *s)
         h.a[i] = s->bg[i];
     }
     h.valid = 1;
+    h.period = period;
+    s->bg_period = period;
     const size_t nf = (size_t)s->g.NX * s->g.ld;
     int rc;
     if ((rc = s->dalloc(&s->d_bg, 1)) || (rc = s->dalloc(&s->ringx, nf)) || (rc = s->dalloc(&s->ringz, nf)))
@@ -738,8 +761,10 @@ int fdfk_set_background(fdfk_solver *s, const float *const bg[12])
     if (!s || !bg) return fail(FDFK_EINVAL, "null argument");
     long n[12];
     bg_sizes(s->g, s->nt, n);
-    int rc = alloc_background(s);
+    int rc = alloc_background(s, s->nt + 1);
     if (rc) return rc;
+    s->bg_block = 0;
+    s->bg_fill = nullptr;
     for (int i = 0; i < 12; ++i) {
         if (!bg[i]) return fail(FDFK_EINVAL, "null background array");
         HIPCHK(hipMemcpyAsync(s->bg[i], bg[i], (size_t)n[i] * 4, hipMemcpyHostToDevice, s->stream));
@@ -753,6 +778,8 @@ int fdfk_get_background(fdfk_solver *s, float *const bg[12])
 {
     if (!s || !bg) return fail(FDFK_EINVAL, "null argument");
     if (!s->bg_valid) return fail(FDFK_EINVAL, "no background on the device");
+    if (s->bg_period != s->nt + 1)
+        return fail(FDFK_EINVAL, "the background is synthesised in blocks; its steps are not all stored");
     for (int i = 0; i < 12; ++i)
         HIPCHK(hipMemcpyAsync(bg[i], s->bg[i], (size_t)s->bg_n[i] * 4, hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));

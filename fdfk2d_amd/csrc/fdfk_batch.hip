@@ -281,6 +281,8 @@ int batch_step(fdfk_batch *B)
         if (int rc = build_tables<NO>(B, act)) return rc;
     const int slot = B->slot;
     HIPCHK(hipEventSynchronize(B->ev[slot]));   // at most R steps ahead of the GPU
+    for (fdfk_solver *s : act)                  // members whose background is synthesised in blocks
+        if (int rc = bg_refill(s, B->stream)) return rc;
     auto *dP = (const BatchPrefix *)B->dtab[B->tslot];
     auto *dJ = (const BatchJob<NO> *)((char *)B->dtab[B->tslot] + kJobOff) + (B->dstep & 1) * B->tn;
     hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;

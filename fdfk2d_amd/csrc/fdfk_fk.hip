@@ -227,7 +227,8 @@ __device__ inline double fk_interp(const double *g, int nfft, double dtf, double
 
 struct FillArgs {
     Geo g;
-    int nt, nfft, nzq;
+    int t0, t1, period;         // steps [t0, t1), step it stored at slab it % period
+    int nfft, nzq;
     double dt, dtf, x0, dx, p, sgn, xe;
     const double *gl, *gr;      // time traces [comp][q][nfft] of the left / right model
     float *ax, *az;             // destination arrays (one strip)
@@ -239,8 +240,8 @@ __global__ void k_fk_fill(FillArgs f)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     long per = (long)f.rows * f.cols;
-    if (t >= per * (f.nt + 1)) return;
-    int it = (int)(t / per);
+    if (t >= per * (f.t1 - f.t0)) return;
+    int it = f.t0 + (int)(t / per);
     long rem = t % per;
     int ix = f.col0 + (int)(rem / f.rows), iz = f.row0 + (int)(rem % f.rows);
     double x = f.x0 + (double)(ix - 1) * f.dx;
@@ -264,7 +265,7 @@ __global__ void k_fk_fill(FillArgs f)
         bx = (1.0 - w) * lx + w * rx;
         bz = (1.0 - w) * lz + w * rz;
     }
-    size_t o = (size_t)(iz - f.row0) + (size_t)f.rows * ((size_t)(ix - f.col0) + (size_t)f.cols * it);
+    size_t o = (size_t)(iz - f.row0) + (size_t)f.rows * ((size_t)(ix - f.col0) + (size_t)f.cols * (it % f.period));
     f.ax[o] = (float)(f.sgn * bx);
     f.az[o] = (float)bz;
 }
@@ -415,30 +416,58 @@ extern "C" int fdfk_compute_background(fdfk_solver *s, const fdfk_fk *fk)
     mr.tref = t0 + (r.src_is_p ? mr.ea[mr.nl - 1].r : mr.eb[mr.nl - 1].r) * (zref - mr.zt[mr.nl - 1]);
     const bool same = fk->nlayer_left == fk->nlayer_right &&
                       std::memcmp(fk->fk_left, fk->fk_right, sizeof(float) * 4 * fk->nlayer_left) == 0;
-    int rc = alloc_background(s);
+    if (fk->block_steps < 0) return fail(FDFK_EINVAL, "block_steps must be >= 0");
+    const int B = fk->block_steps;
+    const bool blocked = B > 0 && 2L * B < (long)nt + 1;
+    const int period = blocked ? 2 * B : nt + 1;
+    int rc = alloc_background(s, period);
     if (rc) return rc;
+    // traces: transient when every step is stored at once, kept with the
+    // solver when later blocks are synthesised from them
     DevBuf gl, gr;
     const size_t ntr = (size_t)2 * nzq * r.nfft;
-    HIPCHK(gl.alloc(ntr * sizeof(double), s->stream));
-    if ((rc = fk_traces_dev(s, ml, r, zq.data(), gl.as<double>()))) return rc;
-    if (!same) {
-        HIPCHK(gr.alloc(ntr * sizeof(double), s->stream));
-        if ((rc = fk_traces_dev(s, mr, r, zq.data(), gr.as<double>()))) return rc;
+    double *tl = nullptr, *tr = nullptr;
+    if (blocked) {
+        if ((rc = s->dalloc(&tl, ntr))) return rc;
+        if (!same && (rc = s->dalloc(&tr, ntr))) return rc;
+    } else {
+        HIPCHK(gl.alloc(ntr * sizeof(double), s->stream));
+        tl = gl.as<double>();
+        if (!same) {
+            HIPCHK(gr.alloc(ntr * sizeof(double), s->stream));
+            tr = gr.as<double>();
+        }
     }
+    if ((rc = fk_traces_dev(s, ml, r, zq.data(), tl))) return rc;
+    if (!same && (rc = fk_traces_dev(s, mr, r, zq.data(), tr))) return rc;
     FillArgs f{};
-    f.g = g; f.nt = nt; f.nfft = r.nfft; f.nzq = nzq; f.dt = dt; f.dtf = r.dtf;
+    f.g = g; f.period = period; f.nfft = r.nfft; f.nzq = nzq; f.dt = dt; f.dtf = r.dtf;
     f.x0 = s->x0; f.dx = g.dx; f.p = p; f.sgn = sgn; f.xe = xe;
-    f.gl = gl.as<double>(); f.gr = same ? gl.as<double>() : gr.as<double>();
+    f.gl = tl; f.gr = same ? tl : tr;
     // strip shapes: (rows, cols, row0, col0) of the twelve arrays, FDFK2D.f90:330-355
-    const int shp[6][4] = {{nz - nbd, nbd, 1, 1}, {nz - nbd, nbd, 1, nx - nbd + 1}, {nbd, nx, nz - nbd + 1, 1},
-                           {nz, nbd, 1, 1 - nbd}, {nz, nbd, 1, nx + 1}, {nbd, nx + 2 * nbd, nz + 1, 1 - nbd}};
-    for (int k = 0; k < 6; ++k) {
-        f.strip = k;
-        f.rows = shp[k][0]; f.cols = shp[k][1]; f.row0 = shp[k][2]; f.col0 = shp[k][3];
-        f.ax = s->bg[2 * k]; f.az = s->bg[2 * k + 1];
-        long n = (long)f.rows * f.cols * (nt + 1);
-        hipLaunchKernelGGL(k_fk_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s->stream, f);
-    }
+    const std::array<std::array<int, 4>, 6> shp = {{{nz - nbd, nbd, 1, 1}, {nz - nbd, nbd, 1, nx - nbd + 1},
+                                                   {nbd, nx, nz - nbd + 1, 1}, {nz, nbd, 1, 1 - nbd},
+                                                   {nz, nbd, 1, nx + 1}, {nbd, nx + 2 * nbd, nz + 1, 1 - nbd}}};
+    std::array<float *, 12> dst;
+    for (int i = 0; i < 12; ++i) dst[i] = s->bg[i];
+    auto fill = [f, shp, dst](hipStream_t st, int t0, int t1) -> int {
+        FillArgs a = f;
+        a.t0 = t0; a.t1 = t1;
+        for (int k = 0; k < 6; ++k) {
+            a.strip = k;
+            a.rows = shp[k][0]; a.cols = shp[k][1]; a.row0 = shp[k][2]; a.col0 = shp[k][3];
+            a.ax = dst[2 * k]; a.az = dst[2 * k + 1];
+            const long n = (long)a.rows * a.cols * (t1 - t0);
+            hipLaunchKernelGGL(k_fk_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a);
+        }
+        hipError_t e = hipGetLastError();
+        return e == hipSuccess ? 0 : fail(FDFK_EDEVICE, std::string("background fill: ") + hipGetErrorString(e));
+    };
+    // the block the run is in and the next one (all steps when not blocked)
+    const int first = blocked ? (s->it / B) * B : 0;
+    if ((rc = fill(s->stream, first, std::min(first + period, nt + 1)))) return rc;
+    s->bg_block = blocked ? B : 0;
+    s->bg_fill = blocked ? std::function<int(hipStream_t, int, int)>(fill) : nullptr;
     HIPCHK(hipStreamSynchronize(s->stream));
     HIPCHK(hipGetLastError());
     s->bg_valid = true;

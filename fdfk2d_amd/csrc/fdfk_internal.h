@@ -80,9 +80,12 @@ struct Strips {
 // Twelve background arrays, reference layout and order:
 // 0 uxl_reg 1 uzl_reg 2 uxr_reg 3 uzr_reg 4 uxb_reg 5 uzb_reg
 // 6 uxl_pml 7 uzl_pml 8 uxr_pml 9 uzr_pml 10 uxb_pml 11 uzb_pml
+// the twelve background arrays; step it is stored at slab it % period
+// (period nt+1: all steps; 2B: a ring of two blocks of B steps)
 struct Background {
     const float *a[12];
     int valid;
+    int period;
 };
 
 // full-order FP32 coefficients of the fused stencils

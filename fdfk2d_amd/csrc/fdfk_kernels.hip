@@ -158,7 +158,7 @@ __device__ __forceinline__ void ring_prep_body(const Geo &g, const Background *b
     default: iz = nz + 1 + (int)(o % nbd); ix = 1 - nbd + (int)(o / nbd); break;
     }
     const int which = k / 3, arr = k % 3;
-    const long src = o + n[k] * (long)it;
+    const long src = o + n[k] * (long)(it % bg->period);
     const size_t c = cell(g, iz, ix);
     ringx[c] = bg->a[6 * which + 2 * arr][src];
     ringz[c] = bg->a[6 * which + 2 * arr + 1][src];

@@ -31,7 +31,7 @@ class FdfkFK(C.Structure):
         ("nlayer_left", C.c_int32), ("nlayer_right", C.c_int32),
         ("fk_left", C.c_void_p), ("fk_right", C.c_void_p),
         ("src_is_p", C.c_int32), ("ds", C.c_float), ("f0", C.c_float),
-        ("rayp", C.c_double), ("bazs", C.c_void_p),
+        ("rayp", C.c_double), ("bazs", C.c_void_p), ("block_steps", C.c_int32),
     ]
 
 
@@ -143,13 +143,15 @@ class Solver:
         ptrs = (C.c_void_p * 12)(*[a.ctypes.data for a in arrs])
         _check(lib().fdfk_set_background(self.h, ptrs))
 
-    def compute_background(self, fk_left, fk_right, src_is_p, f0, ds, rayp, bazs):
+    def compute_background(self, fk_left, fk_right, src_is_p, f0, ds, rayp, bazs, block_steps=0):
+        """FK background on the GPU; block_steps > 0 synthesises it in blocks
+        of that many steps as the run advances instead of storing all steps."""
         fl = np.ascontiguousarray(fk_left, np.float32)
         fr = np.ascontiguousarray(fk_right, np.float32)
         bz = np.ascontiguousarray(bazs, np.float64)
         fk = FdfkFK(nlayer_left=fl.size // 4, nlayer_right=fr.size // 4, fk_left=_ptr(fl),
                     fk_right=_ptr(fr), src_is_p=1 if src_is_p else 0, ds=ds, f0=f0, rayp=rayp,
-                    bazs=_ptr(bz))
+                    bazs=_ptr(bz), block_steps=int(block_steps))
         _check(lib().fdfk_compute_background(self.h, C.byref(fk)))
 
     def get_background(self):

@@ -63,6 +63,13 @@ typedef struct fdfk_fk {
     float ds, f0;                    /* source strength (m) and frequency (Hz) */
     double rayp;                     /* s/m (|rayp| >= 1e-9) */
     const double *bazs;              /* back-azimuth per padded column (rad), NX values */
+    /* 0: keep the background of all nt+1 steps (the reference's twelve
+     * arrays).  B > 0: keep the FP64 time traces and 2B steps of the arrays,
+     * synthesising each next block of B steps in stream order as the run
+     * advances (same values; the background of a 20001-step solve shrinks
+     * from 17.9 GiB to the traces).  Not in the reference (it stores all
+     * steps, src/FDFK2D.f90:320-355). */
+    int32_t block_steps;
 } fdfk_fk;
 
 /* --- handle API: state stays resident on the GPU between calls --- */
@@ -73,7 +80,8 @@ int fdfk_set_background(fdfk_solver *s, const float *const bg[12]);
 /* compute the FK background on the GPU (FP64), replaces
  * comput_background_wavefield (src/FDFK2D.f90:432-434) */
 int fdfk_compute_background(fdfk_solver *s, const fdfk_fk *fk);
-/* copy the device background back in the reference layout (twelve arrays) */
+/* copy the device background back in the reference layout (twelve arrays);
+ * FDFK_EINVAL when it is synthesised in blocks */
 int fdfk_get_background(fdfk_solver *s, float *const bg[12]);
 /* advance nsteps iterations of the loop src/FDFK2D.f90:465-532 (record,
  * then updatedispl); recording stays on the device */

@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 from fdfk2d_amd import serve
-from fdfk2d_amd.lib import Batch, Solver, trim_memory
+from fdfk2d_amd.lib import Batch, FdfkError, Solver, trim_memory
 
 pytestmark = pytest.mark.gpu
 
@@ -16,20 +16,23 @@ def _reqs():
             dict(id=2, arrival=0.0, nx=240, nz=80, dx=50.0, nsteps=200, src_is_p=True, p_skm=0.075, seed=3, npml=20)]
 
 
-def _solver(req, nstp=1):
+def _solver(req, nstp=1, block=0):
     job = serve.build_job(req)
     s = Solver(req["nx"], req["nz"], req["npml"], 3, req["dx"], req["dx"], job["dt"], job["nt"], job["c"],
                rx=job["rx"], rz=job["rz"], nstp=nstp)
-    s.compute_background(job["fk"], job["fk"], req["src_is_p"], job["f0"], job["ds"], job["rayp"], job["bazs"])
+    s.compute_background(job["fk"], job["fk"], req["src_is_p"], job["f0"], job["ds"], job["rayp"], job["bazs"],
+                         block_steps=block)
     return s
 
 
-@pytest.mark.parametrize("nstp", [1, 3])
-def test_batch_equals_alone_with_staggered_joins(nstp):
+@pytest.mark.parametrize("nstp,block", [(1, 0), (3, 0), (1, 32), (3, 32)])
+def test_batch_equals_alone_with_staggered_joins(nstp, block):
     """Joins after 37 and 87 batched steps (not multiples of nstp): the
     batch tables are rebuilt at each join and each finish, with the
-    recording phase of every member carried across."""
-    print(f"case: 3 jobs alone vs batched, staggered joins, nstp {nstp}", flush=True)
+    recording phase of every member carried across.  With block > 0 the
+    members' backgrounds are synthesised in blocks on the batch's stream;
+    the reference runs alone store every step."""
+    print(f"case: 3 jobs alone vs batched, staggered joins, nstp {nstp}, block {block}", flush=True)
     reqs = _reqs()
     alone = []
     for r in reqs:
@@ -38,7 +41,7 @@ def test_batch_equals_alone_with_staggered_joins(nstp):
         alone.append(s.seismograms())
         s.close()
     b = Batch(3)
-    sol = [_solver(r, nstp) for r in reqs]
+    sol = [_solver(r, nstp, block) for r in reqs]
     b.add(sol[0])
     b.run(37)
     b.add(sol[1])
@@ -68,3 +71,27 @@ def test_server_results_equal_plain_path():
     for i in res:
         assert np.array_equal(res[i][0], res_p[i][0]) and np.array_equal(res[i][1], res_p[i][1])
     print("served", serve.summarize(lat, total, 6), "plain", serve.summarize(lat_p, total_p, 6), flush=True)
+
+
+@pytest.mark.parametrize("block", [1, 37, 64])
+def test_blocked_background_equals_stored(block):
+    """Background synthesised in blocks of `block` steps (a ring of two
+    blocks, refilled as the run advances; 37 does not divide the 240
+    iterations) against all steps stored: the same seismograms, bit for bit,
+    also when the run is advanced in uneven chunks."""
+    print(f"case: blocked background, block {block}, 181x61, 240 iterations", flush=True)
+    req = _reqs()[0]
+    full = _solver(req)
+    full.run(full.nt + 1)
+    want = full.seismograms()
+    full.close()
+    s = _solver(req, block=block)
+    s.run(50)
+    s.run(101)
+    s.run(s.nt + 1 - 151)
+    got = s.seismograms()
+    assert np.abs(want[1]).max() > 0
+    assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
+    with pytest.raises(FdfkError, match="blocks"):
+        s.get_background()
+    s.close()
