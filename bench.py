@@ -52,19 +52,26 @@ def step_alg_bytes(nx, nz, npml, norder):
     return 36 * tf + (28 + 72 + 8) * side + (28 + 72 + 16) * bottom, tf + side + bottom
 
 
-def gpu_bench(steps, warmup, workload):
+def gpu_bench(steps, warmup, workload, bg_block=0, nt=0):
+    """nt: iterations the solver (and its background) is set up for, at
+    least the ones run; bg_block > 0 synthesises the background in blocks of
+    that many steps instead of storing all nt+1."""
     from fdfk2d_amd import configs, lib
-    nt = warmup + steps + PROFILED_STEPS + 1
+    nt = max(nt, warmup + steps + PROFILED_STEPS + 1)
     cfg = configs.c2(nt=nt) if workload == "c2" else configs.c1(nt=nt)
     s = lib.Solver(cfg["nx"], cfg["nz"], cfg["npml"], cfg["norder"], cfg["dx"], cfg["dz"], cfg["dt"],
                    nt, cfg["c"], rx=cfg["rx"], rz=cfg["rz"])
-    bg_mode = "fk-gpu"
+    bg_mode = "fk-gpu" if not bg_block else f"fk-gpu, synthesised in blocks of {bg_block} steps"
     try:
+        t_bg = time.perf_counter()
         s.compute_background(cfg["fk_left"], cfg["fk_right"], cfg["src_is_p"], cfg["f0"], cfg["ds"],
-                             cfg["rayp"], cfg["bazs"])
+                             cfg["rayp"], cfg["bazs"], block_steps=bg_block)
+        s.sync()
+        t_bg = time.perf_counter() - t_bg
     except lib.FdfkError:
         bg_mode = "zero"
         s.set_background([np.zeros(n, np.float32) for n in lib.bg_sizes(cfg["nx"], cfg["nz"], 5, nt)])
+    device_bytes = s.device_bytes
     s.run(warmup)
     s.sync()
     t0 = time.perf_counter()
@@ -84,7 +91,8 @@ def gpu_bench(steps, warmup, workload):
     s.close()
     wall = t1 - t0
     return dict(cfg=cfg, NX=NX, NZ=NZ, wall=wall, stats=st, alg_bytes=alg_bytes, alg_cells=alg_cells,
-                bg_mode=bg_mode, finite=finite)
+                bg_mode=bg_mode, finite=finite, nt=nt, device_bytes=device_bytes,
+                background_s=round(t_bg, 3) if bg_mode != "zero" else None)
 
 
 def cpu_baseline(sample_steps):
@@ -217,6 +225,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="c2", choices=["c1", "c2"])
     ap.add_argument("--cpu-steps", type=int, default=201)
+    ap.add_argument("--bg-block", type=int, default=0, metavar="B",
+                    help="synthesise the background in blocks of B steps (default: store all)")
+    ap.add_argument("--nt", type=int, default=0,
+                    help="iterations the solve is set up for (default: just the ones run)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run the copy kernel used to calibrate HBM byte counters")
@@ -238,7 +250,7 @@ def main():
         print("bench.py: this build measures one GPU; the hot path shards as x-slabs "
               "(see DESIGN.md) but multi-GPU runs are not part of it", file=sys.stderr)
         sys.exit(2)
-    r = gpu_bench(a.steps, a.warmup, a.workload)
+    r = gpu_bench(a.steps, a.warmup, a.workload, a.bg_block, a.nt)
     cells = r["NX"] * r["NZ"]
     value = cells * a.steps / r["wall"] / 1e9
     ms = r["wall"] / a.steps * 1e3
@@ -265,6 +277,9 @@ def main():
                         "401 receivers" if a.workload == "c2" else "1001x400, dx=200 m, dt=0.01 s",
             "grid_padded": [r["NX"], r["NZ"]],
             "background": r["bg_mode"],
+            "set_up_for_iterations": r["nt"],
+            "device_gib": round(r["device_bytes"] / 2**30, 3),
+            "background_s": r["background_s"],
             "solves_per_hour_at_20000_steps": round(3600.0 / (ms * 1e-3 * full_steps), 3),
             "finite": r["finite"],
         },
