@@ -161,6 +161,24 @@ class Server:
         preparing, finishing = {}, {}
         lat, results = {}, {}
         t0 = time.perf_counter()
+        try:
+            self._loop(pending, waiting, preparing, finishing, lat, results, keep_results, t0)
+        except BaseException:
+            # a failed request ends the run: leave no solver behind
+            for rid, (req, s) in list(self.active.items()):
+                self.batch.remove(s)
+                s.close()
+            self.active.clear()
+            for f in preparing:
+                if not f.cancelled() and f.exception() is None:
+                    f.result()[0].close()
+            for f in finishing:
+                f.exception()               # read-back and close ran to the end
+            raise
+        total = time.perf_counter() - t0
+        return lat, total, results
+
+    def _loop(self, pending, waiting, preparing, finishing, lat, results, keep_results, t0):
         with cf.ThreadPoolExecutor(self.workers) as pool:
             while pending or waiting or preparing or self.active or finishing:
                 now = time.perf_counter() - t0
@@ -204,8 +222,6 @@ class Server:
                             self.batch.remove(s)
                             del self.active[rid]
                             finishing[pool.submit(self._finish, s)] = req
-        total = time.perf_counter() - t0
-        return lat, total, results
 
     def close(self):
         self.batch.close()

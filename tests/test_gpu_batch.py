@@ -95,3 +95,22 @@ def test_blocked_background_equals_stored(block):
     with pytest.raises(FdfkError, match="blocks"):
         s.get_background()
     s.close()
+
+
+def test_server_failed_request_raises_and_cleans_up():
+    """A request that fdfk_create rejects on the host (a PML narrower than the
+    hybrid ring: npml 3 < 2*norder - 1 = 5) ends the run with that error;
+    no solver stays in the batch, and a new server then serves normally."""
+    print("case: serving with one invalid request, then a normal run", flush=True)
+    reqs = serve.request_mix(4, seed=9, rate=50.0, nx_range=(120, 300), nz_range=(50, 100),
+                             steps_range=(100, 200), dx_choices=(200.0,))
+    bad = dict(reqs[2], npml=3)
+    srv = serve.Server(hbm_budget_gb=50.0, chunk=40)
+    with pytest.raises(FdfkError, match="npml"):
+        srv.run(reqs[:2] + [bad] + reqs[3:])
+    assert not srv.active and srv.batch.active == 0
+    srv.close()
+    srv = serve.Server(hbm_budget_gb=50.0, chunk=40)
+    lat, total, res = srv.run(reqs, keep_results=True)
+    srv.close()
+    assert sorted(lat) == [r["id"] for r in reqs] and all(np.isfinite(res[r["id"]][1]).all() for r in reqs)
