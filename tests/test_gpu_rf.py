@@ -1,0 +1,65 @@
+"""End to end on the GPU against the reference's published P receiver
+function (images/PRF.png): the examples/test2 configuration (two layers,
+Moho at 35 km, P incidence at 4.798 s/deg, 1001 x 251 grid at 200 m, 4501
+iterations, FD order 6) written in the reference's input formats, run
+through the package driver (FK background, time loop, recording, radial
+projection), and the P-RF of a surface receiver from its seismograms.
+
+The published peaks follow from the layer's travel times, with
+eta_s = sqrt(1/Vs^2 - p^2) and eta_p = sqrt(1/Vp^2 - p^2):
+Ps = H (eta_s - eta_p) = 4.397 s, PpPs = H (eta_s + eta_p) = 15.666 s,
+PsPs = 2 H eta_s = 20.064 s with reversed polarity.  The input values are
+those of examples/test2/input (FD_model.dat, Source.dat, FK_model_*.dat,
+Receiver.dat: 401 surface receivers every 500 m) and the models of its
+create_two_layers_model.m (Vp/Vs 6000/3450 m/s above row 176, 8000/4450
+from there, 1-based).
+"""
+
+import numpy as np
+import pytest
+
+from fdfk2d_amd import driver, io, rf
+
+pytestmark = pytest.mark.gpu
+
+
+def test_published_prf_from_gpu_simulation(tmp_path, monkeypatch):
+    print("case: examples/test2 configuration, 1001 x 251, 4501 iterations, P-RF", flush=True)
+    nx, nz, npml, d = 1001, 251, 20, 200.0
+    NX, NZ = nx + 2 * npml, nz + npml
+    ih = int(round(35.0e3 / d))                 # create_two_layers_model.m: rows ih+1.. are mantle
+    vp = np.full((NX, NZ), 6000.0, np.float32)
+    vs = np.full((NX, NZ), 3450.0, np.float32)
+    vp[:, ih:] = 8000.0
+    vs[:, ih:] = 4450.0
+    layers = [(0.0, 6000.0, 3450.0), (35000.0, 8000.0, 4450.0)]
+    rx = np.arange(0.0, 200001.0, 500.0, dtype=np.float32)
+    prefix = str(tmp_path / "input")
+    io.write_inputs(prefix, order=6, x0=0.0, xn=200000.0, z0=0.0, zn=50000.0, dx=d, dz=d, dt=0.01,
+                    tmax=45.0, tstp=0.01, npml=npml, lat_org=45.0, lon_org=0.0, az_org=0.0, f0=1.0,
+                    ds=1e-3, src_is_p=True, rayp_sdeg=4.798, evla=-42.0, evlo=10.0, rx=rx,
+                    rz=np.zeros_like(rx), fk_left=layers, fk_right=layers, vp=vp, vs=vs)
+    monkeypatch.chdir(tmp_path)
+    out = tmp_path / "seismograms"
+    assert driver.main([prefix, "inpar.dat", str(out), "seis"]) == 0
+    sx, hx = io.read_su(str(out / "seisx.su"))
+    sz, _ = io.read_su(str(out / "seisz.su"))
+    assert sx.shape == (401, 4501) and np.isfinite(sx).all() and np.isfinite(sz).all()
+    k = 200                                     # the receiver at x = 100 km, far from both edges
+    r, t, _ = rf.prf(sx[k].astype(np.float64), sz[k].astype(np.float64), 0.01)
+    p = 4.798 * 180.0 / (np.pi * 6371.0e3)
+    H = 35000.0
+    es, ep = np.sqrt(1 / 3450.0 ** 2 - p * p), np.sqrt(1 / 6000.0 ** 2 - p * p)
+    want = [(0.0, 1), (H * (es - ep), 1), (H * (es + ep), 1), (2 * H * es, -1)]
+    assert abs(want[1][0] - 4.397) < 2e-3 and abs(want[2][0] - 15.666) < 2e-3
+    found = []
+    for lag, sign in want:
+        w = np.abs(t - lag) < 0.5
+        j = np.argmax(np.abs(r[w]))
+        found.append((round(lag, 3), round(float(t[w][j]), 3), float(r[w][j])))
+    print("P-RF peaks (expected lag, found lag, amplitude):", found, flush=True)
+    for lag, sign in want:
+        w = np.abs(t - lag) < 0.5
+        j = np.argmax(np.abs(r[w]))
+        assert abs(t[w][j] - lag) < 0.1, (lag, t[w][j])
+        assert np.sign(r[w][j]) == sign, (lag, r[w][j])
