@@ -23,8 +23,9 @@ from fdfk2d_amd import driver, io, rf
 pytestmark = pytest.mark.gpu
 
 
-def test_published_prf_from_gpu_simulation(tmp_path, monkeypatch):
-    print("case: examples/test2 configuration, 1001 x 251, 4501 iterations, P-RF", flush=True)
+def _run_test2(tmp_path, monkeypatch, src_is_p, rayp_sdeg):
+    """examples/test2 inputs (P at 4.798 s/deg there) through the driver;
+    returns the radial and vertical seismograms (401 x 4501)."""
     nx, nz, npml, d = 1001, 251, 20, 200.0
     NX, NZ = nx + 2 * npml, nz + npml
     ih = int(round(35.0e3 / d))                 # create_two_layers_model.m: rows ih+1.. are mantle
@@ -37,7 +38,7 @@ def test_published_prf_from_gpu_simulation(tmp_path, monkeypatch):
     prefix = str(tmp_path / "input")
     io.write_inputs(prefix, order=6, x0=0.0, xn=200000.0, z0=0.0, zn=50000.0, dx=d, dz=d, dt=0.01,
                     tmax=45.0, tstp=0.01, npml=npml, lat_org=45.0, lon_org=0.0, az_org=0.0, f0=1.0,
-                    ds=1e-3, src_is_p=True, rayp_sdeg=4.798, evla=-42.0, evlo=10.0, rx=rx,
+                    ds=1e-3, src_is_p=src_is_p, rayp_sdeg=rayp_sdeg, evla=-42.0, evlo=10.0, rx=rx,
                     rz=np.zeros_like(rx), fk_left=layers, fk_right=layers, vp=vp, vs=vs)
     monkeypatch.chdir(tmp_path)
     out = tmp_path / "seismograms"
@@ -45,6 +46,12 @@ def test_published_prf_from_gpu_simulation(tmp_path, monkeypatch):
     sx, hx = io.read_su(str(out / "seisx.su"))
     sz, _ = io.read_su(str(out / "seisz.su"))
     assert sx.shape == (401, 4501) and np.isfinite(sx).all() and np.isfinite(sz).all()
+    return sx, sz
+
+
+def test_published_prf_from_gpu_simulation(tmp_path, monkeypatch):
+    print("case: examples/test2 configuration, 1001 x 251, 4501 iterations, P-RF", flush=True)
+    sx, sz = _run_test2(tmp_path, monkeypatch, True, 4.798)
     k = 200                                     # the receiver at x = 100 km, far from both edges
     r, t, _ = rf.prf(sx[k].astype(np.float64), sz[k].astype(np.float64), 0.01)
     p = 4.798 * 180.0 / (np.pi * 6371.0e3)
@@ -63,3 +70,26 @@ def test_published_prf_from_gpu_simulation(tmp_path, monkeypatch):
         j = np.argmax(np.abs(r[w]))
         assert abs(t[w][j] - lag) < 0.1, (lag, t[w][j])
         assert np.sign(r[w][j]) == sign, (lag, r[w][j])
+
+
+def test_srf_sp_precursor_from_gpu_simulation(tmp_path, monkeypatch):
+    """SV incidence at 9 s/deg on the same model: the S-RF (vertical over
+    radial, utils/plot_SRF.m lags) of the receiver at 100 km has the direct S
+    at lag 0 and the Moho's Sp conversion ahead of it by
+    H (eta_s - eta_p) = 4.64 s.  Not pinned by a published figure; the lag is
+    fixed by the travel times alone."""
+    print("case: examples/test2 model, SV at 9 s/deg, S-RF", flush=True)
+    sx, sz = _run_test2(tmp_path, monkeypatch, False, 9.0)
+    r, t, _ = rf.srf(sx[200].astype(np.float64), sz[200].astype(np.float64), 0.01)
+    p = 9.0 * 180.0 / (np.pi * 6371.0e3)
+    es, ep = np.sqrt(1 / 3450.0 ** 2 - p * p), np.sqrt(1 / 6000.0 ** 2 - p * p)
+    lead = 35000.0 * (es - ep)
+    assert abs(lead - 4.64) < 0.01
+    j0 = np.argmax(np.abs(r))
+    w = np.abs(t + lead) < 0.5
+    j = np.argmax(np.abs(r[w]))
+    print("S-RF: main peak at %.3f s; Sp expected %.3f s, found %.3f s (amplitude %.3g of %.3g)"
+          % (t[j0], -lead, t[w][j], r[w][j], r[j0]), flush=True)
+    assert abs(t[j0]) < 0.05
+    assert abs(t[w][j] + lead) < 0.1
+    assert abs(r[w][j]) > 0.05 * abs(r[j0])     # a conversion, not noise
