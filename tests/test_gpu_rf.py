@@ -93,3 +93,33 @@ def test_srf_sp_precursor_from_gpu_simulation(tmp_path, monkeypatch):
     assert abs(t[j0]) < 0.05
     assert abs(t[w][j] + lead) < 0.1
     assert abs(r[w][j]) > 0.05 * abs(r[j0])     # a conversion, not noise
+
+
+def test_rf_parity_with_oracle():
+    """Receiver functions from the GPU seismograms against those from the
+    oracle's on the same inputs (a 121 x 60 two-layer model with a Moho at
+    8 km and a Gaussian anomaly, 2001 iterations, 7 surface receivers):
+    P-RF and S-RF of every receiver to rel-L2 1e-5, the seismograms' bar."""
+    print("case: RF parity, 121 x 60, 2001 iterations, 7 receivers", flush=True)
+    from fdfk2d_amd import lib
+    from tests._cases import Case, oracle_run, rel_l2
+    c = Case(nx=121, nz=60, npml=15, dx=200.0, dz=200.0, dt=0.01, nt=2000,
+             layers=((0.0, 6000.0, 3450.0), (8000.0, 8000.0, 4450.0)), nrcv=7)
+    bg = c.oracle_background()
+    st = c.state()
+    ox, oz = oracle_run(c, st, bg, c.nt + 1)
+    s = lib.Solver(c.nx, c.nz, c.npml, c.norder, c.dx, c.dz, c.dt, c.nt, c.c, rx=c.rx, rz=c.rz)
+    s.set_background(bg)
+    s.run(c.nt + 1)
+    gx, gz = s.seismograms()
+    s.close()
+    assert rel_l2(gx, ox) < 1e-5 and rel_l2(gz, oz) < 1e-5
+    worst = 0.0
+    for k in range(len(c.rx)):
+        for f in (rf.prf, rf.srf):
+            want, tw, _ = f(ox[k].astype(np.float64), oz[k].astype(np.float64), c.dt)
+            got, tg, _ = f(gx[k].astype(np.float64), gz[k].astype(np.float64), c.dt)
+            assert np.array_equal(tg, tw) and np.abs(want).max() > 0
+            worst = max(worst, rel_l2(got, want))
+    print("worst RF rel-L2 GPU vs oracle: %.3g" % worst, flush=True)
+    assert worst < 1e-5
