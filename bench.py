@@ -52,13 +52,17 @@ def step_alg_bytes(nx, nz, npml, norder):
     return 36 * tf + (28 + 72 + 8) * side + (28 + 72 + 16) * bottom, tf + side + bottom
 
 
-def gpu_bench(steps, warmup, workload, bg_block=0, nt=0):
+def gpu_bench(steps, warmup, workload, bg_block=0, nt=0, autotune=False):
     """nt: iterations the solver (and its background) is set up for, at
     least the ones run; bg_block > 0 synthesises the background in blocks of
     that many steps instead of storing all nt+1."""
     from fdfk2d_amd import configs, lib
     nt = max(nt, warmup + steps + PROFILED_STEPS + 1)
     cfg = configs.c2(nt=nt) if workload == "c2" else configs.c1(nt=nt)
+    tuned = None
+    if autotune:
+        tuned = lib.autotune(cfg["nx"], cfg["nz"], cfg["npml"], cfg["norder"], cfg["dx"], cfg["dz"], cfg["dt"],
+                             cfg["c"], rx=cfg["rx"], rz=cfg["rz"])
     s = lib.Solver(cfg["nx"], cfg["nz"], cfg["npml"], cfg["norder"], cfg["dx"], cfg["dz"], cfg["dt"],
                    nt, cfg["c"], rx=cfg["rx"], rz=cfg["rz"])
     bg_mode = "fk-gpu" if not bg_block else f"fk-gpu, synthesised in blocks of {bg_block} steps"
@@ -91,7 +95,7 @@ def gpu_bench(steps, warmup, workload, bg_block=0, nt=0):
     s.close()
     wall = t1 - t0
     return dict(cfg=cfg, NX=NX, NZ=NZ, wall=wall, stats=st, alg_bytes=alg_bytes, alg_cells=alg_cells,
-                bg_mode=bg_mode, finite=finite, nt=nt, device_bytes=device_bytes,
+                bg_mode=bg_mode, finite=finite, nt=nt, device_bytes=device_bytes, tuned=tuned,
                 background_s=round(t_bg, 3) if bg_mode != "zero" else None)
 
 
@@ -227,6 +231,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=201)
     ap.add_argument("--bg-block", type=int, default=0, metavar="B",
                     help="synthesise the background in blocks of B steps (default: store all)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="time interior tile widths first (fdfk_autotune) and run with the fastest")
     ap.add_argument("--nt", type=int, default=0,
                     help="iterations the solve is set up for (default: just the ones run)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -250,7 +256,7 @@ def main():
         print("bench.py: this build measures one GPU; the hot path shards as x-slabs "
               "(see DESIGN.md) but multi-GPU runs are not part of it", file=sys.stderr)
         sys.exit(2)
-    r = gpu_bench(a.steps, a.warmup, a.workload, a.bg_block, a.nt)
+    r = gpu_bench(a.steps, a.warmup, a.workload, a.bg_block, a.nt, a.autotune)
     cells = r["NX"] * r["NZ"]
     value = cells * a.steps / r["wall"] / 1e9
     ms = r["wall"] / a.steps * 1e3
@@ -280,6 +286,7 @@ def main():
             "set_up_for_iterations": r["nt"],
             "device_gib": round(r["device_bytes"] / 2**30, 3),
             "background_s": r["background_s"],
+            "tile_autotune": r["tuned"],
             "solves_per_hour_at_20000_steps": round(3600.0 / (ms * 1e-3 * full_steps), 3),
             "finite": r["finite"],
         },

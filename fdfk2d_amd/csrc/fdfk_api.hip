@@ -19,8 +19,10 @@
 #include <array>
 #include <functional>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 namespace fdfk {
@@ -154,6 +156,7 @@ struct fdfk_solver {
     // interior kernel
     int lx = 45, nt_threads = 256;
     bool fused = true;   // interior and frame in one launch (k_step)
+    int lx_model = 0;    // interior tile width of the occupancy model (lx may be tuned)
     // profiling
     bool prof = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_int, ev_step;
@@ -404,6 +407,30 @@ IntPlan plan_lx(const Geo &g, bool fused)
     return best;
 }
 
+// Interior tile widths measured by fdfk_autotune, per geometry and launch
+// form; plan_interior prefers them to the occupancy model.
+struct TunedKey {
+    int nx, nz, npml, norder, fused;
+    bool operator<(const TunedKey &o) const
+    {
+        return std::tie(nx, nz, npml, norder, fused) < std::tie(o.nx, o.nz, o.npml, o.norder, o.fused);
+    }
+};
+static std::mutex g_tuned_mu;
+static std::map<TunedKey, int> *g_tuned = new std::map<TunedKey, int>();
+static TunedKey tuned_key(const fdfk_solver *s)
+{
+    return TunedKey{s->g.nx, s->g.nz, s->g.npml, s->g.norder, s->fused ? 1 : 0};
+}
+static bool tuned_lx(const fdfk_solver *s, int *lx)
+{
+    std::lock_guard<std::mutex> lk(g_tuned_mu);
+    auto it = g_tuned->find(tuned_key(s));
+    if (it == g_tuned->end()) return false;
+    *lx = it->second;
+    return true;
+}
+
 // block height (threads along z) and column width of the interior launch;
 // FDFK_NT / FDFK_LX override them (FDFK_FUSED=0 launches the interior and
 // the frame separately)
@@ -424,7 +451,8 @@ void plan_interior(fdfk_solver *s)
     else with_order(g.norder, [&](auto no) { consider(256, plan_lx<decltype(no)::value, 256>(g, fused)); });
     s->nt_threads = nt;
     s->fused = fused && nt == 256;
-    s->lx = p.lx;
+    s->lx = s->lx_model = p.lx;
+    if (s->nt_threads == 256) tuned_lx(s, &s->lx);
     if (const char *e = std::getenv("FDFK_LX")) s->lx = std::max(4, std::atoi(e));
 }
 
@@ -591,6 +619,76 @@ int validate(const fdfk_job *j)
 extern "C" {
 
 const char *fdfk_last_error(void) { return g_err.c_str(); }
+
+static int alloc_background(fdfk_solver *s, int period);
+
+int fdfk_autotune(const fdfk_job *job, int32_t steps, int32_t *best_lx, float *best_ms, int32_t *model_lx,
+                  float *model_ms)
+{
+    if (!job || steps < 1) return fail(FDFK_EINVAL, "job required and steps >= 1");
+    // scratch solver of the same geometry, set up for every timed step
+    std::vector<int> cand;
+    fdfk_job j = *job;
+    j.nt = 2 * 9 * (steps + 3) + 64;
+    j.nstp = 1;
+    fdfk_solver *s = nullptr;
+    if (int rc = fdfk_create(&j, &s)) return rc;
+    std::unique_ptr<fdfk_solver> own(s);
+    // a zero background of two steps: the hybrid-ring work runs as in a real solve
+    if (int rc = alloc_background(s, 2)) return rc;
+    for (int i = 0; i < 12; ++i) HIPCHK(hipMemsetAsync(s->bg[i], 0, (size_t)s->bg_n[i] * 4, s->stream));
+    s->bg_valid = true;
+    const int ncols = s->g.ix1 - s->g.ix0 + 1;
+    const int m = s->lx_model;
+    for (double f : {1.0, 0.5, 0.67, 0.8, 0.9, 1.1, 1.25, 1.5, 2.0}) {
+        const int c = std::min(ncols, std::max(4, (int)std::lround(m * f)));
+        if (std::find(cand.begin(), cand.end(), c) == cand.end()) cand.push_back(c);
+    }
+    // warm the GPU to its steady clocks at the model's width, then time every
+    // candidate in two passes of opposite order and keep its faster pass
+    s->lx = m;
+    if (int rc = fdfk_run(s, 50)) return rc;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    std::vector<float> best(cand.size(), 0.0f);
+    int rc = 0;
+    for (int pass = 0; pass < 2 && rc == 0; ++pass)
+        for (size_t q = 0; q < cand.size() && rc == 0; ++q) {
+            const size_t k = pass == 0 ? q : cand.size() - 1 - q;
+            s->lx = cand[k];
+            if ((rc = fdfk_run(s, 3))) break;
+            if (hipEventRecord(e0, s->stream) != hipSuccess) { rc = fail(FDFK_EDEVICE, "event"); break; }
+            if ((rc = fdfk_run(s, steps))) break;
+            float ms = 0.0f;
+            if (hipEventRecord(e1, s->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                rc = fail(FDFK_EDEVICE, "event timing");
+                break;
+            }
+            ms /= (float)steps;
+            if (pass == 0 || ms < best[k]) best[k] = ms;
+        }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc) return rc;
+    // leave the model's width (candidate 0) only for a gain of 1% or more
+    size_t bk = 0;
+    for (size_t k = 1; k < cand.size(); ++k)
+        if (best[k] < best[bk]) bk = k;
+    if (best[bk] > 0.99f * best[0]) bk = 0;
+    const int blx = cand[bk];
+    const float bms = best[bk], mms = best[0];
+    {
+        std::lock_guard<std::mutex> lk(g_tuned_mu);
+        (*g_tuned)[tuned_key(s)] = blx;
+    }
+    if (best_lx) *best_lx = blx;
+    if (best_ms) *best_ms = bms;
+    if (model_lx) *model_lx = m;
+    if (model_ms) *model_ms = mms;
+    return FDFK_OK;
+}
 
 int fdfk_trim_memory(void)
 {

@@ -69,6 +69,8 @@ def lib():
             getattr(L, name).argtypes = [C.c_void_p]
         L.fdfk_batch_destroy.restype = None
         L.fdfk_batch_kernel_stats.argtypes = L.fdfk_kernel_stats.argtypes
+        L.fdfk_autotune.argtypes = [C.POINTER(FdfkJob), C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_float),
+                                    C.POINTER(C.c_int32), C.POINTER(C.c_float)]
         L.fdfk_smooth_model.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_float,
                                         C.c_float, C.c_float, C.c_float, C.c_int32, C.c_int32,
                                         C.c_int32, C.c_int32, C.c_int32]
@@ -103,6 +105,22 @@ def bg_sizes(nx, nz, nbd, nt):
     reg_lr, reg_b = (nz - nbd) * nbd * T, nbd * nx * T
     pml_lr, pml_b = nz * nbd * T, nbd * (nx + 2 * nbd) * T
     return [reg_lr] * 4 + [reg_b] * 2 + [pml_lr] * 4 + [pml_b] * 2
+
+
+def autotune(nx, nz, npml, norder, dx, dz, dt, c, x0=0.0, z0=0.0, rx=None, rz=None, R0=1e-3, npower=2,
+             vpml=0.0, steps=20):
+    """Time interior tile widths for this geometry (fdfk_autotune); later
+    Solvers of the same geometry use the fastest.  Returns the chosen and the
+    occupancy model's width with their ms per step."""
+    c = np.ascontiguousarray(c, np.float32)
+    rx = None if rx is None else np.ascontiguousarray(rx, np.float32)
+    rz = None if rz is None else np.ascontiguousarray(rz, np.float32)
+    job = FdfkJob(nx=nx, nz=nz, npml=npml, norder=norder, is_pml_top=0, npower=npower, x0=x0, z0=z0, dx=dx,
+                  dz=dz, dt=dt, R0=R0, vpml=vpml, nt=0, nstp=1, nrcv=0 if rx is None else rx.size,
+                  rx=_ptr(rx), rz=_ptr(rz), c=_ptr(c), precision=FDFK_FP32)
+    blx, bms, mlx, mms = C.c_int32(), C.c_float(), C.c_int32(), C.c_float()
+    _check(lib().fdfk_autotune(C.byref(job), int(steps), C.byref(blx), C.byref(bms), C.byref(mlx), C.byref(mms)))
+    return dict(lx=blx.value, ms=bms.value, model_lx=mlx.value, model_ms=mms.value)
 
 
 class Solver:

@@ -112,6 +112,14 @@ const char *fdfk_last_error(void);
  * solvers and batches running meanwhile); this frees the kept blocks of the
  * current device.  No counterpart in the reference. */
 int fdfk_trim_memory(void);
+/* Tile-width autotuner: builds a scratch solver of the job's geometry (zero
+ * background, same kernels), times `steps` steps at interior tile widths
+ * from 0.5x to 2x the occupancy model's choice, and keeps the fastest for
+ * every later fdfk_create of that geometry in this process (FDFK_LX still
+ * overrides).  Returns the chosen width and its ms per step, and the
+ * model's.  No counterpart in the reference. */
+int fdfk_autotune(const fdfk_job *job, int32_t steps, int32_t *best_lx, float *best_ms, int32_t *model_lx,
+                  float *model_ms);
 
 /* --- batched execution: many resident solvers advanced together with one
  * launch per kernel type per step (tile work-list over jobs).  Members keep

@@ -280,3 +280,27 @@ def test_recording_interval():
     s.close()
     assert gx.shape == (len(c.rx), c.nt // 3 + 1)
     assert rel_l2(gx, ox[:, ::3]) < 1e-5 and rel_l2(gz, oz[:, ::3]) < 1e-5
+
+
+def test_autotuned_tile_width_same_results(case, bg):
+    """fdfk_autotune times interior tile widths for a geometry and later
+    solvers of it use the fastest; a cell's arithmetic does not depend on the
+    width, so the seismograms and final state are bit for bit those of the
+    occupancy model's width."""
+    print("case: autotune on 121 x 60, then 301 iterations", flush=True)
+    def run():
+        s = lib.Solver(case.nx, case.nz, case.npml, case.norder, case.dx, case.dz, case.dt, case.nt, case.c,
+                       rx=case.rx, rz=case.rz)
+        s.set_background(bg)
+        s.run(case.nt + 1)
+        out = s.seismograms() + s.get_state()
+        s.close()
+        return out
+    before = run()
+    t = lib.autotune(case.nx, case.nz, case.npml, case.norder, case.dx, case.dz, case.dt, case.c,
+                     rx=case.rx, rz=case.rz, steps=5)
+    print("autotune:", t, flush=True)
+    assert 4 <= t["lx"] <= case.nx + 2 * case.npml and t["ms"] > 0 and t["ms"] <= t["model_ms"]
+    after = run()
+    for a, b in zip(before, after):
+        assert np.array_equal(a, b)
