@@ -35,7 +35,7 @@ PROFILED_STEPS = 50
 MEASURED_TRAFFIC_BYTES = 726348914
 
 
-def step_alg_bytes(nx, nz, npml, norder):
+def step_alg_bytes(nx, nz, npml, norder, mat_bytes=12):
     """Algorithmic HBM bytes of one k_step launch and its cell count.
 
     Total-field cells (the interior, columns 1..nx x rows 2*norder..nz, and
@@ -49,10 +49,11 @@ def step_alg_bytes(nx, nz, npml, norder):
     tf = nx * nz
     side = 2 * (npml - 1) * nz
     bottom = (nx + 2 * (npml - 1)) * (npml - 1)
-    return 36 * tf + (28 + 72 + 8) * side + (28 + 72 + 16) * bottom, tf + side + bottom
+    tfb = 24 + mat_bytes       # 36 B; 32 B with lam and mu read as FP16 (dt^2/rho stays FP32)
+    return tfb * tf + (28 + 72 + 8) * side + (28 + 72 + 16) * bottom, tf + side + bottom
 
 
-def gpu_bench(steps, warmup, workload, bg_block=0, nt=0, autotune=False):
+def gpu_bench(steps, warmup, workload, bg_block=0, nt=0, autotune=False, precision="fp32"):
     """nt: iterations the solver (and its background) is set up for, at
     least the ones run; bg_block > 0 synthesises the background in blocks of
     that many steps instead of storing all nt+1."""
@@ -64,7 +65,8 @@ def gpu_bench(steps, warmup, workload, bg_block=0, nt=0, autotune=False):
         tuned = lib.autotune(cfg["nx"], cfg["nz"], cfg["npml"], cfg["norder"], cfg["dx"], cfg["dz"], cfg["dt"],
                              cfg["c"], rx=cfg["rx"], rz=cfg["rz"])
     s = lib.Solver(cfg["nx"], cfg["nz"], cfg["npml"], cfg["norder"], cfg["dx"], cfg["dz"], cfg["dt"],
-                   nt, cfg["c"], rx=cfg["rx"], rz=cfg["rz"])
+                   nt, cfg["c"], rx=cfg["rx"], rz=cfg["rz"],
+                   precision=lib.FDFK_FP16 if precision == "fp16" else lib.FDFK_FP32)
     bg_mode = "fk-gpu" if not bg_block else f"fk-gpu, synthesised in blocks of {bg_block} steps"
     try:
         t_bg = time.perf_counter()
@@ -89,7 +91,8 @@ def gpu_bench(steps, warmup, workload, bg_block=0, nt=0, autotune=False):
     s.sync()
     st = s.kernel_stats()
     NX, NZ = cfg["NX"], cfg["NZ"]
-    alg_bytes, alg_cells = step_alg_bytes(cfg["nx"], cfg["nz"], cfg["npml"], cfg["norder"])
+    alg_bytes, alg_cells = step_alg_bytes(cfg["nx"], cfg["nz"], cfg["npml"], cfg["norder"],
+                                          mat_bytes=8 if precision == "fp16" else 12)
     sx, sz = s.seismograms()
     finite = bool(np.isfinite(sx).all() and np.isfinite(sz).all())
     s.close()
@@ -231,6 +234,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=201)
     ap.add_argument("--bg-block", type=int, default=0, metavar="B",
                     help="synthesise the background in blocks of B steps (default: store all)")
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp16"],
+                    help="fp16: lam and mu stored as FP16 with power-of-two scales (arithmetic stays FP32)")
     ap.add_argument("--autotune", action="store_true",
                     help="time interior tile widths first (fdfk_autotune) and run with the fastest")
     ap.add_argument("--nt", type=int, default=0,
@@ -256,7 +261,7 @@ def main():
         print("bench.py: this build measures one GPU; the hot path shards as x-slabs "
               "(see DESIGN.md) but multi-GPU runs are not part of it", file=sys.stderr)
         sys.exit(2)
-    r = gpu_bench(a.steps, a.warmup, a.workload, a.bg_block, a.nt, a.autotune)
+    r = gpu_bench(a.steps, a.warmup, a.workload, a.bg_block, a.nt, a.autotune, a.precision)
     cells = r["NX"] * r["NZ"]
     value = cells * a.steps / r["wall"] / 1e9
     ms = r["wall"] / a.steps * 1e3
@@ -287,6 +292,8 @@ def main():
             "device_gib": round(r["device_bytes"] / 2**30, 3),
             "background_s": r["background_s"],
             "tile_autotune": r["tuned"],
+            "material_storage": ("fp16 lam, mu (power-of-two scales); fp32 dt^2/rho" if a.precision == "fp16"
+                                 else "fp32"),
             "solves_per_hour_at_20000_steps": round(3600.0 / (ms * 1e-3 * full_steps), 3),
             "finite": r["finite"],
         },
@@ -296,7 +303,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": (MEASURED_TRAFFIC_BYTES if a.workload == "c2" else None),
+            "traffic": (MEASURED_TRAFFIC_BYTES if a.workload == "c2" and a.precision == "fp32" else None),
             "traffic_source": "rocprofv3 FETCH_SIZE+WRITE_SIZE per launch, copy-calibrated "
                               "(profiles/k_step_8001x2000_hbm_traffic.md)",
             "kernel": "k_step",

@@ -157,6 +157,11 @@ struct fdfk_solver {
     int lx = 45, nt_threads = 256;
     bool fused = true;   // interior and frame in one launch (k_step)
     int lx_model = 0;    // interior tile width of the occupancy model (lx may be tuned)
+    // 16-bit material storage (FDFK_FP16): FP16 copies of lam, mu for the
+    // interior tiles, value = half * scale (powers of two)
+    bool mat16 = false;
+    unsigned short *lam16 = nullptr, *mu16 = nullptr;
+    float sl = 1.0f, sm = 1.0f;
     // profiling
     bool prof = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_int, ev_step;
@@ -348,6 +353,7 @@ IntArgs<NO> interior_args(fdfk_solver *s)
     a.uxp = s->ux[1 - s->cur]; a.uzp = s->uz[1 - s->cur];
     a.bf = s->bf; a.lam = s->lam; a.mu = s->mu;
     a.ringx = s->ringx; a.ringz = s->ringz;
+    a.lam16 = s->lam16; a.mu16 = s->mu16; a.sl = s->sl; a.sm = s->sm;
     a.corr = s->bg_valid && s->it <= s->nt;
     a.lx = s->lx;
     int_tiling<NO, NT>(g, a.lx, a.nbx, a.nbz, a.nsb, a.zseg);
@@ -382,13 +388,14 @@ size_t step_lds_bytes(const Geo &g)
 }
 
 template <int NO, int NT>
-IntPlan plan_lx(const Geo &g, bool fused)
+IntPlan plan_lx(const Geo &g, bool fused, bool m16 = false)
 {
     IntPlan best;
     int dev = 0, ncu = 0, bpc = 0;
     hipError_t oe;
     if constexpr (NT == 256)
-        oe = fused ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_step<NO>, NT, step_lds_bytes<NO>(g))
+        oe = fused ? (m16 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_step<NO, true>, NT, step_lds_bytes<NO>(g))
+                          : hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_step<NO>, NT, step_lds_bytes<NO>(g)))
                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_interior2<NO, NT>, NT, 0);
     else
         oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_interior2<NO, NT>, NT, 0);
@@ -448,7 +455,7 @@ void plan_interior(fdfk_solver *s)
     };
     if (g.norder == 3 && want == 192) consider(192, plan_lx<3, 192>(g, false));
     else if (g.norder == 3 && want == 128) consider(128, plan_lx<3, 128>(g, false));
-    else with_order(g.norder, [&](auto no) { consider(256, plan_lx<decltype(no)::value, 256>(g, fused)); });
+    else with_order(g.norder, [&](auto no) { consider(256, plan_lx<decltype(no)::value, 256>(g, fused, s->mat16)); });
     s->nt_threads = nt;
     s->fused = fused && nt == 256;
     s->lx = s->lx_model = p.lx;
@@ -519,8 +526,12 @@ void launch_step_t(fdfk_solver *s, const StepArgs &a)
     IntArgs<NO> ia = interior_args<NO, 256>(s);
     ETiles t = frame_tiles(s->g);
     const int nint = ia.nbx * ia.nbz + ia.nsb;
-    hipLaunchKernelGGL(k_step<NO>, dim3((unsigned)(nint + t.off[t.n])), dim3(256), step_lds_bytes<NO>(s->g),
-                       s->stream, ia, a, t, nint);
+    if (s->mat16)
+        hipLaunchKernelGGL((k_step<NO, true>), dim3((unsigned)(nint + t.off[t.n])), dim3(256),
+                           step_lds_bytes<NO>(s->g), s->stream, ia, a, t, nint);
+    else
+        hipLaunchKernelGGL(k_step<NO>, dim3((unsigned)(nint + t.off[t.n])), dim3(256), step_lds_bytes<NO>(s->g),
+                           s->stream, ia, a, t, nint);
 }
 
 void launch_step(fdfk_solver *s, const StepArgs &a)
@@ -610,7 +621,8 @@ int validate(const fdfk_job *j)
     if (j->nt < 0 || j->nstp < 1) return fail(FDFK_EINVAL, "nt >= 0 and nstp >= 1 required");
     if (!j->c) return fail(FDFK_EINVAL, "material array c is required");
     if (j->nrcv < 0 || (j->nrcv > 0 && (!j->rx || !j->rz))) return fail(FDFK_EINVAL, "receiver arrays");
-    if (j->precision != FDFK_FP32) return fail(FDFK_ENOSYS, "only FP32 storage in this entry point");
+    if (j->precision != FDFK_FP32 && j->precision != FDFK_FP16)
+        return fail(FDFK_ENOSYS, "storage precision: FP32, or FP16 materials (BF16 is not built)");
     return 0;
 }
 
@@ -727,7 +739,12 @@ int fdfk_create(const fdfk_job *job, fdfk_solver **out)
     s->nrcv = job->nrcv;
     s->x0 = job->x0; s->z0 = job->z0;
     s->R0 = job->R0; s->npower = job->npower;
+    s->mat16 = job->precision == FDFK_FP16;
     plan_interior(s);
+    if (s->mat16 && !s->fused) {
+        delete s;
+        return fail(FDFK_ENOSYS, "FP16 materials run in the fused step only (FDFK_FUSED=0 is FP32)");
+    }
     const long ncell = (long)g.NX * g.NZ;
     // vpml = max Vp (src/FDFK2D.f90:416)
     float vpml = job->vpml;
@@ -775,6 +792,25 @@ int fdfk_create(const fdfk_job *job, fdfk_solver **out)
     hipLaunchKernelGGL(k_import_material, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, s->stream, g, dc,
                        (float)(g.dt * g.dt), s->bf, s->lam, s->mu);
     hipFreeAsync(dc, s->stream);
+    if (s->mat16) {
+        // per-field power-of-two scales: |value| / scale < 2^15, inside FP16's range
+        double ml = 0.0, mm = 0.0;
+        for (long i = 0; i < ncell; ++i) {
+            ml = std::max(ml, (double)std::fabs(job->c[3 * i + 1]));
+            mm = std::max(mm, (double)std::fabs(job->c[3 * i + 2]));
+        }
+        auto pow2 = [](double m) {
+            int e = 0;
+            if (!(m > 0.0)) return 1.0f;
+            std::frexp(m, &e);
+            return (float)std::ldexp(1.0, e - 15);
+        };
+        s->sl = pow2(ml);
+        s->sm = pow2(mm);
+        if ((rc = s->dalloc(&s->lam16, nf)) || (rc = s->dalloc(&s->mu16, nf))) return bad(rc);
+        hipLaunchKernelGGL(k_round_material, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, s->stream, g,
+                           s->lam, s->mu, s->lam16, s->mu16, s->sl, s->sm);
+    }
     // PML strips
     s->strip_cells[S_L] = s->strip_cells[S_R] = (long)g.npml * g.nz;
     s->strip_cells[S_B] = (long)g.NX * g.npml;

@@ -402,13 +402,14 @@ __global__ __launch_bounds__(kFrameThreads) void k_frame(StepArgs a, ETiles t)
 // tiles release at the end of their march, instead of running after the
 // slowest of them.  Dynamic LDS: the larger of the two needs.
 static_assert(kFrameThreads == 256, "the fused step launches 256-lane blocks");
-template <int NO>
+template <int NO, bool M16 = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kIntWaves<NO>))) void k_step(IntArgs<NO> ia, StepArgs sa,
                                                                                       ETiles ft, int nint)
 {
     extern __shared__ float sm[];
     if ((int)blockIdx.x < nint)
-        IntKernel2<NO, 256>::run(ia, xcd_tile(blockIdx.x, nint), *reinterpret_cast<typename IntKernel2<NO, 256>::Smem *>(sm));
+        IntKernel2<NO, 256, M16>::run(ia, xcd_tile(blockIdx.x, nint),
+                                      *reinterpret_cast<typename IntKernel2<NO, 256, M16>::Smem *>(sm));
     else   // frame tiles grouped per XCD too: neighbouring tiles share halo lines in one L2
         frame_body<NO>(sa, ft, xcd_tile((int)blockIdx.x - nint, (int)gridDim.x - nint));
 }

@@ -138,6 +138,10 @@ struct IntArgs {
     const float *ux, *uz;
     float *uxp, *uzp;
     const float *bf, *lam, *mu;
+    // FP16 copies of lam and mu (16-bit material storage): value = half * scale,
+    // the scales powers of two, so the conversion is exact
+    const unsigned short *lam16, *mu16;
+    float sl, sm;
     const float *ringx, *ringz;  // background on the hybrid ring (total-field view of the padding)
     int corr;                    // ring values valid at this step
     int lx;   // output columns per block

@@ -335,7 +335,9 @@ __device__ __forceinline__ int xcd_tile(int p, int n)
 // the stores are predicated.  Addresses are a per-column scalar base plus a
 // loop-invariant 32-bit lane offset.
 // ---------------------------------------------------------------------
-template <int NO, int NT>
+// M16: lam and mu are read from their FP16 copies (half the bytes; the FP32
+// arrays hold the same rounded values for the other kernels)
+template <int NO, int NT, bool M16 = false>
 struct IntKernel2 {
     static constexpr int HZ = 2 * NO - 1;
     static constexpr int TZ = NT - 2 * HZ;      // output rows per block
@@ -389,7 +391,7 @@ struct IntKernel2 {
     // (c - xb) * ld * 4 + the lane's 32-bit row offset.  Offsets stay far
     // below 4 GiB because a block spans lx + 6*NO columns.
     struct Rsrc {
-        __amdgpu_buffer_rsrc_t ux, uz, uxp, uzp, bf, lam, mu, rx, rz;
+        __amdgpu_buffer_rsrc_t ux, uz, uxp, uzp, bf, lam, mu, rx, rz, lam16, mu16;
         int xb, pitch;   // first column, column pitch in bytes
         __device__ __forceinline__ int soff(int c) const { return (c - xb) * pitch; }
     };
@@ -401,6 +403,27 @@ struct IntKernel2 {
     static __device__ __forceinline__ float ld(__amdgpu_buffer_rsrc_t r, unsigned o, int so)
     {
         return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, o, so, 0));
+    }
+    static __device__ __forceinline__ __amdgpu_buffer_rsrc_t mk16(const Geo &g, const unsigned short *base, int xb)
+    {
+        const unsigned short *cb = base + (size_t)(xb + g.npml - 1) * (size_t)g.ld;
+        return __builtin_amdgcn_make_buffer_rsrc((void *)cb, 0, 0x7fffffff, kBufferFlags);
+    }
+    // a material value at the lane's row of a column: byte offsets of the
+    // 32-bit arrays, halved for the 16-bit copies
+    static __device__ __forceinline__ float ld_lam(const IntArgs<NO> &a, const Rsrc &B, unsigned o, int so)
+    {
+        if constexpr (M16)
+            return (float)__builtin_bit_cast(_Float16, __builtin_amdgcn_raw_buffer_load_b16(B.lam16, o >> 1, so >> 1, 0)) * a.sl;
+        else
+            return ld(B.lam, o, so);
+    }
+    static __device__ __forceinline__ float ld_mu(const IntArgs<NO> &a, const Rsrc &B, unsigned o, int so)
+    {
+        if constexpr (M16)
+            return (float)__builtin_bit_cast(_Float16, __builtin_amdgcn_raw_buffer_load_b16(B.mu16, o >> 1, so >> 1, 0)) * a.sm;
+        else
+            return ld(B.mu, o, so);
     }
     static __device__ __forceinline__ void st(__amdgpu_buffer_rsrc_t r, unsigned o, int so, float v)
     {
@@ -447,8 +470,8 @@ struct IntKernel2 {
                 R.pr[PB][0] = ld(B.rx, o, sn);
                 R.pr[PB][1] = ld(B.rz, o, sn);
             }
-            R.pl[PB][0] = ld(B.lam, o, sm); R.pl[PB][1] = ld(B.mu, o, sm);
-            R.p0[PB][0] = ld(B.lam, o, s0); R.p0[PB][1] = ld(B.mu, o, s0);
+            R.pl[PB][0] = ld_lam(a, B, o, sm); R.pl[PB][1] = ld_mu(a, B, o, sm);
+            R.p0[PB][0] = ld_lam(a, B, o, s0); R.p0[PB][1] = ld_mu(a, B, o, s0);
             R.po[PA][0] = ld(B.bf, o, sx);
             R.po[PA][1] = ld(B.uxp, o, sx);
             R.po[PA][2] = ld(B.uzp, o, sx);
@@ -600,6 +623,7 @@ struct IntKernel2 {
             B.ux = mk(g, a.ux, B.xb); B.uz = mk(g, a.uz, B.xb);
             B.uxp = mk(g, a.uxp, B.xb); B.uzp = mk(g, a.uzp, B.xb);
             B.bf = mk(g, a.bf, B.xb); B.lam = mk(g, a.lam, B.xb); B.mu = mk(g, a.mu, B.xb);
+            if constexpr (M16) { B.lam16 = mk16(g, a.lam16, B.xb); B.mu16 = mk16(g, a.mu16, B.xb); }
             B.rx = mk(g, a.ringx, B.xb); B.rz = mk(g, a.ringz, B.xb);
         }
         // prologue: the loads the first step consumes
@@ -614,8 +638,8 @@ struct IntKernel2 {
                 R.pr[0][0] = ld(B.rx, o, sn);
                 R.pr[0][1] = ld(B.rz, o, sn);
             }
-            R.pl[0][0] = ld(B.lam, o, sm); R.pl[0][1] = ld(B.mu, o, sm);
-            R.p0[0][0] = ld(B.lam, o, s0); R.p0[0][1] = ld(B.mu, o, s0);
+            R.pl[0][0] = ld_lam(a, B, o, sm); R.pl[0][1] = ld_mu(a, B, o, sm);
+            R.p0[0][0] = ld_lam(a, B, o, s0); R.p0[0][1] = ld_mu(a, B, o, s0);
 #pragma unroll
             for (int q = 0; q < 2; ++q) { R.pu[1][q] = 0.0f; R.pl[1][q] = 1.0f; R.p0[1][q] = 1.0f; }
 #pragma unroll
@@ -680,6 +704,23 @@ __global__ void k_import_material(Geo g, const float *c, float dt2, float *bf, f
     bf[d] = dt2 / c[3 * src + 0];
     lam[d] = c[3 * src + 1];
     mu[d] = c[3 * src + 2];
+}
+
+// 16-bit material storage: lam and mu rounded to FP16 with power-of-two
+// scales (round to nearest even); the FP32 arrays take the rounded values,
+// so every kernel sees the same materials
+__global__ void k_round_material(Geo g, float *lam, float *mu, unsigned short *lam16, unsigned short *mu16,
+                                 float sl, float sm)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long n = (long)g.NX * g.NZ;
+    if (t >= n) return;
+    const size_t d = (size_t)(t / g.NZ) * g.ld + (size_t)(t % g.NZ);
+    const _Float16 hl = (_Float16)(lam[d] * (1.0f / sl)), hm = (_Float16)(mu[d] * (1.0f / sm));   // nearest even
+    lam16[d] = __builtin_bit_cast(unsigned short, hl);
+    mu16[d] = __builtin_bit_cast(unsigned short, hm);
+    lam[d] = (float)hl * sl;
+    mu[d] = (float)hm * sm;
 }
 
 // reference (1:2, NZ, NX) <-> two padded arrays (level 1 -> cur, level 2 -> prev)

@@ -304,3 +304,56 @@ def test_autotuned_tile_width_same_results(case, bg):
     after = run()
     for a, b in zip(before, after):
         assert np.array_equal(a, b)
+
+
+def _round_materials(c):
+    """The library's 16-bit material storage restated: lam and mu to FP16 with
+    per-field power-of-two scales (|value| / scale < 2^15), nearest even."""
+    flat = np.array(c, np.float32).ravel().copy()
+    for k in (1, 2):
+        v = flat[k::3]
+        m = float(np.abs(v).max())
+        sc = np.float32(np.ldexp(1.0, np.frexp(m)[1] - 15)) if m > 0 else np.float32(1.0)
+        flat[k::3] = np.float16(v * (np.float32(1.0) / sc)).astype(np.float32) * sc
+    return flat.reshape(np.shape(c))
+
+
+@pytest.mark.parametrize("norder", [3, 8])
+def test_fp16_materials_equal_fp32_on_rounded_materials(norder):
+    """FDFK_FP16: the interior tiles read lam and mu from FP16 copies; every
+    kernel sees the same rounded values, so the run equals, bit for bit, an
+    FP32 run on materials rounded the same way on the host.  The rounding
+    itself moves the seismograms by about the materials' FP16 resolution."""
+    print("case: FP16 materials, norder %d, 121 x 60, 301 iterations" % norder, flush=True)
+    c = Case(nt=300, norder=norder, npml=15 if norder <= 4 else 20)
+    bg = c.oracle_background()
+    vpml = refstate.vp_max(c.c)     # the PML profile from the unrounded model in every run
+    def run(mat, precision):
+        s = lib.Solver(c.nx, c.nz, c.npml, c.norder, c.dx, c.dz, c.dt, c.nt, mat, rx=c.rx, rz=c.rz,
+                       precision=precision, vpml=vpml)
+        s.set_background(bg)
+        s.run(c.nt + 1)
+        out = s.seismograms() + s.get_state()
+        s.close()
+        return out
+    h = run(c.c, lib.FDFK_FP16)
+    r = run(_round_materials(c.c), lib.FDFK_FP32)
+    for a, b in zip(h, r):
+        assert np.array_equal(a, b)
+    f = run(c.c, lib.FDFK_FP32)
+    d = max(rel_l2(h[0], f[0]), rel_l2(h[1], f[1]))
+    print("seismogram rel-L2, FP16 vs FP32 materials: %.3g" % d, flush=True)
+    assert 0 < d < 1e-2
+
+
+def test_fp16_materials_not_batched_nor_separate(monkeypatch):
+    c = Case(nt=20)
+    s = lib.Solver(c.nx, c.nz, c.npml, c.norder, c.dx, c.dz, c.dt, c.nt, c.c, precision=lib.FDFK_FP16)
+    b = lib.Batch(3)
+    with pytest.raises(lib.FdfkError, match="not batched"):
+        b.add(s)
+    b.close()
+    s.close()
+    monkeypatch.setenv("FDFK_FUSED", "0")
+    with pytest.raises(lib.FdfkError, match="fused step only"):
+        lib.Solver(c.nx, c.nz, c.npml, c.norder, c.dx, c.dz, c.dt, c.nt, c.c, precision=lib.FDFK_FP16)
