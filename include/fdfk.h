@@ -51,7 +51,9 @@ typedef struct fdfk_job {
     int32_t nrcv;
     const float *rx, *rz;           /* receiver coordinates (m) */
     const float *c;                 /* rho, lambda, mu (reference layout) */
-    int32_t precision;              /* FDFK_FP32 / FDFK_BF16 / FDFK_FP16 */
+    int32_t precision;              /* FDFK_FP32; FDFK_FP16: lam and mu stored as FP16 with
+                                       power-of-two scales for the interior tiles (single
+                                       solves, fused step); FDFK_BF16: not built */
 } fdfk_job;
 
 /* FK background description: the inputs of comput_background_wavefield
