@@ -33,6 +33,9 @@ PROFILED_STEPS = 50
 # FETCH_SIZE / WRITE_SIZE scaled by a plain device copy
 # (profiles/k_step_8001x2000_hbm_traffic.md)
 MEASURED_TRAFFIC_BYTES = 726348914
+# the same measurement with FP16 materials (profiles/k_step_fp16_8001x2000_hbm_traffic.md):
+# FETCH_SIZE 235573 KiB x 2 + WRITE_SIZE 142402 KiB
+MEASURED_TRAFFIC_BYTES_FP16 = 235573 * 2048 + 142402 * 1024
 
 
 def step_alg_bytes(nx, nz, npml, norder, mat_bytes=12):
@@ -303,9 +306,11 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": (MEASURED_TRAFFIC_BYTES if a.workload == "c2" and a.precision == "fp32" else None),
+            "traffic": (None if a.workload != "c2" else
+                        MEASURED_TRAFFIC_BYTES_FP16 if a.precision == "fp16" else MEASURED_TRAFFIC_BYTES),
             "traffic_source": "rocprofv3 FETCH_SIZE+WRITE_SIZE per launch, copy-calibrated "
-                              "(profiles/k_step_8001x2000_hbm_traffic.md)",
+                              + ("(profiles/k_step_fp16_8001x2000_hbm_traffic.md)" if a.precision == "fp16"
+                                 else "(profiles/k_step_8001x2000_hbm_traffic.md)"),
             "kernel": "k_step",
             "kernel_avg_ms": round(k_ms, 5),
             "alg_bytes_per_launch": r["alg_bytes"],
