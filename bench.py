@@ -175,6 +175,34 @@ def serve_bench(n, seed=5, rate=200.0):
             "max_rel_l2_batched_vs_plain": worst}
 
 
+def compat_bench(steps, warmup=3):
+    """The drop-in path on the single large solve: updatedispl_ called per
+    iteration with the reference driver's host arrays (src/FDFK2D.f90:516-522),
+    which copies both levels of ux, uz and the PML arrays back every call
+    because the driver reads them on the host.  The rate includes those PCIe
+    copies; it is the price of the unchanged driver, not the kernel's."""
+    from fdfk2d_amd import configs, lib, refstate
+    nt = warmup + steps + 1
+    cfg = configs.c2(nt=nt)
+    st = refstate.make_state(cfg["nx"], cfg["nz"], cfg["npml"], cfg["norder"], cfg["dx"], cfg["dz"], cfg["dt"],
+                             nt, cfg["c"])
+    st["bg"] = [np.zeros(n, np.float32) for n in lib.bg_sizes(cfg["nx"], cfg["nz"], 5, nt)]
+    for it in range(warmup):
+        lib.updatedispl(st, it)
+    t0 = time.perf_counter()
+    for it in range(warmup, warmup + steps):
+        lib.updatedispl(st, it)
+    dt = time.perf_counter() - t0
+    lib.lib().fdfk_compat_release()
+    cells = cfg["NX"] * cfg["NZ"]
+    return {"metric": "Gcell-updates/s", "value": round(cells * steps / dt / 1e9, 3), "unit": "Gcell-updates/s",
+            "higher_is_better": True, "data": "synthetic", "dtype": "f32",
+            "config": {"workload": "single large solve 8001x2000 through updatedispl_ (host arrays, "
+                                   "both levels and the PML arrays copied back every call)",
+                       "steps": steps},
+            "ms_per_step": round(dt / steps * 1e3, 3)}
+
+
 def batch_bench(njobs, seed=3, steps=1000):
     """Scaled batched workload: njobs independent solves of mixed sizes
     advanced together (one launch per kernel type per step) against the
@@ -248,6 +276,8 @@ def main():
                     help="only run the copy kernel used to calibrate HBM byte counters")
     ap.add_argument("--serve", type=int, default=0, metavar="N",
                     help="serving measurement with N requests instead of the default line")
+    ap.add_argument("--compat", type=int, default=0, metavar="N",
+                    help="N iterations through the drop-in updatedispl_ (PCIe-inclusive rate)")
     ap.add_argument("--batch", type=int, default=0, metavar="N",
                     help="batched-execution measurement with N jobs instead of the default line")
     a = ap.parse_args()
@@ -259,6 +289,9 @@ def main():
         return
     if a.batch:
         print(json.dumps(batch_bench(a.batch)), flush=True)
+        return
+    if a.compat:
+        print(json.dumps(compat_bench(a.compat)), flush=True)
         return
     if a.gpus != 1:
         print("bench.py: this build measures one GPU; the hot path shards as x-slabs "
