@@ -32,3 +32,14 @@ def test_library_exports_every_declared_symbol():
 def test_library_is_built_for_gfx950():
     data = open(lib.LIB_PATH, "rb").read()
     assert b"gfx950" in data
+
+
+def test_fortran_binding_names_exported_and_declared():
+    """Every bind(C, name=...) of the Fortran module include/fdfk.f90 is a
+    function the header declares and the library exports."""
+    txt = open(os.path.join(ROOT, "include", "fdfk.f90")).read()
+    names = re.findall(r'bind\(C,\s*name="([a-z_0-9]+)"\)', txt)
+    assert len(names) >= 8
+    L = lib.lib()
+    declared = declared_functions()
+    assert not [n for n in names if n not in declared or not hasattr(L, n)]
