@@ -56,7 +56,9 @@ static hipError_t h2d(hipStream_t st, void *dst, const void *src, size_t bytes)
 // server's requests) would stall them all; a destroyed solver's blocks go
 // here instead, once its stream (ordered after any batch steps that used
 // them) is idle.  A request takes the smallest cached block of at least its
-// size and at most twice it, else allocates.  fdfk_trim_memory frees the cache.
+// size and at most twice it, else allocates; when an allocation fails the
+// device's cached blocks are freed and it is tried once more.
+// fdfk_trim_memory frees the cache.
 struct BlockCache {
     std::mutex mu;
     std::map<int, std::multimap<size_t, void *>> free;   // device -> size -> block
@@ -84,7 +86,21 @@ static hipError_t block_get(size_t bytes, void **p, size_t *got)
         }
     }
     *got = bytes;
-    return hipMalloc(p, bytes);
+    e = hipMalloc(p, bytes);
+    if (e == hipErrorOutOfMemory) {
+        // the cache may hold blocks no request fits: give them back and retry once
+        (void)hipGetLastError();
+        std::multimap<size_t, void *> blocks;
+        {
+            BlockCache &c = block_cache();
+            std::lock_guard<std::mutex> lk(c.mu);
+            blocks.swap(c.free[dev]);
+        }
+        if (blocks.empty()) return e;
+        for (auto &b : blocks) (void)hipFree(b.second);
+        e = hipMalloc(p, bytes);
+    }
+    return e;
 }
 static void block_put(int dev, void *p, size_t bytes)
 {
