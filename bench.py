@@ -294,8 +294,8 @@ def main():
         print(json.dumps(compat_bench(a.compat)), flush=True)
         return
     if a.gpus != 1:
-        print("bench.py: this build measures one GPU; the hot path shards as x-slabs "
-              "(see DESIGN.md) but multi-GPU runs are not part of it", file=sys.stderr)
+        print("bench.py measures one GPU: the path would shard as x-slabs with one halo exchange "
+              "per step, which is not built, so N > 1 is not run", file=sys.stderr)
         sys.exit(2)
     r = gpu_bench(a.steps, a.warmup, a.workload, a.bg_block, a.nt, a.autotune, a.precision)
     cells = r["NX"] * r["NZ"]
