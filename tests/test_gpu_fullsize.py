@@ -7,12 +7,17 @@ on the size (the oracle is too slow for this grid):
 - the interior and the frame launched separately equal the fused step;
 - linearity: twice the background gives twice the wavefield, to rel-L2
   1e-5, the parity bar (a missing term or a wrong sign is of order one).
-  The doubling is not exact, although scaling by 2 commutes with every
-  normal FP32 rounding: departures grow to about 3e-6 of the field's scale
-  after 1501 iterations.  Subnormal values at the far edge of the wavefront
-  are the likely cause; it is not confirmed.  The surface seismograms hold
-  only the precursor of a wave still deep in the model and are checked in
-  absolute terms against the field's scale.
+  The doubling is not exact: values at the leading edge of the wavefront
+  pass through the subnormal range, where FP32 rounds on an absolute grid
+  and doubling does not commute with rounding (the kernels keep
+  subnormals), and the arriving wave carries the differences along.  The
+  CPU oracle departs the same way (rel-L2 9.4e-7 after 301 iterations of a
+  small case, 1.0e-6 on the GPU); one step from a random state, with no
+  subnormal values, is exactly linear; and the absolute departure stays
+  constant once the pulse is in (its relative size grows only as the field
+  leaves through the PML).  The surface seismograms hold only the
+  precursor of a wave still deep in the model and are checked in absolute
+  terms against the field's scale.
 """
 
 import numpy as np
