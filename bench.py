@@ -175,6 +175,38 @@ def serve_bench(n, seed=5, rate=200.0):
             "max_rel_l2_batched_vs_plain": worst}
 
 
+def full_solve_bench(nt=20000, block=512):
+    """The whole single large solve, timed end to end on the GPU: FK
+    background (synthesised in blocks), all nt+1 iterations with recording,
+    seismograms back on the host.  Model set-up on the host is outside."""
+    from fdfk2d_amd import configs, lib
+    cfg = configs.c2(nt=nt)
+    t0 = time.perf_counter()
+    s = lib.Solver(cfg["nx"], cfg["nz"], cfg["npml"], cfg["norder"], cfg["dx"], cfg["dz"], cfg["dt"], nt,
+                   cfg["c"], rx=cfg["rx"], rz=cfg["rz"])
+    s.compute_background(cfg["fk_left"], cfg["fk_right"], cfg["src_is_p"], cfg["f0"], cfg["ds"], cfg["rayp"],
+                         cfg["bazs"], block_steps=block)
+    s.sync()
+    t1 = time.perf_counter()
+    s.run(nt + 1)
+    s.sync()
+    t2 = time.perf_counter()
+    sx, sz = s.seismograms()
+    t3 = time.perf_counter()
+    gib = s.device_bytes / 2**30
+    s.close()
+    cells = cfg["NX"] * cfg["NZ"] * (nt + 1)
+    return {"metric": "solves_per_hour", "value": round(3600.0 / (t3 - t0), 1), "unit": "solves/h",
+            "higher_is_better": True, "data": "synthetic", "dtype": "f32",
+            "config": {"workload": f"single large solve 8001x2000, {nt + 1} iterations, 401 receivers, "
+                                   f"background in blocks of {block} steps", "device_gib": round(gib, 2)},
+            "wall_s": round(t3 - t0, 3), "create_and_background_s": round(t1 - t0, 3),
+            "loop_s": round(t2 - t1, 3), "loop_gcells": round(cells / (t2 - t1) / 1e9, 2),
+            "seismograms_s": round(t3 - t2, 3),
+            "finite": bool(np.isfinite(sx).all() and np.isfinite(sz).all()),
+            "max_abs_seis_z": float(np.abs(sz).max())}
+
+
 def compat_bench(steps, warmup=3):
     """The drop-in path on the single large solve: updatedispl_ called per
     iteration with the reference driver's host arrays (src/FDFK2D.f90:516-522),
@@ -276,6 +308,8 @@ def main():
                     help="only run the copy kernel used to calibrate HBM byte counters")
     ap.add_argument("--serve", type=int, default=0, metavar="N",
                     help="serving measurement with N requests instead of the default line")
+    ap.add_argument("--full-solve", action="store_true",
+                    help="time the whole 20001-iteration single large solve end to end")
     ap.add_argument("--compat", type=int, default=0, metavar="N",
                     help="N iterations through the drop-in updatedispl_ (PCIe-inclusive rate)")
     ap.add_argument("--batch", type=int, default=0, metavar="N",
@@ -292,6 +326,9 @@ def main():
         return
     if a.compat:
         print(json.dumps(compat_bench(a.compat)), flush=True)
+        return
+    if a.full_solve:
+        print(json.dumps(full_solve_bench()), flush=True)
         return
     if a.gpus != 1:
         print("bench.py measures one GPU: the path would shard as x-slabs with one halo exchange "
