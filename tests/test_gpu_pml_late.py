@@ -1,0 +1,67 @@
+"""Late-time behaviour of the PML along the free surface (a known issue of
+the formulation, shared by the oracle and the GPU path).
+
+The layering of the single large solve (2 km of Vs 1.5 km/s sediment over
+the crust and mantle) on a small grid, 801 x 200 at 50 m with a 40-cell
+PML, P incidence, 20001 iterations (60 s): after the wave has left, the
+field in the side PML strips should decay, but it grows from about 30 s on
+(0.037 at 60 s against a physical field of 3e-3).  A 20-cell PML on the
+same layering decays.  The growth does not depend on dt; a wider PML makes
+it worse: the late-time instability of classical split-field PMLs for
+elastic waves at a free surface.  Expected to fail until the PML carries a
+multiaxial or frequency-shifted form.
+"""
+import numpy as np
+import pytest
+
+from fdfk2d_amd import configs, lib
+
+pytestmark = pytest.mark.gpu
+
+
+def layered(nx, nz, npml, dt=0.003, dx=50.0):
+    NX, NZ = nx + 2 * npml, nz + npml
+    vsl = [(0.0, 1500.0), (2000.0, 3600.0), (45000.0, 4500.0)]
+    layers = [(zt, 1.73 * v, v) for zt, v in vsl]
+    z = np.arange(NZ) * dx
+    vs1 = np.zeros(NZ, np.float32)
+    for zt, v in vsl:
+        vs1[z >= zt] = v
+    vs = np.repeat(vs1[None, :], NX, 0)
+    vp = (np.float32(1.73) * vs).astype(np.float32)
+    return configs._common(nx, nz, npml, dx, dx, dt, 20000, vp, vs, layers, True, 0.07, 0.5, 41, 50.0 * nx / 41)
+
+
+def side_pml_peaks(npml):
+    nx, nz = 801, 200
+    cfg = layered(nx, nz, npml)
+    NX, NZ = nx + 2 * npml, nz + npml
+    s = lib.Solver(nx, nz, npml, cfg["norder"], cfg["dx"], cfg["dz"], cfg["dt"], 20000, cfg["c"],
+                   rx=cfg["rx"], rz=cfg["rz"])
+    s.compute_background(cfg["fk_left"], cfg["fk_right"], cfg["src_is_p"], cfg["f0"], cfg["ds"], cfg["rayp"],
+                         cfg["bazs"], block_steps=512)
+    peaks, done = [], 0
+    for tsec in (10, 60):
+        it = min(20001, int(round(tsec / 0.003)))
+        s.run(it - done)
+        done = it
+        ux, uz = s.get_state()
+        a = np.maximum(np.abs(ux.reshape(NX, NZ, 2)[:, :, 0]), np.abs(uz.reshape(NX, NZ, 2)[:, :, 0]))
+        side = max(a[:npml - 5, :nz].max(), a[npml + nx + 5:, :nz].max())
+        peaks.append((float(a.max()), float(side)))
+    s.close()
+    print("npml %d: (all, side PML) at 10 s %s, at 60 s %s" % (npml, peaks[0], peaks[1]), flush=True)
+    return peaks
+
+
+def test_thin_pml_decays():
+    print("case: 801 x 200 layered, 20-cell PML, 60 s", flush=True)
+    p10, p60 = side_pml_peaks(20)
+    assert p60[1] < 0.1 * p10[0]
+
+
+@pytest.mark.xfail(reason="late-time growth of the split-field PML at the free surface (known issue)", strict=False)
+def test_thick_pml_decays():
+    print("case: 801 x 200 layered, 40-cell PML, 60 s", flush=True)
+    p10, p60 = side_pml_peaks(40)
+    assert p60[1] < 0.1 * p10[0]
