@@ -7,9 +7,13 @@ PML, P incidence, 20001 iterations (60 s): after the wave has left, the
 field in the side PML strips should decay, but it grows from about 30 s on
 (0.037 at 60 s against a physical field of 3e-3).  A 20-cell PML on the
 same layering decays.  The growth does not depend on dt; a wider PML makes
-it worse: the late-time instability of classical split-field PMLs for
-elastic waves at a free surface.  Expected to fail until the PML carries a
-multiaxial or frequency-shifted form.
+it worse.  It needs the strong low-velocity layer: with a 40-cell PML the
+side strips decay to 1e-10 without the sediment and to 1e-7 with a weak
+one (Vs 3.0 over 3.6), the behaviour of classical PMLs on layered
+waveguides whose backward-propagating guided modes grow inside the
+absorbing layer.  Expected to fail until the absorbing layer is made robust
+to such modes (a multiaxial PML with a sufficient ratio, or a PML combined
+with a stable sponge).
 """
 import numpy as np
 import pytest
