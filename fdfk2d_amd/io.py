@@ -159,7 +159,7 @@ def read_job(prefix, finpar):
     nz = int(round((fd["zn"] - fd["z0"]) / fd["dz"])) + 1
     npml = fd["npml"]
     if fd["pml_top"]:
-        raise ValueError("is_PML_top = .true. is not supported by this build")
+        raise ValueError("is_PML_top = .true. is not supported (free surface only)")
     NX, NZ = nx + 2 * npml, nz + npml
     xmin, xmax = fd["x0"], fd["x0"] + (nx - 1) * fd["dx"]
     zmin, zmax = fd["z0"], fd["z0"] + (nz - 1) * fd["dz"]

@@ -13,7 +13,7 @@
  *   the twelve background arrays uxl_reg..uzb_pml with the shapes of
  *   src/FDFK2D.f90:330-355 (time slowest, 0:nt).
  * Only the free-surface configuration (is_PML_top = .false., izt = 1) is
- * supported by this build.
+ * supported; a top PML is refused with FDFK_ENOSYS.
  */
 #ifndef FDFK_H
 #define FDFK_H
