@@ -20,7 +20,7 @@
 static inline int imin(int a, int b) { return a < b ? a : b; }
 static inline int imax(int a, int b) { return a > b ? a : b; }
 /* Fortran REAL(4) accumulate with a REAL(8) coefficient */
-static inline float acc(float s, double c, float d) { return (float)((double)s + c * (double)d); }
+static inline orc_real acc(orc_real s, double c, orc_real d) { return (orc_real)((double)s + c * (double)d); }
 
 /* ------------------------------------------------------------------ */
 /* FD coefficients, src/Submain.f90:1111-1243                          */
@@ -86,23 +86,23 @@ void orc_fd_coef2(int norder, double *coef)
 }
 
 /* src/Submain.f90:118-164 */
-int orc_check_stability(int nx, int nz, int npml, int izt, int norder, float dx, float dz,
-                        double dt, const double *coef2, const float *c, float *dt_stable)
+int orc_check_stability(int nx, int nz, int npml, int izt, int norder, orc_real dx, orc_real dz,
+                        double dt, const double *coef2, const orc_real *c, orc_real *dt_stable)
 {
-    float h = dx > dz ? dx : dz;
-    float cx = (h / dx) * (h / dx);
-    float cz = (h / dz) * (h / dz);
+    orc_real h = dx > dz ? dx : dz;
+    orc_real cx = (h / dx) * (h / dx);
+    orc_real cz = (h / dz) * (h / dz);
     double sa = 0.0, s = 0.0;
     for (int j = 1; j <= norder; ++j) sa += fabs(CO(coef2, j, norder));
     for (int j = 1; j <= norder; ++j) s += CO(coef2, j, norder);
-    float cfl = (float)sqrt(2.0 / ((double)(cx + cz) * (sa + s)));
+    orc_real cfl = (orc_real)sqrt(2.0 / ((double)(cx + cz) * (sa + s)));
     long ncell = (long)(nz + npml - izt + 1) * (long)(nx + 2 * npml);
-    float vpmax = -INFINITY;
+    orc_real vpmax = -INFINITY;
     for (long i = 0; i < ncell; ++i) {
-        float v = sqrtf((c[3 * i + 1] + 2.0f * c[3 * i + 2]) / c[3 * i + 0]);
+        orc_real v = sqrtf((c[3 * i + 1] + 2.0f * c[3 * i + 2]) / c[3 * i + 0]);
         if (v > vpmax) vpmax = v;
     }
-    float dts = cfl * h / vpmax;
+    orc_real dts = cfl * h / vpmax;
     *dt_stable = dts;
     if (dt > (double)dts) return 1;
     if (dt < (double)(0.4f * dts)) return 2;
@@ -114,15 +114,15 @@ int orc_check_stability(int nx, int nz, int npml, int izt, int norder, float dx,
 #define A2(a, iz, ix) (a)[(size_t)((iz) - izt) + (size_t)NZ * (size_t)((ix) + npml - 1)]
 
 /* src/Submain.f90:167-212 */
-float orc_diff1(int norder, int dire, int nx, int nz, int npml, int izt, int ix, int iz,
-                float d, const double *coef, const float *u)
+orc_real orc_diff1(int norder, int dire, int nx, int nz, int npml, int izt, int ix, int iz,
+                orc_real d, const double *coef, const orc_real *u)
 {
     DIMS;
     int nox = imin(imin(ix + npml - 1, nx + npml - ix), norder);
     int noz = imin(imin(iz - izt, nz + npml - iz), norder);
     int kx = 2 - dire, kz = dire - 1;
     int node = kx * nox + kz * noz;
-    float s = 0.0f;
+    orc_real s = 0.0f;
     for (int k = 1; k <= node; ++k) {
         int lx2 = ix + kx * k, lx1 = ix - kx * k, lz2 = iz + kz * k, lz1 = iz - kz * k;
         s = acc(s, CO(coef, k, node), A2(u, lz2, lx2) - A2(u, lz1, lx1));
@@ -131,10 +131,10 @@ float orc_diff1(int norder, int dire, int nx, int nz, int npml, int izt, int ix,
 }
 
 /* src/Submain.f90:215-249; u(1-npml:nx+npml) */
-float orc_diffx1(int norder, int nx, int npml, int ix, float dx, const double *coef, const float *u)
+orc_real orc_diffx1(int norder, int nx, int npml, int ix, orc_real dx, const double *coef, const orc_real *u)
 {
     int nox = imin(imin(ix + npml - 1, nx + npml - ix), norder);
-    float s = 0.0f;
+    orc_real s = 0.0f;
     for (int k = 1; k <= nox; ++k)
         s = acc(s, CO(coef, k, nox), u[ix + k + npml - 1] - u[ix - k + npml - 1]);
     return s / dx;
@@ -143,27 +143,27 @@ float orc_diffx1(int norder, int nx, int npml, int ix, float dx, const double *c
 /* ------------------------------------------------------------------ */
 /* mixed operators, src/Submain.f90:251-762                            */
 /* ------------------------------------------------------------------ */
-void orc_diffzx_reg(int norder, int nx, int nz, int npml, int izt, float dx, float dz,
-                    const double *coef, const float *mat, const float *u, float *out)
+void orc_diffzx_reg(int norder, int nx, int nz, int npml, int izt, orc_real dx, orc_real dz,
+                    const double *coef, const orc_real *mat, const orc_real *u, orc_real *out)
 {
     DIMS;
     size_t n = (size_t)NZ * (size_t)(nx + 2 * npml);
-    float *v = (float *)calloc(n, sizeof(float));
+    orc_real *v = (orc_real *)calloc(n, sizeof(orc_real));
 #pragma omp parallel for schedule(dynamic)
     for (int iz = 2; iz <= nz; ++iz) {
         int noz = imin(imin(iz - izt, nz + npml - iz), norder);
         for (int ix = 1 - norder; ix <= nx + norder; ++ix) {
-            float s = 0.0f;
+            orc_real s = 0.0f;
             for (int k = 1; k <= noz; ++k) s = acc(s, CO(coef, k, noz), A2(u, iz + k, ix) - A2(u, iz - k, ix));
             A2(v, iz, ix) = A2(mat, iz, ix) * s / dz;
         }
     }
-    memset(out, 0, n * sizeof(float));
+    memset(out, 0, n * sizeof(orc_real));
 #pragma omp parallel for schedule(dynamic)
     for (int ix = 1; ix <= nx; ++ix) {
         int nox = imin(imin(ix + npml - 1, nx + npml - ix), norder);
         for (int iz = 2; iz <= nz; ++iz) {
-            float s = 0.0f;
+            orc_real s = 0.0f;
             for (int k = 1; k <= nox; ++k) s = acc(s, CO(coef, k, nox), A2(v, iz, ix + k) - A2(v, iz, ix - k));
             A2(out, iz, ix) = s / dx;
         }
@@ -171,30 +171,30 @@ void orc_diffzx_reg(int norder, int nx, int nz, int npml, int izt, float dx, flo
     free(v);
 }
 
-void orc_diffzx_pml(int norder, int nx, int nz, int npml, int izt, float dx, float dz,
-                    const double *coef, const float *mat, const float *u, float *out)
+void orc_diffzx_pml(int norder, int nx, int nz, int npml, int izt, orc_real dx, orc_real dz,
+                    const double *coef, const orc_real *mat, const orc_real *u, orc_real *out)
 {
     DIMS;
     size_t n = (size_t)NZ * (size_t)(nx + 2 * npml);
-    float *v = (float *)calloc(n, sizeof(float));
+    orc_real *v = (orc_real *)calloc(n, sizeof(orc_real));
 #pragma omp parallel for schedule(dynamic)
     for (int iz = izt + 1; iz <= nz + npml - 1; ++iz) {
         int noz = imin(imin(iz - izt, nz + npml - iz), norder);
         for (int ix = 2 - npml; ix <= nx + npml - 1; ++ix) {
             /* mask: 1 except (1:nz, norder+1:nx-norder) and row izt */
             if (iz >= 1 && iz <= nz && ix >= norder + 1 && ix <= nx - norder) continue;
-            float s = 0.0f;
+            orc_real s = 0.0f;
             for (int k = 1; k <= noz; ++k) s = acc(s, CO(coef, k, noz), A2(u, iz + k, ix) - A2(u, iz - k, ix));
             A2(v, iz, ix) = A2(mat, iz, ix) * s / dz;
         }
     }
-    memset(out, 0, n * sizeof(float));
+    memset(out, 0, n * sizeof(orc_real));
 #pragma omp parallel for schedule(dynamic)
     for (int ix = 2 - npml; ix <= nx + npml - 1; ++ix) {
         int nox = imin(imin(ix + npml - 1, nx + npml - ix), norder);
         for (int iz = izt + 1; iz <= nz + npml - 1; ++iz) {
             if (iz >= 1 && iz <= nz && ix >= 1 && ix <= nx) continue;
-            float s = 0.0f;
+            orc_real s = 0.0f;
             for (int k = 1; k <= nox; ++k) s = acc(s, CO(coef, k, nox), A2(v, iz, ix + k) - A2(v, iz, ix - k));
             A2(out, iz, ix) = s / dx;
         }
@@ -202,27 +202,27 @@ void orc_diffzx_pml(int norder, int nx, int nz, int npml, int izt, float dx, flo
     free(v);
 }
 
-void orc_diffxz_reg(int norder, int nx, int nz, int npml, int izt, float dx, float dz,
-                    const double *coef, const float *mat, const float *u, float *out)
+void orc_diffxz_reg(int norder, int nx, int nz, int npml, int izt, orc_real dx, orc_real dz,
+                    const double *coef, const orc_real *mat, const orc_real *u, orc_real *out)
 {
     DIMS;
     size_t n = (size_t)NZ * (size_t)(nx + 2 * npml);
-    float *v = (float *)calloc(n, sizeof(float));
+    orc_real *v = (orc_real *)calloc(n, sizeof(orc_real));
 #pragma omp parallel for schedule(dynamic)
     for (int ix = 1; ix <= nx; ++ix) {
         int nox = imin(imin(ix + npml - 1, nx + npml - ix), norder);
         for (int iz = 1; iz <= nz + norder; ++iz) {
-            float s = 0.0f;
+            orc_real s = 0.0f;
             for (int k = 1; k <= nox; ++k) s = acc(s, CO(coef, k, nox), A2(u, iz, ix + k) - A2(u, iz, ix - k));
             A2(v, iz, ix) = A2(mat, iz, ix) * s / dx;
         }
     }
-    memset(out, 0, n * sizeof(float));
+    memset(out, 0, n * sizeof(orc_real));
 #pragma omp parallel for schedule(dynamic)
     for (int iz = 2; iz <= nz; ++iz) {
         int noz = imin(imin(iz - izt, nz + npml - iz), norder);
         for (int ix = 1; ix <= nx; ++ix) {
-            float s = 0.0f;
+            orc_real s = 0.0f;
             for (int k = 1; k <= noz; ++k) s = acc(s, CO(coef, k, noz), A2(v, iz + k, ix) - A2(v, iz - k, ix));
             A2(out, iz, ix) = s / dz;
         }
@@ -230,29 +230,29 @@ void orc_diffxz_reg(int norder, int nx, int nz, int npml, int izt, float dx, flo
     free(v);
 }
 
-void orc_diffxz_pml(int norder, int nx, int nz, int npml, int izt, float dx, float dz,
-                    const double *coef, const float *mat, const float *u, float *out)
+void orc_diffxz_pml(int norder, int nx, int nz, int npml, int izt, orc_real dx, orc_real dz,
+                    const double *coef, const orc_real *mat, const orc_real *u, orc_real *out)
 {
     DIMS;
     size_t n = (size_t)NZ * (size_t)(nx + 2 * npml);
-    float *v = (float *)calloc(n, sizeof(float));
+    orc_real *v = (orc_real *)calloc(n, sizeof(orc_real));
 #pragma omp parallel for schedule(dynamic)
     for (int ix = 2 - npml; ix <= nx + npml - 1; ++ix) {
         int nox = imin(imin(ix + npml - 1, nx + npml - ix), norder);
         for (int iz = izt; iz <= nz + npml - 1; ++iz) {
             if (iz >= 1 && iz <= nz - norder && ix >= 1 && ix <= nx) continue;
-            float s = 0.0f;
+            orc_real s = 0.0f;
             for (int k = 1; k <= nox; ++k) s = acc(s, CO(coef, k, nox), A2(u, iz, ix + k) - A2(u, iz, ix - k));
             A2(v, iz, ix) = A2(mat, iz, ix) * s / dx;
         }
     }
-    memset(out, 0, n * sizeof(float));
+    memset(out, 0, n * sizeof(orc_real));
 #pragma omp parallel for schedule(dynamic)
     for (int iz = izt + 1; iz <= nz + npml - 1; ++iz) {
         int noz = imin(imin(iz - izt, nz + npml - iz), norder);
         for (int ix = 2 - npml; ix <= nx + npml - 1; ++ix) {
             if (iz >= 1 && iz <= nz && ix >= 1 && ix <= nx) continue;
-            float s = 0.0f;
+            orc_real s = 0.0f;
             for (int k = 1; k <= noz; ++k) s = acc(s, CO(coef, k, noz), A2(v, iz + k, ix) - A2(v, iz - k, ix));
             A2(out, iz, ix) = s / dz;
         }
@@ -263,55 +263,55 @@ void orc_diffxz_pml(int norder, int nx, int nz, int npml, int izt, float dx, flo
 /* ------------------------------------------------------------------ */
 /* second-derivative operators, src/Submain.f90:764-1109               */
 /* ------------------------------------------------------------------ */
-static inline float hmean(float a, float b) { return 2.0f / (1.0f / a + 1.0f / b); }
+static inline orc_real hmean(orc_real a, orc_real b) { return 2.0f / (1.0f / a + 1.0f / b); }
 
 /* one output of diffx2 at (iz, ix): the body of the loops at :815-837 */
-static inline float x2_point(int norder, int nx, int npml, int izt, int NZ, const double *coef,
-                             const float *mat, const float *u, int iz, int ix)
+static inline orc_real x2_point(int norder, int nx, int npml, int izt, int NZ, const double *coef,
+                             const orc_real *mat, const orc_real *u, int iz, int ix)
 {
     int nox = imin(imin(ix + npml - 1, nx + npml - ix), norder);
-    float s = 0.0f;
+    orc_real s = 0.0f;
     for (int k = 1; k <= nox; ++k) {
         int i = (2 * k - 1) / 2;
         int n1 = imin(imin(ix + i + npml - 1, nx + npml - ix - i), norder);
-        float s1 = 0.0f;
+        orc_real s1 = 0.0f;
         for (int j = 1; j <= n1; ++j) s1 = acc(s1, CO(coef, j, n1), A2(u, iz, ix + j + k - 1) - A2(u, iz, ix - j + k));
         int n2 = imin(imin(ix - i + npml - 1, nx + npml - ix + i), norder);
-        float s2 = 0.0f;
+        orc_real s2 = 0.0f;
         for (int j = 1; j <= n2; ++j) s2 = acc(s2, CO(coef, j, n2), A2(u, iz, ix + j - k) - A2(u, iz, ix - j - k + 1));
-        float t = hmean(A2(mat, iz, ix + k), A2(mat, iz, ix + k - 1)) * s1 -
+        orc_real t = hmean(A2(mat, iz, ix + k), A2(mat, iz, ix + k - 1)) * s1 -
                   hmean(A2(mat, iz, ix - k + 1), A2(mat, iz, ix - k)) * s2;
         s = acc(s, CO(coef, k, nox), t);
     }
     return s;
 }
 
-static inline float z2_point(int norder, int nz, int npml, int izt, int NZ, const double *coef,
-                             const float *mat, const float *u, int iz, int ix)
+static inline orc_real z2_point(int norder, int nz, int npml, int izt, int NZ, const double *coef,
+                             const orc_real *mat, const orc_real *u, int iz, int ix)
 {
     int noz = imin(imin(iz - izt, nz + npml - iz), norder);
-    float s = 0.0f;
+    orc_real s = 0.0f;
     for (int k = 1; k <= noz; ++k) {
         int i = (2 * k - 1) / 2;
         int n1 = imin(imin(iz + i - izt, nz + npml - iz - i), norder);
-        float s1 = 0.0f;
+        orc_real s1 = 0.0f;
         for (int j = 1; j <= n1; ++j) s1 = acc(s1, CO(coef, j, n1), A2(u, iz + j + k - 1, ix) - A2(u, iz - j + k, ix));
         int n2 = imin(imin(iz - i - izt, nz + npml - iz + i), norder);
-        float s2 = 0.0f;
+        orc_real s2 = 0.0f;
         for (int j = 1; j <= n2; ++j) s2 = acc(s2, CO(coef, j, n2), A2(u, iz + j - k, ix) - A2(u, iz - j - k + 1, ix));
-        float t = hmean(A2(mat, iz + k, ix), A2(mat, iz + k - 1, ix)) * s1 -
+        orc_real t = hmean(A2(mat, iz + k, ix), A2(mat, iz + k - 1, ix)) * s1 -
                   hmean(A2(mat, iz - k + 1, ix), A2(mat, iz - k, ix)) * s2;
         s = acc(s, CO(coef, k, noz), t);
     }
     return s;
 }
 
-void orc_diffx2_reg(int norder, int nx, int nz, int npml, int izt, float dx,
-                    const double *coef, const float *mat, const float *u, float *out)
+void orc_diffx2_reg(int norder, int nx, int nz, int npml, int izt, orc_real dx,
+                    const double *coef, const orc_real *mat, const orc_real *u, orc_real *out)
 {
     DIMS;
-    memset(out, 0, (size_t)NZ * (size_t)(nx + 2 * npml) * sizeof(float));
-    float dx2 = (float)(1.0 / (double)(dx * dx));
+    memset(out, 0, (size_t)NZ * (size_t)(nx + 2 * npml) * sizeof(orc_real));
+    orc_real dx2 = (orc_real)(1.0 / (double)(dx * dx));
 #pragma omp parallel for schedule(dynamic)
     for (int ix = 1; ix <= nx; ++ix)
         for (int iz = izt; iz <= nz; ++iz) {
@@ -320,12 +320,12 @@ void orc_diffx2_reg(int norder, int nx, int nz, int npml, int izt, float dx,
         }
 }
 
-void orc_diffx2_pml(int norder, int nx, int nz, int npml, int izt, float dx,
-                    const double *coef, const float *mat, const float *u, float *out)
+void orc_diffx2_pml(int norder, int nx, int nz, int npml, int izt, orc_real dx,
+                    const double *coef, const orc_real *mat, const orc_real *u, orc_real *out)
 {
     DIMS;
-    memset(out, 0, (size_t)NZ * (size_t)(nx + 2 * npml) * sizeof(float));
-    float dx2 = 1.0f / (dx * dx);
+    memset(out, 0, (size_t)NZ * (size_t)(nx + 2 * npml) * sizeof(orc_real));
+    orc_real dx2 = 1.0f / (dx * dx);
 #pragma omp parallel for schedule(dynamic)
     for (int ix = 2 - npml; ix <= nx + npml - 1; ++ix)
         for (int iz = izt; iz <= nz + npml - 1; ++iz) {
@@ -334,24 +334,24 @@ void orc_diffx2_pml(int norder, int nx, int nz, int npml, int izt, float dx,
         }
 }
 
-void orc_diffz2_reg(int norder, int nx, int nz, int npml, int izt, float dz,
-                    const double *coef, const float *mat, const float *u, float *out)
+void orc_diffz2_reg(int norder, int nx, int nz, int npml, int izt, orc_real dz,
+                    const double *coef, const orc_real *mat, const orc_real *u, orc_real *out)
 {
     DIMS;
-    memset(out, 0, (size_t)NZ * (size_t)(nx + 2 * npml) * sizeof(float));
-    float dz2 = 1.0f / (dz * dz);
+    memset(out, 0, (size_t)NZ * (size_t)(nx + 2 * npml) * sizeof(orc_real));
+    orc_real dz2 = 1.0f / (dz * dz);
 #pragma omp parallel for schedule(dynamic)
     for (int iz = 2; iz <= nz; ++iz)
         for (int ix = 1; ix <= nx; ++ix)
             A2(out, iz, ix) = z2_point(norder, nz, npml, izt, NZ, coef, mat, u, iz, ix) * dz2;
 }
 
-void orc_diffz2_pml(int norder, int nx, int nz, int npml, int izt, float dz,
-                    const double *coef, const float *mat, const float *u, float *out)
+void orc_diffz2_pml(int norder, int nx, int nz, int npml, int izt, orc_real dz,
+                    const double *coef, const orc_real *mat, const orc_real *u, orc_real *out)
 {
     DIMS;
-    memset(out, 0, (size_t)NZ * (size_t)(nx + 2 * npml) * sizeof(float));
-    float dz2 = 1.0f / (dz * dz);
+    memset(out, 0, (size_t)NZ * (size_t)(nx + 2 * npml) * sizeof(orc_real));
+    orc_real dz2 = 1.0f / (dz * dz);
 #pragma omp parallel for schedule(dynamic)
     for (int iz = izt + 1; iz <= nz + npml - 1; ++iz)
         for (int ix = 1 - npml; ix <= nx + npml; ++ix) {
@@ -377,23 +377,23 @@ static double dirac_w(double x)
     return 0.0;
 }
 
-void orc_loadseis(int nx, int nz, int npml, int izt, float x0, float z0, float dx, float dz,
-                  float rx, float rz, const float *ux, const float *uz, float *dsx, float *dsz)
+void orc_loadseis(int nx, int nz, int npml, int izt, orc_real x0, orc_real z0, orc_real dx, orc_real dz,
+                  orc_real rx, orc_real rz, const orc_real *ux, const orc_real *uz, orc_real *dsx, orc_real *dsz)
 {
     int NZ = nz + npml - izt + 1;
     int irx = (int)floorf((rx - x0) / dx) + 1;
     int irz = (int)floorf((rz - z0) / dz) + 1;
-    float sx = 0.0f, sz = 0.0f;
+    orc_real sx = 0.0f, sz = 0.0f;
     for (int ix = imax(1 - npml, irx - 3); ix <= imin(nx + npml, irx + 3); ++ix) {
-        double x = fabs((double)(x0 + (float)(ix - 1) * dx) - (double)rx) / (double)dx;
+        double x = fabs((double)(x0 + (orc_real)(ix - 1) * dx) - (double)rx) / (double)dx;
         double bax = dirac_w(x);
         for (int iz = imax(izt, irz - 3); iz <= imin(nz + npml, irz + 3); ++iz) {
-            double z = fabs((double)(z0 + (float)(iz - 1) * dz) - (double)rz) / (double)dz;
+            double z = fabs((double)(z0 + (orc_real)(iz - 1) * dz) - (double)rz) / (double)dz;
             double baz = dirac_w(z);
             double ba = bax * baz;
             size_t off = 2 * ((size_t)(iz - izt) + (size_t)NZ * (size_t)(ix + npml - 1));
-            sx = (float)((double)sx + ba * (double)ux[off]);
-            sz = (float)((double)sz + ba * (double)uz[off]);
+            sx = (orc_real)((double)sx + ba * (double)ux[off]);
+            sz = (orc_real)((double)sz + ba * (double)uz[off]);
         }
     }
     *dsx = sx;
@@ -401,11 +401,11 @@ void orc_loadseis(int nx, int nz, int npml, int izt, float x0, float z0, float d
 }
 
 /* src/Input.f90:364-378 */
-void orc_material_law(long ncell, float *c)
+void orc_material_law(long ncell, orc_real *c)
 {
     for (long i = 0; i < ncell; ++i) {
-        float vp = c[3 * i + 1], vs = c[3 * i + 2];
-        float rho = (vp + 980.0f) / 2.760f;
+        orc_real vp = c[3 * i + 1], vs = c[3 * i + 2];
+        orc_real rho = (vp + 980.0f) / 2.760f;
         c[3 * i + 0] = rho;
         c[3 * i + 1] = rho * (vp * vp - 2.0f * vs * vs);
         c[3 * i + 2] = rho * vs * vs;
@@ -440,7 +440,7 @@ void orc_backaz(double evla, double evlo, double stla, double stlo, double *baz)
 }
 
 void orc_backazimuth_grids(double lat_org, double lon_org, double az_org, int n,
-                           const float *coordx, double evla, double evlo, double *bazs)
+                           const orc_real *coordx, double evla, double evlo, double *bazs)
 {
     const double deg2rad = O_PI / 180.0, rad2deg = 180.0 / O_PI;
     double az = az_org * deg2rad, lat = lat_org * deg2rad, lon = lon_org * deg2rad;
@@ -508,7 +508,7 @@ void orc_backazimuth_grids(double lat_org, double lon_org, double az_org, int n,
 
 typedef struct {
     int norder, nx, nz, npml, izt, pml_top, npower, nbd, it, nt;
-    float dx, dz;
+    orc_real dx, dz;
     double dt;
     int NZ, NX;
 } gdims;
@@ -518,8 +518,8 @@ static inline size_t g2(const gdims *g, int iz, int ix)
     return (size_t)(iz - g->izt) + (size_t)g->NZ * (size_t)(ix + g->npml - 1);
 }
 
-void orc_damping(int nx, int nz, int npml, int nbd, int is_pml_top, int npower, float dx,
-                 float dz, float vpml, double R0, float *dpx, float *dpz)
+void orc_damping(int nx, int nz, int npml, int nbd, int is_pml_top, int npower, orc_real dx,
+                 orc_real dz, orc_real vpml, double R0, orc_real *dpx, orc_real *dpz)
 {
     int izt = is_pml_top ? 1 - npml : 1;
     int NX = nx + 2 * npml, NZ = nz + npml - izt + 1;
@@ -537,7 +537,7 @@ void orc_damping(int nx, int nz, int npml, int nbd, int is_pml_top, int npower, 
             for (int e = 0; e < npower; ++e) p *= r;
             d = d0x * p;
         }
-        dpx[j] = (float)d;
+        dpx[j] = (orc_real)d;
     }
     for (int j = 0; j < NZ; ++j) {
         int iz = j + izt;
@@ -554,78 +554,78 @@ void orc_damping(int nx, int nz, int npml, int nbd, int is_pml_top, int npower, 
             for (int e = 0; e < npower; ++e) p *= r;
             d = d0t * p;
         }
-        dpz[j] = (float)d;
+        dpz[j] = (orc_real)d;
     }
 }
 
 /* central first derivative along x / z with the reduced-order rule of
  * diff1 (Submain.f90:192-209), times nothing; returns mysum (not divided) */
-static inline float cdx_sum(const gdims *g, const double *coef1, const float *f, int iz, int ix)
+static inline orc_real cdx_sum(const gdims *g, const double *coef1, const orc_real *f, int iz, int ix)
 {
     int norder = g->norder;
     int nox = imin(imin(ix + g->npml - 1, g->nx + g->npml - ix), norder);
-    float s = 0.0f;
+    orc_real s = 0.0f;
     for (int k = 1; k <= nox; ++k) s = acc(s, CO(coef1, k, nox), f[g2(g, iz, ix + k)] - f[g2(g, iz, ix - k)]);
     return s;
 }
-static inline float cdz_sum(const gdims *g, const double *coef1, const float *f, int iz, int ix)
+static inline orc_real cdz_sum(const gdims *g, const double *coef1, const orc_real *f, int iz, int ix)
 {
     int norder = g->norder;
     int noz = imin(imin(iz - g->izt, g->nz + g->npml - iz), norder);
-    float s = 0.0f;
+    orc_real s = 0.0f;
     for (int k = 1; k <= noz; ++k) s = acc(s, CO(coef1, k, noz), f[g2(g, iz + k, ix)] - f[g2(g, iz - k, ix)]);
     return s;
 }
 
 /* modulus of the second-derivative terms: lam+2mu (mode 0) or mu (mode 1) */
-static inline float matval(int mode, float lam, float mu)
+static inline orc_real matval(int mode, orc_real lam, orc_real mu)
 {
     return mode == 0 ? lam + 2.0f * mu : mu;
 }
 
-static float x2_point_mode(const gdims *g, const double *coef0, const float *lam, const float *mu,
-                           int mode, const float *u, int iz, int ix)
+static orc_real x2_point_mode(const gdims *g, const double *coef0, const orc_real *lam, const orc_real *mu,
+                           int mode, const orc_real *u, int iz, int ix)
 {
     int norder = g->norder, nx = g->nx, npml = g->npml;
     int nox = imin(imin(ix + npml - 1, nx + npml - ix), norder);
-    float s = 0.0f;
+    orc_real s = 0.0f;
 #define MAT(jx) matval(mode, lam[g2(g, iz, (jx))], mu[g2(g, iz, (jx))])
 #define U(jx) u[g2(g, iz, (jx))]
     for (int k = 1; k <= nox; ++k) {
         int i = (2 * k - 1) / 2;
         int n1 = imin(imin(ix + i + npml - 1, nx + npml - ix - i), norder);
-        float s1 = 0.0f;
+        orc_real s1 = 0.0f;
         for (int j = 1; j <= n1; ++j) s1 = acc(s1, CO(coef0, j, n1), U(ix + j + k - 1) - U(ix - j + k));
         int n2 = imin(imin(ix - i + npml - 1, nx + npml - ix + i), norder);
-        float s2 = 0.0f;
+        orc_real s2 = 0.0f;
         for (int j = 1; j <= n2; ++j) s2 = acc(s2, CO(coef0, j, n2), U(ix + j - k) - U(ix - j - k + 1));
-        float t = hmean(MAT(ix + k), MAT(ix + k - 1)) * s1 - hmean(MAT(ix - k + 1), MAT(ix - k)) * s2;
+        orc_real t = hmean(MAT(ix + k), MAT(ix + k - 1)) * s1 - hmean(MAT(ix - k + 1), MAT(ix - k)) * s2;
         s = acc(s, CO(coef0, k, nox), t);
     }
 #undef MAT
 #undef U
     int in_reg = (iz >= 1 && iz <= g->nz && ix >= 1 && ix <= nx);
-    float dx2 = in_reg ? (float)(1.0 / (double)(g->dx * g->dx)) : 1.0f / (g->dx * g->dx);
+    orc_real dx2 = in_reg ? (orc_real)(1.0 / (double)(g->dx * g->dx)) : 1.0f / (g->dx * g->dx);
     return s * dx2;
 }
 
-static float z2_point_mode(const gdims *g, const double *coef0, const float *lam, const float *mu,
-                           int mode, const float *u, int iz, int ix)
+static orc_real z2_point_mode(const gdims *g, const double *coef0, const orc_real *lam, const orc_real *mu,
+                           int mode, const orc_real *u, int iz, int ix)
 {
     int norder = g->norder, nz = g->nz, npml = g->npml, izt = g->izt;
     int noz = imin(imin(iz - izt, nz + npml - iz), norder);
-    float s = 0.0f;
+    orc_real s = 0.0f;
 #define MAT(jz) matval(mode, lam[g2(g, (jz), ix)], mu[g2(g, (jz), ix)])
 #define U(jz) u[g2(g, (jz), ix)]
     for (int k = 1; k <= noz; ++k) {
         int i = (2 * k - 1) / 2;
         int n1 = imin(imin(iz + i - izt, nz + npml - iz - i), norder);
-        float s1 = 0.0f;
+        orc_real s1 = 0.0f;
         for (int j = 1; j <= n1; ++j) s1 = acc(s1, CO(coef0, j, n1), U(iz + j + k - 1) - U(iz - j + k));
         int n2 = imin(imin(iz - i - izt, nz + npml - iz + i), norder);
-        float s2 = 0.0f;
+        orc_real s2 = 0.0f;
         for (int j = 1; j <= n2; ++j) s2 = acc(s2, CO(coef0, j, n2), U(iz + j - k) - U(iz - j - k + 1));
-        float t = hmean(MAT(iz + k), MAT(iz + k - 1)) * s1 - hmean(MAT(iz - k + 1), MAT(iz - k)) * s2;
+        orc_real t = hmean(MAT(iz + k), MAT(iz + k - 1)) * s1 - hmean(MAT(iz - k + 1), MAT(iz - k)) * s2;
         s = acc(s, CO(coef0, k, noz), t);
     }
 #undef MAT
@@ -634,56 +634,56 @@ static float z2_point_mode(const gdims *g, const double *coef0, const float *lam
 }
 
 /* Dx( mat * Dz u ): diffzx at one output cell */
-static float zx_point(const gdims *g, const double *coef1, const float *mat, const float *u, int iz, int ix)
+static orc_real zx_point(const gdims *g, const double *coef1, const orc_real *mat, const orc_real *u, int iz, int ix)
 {
     int norder = g->norder;
     int nox = imin(imin(ix + g->npml - 1, g->nx + g->npml - ix), norder);
-    float s = 0.0f;
+    orc_real s = 0.0f;
     for (int k = 1; k <= nox; ++k) {
-        float vp = mat[g2(g, iz, ix + k)] * cdz_sum(g, coef1, u, iz, ix + k) / g->dz;
-        float vm = mat[g2(g, iz, ix - k)] * cdz_sum(g, coef1, u, iz, ix - k) / g->dz;
+        orc_real vp = mat[g2(g, iz, ix + k)] * cdz_sum(g, coef1, u, iz, ix + k) / g->dz;
+        orc_real vm = mat[g2(g, iz, ix - k)] * cdz_sum(g, coef1, u, iz, ix - k) / g->dz;
         s = acc(s, CO(coef1, k, nox), vp - vm);
     }
     return s / g->dx;
 }
 /* Dz( mat * Dx u ): diffxz at one output cell */
-static float xz_point(const gdims *g, const double *coef1, const float *mat, const float *u, int iz, int ix)
+static orc_real xz_point(const gdims *g, const double *coef1, const orc_real *mat, const orc_real *u, int iz, int ix)
 {
     int norder = g->norder;
     int noz = imin(imin(iz - g->izt, g->nz + g->npml - iz), norder);
-    float s = 0.0f;
+    orc_real s = 0.0f;
     for (int k = 1; k <= noz; ++k) {
-        float vp = mat[g2(g, iz + k, ix)] * cdx_sum(g, coef1, u, iz + k, ix) / g->dx;
-        float vm = mat[g2(g, iz - k, ix)] * cdx_sum(g, coef1, u, iz - k, ix) / g->dx;
+        orc_real vp = mat[g2(g, iz + k, ix)] * cdx_sum(g, coef1, u, iz + k, ix) / g->dx;
+        orc_real vm = mat[g2(g, iz - k, ix)] * cdx_sum(g, coef1, u, iz - k, ix) / g->dx;
         s = acc(s, CO(coef1, k, noz), vp - vm);
     }
     return s / g->dz;
 }
 
 /* the six RHS parts at one updated cell; lam/mu 2-D planes */
-static void eval_parts_g(const gdims *g, const double *coef0, const double *coef1, const float *lam,
-                         const float *mu, const float *ux, const float *uz, int iz, int ix, float p[6])
+static void eval_parts_g(const gdims *g, const double *coef0, const double *coef1, const orc_real *lam,
+                         const orc_real *mu, const orc_real *ux, const orc_real *uz, int iz, int ix, orc_real p[6])
 {
     if (iz == g->izt && g->izt == 1) {
         /* free-surface row: half-cell closure from the strain energy of the
          * top layer (rows 1-2, trapezoidal in z), traction free naturally */
         size_t a = g2(g, 1, ix), b = g2(g, 2, ix);
-        float sdz2 = 2.0f / (g->dz * g->dz);
-        float M1 = lam[a] + 2.0f * mu[a], M2 = lam[b] + 2.0f * mu[b];
+        orc_real sdz2 = 2.0f / (g->dz * g->dz);
+        orc_real M1 = lam[a] + 2.0f * mu[a], M2 = lam[b] + 2.0f * mu[b];
         int nox = imin(imin(ix + g->npml - 1, g->nx + g->npml - ix), g->norder);
         int norder = g->norder;
         /* Dx( lbar (uz2-uz1)/dz ) and Dx( mbar (ux2-ux1)/dz ) */
-        float sl = 0.0f, sm = 0.0f;
+        orc_real sl = 0.0f, sm = 0.0f;
         for (int k = 1; k <= nox; ++k) {
             size_t a1 = g2(g, 1, ix + k), b1 = g2(g, 2, ix + k), a2 = g2(g, 1, ix - k), b2 = g2(g, 2, ix - k);
-            float lp = 0.5f * (lam[a1] + lam[b1]), lm = 0.5f * (lam[a2] + lam[b2]);
-            float mp = 0.5f * (mu[a1] + mu[b1]), mm = 0.5f * (mu[a2] + mu[b2]);
+            orc_real lp = 0.5f * (lam[a1] + lam[b1]), lm = 0.5f * (lam[a2] + lam[b2]);
+            orc_real mp = 0.5f * (mu[a1] + mu[b1]), mm = 0.5f * (mu[a2] + mu[b2]);
             sl = acc(sl, CO(coef1, k, nox), lp * (uz[b1] - uz[a1]) - lm * (uz[b2] - uz[a2]));
             sm = acc(sm, CO(coef1, k, nox), mp * (ux[b1] - ux[a1]) - mm * (ux[b2] - ux[a2]));
         }
-        float lbar = 0.5f * (lam[a] + lam[b]), mbar = 0.5f * (mu[a] + mu[b]);
-        float dxuz = (cdx_sum(g, coef1, uz, 1, ix) + cdx_sum(g, coef1, uz, 2, ix)) / g->dx;
-        float dxux = (cdx_sum(g, coef1, ux, 1, ix) + cdx_sum(g, coef1, ux, 2, ix)) / g->dx;
+        orc_real lbar = 0.5f * (lam[a] + lam[b]), mbar = 0.5f * (mu[a] + mu[b]);
+        orc_real dxuz = (cdx_sum(g, coef1, uz, 1, ix) + cdx_sum(g, coef1, uz, 2, ix)) / g->dx;
+        orc_real dxux = (cdx_sum(g, coef1, ux, 1, ix) + cdx_sum(g, coef1, ux, 2, ix)) / g->dx;
         p[0] = x2_point_mode(g, coef0, lam, mu, 0, ux, iz, ix);
         p[1] = (hmean(mu[a], mu[b]) * (ux[b] - ux[a])) * sdz2;
         p[2] = (sl / g->dx + mbar * dxuz) / g->dz;
@@ -700,10 +700,10 @@ static void eval_parts_g(const gdims *g, const double *coef0, const double *coef
     p[5] = zx_point(g, coef1, mu, ux, iz, ix) + xz_point(g, coef1, lam, ux, iz, ix);
 }
 
-void orc_eval_parts(int norder, int nx, int nz, int npml, int izt, float dx, float dz,
-                    const double *coef0, const double *coef1, const float *lam2d,
-                    const float *mu2d, const float *ux2d, const float *uz2d,
-                    int iz, int ix, float parts[6])
+void orc_eval_parts(int norder, int nx, int nz, int npml, int izt, orc_real dx, orc_real dz,
+                    const double *coef0, const double *coef1, const orc_real *lam2d,
+                    const orc_real *mu2d, const orc_real *ux2d, const orc_real *uz2d,
+                    int iz, int ix, orc_real parts[6])
 {
     gdims g;
     memset(&g, 0, sizeof g);
@@ -769,18 +769,18 @@ static inline long bg_off(const gdims *g, int iz, int ix, int it, int *arr, int 
 
 void orc_updatedispl(const int *norder_, const int *nx_, const int *nz_, const int *npml_,
     const int *izt_, const int *is_pml_top_, const int *npower_, const int *nbd_, const int *it_,
-    const int *nt_, const float *dx_, const float *dz_, const double *dt_, const float *vpml_,
+    const int *nt_, const orc_real *dx_, const orc_real *dz_, const double *dt_, const orc_real *vpml_,
     const double *R0_,
-    const float *uxl_reg, const float *uzl_reg, const float *uxr_reg, const float *uzr_reg,
-    const float *uxb_reg, const float *uzb_reg, const float *uxl_pml, const float *uzl_pml,
-    const float *uxr_pml, const float *uzr_pml, const float *uxb_pml, const float *uzb_pml,
-    const float *c, const double *coef0, const double *coef1, float *ux, float *uz,
-    float *bpxlx, float *bpzlx, float *bpxrx, float *bpzrx, float *bpxtx, float *bpxtz,
-    float *bpztx, float *bpztz, float *bpxbx, float *bpxbz, float *bpzbx, float *bpzbz,
-    float *bux1l, float *bux2l, float *bux3l, float *buz1l, float *buz2l, float *buz3l,
-    float *bux1r, float *bux2r, float *bux3r, float *buz1r, float *buz2r, float *buz3r,
-    float *bux1b, float *bux2b, float *bux3b, float *buz1b, float *buz2b, float *buz3b,
-    float *bux1t, float *bux2t, float *bux3t, float *buz1t, float *buz2t, float *buz3t)
+    const orc_real *uxl_reg, const orc_real *uzl_reg, const orc_real *uxr_reg, const orc_real *uzr_reg,
+    const orc_real *uxb_reg, const orc_real *uzb_reg, const orc_real *uxl_pml, const orc_real *uzl_pml,
+    const orc_real *uxr_pml, const orc_real *uzr_pml, const orc_real *uxb_pml, const orc_real *uzb_pml,
+    const orc_real *c, const double *coef0, const double *coef1, orc_real *ux, orc_real *uz,
+    orc_real *bpxlx, orc_real *bpzlx, orc_real *bpxrx, orc_real *bpzrx, orc_real *bpxtx, orc_real *bpxtz,
+    orc_real *bpztx, orc_real *bpztz, orc_real *bpxbx, orc_real *bpxbz, orc_real *bpzbx, orc_real *bpzbz,
+    orc_real *bux1l, orc_real *bux2l, orc_real *bux3l, orc_real *buz1l, orc_real *buz2l, orc_real *buz3l,
+    orc_real *bux1r, orc_real *bux2r, orc_real *bux3r, orc_real *buz1r, orc_real *buz2r, orc_real *buz3r,
+    orc_real *bux1b, orc_real *bux2b, orc_real *bux3b, orc_real *buz1b, orc_real *buz2b, orc_real *buz3b,
+    orc_real *bux1t, orc_real *bux2t, orc_real *bux3t, orc_real *buz1t, orc_real *buz2t, orc_real *buz3t)
 {
     gdims G, *g = &G;
     G.norder = *norder_; G.nx = *nx_; G.nz = *nz_; G.npml = *npml_; G.izt = *izt_;
@@ -792,9 +792,9 @@ void orc_updatedispl(const int *norder_, const int *nx_, const int *nz_, const i
     (void)norder;
 
     /* copy-in of the level-1 fields and the material planes */
-    float *u1x = malloc(ncell * sizeof(float)), *u1z = malloc(ncell * sizeof(float));
-    float *rho = malloc(ncell * sizeof(float)), *lam = malloc(ncell * sizeof(float));
-    float *mu = malloc(ncell * sizeof(float)), *lp2m = malloc(ncell * sizeof(float));
+    orc_real *u1x = malloc(ncell * sizeof(orc_real)), *u1z = malloc(ncell * sizeof(orc_real));
+    orc_real *rho = malloc(ncell * sizeof(orc_real)), *lam = malloc(ncell * sizeof(orc_real));
+    orc_real *mu = malloc(ncell * sizeof(orc_real)), *lp2m = malloc(ncell * sizeof(orc_real));
     /* material seen by the stencils: the damped part of the PML (more than
      * nbd cells into the padding) repeats the outermost undamped cell along
      * the normal, so the layers are normal-invariant (heterogeneous damped
@@ -808,12 +808,12 @@ void orc_updatedispl(const int *norder_, const int *nx_, const int *nz_, const i
             rho[i] = c[3 * j]; lam[i] = c[3 * j + 1]; mu[i] = c[3 * j + 2];
             lp2m[i] = lam[i] + 2.0f * mu[i];
         }
-    float *dpx = malloc(G.NX * sizeof(float)), *dpz = malloc(G.NZ * sizeof(float));
+    orc_real *dpx = malloc(G.NX * sizeof(orc_real)), *dpz = malloc(G.NZ * sizeof(orc_real));
     orc_damping(nx, nz, npml, nbd, G.pml_top, G.npower, G.dx, G.dz, *vpml_, *R0_, dpx, dpz);
 
-    float *bpx[4][2] = {{bpxlx, bpzlx}, {bpxrx, bpzrx}, {bpxbx, bpzbx}, {bpxtx, bpztx}};
-    float *bpz[4][2] = {{NULL, NULL}, {NULL, NULL}, {bpxbz, bpzbz}, {bpxtz, bpztz}};
-    float *bu[4][2][3] = {{{bux1l, bux2l, bux3l}, {buz1l, buz2l, buz3l}},
+    orc_real *bpx[4][2] = {{bpxlx, bpzlx}, {bpxrx, bpzrx}, {bpxbx, bpzbx}, {bpxtx, bpztx}};
+    orc_real *bpz[4][2] = {{NULL, NULL}, {NULL, NULL}, {bpxbz, bpzbz}, {bpxtz, bpztz}};
+    orc_real *bu[4][2][3] = {{{bux1l, bux2l, bux3l}, {buz1l, buz2l, buz3l}},
                           {{bux1r, bux2r, bux3r}, {buz1r, buz2r, buz3r}},
                           {{bux1b, bux2b, bux3b}, {buz1b, buz2b, buz3b}},
                           {{bux1t, bux2t, bux3t}, {buz1t, buz2t, buz3t}}};
@@ -830,18 +830,18 @@ void orc_updatedispl(const int *norder_, const int *nx_, const int *nz_, const i
         for (int ix = ix0; ix <= ix1; ++ix)
             for (int iz = iz0; iz <= iz1; ++iz) {
                 size_t o = strip_off(g, s, iz, ix);
-                float ddx = dpx[ix + npml - 1], ddz = dpz[iz - izt];
+                orc_real ddx = dpx[ix + npml - 1], ddz = dpz[iz - izt];
                 if (ddx > 0.0f) {
-                    float b = (float)exp(-(double)ddx * G.dt), a = 1.0f - b;
-                    float gx = cdx_sum(g, coef1, u1x, iz, ix) / G.dx;
-                    float gz = cdx_sum(g, coef1, u1z, iz, ix) / G.dx;
+                    orc_real b = (orc_real)exp(-(double)ddx * G.dt), a = 1.0f - b;
+                    orc_real gx = cdx_sum(g, coef1, u1x, iz, ix) / G.dx;
+                    orc_real gz = cdx_sum(g, coef1, u1z, iz, ix) / G.dx;
                     bpx[s][0][o] = b * bpx[s][0][o] + a * gx;
                     bpx[s][1][o] = b * bpx[s][1][o] + a * gz;
                 }
                 if (bpz[s][0] && ddz > 0.0f) {
-                    float b = (float)exp(-(double)ddz * G.dt), a = 1.0f - b;
-                    float gx = cdz_sum(g, coef1, u1x, iz, ix) / G.dz;
-                    float gz = cdz_sum(g, coef1, u1z, iz, ix) / G.dz;
+                    orc_real b = (orc_real)exp(-(double)ddz * G.dt), a = 1.0f - b;
+                    orc_real gx = cdz_sum(g, coef1, u1x, iz, ix) / G.dz;
+                    orc_real gz = cdz_sum(g, coef1, u1z, iz, ix) / G.dz;
                     bpz[s][0][o] = b * bpz[s][0][o] + a * gx;
                     bpz[s][1][o] = b * bpz[s][1][o] + a * gz;
                 }
@@ -849,10 +849,10 @@ void orc_updatedispl(const int *norder_, const int *nx_, const int *nz_, const i
     }
 
     /* full-grid psi views (zero outside the strips) for the correction stencils */
-    float *psx[2], *psz[2];
+    orc_real *psx[2], *psz[2];
     for (int q = 0; q < 2; ++q) {
-        psx[q] = calloc(ncell, sizeof(float));
-        psz[q] = calloc(ncell, sizeof(float));
+        psx[q] = calloc(ncell, sizeof(orc_real));
+        psz[q] = calloc(ncell, sizeof(orc_real));
     }
     for (int s = 0; s < nstrip; ++s) {
         int iz0, iz1, ix0, ix1;
@@ -871,10 +871,10 @@ void orc_updatedispl(const int *norder_, const int *nx_, const int *nz_, const i
     }
 
     /* ---- phase B: operator passes (the Submain operators, reg + pml) ---- */
-    float *t1 = malloc(ncell * sizeof(float)), *t2 = malloc(ncell * sizeof(float));
-    float *P[6];
-    for (int q = 0; q < 6; ++q) P[q] = calloc(ncell, sizeof(float));
-    float dxf = G.dx, dzf = G.dz;
+    orc_real *t1 = malloc(ncell * sizeof(orc_real)), *t2 = malloc(ncell * sizeof(orc_real));
+    orc_real *P[6];
+    for (int q = 0; q < 6; ++q) P[q] = calloc(ncell, sizeof(orc_real));
+    orc_real dxf = G.dx, dzf = G.dz;
 #define ADD2(dst) for (size_t i = 0; i < ncell; ++i) dst[i] = dst[i] + (t1[i] + t2[i])
     orc_diffx2_reg(norder, nx, nz, npml, izt, dxf, coef0, lp2m, u1x, t1);
     orc_diffx2_pml(norder, nx, nz, npml, izt, dxf, coef0, lp2m, u1x, t2); ADD2(P[0]);
@@ -895,14 +895,14 @@ void orc_updatedispl(const int *norder_, const int *nx_, const int *nz_, const i
 #undef ADD2
 
     /* masked background fields for the hybrid correction */
-    const float *bgx[2][3] = {{uxl_reg, uxr_reg, uxb_reg}, {uxl_pml, uxr_pml, uxb_pml}};
-    const float *bgz[2][3] = {{uzl_reg, uzr_reg, uzb_reg}, {uzl_pml, uzr_pml, uzb_pml}};
+    const orc_real *bgx[2][3] = {{uxl_reg, uxr_reg, uxb_reg}, {uxl_pml, uxr_pml, uxb_pml}};
+    const orc_real *bgz[2][3] = {{uzl_reg, uzr_reg, uzb_reg}, {uzl_pml, uzr_pml, uzb_pml}};
     int have_bg = uxl_reg && uzl_reg && uxr_reg && uzr_reg && uxb_reg && uzb_reg && uxl_pml &&
                   uzl_pml && uxr_pml && uzr_pml && uxb_pml && uzb_pml && G.it >= 0 && G.it <= G.nt;
-    float *wTx = NULL, *wTz = NULL, *wSx = NULL, *wSz = NULL;
+    orc_real *wTx = NULL, *wTz = NULL, *wSx = NULL, *wSz = NULL;
     if (have_bg) {
-        wTx = calloc(ncell, sizeof(float)); wTz = calloc(ncell, sizeof(float));
-        wSx = calloc(ncell, sizeof(float)); wSz = calloc(ncell, sizeof(float));
+        wTx = calloc(ncell, sizeof(orc_real)); wTz = calloc(ncell, sizeof(orc_real));
+        wSx = calloc(ncell, sizeof(orc_real)); wSz = calloc(ncell, sizeof(orc_real));
         for (int ix = 1 - nbd; ix <= nx + nbd; ++ix)
             for (int iz = 1; iz <= nz + nbd; ++iz) {
                 int arr, which;
@@ -914,17 +914,17 @@ void orc_updatedispl(const int *norder_, const int *nx_, const int *nz_, const i
             }
     }
 
-    const float dt2 = (float)(G.dt * G.dt);
+    const orc_real dt2 = (orc_real)(G.dt * G.dt);
     const int iz_first = (izt == 1) ? 1 : izt + 1;
-    float *newx = malloc(ncell * sizeof(float)), *newz = malloc(ncell * sizeof(float));
-    memcpy(newx, u1x, ncell * sizeof(float)); /* cells not updated keep their value */
-    memcpy(newz, u1z, ncell * sizeof(float));
+    orc_real *newx = malloc(ncell * sizeof(orc_real)), *newz = malloc(ncell * sizeof(orc_real));
+    memcpy(newx, u1x, ncell * sizeof(orc_real)); /* cells not updated keep their value */
+    memcpy(newz, u1z, ncell * sizeof(orc_real));
 
 #pragma omp parallel for schedule(dynamic)
     for (int ix = 2 - npml; ix <= nx + npml - 1; ++ix) {
         for (int iz = iz_first; iz <= nz + npml - 1; ++iz) {
             size_t gi = g2(g, iz, ix);
-            float p[6];
+            orc_real p[6];
             for (int q = 0; q < 6; ++q) p[q] = P[q][gi];
             if (iz == 1 && izt == 1) eval_parts_g(g, coef0, coef1, lam, mu, u1x, u1z, iz, ix, p);
 
@@ -934,10 +934,10 @@ void orc_updatedispl(const int *norder_, const int *nx_, const int *nz_, const i
                 int surf = (iz == 1 && izt == 1);
                 int nox = imin(imin(ix + npml - 1, nx + npml - ix), G.norder);
                 int noz = imin(imin(iz - izt, nz + npml - iz), G.norder);
-                float sx0 = 0.0f, sx1 = 0.0f, sz0 = 0.0f, sz1 = 0.0f;
+                orc_real sx0 = 0.0f, sx1 = 0.0f, sz0 = 0.0f, sz1 = 0.0f;
                 for (int k = 1; k <= nox; ++k) {
                     size_t a = g2(g, iz, ix + k), b = g2(g, iz, ix - k);
-                    float ma = lp2m[a], mb = lp2m[b];
+                    orc_real ma = lp2m[a], mb = lp2m[b];
                     sx0 = acc(sx0, CO(coef1, k, nox), ma * psx[0][a] - mb * psx[0][b]);
                     sx1 = acc(sx1, CO(coef1, k, nox), mu[a] * psx[1][a] - mu[b] * psx[1][b]);
                 }
@@ -955,7 +955,7 @@ void orc_updatedispl(const int *norder_, const int *nx_, const int *nz_, const i
                 int arr, which;
                 long o = bg_off(g, iz, ix, G.it, &arr, &which);
                 if (o >= 0) {
-                    float q[6];
+                    orc_real q[6];
                     if (which == 0) {
                         eval_parts_g(g, coef0, coef1, lam, mu, wSx, wSz, iz, ix, q);
                         for (int m = 0; m < 6; ++m) p[m] = p[m] + q[m];
@@ -966,26 +966,26 @@ void orc_updatedispl(const int *norder_, const int *nx_, const int *nz_, const i
                 }
             }
 
-            float bf = dt2 / rho[gi];
+            orc_real bf = dt2 / rho[gi];
             if (s == S_NONE) {
-                float rx = (p[0] + p[1]) + p[2], rz = (p[3] + p[4]) + p[5];
+                orc_real rx = (p[0] + p[1]) + p[2], rz = (p[3] + p[4]) + p[5];
                 newx[gi] = (2.0f * u1x[gi] - ux[2 * gi + 1]) + bf * rx;
                 newz[gi] = (2.0f * u1z[gi] - uz[2 * gi + 1]) + bf * rz;
             } else {
                 size_t o = strip_off(g, s, iz, ix);
-                float ax = 0.5f * dpx[ix + npml - 1] * (float)G.dt;
-                float az = 0.5f * dpz[iz - izt] * (float)G.dt;
-                float c3 = 2.0f - 4.0f * ax * az;
+                orc_real ax = 0.5f * dpx[ix + npml - 1] * (orc_real)G.dt;
+                orc_real az = 0.5f * dpz[iz - izt] * (orc_real)G.dt;
+                orc_real c3 = 2.0f - 4.0f * ax * az;
                 for (int comp = 0; comp < 2; ++comp) {
-                    float *b1 = bu[s][comp][0], *b2 = bu[s][comp][1], *b3 = bu[s][comp][2];
-                    float X = p[3 * comp], Z = p[3 * comp + 1], Mm = p[3 * comp + 2];
-                    float n1 = (2.0f * b1[2 * o] - (1.0f - ax) * b1[2 * o + 1] + bf * X) / (1.0f + ax);
-                    float n2 = (2.0f * b2[2 * o] - (1.0f - az) * b2[2 * o + 1] + bf * Z) / (1.0f + az);
-                    float n3 = (c3 * b3[2 * o] - (1.0f - ax - az) * b3[2 * o + 1] + bf * Mm) / (1.0f + ax + az);
+                    orc_real *b1 = bu[s][comp][0], *b2 = bu[s][comp][1], *b3 = bu[s][comp][2];
+                    orc_real X = p[3 * comp], Z = p[3 * comp + 1], Mm = p[3 * comp + 2];
+                    orc_real n1 = (2.0f * b1[2 * o] - (1.0f - ax) * b1[2 * o + 1] + bf * X) / (1.0f + ax);
+                    orc_real n2 = (2.0f * b2[2 * o] - (1.0f - az) * b2[2 * o + 1] + bf * Z) / (1.0f + az);
+                    orc_real n3 = (c3 * b3[2 * o] - (1.0f - ax - az) * b3[2 * o + 1] + bf * Mm) / (1.0f + ax + az);
                     b1[2 * o + 1] = b1[2 * o]; b1[2 * o] = n1;
                     b2[2 * o + 1] = b2[2 * o]; b2[2 * o] = n2;
                     b3[2 * o + 1] = b3[2 * o]; b3[2 * o] = n3;
-                    float tot = (n1 + n2) + n3;
+                    orc_real tot = (n1 + n2) + n3;
                     if (comp == 0) newx[gi] = tot; else newz[gi] = tot;
                 }
             }

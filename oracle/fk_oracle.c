@@ -112,7 +112,7 @@ typedef struct {
     cplx (*Q)[4][4]; /* Q_j = E_j^{-1} E_{j+1} */
 } fkmodel;
 
-static int fk_setup(fkmodel *m, int nlayer, const float *fk, double p)
+static int fk_setup(fkmodel *m, int nlayer, const orc_real *fk, double p)
 {
     m->nl = nlayer;
     m->zt = malloc(nlayer * sizeof(double)); m->vp = malloc(nlayer * sizeof(double));
@@ -272,7 +272,7 @@ static int next_pow2(long n)
     return p;
 }
 
-int orc_fk_traces(int nlayer, const float *fkm, int src_is_p, double p, float f0, float ds,
+int orc_fk_traces(int nlayer, const orc_real *fkm, int src_is_p, double p, orc_real f0, orc_real ds,
                   double dt, int ntq, int nzq, const double *z, double zref, double t0,
                   double *ux, double *uz)
 {
@@ -296,12 +296,12 @@ int orc_fk_traces(int nlayer, const float *fkm, int src_is_p, double p, float f0
 
 void orc_comput_background_wavefield(const int *nx_, const int *nz_, const int *npml_,
     const int *izt_, const int *nbd_, const int *nt_, const int *nlayer_left,
-    const int *nlayer_right, const int *src_type, const float *x0_, const float *z0_,
-    const float *dx_, const float *dz_, const double *dt_, const float *ds_, const float *f0_,
-    const double *rayp_, const double *bazs, const float *fk_left, const float *fk_right,
-    float *uxl_reg, float *uzl_reg, float *uxr_reg, float *uzr_reg, float *uxb_reg,
-    float *uzb_reg, float *uxl_pml, float *uzl_pml, float *uxr_pml, float *uzr_pml,
-    float *uxb_pml, float *uzb_pml)
+    const int *nlayer_right, const int *src_type, const orc_real *x0_, const orc_real *z0_,
+    const orc_real *dx_, const orc_real *dz_, const double *dt_, const orc_real *ds_, const orc_real *f0_,
+    const double *rayp_, const double *bazs, const orc_real *fk_left, const orc_real *fk_right,
+    orc_real *uxl_reg, orc_real *uzl_reg, orc_real *uxr_reg, orc_real *uzr_reg, orc_real *uxb_reg,
+    orc_real *uzb_reg, orc_real *uxl_pml, orc_real *uzl_pml, orc_real *uxr_pml, orc_real *uzr_pml,
+    orc_real *uxb_pml, orc_real *uzb_pml)
 {
     const int nx = *nx_, nz = *nz_, npml = *npml_, nbd = *nbd_, nt = *nt_;
     const double x0 = *x0_, z0 = *z0_, dx = *dx_, dz = *dz_, dt = *dt_;
@@ -368,7 +368,7 @@ void orc_comput_background_wavefield(const int *nx_, const int *nz_, const int *
                     int side = ix <= 0 ? 0 : 1;
                     bx = vx[side]; bz = vz[side];
                 }
-                float fx = (float)(sgn * bx), fz = (float)bz;
+                orc_real fx = (orc_real)(sgn * bx), fz = (orc_real)bz;
                 /* scatter into the twelve arrays (FDFK2D.f90:330-355 shapes) */
                 if (iz >= 1 && iz <= nz - nbd && ix >= 1 && ix <= nbd) {
                     size_t o = (size_t)(iz - 1) + (size_t)(nz - nbd) * ((size_t)(ix - 1) + (size_t)nbd * it);

@@ -11,6 +11,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "build", "liboracle.so")
+LIB64_PATH = os.path.join(HERE, "build", "liboracle64.so")   # every FP32 quantity in FP64
 REF_PATH = os.path.join(HERE, "_ref", "libfdfk_ref.so")
 
 _f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
@@ -50,6 +51,27 @@ def lib():
         _lib.orc_check_stability.restype = C.c_int
         _lib.orc_fk_traces.restype = C.c_int
     return _lib
+
+
+_lib64 = None
+
+
+def lib64():
+    """The FP64 build of the same oracle (-DORC_FP64): arrays and the scalars
+    the FP32 build takes as float are double."""
+    global _lib64
+    if _lib64 is None:
+        if not os.path.exists(LIB64_PATH):
+            raise RuntimeError("FP64 oracle not built: run `make -C oracle`")
+        _lib64 = C.CDLL(LIB64_PATH)
+    return _lib64
+
+
+def _real(fp64):
+    """(library, by-reference scalar, by-value scalar type) of one build."""
+    if fp64:
+        return lib64(), _d, C.c_double
+    return lib(), _f, C.c_float
 
 
 def ref_available():
@@ -108,10 +130,12 @@ def ref_op(name, norder, nx, nz, npml, izt, dx, dz, coef, mat, u):
 
 
 def loadseis(nx, nz, npml, izt, x0, z0, dx, dz, rx, rz, ux, uz):
-    sx, sz = C.c_float(), C.c_float()
-    lib().orc_loadseis(C.c_int(nx), C.c_int(nz), C.c_int(npml), C.c_int(izt), C.c_float(x0),
-                       C.c_float(z0), C.c_float(dx), C.c_float(dz), C.c_float(rx),
-                       C.c_float(rz), _p(ux), _p(uz), C.byref(sx), C.byref(sz))
+    """FP64 fields (float64 arrays) run the FP64 build."""
+    L, _, R = _real(ux.dtype == np.float64)
+    sx, sz = R(), R()
+    L.orc_loadseis(C.c_int(nx), C.c_int(nz), C.c_int(npml), C.c_int(izt), R(x0),
+                   R(z0), R(dx), R(dz), R(rx),
+                   R(rz), _p(ux), _p(uz), C.byref(sx), C.byref(sz))
     return sx.value, sz.value
 
 
@@ -174,31 +198,50 @@ def damping(nx, nz, npml, nbd, pml_top, npower, dx, dz, vpml, R0):
 
 def updatedispl(st, it):
     """One oracle time step on a state dict produced by fdfk2d_amd-style
-    allocation (see tests/_state.py).  Arrays are mutated in place."""
+    allocation (see tests/_state.py).  Arrays are mutated in place; a state
+    of float64 arrays (state64) runs the FP64 build."""
     a = st
+    L, fr, _ = _real(a["ux"].dtype == np.float64)
     bg = a.get("bg")
     bgp = [_p(x) for x in bg] if bg is not None else [None] * 12
-    lib().orc_updatedispl(
+    L.orc_updatedispl(
         _i(a["norder"]), _i(a["nx"]), _i(a["nz"]), _i(a["npml"]), _i(a["izt"]),
         _i(a["pml_top"]), _i(a["npower"]), _i(a["nbd"]), _i(it), _i(a["nt"]),
-        _f(a["dx"]), _f(a["dz"]), _d(a["dt"]), _f(a["vpml"]), _d(a["R0"]),
+        fr(a["dx"]), fr(a["dz"]), _d(a["dt"]), fr(a["vpml"]), _d(a["R0"]),
         *bgp, _p(a["c"]), _p(a["coef0"]), _p(a["coef1"]), _p(a["ux"]), _p(a["uz"]),
         *[_p(x) for x in a["bp"]], *[_p(x) for x in a["bu"]])
 
 
 def background(nx, nz, npml, izt, nbd, nt, src_is_p, x0, z0, dx, dz, dt, ds, f0, rayp, bazs,
-               fk_left, fk_right):
-    """Oracle FK background in the reference's twelve-array layout."""
+               fk_left, fk_right, fp64=False):
+    """Oracle FK background in the reference's twelve-array layout (float64
+    arrays from the FP64 build when fp64)."""
+    L, f, _ = _real(fp64)
+    dt_ = np.float64 if fp64 else np.float32
     shapes = bg_shapes(nx, nz, nbd, nt)
-    arrs = [np.zeros(s, np.float32) for s in shapes]
-    fl = np.ascontiguousarray(fk_left, np.float32)
-    fr = np.ascontiguousarray(fk_right, np.float32)
+    arrs = [np.zeros(s, dt_) for s in shapes]
+    fl = np.ascontiguousarray(fk_left, dt_)
+    fr = np.ascontiguousarray(fk_right, dt_)
     bz = np.ascontiguousarray(bazs, np.float64)
-    lib().orc_comput_background_wavefield(
+    L.orc_comput_background_wavefield(
         _i(nx), _i(nz), _i(npml), _i(izt), _i(nbd), _i(nt), _i(fl.size // 4), _i(fr.size // 4),
-        _i(1 if src_is_p else 0), _f(x0), _f(z0), _f(dx), _f(dz), _d(dt), _f(ds), _f(f0),
+        _i(1 if src_is_p else 0), f(x0), f(z0), f(dx), f(dz), _d(dt), f(ds), f(f0),
         _d(rayp), _p(bz), _p(fl), _p(fr), *[_p(x) for x in arrs])
     return arrs
+
+
+def state64(st):
+    """A copy of an oracle state with every float32 array in float64, for
+    the FP64 build (the inputs are the FP32 values, widened exactly)."""
+    out = {}
+    for k, v in st.items():
+        if isinstance(v, np.ndarray) and v.dtype == np.float32:
+            out[k] = v.astype(np.float64)
+        elif isinstance(v, list) and v and isinstance(v[0], np.ndarray):
+            out[k] = [x.astype(np.float64) if x.dtype == np.float32 else x.copy() for x in v]
+        else:
+            out[k] = v
+    return out
 
 
 def bg_shapes(nx, nz, nbd, nt):

@@ -71,7 +71,8 @@ def rel_l2(a, b):
 
 def oracle_run(case, st, bg, nsteps, it0=0, record=True):
     """Advance the oracle nsteps from iteration it0, recording every iteration
-    before the update (src/FDFK2D.f90:497-522)."""
+    before the update (src/FDFK2D.f90:497-522).  A state of float64 arrays
+    runs the FP64 build and records in float64."""
     from oracle import pyoracle as po
     st["bg"] = bg
     sx, sz = [], []
@@ -83,4 +84,5 @@ def oracle_run(case, st, bg, nsteps, it0=0, record=True):
             sx.append([r[0] for r in row])
             sz.append([r[1] for r in row])
         po.updatedispl(st, it)
-    return np.asarray(sx, np.float32).T, np.asarray(sz, np.float32).T
+    dt = st["ux"].dtype
+    return np.asarray(sx, dt).T, np.asarray(sz, dt).T
