@@ -84,3 +84,21 @@ def test_fp16_materials_against_fp64(fp64_runs):
         s.close()
         print("case %d: FP16 materials rel-L2 %.2e against FP64" % (k, e), flush=True)
         assert e < 1e-3
+
+
+@pytest.mark.parametrize("norder", [1, 4, 8])
+def test_fd_orders_within_1e4_of_fp64(norder):
+    """FD orders 2, 8 and 16 (P incidence, 121 x 60, 301 iterations): the FP32
+    GPU path against the FP64 oracle."""
+    print("case: FD order %d, 121 x 60, P, vs FP64 oracle" % (2 * norder), flush=True)
+    c = Case(norder=norder, nt=300, npml=max(15, 2 * (2 * norder - 1)))
+    bg = po.background(c.nx, c.nz, c.npml, 1, c.nbd, c.nt, c.src_is_p, 0.0, 0.0, c.dx, c.dz, c.dt,
+                       c.ds, c.f0, c.rayp, c.bazs, c.fk, c.fk, fp64=True)
+    ref = oracle_run(c, po.state64(c.state()), bg, c.nt + 1)
+    s = _solver(c)
+    s.run(c.nt + 1)
+    e = _err(s.seismograms(), ref)
+    s.close()
+    assert np.abs(ref[1]).max() > 1e-6
+    print("order %d: FP32 rel-L2 %.2e against FP64" % (2 * norder, e), flush=True)
+    assert e < 1e-4
