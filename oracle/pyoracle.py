@@ -1,4 +1,5 @@
-"""ctypes bindings to the CPU oracle (oracle/build/liboracle.so) and to the
+"""ctypes bindings to the CPU oracle (oracle/build/liboracle.so; its FP64
+build oracle/build/liboracle64.so runs on float64 arrays) and to the
 reference's own source-visible Fortran routines (oracle/_ref/libfdfk_ref.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
