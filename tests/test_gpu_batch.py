@@ -114,3 +114,40 @@ def test_server_failed_request_raises_and_cleans_up():
     lat, total, res = srv.run(reqs, keep_results=True)
     srv.close()
     assert sorted(lat) == [r["id"] for r in reqs] and all(np.isfinite(res[r["id"]][1]).all() for r in reqs)
+
+
+def test_batch_equals_alone_at_the_batched_workload_shapes():
+    """Three jobs drawn as in the batched workload (seed 3: nx uniform in
+    2000..8000, nz in 800..2000, dx = dz = 50 m, 40-cell PML, P and SV
+    alternating, ray parameter uniform in 0.04..0.08 s/km, layered model
+    with its random perturbation), 1200 iterations each (3.6 s: the incident
+    pulse, delayed 3 s, is in the grid): batched launches over
+    large uneven grids (many interior tiles per member, one tile width for
+    the batch) give each job's seismograms and final wavefield of the job
+    alone, bit for bit."""
+    rng = np.random.default_rng(3)
+    reqs = [dict(id=k, arrival=0.0, nx=int(rng.integers(2000, 8001)), nz=int(rng.integers(800, 2001)), dx=50.0,
+                 nsteps=1200, src_is_p=(k % 2 == 0), p_skm=float(rng.uniform(0.04, 0.08)), seed=30 + k, npml=40)
+            for k in range(3)]
+    print("case: batched workload shapes %s, 1200 iterations" % [(r["nx"], r["nz"]) for r in reqs], flush=True)
+    alone = []
+    for r in reqs:
+        s = _solver(r)
+        s.run(s.nt + 1)
+        alone.append(s.seismograms() + s.get_state())
+        s.close()
+    b = Batch(3)
+    sol = [_solver(r) for r in reqs]
+    for s in sol:
+        b.add(s)
+    while b.active:
+        b.run(200)
+    b.sync()
+    for s, ref in zip(sol, alone):
+        got = s.seismograms() + s.get_state()
+        print("max |uz| %.3g" % np.abs(ref[3]).max(), flush=True)
+        assert np.abs(ref[3]).max() > 1e-6
+        for a, e in zip(got, ref):
+            assert np.array_equal(a, e)
+        s.close()
+    b.close()
