@@ -359,8 +359,17 @@ def main():
     ms = r["wall"] / a.steps * 1e3
     st = r["stats"]
     k_ms = st["interior_ms"] / max(st["interior_n"], 1)
-    achieved = r["alg_bytes"] / (k_ms * 1e-3) / 1e9
     cfg = r["cfg"]
+    # FDFK_FUSED=0 times the interior launch alone (k_interior2): its own
+    # algorithmic bytes, the interior tiles' total-field cells at 36 B
+    # (32 B with FP16 materials); the measured traffic is the fused launch's
+    fused = os.environ.get("FDFK_FUSED", "1") != "0"
+    if fused:
+        k_name, k_bytes, k_cells = "k_step", r["alg_bytes"], r["alg_cells"]
+    else:
+        k_cells = cfg["nx"] * (cfg["nz"] - 2 * cfg["norder"] + 1)
+        k_name, k_bytes = "k_interior2", (32 if a.precision == "fp16" else 36) * k_cells
+    achieved = k_bytes / (k_ms * 1e-3) / 1e9
     full_steps = 20000
     out = {
         "metric": "Gcell-updates/s",
@@ -395,15 +404,15 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": (None if a.workload != "c2" else
+            "traffic": (None if a.workload != "c2" or not fused else
                         MEASURED_TRAFFIC_BYTES_FP16 if a.precision == "fp16" else MEASURED_TRAFFIC_BYTES),
             "traffic_source": "rocprofv3 FETCH_SIZE+WRITE_SIZE per launch, copy-calibrated "
                               + ("(profiles/k_step_fp16_8001x2000_hbm_traffic.md)" if a.precision == "fp16"
                                  else "(profiles/k_step_8001x2000_hbm_traffic.md)"),
-            "kernel": "k_step",
+            "kernel": k_name,
             "kernel_avg_ms": round(k_ms, 5),
-            "alg_bytes_per_launch": r["alg_bytes"],
-            "cells_per_launch": r["alg_cells"],
+            "alg_bytes_per_launch": k_bytes,
+            "cells_per_launch": k_cells,
         },
     }
     if not a.no_cpu:
