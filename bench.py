@@ -154,14 +154,23 @@ def hbm_probe():
         raise SystemExit("hbm_probe failed with status %d" % r.returncode)
 
 
-def serve_bench(n, seed=5, rate=200.0):
+def serve_bench(n, seed=5, rate=200.0, full=False):
     """Scaled continuous-batching serving run (the request mix of the serving
     workload with smaller grids and step counts so that both the batched and
     the one-at-a-time plain path finish in about a minute): p50/p99 latency
-    and solves per hour for both, plus the equality of their seismograms."""
+    and solves per hour for both, plus the equality of their seismograms.
+    With full, the first n requests of the serving workload itself (4
+    arrivals/s, grids 1000..8000 x 400..2000 log-uniform, 3000..20000 steps,
+    dx 50/100/200 m)."""
     from fdfk2d_amd import serve
-    reqs = serve.request_mix(n, seed=seed, rate=rate, nx_range=(400, 2000), nz_range=(160, 500),
-                             steps_range=(600, 2500), dx_choices=(100.0, 200.0))
+    if full:
+        rate = 4.0
+        reqs = serve.request_mix(n, seed=seed, rate=rate)
+        shapes = "1000..8000 x 400..2000 log-uniform, 3000..20000 steps, dx 50/100/200 m"
+    else:
+        reqs = serve.request_mix(n, seed=seed, rate=rate, nx_range=(400, 2000), nz_range=(160, 500),
+                                 steps_range=(600, 2500), dx_choices=(100.0, 200.0))
+        shapes = "400..2000 x 160..500 log-uniform, 600..2500 steps, dx 100/200 m"
     # untimed pass: rocFFT compiles its kernels for each new FFT length at run
     # time; both timed paths then start with the same warm caches
     serve.serve_plain(reqs)
@@ -180,8 +189,7 @@ def serve_bench(n, seed=5, rate=200.0):
     return {"metric": "solves_per_hour", "value": b["solves_per_hour"], "unit": "solves/h",
             "higher_is_better": True, "data": "synthetic",
             "config": {"workload": f"serving: {n} Poisson arrivals at {rate}/s (seed {seed}), grids "
-                                   "400..2000 x 160..500 log-uniform, 600..2500 steps, dx 100/200 m, "
-                                   "HBM budget 200 GB", "cell_updates": cells},
+                                   + shapes + ", HBM budget 200 GB", "cell_updates": cells},
             "batched": dict(b, prep_workers=srv.workers,
                             prep_s={k: round(v, 3) for k, v in srv.prep_parts.items()}),
             "plain": dict(p, phase_s={k: round(v, 3) for k, v in plain_s.items()}),
@@ -324,6 +332,8 @@ def main():
                     help="only run the copy kernel used to calibrate HBM byte counters")
     ap.add_argument("--serve", type=int, default=0, metavar="N",
                     help="serving measurement with N requests instead of the default line")
+    ap.add_argument("--serve-full", action="store_true",
+                    help="with --serve N: the first N requests of the serving workload at its own sizes")
     ap.add_argument("--full-solve", action="store_true",
                     help="time the whole 20001-iteration single large solve end to end")
     ap.add_argument("--compat", type=int, default=0, metavar="N",
@@ -338,7 +348,7 @@ def main():
         copy_calibration()
         return
     if a.serve:
-        print(json.dumps(serve_bench(a.serve)), flush=True)
+        print(json.dumps(serve_bench(a.serve, full=a.serve_full)), flush=True)
         return
     if a.batch:
         print(json.dumps(batch_bench(a.batch)), flush=True)

@@ -113,9 +113,15 @@ def rel_l2(a, b):
 # serving
 # ---------------------------------------------------------------------------
 class Server:
-    def __init__(self, hbm_budget_gb=200.0, chunk=100, log=None, workers=None):
+    def __init__(self, hbm_budget_gb=200.0, chunk=100, log=None, workers=None, solo_cells=4_000_000):
         self.budget = hbm_budget_gb * 1e9
         self.chunk = chunk
+        # grids of at least solo_cells padded cells fill the GPU alone: they
+        # step on their own streams (the single-job kernels, whose frame
+        # tiles overlap their interior's tail) beside the batch, which takes
+        # the grids too small to fill the GPU
+        self.solo_cells = solo_cells
+        self.solo = {}
         self.workers = workers or max(1, min(8, os.cpu_count() or 1))
         self.batch = Batch(3)
         self.used = 0.0
@@ -169,6 +175,9 @@ class Server:
                 self.batch.remove(s)
                 s.close()
             self.active.clear()
+            for rid, (req, s) in list(self.solo.items()):
+                s.close()
+            self.solo.clear()
             for f in preparing:
                 if not f.cancelled() and f.exception() is None:
                     f.result()[0].close()
@@ -180,7 +189,7 @@ class Server:
 
     def _loop(self, pending, waiting, preparing, finishing, lat, results, keep_results, t0):
         with cf.ThreadPoolExecutor(self.workers) as pool:
-            while pending or waiting or preparing or self.active or finishing:
+            while pending or waiting or preparing or self.active or self.solo or finishing:
                 now = time.perf_counter() - t0
                 while pending and pending[0]["arrival"] <= now:
                     waiting.append(pending.popleft())
@@ -198,8 +207,11 @@ class Server:
                     for k, v in zip(("model", "create", "background"), secs):
                         self.prep_parts[k] += v
                     self.used -= work                      # FK work arrays are gone
-                    self.batch.add(s)
-                    self.active[req["id"]] = (req, s)
+                    if (req["nx"] + 2 * req["npml"]) * (req["nz"] + req["npml"]) >= self.solo_cells:
+                        self.solo[req["id"]] = (req, s)
+                    else:
+                        self.batch.add(s)
+                        self.active[req["id"]] = (req, s)
                 for f in [f for f in finishing if f.done()]:
                     req = finishing.pop(f)
                     (sx, sz), t_done = f.result()
@@ -207,8 +219,17 @@ class Server:
                     lat[req["id"]] = t_done - t0 - req["arrival"]
                     if keep_results:
                         results[req["id"]] = (sx, sz)
+                for rid, (req, s) in list(self.solo.items()):
+                    left = s.nt + 1 - s.iteration
+                    if left > 0:
+                        s.run(min(self.chunk, left))       # queued on the solver's stream
+                    if s.iteration > s.nt:
+                        del self.solo[rid]
+                        finishing[pool.submit(self._finish, s)] = req
                 if not self.active:
                     wait = 0.05 if not pending else max(0.0, min(0.05, pending[0]["arrival"] - now))
+                    if self.solo:
+                        continue
                     if preparing or finishing:
                         cf.wait(list(preparing) + list(finishing), timeout=wait,
                                 return_when=cf.FIRST_COMPLETED)
