@@ -151,3 +151,23 @@ def test_batch_equals_alone_at_the_batched_workload_shapes():
             assert np.array_equal(a, e)
         s.close()
     b.close()
+
+
+def test_server_with_solo_requests_equals_plain_path():
+    """Requests of at least solo_cells padded cells step on their own
+    streams beside the batch (here the threshold is lowered so that the mix
+    splits between the two): every result equals the plain path's, bit for
+    bit, and both kinds finish."""
+    reqs = serve.request_mix(6, seed=7, rate=20.0, nx_range=(120, 400), nz_range=(50, 120),
+                             steps_range=(150, 400), dx_choices=(100.0, 200.0))
+    cells = sorted((r["nx"] + 2 * r["npml"]) * (r["nz"] + r["npml"]) for r in reqs)
+    thr = cells[3]          # the three largest step alone
+    print("case: serving 6 requests, solo threshold %d cells, vs plain serial path" % thr, flush=True)
+    srv = serve.Server(hbm_budget_gb=50.0, chunk=40, solo_cells=thr)
+    lat, total, res = srv.run(reqs, keep_results=True)
+    srv.close()
+    lat_p, total_p, res_p = serve.serve_plain(reqs, keep_results=True)
+    assert sorted(res) == sorted(res_p) == list(range(6))
+    for i in res:
+        assert np.abs(res_p[i][1]).max() > 0
+        assert np.array_equal(res[i][0], res_p[i][0]) and np.array_equal(res[i][1], res_p[i][1])
