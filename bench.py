@@ -141,19 +141,6 @@ def copy_calibration(mib=512, reps=5):
     print(json.dumps({"copy_calibration_bytes": n * 4, "reps": reps}), flush=True)
 
 
-def hbm_probe():
-    """Plain streams at known sizes (fdfk2d_amd/hbm_probe, built from
-    csrc/hbm_probe.hip): copy, four reads with one write (the read/write mix
-    of a time step) and reads only.  The practical HBM ceiling the step
-    kernels' measured traffic is read against."""
-    from fdfk2d_amd import build as b
-    if not os.path.exists(b.PROBE):
-        raise SystemExit("hbm_probe is not built: python -m fdfk2d_amd.build")
-    r = subprocess.run([b.PROBE], timeout=300)
-    if r.returncode != 0:
-        raise SystemExit("hbm_probe failed with status %d" % r.returncode)
-
-
 def serve_bench(n, seed=5, rate=200.0, full=False):
     """Scaled continuous-batching serving run (the request mix of the serving
     workload with smaller grids and step counts so that both the batched and
@@ -326,8 +313,6 @@ def main():
     ap.add_argument("--nt", type=int, default=0,
                     help="iterations the solve is set up for (default: just the ones run)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--hbm-probe", action="store_true",
-                    help="run the streaming-bandwidth probe (copy, 4-read/1-write, read-only) and exit")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run the copy kernel used to calibrate HBM byte counters")
     ap.add_argument("--serve", type=int, default=0, metavar="N",
@@ -341,9 +326,6 @@ def main():
     ap.add_argument("--batch", type=int, default=0, metavar="N",
                     help="batched-execution measurement with N jobs instead of the default line")
     a = ap.parse_args()
-    if a.hbm_probe:
-        hbm_probe()
-        return
     if a.copy_calib:
         copy_calibration()
         return

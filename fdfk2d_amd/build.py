@@ -34,27 +34,7 @@ def build(verbose=False, extra=()):
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("hipcc failed building libfdfk.so")
     os.replace(tmp, LIB)
-    build_probe(verbose)
     return LIB
-
-
-PROBE = os.path.join(HERE, "hbm_probe")
-
-
-def build_probe(verbose=False):
-    """The streaming-bandwidth probe (csrc/hbm_probe.hip), a standalone
-    program run by bench.py --hbm-probe."""
-    cmd = [HIPCC, "--offload-arch=gfx950", "-O3", "-std=c++17",
-           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-           os.path.join(HERE, "csrc", "hbm_probe.hip"), "-o", PROBE + ".tmp"]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed building hbm_probe")
-    os.replace(PROBE + ".tmp", PROBE)
-    return PROBE
 
 
 if __name__ == "__main__":

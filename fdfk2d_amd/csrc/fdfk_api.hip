@@ -142,6 +142,7 @@ struct fdfk_solver {
     std::vector<std::pair<void *, size_t>> allocs;   // device blocks and their sizes
     int device = 0;
     int64_t bytes = 0;
+    bool bg_only = false;   // geometry and background only (comput_background_wavefield_)
     // fields
     float *ux[2] = {nullptr, nullptr}, *uz[2] = {nullptr, nullptr};
     int cur = 0;
@@ -151,8 +152,10 @@ struct fdfk_solver {
     // strips
     float *strip_fields[3][2][3][2] = {};  // [strip][comp][part][buffer]
     float *psx[3][2] = {}, *psz[3][2] = {};
-    float *ax = nullptr, *bx = nullptr, *az = nullptr, *bz = nullptr;
-    std::vector<float> h_ax, h_bx, h_az, h_bz, h_dx, h_dz;
+    float *dpx = nullptr, *dpz = nullptr;   // classical damping profiles per column / row
+    float *bx = nullptr, *bz = nullptr;     // exp(-d dt) of those profiles (memory decay)
+    float pml_ratio = kPmlRatio;            // multiaxial ratio
+    std::vector<float> h_dx, h_dz, h_bx, h_bz;
     long strip_cells[3] = {0, 0, 0};
     Strips *d_strips[2] = {nullptr, nullptr};  // device copies for the two buffer parities
     // background
@@ -218,37 +221,32 @@ int build_damping(fdfk_solver *s)
     const double lr = std::log(s->R0);
     const double d0x = leff > 0 ? -(double)(s->npower + 1) * (double)s->vpml * lr / (2.0 * leff * (double)g.dx) : 0.0;
     const double d0z = leff > 0 ? -(double)(s->npower + 1) * (double)s->vpml * lr / (2.0 * leff * (double)g.dz) : 0.0;
-    s->h_dx.assign(g.NX, 0.f); s->h_ax.assign(g.NX, 0.f); s->h_bx.assign(g.NX, 1.f);
-    s->h_dz.assign(g.NZ, 0.f); s->h_az.assign(g.NZ, 0.f); s->h_bz.assign(g.NZ, 1.f);
+    s->h_dx.assign(g.NX, 0.f); s->h_bx.assign(g.NX, 1.f);
+    s->h_dz.assign(g.NZ, 0.f); s->h_bz.assign(g.NZ, 1.f);
     auto prof = [&](int q, double d0) {
         if (q <= nbd || leff <= 0) return 0.0;
         double r = (double)(q - nbd) / (double)leff, p = 1.0;
         for (int e = 0; e < s->npower; ++e) p *= r;
         return d0 * p;
     };
-    const float fdt = (float)g.dt;
     for (int j = 0; j < g.NX; ++j) {
         int ix = j + 1 - npml;
         int q = ix <= 0 ? 1 - ix : (ix > nx ? ix - nx : 0);
-        float d = (float)prof(q, d0x);
-        s->h_dx[j] = d;
-        s->h_ax[j] = 0.5f * d * fdt;
-        s->h_bx[j] = (float)std::exp(-(double)d * g.dt);
+        s->h_dx[j] = (float)prof(q, d0x);
+        s->h_bx[j] = (float)std::exp(-(double)s->h_dx[j] * g.dt);
     }
     for (int k = 0; k < g.NZ; ++k) {
         int iz = k + 1;
-        float d = iz > nz ? (float)prof(iz - nz, d0z) : 0.0f;
-        s->h_dz[k] = d;
-        s->h_az[k] = 0.5f * d * fdt;
-        s->h_bz[k] = (float)std::exp(-(double)d * g.dt);
+        s->h_dz[k] = iz > nz ? (float)prof(iz - nz, d0z) : 0.0f;
+        s->h_bz[k] = (float)std::exp(-(double)s->h_dz[k] * g.dt);
     }
     int rc;
-    if ((rc = s->dalloc(&s->ax, g.NX)) || (rc = s->dalloc(&s->bx, g.NX)) ||
-        (rc = s->dalloc(&s->az, g.NZ)) || (rc = s->dalloc(&s->bz, g.NZ)))
+    if ((rc = s->dalloc(&s->dpx, g.NX)) || (rc = s->dalloc(&s->dpz, g.NZ)) || (rc = s->dalloc(&s->bx, g.NX)) ||
+        (rc = s->dalloc(&s->bz, g.NZ)))
         return rc;
-    HIPCHK(h2d(s->stream, s->ax, s->h_ax.data(), g.NX * 4));
+    HIPCHK(h2d(s->stream, s->dpx, s->h_dx.data(), g.NX * 4));
+    HIPCHK(h2d(s->stream, s->dpz, s->h_dz.data(), g.NZ * 4));
     HIPCHK(h2d(s->stream, s->bx, s->h_bx.data(), g.NX * 4));
-    HIPCHK(h2d(s->stream, s->az, s->h_az.data(), g.NZ * 4));
     HIPCHK(h2d(s->stream, s->bz, s->h_bz.data(), g.NZ * 4));
     return 0;
 }
@@ -266,7 +264,7 @@ int upload_strips(fdfk_solver *s)
                 h.psx[st][c] = s->psx[st][c];
                 h.psz[st][c] = s->psz[st][c];
             }
-        h.ax = s->ax; h.bx = s->bx; h.az = s->az; h.bz = s->bz;
+        h.dpx = s->dpx; h.bx = s->bx; h.dpz = s->dpz; h.bz = s->bz; h.ratio = s->pml_ratio;
         if (!s->d_strips[par]) {
             int rc = s->dalloc(&s->d_strips[par], 1);
             if (rc) return rc;
@@ -624,19 +622,30 @@ int one_step(fdfk_solver *s, bool record)
     return 0;
 }
 
+// geometry of a job: the checks the background alone needs
+int validate_geometry(int nx, int nz, int npml, int norder, float dx, float dz, double dt, int nt)
+{
+    if (norder < 1 || norder > kMaxOrder) return fail(FDFK_EINVAL, "norder must be 1..8 (FD order 2..16)");
+    const int nbd = 2 * norder - 1;
+    if (npml < nbd) return fail(FDFK_EINVAL, "npml must be >= 2*norder-1");
+    if (nx < 2 * nbd + 2) return fail(FDFK_EINVAL, "nx too small for the hybrid ring");
+    if (nz < nbd + 2 * norder + 1) return fail(FDFK_EINVAL, "nz too small");
+    if (!(dx > 0) || !(dz > 0) || !(dt > 0)) return fail(FDFK_EINVAL, "dx, dz, dt must be > 0");
+    if (nt < 0) return fail(FDFK_EINVAL, "nt >= 0 required");
+    return 0;
+}
+
 int validate(const fdfk_job *j)
 {
     if (!j) return fail(FDFK_EINVAL, "null job");
     if (j->is_pml_top) return fail(FDFK_ENOSYS, "is_PML_top = .true. is not supported (free surface only)");
-    if (j->norder < 1 || j->norder > kMaxOrder) return fail(FDFK_EINVAL, "norder must be 1..8 (FD order 2..16)");
-    int nbd = 2 * j->norder - 1;
-    if (j->npml < nbd) return fail(FDFK_EINVAL, "npml must be >= 2*norder-1");
-    if (j->nx < 2 * nbd + 2) return fail(FDFK_EINVAL, "nx too small for the hybrid ring");
-    if (j->nz < nbd + 2 * j->norder + 1) return fail(FDFK_EINVAL, "nz too small");
-    if (!(j->dx > 0) || !(j->dz > 0) || !(j->dt > 0)) return fail(FDFK_EINVAL, "dx, dz, dt must be > 0");
-    if (j->nt < 0 || j->nstp < 1) return fail(FDFK_EINVAL, "nt >= 0 and nstp >= 1 required");
-    if (!j->c) return fail(FDFK_EINVAL, "material array c is required");
+    if (int rc = validate_geometry(j->nx, j->nz, j->npml, j->norder, j->dx, j->dz, j->dt, j->nt)) return rc;
+    if (j->nstp < 1) return fail(FDFK_EINVAL, "nstp >= 1 required");
     if (j->nrcv < 0 || (j->nrcv > 0 && (!j->rx || !j->rz))) return fail(FDFK_EINVAL, "receiver arrays");
+    for (int i = 0; i < j->nrcv; ++i)
+        if (!std::isfinite(j->rx[i]) || !std::isfinite(j->rz[i]))
+            return fail(FDFK_EINVAL, "receiver coordinates must be finite");
+    if (!j->c) return fail(FDFK_EINVAL, "material array c is required");
     if (j->precision != FDFK_FP32 && j->precision != FDFK_FP16)
         return fail(FDFK_ENOSYS, "storage precision: FP32, or FP16 materials (BF16 is not built)");
     return 0;
@@ -756,6 +765,7 @@ int fdfk_create(const fdfk_job *job, fdfk_solver **out)
     s->x0 = job->x0; s->z0 = job->z0;
     s->R0 = job->R0; s->npower = job->npower;
     s->mat16 = job->precision == FDFK_FP16;
+    if (const char *e = std::getenv("FDFK_PML_RATIO")) s->pml_ratio = (float)std::atof(e);
     plan_interior(s);
     if (s->mat16 && !s->fused) {
         delete s;
@@ -897,11 +907,13 @@ static int alloc_background(fdfk_solver *s, int period)
     s->bg_period = period;
     const size_t nf = (size_t)s->g.NX * s->g.ld;
     int rc;
-    if ((rc = s->dalloc(&s->d_bg, 1)) || (rc = s->dalloc(&s->ringx, nf)) || (rc = s->dalloc(&s->ringz, nf)))
-        return rc;
+    if ((rc = s->dalloc(&s->d_bg, 1))) return rc;
     HIPCHK(h2d(s->stream, s->d_bg, &h, sizeof(Background)));
-    HIPCHK(hipMemsetAsync(s->ringx, 0, nf * 4, s->stream));
-    HIPCHK(hipMemsetAsync(s->ringz, 0, nf * 4, s->stream));
+    if (!s->bg_only) {
+        if ((rc = s->dalloc(&s->ringx, nf)) || (rc = s->dalloc(&s->ringz, nf))) return rc;
+        HIPCHK(hipMemsetAsync(s->ringx, 0, nf * 4, s->stream));
+        HIPCHK(hipMemsetAsync(s->ringz, 0, nf * 4, s->stream));
+    }
     HIPCHK(hipStreamSynchronize(s->stream));
     return 0;
 }
@@ -1101,13 +1113,32 @@ void updatedispl_(const int32_t *norder, const int32_t *nx, const int32_t *nz, c
     (void)coef0; (void)coef1; (void)bpxtx; (void)bpxtz; (void)bpztx; (void)bpztz;
     (void)bux1t; (void)bux2t; (void)bux3t; (void)buz1t; (void)buz2t; (void)buz3t;
     std::lock_guard<std::mutex> lk(g_compat_mu);
-    auto die = [](const std::string &m) { std::fprintf(stderr, "updatedispl (fdfk2d_amd): %s\n", m.c_str()); };
-    if (*is_pml_top || *izt != 1) { die("only is_PML_top = .false. is supported"); return; }
-    if (*nbd != 2 * *norder - 1) { die("nbd must equal 2*norder-1"); return; }
+    // the reference's error contract is print and stop: a failed step must
+    // not return to a driver that would go on writing stale seismograms
+    auto die = [](const std::string &m) {
+        std::fprintf(stderr, "updatedispl (fdfk2d_amd): %s\n", m.c_str());
+        std::fflush(stderr);
+        std::exit(1);
+    };
+    // host checks first: a bad configuration stops before any GPU work
+    if (*is_pml_top || *izt != 1) die("only is_PML_top = .false. is supported");
+    if (*nbd != 2 * *norder - 1) die("nbd must equal 2*norder-1");
+    if (validate_geometry(*nx, *nz, *npml, *norder, *dx, *dz, *dt, *nt)) die(fdfk_last_error());
+    if (*it < 0 || *it > *nt) die("it outside 0..nt");
+    if (!c || !ux || !uz) die("null array");
     auto itr = g_compat.find(ux);
-    bool fresh = itr == g_compat.end() || itr->second.nx != *nx || itr->second.nz != *nz ||
-                 itr->second.npml != *npml || itr->second.norder != *norder || itr->second.nt != *nt ||
-                 itr->second.c != c || itr->second.bg0 != uxl_reg;
+    // a new run (it = 0), a restart (it does not follow the last call) or
+    // other arrays: the host arrays are authoritative, so the materials, the
+    // background and the state are uploaded again into a new mirror
+    const bool fresh = itr == g_compat.end() || itr->second.nx != *nx || itr->second.nz != *nz ||
+                       itr->second.npml != *npml || itr->second.norder != *norder || itr->second.nt != *nt ||
+                       itr->second.c != c || itr->second.bg0 != uxl_reg || *it == 0 ||
+                       itr->second.last_it != *it - 1;
+    float *const pl[2][3] = {{bux1l, bux2l, bux3l}, {buz1l, buz2l, buz3l}};
+    float *const pr[2][3] = {{bux1r, bux2r, bux3r}, {buz1r, buz2r, buz3r}};
+    float *const pb[2][3] = {{bux1b, bux2b, bux3b}, {buz1b, buz2b, buz3b}};
+    float *const xl[2] = {bpxlx, bpzlx}, *const xr[2] = {bpxrx, bpzrx}, *const xb[2] = {bpxbx, bpzbx};
+    float *const zb[2] = {bpxbz, bpzbz}, *const none[2] = {nullptr, nullptr};
     if (fresh) {
         if (itr != g_compat.end()) { fdfk_destroy(itr->second.s); g_compat.erase(itr); }
         fdfk_job j{};
@@ -1115,36 +1146,23 @@ void updatedispl_(const int32_t *norder, const int32_t *nx, const int32_t *nz, c
         j.dx = *dx; j.dz = *dz; j.dt = *dt; j.R0 = *R0; j.vpml = *vpml; j.nt = *nt; j.nstp = 1;
         j.c = c; j.precision = FDFK_FP32;
         fdfk_solver *s = nullptr;
-        if (fdfk_create(&j, &s)) { die(fdfk_last_error()); return; }
+        if (fdfk_create(&j, &s)) die(fdfk_last_error());
         const float *bg[12] = {uxl_reg, uzl_reg, uxr_reg, uzr_reg, uxb_reg, uzb_reg,
                                uxl_pml, uzl_pml, uxr_pml, uzr_pml, uxb_pml, uzb_pml};
-        if (fdfk_set_background(s, bg)) { die(fdfk_last_error()); fdfk_destroy(s); return; }
+        if (fdfk_set_background(s, bg) || fdfk_set_state(s, ux, uz) || strip_import(s, S_L, pl, xl, none) ||
+            strip_import(s, S_R, pr, xr, none) || strip_import(s, S_B, pb, xb, zb))
+            die(fdfk_last_error());
         CompatEntry e;
         e.s = s; e.nx = *nx; e.nz = *nz; e.npml = *npml; e.norder = *norder; e.nt = *nt; e.c = c; e.bg0 = uxl_reg;
         itr = g_compat.emplace(ux, e).first;
     }
     CompatEntry &e = itr->second;
     fdfk_solver *s = e.s;
-    float *const pl[2][3] = {{bux1l, bux2l, bux3l}, {buz1l, buz2l, buz3l}};
-    float *const pr[2][3] = {{bux1r, bux2r, bux3r}, {buz1r, buz2r, buz3r}};
-    float *const pb[2][3] = {{bux1b, bux2b, bux3b}, {buz1b, buz2b, buz3b}};
-    float *const xl[2] = {bpxlx, bpzlx}, *const xr[2] = {bpxrx, bpzrx}, *const xb[2] = {bpxbx, bpzbx};
-    float *const zb[2] = {bpxbz, bpzbz}, *const none[2] = {nullptr, nullptr};
-    if (e.last_it != *it - 1) {
-        // host state is authoritative: import fields, split fields and memories
-        if (fdfk_set_state(s, ux, uz) || strip_import(s, S_L, pl, xl, none) ||
-            strip_import(s, S_R, pr, xr, none) || strip_import(s, S_B, pb, xb, zb)) {
-            die(fdfk_last_error());
-            return;
-        }
-    }
     s->it = *it;
-    if (one_step(s, false)) { die(fdfk_last_error()); return; }
+    if (one_step(s, false)) die(fdfk_last_error());
     if (fdfk_get_state(s, ux, uz) || strip_export(s, S_L, pl, xl, none) ||
-        strip_export(s, S_R, pr, xr, none) || strip_export(s, S_B, pb, xb, zb)) {
+        strip_export(s, S_R, pr, xr, none) || strip_export(s, S_B, pb, xb, zb))
         die(fdfk_last_error());
-        return;
-    }
     e.last_it = *it;
 }
 

@@ -483,25 +483,38 @@ extern "C" void comput_background_wavefield_(const int32_t *nx, const int32_t *n
     float *uzb_reg, float *uxl_pml, float *uzl_pml, float *uxr_pml, float *uzr_pml,
     float *uxb_pml, float *uzb_pml)
 {
-    auto die = [](const char *m) { std::fprintf(stderr, "comput_background_wavefield (fdfk2d_amd): %s\n", m); };
-    if (*izt != 1) { die("only is_PML_top = .false. is supported"); return; }
-    int norder = (*nbd + 1) / 2;
-    // a homogeneous dummy material: only geometry and time axis matter here
-    const int NX = *nx + 2 * *npml, NZ = *nz + *npml;
-    std::vector<float> c((size_t)3 * NX * NZ);
-    for (size_t i = 0; i < c.size(); i += 3) { c[i] = 2000.f; c[i + 1] = 2e10f; c[i + 2] = 1e10f; }
-    fdfk_job j{};
-    j.nx = *nx; j.nz = *nz; j.npml = *npml; j.norder = norder; j.npower = 2;
-    j.x0 = *x0; j.z0 = *z0; j.dx = *dx; j.dz = *dz; j.dt = *dt; j.R0 = 1e-3; j.vpml = 1.f;
-    j.nt = *nt; j.nstp = 1; j.c = c.data(); j.precision = FDFK_FP32;
-    fdfk_solver *s = nullptr;
-    if (fdfk_create(&j, &s)) { die(fdfk_last_error()); return; }
+    // the reference's error contract is print and stop
+    auto die = [](const char *m) {
+        std::fprintf(stderr, "comput_background_wavefield (fdfk2d_amd): %s\n", m);
+        std::fflush(stderr);
+        std::exit(1);
+    };
+    // host checks first: a bad configuration stops before any GPU work
+    if (*izt != 1) die("only is_PML_top = .false. is supported");
+    if (*nbd < 1 || *nbd % 2 == 0) die("nbd must be 2*norder-1");
+    const int norder = (*nbd + 1) / 2;
+    if (validate_geometry(*nx, *nz, *npml, norder, *dx, *dz, *dt, *nt)) die(fdfk_last_error());
+    if (!bazs || !fk_left || !fk_right) die("null input array");
+    float *const out[12] = {uxl_reg, uzl_reg, uxr_reg, uzr_reg, uxb_reg, uzb_reg,
+                            uxl_pml, uzl_pml, uxr_pml, uzr_pml, uxb_pml, uzb_pml};
+    for (float *o : out)
+        if (!o) die("null output array");
+    // a handle with the geometry and the background storage only: no
+    // fields, materials or strips are needed to synthesise the background
+    auto *s = new fdfk_solver();
+    std::unique_ptr<fdfk_solver> own(s);
+    s->bg_only = true;
+    Geo &g = s->g;
+    g.nx = *nx; g.nz = *nz; g.npml = *npml; g.norder = norder; g.nbd = *nbd;
+    g.NX = g.nx + 2 * g.npml; g.NZ = g.nz + g.npml;
+    g.ld = (g.NZ + 31) / 32 * 32;
+    g.dx = *dx; g.dz = *dz; g.dt = *dt;
+    s->nt = *nt; s->x0 = *x0; s->z0 = *z0;
+    if (hipGetDevice(&s->device) != hipSuccess || hipStreamCreate(&s->stream) != hipSuccess)
+        die("no HIP device / stream");
     fdfk_fk fk{};
     fk.nlayer_left = *nlayer_left; fk.nlayer_right = *nlayer_right;
     fk.fk_left = fk_left; fk.fk_right = fk_right; fk.src_is_p = *src_type != 0;
     fk.ds = *ds; fk.f0 = *f0; fk.rayp = *rayp; fk.bazs = bazs;
-    float *const out[12] = {uxl_reg, uzl_reg, uxr_reg, uzr_reg, uxb_reg, uzb_reg,
-                            uxl_pml, uzl_pml, uxr_pml, uzr_pml, uxb_pml, uzb_pml};
     if (fdfk_compute_background(s, &fk) || fdfk_get_background(s, out)) die(fdfk_last_error());
-    fdfk_destroy(s);
 }

@@ -136,8 +136,9 @@ __device__ __forceinline__ void frame_body(const StepArgs &a, const ETiles &et, 
             uprv[0][0] = a.uxp[uc]; uprv[1][0] = a.uzp[uc];
         } else {
             uo = strip_off(g, sb, iz, ix);
-            uax = st->ax[ix + g.npml - 1];
-            uaz = st->az[iz - 1];
+            const float fdt = (float)g.dt;
+            uax = 0.5f * damp_x(st, ix + g.npml - 1, iz - 1) * fdt;
+            uaz = 0.5f * damp_z(st, ix + g.npml - 1, iz - 1) * fdt;
 #pragma unroll
             for (int comp = 0; comp < 2; ++comp)
 #pragma unroll

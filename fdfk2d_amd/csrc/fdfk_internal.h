@@ -29,6 +29,30 @@ __host__ __device__ inline size_t cell(const Geo &g, int iz, int ix)
     return (size_t)(ix + g.npml - 1) * (size_t)g.ld + (size_t)(iz - 1);
 }
 
+// 7x7 window of receiver (rx, rz) for loadseis (src/Submain.f90:61-62,66,
+// 85): columns xa..xb, rows za..zb, clipped to the padded grid; returns the
+// number of window cells, 0 when the receiver lies more than 3 cells outside
+// the grid along either axis (the reference's loops are then empty and it
+// records 0).  The cell indices are clamped in floating point before the
+// conversion to int, so any coordinate (huge, infinite, NaN) gives a window
+// inside the grid or an empty one.
+__host__ __device__ inline int rec_window(const Geo &g, float x0, float z0, float rx, float rz, int &xa,
+                                          int &xb, int &za, int &zb)
+{
+    float fx = floorf((rx - x0) / g.dx), fz = floorf((rz - z0) / g.dz);
+    const float xlo = (float)(-g.npml - 8), xhi = (float)(g.nx + g.npml + 8);
+    const float zlo = -8.0f, zhi = (float)(g.NZ + 8);
+    fx = fx > xlo ? (fx < xhi ? fx : xhi) : xlo;   // NaN compares false: -> xlo
+    fz = fz > zlo ? (fz < zhi ? fz : zhi) : zlo;
+    const int irx = (int)fx + 1, irz = (int)fz + 1;
+    xa = irx - 3 > 1 - g.npml ? irx - 3 : 1 - g.npml;
+    xb = irx + 3 < g.nx + g.npml ? irx + 3 : g.nx + g.npml;
+    za = irz - 3 > 1 ? irz - 3 : 1;
+    zb = irz + 3 < g.NZ ? irz + 3 : g.NZ;
+    if (xb < xa || zb < za) return 0;
+    return (xb - xa + 1) * (zb - za + 1);
+}
+
 // Cells per time step of the six hybrid-ring background arrays in the order
 // of src/FDFK2D.f90:330-355: left/right/bottom inside (reg), then
 // left/right/bottom outside (pml).
@@ -65,6 +89,17 @@ struct Coefs {
 // which is exactly the reference's bux1l/bpxlx/... cell order.
 enum { S_L = 0, S_R = 1, S_B = 2, S_NONE = -1 };
 
+// Multiaxial damping (Meza-Fajardo & Papageorgiou 2008, BSSA 98:1811): the
+// split parts of a strip cell are damped along x by dX = dx(ix) + r dz(iz)
+// and along z by dZ = dz(iz) + r dx(ix), dx / dz the classical profiles per
+// padded column / row, while the first-derivative memories psi keep the
+// classical profiles (psi_x: dx, psi_z: dz).  The classical layer (r = 0)
+// lets guided modes of a strong low-velocity surface layer grow in the side
+// strips at late time; the cross damping removes that growth.  (Driving the
+// memories with dX, dZ as well makes the second-order split system
+// unstable.)  FDFK_PML_RATIO overrides the ratio (experiments only).
+constexpr float kPmlRatio = 0.1f;
+
 struct Strips {
     // split fields u = u1 + u2 + u3: [strip][comp][part] -> cur / prev
     float *cur[3][2][3];
@@ -72,10 +107,16 @@ struct Strips {
     // first-derivative memories psi: [strip][comp] x-direction, z (bottom only)
     float *psx[3][2];
     float *psz[3][2];
-    // damping: per padded column / row
-    const float *ax, *bx;   // 0.5*d*dt and exp(-d*dt) per column (NX)
-    const float *az, *bz;   // same per row (NZ)
+    // classical damping profiles per padded column (NX) / row (NZ) and the
+    // memory decay factors exp(-d dt) of those profiles
+    const float *dpx, *bx;
+    const float *dpz, *bz;
+    float ratio;   // multiaxial ratio r
 };
+
+// damping of strip cell (iz, ix) along x and along z
+__host__ __device__ inline float damp_x(const Strips *st, int jx, int jz) { return st->dpx[jx] + st->ratio * st->dpz[jz]; }
+__host__ __device__ inline float damp_z(const Strips *st, int jx, int jz) { return st->dpz[jz] + st->ratio * st->dpx[jx]; }
 
 // Twelve background arrays, reference layout and order:
 // 0 uxl_reg 1 uzl_reg 2 uxr_reg 3 uzr_reg 4 uxb_reg 5 uzb_reg

@@ -184,7 +184,7 @@ __device__ __forceinline__ void psi_body(const StepArgs &a, long bid)
     FieldRd rx{a.ux, &g}, rz{a.uz, &g};
     const int jx = ix + g.npml - 1;
     const Strips *st = a.st;
-    if (st->ax[jx] > 0.0f) {
+    if (st->dpx[jx] > 0.0f) {
         float b = st->bx[jx], c = 1.0f - b;
         float gx = cdx_sum<NO>(g, a.cf, rx, iz, ix) / g.dx;
         float gz = cdx_sum<NO>(g, a.cf, rz, iz, ix) / g.dx;
@@ -192,7 +192,7 @@ __device__ __forceinline__ void psi_body(const StepArgs &a, long bid)
         p0[o] = b * p0[o] + c * gx;
         p1[o] = b * p1[o] + c * gz;
     }
-    if (s == S_B && st->az[iz - 1] > 0.0f) {
+    if (s == S_B && st->dpz[iz - 1] > 0.0f) {
         float b = st->bz[iz - 1], c = 1.0f - b;
         float gx = cdz_sum<NO>(g, a.cf, rx, iz, ix) / g.dz;
         float gz = cdz_sum<NO>(g, a.cf, rz, iz, ix) / g.dz;
@@ -238,13 +238,9 @@ __device__ __forceinline__ void record_body(const RecArgs &r, long bid, int istp
     if (i >= r.nrcv) return;   // wave-uniform; the exchange below stays inside the wave
     const Geo &g = r.g;
     const float rx = r.rx[i], rz = r.rz[i], x0 = r.x0, z0 = r.z0, dx = g.dx, dz = g.dz;
-    const int irx = (int)floorf((rx - x0) / dx) + 1;
-    const int irz = (int)floorf((rz - z0) / dz) + 1;
-    const int xa = irx - 3 > 1 - g.npml ? irx - 3 : 1 - g.npml;
-    const int xb = irx + 3 < g.nx + g.npml ? irx + 3 : g.nx + g.npml;
-    const int za = irz - 3 > 1 ? irz - 3 : 1;
-    const int zb = irz + 3 < g.nz + g.npml ? irz + 3 : g.nz + g.npml;
-    const int nzr = zb - za + 1, npt = (xb - xa + 1) * nzr;
+    int xa, xb, za, zb;
+    const int npt = rec_window(g, x0, z0, rx, rz, xa, xb, za, zb);   // 0..49
+    const int nzr = zb - za + 1;
     if (l < npt) {
         const int ix = xa + l / nzr, iz = za + l % nzr;
         double x = fabs((double)(x0 + (float)(ix - 1) * dx) - (double)rx) / (double)dx;

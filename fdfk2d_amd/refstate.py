@@ -6,6 +6,11 @@ coefficient tables of src/Submain.f90:1111-1198, restated for the host side.
 """
 import numpy as np
 
+# Multiaxial ratio of the absorbing layer's damping (the split parts of a
+# strip cell are damped by dX = dx + r dz along x and dZ = dz + r dx along z;
+# the library's kPmlRatio, the oracle's ORC_PML_RATIO_DEFAULT).
+PML_RATIO = 0.1
+
 
 def material_law(vp, vs):
     """rho = (vp+980)/2.76, lambda = rho(vp^2-2vs^2), mu = rho vs^2 in FP32,

@@ -26,10 +26,16 @@ def _normpdf(x, mu, sigma):
     return np.exp(-0.5 * ((x - mu) / sigma) ** 2) / (sigma * np.sqrt(2.0 * np.pi))
 
 
-def itdecon(uin, win, dt, minlag, maxlag, tshift=0.0, f0=8.0, itmax=100, minderr=1e-3):
+def itdecon(uin, win, dt, minlag, maxlag, tshift=0.0, f0=8.0, itmax=100, minderr=1e-3, picks=None,
+            record=None):
     """makeRFitdecon(UIN, WIN, DT, MINLAG, MAXLAG, TSHIFT, F0, ITMAX, MINDERR).
 
-    Returns (rf, lag_times, rms)."""
+    Returns (rf, lag_times, rms).  The spike position of each iteration (the
+    argmax of the correlation) and the iteration count are discrete
+    decisions: traces within rounding of each other can pick different
+    spikes where two correlation peaks nearly tie.  picks (a sequence of
+    lag indices) replays given decisions instead, one iteration per pick;
+    record (a list) receives the decisions taken."""
     uin = np.asarray(uin, np.float64).ravel()
     win = np.asarray(win, np.float64).ravel()
     ntin = uin.size
@@ -60,12 +66,17 @@ def itdecon(uin, win, dt, minlag, maxlag, tshift=0.0, f0=8.0, itmax=100, minderr
     sumsq_i = 1.0
     derr = 100.0 * np.sum(U * U) + minderr
     rms = []
-    while abs(derr) > minderr and it < itmax:
+    while (abs(derr) > minderr and it < itmax) if picks is None else it < len(picks):
         it += 1
         RW, lags = _xcorr(R, W, mx)
         RW = RW * inv_pw
-        ok = np.nonzero((lags >= lag1) & (lags <= lag2))[0]
-        idx = ok[np.argmax(np.abs(RW[ok]))]
+        if picks is None:
+            ok = np.nonzero((lags >= lag1) & (lags <= lag2))[0]
+            idx = ok[np.argmax(np.abs(RW[ok]))]
+        else:
+            idx = int(picks[it - 1])
+        if record is not None:
+            record.append(int(idx))
         P0[idx] += RW[idx]
         rfp = _conv_same(P0, gauss)
         UP = _conv_same(win, rfp)
@@ -90,13 +101,13 @@ def itdecon(uin, win, dt, minlag, maxlag, tshift=0.0, f0=8.0, itmax=100, minderr
     return rf, t, np.asarray(rms)
 
 
-def prf(radial, vertical, dt, f0=8.0):
+def prf(radial, vertical, dt, f0=8.0, picks=None, record=None):
     """P receiver function of one station (utils/plot_PRF.m:18-24)."""
     T = (len(radial) - 1) * dt
-    return itdecon(radial, vertical, dt, -2.0, 0.5 * T - 2.0, 0.0, f0, 100, 1e-3)
+    return itdecon(radial, vertical, dt, -2.0, 0.5 * T - 2.0, 0.0, f0, 100, 1e-3, picks, record)
 
 
-def srf(radial, vertical, dt, f0=8.0):
+def srf(radial, vertical, dt, f0=8.0, picks=None, record=None):
     """S receiver function of one station (utils/plot_SRF.m:18-23)."""
     T = (len(radial) - 1) * dt
-    return itdecon(vertical, radial, dt, -6.5, 0.6 * T - 6.5, 0.0, f0, 100, 1e-3)
+    return itdecon(vertical, radial, dt, -6.5, 0.6 * T - 6.5, 0.0, f0, 100, 1e-3, picks, record)

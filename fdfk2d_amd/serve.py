@@ -81,10 +81,17 @@ def build_job(req):
                 f0=0.5, ds=1e-3, rayp=req["p_skm"] / 1000.0)
 
 
-def footprint_bytes(req):
+# background synthesised in blocks of this many steps (fdfk_fk.block_steps):
+# the FP64 traces and two blocks of the twelve arrays stay on the device
+# instead of every step
+BG_BLOCK = 512
+
+
+def footprint_bytes(req, block_steps=BG_BLOCK):
     """(resident, transient) device bytes of a request: fields, materials,
-    ring, PML and the background for all steps stay resident; the FK work
-    arrays (spectra and traces, FP64) exist only while the background is
+    ring, PML and the background stay resident (every step, or with blocks
+    two blocks plus the FP64 traces they are synthesised from); the FK work
+    arrays (spectra and traces, FP64) exist while the background is
     computed.  dt is estimated at 0.9 of the stability limit for Vp <= 9 km/s."""
     nx, nz, npml = req["nx"], req["nz"], req["npml"]
     nbd = 5
@@ -94,12 +101,17 @@ def footprint_bytes(req):
     fields = 9 * NX * ld * 4 + 2 * NX * NZ * 4
     strips = (2 * npml * nz + NX * npml) * 4 * 15
     ring_pts = 2 * (nz - nbd) * nbd + nbd * nx + 2 * nz * nbd + nbd * (nx + 2 * nbd)
-    bg = 2 * ring_pts * (nt + 1) * 4
     dt = 0.9 * 0.5752 * req["dx"] / 9000.0
     f0 = 0.5
     tneed = nt * dt + req["p_skm"] / 1000.0 * (nx + 2 * nbd) * req["dx"] + 3.0 / f0 + 10.0 / f0
     nfft = 1 << int(math.ceil(math.log2(2.0 * tneed / (dt / 2.0))))
     fk_work = 2 * 2 * (nz + nbd) * nfft * 8 * 2
+    if block_steps and 2 * block_steps < nt + 1:
+        # two blocks of the arrays and one side's FP64 traces (the request's
+        # left and right models are the same)
+        bg = 2 * ring_pts * (2 * block_steps) * 4 + 2 * (nz + nbd) * nfft * 8
+    else:
+        bg = 2 * ring_pts * (nt + 1) * 4
     return fields + strips + bg, fk_work
 
 
@@ -113,8 +125,11 @@ def rel_l2(a, b):
 # serving
 # ---------------------------------------------------------------------------
 class Server:
-    def __init__(self, hbm_budget_gb=200.0, chunk=100, log=None, workers=None, solo_cells=4_000_000):
+    def __init__(self, hbm_budget_gb=200.0, chunk=100, log=None, workers=None, solo_cells=4_000_000,
+                 block_steps=BG_BLOCK):
         self.budget = hbm_budget_gb * 1e9
+        self.block_steps = block_steps
+        self.peak_used = 0.0     # largest admitted footprint total (bytes)
         self.chunk = chunk
         # grids of at least solo_cells padded cells fill the GPU alone: they
         # step on their own streams (the single-job kernels, whose frame
@@ -141,7 +156,7 @@ class Server:
         t2 = time.perf_counter()
         try:
             s.compute_background(job["fk"], job["fk"], req["src_is_p"], job["f0"], job["ds"],
-                                 job["rayp"], job["bazs"])
+                                 job["rayp"], job["bazs"], block_steps=self.block_steps)
         except Exception:
             s.close()
             raise
@@ -195,12 +210,15 @@ class Server:
                     waiting.append(pending.popleft())
                 while waiting:
                     req = waiting[0]
-                    res, work = footprint_bytes(req)
-                    if self.used + res + work > self.budget and (self.active or preparing or finishing):
+                    res, work = footprint_bytes(req, self.block_steps)
+                    # over budget: wait while anything holds device memory
+                    # (batched, solo, preparing or finishing requests)
+                    if self.used + res + work > self.budget and (self.active or self.solo or preparing or finishing):
                         break
                     waiting.popleft()
                     preparing[pool.submit(self._prepare, req)] = (req, work)
                     self.used += res + work
+                    self.peak_used = max(self.peak_used, self.used)
                 for f in [f for f in preparing if f.done()]:
                     req, work = preparing.pop(f)
                     s, secs = f.result()
@@ -215,7 +233,7 @@ class Server:
                 for f in [f for f in finishing if f.done()]:
                     req = finishing.pop(f)
                     (sx, sz), t_done = f.result()
-                    self.used -= footprint_bytes(req)[0]
+                    self.used -= footprint_bytes(req, self.block_steps)[0]
                     lat[req["id"]] = t_done - t0 - req["arrival"]
                     if keep_results:
                         results[req["id"]] = (sx, sz)
@@ -248,9 +266,10 @@ class Server:
         self.batch.close()
 
 
-def serve_plain(reqs, keep_results=False, stats=None):
-    """One request at a time with the single-job API, FIFO by arrival.
-    stats, if a dict, accumulates the seconds of each phase."""
+def serve_plain(reqs, keep_results=False, stats=None, block_steps=BG_BLOCK):
+    """One request at a time with the single-job API, FIFO by arrival, the
+    background synthesised in blocks as the server does.  stats, if a dict,
+    accumulates the seconds of each phase."""
     lat, results = {}, {}
     st = stats if stats is not None else {}
     for k in ("model", "create", "background", "run"):
@@ -267,7 +286,7 @@ def serve_plain(reqs, keep_results=False, stats=None):
                    job["c"], rx=job["rx"], rz=job["rz"])
         tc = time.perf_counter()
         s.compute_background(job["fk"], job["fk"], req["src_is_p"], job["f0"], job["ds"], job["rayp"],
-                             job["bazs"])
+                             job["bazs"], block_steps=block_steps)
         td = time.perf_counter()
         s.run(job["nt"] + 1)
         sx, sz = s.seismograms()

@@ -491,14 +491,23 @@ void orc_backazimuth_grids(double lat_org, double lon_org, double az_org, int n,
  *    uz: X = diffx2[m](uz),    Z = 2/dz^2 H(M1,M2)(uz2-uz1),
  *        M = [Dx(mb (ux2-ux1)) + lb (Dx ux1 + Dx ux2)]/dz
  *  PML (stretched coordinates, split u = u1 + u2 + u3 in the padding strips):
- *    rho (u1_tt + dx u1_t)               = X - Dx(Mx psi_x)
- *    rho (u2_tt + dz u2_t)               = Z - Dz(Mz psi_z)
- *    rho (u3_tt + (dx+dz) u3_t + dx dz u3) = M
+ *    rho (u1_tt + dX u1_t)               = X - Dx(Mx psi_x)
+ *    rho (u2_tt + dZ u2_t)               = Z - Dz(Mz psi_z)
+ *    rho (u3_tt + (dX+dZ) u3_t + dX dZ u3) = M
  *    psi_t + d psi = d * D u   (first-derivative memory, the bp* arrays)
- *  Damping d = d0 ((q-nbd)/(npml-nbd))^npower for q > nbd cells into the
- *  left/right/bottom strips (the first nbd cells are an undamped buffer that
- *  holds the hybrid ring), d0 = -(npower+1) vpml ln(R0) / (2 (npml-nbd) h);
- *  the optional top strip has no buffer.  The damped cells use the material
+ *  Damping profiles d = d0 ((q-nbd)/(npml-nbd))^npower for q > nbd cells
+ *  into the left/right/bottom strips (the first nbd cells are an undamped
+ *  buffer that holds the hybrid ring), d0 = -(npower+1) vpml ln(R0) /
+ *  (2 (npml-nbd) h); the optional top strip has no buffer.
+ *  Multiaxial damping (Meza-Fajardo & Papageorgiou 2008, BSSA 98:1811):
+ *  the split parts of a strip cell are damped along both axes,
+ *    dX = dx(ix) + r dz(iz),   dZ = dz(iz) + r dx(ix)   (the dX, dZ above),
+ *  with the ratio r (orc_set_pml_ratio), while the memories psi keep the
+ *  classical profiles (psi_x decays with dx, psi_z with dz).  The classical
+ *  PML (r = 0) lets guided modes of a strong low-velocity surface layer grow
+ *  in the side strips at late time; the cross damping removes that growth.
+ *  Driving the memories with dX, dZ too makes this second-order split system
+ *  unstable, so the strips keep exactly the reference's memory arrays.  The damped cells use the material
  *  of the outermost undamped cell along the normal (normal-invariant layers;
  *  heterogeneous damped corners are unstable otherwise).
  *  Hybrid TF/SF: the physical region 1..nx x 1..nz holds the total field,
@@ -557,6 +566,11 @@ void orc_damping(int nx, int nz, int npml, int nbd, int is_pml_top, int npower, 
         dpz[j] = (orc_real)d;
     }
 }
+
+/* multiaxial ratio r of the damping (see above) */
+static double g_pml_ratio = ORC_PML_RATIO_DEFAULT;
+void orc_set_pml_ratio(double r) { g_pml_ratio = r; }
+double orc_pml_ratio(void) { return g_pml_ratio; }
 
 /* central first derivative along x / z with the reduced-order rule of
  * diff1 (Submain.f90:192-209), times nothing; returns mysum (not divided) */
@@ -813,6 +827,10 @@ void orc_updatedispl(const int *norder_, const int *nx_, const int *nz_, const i
 
     orc_real *bpx[4][2] = {{bpxlx, bpzlx}, {bpxrx, bpzrx}, {bpxbx, bpzbx}, {bpxtx, bpztx}};
     orc_real *bpz[4][2] = {{NULL, NULL}, {NULL, NULL}, {bpxbz, bpzbz}, {bpxtz, bpztz}};
+    const orc_real ratio = (orc_real)g_pml_ratio;
+    /* damping of the split parts of a strip cell along x and z (multiaxial) */
+#define DAMP_X(iz, ix) (dpx[(ix) + npml - 1] + ratio * dpz[(iz) - izt])
+#define DAMP_Z(iz, ix) (dpz[(iz) - izt] + ratio * dpx[(ix) + npml - 1])
     orc_real *bu[4][2][3] = {{{bux1l, bux2l, bux3l}, {buz1l, buz2l, buz3l}},
                           {{bux1r, bux2r, bux3r}, {buz1r, buz2r, buz3r}},
                           {{bux1b, bux2b, bux3b}, {buz1b, buz2b, buz3b}},
@@ -830,6 +848,7 @@ void orc_updatedispl(const int *norder_, const int *nx_, const int *nz_, const i
         for (int ix = ix0; ix <= ix1; ++ix)
             for (int iz = iz0; iz <= iz1; ++iz) {
                 size_t o = strip_off(g, s, iz, ix);
+                /* memories: classical profiles */
                 orc_real ddx = dpx[ix + npml - 1], ddz = dpz[iz - izt];
                 if (ddx > 0.0f) {
                     orc_real b = (orc_real)exp(-(double)ddx * G.dt), a = 1.0f - b;
@@ -973,8 +992,8 @@ void orc_updatedispl(const int *norder_, const int *nx_, const int *nz_, const i
                 newz[gi] = (2.0f * u1z[gi] - uz[2 * gi + 1]) + bf * rz;
             } else {
                 size_t o = strip_off(g, s, iz, ix);
-                orc_real ax = 0.5f * dpx[ix + npml - 1] * (orc_real)G.dt;
-                orc_real az = 0.5f * dpz[iz - izt] * (orc_real)G.dt;
+                orc_real ax = 0.5f * DAMP_X(iz, ix) * (orc_real)G.dt;
+                orc_real az = 0.5f * DAMP_Z(iz, ix) * (orc_real)G.dt;
                 orc_real c3 = 2.0f - 4.0f * ax * az;
                 for (int comp = 0; comp < 2; ++comp) {
                     orc_real *b1 = bu[s][comp][0], *b2 = bu[s][comp][1], *b3 = bu[s][comp][2];
@@ -991,6 +1010,8 @@ void orc_updatedispl(const int *norder_, const int *nx_, const int *nz_, const i
             }
         }
     }
+#undef DAMP_X
+#undef DAMP_Z
     /* level shift: level 2 <- level 1, level 1 <- new */
     for (size_t i = 0; i < ncell; ++i) {
         ux[2 * i + 1] = ux[2 * i]; ux[2 * i] = newx[i];

@@ -101,6 +101,15 @@ void orc_updatedispl(const int *norder, const int *nx, const int *nz, const int 
     orc_real *bux1b, orc_real *bux2b, orc_real *bux3b, orc_real *buz1b, orc_real *buz2b, orc_real *buz3b,
     orc_real *bux1t, orc_real *bux2t, orc_real *bux3t, orc_real *buz1t, orc_real *buz2t, orc_real *buz3t);
 
+/* Multiaxial ratio r of the PML damping: the split parts of a strip cell are
+ * damped by dX = dx + r dz along x and dZ = dz + r dx along z; the memories
+ * keep dx and dz (r = 0: classical PML). */
+#ifndef ORC_PML_RATIO_DEFAULT
+#define ORC_PML_RATIO_DEFAULT 0.1
+#endif
+void orc_set_pml_ratio(double r);
+double orc_pml_ratio(void);
+
 /* Pointwise evaluation of the six right-hand-side parts at one cell:
  * parts = {Xx, Zx, Mx, Xz, Zz, Mz} (x-terms, z-terms, mixed terms of the ux
  * and uz equations).  ux2d/uz2d/c are 2-D (NZ x NX) arrays; c3 holds

@@ -213,6 +213,18 @@ def updatedispl(st, it):
         *[_p(x) for x in a["bp"]], *[_p(x) for x in a["bu"]])
 
 
+def set_pml_ratio(r, fp64=None):
+    """Multiaxial ratio of the oracle's PML (both builds unless fp64 names one)."""
+    libs = [lib()] if fp64 is False else ([lib64()] if fp64 else [lib()] + ([lib64()] if os.path.exists(LIB64_PATH) else []))
+    for L in libs:
+        L.orc_set_pml_ratio(C.c_double(r))
+
+
+def pml_ratio():
+    lib().orc_pml_ratio.restype = C.c_double
+    return lib().orc_pml_ratio()
+
+
 def background(nx, nz, npml, izt, nbd, nt, src_is_p, x0, z0, dx, dz, dt, ds, f0, rayp, bazs,
                fk_left, fk_right, fp64=False):
     """Oracle FK background in the reference's twelve-array layout (float64

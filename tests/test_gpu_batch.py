@@ -171,3 +171,24 @@ def test_server_with_solo_requests_equals_plain_path():
     for i in res:
         assert np.abs(res_p[i][1]).max() > 0
         assert np.array_equal(res[i][0], res_p[i][0]) and np.array_equal(res[i][1], res_p[i][1])
+
+
+def test_server_budget_holds_with_solo_requests():
+    """Admission by footprint counts the requests stepping on their own
+    streams: with a budget that fits one request at a time and every request
+    solo, the admitted total never exceeds the budget.  The background is
+    synthesised in blocks of 64 steps; results equal the plain path's."""
+    reqs = serve.request_mix(4, seed=11, rate=50.0, nx_range=(120, 300), nz_range=(50, 100),
+                             steps_range=(300, 500), dx_choices=(100.0, 200.0))
+    big = max(sum(serve.footprint_bytes(r, 64)) for r in reqs)
+    budget = 1.2 * big
+    print("case: serving 4 solo requests, budget %.3g GB (one request), blocks of 64 steps" % (budget / 1e9),
+          flush=True)
+    srv = serve.Server(hbm_budget_gb=budget / 1e9, chunk=50, solo_cells=1, block_steps=64)
+    lat, total, res = srv.run(reqs, keep_results=True)
+    srv.close()
+    assert srv.peak_used <= srv.budget
+    lat_p, total_p, res_p = serve.serve_plain(reqs, keep_results=True)
+    for i in res_p:
+        assert np.abs(res_p[i][1]).max() > 0
+        assert np.array_equal(res[i][0], res_p[i][0]) and np.array_equal(res[i][1], res_p[i][1])

@@ -264,6 +264,37 @@ def test_receivers_off_grid_at_depth_with_origin():
     assert rel_l2(gx, ox) < 1e-5 and rel_l2(gz, oz) < 1e-5
 
 
+def test_receivers_outside_the_grid_record_zero():
+    """Receivers more than three cells outside the padded grid along one or
+    both axes (beyond nx + npml and above the surface, far left and deep,
+    huge coordinates): the reference's loadseis loops are empty and record 0
+    (src/Submain.f90:66,85); the recorder's window must be empty too, not
+    wrap to a positive count and read outside the fields.  A receiver inside
+    the grid records the same as the oracle's."""
+    print("case: receivers outside the padded grid, 121x60, 301 iterations", flush=True)
+    c = Case(nt=300)
+    far = (c.nx + c.npml + 10) * c.dx
+    rx = np.array([far, -(c.npml + 10) * c.dx, 1.0e30, 5000.0, far, 3000.0], np.float32)
+    rz = np.array([-10 * c.dz, 2000.0, -1.0e30, 0.0, (c.nz + c.npml + 20) * c.dz, 1.0e9], np.float32)
+    bg = c.oracle_background()
+    from oracle import pyoracle as po
+    st = c.state()
+    st["bg"] = bg
+    ox = []
+    for it in range(c.nt + 1):
+        ox.append(po.loadseis(c.nx, c.nz, c.npml, 1, 0.0, 0.0, c.dx, c.dz, 5000.0, 0.0, st["ux"], st["uz"]))
+        po.updatedispl(st, it)
+    s = lib.Solver(c.nx, c.nz, c.npml, c.norder, c.dx, c.dz, c.dt, c.nt, c.c, rx=rx, rz=rz)
+    s.set_background(bg)
+    s.run(c.nt + 1)
+    gx, gz = s.seismograms()
+    s.close()
+    outside = [0, 1, 2, 4, 5]
+    assert not gx[outside].any() and not gz[outside].any()
+    oz = np.array([o[1] for o in ox], np.float32)
+    assert np.abs(oz).max() > 1e-4 and rel_l2(gz[3], oz) < 1e-5
+
+
 def test_recording_interval():
     """nstp = 3 (tstp = 3 dt): nt/3 + 1 samples, taken at iterations 0, 3,
     6, ... before the update (src/FDFK2D.f90:311-317, 497-522), equal to the

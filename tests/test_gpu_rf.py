@@ -19,6 +19,7 @@ import numpy as np
 import pytest
 
 from fdfk2d_amd import driver, io, rf
+from tests._cases import Case, oracle_run, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -98,11 +99,18 @@ def test_srf_sp_precursor_from_gpu_simulation(tmp_path, monkeypatch):
 def test_rf_parity_with_oracle():
     """Receiver functions from the GPU seismograms against those from the
     oracle's on the same inputs (a 121 x 60 two-layer model with a Moho at
-    8 km and a Gaussian anomaly, 2001 iterations, 7 surface receivers):
-    P-RF and S-RF of every receiver to rel-L2 1e-5, the seismograms' bar."""
+    8 km and a Gaussian anomaly, 2001 iterations, 7 surface receivers), to
+    rel-L2 1e-5, the seismograms' bar.  The deconvolution's spike picks are
+    discrete decisions (argmax of a correlation): where two candidates nearly
+    tie, traces within rounding of each other can pick differently (receiver
+    3's P-RF here: 1e-6 relative noise on the oracle's own traces moves it by
+    3.8e-4).  So:
+    - every P-RF and S-RF with the oracle run's picks replayed on the GPU
+      traces (the deconvolution is then continuous in the traces);
+    - free-running, every RF whose GPU run takes the oracle run's picks, and
+      at most two of the 14 taking other picks."""
     print("case: RF parity, 121 x 60, 2001 iterations, 7 receivers", flush=True)
     from fdfk2d_amd import lib
-    from tests._cases import Case, oracle_run, rel_l2
     c = Case(nx=121, nz=60, npml=15, dx=200.0, dz=200.0, dt=0.01, nt=2000,
              layers=((0.0, 6000.0, 3450.0), (8000.0, 8000.0, 4450.0)), nrcv=7)
     bg = c.oracle_background()
@@ -114,12 +122,19 @@ def test_rf_parity_with_oracle():
     gx, gz = s.seismograms()
     s.close()
     assert rel_l2(gx, ox) < 1e-5 and rel_l2(gz, oz) < 1e-5
-    worst = 0.0
+    worst_replay = worst_free = 0.0
+    same = 0
     for k in range(len(c.rx)):
         for f in (rf.prf, rf.srf):
-            want, tw, _ = f(ox[k].astype(np.float64), oz[k].astype(np.float64), c.dt)
-            got, tg, _ = f(gx[k].astype(np.float64), gz[k].astype(np.float64), c.dt)
+            po_, pg = [], []
+            want, tw, _ = f(ox[k].astype(np.float64), oz[k].astype(np.float64), c.dt, record=po_)
+            free, tg, _ = f(gx[k].astype(np.float64), gz[k].astype(np.float64), c.dt, record=pg)
+            got = f(gx[k].astype(np.float64), gz[k].astype(np.float64), c.dt, picks=po_)[0]
             assert np.array_equal(tg, tw) and np.abs(want).max() > 0
-            worst = max(worst, rel_l2(got, want))
-    print("worst RF rel-L2 GPU vs oracle: %.3g" % worst, flush=True)
-    assert worst < 1e-5
+            worst_replay = max(worst_replay, rel_l2(got, want))
+            if pg == po_:
+                same += 1
+                worst_free = max(worst_free, rel_l2(free, want))
+    print("RF rel-L2 GPU vs oracle: replayed picks %.3g (14 RFs), free-running %.3g (%d RFs with the "
+          "oracle's picks)" % (worst_replay, worst_free, same), flush=True)
+    assert worst_replay < 1e-5 and worst_free < 1e-5 and same >= 12

@@ -48,3 +48,17 @@ def test_prf_lags_of_published_figure():
         k = np.argmax(np.abs(r[w]))
         assert abs(t[w][k] - lag) < 0.05
         assert np.sign(r[w][k]) == sign
+
+
+def test_replayed_picks_reproduce_the_rf():
+    """Replaying the spike picks a run recorded gives that run's RF bit for
+    bit (the decisions are the only discrete part of the deconvolution)."""
+    rng = np.random.default_rng(4)
+    dt, n = 0.05, 600
+    t = np.arange(n) * dt
+    vert = np.exp(-((t - 5.0) / 0.4) ** 2) + 0.01 * rng.standard_normal(n)
+    rad = 0.6 * np.roll(vert, 40) - 0.3 * np.roll(vert, 120) + 0.01 * rng.standard_normal(n)
+    picks = []
+    r1, t1, m1 = rf.prf(rad, vert, dt, record=picks)
+    r2, t2, m2 = rf.prf(rad, vert, dt, picks=picks)
+    assert len(picks) == len(m1) and np.array_equal(r1, r2) and np.array_equal(m1, m2)
