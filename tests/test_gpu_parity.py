@@ -292,7 +292,7 @@ def test_receivers_outside_the_grid_record_zero():
     outside = [0, 1, 2, 4, 5]
     assert not gx[outside].any() and not gz[outside].any()
     oz = np.array([o[1] for o in ox], np.float32)
-    assert np.abs(oz).max() > 1e-4 and rel_l2(gz[3], oz) < 1e-5
+    assert np.abs(oz).max() > 1e-5 and rel_l2(gz[3], oz) < 1e-5
 
 
 def test_recording_interval():
