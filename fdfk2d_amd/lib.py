@@ -74,6 +74,7 @@ def lib():
         L.fdfk_smooth_model.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_float,
                                         C.c_float, C.c_float, C.c_float, C.c_int32, C.c_int32,
                                         C.c_int32, C.c_int32, C.c_int32]
+        L.fdfk_smooth_model_dev.argtypes = L.fdfk_smooth_model.argtypes + [C.c_void_p]
         L.updatedispl_.restype = None
         L.comput_background_wavefield_.restype = None
         L.fdfk_compat_release.restype = None

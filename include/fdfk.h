@@ -159,6 +159,21 @@ void fdfk_batch_destroy(fdfk_batch *b);
 int fdfk_smooth_model(const float *vel, float *out, int32_t nx, int32_t nz, float lambdax,
                       float lambdaz, float dx, float dz, int32_t iz0, int32_t iz1, int32_t ix0,
                       int32_t ix1, int32_t method);
+/* Allocation-free variant of fdfk_smooth_model on device buffers.  No
+ * counterpart in the reference; it is for callers that cannot allocate
+ * device memory (code under stream capture, tight serving loops).  vel and
+ * out are device arrays of nx*nz FP32 values (out must not alias vel); the
+ * work is queued on `stream` (a hipStream_t, NULL for the null stream) and
+ * the call returns once the host-side weight tables are on the device.  The
+ * scratch memory (weight tables and the intermediate model of method 2) is a
+ * fixed 64 MiB array per GPU held for the life of the process: a method-2
+ * model may hold up to about 16.7 M cells (the 8081 x 2040 padded model of
+ * the large solve fits), larger requests fail with FDFK_EINVAL.  No device
+ * memory is allocated or freed.  Calls that overlap on one GPU share the
+ * scratch array and race. */
+int fdfk_smooth_model_dev(const float *vel, float *out, int32_t nx, int32_t nz, float lambdax,
+                          float lambdaz, float dx, float dz, int32_t iz0, int32_t iz1, int32_t ix0,
+                          int32_t ix1, int32_t method, void *stream);
 
 /* --- drop-in compatibility: the exact external procedures the reference
  * driver links (all arguments by reference, gfortran/flang mangling).  Device
