@@ -141,7 +141,7 @@ def copy_calibration(mib=512, reps=5):
     print(json.dumps({"copy_calibration_bytes": n * 4, "reps": reps}), flush=True)
 
 
-def serve_bench(n, seed=5, rate=200.0, full=False):
+def serve_bench(n, seed=5, rate=200.0, full=False, log=None):
     """Scaled continuous-batching serving run (the request mix of the serving
     workload with smaller grids and step counts so that both the batched and
     the one-at-a-time plain path finish in about a minute): p50/p99 latency
@@ -159,13 +159,21 @@ def serve_bench(n, seed=5, rate=200.0, full=False):
                                  steps_range=(600, 2500), dx_choices=(100.0, 200.0))
         shapes = "400..2000 x 160..500 log-uniform, 600..2500 steps, dx 100/200 m"
     # untimed pass: rocFFT compiles its kernels for each new FFT length at run
-    # time; both timed paths then start with the same warm caches
-    serve.serve_plain(reqs)
-    srv = serve.Server(hbm_budget_gb=200.0, chunk=100)
-    lat_b, tot_b, res_b = srv.run(reqs, keep_results=True)
+    # time; both timed paths then start with the same warm caches (for long
+    # mixes the first 40 requests cover the lengths)
+    serve.serve_plain(reqs if n <= 40 else [dict(r, arrival=0.0) for r in reqs[:40]])
+    if log:
+        log("serving: warm-up done")
+    # seismograms kept for the equality check: all of them for short mixes,
+    # every tenth request of long ones (a large request's record is 64 MB)
+    keep = True if n <= 40 else set(range(0, n, 10))
+    srv = serve.Server(hbm_budget_gb=200.0, chunk=100, log=log)
+    lat_b, tot_b, res_b = srv.run(reqs, keep_results=keep)
     srv.close()
+    if log:
+        log("serving: batched path done, %.1f s" % tot_b)
     plain_s = {}
-    lat_p, tot_p, res_p = serve.serve_plain(reqs, keep_results=True, stats=plain_s)
+    lat_p, tot_p, res_p = serve.serve_plain(reqs, keep_results=keep, stats=plain_s, log=log)
     worst = 0.0
     for rid in res_p:
         for a_, b_ in zip(res_b[rid], res_p[rid]):
@@ -181,7 +189,7 @@ def serve_bench(n, seed=5, rate=200.0, full=False):
                             prep_s={k: round(v, 3) for k, v in srv.prep_parts.items()}),
             "plain": dict(p, phase_s={k: round(v, 3) for k, v in plain_s.items()}),
             "speedup_solves_per_hour": round(b["solves_per_hour"] / p["solves_per_hour"], 3),
-            "max_rel_l2_batched_vs_plain": worst}
+            "max_rel_l2_batched_vs_plain": worst, "compared_requests": len(res_p)}
 
 
 def full_solve_bench(nt=20000, block=512):
@@ -205,6 +213,26 @@ def full_solve_bench(nt=20000, block=512):
     gib = s.device_bytes / 2**30
     s.close()
     cells = cfg["NX"] * cfg["NZ"] * (nt + 1)
+    # health of the whole record, from a second (untimed) run sampled at
+    # thirds of the record: the side PML strips must stay far below the
+    # surface record and not grow (a growing absorbing layer shows there)
+    NX, NZ, npml, nx, nz = cfg["NX"], cfg["NZ"], cfg["npml"], cfg["nx"], cfg["nz"]
+    s = lib.Solver(cfg["nx"], cfg["nz"], cfg["npml"], cfg["norder"], cfg["dx"], cfg["dz"], cfg["dt"], nt,
+                   cfg["c"], rx=cfg["rx"], rz=cfg["rz"])
+    s.compute_background(cfg["fk_left"], cfg["fk_right"], cfg["src_is_p"], cfg["f0"], cfg["ds"], cfg["rayp"],
+                         cfg["bazs"], block_steps=block)
+    sides, finite = [], bool(np.isfinite(sx).all() and np.isfinite(sz).all())
+    for k in range(3):
+        s.run((nt + 1) // 3 if k < 2 else nt + 1 - 2 * ((nt + 1) // 3))
+        ux, uz = s.get_state()
+        a = np.maximum(np.abs(ux.reshape(NX, NZ, 2)[:, :, 0]), np.abs(uz.reshape(NX, NZ, 2)[:, :, 0]))
+        finite = finite and bool(np.isfinite(a).all())
+        sides.append(float(max(a[:npml - 5, :nz].max(), a[npml + nx + 5:, :nz].max())))
+    s.close()
+    side = sides[-1]
+    peak = float(max(np.abs(sx).max(), np.abs(sz).max()))
+    late_n = int(round(20.0 / cfg["dt"]))
+    late = float(max(np.abs(sx[:, -late_n:]).max(), np.abs(sz[:, -late_n:]).max()))
     return {"metric": "solves_per_hour", "value": round(3600.0 / (t3 - t0), 1), "unit": "solves/h",
             "higher_is_better": True, "data": "synthetic", "dtype": "f32",
             "config": {"workload": f"single large solve 8001x2000, {nt + 1} iterations, 401 receivers, "
@@ -212,8 +240,11 @@ def full_solve_bench(nt=20000, block=512):
             "wall_s": round(t3 - t0, 3), "create_and_background_s": round(t1 - t0, 3),
             "loop_s": round(t2 - t1, 3), "loop_gcells": round(cells / (t2 - t1) / 1e9, 2),
             "seismograms_s": round(t3 - t2, 3),
-            "finite": bool(np.isfinite(sx).all() and np.isfinite(sz).all()),
-            "max_abs_seis_z": float(np.abs(sz).max())}
+            "health": {"finite": finite, "surface_peak": peak,
+                       "surface_max_last_20s": late,
+                       "side_pml_max_at_thirds": sides,
+                       "end_s": round((nt + 1) * cfg["dt"], 3),
+                       "ok": finite and side < 0.1 * peak and side < 2.0 * max(sides[:2])}}
 
 
 def compat_bench(steps, warmup=3):
@@ -297,6 +328,49 @@ def batch_bench(njobs, seed=3, steps=1000):
             "sequential_gcells": round(cells / t_seq / 1e9, 3), "speedup": round(t_seq / t_bat, 3)}
 
 
+def c3_bench(njobs=64, block=512, log=lambda *a: None):
+    """The batched workload (fdfk2d_amd.configs.c3_requests: 64 simulations,
+    nx 2000..8000, nz 800..2000 at 50 m, 10000 iterations, P/S, p 0.04..0.08
+    s/km, all submitted at once): every job one at a time with the
+    single-job API, against the continuous-batching server (graph-replayed
+    batched steps) with the 200 GB HBM budget; the 64 jobs do not fit at
+    once (each keeps about 3 GB of FP64 traces for its background
+    synthesis), so they join the batch as memory frees.  Twice: every job in
+    the batch, and the server's default split (grids of 4 M cells or more
+    step on their own streams beside the batch).  Times include the FK
+    backgrounds (in blocks of `block` steps) and the model set-up; every
+    eighth job's seismograms must be identical on all paths."""
+    from fdfk2d_amd import configs, serve
+    from fdfk2d_amd.lib import trim_memory
+    reqs = configs.c3_requests(njobs)
+    keep = set(range(0, njobs, 8))
+    # warm-up: rocFFT compiles a kernel per FFT length at run time
+    serve.serve_plain(reqs[:2], block_steps=block)
+    lat_p, t_plain, res_p = serve.serve_plain(reqs, keep_results=keep, block_steps=block, log=log)
+    log("c3 plain: %.1f s" % t_plain)
+    trim_memory()
+    out = {}
+    for name, solo in (("batched", 10 ** 12), ("batched_with_solo_streams", 4_000_000)):
+        srv = serve.Server(hbm_budget_gb=200.0, chunk=100, solo_cells=solo, block_steps=block, log=log)
+        lat_b, t_b, res_b = srv.run(reqs, keep_results=keep)
+        srv.close()
+        trim_memory()
+        log("c3 %s: %.1f s" % (name, t_b))
+        same = all(np.array_equal(a_, b_) for rid in res_p for a_, b_ in zip(res_b[rid], res_p[rid]))
+        out[name] = dict(serve.summarize(lat_b, t_b, njobs), peak_admitted_gb=round(srv.peak_used / 1e9, 1),
+                         speedup_solves_per_hour=round(t_plain / t_b, 3), seismograms_identical=same)
+    cells = sum((r["nx"] + 2 * r["npml"]) * (r["nz"] + r["npml"]) * r["nsteps"] for r in reqs)
+    best = max(out.values(), key=lambda d: d["solves_per_hour"])
+    return {"metric": "solves_per_hour", "value": best["solves_per_hour"], "unit": "solves/h",
+            "higher_is_better": True, "data": "synthetic", "dtype": "f32",
+            "config": {"workload": f"batched: {njobs} simulations (seed 3), nx 2000..8000, nz 800..2000, dx 50 m, "
+                                   "10000 iterations, 40-cell PML, P/S alternating, p 0.04..0.08 s/km",
+                       "cell_updates": cells, "background": f"FK on the GPU in blocks of {block} steps",
+                       "hbm_budget_gb": 200.0},
+            "plain": dict(serve.summarize(lat_p, t_plain, njobs), gcells=round(cells / t_plain / 1e9, 2)),
+            **out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -325,15 +399,24 @@ def main():
                     help="N iterations through the drop-in updatedispl_ (PCIe-inclusive rate)")
     ap.add_argument("--batch", type=int, default=0, metavar="N",
                     help="batched-execution measurement with N jobs instead of the default line")
+    ap.add_argument("--c3", type=int, default=0, metavar="N",
+                    help="the batched workload with N of its jobs (64: all), plain vs batched")
+    ap.add_argument("--no-solve", action="store_true",
+                    help="default line without the whole-solve and serving objects")
     a = ap.parse_args()
     if a.copy_calib:
         copy_calibration()
         return
     if a.serve:
-        print(json.dumps(serve_bench(a.serve, full=a.serve_full)), flush=True)
+        log = lambda m: print(m, file=sys.stderr, flush=True)
+        print(json.dumps(serve_bench(a.serve, full=a.serve_full, log=log)), flush=True)
         return
     if a.batch:
         print(json.dumps(batch_bench(a.batch)), flush=True)
+        return
+    if a.c3:
+        log = lambda m: print(m, file=sys.stderr, flush=True)
+        print(json.dumps(c3_bench(a.c3, log=log)), flush=True)
         return
     if a.compat:
         print(json.dumps(compat_bench(a.compat)), flush=True)
@@ -387,7 +470,7 @@ def main():
             "tile_autotune": r["tuned"],
             "material_storage": ("fp16 lam, mu (power-of-two scales); fp32 dt^2/rho" if a.precision == "fp16"
                                  else "fp32"),
-            "solves_per_hour_at_20000_steps": round(3600.0 / (ms * 1e-3 * full_steps), 3),
+            "solves_per_hour_at_20000_steps_loop_only": round(3600.0 / (ms * 1e-3 * full_steps), 3),
             "finite": r["finite"],
         },
         "roofline": {
@@ -407,6 +490,24 @@ def main():
             "cells_per_launch": k_cells,
         },
     }
+    if a.workload == "c2" and not a.no_solve and a.precision == "fp32":
+        # the rest of the metric, measured rather than projected: one whole
+        # solve end to end (its health covers the full 60 s record), and
+        # solves/h with p50/p99 latency of continuous batching against the
+        # plain one-at-a-time path on a bounded request mix
+        fs = full_solve_bench()
+        out["full_solve"] = {k: fs[k] for k in ("value", "unit", "wall_s", "create_and_background_s", "loop_s",
+                                                "loop_gcells", "health")}
+        out["config"]["finite"] = fs["health"]["finite"]
+        sv = serve_bench(24)
+        sf = serve_bench(12, full=True)
+        out["serving"] = {
+            name: {"workload": r["config"]["workload"], "solves_per_hour": r["value"],
+                   "batched": {k: r["batched"][k] for k in ("wall_s", "solves_per_hour", "p50_s", "p99_s")},
+                   "plain": {k: r["plain"][k] for k in ("wall_s", "solves_per_hour", "p50_s", "p99_s")},
+                   "speedup_solves_per_hour": r["speedup_solves_per_hour"],
+                   "max_rel_l2_batched_vs_plain": r["max_rel_l2_batched_vs_plain"]}
+            for name, r in (("scaled_mix_24", sv), ("serving_workload_first_12", sf))}
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(a.cpu_steps)
     print(json.dumps(out), flush=True)

@@ -96,3 +96,23 @@ def c2(nt=19999, perturb=True):
         vs = (vs * (1.0 + 0.05 * d)).astype(np.float32)
     vp = (np.float32(1.73) * vs).astype(np.float32)
     return _common(nx, nz, npml, dx, dx, 0.003, nt, vp, vs, layers, True, 0.07, 0.5, 401, 1000.0)
+
+
+def c3_requests(njobs=64, seed=3, nsteps=10000):
+    """The batched workload: njobs simulations drawn with seed 3, nx uniform
+    in 2000..8000 and nz uniform in 800..2000 points at dx = dz = 50 m, 10000
+    iterations each, 40-cell PML, ray parameter uniform in 0.04..0.08 s/km,
+    P and S incidence alternating, layered + smoothed 5% random models as
+    the single large solve (fdfk2d_amd.serve.make_model, per-job seed).
+    dt is the large solve's 0.003 s, or 0.9 of the reference's stability
+    limit (src/Submain.f90:139-146) when a perturbed model needs less; the
+    request dicts are those of fdfk2d_amd.serve."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(njobs):
+        nx = int(rng.integers(2000, 8001))
+        nz = int(rng.integers(800, 2001))
+        out.append(dict(id=i, arrival=0.0, nx=nx, nz=nz, dx=50.0, nsteps=nsteps, src_is_p=(i % 2 == 0),
+                        p_skm=float(rng.uniform(0.04, 0.08)), seed=int(rng.integers(1 << 30)), npml=40,
+                        dt_max=0.003))
+    return out

@@ -68,6 +68,33 @@ static BlockCache &block_cache()
     static BlockCache *c = new BlockCache();   // outlives static teardown
     return *c;
 }
+// frees every block kept for the current device; false when it held none
+static bool cache_release()
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return false;
+    std::multimap<size_t, void *> blocks;
+    {
+        BlockCache &c = block_cache();
+        std::lock_guard<std::mutex> lk(c.mu);
+        blocks.swap(c.free[dev]);
+    }
+    for (auto &b : blocks) (void)hipFree(b.second);
+    return !blocks.empty();
+}
+
+// stream-ordered allocation that, when the device is out of memory, frees
+// the kept blocks of destroyed solvers and tries once more
+static hipError_t malloc_async_retry(void **p, size_t bytes, hipStream_t st)
+{
+    hipError_t e = hipMallocAsync(p, bytes, st);
+    if (e == hipErrorOutOfMemory) {
+        (void)hipGetLastError();
+        if (cache_release()) e = hipMallocAsync(p, bytes, st);
+    }
+    return e;
+}
+
 static hipError_t block_get(size_t bytes, void **p, size_t *got)
 {
     int dev = 0;
@@ -90,15 +117,7 @@ static hipError_t block_get(size_t bytes, void **p, size_t *got)
     if (e == hipErrorOutOfMemory) {
         // the cache may hold blocks no request fits: give them back and retry once
         (void)hipGetLastError();
-        std::multimap<size_t, void *> blocks;
-        {
-            BlockCache &c = block_cache();
-            std::lock_guard<std::mutex> lk(c.mu);
-            blocks.swap(c.free[dev]);
-        }
-        if (blocks.empty()) return e;
-        for (auto &b : blocks) (void)hipFree(b.second);
-        e = hipMalloc(p, bytes);
+        if (cache_release()) e = hipMalloc(p, bytes);
     }
     return e;
 }
@@ -809,7 +828,7 @@ int fdfk_create(const fdfk_job *job, fdfk_solver **out)
     // materials: reference layout -> dt^2/rho, lambda, mu
     // staging buffer in the stream's order: freeing it waits for no other stream
     float *dc = nullptr;
-    if (hipMallocAsync((void **)&dc, (size_t)3 * ncell * 4, s->stream) != hipSuccess)
+    if (malloc_async_retry((void **)&dc, (size_t)3 * ncell * 4, s->stream) != hipSuccess)
         return bad(fail(FDFK_ENOMEM, "staging"));
     if (h2d(s->stream, dc, job->c, (size_t)3 * ncell * 4) != hipSuccess) {
         hipFreeAsync(dc, s->stream);

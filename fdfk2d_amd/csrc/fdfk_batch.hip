@@ -53,13 +53,23 @@ __device__ __forceinline__ int find_job(const int *pre, int n, int bid)
     return lo;
 }
 
+// The step count since the active set's tables were built lives on the
+// device (ctr[0]): the kernels of a step read it and pick the table of the
+// step's parity themselves, and kb_tick advances it after the step, so the
+// launches of every step are identical and one step can be captured as a
+// graph once per active set and replayed.
+__global__ void kb_tick(int *ctr) { ctr[0] += 1; }
+__global__ void kb_set(int *ctr, int v) { ctr[0] = v; }
+
 // one launch for the per-step preparation of every member: the hybrid ring
 // background, the PML memories and (on recording iterations) the receivers,
 // in that block order; 256-lane blocks, kRecPerBlock receivers per block
 template <int NO>
-__global__ __launch_bounds__(256) void kb_prep(const BatchJob<NO> *J, const BatchPrefix *P, int dstep)
+__global__ __launch_bounds__(256) void kb_prep(const BatchJob<NO> *J0, const BatchPrefix *P, const int *ctr)
 {
     const int n = P->n;
+    const int dstep = ctr[0];
+    const BatchJob<NO> *J = J0 + (dstep & 1) * n;
     int b = blockIdx.x;
     const int nr = P->pre[KB_RING][n], np = P->pre[KB_PSI][n];
     if (b < nr) {
@@ -85,11 +95,13 @@ __global__ __launch_bounds__(256) void kb_prep(const BatchJob<NO> *J, const Batc
 // the interior tiles of every member, then their frame tiles, in one launch
 // (the single-job k_step arrangement)
 template <int NO>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kIntWaves<NO>))) void kb_step(const BatchJob<NO> *J,
-                                                                                       const BatchPrefix *P)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kIntWaves<NO>))) void kb_step(const BatchJob<NO> *J0,
+                                                                                       const BatchPrefix *P,
+                                                                                       const int *ctr)
 {
     extern __shared__ float sm[];
     const int n = P->n;
+    const BatchJob<NO> *J = J0 + (ctr[0] & 1) * n;
     const int ni = P->pre[KB_INT][n];
     if ((int)blockIdx.x < ni) {
         const int t = xcd_tile(blockIdx.x, ni);
@@ -126,10 +138,19 @@ struct fdfk_batch {
     int tn = 0, dstep = 0, nprep = 0, nstepb = 0;
     size_t tsh = 0;
     bool prof = false;
+    // device step counter of the current tables, and the step of the
+    // current active set captured as a graph (rebuilt with the tables)
+    int *dctr = nullptr;
+    bool use_graph = true;
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_int, ev_step;
     ~fdfk_batch()
     {
         if (stream) hipStreamSynchronize(stream);
+        if (gexec) hipGraphExecDestroy(gexec);
+        if (graph) hipGraphDestroy(graph);
+        if (dctr) hipFree(dctr);
         for (int i = 0; i < R; ++i)
             if (ev[i]) hipEventDestroy(ev[i]);
         for (int i = 0; i < TS; ++i) {
@@ -261,6 +282,11 @@ int build_tables(fdfk_batch *B, const std::vector<fdfk_solver *> &act)
     }
     HIPCHK(hipMemcpyAsync(B->dtab[B->tslot], B->htab[B->tslot], kJobOff + 2 * JB * n, hipMemcpyHostToDevice,
                           B->stream));
+    hipLaunchKernelGGL(kb_set, dim3(1), dim3(1), 0, B->stream, B->dctr, 0);
+    HIPCHK(hipGetLastError());
+    // the captured step refers to the old tables: capture again
+    if (B->gexec) { HIPCHK(hipGraphExecDestroy(B->gexec)); B->gexec = nullptr; }
+    if (B->graph) { HIPCHK(hipGraphDestroy(B->graph)); B->graph = nullptr; }
     B->tact = act;
     B->tn = n;
     B->dstep = 0;
@@ -284,20 +310,42 @@ int batch_step(fdfk_batch *B)
     for (fdfk_solver *s : act)                  // members whose background is synthesised in blocks
         if (int rc = bg_refill(s, B->stream)) return rc;
     auto *dP = (const BatchPrefix *)B->dtab[B->tslot];
-    auto *dJ = (const BatchJob<NO> *)((char *)B->dtab[B->tslot] + kJobOff) + (B->dstep & 1) * B->tn;
-    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
+    auto *dJ = (const BatchJob<NO> *)((char *)B->dtab[B->tslot] + kJobOff);
+    // the launches of one step: prep, step, counter tick
+    auto launch = [&](hipStream_t st) {
+        if (B->nprep) hipLaunchKernelGGL(kb_prep<NO>, dim3(B->nprep), dim3(256), 0, st, dJ, dP, (const int *)B->dctr);
+        if (B->nstepb)
+            hipLaunchKernelGGL(kb_step<NO>, dim3(B->nstepb), dim3(256), B->tsh, st, dJ, dP, (const int *)B->dctr);
+        hipLaunchKernelGGL(kb_tick, dim3(1), dim3(1), 0, st, B->dctr);
+    };
     if (B->prof) {
+        hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
         hipEventCreate(&e0); hipEventCreate(&e1); hipEventCreate(&e2); hipEventCreate(&e3);
         hipEventRecord(e0, B->stream);
-    }
-    if (B->nprep) hipLaunchKernelGGL(kb_prep<NO>, dim3(B->nprep), dim3(256), 0, B->stream, dJ, dP, B->dstep);
-    if (B->prof) hipEventRecord(e1, B->stream);
-    if (B->nstepb) hipLaunchKernelGGL(kb_step<NO>, dim3(B->nstepb), dim3(256), B->tsh, B->stream, dJ, dP);
-    if (B->prof) hipEventRecord(e2, B->stream);
-    if (B->prof) {
+        if (B->nprep)
+            hipLaunchKernelGGL(kb_prep<NO>, dim3(B->nprep), dim3(256), 0, B->stream, dJ, dP, (const int *)B->dctr);
+        hipEventRecord(e1, B->stream);
+        if (B->nstepb)
+            hipLaunchKernelGGL(kb_step<NO>, dim3(B->nstepb), dim3(256), B->tsh, B->stream, dJ, dP,
+                               (const int *)B->dctr);
+        hipEventRecord(e2, B->stream);
+        hipLaunchKernelGGL(kb_tick, dim3(1), dim3(1), 0, B->stream, B->dctr);
         hipEventRecord(e3, B->stream);
         B->ev_int.emplace_back(e1, e2);
         B->ev_step.emplace_back(e0, e3);
+    } else if (B->use_graph) {
+        if (!B->gexec) {
+            // capture the step of this active set once; every later step of
+            // the set replays it (its arguments do not change)
+            HIPCHK(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
+            launch(B->stream);
+            hipError_t ce = hipStreamEndCapture(B->stream, &B->graph);
+            if (ce != hipSuccess) return fail(FDFK_EDEVICE, std::string("batch step capture: ") + hipGetErrorString(ce));
+            HIPCHK(hipGraphInstantiate(&B->gexec, B->graph, nullptr, nullptr, 0));
+        }
+        HIPCHK(hipGraphLaunch(B->gexec, B->stream));
+    } else {
+        launch(B->stream);
     }
     HIPCHK(hipEventRecord(B->ev[slot], B->stream));
     hipError_t e = hipGetLastError();
@@ -333,6 +381,12 @@ int fdfk_batch_create(int32_t norder, fdfk_batch **out)
         }
         hipEventRecord(B->ev[i], B->stream);
     }
+    if (const char *e = std::getenv("FDFK_GRAPH")) B->use_graph = std::atoi(e) != 0;
+    if (hipMalloc((void **)&B->dctr, sizeof(int)) != hipSuccess) {
+        delete B;
+        return fail(FDFK_ENOMEM, "batch counter allocation");
+    }
+    hipLaunchKernelGGL(kb_set, dim3(1), dim3(1), 0, B->stream, B->dctr, 0);
     for (int i = 0; i < fdfk_batch::TS; ++i) {
         if (hipMalloc(&B->dtab[i], B->tabbytes) != hipSuccess ||
             hipHostMalloc(&B->htab[i], B->tabbytes, hipHostMallocDefault) != hipSuccess ||

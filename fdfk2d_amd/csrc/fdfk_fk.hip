@@ -283,7 +283,7 @@ struct DevBuf {
     hipError_t alloc(size_t bytes, hipStream_t stream)
     {
         st = stream;
-        return hipMallocAsync(&p, bytes, stream);
+        return malloc_async_retry(&p, bytes, stream);
     }
     ~DevBuf() { if (p) (void)hipFreeAsync(p, st); }
     template <class T> T *as() { return (T *)p; }
@@ -316,7 +316,11 @@ int plan_take(int nfft, int nkp, int batch, hipfftHandle &h)
             }
     }
     int n[1] = {nfft};
-    if (hipfftPlanMany(&h, 1, n, nullptr, 1, nkp, nullptr, 1, nfft, HIPFFT_Z2D, batch) != HIPFFT_SUCCESS)
+    // a plan allocates its work area: with the device full, free the kept
+    // blocks of destroyed solvers and try once more
+    if (hipfftPlanMany(&h, 1, n, nullptr, 1, nkp, nullptr, 1, nfft, HIPFFT_Z2D, batch) != HIPFFT_SUCCESS &&
+        (!cache_release() ||
+         hipfftPlanMany(&h, 1, n, nullptr, 1, nkp, nullptr, 1, nfft, HIPFFT_Z2D, batch) != HIPFFT_SUCCESS))
         return fail(FDFK_EDEVICE, "hipfftPlanMany failed");
     std::lock_guard<std::mutex> lk(P.mu);
     P.plans.push_back({nfft, nkp, batch, h, true});

@@ -73,6 +73,8 @@ def build_job(req):
     vp, vs, fk = make_model(nx, nz, npml, dx, req["seed"], req.get("perturb", True))
     c = refstate.material_law(vp, vs)
     dt = 0.9 * refstate.stable_dt(3, dx, dx, c)
+    if "dt_max" in req:
+        dt = min(dt, req["dt_max"])
     nrcv = max(2, min(401, nx // 20))
     rx = np.linspace(0.0, (nx - 1) * dx, nrcv).astype(np.float32)
     bazs = geometry.column_bazs(nx, npml, 0.0, dx, configs.LAT_ORG, configs.LON_ORG, configs.AZ_ORG,
@@ -113,6 +115,11 @@ def footprint_bytes(req, block_steps=BG_BLOCK):
     else:
         bg = 2 * ring_pts * (nt + 1) * 4
     return fields + strips + bg, fk_work
+
+
+def _keep(keep_results, rid):
+    """keep_results: True (every request), False, or a set of request ids."""
+    return keep_results is True or (bool(keep_results) and not isinstance(keep_results, bool) and rid in keep_results)
 
 
 def rel_l2(a, b):
@@ -235,8 +242,10 @@ class Server:
                     (sx, sz), t_done = f.result()
                     self.used -= footprint_bytes(req, self.block_steps)[0]
                     lat[req["id"]] = t_done - t0 - req["arrival"]
-                    if keep_results:
+                    if _keep(keep_results, req["id"]):
                         results[req["id"]] = (sx, sz)
+                    if len(lat) % 25 == 0:
+                        self.log("served %d requests, %.1f s" % (len(lat), time.perf_counter() - t0))
                 for rid, (req, s) in list(self.solo.items()):
                     left = s.nt + 1 - s.iteration
                     if left > 0:
@@ -266,7 +275,7 @@ class Server:
         self.batch.close()
 
 
-def serve_plain(reqs, keep_results=False, stats=None, block_steps=BG_BLOCK):
+def serve_plain(reqs, keep_results=False, stats=None, block_steps=BG_BLOCK, log=None):
     """One request at a time with the single-job API, FIFO by arrival, the
     background synthesised in blocks as the server does.  stats, if a dict,
     accumulates the seconds of each phase."""
@@ -295,8 +304,10 @@ def serve_plain(reqs, keep_results=False, stats=None, block_steps=BG_BLOCK):
         for k, v in zip(("model", "create", "background", "run"), (tb - ta, tc - tb, td - tc, te - td)):
             st[k] += v
         lat[req["id"]] = time.perf_counter() - t0 - req["arrival"]
-        if keep_results:
+        if _keep(keep_results, req["id"]):
             results[req["id"]] = (sx, sz)
+        if log and len(lat) % 25 == 0:
+            log("plain path: %d requests, %.1f s" % (len(lat), time.perf_counter() - t0))
     return lat, time.perf_counter() - t0, results
 
 

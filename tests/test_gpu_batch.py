@@ -58,6 +58,41 @@ def test_batch_equals_alone_with_staggered_joins(nstp, block):
     b.close()
 
 
+@pytest.mark.parametrize("block", [0, 32])
+def test_graph_replayed_steps_equal_eager_launches(block, monkeypatch):
+    """A batched step is captured as a graph once per active set and
+    replayed (FDFK_GRAPH=1, the default); FDFK_GRAPH=0 launches its kernels
+    one by one.  Both give the same seismograms and final wavefields bit for
+    bit, with table rebuilds at staggered joins and finishes and (block 32)
+    background refills launched eagerly between the replays."""
+    print("case: 3 jobs batched, graph replay vs eager launches, block %d" % block, flush=True)
+
+    def batched(graph):
+        monkeypatch.setenv("FDFK_GRAPH", "1" if graph else "0")
+        b = Batch(3)
+        sol = [_solver(r, 1, block) for r in _reqs()]
+        b.add(sol[0])
+        b.run(41)
+        b.add(sol[1])
+        b.run(33)
+        b.add(sol[2])
+        while b.active:
+            b.run(19)
+        b.sync()
+        out = [s.seismograms() + s.get_state() for s in sol]
+        for s in sol:
+            s.close()
+        b.close()
+        return out
+
+    eager = batched(False)
+    graph = batched(True)
+    for g, e in zip(graph, eager):
+        assert np.abs(e[1]).max() > 0
+        for a, b_ in zip(g, e):
+            assert np.array_equal(a, b_)
+
+
 def test_server_results_equal_plain_path():
     print("case: serving 6 requests vs plain serial path", flush=True)
     reqs = serve.request_mix(6, seed=7, rate=20.0, nx_range=(120, 400), nz_range=(50, 120),
