@@ -68,9 +68,17 @@ static BlockCache &block_cache()
     static BlockCache *c = new BlockCache();   // outlives static teardown
     return *c;
 }
-// frees every block kept for the current device; false when it held none
+}  // namespace fdfk
+namespace {
+bool plan_pool_release_idle();   // fdfk_fk.hip: idle inverse-FFT plans and their work areas
+}
+namespace fdfk {
+
+// frees every block kept for the current device and the idle FFT plans;
+// false when there was nothing to free
 static bool cache_release()
 {
+    const bool plans = plan_pool_release_idle();
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return false;
     std::multimap<size_t, void *> blocks;
@@ -80,7 +88,7 @@ static bool cache_release()
         blocks.swap(c.free[dev]);
     }
     for (auto &b : blocks) (void)hipFree(b.second);
-    return !blocks.empty();
+    return plans || !blocks.empty();
 }
 
 // stream-ordered allocation that, when the device is out of memory, frees
@@ -757,6 +765,7 @@ int fdfk_trim_memory(void)
         blocks.swap(c.free[dev]);
     }
     for (auto &b : blocks) HIPCHK(hipFree(b.second));
+    plan_pool_release_idle();
     return FDFK_OK;
 }
 

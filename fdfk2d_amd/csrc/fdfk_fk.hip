@@ -326,12 +326,28 @@ int plan_take(int nfft, int nkp, int batch, hipfftHandle &h)
     P.plans.push_back({nfft, nkp, batch, h, true});
     return 0;
 }
+// Idle plans kept for reuse; a plan holds a work area of the order of its
+// data, and plans of many different shapes (batches of many grid depths)
+// would otherwise accumulate.  Returned plans beyond this count are
+// destroyed, oldest first (a plan is idle only after its stream finished).
+constexpr int kPlanKeep = 8;
 void plan_give(hipfftHandle h)
 {
     PlanPool &P = plan_pool();
     std::lock_guard<std::mutex> lk(P.mu);
     for (auto &e : P.plans)
         if (e.h == h) e.busy = false;
+    int idle = 0;
+    for (auto &e : P.plans) idle += !e.busy;
+    for (auto it = P.plans.begin(); idle > kPlanKeep && it != P.plans.end();) {
+        if (!it->busy) {
+            (void)hipfftDestroy(it->h);
+            it = P.plans.erase(it);
+            --idle;
+        } else {
+            ++it;
+        }
+    }
 }
 
 int next_pow2(long n)
@@ -381,6 +397,24 @@ int fk_traces_dev(fdfk_solver *s, FKModel &m, const FKRun &r, const double *zq, 
                        2L * nzq);
     HIPCHK(hipStreamSynchronize(s->stream));
     return 0;
+}
+
+// destroys every idle plan; true when one was
+bool plan_pool_release_idle()
+{
+    PlanPool &P = plan_pool();
+    std::lock_guard<std::mutex> lk(P.mu);
+    bool any = false;
+    for (auto it = P.plans.begin(); it != P.plans.end();) {
+        if (!it->busy) {
+            (void)hipfftDestroy(it->h);
+            it = P.plans.erase(it);
+            any = true;
+        } else {
+            ++it;
+        }
+    }
+    return any;
 }
 
 }  // namespace

@@ -112,7 +112,9 @@ const char *fdfk_last_error(void);
 /* fdfk_destroy keeps a solver's device blocks for later solvers instead of
  * freeing them (a free waits for the whole device, which would stall other
  * solvers and batches running meanwhile); this frees the kept blocks of the
- * current device.  No counterpart in the reference. */
+ * current device and the idle inverse-FFT plans of the FK background (up to
+ * 8 are kept for reuse).  An allocation that finds the device full does the
+ * same and tries once more.  No counterpart in the reference. */
 int fdfk_trim_memory(void);
 /* Tile-width autotuner: builds a scratch solver of the job's geometry (zero
  * background, same kernels), times `steps` steps at interior tile widths

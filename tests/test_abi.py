@@ -43,3 +43,10 @@ def test_fortran_binding_names_exported_and_declared():
     L = lib.lib()
     declared = declared_functions()
     assert not [n for n in names if n not in declared or not hasattr(L, n)]
+
+
+def test_library_resolves_every_symbol_at_load():
+    """Loading with RTLD_NOW binds every undefined symbol at once: a
+    declared-but-undefined internal function fails here, not on first use."""
+    import ctypes
+    ctypes.CDLL(lib.LIB_PATH, mode=os.RTLD_NOW | os.RTLD_LOCAL)
