@@ -129,11 +129,33 @@ static hipError_t block_get(size_t bytes, void **p, size_t *got)
     }
     return e;
 }
+// The kept blocks of a device are bounded by an eighth of its memory: a
+// block that would exceed it (mixes of many sizes reuse few blocks) is
+// freed, the largest kept blocks first.  The free waits for the device, as
+// any free does; it happens only when the bound is reached.
 static void block_put(int dev, void *p, size_t bytes)
 {
-    BlockCache &c = block_cache();
-    std::lock_guard<std::mutex> lk(c.mu);
-    c.free[dev].emplace(bytes, p);
+    static size_t cap = 0;
+    if (cap == 0) {
+        size_t fr = 0, total = 0;
+        cap = (hipMemGetInfo(&fr, &total) == hipSuccess && total > 0) ? total / 8 : (size_t(32) << 30);
+    }
+    std::vector<void *> drop;
+    {
+        BlockCache &c = block_cache();
+        std::lock_guard<std::mutex> lk(c.mu);
+        auto &m = c.free[dev];
+        m.emplace(bytes, p);
+        size_t held = 0;
+        for (auto &b : m) held += b.first;
+        while (held > cap && !m.empty()) {
+            auto last = std::prev(m.end());
+            held -= last->first;
+            drop.push_back(last->second);
+            m.erase(last);
+        }
+    }
+    for (void *q : drop) (void)hipFree(q);
 }
 
 // FD coefficients: closed forms of src/Submain.f90:1111-1198 (fd_coef0:
@@ -207,6 +229,7 @@ struct fdfk_solver {
     // interior tiles, value = half * scale (powers of two)
     bool mat16 = false;
     unsigned short *lam16 = nullptr, *mu16 = nullptr;
+    float *lamp = nullptr, *mup = nullptr;   // per-row profiles the residuals are taken against
     float sl = 1.0f, sm = 1.0f;
     // profiling
     bool prof = false;
@@ -394,7 +417,7 @@ IntArgs<NO> interior_args(fdfk_solver *s)
     a.uxp = s->ux[1 - s->cur]; a.uzp = s->uz[1 - s->cur];
     a.bf = s->bf; a.lam = s->lam; a.mu = s->mu;
     a.ringx = s->ringx; a.ringz = s->ringz;
-    a.lam16 = s->lam16; a.mu16 = s->mu16; a.sl = s->sl; a.sm = s->sm;
+    a.lam16 = s->lam16; a.mu16 = s->mu16; a.lamp = s->lamp; a.mup = s->mup; a.sl = s->sl; a.sm = s->sm;
     a.corr = s->bg_valid && s->it <= s->nt;
     a.lx = s->lx;
     int_tiling<NO, NT>(g, a.lx, a.nbx, a.nbz, a.nsb, a.zseg);
@@ -847,11 +870,28 @@ int fdfk_create(const fdfk_job *job, fdfk_solver **out)
                        (float)(g.dt * g.dt), s->bf, s->lam, s->mu);
     hipFreeAsync(dc, s->stream);
     if (s->mat16) {
-        // per-field power-of-two scales: |value| / scale < 2^15, inside FP16's range
+        // Residuals against per-row profiles: the profile of a row is the
+        // midrange of its values over the undamped columns 1-nbd..nx+nbd
+        // (rows below nz+nbd take row nz+nbd's, as their damped cells copy
+        // it); one power-of-two scale per field keeps |residual| / scale
+        // < 2^15 over the undamped cells (the damped ones are copies).
+        const int xa = 1 - g.nbd, xb = g.nx + g.nbd, zb = g.nz + g.nbd;
+        auto C = [&](int ix, int iz, int k) { return job->c[3 * ((size_t)(ix + g.npml - 1) * g.NZ + (iz - 1)) + k]; };
+        std::vector<float> pl(g.NZ), pm(g.NZ);
         double ml = 0.0, mm = 0.0;
-        for (long i = 0; i < ncell; ++i) {
-            ml = std::max(ml, (double)std::fabs(job->c[3 * i + 1]));
-            mm = std::max(mm, (double)std::fabs(job->c[3 * i + 2]));
+        for (int iz = 1; iz <= g.NZ; ++iz) {
+            if (iz > zb) { pl[iz - 1] = pl[zb - 1]; pm[iz - 1] = pm[zb - 1]; continue; }
+            float l0 = C(xa, iz, 1), l1 = l0, m0 = C(xa, iz, 2), m1 = m0;
+            for (int ix = xa; ix <= xb; ++ix) {
+                l0 = std::min(l0, C(ix, iz, 1)); l1 = std::max(l1, C(ix, iz, 1));
+                m0 = std::min(m0, C(ix, iz, 2)); m1 = std::max(m1, C(ix, iz, 2));
+            }
+            pl[iz - 1] = 0.5f * (l0 + l1);
+            pm[iz - 1] = 0.5f * (m0 + m1);
+            for (int ix = xa; ix <= xb; ++ix) {
+                ml = std::max(ml, (double)std::fabs(C(ix, iz, 1) - pl[iz - 1]));
+                mm = std::max(mm, (double)std::fabs(C(ix, iz, 2) - pm[iz - 1]));
+            }
         }
         auto pow2 = [](double m) {
             int e = 0;
@@ -861,9 +901,14 @@ int fdfk_create(const fdfk_job *job, fdfk_solver **out)
         };
         s->sl = pow2(ml);
         s->sm = pow2(mm);
-        if ((rc = s->dalloc(&s->lam16, nf)) || (rc = s->dalloc(&s->mu16, nf))) return bad(rc);
+        if ((rc = s->dalloc(&s->lam16, nf)) || (rc = s->dalloc(&s->mu16, nf)) || (rc = s->dalloc(&s->lamp, g.NZ)) ||
+            (rc = s->dalloc(&s->mup, g.NZ)))
+            return bad(rc);
+        if (h2d(s->stream, s->lamp, pl.data(), g.NZ * 4) != hipSuccess ||
+            h2d(s->stream, s->mup, pm.data(), g.NZ * 4) != hipSuccess)
+            return bad(fail(FDFK_EDEVICE, "material profile upload"));
         hipLaunchKernelGGL(k_round_material, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, s->stream, g,
-                           s->lam, s->mu, s->lam16, s->mu16, s->sl, s->sm);
+                           s->lam, s->mu, s->lam16, s->mu16, s->lamp, s->mup, s->sl, s->sm);
     }
     // PML strips
     s->strip_cells[S_L] = s->strip_cells[S_R] = (long)g.npml * g.nz;

@@ -144,12 +144,22 @@ struct fdfk_batch {
     bool use_graph = true;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
+    // Graphs of earlier active sets whose replays may still be queued: a
+    // graph is destroyed only once the step of its last replay is known to
+    // be complete (the pacing wait below covers steps up to tot - R)
+    struct Retired { hipGraphExec_t exec; hipGraph_t graph; long last; };
+    std::vector<Retired> retired;
+    long tot = 0;      // batched steps queued so far
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_int, ev_step;
     ~fdfk_batch()
     {
         if (stream) hipStreamSynchronize(stream);
         if (gexec) hipGraphExecDestroy(gexec);
         if (graph) hipGraphDestroy(graph);
+        for (auto &r : retired) {
+            hipGraphExecDestroy(r.exec);
+            hipGraphDestroy(r.graph);
+        }
         if (dctr) hipFree(dctr);
         for (int i = 0; i < R; ++i)
             if (ev[i]) hipEventDestroy(ev[i]);
@@ -284,9 +294,11 @@ int build_tables(fdfk_batch *B, const std::vector<fdfk_solver *> &act)
                           B->stream));
     hipLaunchKernelGGL(kb_set, dim3(1), dim3(1), 0, B->stream, B->dctr, 0);
     HIPCHK(hipGetLastError());
-    // the captured step refers to the old tables: capture again
-    if (B->gexec) { HIPCHK(hipGraphExecDestroy(B->gexec)); B->gexec = nullptr; }
-    if (B->graph) { HIPCHK(hipGraphDestroy(B->graph)); B->graph = nullptr; }
+    // the captured step refers to the old tables: capture again; the old
+    // graph may still have replays queued, so it is retired, not destroyed
+    if (B->gexec) B->retired.push_back({B->gexec, B->graph, B->tot - 1});
+    B->gexec = nullptr;
+    B->graph = nullptr;
     B->tact = act;
     B->tn = n;
     B->dstep = 0;
@@ -306,7 +318,17 @@ int batch_step(fdfk_batch *B)
     if (act != B->tact)
         if (int rc = build_tables<NO>(B, act)) return rc;
     const int slot = B->slot;
-    HIPCHK(hipEventSynchronize(B->ev[slot]));   // at most R steps ahead of the GPU
+    HIPCHK(hipEventSynchronize(B->ev[slot]));   // at most R steps ahead of the GPU: steps < tot - R + 1 are done
+    for (size_t i = 0; i < B->retired.size();) {
+        const auto r = B->retired[i];
+        if (r.last <= B->tot - fdfk_batch::R) {
+            HIPCHK(hipGraphExecDestroy(r.exec));
+            HIPCHK(hipGraphDestroy(r.graph));
+            B->retired.erase(B->retired.begin() + i);
+        } else {
+            ++i;
+        }
+    }
     for (fdfk_solver *s : act)                  // members whose background is synthesised in blocks
         if (int rc = bg_refill(s, B->stream)) return rc;
     auto *dP = (const BatchPrefix *)B->dtab[B->tslot];
@@ -352,6 +374,7 @@ int batch_step(fdfk_batch *B)
     if (e != hipSuccess) return fail(FDFK_EDEVICE, std::string("batch launch: ") + hipGetErrorString(e));
     B->slot = (slot + 1) % fdfk_batch::R;
     B->dstep += 1;
+    B->tot += 1;
     // host copies of the members' counters advance as the device tables assume
     for (fdfk_solver *s : act) {
         if ((s->isis == s->it) && (s->istp <= s->nsis) && s->nrcv > 0) { s->istp += 1; s->isis += s->nstp; }

@@ -294,7 +294,7 @@ struct DevBuf {
 // Plan set-up uploads device tables; reusing plans keeps it out of the path
 // of every background computation after the first of a shape.
 struct PlanPool {
-    struct Entry { int nfft, nkp, batch; hipfftHandle h; bool busy; };
+    struct Entry { int nfft, nkp, batch; hipfftHandle h; size_t work; bool busy; };
     std::mutex mu;
     std::vector<Entry> plans;
 };
@@ -303,7 +303,22 @@ PlanPool &plan_pool()
     static PlanPool *pool = new PlanPool();   // never destroyed: plans outlive static teardown
     return *pool;
 }
-int plan_take(int nfft, int nkp, int batch, hipfftHandle &h)
+// A plan of the given shape, made without its own work area: the caller
+// supplies `work` bytes of device memory per execution (stream-ordered), so
+// idle plans hold only their small tables and a device-wide free never
+// follows from dropping one.
+static hipfftResult make_plan(int nfft, int nkp, int batch, hipfftHandle &h, size_t &work)
+{
+    hipfftResult r = hipfftCreate(&h);
+    if (r != HIPFFT_SUCCESS) return r;
+    int n[1] = {nfft};
+    if ((r = hipfftSetAutoAllocation(h, 0)) != HIPFFT_SUCCESS ||
+        (r = hipfftMakePlanMany(h, 1, n, nullptr, 1, nkp, nullptr, 1, nfft, HIPFFT_Z2D, batch, &work)) !=
+            HIPFFT_SUCCESS)
+        (void)hipfftDestroy(h);
+    return r;
+}
+int plan_take(int nfft, int nkp, int batch, hipfftHandle &h, size_t &work)
 {
     PlanPool &P = plan_pool();
     {
@@ -312,25 +327,27 @@ int plan_take(int nfft, int nkp, int batch, hipfftHandle &h)
             if (!e.busy && e.nfft == nfft && e.nkp == nkp && e.batch == batch) {
                 e.busy = true;
                 h = e.h;
+                work = e.work;
                 return 0;
             }
     }
-    int n[1] = {nfft};
-    // a plan allocates its work area: with the device full, free the kept
-    // blocks of destroyed solvers and try once more
-    if (hipfftPlanMany(&h, 1, n, nullptr, 1, nkp, nullptr, 1, nfft, HIPFFT_Z2D, batch) != HIPFFT_SUCCESS &&
-        (!cache_release() ||
-         hipfftPlanMany(&h, 1, n, nullptr, 1, nkp, nullptr, 1, nfft, HIPFFT_Z2D, batch) != HIPFFT_SUCCESS))
-        return fail(FDFK_EDEVICE, "hipfftPlanMany failed");
+    // with the device full, free the kept blocks of destroyed solvers and the
+    // idle plans, clear the failed call's error state and try once more
+    if (make_plan(nfft, nkp, batch, h, work) != HIPFFT_SUCCESS) {
+        (void)hipGetLastError();
+        if (!cache_release() || make_plan(nfft, nkp, batch, h, work) != HIPFFT_SUCCESS) {
+            (void)hipGetLastError();
+            return fail(FDFK_EDEVICE, "hipfftMakePlanMany failed");
+        }
+    }
     std::lock_guard<std::mutex> lk(P.mu);
-    P.plans.push_back({nfft, nkp, batch, h, true});
+    P.plans.push_back({nfft, nkp, batch, h, work, true});
     return 0;
 }
-// Idle plans kept for reuse; a plan holds a work area of the order of its
-// data, and plans of many different shapes (batches of many grid depths)
-// would otherwise accumulate.  Returned plans beyond this count are
+// Idle plans kept for reuse (plans of many shapes, batches of many grid
+// depths, would otherwise accumulate); returned plans beyond this count are
 // destroyed, oldest first (a plan is idle only after its stream finished).
-constexpr int kPlanKeep = 8;
+constexpr int kPlanKeep = 16;
 void plan_give(hipfftHandle h)
 {
     PlanPool &P = plan_pool();
@@ -386,8 +403,18 @@ int fk_traces_dev(fdfk_solver *s, FKModel &m, const FKRun &r, const double *zq, 
     hipLaunchKernelGGL(k_fk_spec, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, s->stream, dm.as<FKModel>(), r,
                        amps.as<zc>(), dlay.as<int>(), ddz.as<double>(), spec.as<zc>());
     hipfftHandle plan;
-    if (int rc = plan_take(r.nfft, (int)nkp, 2 * nzq, plan)) return rc;
+    size_t wbytes = 0;
+    if (int rc = plan_take(r.nfft, (int)nkp, 2 * nzq, plan, wbytes)) return rc;
+    DevBuf wa;
+    if (wbytes > 0) {
+        hipError_t e = wa.alloc(wbytes, s->stream);
+        if (e != hipSuccess) {
+            plan_give(plan);
+            return fail(FDFK_ENOMEM, std::string("FFT work area: ") + hipGetErrorString(e));
+        }
+    }
     hipfftSetStream(plan, s->stream);
+    hipfftSetWorkArea(plan, wa.p);
     hipfftResult fr = hipfftExecZ2D(plan, (hipfftDoubleComplex *)spec.p, (hipfftDoubleReal *)g_out);
     hipStreamSynchronize(s->stream);
     plan_give(plan);

@@ -179,9 +179,11 @@ struct IntArgs {
     const float *ux, *uz;
     float *uxp, *uzp;
     const float *bf, *lam, *mu;
-    // FP16 copies of lam and mu (16-bit material storage): value = half * scale,
-    // the scales powers of two, so the conversion is exact
+    // FP16 residuals of lam and mu against per-row FP32 profiles (16-bit
+    // material storage): value = profile(row) + half * scale, the scales
+    // powers of two
     const unsigned short *lam16, *mu16;
+    const float *lamp, *mup;
     float sl, sm;
     const float *ringx, *ringz;  // background on the hybrid ring (total-field view of the padding)
     int corr;                    // ring values valid at this step

@@ -388,6 +388,7 @@ struct IntKernel2 {
     // below 4 GiB because a block spans lx + 6*NO columns.
     struct Rsrc {
         __amdgpu_buffer_rsrc_t ux, uz, uxp, uzp, bf, lam, mu, rx, rz, lam16, mu16;
+        float plam, pmu;   // M16: the lane row's material profile
         int xb, pitch;   // first column, column pitch in bytes
         __device__ __forceinline__ int soff(int c) const { return (c - xb) * pitch; }
     };
@@ -407,17 +408,18 @@ struct IntKernel2 {
     }
     // a material value at the lane's row of a column: byte offsets of the
     // 32-bit arrays, halved for the 16-bit copies
+    // (M16: the row's profile value plus the scaled FP16 residual)
     static __device__ __forceinline__ float ld_lam(const IntArgs<NO> &a, const Rsrc &B, unsigned o, int so)
     {
         if constexpr (M16)
-            return (float)__builtin_bit_cast(_Float16, __builtin_amdgcn_raw_buffer_load_b16(B.lam16, o >> 1, so >> 1, 0)) * a.sl;
+            return B.plam + (float)__builtin_bit_cast(_Float16, __builtin_amdgcn_raw_buffer_load_b16(B.lam16, o >> 1, so >> 1, 0)) * a.sl;
         else
             return ld(B.lam, o, so);
     }
     static __device__ __forceinline__ float ld_mu(const IntArgs<NO> &a, const Rsrc &B, unsigned o, int so)
     {
         if constexpr (M16)
-            return (float)__builtin_bit_cast(_Float16, __builtin_amdgcn_raw_buffer_load_b16(B.mu16, o >> 1, so >> 1, 0)) * a.sm;
+            return B.pmu + (float)__builtin_bit_cast(_Float16, __builtin_amdgcn_raw_buffer_load_b16(B.mu16, o >> 1, so >> 1, 0)) * a.sm;
         else
             return ld(B.mu, o, so);
     }
@@ -619,7 +621,10 @@ struct IntKernel2 {
             B.ux = mk(g, a.ux, B.xb); B.uz = mk(g, a.uz, B.xb);
             B.uxp = mk(g, a.uxp, B.xb); B.uzp = mk(g, a.uzp, B.xb);
             B.bf = mk(g, a.bf, B.xb); B.lam = mk(g, a.lam, B.xb); B.mu = mk(g, a.mu, B.xb);
-            if constexpr (M16) { B.lam16 = mk16(g, a.lam16, B.xb); B.mu16 = mk16(g, a.mu16, B.xb); }
+            if constexpr (M16) {
+                B.lam16 = mk16(g, a.lam16, B.xb); B.mu16 = mk16(g, a.mu16, B.xb);
+                B.plam = a.lamp[rc - 1]; B.pmu = a.mup[rc - 1];
+            }
             B.rx = mk(g, a.ringx, B.xb); B.rz = mk(g, a.ringz, B.xb);
         }
         // prologue: the loads the first step consumes
@@ -705,18 +710,24 @@ __global__ void k_import_material(Geo g, const float *c, float dt2, float *bf, f
 // 16-bit material storage: lam and mu rounded to FP16 with power-of-two
 // scales (round to nearest even); the FP32 arrays take the rounded values,
 // so every kernel sees the same materials
+// 16-bit material storage: lam = lamp(row) + h * sl with h the FP16 residual
+// against a per-row FP32 profile (power-of-two scale, nearest even), so the
+// rounding falls on the lateral variation only; the FP32 arrays receive the
+// same rounded values (the product h * sl is exact, the sum rounds once)
 __global__ void k_round_material(Geo g, float *lam, float *mu, unsigned short *lam16, unsigned short *mu16,
-                                 float sl, float sm)
+                                 const float *lamp, const float *mup, float sl, float sm)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     long n = (long)g.NX * g.NZ;
     if (t >= n) return;
-    const size_t d = (size_t)(t / g.NZ) * g.ld + (size_t)(t % g.NZ);
-    const _Float16 hl = (_Float16)(lam[d] * (1.0f / sl)), hm = (_Float16)(mu[d] * (1.0f / sm));   // nearest even
+    const int k = (int)(t % g.NZ);
+    const size_t d = (size_t)(t / g.NZ) * g.ld + (size_t)k;
+    const _Float16 hl = (_Float16)((lam[d] - lamp[k]) * (1.0f / sl));
+    const _Float16 hm = (_Float16)((mu[d] - mup[k]) * (1.0f / sm));
     lam16[d] = __builtin_bit_cast(unsigned short, hl);
     mu16[d] = __builtin_bit_cast(unsigned short, hm);
-    lam[d] = (float)hl * sl;
-    mu[d] = (float)hm * sm;
+    lam[d] = lamp[k] + (float)hl * sl;
+    mu[d] = mup[k] + (float)hm * sm;
 }
 
 // reference (1:2, NZ, NX) <-> two padded arrays (level 1 -> cur, level 2 -> prev)

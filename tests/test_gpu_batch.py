@@ -93,6 +93,40 @@ def test_graph_replayed_steps_equal_eager_launches(block, monkeypatch):
             assert np.array_equal(a, b_)
 
 
+def test_graph_rebuilds_while_replays_are_queued():
+    """Twelve members of different lengths joining every 7 steps and leaving
+    as they finish: the active set changes about 20 times while the host
+    keeps up to four graph replays queued ahead of the GPU, so the graph of
+    an active set is replaced while replays of it are still pending (those
+    graphs are destroyed only after their last replay completed).  Every
+    member equals its run alone, bit for bit."""
+    print("case: 12 members, joins every 7 steps, graph replay", flush=True)
+    reqs = []
+    for i in range(12):
+        reqs.append(dict(id=i, arrival=0.0, nx=61 + 10 * i, nz=40 + 3 * i, dx=200.0, nsteps=40 + 17 * i,
+                         src_is_p=(i % 2 == 0), p_skm=0.05 + 0.002 * i, seed=100 + i, npml=20))
+    alone = []
+    for r in reqs:
+        s = _solver(r)
+        s.run(s.nt + 1)
+        alone.append(s.seismograms())
+        s.close()
+    b = Batch(3)
+    sol = [_solver(r) for r in reqs]
+    for s in sol:
+        b.add(s)
+        b.run(7)
+    while b.active:
+        b.run(5)
+    b.sync()
+    for s, (ax, az) in zip(sol, alone):
+        gx, gz = s.seismograms()
+        assert np.abs(az).max() > 0
+        assert np.array_equal(gx, ax) and np.array_equal(gz, az)
+        s.close()
+    b.close()
+
+
 def test_server_results_equal_plain_path():
     print("case: serving 6 requests vs plain serial path", flush=True)
     reqs = serve.request_mix(6, seed=7, rate=20.0, nx_range=(120, 400), nz_range=(50, 120),

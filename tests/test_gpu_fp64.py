@@ -4,10 +4,11 @@ oracle's algorithm with every FP32 quantity in double precision).
 The seismograms of the FP32 GPU path, run alone and in a batch (the served
 form), must lie within rel-L2 1e-4 of the FP64 run on the same inputs; the
 FP32 oracle itself lies about 1e-6 from it (tests/test_oracle_fp64.py).
-The FP16 material storage is measured against the same run: its error is
-the rounding of lambda and mu to FP16 (about 3e-4), stated and bounded by
-1e-3.  Each run computes its own background: the GPU's FK in FP64 with
-hipFFT, the oracle's in double precision.
+The FP16 material storage (per-row FP32 profiles of lambda and mu plus
+FP16 residuals) is measured against the same run and held to the same
+1e-4: the rounding falls on the lateral variation only.  Each run computes
+its own background: the GPU's FK in FP64 with hipFFT, the oracle's in
+double precision.
 """
 import numpy as np
 import pytest
@@ -83,7 +84,7 @@ def test_fp16_materials_against_fp64(fp64_runs):
         e = _err(s.seismograms(), ref)
         s.close()
         print("case %d: FP16 materials rel-L2 %.2e against FP64" % (k, e), flush=True)
-        assert e < 1e-3
+        assert e < 1e-4
 
 
 @pytest.mark.parametrize("norder", [1, 4, 8])

@@ -337,24 +337,35 @@ def test_autotuned_tile_width_same_results(case, bg):
         assert np.array_equal(a, b)
 
 
-def _round_materials(c):
-    """The library's 16-bit material storage restated: lam and mu to FP16 with
-    per-field power-of-two scales (|value| / scale < 2^15), nearest even."""
+def _round_materials(c, nx, nz, npml, nbd):
+    """The library's 16-bit material storage restated: per row a profile, the
+    midrange (min + max) / 2 in FP32 of the row's values over the undamped
+    columns 1-nbd..nx+nbd (rows below nz+nbd take row nz+nbd's); lam and mu
+    become profile + FP16(residual / scale) * scale, one power-of-two scale
+    per field with |residual| / scale < 2^15 over the undamped cells, nearest
+    even.  (The damped cells are copies of undamped ones on the device.)"""
+    NX, NZ = nx + 2 * npml, nz + npml
     flat = np.array(c, np.float32).ravel().copy()
+    x0, x1, zb = npml - nbd, npml + nx + nbd, nz + nbd      # 0-based column range [x0, x1), rows [0, zb)
     for k in (1, 2):
-        v = flat[k::3]
-        m = float(np.abs(v).max())
+        v = flat[k::3].reshape(NX, NZ)
+        und = v[x0:x1, :zb]
+        prof = np.float32(0.5) * (und.min(axis=0) + und.max(axis=0))
+        prof = np.concatenate([prof, np.full(NZ - zb, prof[-1], np.float32)]).astype(np.float32)
+        m = float(np.abs(und - prof[None, :zb]).max())
         sc = np.float32(np.ldexp(1.0, np.frexp(m)[1] - 15)) if m > 0 else np.float32(1.0)
-        flat[k::3] = np.float16(v * (np.float32(1.0) / sc)).astype(np.float32) * sc
+        h = np.float16((v - prof[None, :]) * (np.float32(1.0) / sc)).astype(np.float32)
+        flat[k::3] = (prof[None, :] + h * sc).astype(np.float32).reshape(-1)
     return flat.reshape(np.shape(c))
 
 
 @pytest.mark.parametrize("norder", [3, 8])
 def test_fp16_materials_equal_fp32_on_rounded_materials(norder):
-    """FDFK_FP16: the interior tiles read lam and mu from FP16 copies; every
-    kernel sees the same rounded values, so the run equals, bit for bit, an
-    FP32 run on materials rounded the same way on the host.  The rounding
-    itself moves the seismograms by about the materials' FP16 resolution."""
+    """FDFK_FP16: the interior tiles read lam and mu as per-row FP32 profiles
+    plus FP16 residuals; every kernel sees the same rounded values, so the
+    run equals, bit for bit, an FP32 run on materials rounded the same way on
+    the host.  The rounding itself moves the seismograms by about the FP16
+    resolution of the lateral variation."""
     print("case: FP16 materials, norder %d, 121 x 60, 301 iterations" % norder, flush=True)
     c = Case(nt=300, norder=norder, npml=15 if norder <= 4 else 20)
     bg = c.oracle_background()
@@ -368,7 +379,7 @@ def test_fp16_materials_equal_fp32_on_rounded_materials(norder):
         s.close()
         return out
     h = run(c.c, lib.FDFK_FP16)
-    r = run(_round_materials(c.c), lib.FDFK_FP32)
+    r = run(_round_materials(c.c, c.nx, c.nz, c.npml, c.nbd), lib.FDFK_FP32)
     for a, b in zip(h, r):
         assert np.array_equal(a, b)
     f = run(c.c, lib.FDFK_FP32)
