@@ -10,6 +10,7 @@
 // single translation unit: the kernels are compiled with the host code
 #include "fdfk_kernels.hip"
 #include "fdfk_frame.hip"
+#include "fdfk_interior2.hip"
 
 #include <algorithm>
 #include <cmath>
@@ -23,6 +24,7 @@
 #include <mutex>
 #include <string>
 #include <tuple>
+#include <utility>
 #include <vector>
 
 namespace fdfk {
@@ -225,12 +227,22 @@ struct fdfk_solver {
     int lx = 45, nt_threads = 256;
     bool fused = true;   // interior and frame in one launch (k_step)
     int lx_model = 0;    // interior tile width of the occupancy model (lx may be tuned)
-    // 16-bit material storage (FDFK_FP16): FP16 copies of lam, mu for the
-    // interior tiles, value = half * scale (powers of two)
+    // 16-bit material storage (FDFK_FP16): lam, mu of the interior tiles as
+    // per-row FP32 profiles plus FP16 residuals, value = profile + half *
+    // scale (power-of-two scales)
     bool mat16 = false;
     unsigned short *lam16 = nullptr, *mu16 = nullptr;
     float *lamp = nullptr, *mup = nullptr;   // per-row profiles the residuals are taken against
     float sl = 1.0f, sm = 1.0f;
+    // Two-step pairs (FDFK_STEP2=1): the deep rectangle of the interior is
+    // stepped twice per launch.  The pair writes out of place, into two
+    // spare levels per component that rotate with the two live ones.
+    bool step2 = false;
+    float *uxs[2] = {nullptr, nullptr}, *uzs[2] = {nullptr, nullptr};
+    Geo gdeep{};
+    int d_lx = 0, d_nbx = 0, d_nbz = 0, d_tz = 0;
+    Geo gband[4]{};
+    int band_lx[4] = {0, 0, 0, 0};
     // profiling
     bool prof = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_int, ev_step;
@@ -333,6 +345,7 @@ StepArgs step_args(fdfk_solver *s)
     a.uz = s->uz[s->cur];
     a.uxp = s->ux[1 - s->cur];
     a.uzp = s->uz[1 - s->cur];
+    a.uxo = a.uxp; a.uzo = a.uzp;
     a.bf = s->bf; a.lam = s->lam; a.mu = s->mu;
     a.st = s->d_strips[s->cur];
     a.ringx = s->ringx; a.ringz = s->ringz;
@@ -380,14 +393,16 @@ static_assert(kMaxOrder == 8, "with_order covers norder 1..8");
 // Interior tiling for a column width lx: nbx columns of tiles, nbz bands of
 // full-height tiles (TZ rows); a last band of at most 64 - 2HZ rows goes to
 // one-wave segments, NT/64 of them per block (nsb blocks from row zseg on).
+// A rectangle of fewer rows than that goes to segments entirely when
+// thin_segments is set (the thin bands of a two-step pair).
 template <int NO, int NT>
-void int_tiling(const Geo &g, int lx, int &nbx, int &nbz, int &nsb, int &zseg)
+void int_tiling(const Geo &g, int lx, int &nbx, int &nbz, int &nsb, int &zseg, bool thin_segments = false)
 {
     constexpr int TZ = IntKernel2<NO, NT>::TZ, HZ = IntKernel2<NO, NT>::HZ, W = NT / 64;
     const int ncols = g.ix1 - g.ix0 + 1, nrows = g.iz1 - g.iz0 + 1;
     nbx = (ncols + lx - 1) / lx;
     const int full = nrows / TZ, rem = nrows - full * TZ;
-    if (W > 1 && full > 0 && rem > 0 && rem <= 64 - 2 * HZ) {
+    if (W > 1 && (full > 0 || thin_segments) && rem > 0 && rem <= 64 - 2 * HZ) {
         nbz = full;
         zseg = g.iz0 + full * TZ;
         nsb = (nbx + W - 1) / W;
@@ -415,6 +430,7 @@ IntArgs<NO> interior_args(fdfk_solver *s)
     a.k = make_icoef<NO>(s);
     a.ux = s->ux[s->cur]; a.uz = s->uz[s->cur];
     a.uxp = s->ux[1 - s->cur]; a.uzp = s->uz[1 - s->cur];
+    a.uxo = a.uxp; a.uzo = a.uzp;
     a.bf = s->bf; a.lam = s->lam; a.mu = s->mu;
     a.ringx = s->ringx; a.ringz = s->ringz;
     a.lam16 = s->lam16; a.mu16 = s->mu16; a.lamp = s->lamp; a.mup = s->mup; a.sl = s->sl; a.sm = s->sm;
@@ -614,6 +630,122 @@ void launch_frame(fdfk_solver *s, const StepArgs &a)
     });
 }
 
+// ---------------------------------------------------------------------
+// Two-step pairs over the deep interior
+// ---------------------------------------------------------------------
+template <int NO>
+size_t step2_lds_bytes(const Geo &g, bool deep)
+{
+    size_t b = std::max(sizeof(typename IntKernel2<NO, 256>::Smem), frame_lds_bytes(frame_tiles(g), g.norder));
+    if constexpr (NO <= 4)
+        if (deep) b = std::max(b, sizeof(typename IntKernel22<NO>::Smem));
+    return b;
+}
+
+// The deep rectangle: its tiles compute level n+1 on a halo of HZ cells
+// around it for their second step, and that halo must consist of regular
+// interior cells stepped without the padding's background correction.  So
+// it starts HZ rows below the interior's first row (the rows above the
+// interior read only physical rows), and ends 2HZ cells inside the
+// interior's sides and bottom (the halo's stencils then read no padding).
+// Returns false for orders above 4 (the two-engine march holds ring periods
+// up to 14) or when no deep rectangle worth a launch remains.
+template <int NO>
+bool plan_step2_t(fdfk_solver *s)
+{
+    if constexpr (NO > 4) {
+        (void)s;
+        return false;
+    } else {
+        using K = IntKernel22<NO>;
+        constexpr int HZ = K::HZ;
+        const Geo &g = s->g;
+        Geo d = g;
+        d.ix0 = g.ix0 + 2 * HZ; d.ix1 = g.ix1 - 2 * HZ;
+        d.iz0 = g.iz0 + HZ;     d.iz1 = g.iz1 - 2 * HZ;
+        const int ncols = d.ix1 - d.ix0 + 1, nrows = d.iz1 - d.iz0 + 1;
+        if (ncols < 4 * HZ || nrows < 2 * HZ) return false;
+        int dev = 0, ncu = 0, bpc = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_step2<NO, true>, 256, step2_lds_bytes<NO>(g, true)) !=
+                hipSuccess ||
+            ncu <= 0 || bpc <= 0)
+            return false;
+        s->d_nbz = (nrows + K::TZMAX - 1) / K::TZMAX;
+        s->d_tz = (nrows + s->d_nbz - 1) / s->d_nbz;
+        // column width: fewest (resident rounds) x (march steps of a tile)
+        const long slots = (long)ncu * bpc;
+        long best = -1;
+        for (int lx = 8; lx <= ncols; ++lx) {
+            const long nb = (long)s->d_nbz * ((ncols + lx - 1) / lx);
+            const long cost = (nb + slots - 1) / slots * (lx + K::WARM + K::LAG2);
+            if (best < 0 || cost < best) { best = cost; s->d_lx = lx; }
+        }
+        if (const char *e = std::getenv("FDFK_LX2")) s->d_lx = std::max(8, std::atoi(e));
+        s->d_nbx = (ncols + s->d_lx - 1) / s->d_lx;
+        s->gdeep = d;
+        // bands: above and below it (whole width), left and right of it
+        Geo b[4] = {g, g, g, g};
+        b[0].iz1 = d.iz0 - 1;
+        b[1].iz0 = d.iz1 + 1;
+        b[2].iz0 = b[3].iz0 = d.iz0;
+        b[2].iz1 = b[3].iz1 = d.iz1;
+        b[2].ix1 = d.ix0 - 1;
+        b[3].ix0 = d.ix1 + 1;
+        for (int k = 0; k < 4; ++k) {
+            s->gband[k] = b[k];
+            s->band_lx[k] = k < 2 ? s->lx : b[k].ix1 - b[k].ix0 + 1;
+        }
+        return true;
+    }
+}
+
+template <int NO>
+void launch_step2_t(fdfk_solver *s, const StepArgs &a, bool first)
+{
+    if constexpr (NO > 4) {
+        (void)s; (void)a; (void)first;
+    } else {
+        static_assert(sizeof(DeepArgs<NO>) + sizeof(Bands<NO>) + sizeof(StepArgs) + sizeof(ETiles) + 16 <= 4096,
+                      "the pair launch's arguments exceed the kernel argument limit");
+        ETiles t = frame_tiles(s->g);
+        Bands<NO> bs{};
+        for (int k = 0; k < 4; ++k) {
+            IntArgs<NO> b = interior_args<NO, 256>(s);
+            b.g = s->gband[k];
+            b.lx = s->band_lx[k];
+            int_tiling<NO, 256>(b.g, b.lx, b.nbx, b.nbz, b.nsb, b.zseg, true);
+            b.uxo = a.uxo; b.uzo = a.uzo;
+            bs.b[k] = b;
+            bs.off[k + 1] = bs.off[k] + b.nbx * b.nbz + b.nsb;
+        }
+        DeepArgs<NO> da{};
+        int nd = 0;
+        if (first) {
+            da.g = s->gdeep;
+            da.k = make_icoef<NO>(s);
+            da.ux = s->ux[s->cur]; da.uz = s->uz[s->cur];
+            da.uxp = s->ux[1 - s->cur]; da.uzp = s->uz[1 - s->cur];
+            da.o1x = s->uxs[0]; da.o1z = s->uzs[0];
+            da.o2x = s->uxs[1]; da.o2z = s->uzs[1];
+            da.bf = s->bf; da.lam = s->lam; da.mu = s->mu;
+            da.lx = s->d_lx; da.nbx = s->d_nbx; da.nbz = s->d_nbz; da.tz = s->d_tz;
+            nd = da.nbx * da.nbz;
+            hipLaunchKernelGGL((k_step2<NO, true>), dim3((unsigned)(nd + bs.off[4] + t.off[t.n])), dim3(256),
+                               step2_lds_bytes<NO>(s->g, true), s->stream, da, bs, a, t, nd);
+        } else {
+            hipLaunchKernelGGL((k_step2<NO, false>), dim3((unsigned)(bs.off[4] + t.off[t.n])), dim3(256),
+                               step2_lds_bytes<NO>(s->g, false), s->stream, da, bs, a, t, nd);
+        }
+    }
+}
+
+void launch_step2(fdfk_solver *s, const StepArgs &a, bool first)
+{
+    with_order(s->g.norder, [&](auto no) { launch_step2_t<decltype(no)::value>(s, a, first); });
+}
+
 // Blocked background: entering block j >= 1 (iteration jB), synthesise
 // block j+1 into the half of the ring that held block j-1, whose last step
 // is already queued before this point of the stream.
@@ -625,7 +757,12 @@ int bg_refill(fdfk_solver *s, hipStream_t stream)
     return t0 < t1 ? s->bg_fill(stream, t0, t1) : 0;
 }
 
-int one_step(fdfk_solver *s, bool record)
+// One time step.  half = 0: in place (level n+1 over level n-1).
+// half = 1, 2: the first and second step of a two-step pair, written into
+// the spare level uxs[half-1]; the spare and the replaced level n-1 trade
+// places afterwards.  The first launch of a pair also writes level n+2 of
+// the deep rectangle into uxs[1], which the second launch completes.
+int one_step(fdfk_solver *s, bool record, int half = 0)
 {
     const Geo &g = s->g;
     if (int rc = bg_refill(s, s->stream)) return rc;
@@ -642,6 +779,7 @@ int one_step(fdfk_solver *s, bool record)
         r.seisx = s->seisx; r.seisz = s->seisz;
     }
     StepArgs a = step_args(s);
+    if (half) { a.uxo = s->uxs[half - 1]; a.uzo = s->uzs[half - 1]; }
     {
         const int nring = a.ring_valid ? (int)((ring_count(g) + 255) / 256) : 0;
         const int npsi = (int)((2L * g.npml * g.nz + (long)g.NX * g.npml + 255) / 256);
@@ -652,7 +790,10 @@ int one_step(fdfk_solver *s, bool record)
         });
     }
     if (s->prof) hipEventRecord(e1, s->stream);
-    if (s->fused) {
+    if (half) {
+        launch_step2(s, a, half == 1);
+        if (s->prof) hipEventRecord(e2, s->stream);
+    } else if (s->fused) {
         launch_step(s, a);
         if (s->prof) hipEventRecord(e2, s->stream);
     } else {
@@ -667,6 +808,10 @@ int one_step(fdfk_solver *s, bool record)
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(FDFK_EDEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
+    if (half) {
+        std::swap(s->ux[1 - s->cur], s->uxs[half - 1]);
+        std::swap(s->uz[1 - s->cur], s->uzs[half - 1]);
+    }
     s->cur = 1 - s->cur;
     s->it += 1;
     return 0;
@@ -855,6 +1000,19 @@ int fdfk_create(const fdfk_job *job, fdfk_solver **out)
     if (hipMemsetAsync(s->bf, 0, nf * 4, s->stream) || hipMemsetAsync(s->lam, 0, nf * 4, s->stream) ||
         hipMemsetAsync(s->mu, 0, nf * 4, s->stream))
         return bad(fail(FDFK_EDEVICE, "memset"));
+    if (const char *e = std::getenv("FDFK_STEP2")) {
+        if (std::atoi(e) != 0 && s->fused && !s->mat16 && s->nt_threads == 256)
+            with_order(g.norder, [&](auto no) { s->step2 = plan_step2_t<decltype(no)::value>(s); });
+        if (s->step2) {
+            // spares start at zero like the live levels: cells no step
+            // writes (the outermost padding line) stay zero in every level
+            for (int q = 0; q < 2; ++q) {
+                if ((rc = s->dalloc(&s->uxs[q], nf)) || (rc = s->dalloc(&s->uzs[q], nf))) return bad(rc);
+                if (hipMemsetAsync(s->uxs[q], 0, nf * 4, s->stream) || hipMemsetAsync(s->uzs[q], 0, nf * 4, s->stream))
+                    return bad(fail(FDFK_EDEVICE, "memset"));
+            }
+        }
+    }
     if (h2d(s->stream, s->dcoef, &s->hcoef, sizeof(Coefs)) != hipSuccess)
         return bad(fail(FDFK_EDEVICE, "coef upload"));
     // materials: reference layout -> dt^2/rho, lambda, mu
@@ -1027,10 +1185,15 @@ int fdfk_run(fdfk_solver *s, int32_t nsteps)
     if (nsteps < 0) return fail(FDFK_EINVAL, "nsteps < 0");
     for (int k = 0; k < nsteps; ++k) {
         if (s->it > s->nt) return fail(FDFK_ERANGE, "all iterations done");
-        bool rec = (s->isis == s->it) && (s->istp <= s->nsis);
-        int rc = one_step(s, rec);
-        if (rc) return rc;
-        if (rec) { s->istp += 1; s->isis += s->nstp; }
+        // a pair never spans two calls: between calls both live levels are whole
+        const bool pair = s->step2 && k + 1 < nsteps && s->it + 1 <= s->nt;
+        for (int h = 0; h < (pair ? 2 : 1); ++h) {
+            bool rec = (s->isis == s->it) && (s->istp <= s->nsis);
+            int rc = one_step(s, rec, pair ? h + 1 : 0);
+            if (rc) return rc;
+            if (rec) { s->istp += 1; s->isis += s->nstp; }
+        }
+        if (pair) ++k;
     }
     return 0;
 }

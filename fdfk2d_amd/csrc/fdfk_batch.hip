@@ -56,7 +56,7 @@ __device__ __forceinline__ int find_job(const int *pre, int n, int bid)
 // The step count since the active set's tables were built lives on the
 // device (ctr[0]): the kernels of a step read it and pick the table of the
 // step's parity themselves, and kb_tick advances it after the step, so the
-// launches of every step are identical and one step can be captured as a
+// launches of every step are identical and one step can be built as a
 // graph once per active set and replayed.
 __global__ void kb_tick(int *ctr) { ctr[0] += 1; }
 __global__ void kb_set(int *ctr, int v) { ctr[0] = v; }
@@ -139,7 +139,7 @@ struct fdfk_batch {
     size_t tsh = 0;
     bool prof = false;
     // device step counter of the current tables, and the step of the
-    // current active set captured as a graph (rebuilt with the tables)
+    // current active set as a graph (rebuilt with the tables)
     int *dctr = nullptr;
     bool use_graph = true;
     hipGraphExec_t gexec = nullptr;
@@ -251,9 +251,11 @@ void set_levels(const fdfk_solver *s, BatchJob<NO> &b, int c)
 {
     b.sa.ux = s->ux[c]; b.sa.uz = s->uz[c];
     b.sa.uxp = s->ux[1 - c]; b.sa.uzp = s->uz[1 - c];
+    b.sa.uxo = b.sa.uxp; b.sa.uzo = b.sa.uzp;
     b.sa.st = s->d_strips[c];
     b.ia.ux = s->ux[c]; b.ia.uz = s->uz[c];
     b.ia.uxp = s->ux[1 - c]; b.ia.uzp = s->uz[1 - c];
+    b.ia.uxo = b.ia.uxp; b.ia.uzo = b.ia.uzp;
     b.ra.ux = s->ux[c]; b.ra.uz = s->uz[c];
 }
 
@@ -294,7 +296,7 @@ int build_tables(fdfk_batch *B, const std::vector<fdfk_solver *> &act)
                           B->stream));
     hipLaunchKernelGGL(kb_set, dim3(1), dim3(1), 0, B->stream, B->dctr, 0);
     HIPCHK(hipGetLastError());
-    // the captured step refers to the old tables: capture again; the old
+    // the graph's step refers to the old tables: build it again; the old
     // graph may still have replays queued, so it is retired, not destroyed
     if (B->gexec) B->retired.push_back({B->gexec, B->graph, B->tot - 1});
     B->gexec = nullptr;
@@ -357,13 +359,43 @@ int batch_step(fdfk_batch *B)
         B->ev_step.emplace_back(e0, e3);
     } else if (B->use_graph) {
         if (!B->gexec) {
-            // capture the step of this active set once; every later step of
-            // the set replays it (its arguments do not change)
-            HIPCHK(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
-            launch(B->stream);
-            hipError_t ce = hipStreamEndCapture(B->stream, &B->graph);
-            if (ce != hipSuccess) return fail(FDFK_EDEVICE, std::string("batch step capture: ") + hipGetErrorString(ce));
-            HIPCHK(hipGraphInstantiate(&B->gexec, B->graph, nullptr, nullptr, 0));
+            // the step of this active set as a graph, built once; every later
+            // step of the set replays it (its arguments do not change).  The
+            // graph is assembled node by node rather than captured from the
+            // stream: a capture in progress makes HIP refuse the
+            // stream-ordered allocations other threads make meanwhile for
+            // the solvers they prepare.
+            const int *ctr = B->dctr;
+            void *args[3] = {(void *)&dJ, (void *)&dP, (void *)&ctr};
+            int *tick = B->dctr;
+            void *targs[1] = {(void *)&tick};
+            hipGraph_t gr = nullptr;
+            HIPCHK(hipGraphCreate(&gr, 0));
+            hipGraphNode_t prev = nullptr;
+            auto add = [&](const void *fn, unsigned nb, unsigned nt, size_t sh, void **kargs) -> hipError_t {
+                hipKernelNodeParams kp{};
+                kp.func = const_cast<void *>(fn);
+                kp.gridDim = dim3(nb);
+                kp.blockDim = dim3(nt);
+                kp.sharedMemBytes = (unsigned)sh;
+                kp.kernelParams = kargs;
+                kp.extra = nullptr;
+                hipGraphNode_t n = nullptr;
+                hipError_t e = hipGraphAddKernelNode(&n, gr, prev ? &prev : nullptr, prev ? 1 : 0, &kp);
+                prev = n;
+                return e;
+            };
+            hipError_t ge = hipSuccess;
+            if (B->nprep) ge = add((const void *)kb_prep<NO>, (unsigned)B->nprep, 256, 0, args);
+            if (ge == hipSuccess && B->nstepb) ge = add((const void *)kb_step<NO>, (unsigned)B->nstepb, 256, B->tsh, args);
+            if (ge == hipSuccess) ge = add((const void *)kb_tick, 1, 1, 0, targs);
+            if (ge == hipSuccess) ge = hipGraphInstantiate(&B->gexec, gr, nullptr, nullptr, 0);
+            if (ge != hipSuccess) {
+                (void)hipGraphDestroy(gr);
+                B->gexec = nullptr;
+                return fail(FDFK_EDEVICE, std::string("batch step graph: ") + hipGetErrorString(ge));
+            }
+            B->graph = gr;
         }
         HIPCHK(hipGraphLaunch(B->gexec, B->stream));
     } else {

@@ -370,8 +370,8 @@ __device__ __forceinline__ void frame_body(const StepArgs &a, const ETiles &et, 
     // update: plain leapfrog in the total-field region, split parts in a strip
     if (sb == S_NONE) {
         const float rxs = (p[0] + p[1]) + p[2], rzs = (p[3] + p[4]) + p[5];
-        a.uxp[uc] = (2.0f * ucur[0][0] - uprv[0][0]) + ubf * rxs;
-        a.uzp[uc] = (2.0f * ucur[1][0] - uprv[1][0]) + ubf * rzs;
+        a.uxo[uc] = (2.0f * ucur[0][0] - uprv[0][0]) + ubf * rxs;
+        a.uzo[uc] = (2.0f * ucur[1][0] - uprv[1][0]) + ubf * rzs;
         return;
     }
     const float ax = uax, az = uaz;
@@ -385,7 +385,7 @@ __device__ __forceinline__ void frame_body(const StepArgs &a, const ETiles &et, 
         const float n3 = (c3 * ucur[comp][2] - (1.0f - ax - az) * uprv[comp][2] + ubf * Mm) * r3;
         st->prev[sb][comp][0][uo] = n1; st->prev[sb][comp][1][uo] = n2; st->prev[sb][comp][2][uo] = n3;
         const float tot = (n1 + n2) + n3;
-        if (comp == 0) a.uxp[uc] = tot; else a.uzp[uc] = tot;
+        if (comp == 0) a.uxo[uc] = tot; else a.uzo[uc] = tot;
     }
 }
 

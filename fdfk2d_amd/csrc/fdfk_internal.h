@@ -145,7 +145,8 @@ struct StepArgs {
     Geo g;
     const Coefs *cf;
     const float *ux, *uz;          // level n
-    float *uxp, *uzp;              // level n-1 in, n+1 out
+    float *uxp, *uzp;              // level n-1 in
+    float *uxo, *uzo;              // level n+1 out (uxp, uzp: in place)
     const float *bf, *lam, *mu;    // dt^2/rho, lambda, mu
     const Strips *st;
     const float *ringx, *ringz;    // background u0 on the hybrid ring at this step (0 elsewhere)
@@ -177,7 +178,8 @@ struct IntArgs {
     Geo g;
     ICoef<NO> k;
     const float *ux, *uz;
-    float *uxp, *uzp;
+    float *uxp, *uzp;          // level n-1 in
+    float *uxo, *uzo;          // level n+1 out (uxp, uzp: in place)
     const float *bf, *lam, *mu;
     // FP16 residuals of lam and mu against per-row FP32 profiles (16-bit
     // material storage): value = profile(row) + half * scale, the scales

@@ -331,8 +331,9 @@ __device__ __forceinline__ int xcd_tile(int p, int n)
 // the stores are predicated.  Addresses are a per-column scalar base plus a
 // loop-invariant 32-bit lane offset.
 // ---------------------------------------------------------------------
-// M16: lam and mu are read from their FP16 copies (half the bytes; the FP32
-// arrays hold the same rounded values for the other kernels)
+// M16: lam and mu are read as FP16 residuals against per-row FP32 profiles
+// (half the bytes; the FP32 arrays hold the same rounded values for the
+// other kernels)
 template <int NO, int NT, bool M16 = false>
 struct IntKernel2 {
     static constexpr int HZ = 2 * NO - 1;
@@ -387,7 +388,7 @@ struct IntKernel2 {
     // (c - xb) * ld * 4 + the lane's 32-bit row offset.  Offsets stay far
     // below 4 GiB because a block spans lx + 6*NO columns.
     struct Rsrc {
-        __amdgpu_buffer_rsrc_t ux, uz, uxp, uzp, bf, lam, mu, rx, rz, lam16, mu16;
+        __amdgpu_buffer_rsrc_t ux, uz, uxp, uzp, uxo, uzo, bf, lam, mu, rx, rz, lam16, mu16;
         float plam, pmu;   // M16: the lane row's material profile
         int xb, pitch;   // first column, column pitch in bytes
         __device__ __forceinline__ int soff(int c) const { return (c - xb) * pitch; }
@@ -546,8 +547,8 @@ struct IntKernel2 {
             const float nx_ = (2.0f * R.U[0][sl(PH, -1)] - R.po[PB][1]) + bf * ((Xx + Zx) + Mx);
             const float nz_ = (2.0f * R.U[1][sl(PH, -1)] - R.po[PB][2]) + bf * ((Xz + Zz) + Mz);
             if (L.outrow && ix - 1 >= xs && ix - 1 < xe) {
-                st(B.uxp, o, B.soff(ix - 1), nx_);
-                st(B.uzp, o, B.soff(ix - 1), nz_);
+                st(B.uxo, o, B.soff(ix - 1), nx_);
+                st(B.uzo, o, B.soff(ix - 1), nz_);
             }
         }
     }
@@ -620,6 +621,7 @@ struct IntKernel2 {
             B.pitch = g.ld * 4;
             B.ux = mk(g, a.ux, B.xb); B.uz = mk(g, a.uz, B.xb);
             B.uxp = mk(g, a.uxp, B.xb); B.uzp = mk(g, a.uzp, B.xb);
+            B.uxo = mk(g, a.uxo, B.xb); B.uzo = mk(g, a.uzo, B.xb);
             B.bf = mk(g, a.bf, B.xb); B.lam = mk(g, a.lam, B.xb); B.mu = mk(g, a.mu, B.xb);
             if constexpr (M16) {
                 B.lam16 = mk16(g, a.lam16, B.xb); B.mu16 = mk16(g, a.mu16, B.xb);
@@ -707,9 +709,6 @@ __global__ void k_import_material(Geo g, const float *c, float dt2, float *bf, f
     mu[d] = c[3 * src + 2];
 }
 
-// 16-bit material storage: lam and mu rounded to FP16 with power-of-two
-// scales (round to nearest even); the FP32 arrays take the rounded values,
-// so every kernel sees the same materials
 // 16-bit material storage: lam = lamp(row) + h * sl with h the FP16 residual
 // against a per-row FP32 profile (power-of-two scale, nearest even), so the
 // rounding falls on the lateral variation only; the FP32 arrays receive the
