@@ -226,6 +226,7 @@ struct fdfk_solver {
     // interior kernel
     int lx = 45, nt_threads = 256;
     bool fused = true;   // interior and frame in one launch (k_step)
+    bool frame_full = true;   // full-order frame tiles specialised (FDFK_FRAME_FULL=0: general path only)
     int lx_model = 0;    // interior tile width of the occupancy model (lx may be tuned)
     // 16-bit material storage (FDFK_FP16): lam, mu of the interior tiles as
     // per-row FP32 profiles plus FP16 residuals, value = profile + half *
@@ -351,6 +352,7 @@ StepArgs step_args(fdfk_solver *s)
     a.ringx = s->ringx; a.ringz = s->ringz;
     a.ring_valid = s->bg_valid && s->it <= s->nt;
     a.it = s->it;
+    a.frame_full = s->frame_full ? 1 : 0;
     return a;
 }
 
@@ -693,9 +695,11 @@ bool plan_step2_t(fdfk_solver *s)
         b[2].iz1 = b[3].iz1 = d.iz1;
         b[2].ix1 = d.ix0 - 1;
         b[3].ix0 = d.ix1 + 1;
+        // short marches: the second launch of a pair holds only the bands and
+        // the frame, so a band tile's march is on its critical path
         for (int k = 0; k < 4; ++k) {
             s->gband[k] = b[k];
-            s->band_lx[k] = k < 2 ? s->lx : b[k].ix1 - b[k].ix0 + 1;
+            s->band_lx[k] = k < 2 ? 16 : b[k].ix1 - b[k].ix0 + 1;
         }
         return true;
     }
@@ -962,6 +966,7 @@ int fdfk_create(const fdfk_job *job, fdfk_solver **out)
     s->R0 = job->R0; s->npower = job->npower;
     s->mat16 = job->precision == FDFK_FP16;
     if (const char *e = std::getenv("FDFK_PML_RATIO")) s->pml_ratio = (float)std::atof(e);
+    if (const char *e = std::getenv("FDFK_FRAME_FULL")) s->frame_full = std::atoi(e) != 0;
     plan_interior(s);
     if (s->mat16 && !s->fused) {
         delete s;

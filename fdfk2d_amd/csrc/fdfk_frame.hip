@@ -46,8 +46,13 @@ __device__ __forceinline__ float coefs(const float *tab, int j, int n)
     return j <= n ? c : 0.0f;
 }
 
-template <int NO>
-__device__ __forceinline__ void frame_body(const StepArgs &a, const ETiles &et, long bid)
+// FULL: every face, collocated derivative and cell of the tile carries the
+// full order NO (the tile and its reach stay NO cells from the grid's edges
+// and from the surface row), so the two face fluxes of each face coincide
+// (n1 = n2 = NO: fx2 = fx1, fz2 = fz1, bit for bit) and the coefficients are
+// block constants held in registers
+template <int NO, bool FULL>
+__device__ __forceinline__ void frame_tile(const StepArgs &a, const ETiles &et, long bid)
 {
     extern __shared__ float sm[];   // frame_lds_floats(tx, tz, NO) of the largest tile of the launch
     const Geo &g = a.g;
@@ -73,6 +78,7 @@ __device__ __forceinline__ void frame_body(const StepArgs &a, const ETiles &et, 
     float *dxx = dzz + NDZ * tz, *dxz = dxx + tx * NDX;
     float *mpx0 = dxz + tx * NDX, *mpx1 = mpx0 + NDZ * tz;   // M psi_x on the DZ index space
     float *mpz0 = mpx1 + NDZ * tz, *mpz1 = mpz0 + tx * NDX;  // M psi_z on the DX index space
+    if constexpr (FULL) { fx2x = fx1x; fx2z = fx1z; fz2x = fz1x; fz2z = fz1z; }
     // the band's PML strip (uniform per band): none in the total-field region
     const int sb = view == 0 ? S_NONE : (cz0 > g.nz ? S_B : (cx1 <= 0 ? S_L : S_R));
     // the two coefficient tables, staged once per block
@@ -81,6 +87,15 @@ __device__ __forceinline__ void frame_body(const StepArgs &a, const ETiles &et, 
         c0s[threadIdx.x] = cf->c0f[threadIdx.x];
         c1s[threadIdx.x] = cf->c1f[threadIdx.x];
     }
+    float k0[NO], k1[NO];
+#pragma unroll
+    for (int j = 0; j < NO; ++j) {
+        k0[j] = FULL ? cf->c0f[j + NO * (NO - 1)] : 0.0f;
+        k1[j] = FULL ? cf->c1f[j + NO * (NO - 1)] : 0.0f;
+    }
+    // coefficient j of order n (n = NO throughout a FULL tile)
+    auto C0 = [&](int j, int n) { if constexpr (FULL) return k0[j - 1]; else return coefs<NO>(c0s, j, n); };
+    auto C1 = [&](int j, int n) { if constexpr (FULL) return k1[j - 1]; else return coefs<NO>(c1s, j, n); };
     const FDiv dSZ(SZ), dtz(tz), dNFZ(NFZ), dNDX(NDX);
     // 1/dx and 1/dz are folded into the collocated derivatives and the
     // memory corrections (products instead of divisions)
@@ -189,12 +204,18 @@ __device__ __forceinline__ void frame_body(const StepArgs &a, const ETiles &et, 
         for (int j = 1; j <= NO; ++j) {
             const int p = S(z, f + j), m = S(z, f + 1 - j);
             const float dux = sux[p] - sux[m], duz = suz[p] - suz[m];
-            const float c1 = coefs<NO>(c0s, j, n1), c2 = coefs<NO>(c0s, j, n2);
-            s1x = accf(s1x, c1, dux); s2x = accf(s2x, c2, dux);
-            s1z = accf(s1z, c1, duz); s2z = accf(s2z, c2, duz);
+            const float c1 = C0(j, n1);
+            s1x = accf(s1x, c1, dux);
+            s1z = accf(s1z, c1, duz);
+            if constexpr (!FULL) {
+                const float c2 = C0(j, n2);
+                s2x = accf(s2x, c2, dux);
+                s2z = accf(s2z, c2, duz);
+            }
         }
-        fx1x[e] = ha * s1x; fx2x[e] = ha * s2x;
-        fx1z[e] = hm * s1z; fx2z[e] = hm * s2z;
+        fx1x[e] = ha * s1x;
+        fx1z[e] = hm * s1z;
+        if constexpr (!FULL) { fx2x[e] = ha * s2x; fx2z[e] = hm * s2z; }
     }
     // phase 1b: z-fluxes (mode 1 on ux, mode 0 on uz)
     for (int e = threadIdx.x; e < tx * NFZ; e += kFrameThreads) {
@@ -209,12 +230,18 @@ __device__ __forceinline__ void frame_body(const StepArgs &a, const ETiles &et, 
         for (int j = 1; j <= NO; ++j) {
             const int p = S(f + j, x), m = S(f + 1 - j, x);
             const float dux = sux[p] - sux[m], duz = suz[p] - suz[m];
-            const float c1 = coefs<NO>(c0s, j, n1), c2 = coefs<NO>(c0s, j, n2);
-            s1x = accf(s1x, c1, dux); s2x = accf(s2x, c2, dux);
-            s1z = accf(s1z, c1, duz); s2z = accf(s2z, c2, duz);
+            const float c1 = C0(j, n1);
+            s1x = accf(s1x, c1, dux);
+            s1z = accf(s1z, c1, duz);
+            if constexpr (!FULL) {
+                const float c2 = C0(j, n2);
+                s2x = accf(s2x, c2, dux);
+                s2z = accf(s2z, c2, duz);
+            }
         }
-        fz1x[e] = hm * s1x; fz2x[e] = hm * s2x;
-        fz1z[e] = ha * s1z; fz2z[e] = ha * s2z;
+        fz1x[e] = hm * s1x;
+        fz1z[e] = ha * s1z;
+        if constexpr (!FULL) { fz2x[e] = hm * s2x; fz2z[e] = ha * s2z; }
     }
     // phase 1c: collocated Dz (cdz_sum) at columns dz0.., rows of the tile
     for (int e = threadIdx.x; e < NDZ * tz; e += kFrameThreads) {
@@ -225,7 +252,7 @@ __device__ __forceinline__ void frame_body(const StepArgs &a, const ETiles &et, 
 #pragma unroll
         for (int j = 1; j <= NO; ++j) {
             const int p = S(z + j, x), m = S(z - j, x);
-            const float c = coefs<NO>(c1s, j, n);
+            const float c = C1(j, n);
             sx = accf(sx, c, sux[p] - sux[m]);
             sz = accf(sz, c, suz[p] - suz[m]);
         }
@@ -240,7 +267,7 @@ __device__ __forceinline__ void frame_body(const StepArgs &a, const ETiles &et, 
 #pragma unroll
         for (int j = 1; j <= NO; ++j) {
             const int p = S(z, x + j), m = S(z, x - j);
-            const float c = coefs<NO>(c1s, j, n);
+            const float c = C1(j, n);
             sx = accf(sx, c, sux[p] - sux[m]);
             sz = accf(sz, c, suz[p] - suz[m]);
         }
@@ -301,8 +328,8 @@ __device__ __forceinline__ void frame_body(const StepArgs &a, const ETiles &et, 
     float Xx = 0.0f, Xz = 0.0f, Zx = 0.0f, Zz = 0.0f, zxl = 0.0f, zxm = 0.0f, xzm = 0.0f, xzl = 0.0f;
 #pragma unroll
     for (int k = 1; k <= NO; ++k) {
-        const float c0x = coefs<NO>(c0s, k, nox), c0z = coefs<NO>(c0s, k, noz);
-        const float c1x = coefs<NO>(c1s, k, nox), c1z = coefs<NO>(c1s, k, noz);
+        const float c0x = C0(k, nox), c0z = C0(k, noz);
+        const float c1x = C1(k, nox), c1z = C1(k, noz);
         Xx = accf(Xx, c0x, FX(fx1x, ix + k - 1) - FX(fx2x, ix - k));
         Xz = accf(Xz, c0x, FX(fx1z, ix + k - 1) - FX(fx2z, ix - k));
         Zx = accf(Zx, c0z, FZ(fz1x, iz + k - 1) - FZ(fz2x, iz - k));
@@ -328,7 +355,7 @@ __device__ __forceinline__ void frame_body(const StepArgs &a, const ETiles &et, 
 #pragma unroll
         for (int k = 1; k <= NO; ++k) {
             const int p1 = S(1, ix + k), p2 = S(2, ix + k), m1 = S(1, ix - k), m2 = S(2, ix - k);
-            const float c = coefs<NO>(c1s, k, nox);
+            const float c = C1(k, nox);
             const float lp = 0.5f * (slam[p1] + slam[p2]), lm = 0.5f * (slam[m1] + slam[m2]);
             const float mp = 0.5f * (smu[p1] + smu[p2]), mm = 0.5f * (smu[m1] + smu[m2]);
             sa = accf(sa, c, lp * (suz[p2] - suz[p1]) - lm * (suz[m2] - suz[m1]));
@@ -350,7 +377,7 @@ __device__ __forceinline__ void frame_body(const StepArgs &a, const ETiles &et, 
         float sx0 = 0.0f, sx1 = 0.0f, sz0 = 0.0f, sz1 = 0.0f;
 #pragma unroll
         for (int k = 1; k <= NO; ++k) {
-            const float c = coefs<NO>(c1s, k, nox);
+            const float c = C1(k, nox);
             sx0 = accf(sx0, c, DZ(mpx0, ix + k) - DZ(mpx0, ix - k));
             sx1 = accf(sx1, c, DZ(mpx1, ix + k) - DZ(mpx1, ix - k));
         }
@@ -359,7 +386,7 @@ __device__ __forceinline__ void frame_body(const StepArgs &a, const ETiles &et, 
         if (sb == S_B) {
 #pragma unroll
             for (int k = 1; k <= NO; ++k) {
-                const float c = coefs<NO>(c1s, k, noz);
+                const float c = C1(k, noz);
                 sz0 = accf(sz0, c, DX(mpz0, iz + k) - DX(mpz0, iz - k));
                 sz1 = accf(sz1, c, DX(mpz1, iz + k) - DX(mpz1, iz - k));
             }
@@ -387,6 +414,27 @@ __device__ __forceinline__ void frame_body(const StepArgs &a, const ETiles &et, 
         const float tot = (n1 + n2) + n3;
         if (comp == 0) a.uxo[uc] = tot; else a.uzo[uc] = tot;
     }
+}
+
+// A tile is FULL when the faces and derivatives it evaluates (its cells
+// plus NO on each side, the faces of its reach 2NO) keep the full order:
+// x within [2NO+1-npml, nx+npml-2NO], z within [2NO+1, NZ-2NO]
+template <int NO>
+__device__ __forceinline__ void frame_body(const StepArgs &a, const ETiles &et, long bid)
+{
+    const Geo &g = a.g;
+    int q = 0;
+    while (q + 1 < et.n && bid >= et.off[q + 1]) ++q;
+    const int loc = (int)(bid - et.off[q]);
+    const int bx = loc / et.nbz[q], bz = loc % et.nbz[q];
+    const int cx0 = et.x0[q] + bx * et.tx[q], cx1 = min(cx0 + et.tx[q] - 1, et.x1[q]);
+    const int cz0 = et.z0[q] + bz * et.tz[q], cz1 = min(cz0 + et.tz[q] - 1, et.z1[q]);
+    const bool full = a.frame_full && cx0 >= 2 * NO + 1 - g.npml && cx1 <= g.nx + g.npml - 2 * NO && cz0 >= 2 * NO + 1 &&
+                      cz1 <= g.NZ - 2 * NO;
+    if (full)
+        frame_tile<NO, true>(a, et, bid);
+    else
+        frame_tile<NO, false>(a, et, bid);
 }
 
 template <int NO>

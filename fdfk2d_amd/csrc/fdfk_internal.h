@@ -152,6 +152,7 @@ struct StepArgs {
     const float *ringx, *ringz;    // background u0 on the hybrid ring at this step (0 elsewhere)
     int ring_valid;
     int it;
+    int frame_full;                // full-order frame tiles take their specialised path
 };
 
 // frame-kernel work decomposition: up to four bands (columns x0..x1, rows
