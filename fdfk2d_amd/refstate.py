@@ -63,13 +63,15 @@ def vp_max(c):
     return float(np.sqrt((c[:, 1] + np.float32(2.0) * c[:, 2]) / c[:, 0]).max())
 
 
-def make_state(nx, nz, npml, norder, dx, dz, dt, nt, c, npower=2, R0=1e-3, vpml=None):
-    """All arrays updatedispl receives, zero-initialised (free surface, izt = 1)."""
-    izt = 1
+def make_state(nx, nz, npml, norder, dx, dz, dt, nt, c, npower=2, R0=1e-3, vpml=None, pml_top=False):
+    """All arrays updatedispl receives, zero-initialised: free surface at row 1
+    (izt = 1), or with pml_top an absorbing top layer of npml rows above it
+    (izt = 1-npml, src/Input.f90:179-183); c holds rows izt..nz+npml."""
+    izt = 1 - npml if pml_top else 1
     NZ, NX = nz + npml - izt + 1, nx + 2 * npml
     c = np.ascontiguousarray(c, np.float32)
     assert c.size == 3 * NZ * NX
-    st = dict(norder=norder, nx=nx, nz=nz, npml=npml, izt=izt, pml_top=0, npower=npower,
+    st = dict(norder=norder, nx=nx, nz=nz, npml=npml, izt=izt, pml_top=int(bool(pml_top)), npower=npower,
               nbd=2 * norder - 1, nt=nt, dx=float(dx), dz=float(dz), dt=float(dt),
               vpml=vp_max(c) if vpml is None else float(vpml), R0=float(R0), c=c,
               coef0=fd_coef(norder, True), coef1=fd_coef(norder, False),

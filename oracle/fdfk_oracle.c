@@ -498,7 +498,15 @@ void orc_backazimuth_grids(double lat_org, double lon_org, double az_org, int n,
  *  Damping profiles d = d0 ((q-nbd)/(npml-nbd))^npower for q > nbd cells
  *  into the left/right/bottom strips (the first nbd cells are an undamped
  *  buffer that holds the hybrid ring), d0 = -(npower+1) vpml ln(R0) /
- *  (2 (npml-nbd) h); the optional top strip has no buffer.
+ *  (2 (npml-nbd) h); the optional top strip (is_PML_top, izt = 1-npml) has
+ *  no buffer: d = d0t ((1-iz)/npml)^npower for its rows iz <= 0.
+ *  Top strip: row 1 is then an ordinary cell (no free-surface closure), the
+ *  strip keeps x and z memories like the bottom one, and it holds the field
+ *  of the region beneath it without a hybrid correction: there is no ring
+ *  across the top (the reference's twelve background arrays cover the
+ *  sides and the bottom only), so the background must be one without a
+ *  free surface (the FK background of izt < 1), whose waves leave through
+ *  the top and are absorbed there.
  *  Multiaxial damping (Meza-Fajardo & Papageorgiou 2008, BSSA 98:1811):
  *  the split parts of a strip cell are damped along both axes,
  *    dX = dx(ix) + r dz(iz),   dZ = dz(iz) + r dx(ix)   (the dX, dZ above),
@@ -517,6 +525,7 @@ void orc_backazimuth_grids(double lat_org, double lon_org, double az_org, int n,
 
 typedef struct {
     int norder, nx, nz, npml, izt, pml_top, npower, nbd, it, nt;
+    int top_ring;   /* the top ring's background arrays are present */
     orc_real dx, dz;
     double dt;
     int NZ, NX;
@@ -536,7 +545,7 @@ void orc_damping(int nx, int nz, int npml, int nbd, int is_pml_top, int npower, 
     double lr = log(R0);
     double d0x = leff > 0 ? -(double)(npower + 1) * (double)vpml * lr / (2.0 * leff * (double)dx) : 0.0;
     double d0z = leff > 0 ? -(double)(npower + 1) * (double)vpml * lr / (2.0 * leff * (double)dz) : 0.0;
-    double d0t = -(double)(npower + 1) * (double)vpml * lr / (2.0 * npml * (double)dz);
+
     for (int j = 0; j < NX; ++j) {
         int ix = j + 1 - npml;
         int q = ix <= 0 ? 1 - ix : (ix > nx ? ix - nx : 0);
@@ -559,9 +568,12 @@ void orc_damping(int nx, int nz, int npml, int nbd, int is_pml_top, int npower, 
                 d = d0z * p;
             }
         } else if (iz <= 0) {
-            double r = (double)(1 - iz) / (double)npml, p = 1.0;
-            for (int e = 0; e < npower; ++e) p *= r;
-            d = d0t * p;
+            int q = 1 - iz;
+            if (q > nbd && leff > 0) {
+                double r = (double)(q - nbd) / (double)leff, p = 1.0;
+                for (int e = 0; e < npower; ++e) p *= r;
+                d = d0z * p;
+            }
         }
         dpz[j] = (orc_real)d;
     }
@@ -778,10 +790,22 @@ static inline long bg_off(const gdims *g, int iz, int ix, int it, int *arr, int 
         *arr = 2; *which = 1;
         return (long)((size_t)(iz - (nz + 1)) + (size_t)nbd * ((size_t)(ix - (1 - nbd)) + (size_t)(nx + 2 * nbd) * it));
     }
+    if (g->top_ring) {
+        /* top ring of an absorbing top layer: inside rows 1..nbd between the
+         * side strips, outside rows 1-nbd..0 over cols 1-nbd..nx+nbd */
+        if (iz >= 1 && iz <= nbd && ix >= nbd + 1 && ix <= nx - nbd) {
+            *arr = 3; *which = 0;
+            return (long)((size_t)(iz - 1) + (size_t)nbd * ((size_t)(ix - (nbd + 1)) + (size_t)(nx - 2 * nbd) * it));
+        }
+        if (iz >= 1 - nbd && iz <= 0 && ix >= 1 - nbd && ix <= nx + nbd) {
+            *arr = 3; *which = 1;
+            return (long)((size_t)(iz - (1 - nbd)) + (size_t)nbd * ((size_t)(ix - (1 - nbd)) + (size_t)(nx + 2 * nbd) * it));
+        }
+    }
     return -1;
 }
 
-void orc_updatedispl(const int *norder_, const int *nx_, const int *nz_, const int *npml_,
+static void updatedispl_body(const orc_real *const top[4], const int *norder_, const int *nx_, const int *nz_, const int *npml_,
     const int *izt_, const int *is_pml_top_, const int *npower_, const int *nbd_, const int *it_,
     const int *nt_, const orc_real *dx_, const orc_real *dz_, const double *dt_, const orc_real *vpml_,
     const double *R0_,
@@ -797,6 +821,7 @@ void orc_updatedispl(const int *norder_, const int *nx_, const int *nz_, const i
     orc_real *bux1t, orc_real *bux2t, orc_real *bux3t, orc_real *buz1t, orc_real *buz2t, orc_real *buz3t)
 {
     gdims G, *g = &G;
+    memset(&G, 0, sizeof G);
     G.norder = *norder_; G.nx = *nx_; G.nz = *nz_; G.npml = *npml_; G.izt = *izt_;
     G.pml_top = *is_pml_top_; G.npower = *npower_; G.nbd = *nbd_; G.it = *it_; G.nt = *nt_;
     G.dx = *dx_; G.dz = *dz_; G.dt = *dt_;
@@ -816,7 +841,7 @@ void orc_updatedispl(const int *norder_, const int *nx_, const int *nz_, const i
     for (int ix = 1 - npml; ix <= nx + npml; ++ix)
         for (int iz = izt; iz <= nz + npml; ++iz) {
             int sx = ix < 1 - nbd ? 1 - nbd : (ix > nx + nbd ? nx + nbd : ix);
-            int sz = iz > nz + nbd ? nz + nbd : iz;
+            int sz = iz > nz + nbd ? nz + nbd : (iz < 1 - nbd ? 1 - nbd : iz);
             size_t i = g2(g, iz, ix), j = g2(g, sz, sx);
             u1x[i] = ux[2 * i]; u1z[i] = uz[2 * i];
             rho[i] = c[3 * j]; lam[i] = c[3 * j + 1]; mu[i] = c[3 * j + 2];
@@ -913,17 +938,19 @@ void orc_updatedispl(const int *norder_, const int *nx_, const int *nz_, const i
     orc_diffxz_pml(norder, nx, nz, npml, izt, dxf, dzf, coef1, lam, u1x, t2); ADD2(P[5]);
 #undef ADD2
 
-    /* masked background fields for the hybrid correction */
-    const orc_real *bgx[2][3] = {{uxl_reg, uxr_reg, uxb_reg}, {uxl_pml, uxr_pml, uxb_pml}};
-    const orc_real *bgz[2][3] = {{uzl_reg, uzr_reg, uzb_reg}, {uzl_pml, uzr_pml, uzb_pml}};
+    /* masked background fields for the hybrid correction; the top ring
+     * (arrays top[0..3] = uxt_reg, uzt_reg, uxt_pml, uzt_pml) with a top PML */
+    const orc_real *bgx[2][4] = {{uxl_reg, uxr_reg, uxb_reg, top[0]}, {uxl_pml, uxr_pml, uxb_pml, top[2]}};
+    const orc_real *bgz[2][4] = {{uzl_reg, uzr_reg, uzb_reg, top[1]}, {uzl_pml, uzr_pml, uzb_pml, top[3]}};
     int have_bg = uxl_reg && uzl_reg && uxr_reg && uzr_reg && uxb_reg && uzb_reg && uxl_pml &&
                   uzl_pml && uxr_pml && uzr_pml && uxb_pml && uzb_pml && G.it >= 0 && G.it <= G.nt;
+    G.top_ring = G.pml_top && top[0] && top[1] && top[2] && top[3];
     orc_real *wTx = NULL, *wTz = NULL, *wSx = NULL, *wSz = NULL;
     if (have_bg) {
         wTx = calloc(ncell, sizeof(orc_real)); wTz = calloc(ncell, sizeof(orc_real));
         wSx = calloc(ncell, sizeof(orc_real)); wSz = calloc(ncell, sizeof(orc_real));
         for (int ix = 1 - nbd; ix <= nx + nbd; ++ix)
-            for (int iz = 1; iz <= nz + nbd; ++iz) {
+            for (int iz = G.top_ring ? 1 - nbd : 1; iz <= nz + nbd; ++iz) {
                 int arr, which;
                 long o = bg_off(g, iz, ix, G.it, &arr, &which);
                 if (o < 0) continue;
@@ -945,7 +972,15 @@ void orc_updatedispl(const int *norder_, const int *nx_, const int *nz_, const i
             size_t gi = g2(g, iz, ix);
             orc_real p[6];
             for (int q = 0; q < 6; ++q) p[q] = P[q][gi];
-            if (iz == 1 && izt == 1) eval_parts_g(g, coef0, coef1, lam, mu, u1x, u1z, iz, ix, p);
+            /* Row 1: with a free surface the closure replaces the passes.
+             * Under an absorbing top layer (izt < 1) every cell is evaluated
+             * pointwise: the reference's passes are written for izt = 1 near
+             * the top (diffz2_reg and diffz2_pml leave row 1 of columns 1..nx
+             * without z-terms, Submain.f90:971,1062-1063; diffxz_reg builds
+             * its intermediate from row 1 down, :616, so the full-order Dz of
+             * rows 2..norder reads zeros above it).  The pointwise operators
+             * equal the passes bit for bit wherever those are complete. */
+            if ((iz == 1 && izt == 1) || izt != 1) eval_parts_g(g, coef0, coef1, lam, mu, u1x, u1z, iz, ix, p);
 
             int s = strip_of(g, iz, ix);
             /* PML memory corrections (nonzero only next to damped cells) */
@@ -1025,4 +1060,56 @@ void orc_updatedispl(const int *norder_, const int *nx_, const int *nz_, const i
     for (int q = 0; q < 2; ++q) { free(psx[q]); free(psz[q]); }
     free(dpx); free(dpz);
     free(u1x); free(u1z); free(rho); free(lam); free(mu); free(lp2m);
+}
+
+/* the reference's 68-argument call: no top ring (a top PML then has an open
+ * hybrid contour at the top) */
+void orc_updatedispl(const int *norder_, const int *nx_, const int *nz_, const int *npml_,
+    const int *izt_, const int *is_pml_top_, const int *npower_, const int *nbd_, const int *it_,
+    const int *nt_, const orc_real *dx_, const orc_real *dz_, const double *dt_, const orc_real *vpml_,
+    const double *R0_,
+    const orc_real *uxl_reg, const orc_real *uzl_reg, const orc_real *uxr_reg, const orc_real *uzr_reg,
+    const orc_real *uxb_reg, const orc_real *uzb_reg, const orc_real *uxl_pml, const orc_real *uzl_pml,
+    const orc_real *uxr_pml, const orc_real *uzr_pml, const orc_real *uxb_pml, const orc_real *uzb_pml,
+    const orc_real *c, const double *coef0, const double *coef1, orc_real *ux, orc_real *uz,
+    orc_real *bpxlx, orc_real *bpzlx, orc_real *bpxrx, orc_real *bpzrx, orc_real *bpxtx, orc_real *bpxtz,
+    orc_real *bpztx, orc_real *bpztz, orc_real *bpxbx, orc_real *bpxbz, orc_real *bpzbx, orc_real *bpzbz,
+    orc_real *bux1l, orc_real *bux2l, orc_real *bux3l, orc_real *buz1l, orc_real *buz2l, orc_real *buz3l,
+    orc_real *bux1r, orc_real *bux2r, orc_real *bux3r, orc_real *buz1r, orc_real *buz2r, orc_real *buz3r,
+    orc_real *bux1b, orc_real *bux2b, orc_real *bux3b, orc_real *buz1b, orc_real *buz2b, orc_real *buz3b,
+    orc_real *bux1t, orc_real *bux2t, orc_real *bux3t, orc_real *buz1t, orc_real *buz2t, orc_real *buz3t)
+{
+    const orc_real *const none[4] = {NULL, NULL, NULL, NULL};
+    updatedispl_body(none, norder_, nx_, nz_, npml_, izt_, is_pml_top_, npower_, nbd_, it_, nt_, dx_, dz_, dt_,
+        vpml_, R0_, uxl_reg, uzl_reg, uxr_reg, uzr_reg, uxb_reg, uzb_reg, uxl_pml, uzl_pml, uxr_pml, uzr_pml,
+        uxb_pml, uzb_pml, c, coef0, coef1, ux, uz, bpxlx, bpzlx, bpxrx, bpzrx, bpxtx, bpxtz, bpztx, bpztz,
+        bpxbx, bpxbz, bpzbx, bpzbz, bux1l, bux2l, bux3l, buz1l, buz2l, buz3l, bux1r, bux2r, bux3r, buz1r,
+        buz2r, buz3r, bux1b, bux2b, bux3b, buz1b, buz2b, buz3b, bux1t, bux2t, bux3t, buz1t, buz2t, buz3t);
+}
+
+/* the same with the top ring's four background arrays (uxt_reg, uzt_reg,
+ * uxt_pml, uzt_pml of orc_comput_background_wavefield_ext), which close the
+ * hybrid contour across the top when is_pml_top */
+void orc_updatedispl_ext(const orc_real *uxt_reg, const orc_real *uzt_reg, const orc_real *uxt_pml,
+    const orc_real *uzt_pml, const int *norder_, const int *nx_, const int *nz_, const int *npml_,
+    const int *izt_, const int *is_pml_top_, const int *npower_, const int *nbd_, const int *it_,
+    const int *nt_, const orc_real *dx_, const orc_real *dz_, const double *dt_, const orc_real *vpml_,
+    const double *R0_,
+    const orc_real *uxl_reg, const orc_real *uzl_reg, const orc_real *uxr_reg, const orc_real *uzr_reg,
+    const orc_real *uxb_reg, const orc_real *uzb_reg, const orc_real *uxl_pml, const orc_real *uzl_pml,
+    const orc_real *uxr_pml, const orc_real *uzr_pml, const orc_real *uxb_pml, const orc_real *uzb_pml,
+    const orc_real *c, const double *coef0, const double *coef1, orc_real *ux, orc_real *uz,
+    orc_real *bpxlx, orc_real *bpzlx, orc_real *bpxrx, orc_real *bpzrx, orc_real *bpxtx, orc_real *bpxtz,
+    orc_real *bpztx, orc_real *bpztz, orc_real *bpxbx, orc_real *bpxbz, orc_real *bpzbx, orc_real *bpzbz,
+    orc_real *bux1l, orc_real *bux2l, orc_real *bux3l, orc_real *buz1l, orc_real *buz2l, orc_real *buz3l,
+    orc_real *bux1r, orc_real *bux2r, orc_real *bux3r, orc_real *buz1r, orc_real *buz2r, orc_real *buz3r,
+    orc_real *bux1b, orc_real *bux2b, orc_real *bux3b, orc_real *buz1b, orc_real *buz2b, orc_real *buz3b,
+    orc_real *bux1t, orc_real *bux2t, orc_real *bux3t, orc_real *buz1t, orc_real *buz2t, orc_real *buz3t)
+{
+    const orc_real *const top[4] = {uxt_reg, uzt_reg, uxt_pml, uzt_pml};
+    updatedispl_body(top, norder_, nx_, nz_, npml_, izt_, is_pml_top_, npower_, nbd_, it_, nt_, dx_, dz_, dt_,
+        vpml_, R0_, uxl_reg, uzl_reg, uxr_reg, uzr_reg, uxb_reg, uzb_reg, uxl_pml, uzl_pml, uxr_pml, uzr_pml,
+        uxb_pml, uzb_pml, c, coef0, coef1, ux, uz, bpxlx, bpzlx, bpxrx, bpzrx, bpxtx, bpxtz, bpztx, bpztz,
+        bpxbx, bpxbz, bpzbx, bpzbz, bux1l, bux2l, bux3l, buz1l, buz2l, buz3l, bux1r, bux2r, bux3r, buz1r,
+        buz2r, buz3r, bux1b, bux2b, bux3b, buz1b, buz2b, buz3b, bux1t, bux2t, bux3t, buz1t, buz2t, buz3t);
 }

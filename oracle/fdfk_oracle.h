@@ -101,6 +101,25 @@ void orc_updatedispl(const int *norder, const int *nx, const int *nz, const int 
     orc_real *bux1b, orc_real *bux2b, orc_real *bux3b, orc_real *buz1b, orc_real *buz2b, orc_real *buz3b,
     orc_real *bux1t, orc_real *bux2t, orc_real *bux3t, orc_real *buz1t, orc_real *buz2t, orc_real *buz3t);
 
+/* orc_updatedispl plus the top ring's four background arrays (uxt_reg,
+ * uzt_reg, uxt_pml, uzt_pml), which close the hybrid contour across the top
+ * when is_pml_top (without them the contour is open there) */
+void orc_updatedispl_ext(const orc_real *uxt_reg, const orc_real *uzt_reg, const orc_real *uxt_pml,
+    const orc_real *uzt_pml, const int *norder, const int *nx, const int *nz, const int *npml,
+    const int *izt, const int *is_pml_top, const int *npower, const int *nbd, const int *it,
+    const int *nt, const orc_real *dx, const orc_real *dz, const double *dt, const orc_real *vpml,
+    const double *R0,
+    const orc_real *uxl_reg, const orc_real *uzl_reg, const orc_real *uxr_reg, const orc_real *uzr_reg,
+    const orc_real *uxb_reg, const orc_real *uzb_reg, const orc_real *uxl_pml, const orc_real *uzl_pml,
+    const orc_real *uxr_pml, const orc_real *uzr_pml, const orc_real *uxb_pml, const orc_real *uzb_pml,
+    const orc_real *c, const double *coef0, const double *coef1, orc_real *ux, orc_real *uz,
+    orc_real *bpxlx, orc_real *bpzlx, orc_real *bpxrx, orc_real *bpzrx, orc_real *bpxtx, orc_real *bpxtz,
+    orc_real *bpztx, orc_real *bpztz, orc_real *bpxbx, orc_real *bpxbz, orc_real *bpzbx, orc_real *bpzbz,
+    orc_real *bux1l, orc_real *bux2l, orc_real *bux3l, orc_real *buz1l, orc_real *buz2l, orc_real *buz3l,
+    orc_real *bux1r, orc_real *bux2r, orc_real *bux3r, orc_real *buz1r, orc_real *buz2r, orc_real *buz3r,
+    orc_real *bux1b, orc_real *bux2b, orc_real *bux3b, orc_real *buz1b, orc_real *buz2b, orc_real *buz3b,
+    orc_real *bux1t, orc_real *bux2t, orc_real *bux3t, orc_real *buz1t, orc_real *buz2t, orc_real *buz3t);
+
 /* Multiaxial ratio r of the PML damping: the split parts of a strip cell are
  * damped by dX = dx + r dz along x and dZ = dz + r dx along z; the memories
  * keep dx and dz (r = 0: classical PML). */
@@ -134,12 +153,30 @@ void orc_comput_background_wavefield(const int *nx, const int *nz, const int *np
     orc_real *uzb_reg, orc_real *uxl_pml, orc_real *uzl_pml, orc_real *uxr_pml, orc_real *uzr_pml,
     orc_real *uxb_pml, orc_real *uzb_pml);
 
+/* the same plus the top ring of an absorbing top layer (izt < 1):
+ * uxt_reg/uzt_reg (nbd, nx-2nbd, nt+1) on rows 1..nbd x cols nbd+1..nx-nbd,
+ * uxt_pml/uzt_pml (nbd, nx+2nbd, nt+1) on rows 1-nbd..0 x cols 1-nbd..nx+nbd */
+void orc_comput_background_wavefield_ext(orc_real *uxt_reg, orc_real *uzt_reg, orc_real *uxt_pml,
+    orc_real *uzt_pml, const int *nx, const int *nz, const int *npml,
+    const int *izt, const int *nbd, const int *nt, const int *nlayer_left,
+    const int *nlayer_right, const int *src_type, const orc_real *x0, const orc_real *z0,
+    const orc_real *dx, const orc_real *dz, const double *dt, const orc_real *ds, const orc_real *f0,
+    const double *rayp, const double *bazs, const orc_real *fk_left, const orc_real *fk_right,
+    orc_real *uxl_reg, orc_real *uzl_reg, orc_real *uxr_reg, orc_real *uzr_reg, orc_real *uxb_reg,
+    orc_real *uzb_reg, orc_real *uxl_pml, orc_real *uzl_pml, orc_real *uxr_pml, orc_real *uzr_pml,
+    orc_real *uxb_pml, orc_real *uzb_pml);
+
 /* FK response of one 1-D model at depths z[0..nzq) for one column offset:
  * ux/uz (nzq x ntq, time fastest) of the plane wave at x-offset 0 with the
  * incident wave reaching depth zref at time t0 (seconds).  Exposed for tests. */
 int orc_fk_traces(int nlayer, const orc_real *fkmodel, int src_is_p, double p, orc_real f0, orc_real ds,
                   double dt, int ntq, int nzq, const double *z, double zref, double t0,
                   double *ux, double *uz);
+/* the same with the top condition chosen: free_surface 1 (traction-free
+ * z = 0) or 0 (the top layer extends upward, no down-going wave in it) */
+int orc_fk_traces_fs(int nlayer, const orc_real *fkmodel, int src_is_p, int free_surface, double p, orc_real f0,
+                     orc_real ds, double dt, int ntq, int nzq, const double *z, double zref, double t0,
+                     double *ux, double *uz);
 
 #ifdef __cplusplus
 }

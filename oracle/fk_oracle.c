@@ -12,7 +12,9 @@
  *    S up) referenced at z_j; displacement-traction vector
  *    b = (ux, uz, szz/(iw), sxz/(iw)) = E_j Lambda_j(z - z_j) a_j;
  *  - b is continuous across interfaces; in the half-space the only up-going
- *    wave is the incident one (P or SV); the free surface z = 0 is traction free;
+ *    wave is the incident one (P or SV); the free surface z = 0 is traction
+ *    free (izt = 1), or, with an absorbing top layer (izt < 1), the top
+ *    layer extends upward and carries no down-going wave;
  *  - the incident pulse s(t) = ds exp(-(pi f0 (t - t0))^2), t0 = 1.5/f0, passes
  *    the entry column at depth z_ref (the deeper of the ring bottom and the
  *    half-space top) at t = t0;
@@ -141,8 +143,11 @@ static void fk_free(fkmodel *m)
     free(m->zt); free(m->vp); free(m->vs); free(m->rho); free(m->ea); free(m->eb); free(m->E); free(m->Q);
 }
 
-/* layer amplitudes a[j][0..3] for complex frequency w and incident amplitude ainc */
-static void fk_amplitudes(const fkmodel *m, int src_is_p, cplx w, cplx ainc, cplx (*a)[4])
+/* layer amplitudes a[j][0..3] for complex frequency w and incident amplitude
+ * ainc.  free_surface: the top z_0 is traction free; otherwise the top layer
+ * extends upward without end and carries no down-going wave (the
+ * configuration of an absorbing top layer) */
+static void fk_amplitudes(const fkmodel *m, int src_is_p, int free_surface, cplx w, cplx ainc, cplx (*a)[4])
 {
     int N = m->nl;
     cplx basis[3][4] = {{0}};
@@ -168,12 +173,17 @@ static void fk_amplitudes(const fkmodel *m, int src_is_p, cplx w, cplx ainc, cpl
             memcpy(ab[j][b], t, sizeof t);
         }
     }
-    /* traction at the surface (layer 0, z = z_0 = 0 reference) */
+    /* the two top conditions of each basis vector: the traction at the
+     * surface (layer 0, z = z_0 reference), or the down-going P and S
+     * amplitudes of layer 0 */
     cplx T[3][2];
     for (int b = 0; b < 3; ++b)
         for (int r = 0; r < 2; ++r) {
             cplx s = 0;
-            for (int c = 0; c < 4; ++c) s += m->E[0][2 + r][c] * ab[0][b][c];
+            if (free_surface)
+                for (int c = 0; c < 4; ++c) s += m->E[0][2 + r][c] * ab[0][b][c];
+            else
+                s = ab[0][b][r];
             T[b][r] = s;
         }
     /* T0*ainc + x1*T1 + x2*T2 = 0 */
@@ -200,7 +210,7 @@ typedef struct {
     double *g; /* nzq x nfft x 2 */
 } fktraces;
 
-static int fk_compute(const fkmodel *m, int src_is_p, double p, double f0, double ds, double dtf,
+static int fk_compute(const fkmodel *m, int src_is_p, int free_surface, double p, double f0, double ds, double dtf,
                       int nfft, int nzq, const double *zq, double zref, double t0, fktraces *out)
 {
     int N = m->nl;
@@ -221,7 +231,7 @@ static int fk_compute(const fkmodel *m, int src_is_p, double p, double f0, doubl
         cplx x = w / (2.0 * K_PI * f0);
         cplx S = ds / (f0 * sqrt(K_PI)) * cexp(-x * x);
         cplx ainc = S * cexp(I * w * tref);
-        fk_amplitudes(m, src_is_p, w, ainc, a);
+        fk_amplitudes(m, src_is_p, free_surface, w, ainc, a);
         for (int q = 0; q < nzq; ++q) {
             int j = lay[q];
             double dz = zq[q] - m->zt[j];
@@ -276,13 +286,20 @@ int orc_fk_traces(int nlayer, const orc_real *fkm, int src_is_p, double p, orc_r
                   double dt, int ntq, int nzq, const double *z, double zref, double t0,
                   double *ux, double *uz)
 {
+    return orc_fk_traces_fs(nlayer, fkm, src_is_p, 1, p, f0, ds, dt, ntq, nzq, z, zref, t0, ux, uz);
+}
+
+int orc_fk_traces_fs(int nlayer, const orc_real *fkm, int src_is_p, int free_surface, double p, orc_real f0,
+                     orc_real ds, double dt, int ntq, int nzq, const double *z, double zref, double t0,
+                     double *ux, double *uz)
+{
     fkmodel m;
     if (fk_setup(&m, nlayer, fkm, p)) { fk_free(&m); return -1; }
     double dtf = dt / 2.0;
     double tneed = ntq * dt + 2.0 * t0 + 10.0 / f0;
     int nfft = next_pow2((long)ceil(2.0 * tneed / dtf));
     fktraces tr;
-    fk_compute(&m, src_is_p, p, f0, ds, dtf, nfft, nzq, z, zref, t0, &tr);
+    fk_compute(&m, src_is_p, free_surface, p, f0, ds, dtf, nfft, nzq, z, zref, t0, &tr);
     for (int q = 0; q < nzq; ++q)
         for (int n = 0; n < ntq; ++n) {
             const double *g = tr.g + (size_t)q * nfft * 2;
@@ -294,7 +311,7 @@ int orc_fk_traces(int nlayer, const orc_real *fkm, int src_is_p, double p, orc_r
     return 0;
 }
 
-void orc_comput_background_wavefield(const int *nx_, const int *nz_, const int *npml_,
+static void background_body(orc_real *const top[4], const int *nx_, const int *nz_, const int *npml_,
     const int *izt_, const int *nbd_, const int *nt_, const int *nlayer_left,
     const int *nlayer_right, const int *src_type, const orc_real *x0_, const orc_real *z0_,
     const orc_real *dx_, const orc_real *dz_, const double *dt_, const orc_real *ds_, const orc_real *f0_,
@@ -306,17 +323,22 @@ void orc_comput_background_wavefield(const int *nx_, const int *nz_, const int *
     const int nx = *nx_, nz = *nz_, npml = *npml_, nbd = *nbd_, nt = *nt_;
     const double x0 = *x0_, z0 = *z0_, dx = *dx_, dz = *dz_, dt = *dt_;
     const int src_is_p = *src_type != 0;
-    (void)izt_;
+    /* izt = 1: free surface at row 1; izt < 1 (a top PML): no free surface */
+    const int free_surface = *izt_ == 1;
     double cs = 0.0;
     for (int i = 0; i < nx + 2 * npml; ++i) cs += cos(bazs[i]);
     double sgn = ((*rayp_ < 0.0) ? -1.0 : 1.0) * (cs < 0.0 ? 1.0 : -1.0);
     double p = fabs(*rayp_);
     double t0 = 1.5 / (double)*f0_;
 
-    /* depths of the ring rows 1..nz+nbd */
-    int nzq = nz + nbd;
+    /* the top ring (rows 1-nbd..nbd) with an absorbing top layer */
+    const int top_ring = !free_surface && top[0] && top[1] && top[2] && top[3];
+    /* depths of the ring rows zf..nz+nbd (above the model top, the top
+     * layer continues upward) */
+    const int zf = top_ring ? 1 - nbd : 1;
+    int nzq = nz + nbd - zf + 1;
     double *zq = malloc(nzq * sizeof(double));
-    for (int q = 0; q < nzq; ++q) zq[q] = z0 + (double)q * dz;
+    for (int q = 0; q < nzq; ++q) zq[q] = z0 + (double)(q + zf - 1) * dz;
     double xl = x0 + (double)(1 - nbd - 1) * dx, xr = x0 + (double)(nx + nbd - 1) * dx;
     double xe = sgn > 0 ? xl : xr;
 
@@ -334,8 +356,8 @@ void orc_comput_background_wavefield(const int *nx_, const int *nz_, const int *
     double tneed = nt * dt + p * (xr - xl) + 2.0 * t0 + 10.0 / *f0_;
     int nfft = next_pow2((long)ceil(2.0 * tneed / dtf));
     fktraces tl = {0}, trr = {0};
-    if (okl) fk_compute(&ml, src_is_p, p, *f0_, *ds_, dtf, nfft, nzq, zq, zref, t0, &tl);
-    if (okr) fk_compute(&mr, src_is_p, p, *f0_, *ds_, dtf, nfft, nzq, zq, zref, t0, &trr);
+    if (okl) fk_compute(&ml, src_is_p, free_surface, p, *f0_, *ds_, dtf, nfft, nzq, zq, zref, t0, &tl);
+    if (okr) fk_compute(&mr, src_is_p, free_surface, p, *f0_, *ds_, dtf, nfft, nzq, zq, zref, t0, &trr);
 
 #define TRACE(tr, q) ((tr).g + (size_t)(q) * nfft * 2)
     for (int it = 0; it <= nt; ++it) {
@@ -346,10 +368,10 @@ void orc_comput_background_wavefield(const int *nx_, const int *nz_, const int *
             double w = (double)(ix - 1) / (double)(nx - 1);
             if (w < 0.0) w = 0.0;
             if (w > 1.0) w = 1.0;
-            for (int iz = 1; iz <= nz + nbd; ++iz) {
-                int q = iz - 1;
-                int inL = ix <= nbd, inR = ix >= nx - nbd + 1, inB = iz >= nz - nbd + 1;
-                if (!inL && !inR && !inB) continue;
+            for (int iz = zf; iz <= nz + nbd; ++iz) {
+                int q = iz - zf;
+                int inL = ix <= nbd, inR = ix >= nx - nbd + 1, inB = iz >= nz - nbd + 1, inT = iz <= nbd && top_ring;
+                if (!inL && !inR && !inB && !inT) continue;
                 double vx[2] = {0, 0}, vz[2] = {0, 0};
                 if (okl) { vx[0] = interp(TRACE(tl, q), nfft, dtf, t - tau); vz[0] = interp(TRACE(tl, q) + 1, nfft, dtf, t - tau); }
                 if (okr) { vx[1] = interp(TRACE(trr, q), nfft, dtf, t - tau); vz[1] = interp(TRACE(trr, q) + 1, nfft, dtf, t - tau); }
@@ -364,9 +386,13 @@ void orc_comput_background_wavefield(const int *nx_, const int *nz_, const int *
                     bx = (1.0 - w) * vx[0] + w * vx[1]; bz = (1.0 - w) * vz[0] + w * vz[1];
                 }
                 /* left/right strips at rows 1..nz use the side model */
-                if (iz <= nz && (ix <= 0 || ix > nx)) {
+                if (iz >= 1 && iz <= nz && (ix <= 0 || ix > nx)) {
                     int side = ix <= 0 ? 0 : 1;
                     bx = vx[side]; bz = vz[side];
+                }
+                /* the top ring blends the two models in x, like the bottom */
+                if (top_ring && (iz <= 0 || (iz <= nbd && ix >= nbd + 1 && ix <= nx - nbd))) {
+                    bx = (1.0 - w) * vx[0] + w * vx[1]; bz = (1.0 - w) * vz[0] + w * vz[1];
                 }
                 orc_real fx = (orc_real)(sgn * bx), fz = (orc_real)bz;
                 /* scatter into the twelve arrays (FDFK2D.f90:330-355 shapes) */
@@ -394,6 +420,17 @@ void orc_comput_background_wavefield(const int *nx_, const int *nz_, const int *
                     size_t o = (size_t)(iz - (nz + 1)) + (size_t)nbd * ((size_t)(ix - (1 - nbd)) + (size_t)(nx + 2 * nbd) * it);
                     uxb_pml[o] = fx; uzb_pml[o] = fz;
                 }
+                /* the top ring: inside rows 1..nbd between the side strips,
+                 * outside rows 1-nbd..0; the two models blended in x as in
+                 * the bottom strips */
+                if (top_ring && iz >= 1 && iz <= nbd && ix >= nbd + 1 && ix <= nx - nbd) {
+                    size_t o = (size_t)(iz - 1) + (size_t)nbd * ((size_t)(ix - (nbd + 1)) + (size_t)(nx - 2 * nbd) * it);
+                    top[0][o] = fx; top[1][o] = fz;
+                }
+                if (top_ring && iz >= 1 - nbd && iz <= 0) {
+                    size_t o = (size_t)(iz - (1 - nbd)) + (size_t)nbd * ((size_t)(ix - (1 - nbd)) + (size_t)(nx + 2 * nbd) * it);
+                    top[2][o] = fx; top[3][o] = fz;
+                }
             }
         }
     }
@@ -401,4 +438,38 @@ void orc_comput_background_wavefield(const int *nx_, const int *nz_, const int *
     free(tl.g); free(trr.g);
     fk_free(&ml); fk_free(&mr);
     free(zq);
+}
+
+void orc_comput_background_wavefield(const int *nx_, const int *nz_, const int *npml_,
+    const int *izt_, const int *nbd_, const int *nt_, const int *nlayer_left,
+    const int *nlayer_right, const int *src_type, const orc_real *x0_, const orc_real *z0_,
+    const orc_real *dx_, const orc_real *dz_, const double *dt_, const orc_real *ds_, const orc_real *f0_,
+    const double *rayp_, const double *bazs, const orc_real *fk_left, const orc_real *fk_right,
+    orc_real *uxl_reg, orc_real *uzl_reg, orc_real *uxr_reg, orc_real *uzr_reg, orc_real *uxb_reg,
+    orc_real *uzb_reg, orc_real *uxl_pml, orc_real *uzl_pml, orc_real *uxr_pml, orc_real *uzr_pml,
+    orc_real *uxb_pml, orc_real *uzb_pml)
+{
+    orc_real *const none[4] = {NULL, NULL, NULL, NULL};
+    background_body(none, nx_, nz_, npml_, izt_, nbd_, nt_, nlayer_left, nlayer_right, src_type, x0_, z0_, dx_,
+                    dz_, dt_, ds_, f0_, rayp_, bazs, fk_left, fk_right, uxl_reg, uzl_reg, uxr_reg, uzr_reg,
+                    uxb_reg, uzb_reg, uxl_pml, uzl_pml, uxr_pml, uzr_pml, uxb_pml, uzb_pml);
+}
+
+/* the same plus the top ring (izt < 1): uxt_reg/uzt_reg (nbd, nx-2nbd, nt+1)
+ * for rows 1..nbd x cols nbd+1..nx-nbd, uxt_pml/uzt_pml (nbd, nx+2nbd, nt+1)
+ * for rows 1-nbd..0 x cols 1-nbd..nx+nbd, z fastest */
+void orc_comput_background_wavefield_ext(orc_real *uxt_reg, orc_real *uzt_reg, orc_real *uxt_pml,
+    orc_real *uzt_pml, const int *nx_, const int *nz_, const int *npml_,
+    const int *izt_, const int *nbd_, const int *nt_, const int *nlayer_left,
+    const int *nlayer_right, const int *src_type, const orc_real *x0_, const orc_real *z0_,
+    const orc_real *dx_, const orc_real *dz_, const double *dt_, const orc_real *ds_, const orc_real *f0_,
+    const double *rayp_, const double *bazs, const orc_real *fk_left, const orc_real *fk_right,
+    orc_real *uxl_reg, orc_real *uzl_reg, orc_real *uxr_reg, orc_real *uzr_reg, orc_real *uxb_reg,
+    orc_real *uzb_reg, orc_real *uxl_pml, orc_real *uzl_pml, orc_real *uxr_pml, orc_real *uzr_pml,
+    orc_real *uxb_pml, orc_real *uzb_pml)
+{
+    orc_real *const top[4] = {uxt_reg, uzt_reg, uxt_pml, uzt_pml};
+    background_body(top, nx_, nz_, npml_, izt_, nbd_, nt_, nlayer_left, nlayer_right, src_type, x0_, z0_, dx_,
+                    dz_, dt_, ds_, f0_, rayp_, bazs, fk_left, fk_right, uxl_reg, uzl_reg, uxr_reg, uzr_reg,
+                    uxb_reg, uzb_reg, uxl_pml, uzl_pml, uxr_pml, uzr_pml, uxb_pml, uzb_pml);
 }

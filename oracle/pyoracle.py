@@ -205,7 +205,10 @@ def updatedispl(st, it):
     L, fr, _ = _real(a["ux"].dtype == np.float64)
     bg = a.get("bg")
     bgp = [_p(x) for x in bg] if bg is not None else [None] * 12
-    L.orc_updatedispl(
+    top = a.get("bg_top")
+    head = [_p(x) for x in top] if (top is not None and a["pml_top"]) else None
+    fn = L.orc_updatedispl_ext if head else L.orc_updatedispl
+    fn(*(head or []),
         _i(a["norder"]), _i(a["nx"]), _i(a["nz"]), _i(a["npml"]), _i(a["izt"]),
         _i(a["pml_top"]), _i(a["npower"]), _i(a["nbd"]), _i(it), _i(a["nt"]),
         fr(a["dx"]), fr(a["dz"]), _d(a["dt"]), fr(a["vpml"]), _d(a["R0"]),
@@ -228,7 +231,8 @@ def pml_ratio():
 def background(nx, nz, npml, izt, nbd, nt, src_is_p, x0, z0, dx, dz, dt, ds, f0, rayp, bazs,
                fk_left, fk_right, fp64=False):
     """Oracle FK background in the reference's twelve-array layout (float64
-    arrays from the FP64 build when fp64)."""
+    arrays from the FP64 build when fp64).  With izt != 1 (an absorbing top
+    layer, no free surface) returns (twelve arrays, top ring's four)."""
     L, f, _ = _real(fp64)
     dt_ = np.float64 if fp64 else np.float32
     shapes = bg_shapes(nx, nz, nbd, nt)
@@ -236,10 +240,16 @@ def background(nx, nz, npml, izt, nbd, nt, src_is_p, x0, z0, dx, dz, dt, ds, f0,
     fl = np.ascontiguousarray(fk_left, dt_)
     fr = np.ascontiguousarray(fk_right, dt_)
     bz = np.ascontiguousarray(bazs, np.float64)
-    L.orc_comput_background_wavefield(
+    top = None
+    if izt != 1:   # an absorbing top layer: the top ring as well
+        top = [np.zeros(s, dt_) for s in bg_top_shapes(nx, nbd, nt)]
+    fn = L.orc_comput_background_wavefield_ext if top else L.orc_comput_background_wavefield
+    fn(*([_p(x) for x in top] if top else []),
         _i(nx), _i(nz), _i(npml), _i(izt), _i(nbd), _i(nt), _i(fl.size // 4), _i(fr.size // 4),
         _i(1 if src_is_p else 0), f(x0), f(z0), f(dx), f(dz), _d(dt), f(ds), f(f0),
         _d(rayp), _p(bz), _p(fl), _p(fr), *[_p(x) for x in arrs])
+    if top:
+        return arrs, top
     return arrs
 
 
@@ -268,12 +278,19 @@ def bg_shapes(nx, nz, nbd, nt):
             pml_lr, pml_lr, pml_lr, pml_lr, pml_b, pml_b]
 
 
-def fk_traces(fkmodel, src_is_p, p, f0, ds, dt, ntq, z, zref, t0):
+def bg_top_shapes(nx, nbd, nt):
+    """Element counts of the top ring's uxt_reg, uzt_reg, uxt_pml, uzt_pml."""
+    T = nt + 1
+    return [nbd * (nx - 2 * nbd) * T] * 2 + [nbd * (nx + 2 * nbd) * T] * 2
+
+
+def fk_traces(fkmodel, src_is_p, p, f0, ds, dt, ntq, z, zref, t0, free_surface=True):
     fk = np.ascontiguousarray(fkmodel, np.float32)
     z = np.ascontiguousarray(z, np.float64)
     ux = np.zeros((z.size, ntq), np.float64)
     uz = np.zeros((z.size, ntq), np.float64)
-    rc = lib().orc_fk_traces(C.c_int(fk.size // 4), _p(fk), C.c_int(1 if src_is_p else 0),
+    rc = lib().orc_fk_traces_fs(C.c_int(fk.size // 4), _p(fk), C.c_int(1 if src_is_p else 0),
+                             C.c_int(1 if free_surface else 0),
                              C.c_double(p), C.c_float(f0), C.c_float(ds), C.c_double(dt),
                              C.c_int(ntq), C.c_int(z.size), _p(z), C.c_double(zref),
                              C.c_double(t0), _p(ux), _p(uz))

@@ -14,14 +14,15 @@ def fk_model(layers):
     return np.asarray(out, np.float32)
 
 
-def layered_grid(nx, nz, npml, dx, dz, layers, anomaly=None, seed=0):
-    """vp, vs on the padded grid (NX, NZ): 1-D layers plus an optional
-    Gaussian Vs anomaly (x_c, z_c, radius, fraction)."""
-    NX, NZ = nx + 2 * npml, nz + npml
+def layered_grid(nx, nz, npml, dx, dz, layers, anomaly=None, seed=0, izt=1):
+    """vp, vs on the padded grid (NX, NZ) of rows izt..nz+npml: 1-D layers
+    (the first one also above its top) plus an optional Gaussian Vs anomaly
+    (x_c, z_c, radius, fraction)."""
+    NX, NZ = nx + 2 * npml, nz + npml - izt + 1
     x = (np.arange(NX) - npml) * dx
-    z = np.arange(NZ) * dz
-    vp = np.zeros((NX, NZ), np.float32)
-    vs = np.zeros((NX, NZ), np.float32)
+    z = (np.arange(NZ) + izt - 1) * dz
+    vp = np.full((NX, NZ), layers[0][1], np.float32)
+    vs = np.full((NX, NZ), layers[0][2], np.float32)
     for zt, p, s in layers:
         m = z >= zt
         vp[:, m] = p
@@ -37,12 +38,14 @@ def layered_grid(nx, nz, npml, dx, dz, layers, anomaly=None, seed=0):
 class Case:
     def __init__(self, nx=121, nz=60, npml=15, norder=3, dx=200.0, dz=200.0, dt=0.01, nt=300,
                  layers=((0.0, 6000.0, 3450.0), (8000.0, 8000.0, 4450.0)), anomaly=(12000.0, 5000.0, 2000.0, -0.08),
-                 src_is_p=True, rayp_sdeg=6.6717, f0=1.0, ds=1e-3, baz=np.pi, nrcv=31):
+                 src_is_p=True, rayp_sdeg=6.6717, f0=1.0, ds=1e-3, baz=np.pi, nrcv=31, pml_top=False):
         self.nx, self.nz, self.npml, self.norder = nx, nz, npml, norder
         self.dx, self.dz, self.dt, self.nt = dx, dz, dt, nt
         self.nbd = 2 * norder - 1
-        self.NX, self.NZ = nx + 2 * npml, nz + npml
-        vp, vs = layered_grid(nx, nz, npml, dx, dz, layers, anomaly)
+        self.pml_top = bool(pml_top)
+        self.izt = 1 - npml if pml_top else 1
+        self.NX, self.NZ = nx + 2 * npml, nz + npml - self.izt + 1
+        vp, vs = layered_grid(nx, nz, npml, dx, dz, layers, anomaly, izt=self.izt)
         self.c = refstate.material_law(vp, vs)
         self.fk = fk_model(layers)
         self.src_is_p = src_is_p
@@ -54,11 +57,11 @@ class Case:
 
     def state(self):
         return refstate.make_state(self.nx, self.nz, self.npml, self.norder, self.dx, self.dz,
-                                   self.dt, self.nt, self.c)
+                                   self.dt, self.nt, self.c, pml_top=self.pml_top)
 
     def oracle_background(self):
         from oracle import pyoracle as po
-        return po.background(self.nx, self.nz, self.npml, 1, self.nbd, self.nt, self.src_is_p,
+        return po.background(self.nx, self.nz, self.npml, self.izt, self.nbd, self.nt, self.src_is_p,
                              0.0, 0.0, self.dx, self.dz, self.dt, self.ds, self.f0, self.rayp,
                              self.bazs, self.fk, self.fk)
 
@@ -74,11 +77,14 @@ def oracle_run(case, st, bg, nsteps, it0=0, record=True):
     before the update (src/FDFK2D.f90:497-522).  A state of float64 arrays
     runs the FP64 build and records in float64."""
     from oracle import pyoracle as po
-    st["bg"] = bg
+    if isinstance(bg, tuple):   # the twelve arrays and the top ring (absorbing top layer)
+        st["bg"], st["bg_top"] = bg
+    else:
+        st["bg"] = bg
     sx, sz = [], []
     for it in range(it0, it0 + nsteps):
         if record:
-            row = [po.loadseis(case.nx, case.nz, case.npml, 1, 0.0, 0.0, case.dx, case.dz,
+            row = [po.loadseis(case.nx, case.nz, case.npml, getattr(case, "izt", 1), 0.0, 0.0, case.dx, case.dz,
                                float(rx), float(rz), st["ux"], st["uz"])
                    for rx, rz in zip(case.rx, case.rz)]
             sx.append([r[0] for r in row])
