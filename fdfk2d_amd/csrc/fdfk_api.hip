@@ -11,6 +11,7 @@
 #include "fdfk_kernels.hip"
 #include "fdfk_frame.hip"
 #include "fdfk_interior2.hip"
+#include "fdfk_general.hip"
 
 #include <algorithm>
 #include <cmath>
@@ -244,6 +245,20 @@ struct fdfk_solver {
     int d_lx = 0, d_nbx = 0, d_nbz = 0, d_tz = 0;
     Geo gband[4]{};
     int band_lx[4] = {0, 0, 0, 0};
+    // General per-cell path (fdfk_general.hip): the absorbing top layer
+    // (is_pml_top, rows from zt = 1-npml).  g.NZ and g.ld then count the top
+    // rows; the background arrays, ring fields, receivers and seismograms
+    // are the ones above, the fields, materials and strips these.
+    bool gen = false;
+    int zt = 1;
+    float *gux[2] = {nullptr, nullptr}, *guz[2] = {nullptr, nullptr};
+    float *grho = nullptr, *glam = nullptr, *gmu = nullptr;
+    float *gbu[4][2][3] = {};          // split parts (1:2, cells), reference order
+    float *gbpx[4][2] = {}, *gbpz[4][2] = {};
+    double *gc0 = nullptr, *gc1 = nullptr;
+    float *bg_top[4] = {};             // top ring: uxt_reg, uzt_reg, uxt_pml, uzt_pml
+    long bg_top_n[4] = {};
+    bool bg_top_valid = false;         // without it the hybrid contour is open at the top
     // profiling
     bool prof = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_int, ev_step;
@@ -299,8 +314,8 @@ int build_damping(fdfk_solver *s)
         s->h_bx[j] = (float)std::exp(-(double)s->h_dx[j] * g.dt);
     }
     for (int k = 0; k < g.NZ; ++k) {
-        int iz = k + 1;
-        s->h_dz[k] = iz > nz ? (float)prof(iz - nz, d0z) : 0.0f;
+        int iz = k + s->zt;   // rows from zt: the top strip mirrors the bottom one
+        s->h_dz[k] = iz > nz ? (float)prof(iz - nz, d0z) : (iz <= 0 ? (float)prof(1 - iz, d0z) : 0.0f);
         s->h_bz[k] = (float)std::exp(-(double)s->h_dz[k] * g.dt);
     }
     int rc;
@@ -821,6 +836,69 @@ int one_step(fdfk_solver *s, bool record, int half = 0)
     return 0;
 }
 
+// ---------------------------------------------------------------------
+// General per-cell path (absorbing top layer)
+// ---------------------------------------------------------------------
+GenArgs<float> gen_args(fdfk_solver *s)
+{
+    GenArgs<float> a{};
+    a.g = s->g;
+    a.zt = s->zt;
+    a.top = s->zt != 1;
+    a.c0 = s->gc0; a.c1 = s->gc1;
+    a.ux = s->gux[s->cur]; a.uz = s->guz[s->cur];
+    a.uxp = s->gux[1 - s->cur]; a.uzp = s->guz[1 - s->cur];
+    a.uxo = s->gux[1 - s->cur]; a.uzo = s->guz[1 - s->cur];
+    a.rho = s->grho; a.lam = s->glam; a.mu = s->gmu;
+    for (int st = 0; st < 4; ++st)
+        for (int c = 0; c < 2; ++c) {
+            for (int q = 0; q < 3; ++q) a.bu[st][c][q] = s->gbu[st][c][q];
+            a.bpx[st][c] = s->gbpx[st][c];
+            a.bpz[st][c] = s->gbpz[st][c];
+        }
+    a.dpx = s->dpx; a.dpz = s->dpz; a.bx = s->bx; a.bz = s->bz;
+    a.ratio = s->pml_ratio;
+    a.ringx = s->ringx; a.ringz = s->ringz;
+    a.ring_valid = s->bg_valid && s->it <= s->nt;
+    a.top_ring = s->bg_top_valid;
+    for (int i = 0; i < 12; ++i) a.bg[i] = s->bg[i];
+    for (int i = 0; i < 4; ++i) a.bg[12 + i] = s->bg_top[i];
+    a.it = s->bg_period > 0 ? s->it % s->bg_period : 0;
+    return a;
+}
+
+// One iteration of the general path: prep (receivers before the update,
+// ring values, memories), then the cell update; out of place into level
+// n-1, then the levels trade places.
+int gen_step(fdfk_solver *s, bool record)
+{
+    const Geo &g = s->g;
+    if (int rc = bg_refill(s, s->stream)) return rc;
+    GenArgs<float> a = gen_args(s);
+    GenRec<float> r{};
+    const bool rec = record && s->nrcv > 0 && s->istp <= s->nsis;
+    if (rec) {
+        r.nrcv = s->nrcv; r.nsamp = s->nsamp; r.istp = s->istp;
+        r.x0 = s->x0; r.z0 = s->z0;
+        r.rx = s->rx; r.rz = s->rz;
+        r.seisx = s->seisx; r.seisz = s->seisz;
+    }
+    const int zr = a.top_ring ? 1 - g.nbd : 1;
+    const long nrec = rec ? s->nrcv : 0;
+    const long nring = s->ringx ? (long)(g.nx + 2 * g.nbd) * (g.nz + g.nbd - zr + 1) : 0;
+    const long npsi = 2L * g.npml * g.nz + (s->zt != 1 ? 2L : 1L) * g.NX * g.npml;
+    const long np = nrec + nring + npsi;
+    hipLaunchKernelGGL(k_gen_prep<float>, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s->stream, a, r, nrec,
+                       nring, zr, s->ringx, s->ringz);
+    const long nc = (long)g.NX * g.NZ;
+    hipLaunchKernelGGL(k_gen_step<float>, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s->stream, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(FDFK_EDEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
+    s->cur = 1 - s->cur;
+    s->it += 1;
+    return 0;
+}
+
 // geometry of a job: the checks the background alone needs
 int validate_geometry(int nx, int nz, int npml, int norder, float dx, float dz, double dt, int nt)
 {
@@ -837,7 +915,8 @@ int validate_geometry(int nx, int nz, int npml, int norder, float dx, float dz, 
 int validate(const fdfk_job *j)
 {
     if (!j) return fail(FDFK_EINVAL, "null job");
-    if (j->is_pml_top) return fail(FDFK_ENOSYS, "is_PML_top = .true. is not supported (free surface only)");
+    if (j->is_pml_top && j->precision != FDFK_FP32)
+        return fail(FDFK_ENOSYS, "an absorbing top layer (is_pml_top) runs in FP32 storage only");
     if (int rc = validate_geometry(j->nx, j->nz, j->npml, j->norder, j->dx, j->dz, j->dt, j->nt)) return rc;
     if (j->nstp < 1) return fail(FDFK_EINVAL, "nstp >= 1 required");
     if (j->nrcv < 0 || (j->nrcv > 0 && (!j->rx || !j->rz))) return fail(FDFK_EINVAL, "receiver arrays");
@@ -862,6 +941,7 @@ int fdfk_autotune(const fdfk_job *job, int32_t steps, int32_t *best_lx, float *b
                   float *model_ms)
 {
     if (!job || steps < 1) return fail(FDFK_EINVAL, "job required and steps >= 1");
+    if (job->is_pml_top) return fail(FDFK_ENOSYS, "the tile autotuner covers the free-surface kernels only");
     // scratch solver of the same geometry, set up for every timed step
     std::vector<int> cand;
     fdfk_job j = *job;
@@ -941,12 +1021,110 @@ int fdfk_trim_memory(void)
     return FDFK_OK;
 }
 
+// A solver of the general path: the absorbing top layer (is_pml_top, rows
+// from zt = 1-npml); c holds rows zt..nz+npml
+static int gen_create(const fdfk_job *job, fdfk_solver **out)
+{
+    auto *s = new fdfk_solver();
+    auto bad = [&](int code) { delete s; return code; };
+    int rc = 0;
+    s->gen = true;
+    s->zt = 1 - job->npml;
+    Geo &g = s->g;
+    g.nx = job->nx; g.nz = job->nz; g.npml = job->npml; g.norder = job->norder;
+    g.nbd = 2 * g.norder - 1;
+    g.NX = g.nx + 2 * g.npml; g.NZ = g.nz + g.npml - s->zt + 1;
+    g.ld = (g.NZ + 31) / 32 * 32;
+    g.dx = job->dx; g.dz = job->dz; g.dt = job->dt;
+    s->nt = job->nt; s->nstp = job->nstp;
+    s->nsis = job->nt / job->nstp;
+    s->nsamp = s->nsis + 1;
+    s->nrcv = job->nrcv;
+    s->x0 = job->x0; s->z0 = job->z0;
+    s->R0 = job->R0; s->npower = job->npower;
+    s->fused = false;
+    if (const char *e = std::getenv("FDFK_PML_RATIO")) s->pml_ratio = (float)std::atof(e);
+    const long ncell = (long)g.NX * g.NZ;
+    float vpml = job->vpml;
+    if (!(vpml > 0)) {
+        vpml = 0.0f;
+        for (long i = 0; i < ncell; ++i) {
+            float v = std::sqrt((job->c[3 * i + 1] + 2.0f * job->c[3 * i + 2]) / job->c[3 * i]);
+            if (v > vpml) vpml = v;
+        }
+    }
+    s->vpml = vpml;
+    coef_table(g.norder, true, s->hcoef.c0);
+    coef_table(g.norder, false, s->hcoef.c1);
+    if (hipGetDevice(&s->device) != hipSuccess || hipStreamCreate(&s->stream) != hipSuccess)
+        return bad(fail(FDFK_EDEVICE, "hipStreamCreate failed"));
+    const size_t nf = (size_t)g.NX * g.ld;
+    for (int q = 0; q < 2; ++q) {
+        if ((rc = s->dalloc(&s->gux[q], nf)) || (rc = s->dalloc(&s->guz[q], nf))) return bad(rc);
+        if (hipMemsetAsync(s->gux[q], 0, nf * 4, s->stream) || hipMemsetAsync(s->guz[q], 0, nf * 4, s->stream))
+            return bad(fail(FDFK_EDEVICE, "memset"));
+    }
+    if ((rc = s->dalloc(&s->grho, nf)) || (rc = s->dalloc(&s->glam, nf)) || (rc = s->dalloc(&s->gmu, nf)) ||
+        (rc = s->dalloc(&s->gc0, (size_t)kMaxOrder * kMaxOrder)) || (rc = s->dalloc(&s->gc1, (size_t)kMaxOrder * kMaxOrder)) ||
+        (rc = s->dalloc(&s->stage, (size_t)2 * ncell)))
+        return bad(rc);
+    if (h2d(s->stream, s->gc0, s->hcoef.c0, sizeof(s->hcoef.c0)) != hipSuccess ||
+        h2d(s->stream, s->gc1, s->hcoef.c1, sizeof(s->hcoef.c1)) != hipSuccess)
+        return bad(fail(FDFK_EDEVICE, "coef upload"));
+    {
+        float *dc = nullptr;
+        if (malloc_async_retry((void **)&dc, (size_t)3 * ncell * 4, s->stream) != hipSuccess)
+            return bad(fail(FDFK_ENOMEM, "staging"));
+        if (h2d(s->stream, dc, job->c, (size_t)3 * ncell * 4) != hipSuccess) {
+            hipFreeAsync(dc, s->stream);
+            return bad(fail(FDFK_EDEVICE, "material upload"));
+        }
+        hipLaunchKernelGGL(k_gen_import_material<float>, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0,
+                           s->stream, g, s->zt, dc, s->grho, s->glam, s->gmu);
+        hipFreeAsync(dc, s->stream);
+    }
+    // strips L, R, B, T: split parts with two interleaved levels, x-memories,
+    // z-memories in the bottom and top strips
+    const size_t ncs[4] = {(size_t)g.npml * g.nz, (size_t)g.npml * g.nz, (size_t)g.NX * g.npml, (size_t)g.NX * g.npml};
+    for (int st = 0; st < 4; ++st)
+        for (int c = 0; c < 2; ++c) {
+            for (int q = 0; q < 3; ++q) {
+                if ((rc = s->dalloc(&s->gbu[st][c][q], 2 * ncs[st]))) return bad(rc);
+                hipMemsetAsync(s->gbu[st][c][q], 0, 2 * ncs[st] * 4, s->stream);
+            }
+            if ((rc = s->dalloc(&s->gbpx[st][c], ncs[st]))) return bad(rc);
+            hipMemsetAsync(s->gbpx[st][c], 0, ncs[st] * 4, s->stream);
+            if (st >= 2) {
+                if ((rc = s->dalloc(&s->gbpz[st][c], ncs[st]))) return bad(rc);
+                hipMemsetAsync(s->gbpz[st][c], 0, ncs[st] * 4, s->stream);
+            }
+        }
+    if ((rc = build_damping(s))) return bad(rc);
+    if (s->nrcv > 0) {
+        if ((rc = s->dalloc(&s->rx, s->nrcv)) || (rc = s->dalloc(&s->rz, s->nrcv)) ||
+            (rc = s->dalloc(&s->seisx, (size_t)s->nrcv * s->nsamp)) ||
+            (rc = s->dalloc(&s->seisz, (size_t)s->nrcv * s->nsamp)))
+            return bad(rc);
+        if (h2d(s->stream, s->rx, job->rx, s->nrcv * 4) != hipSuccess ||
+            h2d(s->stream, s->rz, job->rz, s->nrcv * 4) != hipSuccess)
+            return bad(fail(FDFK_EDEVICE, "receiver upload"));
+        hipMemsetAsync(s->seisx, 0, (size_t)s->nrcv * s->nsamp * 4, s->stream);
+        hipMemsetAsync(s->seisz, 0, (size_t)s->nrcv * s->nsamp * 4, s->stream);
+    }
+    hipError_t e = hipStreamSynchronize(s->stream);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e != hipSuccess) return bad(fail(FDFK_EDEVICE, std::string("setup: ") + hipGetErrorString(e)));
+    *out = s;
+    return 0;
+}
+
 int fdfk_create(const fdfk_job *job, fdfk_solver **out)
 {
     if (!out) return fail(FDFK_EINVAL, "null output pointer");
     *out = nullptr;
     int rc = validate(job);
     if (rc) return rc;
+    if (job->is_pml_top) return gen_create(job, out);
     auto *s = new fdfk_solver();
     Geo &g = s->g;
     g.nx = job->nx; g.nz = job->nz; g.npml = job->npml; g.norder = job->norder;
@@ -1138,6 +1316,18 @@ static int alloc_background(fdfk_solver *s, int period)
         s->bg_n[i] = n[i];
         h.a[i] = s->bg[i];
     }
+    if (s->zt != 1) {
+        // top ring: inside rows 1..nbd x cols nbd+1..nx-nbd, outside rows
+        // 1-nbd..0 x cols 1-nbd..nx+nbd, step slowest
+        const long nin = (long)s->g.nbd * (s->g.nx - 2 * s->g.nbd) * period;
+        const long nout = (long)s->g.nbd * (s->g.nx + 2 * s->g.nbd) * period;
+        const long nt4[4] = {nin, nin, nout, nout};
+        for (int i = 0; i < 4; ++i) {
+            int rc = s->dalloc(&s->bg_top[i], (size_t)nt4[i]);
+            if (rc) return rc;
+            s->bg_top_n[i] = nt4[i];
+        }
+    }
     h.valid = 1;
     h.period = period;
     s->bg_period = period;
@@ -1184,10 +1374,46 @@ int fdfk_get_background(fdfk_solver *s, float *const bg[12])
     return 0;
 }
 
+int fdfk_set_top_background(fdfk_solver *s, const float *const top[4])
+{
+    if (!s || !top) return fail(FDFK_EINVAL, "null argument");
+    if (s->zt == 1) return fail(FDFK_EINVAL, "the top ring exists with an absorbing top layer (is_pml_top) only");
+    int rc = alloc_background(s, s->nt + 1);
+    if (rc) return rc;
+    for (int i = 0; i < 4; ++i) {
+        if (!top[i]) return fail(FDFK_EINVAL, "null background array");
+        HIPCHK(hipMemcpyAsync(s->bg_top[i], top[i], (size_t)s->bg_top_n[i] * 4, hipMemcpyHostToDevice, s->stream));
+    }
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->bg_top_valid = true;
+    return 0;
+}
+
+int fdfk_get_top_background(fdfk_solver *s, float *const top[4])
+{
+    if (!s || !top) return fail(FDFK_EINVAL, "null argument");
+    if (!s->bg_top_valid) return fail(FDFK_EINVAL, "no top ring background on the device");
+    if (s->bg_period != s->nt + 1)
+        return fail(FDFK_EINVAL, "the background is synthesised in blocks; its steps are not all stored");
+    for (int i = 0; i < 4; ++i)
+        HIPCHK(hipMemcpyAsync(top[i], s->bg_top[i], (size_t)s->bg_top_n[i] * 4, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
 int fdfk_run(fdfk_solver *s, int32_t nsteps)
 {
     if (!s) return fail(FDFK_EINVAL, "null solver");
     if (nsteps < 0) return fail(FDFK_EINVAL, "nsteps < 0");
+    if (s->gen) {
+        for (int k = 0; k < nsteps; ++k) {
+            if (s->it > s->nt) return fail(FDFK_ERANGE, "all iterations done");
+            const bool rec = (s->isis == s->it) && (s->istp <= s->nsis);
+            if (int rc = gen_step(s, rec)) return rc;
+            if (rec) { s->istp += 1; s->isis += s->nstp; }
+        }
+        return 0;
+    }
     for (int k = 0; k < nsteps; ++k) {
         if (s->it > s->nt) return fail(FDFK_ERANGE, "all iterations done");
         // a pair never spans two calls: between calls both live levels are whole
@@ -1229,6 +1455,16 @@ int fdfk_get_state(fdfk_solver *s, float *ux, float *uz)
     const Geo &g = s->g;
     long n = (long)g.NX * g.NZ;
     dim3 gr((unsigned)((n + 255) / 256)), bl(256);
+    if (s->gen) {
+        for (int c = 0; c < 2; ++c) {
+            float *const *f = c == 0 ? s->gux : s->guz;
+            hipLaunchKernelGGL((k_gen_levels<float, float>), gr, bl, 0, s->stream, g, 1, s->stage, f[s->cur],
+                               f[1 - s->cur]);
+            HIPCHK(hipMemcpyAsync(c == 0 ? ux : uz, s->stage, (size_t)2 * n * 4, hipMemcpyDeviceToHost, s->stream));
+            HIPCHK(hipStreamSynchronize(s->stream));
+        }
+        return 0;
+    }
     hipLaunchKernelGGL(k_export_levels, gr, bl, 0, s->stream, g, s->ux[s->cur], s->ux[1 - s->cur], s->stage);
     HIPCHK(hipMemcpyAsync(ux, s->stage, (size_t)2 * n * 4, hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
@@ -1244,6 +1480,16 @@ int fdfk_set_state(fdfk_solver *s, const float *ux, const float *uz)
     const Geo &g = s->g;
     long n = (long)g.NX * g.NZ;
     dim3 gr((unsigned)((n + 255) / 256)), bl(256);
+    if (s->gen) {
+        for (int c = 0; c < 2; ++c) {
+            float *const *f = c == 0 ? s->gux : s->guz;
+            HIPCHK(hipMemcpyAsync(s->stage, c == 0 ? ux : uz, (size_t)2 * n * 4, hipMemcpyHostToDevice, s->stream));
+            hipLaunchKernelGGL((k_gen_levels<float, float>), gr, bl, 0, s->stream, g, 0, s->stage, f[s->cur],
+                               f[1 - s->cur]);
+            HIPCHK(hipStreamSynchronize(s->stream));
+        }
+        return 0;
+    }
     HIPCHK(hipMemcpyAsync(s->stage, ux, (size_t)2 * n * 4, hipMemcpyHostToDevice, s->stream));
     hipLaunchKernelGGL(k_import_levels, gr, bl, 0, s->stream, g, s->stage, s->ux[s->cur], s->ux[1 - s->cur]);
     HIPCHK(hipStreamSynchronize(s->stream));

@@ -125,7 +125,8 @@ struct FKRun {
     double T, eps, dtf;
     double f0, ds;
     int src_is_p;
-    int nzq;               // ring depths z0 + q dz, q = 0..nzq-1
+    int nzq;               // ring depths z0 + (q + zf - 1) dz, q = 0..nzq-1
+    int free_surface;      // 1: traction-free top; 0: the top layer extends upward (absorbing top layer)
 };
 
 // per frequency: amplitudes of all layers, amps[(k*nl + j)*4 + c]
@@ -158,11 +159,16 @@ __global__ void k_fk_amp(const FKModel *mp, FKRun r, zc *amps, zc *work)
                 ab[(j * 3 + b) * 4 + rr] = inv[rr] * s;
             }
     }
+    // the two top conditions of each basis vector: zero traction at the
+    // surface, or (no free surface) no down-going P and S in the top layer
     zc T[3][2];
     for (int b = 0; b < 3; ++b)
         for (int rr = 0; rr < 2; ++rr) {
             zc s = zmk(0, 0);
-            for (int c = 0; c < 4; ++c) s = s + m.E[0][2 + rr][c] * ab[(0 * 3 + b) * 4 + c];
+            if (r.free_surface)
+                for (int c = 0; c < 4; ++c) s = s + m.E[0][2 + rr][c] * ab[(0 * 3 + b) * 4 + c];
+            else
+                s = ab[(0 * 3 + b) * 4 + rr];
             T[b][rr] = s;
         }
     zc det = T[1][0] * T[2][1] - T[2][0] * T[1][1];
@@ -232,8 +238,9 @@ struct FillArgs {
     double dt, dtf, x0, dx, p, sgn, xe;
     const double *gl, *gr;      // time traces [comp][q][nfft] of the left / right model
     float *ax, *az;             // destination arrays (one strip)
-    int strip;                  // 0 L reg, 1 R reg, 2 B reg, 3 L pml, 4 R pml, 5 B pml
+    int strip;                  // 0 L reg, 1 R reg, 2 B reg, 3 L pml, 4 R pml, 5 B pml, 6 T reg, 7 T pml
     int rows, cols, row0, col0; // strip shape: rows row0.., columns col0..
+    int zf;                     // row of trace depth index 0
 };
 
 __global__ void k_fk_fill(FillArgs f)
@@ -247,7 +254,7 @@ __global__ void k_fk_fill(FillArgs f)
     double x = f.x0 + (double)(ix - 1) * f.dx;
     double tau = f.p * (f.sgn > 0 ? (x - f.xe) : (f.xe - x));
     double tt = it * f.dt - tau;
-    int q = iz - 1;
+    int q = iz - f.zf;
     size_t nz2 = (size_t)f.nzq * f.nfft;
     const double *gxl = f.gl + (size_t)q * f.nfft, *gzl = f.gl + nz2 + (size_t)q * f.nfft;
     const double *gxr = f.gr + (size_t)q * f.nfft, *gzr = f.gr + nz2 + (size_t)q * f.nfft;
@@ -463,9 +470,13 @@ extern "C" int fdfk_compute_background(fdfk_solver *s, const fdfk_fk *fk)
     FKModel ml{}, mr{};
     if (!fk_model_setup(ml, fk->nlayer_left, fk->fk_left, p) || !fk_model_setup(mr, fk->nlayer_right, fk->fk_right, p))
         return fail(FDFK_EINVAL, "invalid FK model (need 1..16 solid layers, vp > vs > 0)");
-    const int nzq = nz + nbd;
+    // depths of the ring rows zf..nz+nbd; an absorbing top layer adds the
+    // top ring (rows 1-nbd..nbd), above the model top the top layer continues
+    const bool top = s->zt != 1;
+    const int zf = top ? 1 - nbd : 1;
+    const int nzq = nz + nbd - zf + 1;
     std::vector<double> zq(nzq);
-    for (int q = 0; q < nzq; ++q) zq[q] = (double)s->z0 + (double)q * g.dz;
+    for (int q = 0; q < nzq; ++q) zq[q] = (double)s->z0 + (double)(q + zf - 1) * g.dz;
     const double xl = (double)s->x0 + (double)(1 - nbd - 1) * g.dx, xr = (double)s->x0 + (double)(nx + nbd - 1) * g.dx;
     const double xe = sgn > 0 ? xl : xr;
     const double zref = std::max(std::max(zq[nzq - 1], ml.zt[ml.nl - 1]), std::max(zq[nzq - 1], mr.zt[mr.nl - 1]));
@@ -477,6 +488,7 @@ extern "C" int fdfk_compute_background(fdfk_solver *s, const fdfk_fk *fk)
     r.T = r.nfft * r.dtf;
     r.eps = 27.6 / r.T;
     r.f0 = f0; r.ds = fk->ds; r.src_is_p = fk->src_is_p ? 1 : 0; r.nzq = nzq;
+    r.free_surface = top ? 0 : 1;
     ml.tref = t0 + (r.src_is_p ? ml.ea[ml.nl - 1].r : ml.eb[ml.nl - 1].r) * (zref - ml.zt[ml.nl - 1]);
     mr.tref = t0 + (r.src_is_p ? mr.ea[mr.nl - 1].r : mr.eb[mr.nl - 1].r) * (zref - mr.zt[mr.nl - 1]);
     const bool same = fk->nlayer_left == fk->nlayer_right &&
@@ -509,16 +521,21 @@ extern "C" int fdfk_compute_background(fdfk_solver *s, const fdfk_fk *fk)
     f.g = g; f.period = period; f.nfft = r.nfft; f.nzq = nzq; f.dt = dt; f.dtf = r.dtf;
     f.x0 = s->x0; f.dx = g.dx; f.p = p; f.sgn = sgn; f.xe = xe;
     f.gl = tl; f.gr = same ? tl : tr;
-    // strip shapes: (rows, cols, row0, col0) of the twelve arrays, FDFK2D.f90:330-355
-    const std::array<std::array<int, 4>, 6> shp = {{{nz - nbd, nbd, 1, 1}, {nz - nbd, nbd, 1, nx - nbd + 1},
+    f.zf = zf;
+    // strip shapes: (rows, cols, row0, col0) of the twelve arrays, FDFK2D.f90:330-355,
+    // then the top ring's inside and outside strips
+    const std::array<std::array<int, 4>, 8> shp = {{{nz - nbd, nbd, 1, 1}, {nz - nbd, nbd, 1, nx - nbd + 1},
                                                    {nbd, nx, nz - nbd + 1, 1}, {nz, nbd, 1, 1 - nbd},
-                                                   {nz, nbd, 1, nx + 1}, {nbd, nx + 2 * nbd, nz + 1, 1 - nbd}}};
-    std::array<float *, 12> dst;
+                                                   {nz, nbd, 1, nx + 1}, {nbd, nx + 2 * nbd, nz + 1, 1 - nbd},
+                                                   {nbd, nx - 2 * nbd, 1, nbd + 1}, {nbd, nx + 2 * nbd, 1 - nbd, 1 - nbd}}};
+    std::array<float *, 16> dst{};
     for (int i = 0; i < 12; ++i) dst[i] = s->bg[i];
-    auto fill = [f, shp, dst](hipStream_t st, int t0, int t1) -> int {
+    for (int i = 0; i < 4; ++i) dst[12 + i] = s->bg_top[i];
+    const int nstrip = top ? 8 : 6;
+    auto fill = [f, shp, dst, nstrip](hipStream_t st, int t0, int t1) -> int {
         FillArgs a = f;
         a.t0 = t0; a.t1 = t1;
-        for (int k = 0; k < 6; ++k) {
+        for (int k = 0; k < nstrip; ++k) {
             a.strip = k;
             a.rows = shp[k][0]; a.cols = shp[k][1]; a.row0 = shp[k][2]; a.col0 = shp[k][3];
             a.ax = dst[2 * k]; a.az = dst[2 * k + 1];
@@ -536,6 +553,7 @@ extern "C" int fdfk_compute_background(fdfk_solver *s, const fdfk_fk *fk)
     HIPCHK(hipStreamSynchronize(s->stream));
     HIPCHK(hipGetLastError());
     s->bg_valid = true;
+    s->bg_top_valid = top;
     return 0;
 }
 

@@ -1,0 +1,544 @@
+// ---------------------------------------------------------------------
+// General per-cell step (included by fdfk_api.hip after the other kernel
+// files).  It serves the absorbing top layer (is_PML_top = .true., izt =
+// 1-npml, src/Input.f90:179-183), which the tiled kernels do not: every
+// updated cell evaluates the six right-hand-side parts with the pointwise
+// operators of src/Submain.f90 (reduced orders by the distance to the
+// padded edges, rows counted from izt), so no operator pass needs a row
+// above the first one.  Under a top layer the reference's passes are not
+// usable as written: diffz2_reg and diffz2_pml leave row 1 of columns 1..nx
+// without z-terms (Submain.f90:971,1062-1063) and diffxz_reg builds its
+// intermediate from row 1 down (:616).
+//
+// Layout: fields, materials and the ring fields are [NX][ld] with rows
+// zt..nz+npml (cell (iz, ix) at (ix+npml-1)*ld + iz-zt); split parts and
+// memories keep the reference's strip arrays, the split parts with their
+// two levels interleaved (1:2, cells) and updated in place.
+//
+// Arithmetic follows the formulation term by term with contraction off and
+// FP64 accumulation of every coefficient product (a REAL(4) sum with a
+// REAL(8) coefficient), so a run in T = float reproduces the FP32 CPU
+// formulation bit for bit and T = double its FP64 build.
+//
+// Launches per step: k_gen_prep (receivers, hybrid ring values, PML
+// memories, all from level n), then k_gen_step (every cell of level n+1).
+// ---------------------------------------------------------------------
+namespace fdfk {
+
+template <class T>
+struct GenArgs {
+    Geo g;                      // NZ rows from zt; ld the column pitch
+    int zt;                     // first row: 1 (free surface) or 1-npml (top layer)
+    int top;                    // the top strip exists (is_pml_top)
+    const double *c0, *c1;      // fd_coef0 / fd_coef1, (norder, norder) column-major
+    const T *ux, *uz;           // level n
+    const T *uxp, *uzp;         // level n-1
+    T *uxo, *uzo;               // level n+1 (may alias uxp, uzp)
+    const T *rho, *lam, *mu;    // material planes (damped cells repeat the undamped edge)
+    T *bu[4][2][3];             // split parts [strip L R B T][comp][part], (1:2, cells)
+    T *bpx[4][2];               // x-memories [strip][comp]
+    T *bpz[4][2];               // z-memories (bottom and top strips)
+    const T *dpx, *dpz;         // classical damping profiles per column / row
+    const T *bx, *bz;           // exp(-d dt) of those profiles
+    T ratio;                    // multiaxial ratio
+    const T *ringx, *ringz;     // background on the ring cells at this step, 0 elsewhere
+    int ring_valid;
+    int top_ring;               // the top ring's background is present
+    const T *bg[16];            // twelve reference arrays + uxt_reg, uzt_reg, uxt_pml, uzt_pml
+    int it;
+};
+
+template <class T>
+__device__ __forceinline__ size_t gcell(const GenArgs<T> &a, int iz, int ix)
+{
+    return (size_t)(ix + a.g.npml - 1) * (size_t)a.g.ld + (size_t)(iz - a.zt);
+}
+__device__ __forceinline__ int gmin(int a, int b) { return a < b ? a : b; }
+__device__ __forceinline__ int gmax(int a, int b) { return a > b ? a : b; }
+__device__ __forceinline__ double gco(const double *c, int norder, int j, int n) { return c[(j - 1) + norder * (n - 1)]; }
+// REAL(T) accumulation with a REAL(8) coefficient
+template <class T>
+__device__ __forceinline__ T gacc(T s, double c, T d)
+{
+#pragma clang fp contract(off)
+    return (T)((double)s + c * (double)d);
+}
+template <class T>
+__device__ __forceinline__ T ghmean(T a, T b)
+{
+#pragma clang fp contract(off)
+    return (T)2.0f / ((T)1.0f / a + (T)1.0f / b);
+}
+
+// strip of a padded cell: 0 L, 1 R, 2 B, 3 T, -1 none
+template <class T>
+__device__ __forceinline__ int gstrip(const GenArgs<T> &a, int iz, int ix)
+{
+    if (iz >= 1 && iz <= a.g.nz) return ix <= 0 ? 0 : (ix > a.g.nx ? 1 : -1);
+    return iz > a.g.nz ? 2 : 3;
+}
+template <class T>
+__device__ __forceinline__ size_t gstrip_off(const GenArgs<T> &a, int s, int iz, int ix)
+{
+    const Geo &g = a.g;
+    switch (s) {
+    case 0: return (size_t)(iz - 1) + (size_t)g.nz * (size_t)(ix - (1 - g.npml));
+    case 1: return (size_t)(iz - 1) + (size_t)g.nz * (size_t)(ix - (g.nx + 1));
+    case 2: return (size_t)(iz - (g.nz + 1)) + (size_t)g.npml * (size_t)(ix - (1 - g.npml));
+    default: return (size_t)(iz - (1 - g.npml)) + (size_t)g.npml * (size_t)(ix - (1 - g.npml));
+    }
+}
+
+// Ring cell lookup (the reference's twelve arrays, src/FDFK2D.f90:330-355,
+// plus the top ring): array index 0..15 of the x component (z is the next
+// one) and the offset of step it, or -1.  inside: the cell is in the
+// total-field region.
+template <class T>
+__device__ __forceinline__ long gring(const GenArgs<T> &a, int iz, int ix, int it, int &arr)
+{
+    const int nx = a.g.nx, nz = a.g.nz, nbd = a.g.nbd;
+    if (iz >= 1 && iz <= nz - nbd && ix >= 1 && ix <= nbd) {
+        arr = 0;
+        return (long)(iz - 1) + (long)(nz - nbd) * ((long)(ix - 1) + (long)nbd * it);
+    }
+    if (iz >= 1 && iz <= nz - nbd && ix >= nx - nbd + 1 && ix <= nx) {
+        arr = 2;
+        return (long)(iz - 1) + (long)(nz - nbd) * ((long)(ix - (nx - nbd + 1)) + (long)nbd * it);
+    }
+    if (iz >= nz - nbd + 1 && iz <= nz && ix >= 1 && ix <= nx) {
+        arr = 4;
+        return (long)(iz - (nz - nbd + 1)) + (long)nbd * ((long)(ix - 1) + (long)nx * it);
+    }
+    if (iz >= 1 && iz <= nz && ix >= 1 - nbd && ix <= 0) {
+        arr = 6;
+        return (long)(iz - 1) + (long)nz * ((long)(ix - (1 - nbd)) + (long)nbd * it);
+    }
+    if (iz >= 1 && iz <= nz && ix >= nx + 1 && ix <= nx + nbd) {
+        arr = 8;
+        return (long)(iz - 1) + (long)nz * ((long)(ix - (nx + 1)) + (long)nbd * it);
+    }
+    if (iz >= nz + 1 && iz <= nz + nbd && ix >= 1 - nbd && ix <= nx + nbd) {
+        arr = 10;
+        return (long)(iz - (nz + 1)) + (long)nbd * ((long)(ix - (1 - nbd)) + (long)(nx + 2 * nbd) * it);
+    }
+    if (a.top_ring) {
+        if (iz >= 1 && iz <= nbd && ix >= nbd + 1 && ix <= nx - nbd) {
+            arr = 12;
+            return (long)(iz - 1) + (long)nbd * ((long)(ix - (nbd + 1)) + (long)(nx - 2 * nbd) * it);
+        }
+        if (iz >= 1 - nbd && iz <= 0 && ix >= 1 - nbd && ix <= nx + nbd) {
+            arr = 14;
+            return (long)(iz - (1 - nbd)) + (long)nbd * ((long)(ix - (1 - nbd)) + (long)(nx + 2 * nbd) * it);
+        }
+    }
+    return -1;
+}
+
+// field accessors: plain level-n fields, or the ring background masked to
+// one side of the hybrid interface (inside = the total-field region)
+template <class T>
+struct GField {
+    const GenArgs<T> *a;
+    const T *f;
+    __device__ __forceinline__ T operator()(int iz, int ix) const { return f[gcell(*a, iz, ix)]; }
+};
+template <class T>
+struct GMasked {
+    const GenArgs<T> *a;
+    const T *f;
+    bool inside;   // keep the values of the total-field region (else of the padding)
+    __device__ __forceinline__ T operator()(int iz, int ix) const
+    {
+        const bool in = iz >= 1 && iz <= a->g.nz && ix >= 1 && ix <= a->g.nx;
+        return in == inside ? f[gcell(*a, iz, ix)] : (T)0.0f;
+    }
+};
+
+// collocated first derivatives with the reduced-order rule of diff1
+// (Submain.f90:192-209), not divided by the spacing
+template <class T, class F>
+__device__ T gcdx(const GenArgs<T> &a, const F &f, int iz, int ix)
+{
+    const int norder = a.g.norder;
+    const int nox = gmin(gmin(ix + a.g.npml - 1, a.g.nx + a.g.npml - ix), norder);
+    T s = (T)0.0f;
+    for (int k = 1; k <= nox; ++k) s = gacc(s, gco(a.c1, norder, k, nox), (T)(f(iz, ix + k) - f(iz, ix - k)));
+    return s;
+}
+template <class T, class F>
+__device__ T gcdz(const GenArgs<T> &a, const F &f, int iz, int ix)
+{
+    const int norder = a.g.norder;
+    const int noz = gmin(gmin(iz - a.zt, a.g.nz + a.g.npml - iz), norder);
+    T s = (T)0.0f;
+    for (int k = 1; k <= noz; ++k) s = gacc(s, gco(a.c1, norder, k, noz), (T)(f(iz + k, ix) - f(iz - k, ix)));
+    return s;
+}
+
+// modulus of the second-derivative terms: lam+2mu (mode 0) or mu (mode 1)
+template <class T>
+__device__ __forceinline__ T gmat(const GenArgs<T> &a, int mode, size_t c)
+{
+#pragma clang fp contract(off)
+    return mode == 0 ? (T)(a.lam[c] + (T)2.0f * a.mu[c]) : a.mu[c];
+}
+
+// diffx2 / diffz2 at one cell (the loop bodies of Submain.f90:815-837,
+// :990-1012 with their reduced orders)
+template <class T, class F>
+__device__ T gx2(const GenArgs<T> &a, int mode, const F &u, int iz, int ix)
+{
+#pragma clang fp contract(off)
+    const int norder = a.g.norder, nx = a.g.nx, npml = a.g.npml;
+    const int nox = gmin(gmin(ix + npml - 1, nx + npml - ix), norder);
+    T s = (T)0.0f;
+    for (int k = 1; k <= nox; ++k) {
+        const int i = (2 * k - 1) / 2;
+        const int n1 = gmin(gmin(ix + i + npml - 1, nx + npml - ix - i), norder);
+        T s1 = (T)0.0f;
+        for (int j = 1; j <= n1; ++j) s1 = gacc(s1, gco(a.c0, norder, j, n1), (T)(u(iz, ix + j + k - 1) - u(iz, ix - j + k)));
+        const int n2 = gmin(gmin(ix - i + npml - 1, nx + npml - ix + i), norder);
+        T s2 = (T)0.0f;
+        for (int j = 1; j <= n2; ++j) s2 = gacc(s2, gco(a.c0, norder, j, n2), (T)(u(iz, ix + j - k) - u(iz, ix - j - k + 1)));
+        const T t = ghmean(gmat(a, mode, gcell(a, iz, ix + k)), gmat(a, mode, gcell(a, iz, ix + k - 1))) * s1 -
+                    ghmean(gmat(a, mode, gcell(a, iz, ix - k + 1)), gmat(a, mode, gcell(a, iz, ix - k))) * s2;
+        s = gacc(s, gco(a.c0, norder, k, nox), t);
+    }
+    const bool in_reg = iz >= 1 && iz <= a.g.nz && ix >= 1 && ix <= nx;
+    const T dxs = (T)a.g.dx;
+    const T dx2 = in_reg ? (T)(1.0 / (double)(dxs * dxs)) : (T)1.0f / (dxs * dxs);
+    return s * dx2;
+}
+template <class T, class F>
+__device__ T gz2(const GenArgs<T> &a, int mode, const F &u, int iz, int ix)
+{
+#pragma clang fp contract(off)
+    const int norder = a.g.norder, nz = a.g.nz, npml = a.g.npml, zt = a.zt;
+    const int noz = gmin(gmin(iz - zt, nz + npml - iz), norder);
+    T s = (T)0.0f;
+    for (int k = 1; k <= noz; ++k) {
+        const int i = (2 * k - 1) / 2;
+        const int n1 = gmin(gmin(iz + i - zt, nz + npml - iz - i), norder);
+        T s1 = (T)0.0f;
+        for (int j = 1; j <= n1; ++j) s1 = gacc(s1, gco(a.c0, norder, j, n1), (T)(u(iz + j + k - 1, ix) - u(iz - j + k, ix)));
+        const int n2 = gmin(gmin(iz - i - zt, nz + npml - iz + i), norder);
+        T s2 = (T)0.0f;
+        for (int j = 1; j <= n2; ++j) s2 = gacc(s2, gco(a.c0, norder, j, n2), (T)(u(iz + j - k, ix) - u(iz - j - k + 1, ix)));
+        const T t = ghmean(gmat(a, mode, gcell(a, iz + k, ix)), gmat(a, mode, gcell(a, iz + k - 1, ix))) * s1 -
+                    ghmean(gmat(a, mode, gcell(a, iz - k + 1, ix)), gmat(a, mode, gcell(a, iz - k, ix))) * s2;
+        s = gacc(s, gco(a.c0, norder, k, noz), t);
+    }
+    const T dzs = (T)a.g.dz;
+    return s * ((T)1.0f / (dzs * dzs));
+}
+// Dx(mat Dz u) (diffzx) and Dz(mat Dx u) (diffxz) at one cell
+template <class T, class F>
+__device__ T gzx(const GenArgs<T> &a, const T *mat, const F &u, int iz, int ix)
+{
+#pragma clang fp contract(off)
+    const int norder = a.g.norder;
+    const int nox = gmin(gmin(ix + a.g.npml - 1, a.g.nx + a.g.npml - ix), norder);
+    const T dx = (T)a.g.dx, dz = (T)a.g.dz;
+    T s = (T)0.0f;
+    for (int k = 1; k <= nox; ++k) {
+        const T vp = mat[gcell(a, iz, ix + k)] * gcdz(a, u, iz, ix + k) / dz;
+        const T vm = mat[gcell(a, iz, ix - k)] * gcdz(a, u, iz, ix - k) / dz;
+        s = gacc(s, gco(a.c1, norder, k, nox), (T)(vp - vm));
+    }
+    return s / dx;
+}
+template <class T, class F>
+__device__ T gxz(const GenArgs<T> &a, const T *mat, const F &u, int iz, int ix)
+{
+#pragma clang fp contract(off)
+    const int norder = a.g.norder;
+    const int noz = gmin(gmin(iz - a.zt, a.g.nz + a.g.npml - iz), norder);
+    const T dx = (T)a.g.dx, dz = (T)a.g.dz;
+    T s = (T)0.0f;
+    for (int k = 1; k <= noz; ++k) {
+        const T vp = mat[gcell(a, iz + k, ix)] * gcdx(a, u, iz + k, ix) / dx;
+        const T vm = mat[gcell(a, iz - k, ix)] * gcdx(a, u, iz - k, ix) / dx;
+        s = gacc(s, gco(a.c1, norder, k, noz), (T)(vp - vm));
+    }
+    return s / dz;
+}
+
+// the six RHS parts {Xx, Zx, Mx, Xz, Zz, Mz} at one cell; the free-surface
+// closure on row 1 when zt = 1 (half-cell energy form, traction free)
+template <class T, class F>
+__device__ void gparts(const GenArgs<T> &a, const F &ux, const F &uz, int iz, int ix, T p[6])
+{
+#pragma clang fp contract(off)
+    if (iz == 1 && a.zt == 1) {
+        const size_t ca = gcell(a, 1, ix), cb = gcell(a, 2, ix);
+        const T dz = (T)a.g.dz, dx = (T)a.g.dx;
+        const T sdz2 = (T)2.0f / (dz * dz);
+        const T M1 = a.lam[ca] + (T)2.0f * a.mu[ca], M2 = a.lam[cb] + (T)2.0f * a.mu[cb];
+        const int norder = a.g.norder;
+        const int nox = gmin(gmin(ix + a.g.npml - 1, a.g.nx + a.g.npml - ix), norder);
+        T sl = (T)0.0f, sm = (T)0.0f;
+        for (int k = 1; k <= nox; ++k) {
+            const size_t a1 = gcell(a, 1, ix + k), b1 = gcell(a, 2, ix + k), a2 = gcell(a, 1, ix - k), b2 = gcell(a, 2, ix - k);
+            const T lp = (T)0.5f * (a.lam[a1] + a.lam[b1]), lm = (T)0.5f * (a.lam[a2] + a.lam[b2]);
+            const T mp = (T)0.5f * (a.mu[a1] + a.mu[b1]), mm = (T)0.5f * (a.mu[a2] + a.mu[b2]);
+            sl = gacc(sl, gco(a.c1, norder, k, nox), (T)(lp * (uz(2, ix + k) - uz(1, ix + k)) - lm * (uz(2, ix - k) - uz(1, ix - k))));
+            sm = gacc(sm, gco(a.c1, norder, k, nox), (T)(mp * (ux(2, ix + k) - ux(1, ix + k)) - mm * (ux(2, ix - k) - ux(1, ix - k))));
+        }
+        const T lbar = (T)0.5f * (a.lam[ca] + a.lam[cb]), mbar = (T)0.5f * (a.mu[ca] + a.mu[cb]);
+        const T dxuz = (gcdx(a, uz, 1, ix) + gcdx(a, uz, 2, ix)) / dx;
+        const T dxux = (gcdx(a, ux, 1, ix) + gcdx(a, ux, 2, ix)) / dx;
+        p[0] = gx2(a, 0, ux, iz, ix);
+        p[1] = (ghmean(a.mu[ca], a.mu[cb]) * (ux(2, ix) - ux(1, ix))) * sdz2;
+        p[2] = (sl / dx + mbar * dxuz) / dz;
+        p[3] = gx2(a, 1, uz, iz, ix);
+        p[4] = (ghmean(M1, M2) * (uz(2, ix) - uz(1, ix))) * sdz2;
+        p[5] = (sm / dx + lbar * dxux) / dz;
+        return;
+    }
+    p[0] = gx2(a, 0, ux, iz, ix);
+    p[1] = gz2(a, 1, ux, iz, ix);
+    p[2] = gzx(a, a.lam, uz, iz, ix) + gxz(a, a.mu, uz, iz, ix);
+    p[3] = gx2(a, 1, uz, iz, ix);
+    p[4] = gz2(a, 0, uz, iz, ix);
+    p[5] = gzx(a, a.mu, ux, iz, ix) + gxz(a, a.lam, ux, iz, ix);
+}
+
+// memory of strip cell (iz, ix) or 0 outside the strips / where a strip
+// keeps none
+template <class T>
+__device__ __forceinline__ T gpsi(const GenArgs<T> &a, int dir, int comp, int iz, int ix)
+{
+    const int s = gstrip(a, iz, ix);
+    if (s < 0 || (s == 3 && !a.top)) return (T)0.0f;
+    const T *p = dir == 0 ? a.bpx[s][comp] : a.bpz[s][comp];
+    return p ? p[gstrip_off(a, s, iz, ix)] : (T)0.0f;
+}
+
+// ---- per-step preparation: receivers (level n), ring values, memories ----
+template <class T>
+struct GenRec {
+    int nrcv, nsamp, istp;
+    T x0, z0;
+    const float *rx, *rz;
+    T *seisx, *seisz;   // (nsamp, nrcv), time fastest
+};
+
+__device__ double gdirac(double x)
+{
+#pragma clang fp contract(off)
+    const double C1_0 = 1.0, C1_2 = -5.0 / 4.0, C1_3 = -35.0 / 12.0, C1_4 = 21.0 / 4.0, C1_5 = -25.0 / 12.0;
+    const double C2_0 = -4.0, C2_1 = 75.0 / 4.0, C2_2 = -245.0 / 8.0, C2_3 = 545.0 / 24.0,
+                 C2_4 = -63.0 / 8.0, C2_5 = 25.0 / 24.0;
+    const double C3_0 = 18.0, C3_1 = -153.0 / 4.0, C3_2 = 255.0 / 8.0, C3_3 = -313.0 / 24.0,
+                 C3_4 = 21.0 / 8.0, C3_5 = -5.0 / 24.0;
+    double x2 = x * x, x3 = x2 * x;
+    if (x <= 1.0) return C1_0 + x2 * (C1_2 + x2 * C1_4) + x3 * (C1_3 + x2 * C1_5);
+    if (x <= 2.0) return C2_0 + C2_1 * x + x2 * (C2_2 + x2 * C2_4) + x3 * (C2_3 + x2 * C2_5);
+    if (x <= 3.0) return C3_0 + C3_1 * x + x2 * (C3_2 + x2 * C3_4) + x3 * (C3_3 + x2 * C3_5);
+    return 0.0;
+}
+
+// loadseis (Submain.f90:3-116) of receiver i, rows from zt: the 7x7
+// window's products summed in the reference's loop order
+template <class T>
+__device__ void grecord(const GenArgs<T> &a, const GenRec<T> &r, int i)
+{
+#pragma clang fp contract(off)
+    const Geo &g = a.g;
+    const T rx = (T)r.rx[i], rz = (T)r.rz[i], dx = (T)g.dx, dz = (T)g.dz;
+    // the window, clamped in floating point (any coordinate gives an empty
+    // or in-grid window)
+    float fx = floorf((float)((rx - r.x0) / dx)), fz = floorf((float)((rz - r.z0) / dz));
+    const float xlo = (float)(-g.npml - 8), xhi = (float)(g.nx + g.npml + 8);
+    const float zlo = (float)(a.zt - 9), zhi = (float)(g.nz + g.npml + 8);
+    fx = fx > xlo ? (fx < xhi ? fx : xhi) : xlo;
+    fz = fz > zlo ? (fz < zhi ? fz : zhi) : zlo;
+    const int irx = (int)fx + 1, irz = (int)fz + 1;
+    T sx = (T)0.0f, sz = (T)0.0f;
+    for (int ix = gmax(1 - g.npml, irx - 3); ix <= gmin(g.nx + g.npml, irx + 3); ++ix) {
+        const double x = fabs((double)(r.x0 + (T)(ix - 1) * dx) - (double)rx) / (double)dx;
+        const double bax = gdirac(x);
+        for (int iz = gmax(a.zt, irz - 3); iz <= gmin(g.nz + g.npml, irz + 3); ++iz) {
+            const double z = fabs((double)(r.z0 + (T)(iz - 1) * dz) - (double)rz) / (double)dz;
+            const double ba = bax * gdirac(z);
+            const size_t c = gcell(a, iz, ix);
+            sx = (T)((double)sx + ba * (double)a.ux[c]);
+            sz = (T)((double)sz + ba * (double)a.uz[c]);
+        }
+    }
+    r.seisx[(size_t)i * r.nsamp + r.istp] = sx;
+    r.seisz[(size_t)i * r.nsamp + r.istp] = sz;
+}
+
+// one thread per receiver, per ring-region cell and per strip cell:
+// threads [0, nrec) record, [nrec, nrec + nring) write the ring values of
+// the cells of rows zr..nz+nbd x columns 1-nbd..nx+nbd (the ring's bounding
+// frame; zero off the ring), the rest update the memories of strip cells
+template <class T>
+__global__ __launch_bounds__(256) void k_gen_prep(GenArgs<T> a, GenRec<T> r, long nrec, long nring, int zr, T *ringx,
+                                                   T *ringz)
+{
+#pragma clang fp contract(off)
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const Geo &g = a.g;
+    if (t < nrec) { grecord(a, r, (int)t); return; }
+    t -= nrec;
+    if (t < nring) {
+        const int rows = g.nz + g.nbd - zr + 1;
+        const int ix = 1 - g.nbd + (int)(t / rows), iz = zr + (int)(t % rows);
+        int arr = 0;
+        const long o = a.ring_valid ? gring(a, iz, ix, a.it, arr) : -1;
+        const size_t c = gcell(a, iz, ix);
+        ringx[c] = o >= 0 ? a.bg[arr][o] : (T)0.0f;
+        ringz[c] = o >= 0 ? a.bg[arr + 1][o] : (T)0.0f;
+        return;
+    }
+    t -= nring;
+    // strip cells: L, R (npml x nz each), B (NX x npml), T (NX x npml)
+    const long nlr = (long)g.npml * g.nz, nb = (long)g.NX * g.npml;
+    int s, iz, ix;
+    if (t < nlr) { s = 0; ix = 1 - g.npml + (int)(t / g.nz); iz = 1 + (int)(t % g.nz); }
+    else if (t < 2 * nlr) { t -= nlr; s = 1; ix = g.nx + 1 + (int)(t / g.nz); iz = 1 + (int)(t % g.nz); }
+    else if (t < 2 * nlr + nb) { t -= 2 * nlr; s = 2; ix = 1 - g.npml + (int)(t / g.npml); iz = g.nz + 1 + (int)(t % g.npml); }
+    else if (a.top && t < 2 * nlr + 2 * nb) { t -= 2 * nlr + nb; s = 3; ix = 1 - g.npml + (int)(t / g.npml); iz = 1 - g.npml + (int)(t % g.npml); }
+    else return;
+    const size_t o = gstrip_off(a, s, iz, ix);
+    const GField<T> fx{&a, a.ux}, fz{&a, a.uz};
+    const int jx = ix + g.npml - 1, jz = iz - a.zt;
+    if (a.dpx[jx] > (T)0.0f) {
+        const T b = a.bx[jx], c = (T)1.0f - b;
+        const T gx = gcdx(a, fx, iz, ix) / (T)g.dx;
+        const T gz = gcdx(a, fz, iz, ix) / (T)g.dx;
+        a.bpx[s][0][o] = b * a.bpx[s][0][o] + c * gx;
+        a.bpx[s][1][o] = b * a.bpx[s][1][o] + c * gz;
+    }
+    if (a.bpz[s][0] && a.dpz[jz] > (T)0.0f) {
+        const T b = a.bz[jz], c = (T)1.0f - b;
+        const T gx = gcdz(a, fx, iz, ix) / (T)g.dz;
+        const T gz = gcdz(a, fz, iz, ix) / (T)g.dz;
+        a.bpz[s][0][o] = b * a.bpz[s][0][o] + c * gx;
+        a.bpz[s][1][o] = b * a.bpz[s][1][o] + c * gz;
+    }
+}
+
+// one thread per cell of the padded grid: level n+1
+template <class T>
+__global__ __launch_bounds__(256) void k_gen_step(GenArgs<T> a)
+{
+#pragma clang fp contract(off)
+    const Geo &g = a.g;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)g.NX * g.NZ) return;
+    const int ix = 1 - g.npml + (int)(t / g.NZ), iz = a.zt + (int)(t % g.NZ);
+    const size_t gi = gcell(a, iz, ix);
+    const int iz_first = a.zt == 1 ? 1 : a.zt + 1;
+    if (ix < 2 - g.npml || ix > g.nx + g.npml - 1 || iz < iz_first || iz > g.nz + g.npml - 1) {
+        // cells on the padded grid's edge keep their value
+        a.uxo[gi] = a.ux[gi];
+        a.uzo[gi] = a.uz[gi];
+        return;
+    }
+    const GField<T> fux{&a, a.ux}, fuz{&a, a.uz};
+    T p[6];
+    gparts(a, fux, fuz, iz, ix, p);
+    const int s = gstrip(a, iz, ix);
+    // PML memory corrections (nonzero only next to damped cells)
+    {
+        const bool surf = iz == 1 && a.zt == 1;
+        const int norder = g.norder;
+        const int nox = gmin(gmin(ix + g.npml - 1, g.nx + g.npml - ix), norder);
+        const int noz = gmin(gmin(iz - a.zt, g.nz + g.npml - iz), norder);
+        T sx0 = (T)0.0f, sx1 = (T)0.0f, sz0 = (T)0.0f, sz1 = (T)0.0f;
+        for (int k = 1; k <= nox; ++k) {
+            const size_t ca = gcell(a, iz, ix + k), cb = gcell(a, iz, ix - k);
+            const T ma = a.lam[ca] + (T)2.0f * a.mu[ca], mb = a.lam[cb] + (T)2.0f * a.mu[cb];
+            sx0 = gacc(sx0, gco(a.c1, norder, k, nox), (T)(ma * gpsi(a, 0, 0, iz, ix + k) - mb * gpsi(a, 0, 0, iz, ix - k)));
+            sx1 = gacc(sx1, gco(a.c1, norder, k, nox),
+                       (T)(a.mu[ca] * gpsi(a, 0, 1, iz, ix + k) - a.mu[cb] * gpsi(a, 0, 1, iz, ix - k)));
+        }
+        for (int k = 1; k <= noz; ++k) {
+            const size_t ca = gcell(a, iz + k, ix), cb = gcell(a, iz - k, ix);
+            const T ma = a.lam[ca] + (T)2.0f * a.mu[ca], mb = a.lam[cb] + (T)2.0f * a.mu[cb];
+            sz0 = gacc(sz0, gco(a.c1, norder, k, noz),
+                       (T)(a.mu[ca] * gpsi(a, 1, 0, iz + k, ix) - a.mu[cb] * gpsi(a, 1, 0, iz - k, ix)));
+            sz1 = gacc(sz1, gco(a.c1, norder, k, noz), (T)(ma * gpsi(a, 1, 1, iz + k, ix) - mb * gpsi(a, 1, 1, iz - k, ix)));
+        }
+        p[0] = p[0] - sx0 / (T)g.dx;
+        p[3] = p[3] - sx1 / (T)g.dx;
+        if (!surf) { p[1] = p[1] - sz0 / (T)g.dz; p[4] = p[4] - sz1 / (T)g.dz; }
+    }
+    // hybrid TF/SF correction: an inside-ring cell adds the operator of the
+    // background on the padding, an outside-ring cell subtracts that of the
+    // background on the total-field region
+    if (a.ring_valid) {
+        int arr = 0;
+        if (gring(a, iz, ix, a.it, arr) >= 0) {
+            const bool inside = iz >= 1 && iz <= g.nz && ix >= 1 && ix <= g.nx;
+            const GMasked<T> wx{&a, a.ringx, !inside}, wz{&a, a.ringz, !inside};
+            T q[6];
+            gparts(a, wx, wz, iz, ix, q);
+            for (int m = 0; m < 6; ++m) p[m] = inside ? p[m] + q[m] : p[m] - q[m];
+        }
+    }
+    const T dt2 = (T)(g.dt * g.dt);
+    const T bf = dt2 / a.rho[gi];
+    if (s < 0) {
+        const T rx = (p[0] + p[1]) + p[2], rz = (p[3] + p[4]) + p[5];
+        a.uxo[gi] = ((T)2.0f * a.ux[gi] - a.uxp[gi]) + bf * rx;
+        a.uzo[gi] = ((T)2.0f * a.uz[gi] - a.uzp[gi]) + bf * rz;
+        return;
+    }
+    const size_t o = gstrip_off(a, s, iz, ix);
+    const int jx = ix + g.npml - 1, jz = iz - a.zt;
+    const T ax = (T)0.5f * (a.dpx[jx] + a.ratio * a.dpz[jz]) * (T)g.dt;
+    const T az = (T)0.5f * (a.dpz[jz] + a.ratio * a.dpx[jx]) * (T)g.dt;
+    const T c3 = (T)2.0f - (T)4.0f * ax * az;
+    for (int comp = 0; comp < 2; ++comp) {
+        T *b1 = a.bu[s][comp][0], *b2 = a.bu[s][comp][1], *b3 = a.bu[s][comp][2];
+        const T X = p[3 * comp], Z = p[3 * comp + 1], Mm = p[3 * comp + 2];
+        const T n1 = ((T)2.0f * b1[2 * o] - ((T)1.0f - ax) * b1[2 * o + 1] + bf * X) / ((T)1.0f + ax);
+        const T n2 = ((T)2.0f * b2[2 * o] - ((T)1.0f - az) * b2[2 * o + 1] + bf * Z) / ((T)1.0f + az);
+        const T n3 = (c3 * b3[2 * o] - ((T)1.0f - ax - az) * b3[2 * o + 1] + bf * Mm) / ((T)1.0f + ax + az);
+        b1[2 * o + 1] = b1[2 * o]; b1[2 * o] = n1;
+        b2[2 * o + 1] = b2[2 * o]; b2[2 * o] = n2;
+        b3[2 * o + 1] = b3[2 * o]; b3[2 * o] = n3;
+        const T tot = (n1 + n2) + n3;
+        if (comp == 0) a.uxo[gi] = tot; else a.uzo[gi] = tot;
+    }
+}
+
+// reference c (1:3, rows zt.., columns) -> rho, lam, mu planes; the damped
+// cells take the material of the outermost undamped cell along the normal
+template <class T>
+__global__ void k_gen_import_material(Geo g, int zt, const float *c, T *rho, T *lam, T *mu)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)g.NX * g.NZ) return;
+    const int j = (int)(t / g.NZ), k = (int)(t % g.NZ);
+    const int ix = j + 1 - g.npml, iz = zt + k;
+    const int sx = ix < 1 - g.nbd ? 1 - g.nbd : (ix > g.nx + g.nbd ? g.nx + g.nbd : ix);
+    const int sz = iz > g.nz + g.nbd ? g.nz + g.nbd : (iz < 1 - g.nbd ? 1 - g.nbd : iz);
+    const long src = (long)(sx + g.npml - 1) * g.NZ + (sz - zt);
+    const size_t d = (size_t)j * g.ld + k;
+    rho[d] = (T)c[3 * src + 0];
+    lam[d] = (T)c[3 * src + 1];
+    mu[d] = (T)c[3 * src + 2];
+}
+
+// reference (1:2, NZ, NX) <-> two padded arrays (level 1 -> cur, level 2 -> prev)
+template <class T, class R>
+__global__ void k_gen_levels(Geo g, int to_ref, R *ref2, T *cur, T *prev)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)g.NX * g.NZ) return;
+    const size_t d = (size_t)(t / g.NZ) * g.ld + (size_t)(t % g.NZ);
+    if (to_ref) {
+        ref2[2 * t] = (R)cur[d];
+        ref2[2 * t + 1] = (R)prev[d];
+    } else {
+        cur[d] = (T)ref2[2 * t];
+        prev[d] = (T)ref2[2 * t + 1];
+    }
+}
+
+}  // namespace fdfk

@@ -54,6 +54,8 @@ def lib():
         L.fdfk_destroy.restype = None
         L.fdfk_set_background.argtypes = [C.c_void_p, C.c_void_p]
         L.fdfk_get_background.argtypes = [C.c_void_p, C.c_void_p]
+        L.fdfk_set_top_background.argtypes = [C.c_void_p, C.c_void_p]
+        L.fdfk_get_top_background.argtypes = [C.c_void_p, C.c_void_p]
         L.fdfk_compute_background.argtypes = [C.c_void_p, C.POINTER(FdfkFK)]
         L.fdfk_seismograms.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.fdfk_get_state.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
@@ -108,6 +110,13 @@ def bg_sizes(nx, nz, nbd, nt):
     return [reg_lr] * 4 + [reg_b] * 2 + [pml_lr] * 4 + [pml_b] * 2
 
 
+def bg_top_sizes(nx, nbd, nt):
+    """Element counts of the top ring's uxt_reg, uzt_reg, uxt_pml, uzt_pml
+    (an absorbing top layer only)."""
+    T = nt + 1
+    return [nbd * (nx - 2 * nbd) * T] * 2 + [nbd * (nx + 2 * nbd) * T] * 2
+
+
 def autotune(nx, nz, npml, norder, dx, dz, dt, c, x0=0.0, z0=0.0, rx=None, rz=None, R0=1e-3, npower=2,
              vpml=0.0, steps=20):
     """Time interior tile widths for this geometry (fdfk_autotune); later
@@ -128,10 +137,15 @@ class Solver:
     """One simulation resident on the GPU (a fdfk_solver handle)."""
 
     def __init__(self, nx, nz, npml, norder, dx, dz, dt, nt, c, x0=0.0, z0=0.0, nstp=1,
-                 rx=None, rz=None, R0=1e-3, npower=2, vpml=0.0, precision=FDFK_FP32):
+                 rx=None, rz=None, R0=1e-3, npower=2, vpml=0.0, precision=FDFK_FP32, pml_top=False):
+        """pml_top: an absorbing top layer of npml rows above row 1 (the
+        reference's is_PML_top, izt = 1-npml) instead of the free surface; c
+        then holds rows izt..nz+npml."""
         self.nx, self.nz, self.npml, self.norder, self.nt = nx, nz, npml, norder, nt
         self.nbd = 2 * norder - 1
-        self.NX, self.NZ = nx + 2 * npml, nz + npml
+        self.pml_top = bool(pml_top)
+        self.izt = 1 - npml if pml_top else 1
+        self.NX, self.NZ = nx + 2 * npml, nz + npml - self.izt + 1
         self.nstp = nstp
         self.nsamp = nt // nstp + 1
         self._c = np.ascontiguousarray(c, np.float32)
@@ -139,7 +153,7 @@ class Solver:
         self._rx = None if rx is None else np.ascontiguousarray(rx, np.float32)
         self._rz = None if rz is None else np.ascontiguousarray(rz, np.float32)
         self.nrcv = 0 if rx is None else self._rx.size
-        job = FdfkJob(nx=nx, nz=nz, npml=npml, norder=norder, is_pml_top=0, npower=npower,
+        job = FdfkJob(nx=nx, nz=nz, npml=npml, norder=norder, is_pml_top=int(self.pml_top), npower=npower,
                       x0=x0, z0=z0, dx=dx, dz=dz, dt=dt, R0=R0, vpml=vpml, nt=nt, nstp=nstp,
                       nrcv=self.nrcv, rx=_ptr(self._rx), rz=_ptr(self._rz), c=_ptr(self._c),
                       precision=precision)
@@ -155,6 +169,17 @@ class Solver:
     __del__ = close
 
     def set_background(self, arrays):
+        """The twelve reference arrays, or (twelve, top ring's four) with an
+        absorbing top layer."""
+        if isinstance(arrays, tuple) and len(arrays) == 2:
+            arrays, top = arrays
+            self.set_background(arrays)
+            tops = [np.ascontiguousarray(a, np.float32) for a in top]
+            for a, n in zip(tops, bg_top_sizes(self.nx, self.nbd, self.nt)):
+                assert a.size == n, (a.size, n)
+            tp = (C.c_void_p * 4)(*[a.ctypes.data for a in tops])
+            _check(lib().fdfk_set_top_background(self.h, tp))
+            return
         arrs = [np.ascontiguousarray(a, np.float32) for a in arrays]
         sizes = bg_sizes(self.nx, self.nz, self.nbd, self.nt)
         for a, n in zip(arrs, sizes):
@@ -174,10 +199,17 @@ class Solver:
         _check(lib().fdfk_compute_background(self.h, C.byref(fk)))
 
     def get_background(self):
+        """The twelve reference arrays; with an absorbing top layer (twelve,
+        top ring's four)."""
         arrs = [np.zeros(n, np.float32) for n in bg_sizes(self.nx, self.nz, self.nbd, self.nt)]
         ptrs = (C.c_void_p * 12)(*[a.ctypes.data for a in arrs])
         _check(lib().fdfk_get_background(self.h, ptrs))
-        return arrs
+        if not self.pml_top:
+            return arrs
+        tops = [np.zeros(n, np.float32) for n in bg_top_sizes(self.nx, self.nbd, self.nt)]
+        tp = (C.c_void_p * 4)(*[a.ctypes.data for a in tops])
+        _check(lib().fdfk_get_top_background(self.h, tp))
+        return arrs, tops
 
     def run(self, nsteps):
         _check(lib().fdfk_run(self.h, int(nsteps)))

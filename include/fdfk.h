@@ -12,8 +12,11 @@
  *   seisx/seisz (0:nsis, 1:nrcv)               time fastest per receiver
  *   the twelve background arrays uxl_reg..uzb_pml with the shapes of
  *   src/FDFK2D.f90:330-355 (time slowest, 0:nt).
- * Only the free-surface configuration (is_PML_top = .false., izt = 1) is
- * supported; a top PML is refused with FDFK_ENOSYS.
+ * izt = 1 with a free surface (is_PML_top = .false.), izt = 1-npml with an
+ * absorbing top layer (is_PML_top = .true., src/Input.f90:179-183).  The top
+ * layer needs a hybrid ring across the top as well, four arrays beyond the
+ * reference's twelve (fdfk_set_top_background), and a background without a
+ * free surface (fdfk_compute_background makes both).
  */
 #ifndef FDFK_H
 #define FDFK_H
@@ -40,7 +43,7 @@ typedef struct fdfk_solver fdfk_solver;
  * of read_data, i.e. the state simulate_wavefield (src/FDFK2D.f90:159) sets up. */
 typedef struct fdfk_job {
     int32_t nx, nz, npml, norder;   /* norder = FD order / 2 (Input.f90:130-131) */
-    int32_t is_pml_top;             /* must be 0 */
+    int32_t is_pml_top;             /* 1: absorbing top layer, rows izt = 1-npml.. (FP32 only, not batched) */
     int32_t npower;                 /* PML profile power (FDFK2D.f90:417: 2) */
     float x0, z0, dx, dz;
     double dt;
@@ -80,8 +83,16 @@ int fdfk_create(const fdfk_job *job, fdfk_solver **out);
 /* upload a host background (twelve reference-layout arrays, nt+1 steps each) */
 int fdfk_set_background(fdfk_solver *s, const float *const bg[12]);
 /* compute the FK background on the GPU (FP64), replaces
- * comput_background_wavefield (src/FDFK2D.f90:432-434) */
+ * comput_background_wavefield (src/FDFK2D.f90:432-434); with an absorbing
+ * top layer the model has no free surface and the top ring is filled too */
 int fdfk_compute_background(fdfk_solver *s, const fdfk_fk *fk);
+/* the top ring of a solver with an absorbing top layer (no counterpart in
+ * the reference, whose twelve arrays leave the hybrid contour open at the
+ * top): uxt_reg, uzt_reg (nbd, nx-2nbd, 0:nt) on rows 1..nbd x columns
+ * nbd+1..nx-nbd, uxt_pml, uzt_pml (nbd, nx+2nbd, 0:nt) on rows 1-nbd..0 x
+ * columns 1-nbd..nx+nbd, z fastest.  Without it the contour stays open. */
+int fdfk_set_top_background(fdfk_solver *s, const float *const top[4]);
+int fdfk_get_top_background(fdfk_solver *s, float *const top[4]);
 /* copy the device background back in the reference layout (twelve arrays);
  * FDFK_EINVAL when it is synthesised in blocks */
 int fdfk_get_background(fdfk_solver *s, float *const bg[12]);
