@@ -1546,6 +1546,35 @@ struct CompatEntry {
 };
 static std::map<const float *, CompatEntry> g_compat;
 static std::mutex g_compat_mu;
+// The top ring of an absorbing top layer has no slot in the reference's
+// arrays: comput_background_wavefield_ keeps the four arrays it computes
+// here, keyed by the caller's uxl_reg array, and updatedispl_ takes them
+// from here for the same uxl_reg (without them the hybrid contour stays
+// open at the top).
+static std::map<const float *, std::array<std::vector<float>, 4>> g_top_ring;
+
+// strips of a general-path solver: the device arrays are the reference's,
+// so the exchange with the caller's arrays is a plain copy
+static int gen_strips_copy(fdfk_solver *s, int st, float *const parts[2][3], float *const px[2],
+                           float *const pz[2], bool to_dev)
+{
+    const Geo &g = s->g;
+    const size_t n = st < 2 ? (size_t)g.npml * g.nz : (size_t)g.NX * g.npml;
+    const hipMemcpyKind k = to_dev ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost;
+    for (int c = 0; c < 2; ++c) {
+        for (int q = 0; q < 3; ++q)
+            HIPCHK(hipMemcpyAsync(to_dev ? (void *)s->gbu[st][c][q] : (void *)parts[c][q],
+                                  to_dev ? (const void *)parts[c][q] : (const void *)s->gbu[st][c][q], 2 * n * 4, k,
+                                  s->stream));
+        HIPCHK(hipMemcpyAsync(to_dev ? (void *)s->gbpx[st][c] : (void *)px[c],
+                              to_dev ? (const void *)px[c] : (const void *)s->gbpx[st][c], n * 4, k, s->stream));
+        if (st >= 2)
+            HIPCHK(hipMemcpyAsync(to_dev ? (void *)s->gbpz[st][c] : (void *)pz[c],
+                                  to_dev ? (const void *)pz[c] : (const void *)s->gbpz[st][c], n * 4, k, s->stream));
+    }
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return 0;
+}
 
 static int strip_import(fdfk_solver *s, int st, float *const parts[2][3], float *const psx[2], float *const psz[2])
 {
@@ -1597,8 +1626,7 @@ void updatedispl_(const int32_t *norder, const int32_t *nx, const int32_t *nz, c
     float *bux1b, float *bux2b, float *bux3b, float *buz1b, float *buz2b, float *buz3b,
     float *bux1t, float *bux2t, float *bux3t, float *buz1t, float *buz2t, float *buz3t)
 {
-    (void)coef0; (void)coef1; (void)bpxtx; (void)bpxtz; (void)bpztx; (void)bpztz;
-    (void)bux1t; (void)bux2t; (void)bux3t; (void)buz1t; (void)buz2t; (void)buz3t;
+    (void)coef0; (void)coef1;
     std::lock_guard<std::mutex> lk(g_compat_mu);
     // the reference's error contract is print and stop: a failed step must
     // not return to a driver that would go on writing stale seismograms
@@ -1608,7 +1636,8 @@ void updatedispl_(const int32_t *norder, const int32_t *nx, const int32_t *nz, c
         std::exit(1);
     };
     // host checks first: a bad configuration stops before any GPU work
-    if (*is_pml_top || *izt != 1) die("only is_PML_top = .false. is supported");
+    const bool top = *is_pml_top != 0;
+    if (*izt != (top ? 1 - *npml : 1)) die("izt must be 1 (free surface) or 1-npml (is_PML_top)");
     if (*nbd != 2 * *norder - 1) die("nbd must equal 2*norder-1");
     if (validate_geometry(*nx, *nz, *npml, *norder, *dx, *dz, *dt, *nt)) die(fdfk_last_error());
     if (*it < 0 || *it > *nt) die("it outside 0..nt");
@@ -1624,21 +1653,45 @@ void updatedispl_(const int32_t *norder, const int32_t *nx, const int32_t *nz, c
     float *const pl[2][3] = {{bux1l, bux2l, bux3l}, {buz1l, buz2l, buz3l}};
     float *const pr[2][3] = {{bux1r, bux2r, bux3r}, {buz1r, buz2r, buz3r}};
     float *const pb[2][3] = {{bux1b, bux2b, bux3b}, {buz1b, buz2b, buz3b}};
+    float *const pt[2][3] = {{bux1t, bux2t, bux3t}, {buz1t, buz2t, buz3t}};
     float *const xl[2] = {bpxlx, bpzlx}, *const xr[2] = {bpxrx, bpzrx}, *const xb[2] = {bpxbx, bpzbx};
     float *const zb[2] = {bpxbz, bpzbz}, *const none[2] = {nullptr, nullptr};
+    float *const xt[2] = {bpxtx, bpztx}, *const ztop[2] = {bpxtz, bpztz};
+    // the caller's strips in and out: four strips (general path) with a top layer
+    auto strips = [&](fdfk_solver *s, bool in) {
+        if (s->gen)
+            return gen_strips_copy(s, 0, pl, xl, none, in) || gen_strips_copy(s, 1, pr, xr, none, in) ||
+                   gen_strips_copy(s, 2, pb, xb, zb, in) || gen_strips_copy(s, 3, pt, xt, ztop, in);
+        if (in)
+            return strip_import(s, S_L, pl, xl, none) || strip_import(s, S_R, pr, xr, none) ||
+                   strip_import(s, S_B, pb, xb, zb);
+        return strip_export(s, S_L, pl, xl, none) || strip_export(s, S_R, pr, xr, none) ||
+               strip_export(s, S_B, pb, xb, zb);
+    };
+    if (top) {
+        for (float *a : {bux1t, bux2t, bux3t, buz1t, buz2t, buz3t, bpxtx, bpxtz, bpztx, bpztz})
+            if (!a) die("null top-strip array");
+    }
     if (fresh) {
         if (itr != g_compat.end()) { fdfk_destroy(itr->second.s); g_compat.erase(itr); }
         fdfk_job j{};
         j.nx = *nx; j.nz = *nz; j.npml = *npml; j.norder = *norder; j.npower = *npower;
+        j.is_pml_top = top ? 1 : 0;
         j.dx = *dx; j.dz = *dz; j.dt = *dt; j.R0 = *R0; j.vpml = *vpml; j.nt = *nt; j.nstp = 1;
         j.c = c; j.precision = FDFK_FP32;
         fdfk_solver *s = nullptr;
         if (fdfk_create(&j, &s)) die(fdfk_last_error());
         const float *bg[12] = {uxl_reg, uzl_reg, uxr_reg, uzr_reg, uxb_reg, uzb_reg,
                                uxl_pml, uzl_pml, uxr_pml, uzr_pml, uxb_pml, uzb_pml};
-        if (fdfk_set_background(s, bg) || fdfk_set_state(s, ux, uz) || strip_import(s, S_L, pl, xl, none) ||
-            strip_import(s, S_R, pr, xr, none) || strip_import(s, S_B, pb, xb, zb))
-            die(fdfk_last_error());
+        if (fdfk_set_background(s, bg) || fdfk_set_state(s, ux, uz) || strips(s, true)) die(fdfk_last_error());
+        if (top) {
+            auto tr = g_top_ring.find(uxl_reg);
+            if (tr != g_top_ring.end()) {
+                const float *t4[4] = {tr->second[0].data(), tr->second[1].data(), tr->second[2].data(),
+                                      tr->second[3].data()};
+                if (fdfk_set_top_background(s, t4)) die(fdfk_last_error());
+            }
+        }
         CompatEntry e;
         e.s = s; e.nx = *nx; e.nz = *nz; e.npml = *npml; e.norder = *norder; e.nt = *nt; e.c = c; e.bg0 = uxl_reg;
         itr = g_compat.emplace(ux, e).first;
@@ -1646,10 +1699,8 @@ void updatedispl_(const int32_t *norder, const int32_t *nx, const int32_t *nz, c
     CompatEntry &e = itr->second;
     fdfk_solver *s = e.s;
     s->it = *it;
-    if (one_step(s, false)) die(fdfk_last_error());
-    if (fdfk_get_state(s, ux, uz) || strip_export(s, S_L, pl, xl, none) ||
-        strip_export(s, S_R, pr, xr, none) || strip_export(s, S_B, pb, xb, zb))
-        die(fdfk_last_error());
+    if (s->gen ? gen_step(s, false) : one_step(s, false)) die(fdfk_last_error());
+    if (fdfk_get_state(s, ux, uz) || strips(s, false)) die(fdfk_last_error());
     e.last_it = *it;
 }
 
@@ -1658,6 +1709,7 @@ void fdfk_compat_release(void)
     std::lock_guard<std::mutex> lk(g_compat_mu);
     for (auto &kv : g_compat) fdfk_destroy(kv.second.s);
     g_compat.clear();
+    g_top_ring.clear();
 }
 
 }  // extern "C"

@@ -573,7 +573,7 @@ extern "C" void comput_background_wavefield_(const int32_t *nx, const int32_t *n
         std::exit(1);
     };
     // host checks first: a bad configuration stops before any GPU work
-    if (*izt != 1) die("only is_PML_top = .false. is supported");
+    if (*izt != 1 && *izt != 1 - *npml) die("izt must be 1 (free surface) or 1-npml (is_PML_top)");
     if (*nbd < 1 || *nbd % 2 == 0) die("nbd must be 2*norder-1");
     const int norder = (*nbd + 1) / 2;
     if (validate_geometry(*nx, *nz, *npml, norder, *dx, *dz, *dt, *nt)) die(fdfk_last_error());
@@ -587,6 +587,7 @@ extern "C" void comput_background_wavefield_(const int32_t *nx, const int32_t *n
     auto *s = new fdfk_solver();
     std::unique_ptr<fdfk_solver> own(s);
     s->bg_only = true;
+    s->zt = *izt;   // izt < 1: no free surface, and the top ring is computed too
     Geo &g = s->g;
     g.nx = *nx; g.nz = *nz; g.npml = *npml; g.norder = norder; g.nbd = *nbd;
     g.NX = g.nx + 2 * g.npml; g.NZ = g.nz + g.npml;
@@ -600,4 +601,13 @@ extern "C" void comput_background_wavefield_(const int32_t *nx, const int32_t *n
     fk.fk_left = fk_left; fk.fk_right = fk_right; fk.src_is_p = *src_type != 0;
     fk.ds = *ds; fk.f0 = *f0; fk.rayp = *rayp; fk.bazs = bazs;
     if (fdfk_compute_background(s, &fk) || fdfk_get_background(s, out)) die(fdfk_last_error());
+    if (s->zt != 1) {
+        // the top ring, for updatedispl_ with the same uxl_reg
+        std::array<std::vector<float>, 4> t;
+        for (int i = 0; i < 4; ++i) t[i].assign((size_t)s->bg_top_n[i], 0.0f);
+        float *const t4[4] = {t[0].data(), t[1].data(), t[2].data(), t[3].data()};
+        if (fdfk_get_top_background(s, t4)) die(fdfk_last_error());
+        std::lock_guard<std::mutex> lk(g_compat_mu);
+        g_top_ring[uxl_reg] = std::move(t);
+    }
 }

@@ -23,16 +23,16 @@ from . import geometry, io, refstate
 from .lib import Batch, Solver
 
 
-def write_snapshot(s, it, nx, nz, npml, dz, snapdir="snapshots"):
+def write_snapshot(s, it, nx, nz, npml, dz, snapdir="snapshots", izt=1):
     """Snapshot at the top of iteration it (src/FDFK2D.f90:467-480): level 2
     of ux(1:2, :, :) and uz, rows 1..nz of columns 1..nx, one SU trace per
     column, header word 58 = nz and 59 = dz, in ./snapshots/<it>ux.su and
-    <it>uz.su (five-digit iteration, I5.5)."""
+    <it>uz.su (five-digit iteration, I5.5).  Rows start at izt."""
     os.makedirs(snapdir, exist_ok=True)
     ux, uz = s.get_state()
-    NX, NZ = nx + 2 * npml, nz + npml
+    NX, NZ = nx + 2 * npml, nz + npml - izt + 1
     for name, u in (("ux", ux), ("uz", uz)):
-        lev2 = u.reshape(NX, NZ, 2)[npml: npml + nx, :nz, 1]
+        lev2 = u.reshape(NX, NZ, 2)[npml: npml + nx, 1 - izt: 1 - izt + nz, 1]
         io.write_su(os.path.join(snapdir, "%05d%s.su" % (it, name)), lev2, int(dz))
 
 
@@ -58,7 +58,7 @@ def _prepare(job, src, log):
         rayp = 1e-9 * np.sign(rayp if rayp != 0 else 1.0)
     s = Solver(nx, nz, npml, norder, fd["dx"], fd["dz"], dt, nt, job["c"], x0=fd["x0"], z0=fd["z0"],
                nstp=nstp, rx=job["rx"], rz=job["rz"], R0=1e-3, npower=2,
-               vpml=refstate.vp_max(job["c"]))
+               vpml=refstate.vp_max(job["c"]), pml_top=fd["pml_top"])
     try:
         s.compute_background(job["fk_left"], job["fk_right"], src["src_is_p"], src["f0"], src["ds"],
                              rayp, bazs)
@@ -93,7 +93,7 @@ def simulate(job, log=print):
             for it in range(fd["nstep"], nt + 1, fd["nstep"]):
                 s.run(it - done)
                 done = it
-                write_snapshot(s, it, nx, nz, npml, fd["dz"])
+                write_snapshot(s, it, nx, nz, npml, fd["dz"], izt=job.get("izt", 1))
             s.run(nt + 1 - done)
         else:
             s.run(nt + 1)
@@ -115,6 +115,17 @@ def simulate_events(job, events, hbm_budget_gb=200.0, log=print):
     fd = job["fd"]
     out, info, k = [], None, 0
     t0 = time.perf_counter()
+    if fd["pml_top"]:
+        # solvers with an absorbing top layer run one at a time (not batched)
+        for ev in events:
+            s, info = _prepare(job, ev, log)
+            try:
+                s.run(info["nt"] + 1)
+                s.sync()
+                out.append(_finish(job, ev, s))
+            finally:
+                s.close()
+        return out, info
     while k < len(events):
         batch = Batch(fd["norder"])
         members = []
@@ -152,8 +163,9 @@ def main(argv=None):
     # *_without_pml.su copies of the models (Input.f90:305-341)
     for key, arr in (("vp_model", job["vp"]), ("vs_model", job["vs"])):
         name = job["names"][key]
+        r1 = 1 - job.get("izt", 1)   # the row of iz = 1
         io.write_su(os.path.join(prefix, name[:-3] + "_without_pml.su"),
-                    arr[job["fd"]["npml"]: job["fd"]["npml"] + job["nx"], : job["nz"]], 1000)
+                    arr[job["fd"]["npml"]: job["fd"]["npml"] + job["nx"], r1: r1 + job["nz"]], 1000)
     os.makedirs(outpath, exist_ok=True)
     if all_events:
         # every line of Source.dat, <fsis>_<k>x.su and z.su for event k (1-based, file order)

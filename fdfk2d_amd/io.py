@@ -158,9 +158,9 @@ def read_job(prefix, finpar):
     nx = int(round((fd["xn"] - fd["x0"]) / fd["dx"])) + 1
     nz = int(round((fd["zn"] - fd["z0"]) / fd["dz"])) + 1
     npml = fd["npml"]
-    if fd["pml_top"]:
-        raise ValueError("is_PML_top = .true. is not supported (free surface only)")
-    NX, NZ = nx + 2 * npml, nz + npml
+    # an absorbing top layer adds npml rows above row 1 (src/Input.f90:179-183)
+    izt = 1 - npml if fd["pml_top"] else 1
+    NX, NZ = nx + 2 * npml, nz + npml - izt + 1
     xmin, xmax = fd["x0"], fd["x0"] + (nx - 1) * fd["dx"]
     zmin, zmax = fd["z0"], fd["z0"] + (nz - 1) * fd["dz"]
     bad = (rx < xmin) | (rx > xmax) | (rz < zmin) | (rz > zmax)
@@ -173,15 +173,16 @@ def read_job(prefix, finpar):
         raise ValueError("velocity models must hold the padded grid (nx+2*npml traces)")
     c = refstate.material_law(vp[:NX], vs[:NX])
     return dict(names=names, fd=fd, src=src, rx=rx, rz=rz, fk_left=fkl, fk_right=fkr, nx=nx,
-                nz=nz, NX=NX, NZ=NZ, c=c, vp=vp[:NX], vs=vs[:NX])
+                nz=nz, NX=NX, NZ=NZ, izt=izt, c=c, vp=vp[:NX], vs=vs[:NX])
 
 
 def write_inputs(prefix, *, order, x0, xn, z0, zn, dx, dz, dt, tmax, tstp, npml, lat_org,
                  lon_org, az_org, f0, ds, src_is_p, rayp_sdeg, evla, evlo, rx, rz, fk_left,
-                 fk_right, vp, vs, outsnap=False, nstep=100, more_events=()):
+                 fk_right, vp, vs, outsnap=False, nstep=100, more_events=(), pml_top=False):
     """Write a complete input directory in the reference's formats.
-    fk_left/right: rows (ztop, vp, vs); vp, vs: padded (NX, NZ) grids;
-    more_events: further (rayp_sdeg, evla, evlo) lines of Source.dat."""
+    fk_left/right: rows (ztop, vp, vs); vp, vs: padded (NX, NZ) grids (rows
+    from izt = 1-npml with pml_top); more_events: further (rayp_sdeg, evla,
+    evlo) lines of Source.dat."""
     os.makedirs(prefix, exist_ok=True)
     L = lambda b: ".true." if b else ".false."
     with open(os.path.join(prefix, "inpar.dat"), "w") as f:
@@ -193,7 +194,7 @@ def write_inputs(prefix, *, order, x0, xn, z0, zn, dx, dz, dt, tmax, tstp, npml,
         f.write("X0(m)        Xn(m)        Z0(m)     Zn(m)       Dx(m)      Dz(m)       Dt(s)\n")
         f.write(f"  {x0!r} {xn!r} {z0!r} {zn!r} {dx!r} {dz!r} {dt!r}\n")
         f.write("tmax(s)      tstep        nPML(number of grid)  is_PML_top\n")
-        f.write(f"  {tmax!r} {tstp!r} {npml} .false.\n")
+        f.write(f"  {tmax!r} {tstp!r} {npml} {L(pml_top)}\n")
         f.write("latitude of (X0,Z0)       longitude of (X0,Z0)  azimuthe of profile\n")
         f.write(f"  {lat_org!r} {lon_org!r} {az_org!r}\n")
         f.write("output snapshot?(.true./.false.)  nstep\n")

@@ -21,17 +21,18 @@ REF_DRIVER = os.path.join(ROOT, "oracle", "_ref", "fdfk2d_ref")
 SNAP_NSTEP = 100
 
 
-def make_inputs(prefix, outsnap=True, event=(6.6717, -42.0, 10.0), more_events=()):
+def make_inputs(prefix, outsnap=True, event=(6.6717, -42.0, 10.0), more_events=(), pml_top=False):
     nx, nz, npml, dx = 101, 50, 14, 200.0
     layers = [(0.0, 6000.0, 3450.0), (6000.0, 8000.0, 4450.0)]
-    vp, vs = layered_grid(nx, nz, npml, dx, dx, layers, anomaly=(9000.0, 4000.0, 1500.0, -0.06))
+    vp, vs = layered_grid(nx, nz, npml, dx, dx, layers, anomaly=(9000.0, 4000.0, 1500.0, -0.06),
+                          izt=1 - npml if pml_top else 1)
     fk = [(0.0, 6000.0, 3450.0), (5900.0, 8000.0, 4450.0)]
     rx = np.arange(0, 20001, 2000, dtype=np.float32)
     io.write_inputs(prefix, order=6, x0=0.0, xn=20000.0, z0=0.0, zn=9800.0, dx=dx, dz=dx, dt=0.01,
                     tmax=2.6, tstp=0.01, npml=npml, lat_org=45.0, lon_org=0.0, az_org=0.0, f0=2.0,
                     ds=1e-3, src_is_p=True, rayp_sdeg=event[0], evla=event[1], evlo=event[2], rx=rx,
                     rz=np.zeros_like(rx), fk_left=fk, fk_right=fk, vp=vp, vs=vs, outsnap=outsnap,
-                    nstep=SNAP_NSTEP, more_events=more_events)
+                    nstep=SNAP_NSTEP, more_events=more_events, pml_top=pml_top)
 
 
 def oracle_pipeline(job):
@@ -40,19 +41,24 @@ def oracle_pipeline(job):
     from fdfk2d_amd import refstate
     fd, src = job["fd"], job["src"]
     nx, nz, npml, norder = job["nx"], job["nz"], fd["npml"], fd["norder"]
+    izt = job["izt"]
     nt = int(np.rint(fd["tmax"] / fd["dt"]))
     ix = np.arange(1 - npml, nx + npml + 1)
     coordx = (np.float32(fd["x0"]) + (ix - 1).astype(np.float32) * np.float32(fd["dx"])).astype(np.float32)
     bazs = po.backazimuth_grids(fd["lat_org"], fd["lon_org"], fd["az_org"], coordx, src["evla"],
                                 src["evlo"]) - fd["az_org"] * np.pi / 180.0
-    bg = po.background(nx, nz, npml, 1, 2 * norder - 1, nt, src["src_is_p"], fd["x0"], fd["z0"],
+    bg = po.background(nx, nz, npml, izt, 2 * norder - 1, nt, src["src_is_p"], fd["x0"], fd["z0"],
                        fd["dx"], fd["dz"], fd["dt"], src["ds"], src["f0"], src["rayp"], bazs,
                        job["fk_left"], job["fk_right"])
-    st = refstate.make_state(nx, nz, npml, norder, fd["dx"], fd["dz"], fd["dt"], nt, job["c"])
-    st["bg"] = bg
+    st = refstate.make_state(nx, nz, npml, norder, fd["dx"], fd["dz"], fd["dt"], nt, job["c"],
+                             pml_top=izt != 1)
+    if isinstance(bg, tuple):   # with the top ring of an absorbing top layer
+        st["bg"], st["bg_top"] = bg
+    else:
+        st["bg"] = bg
     sx, sz = [], []
     for it in range(nt + 1):
-        row = [po.loadseis(nx, nz, npml, 1, fd["x0"], fd["z0"], fd["dx"], fd["dz"], float(a), float(b),
+        row = [po.loadseis(nx, nz, npml, izt, fd["x0"], fd["z0"], fd["dx"], fd["dz"], float(a), float(b),
                            st["ux"], st["uz"]) for a, b in zip(job["rx"], job["rz"])]
         sx.append([r[0] for r in row])
         sz.append([r[1] for r in row])
@@ -64,12 +70,14 @@ def oracle_pipeline(job):
     return (sx.astype(np.float64) * np.cos(br)[:, None]).astype(np.float32), sz
 
 
-def test_reference_driver_dropin(tmp_path, monkeypatch):
-    print("case: reference Fortran driver linked to libfdfk.so, 101x50, 261 iterations", flush=True)
+@pytest.mark.parametrize("pml_top", [False, True])
+def test_reference_driver_dropin(tmp_path, monkeypatch, pml_top):
+    print(f"case: reference Fortran driver linked to libfdfk.so, 101x50, 261 iterations, is_PML_top={pml_top}",
+          flush=True)
     if not os.path.exists(REF_DRIVER):
         pytest.skip("oracle/_ref/fdfk2d_ref not built (reference tree absent at build time)")
     prefix = str(tmp_path / "input")
-    make_inputs(prefix)
+    make_inputs(prefix, pml_top=pml_top)
     out_ref = tmp_path / "seis_ref"
     out_ref.mkdir()
     (tmp_path / "snapshots").mkdir()   # the reference opens ./snapshots/ without creating it
