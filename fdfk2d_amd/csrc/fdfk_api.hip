@@ -249,13 +249,21 @@ struct fdfk_solver {
     // (is_pml_top, rows from zt = 1-npml).  g.NZ and g.ld then count the top
     // rows; the background arrays, ring fields, receivers and seismograms
     // are the ones above, the fields, materials and strips these.
+    // FDFK_FP64 runs it in double (every array below then holds doubles;
+    // gsize is the element size).
     bool gen = false;
     int zt = 1;
-    float *gux[2] = {nullptr, nullptr}, *guz[2] = {nullptr, nullptr};
-    float *grho = nullptr, *glam = nullptr, *gmu = nullptr;
-    float *gbu[4][2][3] = {};          // split parts (1:2, cells), reference order
-    float *gbpx[4][2] = {}, *gbpz[4][2] = {};
+    int gsize = 4;
+    void *gux[2] = {nullptr, nullptr}, *guz[2] = {nullptr, nullptr};
+    void *grho = nullptr, *glam = nullptr, *gmu = nullptr;
+    void *gbu[4][2][3] = {};           // split parts (1:2, cells), reference order
+    void *gbpx[4][2] = {}, *gbpz[4][2] = {};
     double *gc0 = nullptr, *gc1 = nullptr;
+    void *gdamp[4] = {};               // dpx, dpz, bx, bz
+    void *gring[2] = {};               // ring fields
+    void *gbg[16] = {};                // the twelve arrays and the top ring (FP32: bg[], bg_top[])
+    void *gseis[2] = {};               // seismograms (FP32: seisx, seisz)
+    double pml_ratio_d = 0.1;          // multiaxial ratio of the FP64 path
     float *bg_top[4] = {};             // top ring: uxt_reg, uzt_reg, uxt_pml, uzt_pml
     long bg_top_n[4] = {};
     bool bg_top_valid = false;         // without it the hybrid contour is open at the top
@@ -839,30 +847,31 @@ int one_step(fdfk_solver *s, bool record, int half = 0)
 // ---------------------------------------------------------------------
 // General per-cell path (absorbing top layer)
 // ---------------------------------------------------------------------
-GenArgs<float> gen_args(fdfk_solver *s)
+template <class T>
+GenArgs<T> gen_args(fdfk_solver *s)
 {
-    GenArgs<float> a{};
+    GenArgs<T> a{};
     a.g = s->g;
     a.zt = s->zt;
     a.top = s->zt != 1;
     a.c0 = s->gc0; a.c1 = s->gc1;
-    a.ux = s->gux[s->cur]; a.uz = s->guz[s->cur];
-    a.uxp = s->gux[1 - s->cur]; a.uzp = s->guz[1 - s->cur];
-    a.uxo = s->gux[1 - s->cur]; a.uzo = s->guz[1 - s->cur];
-    a.rho = s->grho; a.lam = s->glam; a.mu = s->gmu;
+    a.ux = (const T *)s->gux[s->cur]; a.uz = (const T *)s->guz[s->cur];
+    a.uxp = (const T *)s->gux[1 - s->cur]; a.uzp = (const T *)s->guz[1 - s->cur];
+    a.uxo = (T *)s->gux[1 - s->cur]; a.uzo = (T *)s->guz[1 - s->cur];
+    a.rho = (const T *)s->grho; a.lam = (const T *)s->glam; a.mu = (const T *)s->gmu;
     for (int st = 0; st < 4; ++st)
         for (int c = 0; c < 2; ++c) {
-            for (int q = 0; q < 3; ++q) a.bu[st][c][q] = s->gbu[st][c][q];
-            a.bpx[st][c] = s->gbpx[st][c];
-            a.bpz[st][c] = s->gbpz[st][c];
+            for (int q = 0; q < 3; ++q) a.bu[st][c][q] = (T *)s->gbu[st][c][q];
+            a.bpx[st][c] = (T *)s->gbpx[st][c];
+            a.bpz[st][c] = (T *)s->gbpz[st][c];
         }
-    a.dpx = s->dpx; a.dpz = s->dpz; a.bx = s->bx; a.bz = s->bz;
-    a.ratio = s->pml_ratio;
-    a.ringx = s->ringx; a.ringz = s->ringz;
+    a.dpx = (const T *)s->gdamp[0]; a.dpz = (const T *)s->gdamp[1];
+    a.bx = (const T *)s->gdamp[2]; a.bz = (const T *)s->gdamp[3];
+    a.ratio = sizeof(T) == 8 ? (T)s->pml_ratio_d : (T)s->pml_ratio;
+    a.ringx = (const T *)s->gring[0]; a.ringz = (const T *)s->gring[1];
     a.ring_valid = s->bg_valid && s->it <= s->nt;
     a.top_ring = s->bg_top_valid;
-    for (int i = 0; i < 12; ++i) a.bg[i] = s->bg[i];
-    for (int i = 0; i < 4; ++i) a.bg[12 + i] = s->bg_top[i];
+    for (int i = 0; i < 16; ++i) a.bg[i] = (const T *)s->gbg[i];
     a.it = s->bg_period > 0 ? s->it % s->bg_period : 0;
     return a;
 }
@@ -870,32 +879,76 @@ GenArgs<float> gen_args(fdfk_solver *s)
 // One iteration of the general path: prep (receivers before the update,
 // ring values, memories), then the cell update; out of place into level
 // n-1, then the levels trade places.
-int gen_step(fdfk_solver *s, bool record)
+template <class T>
+int gen_step_t(fdfk_solver *s, bool record)
 {
     const Geo &g = s->g;
     if (int rc = bg_refill(s, s->stream)) return rc;
-    GenArgs<float> a = gen_args(s);
-    GenRec<float> r{};
+    GenArgs<T> a = gen_args<T>(s);
+    GenRec<T> r{};
     const bool rec = record && s->nrcv > 0 && s->istp <= s->nsis;
     if (rec) {
         r.nrcv = s->nrcv; r.nsamp = s->nsamp; r.istp = s->istp;
-        r.x0 = s->x0; r.z0 = s->z0;
+        r.x0 = (T)s->x0; r.z0 = (T)s->z0;
         r.rx = s->rx; r.rz = s->rz;
-        r.seisx = s->seisx; r.seisz = s->seisz;
+        r.seisx = (T *)s->gseis[0]; r.seisz = (T *)s->gseis[1];
     }
     const int zr = a.top_ring ? 1 - g.nbd : 1;
     const long nrec = rec ? s->nrcv : 0;
-    const long nring = s->ringx ? (long)(g.nx + 2 * g.nbd) * (g.nz + g.nbd - zr + 1) : 0;
+    const long nring = s->gring[0] ? (long)(g.nx + 2 * g.nbd) * (g.nz + g.nbd - zr + 1) : 0;
     const long npsi = 2L * g.npml * g.nz + (s->zt != 1 ? 2L : 1L) * g.NX * g.npml;
     const long np = nrec + nring + npsi;
-    hipLaunchKernelGGL(k_gen_prep<float>, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s->stream, a, r, nrec,
-                       nring, zr, s->ringx, s->ringz);
+    hipLaunchKernelGGL(k_gen_prep<T>, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s->stream, a, r, nrec, nring,
+                       zr, (T *)s->gring[0], (T *)s->gring[1]);
     const long nc = (long)g.NX * g.NZ;
-    hipLaunchKernelGGL(k_gen_step<float>, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s->stream, a);
+    hipLaunchKernelGGL(k_gen_step<T>, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s->stream, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(FDFK_EDEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
     s->cur = 1 - s->cur;
     s->it += 1;
+    return 0;
+}
+int gen_step(fdfk_solver *s, bool record)
+{
+    return s->gsize == 8 ? gen_step_t<double>(s, record) : gen_step_t<float>(s, record);
+}
+
+// damping profiles of the general path in its own type (the FP64 path keeps
+// them unrounded, like the formulation's FP64 build)
+template <class T>
+int gen_damping(fdfk_solver *s)
+{
+    const Geo &g = s->g;
+    const int nx = g.nx, nz = g.nz, npml = g.npml, nbd = g.nbd;
+    const int leff = npml - nbd;
+    const double lr = std::log(s->R0);
+    const double d0x = leff > 0 ? -(double)(s->npower + 1) * (double)s->vpml * lr / (2.0 * leff * (double)g.dx) : 0.0;
+    const double d0z = leff > 0 ? -(double)(s->npower + 1) * (double)s->vpml * lr / (2.0 * leff * (double)g.dz) : 0.0;
+    auto prof = [&](int q, double d0) {
+        if (q <= nbd || leff <= 0) return 0.0;
+        double r = (double)(q - nbd) / (double)leff, p = 1.0;
+        for (int e = 0; e < s->npower; ++e) p *= r;
+        return d0 * p;
+    };
+    std::vector<T> dx(g.NX), bx(g.NX), dz(g.NZ), bz(g.NZ);
+    for (int j = 0; j < g.NX; ++j) {
+        const int ix = j + 1 - npml;
+        const int q = ix <= 0 ? 1 - ix : (ix > nx ? ix - nx : 0);
+        dx[j] = (T)prof(q, d0x);
+        bx[j] = (T)std::exp(-(double)dx[j] * g.dt);
+    }
+    for (int k = 0; k < g.NZ; ++k) {
+        const int iz = k + s->zt;
+        dz[k] = iz > nz ? (T)prof(iz - nz, d0z) : (iz <= 0 ? (T)prof(1 - iz, d0z) : (T)0);
+        bz[k] = (T)std::exp(-(double)dz[k] * g.dt);
+    }
+    const std::vector<T> *src[4] = {&dx, &dz, &bx, &bz};
+    for (int i = 0; i < 4; ++i) {
+        T *d = nullptr;
+        if (int rc = s->dalloc(&d, src[i]->size())) return rc;
+        HIPCHK(h2d(s->stream, d, src[i]->data(), src[i]->size() * sizeof(T)));
+        s->gdamp[i] = d;
+    }
     return 0;
 }
 
@@ -915,8 +968,8 @@ int validate_geometry(int nx, int nz, int npml, int norder, float dx, float dz, 
 int validate(const fdfk_job *j)
 {
     if (!j) return fail(FDFK_EINVAL, "null job");
-    if (j->is_pml_top && j->precision != FDFK_FP32)
-        return fail(FDFK_ENOSYS, "an absorbing top layer (is_pml_top) runs in FP32 storage only");
+    if (j->is_pml_top && j->precision == FDFK_FP16)
+        return fail(FDFK_ENOSYS, "an absorbing top layer (is_pml_top) runs in FP32 or FP64 storage");
     if (int rc = validate_geometry(j->nx, j->nz, j->npml, j->norder, j->dx, j->dz, j->dt, j->nt)) return rc;
     if (j->nstp < 1) return fail(FDFK_EINVAL, "nstp >= 1 required");
     if (j->nrcv < 0 || (j->nrcv > 0 && (!j->rx || !j->rz))) return fail(FDFK_EINVAL, "receiver arrays");
@@ -924,8 +977,8 @@ int validate(const fdfk_job *j)
         if (!std::isfinite(j->rx[i]) || !std::isfinite(j->rz[i]))
             return fail(FDFK_EINVAL, "receiver coordinates must be finite");
     if (!j->c) return fail(FDFK_EINVAL, "material array c is required");
-    if (j->precision != FDFK_FP32 && j->precision != FDFK_FP16)
-        return fail(FDFK_ENOSYS, "storage precision: FP32, or FP16 materials (BF16 is not built)");
+    if (j->precision != FDFK_FP32 && j->precision != FDFK_FP16 && j->precision != FDFK_FP64)
+        return fail(FDFK_ENOSYS, "storage precision: FP32, FP16 materials or FP64 (BF16 is not built)");
     return 0;
 }
 
@@ -941,7 +994,8 @@ int fdfk_autotune(const fdfk_job *job, int32_t steps, int32_t *best_lx, float *b
                   float *model_ms)
 {
     if (!job || steps < 1) return fail(FDFK_EINVAL, "job required and steps >= 1");
-    if (job->is_pml_top) return fail(FDFK_ENOSYS, "the tile autotuner covers the free-surface kernels only");
+    if (job->is_pml_top || job->precision == FDFK_FP64)
+        return fail(FDFK_ENOSYS, "the tile autotuner covers the tiled FP32 kernels only");
     // scratch solver of the same geometry, set up for every timed step
     std::vector<int> cand;
     fdfk_job j = *job;
@@ -1021,15 +1075,20 @@ int fdfk_trim_memory(void)
     return FDFK_OK;
 }
 
-// A solver of the general path: the absorbing top layer (is_pml_top, rows
-// from zt = 1-npml); c holds rows zt..nz+npml
+// A solver of the general path (fdfk_general.hip) in element type T: the
+// absorbing top layer (is_pml_top, rows from zt = 1-npml; c holds rows
+// zt..nz+npml) and the FP64 storage mode (FDFK_FP64, either surface)
+extern "C++" {
+template <class T>
 static int gen_create(const fdfk_job *job, fdfk_solver **out)
 {
     auto *s = new fdfk_solver();
     auto bad = [&](int code) { delete s; return code; };
     int rc = 0;
     s->gen = true;
-    s->zt = 1 - job->npml;
+    s->gsize = (int)sizeof(T);
+    s->precision = job->precision;
+    s->zt = job->is_pml_top ? 1 - job->npml : 1;
     Geo &g = s->g;
     g.nx = job->nx; g.nz = job->nz; g.npml = job->npml; g.norder = job->norder;
     g.nbd = 2 * g.norder - 1;
@@ -1043,7 +1102,10 @@ static int gen_create(const fdfk_job *job, fdfk_solver **out)
     s->x0 = job->x0; s->z0 = job->z0;
     s->R0 = job->R0; s->npower = job->npower;
     s->fused = false;
-    if (const char *e = std::getenv("FDFK_PML_RATIO")) s->pml_ratio = (float)std::atof(e);
+    if (const char *e = std::getenv("FDFK_PML_RATIO")) {
+        s->pml_ratio = (float)std::atof(e);
+        s->pml_ratio_d = std::atof(e);
+    }
     const long ncell = (long)g.NX * g.NZ;
     float vpml = job->vpml;
     if (!(vpml > 0)) {
@@ -1059,14 +1121,18 @@ static int gen_create(const fdfk_job *job, fdfk_solver **out)
     if (hipGetDevice(&s->device) != hipSuccess || hipStreamCreate(&s->stream) != hipSuccess)
         return bad(fail(FDFK_EDEVICE, "hipStreamCreate failed"));
     const size_t nf = (size_t)g.NX * g.ld;
-    for (int q = 0; q < 2; ++q) {
-        if ((rc = s->dalloc(&s->gux[q], nf)) || (rc = s->dalloc(&s->guz[q], nf))) return bad(rc);
-        if (hipMemsetAsync(s->gux[q], 0, nf * 4, s->stream) || hipMemsetAsync(s->guz[q], 0, nf * 4, s->stream))
-            return bad(fail(FDFK_EDEVICE, "memset"));
-    }
-    if ((rc = s->dalloc(&s->grho, nf)) || (rc = s->dalloc(&s->glam, nf)) || (rc = s->dalloc(&s->gmu, nf)) ||
+    auto zalloc = [&](void **dst, size_t n) -> int {
+        T *p = nullptr;
+        if (int r = s->dalloc(&p, n)) return r;
+        if (hipMemsetAsync(p, 0, n * sizeof(T), s->stream) != hipSuccess) return fail(FDFK_EDEVICE, "memset");
+        *dst = p;
+        return 0;
+    };
+    for (int q = 0; q < 2; ++q)
+        if ((rc = zalloc(&s->gux[q], nf)) || (rc = zalloc(&s->guz[q], nf))) return bad(rc);
+    if ((rc = zalloc(&s->grho, nf)) || (rc = zalloc(&s->glam, nf)) || (rc = zalloc(&s->gmu, nf)) ||
         (rc = s->dalloc(&s->gc0, (size_t)kMaxOrder * kMaxOrder)) || (rc = s->dalloc(&s->gc1, (size_t)kMaxOrder * kMaxOrder)) ||
-        (rc = s->dalloc(&s->stage, (size_t)2 * ncell)))
+        (rc = s->dalloc(&s->stage, (size_t)2 * ncell * (sizeof(T) / 4))))
         return bad(rc);
     if (h2d(s->stream, s->gc0, s->hcoef.c0, sizeof(s->hcoef.c0)) != hipSuccess ||
         h2d(s->stream, s->gc1, s->hcoef.c1, sizeof(s->hcoef.c1)) != hipSuccess)
@@ -1079,37 +1145,30 @@ static int gen_create(const fdfk_job *job, fdfk_solver **out)
             hipFreeAsync(dc, s->stream);
             return bad(fail(FDFK_EDEVICE, "material upload"));
         }
-        hipLaunchKernelGGL(k_gen_import_material<float>, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0,
-                           s->stream, g, s->zt, dc, s->grho, s->glam, s->gmu);
+        hipLaunchKernelGGL(k_gen_import_material<T>, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0,
+                           s->stream, g, s->zt, dc, (T *)s->grho, (T *)s->glam, (T *)s->gmu);
         hipFreeAsync(dc, s->stream);
     }
-    // strips L, R, B, T: split parts with two interleaved levels, x-memories,
-    // z-memories in the bottom and top strips
+    // strips L, R, B (and T with a top layer): split parts with two
+    // interleaved levels, x-memories, z-memories in the bottom and top strips
     const size_t ncs[4] = {(size_t)g.npml * g.nz, (size_t)g.npml * g.nz, (size_t)g.NX * g.npml, (size_t)g.NX * g.npml};
-    for (int st = 0; st < 4; ++st)
+    for (int st = 0; st < (s->zt != 1 ? 4 : 3); ++st)
         for (int c = 0; c < 2; ++c) {
-            for (int q = 0; q < 3; ++q) {
-                if ((rc = s->dalloc(&s->gbu[st][c][q], 2 * ncs[st]))) return bad(rc);
-                hipMemsetAsync(s->gbu[st][c][q], 0, 2 * ncs[st] * 4, s->stream);
-            }
-            if ((rc = s->dalloc(&s->gbpx[st][c], ncs[st]))) return bad(rc);
-            hipMemsetAsync(s->gbpx[st][c], 0, ncs[st] * 4, s->stream);
-            if (st >= 2) {
-                if ((rc = s->dalloc(&s->gbpz[st][c], ncs[st]))) return bad(rc);
-                hipMemsetAsync(s->gbpz[st][c], 0, ncs[st] * 4, s->stream);
-            }
+            for (int q = 0; q < 3; ++q)
+                if ((rc = zalloc(&s->gbu[st][c][q], 2 * ncs[st]))) return bad(rc);
+            if ((rc = zalloc(&s->gbpx[st][c], ncs[st]))) return bad(rc);
+            if (st >= 2 && (rc = zalloc(&s->gbpz[st][c], ncs[st]))) return bad(rc);
         }
-    if ((rc = build_damping(s))) return bad(rc);
+    if ((rc = gen_damping<T>(s))) return bad(rc);
     if (s->nrcv > 0) {
         if ((rc = s->dalloc(&s->rx, s->nrcv)) || (rc = s->dalloc(&s->rz, s->nrcv)) ||
-            (rc = s->dalloc(&s->seisx, (size_t)s->nrcv * s->nsamp)) ||
-            (rc = s->dalloc(&s->seisz, (size_t)s->nrcv * s->nsamp)))
+            (rc = zalloc(&s->gseis[0], (size_t)s->nrcv * s->nsamp)) ||
+            (rc = zalloc(&s->gseis[1], (size_t)s->nrcv * s->nsamp)))
             return bad(rc);
         if (h2d(s->stream, s->rx, job->rx, s->nrcv * 4) != hipSuccess ||
             h2d(s->stream, s->rz, job->rz, s->nrcv * 4) != hipSuccess)
             return bad(fail(FDFK_EDEVICE, "receiver upload"));
-        hipMemsetAsync(s->seisx, 0, (size_t)s->nrcv * s->nsamp * 4, s->stream);
-        hipMemsetAsync(s->seisz, 0, (size_t)s->nrcv * s->nsamp * 4, s->stream);
+        if (sizeof(T) == 4) { s->seisx = (float *)s->gseis[0]; s->seisz = (float *)s->gseis[1]; }
     }
     hipError_t e = hipStreamSynchronize(s->stream);
     if (e == hipSuccess) e = hipGetLastError();
@@ -1117,6 +1176,7 @@ static int gen_create(const fdfk_job *job, fdfk_solver **out)
     *out = s;
     return 0;
 }
+}  // extern "C++"
 
 int fdfk_create(const fdfk_job *job, fdfk_solver **out)
 {
@@ -1124,7 +1184,8 @@ int fdfk_create(const fdfk_job *job, fdfk_solver **out)
     *out = nullptr;
     int rc = validate(job);
     if (rc) return rc;
-    if (job->is_pml_top) return gen_create(job, out);
+    if (job->precision == FDFK_FP64) return gen_create<double>(job, out);
+    if (job->is_pml_top) return gen_create<float>(job, out);
     auto *s = new fdfk_solver();
     Geo &g = s->g;
     g.nx = job->nx; g.nz = job->nz; g.npml = job->npml; g.norder = job->norder;
@@ -1328,6 +1389,19 @@ static int alloc_background(fdfk_solver *s, int period)
             s->bg_top_n[i] = nt4[i];
         }
     }
+    if (s->gen) {
+        // the general path's background in its own type: FP32 shares the
+        // arrays above, FP64 has double copies
+        for (int i = 0; i < 16; ++i) {
+            float *f = i < 12 ? s->bg[i] : s->bg_top[i - 12];
+            const long n = i < 12 ? s->bg_n[i] : s->bg_top_n[i - 12];
+            if (!f) continue;
+            if (s->gsize == 4) { s->gbg[i] = f; continue; }
+            double *d = nullptr;
+            if (int rc = s->dalloc(&d, (size_t)n)) return rc;
+            s->gbg[i] = d;
+        }
+    }
     h.valid = 1;
     h.period = period;
     s->bg_period = period;
@@ -1339,6 +1413,14 @@ static int alloc_background(fdfk_solver *s, int period)
         if ((rc = s->dalloc(&s->ringx, nf)) || (rc = s->dalloc(&s->ringz, nf))) return rc;
         HIPCHK(hipMemsetAsync(s->ringx, 0, nf * 4, s->stream));
         HIPCHK(hipMemsetAsync(s->ringz, 0, nf * 4, s->stream));
+        s->gring[0] = s->ringx; s->gring[1] = s->ringz;
+        if (s->gen && s->gsize == 8) {
+            double *rx = nullptr, *rz = nullptr;
+            if ((rc = s->dalloc(&rx, nf)) || (rc = s->dalloc(&rz, nf))) return rc;
+            HIPCHK(hipMemsetAsync(rx, 0, nf * 8, s->stream));
+            HIPCHK(hipMemsetAsync(rz, 0, nf * 8, s->stream));
+            s->gring[0] = rx; s->gring[1] = rz;
+        }
     }
     HIPCHK(hipStreamSynchronize(s->stream));
     return 0;
@@ -1356,6 +1438,26 @@ int fdfk_set_background(fdfk_solver *s, const float *const bg[12])
     for (int i = 0; i < 12; ++i) {
         if (!bg[i]) return fail(FDFK_EINVAL, "null background array");
         HIPCHK(hipMemcpyAsync(s->bg[i], bg[i], (size_t)n[i] * 4, hipMemcpyHostToDevice, s->stream));
+        if (s->gen && s->gsize == 8)
+            hipLaunchKernelGGL((k_convert<double, float>), dim3((unsigned)((n[i] + 255) / 256)), dim3(256), 0,
+                               s->stream, n[i], s->bg[i], (double *)s->gbg[i]);
+    }
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->bg_valid = true;
+    return 0;
+}
+
+int fdfk_set_background64(fdfk_solver *s, const double *const bg[12])
+{
+    if (!s || !bg) return fail(FDFK_EINVAL, "null argument");
+    if (!(s->gen && s->gsize == 8)) return fail(FDFK_EINVAL, "an FP64 background needs an FDFK_FP64 solver");
+    int rc = alloc_background(s, s->nt + 1);
+    if (rc) return rc;
+    s->bg_block = 0;
+    s->bg_fill = nullptr;
+    for (int i = 0; i < 12; ++i) {
+        if (!bg[i]) return fail(FDFK_EINVAL, "null background array");
+        HIPCHK(hipMemcpyAsync(s->gbg[i], bg[i], (size_t)s->bg_n[i] * 8, hipMemcpyHostToDevice, s->stream));
     }
     HIPCHK(hipStreamSynchronize(s->stream));
     s->bg_valid = true;
@@ -1366,6 +1468,7 @@ int fdfk_get_background(fdfk_solver *s, float *const bg[12])
 {
     if (!s || !bg) return fail(FDFK_EINVAL, "null argument");
     if (!s->bg_valid) return fail(FDFK_EINVAL, "no background on the device");
+    if (s->gen && s->gsize == 8) return fail(FDFK_EINVAL, "an FDFK_FP64 solver keeps its background in FP64 only");
     if (s->bg_period != s->nt + 1)
         return fail(FDFK_EINVAL, "the background is synthesised in blocks; its steps are not all stored");
     for (int i = 0; i < 12; ++i)
@@ -1383,6 +1486,25 @@ int fdfk_set_top_background(fdfk_solver *s, const float *const top[4])
     for (int i = 0; i < 4; ++i) {
         if (!top[i]) return fail(FDFK_EINVAL, "null background array");
         HIPCHK(hipMemcpyAsync(s->bg_top[i], top[i], (size_t)s->bg_top_n[i] * 4, hipMemcpyHostToDevice, s->stream));
+        if (s->gen && s->gsize == 8)
+            hipLaunchKernelGGL((k_convert<double, float>), dim3((unsigned)((s->bg_top_n[i] + 255) / 256)), dim3(256),
+                               0, s->stream, s->bg_top_n[i], s->bg_top[i], (double *)s->gbg[12 + i]);
+    }
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->bg_top_valid = true;
+    return 0;
+}
+
+int fdfk_set_top_background64(fdfk_solver *s, const double *const top[4])
+{
+    if (!s || !top) return fail(FDFK_EINVAL, "null argument");
+    if (!(s->gen && s->gsize == 8)) return fail(FDFK_EINVAL, "an FP64 background needs an FDFK_FP64 solver");
+    if (s->zt == 1) return fail(FDFK_EINVAL, "the top ring exists with an absorbing top layer (is_pml_top) only");
+    int rc = alloc_background(s, s->nt + 1);
+    if (rc) return rc;
+    for (int i = 0; i < 4; ++i) {
+        if (!top[i]) return fail(FDFK_EINVAL, "null background array");
+        HIPCHK(hipMemcpyAsync(s->gbg[12 + i], top[i], (size_t)s->bg_top_n[i] * 8, hipMemcpyHostToDevice, s->stream));
     }
     HIPCHK(hipStreamSynchronize(s->stream));
     s->bg_top_valid = true;
@@ -1438,10 +1560,28 @@ int fdfk_sync(fdfk_solver *s)
     return 0;
 }
 
+int fdfk_seismograms64(fdfk_solver *s, double *sx, double *sz)
+{
+    if (!s || !sx || !sz) return fail(FDFK_EINVAL, "null argument");
+    if (!(s->gen && s->gsize == 8)) return fail(FDFK_EINVAL, "FP64 seismograms need an FDFK_FP64 solver");
+    if (s->nrcv == 0) return 0;
+    size_t n = (size_t)s->nrcv * s->nsamp * 8;
+    HIPCHK(hipMemcpyAsync(sx, s->gseis[0], n, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipMemcpyAsync(sz, s->gseis[1], n, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
 int fdfk_seismograms(fdfk_solver *s, float *sx, float *sz)
 {
     if (!s || !sx || !sz) return fail(FDFK_EINVAL, "null argument");
     if (s->nrcv == 0) return 0;
+    if (s->gen && s->gsize == 8) {
+        std::vector<double> a((size_t)s->nrcv * s->nsamp), b(a.size());
+        if (int rc = fdfk_seismograms64(s, a.data(), b.data())) return rc;
+        for (size_t i = 0; i < a.size(); ++i) { sx[i] = (float)a[i]; sz[i] = (float)b[i]; }
+        return 0;
+    }
     size_t n = (size_t)s->nrcv * s->nsamp * 4;
     HIPCHK(hipMemcpyAsync(sx, s->seisx, n, hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipMemcpyAsync(sz, s->seisz, n, hipMemcpyDeviceToHost, s->stream));
@@ -1457,9 +1597,13 @@ int fdfk_get_state(fdfk_solver *s, float *ux, float *uz)
     dim3 gr((unsigned)((n + 255) / 256)), bl(256);
     if (s->gen) {
         for (int c = 0; c < 2; ++c) {
-            float *const *f = c == 0 ? s->gux : s->guz;
-            hipLaunchKernelGGL((k_gen_levels<float, float>), gr, bl, 0, s->stream, g, 1, s->stage, f[s->cur],
-                               f[1 - s->cur]);
+            void *const *f = c == 0 ? s->gux : s->guz;
+            if (s->gsize == 8)
+                hipLaunchKernelGGL((k_gen_levels<double, float>), gr, bl, 0, s->stream, g, 1, s->stage,
+                                   (double *)f[s->cur], (double *)f[1 - s->cur]);
+            else
+                hipLaunchKernelGGL((k_gen_levels<float, float>), gr, bl, 0, s->stream, g, 1, s->stage,
+                                   (float *)f[s->cur], (float *)f[1 - s->cur]);
             HIPCHK(hipMemcpyAsync(c == 0 ? ux : uz, s->stage, (size_t)2 * n * 4, hipMemcpyDeviceToHost, s->stream));
             HIPCHK(hipStreamSynchronize(s->stream));
         }
@@ -1474,6 +1618,23 @@ int fdfk_get_state(fdfk_solver *s, float *ux, float *uz)
     return 0;
 }
 
+int fdfk_get_state64(fdfk_solver *s, double *ux, double *uz)
+{
+    if (!s || !ux || !uz) return fail(FDFK_EINVAL, "null argument");
+    if (!(s->gen && s->gsize == 8)) return fail(FDFK_EINVAL, "FP64 fields need an FDFK_FP64 solver");
+    const Geo &g = s->g;
+    long n = (long)g.NX * g.NZ;
+    dim3 gr((unsigned)((n + 255) / 256)), bl(256);
+    for (int c = 0; c < 2; ++c) {
+        void *const *f = c == 0 ? s->gux : s->guz;
+        hipLaunchKernelGGL((k_gen_levels<double, double>), gr, bl, 0, s->stream, g, 1, (double *)s->stage,
+                           (double *)f[s->cur], (double *)f[1 - s->cur]);
+        HIPCHK(hipMemcpyAsync(c == 0 ? ux : uz, s->stage, (size_t)2 * n * 8, hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+    }
+    return 0;
+}
+
 int fdfk_set_state(fdfk_solver *s, const float *ux, const float *uz)
 {
     if (!s || !ux || !uz) return fail(FDFK_EINVAL, "null argument");
@@ -1482,10 +1643,14 @@ int fdfk_set_state(fdfk_solver *s, const float *ux, const float *uz)
     dim3 gr((unsigned)((n + 255) / 256)), bl(256);
     if (s->gen) {
         for (int c = 0; c < 2; ++c) {
-            float *const *f = c == 0 ? s->gux : s->guz;
+            void *const *f = c == 0 ? s->gux : s->guz;
             HIPCHK(hipMemcpyAsync(s->stage, c == 0 ? ux : uz, (size_t)2 * n * 4, hipMemcpyHostToDevice, s->stream));
-            hipLaunchKernelGGL((k_gen_levels<float, float>), gr, bl, 0, s->stream, g, 0, s->stage, f[s->cur],
-                               f[1 - s->cur]);
+            if (s->gsize == 8)
+                hipLaunchKernelGGL((k_gen_levels<double, float>), gr, bl, 0, s->stream, g, 0, s->stage,
+                                   (double *)f[s->cur], (double *)f[1 - s->cur]);
+            else
+                hipLaunchKernelGGL((k_gen_levels<float, float>), gr, bl, 0, s->stream, g, 0, s->stage,
+                                   (float *)f[s->cur], (float *)f[1 - s->cur]);
             HIPCHK(hipStreamSynchronize(s->stream));
         }
         return 0;

@@ -237,13 +237,15 @@ struct FillArgs {
     int nfft, nzq;
     double dt, dtf, x0, dx, p, sgn, xe;
     const double *gl, *gr;      // time traces [comp][q][nfft] of the left / right model
-    float *ax, *az;             // destination arrays (one strip)
     int strip;                  // 0 L reg, 1 R reg, 2 B reg, 3 L pml, 4 R pml, 5 B pml, 6 T reg, 7 T pml
     int rows, cols, row0, col0; // strip shape: rows row0.., columns col0..
     int zf;                     // row of trace depth index 0
 };
 
-__global__ void k_fk_fill(FillArgs f)
+// ax, az: the destination arrays of the strip, FP32 (the reference's
+// arrays) or FP64 (the FP64 storage mode)
+template <class R>
+__global__ void k_fk_fill(FillArgs f, R *ax, R *az)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     long per = (long)f.rows * f.cols;
@@ -273,8 +275,8 @@ __global__ void k_fk_fill(FillArgs f)
         bz = (1.0 - w) * lz + w * rz;
     }
     size_t o = (size_t)(iz - f.row0) + (size_t)f.rows * ((size_t)(ix - f.col0) + (size_t)f.cols * (it % f.period));
-    f.ax[o] = (float)(f.sgn * bx);
-    f.az[o] = (float)bz;
+    ax[o] = (R)(f.sgn * bx);
+    az[o] = (R)bz;
 }
 
 }  // namespace fdfk
@@ -528,19 +530,25 @@ extern "C" int fdfk_compute_background(fdfk_solver *s, const fdfk_fk *fk)
                                                    {nbd, nx, nz - nbd + 1, 1}, {nz, nbd, 1, 1 - nbd},
                                                    {nz, nbd, 1, nx + 1}, {nbd, nx + 2 * nbd, nz + 1, 1 - nbd},
                                                    {nbd, nx - 2 * nbd, 1, nbd + 1}, {nbd, nx + 2 * nbd, 1 - nbd, 1 - nbd}}};
-    std::array<float *, 16> dst{};
-    for (int i = 0; i < 12; ++i) dst[i] = s->bg[i];
-    for (int i = 0; i < 4; ++i) dst[12 + i] = s->bg_top[i];
+    // destinations: the reference's FP32 arrays, or the FP64 path's copies
+    const bool f64 = s->gen && s->gsize == 8;
+    std::array<void *, 16> dst{};
+    for (int i = 0; i < 12; ++i) dst[i] = f64 ? s->gbg[i] : (void *)s->bg[i];
+    for (int i = 0; i < 4; ++i) dst[12 + i] = f64 ? s->gbg[12 + i] : (void *)s->bg_top[i];
     const int nstrip = top ? 8 : 6;
-    auto fill = [f, shp, dst, nstrip](hipStream_t st, int t0, int t1) -> int {
+    auto fill = [f, shp, dst, nstrip, f64](hipStream_t st, int t0, int t1) -> int {
         FillArgs a = f;
         a.t0 = t0; a.t1 = t1;
         for (int k = 0; k < nstrip; ++k) {
             a.strip = k;
             a.rows = shp[k][0]; a.cols = shp[k][1]; a.row0 = shp[k][2]; a.col0 = shp[k][3];
-            a.ax = dst[2 * k]; a.az = dst[2 * k + 1];
             const long n = (long)a.rows * a.cols * (t1 - t0);
-            hipLaunchKernelGGL(k_fk_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a);
+            if (f64)
+                hipLaunchKernelGGL(k_fk_fill<double>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a,
+                                   (double *)dst[2 * k], (double *)dst[2 * k + 1]);
+            else
+                hipLaunchKernelGGL(k_fk_fill<float>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a,
+                                   (float *)dst[2 * k], (float *)dst[2 * k + 1]);
         }
         hipError_t e = hipGetLastError();
         return e == hipSuccess ? 0 : fail(FDFK_EDEVICE, std::string("background fill: ") + hipGetErrorString(e));

@@ -541,4 +541,13 @@ __global__ void k_gen_levels(Geo g, int to_ref, R *ref2, T *cur, T *prev)
     }
 }
 
+// element-type conversion of a contiguous array (widen FP32 inputs for the
+// FP64 path, round FP64 results for FP32 getters)
+template <class D, class S>
+__global__ void k_convert(long n, const S *src, D *dst)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) dst[t] = (D)src[t];
+}
+
 }  // namespace fdfk

@@ -11,7 +11,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfdfk.so")
 
-FDFK_FP32, FDFK_BF16, FDFK_FP16 = 0, 1, 2
+FDFK_FP32, FDFK_BF16, FDFK_FP16, FDFK_FP64 = 0, 1, 2, 3
 
 
 class FdfkJob(C.Structure):
@@ -56,6 +56,10 @@ def lib():
         L.fdfk_get_background.argtypes = [C.c_void_p, C.c_void_p]
         L.fdfk_set_top_background.argtypes = [C.c_void_p, C.c_void_p]
         L.fdfk_get_top_background.argtypes = [C.c_void_p, C.c_void_p]
+        for name in ("fdfk_set_background64", "fdfk_set_top_background64"):
+            getattr(L, name).argtypes = [C.c_void_p, C.c_void_p]
+        for name in ("fdfk_seismograms64", "fdfk_get_state64"):
+            getattr(L, name).argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.fdfk_compute_background.argtypes = [C.c_void_p, C.POINTER(FdfkFK)]
         L.fdfk_seismograms.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.fdfk_get_state.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
@@ -143,6 +147,7 @@ class Solver:
         then holds rows izt..nz+npml."""
         self.nx, self.nz, self.npml, self.norder, self.nt = nx, nz, npml, norder, nt
         self.nbd = 2 * norder - 1
+        self.precision = precision
         self.pml_top = bool(pml_top)
         self.izt = 1 - npml if pml_top else 1
         self.NX, self.NZ = nx + 2 * npml, nz + npml - self.izt + 1
@@ -170,15 +175,26 @@ class Solver:
 
     def set_background(self, arrays):
         """The twelve reference arrays, or (twelve, top ring's four) with an
-        absorbing top layer."""
+        absorbing top layer.  float64 arrays go to an FDFK_FP64 solver
+        unrounded."""
+        wide = self.precision == FDFK_FP64 and np.asarray(arrays[0][0] if isinstance(arrays, tuple)
+                                                           else arrays[0]).dtype == np.float64
+        dt, suffix = (np.float64, "64") if wide else (np.float32, "")
         if isinstance(arrays, tuple) and len(arrays) == 2:
             arrays, top = arrays
             self.set_background(arrays)
-            tops = [np.ascontiguousarray(a, np.float32) for a in top]
+            tops = [np.ascontiguousarray(a, dt) for a in top]
             for a, n in zip(tops, bg_top_sizes(self.nx, self.nbd, self.nt)):
                 assert a.size == n, (a.size, n)
             tp = (C.c_void_p * 4)(*[a.ctypes.data for a in tops])
-            _check(lib().fdfk_set_top_background(self.h, tp))
+            _check(getattr(lib(), "fdfk_set_top_background" + suffix)(self.h, tp))
+            return
+        if wide:
+            arrs = [np.ascontiguousarray(a, np.float64) for a in arrays]
+            for a, n in zip(arrs, bg_sizes(self.nx, self.nz, self.nbd, self.nt)):
+                assert a.size == n, (a.size, n)
+            ptrs = (C.c_void_p * 12)(*[a.ctypes.data for a in arrs])
+            _check(lib().fdfk_set_background64(self.h, ptrs))
             return
         arrs = [np.ascontiguousarray(a, np.float32) for a in arrays]
         sizes = bg_sizes(self.nx, self.nz, self.nbd, self.nt)
@@ -222,13 +238,25 @@ class Solver:
         return lib().fdfk_iteration(self.h)
 
     def seismograms(self):
+        """(nrcv, nsamp) traces; float64 from an FDFK_FP64 solver."""
+        if self.precision == FDFK_FP64:
+            sx = np.zeros((self.nrcv, self.nsamp), np.float64)
+            sz = np.zeros((self.nrcv, self.nsamp), np.float64)
+            _check(lib().fdfk_seismograms64(self.h, _ptr(sx), _ptr(sz)))
+            return sx, sz
         sx = np.zeros((self.nrcv, self.nsamp), np.float32)
         sz = np.zeros((self.nrcv, self.nsamp), np.float32)
         _check(lib().fdfk_seismograms(self.h, _ptr(sx), _ptr(sz)))
         return sx, sz
 
     def get_state(self):
+        """Both levels of ux, uz in the reference layout; float64 from an
+        FDFK_FP64 solver."""
         n = 2 * self.NX * self.NZ
+        if self.precision == FDFK_FP64:
+            ux, uz = np.zeros(n, np.float64), np.zeros(n, np.float64)
+            _check(lib().fdfk_get_state64(self.h, _ptr(ux), _ptr(uz)))
+            return ux, uz
         ux, uz = np.zeros(n, np.float32), np.zeros(n, np.float32)
         _check(lib().fdfk_get_state(self.h, _ptr(ux), _ptr(uz)))
         return ux, uz

@@ -32,10 +32,15 @@ extern "C" {
 #define FDFK_ERANGE (-34)
 #define FDFK_ENOSYS (-38)
 
-/* storage precision of wavefield and material arrays (arithmetic is FP32) */
+/* storage precision of wavefield and material arrays (arithmetic is FP32,
+ * FP64 for FDFK_FP64) */
 #define FDFK_FP32 0
 #define FDFK_BF16 1
 #define FDFK_FP16 2
+/* every array and all arithmetic in FP64 (per-cell kernels; the reference
+ * of the FP32 and 16-bit storage modes at full size; no counterpart in the
+ * reference, which is FP32) */
+#define FDFK_FP64 3
 
 typedef struct fdfk_solver fdfk_solver;
 
@@ -56,7 +61,8 @@ typedef struct fdfk_job {
     const float *c;                 /* rho, lambda, mu (reference layout) */
     int32_t precision;              /* FDFK_FP32; FDFK_FP16: lam and mu stored as FP16 with
                                        power-of-two scales for the interior tiles (single
-                                       solves, fused step); FDFK_BF16: not built */
+                                       solves, fused step); FDFK_FP64: everything in FP64
+                                       (per-cell kernels, single solves); FDFK_BF16: not built */
 } fdfk_job;
 
 /* FK background description: the inputs of comput_background_wavefield
@@ -93,6 +99,13 @@ int fdfk_compute_background(fdfk_solver *s, const fdfk_fk *fk);
  * columns 1-nbd..nx+nbd, z fastest.  Without it the contour stays open. */
 int fdfk_set_top_background(fdfk_solver *s, const float *const top[4]);
 int fdfk_get_top_background(fdfk_solver *s, float *const top[4]);
+/* FDFK_FP64 solvers: the background in double (the FP32 setters widen),
+ * seismograms and fields in double (the FP32 getters round); no counterpart
+ * in the reference */
+int fdfk_set_background64(fdfk_solver *s, const double *const bg[12]);
+int fdfk_set_top_background64(fdfk_solver *s, const double *const top[4]);
+int fdfk_seismograms64(fdfk_solver *s, double *seisx, double *seisz);
+int fdfk_get_state64(fdfk_solver *s, double *ux, double *uz);
 /* copy the device background back in the reference layout (twelve arrays);
  * FDFK_EINVAL when it is synthesised in blocks */
 int fdfk_get_background(fdfk_solver *s, float *const bg[12]);
