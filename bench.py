@@ -371,6 +371,76 @@ def c3_bench(njobs=64, block=512, log=lambda *a: None):
             **out}
 
 
+def c4_bench(nsteps=8000, rf_every=20, block=512, log=lambda *a: None):
+    """The storage-precision workload: the single large solve (8001 x 2000)
+    for nsteps iterations in FP64 (the general per-cell kernels in double,
+    the reference), FP32 (the tiled kernels) and FP32 with FP16 materials;
+    rel-L2 of every receiver trace and of the P receiver functions of every
+    rf_every-th receiver against the FP64 run (RF spike picks of the FP64
+    run replayed, so that the comparison measures the traces' precision and
+    not the discrete decisions of the deconvolution)."""
+    from fdfk2d_amd import configs, lib, rf
+    cfg = configs.c2(nt=nsteps)
+    NX, NZ = cfg["NX"], cfg["NZ"]
+    runs = {}
+    for name, prec in (("fp64", lib.FDFK_FP64), ("fp32", lib.FDFK_FP32), ("fp16_materials", lib.FDFK_FP16)):
+        log(f"c4: {name}, {nsteps + 1} iterations")
+        s = lib.Solver(cfg["nx"], cfg["nz"], cfg["npml"], cfg["norder"], cfg["dx"], cfg["dz"], cfg["dt"], nsteps,
+                       cfg["c"], rx=cfg["rx"], rz=cfg["rz"], precision=prec)
+        try:
+            s.compute_background(cfg["fk_left"], cfg["fk_right"], cfg["src_is_p"], cfg["f0"], cfg["ds"],
+                                 cfg["rayp"], cfg["bazs"], block_steps=block)
+            s.sync()
+            t0 = time.perf_counter()
+            s.run(nsteps + 1)
+            s.sync()
+            t = time.perf_counter() - t0
+            sx, sz = s.seismograms()
+            dev = s.device_bytes
+        finally:
+            s.close()
+        runs[name] = (np.asarray(sx, np.float64), np.asarray(sz, np.float64), t, dev)
+        log(f"c4: {name} {t:.2f} s ({NX * NZ * (nsteps + 1) / t / 1e9:.2f} Gcell/s)")
+    rx64, rz64 = runs["fp64"][0], runs["fp64"][1]
+    ref_all = np.concatenate([rx64.ravel(), rz64.ravel()])
+    # receivers the incident wave has reached within the run (it crosses the
+    # profile from one end at the apparent velocity 1/p)
+    peak = np.abs(rz64).max()
+    sel = [i for i in range(0, rx64.shape[0], rf_every) if np.abs(rz64[i]).max() > 1e-2 * peak]
+    dt = cfg["dt"]
+    picks, rf64 = {}, {}
+    for i in sel:
+        rec = []
+        rf64[i] = rf.prf(rx64[i], rz64[i], dt, record=rec)[0]
+        picks[i] = rec
+    out = dict(workload="single large solve 8001x2000, dx=dz=50 m, dt=0.003 s, 40-cell PML, order 6, 401 receivers",
+               iterations=nsteps + 1, reference="FDFK_FP64 (general per-cell kernels in double)",
+               rf="P-RF (iterative deconvolution, Gaussian 8, FP64 run's spike picks replayed) of "
+                  f"{len(sel)} receivers (every {rf_every}th the incident wave reached)", modes={})
+    for name, (sx, sz, t, dev) in runs.items():
+        m = dict(seconds=round(t, 3), gcell_s=round(NX * NZ * (nsteps + 1) / t / 1e9, 3),
+                 device_gib=round(dev / 2 ** 30, 3))
+        if name != "fp64":
+            err = np.concatenate([sx.ravel(), sz.ravel()]) - ref_all
+            m["rel_l2_traces"] = float(np.linalg.norm(err) / np.linalg.norm(ref_all))
+            m["rel_l2_x"] = float(np.linalg.norm(sx - rx64) / np.linalg.norm(rx64))
+            m["rel_l2_z"] = float(np.linalg.norm(sz - rz64) / np.linalg.norm(rz64))
+            num = den = 0.0
+            per = {}
+            big = max(np.linalg.norm(rf64[i]) for i in sel)
+            for i in sel:
+                r = rf.prf(sx[i], sz[i], dt, picks=picks[i])[0]
+                num += float(np.sum((r - rf64[i]) ** 2))
+                den += float(np.sum(rf64[i] ** 2))
+                n64 = float(np.linalg.norm(rf64[i]))
+                per[str(i + 1)] = dict(rel_l2=float(np.linalg.norm(r - rf64[i]) / max(n64, 1e-300)),
+                                       rf_norm_over_largest=n64 / big)
+            m["rel_l2_prf"] = float(np.sqrt(num / den))
+            m["rel_l2_prf_per_receiver"] = per
+        out["modes"][name] = m
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -401,6 +471,9 @@ def main():
                     help="batched-execution measurement with N jobs instead of the default line")
     ap.add_argument("--c3", type=int, default=0, metavar="N",
                     help="the batched workload with N of its jobs (64: all), plain vs batched")
+    ap.add_argument("--c4", type=int, default=0, metavar="N",
+                    help="storage precision on the single large solve: N iterations in FP64, FP32 and FP16 "
+                         "materials, rel-L2 of traces and P-RFs against FP64")
     ap.add_argument("--no-solve", action="store_true",
                     help="default line without the whole-solve and serving objects")
     a = ap.parse_args()
@@ -413,6 +486,10 @@ def main():
         return
     if a.batch:
         print(json.dumps(batch_bench(a.batch)), flush=True)
+        return
+    if a.c4:
+        log = lambda m: print(m, file=sys.stderr, flush=True)
+        print(json.dumps(c4_bench(a.c4, log=log)), flush=True)
         return
     if a.c3:
         log = lambda m: print(m, file=sys.stderr, flush=True)
