@@ -8,12 +8,12 @@ module fdfk
 
   integer(c_int), parameter, public :: FDFK_OK = 0, FDFK_EINVAL = -22, FDFK_ENOMEM = -12, &
                                        FDFK_EDEVICE = -5, FDFK_ERANGE = -34, FDFK_ENOSYS = -38
-  integer(c_int32_t), parameter, public :: FDFK_FP32 = 0, FDFK_BF16 = 1, FDFK_FP16 = 2
+  integer(c_int32_t), parameter, public :: FDFK_FP32 = 0, FDFK_BF16 = 1, FDFK_FP16 = 2, FDFK_FP64 = 3
 
   ! one simulation: the scalars of get_parameter plus the arrays of read_data
   type, bind(C), public :: fdfk_job
     integer(c_int32_t) :: nx, nz, npml, norder   ! norder = FD order / 2
-    integer(c_int32_t) :: is_pml_top             ! must be 0
+    integer(c_int32_t) :: is_pml_top             ! 1: absorbing top layer, rows from izt = 1-npml
     integer(c_int32_t) :: npower                 ! PML profile power (2)
     real(c_float)      :: x0, z0, dx, dz
     real(c_double)     :: dt
@@ -24,7 +24,7 @@ module fdfk
     integer(c_int32_t) :: nrcv
     type(c_ptr)        :: rx, rz                 ! receiver coordinates (m), real(c_float)
     type(c_ptr)        :: c                      ! rho, lambda, mu (reference layout), real(c_float)
-    integer(c_int32_t) :: precision              ! FDFK_FP32 or FDFK_FP16 (materials)
+    integer(c_int32_t) :: precision              ! FDFK_FP32, FDFK_FP16 (materials) or FDFK_FP64
   end type
 
   ! FK background: the inputs of comput_background_wavefield
