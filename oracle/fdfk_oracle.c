@@ -1113,3 +1113,166 @@ void orc_updatedispl_ext(const orc_real *uxt_reg, const orc_real *uzt_reg, const
         bpxbx, bpxbz, bpzbx, bpzbz, bux1l, bux2l, bux3l, buz1l, buz2l, buz3l, bux1r, bux2r, bux3r, buz1r,
         buz2r, buz3r, bux1b, bux2b, bux3b, buz1b, buz2b, buz3b, bux1t, bux2t, bux3t, buz1t, buz2t, buz3t);
 }
+
+/* ================================================================== */
+/* (3) SH (transverse) motion: this project's formulation, no         */
+/*     counterpart in the reference (it keeps the in-plane P-SV part)  */
+/* ================================================================== */
+/*
+ *  rho uy_tt = Dx(mu Dx uy) + Dz(mu Dz uy): the X and Z terms of the uz
+ *  equation with modulus mu (diffx2 / diffz2 of Submain.f90 with mat = mu).
+ *  Free surface (izt = 1), row 1: Z = 2/dz^2 H(mu1, mu2)(uy2 - uy1) (zero
+ *  traction mu duy/dz from the same half-cell energy form).  PML: split
+ *  uy = u1 + u2, rho(u1_tt + dX u1_t) = X - Dx(mu psi_x),
+ *  rho(u2_tt + dZ u2_t) = Z - Dz(mu psi_z), the memories and the multiaxial
+ *  damping of the P-SV formulation.  Hybrid ring: as for P-SV, one
+ *  component.  Every cell is evaluated with the pointwise operators.
+ *  bg: uyl_reg, uyr_reg, uyb_reg, uyt_reg, uyl_pml, uyr_pml, uyb_pml,
+ *  uyt_pml (the reference's strip shapes; the top two with a top layer,
+ *  NULL: the contour stays open there).  bp: x-memories of the L, R, B, T
+ *  strips, then z-memories of B and T.  bu[s][0/1]: the split parts u1, u2
+ *  of strip s (1:2, cells).
+ */
+void orc_updatedispl_sh(int norder, int nx, int nz, int npml, int pml_top, int npower, int it, int nt,
+                        orc_real dx, orc_real dz, double dt, orc_real vpml, double R0,
+                        const orc_real *const bg[8], const orc_real *c, const double *coef0,
+                        const double *coef1, orc_real *uy, orc_real *const bp[6], orc_real *const bu[4][2])
+{
+    gdims G, *g = &G;
+    memset(&G, 0, sizeof G);
+    const int izt = pml_top ? 1 - npml : 1, nbd = 2 * norder - 1;
+    G.norder = norder; G.nx = nx; G.nz = nz; G.npml = npml; G.izt = izt; G.pml_top = pml_top;
+    G.npower = npower; G.nbd = nbd; G.it = it; G.nt = nt; G.dx = dx; G.dz = dz; G.dt = dt;
+    G.NZ = nz + npml - izt + 1; G.NX = nx + 2 * npml;
+    const size_t ncell = (size_t)G.NZ * (size_t)G.NX;
+    orc_real *u1 = malloc(ncell * sizeof(orc_real)), *rho = malloc(ncell * sizeof(orc_real));
+    orc_real *mu = malloc(ncell * sizeof(orc_real)), *lam = malloc(ncell * sizeof(orc_real));
+    for (int ix = 1 - npml; ix <= nx + npml; ++ix)
+        for (int iz = izt; iz <= nz + npml; ++iz) {
+            int sx = ix < 1 - nbd ? 1 - nbd : (ix > nx + nbd ? nx + nbd : ix);
+            int sz = iz > nz + nbd ? nz + nbd : (iz < 1 - nbd ? 1 - nbd : iz);
+            size_t i = g2(g, iz, ix), j = g2(g, sz, sx);
+            u1[i] = uy[2 * i];
+            rho[i] = c[3 * j]; lam[i] = c[3 * j + 1]; mu[i] = c[3 * j + 2];
+        }
+    orc_real *dpx = malloc(G.NX * sizeof(orc_real)), *dpz = malloc(G.NZ * sizeof(orc_real));
+    orc_damping(nx, nz, npml, nbd, pml_top, npower, dx, dz, vpml, R0, dpx, dpz);
+    const int nstrip = pml_top ? 4 : 3;
+    orc_real *const bpx[4] = {bp[0], bp[1], bp[2], bp[3]};
+    orc_real *const bpz[4] = {NULL, NULL, bp[4], bp[5]};
+    /* memories (level n) */
+    for (int s = 0; s < nstrip; ++s) {
+        int iz0, iz1, ix0, ix1;
+        if (s == S_L) { iz0 = 1; iz1 = nz; ix0 = 1 - npml; ix1 = 0; }
+        else if (s == S_R) { iz0 = 1; iz1 = nz; ix0 = nx + 1; ix1 = nx + npml; }
+        else if (s == S_B) { iz0 = nz + 1; iz1 = nz + npml; ix0 = 1 - npml; ix1 = nx + npml; }
+        else { iz0 = 1 - npml; iz1 = 0; ix0 = 1 - npml; ix1 = nx + npml; }
+        for (int ix = ix0; ix <= ix1; ++ix)
+            for (int iz = iz0; iz <= iz1; ++iz) {
+                size_t o = strip_off(g, s, iz, ix);
+                orc_real ddx = dpx[ix + npml - 1], ddz = dpz[iz - izt];
+                if (ddx > 0.0f) {
+                    orc_real b = (orc_real)exp(-(double)ddx * dt), a = 1.0f - b;
+                    bpx[s][o] = b * bpx[s][o] + a * (cdx_sum(g, coef1, u1, iz, ix) / dx);
+                }
+                if (bpz[s] && ddz > 0.0f) {
+                    orc_real b = (orc_real)exp(-(double)ddz * dt), a = 1.0f - b;
+                    bpz[s][o] = b * bpz[s][o] + a * (cdz_sum(g, coef1, u1, iz, ix) / dz);
+                }
+            }
+    }
+    orc_real *psx = calloc(ncell, sizeof(orc_real)), *psz = calloc(ncell, sizeof(orc_real));
+    for (int s = 0; s < nstrip; ++s) {
+        int iz0, iz1, ix0, ix1;
+        if (s == S_L) { iz0 = 1; iz1 = nz; ix0 = 1 - npml; ix1 = 0; }
+        else if (s == S_R) { iz0 = 1; iz1 = nz; ix0 = nx + 1; ix1 = nx + npml; }
+        else if (s == S_B) { iz0 = nz + 1; iz1 = nz + npml; ix0 = 1 - npml; ix1 = nx + npml; }
+        else { iz0 = 1 - npml; iz1 = 0; ix0 = 1 - npml; ix1 = nx + npml; }
+        for (int ix = ix0; ix <= ix1; ++ix)
+            for (int iz = iz0; iz <= iz1; ++iz) {
+                size_t o = strip_off(g, s, iz, ix), gi = g2(g, iz, ix);
+                psx[gi] = bpx[s][o];
+                if (bpz[s]) psz[gi] = bpz[s][o];
+            }
+    }
+    /* masked background (inside: wT, outside: wS) */
+    int have_bg = bg && bg[0] && bg[1] && bg[2] && bg[4] && bg[5] && bg[6] && it >= 0 && it <= nt;
+    G.top_ring = pml_top && have_bg && bg[3] && bg[7];
+    orc_real *wT = calloc(ncell, sizeof(orc_real)), *wS = calloc(ncell, sizeof(orc_real));
+    if (have_bg)
+        for (int ix = 1 - nbd; ix <= nx + nbd; ++ix)
+            for (int iz = G.top_ring ? 1 - nbd : 1; iz <= nz + nbd; ++iz) {
+                int arr, which;
+                long o = bg_off(g, iz, ix, it, &arr, &which);
+                if (o < 0) continue;
+                (which == 0 ? wT : wS)[g2(g, iz, ix)] = bg[which * 4 + arr][o];
+            }
+    const orc_real dt2 = (orc_real)(dt * dt);
+    const int iz_first = izt == 1 ? 1 : izt + 1;
+    orc_real *unew = malloc(ncell * sizeof(orc_real));
+    memcpy(unew, u1, ncell * sizeof(orc_real));
+    const orc_real ratio = (orc_real)g_pml_ratio;
+#pragma omp parallel for schedule(dynamic)
+    for (int ix = 2 - npml; ix <= nx + npml - 1; ++ix)
+        for (int iz = iz_first; iz <= nz + npml - 1; ++iz) {
+            size_t gi = g2(g, iz, ix);
+            const int surf = iz == 1 && izt == 1;
+            orc_real X = x2_point_mode(g, coef0, lam, mu, 1, u1, iz, ix), Z;
+            if (surf) {
+                size_t a = g2(g, 1, ix), b = g2(g, 2, ix);
+                Z = (hmean(mu[a], mu[b]) * (u1[b] - u1[a])) * (2.0f / (dz * dz));
+            } else {
+                Z = z2_point_mode(g, coef0, lam, mu, 1, u1, iz, ix);
+            }
+            {
+                int nox = imin(imin(ix + npml - 1, nx + npml - ix), norder);
+                int noz = imin(imin(iz - izt, nz + npml - iz), norder);
+                orc_real sx = 0.0f, sz = 0.0f;
+                for (int k = 1; k <= nox; ++k) {
+                    size_t a = g2(g, iz, ix + k), b = g2(g, iz, ix - k);
+                    sx = acc(sx, CO(coef1, k, nox), mu[a] * psx[a] - mu[b] * psx[b]);
+                }
+                for (int k = 1; k <= noz; ++k) {
+                    size_t a = g2(g, iz + k, ix), b = g2(g, iz - k, ix);
+                    sz = acc(sz, CO(coef1, k, noz), mu[a] * psz[a] - mu[b] * psz[b]);
+                }
+                X = X - sx / dx;
+                if (!surf) Z = Z - sz / dz;
+            }
+            if (have_bg) {
+                int arr, which;
+                if (bg_off(g, iz, ix, it, &arr, &which) >= 0) {
+                    /* inside: the operator of the background on the padding;
+                     * outside: minus that of the background inside */
+                    const orc_real *w = which == 0 ? wS : wT;
+                    orc_real qx = x2_point_mode(g, coef0, lam, mu, 1, w, iz, ix), qz;
+                    if (surf) {
+                        size_t a = g2(g, 1, ix), b = g2(g, 2, ix);
+                        qz = (hmean(mu[a], mu[b]) * (w[b] - w[a])) * (2.0f / (dz * dz));
+                    } else {
+                        qz = z2_point_mode(g, coef0, lam, mu, 1, w, iz, ix);
+                    }
+                    if (which == 0) { X = X + qx; Z = Z + qz; }
+                    else { X = X - qx; Z = Z - qz; }
+                }
+            }
+            orc_real bf = dt2 / rho[gi];
+            int s = strip_of(g, iz, ix);
+            if (s == S_NONE) {
+                unew[gi] = (2.0f * u1[gi] - uy[2 * gi + 1]) + bf * (X + Z);
+            } else {
+                size_t o = strip_off(g, s, iz, ix);
+                orc_real ax = 0.5f * (dpx[ix + npml - 1] + ratio * dpz[iz - izt]) * (orc_real)dt;
+                orc_real az = 0.5f * (dpz[iz - izt] + ratio * dpx[ix + npml - 1]) * (orc_real)dt;
+                orc_real *b1 = bu[s][0], *b2 = bu[s][1];
+                orc_real n1 = (2.0f * b1[2 * o] - (1.0f - ax) * b1[2 * o + 1] + bf * X) / (1.0f + ax);
+                orc_real n2 = (2.0f * b2[2 * o] - (1.0f - az) * b2[2 * o + 1] + bf * Z) / (1.0f + az);
+                b1[2 * o + 1] = b1[2 * o]; b1[2 * o] = n1;
+                b2[2 * o + 1] = b2[2 * o]; b2[2 * o] = n2;
+                unew[gi] = n1 + n2;
+            }
+        }
+    for (size_t i = 0; i < ncell; ++i) { uy[2 * i + 1] = uy[2 * i]; uy[2 * i] = unew[i]; }
+    free(unew); free(wT); free(wS); free(psx); free(psz); free(dpx); free(dpz);
+    free(u1); free(rho); free(mu); free(lam);
+}

@@ -178,6 +178,19 @@ int orc_fk_traces_fs(int nlayer, const orc_real *fkmodel, int src_is_p, int free
                      orc_real ds, double dt, int ntq, int nzq, const double *z, double zref, double t0,
                      double *ux, double *uz);
 
+/* ---- SH (transverse) motion, this project's formulation (no counterpart
+ * in the reference, which keeps the in-plane P-SV motion only) ---- */
+void orc_updatedispl_sh(int norder, int nx, int nz, int npml, int pml_top, int npower, int it, int nt,
+                        orc_real dx, orc_real dz, double dt, orc_real vpml, double R0,
+                        const orc_real *const bg[8], const orc_real *c, const double *coef0,
+                        const double *coef1, orc_real *uy, orc_real *const bp[6], orc_real *const bu[4][2]);
+void orc_background_sh(int nx, int nz, int npml, int izt, int nbd, int nt, orc_real x0, orc_real z0,
+                       orc_real dx, orc_real dz, double dt, orc_real ds, orc_real f0, double rayp,
+                       const double *bazs, int nlayer_left, const orc_real *fk_left, int nlayer_right,
+                       const orc_real *fk_right, orc_real *const out[8]);
+int orc_fk_traces_sh(int nlayer, const orc_real *fkm, int free_surface, double p, orc_real f0, orc_real ds,
+                     double dt, int ntq, int nzq, const double *z, double zref, double t0, double *uy);
+
 #ifdef __cplusplus
 }
 #endif

@@ -473,3 +473,175 @@ void orc_comput_background_wavefield_ext(orc_real *uxt_reg, orc_real *uzt_reg, o
                     dz_, dt_, ds_, f0_, rayp_, bazs, fk_left, fk_right, uxl_reg, uzl_reg, uxr_reg, uzr_reg,
                     uxb_reg, uzb_reg, uxl_pml, uzl_pml, uxr_pml, uzr_pml, uxb_pml, uzb_pml);
 }
+
+/* ------------------------------------------------------------------ */
+/* SH plane wave (transverse motion uy): b = (uy, syz/(iw)) = E a with   */
+/* a = (S down, S up), E = [[1, 1], [mu eta, -mu eta]] per layer; the  */
+/* incident S up-going in the half-space; top: traction free, or no    */
+/* down-going wave in the top layer (absorbing top layer).  No         */
+/* counterpart in the reference.                                       */
+/* ------------------------------------------------------------------ */
+static void sh_emat(const fkmodel *m, int j, cplx E[2][2])
+{
+    cplx mh = m->rho[j] * m->vs[j] * m->vs[j] * m->eb[j];
+    E[0][0] = 1.0; E[0][1] = 1.0;
+    E[1][0] = mh;  E[1][1] = -mh;
+}
+
+static void sh_amplitudes(const fkmodel *m, int free_surface, cplx w, cplx ainc, cplx (*a)[2])
+{
+    int N = m->nl;
+    cplx (*ab)[2][2] = malloc(N * sizeof(*ab));   /* [layer][basis][comp] */
+    ab[N - 1][0][0] = 0.0; ab[N - 1][0][1] = 1.0;   /* incident up-going S */
+    ab[N - 1][1][0] = 1.0; ab[N - 1][1][1] = 0.0;   /* down-going S leaving the stack */
+    for (int j = N - 2; j >= 0; --j) {
+        cplx Ej[2][2], Ek[2][2];
+        sh_emat(m, j, Ej);
+        sh_emat(m, j + 1, Ek);
+        /* Q = Ej^-1 Ek */
+        cplx det = Ej[0][0] * Ej[1][1] - Ej[0][1] * Ej[1][0];
+        cplx inv[2][2] = {{Ej[1][1] / det, -Ej[0][1] / det}, {-Ej[1][0] / det, Ej[0][0] / det}};
+        double h = m->zt[j + 1] - m->zt[j];
+        cplx ph[2] = {cexp(-I * w * m->eb[j] * h), cexp(I * w * m->eb[j] * h)};
+        for (int b = 0; b < 2; ++b) {
+            cplx v[2];
+            for (int r = 0; r < 2; ++r) v[r] = Ek[r][0] * ab[j + 1][b][0] + Ek[r][1] * ab[j + 1][b][1];
+            for (int r = 0; r < 2; ++r) ab[j][b][r] = ph[r] * (inv[r][0] * v[0] + inv[r][1] * v[1]);
+        }
+    }
+    cplx E0[2][2];
+    sh_emat(m, 0, E0);
+    cplx T[2];
+    for (int b = 0; b < 2; ++b)
+        T[b] = free_surface ? E0[1][0] * ab[0][b][0] + E0[1][1] * ab[0][b][1] : ab[0][b][0];
+    cplx x1 = -T[0] * ainc / T[1];
+    for (int j = 0; j < N; ++j)
+        for (int c = 0; c < 2; ++c) a[j][c] = ainc * ab[j][0][c] + x1 * ab[j][1][c];
+    free(ab);
+}
+
+/* traces uy(z, t) on the oversampled grid, out->g[(q*nfft + n)*2] (the
+ * second slot of each pair left zero) */
+static int sh_compute(const fkmodel *m, int free_surface, double p, double f0, double ds, double dtf, int nfft,
+                      int nzq, const double *zq, double zref, double t0, fktraces *out)
+{
+    int N = m->nl;
+    double T = nfft * dtf;
+    double eps = 27.6 / T;
+    double tref = t0 + creal(m->eb[N - 1]) * (zref - m->zt[N - 1]);
+    (void)p;
+    out->nfft = nfft; out->nzq = nzq; out->dtf = dtf; out->eps = eps;
+    out->g = calloc((size_t)nzq * nfft * 2, sizeof(double));
+    cplx *spec = calloc((size_t)nzq * (nfft / 2 + 1), sizeof(cplx));
+    cplx (*a)[2] = malloc(N * sizeof(*a));
+    for (int k = 0; k < nfft / 2; ++k) {
+        double wr = 2.0 * K_PI * k / T;
+        cplx w = wr + I * eps;
+        cplx x = w / (2.0 * K_PI * f0);
+        cplx S = ds / (f0 * sqrt(K_PI)) * cexp(-x * x);
+        cplx ainc = S * cexp(I * w * tref);
+        sh_amplitudes(m, free_surface, w, ainc, a);
+        for (int q = 0; q < nzq; ++q) {
+            int j = layer_of(m, zq[q]);
+            double dz = zq[q] - m->zt[j];
+            spec[(size_t)q * (nfft / 2 + 1) + k] = cexp(I * w * m->eb[j] * dz) * a[j][0] + cexp(-I * w * m->eb[j] * dz) * a[j][1];
+        }
+    }
+    cplx *buf = malloc(nfft * sizeof(cplx));
+    for (int q = 0; q < nzq; ++q) {
+        const cplx *sp = spec + (size_t)q * (nfft / 2 + 1);
+        buf[0] = sp[0];
+        for (int k = 1; k < nfft / 2; ++k) { buf[k] = sp[k]; buf[nfft - k] = conj(sp[k]); }
+        buf[nfft / 2] = 0.0;
+        fft_inplace(buf, nfft, -1);
+        for (int n = 0; n < nfft; ++n) out->g[((size_t)q * nfft + n) * 2] = creal(buf[n]) * exp(eps * n * dtf) / T;
+    }
+    free(buf); free(spec); free(a);
+    return 0;
+}
+
+int orc_fk_traces_sh(int nlayer, const orc_real *fkm, int free_surface, double p, orc_real f0, orc_real ds,
+                     double dt, int ntq, int nzq, const double *z, double zref, double t0, double *uy)
+{
+    fkmodel m;
+    if (fk_setup(&m, nlayer, fkm, p)) { fk_free(&m); return -1; }
+    double dtf = dt / 2.0;
+    double tneed = ntq * dt + 2.0 * t0 + 10.0 / f0;
+    int nfft = next_pow2((long)ceil(2.0 * tneed / dtf));
+    fktraces tr;
+    sh_compute(&m, free_surface, p, f0, ds, dtf, nfft, nzq, z, zref, t0, &tr);
+    for (int q = 0; q < nzq; ++q)
+        for (int n = 0; n < ntq; ++n) uy[(size_t)q * ntq + n] = interp(tr.g + (size_t)q * nfft * 2, nfft, dtf, n * dt);
+    free(tr.g);
+    fk_free(&m);
+    return 0;
+}
+
+/* SH background on the ring, the P-SV background's geometry and
+ * conventions (translation by p |x - x_entry|, side models on the side
+ * strips, blended in x on the bottom and top strips, the direction sign
+ * applied to uy as to ux); out[8]: uyl_reg, uyr_reg, uyb_reg, uyt_reg,
+ * uyl_pml, uyr_pml, uyb_pml, uyt_pml (the top two with izt < 1 only) */
+void orc_background_sh(int nx, int nz, int npml, int izt, int nbd, int nt, orc_real x0, orc_real z0,
+                       orc_real dx, orc_real dz, double dt, orc_real ds, orc_real f0, double rayp,
+                       const double *bazs, int nlayer_left, const orc_real *fk_left, int nlayer_right,
+                       const orc_real *fk_right, orc_real *const out[8])
+{
+    const int free_surface = izt == 1, top = !free_surface && out[3] && out[7];
+    double cs = 0.0;
+    for (int i = 0; i < nx + 2 * npml; ++i) cs += cos(bazs[i]);
+    double sgn = ((rayp < 0.0) ? -1.0 : 1.0) * (cs < 0.0 ? 1.0 : -1.0);
+    double p = fabs(rayp), t0 = 1.5 / (double)f0;
+    const int zf = top ? 1 - nbd : 1;
+    int nzq = nz + nbd - zf + 1;
+    double *zq = malloc(nzq * sizeof(double));
+    for (int q = 0; q < nzq; ++q) zq[q] = z0 + (double)(q + zf - 1) * dz;
+    double xl = x0 + (double)(1 - nbd - 1) * dx, xr = x0 + (double)(nx + nbd - 1) * dx;
+    double xe = sgn > 0 ? xl : xr;
+    fkmodel ml, mr;
+    if (fk_setup(&ml, nlayer_left, fk_left, p) || fk_setup(&mr, nlayer_right, fk_right, p)) {
+        fk_free(&ml); fk_free(&mr); free(zq);
+        return;
+    }
+    double zref = fmax(fmax(zq[nzq - 1], ml.zt[ml.nl - 1]), fmax(zq[nzq - 1], mr.zt[mr.nl - 1]));
+    double dtf = dt / 2.0;
+    double tneed = nt * dt + p * (xr - xl) + 2.0 * t0 + 10.0 / f0;
+    int nfft = next_pow2((long)ceil(2.0 * tneed / dtf));
+    fktraces tl, trr;
+    sh_compute(&ml, free_surface, p, f0, ds, dtf, nfft, nzq, zq, zref, t0, &tl);
+    sh_compute(&mr, free_surface, p, f0, ds, dtf, nfft, nzq, zq, zref, t0, &trr);
+    for (int it = 0; it <= nt; ++it)
+        for (int ix = 1 - nbd; ix <= nx + nbd; ++ix) {
+            double x = x0 + (double)(ix - 1) * dx;
+            double tau = p * (sgn > 0 ? (x - xe) : (xe - x));
+            double w = (double)(ix - 1) / (double)(nx - 1);
+            w = w < 0.0 ? 0.0 : (w > 1.0 ? 1.0 : w);
+            for (int iz = zf; iz <= nz + nbd; ++iz) {
+                int q = iz - zf;
+                double vl = interp(tl.g + (size_t)q * nfft * 2, nfft, dtf, it * dt - tau);
+                double vr = interp(trr.g + (size_t)q * nfft * 2, nfft, dtf, it * dt - tau);
+                const int side = (iz >= 1 && iz <= nz) && (ix <= nbd || ix >= nx - nbd + 1) && !(iz > nz - nbd && ix >= 1 && ix <= nx);
+                double v = side ? (ix <= nbd ? vl : vr) : (1.0 - w) * vl + w * vr;
+                orc_real f = (orc_real)(sgn * v);
+                if (iz >= 1 && iz <= nz - nbd && ix >= 1 && ix <= nbd)
+                    out[0][(size_t)(iz - 1) + (size_t)(nz - nbd) * ((size_t)(ix - 1) + (size_t)nbd * it)] = f;
+                if (iz >= 1 && iz <= nz - nbd && ix >= nx - nbd + 1 && ix <= nx)
+                    out[1][(size_t)(iz - 1) + (size_t)(nz - nbd) * ((size_t)(ix - (nx - nbd + 1)) + (size_t)nbd * it)] = f;
+                if (iz >= nz - nbd + 1 && iz <= nz && ix >= 1 && ix <= nx)
+                    out[2][(size_t)(iz - (nz - nbd + 1)) + (size_t)nbd * ((size_t)(ix - 1) + (size_t)nx * it)] = f;
+                if (top && iz >= 1 && iz <= nbd && ix >= nbd + 1 && ix <= nx - nbd)
+                    out[3][(size_t)(iz - 1) + (size_t)nbd * ((size_t)(ix - (nbd + 1)) + (size_t)(nx - 2 * nbd) * it)] = f;
+                if (iz >= 1 && iz <= nz && ix >= 1 - nbd && ix <= 0)
+                    out[4][(size_t)(iz - 1) + (size_t)nz * ((size_t)(ix - (1 - nbd)) + (size_t)nbd * it)] = f;
+                if (iz >= 1 && iz <= nz && ix >= nx + 1 && ix <= nx + nbd)
+                    out[5][(size_t)(iz - 1) + (size_t)nz * ((size_t)(ix - (nx + 1)) + (size_t)nbd * it)] = f;
+                if (iz >= nz + 1 && iz <= nz + nbd && ix >= 1 - nbd && ix <= nx + nbd)
+                    out[6][(size_t)(iz - (nz + 1)) + (size_t)nbd * ((size_t)(ix - (1 - nbd)) + (size_t)(nx + 2 * nbd) * it)] = f;
+                if (top && iz >= 1 - nbd && iz <= 0)
+                    out[7][(size_t)(iz - (1 - nbd)) + (size_t)nbd * ((size_t)(ix - (1 - nbd)) + (size_t)(nx + 2 * nbd) * it)] = f;
+            }
+        }
+    free(tl.g); free(trr.g);
+    fk_free(&ml); fk_free(&mr);
+    free(zq);
+}

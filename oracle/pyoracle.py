@@ -297,3 +297,69 @@ def fk_traces(fkmodel, src_is_p, p, f0, ds, dt, ntq, z, zref, t0, free_surface=T
     if rc:
         raise ValueError("invalid FK model")
     return ux, uz
+
+
+# ---- SH (transverse) motion ----
+def sh_state(nx, nz, npml, norder, dx, dz, dt, nt, c, pml_top=False, npower=2, R0=1e-3, vpml=None):
+    """Zeroed SH state for updatedispl_sh: uy (1:2, NZ, NX), memories bp
+    (x: L, R, B, T; z: B, T), split parts bu[strip][u1, u2] (1:2, cells)."""
+    from fdfk2d_amd import refstate
+    izt = 1 - npml if pml_top else 1
+    NZ, NX = nz + npml - izt + 1, nx + 2 * npml
+    c = np.ascontiguousarray(c, np.float32)
+    assert c.size == 3 * NZ * NX
+    lr, tb = nz * npml, npml * NX
+    cells = [lr, lr, tb, tb]
+    return dict(norder=norder, nx=nx, nz=nz, npml=npml, izt=izt, pml_top=int(bool(pml_top)), npower=npower,
+                nt=nt, dx=float(dx), dz=float(dz), dt=float(dt),
+                vpml=refstate.vp_max(c) if vpml is None else float(vpml), R0=float(R0), c=c,
+                coef0=refstate.fd_coef(norder, True), coef1=refstate.fd_coef(norder, False),
+                uy=np.zeros(2 * NZ * NX, np.float32),
+                bp=[np.zeros(n, np.float32) for n in cells] + [np.zeros(tb, np.float32) for _ in range(2)],
+                bu=[[np.zeros(2 * n, np.float32) for _ in range(2)] for n in cells])
+
+
+def updatedispl_sh(st, it, bg=None):
+    """One SH step of the oracle on a sh_state (FP32); bg: the eight ring
+    arrays (uyl_reg, uyr_reg, uyb_reg, uyt_reg, uyl_pml, uyr_pml, uyb_pml,
+    uyt_pml; the top two may be None) or None."""
+    a = st
+    bgp = (C.c_void_p * 8)(*[None if x is None else x.ctypes.data for x in bg]) if bg is not None else None
+    bpp = (C.c_void_p * 6)(*[x.ctypes.data for x in a["bp"]])
+    bup = ((C.c_void_p * 2) * 4)(*[(C.c_void_p * 2)(*[x.ctypes.data for x in pair]) for pair in a["bu"]])
+    lib().orc_updatedispl_sh(
+        C.c_int(a["norder"]), C.c_int(a["nx"]), C.c_int(a["nz"]), C.c_int(a["npml"]), C.c_int(a["pml_top"]),
+        C.c_int(a["npower"]), C.c_int(it), C.c_int(a["nt"]), C.c_float(a["dx"]), C.c_float(a["dz"]),
+        C.c_double(a["dt"]), C.c_float(a["vpml"]), C.c_double(a["R0"]), bgp, _p(a["c"]), _p(a["coef0"]),
+        _p(a["coef1"]), _p(a["uy"]), bpp, bup)
+
+
+def background_sh(nx, nz, npml, izt, nbd, nt, x0, z0, dx, dz, dt, ds, f0, rayp, bazs, fk_left, fk_right):
+    """SH ring background: eight arrays (the top two None without a top layer)."""
+    shapes = bg_shapes(nx, nz, nbd, nt)
+    T = nt + 1
+    arrs = [np.zeros(shapes[0], np.float32), np.zeros(shapes[2], np.float32), np.zeros(shapes[4], np.float32),
+            np.zeros(nbd * (nx - 2 * nbd) * T, np.float32) if izt != 1 else None,
+            np.zeros(shapes[6], np.float32), np.zeros(shapes[8], np.float32), np.zeros(shapes[10], np.float32),
+            np.zeros(nbd * (nx + 2 * nbd) * T, np.float32) if izt != 1 else None]
+    fl = np.ascontiguousarray(fk_left, np.float32)
+    fr = np.ascontiguousarray(fk_right, np.float32)
+    bz = np.ascontiguousarray(bazs, np.float64)
+    outp = (C.c_void_p * 8)(*[None if x is None else x.ctypes.data for x in arrs])
+    lib().orc_background_sh(C.c_int(nx), C.c_int(nz), C.c_int(npml), C.c_int(izt), C.c_int(nbd), C.c_int(nt),
+                            C.c_float(x0), C.c_float(z0), C.c_float(dx), C.c_float(dz), C.c_double(dt),
+                            C.c_float(ds), C.c_float(f0), C.c_double(rayp), _p(bz), C.c_int(fl.size // 4), _p(fl),
+                            C.c_int(fr.size // 4), _p(fr), outp)
+    return arrs
+
+
+def fk_traces_sh(fkmodel, p, f0, ds, dt, ntq, z, zref, t0, free_surface=True):
+    fk = np.ascontiguousarray(fkmodel, np.float32)
+    z = np.ascontiguousarray(z, np.float64)
+    uy = np.zeros((z.size, ntq), np.float64)
+    rc = lib().orc_fk_traces_sh(C.c_int(fk.size // 4), _p(fk), C.c_int(1 if free_surface else 0), C.c_double(p),
+                                C.c_float(f0), C.c_float(ds), C.c_double(dt), C.c_int(ntq), C.c_int(z.size), _p(z),
+                                C.c_double(zref), C.c_double(t0), _p(uy))
+    if rc:
+        raise ValueError("invalid FK model")
+    return uy
