@@ -252,6 +252,7 @@ struct fdfk_solver {
     // FDFK_FP64 runs it in double (every array below then holds doubles;
     // gsize is the element size).
     bool gen = false;
+    bool sh = false;                   // transverse SH motion (one component, in the ux slots)
     int zt = 1;
     int gsize = 4;
     void *gux[2] = {nullptr, nullptr}, *guz[2] = {nullptr, nullptr};
@@ -898,10 +899,16 @@ int gen_step_t(fdfk_solver *s, bool record)
     const long nring = s->gring[0] ? (long)(g.nx + 2 * g.nbd) * (g.nz + g.nbd - zr + 1) : 0;
     const long npsi = 2L * g.npml * g.nz + (s->zt != 1 ? 2L : 1L) * g.NX * g.npml;
     const long np = nrec + nring + npsi;
-    hipLaunchKernelGGL(k_gen_prep<T>, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s->stream, a, r, nrec, nring,
-                       zr, (T *)s->gring[0], (T *)s->gring[1]);
     const long nc = (long)g.NX * g.NZ;
-    hipLaunchKernelGGL(k_gen_step<T>, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s->stream, a);
+    if (s->sh) {
+        hipLaunchKernelGGL(k_gen_prep_sh<T>, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s->stream, a, r, nrec,
+                           nring, zr, (T *)s->gring[0]);
+        hipLaunchKernelGGL(k_gen_step_sh<T>, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s->stream, a);
+    } else {
+        hipLaunchKernelGGL(k_gen_prep<T>, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s->stream, a, r, nrec,
+                           nring, zr, (T *)s->gring[0], (T *)s->gring[1]);
+        hipLaunchKernelGGL(k_gen_step<T>, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s->stream, a);
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(FDFK_EDEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
     s->cur = 1 - s->cur;
@@ -970,6 +977,9 @@ int validate(const fdfk_job *j)
     if (!j) return fail(FDFK_EINVAL, "null job");
     if (j->is_pml_top && j->precision == FDFK_FP16)
         return fail(FDFK_ENOSYS, "an absorbing top layer (is_pml_top) runs in FP32 or FP64 storage");
+    if (j->wave != FDFK_PSV && j->wave != FDFK_SH) return fail(FDFK_EINVAL, "wave must be FDFK_PSV or FDFK_SH");
+    if (j->wave == FDFK_SH && j->precision == FDFK_FP16)
+        return fail(FDFK_ENOSYS, "SH solvers run in FP32 or FP64 storage");
     if (int rc = validate_geometry(j->nx, j->nz, j->npml, j->norder, j->dx, j->dz, j->dt, j->nt)) return rc;
     if (j->nstp < 1) return fail(FDFK_EINVAL, "nstp >= 1 required");
     if (j->nrcv < 0 || (j->nrcv > 0 && (!j->rx || !j->rz))) return fail(FDFK_EINVAL, "receiver arrays");
@@ -994,7 +1004,7 @@ int fdfk_autotune(const fdfk_job *job, int32_t steps, int32_t *best_lx, float *b
                   float *model_ms)
 {
     if (!job || steps < 1) return fail(FDFK_EINVAL, "job required and steps >= 1");
-    if (job->is_pml_top || job->precision == FDFK_FP64)
+    if (job->is_pml_top || job->precision == FDFK_FP64 || job->wave == FDFK_SH)
         return fail(FDFK_ENOSYS, "the tile autotuner covers the tiled FP32 kernels only");
     // scratch solver of the same geometry, set up for every timed step
     std::vector<int> cand;
@@ -1086,6 +1096,7 @@ static int gen_create(const fdfk_job *job, fdfk_solver **out)
     auto bad = [&](int code) { delete s; return code; };
     int rc = 0;
     s->gen = true;
+    s->sh = job->wave == FDFK_SH;
     s->gsize = (int)sizeof(T);
     s->precision = job->precision;
     s->zt = job->is_pml_top ? 1 - job->npml : 1;
@@ -1185,7 +1196,7 @@ int fdfk_create(const fdfk_job *job, fdfk_solver **out)
     int rc = validate(job);
     if (rc) return rc;
     if (job->precision == FDFK_FP64) return gen_create<double>(job, out);
-    if (job->is_pml_top) return gen_create<float>(job, out);
+    if (job->is_pml_top || job->wave == FDFK_SH) return gen_create<float>(job, out);
     auto *s = new fdfk_solver();
     Geo &g = s->g;
     g.nx = job->nx; g.nz = job->nz; g.npml = job->npml; g.norder = job->norder;
@@ -1552,6 +1563,15 @@ int fdfk_run(fdfk_solver *s, int32_t nsteps)
 }
 
 int fdfk_iteration(const fdfk_solver *s) { return s ? s->it : FDFK_EINVAL; }
+
+int fdfk_query(fdfk_solver *s)
+{
+    if (!s) return fail(FDFK_EINVAL, "null solver");
+    const hipError_t e = hipStreamQuery(s->stream);
+    if (e == hipSuccess) return 1;
+    if (e == hipErrorNotReady) return 0;
+    return fail(FDFK_EDEVICE, std::string("hipStreamQuery: ") + hipGetErrorString(e));
+}
 
 int fdfk_sync(fdfk_solver *s)
 {

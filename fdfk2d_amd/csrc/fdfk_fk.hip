@@ -48,6 +48,7 @@ constexpr int kMaxLayer = 16;
 struct FKModel {
     int nl;
     double zt[kMaxLayer];
+    double mu[kMaxLayer];      // rho vs^2 (SH)
     zc ea[kMaxLayer], eb[kMaxLayer];
     zc E[kMaxLayer][4][4];
     zc Q[kMaxLayer][4][4];
@@ -105,6 +106,7 @@ static bool fk_model_setup(FKModel &m, int nl, const float *fk, double p)
         double zt = fk[4 * j], vp = fk[4 * j + 1], vs = fk[4 * j + 2], rho = fk[4 * j + 3];
         if (!(vs > 0.0) || !(vp > vs) || !(rho > 0.0)) return false;
         m.zt[j] = zt;
+        m.mu[j] = rho * vs * vs;
         m.ea[j] = vslow_h(vp, p);
         m.eb[j] = vslow_h(vs, p);
         emat_h(vp, vs, rho, p, m.ea[j], m.eb[j], m.E[j]);
@@ -127,6 +129,7 @@ struct FKRun {
     int src_is_p;
     int nzq;               // ring depths z0 + (q + zf - 1) dz, q = 0..nzq-1
     int free_surface;      // 1: traction-free top; 0: the top layer extends upward (absorbing top layer)
+    int sh;                // SH incidence: one component (uy), stored in the x slots
 };
 
 // per frequency: amplitudes of all layers, amps[(k*nl + j)*4 + c]
@@ -179,6 +182,70 @@ __global__ void k_fk_amp(const FKModel *mp, FKRun r, zc *amps, zc *work)
         for (int c = 0; c < 4; ++c)
             amps[((size_t)k * N + j) * 4 + c] =
                 ainc * ab[(j * 3 + 0) * 4 + c] + x1 * ab[(j * 3 + 1) * 4 + c] + x2 * ab[(j * 3 + 2) * 4 + c];
+}
+
+// SH plane wave, per frequency: b = (uy, syz/(iw)) = E a, a = (S down, S
+// up), E = [[1, 1], [mu eta, -mu eta]]; E^-1 b = (u + s/(mu eta), u - s/(mu
+// eta)) / 2.  The incident S up-going and the leaving down-going S of the
+// half-space propagated up; top: traction free or no down-going wave.
+// amps[(k*nl + j)*4 + c], c = 0 down, 1 up.
+__global__ void k_fk_amp_sh(const FKModel *mp, FKRun r, zc *amps, zc *work)
+{
+    const FKModel &m = *mp;
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= r.nk) return;
+    const int N = m.nl;
+    const double PI = 3.14159265358979323846;
+    zc w = zmk(2.0 * PI * k / r.T, r.eps);
+    zc x = zmk(w.r / (2.0 * PI * r.f0), w.i / (2.0 * PI * r.f0));
+    zc S = (r.ds / (r.f0 * sqrt(PI))) * zexp(zmk(-(x.r * x.r - x.i * x.i), -2.0 * x.r * x.i));
+    zc ainc = S * zexp(zI(w * zmk(m.tref, 0.0)));
+    zc *ab = work + (size_t)k * N * 4;   // [layer][basis][comp]
+    ab[((N - 1) * 2 + 0) * 2 + 0] = zmk(0, 0); ab[((N - 1) * 2 + 0) * 2 + 1] = zmk(1, 0);
+    ab[((N - 1) * 2 + 1) * 2 + 0] = zmk(1, 0); ab[((N - 1) * 2 + 1) * 2 + 1] = zmk(0, 0);
+    for (int j = N - 2; j >= 0; --j) {
+        const double h = m.zt[j + 1] - m.zt[j];
+        const zc mek = m.mu[j + 1] * m.eb[j + 1], mej = m.mu[j] * m.eb[j];
+        const zc iwh = zI(w * m.eb[j]);
+        const zc ph[2] = {zexp(-h * iwh), zexp(h * iwh)};
+        for (int b = 0; b < 2; ++b) {
+            const zc ad = ab[((j + 1) * 2 + b) * 2 + 0], au = ab[((j + 1) * 2 + b) * 2 + 1];
+            const zc u = ad + au, sg = mek * (ad - au);
+            const zc q = zdiv(sg, mej);
+            ab[(j * 2 + b) * 2 + 0] = ph[0] * (0.5 * (u + q));
+            ab[(j * 2 + b) * 2 + 1] = ph[1] * (0.5 * (u - q));
+        }
+    }
+    const zc me0 = m.mu[0] * m.eb[0];
+    zc T[2];
+    for (int b = 0; b < 2; ++b)
+        T[b] = r.free_surface ? me0 * (ab[b * 2 + 0] - ab[b * 2 + 1]) : ab[b * 2 + 0];
+    const zc x1 = zdiv(zmk(0, 0) - T[0] * ainc, T[1]);
+    for (int j = 0; j < N; ++j)
+        for (int c = 0; c < 2; ++c)
+            amps[((size_t)k * N + j) * 4 + c] = ainc * ab[(j * 2 + 0) * 2 + c] + x1 * ab[(j * 2 + 1) * 2 + c];
+}
+
+// SH displacement spectra at the ring depths (uy in the first component,
+// zero in the second)
+__global__ void k_fk_spec_sh(const FKModel *mp, FKRun r, const zc *amps, const int *lay, const double *dzl, zc *spec)
+{
+    const FKModel &m = *mp;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long nkp = r.nfft / 2 + 1;
+    if (t >= (long)r.nzq * nkp) return;
+    int q = (int)(t / nkp), k = (int)(t % nkp);
+    zc uy = zmk(0, 0);
+    if (k < r.nk) {
+        const double PI = 3.14159265358979323846;
+        zc w = zmk(2.0 * PI * k / r.T, r.eps);
+        int j = lay[q];
+        zc iwb = zI(w * m.eb[j]);
+        const zc *a = amps + ((size_t)k * m.nl + j) * 4;
+        uy = zexp(dzl[q] * iwb) * a[0] + zexp(-dzl[q] * iwb) * a[1];
+    }
+    spec[(size_t)q * nkp + k] = zconj(uy);
+    spec[((size_t)r.nzq + q) * nkp + k] = zmk(0, 0);
 }
 
 // displacement spectra at the ring depths, conjugated for the e^{+} inverse
@@ -406,11 +473,18 @@ int fk_traces_dev(fdfk_solver *s, FKModel &m, const FKRun &r, const double *zq, 
     HIPCHK(spec.alloc((size_t)2 * nzq * nkp * sizeof(zc), s->stream));
     HIPCHK(h2d(s->stream, dlay.p, lay.data(), nzq * sizeof(int)));
     HIPCHK(h2d(s->stream, ddz.p, dzl.data(), nzq * sizeof(double)));
-    hipLaunchKernelGGL(k_fk_amp, dim3((r.nk + 127) / 128), dim3(128), 0, s->stream, dm.as<FKModel>(), r, amps.as<zc>(),
-                       work.as<zc>());
     long ns = (long)nzq * nkp;
-    hipLaunchKernelGGL(k_fk_spec, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, s->stream, dm.as<FKModel>(), r,
-                       amps.as<zc>(), dlay.as<int>(), ddz.as<double>(), spec.as<zc>());
+    if (r.sh) {
+        hipLaunchKernelGGL(k_fk_amp_sh, dim3((r.nk + 127) / 128), dim3(128), 0, s->stream, dm.as<FKModel>(), r,
+                           amps.as<zc>(), work.as<zc>());
+        hipLaunchKernelGGL(k_fk_spec_sh, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, s->stream,
+                           dm.as<FKModel>(), r, amps.as<zc>(), dlay.as<int>(), ddz.as<double>(), spec.as<zc>());
+    } else {
+        hipLaunchKernelGGL(k_fk_amp, dim3((r.nk + 127) / 128), dim3(128), 0, s->stream, dm.as<FKModel>(), r,
+                           amps.as<zc>(), work.as<zc>());
+        hipLaunchKernelGGL(k_fk_spec, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, s->stream,
+                           dm.as<FKModel>(), r, amps.as<zc>(), dlay.as<int>(), ddz.as<double>(), spec.as<zc>());
+    }
     hipfftHandle plan;
     size_t wbytes = 0;
     if (int rc = plan_take(r.nfft, (int)nkp, 2 * nzq, plan, wbytes)) return rc;
@@ -491,6 +565,8 @@ extern "C" int fdfk_compute_background(fdfk_solver *s, const fdfk_fk *fk)
     r.eps = 27.6 / r.T;
     r.f0 = f0; r.ds = fk->ds; r.src_is_p = fk->src_is_p ? 1 : 0; r.nzq = nzq;
     r.free_surface = top ? 0 : 1;
+    r.sh = s->sh ? 1 : 0;
+    if (r.sh) r.src_is_p = 0;   // an SH solver's incident wave is an S wave
     ml.tref = t0 + (r.src_is_p ? ml.ea[ml.nl - 1].r : ml.eb[ml.nl - 1].r) * (zref - ml.zt[ml.nl - 1]);
     mr.tref = t0 + (r.src_is_p ? mr.ea[mr.nl - 1].r : mr.eb[mr.nl - 1].r) * (zref - mr.zt[mr.nl - 1]);
     const bool same = fk->nlayer_left == fk->nlayer_right &&

@@ -340,7 +340,7 @@ __device__ double gdirac(double x)
 
 // loadseis (Submain.f90:3-116) of receiver i, rows from zt: the 7x7
 // window's products summed in the reference's loop order
-template <class T>
+template <class T, int NC = 2>
 __device__ void grecord(const GenArgs<T> &a, const GenRec<T> &r, int i)
 {
 #pragma clang fp contract(off)
@@ -363,11 +363,11 @@ __device__ void grecord(const GenArgs<T> &a, const GenRec<T> &r, int i)
             const double ba = bax * gdirac(z);
             const size_t c = gcell(a, iz, ix);
             sx = (T)((double)sx + ba * (double)a.ux[c]);
-            sz = (T)((double)sz + ba * (double)a.uz[c]);
+            if (NC == 2) sz = (T)((double)sz + ba * (double)a.uz[c]);
         }
     }
     r.seisx[(size_t)i * r.nsamp + r.istp] = sx;
-    r.seisz[(size_t)i * r.nsamp + r.istp] = sz;
+    if (NC == 2) r.seisz[(size_t)i * r.nsamp + r.istp] = sz;
 }
 
 // one thread per receiver, per ring-region cell and per strip cell:
@@ -505,6 +505,125 @@ __global__ __launch_bounds__(256) void k_gen_step(GenArgs<T> a)
         const T tot = (n1 + n2) + n3;
         if (comp == 0) a.uxo[gi] = tot; else a.uzo[gi] = tot;
     }
+}
+
+// ---- SH (transverse) motion: uy in the ux slots (fields, ring values,
+// x background arrays, seisx), one split part pair per strip cell (parts 0
+// and 1 of component 0), memories of component 0.  rho uy_tt = X + Z with X
+// = diffx2[mu] uy, Z = diffz2[mu] uy, the free-surface closure Z = 2/dz^2
+// H(mu1, mu2)(uy2 - uy1) on row 1 when zt = 1.
+template <class T, class F>
+__device__ void gparts_sh(const GenArgs<T> &a, const F &u, int iz, int ix, T &X, T &Z)
+{
+#pragma clang fp contract(off)
+    X = gx2(a, 1, u, iz, ix);
+    if (iz == 1 && a.zt == 1) {
+        const size_t ca = gcell(a, 1, ix), cb = gcell(a, 2, ix);
+        const T dz = (T)a.g.dz;
+        Z = (ghmean(a.mu[ca], a.mu[cb]) * (u(2, ix) - u(1, ix))) * ((T)2.0f / (dz * dz));
+    } else {
+        Z = gz2(a, 1, u, iz, ix);
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_gen_prep_sh(GenArgs<T> a, GenRec<T> r, long nrec, long nring, int zr, T *ringx)
+{
+#pragma clang fp contract(off)
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const Geo &g = a.g;
+    if (t < nrec) { grecord<T, 1>(a, r, (int)t); return; }
+    t -= nrec;
+    if (t < nring) {
+        const int rows = g.nz + g.nbd - zr + 1;
+        const int ix = 1 - g.nbd + (int)(t / rows), iz = zr + (int)(t % rows);
+        int arr = 0;
+        const long o = a.ring_valid ? gring(a, iz, ix, a.it, arr) : -1;
+        ringx[gcell(a, iz, ix)] = o >= 0 ? a.bg[arr][o] : (T)0.0f;
+        return;
+    }
+    t -= nring;
+    const long nlr = (long)g.npml * g.nz, nb = (long)g.NX * g.npml;
+    int s, iz, ix;
+    if (t < nlr) { s = 0; ix = 1 - g.npml + (int)(t / g.nz); iz = 1 + (int)(t % g.nz); }
+    else if (t < 2 * nlr) { t -= nlr; s = 1; ix = g.nx + 1 + (int)(t / g.nz); iz = 1 + (int)(t % g.nz); }
+    else if (t < 2 * nlr + nb) { t -= 2 * nlr; s = 2; ix = 1 - g.npml + (int)(t / g.npml); iz = g.nz + 1 + (int)(t % g.npml); }
+    else if (a.top && t < 2 * nlr + 2 * nb) { t -= 2 * nlr + nb; s = 3; ix = 1 - g.npml + (int)(t / g.npml); iz = 1 - g.npml + (int)(t % g.npml); }
+    else return;
+    const size_t o = gstrip_off(a, s, iz, ix);
+    const GField<T> fu{&a, a.ux};
+    const int jx = ix + g.npml - 1, jz = iz - a.zt;
+    if (a.dpx[jx] > (T)0.0f) {
+        const T b = a.bx[jx], c = (T)1.0f - b;
+        a.bpx[s][0][o] = b * a.bpx[s][0][o] + c * (gcdx(a, fu, iz, ix) / (T)g.dx);
+    }
+    if (a.bpz[s][0] && a.dpz[jz] > (T)0.0f) {
+        const T b = a.bz[jz], c = (T)1.0f - b;
+        a.bpz[s][0][o] = b * a.bpz[s][0][o] + c * (gcdz(a, fu, iz, ix) / (T)g.dz);
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_gen_step_sh(GenArgs<T> a)
+{
+#pragma clang fp contract(off)
+    const Geo &g = a.g;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)g.NX * g.NZ) return;
+    const int ix = 1 - g.npml + (int)(t / g.NZ), iz = a.zt + (int)(t % g.NZ);
+    const size_t gi = gcell(a, iz, ix);
+    const int iz_first = a.zt == 1 ? 1 : a.zt + 1;
+    if (ix < 2 - g.npml || ix > g.nx + g.npml - 1 || iz < iz_first || iz > g.nz + g.npml - 1) {
+        a.uxo[gi] = a.ux[gi];
+        return;
+    }
+    const GField<T> fu{&a, a.ux};
+    T X, Z;
+    gparts_sh(a, fu, iz, ix, X, Z);
+    const bool surf = iz == 1 && a.zt == 1;
+    {
+        const int norder = g.norder;
+        const int nox = gmin(gmin(ix + g.npml - 1, g.nx + g.npml - ix), norder);
+        const int noz = gmin(gmin(iz - a.zt, g.nz + g.npml - iz), norder);
+        T sx = (T)0.0f, sz = (T)0.0f;
+        for (int k = 1; k <= nox; ++k) {
+            const size_t ca = gcell(a, iz, ix + k), cb = gcell(a, iz, ix - k);
+            sx = gacc(sx, gco(a.c1, norder, k, nox), (T)(a.mu[ca] * gpsi(a, 0, 0, iz, ix + k) - a.mu[cb] * gpsi(a, 0, 0, iz, ix - k)));
+        }
+        for (int k = 1; k <= noz; ++k) {
+            const size_t ca = gcell(a, iz + k, ix), cb = gcell(a, iz - k, ix);
+            sz = gacc(sz, gco(a.c1, norder, k, noz), (T)(a.mu[ca] * gpsi(a, 1, 0, iz + k, ix) - a.mu[cb] * gpsi(a, 1, 0, iz - k, ix)));
+        }
+        X = X - sx / (T)g.dx;
+        if (!surf) Z = Z - sz / (T)g.dz;
+    }
+    if (a.ring_valid) {
+        int arr = 0;
+        if (gring(a, iz, ix, a.it, arr) >= 0) {
+            const bool inside = iz >= 1 && iz <= g.nz && ix >= 1 && ix <= g.nx;
+            const GMasked<T> w{&a, a.ringx, !inside};
+            T qx, qz;
+            gparts_sh(a, w, iz, ix, qx, qz);
+            if (inside) { X = X + qx; Z = Z + qz; }
+            else { X = X - qx; Z = Z - qz; }
+        }
+    }
+    const T bf = (T)(g.dt * g.dt) / a.rho[gi];
+    const int s = gstrip(a, iz, ix);
+    if (s < 0) {
+        a.uxo[gi] = ((T)2.0f * a.ux[gi] - a.uxp[gi]) + bf * (X + Z);
+        return;
+    }
+    const size_t o = gstrip_off(a, s, iz, ix);
+    const int jx = ix + g.npml - 1, jz = iz - a.zt;
+    const T ax = (T)0.5f * (a.dpx[jx] + a.ratio * a.dpz[jz]) * (T)g.dt;
+    const T az = (T)0.5f * (a.dpz[jz] + a.ratio * a.dpx[jx]) * (T)g.dt;
+    T *b1 = a.bu[s][0][0], *b2 = a.bu[s][0][1];
+    const T n1 = ((T)2.0f * b1[2 * o] - ((T)1.0f - ax) * b1[2 * o + 1] + bf * X) / ((T)1.0f + ax);
+    const T n2 = ((T)2.0f * b2[2 * o] - ((T)1.0f - az) * b2[2 * o + 1] + bf * Z) / ((T)1.0f + az);
+    b1[2 * o + 1] = b1[2 * o]; b1[2 * o] = n1;
+    b2[2 * o + 1] = b2[2 * o]; b2[2 * o] = n2;
+    a.uxo[gi] = n1 + n2;
 }
 
 // reference c (1:3, rows zt.., columns) -> rho, lam, mu planes; the damped

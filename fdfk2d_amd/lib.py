@@ -12,6 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfdfk.so")
 
 FDFK_FP32, FDFK_BF16, FDFK_FP16, FDFK_FP64 = 0, 1, 2, 3
+FDFK_PSV, FDFK_SH = 0, 1
 
 
 class FdfkJob(C.Structure):
@@ -22,7 +23,7 @@ class FdfkJob(C.Structure):
         ("dt", C.c_double), ("R0", C.c_double), ("vpml", C.c_float),
         ("nt", C.c_int32), ("nstp", C.c_int32), ("nrcv", C.c_int32),
         ("rx", C.c_void_p), ("rz", C.c_void_p), ("c", C.c_void_p),
-        ("precision", C.c_int32),
+        ("precision", C.c_int32), ("wave", C.c_int32),
     ]
 
 
@@ -49,7 +50,7 @@ def lib():
         L.fdfk_create.argtypes = [C.POINTER(FdfkJob), C.POINTER(C.c_void_p)]
         for name in ("fdfk_run", "fdfk_profile"):
             getattr(L, name).argtypes = [C.c_void_p, C.c_int32]
-        for name in ("fdfk_sync", "fdfk_iteration", "fdfk_destroy", "fdfk_device_bytes"):
+        for name in ("fdfk_sync", "fdfk_iteration", "fdfk_destroy", "fdfk_device_bytes", "fdfk_query"):
             getattr(L, name).argtypes = [C.c_void_p]
         L.fdfk_destroy.restype = None
         L.fdfk_set_background.argtypes = [C.c_void_p, C.c_void_p]
@@ -141,10 +142,13 @@ class Solver:
     """One simulation resident on the GPU (a fdfk_solver handle)."""
 
     def __init__(self, nx, nz, npml, norder, dx, dz, dt, nt, c, x0=0.0, z0=0.0, nstp=1,
-                 rx=None, rz=None, R0=1e-3, npower=2, vpml=0.0, precision=FDFK_FP32, pml_top=False):
+                 rx=None, rz=None, R0=1e-3, npower=2, vpml=0.0, precision=FDFK_FP32, pml_top=False,
+                 wave=FDFK_PSV):
         """pml_top: an absorbing top layer of npml rows above row 1 (the
         reference's is_PML_top, izt = 1-npml) instead of the free surface; c
-        then holds rows izt..nz+npml."""
+        then holds rows izt..nz+npml.  wave=FDFK_SH: the transverse motion uy
+        of an incident SH wave (seismograms() gives it as the first array)."""
+        self.wave = wave
         self.nx, self.nz, self.npml, self.norder, self.nt = nx, nz, npml, norder, nt
         self.nbd = 2 * norder - 1
         self.precision = precision
@@ -161,7 +165,7 @@ class Solver:
         job = FdfkJob(nx=nx, nz=nz, npml=npml, norder=norder, is_pml_top=int(self.pml_top), npower=npower,
                       x0=x0, z0=z0, dx=dx, dz=dz, dt=dt, R0=R0, vpml=vpml, nt=nt, nstp=nstp,
                       nrcv=self.nrcv, rx=_ptr(self._rx), rz=_ptr(self._rz), c=_ptr(self._c),
-                      precision=precision)
+                      precision=precision, wave=wave)
         h = C.c_void_p()
         _check(lib().fdfk_create(C.byref(job), C.byref(h)))
         self.h = h
@@ -232,6 +236,13 @@ class Solver:
 
     def sync(self):
         _check(lib().fdfk_sync(self.h))
+
+    def idle(self):
+        """True when every queued iteration has completed (does not wait)."""
+        rc = lib().fdfk_query(self.h)
+        if rc < 0:
+            _check(rc)
+        return rc == 1
 
     @property
     def iteration(self):

@@ -131,6 +131,9 @@ def rel_l2(a, b):
 # ---------------------------------------------------------------------------
 # serving
 # ---------------------------------------------------------------------------
+SOLO_AHEAD = 2   # chunks of iterations queued per solo stream at most
+
+
 class Server:
     def __init__(self, hbm_budget_gb=200.0, chunk=100, log=None, workers=None, solo_cells=4_000_000,
                  block_steps=BG_BLOCK):
@@ -144,6 +147,10 @@ class Server:
         # the grids too small to fill the GPU
         self.solo_cells = solo_cells
         self.solo = {}
+        # chunks queued on each solo stream since it was last seen idle: at
+        # most SOLO_AHEAD, so that a large request's thousands of steps do
+        # not fill the hardware queues ahead of the batch and new requests
+        self.solo_queued = {}
         self.workers = workers or max(1, min(8, os.cpu_count() or 1))
         self.batch = Batch(3)
         self.used = 0.0
@@ -246,16 +253,26 @@ class Server:
                         results[req["id"]] = (sx, sz)
                     if len(lat) % 25 == 0:
                         self.log("served %d requests, %.1f s" % (len(lat), time.perf_counter() - t0))
+                queued_solo = False
                 for rid, (req, s) in list(self.solo.items()):
                     left = s.nt + 1 - s.iteration
                     if left > 0:
-                        s.run(min(self.chunk, left))       # queued on the solver's stream
+                        n = self.solo_queued.get(rid, 0)
+                        if n >= SOLO_AHEAD and s.idle():
+                            n = 0
+                        if n < SOLO_AHEAD:
+                            s.run(min(self.chunk, left))   # queued on the solver's stream
+                            self.solo_queued[rid] = n + 1
+                            queued_solo = True
                     if s.iteration > s.nt:
                         del self.solo[rid]
+                        self.solo_queued.pop(rid, None)
                         finishing[pool.submit(self._finish, s)] = req
                 if not self.active:
                     wait = 0.05 if not pending else max(0.0, min(0.05, pending[0]["arrival"] - now))
                     if self.solo:
+                        if not queued_solo:
+                            time.sleep(0.0005)   # every solo stream holds SOLO_AHEAD chunks
                         continue
                     if preparing or finishing:
                         cf.wait(list(preparing) + list(finishing), timeout=wait,

@@ -9,6 +9,7 @@ module fdfk
   integer(c_int), parameter, public :: FDFK_OK = 0, FDFK_EINVAL = -22, FDFK_ENOMEM = -12, &
                                        FDFK_EDEVICE = -5, FDFK_ERANGE = -34, FDFK_ENOSYS = -38
   integer(c_int32_t), parameter, public :: FDFK_FP32 = 0, FDFK_BF16 = 1, FDFK_FP16 = 2, FDFK_FP64 = 3
+  integer(c_int32_t), parameter, public :: FDFK_PSV = 0, FDFK_SH = 1
 
   ! one simulation: the scalars of get_parameter plus the arrays of read_data
   type, bind(C), public :: fdfk_job
@@ -25,6 +26,7 @@ module fdfk
     type(c_ptr)        :: rx, rz                 ! receiver coordinates (m), real(c_float)
     type(c_ptr)        :: c                      ! rho, lambda, mu (reference layout), real(c_float)
     integer(c_int32_t) :: precision              ! FDFK_FP32, FDFK_FP16 (materials) or FDFK_FP64
+    integer(c_int32_t) :: wave                   ! FDFK_PSV (in-plane) or FDFK_SH (transverse)
   end type
 
   ! FK background: the inputs of comput_background_wavefield

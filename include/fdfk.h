@@ -42,6 +42,13 @@ extern "C" {
  * reference, which is FP32) */
 #define FDFK_FP64 3
 
+/* motion of a solver: the in-plane P-SV motion of the reference (ux, uz), or
+ * the transverse SH motion uy (this library only: an incident SH wave, one
+ * component, on the general per-cell kernels; uy takes the place of ux in
+ * the background arrays, fields and seismograms, uz stays zero) */
+#define FDFK_PSV 0
+#define FDFK_SH 1
+
 typedef struct fdfk_solver fdfk_solver;
 
 /* One simulation: the scalars of src/Input.f90:get_parameter plus the arrays
@@ -63,6 +70,7 @@ typedef struct fdfk_job {
                                        power-of-two scales for the interior tiles (single
                                        solves, fused step); FDFK_FP64: everything in FP64
                                        (per-cell kernels, single solves); FDFK_BF16: not built */
+    int32_t wave;                   /* FDFK_PSV or FDFK_SH (FP32 or FP64, single solves) */
 } fdfk_job;
 
 /* FK background description: the inputs of comput_background_wavefield
@@ -122,6 +130,10 @@ int fdfk_get_state(fdfk_solver *s, float *ux, float *uz);
 int fdfk_set_state(fdfk_solver *s, const float *ux, const float *uz);
 /* wait for all queued work of this handle */
 int fdfk_sync(fdfk_solver *s);
+/* without waiting: 1 when all queued work of this handle has completed, 0
+ * while some is still queued or running (pacing of callers that queue
+ * chunks of iterations; no counterpart in the reference) */
+int fdfk_query(fdfk_solver *s);
 /* kernel timing of the stencil launch of each step (HIP events on the
  * handle's stream): k_step, the interior and the frame in one launch (or
  * k_interior2 alone when the environment sets FDFK_FUSED=0); enable before

@@ -36,7 +36,7 @@ program handle_api_run
   close(u)
 
   job = fdfk_job(nx, nz, npml, norder, 0, 2, x0, z0, dx, dz, dt, 1.0d-3, 0.0, nt, nstp, nrcv, &
-                 c_loc(rx), c_loc(rz), c_loc(c), FDFK_FP32)
+                 c_loc(rx), c_loc(rz), c_loc(c), FDFK_FP32, FDFK_PSV)
   ier = fdfk_create(job, h)
   if (ier /= FDFK_OK) call fail('fdfk_create')
   fk = fdfk_fk(nll, nlr, c_loc(fkl), c_loc(fkr), srcp, ds, f0, rayp, c_loc(bazs), 0)

@@ -1,0 +1,104 @@
+"""GPU path of the SH (transverse) motion against the CPU oracle's
+formulation of it (no counterpart in the reference).
+
+The SH kernels evaluate the oracle's pointwise operators with contraction
+off and FP64 accumulation in the same order, so the FP32 run equals the
+oracle bit for bit (seismograms and final fields), with the free surface
+and with the absorbing top layer.  The GPU SH FK background is held to 1e-6
+of its peak against the oracle's.  uy takes the place of ux in the
+background arrays, the fields and the seismograms.
+"""
+import numpy as np
+import pytest
+
+from fdfk2d_amd import lib, refstate
+from oracle import pyoracle as po
+from tests._cases import Case, fk_model, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _sh_background(case):
+    izt = case.izt
+    b = po.background_sh(case.nx, case.nz, case.npml, izt, case.nbd, case.nt, 0.0, 0.0, case.dx, case.dz,
+                         case.dt, case.ds, case.f0, case.rayp, case.bazs, case.fk, case.fk)
+    z = lambda a: np.zeros_like(a)
+    twelve = [b[0], z(b[0]), b[1], z(b[1]), b[2], z(b[2]), b[4], z(b[4]), b[5], z(b[5]), b[6], z(b[6])]
+    if izt == 1:
+        return b, twelve
+    return b, (twelve, [b[3], z(b[3]), b[7], z(b[7])])
+
+
+def _oracle_sh(case, bg8):
+    st = po.sh_state(case.nx, case.nz, case.npml, case.norder, case.dx, case.dz, case.dt, case.nt, case.c,
+                     pml_top=case.pml_top, vpml=refstate.vp_max(case.c))
+    tr = []
+    for it in range(case.nt + 1):
+        tr.append([po.loadseis(case.nx, case.nz, case.npml, case.izt, 0.0, 0.0, case.dx, case.dz, float(a), float(b),
+                               st["uy"], st["uy"])[0] for a, b in zip(case.rx, case.rz)])
+        po.updatedispl_sh(st, it, bg8)
+    return np.asarray(tr, np.float32).T, st
+
+
+def _solver(case, precision=lib.FDFK_FP32):
+    return lib.Solver(case.nx, case.nz, case.npml, case.norder, case.dx, case.dz, case.dt, case.nt, case.c,
+                      rx=case.rx, rz=case.rz, precision=precision, vpml=refstate.vp_max(case.c),
+                      pml_top=case.pml_top, wave=lib.FDFK_SH)
+
+
+@pytest.mark.parametrize("pml_top,norder", [(False, 3), (True, 3), (False, 1), (False, 6)])
+def test_sh_equals_oracle_bit_for_bit(pml_top, norder):
+    print(f"case: SH, 121x60, top={pml_top}, order {2 * norder}, 300 iterations", flush=True)
+    case = Case(nt=300, pml_top=pml_top, norder=norder, nrcv=13, src_is_p=False, npml=max(15, 2 * norder - 1))
+    bg8, bg = _sh_background(case)
+    oy, st = _oracle_sh(case, bg8)
+    s = _solver(case)
+    s.set_background(bg)
+    s.run(case.nt + 1)
+    gy, gz = s.seismograms()
+    uy, _ = s.get_state()
+    s.close()
+    assert np.abs(oy).max() > 0 and not np.any(gz)
+    assert np.array_equal(gy.view(np.uint32), oy.view(np.uint32))
+    assert np.array_equal(uy.view(np.uint32), st["uy"].view(np.uint32))
+
+
+@pytest.mark.parametrize("pml_top", [False, True])
+def test_sh_gpu_fk_matches_oracle(pml_top):
+    print(f"case: SH FK background on the GPU, top={pml_top}", flush=True)
+    case = Case(nt=300, pml_top=pml_top, src_is_p=False)
+    bg8, bg = _sh_background(case)
+    s = _solver(case)
+    s.compute_background(case.fk, case.fk, False, case.f0, case.ds, case.rayp, case.bazs)
+    got = s.get_background()
+    s.close()
+    want = bg if not pml_top else bg[0] + bg[1]
+    got = got if not pml_top else got[0] + got[1]
+    peak = max(np.abs(a).max() for a in want)
+    assert peak > 0
+    for a, b in zip(got, want):
+        assert np.abs(a.astype(np.float64) - b).max() <= 1e-6 * peak
+
+
+def test_sh_pipeline_free_surface_against_fk():
+    """GPU FK + GPU run, FD model == FK model (one layer over a half-space):
+    the surface records equal the SH FK surface response."""
+    print("case: SH pipeline, 151x60, 700 iterations", flush=True)
+    layers = ((0.0, 6000.0, 3450.0), (6000.0, 8000.0, 4450.0))
+    case = Case(nx=151, nz=60, npml=20, nt=700, layers=layers, anomaly=None, f0=1.0, nrcv=7, src_is_p=False,
+                rayp_sdeg=6.0)
+    # the FD material change between rows at 5800 and 6000 m sits at 5900 m
+    case.fk = fk_model(((0.0, 6000.0, 3450.0), (5900.0, 8000.0, 4450.0)))
+    s = _solver(case)
+    s.compute_background(case.fk, case.fk, False, case.f0, case.ds, case.rayp, case.bazs)
+    s.run(case.nt + 1)
+    gy, _ = s.seismograms()
+    s.close()
+    p = case.rayp
+    xe = (1 - case.nbd - 1) * case.dx
+    u = po.fk_traces_sh(case.fk, p, case.f0, case.ds, case.dt, case.nt + 400, np.array([0.0]),
+                        (case.nz + case.nbd - 1) * case.dz, 1.5 / case.f0, True)[0]
+    n = np.arange(case.nt + 1)
+    for i, rx in enumerate(case.rx):
+        k = np.interp(n - p * (rx - xe) / case.dt, np.arange(u.size), u)
+        assert rel_l2(gy[i], k) < 0.005
