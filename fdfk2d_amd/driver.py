@@ -12,6 +12,22 @@ With --all-events every event line of a multi-event Source.dat is simulated
 (the reference reads only the first, src/Input.f90:142-146), the events
 advanced together as batches of resident solvers, and event k is written to
 <outpath>/<prefix>_<k>x.su and z.su.
+
+Three components (--3c, optionally --spol DEG).  The reference keeps the
+in-plane horizontal motion u_x rotated by the receiver's relative
+back-azimuth phi = baz - az_org and projected on the profile, x = u_x cos phi
+(src/FDFK2D.f90:537-543).  With --3c the same rotation is applied to the
+whole horizontal vector (u_x, u_y) and both of its components are written:
+
+    x = u_x cos phi - u_y sin phi     (<prefix>x.su, the reference's file)
+    y = u_x sin phi + u_y cos phi     (<prefix>y.su, 90 deg clockwise of x)
+
+u_y is the transverse (SH) motion.  An incident S wave polarised at psi
+degrees from SV toward SH (--spol, default 0: the reference's pure SV) is
+simulated as an SV solve of strength ds cos psi and an SH solve (FDFK_SH) of
+strength ds sin psi; in the 2-D model the two motions do not couple, and an
+incident P wave has no SH part.  With psi = 0 x.su and z.su are the
+reference's files bit for bit and y.su is u_x sin phi.
 """
 import os
 import sys
@@ -20,24 +36,48 @@ import time
 import numpy as np
 
 from . import geometry, io, refstate
-from .lib import Batch, Solver
+from .lib import FDFK_PSV, FDFK_SH, Batch, Solver
 
 
 def write_snapshot(s, it, nx, nz, npml, dz, snapdir="snapshots", izt=1):
     """Snapshot at the top of iteration it (src/FDFK2D.f90:467-480): level 2
     of ux(1:2, :, :) and uz, rows 1..nz of columns 1..nx, one SU trace per
     column, header word 58 = nz and 59 = dz, in ./snapshots/<it>ux.su and
-    <it>uz.su (five-digit iteration, I5.5).  Rows start at izt."""
+    <it>uz.su (five-digit iteration, I5.5).  Rows start at izt.  An SH
+    solver writes its one component to <it>uy.su."""
     os.makedirs(snapdir, exist_ok=True)
     ux, uz = s.get_state()
     NX, NZ = nx + 2 * npml, nz + npml - izt + 1
-    for name, u in (("ux", ux), ("uz", uz)):
+    fields = (("uy", ux),) if s.wave == FDFK_SH else (("ux", ux), ("uz", uz))
+    for name, u in fields:
         lev2 = u.reshape(NX, NZ, 2)[npml: npml + nx, 1 - izt: 1 - izt + nz, 1]
         io.write_su(os.path.join(snapdir, "%05d%s.su" % (it, name)), lev2, int(dz))
 
 
-def _prepare(job, src, log):
-    """Solver with its FK background for one event (src/FDFK2D.f90:311-434)."""
+def rotate_horizontal(sx, sy, phi):
+    """(x, y) = R(phi) (u_x, u_y) per receiver, in FP64, rounded to FP32;
+    sy None: no transverse motion, x = u_x cos phi exactly as
+    src/FDFK2D.f90:540-542 computes it."""
+    c, s = np.cos(phi)[:, None], np.sin(phi)[:, None]
+    ux = sx.astype(np.float64)
+    if sy is None:
+        return (ux * c).astype(np.float32), (ux * s).astype(np.float32)
+    uy = sy.astype(np.float64)
+    return (ux * c - uy * s).astype(np.float32), (ux * s + uy * c).astype(np.float32)
+
+
+def split_strength(src, spol_deg):
+    """Strengths of the SV and SH solves of one event: (ds_sv, ds_sh); an
+    incident P wave has no SH part."""
+    if src["src_is_p"] or spol_deg == 0.0:
+        return src["ds"], 0.0
+    psi = np.deg2rad(spol_deg)
+    return src["ds"] * float(np.cos(psi)), src["ds"] * float(np.sin(psi))
+
+
+def _prepare(job, src, log, wave=FDFK_PSV):
+    """Solver with its FK background for one event (src/FDFK2D.f90:311-434);
+    wave FDFK_SH: the SH solver of an S event (src["ds"] its strength)."""
     fd = job["fd"]
     nx, nz, npml, norder = job["nx"], job["nz"], fd["npml"], fd["norder"]
     dt = fd["dt"]
@@ -58,10 +98,10 @@ def _prepare(job, src, log):
         rayp = 1e-9 * np.sign(rayp if rayp != 0 else 1.0)
     s = Solver(nx, nz, npml, norder, fd["dx"], fd["dz"], dt, nt, job["c"], x0=fd["x0"], z0=fd["z0"],
                nstp=nstp, rx=job["rx"], rz=job["rz"], R0=1e-3, npower=2,
-               vpml=refstate.vp_max(job["c"]), pml_top=fd["pml_top"])
+               vpml=refstate.vp_max(job["c"]), pml_top=fd["pml_top"], wave=wave)
     try:
-        s.compute_background(job["fk_left"], job["fk_right"], src["src_is_p"], src["f0"], src["ds"],
-                             rayp, bazs)
+        s.compute_background(job["fk_left"], job["fk_right"], src["src_is_p"] and wave == FDFK_PSV, src["f0"],
+                             src["ds"], rayp, bazs)
         s.sync()
     except Exception:
         s.close()
@@ -69,60 +109,102 @@ def _prepare(job, src, log):
     return s, dict(nt=nt, nsis=nsis, tstp=float(tstp))
 
 
-def _finish(job, src, s):
-    """Seismograms with the radial projection by cos(baz) (FDFK2D.f90:537-543)."""
+def _receiver_phi(job, src):
+    """Relative back-azimuth of each receiver (src/FDFK2D.f90:537-539)."""
     fd = job["fd"]
+    return geometry.backazimuth_grids(fd["lat_org"], fd["lon_org"], fd["az_org"], job["rx"],
+                                      src["evla"], src["evlo"]) - fd["az_org"] * np.pi / 180.0
+
+
+def _finish(job, src, s, sy=None, three=False):
+    """Seismograms with the radial projection by cos(baz) (FDFK2D.f90:537-543):
+    (x, z); three components: (x, y, z), sy the SH solve's u_y or None."""
     sx, sz = s.seismograms()
-    br = geometry.backazimuth_grids(fd["lat_org"], fd["lon_org"], fd["az_org"], job["rx"],
-                                    src["evla"], src["evlo"]) - fd["az_org"] * np.pi / 180.0
-    sx = (sx.astype(np.float64) * np.cos(br)[:, None]).astype(np.float32)
-    return sx, sz
+    x, y = rotate_horizontal(sx, sy, _receiver_phi(job, src))
+    return (x, y, sz) if three else (x, sz)
 
 
-def simulate(job, log=print):
-    fd, src = job["fd"], job["src"]
-    nx, nz, npml = job["nx"], job["nz"], fd["npml"]
-    t0 = time.perf_counter()
-    s, info = _prepare(job, src, log)
-    nt = info["nt"]
-    t1 = time.perf_counter()
-    log(f"background (FK, GPU FP64): {t1 - t0:.2f} s")
+def _sh_part(job, src, spol, log):
+    """u_y seismograms of an event's SH solve, None when it has no SH part."""
+    _, ds_sh = split_strength(src, spol)
+    if ds_sh == 0.0:
+        return None
+    s, info = _prepare(job, dict(src, ds=ds_sh), log, wave=FDFK_SH)
     try:
-        if fd.get("outsnap") and fd.get("nstep", 0) > 0:
-            done = 0
-            for it in range(fd["nstep"], nt + 1, fd["nstep"]):
-                s.run(it - done)
-                done = it
-                write_snapshot(s, it, nx, nz, npml, fd["dz"], izt=job.get("izt", 1))
-            s.run(nt + 1 - done)
-        else:
-            s.run(nt + 1)
+        s.run(info["nt"] + 1)
         s.sync()
-        t2 = time.perf_counter()
-        cells = (nx + 2 * npml) * (nz + npml) * (nt + 1)
-        log(f"time loop: {nt + 1} iterations in {t2 - t1:.2f} s ({cells / (t2 - t1) / 1e9:.2f} Gcell/s)")
-        sx, sz = _finish(job, src, s)
+        return s.seismograms()[0]
     finally:
         s.close()
-    return sx, sz, info
 
 
-def simulate_events(job, events, hbm_budget_gb=200.0, log=print):
+def _advance(job, solvers, nt, snap):
+    """Queue every iteration of the solvers (each on its own stream, so they
+    overlap on the GPU), with the snapshots at every nstep iterations."""
+    fd = job["fd"]
+    if snap:
+        done = 0
+        for it in range(fd["nstep"], nt + 1, fd["nstep"]):
+            for s in solvers:
+                s.run(it - done)
+            done = it
+            for s in solvers:
+                write_snapshot(s, it, job["nx"], job["nz"], fd["npml"], fd["dz"], izt=job.get("izt", 1))
+        for s in solvers:
+            s.run(nt + 1 - done)
+    else:
+        for s in solvers:
+            s.run(nt + 1)
+    for s in solvers:
+        s.sync()
+
+
+def simulate(job, log=print, three=False, spol=0.0):
+    """One event: (x, z) seismograms and the run's info; three components
+    (three=True): (x, y, z).  spol: polarisation of an incident S wave in
+    degrees from SV toward SH (the SH part runs on its own solver)."""
+    fd, src = job["fd"], job["src"]
+    nx, nz, npml = job["nx"], job["nz"], fd["npml"]
+    ds_sv, ds_sh = split_strength(src, spol)
+    t0 = time.perf_counter()
+    s, info = _prepare(job, dict(src, ds=ds_sv), log)
+    s_sh = None
+    nt = info["nt"]
+    try:
+        if ds_sh != 0.0:
+            s_sh, _ = _prepare(job, dict(src, ds=ds_sh), log, wave=FDFK_SH)
+        t1 = time.perf_counter()
+        log(f"background (FK, GPU FP64): {t1 - t0:.2f} s")
+        _advance(job, [s] + ([s_sh] if s_sh else []), nt, fd.get("outsnap") and fd.get("nstep", 0) > 0)
+        t2 = time.perf_counter()
+        cells = (nx + 2 * npml) * (nz + npml - job.get("izt", 1) + 1) * (nt + 1) * (2 if s_sh else 1)
+        log(f"time loop: {nt + 1} iterations in {t2 - t1:.2f} s ({cells / (t2 - t1) / 1e9:.2f} Gcell/s)")
+        out = _finish(job, src, s, s_sh.seismograms()[0] if s_sh else None, three)
+    finally:
+        s.close()
+        if s_sh is not None:
+            s_sh.close()
+    return out + (info,)
+
+
+def simulate_events(job, events, hbm_budget_gb=200.0, log=print, three=False, spol=0.0):
     """Every event of a multi-event Source.dat on the same model, advanced
     together as batches of resident solvers (fdfk_batch).  Events are taken
     in file order, as many per batch as the HBM budget holds; each result is
-    bit-identical to the event run alone.  Returns [(sx, sz)], info."""
+    bit-identical to the event run alone.  The SH solves of S events
+    (spol != 0) run one at a time after their batch.  Returns [(x, z)] or,
+    three components, [(x, y, z)], and info."""
     fd = job["fd"]
     out, info, k = [], None, 0
     t0 = time.perf_counter()
     if fd["pml_top"]:
         # solvers with an absorbing top layer run one at a time (not batched)
         for ev in events:
-            s, info = _prepare(job, ev, log)
+            s, info = _prepare(job, dict(ev, ds=split_strength(ev, spol)[0]), log)
             try:
                 s.run(info["nt"] + 1)
                 s.sync()
-                out.append(_finish(job, ev, s))
+                out.append(_finish(job, ev, s, _sh_part(job, ev, spol, log), three))
             finally:
                 s.close()
         return out, info
@@ -131,7 +213,7 @@ def simulate_events(job, events, hbm_budget_gb=200.0, log=print):
         members = []
         try:
             while k < len(events):
-                s, info = _prepare(job, events[k], log)
+                s, info = _prepare(job, dict(events[k], ds=split_strength(events[k], spol)[0]), log)
                 members.append((events[k], s))
                 batch.add(s)
                 k += 1
@@ -140,7 +222,7 @@ def simulate_events(job, events, hbm_budget_gb=200.0, log=print):
             batch.run(info["nt"] + 1)
             batch.sync()
             for ev, s in members:
-                out.append(_finish(job, ev, s))
+                out.append(_finish(job, ev, s, _sh_part(job, ev, spol, log), three))
         finally:
             batch.close()
             for _, s in members:
@@ -150,15 +232,35 @@ def simulate_events(job, events, hbm_budget_gb=200.0, log=print):
     return out, info
 
 
+def _options(argv):
+    """--all-events, --3c and --spol DEG (or --spol=DEG) out of argv."""
+    opts = dict(all_events=False, three=False, spol=0.0)
+    rest, i = [], 0
+    while i < len(argv):
+        a = argv[i]
+        if a == "--all-events":
+            opts["all_events"] = True
+        elif a == "--3c":
+            opts["three"] = True
+        elif a.startswith("--spol="):
+            opts["spol"] = float(a.split("=", 1)[1])
+        elif a == "--spol" and i + 1 < len(argv):
+            opts["spol"] = float(argv[i + 1])
+            i += 1
+        else:
+            rest.append(a)
+        i += 1
+    return opts, rest
+
+
 def main(argv=None):
-    argv = list(sys.argv[1:] if argv is None else argv)
-    all_events = "--all-events" in argv
-    argv = [a for a in argv if a != "--all-events"]
+    opts, argv = _options(list(sys.argv[1:] if argv is None else argv))
     if len(argv) < 3:
-        print("FDFK2D (MI355X path)\n./driver ./inpath inpar.dat ./outpath seism [--all-events]")
+        print("FDFK2D (MI355X path)\n./driver ./inpath inpar.dat ./outpath seism [--all-events] [--3c] [--spol DEG]")
         return 0
     prefix, finpar, outpath = argv[:3]
     fsis = argv[3] if len(argv) > 3 else "seis"
+    three, spol = opts["three"], opts["spol"]
     job = io.read_job(prefix, finpar)
     # *_without_pml.su copies of the models (Input.f90:305-341)
     for key, arr in (("vp_model", job["vp"]), ("vs_model", job["vs"])):
@@ -167,21 +269,22 @@ def main(argv=None):
         io.write_su(os.path.join(prefix, name[:-3] + "_without_pml.su"),
                     arr[job["fd"]["npml"]: job["fd"]["npml"] + job["nx"], r1: r1 + job["nz"]], 1000)
     os.makedirs(outpath, exist_ok=True)
-    if all_events:
+    comps = "xyz" if three else "xz"
+    if opts["all_events"]:
         # every line of Source.dat, <fsis>_<k>x.su and z.su for event k (1-based, file order)
         events = io.read_events(os.path.join(prefix, job["names"]["source"]))
         if job["fd"].get("outsnap"):
             print("snapshots are written for single-event runs only")
-        results, info = simulate_events(job, events)
+        results, info = simulate_events(job, events, three=three, spol=spol)
         dh = int(np.float32(info["tstp"]) * np.float32(1e6))
-        for k, (sx, sz) in enumerate(results, 1):
-            io.write_su(os.path.join(outpath, f"{fsis}_{k}x.su"), sx, dh)
-            io.write_su(os.path.join(outpath, f"{fsis}_{k}z.su"), sz, dh)
+        for k, traces in enumerate(results, 1):
+            for c, tr in zip(comps, traces):
+                io.write_su(os.path.join(outpath, f"{fsis}_{k}{c}.su"), tr, dh)
         return 0
-    sx, sz, info = simulate(job)
+    *traces, info = simulate(job, three=three, spol=spol)
     dh = int(np.float32(info["tstp"]) * np.float32(1e6))
-    io.write_su(os.path.join(outpath, fsis + "x.su"), sx, dh)
-    io.write_su(os.path.join(outpath, fsis + "z.su"), sz, dh)
+    for c, tr in zip(comps, traces):
+        io.write_su(os.path.join(outpath, fsis + c + ".su"), tr, dh)
     return 0
 
 
