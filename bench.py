@@ -106,6 +106,59 @@ def gpu_bench(steps, warmup, workload, bg_block=0, nt=0, autotune=False, precisi
                 background_s=round(t_bg, 3) if bg_mode != "zero" else None)
 
 
+def variant_bench(variant, steps=200, warmup=10, tiled=True):
+    """The single large solve with the absorbing top layer ("top", P-SV), as
+    SH motion ("sh", free surface) or both ("sh-top"), on the general-path
+    solver: the deep interior on the tiled interior kernel and the cells
+    around it on the per-cell kernels, or (tiled=False) every cell on the
+    per-cell kernels.  value = padded cells (rows from izt) per second."""
+    from fdfk2d_amd import configs, lib
+    top, sh = variant in ("top", "sh-top"), variant.startswith("sh")
+    nt = warmup + steps + 1
+    cfg = configs.c2(nt=nt)
+    c, NX, NZ, npml = cfg["c"], cfg["NX"], cfg["NZ"], cfg["npml"]
+    if top:
+        # npml rows above the surface repeating row 1 (the damped cells take
+        # the material of the outermost undamped one)
+        c3 = c.reshape(NX, NZ, 3)
+        c = np.ascontiguousarray(np.concatenate([np.repeat(c3[:, :1], npml, axis=1), c3], axis=1)).ravel()
+        NZ += npml
+    old = os.environ.get("FDFK_GEN_TILED")
+    os.environ["FDFK_GEN_TILED"] = "1" if tiled else "0"
+    try:
+        s = lib.Solver(cfg["nx"], cfg["nz"], npml, cfg["norder"], cfg["dx"], cfg["dz"], cfg["dt"], nt, c,
+                       rx=cfg["rx"], rz=cfg["rz"], pml_top=top, wave=lib.FDFK_SH if sh else lib.FDFK_PSV)
+    finally:
+        if old is None:
+            del os.environ["FDFK_GEN_TILED"]
+        else:
+            os.environ["FDFK_GEN_TILED"] = old
+    try:
+        s.compute_background(cfg["fk_left"], cfg["fk_right"], cfg["src_is_p"] and not sh, cfg["f0"], cfg["ds"],
+                             cfg["rayp"], cfg["bazs"])
+        s.run(warmup)
+        s.sync()
+        t0 = time.perf_counter()
+        s.run(steps)
+        s.sync()
+        wall = time.perf_counter() - t0
+        sx = s.seismograms()[0]
+        finite = bool(np.isfinite(sx).all()) and float(np.abs(sx).max()) > 0
+        dev = s.device_bytes
+    finally:
+        s.close()
+    cells = NX * NZ
+    return {"metric": "Gcell-updates/s", "value": round(cells * steps / wall / 1e9, 3), "unit": "Gcell-updates/s",
+            "ms_per_step": round(wall / steps * 1e3, 4), "steps": steps, "warmup": warmup, "dtype": "f32",
+            "data": "synthetic",
+            "config": {"workload": "single large solve 8001x2000, dx=dz=50 m, dt=0.003 s, 40-cell PML, order 6, "
+                                   "401 receivers; " + {"top": "absorbing top layer, P-SV",
+                                                        "sh": "SH motion, free surface",
+                                                        "sh-top": "SH motion, absorbing top layer"}[variant],
+                       "grid_padded": [NX, NZ], "interior": "tiled" if tiled else "per-cell kernels only",
+                       "device_gib": round(dev / 2**30, 3), "finite": finite}}
+
+
 def cpu_baseline(sample_steps):
     from fdfk2d_amd import configs, refstate
     from oracle import pyoracle as po
@@ -476,7 +529,15 @@ def main():
                          "materials, rel-L2 of traces and P-RFs against FP64")
     ap.add_argument("--no-solve", action="store_true",
                     help="default line without the whole-solve and serving objects")
+    ap.add_argument("--variant", default="", choices=["", "top", "sh", "sh-top"],
+                    help="the single large solve with the absorbing top layer, as SH motion, or both "
+                         "(general-path solver) instead of the default line")
+    ap.add_argument("--per-cell", action="store_true",
+                    help="with --variant: every cell on the per-cell kernels (no tiled interior)")
     a = ap.parse_args()
+    if a.variant:
+        print(json.dumps(variant_bench(a.variant, a.steps, a.warmup, tiled=not a.per_cell)), flush=True)
+        return
     if a.copy_calib:
         copy_calibration()
         return

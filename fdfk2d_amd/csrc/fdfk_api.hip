@@ -11,6 +11,7 @@
 #include "fdfk_kernels.hip"
 #include "fdfk_frame.hip"
 #include "fdfk_interior2.hip"
+#include "fdfk_interior_sh.hip"
 #include "fdfk_general.hip"
 
 #include <algorithm>
@@ -268,6 +269,15 @@ struct fdfk_solver {
     float *bg_top[4] = {};             // top ring: uxt_reg, uzt_reg, uxt_pml, uzt_pml
     long bg_top_n[4] = {};
     bool bg_top_valid = false;         // without it the hybrid contour is open at the top
+    // FP32 general path: the deep interior (columns 1..nx, rows 2*norder..nz)
+    // runs on the tiled interior kernel (P-SV: k_interior2, SH:
+    // k_interior_sh) over a view of the planes that starts at row 1, the
+    // general kernels update the cells around it (FDFK_GEN_TILED=0: the
+    // general kernels update every cell)
+    bool gtiled = false;
+    Geo gview{};
+    float *gbf = nullptr;              // dt^2/rho of the tiled kernel
+    int glx = 45;
     // profiling
     bool prof = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_int, ev_step;
@@ -854,6 +864,7 @@ GenArgs<T> gen_args(fdfk_solver *s)
     GenArgs<T> a{};
     a.g = s->g;
     a.zt = s->zt;
+    a.hx0 = 1; a.hx1 = 0;   // no hole: the step kernel updates every cell
     a.top = s->zt != 1;
     a.c0 = s->gc0; a.c1 = s->gc1;
     a.ux = (const T *)s->gux[s->cur]; a.uz = (const T *)s->guz[s->cur];
@@ -877,9 +888,52 @@ GenArgs<T> gen_args(fdfk_solver *s)
     return a;
 }
 
+// Tiled interior of an FP32 general-path solver: the kernel arguments over
+// the view of the planes from row 1 (offset 1-zt elements into each plane)
+template <int NO>
+IntArgs<NO> gen_interior_args(fdfk_solver *s)
+{
+    IntArgs<NO> a{};
+    const size_t off = (size_t)(1 - s->zt);
+    auto view = [off](void *p) { return p ? (float *)p + off : nullptr; };
+    a.g = s->gview;
+    a.k = make_icoef<NO>(s);
+    a.ux = view(s->gux[s->cur]); a.uz = view(s->guz[s->cur]);
+    a.uxp = view(s->gux[1 - s->cur]); a.uzp = view(s->guz[1 - s->cur]);
+    a.uxo = a.uxp; a.uzo = a.uzp;
+    a.bf = s->gbf + off; a.lam = view(s->glam); a.mu = view(s->gmu);
+    a.ringx = view(s->gring[0]); a.ringz = view(s->gring[1]);
+    a.corr = s->bg_valid && s->it <= s->nt;
+    a.lx = s->glx;
+    int_tiling<NO, 256>(a.g, a.lx, a.nbx, a.nbz, a.nsb, a.zseg);
+    return a;
+}
+
+// its tile width: the occupancy model of plan_lx for the kernel it launches
+template <int NO>
+int gen_plan_lx(const Geo &v, bool sh)
+{
+    int dev = 0, ncu = 0, bpc = 0;
+    const hipError_t oe = sh ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_interior_sh<NO, 256>, 256, 0)
+                             : hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_interior2<NO, 256>, 256, 0);
+    if (oe != hipSuccess || hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0 || bpc <= 0)
+        return 45;
+    constexpr int WARM = IntKernel2<NO, 256>::WARM;
+    const long slots = (long)ncu * bpc;
+    long best = -1;
+    int blx = 45;
+    for (int lx = 4; lx <= std::max(4, v.ix1 - v.ix0 + 1); ++lx) {
+        const long cost = ((int_blocks<NO, 256>(v, lx) + slots - 1) / slots) * (lx + WARM);
+        if (best < 0 || cost < best) { best = cost; blx = lx; }
+    }
+    return blx;
+}
+
 // One iteration of the general path: prep (receivers before the update,
-// ring values, memories), then the cell update; out of place into level
-// n-1, then the levels trade places.
+// ring values, memories), then the cell update (FP32 with the tiled
+// interior: the interior rectangle, then the cells around it); out of place
+// into level n-1, then the levels trade places.
 template <class T>
 int gen_step_t(fdfk_solver *s, bool record)
 {
@@ -894,19 +948,45 @@ int gen_step_t(fdfk_solver *s, bool record)
         r.rx = s->rx; r.rz = s->rz;
         r.seisx = (T *)s->gseis[0]; r.seisz = (T *)s->gseis[1];
     }
-    const int zr = a.top_ring ? 1 - g.nbd : 1;
     const long nrec = rec ? s->nrcv : 0;
-    const long nring = s->gring[0] ? (long)(g.nx + 2 * g.nbd) * (g.nz + g.nbd - zr + 1) : 0;
+    int rb[8];
+    gring_box(g, a.top_ring, rb);
+    const long nring = s->gring[0] ? gbox_count(rb[0], rb[1], rb[2], rb[3], rb[4], rb[5], rb[6], rb[7]) : 0;
     const long npsi = 2L * g.npml * g.nz + (s->zt != 1 ? 2L : 1L) * g.NX * g.npml;
     const long np = nrec + nring + npsi;
-    const long nc = (long)g.NX * g.NZ;
+    long nc = (long)g.NX * g.NZ;
+    if constexpr (sizeof(T) == 4) {
+        if (s->gtiled) {
+            const Geo &v = s->gview;
+            a.hx0 = v.ix0; a.hx1 = v.ix1; a.hz0 = v.iz0; a.hz1 = v.iz1;
+            const long nm = v.iz1 - v.iz0 + 1;
+            nc = (long)(v.iz0 - s->zt) * g.NX + (long)(s->zt + g.NZ - 1 - v.iz1) * g.NX +
+                 (long)(g.NX - (v.ix1 - v.ix0 + 1)) * nm;
+        }
+    }
+    auto interior = [&]() {
+        if constexpr (sizeof(T) == 4) {
+            if (!s->gtiled) return;
+            with_order(g.norder, [&](auto no) {
+                constexpr int NO = decltype(no)::value;
+                const IntArgs<NO> ia = gen_interior_args<NO>(s);
+                const unsigned nb = (unsigned)(ia.nbx * ia.nbz + ia.nsb);
+                if (s->sh)
+                    hipLaunchKernelGGL((k_interior_sh<NO, 256>), dim3(nb), dim3(256), 0, s->stream, ia);
+                else
+                    hipLaunchKernelGGL((k_interior2<NO, 256>), dim3(nb), dim3(256), 0, s->stream, ia);
+            });
+        }
+    };
     if (s->sh) {
         hipLaunchKernelGGL(k_gen_prep_sh<T>, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s->stream, a, r, nrec,
-                           nring, zr, (T *)s->gring[0]);
+                           nring, (T *)s->gring[0]);
+        interior();
         hipLaunchKernelGGL(k_gen_step_sh<T>, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s->stream, a);
     } else {
         hipLaunchKernelGGL(k_gen_prep<T>, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s->stream, a, r, nrec,
-                           nring, zr, (T *)s->gring[0], (T *)s->gring[1]);
+                           nring, (T *)s->gring[0], (T *)s->gring[1]);
+        interior();
         hipLaunchKernelGGL(k_gen_step<T>, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s->stream, a);
     }
     hipError_t e = hipGetLastError();
@@ -1159,6 +1239,22 @@ static int gen_create(const fdfk_job *job, fdfk_solver **out)
         hipLaunchKernelGGL(k_gen_import_material<T>, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0,
                            s->stream, g, s->zt, dc, (T *)s->grho, (T *)s->glam, (T *)s->gmu);
         hipFreeAsync(dc, s->stream);
+    }
+    if constexpr (sizeof(T) == 4) {
+        // the tiled interior: columns 1..nx, rows 2*norder..nz (full-order
+        // stencils whose reach stays in rows >= 1), as on the tiled path
+        Geo v = g;
+        v.NZ = g.nz + g.npml;
+        v.ix0 = 1; v.ix1 = g.nx; v.iz0 = 2 * g.norder; v.iz1 = g.nz;
+        const char *e = std::getenv("FDFK_GEN_TILED");
+        if (!(e && std::atoi(e) == 0) && v.iz1 >= v.iz0) {
+            s->gtiled = true;
+            s->gview = v;
+            if ((rc = s->dalloc(&s->gbf, nf))) return bad(rc);
+            hipLaunchKernelGGL(k_gen_bf, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, s->stream, (long)nf,
+                               (float)(g.dt * g.dt), (const float *)s->grho, s->gbf);
+            with_order(g.norder, [&](auto no) { s->glx = gen_plan_lx<decltype(no)::value>(v, s->sh); });
+        }
     }
     // strips L, R, B (and T with a top layer): split parts with two
     // interleaved levels, x-memories, z-memories in the bottom and top strips

@@ -21,7 +21,9 @@
 // formulation bit for bit and T = double its FP64 build.
 //
 // Launches per step: k_gen_prep (receivers, hybrid ring values, PML
-// memories, all from level n), then k_gen_step (every cell of level n+1).
+// memories, all from level n), then k_gen_step (every cell of level n+1,
+// or, in FP32, the cells around the deep interior rectangle that the tiled
+// interior kernel updates in the same step).
 // ---------------------------------------------------------------------
 namespace fdfk {
 
@@ -46,6 +48,9 @@ struct GenArgs {
     int top_ring;               // the top ring's background is present
     const T *bg[16];            // twelve reference arrays + uxt_reg, uzt_reg, uxt_pml, uzt_pml
     int it;
+    // hole: the rectangle columns hx0..hx1 x rows hz0..hz1 that the tiled
+    // interior kernel updates in the same step (none when hx0 > hx1)
+    int hx0, hx1, hz0, hz1;
 };
 
 template <class T>
@@ -68,6 +73,57 @@ __device__ __forceinline__ T ghmean(T a, T b)
 {
 #pragma clang fp contract(off)
     return (T)2.0f / ((T)1.0f / a + (T)1.0f / b);
+}
+
+// Cell t of the box columns x0..x1 x rows z0..z1 minus the hole columns
+// hx0..hx1 x rows hz0..hz1 (inside the box; none when hx0 > hx1 or hz0 >
+// hz1), band by band: rows above the hole, rows below it, then the columns
+// left and right of it; false past the last one.
+__device__ __forceinline__ bool gbox_cell(long t, int x0, int x1, int z0, int z1, int hx0, int hx1, int hz0, int hz1,
+                                          int &iz, int &ix)
+{
+    const long W = x1 - x0 + 1, H = z1 - z0 + 1;
+    if (hx0 > hx1 || hz0 > hz1) {
+        if (t >= W * H) return false;
+        ix = x0 + (int)(t / H); iz = z0 + (int)(t % H);
+        return true;
+    }
+    const long na = hz0 - z0, nb = z1 - hz1, nm = hz1 - hz0 + 1;
+    if (t < na * W) { ix = x0 + (int)(t / na); iz = z0 + (int)(t % na); return true; }
+    t -= na * W;
+    if (t < nb * W) { ix = x0 + (int)(t / nb); iz = hz1 + 1 + (int)(t % nb); return true; }
+    t -= nb * W;
+    const long nl = (long)(hx0 - x0) * nm;
+    if (t < nl) { ix = x0 + (int)(t / nm); iz = hz0 + (int)(t % nm); return true; }
+    t -= nl;
+    if (t < (long)(x1 - hx1) * nm) { ix = hx1 + 1 + (int)(t / nm); iz = hz0 + (int)(t % nm); return true; }
+    return false;
+}
+__host__ __device__ inline long gbox_count(int x0, int x1, int z0, int z1, int hx0, int hx1, int hz0, int hz1)
+{
+    long n = (long)(x1 - x0 + 1) * (z1 - z0 + 1);
+    if (hx0 <= hx1 && hz0 <= hz1) n -= (long)(hx1 - hx0 + 1) * (hz1 - hz0 + 1);
+    return n;
+}
+
+// The ring cells: the box columns 1-nbd..nx+nbd x rows zr..nz+nbd (zr =
+// 1-nbd with the top ring, else 1) minus the cells farther than nbd from
+// every edge of the physical region it closes (columns nbd+1..nx-nbd x rows
+// zi..nz-nbd, zi = nbd+1 with the top ring, else 1).  b: x0 x1 z0 z1 hx0
+// hx1 hz0 hz1.
+__host__ __device__ inline void gring_box(const Geo &g, int top_ring, int b[8])
+{
+    b[0] = 1 - g.nbd; b[1] = g.nx + g.nbd; b[2] = top_ring ? 1 - g.nbd : 1; b[3] = g.nz + g.nbd;
+    b[4] = g.nbd + 1; b[5] = g.nx - g.nbd; b[6] = top_ring ? g.nbd + 1 : 1; b[7] = g.nz - g.nbd;
+}
+
+// Cell t of a step launch: every cell, or with a hole (the rectangle the
+// tiled interior kernel updates) the cells around it.
+template <class T>
+__device__ __forceinline__ bool gstep_cell(const GenArgs<T> &a, long t, int &iz, int &ix)
+{
+    const Geo &g = a.g;
+    return gbox_cell(t, 1 - g.npml, g.nx + g.npml, a.zt, a.zt + g.NZ - 1, a.hx0, a.hx1, a.hz0, a.hz1, iz, ix);
 }
 
 // strip of a padded cell: 0 L, 1 R, 2 B, 3 T, -1 none
@@ -370,12 +426,12 @@ __device__ void grecord(const GenArgs<T> &a, const GenRec<T> &r, int i)
     if (NC == 2) r.seisz[(size_t)i * r.nsamp + r.istp] = sz;
 }
 
-// one thread per receiver, per ring-region cell and per strip cell:
-// threads [0, nrec) record, [nrec, nrec + nring) write the ring values of
-// the cells of rows zr..nz+nbd x columns 1-nbd..nx+nbd (the ring's bounding
-// frame; zero off the ring), the rest update the memories of strip cells
+// one thread per receiver, per ring cell and per strip cell: threads
+// [0, nrec) record, [nrec, nrec + nring) write the ring values of the cells
+// of gring_box (the other cells of the ring fields stay zero from their
+// allocation), the rest update the memories of strip cells
 template <class T>
-__global__ __launch_bounds__(256) void k_gen_prep(GenArgs<T> a, GenRec<T> r, long nrec, long nring, int zr, T *ringx,
+__global__ __launch_bounds__(256) void k_gen_prep(GenArgs<T> a, GenRec<T> r, long nrec, long nring, T *ringx,
                                                    T *ringz)
 {
 #pragma clang fp contract(off)
@@ -384,8 +440,9 @@ __global__ __launch_bounds__(256) void k_gen_prep(GenArgs<T> a, GenRec<T> r, lon
     if (t < nrec) { grecord(a, r, (int)t); return; }
     t -= nrec;
     if (t < nring) {
-        const int rows = g.nz + g.nbd - zr + 1;
-        const int ix = 1 - g.nbd + (int)(t / rows), iz = zr + (int)(t % rows);
+        int b[8], iz, ix;
+        gring_box(g, a.top_ring, b);
+        gbox_cell(t, b[0], b[1], b[2], b[3], b[4], b[5], b[6], b[7], iz, ix);
         int arr = 0;
         const long o = a.ring_valid ? gring(a, iz, ix, a.it, arr) : -1;
         const size_t c = gcell(a, iz, ix);
@@ -427,9 +484,8 @@ __global__ __launch_bounds__(256) void k_gen_step(GenArgs<T> a)
 {
 #pragma clang fp contract(off)
     const Geo &g = a.g;
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long)g.NX * g.NZ) return;
-    const int ix = 1 - g.npml + (int)(t / g.NZ), iz = a.zt + (int)(t % g.NZ);
+    int iz, ix;
+    if (!gstep_cell(a, (long)blockIdx.x * blockDim.x + threadIdx.x, iz, ix)) return;
     const size_t gi = gcell(a, iz, ix);
     const int iz_first = a.zt == 1 ? 1 : a.zt + 1;
     if (ix < 2 - g.npml || ix > g.nx + g.npml - 1 || iz < iz_first || iz > g.nz + g.npml - 1) {
@@ -527,7 +583,7 @@ __device__ void gparts_sh(const GenArgs<T> &a, const F &u, int iz, int ix, T &X,
 }
 
 template <class T>
-__global__ __launch_bounds__(256) void k_gen_prep_sh(GenArgs<T> a, GenRec<T> r, long nrec, long nring, int zr, T *ringx)
+__global__ __launch_bounds__(256) void k_gen_prep_sh(GenArgs<T> a, GenRec<T> r, long nrec, long nring, T *ringx)
 {
 #pragma clang fp contract(off)
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -535,8 +591,9 @@ __global__ __launch_bounds__(256) void k_gen_prep_sh(GenArgs<T> a, GenRec<T> r, 
     if (t < nrec) { grecord<T, 1>(a, r, (int)t); return; }
     t -= nrec;
     if (t < nring) {
-        const int rows = g.nz + g.nbd - zr + 1;
-        const int ix = 1 - g.nbd + (int)(t / rows), iz = zr + (int)(t % rows);
+        int b[8], iz, ix;
+        gring_box(g, a.top_ring, b);
+        gbox_cell(t, b[0], b[1], b[2], b[3], b[4], b[5], b[6], b[7], iz, ix);
         int arr = 0;
         const long o = a.ring_valid ? gring(a, iz, ix, a.it, arr) : -1;
         ringx[gcell(a, iz, ix)] = o >= 0 ? a.bg[arr][o] : (T)0.0f;
@@ -568,9 +625,8 @@ __global__ __launch_bounds__(256) void k_gen_step_sh(GenArgs<T> a)
 {
 #pragma clang fp contract(off)
     const Geo &g = a.g;
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long)g.NX * g.NZ) return;
-    const int ix = 1 - g.npml + (int)(t / g.NZ), iz = a.zt + (int)(t % g.NZ);
+    int iz, ix;
+    if (!gstep_cell(a, (long)blockIdx.x * blockDim.x + threadIdx.x, iz, ix)) return;
     const size_t gi = gcell(a, iz, ix);
     const int iz_first = a.zt == 1 ? 1 : a.zt + 1;
     if (ix < 2 - g.npml || ix > g.nx + g.npml - 1 || iz < iz_first || iz > g.nz + g.npml - 1) {
@@ -642,6 +698,14 @@ __global__ void k_gen_import_material(Geo g, int zt, const float *c, T *rho, T *
     rho[d] = (T)c[3 * src + 0];
     lam[d] = (T)c[3 * src + 1];
     mu[d] = (T)c[3 * src + 2];
+}
+
+// dt^2/rho for the tiled interior kernels of the FP32 general path (the
+// update factor the general kernels form per cell)
+__global__ void k_gen_bf(long n, float dt2, const float *rho, float *bf)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) bf[t] = dt2 / rho[t];
 }
 
 // reference (1:2, NZ, NX) <-> two padded arrays (level 1 -> cur, level 2 -> prev)

@@ -2,9 +2,11 @@
 formulation of it (no counterpart in the reference).
 
 The SH kernels evaluate the oracle's pointwise operators with contraction
-off and FP64 accumulation in the same order, so the FP32 run equals the
-oracle bit for bit (seismograms and final fields), with the free surface
-and with the absorbing top layer.  The GPU SH FK background is held to 1e-6
+off and FP64 accumulation in the same order, so with every cell on them
+(FDFK_GEN_TILED=0) the FP32 run equals the oracle bit for bit (seismograms
+and final fields), with the free surface and with the absorbing top layer.
+By default the deep interior runs on the tiled SH interior kernel (FP32
+with FMA contraction and hardware reciprocals), held to rel-L2 1e-5.  The GPU SH FK background is held to 1e-6
 of its peak against the oracle's.  uy takes the place of ux in the
 background arrays, the fields and the seismograms.
 """
@@ -46,9 +48,13 @@ def _solver(case, precision=lib.FDFK_FP32):
                       pml_top=case.pml_top, wave=lib.FDFK_SH)
 
 
-@pytest.mark.parametrize("pml_top,norder", [(False, 3), (True, 3), (False, 1), (False, 6)])
-def test_sh_equals_oracle_bit_for_bit(pml_top, norder):
-    print(f"case: SH, 121x60, top={pml_top}, order {2 * norder}, 300 iterations", flush=True)
+SH_CASES = [(False, 3), (True, 3), (False, 1), (False, 6)]
+
+
+@pytest.mark.parametrize("pml_top,norder", SH_CASES)
+def test_sh_equals_oracle_bit_for_bit(monkeypatch, pml_top, norder):
+    print(f"case: SH, per-cell kernels only, 121x60, top={pml_top}, order {2 * norder}, 300 iterations", flush=True)
+    monkeypatch.setenv("FDFK_GEN_TILED", "0")
     case = Case(nt=300, pml_top=pml_top, norder=norder, nrcv=13, src_is_p=False, npml=max(15, 2 * norder - 1))
     bg8, bg = _sh_background(case)
     oy, st = _oracle_sh(case, bg8)
@@ -61,6 +67,24 @@ def test_sh_equals_oracle_bit_for_bit(pml_top, norder):
     assert np.abs(oy).max() > 0 and not np.any(gz)
     assert np.array_equal(gy.view(np.uint32), oy.view(np.uint32))
     assert np.array_equal(uy.view(np.uint32), st["uy"].view(np.uint32))
+
+
+@pytest.mark.parametrize("pml_top,norder", SH_CASES + [(True, 5), (False, 8)])
+def test_sh_tiled_interior_against_oracle(pml_top, norder):
+    print(f"case: SH, tiled interior, 121x60, top={pml_top}, order {2 * norder}, 300 iterations", flush=True)
+    case = Case(nt=300, pml_top=pml_top, norder=norder, nrcv=13, src_is_p=False, npml=max(15, 2 * norder + 2))
+    case.rz = np.linspace(0.0, 50 * case.dz, 13).astype(np.float32)
+    bg8, bg = _sh_background(case)
+    oy, st = _oracle_sh(case, bg8)
+    s = _solver(case)
+    s.set_background(bg)
+    s.run(case.nt + 1)
+    gy, _ = s.seismograms()
+    uy, _ = s.get_state()
+    s.close()
+    e = max(rel_l2(gy, oy), rel_l2(uy, st["uy"]))
+    print(f"tiled SH vs oracle: rel-L2 {e:.2e}", flush=True)
+    assert np.abs(oy).max() > 0 and e < 1e-5
 
 
 @pytest.mark.parametrize("pml_top", [False, True])

@@ -2,10 +2,13 @@
 against the CPU oracle's formulation of it.
 
 The general per-cell kernels evaluate the oracle's pointwise operators with
-contraction off and FP64 accumulation in the same order, so the FP32 run
-equals the oracle bit for bit (fields, strips and seismograms).  The GPU FK
-background (top layer extended upward, no free surface) is held to 1e-6 of
-its peak against the oracle's, like the free-surface background.
+contraction off and FP64 accumulation in the same order, so with every cell
+on them (FDFK_GEN_TILED=0) the FP32 run equals the oracle bit for bit
+(fields, strips and seismograms).  By default the deep interior runs on the
+tiled interior kernel (FP32 with FMA contraction and hardware reciprocals),
+held to rel-L2 1e-5 against the oracle like the free-surface tiled path.
+The GPU FK background (top layer extended upward, no free surface) is held
+to 1e-6 of its peak against the oracle's, like the free-surface background.
 """
 import numpy as np
 import pytest
@@ -22,9 +25,14 @@ def _solver(case):
                       rx=case.rx, rz=case.rz, pml_top=True)
 
 
-@pytest.mark.parametrize("norder,nx,nz,npml,nt", [(3, 121, 60, 15, 300), (1, 90, 40, 12, 200), (5, 110, 50, 16, 220)])
-def test_top_layer_equals_oracle_bit_for_bit(norder, nx, nz, npml, nt):
-    print(f"case: top layer, order {2 * norder}, {nx}x{nz}, npml {npml}, {nt} iterations", flush=True)
+CASES = [(3, 121, 60, 15, 300), (1, 90, 40, 12, 200), (5, 110, 50, 16, 220)]
+
+
+@pytest.mark.parametrize("norder,nx,nz,npml,nt", CASES)
+def test_top_layer_equals_oracle_bit_for_bit(monkeypatch, norder, nx, nz, npml, nt):
+    print(f"case: top layer, per-cell kernels only, order {2 * norder}, {nx}x{nz}, npml {npml}, {nt} iterations",
+          flush=True)
+    monkeypatch.setenv("FDFK_GEN_TILED", "0")
     case = Case(nx=nx, nz=nz, npml=npml, norder=norder, nt=nt, nrcv=13, pml_top=True,
                 rayp_sdeg=6.0, f0=1.2)
     bg = case.oracle_background()
@@ -41,6 +49,25 @@ def test_top_layer_equals_oracle_bit_for_bit(norder, nx, nz, npml, nt):
     assert np.array_equal(gz.view(np.uint32), oz.view(np.uint32))
     assert np.array_equal(ux.view(np.uint32), st["ux"].view(np.uint32))
     assert np.array_equal(uz.view(np.uint32), st["uz"].view(np.uint32))
+
+
+@pytest.mark.parametrize("norder,nx,nz,npml,nt", CASES + [(6, 100, 45, 14, 200)])
+def test_top_layer_tiled_interior_against_oracle(norder, nx, nz, npml, nt):
+    print(f"case: top layer, tiled interior, order {2 * norder}, {nx}x{nz}, npml {npml}, {nt} iterations", flush=True)
+    case = Case(nx=nx, nz=nz, npml=npml, norder=norder, nt=nt, nrcv=13, pml_top=True, rayp_sdeg=6.0, f0=1.2)
+    case.rz = np.linspace(0.0, (nz - 1) * case.dz, 13).astype(np.float32)   # receivers at depth too
+    bg = case.oracle_background()
+    st = case.state()
+    ox, oz = oracle_run(case, st, bg, nt + 1)
+    s = _solver(case)
+    s.set_background(bg)
+    s.run(nt + 1)
+    gx, gz = s.seismograms()
+    ux, uz = s.get_state()
+    s.close()
+    e = max(rel_l2(gx, ox), rel_l2(gz, oz), rel_l2(ux, st["ux"]), rel_l2(uz, st["uz"]))
+    print(f"tiled top layer vs oracle: rel-L2 {e:.2e}", flush=True)
+    assert np.abs(ox).max() > 0 and e < 1e-5
 
 
 def test_top_layer_fk_background_matches_oracle():
