@@ -646,6 +646,13 @@ def main():
                    "speedup_solves_per_hour": r["speedup_solves_per_hour"],
                    "max_rel_l2_batched_vs_plain": r["max_rel_l2_batched_vs_plain"]}
             for name, r in (("scaled_mix_24", sv), ("serving_workload_first_12", sf))}
+        # the same solve on the general-path solver: absorbing top layer
+        # (P-SV) and SH motion (both top boundaries)
+        out["variants"] = {}
+        for v in ("top", "sh", "sh-top"):
+            vr = variant_bench(v, steps=100, warmup=5)
+            out["variants"][v] = {"value": vr["value"], "unit": vr["unit"], "ms_per_step": vr["ms_per_step"],
+                                  "grid_padded": vr["config"]["grid_padded"], "finite": vr["config"]["finite"]}
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(a.cpu_steps)
     print(json.dumps(out), flush=True)
