@@ -258,6 +258,7 @@ struct fdfk_solver {
     int gsize = 4;
     void *gux[2] = {nullptr, nullptr}, *guz[2] = {nullptr, nullptr};
     void *grho = nullptr, *glam = nullptr, *gmu = nullptr;
+    void *ghx[2] = {}, *ghz[2] = {};   // face harmonic means [lam+2mu, mu]
     void *gbu[4][2][3] = {};           // split parts (1:2, cells), reference order
     void *gbpx[4][2] = {}, *gbpz[4][2] = {};
     double *gc0 = nullptr, *gc1 = nullptr;
@@ -871,6 +872,7 @@ GenArgs<T> gen_args(fdfk_solver *s)
     a.uxp = (const T *)s->gux[1 - s->cur]; a.uzp = (const T *)s->guz[1 - s->cur];
     a.uxo = (T *)s->gux[1 - s->cur]; a.uzo = (T *)s->guz[1 - s->cur];
     a.rho = (const T *)s->grho; a.lam = (const T *)s->glam; a.mu = (const T *)s->gmu;
+    for (int m = 0; m < 2; ++m) { a.hx[m] = (const T *)s->ghx[m]; a.hz[m] = (const T *)s->ghz[m]; }
     for (int st = 0; st < 4; ++st)
         for (int c = 0; c < 2; ++c) {
             for (int q = 0; q < 3; ++q) a.bu[st][c][q] = (T *)s->gbu[st][c][q];
@@ -1239,6 +1241,15 @@ static int gen_create(const fdfk_job *job, fdfk_solver **out)
         hipLaunchKernelGGL(k_gen_import_material<T>, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0,
                            s->stream, g, s->zt, dc, (T *)s->grho, (T *)s->glam, (T *)s->gmu);
         hipFreeAsync(dc, s->stream);
+    }
+    {
+        // face harmonic means (SH uses the mu faces only)
+        const int m0 = s->sh ? 1 : 0;
+        for (int m = m0; m < 2; ++m)
+            if ((rc = s->dalloc((T **)&s->ghx[m], nf)) || (rc = s->dalloc((T **)&s->ghz[m], nf))) return bad(rc);
+        hipLaunchKernelGGL(k_gen_faces<T>, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, s->stream, g, s->zt,
+                           (const T *)s->glam, (const T *)s->gmu, m0, 1, (T *)s->ghx[0], (T *)s->ghz[0],
+                           (T *)s->ghx[1], (T *)s->ghz[1]);
     }
     if constexpr (sizeof(T) == 4) {
         // the tiled interior: columns 1..nx, rows 2*norder..nz (full-order

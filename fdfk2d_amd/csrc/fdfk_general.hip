@@ -51,6 +51,10 @@ struct GenArgs {
     // hole: the rectangle columns hx0..hx1 x rows hz0..hz1 that the tiled
     // interior kernel updates in the same step (none when hx0 > hx1)
     int hx0, hx1, hz0, hz1;
+    // harmonic means of the moduli at the cell faces, formed once at set-up:
+    // [mode] 0 lam+2mu, 1 mu; hx at cell (iz, ix) for the face (ix, ix+1),
+    // hz for the face (iz, iz+1) (the same quotients the operators form)
+    const T *hx[2], *hz[2];
 };
 
 template <class T>
@@ -256,8 +260,7 @@ __device__ T gx2(const GenArgs<T> &a, int mode, const F &u, int iz, int ix)
         const int n2 = gmin(gmin(ix - i + npml - 1, nx + npml - ix + i), norder);
         T s2 = (T)0.0f;
         for (int j = 1; j <= n2; ++j) s2 = gacc(s2, gco(a.c0, norder, j, n2), (T)(u(iz, ix + j - k) - u(iz, ix - j - k + 1)));
-        const T t = ghmean(gmat(a, mode, gcell(a, iz, ix + k)), gmat(a, mode, gcell(a, iz, ix + k - 1))) * s1 -
-                    ghmean(gmat(a, mode, gcell(a, iz, ix - k + 1)), gmat(a, mode, gcell(a, iz, ix - k))) * s2;
+        const T t = a.hx[mode][gcell(a, iz, ix + k - 1)] * s1 - a.hx[mode][gcell(a, iz, ix - k)] * s2;
         s = gacc(s, gco(a.c0, norder, k, nox), t);
     }
     const bool in_reg = iz >= 1 && iz <= a.g.nz && ix >= 1 && ix <= nx;
@@ -280,8 +283,7 @@ __device__ T gz2(const GenArgs<T> &a, int mode, const F &u, int iz, int ix)
         const int n2 = gmin(gmin(iz - i - zt, nz + npml - iz + i), norder);
         T s2 = (T)0.0f;
         for (int j = 1; j <= n2; ++j) s2 = gacc(s2, gco(a.c0, norder, j, n2), (T)(u(iz + j - k, ix) - u(iz - j - k + 1, ix)));
-        const T t = ghmean(gmat(a, mode, gcell(a, iz + k, ix)), gmat(a, mode, gcell(a, iz + k - 1, ix))) * s1 -
-                    ghmean(gmat(a, mode, gcell(a, iz - k + 1, ix)), gmat(a, mode, gcell(a, iz - k, ix))) * s2;
+        const T t = a.hz[mode][gcell(a, iz + k - 1, ix)] * s1 - a.hz[mode][gcell(a, iz - k, ix)] * s2;
         s = gacc(s, gco(a.c0, norder, k, noz), t);
     }
     const T dzs = (T)a.g.dz;
@@ -329,7 +331,6 @@ __device__ void gparts(const GenArgs<T> &a, const F &ux, const F &uz, int iz, in
         const size_t ca = gcell(a, 1, ix), cb = gcell(a, 2, ix);
         const T dz = (T)a.g.dz, dx = (T)a.g.dx;
         const T sdz2 = (T)2.0f / (dz * dz);
-        const T M1 = a.lam[ca] + (T)2.0f * a.mu[ca], M2 = a.lam[cb] + (T)2.0f * a.mu[cb];
         const int norder = a.g.norder;
         const int nox = gmin(gmin(ix + a.g.npml - 1, a.g.nx + a.g.npml - ix), norder);
         T sl = (T)0.0f, sm = (T)0.0f;
@@ -344,10 +345,10 @@ __device__ void gparts(const GenArgs<T> &a, const F &ux, const F &uz, int iz, in
         const T dxuz = (gcdx(a, uz, 1, ix) + gcdx(a, uz, 2, ix)) / dx;
         const T dxux = (gcdx(a, ux, 1, ix) + gcdx(a, ux, 2, ix)) / dx;
         p[0] = gx2(a, 0, ux, iz, ix);
-        p[1] = (ghmean(a.mu[ca], a.mu[cb]) * (ux(2, ix) - ux(1, ix))) * sdz2;
+        p[1] = (a.hz[1][ca] * (ux(2, ix) - ux(1, ix))) * sdz2;
         p[2] = (sl / dx + mbar * dxuz) / dz;
         p[3] = gx2(a, 1, uz, iz, ix);
-        p[4] = (ghmean(M1, M2) * (uz(2, ix) - uz(1, ix))) * sdz2;
+        p[4] = (a.hz[0][ca] * (uz(2, ix) - uz(1, ix))) * sdz2;
         p[5] = (sm / dx + lbar * dxux) / dz;
         return;
     }
@@ -576,7 +577,7 @@ __device__ void gparts_sh(const GenArgs<T> &a, const F &u, int iz, int ix, T &X,
     if (iz == 1 && a.zt == 1) {
         const size_t ca = gcell(a, 1, ix), cb = gcell(a, 2, ix);
         const T dz = (T)a.g.dz;
-        Z = (ghmean(a.mu[ca], a.mu[cb]) * (u(2, ix) - u(1, ix))) * ((T)2.0f / (dz * dz));
+        Z = (a.hz[1][ca] * (u(2, ix) - u(1, ix))) * ((T)2.0f / (dz * dz));
     } else {
         Z = gz2(a, 1, u, iz, ix);
     }
@@ -698,6 +699,25 @@ __global__ void k_gen_import_material(Geo g, int zt, const float *c, T *rho, T *
     rho[d] = (T)c[3 * src + 0];
     lam[d] = (T)c[3 * src + 1];
     mu[d] = (T)c[3 * src + 2];
+}
+
+// harmonic means at the x- and z-faces of every cell for the moduli of
+// the modes in [m0, m1] (0 lam+2mu, 1 mu); the last column / row has no
+// face and holds 0
+template <class T>
+__global__ void k_gen_faces(Geo g, int zt, const T *lam, const T *mu, int m0, int m1, T *hx0, T *hz0, T *hx1, T *hz1)
+{
+#pragma clang fp contract(off)
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)g.NX * g.NZ) return;
+    const int j = (int)(t / g.NZ), k = (int)(t % g.NZ);
+    const size_t c = (size_t)j * g.ld + k, cx = c + g.ld, cz = c + 1;
+    for (int m = m0; m <= m1; ++m) {
+        auto mat = [&](size_t q) { return m == 0 ? (T)(lam[q] + (T)2.0f * mu[q]) : mu[q]; };
+        T *hx = m == 0 ? hx0 : hx1, *hz = m == 0 ? hz0 : hz1;
+        hx[c] = j + 1 < g.NX ? ghmean(mat(cx), mat(c)) : (T)0.0f;
+        hz[c] = k + 1 < g.NZ ? ghmean(mat(cz), mat(c)) : (T)0.0f;
+    }
 }
 
 // dt^2/rho for the tiled interior kernels of the FP32 general path (the
