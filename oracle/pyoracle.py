@@ -256,15 +256,13 @@ def background(nx, nz, npml, izt, nbd, nt, src_is_p, x0, z0, dx, dz, dt, ds, f0,
 def state64(st):
     """A copy of an oracle state with every float32 array in float64, for
     the FP64 build (the inputs are the FP32 values, widened exactly)."""
-    out = {}
-    for k, v in st.items():
-        if isinstance(v, np.ndarray) and v.dtype == np.float32:
-            out[k] = v.astype(np.float64)
-        elif isinstance(v, list) and v and isinstance(v[0], np.ndarray):
-            out[k] = [x.astype(np.float64) if x.dtype == np.float32 else x.copy() for x in v]
-        else:
-            out[k] = v
-    return out
+    def wide(v):
+        if isinstance(v, np.ndarray):
+            return v.astype(np.float64) if v.dtype == np.float32 else v.copy()
+        if isinstance(v, list):   # arrays, or lists of them (the SH split parts)
+            return [wide(x) for x in v]
+        return v
+    return {k: wide(v) for k, v in st.items()}
 
 
 def bg_shapes(nx, nz, nbd, nt):
@@ -320,36 +318,40 @@ def sh_state(nx, nz, npml, norder, dx, dz, dt, nt, c, pml_top=False, npower=2, R
 
 
 def updatedispl_sh(st, it, bg=None):
-    """One SH step of the oracle on a sh_state (FP32); bg: the eight ring
-    arrays (uyl_reg, uyr_reg, uyb_reg, uyt_reg, uyl_pml, uyr_pml, uyb_pml,
-    uyt_pml; the top two may be None) or None."""
+    """One SH step of the oracle on a sh_state (a state64 of it runs the FP64
+    build); bg: the eight ring arrays (uyl_reg, uyr_reg, uyb_reg, uyt_reg,
+    uyl_pml, uyr_pml, uyb_pml, uyt_pml; the top two may be None) or None."""
     a = st
+    L, _, fv = _real(a["uy"].dtype == np.float64)
     bgp = (C.c_void_p * 8)(*[None if x is None else x.ctypes.data for x in bg]) if bg is not None else None
     bpp = (C.c_void_p * 6)(*[x.ctypes.data for x in a["bp"]])
     bup = ((C.c_void_p * 2) * 4)(*[(C.c_void_p * 2)(*[x.ctypes.data for x in pair]) for pair in a["bu"]])
-    lib().orc_updatedispl_sh(
+    L.orc_updatedispl_sh(
         C.c_int(a["norder"]), C.c_int(a["nx"]), C.c_int(a["nz"]), C.c_int(a["npml"]), C.c_int(a["pml_top"]),
-        C.c_int(a["npower"]), C.c_int(it), C.c_int(a["nt"]), C.c_float(a["dx"]), C.c_float(a["dz"]),
-        C.c_double(a["dt"]), C.c_float(a["vpml"]), C.c_double(a["R0"]), bgp, _p(a["c"]), _p(a["coef0"]),
+        C.c_int(a["npower"]), C.c_int(it), C.c_int(a["nt"]), fv(a["dx"]), fv(a["dz"]),
+        C.c_double(a["dt"]), fv(a["vpml"]), C.c_double(a["R0"]), bgp, _p(a["c"]), _p(a["coef0"]),
         _p(a["coef1"]), _p(a["uy"]), bpp, bup)
 
 
-def background_sh(nx, nz, npml, izt, nbd, nt, x0, z0, dx, dz, dt, ds, f0, rayp, bazs, fk_left, fk_right):
-    """SH ring background: eight arrays (the top two None without a top layer)."""
+def background_sh(nx, nz, npml, izt, nbd, nt, x0, z0, dx, dz, dt, ds, f0, rayp, bazs, fk_left, fk_right,
+                  fp64=False):
+    """SH ring background: eight arrays (the top two None without a top
+    layer), float64 from the FP64 build when fp64."""
+    L, _, fv = _real(fp64)
+    dt_ = np.float64 if fp64 else np.float32
     shapes = bg_shapes(nx, nz, nbd, nt)
     T = nt + 1
-    arrs = [np.zeros(shapes[0], np.float32), np.zeros(shapes[2], np.float32), np.zeros(shapes[4], np.float32),
-            np.zeros(nbd * (nx - 2 * nbd) * T, np.float32) if izt != 1 else None,
-            np.zeros(shapes[6], np.float32), np.zeros(shapes[8], np.float32), np.zeros(shapes[10], np.float32),
-            np.zeros(nbd * (nx + 2 * nbd) * T, np.float32) if izt != 1 else None]
-    fl = np.ascontiguousarray(fk_left, np.float32)
-    fr = np.ascontiguousarray(fk_right, np.float32)
+    arrs = [np.zeros(shapes[0], dt_), np.zeros(shapes[2], dt_), np.zeros(shapes[4], dt_),
+            np.zeros(nbd * (nx - 2 * nbd) * T, dt_) if izt != 1 else None,
+            np.zeros(shapes[6], dt_), np.zeros(shapes[8], dt_), np.zeros(shapes[10], dt_),
+            np.zeros(nbd * (nx + 2 * nbd) * T, dt_) if izt != 1 else None]
+    fl = np.ascontiguousarray(fk_left, dt_)
+    fr = np.ascontiguousarray(fk_right, dt_)
     bz = np.ascontiguousarray(bazs, np.float64)
     outp = (C.c_void_p * 8)(*[None if x is None else x.ctypes.data for x in arrs])
-    lib().orc_background_sh(C.c_int(nx), C.c_int(nz), C.c_int(npml), C.c_int(izt), C.c_int(nbd), C.c_int(nt),
-                            C.c_float(x0), C.c_float(z0), C.c_float(dx), C.c_float(dz), C.c_double(dt),
-                            C.c_float(ds), C.c_float(f0), C.c_double(rayp), _p(bz), C.c_int(fl.size // 4), _p(fl),
-                            C.c_int(fr.size // 4), _p(fr), outp)
+    L.orc_background_sh(C.c_int(nx), C.c_int(nz), C.c_int(npml), C.c_int(izt), C.c_int(nbd), C.c_int(nt),
+                        fv(x0), fv(z0), fv(dx), fv(dz), C.c_double(dt), fv(ds), fv(f0), C.c_double(rayp), _p(bz),
+                        C.c_int(fl.size // 4), _p(fl), C.c_int(fr.size // 4), _p(fr), outp)
     return arrs
 
 

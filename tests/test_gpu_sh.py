@@ -20,10 +20,10 @@ from tests._cases import Case, fk_model, rel_l2
 pytestmark = pytest.mark.gpu
 
 
-def _sh_background(case):
+def _sh_background(case, fp64=False):
     izt = case.izt
     b = po.background_sh(case.nx, case.nz, case.npml, izt, case.nbd, case.nt, 0.0, 0.0, case.dx, case.dz,
-                         case.dt, case.ds, case.f0, case.rayp, case.bazs, case.fk, case.fk)
+                         case.dt, case.ds, case.f0, case.rayp, case.bazs, case.fk, case.fk, fp64=fp64)
     z = lambda a: np.zeros_like(a)
     twelve = [b[0], z(b[0]), b[1], z(b[1]), b[2], z(b[2]), b[4], z(b[4]), b[5], z(b[5]), b[6], z(b[6])]
     if izt == 1:
@@ -31,15 +31,17 @@ def _sh_background(case):
     return b, (twelve, [b[3], z(b[3]), b[7], z(b[7])])
 
 
-def _oracle_sh(case, bg8):
+def _oracle_sh(case, bg8, fp64=False):
     st = po.sh_state(case.nx, case.nz, case.npml, case.norder, case.dx, case.dz, case.dt, case.nt, case.c,
                      pml_top=case.pml_top, vpml=refstate.vp_max(case.c))
+    if fp64:
+        st = po.state64(st)
     tr = []
     for it in range(case.nt + 1):
         tr.append([po.loadseis(case.nx, case.nz, case.npml, case.izt, 0.0, 0.0, case.dx, case.dz, float(a), float(b),
                                st["uy"], st["uy"])[0] for a, b in zip(case.rx, case.rz)])
         po.updatedispl_sh(st, it, bg8)
-    return np.asarray(tr, np.float32).T, st
+    return np.asarray(tr, st["uy"].dtype).T, st
 
 
 def _solver(case, precision=lib.FDFK_FP32):
@@ -67,6 +69,22 @@ def test_sh_equals_oracle_bit_for_bit(monkeypatch, pml_top, norder):
     assert np.abs(oy).max() > 0 and not np.any(gz)
     assert np.array_equal(gy.view(np.uint32), oy.view(np.uint32))
     assert np.array_equal(uy.view(np.uint32), st["uy"].view(np.uint32))
+
+
+@pytest.mark.parametrize("pml_top", [False, True])
+def test_sh_fp64_equals_fp64_oracle_bit_for_bit(pml_top):
+    print(f"case: SH, FDFK_FP64, 121x60, top={pml_top}, order 6, 300 iterations", flush=True)
+    case = Case(nt=300, pml_top=pml_top, nrcv=13, src_is_p=False)
+    bg8, bg = _sh_background(case, fp64=True)
+    oy, st = _oracle_sh(case, bg8, fp64=True)
+    s = _solver(case, precision=lib.FDFK_FP64)
+    s.set_background(bg)
+    s.run(case.nt + 1)
+    gy, _ = s.seismograms()
+    uy, _ = s.get_state()
+    s.close()
+    assert gy.dtype == np.float64 and np.abs(oy).max() > 0
+    assert np.array_equal(gy, oy) and np.array_equal(uy, st["uy"])
 
 
 @pytest.mark.parametrize("pml_top,norder", SH_CASES + [(True, 5), (False, 8)])

@@ -7,9 +7,10 @@ with the FP32 build to FP32 rounding (a dropped term or a wrong sign in
 either would be of order one), yet differ from it (it really computes in
 double: the FP32 build's errors are above FP64 rounding by many orders),
 and a plane wave's background in double precision matches the FP32 one
-step by step.
+step by step.  The same for the SH formulation (both top boundaries).
 """
 import numpy as np
+import pytest
 
 from oracle import pyoracle as po
 from tests._cases import Case, oracle_run, rel_l2
@@ -44,3 +45,24 @@ def test_fp64_run_agrees_with_fp32_to_fp32_rounding():
         print("FP32 oracle vs FP64 oracle, P %s: rel-L2 x %.2e z %.2e" % (src_is_p, ex, ez))
         assert ex < 1e-5 and ez < 1e-5
         assert ex > 1e-9 and ez > 1e-9
+
+
+@pytest.mark.parametrize("top", [False, True])
+def test_fp64_sh_run_agrees_with_fp32_to_fp32_rounding(top):
+    from fdfk2d_amd import refstate
+    c = Case(nx=81, nz=40, npml=12, nt=300, src_is_p=False, pml_top=top)
+    out = {}
+    for fp64 in (False, True):
+        b = po.background_sh(c.nx, c.nz, c.npml, c.izt, c.nbd, c.nt, 0.0, 0.0, c.dx, c.dz, c.dt, c.ds, c.f0,
+                             c.rayp, c.bazs, c.fk, c.fk, fp64=fp64)
+        st = po.sh_state(c.nx, c.nz, c.npml, c.norder, c.dx, c.dz, c.dt, c.nt, c.c, pml_top=top,
+                         vpml=refstate.vp_max(c.c))
+        if fp64:
+            st = po.state64(st)
+            assert all(x.dtype == np.float64 for pair in st["bu"] for x in pair)
+        for it in range(c.nt + 1):
+            po.updatedispl_sh(st, it, b)
+        out[fp64] = st["uy"]
+    assert out[True].dtype == np.float64 and np.abs(out[True]).max() > 1e-6
+    e = rel_l2(out[False], out[True])
+    assert 1e-9 < e < 1e-5, e
