@@ -261,3 +261,23 @@ def test_server_budget_holds_with_solo_requests():
     for i in res_p:
         assert np.abs(res_p[i][1]).max() > 0
         assert np.array_equal(res[i][0], res_p[i][0]) and np.array_equal(res[i][1], res_p[i][1])
+
+
+def test_server_serving_workload_requests_at_their_own_sizes():
+    """The first four requests of the serving workload itself (seed 5, 4
+    arrivals/s: 2338x772 to 6467x1696 cells, 3769 to 17352 steps, dx 50..200 m,
+    253 G cell-updates), served with the default split (grids of >= 4 M
+    padded cells on their own streams, the rest batched, background in
+    blocks) and one at a time: every seismogram identical."""
+    reqs = serve.request_mix(4, seed=5, rate=4.0)
+    print("case: the serving workload's first 4 requests at their own sizes, server vs plain", flush=True)
+    srv = serve.Server(hbm_budget_gb=200.0, chunk=100)
+    lat, total, res = srv.run(reqs, keep_results=True)
+    srv.close()
+    print("server: %.2f s" % total, flush=True)
+    lat_p, total_p, res_p = serve.serve_plain(reqs, keep_results=True)
+    print("plain: %.2f s" % total_p, flush=True)
+    assert sorted(res) == sorted(res_p) == list(range(4))
+    for i in res:
+        assert np.abs(res_p[i][1]).max() > 0
+        assert np.array_equal(res[i][0], res_p[i][0]) and np.array_equal(res[i][1], res_p[i][1])
