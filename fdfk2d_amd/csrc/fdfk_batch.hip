@@ -57,7 +57,11 @@ __device__ __forceinline__ int find_job(const int *pre, int n, int bid)
 // device (ctr[0]): the kernels of a step read it and pick the table of the
 // step's parity themselves, and kb_tick advances it after the step, so the
 // launches of every step are identical and one step can be built as a
-// graph once per active set and replayed.
+// graph once per active set and replayed (FDFK_GRAPH=1).  Eager launches
+// are the default: on this runtime a replayed three-node graph was not
+// faster than the three launches (16 jobs of one active set: 0.220 s
+// against 0.212-0.215 s), and the rebuilds at every join and finish cost
+// a mix of small requests 12%.
 __global__ void kb_tick(int *ctr) { ctr[0] += 1; }
 __global__ void kb_set(int *ctr, int v) { ctr[0] = v; }
 
@@ -141,7 +145,7 @@ struct fdfk_batch {
     // device step counter of the current tables, and the step of the
     // current active set as a graph (rebuilt with the tables)
     int *dctr = nullptr;
-    bool use_graph = true;
+    bool use_graph = false;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
     // Graphs of earlier active sets whose replays may still be queued: a

@@ -60,9 +60,9 @@ def test_batch_equals_alone_with_staggered_joins(nstp, block):
 
 @pytest.mark.parametrize("block", [0, 32])
 def test_graph_replayed_steps_equal_eager_launches(block, monkeypatch):
-    """A batched step is captured as a graph once per active set and
-    replayed (FDFK_GRAPH=1, the default); FDFK_GRAPH=0 launches its kernels
-    one by one.  Both give the same seismograms and final wavefields bit for
+    """A batched step is built as a graph once per active set and replayed
+    (FDFK_GRAPH=1); FDFK_GRAPH=0 (the default) launches its kernels one by
+    one.  Both give the same seismograms and final wavefields bit for
     bit, with table rebuilds at staggered joins and finishes and (block 32)
     background refills launched eagerly between the replays."""
     print("case: 3 jobs batched, graph replay vs eager launches, block %d" % block, flush=True)
