@@ -1265,6 +1265,10 @@ static int gen_create(const fdfk_job *job, fdfk_solver **out)
             hipLaunchKernelGGL(k_gen_bf, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, s->stream, (long)nf,
                                (float)(g.dt * g.dt), (const float *)s->grho, s->gbf);
             with_order(g.norder, [&](auto no) { s->glx = gen_plan_lx<decltype(no)::value>(v, s->sh); });
+            if (const char *el = std::getenv("FDFK_LX")) {
+                const int lx = std::atoi(el);
+                if (lx >= 4) s->glx = lx;
+            }
         }
     }
     // strips L, R, B (and T with a top layer): split parts with two
