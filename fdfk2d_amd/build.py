@@ -8,9 +8,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libfdfk.so")
-SOURCES = ["csrc/fdfk_api.hip"]
-DEPS = ["csrc/fdfk_api.hip", "csrc/fdfk_kernels.hip", "csrc/fdfk_internal.h", "csrc/fdfk_frame.hip", "csrc/fdfk_batch.hip",
-        "csrc/fdfk_fk.hip", "csrc/fdfk_smooth.hip", "csrc/fdfk_interior2.hip", "../include/fdfk.h"]
+SOURCES = ["csrc/fdfk_api.hip"]   # includes the other csrc/*.hip files
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
