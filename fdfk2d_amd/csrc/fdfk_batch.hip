@@ -53,11 +53,11 @@ __device__ __forceinline__ int find_job(const int *pre, int n, int bid)
     return lo;
 }
 
-// The step count since the active set's tables were built lives on the
-// device (ctr[0]): the kernels of a step read it and pick the table of the
-// step's parity themselves, and kb_tick advances it after the step, so the
-// launches of every step are identical and one step can be built as a
-// graph once per active set and replayed (FDFK_GRAPH=1).  Eager launches
+// The step count since the active set's tables were built: launched
+// eagerly the kernels take it as an argument (hstep); replayed as a graph
+// (FDFK_GRAPH=1) they read it from the device (ctr[0]), which kb_tick
+// advances after each step, so that the launches of every step are
+// identical and one step can be built as a graph once per active set.  Eager launches
 // are the default: on this runtime a replayed three-node graph was not
 // faster than the three launches (16 jobs of one active set: 0.220 s
 // against 0.212-0.215 s), and the rebuilds at every join and finish cost
@@ -69,10 +69,11 @@ __global__ void kb_set(int *ctr, int v) { ctr[0] = v; }
 // background, the PML memories and (on recording iterations) the receivers,
 // in that block order; 256-lane blocks, kRecPerBlock receivers per block
 template <int NO>
-__global__ __launch_bounds__(256) void kb_prep(const BatchJob<NO> *J0, const BatchPrefix *P, const int *ctr)
+__global__ __launch_bounds__(256) void kb_prep(const BatchJob<NO> *J0, const BatchPrefix *P, const int *ctr,
+                                                int hstep)
 {
     const int n = P->n;
-    const int dstep = ctr[0];
+    const int dstep = ctr ? ctr[0] : hstep;
     const BatchJob<NO> *J = J0 + (dstep & 1) * n;
     int b = blockIdx.x;
     const int nr = P->pre[KB_RING][n], np = P->pre[KB_PSI][n];
@@ -101,11 +102,11 @@ __global__ __launch_bounds__(256) void kb_prep(const BatchJob<NO> *J0, const Bat
 template <int NO>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kIntWaves<NO>))) void kb_step(const BatchJob<NO> *J0,
                                                                                        const BatchPrefix *P,
-                                                                                       const int *ctr)
+                                                                                       const int *ctr, int hstep)
 {
     extern __shared__ float sm[];
     const int n = P->n;
-    const BatchJob<NO> *J = J0 + (ctr[0] & 1) * n;
+    const BatchJob<NO> *J = J0 + ((ctr ? ctr[0] : hstep) & 1) * n;
     const int ni = P->pre[KB_INT][n];
     if ((int)blockIdx.x < ni) {
         const int t = xcd_tile(blockIdx.x, ni);
@@ -126,7 +127,9 @@ struct fdfk_batch {
     hipStream_t stream = nullptr;
     std::vector<fdfk_solver *> members;
     // pacing: the host runs at most R steps ahead of the GPU, so that a
-    // solver added to the batch joins within a few steps
+    // solver added to the batch joins within a few steps (an event after
+    // every fourth step instead was 2 us per step faster on a one-member
+    // batch and 3% slower on a mix of small requests)
     static constexpr int R = 4;
     hipEvent_t ev[R] = {};
     int slot = 0;
@@ -213,7 +216,7 @@ int batch_lx(const std::vector<fdfk_solver *> &act)
     }
     int best = 64;
     long best_cost = -1;
-    for (int lx = 8; lx <= 256; lx += 8) {
+    for (int lx = 4; lx <= 256; lx += (lx < 32 ? 1 : 8)) {
         long nb = 0;
         for (auto *s : act) nb += int_blocks<NO, NT>(s->g, lx);
         const long cost = ((nb + slots - 1) / slots) * (lx + WARM);
@@ -339,25 +342,26 @@ int batch_step(fdfk_batch *B)
         if (int rc = bg_refill(s, B->stream)) return rc;
     auto *dP = (const BatchPrefix *)B->dtab[B->tslot];
     auto *dJ = (const BatchJob<NO> *)((char *)B->dtab[B->tslot] + kJobOff);
-    // the launches of one step: prep, step, counter tick
+    // the launches of one step: prep and step, with the step count as an
+    // argument (no device counter)
+    const int *nctr = nullptr;
+    const int hs = B->dstep;
     auto launch = [&](hipStream_t st) {
-        if (B->nprep) hipLaunchKernelGGL(kb_prep<NO>, dim3(B->nprep), dim3(256), 0, st, dJ, dP, (const int *)B->dctr);
-        if (B->nstepb)
-            hipLaunchKernelGGL(kb_step<NO>, dim3(B->nstepb), dim3(256), B->tsh, st, dJ, dP, (const int *)B->dctr);
-        hipLaunchKernelGGL(kb_tick, dim3(1), dim3(1), 0, st, B->dctr);
+        if (B->nprep) hipLaunchKernelGGL(kb_prep<NO>, dim3(B->nprep), dim3(256), 0, st, dJ, dP, nctr, hs);
+        if (B->nstepb) hipLaunchKernelGGL(kb_step<NO>, dim3(B->nstepb), dim3(256), B->tsh, st, dJ, dP, nctr, hs);
     };
     if (B->prof) {
         hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
         hipEventCreate(&e0); hipEventCreate(&e1); hipEventCreate(&e2); hipEventCreate(&e3);
         hipEventRecord(e0, B->stream);
-        if (B->nprep)
-            hipLaunchKernelGGL(kb_prep<NO>, dim3(B->nprep), dim3(256), 0, B->stream, dJ, dP, (const int *)B->dctr);
+        if (B->nprep) hipLaunchKernelGGL(kb_prep<NO>, dim3(B->nprep), dim3(256), 0, B->stream, dJ, dP, nctr, hs);
         hipEventRecord(e1, B->stream);
         if (B->nstepb)
-            hipLaunchKernelGGL(kb_step<NO>, dim3(B->nstepb), dim3(256), B->tsh, B->stream, dJ, dP,
-                               (const int *)B->dctr);
+            hipLaunchKernelGGL(kb_step<NO>, dim3(B->nstepb), dim3(256), B->tsh, B->stream, dJ, dP, nctr, hs);
         hipEventRecord(e2, B->stream);
-        hipLaunchKernelGGL(kb_tick, dim3(1), dim3(1), 0, B->stream, B->dctr);
+        // with graph replay the device counter must stay in step for the
+        // replays after profiling is switched off
+        if (B->use_graph) hipLaunchKernelGGL(kb_tick, dim3(1), dim3(1), 0, B->stream, B->dctr);
         hipEventRecord(e3, B->stream);
         B->ev_int.emplace_back(e1, e2);
         B->ev_step.emplace_back(e0, e3);
@@ -370,7 +374,8 @@ int batch_step(fdfk_batch *B)
             // stream-ordered allocations other threads make meanwhile for
             // the solvers they prepare.
             const int *ctr = B->dctr;
-            void *args[3] = {(void *)&dJ, (void *)&dP, (void *)&ctr};
+            int zero = 0;
+            void *args[4] = {(void *)&dJ, (void *)&dP, (void *)&ctr, (void *)&zero};
             int *tick = B->dctr;
             void *targs[1] = {(void *)&tick};
             hipGraph_t gr = nullptr;

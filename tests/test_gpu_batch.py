@@ -93,14 +93,16 @@ def test_graph_replayed_steps_equal_eager_launches(block, monkeypatch):
             assert np.array_equal(a, b_)
 
 
-def test_graph_rebuilds_while_replays_are_queued():
+def test_graph_rebuilds_while_replays_are_queued(monkeypatch):
     """Twelve members of different lengths joining every 7 steps and leaving
-    as they finish: the active set changes about 20 times while the host
-    keeps up to four graph replays queued ahead of the GPU, so the graph of
-    an active set is replaced while replays of it are still pending (those
-    graphs are destroyed only after their last replay completed).  Every
-    member equals its run alone, bit for bit."""
+    as they finish, steps replayed as graphs (FDFK_GRAPH=1): the active set
+    changes about 20 times while the host keeps up to four replays queued
+    ahead of the GPU, so the graph of an active set is replaced while
+    replays of it are still pending (those graphs are destroyed only after
+    their last replay completed).  Every member equals its run alone, bit
+    for bit."""
     print("case: 12 members, joins every 7 steps, graph replay", flush=True)
+    monkeypatch.setenv("FDFK_GRAPH", "1")
     reqs = []
     for i in range(12):
         reqs.append(dict(id=i, arrival=0.0, nx=61 + 10 * i, nz=40 + 3 * i, dx=200.0, nsteps=40 + 17 * i,
