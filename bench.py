@@ -70,7 +70,7 @@ def gpu_bench(steps, warmup, workload, bg_block=0, nt=0, autotune=False, precisi
                              cfg["c"], rx=cfg["rx"], rz=cfg["rz"])
     s = lib.Solver(cfg["nx"], cfg["nz"], cfg["npml"], cfg["norder"], cfg["dx"], cfg["dz"], cfg["dt"],
                    nt, cfg["c"], rx=cfg["rx"], rz=cfg["rz"],
-                   precision=lib.FDFK_FP16 if precision == "fp16" else lib.FDFK_FP32)
+                   precision={"fp16": lib.FDFK_FP16, "bf16": lib.FDFK_BF16}.get(precision, lib.FDFK_FP32))
     bg_mode = "fk-gpu" if not bg_block else f"fk-gpu, synthesised in blocks of {bg_block} steps"
     try:
         t_bg = time.perf_counter()
@@ -96,7 +96,7 @@ def gpu_bench(steps, warmup, workload, bg_block=0, nt=0, autotune=False, precisi
     st = s.kernel_stats()
     NX, NZ = cfg["NX"], cfg["NZ"]
     alg_bytes, alg_cells = step_alg_bytes(cfg["nx"], cfg["nz"], cfg["npml"], cfg["norder"],
-                                          mat_bytes=8 if precision == "fp16" else 12)
+                                          mat_bytes=8 if precision in ("fp16", "bf16") else 12)
     sx, sz = s.seismograms()
     finite = bool(np.isfinite(sx).all() and np.isfinite(sz).all())
     s.close()
@@ -436,7 +436,8 @@ def c4_bench(nsteps=8000, rf_every=20, block=512, log=lambda *a: None):
     cfg = configs.c2(nt=nsteps)
     NX, NZ = cfg["NX"], cfg["NZ"]
     runs = {}
-    for name, prec in (("fp64", lib.FDFK_FP64), ("fp32", lib.FDFK_FP32), ("fp16_materials", lib.FDFK_FP16)):
+    for name, prec in (("fp64", lib.FDFK_FP64), ("fp32", lib.FDFK_FP32), ("fp16_materials", lib.FDFK_FP16),
+                       ("bf16_materials", lib.FDFK_BF16)):
         log(f"c4: {name}, {nsteps + 1} iterations")
         s = lib.Solver(cfg["nx"], cfg["nz"], cfg["npml"], cfg["norder"], cfg["dx"], cfg["dz"], cfg["dt"], nsteps,
                        cfg["c"], rx=cfg["rx"], rz=cfg["rz"], precision=prec)
@@ -503,8 +504,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=201)
     ap.add_argument("--bg-block", type=int, default=0, metavar="B",
                     help="synthesise the background in blocks of B steps (default: store all)")
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp16"],
-                    help="fp16: lam and mu stored as FP16 with power-of-two scales (arithmetic stays FP32)")
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp16", "bf16"],
+                    help="fp16 / bf16: lam and mu stored as per-row profiles plus FP16 / BF16 residuals with "
+                         "power-of-two scales (arithmetic stays FP32)")
     ap.add_argument("--autotune", action="store_true",
                     help="time interior tile widths first (fdfk_autotune) and run with the fastest")
     ap.add_argument("--nt", type=int, default=0,
@@ -581,7 +583,7 @@ def main():
         k_name, k_bytes, k_cells = "k_step", r["alg_bytes"], r["alg_cells"]
     else:
         k_cells = cfg["nx"] * (cfg["nz"] - 2 * cfg["norder"] + 1)
-        k_name, k_bytes = "k_interior2", (32 if a.precision == "fp16" else 36) * k_cells
+        k_name, k_bytes = "k_interior2", (32 if a.precision in ("fp16", "bf16") else 36) * k_cells
     achieved = k_bytes / (k_ms * 1e-3) / 1e9
     full_steps = 20000
     out = {
@@ -606,8 +608,8 @@ def main():
             "device_gib": round(r["device_bytes"] / 2**30, 3),
             "background_s": r["background_s"],
             "tile_autotune": r["tuned"],
-            "material_storage": ("fp16 lam, mu (power-of-two scales); fp32 dt^2/rho" if a.precision == "fp16"
-                                 else "fp32"),
+            "material_storage": ("%s lam, mu residuals on per-row fp32 profiles (power-of-two scales); fp32 dt^2/rho"
+                                 % a.precision if a.precision in ("fp16", "bf16") else "fp32"),
             "solves_per_hour_at_20000_steps_loop_only": round(3600.0 / (ms * 1e-3 * full_steps), 3),
             "finite": r["finite"],
         },
@@ -617,10 +619,11 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
+            # the BF16 mode reads the same bytes as the FP16 one
             "traffic": (None if a.workload != "c2" or not fused else
-                        MEASURED_TRAFFIC_BYTES_FP16 if a.precision == "fp16" else MEASURED_TRAFFIC_BYTES),
+                        MEASURED_TRAFFIC_BYTES_FP16 if a.precision in ("fp16", "bf16") else MEASURED_TRAFFIC_BYTES),
             "traffic_source": "rocprofv3 FETCH_SIZE+WRITE_SIZE per launch, copy-calibrated "
-                              + ("(profiles/k_step_fp16_8001x2000_hbm_traffic.md)" if a.precision == "fp16"
+                              + ("(profiles/k_step_fp16_8001x2000_hbm_traffic.md)" if a.precision in ("fp16", "bf16")
                                  else "(profiles/k_step_8001x2000_hbm_traffic.md)"),
             "kernel": k_name,
             "kernel_avg_ms": round(k_ms, 5),

@@ -230,10 +230,11 @@ struct fdfk_solver {
     bool fused = true;   // interior and frame in one launch (k_step)
     bool frame_full = true;   // full-order frame tiles specialised (FDFK_FRAME_FULL=0: general path only)
     int lx_model = 0;    // interior tile width of the occupancy model (lx may be tuned)
-    // 16-bit material storage (FDFK_FP16): lam, mu of the interior tiles as
-    // per-row FP32 profiles plus FP16 residuals, value = profile + half *
-    // scale (power-of-two scales)
+    // 16-bit material storage (FDFK_FP16, FDFK_BF16): lam, mu of the
+    // interior tiles as per-row FP32 profiles plus 16-bit residuals, value =
+    // profile + residual * scale (power-of-two scales)
     bool mat16 = false;
+    bool matbf = false;                // the residuals in BF16 (FDFK_BF16) instead of FP16
     unsigned short *lam16 = nullptr, *mu16 = nullptr;
     float *lamp = nullptr, *mup = nullptr;   // per-row profiles the residuals are taken against
     float sl = 1.0f, sm = 1.0f;
@@ -511,7 +512,7 @@ IntPlan plan_lx(const Geo &g, bool fused, bool m16 = false)
     int dev = 0, ncu = 0, bpc = 0;
     hipError_t oe;
     if constexpr (NT == 256)
-        oe = fused ? (m16 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_step<NO, true>, NT, step_lds_bytes<NO>(g))
+        oe = fused ? (m16 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_step<NO, 1>, NT, step_lds_bytes<NO>(g))
                           : hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_step<NO>, NT, step_lds_bytes<NO>(g)))
                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_interior2<NO, NT>, NT, 0);
     else
@@ -643,8 +644,11 @@ void launch_step_t(fdfk_solver *s, const StepArgs &a)
     IntArgs<NO> ia = interior_args<NO, 256>(s);
     ETiles t = frame_tiles(s->g);
     const int nint = ia.nbx * ia.nbz + ia.nsb;
-    if (s->mat16)
-        hipLaunchKernelGGL((k_step<NO, true>), dim3((unsigned)(nint + t.off[t.n])), dim3(256),
+    if (s->matbf)
+        hipLaunchKernelGGL((k_step<NO, 2>), dim3((unsigned)(nint + t.off[t.n])), dim3(256),
+                           step_lds_bytes<NO>(s->g), s->stream, ia, a, t, nint);
+    else if (s->mat16)
+        hipLaunchKernelGGL((k_step<NO, 1>), dim3((unsigned)(nint + t.off[t.n])), dim3(256),
                            step_lds_bytes<NO>(s->g), s->stream, ia, a, t, nint);
     else
         hipLaunchKernelGGL(k_step<NO>, dim3((unsigned)(nint + t.off[t.n])), dim3(256), step_lds_bytes<NO>(s->g),
@@ -1057,10 +1061,11 @@ int validate_geometry(int nx, int nz, int npml, int norder, float dx, float dz, 
 int validate(const fdfk_job *j)
 {
     if (!j) return fail(FDFK_EINVAL, "null job");
-    if (j->is_pml_top && j->precision == FDFK_FP16)
+    const bool m16 = j->precision == FDFK_FP16 || j->precision == FDFK_BF16;
+    if (j->is_pml_top && m16)
         return fail(FDFK_ENOSYS, "an absorbing top layer (is_pml_top) runs in FP32 or FP64 storage");
     if (j->wave != FDFK_PSV && j->wave != FDFK_SH) return fail(FDFK_EINVAL, "wave must be FDFK_PSV or FDFK_SH");
-    if (j->wave == FDFK_SH && j->precision == FDFK_FP16)
+    if (j->wave == FDFK_SH && m16)
         return fail(FDFK_ENOSYS, "SH solvers run in FP32 or FP64 storage");
     if (int rc = validate_geometry(j->nx, j->nz, j->npml, j->norder, j->dx, j->dz, j->dt, j->nt)) return rc;
     if (j->nstp < 1) return fail(FDFK_EINVAL, "nstp >= 1 required");
@@ -1069,8 +1074,8 @@ int validate(const fdfk_job *j)
         if (!std::isfinite(j->rx[i]) || !std::isfinite(j->rz[i]))
             return fail(FDFK_EINVAL, "receiver coordinates must be finite");
     if (!j->c) return fail(FDFK_EINVAL, "material array c is required");
-    if (j->precision != FDFK_FP32 && j->precision != FDFK_FP16 && j->precision != FDFK_FP64)
-        return fail(FDFK_ENOSYS, "storage precision: FP32, FP16 materials or FP64 (BF16 is not built)");
+    if (j->precision != FDFK_FP32 && !m16 && j->precision != FDFK_FP64)
+        return fail(FDFK_EINVAL, "storage precision: FDFK_FP32, FDFK_FP16, FDFK_BF16 or FDFK_FP64");
     return 0;
 }
 
@@ -1325,13 +1330,14 @@ int fdfk_create(const fdfk_job *job, fdfk_solver **out)
     s->nrcv = job->nrcv;
     s->x0 = job->x0; s->z0 = job->z0;
     s->R0 = job->R0; s->npower = job->npower;
-    s->mat16 = job->precision == FDFK_FP16;
+    s->mat16 = job->precision == FDFK_FP16 || job->precision == FDFK_BF16;
+    s->matbf = job->precision == FDFK_BF16;
     if (const char *e = std::getenv("FDFK_PML_RATIO")) s->pml_ratio = (float)std::atof(e);
     if (const char *e = std::getenv("FDFK_FRAME_FULL")) s->frame_full = std::atoi(e) != 0;
     plan_interior(s);
     if (s->mat16 && !s->fused) {
         delete s;
-        return fail(FDFK_ENOSYS, "FP16 materials run in the fused step only (FDFK_FUSED=0 is FP32)");
+        return fail(FDFK_ENOSYS, "16-bit materials run in the fused step only (FDFK_FUSED=0 is FP32)");
     }
     const long ncell = (long)g.NX * g.NZ;
     // vpml = max Vp (src/FDFK2D.f90:416)
@@ -1432,7 +1438,7 @@ int fdfk_create(const fdfk_job *job, fdfk_solver **out)
             h2d(s->stream, s->mup, pm.data(), g.NZ * 4) != hipSuccess)
             return bad(fail(FDFK_EDEVICE, "material profile upload"));
         hipLaunchKernelGGL(k_round_material, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, s->stream, g,
-                           s->lam, s->mu, s->lam16, s->mu16, s->lamp, s->mup, s->sl, s->sm);
+                           s->lam, s->mu, s->lam16, s->mu16, s->lamp, s->mup, s->sl, s->sm, s->matbf ? 1 : 0);
     }
     // PML strips
     s->strip_cells[S_L] = s->strip_cells[S_R] = (long)g.npml * g.nz;

@@ -468,7 +468,7 @@ int fdfk_batch_add(fdfk_batch *B, fdfk_solver *s)
 {
     if (!B || !s) return fail(FDFK_EINVAL, "null argument");
     if (s->g.norder != B->norder) return fail(FDFK_EINVAL, "solver FD order differs from the batch");
-    if (s->mat16) return fail(FDFK_ENOSYS, "FP16-material solvers are not batched");
+    if (s->mat16) return fail(FDFK_ENOSYS, "16-bit-material solvers are not batched");
     if (s->gen) return fail(FDFK_ENOSYS, "solvers with an absorbing top layer are not batched");
     if ((int)B->members.size() >= kMaxBatch) return fail(FDFK_ERANGE, "batch is full");
     for (auto *m : B->members)

@@ -451,7 +451,7 @@ __global__ __launch_bounds__(kFrameThreads) void k_frame(StepArgs a, ETiles t)
 // tiles release at the end of their march, instead of running after the
 // slowest of them.  Dynamic LDS: the larger of the two needs.
 static_assert(kFrameThreads == 256, "the fused step launches 256-lane blocks");
-template <int NO, bool M16 = false>
+template <int NO, int M16 = 0>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kIntWaves<NO>))) void k_step(IntArgs<NO> ia, StepArgs sa,
                                                                                       ETiles ft, int nint)
 {

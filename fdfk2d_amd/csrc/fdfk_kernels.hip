@@ -331,10 +331,10 @@ __device__ __forceinline__ int xcd_tile(int p, int n)
 // the stores are predicated.  Addresses are a per-column scalar base plus a
 // loop-invariant 32-bit lane offset.
 // ---------------------------------------------------------------------
-// M16: lam and mu are read as FP16 residuals against per-row FP32 profiles
-// (half the bytes; the FP32 arrays hold the same rounded values for the
-// other kernels)
-template <int NO, int NT, bool M16 = false>
+// M16: lam and mu are read as 16-bit residuals against per-row FP32
+// profiles, FP16 (M16 = 1) or BF16 (M16 = 2) (half the bytes; the FP32
+// arrays hold the same rounded values for the other kernels)
+template <int NO, int NT, int M16 = 0>
 struct IntKernel2 {
     static constexpr int HZ = 2 * NO - 1;
     static constexpr int TZ = NT - 2 * HZ;      // output rows per block
@@ -409,18 +409,25 @@ struct IntKernel2 {
     }
     // a material value at the lane's row of a column: byte offsets of the
     // 32-bit arrays, halved for the 16-bit copies
-    // (M16: the row's profile value plus the scaled FP16 residual)
+    // (M16: the row's profile value plus the scaled 16-bit residual)
+    static __device__ __forceinline__ float h16(unsigned short h)
+    {
+        if constexpr (M16 == 2)
+            return __uint_as_float((unsigned)h << 16);
+        else
+            return (float)__builtin_bit_cast(_Float16, h);
+    }
     static __device__ __forceinline__ float ld_lam(const IntArgs<NO> &a, const Rsrc &B, unsigned o, int so)
     {
-        if constexpr (M16)
-            return B.plam + (float)__builtin_bit_cast(_Float16, __builtin_amdgcn_raw_buffer_load_b16(B.lam16, o >> 1, so >> 1, 0)) * a.sl;
+        if constexpr (M16 != 0)
+            return B.plam + h16(__builtin_amdgcn_raw_buffer_load_b16(B.lam16, o >> 1, so >> 1, 0)) * a.sl;
         else
             return ld(B.lam, o, so);
     }
     static __device__ __forceinline__ float ld_mu(const IntArgs<NO> &a, const Rsrc &B, unsigned o, int so)
     {
-        if constexpr (M16)
-            return B.pmu + (float)__builtin_bit_cast(_Float16, __builtin_amdgcn_raw_buffer_load_b16(B.mu16, o >> 1, so >> 1, 0)) * a.sm;
+        if constexpr (M16 != 0)
+            return B.pmu + h16(__builtin_amdgcn_raw_buffer_load_b16(B.mu16, o >> 1, so >> 1, 0)) * a.sm;
         else
             return ld(B.mu, o, so);
     }
@@ -623,7 +630,7 @@ struct IntKernel2 {
             B.uxp = mk(g, a.uxp, B.xb); B.uzp = mk(g, a.uzp, B.xb);
             B.uxo = mk(g, a.uxo, B.xb); B.uzo = mk(g, a.uzo, B.xb);
             B.bf = mk(g, a.bf, B.xb); B.lam = mk(g, a.lam, B.xb); B.mu = mk(g, a.mu, B.xb);
-            if constexpr (M16) {
+            if constexpr (M16 != 0) {
                 B.lam16 = mk16(g, a.lam16, B.xb); B.mu16 = mk16(g, a.mu16, B.xb);
                 B.plam = a.lamp[rc - 1]; B.pmu = a.mup[rc - 1];
             }
@@ -709,24 +716,37 @@ __global__ void k_import_material(Geo g, const float *c, float dt2, float *bf, f
     mu[d] = c[3 * src + 2];
 }
 
-// 16-bit material storage: lam = lamp(row) + h * sl with h the FP16 residual
-// against a per-row FP32 profile (power-of-two scale, nearest even), so the
-// rounding falls on the lateral variation only; the FP32 arrays receive the
-// same rounded values (the product h * sl is exact, the sum rounds once)
+// 16-bit material storage: lam = lamp(row) + h * sl with h the FP16 (bf = 0)
+// or BF16 (bf = 1) residual against a per-row FP32 profile (power-of-two
+// scale, nearest even), so the rounding falls on the lateral variation
+// only; the FP32 arrays receive the same rounded values (the product h * sl
+// is exact, the sum rounds once)
+__device__ __forceinline__ unsigned short round16(float x, int bf, float &v)
+{
+    if (bf) {
+        unsigned u = __float_as_uint(x);
+        u += 0x7FFFu + ((u >> 16) & 1u);   // nearest even (finite values)
+        const unsigned short h = (unsigned short)(u >> 16);
+        v = __uint_as_float((unsigned)h << 16);
+        return h;
+    }
+    const _Float16 h = (_Float16)x;
+    v = (float)h;
+    return __builtin_bit_cast(unsigned short, h);
+}
 __global__ void k_round_material(Geo g, float *lam, float *mu, unsigned short *lam16, unsigned short *mu16,
-                                 const float *lamp, const float *mup, float sl, float sm)
+                                 const float *lamp, const float *mup, float sl, float sm, int bf)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     long n = (long)g.NX * g.NZ;
     if (t >= n) return;
     const int k = (int)(t % g.NZ);
     const size_t d = (size_t)(t / g.NZ) * g.ld + (size_t)k;
-    const _Float16 hl = (_Float16)((lam[d] - lamp[k]) * (1.0f / sl));
-    const _Float16 hm = (_Float16)((mu[d] - mup[k]) * (1.0f / sm));
-    lam16[d] = __builtin_bit_cast(unsigned short, hl);
-    mu16[d] = __builtin_bit_cast(unsigned short, hm);
-    lam[d] = lamp[k] + (float)hl * sl;
-    mu[d] = mup[k] + (float)hm * sm;
+    float vl, vm;
+    lam16[d] = round16((lam[d] - lamp[k]) * (1.0f / sl), bf, vl);
+    mu16[d] = round16((mu[d] - mup[k]) * (1.0f / sm), bf, vm);
+    lam[d] = lamp[k] + vl * sl;
+    mu[d] = mup[k] + vm * sm;
 }
 
 // reference (1:2, NZ, NX) <-> two padded arrays (level 1 -> cur, level 2 -> prev)

@@ -66,10 +66,11 @@ typedef struct fdfk_job {
     int32_t nrcv;
     const float *rx, *rz;           /* receiver coordinates (m) */
     const float *c;                 /* rho, lambda, mu (reference layout) */
-    int32_t precision;              /* FDFK_FP32; FDFK_FP16: lam and mu stored as FP16 with
-                                       power-of-two scales for the interior tiles (single
-                                       solves, fused step); FDFK_FP64: everything in FP64
-                                       (per-cell kernels, single solves); FDFK_BF16: not built */
+    int32_t precision;              /* FDFK_FP32; FDFK_FP16 / FDFK_BF16: lam and mu stored for
+                                       the interior tiles as per-row FP32 profiles plus FP16 /
+                                       BF16 residuals with power-of-two scales (single solves,
+                                       fused step; the wavefields stay FP32); FDFK_FP64:
+                                       everything in FP64 (per-cell kernels, single solves) */
     int32_t wave;                   /* FDFK_PSV or FDFK_SH (FP32 or FP64, single solves) */
 } fdfk_job;
 

@@ -1,6 +1,6 @@
 """Storage precision at the size of the single large solve (8001 x 2000,
 dx = dz = 50 m, dt = 0.003 s, 40-cell PML, order 6, 401 receivers): the FP32
-tiled path and FP32 with FP16 materials against the FDFK_FP64 run of the same
+tiled path and FP32 with FP16 or BF16 materials against the FDFK_FP64 run of the same
 workload (the general per-cell kernels in double, themselves equal to the
 FP64 oracle bit for bit on small grids, tests/test_gpu_fp64_path.py).
 
@@ -35,7 +35,7 @@ def _run(cfg, precision):
 
 
 def test_storage_modes_against_fp64_at_full_size():
-    print(f"case: 8001x2000, {NSTEPS + 1} iterations, FP64 / FP32 / FP16 materials", flush=True)
+    print(f"case: 8001x2000, {NSTEPS + 1} iterations, FP64 / FP32 / FP16 and BF16 materials", flush=True)
     cfg = configs.c2(nt=NSTEPS)
     x64, z64 = _run(cfg, lib.FDFK_FP64)
     peak = np.abs(z64).max()
@@ -48,7 +48,7 @@ def test_storage_modes_against_fp64_at_full_size():
         rf64[i] = rf.prf(x64[i], z64[i], cfg["dt"], record=rec)[0]
         picks[i] = rec
     ref = np.concatenate([x64.ravel(), z64.ravel()])
-    for name, prec in (("fp32", lib.FDFK_FP32), ("fp16 materials", lib.FDFK_FP16)):
+    for name, prec in (("fp32", lib.FDFK_FP32), ("fp16 materials", lib.FDFK_FP16), ("bf16 materials", lib.FDFK_BF16)):
         x, z = _run(cfg, prec)
         e = np.linalg.norm(np.concatenate([x.ravel(), z.ravel()]) - ref) / np.linalg.norm(ref)
         num = den = 0.0

@@ -337,13 +337,21 @@ def test_autotuned_tile_width_same_results(case, bg):
         assert np.array_equal(a, b)
 
 
-def _round_materials(c, nx, nz, npml, nbd):
+def _bf16(x):
+    """float32 -> BF16 (nearest even, finite values) -> float32."""
+    b = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    b = (b + 0x7FFF + ((b >> 16) & 1)) & 0xFFFF0000
+    return b.astype(np.uint32).view(np.float32)
+
+
+def _round_materials(c, nx, nz, npml, nbd, kind="fp16"):
     """The library's 16-bit material storage restated: per row a profile, the
     midrange (min + max) / 2 in FP32 of the row's values over the undamped
     columns 1-nbd..nx+nbd (rows below nz+nbd take row nz+nbd's); lam and mu
-    become profile + FP16(residual / scale) * scale, one power-of-two scale
-    per field with |residual| / scale < 2^15 over the undamped cells, nearest
-    even.  (The damped cells are copies of undamped ones on the device.)"""
+    become profile + FP16 or BF16(residual / scale) * scale, one power-of-two
+    scale per field with |residual| / scale < 2^15 over the undamped cells,
+    nearest even.  (The damped cells are copies of undamped ones on the
+    device.)"""
     NX, NZ = nx + 2 * npml, nz + npml
     flat = np.array(c, np.float32).ravel().copy()
     x0, x1, zb = npml - nbd, npml + nx + nbd, nz + nbd      # 0-based column range [x0, x1), rows [0, zb)
@@ -354,19 +362,21 @@ def _round_materials(c, nx, nz, npml, nbd):
         prof = np.concatenate([prof, np.full(NZ - zb, prof[-1], np.float32)]).astype(np.float32)
         m = float(np.abs(und - prof[None, :zb]).max())
         sc = np.float32(np.ldexp(1.0, np.frexp(m)[1] - 15)) if m > 0 else np.float32(1.0)
-        h = np.float16((v - prof[None, :]) * (np.float32(1.0) / sc)).astype(np.float32)
+        r = ((v - prof[None, :]) * (np.float32(1.0) / sc)).astype(np.float32)
+        h = _bf16(r) if kind == "bf16" else np.float16(r).astype(np.float32)
         flat[k::3] = (prof[None, :] + h * sc).astype(np.float32).reshape(-1)
     return flat.reshape(np.shape(c))
 
 
-@pytest.mark.parametrize("norder", [3, 8])
-def test_fp16_materials_equal_fp32_on_rounded_materials(norder):
-    """FDFK_FP16: the interior tiles read lam and mu as per-row FP32 profiles
-    plus FP16 residuals; every kernel sees the same rounded values, so the
-    run equals, bit for bit, an FP32 run on materials rounded the same way on
-    the host.  The rounding itself moves the seismograms by about the FP16
-    resolution of the lateral variation."""
-    print("case: FP16 materials, norder %d, 121 x 60, 301 iterations" % norder, flush=True)
+@pytest.mark.parametrize("norder,kind", [(3, "fp16"), (8, "fp16"), (3, "bf16"), (8, "bf16")])
+def test_fp16_materials_equal_fp32_on_rounded_materials(norder, kind):
+    """FDFK_FP16 / FDFK_BF16: the interior tiles read lam and mu as per-row
+    FP32 profiles plus FP16 or BF16 residuals; every kernel sees the same
+    rounded values, so the run equals, bit for bit, an FP32 run on materials
+    rounded the same way on the host.  The rounding itself moves the
+    seismograms by about the 16-bit resolution of the lateral variation."""
+    print("case: %s materials, norder %d, 121 x 60, 301 iterations" % (kind, norder), flush=True)
+    prec = lib.FDFK_BF16 if kind == "bf16" else lib.FDFK_FP16
     c = Case(nt=300, norder=norder, npml=15 if norder <= 4 else 20)
     bg = c.oracle_background()
     vpml = refstate.vp_max(c.c)     # the PML profile from the unrounded model in every run
@@ -378,19 +388,20 @@ def test_fp16_materials_equal_fp32_on_rounded_materials(norder):
         out = s.seismograms() + s.get_state()
         s.close()
         return out
-    h = run(c.c, lib.FDFK_FP16)
-    r = run(_round_materials(c.c, c.nx, c.nz, c.npml, c.nbd), lib.FDFK_FP32)
+    h = run(c.c, prec)
+    r = run(_round_materials(c.c, c.nx, c.nz, c.npml, c.nbd, kind), lib.FDFK_FP32)
     for a, b in zip(h, r):
         assert np.array_equal(a, b)
     f = run(c.c, lib.FDFK_FP32)
     d = max(rel_l2(h[0], f[0]), rel_l2(h[1], f[1]))
-    print("seismogram rel-L2, FP16 vs FP32 materials: %.3g" % d, flush=True)
+    print("seismogram rel-L2, %s vs FP32 materials: %.3g" % (kind, d), flush=True)
     assert 0 < d < 1e-2
 
 
-def test_fp16_materials_not_batched_nor_separate(monkeypatch):
+@pytest.mark.parametrize("prec", [lib.FDFK_FP16, lib.FDFK_BF16])
+def test_fp16_materials_not_batched_nor_separate(monkeypatch, prec):
     c = Case(nt=20)
-    s = lib.Solver(c.nx, c.nz, c.npml, c.norder, c.dx, c.dz, c.dt, c.nt, c.c, precision=lib.FDFK_FP16)
+    s = lib.Solver(c.nx, c.nz, c.npml, c.norder, c.dx, c.dz, c.dt, c.nt, c.c, precision=prec)
     b = lib.Batch(3)
     with pytest.raises(lib.FdfkError, match="not batched"):
         b.add(s)
@@ -398,4 +409,4 @@ def test_fp16_materials_not_batched_nor_separate(monkeypatch):
     s.close()
     monkeypatch.setenv("FDFK_FUSED", "0")
     with pytest.raises(lib.FdfkError, match="fused step only"):
-        lib.Solver(c.nx, c.nz, c.npml, c.norder, c.dx, c.dz, c.dt, c.nt, c.c, precision=lib.FDFK_FP16)
+        lib.Solver(c.nx, c.nz, c.npml, c.norder, c.dx, c.dz, c.dt, c.nt, c.c, precision=prec)
