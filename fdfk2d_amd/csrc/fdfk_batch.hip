@@ -50,7 +50,7 @@ __device__ __forceinline__ int find_job(const int *pre, int n, int bid)
         int mid = (lo + hi) >> 1;
         if (pre[mid] <= bid) lo = mid; else hi = mid;
     }
-    return lo;
+    return __builtin_amdgcn_readfirstlane(lo);   // block-uniform
 }
 
 // The step count since the active set's tables were built: launched
@@ -69,8 +69,8 @@ __global__ void kb_set(int *ctr, int v) { ctr[0] = v; }
 // background, the PML memories and (on recording iterations) the receivers,
 // in that block order; 256-lane blocks, kRecPerBlock receivers per block
 template <int NO>
-__global__ __launch_bounds__(256) void kb_prep(const BatchJob<NO> *J0, const BatchPrefix *P, const int *ctr,
-                                                int hstep)
+__global__ __launch_bounds__(256) void kb_prep(const BatchJob<NO> *__restrict__ J0, const BatchPrefix *__restrict__ P,
+                                                const int *ctr, int hstep)
 {
     const int n = P->n;
     const int dstep = ctr ? ctr[0] : hstep;
@@ -98,11 +98,14 @@ __global__ __launch_bounds__(256) void kb_prep(const BatchJob<NO> *J0, const Bat
 }
 
 // the interior tiles of every member, then their frame tiles, in one launch
-// (the single-job k_step arrangement)
+// (the single-job k_step arrangement).  The job table is read-only in these
+// kernels and aliases none of the arrays they write (__restrict__): its
+// fields are then read through the scalar cache wherever the march needs
+// them; without it, every field read after the block's first store was a
+// per-lane vector load (+60% time on a one-member batch)
 template <int NO>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kIntWaves<NO>))) void kb_step(const BatchJob<NO> *J0,
-                                                                                       const BatchPrefix *P,
-                                                                                       const int *ctr, int hstep)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kIntWaves<NO>))) void kb_step(
+    const BatchJob<NO> *__restrict__ J0, const BatchPrefix *__restrict__ P, const int *ctr, int hstep)
 {
     extern __shared__ float sm[];
     const int n = P->n;
