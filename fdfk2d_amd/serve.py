@@ -131,7 +131,16 @@ def rel_l2(a, b):
 # ---------------------------------------------------------------------------
 # serving
 # ---------------------------------------------------------------------------
-SOLO_AHEAD = 2   # chunks of iterations queued per solo stream at most
+# Solo streams (grids of at least solo_cells) are fed in chunks of
+# SOLO_CHUNK iterations, at most SOLO_AHEAD chunks queued per stream, so that
+# a large request's thousands of steps do not fill the hardware queues ahead
+# of the batch and of new requests; while solo streams run, the batch steps
+# in chunks of at most BATCH_CHUNK_WITH_SOLO, so that the loop comes back to
+# refill them before they drain (a stream is refilled only once idle).
+SOLO_AHEAD = 2
+SOLO_CHUNK = 500
+BATCH_CHUNK_WITH_SOLO = 25
+PROGRESS_S = 20.0   # a progress line at least this often on long runs
 
 
 class Server:
@@ -218,8 +227,13 @@ class Server:
 
     def _loop(self, pending, waiting, preparing, finishing, lat, results, keep_results, t0):
         with cf.ThreadPoolExecutor(self.workers) as pool:
+            last_log = 0.0
             while pending or waiting or preparing or self.active or self.solo or finishing:
                 now = time.perf_counter() - t0
+                if now - last_log >= PROGRESS_S:
+                    last_log = now
+                    self.log("%.0f s: %d served, %d batched, %d solo, %d preparing, %d waiting" % (
+                        now, len(lat), len(self.active), len(self.solo), len(preparing), len(waiting) + len(pending)))
                 while pending and pending[0]["arrival"] <= now:
                     waiting.append(pending.popleft())
                 while waiting:
@@ -261,7 +275,7 @@ class Server:
                         if n >= SOLO_AHEAD and s.idle():
                             n = 0
                         if n < SOLO_AHEAD:
-                            s.run(min(self.chunk, left))   # queued on the solver's stream
+                            s.run(min(SOLO_CHUNK, left))   # queued on the solver's stream
                             self.solo_queued[rid] = n + 1
                             queued_solo = True
                     if s.iteration > s.nt:
@@ -280,7 +294,7 @@ class Server:
                     elif pending:
                         time.sleep(wait)
                     continue
-                self.batch.run(self.chunk)
+                self.batch.run(min(self.chunk, BATCH_CHUNK_WITH_SOLO) if self.solo else self.chunk)
                 if self.batch.active < len(self.active):
                     for rid, (req, s) in list(self.active.items()):
                         if s.iteration > s.nt:
