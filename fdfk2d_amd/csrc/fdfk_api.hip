@@ -260,6 +260,8 @@ struct fdfk_solver {
     void *gux[2] = {nullptr, nullptr}, *guz[2] = {nullptr, nullptr};
     void *grho = nullptr, *glam = nullptr, *gmu = nullptr;
     void *ghx[2] = {}, *ghz[2] = {};   // face harmonic means [lam+2mu, mu]
+    void *gst[8] = {};                 // staged terms of level n (P-SV 8 planes, SH 2)
+    void *grq[6] = {};                 // ring correction terms (P-SV 6, SH 2), ring-cell order
     void *gbu[4][2][3] = {};           // split parts (1:2, cells), reference order
     void *gbpx[4][2] = {}, *gbpz[4][2] = {};
     double *gc0 = nullptr, *gc1 = nullptr;
@@ -891,6 +893,9 @@ GenArgs<T> gen_args(fdfk_solver *s)
     a.top_ring = s->bg_top_valid;
     for (int i = 0; i < 16; ++i) a.bg[i] = (const T *)s->gbg[i];
     a.it = s->bg_period > 0 ? s->it % s->bg_period : 0;
+    for (int i = 0; i < 8; ++i) a.st[i] = (T *)s->gst[i];
+    for (int i = 0; i < 6; ++i) a.rq[i] = (T *)s->grq[i];
+    a.ex0 = 1; a.ex1 = 0; a.ez0 = 1; a.ez1 = 0;   // staged over every cell
     return a;
 }
 
@@ -959,17 +964,19 @@ int gen_step_t(fdfk_solver *s, bool record)
     gring_box(g, a.top_ring, rb);
     const long nring = s->gring[0] ? gbox_count(rb[0], rb[1], rb[2], rb[3], rb[4], rb[5], rb[6], rb[7]) : 0;
     const long npsi = 2L * g.npml * g.nz + (s->zt != 1 ? 2L : 1L) * g.NX * g.npml;
-    const long np = nrec + nring + npsi;
     long nc = (long)g.NX * g.NZ;
     if constexpr (sizeof(T) == 4) {
         if (s->gtiled) {
             const Geo &v = s->gview;
             a.hx0 = v.ix0; a.hx1 = v.ix1; a.hz0 = v.iz0; a.hz1 = v.iz1;
-            const long nm = v.iz1 - v.iz0 + 1;
-            nc = (long)(v.iz0 - s->zt) * g.NX + (long)(s->zt + g.NZ - 1 - v.iz1) * g.NX +
-                 (long)(g.NX - (v.ix1 - v.ix0 + 1)) * nm;
+            nc = gbox_count(1 - g.npml, g.nx + g.npml, s->zt, s->zt + g.NZ - 1, a.hx0, a.hx1, a.hz0, a.hz1);
+            // the step kernel reaches norder cells into the hole
+            a.ex0 = v.ix0 + g.norder; a.ex1 = v.ix1 - g.norder;
+            a.ez0 = v.iz0 + g.norder; a.ez1 = v.iz1 - g.norder;
         }
     }
+    const long nst = gbox_count(1 - g.npml, g.nx + g.npml, s->zt, s->zt + g.NZ - 1, a.ex0, a.ex1, a.ez0, a.ez1);
+    const long np = nrec + nring + npsi + nst;
     auto interior = [&]() {
         if constexpr (sizeof(T) == 4) {
             if (!s->gtiled) return;
@@ -984,14 +991,18 @@ int gen_step_t(fdfk_solver *s, bool record)
             });
         }
     };
+    // the ring correction terms, from the ring values the prep launch writes
+    const long nq = a.ring_valid && s->grq[0] ? nring : 0;
     if (s->sh) {
         hipLaunchKernelGGL(k_gen_prep_sh<T>, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s->stream, a, r, nrec,
-                           nring, (T *)s->gring[0]);
+                           nring, npsi, (T *)s->gring[0]);
+        if (nq) hipLaunchKernelGGL(k_gen_ringq_sh<T>, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s->stream, a);
         interior();
         hipLaunchKernelGGL(k_gen_step_sh<T>, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s->stream, a);
     } else {
         hipLaunchKernelGGL(k_gen_prep<T>, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s->stream, a, r, nrec,
-                           nring, (T *)s->gring[0], (T *)s->gring[1]);
+                           nring, npsi, (T *)s->gring[0], (T *)s->gring[1]);
+        if (nq) hipLaunchKernelGGL(k_gen_ringq<T>, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s->stream, a);
         interior();
         hipLaunchKernelGGL(k_gen_step<T>, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s->stream, a);
     }
@@ -1255,6 +1266,9 @@ static int gen_create(const fdfk_job *job, fdfk_solver **out)
         hipLaunchKernelGGL(k_gen_faces<T>, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, s->stream, g, s->zt,
                            (const T *)s->glam, (const T *)s->gmu, m0, 1, (T *)s->ghx[0], (T *)s->ghz[0],
                            (T *)s->ghx[1], (T *)s->ghz[1]);
+        // staged terms of level n (written by every prep launch before they are read)
+        for (int i = 0; i < (s->sh ? 2 : 8); ++i)
+            if ((rc = s->dalloc((T **)&s->gst[i], nf))) return bad(rc);
     }
     if constexpr (sizeof(T) == 4) {
         // the tiled interior: columns 1..nx, rows 2*norder..nz (full-order
@@ -1548,6 +1562,14 @@ static int alloc_background(fdfk_solver *s, int period)
             HIPCHK(hipMemsetAsync(rx, 0, nf * 8, s->stream));
             HIPCHK(hipMemsetAsync(rz, 0, nf * 8, s->stream));
             s->gring[0] = rx; s->gring[1] = rz;
+        }
+        if (s->gen && !s->grq[0]) {
+            // ring correction terms: every ring cell of the largest ring box
+            int b[8];
+            gring_box(s->g, s->zt != 1, b);
+            const size_t nq = (size_t)gbox_count(b[0], b[1], b[2], b[3], b[4], b[5], b[6], b[7]);
+            for (int i = 0; i < (s->sh ? 2 : 6); ++i)
+                if ((rc = s->dalloc((char **)&s->grq[i], nq * (size_t)s->gsize))) return rc;
         }
     }
     HIPCHK(hipStreamSynchronize(s->stream));

@@ -21,9 +21,10 @@
 // formulation bit for bit and T = double its FP64 build.
 //
 // Launches per step: k_gen_prep (receivers, hybrid ring values, PML
-// memories, all from level n), then k_gen_step (every cell of level n+1,
-// or, in FP32, the cells around the deep interior rectangle that the tiled
-// interior kernel updates in the same step).
+// memories and the staged operator terms, all from level n), k_gen_ringq
+// (the hybrid correction terms of the ring cells), then k_gen_step (every
+// cell of level n+1, or, in FP32, the cells around the deep interior
+// rectangle that the tiled interior kernel updates in the same step).
 // ---------------------------------------------------------------------
 namespace fdfk {
 
@@ -55,6 +56,20 @@ struct GenArgs {
     // [mode] 0 lam+2mu, 1 mu; hx at cell (iz, ix) for the face (ix, ix+1),
     // hz for the face (iz, iz+1) (the same quotients the operators form)
     const T *hx[2], *hz[2];
+    // Staged per-cell terms of level n, formed by the prep launch over the
+    // cells the step kernel reaches (the box minus the rectangle ex0..ex1 x
+    // ez0..ez1, which is the hole shrunk by norder): a face value of the
+    // staggered second-derivative operators, face modulus times staggered
+    // difference, and the undivided collocated first derivatives the mixed
+    // operators take.  P-SV: 0 x-face ux [lam+2mu], 1 z-face ux [mu],
+    // 2 x-face uz [mu], 3 z-face uz [lam+2mu], 4 dz ux, 5 dx ux, 6 dz uz,
+    // 7 dx uz; SH: 0 x-face uy [mu], 1 z-face uy [mu].
+    T *st[8];
+    int ex0, ex1, ez0, ez1;
+    // hybrid-ring correction terms of level n (the operator parts of the
+    // masked ring background, P-SV 6, SH 2) in the ring cells' order of
+    // gring_box, formed by a launch of their own before the step
+    T *rq[6];
 };
 
 template <class T>
@@ -102,6 +117,18 @@ __device__ __forceinline__ bool gbox_cell(long t, int x0, int x1, int z0, int z1
     t -= nl;
     if (t < (long)(x1 - hx1) * nm) { ix = hx1 + 1 + (int)(t / nm); iz = hz0 + (int)(t % nm); return true; }
     return false;
+}
+// the inverse: the index t of cell (iz, ix) of the box minus the hole
+__device__ __forceinline__ long gbox_index(int iz, int ix, int x0, int x1, int z0, int z1, int hx0, int hx1, int hz0,
+                                           int hz1)
+{
+    const long W = x1 - x0 + 1, H = z1 - z0 + 1;
+    if (hx0 > hx1 || hz0 > hz1) return (long)(ix - x0) * H + (iz - z0);
+    const long na = hz0 - z0, nb = z1 - hz1, nm = hz1 - hz0 + 1;
+    if (iz < hz0) return (long)(ix - x0) * na + (iz - z0);
+    if (iz > hz1) return na * W + (long)(ix - x0) * nb + (iz - hz1 - 1);
+    if (ix < hx0) return (na + nb) * W + (long)(ix - x0) * nm + (iz - hz0);
+    return (na + nb) * W + (long)(hx0 - x0) * nm + (long)(ix - hx1 - 1) * nm + (iz - hz0);
 }
 __host__ __device__ inline long gbox_count(int x0, int x1, int z0, int z1, int hx0, int hx1, int hz0, int hz1)
 {
@@ -192,6 +219,15 @@ __device__ __forceinline__ long gring(const GenArgs<T> &a, int iz, int ix, int i
         }
     }
     return -1;
+}
+
+// index of a ring cell in the order of gring_box
+template <class T>
+__device__ __forceinline__ long gring_index(const GenArgs<T> &a, int iz, int ix)
+{
+    int b[8];
+    gring_box(a.g, a.top_ring, b);
+    return gbox_index(iz, ix, b[0], b[1], b[2], b[3], b[4], b[5], b[6], b[7]);
 }
 
 // field accessors: plain level-n fields, or the ring background masked to
@@ -321,6 +357,149 @@ __device__ T gxz(const GenArgs<T> &a, const T *mat, const F &u, int iz, int ix)
     return s / dz;
 }
 
+// ---- staged operators: the same sums as gx2, gz2, gzx and gxz, term by
+// term, with the inner sums read from the staged planes ----
+// Order of the staggered difference at the face between p and p+1 as the
+// operators of diffx2/diffz2 choose it: the s1 sums of gx2/gz2 take
+// gnx(p) / gnz(p) at the face p = i+k-1, the s2 sums gnx(h+1) / gnz(h+1) at
+// the face h = i-k.
+template <class T>
+__device__ __forceinline__ int gnx(const GenArgs<T> &a, int p)
+{
+    return gmin(gmin(p + a.g.npml - 1, a.g.nx + a.g.npml - p), a.g.norder);
+}
+template <class T>
+__device__ __forceinline__ int gnz(const GenArgs<T> &a, int p)
+{
+    return gmin(gmin(p - a.zt, a.g.nz + a.g.npml - p), a.g.norder);
+}
+// staggered difference at the face (p, p+1) along x / z with n terms
+template <class T, class F>
+__device__ T gsx(const GenArgs<T> &a, const F &u, int iz, int p, int n)
+{
+    T s = (T)0.0f;
+    for (int j = 1; j <= n; ++j) s = gacc(s, gco(a.c0, a.g.norder, j, n), (T)(u(iz, p + j) - u(iz, p - j + 1)));
+    return s;
+}
+template <class T, class F>
+__device__ T gsz(const GenArgs<T> &a, const F &u, int p, int ix, int n)
+{
+    T s = (T)0.0f;
+    for (int j = 1; j <= n; ++j) s = gacc(s, gco(a.c0, a.g.norder, j, n), (T)(u(p + j, ix) - u(p - j + 1, ix)));
+    return s;
+}
+// gx2 from the staged x-face values of plane arr (field u, modulus mode)
+template <class T>
+__device__ T gx2s(const GenArgs<T> &a, int mode, int arr, const T *u, int iz, int ix)
+{
+#pragma clang fp contract(off)
+    const int norder = a.g.norder, nx = a.g.nx, npml = a.g.npml;
+    const int nox = gmin(gmin(ix + npml - 1, nx + npml - ix), norder);
+    const T *F = a.st[arr];
+    T s = (T)0.0f;
+    for (int k = 1; k <= nox; ++k) {
+        const int h = ix - k, nh = gnx(a, h + 1);
+        // the face value of plane arr has gnx(h) terms; near the padded
+        // edges the s2 sum takes another count and is formed here
+        const T pb = nh == gnx(a, h) ? F[gcell(a, iz, h)]
+                                     : a.hx[mode][gcell(a, iz, h)] * gsx(a, GField<T>{&a, u}, iz, h, nh);
+        s = gacc(s, gco(a.c0, norder, k, nox), (T)(F[gcell(a, iz, ix + k - 1)] - pb));
+    }
+    const bool in_reg = iz >= 1 && iz <= a.g.nz && ix >= 1 && ix <= nx;
+    const T dxs = (T)a.g.dx;
+    const T dx2 = in_reg ? (T)(1.0 / (double)(dxs * dxs)) : (T)1.0f / (dxs * dxs);
+    return s * dx2;
+}
+template <class T>
+__device__ T gz2s(const GenArgs<T> &a, int mode, int arr, const T *u, int iz, int ix)
+{
+#pragma clang fp contract(off)
+    const int norder = a.g.norder, nz = a.g.nz, npml = a.g.npml, zt = a.zt;
+    const int noz = gmin(gmin(iz - zt, nz + npml - iz), norder);
+    const T *F = a.st[arr];
+    T s = (T)0.0f;
+    for (int k = 1; k <= noz; ++k) {
+        const int h = iz - k, nh = gnz(a, h + 1);
+        const T pb = nh == gnz(a, h) ? F[gcell(a, h, ix)]
+                                     : a.hz[mode][gcell(a, h, ix)] * gsz(a, GField<T>{&a, u}, h, ix, nh);
+        s = gacc(s, gco(a.c0, norder, k, noz), (T)(F[gcell(a, iz + k - 1, ix)] - pb));
+    }
+    const T dzs = (T)a.g.dz;
+    return s * ((T)1.0f / (dzs * dzs));
+}
+// gzx / gxz from the staged undivided first derivatives of plane arr
+template <class T>
+__device__ T gzxs(const GenArgs<T> &a, const T *mat, int arr, int iz, int ix)
+{
+#pragma clang fp contract(off)
+    const int norder = a.g.norder;
+    const int nox = gmin(gmin(ix + a.g.npml - 1, a.g.nx + a.g.npml - ix), norder);
+    const T dx = (T)a.g.dx, dz = (T)a.g.dz;
+    const T *D = a.st[arr];
+    T s = (T)0.0f;
+    for (int k = 1; k <= nox; ++k) {
+        const size_t cp = gcell(a, iz, ix + k), cm = gcell(a, iz, ix - k);
+        const T vp = mat[cp] * D[cp] / dz;
+        const T vm = mat[cm] * D[cm] / dz;
+        s = gacc(s, gco(a.c1, norder, k, nox), (T)(vp - vm));
+    }
+    return s / dx;
+}
+template <class T>
+__device__ T gxzs(const GenArgs<T> &a, const T *mat, int arr, int iz, int ix)
+{
+#pragma clang fp contract(off)
+    const int norder = a.g.norder;
+    const int noz = gmin(gmin(iz - a.zt, a.g.nz + a.g.npml - iz), norder);
+    const T dx = (T)a.g.dx, dz = (T)a.g.dz;
+    const T *D = a.st[arr];
+    T s = (T)0.0f;
+    for (int k = 1; k <= noz; ++k) {
+        const size_t cp = gcell(a, iz + k, ix), cm = gcell(a, iz - k, ix);
+        const T vp = mat[cp] * D[cp] / dx;
+        const T vm = mat[cm] * D[cm] / dx;
+        s = gacc(s, gco(a.c1, norder, k, noz), (T)(vp - vm));
+    }
+    return s / dz;
+}
+// Cell t of the staged region (the box minus the shrunk hole)
+template <class T>
+__device__ __forceinline__ bool gstage_cell(const GenArgs<T> &a, long t, int &iz, int &ix)
+{
+    const Geo &g = a.g;
+    return gbox_cell(t, 1 - g.npml, g.nx + g.npml, a.zt, a.zt + g.NZ - 1, a.ex0, a.ex1, a.ez0, a.ez1, iz, ix);
+}
+// the staged terms of one cell from level n
+template <class T>
+__device__ void gstage(const GenArgs<T> &a, long t)
+{
+#pragma clang fp contract(off)
+    int iz, ix;
+    if (!gstage_cell(a, t, iz, ix)) return;
+    const size_t c = gcell(a, iz, ix);
+    const GField<T> fux{&a, a.ux}, fuz{&a, a.uz};
+    const int nfx = gnx(a, ix), nfz = gnz(a, iz);
+    a.st[0][c] = a.hx[0][c] * gsx(a, fux, iz, ix, nfx);
+    a.st[1][c] = a.hz[1][c] * gsz(a, fux, iz, ix, nfz);
+    a.st[2][c] = a.hx[1][c] * gsx(a, fuz, iz, ix, nfx);
+    a.st[3][c] = a.hz[0][c] * gsz(a, fuz, iz, ix, nfz);
+    a.st[4][c] = gcdz(a, fux, iz, ix);
+    a.st[5][c] = gcdx(a, fux, iz, ix);
+    a.st[6][c] = gcdz(a, fuz, iz, ix);
+    a.st[7][c] = gcdx(a, fuz, iz, ix);
+}
+template <class T>
+__device__ void gstage_sh(const GenArgs<T> &a, long t)
+{
+#pragma clang fp contract(off)
+    int iz, ix;
+    if (!gstage_cell(a, t, iz, ix)) return;
+    const size_t c = gcell(a, iz, ix);
+    const GField<T> fu{&a, a.ux};
+    a.st[0][c] = a.hx[1][c] * gsx(a, fu, iz, ix, gnx(a, ix));
+    a.st[1][c] = a.hz[1][c] * gsz(a, fu, iz, ix, gnz(a, iz));
+}
+
 // the six RHS parts {Xx, Zx, Mx, Xz, Zz, Mz} at one cell; the free-surface
 // closure on row 1 when zt = 1 (half-cell energy form, traction free)
 template <class T, class F>
@@ -358,6 +537,44 @@ __device__ void gparts(const GenArgs<T> &a, const F &ux, const F &uz, int iz, in
     p[3] = gx2(a, 1, uz, iz, ix);
     p[4] = gz2(a, 0, uz, iz, ix);
     p[5] = gzx(a, a.mu, ux, iz, ix) + gxz(a, a.lam, ux, iz, ix);
+}
+// gparts of the level-n fields from the staged terms (same values)
+template <class T>
+__device__ void gparts_st(const GenArgs<T> &a, int iz, int ix, T p[6])
+{
+#pragma clang fp contract(off)
+    if (iz == 1 && a.zt == 1) {
+        const GField<T> ux{&a, a.ux}, uz{&a, a.uz};
+        const size_t ca = gcell(a, 1, ix), cb = gcell(a, 2, ix);
+        const T dz = (T)a.g.dz, dx = (T)a.g.dx;
+        const T sdz2 = (T)2.0f / (dz * dz);
+        const int norder = a.g.norder;
+        const int nox = gmin(gmin(ix + a.g.npml - 1, a.g.nx + a.g.npml - ix), norder);
+        T sl = (T)0.0f, sm = (T)0.0f;
+        for (int k = 1; k <= nox; ++k) {
+            const size_t a1 = gcell(a, 1, ix + k), b1 = gcell(a, 2, ix + k), a2 = gcell(a, 1, ix - k), b2 = gcell(a, 2, ix - k);
+            const T lp = (T)0.5f * (a.lam[a1] + a.lam[b1]), lm = (T)0.5f * (a.lam[a2] + a.lam[b2]);
+            const T mp = (T)0.5f * (a.mu[a1] + a.mu[b1]), mm = (T)0.5f * (a.mu[a2] + a.mu[b2]);
+            sl = gacc(sl, gco(a.c1, norder, k, nox), (T)(lp * (uz(2, ix + k) - uz(1, ix + k)) - lm * (uz(2, ix - k) - uz(1, ix - k))));
+            sm = gacc(sm, gco(a.c1, norder, k, nox), (T)(mp * (ux(2, ix + k) - ux(1, ix + k)) - mm * (ux(2, ix - k) - ux(1, ix - k))));
+        }
+        const T lbar = (T)0.5f * (a.lam[ca] + a.lam[cb]), mbar = (T)0.5f * (a.mu[ca] + a.mu[cb]);
+        const T dxuz = (a.st[7][ca] + a.st[7][cb]) / dx;
+        const T dxux = (a.st[5][ca] + a.st[5][cb]) / dx;
+        p[0] = gx2s(a, 0, 0, a.ux, iz, ix);
+        p[1] = (a.hz[1][ca] * (ux(2, ix) - ux(1, ix))) * sdz2;
+        p[2] = (sl / dx + mbar * dxuz) / dz;
+        p[3] = gx2s(a, 1, 2, a.uz, iz, ix);
+        p[4] = (a.hz[0][ca] * (uz(2, ix) - uz(1, ix))) * sdz2;
+        p[5] = (sm / dx + lbar * dxux) / dz;
+        return;
+    }
+    p[0] = gx2s(a, 0, 0, a.ux, iz, ix);
+    p[1] = gz2s(a, 1, 1, a.ux, iz, ix);
+    p[2] = gzxs(a, a.lam, 6, iz, ix) + gxzs(a, a.mu, 7, iz, ix);
+    p[3] = gx2s(a, 1, 2, a.uz, iz, ix);
+    p[4] = gz2s(a, 0, 3, a.uz, iz, ix);
+    p[5] = gzxs(a, a.mu, 4, iz, ix) + gxzs(a, a.lam, 5, iz, ix);
 }
 
 // memory of strip cell (iz, ix) or 0 outside the strips / where a strip
@@ -432,8 +649,8 @@ __device__ void grecord(const GenArgs<T> &a, const GenRec<T> &r, int i)
 // of gring_box (the other cells of the ring fields stay zero from their
 // allocation), the rest update the memories of strip cells
 template <class T>
-__global__ __launch_bounds__(256) void k_gen_prep(GenArgs<T> a, GenRec<T> r, long nrec, long nring, T *ringx,
-                                                   T *ringz)
+__global__ __launch_bounds__(256) void k_gen_prep(GenArgs<T> a, GenRec<T> r, long nrec, long nring, long npsi,
+                                                   T *ringx, T *ringz)
 {
 #pragma clang fp contract(off)
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -452,6 +669,7 @@ __global__ __launch_bounds__(256) void k_gen_prep(GenArgs<T> a, GenRec<T> r, lon
         return;
     }
     t -= nring;
+    if (t >= npsi) { gstage(a, t - npsi); return; }
     // strip cells: L, R (npml x nz each), B (NX x npml), T (NX x npml)
     const long nlr = (long)g.npml * g.nz, nb = (long)g.NX * g.npml;
     int s, iz, ix;
@@ -495,9 +713,8 @@ __global__ __launch_bounds__(256) void k_gen_step(GenArgs<T> a)
         a.uzo[gi] = a.uz[gi];
         return;
     }
-    const GField<T> fux{&a, a.ux}, fuz{&a, a.uz};
     T p[6];
-    gparts(a, fux, fuz, iz, ix, p);
+    gparts_st(a, iz, ix, p);
     const int s = gstrip(a, iz, ix);
     // PML memory corrections (nonzero only next to damped cells)
     {
@@ -531,10 +748,8 @@ __global__ __launch_bounds__(256) void k_gen_step(GenArgs<T> a)
         int arr = 0;
         if (gring(a, iz, ix, a.it, arr) >= 0) {
             const bool inside = iz >= 1 && iz <= g.nz && ix >= 1 && ix <= g.nx;
-            const GMasked<T> wx{&a, a.ringx, !inside}, wz{&a, a.ringz, !inside};
-            T q[6];
-            gparts(a, wx, wz, iz, ix, q);
-            for (int m = 0; m < 6; ++m) p[m] = inside ? p[m] + q[m] : p[m] - q[m];
+            const long q = gring_index(a, iz, ix);
+            for (int m = 0; m < 6; ++m) p[m] = inside ? p[m] + a.rq[m][q] : p[m] - a.rq[m][q];
         }
     }
     const T dt2 = (T)(g.dt * g.dt);
@@ -564,6 +779,38 @@ __global__ __launch_bounds__(256) void k_gen_step(GenArgs<T> a)
     }
 }
 
+// ---- hybrid-ring correction terms: one thread per ring cell that the step
+// kernel updates (the operator parts of the ring background masked to the
+// other side of the interface), so that the step kernel's waves, most of
+// which hold a few ring cells, read six values instead of evaluating the
+// operators on the masked background ----
+template <class T>
+__device__ __forceinline__ bool gring_job(const GenArgs<T> &a, long t, int &iz, int &ix)
+{
+    const Geo &g = a.g;
+    int b[8];
+    gring_box(g, a.top_ring, b);
+    if (t >= gbox_count(b[0], b[1], b[2], b[3], b[4], b[5], b[6], b[7])) return false;
+    gbox_cell(t, b[0], b[1], b[2], b[3], b[4], b[5], b[6], b[7], iz, ix);
+    // cells the step kernel leaves: the padded grid's edge and the hole
+    const int iz_first = a.zt == 1 ? 1 : a.zt + 1;
+    if (ix < 2 - g.npml || ix > g.nx + g.npml - 1 || iz < iz_first || iz > g.nz + g.npml - 1) return false;
+    return !(ix >= a.hx0 && ix <= a.hx1 && iz >= a.hz0 && iz <= a.hz1);
+}
+template <class T>
+__global__ __launch_bounds__(256) void k_gen_ringq(GenArgs<T> a)
+{
+#pragma clang fp contract(off)
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int iz, ix;
+    if (!gring_job(a, t, iz, ix)) return;
+    const bool inside = iz >= 1 && iz <= a.g.nz && ix >= 1 && ix <= a.g.nx;
+    const GMasked<T> wx{&a, a.ringx, !inside}, wz{&a, a.ringz, !inside};
+    T q[6];
+    gparts(a, wx, wz, iz, ix, q);
+    for (int m = 0; m < 6; ++m) a.rq[m][t] = q[m];
+}
+
 // ---- SH (transverse) motion: uy in the ux slots (fields, ring values,
 // x background arrays, seisx), one split part pair per strip cell (parts 0
 // and 1 of component 0), memories of component 0.  rho uy_tt = X + Z with X
@@ -584,7 +831,23 @@ __device__ void gparts_sh(const GenArgs<T> &a, const F &u, int iz, int ix, T &X,
 }
 
 template <class T>
-__global__ __launch_bounds__(256) void k_gen_prep_sh(GenArgs<T> a, GenRec<T> r, long nrec, long nring, T *ringx)
+__global__ __launch_bounds__(256) void k_gen_ringq_sh(GenArgs<T> a)
+{
+#pragma clang fp contract(off)
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int iz, ix;
+    if (!gring_job(a, t, iz, ix)) return;
+    const bool inside = iz >= 1 && iz <= a.g.nz && ix >= 1 && ix <= a.g.nx;
+    const GMasked<T> w{&a, a.ringx, !inside};
+    T qx, qz;
+    gparts_sh(a, w, iz, ix, qx, qz);
+    a.rq[0][t] = qx;
+    a.rq[1][t] = qz;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_gen_prep_sh(GenArgs<T> a, GenRec<T> r, long nrec, long nring, long npsi,
+                                                      T *ringx)
 {
 #pragma clang fp contract(off)
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -601,6 +864,7 @@ __global__ __launch_bounds__(256) void k_gen_prep_sh(GenArgs<T> a, GenRec<T> r, 
         return;
     }
     t -= nring;
+    if (t >= npsi) { gstage_sh(a, t - npsi); return; }
     const long nlr = (long)g.npml * g.nz, nb = (long)g.NX * g.npml;
     int s, iz, ix;
     if (t < nlr) { s = 0; ix = 1 - g.npml + (int)(t / g.nz); iz = 1 + (int)(t % g.nz); }
@@ -634,9 +898,14 @@ __global__ __launch_bounds__(256) void k_gen_step_sh(GenArgs<T> a)
         a.uxo[gi] = a.ux[gi];
         return;
     }
-    const GField<T> fu{&a, a.ux};
-    T X, Z;
-    gparts_sh(a, fu, iz, ix, X, Z);
+    T X = gx2s(a, 1, 0, a.ux, iz, ix), Z;
+    if (iz == 1 && a.zt == 1) {
+        const size_t ca = gcell(a, 1, ix);
+        const T dz = (T)g.dz;
+        Z = (a.hz[1][ca] * (a.ux[gcell(a, 2, ix)] - a.ux[ca])) * ((T)2.0f / (dz * dz));
+    } else {
+        Z = gz2s(a, 1, 1, a.ux, iz, ix);
+    }
     const bool surf = iz == 1 && a.zt == 1;
     {
         const int norder = g.norder;
@@ -658,9 +927,8 @@ __global__ __launch_bounds__(256) void k_gen_step_sh(GenArgs<T> a)
         int arr = 0;
         if (gring(a, iz, ix, a.it, arr) >= 0) {
             const bool inside = iz >= 1 && iz <= g.nz && ix >= 1 && ix <= g.nx;
-            const GMasked<T> w{&a, a.ringx, !inside};
-            T qx, qz;
-            gparts_sh(a, w, iz, ix, qx, qz);
+            const long q = gring_index(a, iz, ix);
+            const T qx = a.rq[0][q], qz = a.rq[1][q];
             if (inside) { X = X + qx; Z = Z + qz; }
             else { X = X - qx; Z = Z - qz; }
         }
