@@ -192,6 +192,11 @@ struct fdfk_solver {
     int npower = 2;
     hipStream_t stream = nullptr;
     hipEvent_t ev_order = nullptr;   // orders this stream against a batch's (join, leave)
+    // FP32 general path with the tiled interior: the per-cell kernels of a
+    // step run on a second stream beside the interior kernel (fork after the
+    // prep launch, join before the next one)
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::vector<std::pair<void *, size_t>> allocs;   // device blocks and their sizes
     int device = 0;
     int64_t bytes = 0;
@@ -304,11 +309,15 @@ struct fdfk_solver {
     ~fdfk_solver()
     {
         if (stream) hipStreamSynchronize(stream);
+        if (stream2) hipStreamSynchronize(stream2);
         if (ev_order) hipEventDestroy(ev_order);
+        if (ev_fork) hipEventDestroy(ev_fork);
+        if (ev_join) hipEventDestroy(ev_join);
         for (auto &e : ev_int) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
         for (auto &e : ev_step) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
         for (auto &a : allocs) block_put(device, a.first, a.second);   // stream idle: reusable
         if (stream) hipStreamDestroy(stream);
+        if (stream2) hipStreamDestroy(stream2);
     }
 };
 
@@ -993,18 +1002,37 @@ int gen_step_t(fdfk_solver *s, bool record)
     };
     // the ring correction terms, from the ring values the prep launch writes
     const long nq = a.ring_valid && s->grq[0] ? nring : 0;
-    if (s->sh) {
+    // The interior kernel and the per-cell kernels write disjoint cells of
+    // level n+1 and read level n and the prep launch's output only, so with
+    // a second stream they run side by side: the interior is bound by HBM,
+    // the per-cell kernels by cache latency, and their workgroups (no LDS,
+    // few registers) fit beside the interior's on the same CUs.  The interior
+    // is queued first so that its workgroups take their slots first.
+    const bool fork = s->stream2 != nullptr;
+    hipStream_t sc = fork ? s->stream2 : s->stream;
+    if (s->sh)
         hipLaunchKernelGGL(k_gen_prep_sh<T>, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s->stream, a, r, nrec,
                            nring, npsi, (T *)s->gring[0]);
-        if (nq) hipLaunchKernelGGL(k_gen_ringq_sh<T>, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s->stream, a);
-        interior();
-        hipLaunchKernelGGL(k_gen_step_sh<T>, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s->stream, a);
-    } else {
+    else
         hipLaunchKernelGGL(k_gen_prep<T>, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s->stream, a, r, nrec,
                            nring, npsi, (T *)s->gring[0], (T *)s->gring[1]);
-        if (nq) hipLaunchKernelGGL(k_gen_ringq<T>, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s->stream, a);
+    if (fork) {
+        HIPCHK(hipEventRecord(s->ev_fork, s->stream));
+        HIPCHK(hipStreamWaitEvent(s->stream2, s->ev_fork, 0));
         interior();
-        hipLaunchKernelGGL(k_gen_step<T>, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s->stream, a);
+    }
+    if (s->sh) {
+        if (nq) hipLaunchKernelGGL(k_gen_ringq_sh<T>, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, sc, a);
+        if (!fork) interior();
+        hipLaunchKernelGGL(k_gen_step_sh<T>, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, sc, a);
+    } else {
+        if (nq) hipLaunchKernelGGL(k_gen_ringq<T>, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, sc, a);
+        if (!fork) interior();
+        hipLaunchKernelGGL(k_gen_step<T>, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, sc, a);
+    }
+    if (fork) {
+        HIPCHK(hipEventRecord(s->ev_join, s->stream2));
+        HIPCHK(hipStreamWaitEvent(s->stream, s->ev_join, 0));
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(FDFK_EDEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
@@ -1280,6 +1308,13 @@ static int gen_create(const fdfk_job *job, fdfk_solver **out)
         if (!(e && std::atoi(e) == 0) && v.iz1 >= v.iz0) {
             s->gtiled = true;
             s->gview = v;
+            // the per-cell kernels beside the interior (FDFK_GEN_OVERLAP=0: in line)
+            const char *eo = std::getenv("FDFK_GEN_OVERLAP");
+            if (!(eo && std::atoi(eo) == 0) &&
+                (hipStreamCreate(&s->stream2) != hipSuccess ||
+                 hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming) != hipSuccess ||
+                 hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming) != hipSuccess))
+                return bad(fail(FDFK_EDEVICE, "second stream"));
             if ((rc = s->dalloc(&s->gbf, nf))) return bad(rc);
             hipLaunchKernelGGL(k_gen_bf, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, s->stream, (long)nf,
                                (float)(g.dt * g.dt), (const float *)s->grho, s->gbf);
