@@ -265,8 +265,9 @@ struct fdfk_solver {
     void *gux[2] = {nullptr, nullptr}, *guz[2] = {nullptr, nullptr};
     void *grho = nullptr, *glam = nullptr, *gmu = nullptr;
     void *ghx[2] = {}, *ghz[2] = {};   // face harmonic means [lam+2mu, mu]
-    void *gst[8] = {};                 // staged terms of level n (P-SV 8 planes, SH 2)
+    void *gst[12] = {};                // staged terms of level n (P-SV 12 planes, SH 4)
     void *grq[6] = {};                 // ring correction terms (P-SV 6, SH 2), ring-cell order
+    void *gmq[16] = {};                // staged terms of the masked ring fields (ring neighbourhood)
     void *gbu[4][2][3] = {};           // split parts (1:2, cells), reference order
     void *gbpx[4][2] = {}, *gbpz[4][2] = {};
     double *gc0 = nullptr, *gc1 = nullptr;
@@ -902,8 +903,9 @@ GenArgs<T> gen_args(fdfk_solver *s)
     a.top_ring = s->bg_top_valid;
     for (int i = 0; i < 16; ++i) a.bg[i] = (const T *)s->gbg[i];
     a.it = s->bg_period > 0 ? s->it % s->bg_period : 0;
-    for (int i = 0; i < 8; ++i) a.st[i] = (T *)s->gst[i];
+    for (int i = 0; i < 12; ++i) a.st[i] = (T *)s->gst[i];
     for (int i = 0; i < 6; ++i) a.rq[i] = (T *)s->grq[i];
+    for (int i = 0; i < 16; ++i) a.mq[i] = (T *)s->gmq[i];
     a.ex0 = 1; a.ex1 = 0; a.ez0 = 1; a.ez1 = 0;   // staged over every cell
     return a;
 }
@@ -972,7 +974,6 @@ int gen_step_t(fdfk_solver *s, bool record)
     int rb[8];
     gring_box(g, a.top_ring, rb);
     const long nring = s->gring[0] ? gbox_count(rb[0], rb[1], rb[2], rb[3], rb[4], rb[5], rb[6], rb[7]) : 0;
-    const long npsi = 2L * g.npml * g.nz + (s->zt != 1 ? 2L : 1L) * g.NX * g.npml;
     long nc = (long)g.NX * g.NZ;
     if constexpr (sizeof(T) == 4) {
         if (s->gtiled) {
@@ -985,7 +986,7 @@ int gen_step_t(fdfk_solver *s, bool record)
         }
     }
     const long nst = gbox_count(1 - g.npml, g.nx + g.npml, s->zt, s->zt + g.NZ - 1, a.ex0, a.ex1, a.ez0, a.ez1);
-    const long np = nrec + nring + npsi + nst;
+    const long np = nrec + nring + nst;   // the memories are updated by the staging threads of strip cells
     auto interior = [&]() {
         if constexpr (sizeof(T) == 4) {
             if (!s->gtiled) return;
@@ -1012,21 +1013,31 @@ int gen_step_t(fdfk_solver *s, bool record)
     hipStream_t sc = fork ? s->stream2 : s->stream;
     if (s->sh)
         hipLaunchKernelGGL(k_gen_prep_sh<T>, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s->stream, a, r, nrec,
-                           nring, npsi, (T *)s->gring[0]);
+                           nring, (T *)s->gring[0]);
     else
         hipLaunchKernelGGL(k_gen_prep<T>, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s->stream, a, r, nrec,
-                           nring, npsi, (T *)s->gring[0], (T *)s->gring[1]);
+                           nring, (T *)s->gring[0], (T *)s->gring[1]);
     if (fork) {
         HIPCHK(hipEventRecord(s->ev_fork, s->stream));
         HIPCHK(hipStreamWaitEvent(s->stream2, s->ev_fork, 0));
         interior();
+    }
+    long nm = 0;   // cells of the ring's neighbourhood (P-SV's staged masked terms)
+    if (nq && !s->sh) {
+        const int no = g.norder;
+        nm = gbox_count(std::max(rb[0] - no, 1 - g.npml), std::min(rb[1] + no, g.nx + g.npml),
+                        std::max(rb[2] - no, s->zt), std::min(rb[3] + no, s->zt + g.NZ - 1), rb[4] + no, rb[5] - no,
+                        rb[6] + no, rb[7] - no);
     }
     if (s->sh) {
         if (nq) hipLaunchKernelGGL(k_gen_ringq_sh<T>, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, sc, a);
         if (!fork) interior();
         hipLaunchKernelGGL(k_gen_step_sh<T>, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, sc, a);
     } else {
-        if (nq) hipLaunchKernelGGL(k_gen_ringq<T>, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, sc, a);
+        if (nq) {
+            hipLaunchKernelGGL(k_gen_ringst<T>, dim3((unsigned)((nm + 255) / 256)), dim3(256), 0, sc, a);
+            hipLaunchKernelGGL(k_gen_ringq<T>, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, sc, a);
+        }
         if (!fork) interior();
         hipLaunchKernelGGL(k_gen_step<T>, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, sc, a);
     }
@@ -1295,8 +1306,12 @@ static int gen_create(const fdfk_job *job, fdfk_solver **out)
                            (const T *)s->glam, (const T *)s->gmu, m0, 1, (T *)s->ghx[0], (T *)s->ghz[0],
                            (T *)s->ghx[1], (T *)s->ghz[1]);
         // staged terms of level n (written by every prep launch before they are read)
-        for (int i = 0; i < (s->sh ? 2 : 8); ++i)
+        for (int i = 0; i < (s->sh ? 4 : 12); ++i)
             if ((rc = s->dalloc((T **)&s->gst[i], nf))) return bad(rc);
+        // the weighted memories are written on strip cells only: 0 elsewhere
+        for (int i = s->sh ? 2 : 8; i < (s->sh ? 4 : 12); ++i)
+            if (hipMemsetAsync(s->gst[i], 0, nf * sizeof(T), s->stream) != hipSuccess)
+                return bad(fail(FDFK_EDEVICE, "memset"));
     }
     if constexpr (sizeof(T) == 4) {
         // the tiled interior: columns 1..nx, rows 2*norder..nz (full-order
@@ -1605,6 +1620,16 @@ static int alloc_background(fdfk_solver *s, int period)
             const size_t nq = (size_t)gbox_count(b[0], b[1], b[2], b[3], b[4], b[5], b[6], b[7]);
             for (int i = 0; i < (s->sh ? 2 : 6); ++i)
                 if ((rc = s->dalloc((char **)&s->grq[i], nq * (size_t)s->gsize))) return rc;
+            // staged masked terms over the ring's neighbourhood (the box grown
+            // and the hole shrunk by norder, within the grid)
+            const Geo &g = s->g;
+            const int no = g.norder;
+            const size_t nm = (size_t)gbox_count(std::max(b[0] - no, 1 - g.npml), std::min(b[1] + no, g.nx + g.npml),
+                                                 std::max(b[2] - no, s->zt), std::min(b[3] + no, s->zt + g.NZ - 1),
+                                                 b[4] + no, b[5] - no, b[6] + no, b[7] - no);
+            for (int k = 0; k < 2 && !s->sh; ++k)
+                for (int i = 0; i < 8; ++i)
+                    if ((rc = s->dalloc((char **)&s->gmq[8 * k + i], nm * (size_t)s->gsize))) return rc;
         }
     }
     HIPCHK(hipStreamSynchronize(s->stream));

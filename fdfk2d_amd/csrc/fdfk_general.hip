@@ -21,7 +21,8 @@
 // formulation bit for bit and T = double its FP64 build.
 //
 // Launches per step: k_gen_prep (receivers, hybrid ring values, PML
-// memories and the staged operator terms, all from level n), k_gen_ringq
+// memories and the staged operator terms, all from level n), for P-SV
+// k_gen_ringst (the staged terms of the masked ring fields), k_gen_ringq
 // (the hybrid correction terms of the ring cells), then k_gen_step (every
 // cell of level n+1, or, in FP32, the cells around the deep interior
 // rectangle that the tiled interior kernel updates in the same step).
@@ -63,13 +64,20 @@ struct GenArgs {
     // difference, and the undivided collocated first derivatives the mixed
     // operators take.  P-SV: 0 x-face ux [lam+2mu], 1 z-face ux [mu],
     // 2 x-face uz [mu], 3 z-face uz [lam+2mu], 4 dz ux, 5 dx ux, 6 dz uz,
-    // 7 dx uz; SH: 0 x-face uy [mu], 1 z-face uy [mu].
-    T *st[8];
+    // 7 dx uz, 8..11 weighted memories (gstage); SH: 0 x-face uy [mu], 1
+    // z-face uy [mu], 2..3 weighted memories.
+    T *st[12];
     int ex0, ex1, ez0, ez1;
     // hybrid-ring correction terms of level n (the operator parts of the
     // masked ring background, P-SV 6, SH 2) in the ring cells' order of
     // gring_box, formed by a launch of their own before the step
     T *rq[6];
+    // P-SV: staged terms of the masked ring field (the other side's values
+    // only) over the ring's neighbourhood (gring_region, compact): planes
+    // 0..7 of the padding's values (for cells of the total-field region),
+    // 8..15 of the total-field region's (for padding cells).  SH evaluates
+    // its two masked operators directly (cheaper than staging them).
+    T *mq[16];
 };
 
 template <class T>
@@ -388,76 +396,83 @@ __device__ T gsz(const GenArgs<T> &a, const F &u, int p, int ix, int n)
     for (int j = 1; j <= n; ++j) s = gacc(s, gco(a.c0, a.g.norder, j, n), (T)(u(p + j, ix) - u(p - j + 1, ix)));
     return s;
 }
-// gx2 from the staged x-face values of plane arr (field u, modulus mode)
+// Indexers of staged planes: full planes in cell order, or compact planes
+// over a box minus a hole (the ring's neighbourhood)
 template <class T>
-__device__ T gx2s(const GenArgs<T> &a, int mode, int arr, const T *u, int iz, int ix)
+struct GIdxCell {
+    const GenArgs<T> *a;
+    __device__ __forceinline__ size_t operator()(int iz, int ix) const { return gcell(*a, iz, ix); }
+};
+struct GIdxBox {
+    int b[8];
+    __device__ __forceinline__ long operator()(int iz, int ix) const
+    {
+        return gbox_index(iz, ix, b[0], b[1], b[2], b[3], b[4], b[5], b[6], b[7]);
+    }
+};
+// gx2 from the staged x-face values F of field u (modulus mode); F at
+// idx(cell) holds the face value of the face (cell, next cell)
+template <class T, class I, class U>
+__device__ T gx2s(const GenArgs<T> &a, int mode, const T *F, const I &idx, const U &u, int iz, int ix)
 {
 #pragma clang fp contract(off)
     const int norder = a.g.norder, nx = a.g.nx, npml = a.g.npml;
     const int nox = gmin(gmin(ix + npml - 1, nx + npml - ix), norder);
-    const T *F = a.st[arr];
     T s = (T)0.0f;
     for (int k = 1; k <= nox; ++k) {
         const int h = ix - k, nh = gnx(a, h + 1);
-        // the face value of plane arr has gnx(h) terms; near the padded
-        // edges the s2 sum takes another count and is formed here
-        const T pb = nh == gnx(a, h) ? F[gcell(a, iz, h)]
-                                     : a.hx[mode][gcell(a, iz, h)] * gsx(a, GField<T>{&a, u}, iz, h, nh);
-        s = gacc(s, gco(a.c0, norder, k, nox), (T)(F[gcell(a, iz, ix + k - 1)] - pb));
+        // the face value holds gnx(h) terms; near the padded edges the s2
+        // sum takes another count and is formed here
+        const T pb = nh == gnx(a, h) ? F[idx(iz, h)] : a.hx[mode][gcell(a, iz, h)] * gsx(a, u, iz, h, nh);
+        s = gacc(s, gco(a.c0, norder, k, nox), (T)(F[idx(iz, ix + k - 1)] - pb));
     }
     const bool in_reg = iz >= 1 && iz <= a.g.nz && ix >= 1 && ix <= nx;
     const T dxs = (T)a.g.dx;
     const T dx2 = in_reg ? (T)(1.0 / (double)(dxs * dxs)) : (T)1.0f / (dxs * dxs);
     return s * dx2;
 }
-template <class T>
-__device__ T gz2s(const GenArgs<T> &a, int mode, int arr, const T *u, int iz, int ix)
+template <class T, class I, class U>
+__device__ T gz2s(const GenArgs<T> &a, int mode, const T *F, const I &idx, const U &u, int iz, int ix)
 {
 #pragma clang fp contract(off)
     const int norder = a.g.norder, nz = a.g.nz, npml = a.g.npml, zt = a.zt;
     const int noz = gmin(gmin(iz - zt, nz + npml - iz), norder);
-    const T *F = a.st[arr];
     T s = (T)0.0f;
     for (int k = 1; k <= noz; ++k) {
         const int h = iz - k, nh = gnz(a, h + 1);
-        const T pb = nh == gnz(a, h) ? F[gcell(a, h, ix)]
-                                     : a.hz[mode][gcell(a, h, ix)] * gsz(a, GField<T>{&a, u}, h, ix, nh);
-        s = gacc(s, gco(a.c0, norder, k, noz), (T)(F[gcell(a, iz + k - 1, ix)] - pb));
+        const T pb = nh == gnz(a, h) ? F[idx(h, ix)] : a.hz[mode][gcell(a, h, ix)] * gsz(a, u, h, ix, nh);
+        s = gacc(s, gco(a.c0, norder, k, noz), (T)(F[idx(iz + k - 1, ix)] - pb));
     }
     const T dzs = (T)a.g.dz;
     return s * ((T)1.0f / (dzs * dzs));
 }
-// gzx / gxz from the staged undivided first derivatives of plane arr
-template <class T>
-__device__ T gzxs(const GenArgs<T> &a, const T *mat, int arr, int iz, int ix)
+// gzx / gxz from the staged undivided first derivatives D
+template <class T, class I>
+__device__ T gzxs(const GenArgs<T> &a, const T *mat, const T *D, const I &idx, int iz, int ix)
 {
 #pragma clang fp contract(off)
     const int norder = a.g.norder;
     const int nox = gmin(gmin(ix + a.g.npml - 1, a.g.nx + a.g.npml - ix), norder);
     const T dx = (T)a.g.dx, dz = (T)a.g.dz;
-    const T *D = a.st[arr];
     T s = (T)0.0f;
     for (int k = 1; k <= nox; ++k) {
-        const size_t cp = gcell(a, iz, ix + k), cm = gcell(a, iz, ix - k);
-        const T vp = mat[cp] * D[cp] / dz;
-        const T vm = mat[cm] * D[cm] / dz;
+        const T vp = mat[gcell(a, iz, ix + k)] * D[idx(iz, ix + k)] / dz;
+        const T vm = mat[gcell(a, iz, ix - k)] * D[idx(iz, ix - k)] / dz;
         s = gacc(s, gco(a.c1, norder, k, nox), (T)(vp - vm));
     }
     return s / dx;
 }
-template <class T>
-__device__ T gxzs(const GenArgs<T> &a, const T *mat, int arr, int iz, int ix)
+template <class T, class I>
+__device__ T gxzs(const GenArgs<T> &a, const T *mat, const T *D, const I &idx, int iz, int ix)
 {
 #pragma clang fp contract(off)
     const int norder = a.g.norder;
     const int noz = gmin(gmin(iz - a.zt, a.g.nz + a.g.npml - iz), norder);
     const T dx = (T)a.g.dx, dz = (T)a.g.dz;
-    const T *D = a.st[arr];
     T s = (T)0.0f;
     for (int k = 1; k <= noz; ++k) {
-        const size_t cp = gcell(a, iz + k, ix), cm = gcell(a, iz - k, ix);
-        const T vp = mat[cp] * D[cp] / dx;
-        const T vm = mat[cm] * D[cm] / dx;
+        const T vp = mat[gcell(a, iz + k, ix)] * D[idx(iz + k, ix)] / dx;
+        const T vm = mat[gcell(a, iz - k, ix)] * D[idx(iz - k, ix)] / dx;
         s = gacc(s, gco(a.c1, norder, k, noz), (T)(vp - vm));
     }
     return s / dz;
@@ -469,7 +484,36 @@ __device__ __forceinline__ bool gstage_cell(const GenArgs<T> &a, long t, int &iz
     const Geo &g = a.g;
     return gbox_cell(t, 1 - g.npml, g.nx + g.npml, a.zt, a.zt + g.NZ - 1, a.ex0, a.ex1, a.ez0, a.ez1, iz, ix);
 }
-// the staged terms of one cell from level n
+// the staged terms of one cell (iz, ix) of fields ux, uz into planes P at o
+template <class T, class U>
+__device__ void gstage_terms(const GenArgs<T> &a, const U &ux, const U &uz, int iz, int ix, T *const *P, size_t o)
+{
+#pragma clang fp contract(off)
+    const size_t c = gcell(a, iz, ix);
+    const int nfx = gnx(a, ix), nfz = gnz(a, iz);
+    P[0][o] = a.hx[0][c] * gsx(a, ux, iz, ix, nfx);
+    P[1][o] = a.hz[1][c] * gsz(a, ux, iz, ix, nfz);
+    P[2][o] = a.hx[1][c] * gsx(a, uz, iz, ix, nfx);
+    P[3][o] = a.hz[0][c] * gsz(a, uz, iz, ix, nfz);
+    P[4][o] = gcdz(a, ux, iz, ix);
+    P[5][o] = gcdx(a, ux, iz, ix);
+    P[6][o] = gcdz(a, uz, iz, ix);
+    P[7][o] = gcdx(a, uz, iz, ix);
+}
+template <class T, class U>
+__device__ void gstage_terms_sh(const GenArgs<T> &a, const U &u, int iz, int ix, T *const *P, size_t o)
+{
+#pragma clang fp contract(off)
+    const size_t c = gcell(a, iz, ix);
+    P[0][o] = a.hx[1][c] * gsx(a, u, iz, ix, gnx(a, ix));
+    P[1][o] = a.hz[1][c] * gsz(a, u, iz, ix, gnz(a, iz));
+}
+// The staged terms of level n at cell t of the staged region, and on a
+// strip cell the update of its PML memories (the first derivatives of
+// level n, the formulation's phase A); every cell then stores its memories
+// weighted by the modulus the memory corrections of the step apply (0 off
+// the strips): P-SV planes 8 (lam+2mu) psi_x[ux], 9 mu psi_x[uz], 10 mu
+// psi_z[ux], 11 (lam+2mu) psi_z[uz]; SH planes 2 mu psi_x, 3 mu psi_z.
 template <class T>
 __device__ void gstage(const GenArgs<T> &a, long t)
 {
@@ -477,16 +521,35 @@ __device__ void gstage(const GenArgs<T> &a, long t)
     int iz, ix;
     if (!gstage_cell(a, t, iz, ix)) return;
     const size_t c = gcell(a, iz, ix);
-    const GField<T> fux{&a, a.ux}, fuz{&a, a.uz};
-    const int nfx = gnx(a, ix), nfz = gnz(a, iz);
-    a.st[0][c] = a.hx[0][c] * gsx(a, fux, iz, ix, nfx);
-    a.st[1][c] = a.hz[1][c] * gsz(a, fux, iz, ix, nfz);
-    a.st[2][c] = a.hx[1][c] * gsx(a, fuz, iz, ix, nfx);
-    a.st[3][c] = a.hz[0][c] * gsz(a, fuz, iz, ix, nfz);
-    a.st[4][c] = gcdz(a, fux, iz, ix);
-    a.st[5][c] = gcdx(a, fux, iz, ix);
-    a.st[6][c] = gcdz(a, fuz, iz, ix);
-    a.st[7][c] = gcdx(a, fuz, iz, ix);
+    gstage_terms(a, GField<T>{&a, a.ux}, GField<T>{&a, a.uz}, iz, ix, a.st, c);
+    const Geo &g = a.g;
+    const int s = gstrip(a, iz, ix);
+    // off the strips the weighted memories stay 0 from set-up
+    if (s >= 0) {
+        T w[4] = {(T)0.0f, (T)0.0f, (T)0.0f, (T)0.0f};
+        const size_t o = gstrip_off(a, s, iz, ix);
+        const int jx = ix + g.npml - 1, jz = iz - a.zt;
+        T *px0 = a.bpx[s][0], *px1 = a.bpx[s][1];
+        if (a.dpx[jx] > (T)0.0f) {
+            const T b = a.bx[jx], cc = (T)1.0f - b;
+            px0[o] = b * px0[o] + cc * (a.st[5][c] / (T)g.dx);
+            px1[o] = b * px1[o] + cc * (a.st[7][c] / (T)g.dx);
+        }
+        const T ma = a.lam[c] + (T)2.0f * a.mu[c], mu = a.mu[c];
+        w[0] = ma * px0[o];
+        w[1] = mu * px1[o];
+        if (T *pz0 = a.bpz[s][0]) {
+            T *pz1 = a.bpz[s][1];
+            if (a.dpz[jz] > (T)0.0f) {
+                const T b = a.bz[jz], cc = (T)1.0f - b;
+                pz0[o] = b * pz0[o] + cc * (a.st[4][c] / (T)g.dz);
+                pz1[o] = b * pz1[o] + cc * (a.st[6][c] / (T)g.dz);
+            }
+            w[2] = mu * pz0[o];
+            w[3] = ma * pz1[o];
+        }
+        for (int k = 0; k < 4; ++k) a.st[8 + k][c] = w[k];
+    }
 }
 template <class T>
 __device__ void gstage_sh(const GenArgs<T> &a, long t)
@@ -496,8 +559,29 @@ __device__ void gstage_sh(const GenArgs<T> &a, long t)
     if (!gstage_cell(a, t, iz, ix)) return;
     const size_t c = gcell(a, iz, ix);
     const GField<T> fu{&a, a.ux};
-    a.st[0][c] = a.hx[1][c] * gsx(a, fu, iz, ix, gnx(a, ix));
-    a.st[1][c] = a.hz[1][c] * gsz(a, fu, iz, ix, gnz(a, iz));
+    gstage_terms_sh(a, fu, iz, ix, a.st, c);
+    const Geo &g = a.g;
+    const int s = gstrip(a, iz, ix);
+    if (s >= 0) {   // off the strips the weighted memories stay 0 from set-up
+        T wx = (T)0.0f, wz = (T)0.0f;
+        const size_t o = gstrip_off(a, s, iz, ix);
+        const int jx = ix + g.npml - 1, jz = iz - a.zt;
+        T *px = a.bpx[s][0];
+        if (a.dpx[jx] > (T)0.0f) {
+            const T b = a.bx[jx], cc = (T)1.0f - b;
+            px[o] = b * px[o] + cc * (gcdx(a, fu, iz, ix) / (T)g.dx);
+        }
+        wx = a.mu[c] * px[o];
+        if (T *pz = a.bpz[s][0]) {
+            if (a.dpz[jz] > (T)0.0f) {
+                const T b = a.bz[jz], cc = (T)1.0f - b;
+                pz[o] = b * pz[o] + cc * (gcdz(a, fu, iz, ix) / (T)g.dz);
+            }
+            wz = a.mu[c] * pz[o];
+        }
+        a.st[2][c] = wx;
+        a.st[3][c] = wz;
+    }
 }
 
 // the six RHS parts {Xx, Zx, Mx, Xz, Zz, Mz} at one cell; the free-surface
@@ -538,13 +622,13 @@ __device__ void gparts(const GenArgs<T> &a, const F &ux, const F &uz, int iz, in
     p[4] = gz2(a, 0, uz, iz, ix);
     p[5] = gzx(a, a.mu, ux, iz, ix) + gxz(a, a.lam, ux, iz, ix);
 }
-// gparts of the level-n fields from the staged terms (same values)
-template <class T>
-__device__ void gparts_st(const GenArgs<T> &a, int iz, int ix, T p[6])
+// gparts of fields ux, uz from their staged terms P (same values)
+template <class T, class I, class U>
+__device__ void gparts_st(const GenArgs<T> &a, const T *const *P, const I &idx, const U &ux, const U &uz, int iz,
+                          int ix, T p[6])
 {
 #pragma clang fp contract(off)
     if (iz == 1 && a.zt == 1) {
-        const GField<T> ux{&a, a.ux}, uz{&a, a.uz};
         const size_t ca = gcell(a, 1, ix), cb = gcell(a, 2, ix);
         const T dz = (T)a.g.dz, dx = (T)a.g.dx;
         const T sdz2 = (T)2.0f / (dz * dz);
@@ -559,33 +643,37 @@ __device__ void gparts_st(const GenArgs<T> &a, int iz, int ix, T p[6])
             sm = gacc(sm, gco(a.c1, norder, k, nox), (T)(mp * (ux(2, ix + k) - ux(1, ix + k)) - mm * (ux(2, ix - k) - ux(1, ix - k))));
         }
         const T lbar = (T)0.5f * (a.lam[ca] + a.lam[cb]), mbar = (T)0.5f * (a.mu[ca] + a.mu[cb]);
-        const T dxuz = (a.st[7][ca] + a.st[7][cb]) / dx;
-        const T dxux = (a.st[5][ca] + a.st[5][cb]) / dx;
-        p[0] = gx2s(a, 0, 0, a.ux, iz, ix);
+        const T dxuz = (P[7][idx(1, ix)] + P[7][idx(2, ix)]) / dx;
+        const T dxux = (P[5][idx(1, ix)] + P[5][idx(2, ix)]) / dx;
+        p[0] = gx2s(a, 0, P[0], idx, ux, iz, ix);
         p[1] = (a.hz[1][ca] * (ux(2, ix) - ux(1, ix))) * sdz2;
         p[2] = (sl / dx + mbar * dxuz) / dz;
-        p[3] = gx2s(a, 1, 2, a.uz, iz, ix);
+        p[3] = gx2s(a, 1, P[2], idx, uz, iz, ix);
         p[4] = (a.hz[0][ca] * (uz(2, ix) - uz(1, ix))) * sdz2;
         p[5] = (sm / dx + lbar * dxux) / dz;
         return;
     }
-    p[0] = gx2s(a, 0, 0, a.ux, iz, ix);
-    p[1] = gz2s(a, 1, 1, a.ux, iz, ix);
-    p[2] = gzxs(a, a.lam, 6, iz, ix) + gxzs(a, a.mu, 7, iz, ix);
-    p[3] = gx2s(a, 1, 2, a.uz, iz, ix);
-    p[4] = gz2s(a, 0, 3, a.uz, iz, ix);
-    p[5] = gzxs(a, a.mu, 4, iz, ix) + gxzs(a, a.lam, 5, iz, ix);
+    p[0] = gx2s(a, 0, P[0], idx, ux, iz, ix);
+    p[1] = gz2s(a, 1, P[1], idx, ux, iz, ix);
+    p[2] = gzxs(a, a.lam, P[6], idx, iz, ix) + gxzs(a, a.mu, P[7], idx, iz, ix);
+    p[3] = gx2s(a, 1, P[2], idx, uz, iz, ix);
+    p[4] = gz2s(a, 0, P[3], idx, uz, iz, ix);
+    p[5] = gzxs(a, a.mu, P[4], idx, iz, ix) + gxzs(a, a.lam, P[5], idx, iz, ix);
 }
-
-// memory of strip cell (iz, ix) or 0 outside the strips / where a strip
-// keeps none
-template <class T>
-__device__ __forceinline__ T gpsi(const GenArgs<T> &a, int dir, int comp, int iz, int ix)
+// gparts_sh from the staged terms
+template <class T, class I, class U>
+__device__ void gparts_sh_st(const GenArgs<T> &a, const T *const *P, const I &idx, const U &u, int iz, int ix, T &X,
+                             T &Z)
 {
-    const int s = gstrip(a, iz, ix);
-    if (s < 0 || (s == 3 && !a.top)) return (T)0.0f;
-    const T *p = dir == 0 ? a.bpx[s][comp] : a.bpz[s][comp];
-    return p ? p[gstrip_off(a, s, iz, ix)] : (T)0.0f;
+#pragma clang fp contract(off)
+    X = gx2s(a, 1, P[0], idx, u, iz, ix);
+    if (iz == 1 && a.zt == 1) {
+        const size_t ca = gcell(a, 1, ix);
+        const T dz = (T)a.g.dz;
+        Z = (a.hz[1][ca] * (u(2, ix) - u(1, ix))) * ((T)2.0f / (dz * dz));
+    } else {
+        Z = gz2s(a, 1, P[1], idx, u, iz, ix);
+    }
 }
 
 // ---- per-step preparation: receivers (level n), ring values, memories ----
@@ -644,13 +732,14 @@ __device__ void grecord(const GenArgs<T> &a, const GenRec<T> &r, int i)
     if (NC == 2) r.seisz[(size_t)i * r.nsamp + r.istp] = sz;
 }
 
-// one thread per receiver, per ring cell and per strip cell: threads
-// [0, nrec) record, [nrec, nrec + nring) write the ring values of the cells
-// of gring_box (the other cells of the ring fields stay zero from their
-// allocation), the rest update the memories of strip cells
+// one thread per receiver, per ring cell and per cell of the staged region:
+// threads [0, nrec) record, [nrec, nrec + nring) write the ring values of
+// the cells of gring_box (the other cells of the ring fields stay zero from
+// their allocation), the rest stage the terms of level n (gstage: a strip
+// cell also updates its memories)
 template <class T>
-__global__ __launch_bounds__(256) void k_gen_prep(GenArgs<T> a, GenRec<T> r, long nrec, long nring, long npsi,
-                                                   T *ringx, T *ringz)
+__global__ __launch_bounds__(256) void k_gen_prep(GenArgs<T> a, GenRec<T> r, long nrec, long nring, T *ringx,
+                                                   T *ringz)
 {
 #pragma clang fp contract(off)
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -669,32 +758,7 @@ __global__ __launch_bounds__(256) void k_gen_prep(GenArgs<T> a, GenRec<T> r, lon
         return;
     }
     t -= nring;
-    if (t >= npsi) { gstage(a, t - npsi); return; }
-    // strip cells: L, R (npml x nz each), B (NX x npml), T (NX x npml)
-    const long nlr = (long)g.npml * g.nz, nb = (long)g.NX * g.npml;
-    int s, iz, ix;
-    if (t < nlr) { s = 0; ix = 1 - g.npml + (int)(t / g.nz); iz = 1 + (int)(t % g.nz); }
-    else if (t < 2 * nlr) { t -= nlr; s = 1; ix = g.nx + 1 + (int)(t / g.nz); iz = 1 + (int)(t % g.nz); }
-    else if (t < 2 * nlr + nb) { t -= 2 * nlr; s = 2; ix = 1 - g.npml + (int)(t / g.npml); iz = g.nz + 1 + (int)(t % g.npml); }
-    else if (a.top && t < 2 * nlr + 2 * nb) { t -= 2 * nlr + nb; s = 3; ix = 1 - g.npml + (int)(t / g.npml); iz = 1 - g.npml + (int)(t % g.npml); }
-    else return;
-    const size_t o = gstrip_off(a, s, iz, ix);
-    const GField<T> fx{&a, a.ux}, fz{&a, a.uz};
-    const int jx = ix + g.npml - 1, jz = iz - a.zt;
-    if (a.dpx[jx] > (T)0.0f) {
-        const T b = a.bx[jx], c = (T)1.0f - b;
-        const T gx = gcdx(a, fx, iz, ix) / (T)g.dx;
-        const T gz = gcdx(a, fz, iz, ix) / (T)g.dx;
-        a.bpx[s][0][o] = b * a.bpx[s][0][o] + c * gx;
-        a.bpx[s][1][o] = b * a.bpx[s][1][o] + c * gz;
-    }
-    if (a.bpz[s][0] && a.dpz[jz] > (T)0.0f) {
-        const T b = a.bz[jz], c = (T)1.0f - b;
-        const T gx = gcdz(a, fx, iz, ix) / (T)g.dz;
-        const T gz = gcdz(a, fz, iz, ix) / (T)g.dz;
-        a.bpz[s][0][o] = b * a.bpz[s][0][o] + c * gx;
-        a.bpz[s][1][o] = b * a.bpz[s][1][o] + c * gz;
-    }
+    gstage(a, t);
 }
 
 // one thread per cell of the padded grid: level n+1
@@ -714,7 +778,7 @@ __global__ __launch_bounds__(256) void k_gen_step(GenArgs<T> a)
         return;
     }
     T p[6];
-    gparts_st(a, iz, ix, p);
+    gparts_st(a, a.st, GIdxCell<T>{&a}, GField<T>{&a, a.ux}, GField<T>{&a, a.uz}, iz, ix, p);
     const int s = gstrip(a, iz, ix);
     // PML memory corrections (nonzero only next to damped cells)
     {
@@ -723,19 +787,19 @@ __global__ __launch_bounds__(256) void k_gen_step(GenArgs<T> a)
         const int nox = gmin(gmin(ix + g.npml - 1, g.nx + g.npml - ix), norder);
         const int noz = gmin(gmin(iz - a.zt, g.nz + g.npml - iz), norder);
         T sx0 = (T)0.0f, sx1 = (T)0.0f, sz0 = (T)0.0f, sz1 = (T)0.0f;
-        for (int k = 1; k <= nox; ++k) {
+        const T *W0 = a.st[8], *W1 = a.st[9], *W2 = a.st[10], *W3 = a.st[11];
+        // a sum whose cells are all off the strips is a sum of zeros
+        const bool xs = iz < 1 || iz > g.nz || ix - nox < 1 || ix + nox > g.nx;
+        const bool zs = ix < 1 || ix > g.nx || iz - noz < 1 || iz + noz > g.nz;
+        for (int k = 1; xs && k <= nox; ++k) {
             const size_t ca = gcell(a, iz, ix + k), cb = gcell(a, iz, ix - k);
-            const T ma = a.lam[ca] + (T)2.0f * a.mu[ca], mb = a.lam[cb] + (T)2.0f * a.mu[cb];
-            sx0 = gacc(sx0, gco(a.c1, norder, k, nox), (T)(ma * gpsi(a, 0, 0, iz, ix + k) - mb * gpsi(a, 0, 0, iz, ix - k)));
-            sx1 = gacc(sx1, gco(a.c1, norder, k, nox),
-                       (T)(a.mu[ca] * gpsi(a, 0, 1, iz, ix + k) - a.mu[cb] * gpsi(a, 0, 1, iz, ix - k)));
+            sx0 = gacc(sx0, gco(a.c1, norder, k, nox), (T)(W0[ca] - W0[cb]));
+            sx1 = gacc(sx1, gco(a.c1, norder, k, nox), (T)(W1[ca] - W1[cb]));
         }
-        for (int k = 1; k <= noz; ++k) {
+        for (int k = 1; zs && k <= noz; ++k) {
             const size_t ca = gcell(a, iz + k, ix), cb = gcell(a, iz - k, ix);
-            const T ma = a.lam[ca] + (T)2.0f * a.mu[ca], mb = a.lam[cb] + (T)2.0f * a.mu[cb];
-            sz0 = gacc(sz0, gco(a.c1, norder, k, noz),
-                       (T)(a.mu[ca] * gpsi(a, 1, 0, iz + k, ix) - a.mu[cb] * gpsi(a, 1, 0, iz - k, ix)));
-            sz1 = gacc(sz1, gco(a.c1, norder, k, noz), (T)(ma * gpsi(a, 1, 1, iz + k, ix) - mb * gpsi(a, 1, 1, iz - k, ix)));
+            sz0 = gacc(sz0, gco(a.c1, norder, k, noz), (T)(W2[ca] - W2[cb]));
+            sz1 = gacc(sz1, gco(a.c1, norder, k, noz), (T)(W3[ca] - W3[cb]));
         }
         p[0] = p[0] - sx0 / (T)g.dx;
         p[3] = p[3] - sx1 / (T)g.dx;
@@ -797,6 +861,35 @@ __device__ __forceinline__ bool gring_job(const GenArgs<T> &a, long t, int &iz, 
     if (ix < 2 - g.npml || ix > g.nx + g.npml - 1 || iz < iz_first || iz > g.nz + g.npml - 1) return false;
     return !(ix >= a.hx0 && ix <= a.hx1 && iz >= a.hz0 && iz <= a.hz1);
 }
+// The ring's neighbourhood: the ring box grown and its hole shrunk by
+// norder, within the padded grid; the ring cells' operators read no cell
+// outside it
+template <class T>
+__device__ __forceinline__ GIdxBox gring_region(const GenArgs<T> &a)
+{
+    const Geo &g = a.g;
+    int r[8];
+    gring_box(g, a.top_ring, r);
+    const int no = g.norder;
+    GIdxBox b;
+    b.b[0] = gmax(r[0] - no, 1 - g.npml); b.b[1] = gmin(r[1] + no, g.nx + g.npml);
+    b.b[2] = gmax(r[2] - no, a.zt); b.b[3] = gmin(r[3] + no, a.zt + g.NZ - 1);
+    b.b[4] = r[4] + no; b.b[5] = r[5] - no; b.b[6] = r[6] + no; b.b[7] = r[7] - no;
+    return b;
+}
+// staged terms of both masked ring fields at cell t of the ring's neighbourhood
+template <class T>
+__global__ __launch_bounds__(256) void k_gen_ringst(GenArgs<T> a)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const GIdxBox b = gring_region(a);
+    if (t >= gbox_count(b.b[0], b.b[1], b.b[2], b.b[3], b.b[4], b.b[5], b.b[6], b.b[7])) return;
+    int iz, ix;
+    gbox_cell(t, b.b[0], b.b[1], b.b[2], b.b[3], b.b[4], b.b[5], b.b[6], b.b[7], iz, ix);
+    for (int keep = 0; keep < 2; ++keep)
+        gstage_terms(a, GMasked<T>{&a, a.ringx, keep == 1}, GMasked<T>{&a, a.ringz, keep == 1}, iz, ix,
+                     a.mq + 8 * keep, (size_t)t);
+}
 template <class T>
 __global__ __launch_bounds__(256) void k_gen_ringq(GenArgs<T> a)
 {
@@ -807,7 +900,7 @@ __global__ __launch_bounds__(256) void k_gen_ringq(GenArgs<T> a)
     const bool inside = iz >= 1 && iz <= a.g.nz && ix >= 1 && ix <= a.g.nx;
     const GMasked<T> wx{&a, a.ringx, !inside}, wz{&a, a.ringz, !inside};
     T q[6];
-    gparts(a, wx, wz, iz, ix, q);
+    gparts_st(a, a.mq + (inside ? 0 : 8), gring_region(a), wx, wz, iz, ix, q);
     for (int m = 0; m < 6; ++m) a.rq[m][t] = q[m];
 }
 
@@ -846,8 +939,7 @@ __global__ __launch_bounds__(256) void k_gen_ringq_sh(GenArgs<T> a)
 }
 
 template <class T>
-__global__ __launch_bounds__(256) void k_gen_prep_sh(GenArgs<T> a, GenRec<T> r, long nrec, long nring, long npsi,
-                                                      T *ringx)
+__global__ __launch_bounds__(256) void k_gen_prep_sh(GenArgs<T> a, GenRec<T> r, long nrec, long nring, T *ringx)
 {
 #pragma clang fp contract(off)
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -864,25 +956,7 @@ __global__ __launch_bounds__(256) void k_gen_prep_sh(GenArgs<T> a, GenRec<T> r, 
         return;
     }
     t -= nring;
-    if (t >= npsi) { gstage_sh(a, t - npsi); return; }
-    const long nlr = (long)g.npml * g.nz, nb = (long)g.NX * g.npml;
-    int s, iz, ix;
-    if (t < nlr) { s = 0; ix = 1 - g.npml + (int)(t / g.nz); iz = 1 + (int)(t % g.nz); }
-    else if (t < 2 * nlr) { t -= nlr; s = 1; ix = g.nx + 1 + (int)(t / g.nz); iz = 1 + (int)(t % g.nz); }
-    else if (t < 2 * nlr + nb) { t -= 2 * nlr; s = 2; ix = 1 - g.npml + (int)(t / g.npml); iz = g.nz + 1 + (int)(t % g.npml); }
-    else if (a.top && t < 2 * nlr + 2 * nb) { t -= 2 * nlr + nb; s = 3; ix = 1 - g.npml + (int)(t / g.npml); iz = 1 - g.npml + (int)(t % g.npml); }
-    else return;
-    const size_t o = gstrip_off(a, s, iz, ix);
-    const GField<T> fu{&a, a.ux};
-    const int jx = ix + g.npml - 1, jz = iz - a.zt;
-    if (a.dpx[jx] > (T)0.0f) {
-        const T b = a.bx[jx], c = (T)1.0f - b;
-        a.bpx[s][0][o] = b * a.bpx[s][0][o] + c * (gcdx(a, fu, iz, ix) / (T)g.dx);
-    }
-    if (a.bpz[s][0] && a.dpz[jz] > (T)0.0f) {
-        const T b = a.bz[jz], c = (T)1.0f - b;
-        a.bpz[s][0][o] = b * a.bpz[s][0][o] + c * (gcdz(a, fu, iz, ix) / (T)g.dz);
-    }
+    gstage_sh(a, t);
 }
 
 template <class T>
@@ -898,28 +972,22 @@ __global__ __launch_bounds__(256) void k_gen_step_sh(GenArgs<T> a)
         a.uxo[gi] = a.ux[gi];
         return;
     }
-    T X = gx2s(a, 1, 0, a.ux, iz, ix), Z;
-    if (iz == 1 && a.zt == 1) {
-        const size_t ca = gcell(a, 1, ix);
-        const T dz = (T)g.dz;
-        Z = (a.hz[1][ca] * (a.ux[gcell(a, 2, ix)] - a.ux[ca])) * ((T)2.0f / (dz * dz));
-    } else {
-        Z = gz2s(a, 1, 1, a.ux, iz, ix);
-    }
+    T X, Z;
+    gparts_sh_st(a, a.st, GIdxCell<T>{&a}, GField<T>{&a, a.ux}, iz, ix, X, Z);
     const bool surf = iz == 1 && a.zt == 1;
     {
         const int norder = g.norder;
         const int nox = gmin(gmin(ix + g.npml - 1, g.nx + g.npml - ix), norder);
         const int noz = gmin(gmin(iz - a.zt, g.nz + g.npml - iz), norder);
         T sx = (T)0.0f, sz = (T)0.0f;
-        for (int k = 1; k <= nox; ++k) {
-            const size_t ca = gcell(a, iz, ix + k), cb = gcell(a, iz, ix - k);
-            sx = gacc(sx, gco(a.c1, norder, k, nox), (T)(a.mu[ca] * gpsi(a, 0, 0, iz, ix + k) - a.mu[cb] * gpsi(a, 0, 0, iz, ix - k)));
-        }
-        for (int k = 1; k <= noz; ++k) {
-            const size_t ca = gcell(a, iz + k, ix), cb = gcell(a, iz - k, ix);
-            sz = gacc(sz, gco(a.c1, norder, k, noz), (T)(a.mu[ca] * gpsi(a, 1, 0, iz + k, ix) - a.mu[cb] * gpsi(a, 1, 0, iz - k, ix)));
-        }
+        const T *Wx = a.st[2], *Wz = a.st[3];
+        // a sum whose cells are all off the strips is a sum of zeros
+        const bool xs = iz < 1 || iz > g.nz || ix - nox < 1 || ix + nox > g.nx;
+        const bool zs = ix < 1 || ix > g.nx || iz - noz < 1 || iz + noz > g.nz;
+        for (int k = 1; xs && k <= nox; ++k)
+            sx = gacc(sx, gco(a.c1, norder, k, nox), (T)(Wx[gcell(a, iz, ix + k)] - Wx[gcell(a, iz, ix - k)]));
+        for (int k = 1; zs && k <= noz; ++k)
+            sz = gacc(sz, gco(a.c1, norder, k, noz), (T)(Wz[gcell(a, iz + k, ix)] - Wz[gcell(a, iz - k, ix)]));
         X = X - sx / (T)g.dx;
         if (!surf) Z = Z - sz / (T)g.dz;
     }
