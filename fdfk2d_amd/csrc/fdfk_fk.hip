@@ -450,62 +450,74 @@ int next_pow2(long n)
     return p;
 }
 
-// traces of one model on the device: g[comp][q][nfft]
+// traces of one model on the device: g[comp][q][nfft].  One stream
+// synchronisation in all (after the scaling): the model, the layer of every
+// depth and its depth below the layer top go up in one packed copy from a
+// host buffer that lives until then, and the plan and its work area are
+// given back only after it.  Under a serving load every synchronisation
+// waits behind the other streams' work on the shared hardware queues.
 int fk_traces_dev(fdfk_solver *s, FKModel &m, const FKRun &r, const double *zq, double *g_out)
 {
     const int nzq = r.nzq;
-    std::vector<int> lay(nzq);
-    std::vector<double> dzl(nzq);
+    const long nkp = r.nfft / 2 + 1;
+    hipfftHandle plan;
+    size_t wbytes = 0;
+    if (int rc = plan_take(r.nfft, (int)nkp, 2 * nzq, plan, wbytes)) return rc;
+    const size_t ol = (sizeof(FKModel) + 15) / 16 * 16;
+    const size_t od = ol + ((size_t)nzq * sizeof(int) + 15) / 16 * 16;
+    std::vector<char> host(od + (size_t)nzq * sizeof(double));
+    std::memcpy(host.data(), &m, sizeof(FKModel));
+    int *lay = (int *)(host.data() + ol);
+    double *dzl = (double *)(host.data() + od);
     for (int q = 0; q < nzq; ++q) {
         int j = 0;
         while (j + 1 < m.nl && zq[q] >= m.zt[j + 1]) ++j;
         lay[q] = j;
         dzl[q] = zq[q] - m.zt[j];
     }
-    DevBuf amps, work, dlay, ddz, spec, dm;
-    const long nkp = r.nfft / 2 + 1;
-    HIPCHK(dm.alloc(sizeof(FKModel), s->stream));
-    HIPCHK(h2d(s->stream, dm.p, &m, sizeof(FKModel)));
-    HIPCHK(amps.alloc((size_t)r.nk * m.nl * 4 * sizeof(zc), s->stream));
-    HIPCHK(work.alloc((size_t)r.nk * m.nl * 12 * sizeof(zc), s->stream));
-    HIPCHK(dlay.alloc(nzq * sizeof(int), s->stream));
-    HIPCHK(ddz.alloc(nzq * sizeof(double), s->stream));
-    HIPCHK(spec.alloc((size_t)2 * nzq * nkp * sizeof(zc), s->stream));
-    HIPCHK(h2d(s->stream, dlay.p, lay.data(), nzq * sizeof(int)));
-    HIPCHK(h2d(s->stream, ddz.p, dzl.data(), nzq * sizeof(double)));
+    DevBuf up, amps, work, spec, wa;
+    hipError_t e = up.alloc(host.size(), s->stream);
+    if (e == hipSuccess) e = amps.alloc((size_t)r.nk * m.nl * 4 * sizeof(zc), s->stream);
+    if (e == hipSuccess) e = work.alloc((size_t)r.nk * m.nl * 12 * sizeof(zc), s->stream);
+    if (e == hipSuccess) e = spec.alloc((size_t)2 * nzq * nkp * sizeof(zc), s->stream);
+    if (e == hipSuccess && wbytes > 0) e = wa.alloc(wbytes, s->stream);
+    if (e != hipSuccess) {
+        plan_give(plan);
+        return fail(FDFK_ENOMEM, std::string("FK work arrays: ") + hipGetErrorString(e));
+    }
+    e = hipMemcpyAsync(up.p, host.data(), host.size(), hipMemcpyHostToDevice, s->stream);
+    if (e != hipSuccess) {
+        (void)hipStreamSynchronize(s->stream);
+        plan_give(plan);
+        return fail(FDFK_EDEVICE, std::string("FK model upload: ") + hipGetErrorString(e));
+    }
+    const FKModel *dm = (const FKModel *)up.p;
+    const int *dlay = (const int *)((char *)up.p + ol);
+    const double *ddz = (const double *)((char *)up.p + od);
     long ns = (long)nzq * nkp;
     if (r.sh) {
-        hipLaunchKernelGGL(k_fk_amp_sh, dim3((r.nk + 127) / 128), dim3(128), 0, s->stream, dm.as<FKModel>(), r,
-                           amps.as<zc>(), work.as<zc>());
-        hipLaunchKernelGGL(k_fk_spec_sh, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, s->stream,
-                           dm.as<FKModel>(), r, amps.as<zc>(), dlay.as<int>(), ddz.as<double>(), spec.as<zc>());
+        hipLaunchKernelGGL(k_fk_amp_sh, dim3((r.nk + 127) / 128), dim3(128), 0, s->stream, dm, r, amps.as<zc>(),
+                           work.as<zc>());
+        hipLaunchKernelGGL(k_fk_spec_sh, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, s->stream, dm, r,
+                           amps.as<zc>(), dlay, ddz, spec.as<zc>());
     } else {
-        hipLaunchKernelGGL(k_fk_amp, dim3((r.nk + 127) / 128), dim3(128), 0, s->stream, dm.as<FKModel>(), r,
-                           amps.as<zc>(), work.as<zc>());
-        hipLaunchKernelGGL(k_fk_spec, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, s->stream,
-                           dm.as<FKModel>(), r, amps.as<zc>(), dlay.as<int>(), ddz.as<double>(), spec.as<zc>());
-    }
-    hipfftHandle plan;
-    size_t wbytes = 0;
-    if (int rc = plan_take(r.nfft, (int)nkp, 2 * nzq, plan, wbytes)) return rc;
-    DevBuf wa;
-    if (wbytes > 0) {
-        hipError_t e = wa.alloc(wbytes, s->stream);
-        if (e != hipSuccess) {
-            plan_give(plan);
-            return fail(FDFK_ENOMEM, std::string("FFT work area: ") + hipGetErrorString(e));
-        }
+        hipLaunchKernelGGL(k_fk_amp, dim3((r.nk + 127) / 128), dim3(128), 0, s->stream, dm, r, amps.as<zc>(),
+                           work.as<zc>());
+        hipLaunchKernelGGL(k_fk_spec, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, s->stream, dm, r,
+                           amps.as<zc>(), dlay, ddz, spec.as<zc>());
     }
     hipfftSetStream(plan, s->stream);
     hipfftSetWorkArea(plan, wa.p);
-    hipfftResult fr = hipfftExecZ2D(plan, (hipfftDoubleComplex *)spec.p, (hipfftDoubleReal *)g_out);
-    hipStreamSynchronize(s->stream);
+    const hipfftResult fr = hipfftExecZ2D(plan, (hipfftDoubleComplex *)spec.p, (hipfftDoubleReal *)g_out);
+    const long tot = 2L * nzq * r.nfft;
+    if (fr == HIPFFT_SUCCESS)
+        hipLaunchKernelGGL(k_fk_scale, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s->stream, r, g_out,
+                           2L * nzq);
+    e = hipStreamSynchronize(s->stream);
     plan_give(plan);
     if (fr != HIPFFT_SUCCESS) return fail(FDFK_EDEVICE, "hipfftExecZ2D failed");
-    long tot = 2L * nzq * r.nfft;
-    hipLaunchKernelGGL(k_fk_scale, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s->stream, r, g_out,
-                       2L * nzq);
-    HIPCHK(hipStreamSynchronize(s->stream));
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e != hipSuccess) return fail(FDFK_EDEVICE, std::string("FK traces: ") + hipGetErrorString(e));
     return 0;
 }
 
