@@ -360,25 +360,35 @@ def batch_bench(njobs, seed=3, steps=1000):
     t_seq = time.perf_counter() - t0
     for s in sol:
         s.close()
-    sol = [make(r) for r in reqs]
-    B = Batch(3)
-    for s in sol:
-        B.add(s)
-    t0 = time.perf_counter()
-    while B.active:
-        B.run(100)
-    B.sync()
-    t_bat = time.perf_counter() - t0
-    for s in sol:
-        s.close()
-    B.close()
+    def batched(tune):
+        sol = [make(r) for r in reqs]
+        B = Batch(3)
+        B.autotune(tune)
+        for s in sol:
+            B.add(s)
+        t0 = time.perf_counter()
+        while B.active:
+            B.run(100)
+            width = B.tile_width
+        B.sync()
+        t = time.perf_counter() - t0
+        for s in sol:
+            s.close()
+        B.close()
+        return t, width
+
+    t_bat, w_model = batched(False)
+    t_tuned, w_tuned = batched(True)     # the tuner's exploration steps are inside the timed region
     cells = sum((r["nx"] + 40) * (r["nz"] + 20) * r["nsteps"] for r in reqs)
     return {"metric": "Gcell-updates/s", "value": round(cells / t_bat / 1e9, 3), "unit": "Gcell-updates/s",
             "higher_is_better": True, "data": "synthetic",
             "config": {"workload": f"batch of {njobs} solves (seed {seed}), nx 500..2000, nz 200..500, "
                                    f"dx 100 m, {steps} steps each"},
             "batched_s": round(t_bat, 3), "sequential_s": round(t_seq, 3),
-            "sequential_gcells": round(cells / t_seq / 1e9, 3), "speedup": round(t_seq / t_bat, 3)}
+            "sequential_gcells": round(cells / t_seq / 1e9, 3), "speedup": round(t_seq / t_bat, 3),
+            "tile_width_model": w_model,
+            "autotuned": {"batched_s": round(t_tuned, 3), "tile_width": w_tuned,
+                          "speedup": round(t_seq / t_tuned, 3)}}
 
 
 def c3_bench(njobs=64, block=512, log=lambda *a: None):

@@ -161,9 +161,26 @@ struct fdfk_batch {
     std::vector<Retired> retired;
     long tot = 0;      // batched steps queued so far
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_int, ev_step;
+    // Interior tile width of the current tables, and the online tuner
+    // (fdfk_batch_autotune): for an active set not seen before, the first
+    // batched steps run on candidate widths around the occupancy model's,
+    // each timed over kTuneSteps steps after one untimed step, and the set
+    // keeps the fastest (the model's unless another is 1% faster).  The
+    // width changes the tiling only, not a cell's arithmetic.
+    int tlx = 0;
+    bool tune = false;
+    static constexpr int kTuneSteps = 4;
+    std::vector<int> tcand;                  // candidates of the set being tuned
+    std::vector<float> tms;                  // their ms per step
+    int tk = -1, tc = 0;                     // candidate, steps on its tables
+    hipEvent_t te0 = nullptr, te1 = nullptr;
+    std::vector<long> tkey;                  // shape of the set being tuned
+    std::map<std::vector<long>, int> tuned;  // shape of an active set -> width
     ~fdfk_batch()
     {
         if (stream) hipStreamSynchronize(stream);
+        if (te0) hipEventDestroy(te0);
+        if (te1) hipEventDestroy(te1);
         if (gexec) hipGraphExecDestroy(gexec);
         if (graph) hipGraphDestroy(graph);
         for (auto &r : retired) {
@@ -271,7 +288,7 @@ void set_levels(const fdfk_solver *s, BatchJob<NO> &b, int c)
 
 // build and upload the job tables of a new active set
 template <int NO>
-int build_tables(fdfk_batch *B, const std::vector<fdfk_solver *> &act)
+int build_tables(fdfk_batch *B, const std::vector<fdfk_solver *> &act, int lx_force = 0)
 {
     constexpr size_t JB = sizeof(BatchJob<NO>);
     // last step on the old table (recorded even when add/remove cleared tact:
@@ -285,7 +302,8 @@ int build_tables(fdfk_batch *B, const std::vector<fdfk_solver *> &act)
     auto *J1 = J0 + n;
     P->n = n;
     for (int k = 0; k < KB_N; ++k) P->pre[k][0] = 0;
-    const int lx = batch_lx<NO>(act);
+    const int lx = lx_force > 0 ? lx_force : batch_lx<NO>(act);
+    B->tlx = lx;
     size_t sh = sizeof(typename IntKernel2<NO, 256>::Smem);
     for (int j = 0; j < n; ++j) {
         fdfk_solver *s = act[j];
@@ -320,6 +338,67 @@ int build_tables(fdfk_batch *B, const std::vector<fdfk_solver *> &act)
     return 0;
 }
 
+// shape of an active set for the tuner: the members' grids, sorted
+static std::vector<long> set_shape(const std::vector<fdfk_solver *> &act)
+{
+    std::vector<long> k;
+    for (auto *s : act) k.push_back(((long)s->g.nx << 32) | ((long)s->g.nz << 12) | (long)s->g.npml);
+    std::sort(k.begin(), k.end());
+    return k;
+}
+
+// Before a step: new tables for a new active set (its tuned width when
+// known, the first candidate when it is to be tuned), and the tuner's
+// events and switches between candidates
+template <int NO>
+int tune_tables(fdfk_batch *B, const std::vector<fdfk_solver *> &act)
+{
+    if (act != B->tact) {
+        B->tk = -1;
+        int lx = 0;
+        if (B->tune) {
+            std::vector<long> key = set_shape(act);
+            auto it = B->tuned.find(key);
+            if (it != B->tuned.end()) {
+                lx = it->second;
+            } else {
+                const int m = batch_lx<NO>(act);
+                B->tcand.clear();
+                for (double f : {1.0, 0.5, 0.67, 0.8, 1.25, 1.5, 2.0}) {
+                    const int c = std::min(256, std::max(4, (int)std::lround(m * f)));
+                    if (std::find(B->tcand.begin(), B->tcand.end(), c) == B->tcand.end()) B->tcand.push_back(c);
+                }
+                B->tms.assign(B->tcand.size(), 0.0f);
+                B->tk = 0;
+                B->tc = 0;
+                B->tkey = key;
+                lx = B->tcand[0];
+            }
+        }
+        return build_tables<NO>(B, act, lx);
+    }
+    if (B->tk < 0) return 0;
+    // one untimed step on the candidate's tables, then kTuneSteps timed
+    if (B->tc == 1) HIPCHK(hipEventRecord(B->te0, B->stream));
+    if (B->tc == 1 + fdfk_batch::kTuneSteps) {
+        HIPCHK(hipEventRecord(B->te1, B->stream));
+        float ms = 0.0f;
+        HIPCHK(hipEventSynchronize(B->te1));
+        HIPCHK(hipEventElapsedTime(&ms, B->te0, B->te1));
+        B->tms[B->tk] = ms / fdfk_batch::kTuneSteps;
+        B->tc = 0;
+        if (++B->tk < (int)B->tcand.size()) return build_tables<NO>(B, act, B->tcand[B->tk]);
+        size_t bk = 0;
+        for (size_t k = 1; k < B->tcand.size(); ++k)
+            if (B->tms[k] < B->tms[bk]) bk = k;
+        if (B->tms[bk] > 0.99f * B->tms[0]) bk = 0;
+        B->tuned[B->tkey] = B->tcand[bk];
+        B->tk = -1;
+        return build_tables<NO>(B, act, B->tcand[bk]);
+    }
+    return 0;
+}
+
 template <int NO>
 int batch_step(fdfk_batch *B)
 {
@@ -327,8 +406,8 @@ int batch_step(fdfk_batch *B)
     for (auto *s : B->members)
         if (s->it <= s->nt) act.push_back(s);
     if (act.empty()) return 0;
-    if (act != B->tact)
-        if (int rc = build_tables<NO>(B, act)) return rc;
+    if (int rc = tune_tables<NO>(B, act)) return rc;
+    if (B->tk >= 0) B->tc += 1;
     const int slot = B->slot;
     HIPCHK(hipEventSynchronize(B->ev[slot]));   // at most R steps ahead of the GPU: steps < tot - R + 1 are done
     for (size_t i = 0; i < B->retired.size();) {
@@ -527,6 +606,24 @@ int fdfk_batch_sync(fdfk_batch *B)
     if (!B) return fail(FDFK_EINVAL, "null batch");
     HIPCHK(hipStreamSynchronize(B->stream));
     return 0;
+}
+
+int fdfk_batch_autotune(fdfk_batch *B, int32_t enable)
+{
+    if (!B) return fail(FDFK_EINVAL, "null batch");
+    if (enable && !B->te0) {
+        if (hipEventCreate(&B->te0) != hipSuccess || hipEventCreate(&B->te1) != hipSuccess)
+            return fail(FDFK_EDEVICE, "tuner events");
+    }
+    B->tune = enable != 0;
+    if (!B->tune) B->tk = -1;
+    return 0;
+}
+
+int fdfk_batch_tile_width(const fdfk_batch *B)
+{
+    if (!B) return FDFK_EINVAL;
+    return B->tlx;
 }
 
 int fdfk_batch_profile(fdfk_batch *B, int32_t enable)

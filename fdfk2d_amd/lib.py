@@ -70,8 +70,9 @@ def lib():
         L.fdfk_batch_create.argtypes = [C.c_int32, C.POINTER(C.c_void_p)]
         for name in ("fdfk_batch_add", "fdfk_batch_remove"):
             getattr(L, name).argtypes = [C.c_void_p, C.c_void_p]
-        for name in ("fdfk_batch_run", "fdfk_batch_profile"):
+        for name in ("fdfk_batch_run", "fdfk_batch_profile", "fdfk_batch_autotune"):
             getattr(L, name).argtypes = [C.c_void_p, C.c_int32]
+        L.fdfk_batch_tile_width.argtypes = [C.c_void_p]
         for name in ("fdfk_batch_active", "fdfk_batch_sync", "fdfk_batch_destroy"):
             getattr(L, name).argtypes = [C.c_void_p]
         L.fdfk_batch_destroy.restype = None
@@ -316,6 +317,15 @@ class Batch:
 
     def sync(self):
         _check(lib().fdfk_batch_sync(self.h))
+
+    def autotune(self, on=True):
+        """Time candidate interior tile widths on the first steps of every
+        active set of a new shape and keep the fastest (fdfk_batch_autotune)."""
+        _check(lib().fdfk_batch_autotune(self.h, 1 if on else 0))
+
+    @property
+    def tile_width(self):
+        return lib().fdfk_batch_tile_width(self.h)
 
     def profile(self, on=True):
         _check(lib().fdfk_batch_profile(self.h, 1 if on else 0))

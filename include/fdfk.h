@@ -180,6 +180,17 @@ int fdfk_batch_run(fdfk_batch *b, int32_t nsteps);
 /* number of members with iterations left */
 int fdfk_batch_active(const fdfk_batch *b);
 int fdfk_batch_sync(fdfk_batch *b);
+/* Online tile-width tuner of the batched steps (no counterpart in the
+ * reference, whose steps are not batched): with enable != 0, an active set
+ * of a shape (the members' grids) not seen before by this batch runs its
+ * first batched steps on candidate interior tile widths around the
+ * occupancy model's, timing each over 4 steps after one untimed step, and
+ * then keeps the fastest; a set of a known shape takes its tuned width at
+ * once.  Results do not depend on the width.  Off by default: a server
+ * whose active set changes every few steps would spend its steps tuning. */
+int fdfk_batch_autotune(fdfk_batch *b, int32_t enable);
+/* interior tile width of the batch's current tables (0 before its first step) */
+int fdfk_batch_tile_width(const fdfk_batch *b);
 int fdfk_batch_profile(fdfk_batch *b, int32_t enable);
 int fdfk_batch_kernel_stats(fdfk_batch *b, double *interior_ms_total, int64_t *interior_launches,
                             double *step_ms_total, int64_t *steps);

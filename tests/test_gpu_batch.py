@@ -58,6 +58,45 @@ def test_batch_equals_alone_with_staggered_joins(nstp, block):
     b.close()
 
 
+def test_autotuned_batch_equals_alone():
+    """With the online tile-width tuner on, the active sets' first batched
+    steps run on several interior tile widths (the tables are rebuilt at
+    every candidate) before each set keeps its fastest; a set of a shape
+    tuned before takes its width at once.  Every job still gives its
+    seismograms alone bit for bit, and the width kept is one of the
+    candidates around the occupancy model's."""
+    print("case: 3 jobs alone vs batched with the tile-width tuner, joins after 60 and 110 steps", flush=True)
+    reqs = _reqs()
+    alone = []
+    for r in reqs:
+        s = _solver(r)
+        s.run(s.nt + 1)
+        alone.append(s.seismograms())
+        s.close()
+    b = Batch(3)
+    b.autotune(True)
+    sol = [_solver(r) for r in reqs]
+    b.add(sol[0])
+    b.run(60)               # the one-member set: 7 candidates x 5 steps, then its width
+    w1 = b.tile_width
+    b.add(sol[1])
+    b.run(50)
+    b.add(sol[2])
+    widths = set()
+    while b.active:
+        b.run(5)
+        widths.add(b.tile_width)
+    b.sync()
+    for s, (ax, az) in zip(sol, alone):
+        gx, gz = s.seismograms()
+        assert np.abs(az).max() > 0
+        assert np.array_equal(gx, ax) and np.array_equal(gz, az)
+        s.close()
+    b.close()
+    assert 4 <= w1 <= 256
+    assert len(widths) >= 2      # the three-member set was tuned over several widths
+
+
 @pytest.mark.parametrize("block", [0, 32])
 def test_graph_replayed_steps_equal_eager_launches(block, monkeypatch):
     """A batched step is built as a graph once per active set and replayed
