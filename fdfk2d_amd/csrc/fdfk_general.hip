@@ -333,40 +333,8 @@ __device__ T gz2(const GenArgs<T> &a, int mode, const F &u, int iz, int ix)
     const T dzs = (T)a.g.dz;
     return s * ((T)1.0f / (dzs * dzs));
 }
-// Dx(mat Dz u) (diffzx) and Dz(mat Dx u) (diffxz) at one cell
-template <class T, class F>
-__device__ T gzx(const GenArgs<T> &a, const T *mat, const F &u, int iz, int ix)
-{
-#pragma clang fp contract(off)
-    const int norder = a.g.norder;
-    const int nox = gmin(gmin(ix + a.g.npml - 1, a.g.nx + a.g.npml - ix), norder);
-    const T dx = (T)a.g.dx, dz = (T)a.g.dz;
-    T s = (T)0.0f;
-    for (int k = 1; k <= nox; ++k) {
-        const T vp = mat[gcell(a, iz, ix + k)] * gcdz(a, u, iz, ix + k) / dz;
-        const T vm = mat[gcell(a, iz, ix - k)] * gcdz(a, u, iz, ix - k) / dz;
-        s = gacc(s, gco(a.c1, norder, k, nox), (T)(vp - vm));
-    }
-    return s / dx;
-}
-template <class T, class F>
-__device__ T gxz(const GenArgs<T> &a, const T *mat, const F &u, int iz, int ix)
-{
-#pragma clang fp contract(off)
-    const int norder = a.g.norder;
-    const int noz = gmin(gmin(iz - a.zt, a.g.nz + a.g.npml - iz), norder);
-    const T dx = (T)a.g.dx, dz = (T)a.g.dz;
-    T s = (T)0.0f;
-    for (int k = 1; k <= noz; ++k) {
-        const T vp = mat[gcell(a, iz + k, ix)] * gcdx(a, u, iz + k, ix) / dx;
-        const T vm = mat[gcell(a, iz - k, ix)] * gcdx(a, u, iz - k, ix) / dx;
-        s = gacc(s, gco(a.c1, norder, k, noz), (T)(vp - vm));
-    }
-    return s / dz;
-}
-
-// ---- staged operators: the same sums as gx2, gz2, gzx and gxz, term by
-// term, with the inner sums read from the staged planes ----
+// ---- staged operators: the sums of gx2 and gz2 and of the mixed
+// operators, term by term, with the inner sums read from the staged planes ----
 // Order of the staggered difference at the face between p and p+1 as the
 // operators of diffx2/diffz2 choose it: the s1 sums of gx2/gz2 take
 // gnx(p) / gnz(p) at the face p = i+k-1, the s2 sums gnx(h+1) / gnz(h+1) at
@@ -446,7 +414,8 @@ __device__ T gz2s(const GenArgs<T> &a, int mode, const T *F, const I &idx, const
     const T dzs = (T)a.g.dz;
     return s * ((T)1.0f / (dzs * dzs));
 }
-// gzx / gxz from the staged undivided first derivatives D
+// Dx(mat Dz u) (diffzx) and Dz(mat Dx u) (diffxz) at one cell from the staged
+// undivided first derivatives D (Dz u, Dx u at the cells of the outer sum)
 template <class T, class I>
 __device__ T gzxs(const GenArgs<T> &a, const T *mat, const T *D, const I &idx, int iz, int ix)
 {
@@ -584,45 +553,9 @@ __device__ void gstage_sh(const GenArgs<T> &a, long t)
     }
 }
 
-// the six RHS parts {Xx, Zx, Mx, Xz, Zz, Mz} at one cell; the free-surface
-// closure on row 1 when zt = 1 (half-cell energy form, traction free)
-template <class T, class F>
-__device__ void gparts(const GenArgs<T> &a, const F &ux, const F &uz, int iz, int ix, T p[6])
-{
-#pragma clang fp contract(off)
-    if (iz == 1 && a.zt == 1) {
-        const size_t ca = gcell(a, 1, ix), cb = gcell(a, 2, ix);
-        const T dz = (T)a.g.dz, dx = (T)a.g.dx;
-        const T sdz2 = (T)2.0f / (dz * dz);
-        const int norder = a.g.norder;
-        const int nox = gmin(gmin(ix + a.g.npml - 1, a.g.nx + a.g.npml - ix), norder);
-        T sl = (T)0.0f, sm = (T)0.0f;
-        for (int k = 1; k <= nox; ++k) {
-            const size_t a1 = gcell(a, 1, ix + k), b1 = gcell(a, 2, ix + k), a2 = gcell(a, 1, ix - k), b2 = gcell(a, 2, ix - k);
-            const T lp = (T)0.5f * (a.lam[a1] + a.lam[b1]), lm = (T)0.5f * (a.lam[a2] + a.lam[b2]);
-            const T mp = (T)0.5f * (a.mu[a1] + a.mu[b1]), mm = (T)0.5f * (a.mu[a2] + a.mu[b2]);
-            sl = gacc(sl, gco(a.c1, norder, k, nox), (T)(lp * (uz(2, ix + k) - uz(1, ix + k)) - lm * (uz(2, ix - k) - uz(1, ix - k))));
-            sm = gacc(sm, gco(a.c1, norder, k, nox), (T)(mp * (ux(2, ix + k) - ux(1, ix + k)) - mm * (ux(2, ix - k) - ux(1, ix - k))));
-        }
-        const T lbar = (T)0.5f * (a.lam[ca] + a.lam[cb]), mbar = (T)0.5f * (a.mu[ca] + a.mu[cb]);
-        const T dxuz = (gcdx(a, uz, 1, ix) + gcdx(a, uz, 2, ix)) / dx;
-        const T dxux = (gcdx(a, ux, 1, ix) + gcdx(a, ux, 2, ix)) / dx;
-        p[0] = gx2(a, 0, ux, iz, ix);
-        p[1] = (a.hz[1][ca] * (ux(2, ix) - ux(1, ix))) * sdz2;
-        p[2] = (sl / dx + mbar * dxuz) / dz;
-        p[3] = gx2(a, 1, uz, iz, ix);
-        p[4] = (a.hz[0][ca] * (uz(2, ix) - uz(1, ix))) * sdz2;
-        p[5] = (sm / dx + lbar * dxux) / dz;
-        return;
-    }
-    p[0] = gx2(a, 0, ux, iz, ix);
-    p[1] = gz2(a, 1, ux, iz, ix);
-    p[2] = gzx(a, a.lam, uz, iz, ix) + gxz(a, a.mu, uz, iz, ix);
-    p[3] = gx2(a, 1, uz, iz, ix);
-    p[4] = gz2(a, 0, uz, iz, ix);
-    p[5] = gzx(a, a.mu, ux, iz, ix) + gxz(a, a.lam, ux, iz, ix);
-}
-// gparts of fields ux, uz from their staged terms P (same values)
+// The six RHS parts {Xx, Zx, Mx, Xz, Zz, Mz} of fields ux, uz at one cell
+// from their staged terms P; the free-surface closure on row 1 when zt = 1
+// (half-cell energy form, traction free)
 template <class T, class I, class U>
 __device__ void gparts_st(const GenArgs<T> &a, const T *const *P, const I &idx, const U &ux, const U &uz, int iz,
                           int ix, T p[6])
