@@ -61,11 +61,12 @@ struct GenArgs {
     // cells the step kernel reaches (the box minus the rectangle ex0..ex1 x
     // ez0..ez1, which is the hole shrunk by norder): a face value of the
     // staggered second-derivative operators, face modulus times staggered
-    // difference, and the undivided collocated first derivatives the mixed
-    // operators take.  P-SV: 0 x-face ux [lam+2mu], 1 z-face ux [mu],
-    // 2 x-face uz [mu], 3 z-face uz [lam+2mu], 4 dz ux, 5 dx ux, 6 dz uz,
-    // 7 dx uz, 8..11 weighted memories (gstage); SH: 0 x-face uy [mu], 1
-    // z-face uy [mu], 2..3 weighted memories.
+    // difference, and the products modulus times first derivative over
+    // the spacing that the mixed operators sum.  P-SV: 0 x-face ux
+    // [lam+2mu], 1 z-face ux [mu], 2 x-face uz [mu], 3 z-face uz [lam+2mu],
+    // 4 mu dz ux / dz, 5 lam dx ux / dx, 6 lam dz uz / dz, 7 mu dx uz / dx,
+    // 8..11 weighted memories (gstage); SH: 0 x-face uy [mu], 1 z-face uy
+    // [mu], 2..3 weighted memories.
     T *st[12];
     int ex0, ex1, ez0, ez1;
     // hybrid-ring correction terms of level n (the operator parts of the
@@ -415,36 +416,29 @@ __device__ T gz2s(const GenArgs<T> &a, int mode, const T *F, const I &idx, const
     return s * ((T)1.0f / (dzs * dzs));
 }
 // Dx(mat Dz u) (diffzx) and Dz(mat Dx u) (diffxz) at one cell from the staged
-// undivided first derivatives D (Dz u, Dx u at the cells of the outer sum)
+// products V = mat Dz u / dz (gzxs) or mat Dx u / dx (gxzs) at the cells of
+// the outer sum
 template <class T, class I>
-__device__ T gzxs(const GenArgs<T> &a, const T *mat, const T *D, const I &idx, int iz, int ix)
+__device__ T gzxs(const GenArgs<T> &a, const T *V, const I &idx, int iz, int ix)
 {
 #pragma clang fp contract(off)
     const int norder = a.g.norder;
     const int nox = gmin(gmin(ix + a.g.npml - 1, a.g.nx + a.g.npml - ix), norder);
-    const T dx = (T)a.g.dx, dz = (T)a.g.dz;
     T s = (T)0.0f;
-    for (int k = 1; k <= nox; ++k) {
-        const T vp = mat[gcell(a, iz, ix + k)] * D[idx(iz, ix + k)] / dz;
-        const T vm = mat[gcell(a, iz, ix - k)] * D[idx(iz, ix - k)] / dz;
-        s = gacc(s, gco(a.c1, norder, k, nox), (T)(vp - vm));
-    }
-    return s / dx;
+    for (int k = 1; k <= nox; ++k)
+        s = gacc(s, gco(a.c1, norder, k, nox), (T)(V[idx(iz, ix + k)] - V[idx(iz, ix - k)]));
+    return s / (T)a.g.dx;
 }
 template <class T, class I>
-__device__ T gxzs(const GenArgs<T> &a, const T *mat, const T *D, const I &idx, int iz, int ix)
+__device__ T gxzs(const GenArgs<T> &a, const T *V, const I &idx, int iz, int ix)
 {
 #pragma clang fp contract(off)
     const int norder = a.g.norder;
     const int noz = gmin(gmin(iz - a.zt, a.g.nz + a.g.npml - iz), norder);
-    const T dx = (T)a.g.dx, dz = (T)a.g.dz;
     T s = (T)0.0f;
-    for (int k = 1; k <= noz; ++k) {
-        const T vp = mat[gcell(a, iz + k, ix)] * D[idx(iz + k, ix)] / dx;
-        const T vm = mat[gcell(a, iz - k, ix)] * D[idx(iz - k, ix)] / dx;
-        s = gacc(s, gco(a.c1, norder, k, noz), (T)(vp - vm));
-    }
-    return s / dz;
+    for (int k = 1; k <= noz; ++k)
+        s = gacc(s, gco(a.c1, norder, k, noz), (T)(V[idx(iz + k, ix)] - V[idx(iz - k, ix)]));
+    return s / (T)a.g.dz;
 }
 // Cell t of the staged region (the box minus the shrunk hole)
 template <class T>
@@ -453,9 +447,11 @@ __device__ __forceinline__ bool gstage_cell(const GenArgs<T> &a, long t, int &iz
     const Geo &g = a.g;
     return gbox_cell(t, 1 - g.npml, g.nx + g.npml, a.zt, a.zt + g.NZ - 1, a.ex0, a.ex1, a.ez0, a.ez1, iz, ix);
 }
-// the staged terms of one cell (iz, ix) of fields ux, uz into planes P at o
+// the staged terms of one cell (iz, ix) of fields ux, uz into planes P at o;
+// d: the undivided first derivatives dz ux, dx ux, dz uz, dx uz
 template <class T, class U>
-__device__ void gstage_terms(const GenArgs<T> &a, const U &ux, const U &uz, int iz, int ix, T *const *P, size_t o)
+__device__ void gstage_terms(const GenArgs<T> &a, const U &ux, const U &uz, int iz, int ix, T *const *P, size_t o,
+                             T d[4])
 {
 #pragma clang fp contract(off)
     const size_t c = gcell(a, iz, ix);
@@ -464,10 +460,15 @@ __device__ void gstage_terms(const GenArgs<T> &a, const U &ux, const U &uz, int 
     P[1][o] = a.hz[1][c] * gsz(a, ux, iz, ix, nfz);
     P[2][o] = a.hx[1][c] * gsx(a, uz, iz, ix, nfx);
     P[3][o] = a.hz[0][c] * gsz(a, uz, iz, ix, nfz);
-    P[4][o] = gcdz(a, ux, iz, ix);
-    P[5][o] = gcdx(a, ux, iz, ix);
-    P[6][o] = gcdz(a, uz, iz, ix);
-    P[7][o] = gcdx(a, uz, iz, ix);
+    d[0] = gcdz(a, ux, iz, ix);
+    d[1] = gcdx(a, ux, iz, ix);
+    d[2] = gcdz(a, uz, iz, ix);
+    d[3] = gcdx(a, uz, iz, ix);
+    const T dx = (T)a.g.dx, dz = (T)a.g.dz;
+    P[4][o] = a.mu[c] * d[0] / dz;    // gzx[mu] of ux
+    P[5][o] = a.lam[c] * d[1] / dx;   // gxz[lam] of ux
+    P[6][o] = a.lam[c] * d[2] / dz;   // gzx[lam] of uz
+    P[7][o] = a.mu[c] * d[3] / dx;    // gxz[mu] of uz
 }
 template <class T, class U>
 __device__ void gstage_terms_sh(const GenArgs<T> &a, const U &u, int iz, int ix, T *const *P, size_t o)
@@ -490,7 +491,8 @@ __device__ void gstage(const GenArgs<T> &a, long t)
     int iz, ix;
     if (!gstage_cell(a, t, iz, ix)) return;
     const size_t c = gcell(a, iz, ix);
-    gstage_terms(a, GField<T>{&a, a.ux}, GField<T>{&a, a.uz}, iz, ix, a.st, c);
+    T d[4];
+    gstage_terms(a, GField<T>{&a, a.ux}, GField<T>{&a, a.uz}, iz, ix, a.st, c, d);
     const Geo &g = a.g;
     const int s = gstrip(a, iz, ix);
     // off the strips the weighted memories stay 0 from set-up
@@ -501,8 +503,8 @@ __device__ void gstage(const GenArgs<T> &a, long t)
         T *px0 = a.bpx[s][0], *px1 = a.bpx[s][1];
         if (a.dpx[jx] > (T)0.0f) {
             const T b = a.bx[jx], cc = (T)1.0f - b;
-            px0[o] = b * px0[o] + cc * (a.st[5][c] / (T)g.dx);
-            px1[o] = b * px1[o] + cc * (a.st[7][c] / (T)g.dx);
+            px0[o] = b * px0[o] + cc * (d[1] / (T)g.dx);
+            px1[o] = b * px1[o] + cc * (d[3] / (T)g.dx);
         }
         const T ma = a.lam[c] + (T)2.0f * a.mu[c], mu = a.mu[c];
         w[0] = ma * px0[o];
@@ -511,8 +513,8 @@ __device__ void gstage(const GenArgs<T> &a, long t)
             T *pz1 = a.bpz[s][1];
             if (a.dpz[jz] > (T)0.0f) {
                 const T b = a.bz[jz], cc = (T)1.0f - b;
-                pz0[o] = b * pz0[o] + cc * (a.st[4][c] / (T)g.dz);
-                pz1[o] = b * pz1[o] + cc * (a.st[6][c] / (T)g.dz);
+                pz0[o] = b * pz0[o] + cc * (d[0] / (T)g.dz);
+                pz1[o] = b * pz1[o] + cc * (d[2] / (T)g.dz);
             }
             w[2] = mu * pz0[o];
             w[3] = ma * pz1[o];
@@ -576,8 +578,8 @@ __device__ void gparts_st(const GenArgs<T> &a, const T *const *P, const I &idx, 
             sm = gacc(sm, gco(a.c1, norder, k, nox), (T)(mp * (ux(2, ix + k) - ux(1, ix + k)) - mm * (ux(2, ix - k) - ux(1, ix - k))));
         }
         const T lbar = (T)0.5f * (a.lam[ca] + a.lam[cb]), mbar = (T)0.5f * (a.mu[ca] + a.mu[cb]);
-        const T dxuz = (P[7][idx(1, ix)] + P[7][idx(2, ix)]) / dx;
-        const T dxux = (P[5][idx(1, ix)] + P[5][idx(2, ix)]) / dx;
+        const T dxuz = (gcdx(a, uz, 1, ix) + gcdx(a, uz, 2, ix)) / dx;
+        const T dxux = (gcdx(a, ux, 1, ix) + gcdx(a, ux, 2, ix)) / dx;
         p[0] = gx2s(a, 0, P[0], idx, ux, iz, ix);
         p[1] = (a.hz[1][ca] * (ux(2, ix) - ux(1, ix))) * sdz2;
         p[2] = (sl / dx + mbar * dxuz) / dz;
@@ -588,10 +590,10 @@ __device__ void gparts_st(const GenArgs<T> &a, const T *const *P, const I &idx, 
     }
     p[0] = gx2s(a, 0, P[0], idx, ux, iz, ix);
     p[1] = gz2s(a, 1, P[1], idx, ux, iz, ix);
-    p[2] = gzxs(a, a.lam, P[6], idx, iz, ix) + gxzs(a, a.mu, P[7], idx, iz, ix);
+    p[2] = gzxs(a, P[6], idx, iz, ix) + gxzs(a, P[7], idx, iz, ix);
     p[3] = gx2s(a, 1, P[2], idx, uz, iz, ix);
     p[4] = gz2s(a, 0, P[3], idx, uz, iz, ix);
-    p[5] = gzxs(a, a.mu, P[4], idx, iz, ix) + gxzs(a, a.lam, P[5], idx, iz, ix);
+    p[5] = gzxs(a, P[4], idx, iz, ix) + gxzs(a, P[5], idx, iz, ix);
 }
 // gparts_sh from the staged terms
 template <class T, class I, class U>
@@ -819,9 +821,10 @@ __global__ __launch_bounds__(256) void k_gen_ringst(GenArgs<T> a)
     if (t >= gbox_count(b.b[0], b.b[1], b.b[2], b.b[3], b.b[4], b.b[5], b.b[6], b.b[7])) return;
     int iz, ix;
     gbox_cell(t, b.b[0], b.b[1], b.b[2], b.b[3], b.b[4], b.b[5], b.b[6], b.b[7], iz, ix);
+    T d[4];
     for (int keep = 0; keep < 2; ++keep)
         gstage_terms(a, GMasked<T>{&a, a.ringx, keep == 1}, GMasked<T>{&a, a.ringz, keep == 1}, iz, ix,
-                     a.mq + 8 * keep, (size_t)t);
+                     a.mq + 8 * keep, (size_t)t, d);
 }
 template <class T>
 __global__ __launch_bounds__(256) void k_gen_ringq(GenArgs<T> a)
