@@ -197,6 +197,7 @@ struct fdfk_solver {
     // prep launch, join before the next one)
     hipStream_t stream2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    int gen_overlap = 2;   // 2: the per-cell kernels first on the solver's stream, 1: the interior first
     std::vector<std::pair<void *, size_t>> allocs;   // device blocks and their sizes
     int device = 0;
     int64_t bytes = 0;
@@ -986,8 +987,7 @@ int gen_step_t(fdfk_solver *s, bool record)
         }
     }
     const long nst = gbox_count(1 - g.npml, g.nx + g.npml, s->zt, s->zt + g.NZ - 1, a.ex0, a.ex1, a.ez0, a.ez1);
-    const long np = nrec + nring + nst;   // the memories are updated by the staging threads of strip cells
-    auto interior = [&]() {
+    auto interior = [&](hipStream_t st) {
         if constexpr (sizeof(T) == 4) {
             if (!s->gtiled) return;
             with_order(g.norder, [&](auto no) {
@@ -995,9 +995,9 @@ int gen_step_t(fdfk_solver *s, bool record)
                 const IntArgs<NO> ia = gen_interior_args<NO>(s);
                 const unsigned nb = (unsigned)(ia.nbx * ia.nbz + ia.nsb);
                 if (s->sh)
-                    hipLaunchKernelGGL((k_interior_sh<NO, 256>), dim3(nb), dim3(256), 0, s->stream, ia);
+                    hipLaunchKernelGGL((k_interior_sh<NO, 256>), dim3(nb), dim3(256), 0, st, ia);
                 else
-                    hipLaunchKernelGGL((k_interior2<NO, 256>), dim3(nb), dim3(256), 0, s->stream, ia);
+                    hipLaunchKernelGGL((k_interior2<NO, 256>), dim3(nb), dim3(256), 0, st, ia);
             });
         }
     };
@@ -1009,19 +1009,33 @@ int gen_step_t(fdfk_solver *s, bool record)
     // the per-cell kernels by cache latency, and their workgroups (no LDS,
     // few registers) fit beside the interior's on the same CUs.  The interior
     // is queued first so that its workgroups take their slots first.
-    const bool fork = s->stream2 != nullptr;
-    hipStream_t sc = fork ? s->stream2 : s->stream;
-    if (s->sh)
-        hipLaunchKernelGGL(k_gen_prep_sh<T>, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s->stream, a, r, nrec,
-                           nring, (T *)s->gring[0]);
-    else
-        hipLaunchKernelGGL(k_gen_prep<T>, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s->stream, a, r, nrec,
-                           nring, (T *)s->gring[0], (T *)s->gring[1]);
+    // mode 0: one stream; 1: the interior first, on the solver's stream,
+    // the per-cell kernels on the second; 2 (the default): the per-cell
+    // kernels first, on the solver's stream, the interior on the second
+    const int mode = s->stream2 ? s->gen_overlap : 0;
+    const bool fork = mode != 0;
+    hipStream_t sc = mode == 1 ? s->stream2 : s->stream;
+    hipStream_t si = mode == 2 ? s->stream2 : s->stream;
+    // the prep launch: receivers, ring values, then the staged terms (one
+    // launch: starting the interior after the ring values alone, beside a
+    // second launch for the staging, was slower)
+    auto prep = [&](long nr, long ng, long ns) {
+        const long ntot = nr + ng + ns;
+        if (ntot <= 0) return;
+        if (s->sh)
+            hipLaunchKernelGGL(k_gen_prep_sh<T>, dim3((unsigned)((ntot + 255) / 256)), dim3(256), 0, s->stream, a, r,
+                               nr, ng, ns, (T *)s->gring[0]);
+        else
+            hipLaunchKernelGGL(k_gen_prep<T>, dim3((unsigned)((ntot + 255) / 256)), dim3(256), 0, s->stream, a, r,
+                               nr, ng, ns, (T *)s->gring[0], (T *)s->gring[1]);
+    };
+    prep(nrec, nring, nst);
     if (fork) {
         HIPCHK(hipEventRecord(s->ev_fork, s->stream));
         HIPCHK(hipStreamWaitEvent(s->stream2, s->ev_fork, 0));
-        interior();
+        if (mode == 1) interior(si);
     }
+
     long nm = 0;   // cells of the ring's neighbourhood (P-SV's staged masked terms)
     if (nq && !s->sh) {
         const int no = g.norder;
@@ -1031,16 +1045,19 @@ int gen_step_t(fdfk_solver *s, bool record)
     }
     if (s->sh) {
         if (nq) hipLaunchKernelGGL(k_gen_ringq_sh<T>, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, sc, a);
-        if (!fork) interior();
+        if (!fork) interior(si);
         hipLaunchKernelGGL(k_gen_step_sh<T>, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, sc, a);
     } else {
         if (nq) {
             hipLaunchKernelGGL(k_gen_ringst<T>, dim3((unsigned)((nm + 255) / 256)), dim3(256), 0, sc, a);
             hipLaunchKernelGGL(k_gen_ringq<T>, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, sc, a);
         }
-        if (!fork) interior();
+        if (!fork) interior(si);
         hipLaunchKernelGGL(k_gen_step<T>, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, sc, a);
     }
+    // mode 2: the interior is queued after the per-cell kernels, whose
+    // workgroups (no LDS, few registers) then take their places first
+    if (mode == 2) interior(si);
     if (fork) {
         HIPCHK(hipEventRecord(s->ev_join, s->stream2));
         HIPCHK(hipStreamWaitEvent(s->stream, s->ev_join, 0));
@@ -1325,6 +1342,7 @@ static int gen_create(const fdfk_job *job, fdfk_solver **out)
             s->gview = v;
             // the per-cell kernels beside the interior (FDFK_GEN_OVERLAP=0: in line)
             const char *eo = std::getenv("FDFK_GEN_OVERLAP");
+            if (eo && std::atoi(eo) == 1) s->gen_overlap = 1;
             if (!(eo && std::atoi(eo) == 0) &&
                 (hipStreamCreate(&s->stream2) != hipSuccess ||
                  hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming) != hipSuccess ||

@@ -670,11 +670,11 @@ __device__ void grecord(const GenArgs<T> &a, const GenRec<T> &r, int i)
 // one thread per receiver, per ring cell and per cell of the staged region:
 // threads [0, nrec) record, [nrec, nrec + nring) write the ring values of
 // the cells of gring_box (the other cells of the ring fields stay zero from
-// their allocation), the rest stage the terms of level n (gstage: a strip
-// cell also updates its memories)
+// their allocation), the next nst stage the terms of level n (gstage: a
+// strip cell also updates its memories)
 template <class T>
-__global__ __launch_bounds__(256) void k_gen_prep(GenArgs<T> a, GenRec<T> r, long nrec, long nring, T *ringx,
-                                                   T *ringz)
+__global__ __launch_bounds__(256) void k_gen_prep(GenArgs<T> a, GenRec<T> r, long nrec, long nring, long nst,
+                                                   T *ringx, T *ringz)
 {
 #pragma clang fp contract(off)
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -693,7 +693,7 @@ __global__ __launch_bounds__(256) void k_gen_prep(GenArgs<T> a, GenRec<T> r, lon
         return;
     }
     t -= nring;
-    gstage(a, t);
+    if (t < nst) gstage(a, t);
 }
 
 // one thread per cell of the padded grid: level n+1
@@ -875,7 +875,8 @@ __global__ __launch_bounds__(256) void k_gen_ringq_sh(GenArgs<T> a)
 }
 
 template <class T>
-__global__ __launch_bounds__(256) void k_gen_prep_sh(GenArgs<T> a, GenRec<T> r, long nrec, long nring, T *ringx)
+__global__ __launch_bounds__(256) void k_gen_prep_sh(GenArgs<T> a, GenRec<T> r, long nrec, long nring, long nst,
+                                                      T *ringx)
 {
 #pragma clang fp contract(off)
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -892,7 +893,7 @@ __global__ __launch_bounds__(256) void k_gen_prep_sh(GenArgs<T> a, GenRec<T> r, 
         return;
     }
     t -= nring;
-    gstage_sh(a, t);
+    if (t < nst) gstage_sh(a, t);
 }
 
 template <class T>

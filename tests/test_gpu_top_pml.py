@@ -124,19 +124,24 @@ def test_top_layer_refusals():
 
 @pytest.mark.parametrize("wave,pml_top", [("psv", True), ("sh", True), ("sh", False)])
 def test_second_stream_equals_one_stream(monkeypatch, wave, pml_top):
-    """The per-cell kernels of an FP32 general-path step run on a second
-    stream beside the tiled interior kernel by default; with
-    FDFK_GEN_OVERLAP=0 they run in line on the solver's stream.  The two
-    schedules write disjoint cells from the same inputs, so seismograms and
-    final fields are identical."""
+    """An FP32 general-path step runs its per-cell kernels and the tiled
+    interior kernel on two streams side by side: by default the per-cell
+    kernels first on the solver's stream and the interior on the second;
+    with FDFK_GEN_OVERLAP=1 the interior first on the solver's stream; with
+    FDFK_GEN_OVERLAP=0 everything in line on one stream.  The schedules
+    write disjoint cells from the same inputs, so seismograms and final
+    fields are identical."""
     print(f"case: {wave}, top layer {pml_top}, second stream against one stream, 121x60, 260 iterations", flush=True)
     case = Case(nx=121, nz=60, npml=15, norder=3, nt=260, nrcv=13, pml_top=pml_top, rayp_sdeg=6.0, f0=1.2)
     case.rz = np.linspace(0.0, (case.nz - 1) * case.dz, 13).astype(np.float32)
     bg = case.oracle_background()
     w = lib.FDFK_SH if wave == "sh" else lib.FDFK_PSV
     out = []
-    for overlap in ("1", "0"):
-        monkeypatch.setenv("FDFK_GEN_OVERLAP", overlap)
+    for overlap in (None, "1", "0"):
+        if overlap is None:
+            monkeypatch.delenv("FDFK_GEN_OVERLAP", raising=False)
+        else:
+            monkeypatch.setenv("FDFK_GEN_OVERLAP", overlap)
         s = lib.Solver(case.nx, case.nz, case.npml, case.norder, case.dx, case.dz, case.dt, case.nt, case.c,
                        rx=case.rx, rz=case.rz, pml_top=pml_top, wave=w)
         s.set_background(bg)
@@ -144,5 +149,6 @@ def test_second_stream_equals_one_stream(monkeypatch, wave, pml_top):
         out.append(s.seismograms() + s.get_state())
         s.close()
     assert np.abs(out[0][0]).max() > 0
-    for a, b in zip(out[0], out[1]):
-        assert np.array_equal(np.asarray(a).view(np.uint32), np.asarray(b).view(np.uint32))
+    for other in out[1:]:
+        for a, b in zip(out[0], other):
+            assert np.array_equal(np.asarray(a).view(np.uint32), np.asarray(b).view(np.uint32))
