@@ -267,7 +267,6 @@ struct fdfk_solver {
     void *grho = nullptr, *glam = nullptr, *gmu = nullptr;
     void *ghx[2] = {}, *ghz[2] = {};   // face harmonic means [lam+2mu, mu]
     void *gst[12] = {};                // staged terms of level n (P-SV 12 planes, SH 4)
-    void *grq[6] = {};                 // ring correction terms (P-SV 6, SH 2), ring-cell order
     void *gmq[16] = {};                // staged terms of the masked ring fields (ring neighbourhood)
     void *gbu[4][2][3] = {};           // split parts (1:2, cells), reference order
     void *gbpx[4][2] = {}, *gbpz[4][2] = {};
@@ -905,7 +904,6 @@ GenArgs<T> gen_args(fdfk_solver *s)
     for (int i = 0; i < 16; ++i) a.bg[i] = (const T *)s->gbg[i];
     a.it = s->bg_period > 0 ? s->it % s->bg_period : 0;
     for (int i = 0; i < 12; ++i) a.st[i] = (T *)s->gst[i];
-    for (int i = 0; i < 6; ++i) a.rq[i] = (T *)s->grq[i];
     for (int i = 0; i < 16; ++i) a.mq[i] = (T *)s->gmq[i];
     a.ex0 = 1; a.ex1 = 0; a.ez0 = 1; a.ez1 = 0;   // staged over every cell
     return a;
@@ -1001,17 +999,19 @@ int gen_step_t(fdfk_solver *s, bool record)
             });
         }
     };
-    // the ring correction terms, from the ring values the prep launch writes
-    const long nq = a.ring_valid && s->grq[0] ? nring : 0;
+    // ring cells whose hybrid correction the step launch evaluates first
+    const long nq = a.ring_valid ? nring : 0;
+    const long nqb = (nq + 255) / 256 * 256;
     // The interior kernel and the per-cell kernels write disjoint cells of
     // level n+1 and read level n and the prep launch's output only, so with
     // a second stream they run side by side: the interior is bound by HBM,
     // the per-cell kernels by cache latency, and their workgroups (no LDS,
     // few registers) fit beside the interior's on the same CUs.  The interior
     // is queued first so that its workgroups take their slots first.
-    // mode 0: one stream; 1: the interior first, on the solver's stream,
-    // the per-cell kernels on the second; 2 (the default): the per-cell
-    // kernels first, on the solver's stream, the interior on the second
+    // mode 0: one stream; 1 (SH's default): the interior first, on the
+    // solver's stream, the per-cell kernels on the second; 2 (P-SV's
+    // default): the per-cell kernels first, on the solver's stream, the
+    // interior on the second
     const int mode = s->stream2 ? s->gen_overlap : 0;
     const bool fork = mode != 0;
     hipStream_t sc = mode == 1 ? s->stream2 : s->stream;
@@ -1044,16 +1044,12 @@ int gen_step_t(fdfk_solver *s, bool record)
                         rb[6] + no, rb[7] - no);
     }
     if (s->sh) {
-        if (nq) hipLaunchKernelGGL(k_gen_ringq_sh<T>, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, sc, a);
         if (!fork) interior(si);
-        hipLaunchKernelGGL(k_gen_step_sh<T>, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, sc, a);
+        hipLaunchKernelGGL(k_gen_step_sh<T>, dim3((unsigned)((nqb + nc + 255) / 256)), dim3(256), 0, sc, a, nq);
     } else {
-        if (nq) {
-            hipLaunchKernelGGL(k_gen_ringst<T>, dim3((unsigned)((nm + 255) / 256)), dim3(256), 0, sc, a);
-            hipLaunchKernelGGL(k_gen_ringq<T>, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, sc, a);
-        }
+        if (nq) hipLaunchKernelGGL(k_gen_ringst<T>, dim3((unsigned)((nm + 255) / 256)), dim3(256), 0, sc, a);
         if (!fork) interior(si);
-        hipLaunchKernelGGL(k_gen_step<T>, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, sc, a);
+        hipLaunchKernelGGL(k_gen_step<T>, dim3((unsigned)((nqb + nc + 255) / 256)), dim3(256), 0, sc, a, nq);
     }
     // mode 2: the interior is queued after the per-cell kernels, whose
     // workgroups (no LDS, few registers) then take their places first
@@ -1342,7 +1338,10 @@ static int gen_create(const fdfk_job *job, fdfk_solver **out)
             s->gview = v;
             // the per-cell kernels beside the interior (FDFK_GEN_OVERLAP=0: in line)
             const char *eo = std::getenv("FDFK_GEN_OVERLAP");
-            if (eo && std::atoi(eo) == 1) s->gen_overlap = 1;
+            // default: P-SV the per-cell kernels first, SH the interior
+            // first (the faster order of each, measured on the 8001x2000 solve)
+            s->gen_overlap = s->sh ? 1 : 2;
+            if (eo && (std::atoi(eo) == 1 || std::atoi(eo) == 2)) s->gen_overlap = std::atoi(eo);
             if (!(eo && std::atoi(eo) == 0) &&
                 (hipStreamCreate(&s->stream2) != hipSuccess ||
                  hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming) != hipSuccess ||
@@ -1631,21 +1630,18 @@ static int alloc_background(fdfk_solver *s, int period)
             HIPCHK(hipMemsetAsync(rz, 0, nf * 8, s->stream));
             s->gring[0] = rx; s->gring[1] = rz;
         }
-        if (s->gen && !s->grq[0]) {
-            // ring correction terms: every ring cell of the largest ring box
+        if (s->gen && !s->sh && !s->gmq[0]) {
+            // P-SV: staged terms of the masked ring fields over the ring's
+            // neighbourhood (the largest ring box grown and its hole shrunk
+            // by norder, within the grid)
             int b[8];
             gring_box(s->g, s->zt != 1, b);
-            const size_t nq = (size_t)gbox_count(b[0], b[1], b[2], b[3], b[4], b[5], b[6], b[7]);
-            for (int i = 0; i < (s->sh ? 2 : 6); ++i)
-                if ((rc = s->dalloc((char **)&s->grq[i], nq * (size_t)s->gsize))) return rc;
-            // staged masked terms over the ring's neighbourhood (the box grown
-            // and the hole shrunk by norder, within the grid)
             const Geo &g = s->g;
             const int no = g.norder;
             const size_t nm = (size_t)gbox_count(std::max(b[0] - no, 1 - g.npml), std::min(b[1] + no, g.nx + g.npml),
                                                  std::max(b[2] - no, s->zt), std::min(b[3] + no, s->zt + g.NZ - 1),
                                                  b[4] + no, b[5] - no, b[6] + no, b[7] - no);
-            for (int k = 0; k < 2 && !s->sh; ++k)
+            for (int k = 0; k < 2; ++k)
                 for (int i = 0; i < 8; ++i)
                     if ((rc = s->dalloc((char **)&s->gmq[8 * k + i], nm * (size_t)s->gsize))) return rc;
         }

@@ -22,10 +22,11 @@
 //
 // Launches per step: k_gen_prep (receivers, hybrid ring values, PML
 // memories and the staged operator terms, all from level n), for P-SV
-// k_gen_ringst (the staged terms of the masked ring fields), k_gen_ringq
-// (the hybrid correction terms of the ring cells), then k_gen_step (every
-// cell of level n+1, or, in FP32, the cells around the deep interior
-// rectangle that the tiled interior kernel updates in the same step).
+// k_gen_ringst (the staged terms of the masked ring fields), then
+// k_gen_step (every cell of level n+1, or, in FP32, the cells around the
+// deep interior rectangle that the tiled interior kernel updates in the
+// same step; the ring cells first, in waves of their own, with their hybrid
+// correction).
 // ---------------------------------------------------------------------
 namespace fdfk {
 
@@ -69,10 +70,6 @@ struct GenArgs {
     // [mu], 2..3 weighted memories.
     T *st[12];
     int ex0, ex1, ez0, ez1;
-    // hybrid-ring correction terms of level n (the operator parts of the
-    // masked ring background, P-SV 6, SH 2) in the ring cells' order of
-    // gring_box, formed by a launch of their own before the step
-    T *rq[6];
     // P-SV: staged terms of the masked ring field (the other side's values
     // only) over the ring's neighbourhood (gring_region, compact): planes
     // 0..7 of the padding's values (for cells of the total-field region),
@@ -157,6 +154,9 @@ __host__ __device__ inline void gring_box(const Geo &g, int top_ring, int b[8])
     b[4] = g.nbd + 1; b[5] = g.nx - g.nbd; b[6] = top_ring ? g.nbd + 1 : 1; b[7] = g.nz - g.nbd;
 }
 
+// row of "no cell" for a thread of a step launch that has none
+constexpr int kNoCell = -(1 << 30);
+
 // Cell t of a step launch: every cell, or with a hole (the rectangle the
 // tiled interior kernel updates) the cells around it.
 template <class T>
@@ -228,15 +228,6 @@ __device__ __forceinline__ long gring(const GenArgs<T> &a, int iz, int ix, int i
         }
     }
     return -1;
-}
-
-// index of a ring cell in the order of gring_box
-template <class T>
-__device__ __forceinline__ long gring_index(const GenArgs<T> &a, int iz, int ix)
-{
-    int b[8];
-    gring_box(a.g, a.top_ring, b);
-    return gbox_index(iz, ix, b[0], b[1], b[2], b[3], b[4], b[5], b[6], b[7]);
 }
 
 // field accessors: plain level-n fields, or the ring background masked to
@@ -698,12 +689,21 @@ __global__ __launch_bounds__(256) void k_gen_prep(GenArgs<T> a, GenRec<T> r, lon
 
 // one thread per cell of the padded grid: level n+1
 template <class T>
-__global__ __launch_bounds__(256) void k_gen_step(GenArgs<T> a)
+__global__ __launch_bounds__(256) void k_gen_step(GenArgs<T> a, long nq)
 {
 #pragma clang fp contract(off)
     const Geo &g = a.g;
     int iz, ix;
-    if (!gstep_cell(a, (long)blockIdx.x * blockDim.x + threadIdx.x, iz, ix)) return;
+    T q[6];
+    const bool ringcell = gstep_job(a, nq, iz, ix);
+    if (iz == kNoCell) return;
+    if (ringcell) {
+        // the hybrid correction: the operators of the ring field masked to
+        // the other side of the interface, from its staged terms
+        const bool inside = iz >= 1 && iz <= g.nz && ix >= 1 && ix <= g.nx;
+        const GMasked<T> wx{&a, a.ringx, !inside}, wz{&a, a.ringz, !inside};
+        gparts_st(a, a.mq + (inside ? 0 : 8), gring_region(a), wx, wz, iz, ix, q);
+    }
     const size_t gi = gcell(a, iz, ix);
     const int iz_first = a.zt == 1 ? 1 : a.zt + 1;
     if (ix < 2 - g.npml || ix > g.nx + g.npml - 1 || iz < iz_first || iz > g.nz + g.npml - 1) {
@@ -743,13 +743,9 @@ __global__ __launch_bounds__(256) void k_gen_step(GenArgs<T> a)
     // hybrid TF/SF correction: an inside-ring cell adds the operator of the
     // background on the padding, an outside-ring cell subtracts that of the
     // background on the total-field region
-    if (a.ring_valid) {
-        int arr = 0;
-        if (gring(a, iz, ix, a.it, arr) >= 0) {
-            const bool inside = iz >= 1 && iz <= g.nz && ix >= 1 && ix <= g.nx;
-            const long q = gring_index(a, iz, ix);
-            for (int m = 0; m < 6; ++m) p[m] = inside ? p[m] + a.rq[m][q] : p[m] - a.rq[m][q];
-        }
+    if (ringcell) {
+        const bool inside = iz >= 1 && iz <= g.nz && ix >= 1 && ix <= g.nx;
+        for (int m = 0; m < 6; ++m) p[m] = inside ? p[m] + q[m] : p[m] - q[m];
     }
     const T dt2 = (T)(g.dt * g.dt);
     const T bf = dt2 / a.rho[gi];
@@ -826,18 +822,35 @@ __global__ __launch_bounds__(256) void k_gen_ringst(GenArgs<T> a)
         gstage_terms(a, GMasked<T>{&a, a.ringx, keep == 1}, GMasked<T>{&a, a.ringz, keep == 1}, iz, ix,
                      a.mq + 8 * keep, (size_t)t, d);
 }
+// Cell of thread t of a step launch.  With nq > 0 (the ring values are
+// valid) the launch starts with the ring cells that the step updates, nq
+// threads in the order of gring_box rounded up to whole blocks, so that the
+// waves evaluating the hybrid correction hold ring cells only; the cells of
+// the band around the interior follow, the ring cells among them left to
+// the first range.  Returns true for a ring cell of the first range; iz =
+// kNoCell for a thread without a cell.
 template <class T>
-__global__ __launch_bounds__(256) void k_gen_ringq(GenArgs<T> a)
+__device__ __forceinline__ bool gstep_job(const GenArgs<T> &a, long nq, int &iz, int &ix)
 {
-#pragma clang fp contract(off)
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    int iz, ix;
-    if (!gring_job(a, t, iz, ix)) return;
-    const bool inside = iz >= 1 && iz <= a.g.nz && ix >= 1 && ix <= a.g.nx;
-    const GMasked<T> wx{&a, a.ringx, !inside}, wz{&a, a.ringz, !inside};
-    T q[6];
-    gparts_st(a, a.mq + (inside ? 0 : 8), gring_region(a), wx, wz, iz, ix, q);
-    for (int m = 0; m < 6; ++m) a.rq[m][t] = q[m];
+    const long nqb = (nq + 255) / 256 * 256;
+    iz = kNoCell;
+    if (t < nqb) {
+        if (t >= nq || !gring_job(a, t, iz, ix)) iz = kNoCell;
+        return iz != kNoCell;
+    }
+    if (!gstep_cell(a, t - nqb, iz, ix)) {
+        iz = kNoCell;
+        return false;
+    }
+    if (nq > 0) {
+        const Geo &g = a.g;
+        const int iz_first = a.zt == 1 ? 1 : a.zt + 1;
+        const bool edge = ix < 2 - g.npml || ix > g.nx + g.npml - 1 || iz < iz_first || iz > g.nz + g.npml - 1;
+        int arr = 0;
+        if (!edge && gring(a, iz, ix, a.it, arr) >= 0) iz = kNoCell;   // updated by the first range
+    }
+    return false;
 }
 
 // ---- SH (transverse) motion: uy in the ux slots (fields, ring values,
@@ -857,21 +870,6 @@ __device__ void gparts_sh(const GenArgs<T> &a, const F &u, int iz, int ix, T &X,
     } else {
         Z = gz2(a, 1, u, iz, ix);
     }
-}
-
-template <class T>
-__global__ __launch_bounds__(256) void k_gen_ringq_sh(GenArgs<T> a)
-{
-#pragma clang fp contract(off)
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    int iz, ix;
-    if (!gring_job(a, t, iz, ix)) return;
-    const bool inside = iz >= 1 && iz <= a.g.nz && ix >= 1 && ix <= a.g.nx;
-    const GMasked<T> w{&a, a.ringx, !inside};
-    T qx, qz;
-    gparts_sh(a, w, iz, ix, qx, qz);
-    a.rq[0][t] = qx;
-    a.rq[1][t] = qz;
 }
 
 template <class T>
@@ -897,12 +895,18 @@ __global__ __launch_bounds__(256) void k_gen_prep_sh(GenArgs<T> a, GenRec<T> r, 
 }
 
 template <class T>
-__global__ __launch_bounds__(256) void k_gen_step_sh(GenArgs<T> a)
+__global__ __launch_bounds__(256) void k_gen_step_sh(GenArgs<T> a, long nq)
 {
 #pragma clang fp contract(off)
     const Geo &g = a.g;
     int iz, ix;
-    if (!gstep_cell(a, (long)blockIdx.x * blockDim.x + threadIdx.x, iz, ix)) return;
+    T qx = (T)0.0f, qz = (T)0.0f;
+    const bool ringcell = gstep_job(a, nq, iz, ix);
+    if (iz == kNoCell) return;
+    if (ringcell) {
+        const bool inside = iz >= 1 && iz <= g.nz && ix >= 1 && ix <= g.nx;
+        gparts_sh(a, GMasked<T>{&a, a.ringx, !inside}, iz, ix, qx, qz);
+    }
     const size_t gi = gcell(a, iz, ix);
     const int iz_first = a.zt == 1 ? 1 : a.zt + 1;
     if (ix < 2 - g.npml || ix > g.nx + g.npml - 1 || iz < iz_first || iz > g.nz + g.npml - 1) {
@@ -928,15 +932,10 @@ __global__ __launch_bounds__(256) void k_gen_step_sh(GenArgs<T> a)
         X = X - sx / (T)g.dx;
         if (!surf) Z = Z - sz / (T)g.dz;
     }
-    if (a.ring_valid) {
-        int arr = 0;
-        if (gring(a, iz, ix, a.it, arr) >= 0) {
-            const bool inside = iz >= 1 && iz <= g.nz && ix >= 1 && ix <= g.nx;
-            const long q = gring_index(a, iz, ix);
-            const T qx = a.rq[0][q], qz = a.rq[1][q];
-            if (inside) { X = X + qx; Z = Z + qz; }
-            else { X = X - qx; Z = Z - qz; }
-        }
+    if (ringcell) {
+        const bool inside = iz >= 1 && iz <= g.nz && ix >= 1 && ix <= g.nx;
+        if (inside) { X = X + qx; Z = Z + qz; }
+        else { X = X - qx; Z = Z - qz; }
     }
     const T bf = (T)(g.dt * g.dt) / a.rho[gi];
     const int s = gstrip(a, iz, ix);

@@ -125,19 +125,19 @@ def test_top_layer_refusals():
 @pytest.mark.parametrize("wave,pml_top", [("psv", True), ("sh", True), ("sh", False)])
 def test_second_stream_equals_one_stream(monkeypatch, wave, pml_top):
     """An FP32 general-path step runs its per-cell kernels and the tiled
-    interior kernel on two streams side by side: by default the per-cell
-    kernels first on the solver's stream and the interior on the second;
-    with FDFK_GEN_OVERLAP=1 the interior first on the solver's stream; with
-    FDFK_GEN_OVERLAP=0 everything in line on one stream.  The schedules
-    write disjoint cells from the same inputs, so seismograms and final
-    fields are identical."""
+    interior kernel on two streams side by side: with FDFK_GEN_OVERLAP=2
+    (P-SV's default) the per-cell kernels first on the solver's stream and
+    the interior on the second, with FDFK_GEN_OVERLAP=1 (SH's default) the
+    interior first, with FDFK_GEN_OVERLAP=0 everything in line on one
+    stream.  The schedules write disjoint cells from the same inputs, so
+    seismograms and final fields are identical."""
     print(f"case: {wave}, top layer {pml_top}, second stream against one stream, 121x60, 260 iterations", flush=True)
     case = Case(nx=121, nz=60, npml=15, norder=3, nt=260, nrcv=13, pml_top=pml_top, rayp_sdeg=6.0, f0=1.2)
     case.rz = np.linspace(0.0, (case.nz - 1) * case.dz, 13).astype(np.float32)
     bg = case.oracle_background()
     w = lib.FDFK_SH if wave == "sh" else lib.FDFK_PSV
     out = []
-    for overlap in (None, "1", "0"):
+    for overlap in (None, "2", "1", "0"):
         if overlap is None:
             monkeypatch.delenv("FDFK_GEN_OVERLAP", raising=False)
         else:
